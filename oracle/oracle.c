@@ -1,0 +1,1312 @@
+/*
+ * oracle.c -- CPU FP64 restatement of the Schwinger_MG hot path (C99 + OpenMP).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library and the timed CPU
+ * baseline.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg load it.  Each function cites the reference file:line it restates; loop
+ * order and arithmetic order follow the reference so that, single-threaded, most
+ * results agree with the compiled reference to the last bit.
+ *
+ * Deviation (documented, deliberate): the reference builds its Galerkin
+ * operators by probing one coarse column at a time (src/coarse_op.h:26-43,
+ * 241-279); orc_coarse_build(colored=1) probes one colour class of aggregates
+ * at a time instead (9*k probes instead of dim), which yields the same 9-point
+ * stencil whenever every coarse direction is at least 3 aggregates long.
+ */
+#include "oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define OMP_MIN 4096
+#define OMP_FOR _Pragma("omp parallel for schedule(static) if(n > OMP_MIN)")
+
+static void* xcalloc(size_t n, size_t sz) {
+    void* p = calloc(n ? n : 1, sz);
+    if (!p) abort();
+    return p;
+}
+
+/* ================================================================ geometry */
+/* src/lattice.h:19-26 */
+static inline int site_idx(int L, int x, int y) { return ((x % L + L) % L) + L * ((y % L + L) % L); }
+static inline int xp(int L, int s) { return site_idx(L, s % L + 1, s / L); }
+static inline int xm(int L, int s) { return site_idx(L, s % L - 1, s / L); }
+static inline int yp(int L, int s) { return site_idx(L, s % L, s / L + 1); }
+static inline int ym(int L, int s) { return site_idx(L, s % L, s / L - 1); }
+
+/* src/lattice.h:28-41 */
+void orc_eo_maps(int L, int* even_sites, int* odd_sites, int* eo_index) {
+    int V = L * L, ne = 0, no = 0;
+    for (int s = 0; s < V; s++) {
+        int par = ((s % L) + (s / L)) % 2;
+        if (par == 0) { eo_index[s] = ne; even_sites[ne++] = s; }
+        else          { eo_index[s] = no; odd_sites[no++] = s; }
+    }
+}
+
+/* src/gauge.h:26-29 */
+static inline ocx plaq(int L, const ocx* U, int s) {
+    int V = L * L;
+    int sp0 = xp(L, s), sp1 = yp(L, s);
+    return U[s] * U[V + sp0] * conj(U[sp1]) * conj(U[V + s]);
+}
+
+/* src/gauge.h:31-36 */
+double orc_avg_plaq(int L, const ocx* U) {
+    int V = L * L;
+    double acc = 0;
+    for (int i = 0; i < V; i++) acc += creal(plaq(L, U, i));
+    return acc / V;
+}
+
+/* src/hmc_forces.cpp:80-86 */
+double orc_gauge_action(int L, const ocx* U, double beta) {
+    int V = L * L;
+    double acc = 0;
+    for (int i = 0; i < V; i++) acc += creal(plaq(L, U, i));
+    return -beta * acc;
+}
+
+/* src/hmc_forces.cpp:88-111 */
+void orc_gauge_force(int L, const ocx* U, double beta, double* F) {
+    int V = L * L, n = V;
+    const ocx* U0 = U; const ocx* U1 = U + V;
+    OMP_FOR
+    for (int s = 0; s < V; s++) {
+        {
+            int sx = xp(L, s), sy = yp(L, s), smy = ym(L, s);
+            int sxmy = ym(L, sx);
+            ocx staple = U1[sx] * conj(U0[sy]) * conj(U1[s]) + conj(U1[sxmy]) * conj(U0[smy]) * U1[smy];
+            F[s] = -beta * cimag(U0[s] * staple);
+        }
+        {
+            int sy = yp(L, s), sx = xp(L, s), smx = xm(L, s);
+            int symx = xm(L, sy);
+            ocx staple = U0[sy] * conj(U1[sx]) * conj(U0[s]) + conj(U0[symx]) * conj(U1[smx]) * U0[smx];
+            F[V + s] = -beta * cimag(U1[s] * staple);
+        }
+    }
+}
+
+/* src/dirac.h:26-52 */
+void orc_clover(int L, const ocx* U, double* F01) {
+    int V = L * L, n = V;
+    const ocx* U0 = U; const ocx* U1 = U + V;
+    OMP_FOR
+    for (int s = 0; s < V; s++) {
+        int sxp = xp(L, s), sxm = xm(L, s), syp = yp(L, s), sym = ym(L, s);
+        ocx P_ur = U0[s] * U1[sxp] * conj(U0[syp]) * conj(U1[s]);
+        int sxmyp = yp(L, sxm);
+        ocx P_ul = U1[s] * conj(U0[sxmyp]) * conj(U1[sxm]) * U0[sxm];
+        int sxmym = ym(L, sxm);
+        ocx P_ll = conj(U0[sxm]) * conj(U1[sxmym]) * U0[sxmym] * U1[sym];
+        int sxpym = ym(L, sxp);
+        ocx P_lr = conj(U1[sym]) * U0[sym] * U1[sxpym] * conj(U0[s]);
+        ocx Q = P_ur + P_ul + P_ll + P_lr;
+        F01[s] = cimag(Q) / 4.0;
+    }
+}
+
+/* ================================================================ Dirac */
+struct orc_dirac {
+    int L, V;
+    ocx* U;
+    double* F01;
+    double mass, r, csw, mu_t;
+    int *even, *odd, *eoi;
+};
+
+orc_dirac* orc_dirac_new(int L, const ocx* U, double mass, double r, double csw, double mu_t) {
+    orc_dirac* D = xcalloc(1, sizeof(orc_dirac));
+    D->L = L; D->V = L * L;
+    D->U = xcalloc(2 * (size_t)D->V, sizeof(ocx));
+    memcpy(D->U, U, 2 * (size_t)D->V * sizeof(ocx));
+    D->mass = mass; D->r = r; D->csw = csw; D->mu_t = mu_t;
+    D->F01 = xcalloc(D->V, sizeof(double));
+    if (csw != 0.0) orc_clover(L, D->U, D->F01);   /* src/dirac.h:17-22 */
+    D->even = xcalloc(D->V / 2 + 1, sizeof(int));
+    D->odd = xcalloc(D->V / 2 + 1, sizeof(int));
+    D->eoi = xcalloc(D->V, sizeof(int));
+    orc_eo_maps(L, D->even, D->odd, D->eoi);
+    return D;
+}
+
+void orc_dirac_free(orc_dirac* D) {
+    if (!D) return;
+    free(D->U); free(D->F01); free(D->even); free(D->odd); free(D->eoi); free(D);
+}
+
+const double* orc_dirac_clover(const orc_dirac* D) { return D->F01; }
+
+/* src/dirac.h:58-89 -- the four Wilson hops at s, with optional phase scales */
+static inline void accumulate_hops(ocx* d0, ocx* d1, int s, const ocx* src, const orc_dirac* D,
+                                   ocx fwd_x, ocx bwd_x, ocx fwd_y, ocx bwd_y) {
+    const int L = D->L, V = D->V;
+    const double r = D->r;
+    const ocx* U0 = D->U; const ocx* U1 = D->U + V;
+    const ocx ii = I;
+    { int sf = xp(L, s);
+      ocx u = fwd_x * U0[s];
+      ocx up0 = u * src[2 * sf], up1 = u * src[2 * sf + 1];
+      *d0 -= 0.5 * (r * up0 - up1);
+      *d1 -= 0.5 * (-up0 + r * up1); }
+    { int sb = xm(L, s);
+      ocx ud = bwd_x * conj(U0[sb]);
+      ocx udq0 = ud * src[2 * sb], udq1 = ud * src[2 * sb + 1];
+      *d0 -= 0.5 * (r * udq0 + udq1);
+      *d1 -= 0.5 * (udq0 + r * udq1); }
+    { int sf = yp(L, s);
+      ocx u = fwd_y * U1[s];
+      ocx up0 = u * src[2 * sf], up1 = u * src[2 * sf + 1];
+      *d0 -= 0.5 * (r * up0 + ii * up1);
+      *d1 -= 0.5 * (-ii * up0 + r * up1); }
+    { int sb = ym(L, s);
+      ocx ud = bwd_y * conj(U1[sb]);
+      ocx udq0 = ud * src[2 * sb], udq1 = ud * src[2 * sb + 1];
+      *d0 -= 0.5 * (r * udq0 - ii * udq1);
+      *d1 -= 0.5 * (ii * udq0 + r * udq1); }
+}
+
+/* src/dirac.h:93-113 */
+static void apply_impl(const orc_dirac* D, const ocx* src, ocx* dst, double mu_eff) {
+    int V = D->V, n = V;
+    int has_clover = (D->csw != 0.0), has_twist = (mu_eff != 0.0);
+    OMP_FOR
+    for (int s = 0; s < V; s++) {
+        ocx s0 = src[2 * s], s1 = src[2 * s + 1];
+        double diag = 2.0 * D->r + D->mass;
+        ocx a0, a1;
+        if (has_clover || has_twist) {
+            double C = has_clover ? 0.5 * D->csw * D->F01[s] : 0.0;
+            ocx e0 = CMPLX(diag + C, mu_eff);
+            ocx e1 = CMPLX(diag - C, -mu_eff);
+            a0 = e0 * s0; a1 = e1 * s1;
+        } else {
+            a0 = diag * s0; a1 = diag * s1;
+        }
+        accumulate_hops(&a0, &a1, s, src, D, 1.0, 1.0, 1.0, 1.0);
+        dst[2 * s] = a0; dst[2 * s + 1] = a1;
+    }
+}
+
+/* src/dirac.h:120-134: D^dag = g5 D(-mu) g5 */
+static void apply_dag(const orc_dirac* D, const ocx* src, ocx* dst) {
+    int V = D->V, n = V;
+    ocx* g5src = xcalloc(2 * (size_t)V, sizeof(ocx));
+    ocx* g5dst = xcalloc(2 * (size_t)V, sizeof(ocx));
+    OMP_FOR
+    for (int s = 0; s < V; s++) { g5src[2 * s] = src[2 * s]; g5src[2 * s + 1] = -src[2 * s + 1]; }
+    apply_impl(D, g5src, g5dst, -D->mu_t);
+    OMP_FOR
+    for (int s = 0; s < V; s++) { dst[2 * s] = g5dst[2 * s]; dst[2 * s + 1] = -g5dst[2 * s + 1]; }
+    free(g5src); free(g5dst);
+}
+
+void orc_dirac_apply(const orc_dirac* D, int kind, const ocx* src, ocx* dst) {
+    int V = D->V, n = V;
+    switch (kind) {
+    case ORC_D: apply_impl(D, src, dst, D->mu_t); break;      /* src/dirac.h:115-117 */
+    case ORC_DDAG: apply_dag(D, src, dst); break;
+    case ORC_DDAGD: {                                          /* src/dirac.h:136-140 */
+        ocx* tmp = xcalloc(2 * (size_t)V, sizeof(ocx));
+        apply_impl(D, src, tmp, D->mu_t);
+        apply_dag(D, tmp, dst);
+        free(tmp);
+        break; }
+    case ORC_G5D:                                              /* src/dirac.h:145-152 */
+        apply_impl(D, src, dst, D->mu_t);
+        OMP_FOR
+        for (int s = 0; s < V; s++) dst[2 * s + 1] = -dst[2 * s + 1];
+        break;
+    default: abort();
+    }
+}
+
+/* src/dirac.h:159-174 */
+void orc_dirac_delta(const orc_dirac* D, const ocx* src, ocx* dst, const double* pi, double dt) {
+    int V = D->V, n = V, L = D->L;
+    OMP_FOR
+    for (int s = 0; s < V; s++) {
+        ocx a0 = 0.0, a1 = 0.0;
+        ocx fwd_x = cexp(CMPLX(0, dt * pi[s])) - 1.0;
+        ocx bwd_x = cexp(CMPLX(0, -dt * pi[xm(L, s)])) - 1.0;
+        ocx fwd_y = cexp(CMPLX(0, dt * pi[V + s])) - 1.0;
+        ocx bwd_y = cexp(CMPLX(0, -dt * pi[V + ym(L, s)])) - 1.0;
+        accumulate_hops(&a0, &a1, s, src, D, fwd_x, bwd_x, fwd_y, bwd_y);
+        dst[2 * s] = a0; dst[2 * s + 1] = a1;
+    }
+}
+
+/* ================================================================ even-odd */
+/* src/dirac.h:198-250: dst (dst_sites parity) = hopping of src (other parity) */
+static void eo_hop(const orc_dirac* D, const ocx* src, ocx* dst, const int* dst_sites) {
+    int nh = D->V / 2, n = nh * 8, L = D->L, V = D->V;
+    const double r = D->r;
+    const ocx* U0 = D->U; const ocx* U1 = D->U + V;
+    const ocx ii = I;
+    OMP_FOR
+    for (int id = 0; id < nh; id++) {
+        int s = dst_sites[id];
+        ocx a0 = 0.0, a1 = 0.0;
+        { int is = D->eoi[xp(L, s)];
+          ocx u = U0[s];
+          ocx up0 = u * src[2 * is], up1 = u * src[2 * is + 1];
+          a0 -= 0.5 * (r * up0 - up1);
+          a1 -= 0.5 * (-up0 + r * up1); }
+        { int sb = xm(L, s); int is = D->eoi[sb];
+          ocx ud = conj(U0[sb]);
+          ocx udq0 = ud * src[2 * is], udq1 = ud * src[2 * is + 1];
+          a0 -= 0.5 * (r * udq0 + udq1);
+          a1 -= 0.5 * (udq0 + r * udq1); }
+        { int is = D->eoi[yp(L, s)];
+          ocx u = U1[s];
+          ocx up0 = u * src[2 * is], up1 = u * src[2 * is + 1];
+          a0 -= 0.5 * (r * up0 + ii * up1);
+          a1 -= 0.5 * (-ii * up0 + r * up1); }
+        { int sb = ym(L, s); int is = D->eoi[sb];
+          ocx ud = conj(U1[sb]);
+          ocx udq0 = ud * src[2 * is], udq1 = ud * src[2 * is + 1];
+          a0 -= 0.5 * (r * udq0 - ii * udq1);
+          a1 -= 0.5 * (ii * udq0 + r * udq1); }
+        dst[2 * id] = a0; dst[2 * id + 1] = a1;
+    }
+}
+
+/* src/dirac.h:277-300, 370-380 */
+static void diag_inv(const orc_dirac* D, const ocx* src, ocx* dst, const int* sites, double mu_eff) {
+    int nh = D->V / 2, n = nh * 8;
+    double diag = 2.0 * D->r + D->mass;
+    int hc = (D->csw != 0.0), ht = (mu_eff != 0.0);
+    if (!hc && !ht) {
+        double inv = 1.0 / diag;
+        OMP_FOR
+        for (int i = 0; i < nh; i++) { dst[2 * i] = inv * src[2 * i]; dst[2 * i + 1] = inv * src[2 * i + 1]; }
+        return;
+    }
+    OMP_FOR
+    for (int i = 0; i < nh; i++) {
+        int s = sites[i];
+        double C = hc ? 0.5 * D->csw * D->F01[s] : 0.0;
+        ocx d0 = CMPLX(diag + C, mu_eff), d1 = CMPLX(diag - C, -mu_eff);
+        dst[2 * i] = src[2 * i] / d0;
+        dst[2 * i + 1] = src[2 * i + 1] / d1;
+    }
+}
+
+/* src/dirac.h:302-323, 382-392 */
+static void diag_mul(const orc_dirac* D, const ocx* src, ocx* dst, const int* sites, double mu_eff) {
+    int nh = D->V / 2, n = nh * 8;
+    double diag = 2.0 * D->r + D->mass;
+    int hc = (D->csw != 0.0), ht = (mu_eff != 0.0);
+    OMP_FOR
+    for (int i = 0; i < nh; i++) {
+        if (!hc && !ht) { dst[2 * i] = diag * src[2 * i]; dst[2 * i + 1] = diag * src[2 * i + 1]; continue; }
+        int s = sites[i];
+        double C = hc ? 0.5 * D->csw * D->F01[s] : 0.0;
+        ocx d0 = CMPLX(diag + C, mu_eff), d1 = CMPLX(diag - C, -mu_eff);
+        dst[2 * i] = d0 * src[2 * i];
+        dst[2 * i + 1] = d1 * src[2 * i + 1];
+    }
+}
+
+/* src/dirac.h:399-415 */
+static void schur_impl(const orc_dirac* D, const ocx* src_o, ocx* dst_o, double mu_eff) {
+    int nh = D->V / 2, n = nh * 8;
+    ocx* t1 = xcalloc(2 * (size_t)nh, sizeof(ocx));
+    ocx* t2 = xcalloc(2 * (size_t)nh, sizeof(ocx));
+    eo_hop(D, src_o, t1, D->even);            /* apply_oe */
+    diag_inv(D, t1, t2, D->even, mu_eff);
+    eo_hop(D, t2, dst_o, D->odd);             /* apply_eo */
+    double diag = 2.0 * D->r + D->mass;
+    int hc = (D->csw != 0.0), ht = (mu_eff != 0.0);
+    OMP_FOR
+    for (int i = 0; i < nh; i++) {            /* schur_diag_loop, src/dirac.h:325-346 */
+        if (!hc && !ht) {
+            dst_o[2 * i] = diag * src_o[2 * i] - dst_o[2 * i];
+            dst_o[2 * i + 1] = diag * src_o[2 * i + 1] - dst_o[2 * i + 1];
+            continue;
+        }
+        int s = D->odd[i];
+        double C = hc ? 0.5 * D->csw * D->F01[s] : 0.0;
+        ocx d0 = CMPLX(diag + C, mu_eff), d1 = CMPLX(diag - C, -mu_eff);
+        dst_o[2 * i] = d0 * src_o[2 * i] - dst_o[2 * i];
+        dst_o[2 * i + 1] = d1 * src_o[2 * i + 1] - dst_o[2 * i + 1];
+    }
+    free(t1); free(t2);
+}
+
+/* src/dirac.h:422-437 */
+static void schur_dag(const orc_dirac* D, const ocx* src_o, ocx* dst_o) {
+    int nh = D->V / 2, n = nh * 8;
+    ocx* g = xcalloc(2 * (size_t)nh, sizeof(ocx));
+    ocx* h = xcalloc(2 * (size_t)nh, sizeof(ocx));
+    OMP_FOR
+    for (int i = 0; i < nh; i++) { g[2 * i] = src_o[2 * i]; g[2 * i + 1] = -src_o[2 * i + 1]; }
+    schur_impl(D, g, h, -D->mu_t);
+    OMP_FOR
+    for (int i = 0; i < nh; i++) { dst_o[2 * i] = h[2 * i]; dst_o[2 * i + 1] = -h[2 * i + 1]; }
+    free(g); free(h);
+}
+
+void orc_eo_apply(const orc_dirac* D, int kind, const ocx* src, ocx* dst) {
+    int nh = D->V / 2;
+    switch (kind) {
+    case ORC_EO_HOP_EO: eo_hop(D, src, dst, D->odd); break;            /* src/dirac.h:253-255 */
+    case ORC_EO_HOP_OE: eo_hop(D, src, dst, D->even); break;           /* src/dirac.h:258-260 */
+    case ORC_EO_EE_INV: diag_inv(D, src, dst, D->even, D->mu_t); break; /* src/dirac.h:263-265 */
+    case ORC_EO_OO_INV: diag_inv(D, src, dst, D->odd, D->mu_t); break;
+    case ORC_EO_OO: diag_mul(D, src, dst, D->odd, D->mu_t); break;     /* src/dirac.h:394-396 */
+    case ORC_EO_SCHUR: schur_impl(D, src, dst, D->mu_t); break;        /* src/dirac.h:417-419 */
+    case ORC_EO_SCHUR_DAG: schur_dag(D, src, dst); break;
+    case ORC_EO_SCHUR_DAG_SCHUR: {                                      /* src/dirac.h:440-445 */
+        ocx* t = xcalloc(2 * (size_t)nh, sizeof(ocx));
+        schur_impl(D, src, t, D->mu_t);
+        schur_dag(D, t, dst);
+        free(t);
+        break; }
+    default: abort();
+    }
+}
+
+/* src/dirac.h:448-463 */
+static void eo_scatter(const orc_dirac* D, const ocx* he, const ocx* ho, ocx* full) {
+    int nh = D->V / 2;
+    for (int i = 0; i < nh; i++) {
+        int se = D->even[i];
+        full[2 * se] = he[2 * i]; full[2 * se + 1] = he[2 * i + 1];
+    }
+    for (int i = 0; i < nh; i++) {
+        int so = D->odd[i];
+        full[2 * so] = ho[2 * i]; full[2 * so + 1] = ho[2 * i + 1];
+    }
+}
+
+/* src/dirac.h:541-552 */
+void orc_eo_reconstruct(const orc_dirac* D, const ocx* x_o, const ocx* phi, ocx* out) {
+    int nh = D->V / 2;
+    ocx* hop = xcalloc(2 * (size_t)nh, sizeof(ocx));
+    ocx* xe = xcalloc(2 * (size_t)nh, sizeof(ocx));
+    eo_hop(D, x_o, hop, D->even);
+    for (int i = 0; i < nh; i++) {
+        int s = D->even[i];
+        hop[2 * i] = phi[2 * s] - hop[2 * i];
+        hop[2 * i + 1] = phi[2 * s + 1] - hop[2 * i + 1];
+    }
+    diag_inv(D, hop, xe, D->even, D->mu_t);
+    eo_scatter(D, xe, x_o, out);
+    free(hop); free(xe);
+}
+
+/* src/dirac.h:515-523 */
+void orc_eo_reconstruct_full(const orc_dirac* D, const ocx* x_o, ocx* out) {
+    int nh = D->V / 2;
+    ocx* hop = xcalloc(2 * (size_t)nh, sizeof(ocx));
+    ocx* xe = xcalloc(2 * (size_t)nh, sizeof(ocx));
+    eo_hop(D, x_o, hop, D->even);
+    for (int i = 0; i < 2 * nh; i++) hop[i] = -hop[i];
+    diag_inv(D, hop, xe, D->even, D->mu_t);
+    eo_scatter(D, xe, x_o, out);
+    free(hop); free(xe);
+}
+
+/* src/dirac.h:348-366, 528-537 */
+double orc_eo_logdet(const orc_dirac* D) {
+    double diag = 2.0 * D->r + D->mass;
+    int nh = D->V / 2;
+    int hc = (D->csw != 0.0), ht = (D->mu_t != 0.0);
+    if (!hc && !ht) return nh * 2.0 * log(diag);
+    double mu2 = D->mu_t * D->mu_t, acc = 0;
+    for (int i = 0; i < nh; i++) {
+        int s = D->even[i];
+        double C = hc ? 0.5 * D->csw * D->F01[s] : 0.0;
+        double d0 = diag + C, d1 = diag - C;
+        acc += 0.5 * log(d0 * d0 + mu2) + 0.5 * log(d1 * d1 + mu2);
+    }
+    return acc;
+}
+
+/* ================================================================ BLAS-1 */
+/* src/linalg.cpp:8-18 */
+ocx orc_dot(int n, const ocx* a, const ocx* b) {
+    double re = 0, im = 0;
+    #pragma omp parallel for reduction(+:re,im) schedule(static) if(n > OMP_MIN)
+    for (int i = 0; i < n; i++) {
+        ocx p = conj(a[i]) * b[i];
+        re += creal(p); im += cimag(p);
+    }
+    return CMPLX(re, im);
+}
+void orc_dot2(int n, const ocx* a, const ocx* b, double* out) {
+    ocx d = orc_dot(n, a, b);
+    out[0] = creal(d); out[1] = cimag(d);
+}
+/* src/linalg.cpp:20 */
+double orc_norm(int n, const ocx* a) { return sqrt(creal(orc_dot(n, a, a))); }
+/* src/linalg.cpp:22-26 */
+static void axpy(int n, ocx a, const ocx* x, ocx* y) {
+    OMP_FOR
+    for (int i = 0; i < n; i++) y[i] += a * x[i];
+}
+static void vscale(int n, ocx* v, ocx a) {
+    OMP_FOR
+    for (int i = 0; i < n; i++) v[i] *= a;
+}
+
+/* ================================================================ operators */
+struct orc_op {
+    int n;
+    void (*fn)(const orc_op*, const ocx*, ocx*);
+    const void* ctx;
+    int kind;
+};
+static void op_dirac_fn(const orc_op* A, const ocx* s, ocx* d) {
+    const orc_dirac* D = A->ctx;
+    if (A->kind == 100) orc_eo_apply(D, ORC_EO_SCHUR_DAG_SCHUR, s, d);
+    else orc_dirac_apply(D, A->kind, s, d);
+}
+orc_op* orc_op_dirac(const orc_dirac* D, int kind) {
+    orc_op* A = xcalloc(1, sizeof(orc_op));
+    A->n = (kind == 100) ? D->V : 2 * D->V;
+    A->fn = op_dirac_fn; A->ctx = D; A->kind = kind;
+    return A;
+}
+void orc_op_apply(const orc_op* A, const ocx* src, ocx* dst) { A->fn(A, src, dst); }
+int orc_op_dim(const orc_op* A) { return A->n; }
+void orc_op_free(orc_op* A) { free(A); }
+
+/* ================================================================ prolongator */
+/* src/prolongator.h:14-142 (geometric, nd = 2 fine dof per site) and
+ * src/prolongator.h:149-228 (linear blocking of block_k consecutive coarse dofs).
+ * The linear blocking of 4*k consecutive dofs on a coarse lattice whose x extent
+ * is a multiple of 4 is the geometric 4x1 aggregation with nd = k, so one
+ * geometric implementation with nd dof per site restates both. */
+struct orc_prolong {
+    int nx, ny, nd, bx, by, k, nbx, nby, nblocks, bsites, bsize;
+    ocx* bv;          /* [nblocks][k][bsize], bsize = bx*by*nd */
+    int* s2b; int* s2l;
+    int* blist;       /* [nblocks][bsites] ascending site index */
+};
+
+orc_prolong* orc_prolong_new(int nx, int ny, int nd, int bx, int by, int k) {
+    orc_prolong* P = xcalloc(1, sizeof(orc_prolong));
+    P->nx = nx; P->ny = ny; P->nd = nd; P->bx = bx; P->by = by; P->k = k;
+    P->nbx = nx / bx; P->nby = ny / by; P->nblocks = P->nbx * P->nby;
+    P->bsites = bx * by; P->bsize = bx * by * nd;
+    P->bv = xcalloc((size_t)P->nblocks * k * P->bsize, sizeof(ocx));
+    int V = nx * ny;
+    P->s2b = xcalloc(V, sizeof(int)); P->s2l = xcalloc(V, sizeof(int));
+    P->blist = xcalloc((size_t)V, sizeof(int));
+    int* fill = xcalloc(P->nblocks, sizeof(int));
+    for (int s = 0; s < V; s++) {                 /* src/prolongator.h:50-57 */
+        int x = s % nx, y = s / nx;
+        int b = x / bx + P->nbx * (y / by);
+        P->s2b[s] = b; P->s2l[s] = x % bx + bx * (y % by);
+        P->blist[b * P->bsites + fill[b]++] = s;
+    }
+    free(fill);
+    return P;
+}
+void orc_prolong_free(orc_prolong* P) {
+    if (!P) return;
+    free(P->bv); free(P->s2b); free(P->s2l); free(P->blist); free(P);
+}
+int orc_prolong_coarse_dim(const orc_prolong* P) { return P->nblocks * P->k; }
+void orc_prolong_get(const orc_prolong* P, ocx* bv) {
+    memcpy(bv, P->bv, (size_t)P->nblocks * P->k * P->bsize * sizeof(ocx));
+}
+void orc_prolong_set(orc_prolong* P, const ocx* bv) {
+    memcpy(P->bv, bv, (size_t)P->nblocks * P->k * P->bsize * sizeof(ocx));
+}
+
+/* src/prolongator.h:111-130 */
+static void orthonormalise_block(orc_prolong* P, int b) {
+    int bd = P->bsize;
+    ocx* base = P->bv + (size_t)b * P->k * bd;
+    for (int k = 0; k < P->k; k++) {
+        ocx* v = base + (size_t)k * bd;
+        for (int j = 0; j < k; j++) {
+            const ocx* w = base + (size_t)j * bd;
+            ocx proj = 0;
+            for (int i = 0; i < bd; i++) proj += conj(w[i]) * v[i];
+            for (int i = 0; i < bd; i++) v[i] -= proj * w[i];
+        }
+        double nn = 0;
+        for (int i = 0; i < bd; i++) nn += creal(v[i]) * creal(v[i]) + cimag(v[i]) * cimag(v[i]);
+        nn = sqrt(nn);
+        if (nn > 1e-14)
+            for (int i = 0; i < bd; i++) v[i] /= nn;
+    }
+}
+
+/* src/prolongator.h:63-74, 133-141 */
+void orc_prolong_build(orc_prolong* P, const ocx* null_vecs) {
+    int V = P->nx * P->ny, nd = P->nd;
+    size_t fd = (size_t)V * nd;
+    for (int b = 0; b < P->nblocks; b++) {
+        for (int k = 0; k < P->k; k++) {
+            ocx* out = P->bv + ((size_t)b * P->k + k) * P->bsize;
+            for (int t = 0; t < P->bsites; t++) {
+                int s = P->blist[b * P->bsites + t];
+                int loc = P->s2l[s];
+                for (int c = 0; c < nd; c++) out[loc * nd + c] = null_vecs[k * fd + (size_t)s * nd + c];
+            }
+        }
+        orthonormalise_block(P, b);
+    }
+}
+
+/* src/prolongator.h:77-90 */
+void orc_prolong_apply(const orc_prolong* P, const ocx* coarse, ocx* fine) {
+    int V = P->nx * P->ny, n = V, nd = P->nd;
+    OMP_FOR
+    for (int s = 0; s < V; s++) {
+        int b = P->s2b[s], loc = P->s2l[s];
+        for (int c = 0; c < nd; c++) fine[(size_t)s * nd + c] = 0.0;
+        for (int k = 0; k < P->k; k++) {
+            ocx cc = coarse[b * P->k + k];
+            const ocx* v = P->bv + ((size_t)b * P->k + k) * P->bsize + loc * nd;
+            for (int c = 0; c < nd; c++) fine[(size_t)s * nd + c] += cc * v[c];
+        }
+    }
+}
+
+/* src/prolongator.h:94-108 */
+void orc_restrict_apply(const orc_prolong* P, const ocx* fine, ocx* coarse) {
+    int nb = P->nblocks, n = nb * 16, nd = P->nd;
+    OMP_FOR
+    for (int b = 0; b < nb; b++) {
+        for (int k = 0; k < P->k; k++) coarse[b * P->k + k] = 0.0;
+        for (int t = 0; t < P->bsites; t++) {
+            int s = P->blist[b * P->bsites + t];
+            int loc = P->s2l[s];
+            for (int k = 0; k < P->k; k++) {
+                const ocx* v = P->bv + ((size_t)b * P->k + k) * P->bsize + loc * nd;
+                ocx acc = conj(v[0]) * fine[(size_t)s * nd];
+                for (int c = 1; c < nd; c++) acc += conj(v[c]) * fine[(size_t)s * nd + c];
+                coarse[b * P->k + k] += acc;
+            }
+        }
+    }
+}
+
+/* ================================================================ coarse op */
+struct orc_coarse {
+    int nx, ny, k, nb;
+    ocx* link;   /* [nb][9][k*k] row-major k x k blocks */
+};
+
+/* src/coarse_op.h:205-222 */
+static inline int cnb(int nbx, int nby, int b, int dir) {
+    static const int DX[9] = {0, 1, -1, 0, 0, 1, 1, -1, -1};
+    static const int DY[9] = {0, 0, 0, 1, -1, 1, -1, 1, -1};
+    int bxi = b % nbx, byi = b / nbx;
+    int nx = (bxi + DX[dir] + nbx) % nbx, ny = (byi + DY[dir] + nby) % nby;
+    return nx + ny * nbx;
+}
+/* src/coarse_op.h:225-229 */
+static const int REVDIR[9] = {0, 2, 1, 4, 3, 8, 7, 6, 5};
+
+static int color_period(int nb) {
+    if (nb < 3) return nb;
+    for (int c = 3; c <= nb; c++) if (nb % c == 0) return c;
+    return nb;
+}
+
+/* src/coarse_op.h:241-279 (colored = 0); colored = 1 probes every aggregate of
+ * one colour class at once. */
+orc_coarse* orc_coarse_build(const orc_op* A, const orc_prolong* P, int colored) {
+    orc_coarse* C = xcalloc(1, sizeof(orc_coarse));
+    int k = P->k, nbx = P->nbx, nby = P->nby, nb = P->nblocks, dim = nb * k;
+    C->nx = nbx; C->ny = nby; C->k = k; C->nb = nb;
+    C->link = xcalloc((size_t)nb * 9 * k * k, sizeof(ocx));
+    int fdim = A->n;
+    ocx* ej = xcalloc(dim, sizeof(ocx));
+    ocx* fine = xcalloc(fdim, sizeof(ocx));
+    ocx* Afine = xcalloc(fdim, sizeof(ocx));
+    ocx* col = xcalloc(dim, sizeof(ocx));
+    int cx_ = colored ? color_period(nbx) : nbx, cy_ = colored ? color_period(nby) : nby;
+    int nclass = colored ? cx_ * cy_ : nb;
+    for (int cls = 0; cls < nclass; cls++) {
+        for (int kj = 0; kj < k; kj++) {
+            memset(ej, 0, dim * sizeof(ocx));
+            for (int J = 0; J < nb; J++) {
+                int in_class = colored ? ((J % nbx) % cx_ == cls % cx_ && (J / nbx) % cy_ == cls / cx_) : (J == cls);
+                if (in_class) ej[J * k + kj] = 1.0;
+            }
+            orc_prolong_apply(P, ej, fine);
+            A->fn(A, fine, Afine);
+            orc_restrict_apply(P, Afine, col);
+            for (int J = 0; J < nb; J++) {
+                int in_class = colored ? ((J % nbx) % cx_ == cls % cx_ && (J / nbx) % cy_ == cls / cx_) : (J == cls);
+                if (!in_class) continue;
+                for (int d = 0; d < 9; d++) {
+                    int Ib = cnb(nbx, nby, J, d);
+                    int rd = REVDIR[d];
+                    /* On a coarse lattice with fewer than 3 aggregates along a
+                     * direction, several stencil directions reach the same
+                     * aggregate; the coupling is stored once, in the first of
+                     * them, so that the stencil equals the dense Galerkin
+                     * operator of src/coarse_op.h:26-43.  (The reference's
+                     * SparseCoarseOp::build stores it in every such direction.) */
+                    int dup = 0;
+                    for (int d2 = 0; d2 < d; d2++) if (cnb(nbx, nby, J, d2) == Ib) dup = 1;
+                    if (dup) continue;
+                    for (int ki = 0; ki < k; ki++)
+                        C->link[((size_t)Ib * 9 + rd) * k * k + ki * k + kj] = col[Ib * k + ki];
+                }
+            }
+        }
+    }
+    free(ej); free(fine); free(Afine); free(col);
+    return C;
+}
+void orc_coarse_free(orc_coarse* C) { if (C) { free(C->link); free(C); } }
+void orc_coarse_get(const orc_coarse* C, ocx* link) { memcpy(link, C->link, (size_t)C->nb * 9 * C->k * C->k * sizeof(ocx)); }
+void orc_coarse_set(orc_coarse* C, const ocx* link) { memcpy(C->link, link, (size_t)C->nb * 9 * C->k * C->k * sizeof(ocx)); }
+
+/* src/coarse_op.h:282-297 */
+static void coarse_apply(const orc_coarse* C, const ocx* src, ocx* dst) {
+    int nb = C->nb, k = C->k, n = nb * 16;
+    OMP_FOR
+    for (int b = 0; b < nb; b++) {
+        for (int ki = 0; ki < k; ki++) dst[b * k + ki] = 0.0;
+        for (int d = 0; d < 9; d++) {
+            int nbb = cnb(C->nx, C->ny, b, d);
+            const ocx* Lk = C->link + ((size_t)b * 9 + d) * k * k;
+            for (int ki = 0; ki < k; ki++) {
+                ocx sum = 0.0;
+                for (int kj = 0; kj < k; kj++) sum += Lk[ki * k + kj] * src[nbb * k + kj];
+                dst[b * k + ki] += sum;
+            }
+        }
+    }
+}
+static void op_coarse_fn(const orc_op* A, const ocx* s, ocx* d) { coarse_apply(A->ctx, s, d); }
+orc_op* orc_op_coarse(const orc_coarse* C) {
+    orc_op* A = xcalloc(1, sizeof(orc_op));
+    A->n = C->nb * C->k; A->fn = op_coarse_fn; A->ctx = C; A->kind = 200;
+    return A;
+}
+
+/* ---------------- dense coarse op, src/coarse_op.h:18-137 ---------------- */
+struct orc_dense { int dim; ocx* mat; /* column-major: mat[j*dim + i] */ };
+
+/* src/coarse_op.h:26-37.  P == NULL: the dense matrix of A itself. */
+orc_dense* orc_dense_build(const orc_op* A, const orc_prolong* P) {
+    orc_dense* M = xcalloc(1, sizeof(orc_dense));
+    int dim = P ? orc_prolong_coarse_dim(P) : A->n, fdim = A->n;
+    M->dim = dim;
+    M->mat = xcalloc((size_t)dim * dim, sizeof(ocx));
+    ocx* ej = xcalloc(dim, sizeof(ocx));
+    ocx* fine = xcalloc(fdim, sizeof(ocx));
+    ocx* Af = xcalloc(fdim, sizeof(ocx));
+    for (int j = 0; j < dim; j++) {
+        memset(ej, 0, dim * sizeof(ocx));
+        ej[j] = 1.0;
+        if (P) {
+            orc_prolong_apply(P, ej, fine);
+            A->fn(A, fine, Af);
+            orc_restrict_apply(P, Af, M->mat + (size_t)j * dim);
+        } else {
+            A->fn(A, ej, M->mat + (size_t)j * dim);
+        }
+    }
+    free(ej); free(fine); free(Af);
+    return M;
+}
+void orc_dense_free(orc_dense* M) { if (M) { free(M->mat); free(M); } }
+void orc_dense_get(const orc_dense* M, ocx* cols) { memcpy(cols, M->mat, (size_t)M->dim * M->dim * sizeof(ocx)); }
+/* src/coarse_op.h:46-54 */
+void orc_dense_apply(const orc_dense* M, const ocx* x, ocx* y) {
+    int dim = M->dim;
+    for (int i = 0; i < dim; i++) y[i] = 0.0;
+    for (int j = 0; j < dim; j++) {
+        ocx xj = x[j];
+        for (int i = 0; i < dim; i++) y[i] += M->mat[(size_t)j * dim + i] * xj;
+    }
+}
+/* src/coarse_op.h:101-137 */
+void orc_dense_solve(const orc_dense* M, const ocx* b, ocx* x) {
+    int n = M->dim;
+    ocx** A = xcalloc(n, sizeof(ocx*));
+    for (int i = 0; i < n; i++) {
+        A[i] = xcalloc(n + 1, sizeof(ocx));
+        for (int j = 0; j < n; j++) A[i][j] = M->mat[(size_t)j * n + i];
+        A[i][n] = b[i];
+    }
+    for (int k = 0; k < n; k++) {
+        double best = 0; int br = k;
+        for (int i = k; i < n; i++) { double v = cabs(A[i][k]); if (v > best) { best = v; br = i; } }
+        ocx* t = A[k]; A[k] = A[br]; A[br] = t;
+        if (best < 1e-30) continue;
+        for (int i = k + 1; i < n; i++) {
+            ocx f = A[i][k] / A[k][k];
+            for (int j = k; j <= n; j++) A[i][j] -= f * A[k][j];
+        }
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        ocx s = A[i][n];
+        for (int j = i + 1; j < n; j++) s -= A[i][j] * x[j];
+        x[i] = (cabs(A[i][i]) > 1e-30) ? s / A[i][i] : 0.0;
+    }
+    for (int i = 0; i < n; i++) free(A[i]);
+    free(A);
+}
+
+/* ================================================================ smoothers */
+/* src/smoother.cpp:26-40 (and :9-23, the same loop with A = D^dag D) */
+void orc_mr_smooth(const orc_op* A, ocx* x, const ocx* b, int niter) {
+    int n = A->n;
+    ocx* r = xcalloc(n, sizeof(ocx));
+    ocx* Ar = xcalloc(n, sizeof(ocx));
+    for (int it = 0; it < niter; it++) {
+        A->fn(A, x, r);
+        OMP_FOR
+        for (int i = 0; i < n; i++) r[i] = b[i] - r[i];
+        A->fn(A, r, Ar);
+        ocx rAr = orc_dot(n, Ar, r);
+        ocx ArAr = orc_dot(n, Ar, Ar);
+        if (cabs(ArAr) < 1e-30) break;
+        ocx omega = rAr / ArAr;
+        axpy(n, omega, r, x);
+    }
+    free(r); free(Ar);
+}
+/* src/smoother.cpp:63-73 */
+void orc_richardson(const orc_op* A, ocx* x, const ocx* b, int niter, double omega) {
+    int n = A->n;
+    ocx* r = xcalloc(n, sizeof(ocx));
+    for (int it = 0; it < niter; it++) {
+        A->fn(A, x, r);
+        OMP_FOR
+        for (int i = 0; i < n; i++) x[i] += omega * (b[i] - r[i]);
+    }
+    free(r);
+}
+/* src/smoother.cpp:43-59 with the start vector supplied explicitly */
+double orc_lambda_max(const orc_op* A, const ocx* v0, int niter) {
+    int n = A->n;
+    ocx* v = xcalloc(n, sizeof(ocx));
+    ocx* Av = xcalloc(n, sizeof(ocx));
+    memcpy(v, v0, n * sizeof(ocx));
+    double vn = orc_norm(n, v);
+    vscale(n, v, 1.0 / vn);
+    double lambda = 0;
+    for (int i = 0; i < niter; i++) {
+        A->fn(A, v, Av);
+        lambda = orc_norm(n, Av);
+        if (lambda < 1e-30) break;
+        vscale(n, Av, 1.0 / lambda);
+        memcpy(v, Av, n * sizeof(ocx));
+    }
+    free(v); free(Av);
+    return lambda;
+}
+
+/* src/multigrid.cpp:545-587 (and :435-484, identical with A = D^dag D), all k
+ * start vectors supplied explicitly (the reference's warm_start argument). */
+void orc_null_space(const orc_op* A, int k, int outer_iters, const ocx* starts, ocx* out) {
+    int n = A->n;
+    ocx* xnew = xcalloc(n, sizeof(ocx));
+    ocx* rhs = xcalloc(n, sizeof(ocx));
+    for (int i = 0; i < k; i++) {
+        ocx* v = out + (size_t)i * n;
+        memcpy(v, starts + (size_t)i * n, n * sizeof(ocx));
+        double nv = orc_norm(n, v);
+        vscale(n, v, 1.0 / nv);
+        for (int it = 0; it < outer_iters; it++) {
+            memcpy(rhs, v, n * sizeof(ocx));
+            memset(xnew, 0, n * sizeof(ocx));
+            orc_mr_smooth(A, xnew, rhs, 5);
+            for (int j = 0; j < i; j++) {
+                ocx proj = orc_dot(n, out + (size_t)j * n, xnew);
+                axpy(n, -proj, out + (size_t)j * n, xnew);
+            }
+            nv = orc_norm(n, xnew);
+            if (nv > 1e-14) { vscale(n, xnew, 1.0 / nv); memcpy(v, xnew, n * sizeof(ocx)); }
+            else break;
+        }
+    }
+    for (int i = 0; i < k; i++) {
+        ocx* v = out + (size_t)i * n;
+        for (int j = 0; j < i; j++) {
+            ocx proj = orc_dot(n, out + (size_t)j * n, v);
+            axpy(n, -proj, out + (size_t)j * n, v);
+        }
+        double nv = orc_norm(n, v);
+        if (nv > 1e-14) vscale(n, v, 1.0 / nv);
+    }
+    free(xnew); free(rhs);
+}
+
+/* ================================================================ multigrid */
+#define MAXLEV 5
+struct orc_mg {
+    orc_mg_params prm;
+    int nlev;
+    const orc_op* op[MAXLEV];
+    orc_op* own_op[MAXLEV];
+    int dim[MAXLEV];
+    orc_prolong* P[MAXLEV];
+    orc_coarse* C[MAXLEV];
+    ocx* nullv[MAXLEV];
+    double omega[MAXLEV];
+    orc_dense* dense;
+};
+
+/* src/multigrid.cpp:595-687, with geometric aggregation at every level and the
+ * 9-point operator (src/coarse_op.h:154-297) as the coarse operator. */
+orc_mg* orc_mg_new(const orc_op* fine, int nx, int ny, int nd, const orc_mg_params* prm, const ocx* starts) {
+    orc_mg* mg = xcalloc(1, sizeof(orc_mg));
+    mg->prm = *prm;
+    mg->nlev = prm->n_levels;
+    mg->op[0] = fine;
+    mg->dim[0] = fine->n;
+    int cnx = nx, cny = ny, cnd = nd;
+    size_t soff = 0;
+    for (int l = 0; l + 1 < mg->nlev; l++) {
+        int k = prm->k[l], n = mg->dim[l];
+        mg->nullv[l] = xcalloc((size_t)k * n, sizeof(ocx));
+        orc_null_space(mg->op[l], k, prm->null_iters, starts + soff, mg->nullv[l]);
+        soff += (size_t)k * n;
+        mg->P[l] = orc_prolong_new(cnx, cny, cnd, prm->bx[l], prm->by[l], k);
+        orc_prolong_build(mg->P[l], mg->nullv[l]);
+        mg->C[l] = orc_coarse_build(mg->op[l], mg->P[l], prm->colored_build);
+        mg->own_op[l + 1] = orc_op_coarse(mg->C[l]);
+        mg->op[l + 1] = mg->own_op[l + 1];
+        mg->dim[l + 1] = mg->P[l]->nblocks * k;
+        cnx = mg->P[l]->nbx; cny = mg->P[l]->nby; cnd = k;
+    }
+    if (prm->coarse_solver == 1 && mg->nlev >= 2)
+        mg->dense = orc_dense_build(mg->op[mg->nlev - 1], NULL);
+    return mg;
+}
+void orc_mg_free(orc_mg* mg) {
+    if (!mg) return;
+    for (int l = 0; l < MAXLEV; l++) {
+        orc_prolong_free(mg->P[l]); orc_coarse_free(mg->C[l]);
+        if (mg->own_op[l]) orc_op_free(mg->own_op[l]);
+        free(mg->nullv[l]);
+    }
+    orc_dense_free(mg->dense);
+    free(mg);
+}
+const orc_prolong* orc_mg_prolong(const orc_mg* mg, int l) { return mg->P[l]; }
+const orc_coarse* orc_mg_coarse(const orc_mg* mg, int l) { return mg->C[l]; }
+double orc_mg_omega(const orc_mg* mg, int l) { return mg->omega[l]; }
+void orc_mg_null_vecs(const orc_mg* mg, int l, ocx* out) {
+    memcpy(out, mg->nullv[l], (size_t)mg->prm.k[l] * mg->dim[l] * sizeof(ocx));
+}
+
+/* src/multigrid.cpp:146-152 with explicit power-iteration starts per level */
+void orc_mg_set_symmetric(orc_mg* mg, double damping, const ocx* starts) {
+    size_t off = 0;
+    for (int l = 0; l + 1 < mg->nlev; l++) {
+        double lmax = orc_lambda_max(mg->op[l], starts + off, 30);
+        mg->omega[l] = damping / lmax;
+        off += mg->dim[l];
+    }
+}
+
+/* src/coarse_op.h:492-547 without deflation vectors (x0 = 0) */
+int orc_coarse_cg(const orc_op* A, const ocx* b, int maxit, double tol, ocx* x) {
+    int n = A->n;
+    memset(x, 0, n * sizeof(ocx));
+    double bnorm = orc_norm(n, b);
+    if (bnorm < 1e-30) return 0;
+    ocx* Ax = xcalloc(n, sizeof(ocx));
+    ocx* r = xcalloc(n, sizeof(ocx));
+    ocx* p = xcalloc(n, sizeof(ocx));
+    ocx* Ap = xcalloc(n, sizeof(ocx));
+    A->fn(A, x, Ax);
+    for (int i = 0; i < n; i++) r[i] = b[i] - Ax[i];
+    memcpy(p, r, n * sizeof(ocx));
+    ocx rr = orc_dot(n, r, r);
+    int iter = 0;
+    while (iter < maxit) {
+        A->fn(A, p, Ap);
+        ocx pAp = orc_dot(n, p, Ap);
+        if (cabs(pAp) < 1e-30) break;
+        ocx alpha = rr / pAp;
+        for (int i = 0; i < n; i++) { x[i] += alpha * p[i]; r[i] -= alpha * Ap[i]; }
+        iter++;
+        double rn = orc_norm(n, r);
+        if (rn / bnorm < tol) break;
+        if (isnan(rn) || isinf(rn)) break;
+        ocx rn2 = orc_dot(n, r, r);
+        ocx beta = rn2 / rr;
+        rr = rn2;
+        for (int i = 0; i < n; i++) p[i] = r[i] + beta * p[i];
+    }
+    free(Ax); free(r); free(p); free(Ap);
+    return iter;
+}
+
+/* src/multigrid.cpp:43-102 */
+static void mg_cycle(orc_mg* mg, int lev, const ocx* b, ocx* x) {
+    int n = mg->dim[lev], last = mg->nlev - 1;
+    const orc_op* A = mg->op[lev];
+    if (lev == last) {
+        if (mg->dense) orc_dense_solve(mg->dense, b, x);
+        else orc_coarse_cg(A, b, mg->prm.coarse_maxit, mg->prm.coarse_tol, x);
+        return;
+    }
+    memset(x, 0, n * sizeof(ocx));
+    if (mg->omega[lev] > 0) orc_richardson(A, x, b, mg->prm.pre_smooth, mg->omega[lev]);
+    else orc_mr_smooth(A, x, b, mg->prm.pre_smooth);
+    ocx* r = xcalloc(n, sizeof(ocx));
+    A->fn(A, x, r);
+    OMP_FOR
+    for (int i = 0; i < n; i++) r[i] = b[i] - r[i];
+    int nc = mg->dim[lev + 1];
+    ocx* rc = xcalloc(nc, sizeof(ocx));
+    ocx* ec = xcalloc(nc, sizeof(ocx));
+    orc_restrict_apply(mg->P[lev], r, rc);
+    mg_cycle(mg, lev + 1, rc, ec);
+    if (mg->prm.w_cycle && lev + 1 < last) {
+        ocx* t = xcalloc(nc, sizeof(ocx));
+        ocx* ec2 = xcalloc(nc, sizeof(ocx));
+        mg->op[lev + 1]->fn(mg->op[lev + 1], ec, t);
+        for (int i = 0; i < nc; i++) t[i] = rc[i] - t[i];
+        mg_cycle(mg, lev + 1, t, ec2);
+        for (int i = 0; i < nc; i++) ec[i] += ec2[i];
+        free(t); free(ec2);
+    }
+    ocx* e = r;   /* reuse */
+    orc_prolong_apply(mg->P[lev], ec, e);
+    OMP_FOR
+    for (int i = 0; i < n; i++) x[i] += e[i];
+    if (mg->omega[lev] > 0) orc_richardson(A, x, b, mg->prm.post_smooth, mg->omega[lev]);
+    else orc_mr_smooth(A, x, b, mg->prm.post_smooth);
+    free(r); free(rc); free(ec);
+}
+void orc_mg_precondition(orc_mg* mg, const ocx* b, ocx* x) { mg_cycle(mg, 0, b, x); }
+
+static void precond(orc_mg* mg, int n, const ocx* r, ocx* z) {
+    if (mg) orc_mg_precondition(mg, r, z);
+    else memcpy(z, r, n * sizeof(ocx));
+}
+
+/* ================================================================ Krylov */
+/* src/solvers.cpp:16-72 */
+int orc_cg(const orc_op* A, const ocx* rhs, const ocx* x0, orc_mg* mg, int maxit, double tol,
+           ocx* x, double* final_res) {
+    int n = A->n;
+    double rhs_norm = orc_norm(n, rhs);
+    if (rhs_norm < 1e-30) { memset(x, 0, n * sizeof(ocx)); *final_res = 0; return 0; }
+    ocx* r = xcalloc(n, sizeof(ocx));
+    ocx* z = xcalloc(n, sizeof(ocx));
+    ocx* p = xcalloc(n, sizeof(ocx));
+    ocx* Ap = xcalloc(n, sizeof(ocx));
+    if (x0) {
+        memcpy(x, x0, n * sizeof(ocx));
+        A->fn(A, x, Ap);
+        OMP_FOR
+        for (int i = 0; i < n; i++) r[i] = rhs[i] - Ap[i];
+    } else {
+        memset(x, 0, n * sizeof(ocx));
+        memcpy(r, rhs, n * sizeof(ocx));
+    }
+    precond(mg, n, r, z);
+    memcpy(p, z, n * sizeof(ocx));
+    ocx rz = orc_dot(n, r, z);
+    int iter = 0;
+    while (iter < maxit) {
+        A->fn(A, p, Ap);
+        ocx pAp = orc_dot(n, p, Ap);
+        ocx alpha = rz / pAp;
+        OMP_FOR
+        for (int i = 0; i < n; i++) { x[i] += alpha * p[i]; r[i] -= alpha * Ap[i]; }
+        iter++;
+        double rn = orc_norm(n, r);
+        if (rn / rhs_norm < tol) break;
+        precond(mg, n, r, z);
+        ocx rz_new = orc_dot(n, r, z);
+        ocx beta = rz_new / rz;
+        rz = rz_new;
+        OMP_FOR
+        for (int i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+    }
+    *final_res = orc_norm(n, r) / rhs_norm;
+    free(r); free(z); free(p); free(Ap);
+    return iter;
+}
+
+/* src/solvers.cpp:223-300 (truncated GCR, sliding window of trunc directions) */
+int orc_fcg(const orc_op* A, const ocx* rhs, orc_mg* mg, int maxit, double tol, int trunc,
+            int ref_projection, ocx* x, double* final_res) {
+    int n = A->n;
+    double rhs_norm = orc_norm(n, rhs);
+    memset(x, 0, n * sizeof(ocx));
+    if (rhs_norm < 1e-30) { *final_res = 0; return 0; }
+    ocx* r = xcalloc(n, sizeof(ocx));
+    memcpy(r, rhs, n * sizeof(ocx));
+    int cap = trunc + 1;
+    ocx** Pd = xcalloc(cap, sizeof(ocx*));
+    ocx** APd = xcalloc(cap, sizeof(ocx*));
+    double* ApAp = xcalloc(cap, sizeof(double));
+    for (int i = 0; i < cap; i++) { Pd[i] = xcalloc(n, sizeof(ocx)); APd[i] = xcalloc(n, sizeof(ocx)); }
+    ocx* z = xcalloc(n, sizeof(ocx));
+    ocx* Az = xcalloc(n, sizeof(ocx));
+    int nd = 0, iter = 0;
+    while (iter < maxit) {
+        double rn = orc_norm(n, r);
+        if (rn / rhs_norm < tol) break;
+        precond(mg, n, r, z);
+        A->fn(A, z, Az);
+        ocx* p = Pd[nd];
+        ocx* Ap = APd[nd];
+        memcpy(p, z, n * sizeof(ocx));
+        memcpy(Ap, Az, n * sizeof(ocx));
+        for (int i = 0; i < nd; i++) {
+            /* src/solvers.cpp:261 computes dot(Az, AP_i) = <Az, AP_i>, the complex
+             * conjugate of the GCR projection coefficient <AP_i, Az>; ref_projection
+             * selects that form, otherwise the textbook coefficient is used. */
+            ocx beta = (ref_projection ? orc_dot(n, Az, APd[i]) : orc_dot(n, APd[i], Az)) / ApAp[i];
+            OMP_FOR
+            for (int j = 0; j < n; j++) { p[j] -= beta * Pd[i][j]; Ap[j] -= beta * APd[i][j]; }
+        }
+        double apap = creal(orc_dot(n, Ap, Ap));
+        if (apap < 1e-30) { nd = 0; continue; }
+        ocx alpha = orc_dot(n, Ap, r) / apap;
+        OMP_FOR
+        for (int j = 0; j < n; j++) { x[j] += alpha * p[j]; r[j] -= alpha * Ap[j]; }
+        iter++;
+        ApAp[nd] = apap;
+        nd++;
+        if (nd > trunc) {   /* erase the oldest direction (the reference erases before push) */
+            ocx* t0 = Pd[0]; ocx* t1 = APd[0];
+            for (int i = 0; i < nd - 1; i++) { Pd[i] = Pd[i + 1]; APd[i] = APd[i + 1]; ApAp[i] = ApAp[i + 1]; }
+            Pd[nd - 1] = t0; APd[nd - 1] = t1;
+            nd--;
+        }
+    }
+    *final_res = orc_norm(n, r) / rhs_norm;
+    for (int i = 0; i < cap; i++) { free(Pd[i]); free(APd[i]); }
+    free(Pd); free(APd); free(ApAp); free(z); free(Az); free(r);
+    return iter;
+}
+
+/* src/solvers.cpp:555-772 with n_ritz_harvest = 0 (no Ritz harvest). */
+int orc_fgmres(const orc_op* A, const ocx* rhs, orc_mg* mg, int restart, int maxit, double tol,
+               ocx* x, double* final_res) {
+    int n = A->n;
+    double rhs_norm = orc_norm(n, rhs);
+    memset(x, 0, n * sizeof(ocx));
+    if (rhs_norm < 1e-30) { *final_res = 0; return 0; }
+    int total = 0;
+    ocx* Ax = xcalloc(n, sizeof(ocx));
+    ocx* w = xcalloc(n, sizeof(ocx));
+    int m0 = restart;
+    ocx** V = xcalloc(m0 + 1, sizeof(ocx*));
+    ocx** Z = xcalloc(m0, sizeof(ocx*));
+    for (int i = 0; i <= m0; i++) V[i] = xcalloc(n, sizeof(ocx));
+    for (int i = 0; i < m0; i++) Z[i] = xcalloc(n, sizeof(ocx));
+    ocx* H = xcalloc((size_t)(m0 + 1) * m0, sizeof(ocx));   /* H[i*m0 + j] */
+    ocx* g = xcalloc(m0 + 1, sizeof(ocx));
+    ocx* cs = xcalloc(m0, sizeof(ocx));
+    ocx* sn = xcalloc(m0, sizeof(ocx));
+    ocx* y = xcalloc(m0, sizeof(ocx));
+    while (total < maxit) {
+        A->fn(A, x, Ax);
+        for (int i = 0; i < n; i++) V[0][i] = rhs[i] - Ax[i];
+        double beta = orc_norm(n, V[0]);
+        if (beta / rhs_norm < tol) break;
+        int m = restart < maxit - total ? restart : maxit - total;
+        vscale(n, V[0], 1.0 / beta);
+        memset(H, 0, (size_t)(m0 + 1) * m0 * sizeof(ocx));
+        memset(g, 0, (m0 + 1) * sizeof(ocx));
+        g[0] = beta;
+        int k;
+        for (k = 0; k < m; k++) {
+            precond(mg, n, V[k], Z[k]);
+            A->fn(A, Z[k], w);
+            for (int j = 0; j <= k; j++) {
+                H[j * m0 + k] = orc_dot(n, V[j], w);
+                axpy(n, -H[j * m0 + k], V[j], w);
+            }
+            for (int j = 0; j <= k; j++) {
+                ocx c = orc_dot(n, V[j], w);
+                H[j * m0 + k] += c;
+                axpy(n, -c, V[j], w);
+            }
+            H[(k + 1) * m0 + k] = orc_norm(n, w);
+            if (cabs(H[(k + 1) * m0 + k]) < 1e-14) { k++; break; }
+            memcpy(V[k + 1], w, n * sizeof(ocx));
+            vscale(n, V[k + 1], 1.0 / H[(k + 1) * m0 + k]);
+            for (int j = 0; j < k; j++) {
+                ocx hj = H[j * m0 + k], hj1 = H[(j + 1) * m0 + k];
+                H[j * m0 + k] = conj(cs[j]) * hj + conj(sn[j]) * hj1;
+                H[(j + 1) * m0 + k] = -sn[j] * hj + cs[j] * hj1;
+            }
+            double a = cabs(H[k * m0 + k]), bb = cabs(H[(k + 1) * m0 + k]);
+            double den = sqrt(a * a + bb * bb);
+            cs[k] = H[k * m0 + k] / den;
+            sn[k] = H[(k + 1) * m0 + k] / den;
+            g[k + 1] = -sn[k] * g[k];
+            g[k] = conj(cs[k]) * g[k];
+            H[k * m0 + k] = den;
+            H[(k + 1) * m0 + k] = 0.0;
+            total++;
+            if (cabs(g[k + 1]) / rhs_norm < tol) { k++; break; }
+        }
+        int kk = k;
+        for (int i = kk - 1; i >= 0; i--) {
+            ocx s = g[i];
+            for (int j = i + 1; j < kk; j++) s -= H[i * m0 + j] * y[j];
+            y[i] = s / H[i * m0 + i];
+        }
+        for (int j = 0; j < kk; j++) axpy(n, y[j], Z[j], x);
+        if (cabs(g[kk]) / rhs_norm < tol) break;
+    }
+    A->fn(A, x, Ax);
+    double res = 0;
+    for (int i = 0; i < n; i++) { ocx d = rhs[i] - Ax[i]; res += creal(d) * creal(d) + cimag(d) * cimag(d); }
+    *final_res = sqrt(res) / rhs_norm;
+    for (int i = 0; i <= m0; i++) free(V[i]);
+    for (int i = 0; i < m0; i++) free(Z[i]);
+    free(V); free(Z); free(H); free(g); free(cs); free(sn); free(y); free(Ax); free(w);
+    return total;
+}
+
+/* ================================================================ forces */
+/* src/hmc_forces.cpp:540-569 */
+void orc_hopping_force(const orc_dirac* D, const ocx* chi, const ocx* X, double* F) {
+    int L = D->L, V = D->V, n = V;
+    const double r = D->r;
+    const ocx* U0 = D->U; const ocx* U1 = D->U + V;
+    const ocx ii = I;
+    OMP_FOR
+    for (int s = 0; s < V; s++) {
+        ocx c0 = chi[2 * s], c1 = chi[2 * s + 1], x0 = X[2 * s], x1 = X[2 * s + 1];
+        { int sf = xp(L, s);
+          ocx u = U0[s], ud = conj(u);
+          ocx ux0 = u * X[2 * sf], ux1 = u * X[2 * sf + 1];
+          ocx f0 = r * ux0 - ux1, f1 = -ux0 + r * ux1;
+          ocx c_fwd = conj(c0) * f0 + conj(c1) * f1;
+          ocx udx0 = ud * x0, udx1 = ud * x1;
+          ocx b0 = r * udx0 + udx1, b1 = udx0 + r * udx1;
+          ocx c_bwd = conj(chi[2 * sf]) * b0 + conj(chi[2 * sf + 1]) * b1;
+          F[s] = creal(ii * c_bwd - ii * c_fwd); }
+        { int sf = yp(L, s);
+          ocx u = U1[s], ud = conj(u);
+          ocx ux0 = u * X[2 * sf], ux1 = u * X[2 * sf + 1];
+          ocx f0 = r * ux0 + ii * ux1, f1 = -ii * ux0 + r * ux1;
+          ocx c_fwd = conj(c0) * f0 + conj(c1) * f1;
+          ocx udx0 = ud * x0, udx1 = ud * x1;
+          ocx b0 = r * udx0 - ii * udx1, b1 = ii * udx0 + r * udx1;
+          ocx c_bwd = conj(chi[2 * sf]) * b0 + conj(chi[2 * sf + 1]) * b1;
+          F[V + s] = creal(ii * c_bwd - ii * c_fwd); }
+    }
+}
+
+/* src/hmc_forces.cpp:501-534 */
+void orc_clover_deriv_insert(const orc_dirac* D, const double* w, double factor, double* F) {
+    int L = D->L, V = D->V, n = V;
+    const ocx* U0 = D->U; const ocx* U1 = D->U + V;
+    OMP_FOR
+    for (int s = 0; s < V; s++) {
+        int sxp = xp(L, s), sxm = xm(L, s), syp = yp(L, s), sym = ym(L, s);
+        int sxpym = ym(L, sxp), sxmyp = yp(L, sxm), sxpyp = yp(L, sxp);
+        ocx P_fwd = U0[s] * U1[sxp] * conj(U0[syp]) * conj(U1[s]);
+        double re_fwd = creal(P_fwd);
+        double w_fwd = w[s] + w[sxp] + w[sxpyp] + w[syp];
+        ocx P_bwd = U0[sym] * U1[sxpym] * conj(U0[s]) * conj(U1[sym]);
+        double re_bwd = creal(P_bwd);
+        double w_bwd = w[sym] + w[sxpym] + w[sxp] + w[s];
+        F[s] += factor * (re_fwd * w_fwd - re_bwd * w_bwd);
+        ocx P_fwd1 = U0[sxm] * U1[s] * conj(U0[sxmyp]) * conj(U1[sxm]);
+        double re_fwd1 = creal(P_fwd1);
+        double w_fwd1 = w[sxm] + w[s] + w[syp] + w[sxmyp];
+        double re_bwd1 = re_fwd;
+        double w_bwd1 = w[s] + w[sxp] + w[sxpyp] + w[syp];
+        F[V + s] += factor * (re_fwd1 * w_fwd1 - re_bwd1 * w_bwd1);
+    }
+}
+
+/* src/hmc_forces.cpp:141-194 + 198-495 (the clover part is the insertion with
+ * w = Re(chi0* X0 - chi1* X1) and factor c_sw/4) */
+void orc_fermion_force_bilinear(const orc_dirac* D, const ocx* chi, const ocx* X, double* F) {
+    orc_hopping_force(D, chi, X, F);
+    if (D->csw != 0.0) {
+        int V = D->V;
+        double* w = xcalloc(V, sizeof(double));
+        for (int s = 0; s < V; s++)
+            w[s] = creal(conj(chi[2 * s]) * X[2 * s] - conj(chi[2 * s + 1]) * X[2 * s + 1]);
+        orc_clover_deriv_insert(D, w, D->csw / 4.0, F);
+        free(w);
+    }
+}
+
+/* src/hmc_forces.cpp:134-139 */
+void orc_fermion_force(const orc_dirac* D, const ocx* X, double* F) {
+    ocx* chi = xcalloc(2 * (size_t)D->V, sizeof(ocx));
+    orc_dirac_apply(D, ORC_D, X, chi);
+    orc_fermion_force_bilinear(D, chi, X, F);
+    free(chi);
+}
+
+/* src/hmc_forces.cpp:578-637 */
+void orc_logdet_ee_force(const orc_dirac* D, double* F) {
+    int V = D->V;
+    memset(F, 0, 2 * (size_t)V * sizeof(double));
+    if (D->csw == 0.0) return;
+    double csw = D->csw, diag = 2.0 * D->r + D->mass, mu2 = D->mu_t * D->mu_t;
+    double* w = xcalloc(V, sizeof(double));
+    for (int ie = 0; ie < V / 2; ie++) {
+        int e = D->even[ie];
+        double C = 0.5 * csw * D->F01[e];
+        double d0 = diag + C, d1 = diag - C;
+        w[e] = 0.5 * csw * (d0 / (d0 * d0 + mu2) - d1 / (d1 * d1 + mu2));
+    }
+    orc_clover_deriv_insert(D, w, 0.5, F);
+    free(w);
+}
+
+/* src/hmc_forces.cpp:763-828 */
+static void schur_deriv_plus(const orc_dirac* D, const ocx* chi_o, const ocx* psi_o, double* F) {
+    int V = D->V, nh = V / 2;
+    ocx* t1 = xcalloc(2 * (size_t)nh, sizeof(ocx));
+    ocx* tmp2 = xcalloc(2 * (size_t)nh, sizeof(ocx));
+    eo_hop(D, psi_o, t1, D->even);
+    diag_inv(D, t1, tmp2, D->even, D->mu_t);
+    ocx* g5c = xcalloc(2 * (size_t)nh, sizeof(ocx));
+    for (int i = 0; i < nh; i++) { g5c[2 * i] = chi_o[2 * i]; g5c[2 * i + 1] = -chi_o[2 * i + 1]; }
+    ocx* hop = xcalloc(2 * (size_t)nh, sizeof(ocx));
+    eo_hop(D, g5c, hop, D->even);
+    for (int i = 0; i < nh; i++) hop[2 * i + 1] = -hop[2 * i + 1];
+    ocx* tmp3 = xcalloc(2 * (size_t)nh, sizeof(ocx));
+    diag_inv(D, hop, tmp3, D->even, -D->mu_t);
+    if (D->csw != 0.0) {
+        double* w = xcalloc(V, sizeof(double));
+        for (int io = 0; io < nh; io++)
+            w[D->odd[io]] = creal(conj(chi_o[2 * io]) * psi_o[2 * io] - conj(chi_o[2 * io + 1]) * psi_o[2 * io + 1]);
+        for (int ie = 0; ie < nh; ie++)
+            w[D->even[ie]] = creal(conj(tmp3[2 * ie]) * tmp2[2 * ie] - conj(tmp3[2 * ie + 1]) * tmp2[2 * ie + 1]);
+        orc_clover_deriv_insert(D, w, D->csw / 8.0, F);
+        free(w);
+    }
+    ocx* chif = xcalloc(2 * (size_t)V, sizeof(ocx));
+    ocx* psif = xcalloc(2 * (size_t)V, sizeof(ocx));
+    eo_scatter(D, tmp3, chi_o, chif);
+    eo_scatter(D, tmp2, psi_o, psif);
+    double* ff = xcalloc(2 * (size_t)V, sizeof(double));
+    orc_hopping_force(D, chif, psif, ff);
+    for (int i = 0; i < 2 * V; i++) F[i] -= 0.5 * ff[i];
+    free(t1); free(tmp2); free(g5c); free(hop); free(tmp3); free(chif); free(psif); free(ff);
+}
+
+/* src/hmc_forces.cpp:830-867 */
+void orc_eo_fermion_force(const orc_dirac* D, const ocx* x_o, const ocx* y_o, double* F) {
+    int V = D->V;
+    memset(F, 0, 2 * (size_t)V * sizeof(double));
+    schur_deriv_plus(D, y_o, x_o, F);
+    for (int i = 0; i < 2 * V; i++) F[i] *= 2.0;
+    if (D->csw != 0.0) {
+        double* fl = xcalloc(2 * (size_t)V, sizeof(double));
+        orc_logdet_ee_force(D, fl);
+        for (int i = 0; i < 2 * V; i++) F[i] += fl[i];
+        free(fl);
+    }
+}

@@ -1,0 +1,196 @@
+"""ctypes bindings for oracle/_ref/libref.so: the reference's own code, compiled
+from its sources by oracle/Makefile (see ref_harness.cpp).
+
+TEST INFRASTRUCTURE ONLY: used to pin the oracle and to write tests/golden/
+fixtures.  ``available()`` is False wherever the library was not built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PATH = os.path.join(HERE, "_ref", "libref.so")
+_P, _I, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+_lib = None
+
+
+def available():
+    return os.path.exists(PATH)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P) if a is not None else None
+
+
+def _c(a, dt=np.complex128):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = ctypes.CDLL(PATH)
+        L.ref_random_vec.argtypes = [_I, ctypes.c_uint, _P]
+        L.ref_eo_maps.argtypes = [_I, _P, _P, _P]
+        L.ref_clover.argtypes = [_I, _P, _P]
+        L.ref_avg_plaq.restype = _D
+        L.ref_avg_plaq.argtypes = [_I, _P]
+        L.ref_dirac_apply.argtypes = [_I, _P, _D, _D, _D, _D, _I, _P, _P]
+        L.ref_dirac_delta.argtypes = [_I, _P, _D, _D, _D, _D, _P, _P, _P, _D]
+        L.ref_eo_apply.argtypes = [_I, _P, _D, _D, _D, _D, _I, _P, _P]
+        L.ref_eo_reconstruct.argtypes = [_I, _P, _D, _D, _D, _D, _P, _P, _P]
+        L.ref_eo_logdet.restype = _D
+        L.ref_eo_logdet.argtypes = [_I, _P, _D, _D, _D, _D]
+        L.ref_prolong_build.argtypes = [_I, _I, _I, _P, _P]
+        L.ref_prolong_apply.argtypes = [_I, _I, _I, _P, _I, _P, _P]
+        L.ref_coarse_prolong_build.argtypes = [_I, _I, _I, _P, _P]
+        L.ref_galerkin.argtypes = [_I, _P, _D, _D, _D, _D, _I, _I, _P, _P, _P]
+        L.ref_dense_solve.argtypes = [_I, _P, _P, _P]
+        L.ref_mr_smooth.argtypes = [_I, _P, _D, _D, _D, _D, _P, _P, _I]
+        L.ref_lambda_max.restype = _D
+        L.ref_lambda_max.argtypes = [_I, _P, _D, _D, _D, _D, _I]
+        L.ref_krylov.restype = _I
+        L.ref_krylov.argtypes = [_I, _P, _D, _D, _D, _D, _I, _P, _I, _D, _I, _P, _P]
+        L.ref_mg.restype = _I
+        L.ref_mg.argtypes = [_I, _P, _D, _D, _D, _D, _I, _I, _I, _I, _I, _P, _I, _I, _I, _I,
+                             _P, _P, _I, _D, _I, _P, _P]
+        L.ref_gauge_force.argtypes = [_I, _P, _D, _P]
+        L.ref_gauge_action.restype = _D
+        L.ref_gauge_action.argtypes = [_I, _P, _D]
+        L.ref_fermion_force.argtypes = [_I, _P, _D, _D, _D, _D, _I, _P, _P, _P]
+        L.ref_fermion_action.restype = _D
+        L.ref_fermion_action.argtypes = [_I, _P, _D, _D, _D, _D, _P, _I, _D]
+        _lib = L
+    return _lib
+
+
+def random_vec(n, seed):
+    """random_vec(n, std::mt19937(seed)) -- src/linalg.cpp:44-49."""
+    out = np.zeros(n, np.complex128)
+    lib().ref_random_vec(n, seed, _ptr(out))
+    return out
+
+
+def eo_maps(L):
+    V = L * L
+    e = np.zeros(V // 2, np.int32); o = np.zeros(V // 2, np.int32); i = np.zeros(V, np.int32)
+    lib().ref_eo_maps(L, _ptr(e), _ptr(o), _ptr(i))
+    return e, o, i
+
+
+def clover(L, U):
+    U = _c(U); F = np.zeros(L * L)
+    lib().ref_clover(L, _ptr(U), _ptr(F))
+    return F
+
+
+def avg_plaq(L, U):
+    return lib().ref_avg_plaq(L, _ptr(_c(U)))
+
+
+class Dirac:
+    def __init__(self, L, U, mass, r=1.0, csw=0.0, mu_t=0.0):
+        self.L, self.V, self.U = L, L * L, _c(U)
+        self.args = (mass, r, csw, mu_t)
+
+    def apply(self, kind, src):
+        src = _c(src); dst = np.zeros(2 * self.V, np.complex128)
+        lib().ref_dirac_apply(self.L, _ptr(self.U), *self.args, kind, _ptr(src), _ptr(dst))
+        return dst
+
+    def delta(self, src, pi, dt):
+        src = _c(src); pi = _c(pi, np.float64); dst = np.zeros(2 * self.V, np.complex128)
+        lib().ref_dirac_delta(self.L, _ptr(self.U), *self.args, _ptr(src), _ptr(dst), _ptr(pi), dt)
+        return dst
+
+    def eo(self, kind, src):
+        src = _c(src); dst = np.zeros(self.V, np.complex128)
+        lib().ref_eo_apply(self.L, _ptr(self.U), *self.args, kind, _ptr(src), _ptr(dst))
+        return dst
+
+    def reconstruct(self, x_o, phi=None):
+        x_o = _c(x_o); out = np.zeros(2 * self.V, np.complex128)
+        lib().ref_eo_reconstruct(self.L, _ptr(self.U), *self.args, _ptr(x_o),
+                                 _ptr(_c(phi)) if phi is not None else None, _ptr(out))
+        return out
+
+    def logdet(self):
+        return lib().ref_eo_logdet(self.L, _ptr(self.U), *self.args)
+
+    def galerkin(self, b, k, nulls, dense=False):
+        nulls = _c(nulls)
+        nb = (self.L // b) ** 2
+        links = np.zeros(nb * 9 * k * k, np.complex128)
+        dim = nb * k
+        cols = np.zeros(dim * dim, np.complex128) if dense else None
+        lib().ref_galerkin(self.L, _ptr(self.U), *self.args, b, k, _ptr(nulls), _ptr(links), _ptr(cols))
+        return links, (cols.reshape(dim, dim) if dense else None)
+
+    def mr_smooth(self, x, b, niter):
+        x = _c(x).copy(); b = _c(b)
+        lib().ref_mr_smooth(self.L, _ptr(self.U), *self.args, _ptr(x), _ptr(b), niter)
+        return x
+
+    def lambda_max(self, niter):
+        return lib().ref_lambda_max(self.L, _ptr(self.U), *self.args, niter)
+
+    def krylov(self, solver, rhs, maxit, tol, trunc=30):
+        rhs = _c(rhs); x = np.zeros(2 * self.V, np.complex128); res = np.zeros(1)
+        it = lib().ref_krylov(self.L, _ptr(self.U), *self.args, solver, _ptr(rhs), maxit, tol, trunc,
+                              _ptr(x), _ptr(res))
+        return x, it, res[0]
+
+    def mg(self, n_levels, block, k, coarse_block, null_iters, starts, pre, post, w_cycle, mode, b,
+           maxit=100, tol=1e-10, trunc=20, sym_starts=None):
+        starts = _c(starts); b = _c(b); x = np.zeros(2 * self.V, np.complex128); res = np.zeros(1)
+        it = lib().ref_mg(self.L, _ptr(self.U), *self.args, n_levels, block, k, coarse_block, null_iters,
+                          _ptr(starts), pre, post, int(w_cycle), mode, _ptr(b), _ptr(x), maxit, tol, trunc,
+                          _ptr(_c(sym_starts)) if sym_starts is not None else None, _ptr(res))
+        return x, it, res[0]
+
+    def force(self, kind, X=None, chi=None):
+        F = np.zeros(2 * self.V)
+        lib().ref_fermion_force(self.L, _ptr(self.U), *self.args, kind,
+                                _ptr(_c(X)) if X is not None else None,
+                                _ptr(_c(chi)) if chi is not None else None, _ptr(F))
+        return F
+
+    def fermion_action(self, phi, maxit, tol):
+        return lib().ref_fermion_action(self.L, _ptr(self.U), *self.args, _ptr(_c(phi)), maxit, tol)
+
+
+def prolong_build(L, b, k, nulls):
+    nulls = _c(nulls); bv = np.zeros((L // b) ** 2 * k * 2 * b * b, np.complex128)
+    lib().ref_prolong_build(L, b, k, _ptr(nulls), _ptr(bv))
+    return bv
+
+
+def prolong_apply(L, b, k, nulls, direction, vin):
+    nulls = _c(nulls); vin = _c(vin)
+    n = 2 * L * L if direction == 0 else (L // b) ** 2 * k
+    out = np.zeros(n, np.complex128)
+    lib().ref_prolong_apply(L, b, k, _ptr(nulls), direction, _ptr(vin), _ptr(out))
+    return out
+
+
+def coarse_prolong_build(fine_dim, block_k, k, nulls):
+    nulls = _c(nulls); bv = np.zeros(fine_dim // block_k * k * block_k, np.complex128)
+    lib().ref_coarse_prolong_build(fine_dim, block_k, k, _ptr(nulls), _ptr(bv))
+    return bv
+
+
+def dense_solve(cols, b):
+    cols = _c(cols); b = _c(b); x = np.zeros(b.size, np.complex128)
+    lib().ref_dense_solve(b.size, _ptr(cols), _ptr(b), _ptr(x))
+    return x
+
+
+def gauge_force(L, U, beta):
+    F = np.zeros(2 * L * L)
+    lib().ref_gauge_force(L, _ptr(_c(U)), beta, _ptr(F))
+    return F
+
+
+def gauge_action(L, U, beta):
+    return lib().ref_gauge_action(L, _ptr(_c(U)), beta)

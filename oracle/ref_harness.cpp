@@ -1,0 +1,349 @@
+// ref_harness.cpp -- extern "C" entry points into the reference's own code.
+//
+// TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile from the reference
+// sources where they lie (never copied) into oracle/_ref/libref.so; used to
+// pin the oracle and to write the golden fixtures under tests/golden/.  The
+// reference's FEAST paths are never referenced here, and the linker's section
+// garbage collection drops them, so no FEAST or LAPACK is needed.
+#include "dirac.h"
+#include "prolongator.h"
+#include "coarse_op.h"
+#include "smoother.h"
+#include "solvers.h"
+#include "multigrid.h"
+#include "hmc.h"
+#include <cstring>
+#include <deque>
+#include <random>
+
+#define API extern "C" __attribute__((visibility("default")))
+
+namespace {
+Vec to_vec(const double* p, size_t n) {
+    Vec v(n);
+    std::memcpy(v.data(), p, n * sizeof(cx));
+    return v;
+}
+void from_vec(const Vec& v, double* p) { std::memcpy(p, v.data(), v.size() * sizeof(cx)); }
+void load_gauge_arr(GaugeField& g, const double* U) {
+    int V = g.lat.V;
+    g.U[0] = to_vec(U, V);
+    g.U[1] = to_vec(U + 2 * V, V);
+}
+struct Ctx {
+    Lattice lat;
+    GaugeField g;
+    DiracOp D;
+    Ctx(int L, const double* U, double m, double r, double csw, double mu)
+        : lat(L), g(lat), D(lat, (load_gauge_arr(g, U), g), m, r, csw, mu) {}
+};
+void force_out(const std::array<RVec, 2>& F, double* out, int V) {
+    std::memcpy(out, F[0].data(), V * sizeof(double));
+    std::memcpy(out + V, F[1].data(), V * sizeof(double));
+}
+}  // namespace
+
+API void ref_random_vec(int n, unsigned seed, double* out) {
+    std::mt19937 rng(seed);
+    from_vec(random_vec(n, rng), out);
+}
+
+API void ref_eo_maps(int L, int* even, int* odd, int* eoi) {
+    Lattice lat(L);
+    std::memcpy(even, lat.even_sites.data(), lat.V_half * sizeof(int));
+    std::memcpy(odd, lat.odd_sites.data(), lat.V_half * sizeof(int));
+    std::memcpy(eoi, lat.eo_index.data(), lat.V * sizeof(int));
+}
+
+API void ref_clover(int L, const double* U, double* F01) {
+    Ctx c(L, U, 0.1, 1.0, 1.0, 0.0);
+    std::memcpy(F01, c.D.clover_field.data(), c.lat.V * sizeof(double));
+}
+
+API double ref_avg_plaq(int L, const double* U) {
+    Lattice lat(L); GaugeField g(lat); load_gauge_arr(g, U);
+    return g.avg_plaq();
+}
+
+// kind: 0 D, 1 D^dag, 2 D^dag D, 3 g5 D
+API void ref_dirac_apply(int L, const double* U, double m, double r, double csw, double mu,
+                         int kind, const double* src, double* dst) {
+    Ctx c(L, U, m, r, csw, mu);
+    Vec s = to_vec(src, c.lat.ndof), d(c.lat.ndof);
+    if (kind == 0) c.D.apply(s, d);
+    else if (kind == 1) c.D.apply_dag(s, d);
+    else if (kind == 2) c.D.apply_DdagD(s, d);
+    else c.D.apply_g5D(s, d);
+    from_vec(d, dst);
+}
+
+API void ref_dirac_delta(int L, const double* U, double m, double r, double csw, double mu,
+                         const double* src, double* dst, const double* pi, double dt) {
+    Ctx c(L, U, m, r, csw, mu);
+    Vec s = to_vec(src, c.lat.ndof), d(c.lat.ndof);
+    std::array<RVec, 2> p;
+    p[0].assign(pi, pi + c.lat.V);
+    p[1].assign(pi + c.lat.V, pi + 2 * c.lat.V);
+    c.D.apply_delta_D(s, d, p, dt);
+    from_vec(d, dst);
+}
+
+// kind: 0 hop e->o, 1 hop o->e, 2 ee_inv, 3 oo_inv, 4 oo, 5 M, 6 M^dag, 7 M^dag M
+API void ref_eo_apply(int L, const double* U, double m, double r, double csw, double mu,
+                      int kind, const double* src, double* dst) {
+    Ctx c(L, U, m, r, csw, mu);
+    EvenOddDiracOp eo(c.D);
+    int nh = 2 * c.lat.V_half;
+    Vec s = to_vec(src, nh), d(nh);
+    switch (kind) {
+    case 0: eo.apply_eo(s, d); break;
+    case 1: eo.apply_oe(s, d); break;
+    case 2: eo.apply_ee_inv(s, d); break;
+    case 3: eo.apply_oo_inv(s, d); break;
+    case 4: eo.apply_oo(s, d); break;
+    case 5: eo.apply_schur(s, d); break;
+    case 6: eo.apply_schur_dag(s, d); break;
+    default: eo.apply_schur_dag_schur(s, d); break;
+    }
+    from_vec(d, dst);
+}
+
+API void ref_eo_reconstruct(int L, const double* U, double m, double r, double csw, double mu,
+                            const double* x_o, const double* phi, double* out) {
+    Ctx c(L, U, m, r, csw, mu);
+    EvenOddDiracOp eo(c.D);
+    Vec xo = to_vec(x_o, 2 * c.lat.V_half);
+    Vec res = phi ? eo.reconstruct(xo, to_vec(phi, c.lat.ndof)) : eo.reconstruct_full(xo);
+    from_vec(res, out);
+}
+
+API double ref_eo_logdet(int L, const double* U, double m, double r, double csw, double mu) {
+    Ctx c(L, U, m, r, csw, mu);
+    EvenOddDiracOp eo(c.D);
+    return eo.log_det_ee();
+}
+
+// Geometric prolongator on the fine lattice: block vectors out [nblocks][k][2*b*b]
+API void ref_prolong_build(int L, int b, int k, const double* nulls, double* bv) {
+    Lattice lat(L);
+    Prolongator P(lat, b, b, k);
+    std::vector<Vec> nv(k);
+    for (int i = 0; i < k; i++) nv[i] = to_vec(nulls + 2 * (size_t)i * lat.ndof, lat.ndof);
+    P.build_from_vectors(nv);
+    size_t bd = P.block_fine_dim();
+    for (int bb = 0; bb < P.nblocks; bb++)
+        for (int i = 0; i < k; i++)
+            from_vec(P.block_vecs[bb][i], bv + 2 * ((size_t)bb * k + i) * bd);
+}
+
+// direction: 0 prolong (coarse -> fine), 1 restrict (fine -> coarse)
+API void ref_prolong_apply(int L, int b, int k, const double* nulls, int direction,
+                           const double* in, double* out) {
+    Lattice lat(L);
+    Prolongator P(lat, b, b, k);
+    std::vector<Vec> nv(k);
+    for (int i = 0; i < k; i++) nv[i] = to_vec(nulls + 2 * (size_t)i * lat.ndof, lat.ndof);
+    P.build_from_vectors(nv);
+    if (direction == 0) from_vec(P.prolong(to_vec(in, P.coarse_dim)), out);
+    else from_vec(P.restrict_vec(to_vec(in, lat.ndof)), out);
+}
+
+// Linear-block prolongator of the coarse levels: block vectors [nblocks][k][block_k]
+API void ref_coarse_prolong_build(int fine_dim, int block_k, int k, const double* nulls, double* bv) {
+    CoarseProlongator P(fine_dim, block_k, k);
+    std::vector<Vec> nv(k);
+    for (int i = 0; i < k; i++) nv[i] = to_vec(nulls + 2 * (size_t)i * fine_dim, fine_dim);
+    P.build_from_vectors(nv);
+    for (int bb = 0; bb < P.nblocks; bb++)
+        for (int i = 0; i < k; i++)
+            from_vec(P.block_vecs[bb][i], bv + 2 * ((size_t)bb * k + i) * block_k);
+}
+
+// Galerkin A_c = P^dag (D^dag D) P: sparse 9-point links [nb][9][k*k] and/or dense columns
+API void ref_galerkin(int L, const double* U, double m, double r, double csw, double mu,
+                      int b, int k, const double* nulls, double* links, double* dense_cols) {
+    Ctx c(L, U, m, r, csw, mu);
+    Prolongator P(c.lat, b, b, k);
+    std::vector<Vec> nv(k);
+    for (int i = 0; i < k; i++) nv[i] = to_vec(nulls + 2 * (size_t)i * c.lat.ndof, c.lat.ndof);
+    P.build_from_vectors(nv);
+    if (links) {
+        SparseCoarseOp S;
+        S.build(P, c.D.as_DdagD_op(), c.lat.ndof);
+        size_t kk = (size_t)k * k;
+        for (int bb = 0; bb < S.nblocks; bb++)
+            for (int d = 0; d < 9; d++)
+                from_vec(S.link[bb][d], links + 2 * ((size_t)bb * 9 + d) * kk);
+    }
+    if (dense_cols) {
+        CoarseOp A;
+        A.build(c.D, P);
+        for (int j = 0; j < A.dim; j++) from_vec(A.mat[j], dense_cols + 2 * (size_t)j * A.dim);
+    }
+}
+
+API void ref_dense_solve(int dim, const double* cols, const double* b, double* x) {
+    CoarseOp A;
+    A.dim = dim;
+    A.mat.resize(dim);
+    for (int j = 0; j < dim; j++) A.mat[j] = to_vec(cols + 2 * (size_t)j * dim, dim);
+    from_vec(A.solve(to_vec(b, dim)), x);
+}
+
+API void ref_mr_smooth(int L, const double* U, double m, double r, double csw, double mu,
+                       double* x, const double* b, int niter) {
+    Ctx c(L, U, m, r, csw, mu);
+    Vec xv = to_vec(x, c.lat.ndof);
+    mr_smooth(c.D, xv, to_vec(b, c.lat.ndof), niter);
+    from_vec(xv, x);
+}
+
+API double ref_lambda_max(int L, const double* U, double m, double r, double csw, double mu, int niter) {
+    Ctx c(L, U, m, r, csw, mu);
+    return estimate_lambda_max(c.D.as_DdagD_op(), c.lat.ndof, niter);
+}
+
+// solver: 0 CG, 1 GCR (fcg_solve), 2 FGMRES; operator D^dag D; identity preconditioner
+API int ref_krylov(int L, const double* U, double m, double r, double csw, double mu, int solver,
+                   const double* rhs, int maxit, double tol, int trunc, double* x, double* res) {
+    Ctx c(L, U, m, r, csw, mu);
+    OpApply A = c.D.as_DdagD_op();
+    Vec b = to_vec(rhs, c.lat.ndof);
+    auto ident = [](const Vec& v) { return v; };
+    if (solver == 0) {
+        auto o = cg_solve(A, c.lat.ndof, b, maxit, tol);
+        from_vec(o.solution, x); *res = o.final_residual; return o.iterations;
+    } else if (solver == 1) {
+        auto o = fcg_solve(A, c.lat.ndof, b, ident, maxit, tol, trunc);
+        from_vec(o.solution, x); *res = o.final_residual; return o.iterations;
+    }
+    auto o = fgmres_solve_generic(A, c.lat.ndof, b, ident, trunc, maxit, tol, 0);
+    from_vec(o.solution, x); *res = o.final_residual; return o.iterations;
+}
+
+// Multigrid built from the reference's building blocks exactly as
+// build_mg_hierarchy (src/multigrid.cpp:595-687) assembles them, but with every
+// level's null-space start vectors supplied (the reference's warm_start).
+// Levels >= 1 use CoarseProlongator with block_k = coarse_block.  Then either one
+// preconditioner application (mode 0), or a solve: 1 = fcg_solve_mg,
+// 2 = CG with the symmetric (Richardson) cycle, 3 = fgmres_solve_mg.
+// sym_starts: power-iteration start vectors per non-coarsest level (mode 2).
+API int ref_mg(int L, const double* U, double m, double r, double csw, double mu,
+               int n_levels, int block, int k, int coarse_block, int null_iters,
+               const double* starts, int pre, int post, int w_cycle, int mode,
+               const double* b, double* x, int maxit, double tol, int trunc,
+               const double* sym_starts, double* res_out) {
+    Ctx c(L, U, m, r, csw, mu);
+    std::mt19937 rng(1);
+    std::vector<MGLevel> levels(n_levels);
+    std::deque<Prolongator> gp;
+    std::deque<CoarseProlongator> cp;
+    std::deque<CoarseOp> acs;
+    size_t off = 0;
+    levels[0].op = c.D.as_DdagD_op();
+    levels[0].dim = c.lat.ndof;
+    levels[0].pre_smooth = pre; levels[0].post_smooth = post;
+    std::vector<Vec> warm(k);
+    for (int i = 0; i < k; i++) warm[i] = to_vec(starts + 2 * (off + (size_t)i * c.lat.ndof), c.lat.ndof);
+    off += (size_t)k * c.lat.ndof;
+    auto nv = compute_near_null_space(c.D, k, null_iters, rng, &warm);
+    gp.emplace_back(c.lat, block, block, k);
+    gp.back().build_from_vectors(nv);
+    Prolongator& P0 = gp.back();
+    levels[0].restrict_fn = [&P0](const Vec& v) { return P0.restrict_vec(v); };
+    levels[0].prolong_fn = [&P0](const Vec& v) { return P0.prolong(v); };
+    acs.emplace_back();
+    acs.back().build(c.D, P0);
+    levels[0].Ac = acs.back();
+    int prev = P0.coarse_dim;
+    for (int l = 1; l < n_levels - 1; l++) {
+        CoarseOp& Aab = acs.back();
+        OpApply op = [&Aab](const Vec& s, Vec& d) { d = Aab.apply(s); };
+        levels[l].op = op; levels[l].dim = prev;
+        levels[l].pre_smooth = pre; levels[l].post_smooth = post;
+        int blk = coarse_block;
+        while (prev % blk != 0 && blk > 1) blk--;
+        std::vector<Vec> w(k);
+        for (int i = 0; i < k; i++) w[i] = to_vec(starts + 2 * (off + (size_t)i * prev), prev);
+        off += (size_t)k * prev;
+        auto cn = compute_near_null_space_generic(op, prev, k, null_iters, rng, &w);
+        cp.emplace_back(prev, blk, k);
+        cp.back().build_from_vectors(cn);
+        CoarseProlongator& Pc = cp.back();
+        levels[l].restrict_fn = [&Pc](const Vec& v) { return Pc.restrict_vec(v); };
+        levels[l].prolong_fn = [&Pc](const Vec& v) { return Pc.prolong(v); };
+        acs.emplace_back();
+        acs.back().build_generic(op, Pc, prev, Pc.coarse_dim);
+        levels[l].Ac = acs.back();
+        prev = Pc.coarse_dim;
+    }
+    {
+        CoarseOp& Aab = acs.back();
+        levels[n_levels - 1].op = [&Aab](const Vec& s, Vec& d) { d = Aab.apply(s); };
+        levels[n_levels - 1].dim = prev;
+        levels[n_levels - 1].Ac = Aab;
+        levels[n_levels - 1].pre_smooth = 0;
+        levels[n_levels - 1].post_smooth = 0;
+    }
+    if (mode == 2) {   // set_symmetric(0.8) with supplied power-iteration starts
+        size_t so = 0;
+        for (int l = 0; l < n_levels - 1; l++) {
+            int n = levels[l].dim;
+            Vec v = to_vec(sym_starts + 2 * so, n);
+            so += n;
+            double vn = norm(v); scale(v, 1.0 / vn);
+            Vec Av(n); double lambda = 0;
+            for (int i = 0; i < 30; i++) {
+                levels[l].op(v, Av);
+                lambda = norm(Av);
+                if (lambda < 1e-30) break;
+                scale(Av, 1.0 / lambda); v = Av;
+            }
+            levels[l].richardson_omega = 0.8 / lambda;
+        }
+    }
+    bool wc = w_cycle != 0;
+    auto pc = [&levels, wc](const Vec& v) { return mg_cycle(levels, 0, v, wc); };
+    Vec bv = to_vec(b, c.lat.ndof);
+    if (mode == 0) { from_vec(pc(bv), x); return 0; }
+    if (mode == 1) {
+        auto o = fcg_solve(levels[0].op, c.lat.ndof, bv, pc, maxit, tol, trunc);
+        from_vec(o.solution, x); *res_out = o.final_residual; return o.iterations;
+    }
+    if (mode == 2) {
+        auto o = cg_solve_precond(levels[0].op, c.lat.ndof, bv, pc, maxit, tol);
+        from_vec(o.solution, x); *res_out = o.final_residual; return o.iterations;
+    }
+    auto o = fgmres_solve_generic(levels[0].op, c.lat.ndof, bv, pc, trunc, maxit, tol, 0);
+    from_vec(o.solution, x); *res_out = o.final_residual; return o.iterations;
+}
+
+// forces (F = -dS/dtheta, layout F[mu*V + s])
+API void ref_gauge_force(int L, const double* U, double beta, double* F) {
+    Lattice lat(L); GaugeField g(lat); load_gauge_arr(g, U);
+    std::array<RVec, 2> f; gauge_force(g, beta, f); force_out(f, F, lat.V);
+}
+API double ref_gauge_action(int L, const double* U, double beta) {
+    Lattice lat(L); GaugeField g(lat); load_gauge_arr(g, U);
+    return gauge_action(g, beta);
+}
+// kind: 0 fermion_force(X), 1 fermion_force_bilinear(chi, X), 2 hopping_force_bilinear(chi, X),
+//       3 logdet_ee_force, 4 eo_fermion_force(x_o = X, y_o = chi)
+API void ref_fermion_force(int L, const double* U, double m, double r, double csw, double mu,
+                           int kind, const double* X, const double* chi, double* F) {
+    Ctx c(L, U, m, r, csw, mu);
+    std::array<RVec, 2> f;
+    int n = c.lat.ndof, nh = 2 * c.lat.V_half;
+    if (kind == 0) fermion_force(c.D, to_vec(X, n), f);
+    else if (kind == 1) fermion_force_bilinear(c.D, to_vec(chi, n), to_vec(X, n), f);
+    else if (kind == 2) hopping_force_bilinear(c.D, to_vec(chi, n), to_vec(X, n), f);
+    else if (kind == 3) logdet_ee_force(c.D, f);
+    else { EvenOddDiracOp eo(c.D); eo_fermion_force(c.D, eo, to_vec(X, nh), to_vec(chi, nh), f); }
+    force_out(f, F, c.lat.V);
+}
+
+API double ref_fermion_action(int L, const double* U, double m, double r, double csw, double mu,
+                              const double* phi, int maxit, double tol) {
+    Ctx c(L, U, m, r, csw, mu);
+    return fermion_action(c.D, to_vec(phi, c.lat.ndof), maxit, tol).action;
+}
