@@ -1,0 +1,218 @@
+"""Benchmark: MG-GCR solves/s on a 512 x 512 Wilson-clover Schwinger lattice.
+
+One step = one right-hand side solved to a true residual |phi - A x|/|phi| <= 1e-10
+(A = D^dag D, the reference's system, src/solvers.cpp:55-59) by mixed-precision
+GCR + multigrid: FP32 GCR and FP32 W-cycle with FP16 coarse links inside FP64
+iterative refinement on the FP64 true residual.  The same hierarchy solved in
+plain FP64 (FP64 GCR + FP64 cycle) is timed beside it, the Wilson-clover stencil
+is timed alone (HIP events on the library's stream), and the oracle's FP64 CPU
+MG-GCR solve is timed on the host cores.  Inputs are synthetic (seeded U(1)
+links theta ~ N(0, 0.4^2), Gaussian source) and resident in HBM when the timed
+region starts.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--L", type=int, default=512)
+    ap.add_argument("--mass", type=float, default=0.05)
+    ap.add_argument("--sigma", type=float, default=0.4)
+    ap.add_argument("--levels", type=int, default=3)
+    ap.add_argument("--k", type=int, default=8)
+    ap.add_argument("--null-iters", type=int, default=10)
+    ap.add_argument("--pre", type=int, default=3)
+    ap.add_argument("--post", type=int, default=3)
+    ap.add_argument("--coarse-maxit", type=int, default=50)
+    ap.add_argument("--coarse-tol", type=float, default=1e-2)
+    ap.add_argument("--trunc", type=int, default=10)
+    ap.add_argument("--inner-tol", type=float, default=1e-5)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def stencil_bytes_per_site(prec, nrhs, csw):
+    sc = 16 if prec == "f64" else 8      # complex
+    sr = 8 if prec == "f64" else 4       # real
+    return nrhs * 2 * 2 * sc + 2 * sc + (sr if csw else 0)
+
+
+def time_stencil(smg, torch, D, prec, nrhs, V, csw, launches=50, warm=5):
+    """Average duration of one Wilson-clover launch, HIP events on the library stream."""
+    dt = torch.complex128 if prec == "f64" else torch.complex64
+    src = torch.randn(nrhs, 2 * V, dtype=dt, device="cuda")
+    dst = torch.empty_like(src)
+    p = smg.F64 if prec == "f64" else smg.F32
+    st = torch.cuda.ExternalStream(smg.library_stream())
+    for _ in range(warm):
+        D.apply_dev(smg.D_, p, src.data_ptr(), dst.data_ptr(), nrhs, None)
+    smg.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(launches):
+        D.apply_dev(smg.D_, p, src.data_ptr(), dst.data_ptr(), nrhs, None)
+    e1.record(st)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / launches
+    byt = stencil_bytes_per_site(prec, nrhs, csw) * V
+    return {"prec": prec, "nrhs": nrhs, "us_per_launch": ms * 1e3, "bytes_per_launch": byt,
+            "GBs": byt / (ms * 1e-3) / 1e9}
+
+
+def cpu_baseline(args, U, rhs, starts, threads):
+    """The oracle's FP64 MG-GCR solve (same hierarchy shape) on the host cores."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    import oracle as O
+    L = args.L
+    D = O.Dirac(L, U, args.mass, 1.0, 1.0, 0.0)
+    A = D.op()
+    blocks = ((4, 4),) + ((2, 2),) * (args.levels - 2)
+    t0 = time.time()
+    mg = O.MG(A, L, L, 2, starts, n_levels=args.levels, block=blocks, k=(args.k,) * (args.levels - 1),
+              null_iters=args.null_iters, pre=args.pre, post=args.post, coarse_solver=0,
+              coarse_maxit=args.coarse_maxit, coarse_tol=args.coarse_tol)
+    setup = time.time() - t0
+    t0 = time.time()
+    x, it, res = O.gcr(A, rhs, 200, 1e-10, args.trunc, mg, ref_projection=False)
+    dt = time.time() - t0
+    tres = np.linalg.norm(rhs - A(x)) / np.linalg.norm(rhs)
+    return {"value": 1.0 / dt, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"1 FP64 MG-GCR solve at L={L} after a {setup:.1f} s setup "
+                      f"({it} GCR iterations, true residual {tres:.2e})"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import schwinger_mg_amd as smg
+    from schwinger_mg_amd.fields import gaussian_vec, gaussian_vecs, random_gauge
+
+    L, V = args.L, args.L * args.L
+    seed = 2024
+    U = random_gauge(L, args.sigma, seed)
+    rhs = gaussian_vec(2 * V, seed + 1)
+    dims = [2 * V]
+    nx = L
+    for lev in range(args.levels - 1):
+        b = 4 if lev == 0 else 2
+        nx //= b
+        dims.append(nx * nx * args.k)
+    starts = np.concatenate([gaussian_vecs(args.k, n, seed + 10 + i).ravel() for i, n in enumerate(dims[:-1])])
+
+    D = smg.DiracOp(L, U, args.mass, 1.0, 1.0, 0.0)
+    blocks = ((4, 4),) + ((2, 2),) * (args.levels - 2)
+    t0 = time.time()
+    mg = smg.MGHierarchy(D, starts, n_levels=args.levels, blocks=blocks, k=(args.k,) * (args.levels - 1),
+                         null_iters=args.null_iters, pre_smooth=args.pre, post_smooth=args.post,
+                         coarse_maxit=args.coarse_maxit, coarse_tol=args.coarse_tol, coarse_half=True)
+    smg.synchronize()
+    setup_s = time.time() - t0
+    log(f"MG setup {setup_s:.2f} s")
+
+    b = torch.from_numpy(rhs).cuda()
+    x = torch.zeros_like(b)
+    info = smg.SolveInfo()
+    import ctypes
+
+    def solve(prec):
+        smg._chk(smg.lib().smg_gcr_solve_dev(D.h, mg.h, prec, b.data_ptr(), x.data_ptr(), 500, 1e-10, args.trunc,
+                                             args.inner_tol, ctypes.byref(info), None))
+
+    for _ in range(args.warmup):
+        solve(smg.F32)
+    torch.cuda.synchronize(); smg.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        solve(smg.F32)
+    torch.cuda.synchronize(); smg.synchronize()
+    t_mixed = (time.perf_counter() - t0) / args.steps
+    mixed = {"iterations": info.iterations, "restarts": info.restarts, "true_residual": info.true_residual}
+    log(f"mixed: {t_mixed * 1e3:.1f} ms/solve, {mixed}")
+    x_mixed = x.cpu().numpy()
+
+    solve(smg.F64)
+    torch.cuda.synchronize(); smg.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.fp64_steps):
+        solve(smg.F64)
+    torch.cuda.synchronize(); smg.synchronize()
+    t_f64 = (time.perf_counter() - t0) / args.fp64_steps
+    fp64 = {"iterations": info.iterations, "true_residual": info.true_residual}
+    log(f"fp64: {t_f64 * 1e3:.1f} ms/solve, {fp64}")
+    x_f64 = x.cpu().numpy()
+    err_vs_f64 = float(np.linalg.norm(x_mixed - x_f64) / np.linalg.norm(x_f64))
+
+    stencil = [time_stencil(smg, torch, D, p, n, V, True) for p, n in
+               [("f32", 1), ("f64", 1), ("f32", 32), ("f64", 32)]]
+    dom = stencil[3]     # FP64, 32 RHS: a 540 MB working set (beyond the 256 MB Infinity Cache)
+    log("stencil: " + ", ".join(f"{s['prec']}x{s['nrhs']} {s['GBs']:.0f} GB/s" for s in stencil))
+    roofline = {"bound": "hbm", "achieved": round(dom["GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_dslash<double> (Wilson-clover D, 32 RHS, L=512)"}
+
+    cpu = None
+    if args.cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {threads} threads ...")
+        cpu = cpu_baseline(args, U, rhs, starts, threads)
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": "MG-GCR solves/sec at L=512, 1e-10 resid: mixed vs FP64 GPU; Dslash GB/s",
+        "value": round(1.0 / t_mixed, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_mixed * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP16 coarse links) in f64 refinement",
+        "data": "synthetic",
+        "config": {"workload": f"L={L} Wilson-clover m={args.mass} c_sw=1, theta~N(0,{args.sigma}^2), 1 RHS, "
+                               f"{args.levels}-level MG (4x4 then 2x2 aggregates, k={args.k}), GCR({args.trunc}), "
+                               "tol 1e-10 true residual",
+                   "L": L, "levels": args.levels, "k": args.k, "pre": args.pre, "post": args.post,
+                   "coarse_maxit": args.coarse_maxit, "coarse_tol": args.coarse_tol},
+        "mixed": mixed,
+        "fp64_solves_per_s": round(1.0 / t_f64, 3),
+        "fp64": fp64,
+        "speedup_mixed_vs_fp64": round(t_f64 / t_mixed, 3),
+        "mixed_vs_fp64_solution_relerr": err_vs_f64,
+        "setup_s": round(setup_s, 3),
+        "stencil": stencil,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

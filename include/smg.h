@@ -1,0 +1,231 @@
+/*
+ * smg.h -- C-ABI of the MI355X (gfx950) Schwinger-model Wilson-clover / multigrid /
+ * fermion-force library (libsmg.so).
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success and a negative code on failure
+ *    (-1 invalid argument, -2 HIP runtime error, -3 internal error); the message
+ *    of the last failure on the calling thread is returned by smg_last_error().
+ *    No C++ exception crosses this boundary.
+ *  - Complex arrays are interleaved (re, im) doubles: the bytes of the
+ *    reference's std::vector<std::complex<double>> (src/types.h:7-8).  Layouts are
+ *    the reference's: spinor psi[2*s + spin] with s = x + L*y (src/lattice.h:19,
+ *    src/dirac.h:99); gauge U[mu*V + s] (src/gauge.h:12); forces F[mu*V + s];
+ *    coarse vectors c[b*k + j] (src/prolongator.h:13); coarse links
+ *    link[(b*9 + d)*k*k + ki*k + kj] (src/coarse_op.h:159-166); half-lattice
+ *    vectors are indexed by eo_index (src/lattice.h:28-41).
+ *  - Unsuffixed calls take HOST arrays (the reference's Vec semantics: inputs
+ *    const, outputs pre-sized by the caller).  *_dev calls take DEVICE pointers
+ *    of precision `prec` and an optional hipStream_t (NULL: the library orders
+ *    itself after all prior work of the calling thread's null stream).
+ *  - Handles are opaque, owned by the caller, and used by one host thread at a
+ *    time (the reference's objects carry the same restriction).
+ */
+#ifndef SMG_H
+#define SMG_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMG_F64 0
+#define SMG_F32 1
+#define SMG_F16 2 /* coarse-link storage only (FP32 arithmetic) */
+
+/* Dirac operator kinds: D, D^dag, D^dag D, gamma5 D (src/dirac.h:115-152) */
+#define SMG_D 0
+#define SMG_DDAG 1
+#define SMG_DDAGD 2
+#define SMG_G5D 3
+
+/* even-odd kinds (src/dirac.h:253-445) */
+#define SMG_EO_HOP_EO 0          /* apply_eo: even -> odd hopping      */
+#define SMG_EO_HOP_OE 1          /* apply_oe: odd -> even hopping      */
+#define SMG_EO_EE_INV 2          /* apply_ee_inv                       */
+#define SMG_EO_OO_INV 3          /* apply_oo_inv                       */
+#define SMG_EO_OO 4              /* apply_oo                           */
+#define SMG_EO_SCHUR 5           /* apply_schur: M                     */
+#define SMG_EO_SCHUR_DAG 6       /* apply_schur_dag: M^dag             */
+#define SMG_EO_SCHUR_DAG_SCHUR 7 /* apply_schur_dag_schur: M^dag M     */
+
+/* solver operators */
+#define SMG_OP_DDAGD 0 /* D^dag D on the full lattice (DiracOp::as_DdagD_op, src/dirac.h:177-179) */
+#define SMG_OP_SCHUR 1 /* M^dag M on odd sites (EvenOddDiracOp::apply_schur_dag_schur) */
+
+/* smoother kinds (src/smoother.cpp) */
+#define SMG_SMOOTH_MR 0
+#define SMG_SMOOTH_RICHARDSON 1
+
+typedef struct smg_dirac_s smg_dirac;
+typedef struct smg_prolongator_s smg_prolongator;
+typedef struct smg_coarse_s smg_coarse;
+typedef struct smg_mg_s smg_mg;
+
+/* ------------------------------------------------------------------ runtime */
+/* Select the HIP device and create the library's stream.  Optional: the first
+ * call of any other entry point initialises device 0. */
+int smg_init(int device);
+const char* smg_last_error(void);
+int smg_version(void);
+/* Block until all library work has completed. */
+int smg_synchronize(void);
+/* The hipStream_t the library launches on (for event timing by the caller). */
+int smg_get_stream(void** stream);
+
+/* ------------------------------------------------------------------ geometry */
+/* Lattice::init_eo, src/lattice.h:28-41 (host arrays of V/2, V/2 and V ints) */
+int smg_eo_maps(int L, int* even_sites, int* odd_sites, int* eo_index);
+/* GaugeField::avg_plaq, src/gauge.h:31-36 */
+int smg_avg_plaq(int L, const double* U, double* out);
+/* gauge_action, src/hmc_forces.cpp:80-86 */
+int smg_gauge_action(int L, const double* U, double beta, double* out);
+/* gauge_force, src/hmc_forces.cpp:88-111 (F = -dS_g/dtheta) */
+int smg_gauge_force(int L, const double* U, double beta, double* F);
+
+/* ------------------------------------------------------------------ Dirac operator */
+/* DiracOp(lat, gauge, m, r, csw, mu_t), src/dirac.h:17-22 (U copied to the device,
+ * clover field computed on the device, src/dirac.h:26-52) */
+int smg_dirac_create(smg_dirac** out, int L, const double* U, double mass, double r, double csw, double mu_t);
+int smg_dirac_destroy(smg_dirac* D);
+/* new gauge field for the same operator (clover recomputed) */
+int smg_dirac_set_gauge(smg_dirac* D, const double* U);
+/* DiracOp::clover_field (V doubles) */
+int smg_dirac_clover(smg_dirac* D, double* F01);
+/* DiracOp::apply / apply_dag / apply_DdagD / apply_g5D, src/dirac.h:115-152 */
+int smg_dirac_apply(smg_dirac* D, int kind, const double* src, double* dst);
+/* the same on device vectors of precision prec, nrhs vectors of 2*V complex each */
+int smg_dirac_apply_dev(smg_dirac* D, int kind, int prec, const void* src, void* dst, int nrhs, void* stream);
+/* DiracOp::apply_delta_D, src/dirac.h:159-174 (pi: 2*V doubles) */
+int smg_dirac_delta(smg_dirac* D, const double* src, double* dst, const double* pi, double dt);
+/* EvenOddDiracOp operators (half-lattice vectors of V complex), src/dirac.h:253-445 */
+int smg_eo_apply(smg_dirac* D, int kind, const double* src, double* dst);
+int smg_eo_apply_dev(smg_dirac* D, int kind, int prec, const void* src, void* dst, void* stream);
+/* EvenOddDiracOp::reconstruct (phi != NULL) / reconstruct_full (phi == NULL), src/dirac.h:515-552 */
+int smg_eo_reconstruct(smg_dirac* D, const double* x_o, const double* phi, double* out_full);
+/* EvenOddDiracOp::log_det_ee, src/dirac.h:528-537 */
+int smg_eo_logdet(smg_dirac* D, double* out);
+
+/* ------------------------------------------------------------------ BLAS-1 */
+/* dot (src/linalg.cpp:8-18): out2 = {Re, Im} of sum conj(a) b; device vectors of n complex */
+int smg_dot_dev(const void* a, const void* b, size_t n, int prec, double* out2, void* stream);
+/* norm (src/linalg.cpp:20) */
+int smg_norm_dev(const void* a, size_t n, int prec, double* out, void* stream);
+
+/* ------------------------------------------------------------------ prolongator */
+/* Prolongator(lat, bx, by, k), src/prolongator.h:35-58, generalised to nd dof per
+ * site so that it also restates CoarseProlongator (src/prolongator.h:149-228):
+ * linear blocks of 4*k consecutive coarse dofs on an nx x ny coarse lattice are
+ * the 4 x 1 aggregates with nd = k. */
+int smg_prolongator_create(smg_prolongator** out, int nx, int ny, int nd, int bx, int by, int k);
+int smg_prolongator_destroy(smg_prolongator* P);
+/* Prolongator::build_from_vectors, src/prolongator.h:133-141 (k host vectors of nx*ny*nd complex) */
+int smg_prolongator_build(smg_prolongator* P, const double* null_vecs);
+/* block vectors [nblocks][k][bx*by*nd] (Prolongator::block_vecs) */
+int smg_prolongator_get(smg_prolongator* P, double* bv);
+int smg_prolongator_set(smg_prolongator* P, const double* bv);
+/* Prolongator::prolong / restrict_vec, src/prolongator.h:77-108 */
+int smg_prolong(smg_prolongator* P, const double* coarse, double* fine);
+int smg_restrict(smg_prolongator* P, const double* fine, double* coarse);
+
+/* ------------------------------------------------------------------ coarse operator */
+/* Galerkin A_c = P^dag (D^dag D) P as the 9-point operator (SparseCoarseOp::build,
+ * src/coarse_op.h:241-279; equal to CoarseOp::build, src/coarse_op.h:40-43, for
+ * aggregates of at least 2 x 2 sites) */
+int smg_coarse_create_galerkin(smg_coarse** out, smg_dirac* D, smg_prolongator* P);
+/* next level: A_c = P^dag A P for a coarse operator A (CoarseOp::build_generic, src/coarse_op.h:26-37) */
+int smg_coarse_create_from_coarse(smg_coarse** out, smg_coarse* A, smg_prolongator* P);
+/* a coarse operator from explicit links */
+int smg_coarse_create_links(smg_coarse** out, int nx, int ny, int k, const double* links);
+int smg_coarse_destroy(smg_coarse* C);
+int smg_coarse_get_links(smg_coarse* C, double* links);
+/* SparseCoarseOp::apply_to, src/coarse_op.h:282-297, links read in precision prec */
+int smg_coarse_apply(smg_coarse* C, int prec, const double* src, double* dst);
+/* SparseCoarseOp::solve without deflation (CG from x0 = 0), src/coarse_op.h:492-547 */
+int smg_coarse_solve(smg_coarse* C, const double* b, double* x, int maxit, double tol, int* iters);
+
+/* ------------------------------------------------------------------ smoothers */
+/* mr_smooth (src/smoother.cpp:9-23) / richardson_smooth_op (:63-73) on D^dag D; x in/out */
+int smg_smooth_dirac(smg_dirac* D, int kind, double* x, const double* b, int niter, double omega);
+/* mr_smooth_op / richardson_smooth_op on a coarse operator */
+int smg_smooth_coarse(smg_coarse* C, int kind, double* x, const double* b, int niter, double omega);
+/* estimate_lambda_max (src/smoother.cpp:43-59) of D^dag D from the start vector v0 */
+int smg_lambda_max_dirac(smg_dirac* D, const double* v0, int niter, double* out);
+
+/* ------------------------------------------------------------------ multigrid */
+typedef struct {
+    int n_levels;          /* 2..5 */
+    int bx[4], by[4];      /* aggregate shape of transfer l (level l -> l+1) */
+    int k[4];              /* near-null vectors of transfer l */
+    int null_iters;        /* outer inverse-iteration steps (5 MR sweeps each) */
+    int pre_smooth, post_smooth;
+    int w_cycle;
+    int coarse_maxit;      /* coarsest CG */
+    double coarse_tol;
+    int coarse_half;       /* FP32 cycle: coarse links in FP16 */
+} smg_mg_params;
+/* build_mg_hierarchy, src/multigrid.cpp:595-687, with every level's null-space start
+ * vectors supplied (k[l] vectors of level-l dimension, level 0 first), geometric
+ * aggregation at every level and 9-point Galerkin operators */
+int smg_mg_setup(smg_mg** out, smg_dirac* D, const smg_mg_params* prm, const double* starts);
+int smg_mg_destroy(smg_mg* mg);
+/* MGHierarchy::set_symmetric, src/multigrid.cpp:146-152 (power-iteration start
+ * vectors for every non-coarsest level, concatenated) */
+int smg_mg_set_symmetric(smg_mg* mg, double damping, const double* starts);
+/* MGHierarchy::precondition, src/multigrid.cpp:155-157; the cycle runs in prec */
+int smg_mg_precondition(smg_mg* mg, int prec, const double* b, double* x);
+int smg_mg_precondition_dev(smg_mg* mg, int prec, const void* b, void* x, void* stream);
+int smg_mg_level_dim(smg_mg* mg, int level, size_t* n);
+int smg_mg_null_vecs(smg_mg* mg, int level, double* out);
+int smg_mg_block_vecs(smg_mg* mg, int level, double* out);
+int smg_mg_coarse_links(smg_mg* mg, int level, double* out);
+int smg_mg_omega(smg_mg* mg, int level, double* out);
+/* change the cycle's smoothing steps and coarsest-solve budget after setup */
+int smg_mg_set_cycle(smg_mg* mg, int pre_smooth, int post_smooth, int coarse_maxit, double coarse_tol);
+
+/* ------------------------------------------------------------------ Krylov solvers */
+typedef struct {
+    int iterations;
+    double final_residual; /* the solver's own residual, as the reference reports it */
+    double true_residual;  /* |b - A x|/|b| recomputed in FP64 on the device */
+    int restarts;          /* outer FP64 refinements (mixed precision) */
+} smg_solve_info;
+/* cg_solve / cg_solve_x0 / cg_solve_precond / cg_solve_x0_precond, src/solvers.cpp:16-96
+ * (x0 and mg may be NULL); op = SMG_OP_DDAGD or SMG_OP_SCHUR (mg must be NULL) */
+int smg_cg_solve(smg_dirac* D, int op, smg_mg* mg, const double* b, const double* x0, double* x, int maxit,
+                 double tol, smg_solve_info* info);
+/* fcg_solve / fcg_solve_mg (truncated GCR), src/solvers.cpp:223-300, 803-811.
+ * prec = SMG_F64: FP64 GCR(trunc) with the FP64 cycle.  prec = SMG_F32: mixed
+ * precision -- FP64 iterative refinement on the FP64 true residual around FP32
+ * GCR(trunc) + FP32 (optionally FP16-link) cycle, each inner solve to inner_tol.
+ * ref_projection = 1 uses the reference's conjugated projection coefficient
+ * (src/solvers.cpp:261); 0 the textbook GCR coefficient. */
+int smg_gcr_solve(smg_dirac* D, smg_mg* mg, int prec, const double* b, double* x, int maxit, double tol, int trunc,
+                  int ref_projection, double inner_tol, smg_solve_info* info);
+int smg_gcr_solve_dev(smg_dirac* D, smg_mg* mg, int prec, const void* b, void* x, int maxit, double tol, int trunc,
+                      double inner_tol, smg_solve_info* info, void* stream);
+/* fgmres_solve_generic / fgmres_solve_mg with n_ritz_harvest = 0, src/solvers.cpp:555-800 */
+int smg_fgmres_solve(smg_dirac* D, smg_mg* mg, const double* b, double* x, int restart, int maxit, double tol,
+                     smg_solve_info* info);
+
+/* ------------------------------------------------------------------ forces (F = -dS/dtheta) */
+/* forward half of fermion_force, src/hmc_forces.cpp:134-139: chi = D X */
+int smg_force_forward(smg_dirac* D, const double* X, double* chi);
+/* backward half: fermion_force_bilinear (hopping + clover), src/hmc_forces.cpp:141-495 */
+int smg_force_backward(smg_dirac* D, const double* chi, const double* X, double* F);
+/* fermion_force, src/hmc_forces.cpp:134-139: forward + backward with chi kept on the device */
+int smg_fermion_force(smg_dirac* D, const double* X, double* F);
+int smg_fermion_force_dev(smg_dirac* D, const void* X, void* F, void* stream);
+/* hopping_force_bilinear, src/hmc_forces.cpp:540-569 */
+int smg_hopping_force(smg_dirac* D, const double* chi, const double* X, double* F);
+/* logdet_ee_force, src/hmc_forces.cpp:578-637 */
+int smg_logdet_force(smg_dirac* D, double* F);
+/* eo_fermion_force, src/hmc_forces.cpp:830-867 (x_o, y_o half-lattice) */
+int smg_eo_fermion_force(smg_dirac* D, const double* x_o, const double* y_o, double* F);
+/* fermion_action, src/hmc_forces.cpp:113-119: S = Re phi^dag (D^dag D)^-1 phi via CG */
+int smg_fermion_action(smg_dirac* D, const double* phi, int maxit, double tol, double* S, double* X);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

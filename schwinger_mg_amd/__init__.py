@@ -1,0 +1,586 @@
+"""schwinger_mg_amd -- MI355X (gfx950) Wilson-clover operator, adaptive multigrid and
+HMC fermion force for the 2D Schwinger model.
+
+The compute path is libsmg.so (hand-written HIP kernels behind the C-ABI in
+include/smg.h).  This module is the Python host mirror of the reference's C++
+operator API: ``DiracOp`` and ``EvenOddDiracOp`` (src/dirac.h), ``Prolongator``
+(src/prolongator.h), ``SparseCoarseOp`` (src/coarse_op.h), ``MGHierarchy`` /
+``build_mg_hierarchy`` (src/multigrid.h), the smoothers (src/smoother.h), the
+Krylov entry points (src/solvers.h) and the force routines (src/hmc.h), with the
+same names, argument meaning and layouts.  Vectors are numpy complex128 arrays
+in the reference layouts; every call runs on the GPU.  There is no CPU fallback:
+if libsmg.so is missing or no GPU is present the first call raises.
+"""
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsmg.so")
+
+F64, F32, F16 = 0, 1, 2
+D_, DDAG, DDAGD, G5D = 0, 1, 2, 3
+OP_DDAGD, OP_SCHUR = 0, 1
+SMOOTH_MR, SMOOTH_RICHARDSON = 0, 1
+EO_HOP_EO, EO_HOP_OE, EO_EE_INV, EO_OO_INV, EO_OO, EO_SCHUR, EO_SCHUR_DAG, EO_SCHUR_DAG_SCHUR = range(8)
+
+_P, _I, _D, _SZ = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_size_t
+
+
+class SMGError(RuntimeError):
+    pass
+
+
+class MGParams(ctypes.Structure):
+    _fields_ = [("n_levels", _I), ("bx", _I * 4), ("by", _I * 4), ("k", _I * 4), ("null_iters", _I),
+                ("pre_smooth", _I), ("post_smooth", _I), ("w_cycle", _I), ("coarse_maxit", _I),
+                ("coarse_tol", _D), ("coarse_half", _I)]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("iterations", _I), ("final_residual", _D), ("true_residual", _D), ("restarts", _I)]
+
+
+# (name, restype, argtypes) of every entry point declared in include/smg.h
+SIGNATURES = [
+    ("smg_init", _I, [_I]), ("smg_last_error", ctypes.c_char_p, []), ("smg_version", _I, []),
+    ("smg_synchronize", _I, []), ("smg_get_stream", _I, [_P]),
+    ("smg_eo_maps", _I, [_I, _P, _P, _P]), ("smg_avg_plaq", _I, [_I, _P, _P]),
+    ("smg_gauge_action", _I, [_I, _P, _D, _P]), ("smg_gauge_force", _I, [_I, _P, _D, _P]),
+    ("smg_dirac_create", _I, [_P, _I, _P, _D, _D, _D, _D]), ("smg_dirac_destroy", _I, [_P]),
+    ("smg_dirac_set_gauge", _I, [_P, _P]), ("smg_dirac_clover", _I, [_P, _P]),
+    ("smg_dirac_apply", _I, [_P, _I, _P, _P]), ("smg_dirac_apply_dev", _I, [_P, _I, _I, _P, _P, _I, _P]),
+    ("smg_dirac_delta", _I, [_P, _P, _P, _P, _D]),
+    ("smg_eo_apply", _I, [_P, _I, _P, _P]), ("smg_eo_apply_dev", _I, [_P, _I, _I, _P, _P, _P]),
+    ("smg_eo_reconstruct", _I, [_P, _P, _P, _P]), ("smg_eo_logdet", _I, [_P, _P]),
+    ("smg_dot_dev", _I, [_P, _P, _SZ, _I, _P, _P]), ("smg_norm_dev", _I, [_P, _SZ, _I, _P, _P]),
+    ("smg_prolongator_create", _I, [_P, _I, _I, _I, _I, _I, _I]), ("smg_prolongator_destroy", _I, [_P]),
+    ("smg_prolongator_build", _I, [_P, _P]), ("smg_prolongator_get", _I, [_P, _P]),
+    ("smg_prolongator_set", _I, [_P, _P]), ("smg_prolong", _I, [_P, _P, _P]), ("smg_restrict", _I, [_P, _P, _P]),
+    ("smg_coarse_create_galerkin", _I, [_P, _P, _P]), ("smg_coarse_create_from_coarse", _I, [_P, _P, _P]),
+    ("smg_coarse_create_links", _I, [_P, _I, _I, _I, _P]), ("smg_coarse_destroy", _I, [_P]),
+    ("smg_coarse_get_links", _I, [_P, _P]), ("smg_coarse_apply", _I, [_P, _I, _P, _P]),
+    ("smg_coarse_solve", _I, [_P, _P, _P, _I, _D, _P]),
+    ("smg_smooth_dirac", _I, [_P, _I, _P, _P, _I, _D]), ("smg_smooth_coarse", _I, [_P, _I, _P, _P, _I, _D]),
+    ("smg_lambda_max_dirac", _I, [_P, _P, _I, _P]),
+    ("smg_mg_setup", _I, [_P, _P, _P, _P]), ("smg_mg_destroy", _I, [_P]),
+    ("smg_mg_set_symmetric", _I, [_P, _D, _P]), ("smg_mg_precondition", _I, [_P, _I, _P, _P]),
+    ("smg_mg_precondition_dev", _I, [_P, _I, _P, _P, _P]), ("smg_mg_level_dim", _I, [_P, _I, _P]),
+    ("smg_mg_null_vecs", _I, [_P, _I, _P]), ("smg_mg_block_vecs", _I, [_P, _I, _P]),
+    ("smg_mg_coarse_links", _I, [_P, _I, _P]), ("smg_mg_omega", _I, [_P, _I, _P]),
+    ("smg_mg_set_cycle", _I, [_P, _I, _I, _I, _D]),
+    ("smg_cg_solve", _I, [_P, _I, _P, _P, _P, _P, _I, _D, _P]),
+    ("smg_gcr_solve", _I, [_P, _P, _I, _P, _P, _I, _D, _I, _I, _D, _P]),
+    ("smg_gcr_solve_dev", _I, [_P, _P, _I, _P, _P, _I, _D, _I, _D, _P, _P]),
+    ("smg_fgmres_solve", _I, [_P, _P, _P, _P, _I, _I, _D, _P]),
+    ("smg_force_forward", _I, [_P, _P, _P]), ("smg_force_backward", _I, [_P, _P, _P, _P]),
+    ("smg_fermion_force", _I, [_P, _P, _P]), ("smg_fermion_force_dev", _I, [_P, _P, _P, _P]),
+    ("smg_hopping_force", _I, [_P, _P, _P, _P]), ("smg_logdet_force", _I, [_P, _P]),
+    ("smg_eo_fermion_force", _I, [_P, _P, _P, _P]), ("smg_fermion_action", _I, [_P, _P, _I, _D, _P, _P]),
+]
+
+_lib = None
+
+
+def load_library():
+    """Load libsmg.so and bind every C-ABI entry point (no GPU call)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SMGError(f"{LIB_PATH} is not built; run `python -m schwinger_mg_amd.build`")
+        # One HIP runtime per process: PyTorch ships its own libamdhip64 with the
+        # same soname (libamdhip64.so.7) as /opt/rocm's.  Loading torch first makes
+        # libsmg.so bind to the runtime torch already mapped, so device pointers
+        # and streams are shared instead of two runtimes competing for the GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+_initialised = False
+
+
+def lib():
+    global _initialised
+    L = load_library()
+    if not _initialised:
+        rc = L.smg_init(0)
+        if rc != 0:
+            raise SMGError("smg_init failed: " + L.smg_last_error().decode())
+        _initialised = True
+    return L
+
+
+def _chk(rc):
+    if rc != 0:
+        raise SMGError(lib().smg_last_error().decode())
+
+
+def _c(a, dt=np.complex128):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def _p(a):
+    return a.ctypes.data_as(_P) if a is not None else None
+
+
+def synchronize():
+    _chk(lib().smg_synchronize())
+
+
+def library_stream():
+    """The hipStream_t (as int) the library launches on."""
+    s = _P()
+    _chk(lib().smg_get_stream(ctypes.byref(s)))
+    return s.value or 0
+
+
+# ------------------------------------------------------------------ gauge
+def avg_plaq(L, U):
+    out = _D()
+    _chk(lib().smg_avg_plaq(L, _p(_c(U)), ctypes.byref(out)))
+    return out.value
+
+
+def gauge_action(L, U, beta):
+    out = _D()
+    _chk(lib().smg_gauge_action(L, _p(_c(U)), beta, ctypes.byref(out)))
+    return out.value
+
+
+def gauge_force(L, U, beta):
+    F = np.zeros(2 * L * L)
+    _chk(lib().smg_gauge_force(L, _p(_c(U)), beta, _p(F)))
+    return F
+
+
+def eo_maps(L):
+    V = L * L
+    e = np.zeros(V // 2, np.int32); o = np.zeros(V // 2, np.int32); i = np.zeros(V, np.int32)
+    _chk(lib().smg_eo_maps(L, _p(e), _p(o), _p(i)))
+    return e, o, i
+
+
+# ------------------------------------------------------------------ Dirac
+class DiracOp:
+    """DiracOp(lat, gauge, m, r, csw, mu_t) -- src/dirac.h:8-183."""
+
+    def __init__(self, L, U, mass, r=1.0, csw=0.0, mu_t=0.0):
+        self.L, self.V = L, L * L
+        self.ndof = 2 * self.V
+        self.mass, self.r, self.c_sw, self.mu_t = mass, r, csw, mu_t
+        h = _P()
+        _chk(lib().smg_dirac_create(ctypes.byref(h), L, _p(_c(U)), mass, r, csw, mu_t))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.smg_dirac_destroy(self.h)
+            self.h = None
+
+    def set_gauge(self, U):
+        _chk(lib().smg_dirac_set_gauge(self.h, _p(_c(U))))
+
+    @property
+    def clover_field(self):
+        F = np.zeros(self.V)
+        _chk(lib().smg_dirac_clover(self.h, _p(F)))
+        return F
+
+    def _apply(self, kind, src):
+        dst = np.zeros(self.ndof, np.complex128)
+        _chk(lib().smg_dirac_apply(self.h, kind, _p(_c(src)), _p(dst)))
+        return dst
+
+    def apply(self, src):
+        return self._apply(D_, src)
+
+    def apply_dag(self, src):
+        return self._apply(DDAG, src)
+
+    def apply_DdagD(self, src):
+        return self._apply(DDAGD, src)
+
+    def apply_g5D(self, src):
+        return self._apply(G5D, src)
+
+    def apply_delta_D(self, src, pi, dt):
+        dst = np.zeros(self.ndof, np.complex128)
+        _chk(lib().smg_dirac_delta(self.h, _p(_c(src)), _p(dst), _p(_c(pi, np.float64)), dt))
+        return dst
+
+    def apply_dev(self, kind, prec, src_ptr, dst_ptr, nrhs=1, stream=None):
+        _chk(lib().smg_dirac_apply_dev(self.h, kind, prec, src_ptr, dst_ptr, nrhs, stream))
+
+    def as_DdagD_op(self):
+        return self.apply_DdagD
+
+
+class EvenOddDiracOp:
+    """EvenOddDiracOp(D) -- src/dirac.h:190-553 (half-lattice vectors of V complex)."""
+
+    def __init__(self, D):
+        self.D = D
+        self.nh = D.V
+
+    def _eo(self, kind, src):
+        dst = np.zeros(self.nh, np.complex128)
+        _chk(lib().smg_eo_apply(self.D.h, kind, _p(_c(src)), _p(dst)))
+        return dst
+
+    def apply_eo(self, src_e):
+        return self._eo(EO_HOP_EO, src_e)
+
+    def apply_oe(self, src_o):
+        return self._eo(EO_HOP_OE, src_o)
+
+    def apply_ee_inv(self, src):
+        return self._eo(EO_EE_INV, src)
+
+    def apply_oo_inv(self, src):
+        return self._eo(EO_OO_INV, src)
+
+    def apply_oo(self, src):
+        return self._eo(EO_OO, src)
+
+    def apply_schur(self, src):
+        return self._eo(EO_SCHUR, src)
+
+    def apply_schur_dag(self, src):
+        return self._eo(EO_SCHUR_DAG, src)
+
+    def apply_schur_dag_schur(self, src):
+        return self._eo(EO_SCHUR_DAG_SCHUR, src)
+
+    def reconstruct(self, x_o, phi):
+        out = np.zeros(self.D.ndof, np.complex128)
+        _chk(lib().smg_eo_reconstruct(self.D.h, _p(_c(x_o)), _p(_c(phi)), _p(out)))
+        return out
+
+    def reconstruct_full(self, x_o):
+        out = np.zeros(self.D.ndof, np.complex128)
+        _chk(lib().smg_eo_reconstruct(self.D.h, _p(_c(x_o)), None, _p(out)))
+        return out
+
+    def log_det_ee(self):
+        out = _D()
+        _chk(lib().smg_eo_logdet(self.D.h, ctypes.byref(out)))
+        return out.value
+
+
+# ------------------------------------------------------------------ prolongator / coarse op
+class Prolongator:
+    """Prolongator(lat, bx, by, k) -- src/prolongator.h:14-142, with nd dof per site
+    (nd = 2 on the fine lattice; nd = k of the level above on a coarse lattice,
+    which also covers CoarseProlongator, src/prolongator.h:149-228)."""
+
+    def __init__(self, nx, ny, nd, bx, by, k):
+        self.nx, self.ny, self.nd, self.bx, self.by, self.k_vec = nx, ny, nd, bx, by, k
+        self.nbx, self.nby = nx // bx, ny // by
+        self.nblocks = self.nbx * self.nby
+        self.coarse_dim = self.nblocks * k
+        self.fine_dim = nx * ny * nd
+        h = _P()
+        _chk(lib().smg_prolongator_create(ctypes.byref(h), nx, ny, nd, bx, by, k))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.smg_prolongator_destroy(self.h)
+            self.h = None
+
+    def build_from_vectors(self, null_vecs):
+        nv = _c(np.asarray(null_vecs)[: self.k_vec]).ravel()
+        _chk(lib().smg_prolongator_build(self.h, _p(nv)))
+        return self
+
+    @property
+    def block_vecs(self):
+        out = np.zeros(self.nblocks * self.k_vec * self.bx * self.by * self.nd, np.complex128)
+        _chk(lib().smg_prolongator_get(self.h, _p(out)))
+        return out
+
+    def set_block_vecs(self, bv):
+        _chk(lib().smg_prolongator_set(self.h, _p(_c(bv))))
+
+    def prolong(self, coarse):
+        out = np.zeros(self.fine_dim, np.complex128)
+        _chk(lib().smg_prolong(self.h, _p(_c(coarse)), _p(out)))
+        return out
+
+    def restrict_vec(self, fine):
+        out = np.zeros(self.coarse_dim, np.complex128)
+        _chk(lib().smg_restrict(self.h, _p(_c(fine)), _p(out)))
+        return out
+
+
+class SparseCoarseOp:
+    """9-point coarse operator -- src/coarse_op.h:154-548 (built by Galerkin projection)."""
+
+    def __init__(self, handle, nx, ny, k, keep=()):
+        self.h, self.nbx, self.nby, self.k_vec = handle, nx, ny, k
+        self.nblocks = nx * ny
+        self.dim = self.nblocks * k
+        self._keep = keep
+
+    @classmethod
+    def build(cls, P, D):
+        """A_c = P^dag (D^dag D) P (src/coarse_op.h:241-279 with A = D.as_DdagD_op())."""
+        h = _P()
+        _chk(lib().smg_coarse_create_galerkin(ctypes.byref(h), D.h, P.h))
+        return cls(h, P.nbx, P.nby, P.k_vec)
+
+    @classmethod
+    def build_from_coarse(cls, P, A):
+        h = _P()
+        _chk(lib().smg_coarse_create_from_coarse(ctypes.byref(h), A.h, P.h))
+        return cls(h, P.nbx, P.nby, P.k_vec)
+
+    @classmethod
+    def from_links(cls, nx, ny, k, links):
+        h = _P()
+        _chk(lib().smg_coarse_create_links(ctypes.byref(h), nx, ny, k, _p(_c(links))))
+        return cls(h, nx, ny, k)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.smg_coarse_destroy(self.h)
+            self.h = None
+
+    @property
+    def link(self):
+        out = np.zeros(self.nblocks * 9 * self.k_vec ** 2, np.complex128)
+        _chk(lib().smg_coarse_get_links(self.h, _p(out)))
+        return out
+
+    def apply_to(self, src, prec=F64):
+        out = np.zeros(self.dim, np.complex128)
+        _chk(lib().smg_coarse_apply(self.h, prec, _p(_c(src)), _p(out)))
+        return out
+
+    def solve(self, b, max_iter=200, cg_tol=1e-12):
+        x = np.zeros(self.dim, np.complex128)
+        it = _I()
+        _chk(lib().smg_coarse_solve(self.h, _p(_c(b)), _p(x), max_iter, cg_tol, ctypes.byref(it)))
+        self.last_iterations = it.value
+        return x
+
+    def as_op(self):
+        return self.apply_to
+
+
+# ------------------------------------------------------------------ smoothers
+def mr_smooth(D, x, b, niter):
+    """mr_smooth(D, x, b, niter) -- src/smoother.cpp:9-23; returns the updated x."""
+    x = _c(x).copy()
+    _chk(lib().smg_smooth_dirac(D.h, SMOOTH_MR, _p(x), _p(_c(b)), niter, 0.0))
+    return x
+
+
+def richardson_smooth(D, x, b, niter, omega):
+    x = _c(x).copy()
+    _chk(lib().smg_smooth_dirac(D.h, SMOOTH_RICHARDSON, _p(x), _p(_c(b)), niter, omega))
+    return x
+
+
+def mr_smooth_op(C, x, b, niter):
+    """mr_smooth_op on a SparseCoarseOp -- src/smoother.cpp:26-40."""
+    x = _c(x).copy()
+    _chk(lib().smg_smooth_coarse(C.h, SMOOTH_MR, _p(x), _p(_c(b)), niter, 0.0))
+    return x
+
+
+def richardson_smooth_op(C, x, b, niter, omega):
+    x = _c(x).copy()
+    _chk(lib().smg_smooth_coarse(C.h, SMOOTH_RICHARDSON, _p(x), _p(_c(b)), niter, omega))
+    return x
+
+
+def estimate_lambda_max(D, v0, niter):
+    """estimate_lambda_max of D^dag D (src/smoother.cpp:43-59) from start vector v0."""
+    out = _D()
+    _chk(lib().smg_lambda_max_dirac(D.h, _p(_c(v0)), niter, ctypes.byref(out)))
+    return out.value
+
+
+# ------------------------------------------------------------------ multigrid
+class MGHierarchy:
+    """MGHierarchy (src/multigrid.h:41-107) built by build_mg_hierarchy."""
+
+    def __init__(self, D, starts, n_levels=2, blocks=((4, 4),), k=(4,), null_iters=20, pre_smooth=3,
+                 post_smooth=3, w_cycle=True, coarse_maxit=200, coarse_tol=1e-12, coarse_half=False):
+        p = MGParams()
+        p.n_levels = n_levels
+        blocks = list(blocks) + [blocks[-1]] * (4 - len(blocks))
+        k = list(k) + [k[-1]] * (4 - len(k))
+        for i in range(4):
+            p.bx[i], p.by[i] = blocks[i]
+            p.k[i] = k[i]
+        p.null_iters, p.pre_smooth, p.post_smooth = null_iters, pre_smooth, post_smooth
+        p.w_cycle, p.coarse_maxit, p.coarse_tol, p.coarse_half = int(w_cycle), coarse_maxit, coarse_tol, int(coarse_half)
+        self.params, self.D, self.n_levels = p, D, n_levels
+        h = _P()
+        _chk(lib().smg_mg_setup(ctypes.byref(h), D.h, ctypes.byref(p), _p(_c(starts))))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.smg_mg_destroy(self.h)
+            self.h = None
+
+    def level_dim(self, level):
+        n = _SZ()
+        _chk(lib().smg_mg_level_dim(self.h, level, ctypes.byref(n)))
+        return n.value
+
+    def set_symmetric(self, damping, starts):
+        """set_symmetric (src/multigrid.cpp:146-152) with explicit power-iteration starts."""
+        _chk(lib().smg_mg_set_symmetric(self.h, damping, _p(_c(starts))))
+
+    def precondition(self, b, prec=F64):
+        x = np.zeros(self.D.ndof, np.complex128)
+        _chk(lib().smg_mg_precondition(self.h, prec, _p(_c(b)), _p(x)))
+        return x
+
+    def null_vecs(self, level=0):
+        n = self.level_dim(level)
+        out = np.zeros(self.params.k[level] * n, np.complex128)
+        _chk(lib().smg_mg_null_vecs(self.h, level, _p(out)))
+        return out.reshape(self.params.k[level], n)
+
+    def block_vecs(self, level=0):
+        nc = self.level_dim(level + 1)
+        nf = self.level_dim(level)
+        out = np.zeros(nc // self.params.k[level] * self.params.k[level] * (nf * self.params.k[level] // nc),
+                       np.complex128)
+        _chk(lib().smg_mg_block_vecs(self.h, level, _p(out)))
+        return out
+
+    def coarse_links(self, level=0):
+        k = self.params.k[level]
+        nb = self.level_dim(level + 1) // k
+        out = np.zeros(nb * 9 * k * k, np.complex128)
+        _chk(lib().smg_mg_coarse_links(self.h, level, _p(out)))
+        return out
+
+    def set_cycle(self, pre_smooth, post_smooth, coarse_maxit, coarse_tol):
+        _chk(lib().smg_mg_set_cycle(self.h, pre_smooth, post_smooth, coarse_maxit, coarse_tol))
+
+    def omega(self, level):
+        out = _D()
+        _chk(lib().smg_mg_omega(self.h, level, ctypes.byref(out)))
+        return out.value
+
+
+def build_mg_hierarchy(D, n_levels, block_size, k_null, null_iters, starts, w_cycle=True, pre_smooth=3,
+                       post_smooth=3, coarse_blocks=None, **kw):
+    """build_mg_hierarchy (src/multigrid.cpp:595-687) with explicit null-space starts."""
+    blocks = [(block_size, block_size)] + list(coarse_blocks or [(2, 2)] * 3)
+    return MGHierarchy(D, starts, n_levels=n_levels, blocks=blocks[:4], k=(k_null,) * 4, null_iters=null_iters,
+                       pre_smooth=pre_smooth, post_smooth=post_smooth, w_cycle=w_cycle, **kw)
+
+
+# ------------------------------------------------------------------ solvers
+@dataclass
+class SolveResult:
+    solution: np.ndarray
+    iterations: int
+    final_residual: float
+    true_residual: float
+    restarts: int = 0
+
+
+def _res(x, info):
+    return SolveResult(x, info.iterations, info.final_residual, info.true_residual, info.restarts)
+
+
+def cg_solve(D, rhs, max_iter, tol, x0=None, mg=None, op=OP_DDAGD):
+    """cg_solve / cg_solve_x0 / cg_solve_precond (src/solvers.cpp:16-96)."""
+    n = D.ndof if op == OP_DDAGD else D.V
+    x = np.zeros(n, np.complex128)
+    info = SolveInfo()
+    _chk(lib().smg_cg_solve(D.h, op, mg.h if mg else None, _p(_c(rhs)), _p(_c(x0)) if x0 is not None else None,
+                            _p(x), max_iter, tol, ctypes.byref(info)))
+    return _res(x, info)
+
+
+def fcg_solve(D, rhs, mg, max_iter, tol, trunc=30, prec=F64, ref_projection=False, inner_tol=1e-5):
+    """fcg_solve / fcg_solve_mg (truncated GCR, src/solvers.cpp:223-300, 803-811)."""
+    x = np.zeros(D.ndof, np.complex128)
+    info = SolveInfo()
+    _chk(lib().smg_gcr_solve(D.h, mg.h if mg else None, prec, _p(_c(rhs)), _p(x), max_iter, tol, trunc,
+                             int(ref_projection), inner_tol, ctypes.byref(info)))
+    return _res(x, info)
+
+
+gcr_solve = fcg_solve
+
+
+def fgmres_solve(D, rhs, mg, restart, max_iter, tol):
+    """fgmres_solve_generic / fgmres_solve_mg with no Ritz harvest (src/solvers.cpp:555-800)."""
+    x = np.zeros(D.ndof, np.complex128)
+    info = SolveInfo()
+    _chk(lib().smg_fgmres_solve(D.h, mg.h if mg else None, _p(_c(rhs)), _p(x), restart, max_iter, tol,
+                                ctypes.byref(info)))
+    return _res(x, info)
+
+
+# ------------------------------------------------------------------ forces
+def fermion_force(D, X):
+    """fermion_force (src/hmc_forces.cpp:134-139): F = -dS_f/dtheta, F[mu*V + s]."""
+    F = np.zeros(2 * D.V)
+    _chk(lib().smg_fermion_force(D.h, _p(_c(X)), _p(F)))
+    return F
+
+
+def force_forward(D, X):
+    chi = np.zeros(D.ndof, np.complex128)
+    _chk(lib().smg_force_forward(D.h, _p(_c(X)), _p(chi)))
+    return chi
+
+
+def fermion_force_bilinear(D, chi, X):
+    """fermion_force_bilinear (src/hmc_forces.cpp:141-194) -- the backward half."""
+    F = np.zeros(2 * D.V)
+    _chk(lib().smg_force_backward(D.h, _p(_c(chi)), _p(_c(X)), _p(F)))
+    return F
+
+
+force_backward = fermion_force_bilinear
+
+
+def hopping_force_bilinear(D, chi, X):
+    F = np.zeros(2 * D.V)
+    _chk(lib().smg_hopping_force(D.h, _p(_c(chi)), _p(_c(X)), _p(F)))
+    return F
+
+
+def logdet_ee_force(D):
+    F = np.zeros(2 * D.V)
+    _chk(lib().smg_logdet_force(D.h, _p(F)))
+    return F
+
+
+def eo_fermion_force(D, x_o, y_o):
+    """eo_fermion_force (src/hmc_forces.cpp:830-867)."""
+    F = np.zeros(2 * D.V)
+    _chk(lib().smg_eo_fermion_force(D.h, _p(_c(x_o)), _p(_c(y_o)), _p(F)))
+    return F
+
+
+def fermion_action(D, phi, max_iter, tol):
+    """fermion_action (src/hmc_forces.cpp:113-119): returns (S, X)."""
+    S = _D()
+    X = np.zeros(D.ndof, np.complex128)
+    _chk(lib().smg_fermion_action(D.h, _p(_c(phi)), max_iter, tol, ctypes.byref(S), _p(X)))
+    return S.value, X

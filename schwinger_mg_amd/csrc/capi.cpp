@@ -1,0 +1,789 @@
+// capi.cpp -- the extern "C" boundary (include/smg.h) over the C++ runtime.
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "../../include/smg.h"
+#include "runtime.h"
+
+using namespace smg;
+
+struct smg_dirac_s {
+    std::unique_ptr<Dirac> D;
+    std::unique_ptr<DdagDOp<double>> op64;
+    std::unique_ptr<DdagDOp<float>> op32;
+    std::unique_ptr<SchurOp<double>> schur64;
+    std::unique_ptr<SchurOp<float>> schur32;
+};
+struct smg_prolongator_s {
+    std::unique_ptr<Aggregation> P;
+};
+struct smg_coarse_s {
+    std::unique_ptr<CoarseOperator> C;
+};
+struct smg_mg_s {
+    std::unique_ptr<MG> mg;
+    smg_dirac* D;
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -3;
+    } catch (...) {
+        g_err = "unknown error";
+        return -3;
+    }
+}
+
+hipStream_t S() { return Context::get().stream; }
+
+Field<double> up(const double* h, size_t n) {
+    SMG_REQUIRE(h != nullptr, "null input array");
+    Field<double> f(n);
+    SMG_HIP(hipMemcpyAsync(f.p(), h, n * sizeof(c64), hipMemcpyHostToDevice, S()));
+    return f;
+}
+void down(double* h, const void* d, size_t bytes) {
+    SMG_REQUIRE(h != nullptr, "null output array");
+    SMG_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, S()));
+    SMG_HIP(hipStreamSynchronize(S()));
+}
+void downc(double* h, const c64* d, size_t n) { down(h, d, n * sizeof(c64)); }
+
+// Orders the library stream after the caller's stream on entry, and the caller's
+// stream after the library's work on exit.
+struct StreamJoin {
+    hipStream_t user;
+    explicit StreamJoin(void* s) : user(reinterpret_cast<hipStream_t>(s)) {
+        hipEvent_t ev;
+        SMG_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        SMG_HIP(hipEventRecord(ev, user));
+        SMG_HIP(hipStreamWaitEvent(S(), ev, 0));
+        SMG_HIP(hipEventDestroy(ev));
+    }
+    ~StreamJoin() {
+        hipEvent_t ev;
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess) {
+            (void)hipEventRecord(ev, S());
+            (void)hipStreamWaitEvent(user, ev, 0);
+            (void)hipEventDestroy(ev);
+        }
+    }
+};
+
+void check_prec(int prec) { SMG_REQUIRE(prec == SMG_F64 || prec == SMG_F32, "prec must be SMG_F64 or SMG_F32"); }
+}  // namespace
+
+// Every smg_* definition below has C linkage from its declaration in smg.h.
+
+int smg_init(int device) {
+    return guard([&] {
+        SMG_HIP(hipSetDevice(device));
+        Context::get();
+    });
+}
+const char* smg_last_error(void) { return g_err.c_str(); }
+int smg_version(void) { return 1; }
+int smg_synchronize(void) { return guard([] { Context::get().sync(); }); }
+int smg_get_stream(void** stream) {
+    return guard([&] {
+        SMG_REQUIRE(stream, "null argument");
+        *stream = reinterpret_cast<void*>(Context::get().stream);
+    });
+}
+
+int smg_eo_maps(int L, int* even, int* odd, int* eoi) {
+    return guard([&] {
+        SMG_REQUIRE(L >= 2 && even && odd && eoi, "bad arguments");
+        int ne = 0, no = 0;
+        for (int s = 0; s < L * L; s++) {
+            if (((s % L) + (s / L)) % 2 == 0) { eoi[s] = ne; even[ne++] = s; }
+            else { eoi[s] = no; odd[no++] = s; }
+        }
+    });
+}
+
+static double reduce_read(int g, hipStream_t st) {
+    Context& cx = Context::get();
+    DevMem out(8 * sizeof(double));
+    launch_finalize(cx.part_ptr(), g, 1, out.as<double>(), FIN_STORE, nullptr, st);
+    double h;
+    cx.read(out.as<double>(), &h, 1);
+    return h;
+}
+
+int smg_avg_plaq(int L, const double* U, double* out) {
+    return guard([&] {
+        Field<double> u = up(U, 2 * (size_t)L * L);
+        int g = launch_plaq_sum(L, u.p(), Context::get().part_ptr(), S());
+        *out = reduce_read(g, S()) / ((double)L * L);
+    });
+}
+int smg_gauge_action(int L, const double* U, double beta, double* out) {
+    return guard([&] {
+        Field<double> u = up(U, 2 * (size_t)L * L);
+        int g = launch_plaq_sum(L, u.p(), Context::get().part_ptr(), S());
+        *out = -beta * reduce_read(g, S());
+    });
+}
+int smg_gauge_force(int L, const double* U, double beta, double* F) {
+    return guard([&] {
+        Field<double> u = up(U, 2 * (size_t)L * L);
+        DevMem f(2 * (size_t)L * L * sizeof(double));
+        launch_gauge_force(L, u.p(), beta, f.as<double>(), S());
+        down(F, f.p, 2 * (size_t)L * L * sizeof(double));
+    });
+}
+
+// ------------------------------------------------------------------ Dirac
+int smg_dirac_create(smg_dirac** out, int L, const double* U, double mass, double r, double csw, double mu_t) {
+    return guard([&] {
+        SMG_REQUIRE(out && U, "null argument");
+        SMG_REQUIRE(L >= 2, "L must be >= 2");
+        auto* h = new smg_dirac_s();
+        try {
+            h->D.reset(new Dirac(L, reinterpret_cast<const c64*>(U), mass, r, csw, mu_t));
+            h->op64.reset(new DdagDOp<double>(*h->D));
+            h->op32.reset(new DdagDOp<float>(*h->D));
+            if (L % 2 == 0) {
+                h->schur64.reset(new SchurOp<double>(*h->D));
+                h->schur32.reset(new SchurOp<float>(*h->D));
+            }
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+int smg_dirac_destroy(smg_dirac* D) {
+    return guard([&] { delete D; });
+}
+int smg_dirac_set_gauge(smg_dirac* D, const double* U) {
+    return guard([&] {
+        SMG_REQUIRE(D && U, "null argument");
+        D->D->set_gauge(reinterpret_cast<const c64*>(U), false);
+    });
+}
+int smg_dirac_clover(smg_dirac* D, double* F01) {
+    return guard([&] {
+        SMG_REQUIRE(D, "null handle");
+        down(F01, D->D->F01.p, D->D->V * sizeof(double));
+    });
+}
+
+template <typename T>
+static void dirac_kind(const Dirac& d, LinOp<T>& op, int kind, const cplx<T>* src, cplx<T>* dst, int nrhs,
+                       hipStream_t st) {
+    const DiracView<T> v = d.view<T>();
+    const size_t n = 2 * (size_t)d.V;
+    const T mu = (T)d.mu_t;
+    switch (kind) {
+    case SMG_D:
+        launch_dslash<T>(v, dst, src, mu, 0, 0, EPI_NONE, nullptr, nullptr, nullptr, nrhs, n, st);
+        break;
+    case SMG_DDAG:
+        launch_dslash<T>(v, dst, src, -mu, 1, 1, EPI_NONE, nullptr, nullptr, nullptr, nrhs, n, st);
+        break;
+    case SMG_G5D:
+        launch_dslash<T>(v, dst, src, mu, 0, 1, EPI_NONE, nullptr, nullptr, nullptr, nrhs, n, st);
+        break;
+    case SMG_DDAGD:
+        for (int i = 0; i < nrhs; i++) op.apply(dst + i * n, src + i * n, EPI_NONE, nullptr, nullptr, nullptr, st);
+        break;
+    default:
+        SMG_REQUIRE(false, "unknown Dirac kind");
+    }
+}
+
+int smg_dirac_apply(smg_dirac* D, int kind, const double* src, double* dst) {
+    return guard([&] {
+        SMG_REQUIRE(D, "null handle");
+        const size_t n = 2 * (size_t)D->D->V;
+        Field<double> s = up(src, n), d(n);
+        dirac_kind<double>(*D->D, *D->op64, kind, s.p(), d.p(), 1, S());
+        downc(dst, d.p(), n);
+    });
+}
+int smg_dirac_apply_dev(smg_dirac* D, int kind, int prec, const void* src, void* dst, int nrhs, void* stream) {
+    return guard([&] {
+        SMG_REQUIRE(D && src && dst && nrhs >= 1, "bad arguments");
+        check_prec(prec);
+        StreamJoin j(stream);
+        if (prec == SMG_F64)
+            dirac_kind<double>(*D->D, *D->op64, kind, (const c64*)src, (c64*)dst, nrhs, S());
+        else
+            dirac_kind<float>(*D->D, *D->op32, kind, (const c32*)src, (c32*)dst, nrhs, S());
+    });
+}
+int smg_dirac_delta(smg_dirac* D, const double* src, double* dst, const double* pi, double dt) {
+    return guard([&] {
+        SMG_REQUIRE(D && pi, "null argument");
+        const size_t n = 2 * (size_t)D->D->V;
+        Field<double> s = up(src, n), d(n);
+        DevMem p(n * sizeof(double));
+        SMG_HIP(hipMemcpyAsync(p.p, pi, n * sizeof(double), hipMemcpyHostToDevice, S()));
+        launch_delta_dirac(D->D->v64(), d.p(), s.p(), p.as<double>(), dt, S());
+        downc(dst, d.p(), n);
+    });
+}
+
+template <typename T>
+static void eo_kind(smg_dirac* D, int kind, const cplx<T>* s, cplx<T>* d, hipStream_t st) {
+    const Dirac& dd = *D->D;
+    SMG_REQUIRE(dd.L % 2 == 0, "even-odd operators need an even L");
+    const DiracView<T> v = dd.view<T>();
+    const T mu = (T)dd.mu_t;
+    SchurOp<T>* sop;
+    if constexpr (std::is_same<T, double>::value) sop = D->schur64.get();
+    else sop = D->schur32.get();
+    switch (kind) {
+    case SMG_EO_HOP_EO:
+        launch_eo_hop<T>(v, d, s, 1, 0, mu, 0, 0, nullptr, EPI_NONE, nullptr, nullptr, nullptr, st);
+        break;
+    case SMG_EO_HOP_OE:
+        launch_eo_hop<T>(v, d, s, 0, 0, mu, 0, 0, nullptr, EPI_NONE, nullptr, nullptr, nullptr, st);
+        break;
+    case SMG_EO_EE_INV: launch_eo_diag<T>(v, d, s, 0, 1, mu, st); break;
+    case SMG_EO_OO_INV: launch_eo_diag<T>(v, d, s, 1, 1, mu, st); break;
+    case SMG_EO_OO: launch_eo_diag<T>(v, d, s, 1, 0, mu, st); break;
+    case SMG_EO_SCHUR: sop->apply_m(d, s, 0, st); break;
+    case SMG_EO_SCHUR_DAG: sop->apply_m(d, s, 1, st); break;
+    case SMG_EO_SCHUR_DAG_SCHUR: sop->apply(d, s, EPI_NONE, nullptr, nullptr, nullptr, st); break;
+    default: SMG_REQUIRE(false, "unknown even-odd kind");
+    }
+}
+int smg_eo_apply(smg_dirac* D, int kind, const double* src, double* dst) {
+    return guard([&] {
+        SMG_REQUIRE(D, "null handle");
+        const size_t n = (size_t)D->D->V;
+        Field<double> s = up(src, n), d(n);
+        eo_kind<double>(D, kind, s.p(), d.p(), S());
+        downc(dst, d.p(), n);
+    });
+}
+int smg_eo_apply_dev(smg_dirac* D, int kind, int prec, const void* src, void* dst, void* stream) {
+    return guard([&] {
+        SMG_REQUIRE(D && src && dst, "bad arguments");
+        check_prec(prec);
+        StreamJoin j(stream);
+        if (prec == SMG_F64) eo_kind<double>(D, kind, (const c64*)src, (c64*)dst, S());
+        else eo_kind<float>(D, kind, (const c32*)src, (c32*)dst, S());
+    });
+}
+int smg_eo_reconstruct(smg_dirac* D, const double* x_o, const double* phi, double* out) {
+    return guard([&] {
+        SMG_REQUIRE(D, "null handle");
+        const Dirac& dd = *D->D;
+        SMG_REQUIRE(dd.L % 2 == 0, "even-odd operators need an even L");
+        const size_t nh = (size_t)dd.V, n = 2 * nh;
+        Field<double> xo = up(x_o, nh), h(nh), te(nh), xe(nh), full(n);
+        const DiracView<double> v = dd.v64();
+        launch_eo_hop<double>(v, h.p(), xo.p(), 0, 0, dd.mu_t, 0, 0, nullptr, EPI_NONE, nullptr, nullptr, nullptr,
+                              S());
+        if (phi) {
+            Field<double> ph = up(phi, n);
+            launch_eo_gather<double>(dd.L, te.p(), ph.p(), 0, S());
+            launch_axpby<double>(te.p(), h.p(), c64{-1, 0}, c64{1, 0}, nh, S());   // phi_e - H x_o
+        } else {
+            launch_axpby<double>(te.p(), h.p(), c64{-1, 0}, c64{0, 0}, nh, S());   // -H x_o
+        }
+        launch_eo_diag<double>(v, xe.p(), te.p(), 0, 1, dd.mu_t, S());
+        launch_eo_scatter<double>(dd.L, full.p(), xe.p(), 0, S());
+        launch_eo_scatter<double>(dd.L, full.p(), xo.p(), 1, S());
+        downc(out, full.p(), n);
+    });
+}
+int smg_eo_logdet(smg_dirac* D, double* out) {
+    return guard([&] {
+        SMG_REQUIRE(D && out, "null argument");
+        SMG_REQUIRE(D->D->L % 2 == 0, "even-odd operators need an even L");
+        int g = launch_logdet(D->D->v64(), D->D->mu_t, Context::get().part_ptr(), S());
+        *out = reduce_read(g, S());
+    });
+}
+
+// ------------------------------------------------------------------ BLAS-1
+int smg_dot_dev(const void* a, const void* b, size_t n, int prec, double* out2, void* stream) {
+    return guard([&] {
+        check_prec(prec);
+        StreamJoin j(stream);
+        c64 d = prec == SMG_F64 ? dev_dot<double>((const c64*)a, (const c64*)b, n, S())
+                                : dev_dot<float>((const c32*)a, (const c32*)b, n, S());
+        out2[0] = d.re;
+        out2[1] = d.im;
+    });
+}
+int smg_norm_dev(const void* a, size_t n, int prec, double* out, void* stream) {
+    return guard([&] {
+        check_prec(prec);
+        StreamJoin j(stream);
+        c64 d = prec == SMG_F64 ? dev_dot<double>((const c64*)a, (const c64*)a, n, S())
+                                : dev_dot<float>((const c32*)a, (const c32*)a, n, S());
+        *out = std::sqrt(d.re);
+    });
+}
+
+// ------------------------------------------------------------------ prolongator
+int smg_prolongator_create(smg_prolongator** out, int nx, int ny, int nd, int bx, int by, int k) {
+    return guard([&] {
+        SMG_REQUIRE(out && nx > 0 && ny > 0 && nd > 0, "bad arguments");
+        auto* h = new smg_prolongator_s();
+        try { h->P.reset(new Aggregation(nx, ny, nd, bx, by, k)); } catch (...) { delete h; throw; }
+        *out = h;
+    });
+}
+int smg_prolongator_destroy(smg_prolongator* P) { return guard([&] { delete P; }); }
+int smg_prolongator_build(smg_prolongator* P, const double* null_vecs) {
+    return guard([&] {
+        SMG_REQUIRE(P, "null handle");
+        Field<double> nv = up(null_vecs, P->P->fine_n() * P->P->A.k);
+        P->P->build(nv.p(), S());
+        Context::get().sync();
+    });
+}
+int smg_prolongator_get(smg_prolongator* P, double* bv) {
+    return guard([&] { SMG_REQUIRE(P, "null handle"); downc(bv, P->P->bv64.p(), P->P->bv64.n); });
+}
+int smg_prolongator_set(smg_prolongator* P, const double* bv) {
+    return guard([&] {
+        SMG_REQUIRE(P, "null handle");
+        Field<double> b = up(bv, P->P->bv64.n);
+        P->P->set_block_vecs(b.p(), S());
+        Context::get().sync();
+    });
+}
+int smg_prolong(smg_prolongator* P, const double* coarse, double* fine) {
+    return guard([&] {
+        SMG_REQUIRE(P, "null handle");
+        Field<double> c = up(coarse, P->P->coarse_n()), f(P->P->fine_n());
+        launch_prolong<double>(P->P->A, P->P->bv64.p(), c.p(), f.p(), 0, S());
+        downc(fine, f.p(), f.n);
+    });
+}
+int smg_restrict(smg_prolongator* P, const double* fine, double* coarse) {
+    return guard([&] {
+        SMG_REQUIRE(P, "null handle");
+        Field<double> f = up(fine, P->P->fine_n()), c(P->P->coarse_n());
+        launch_restrict<double>(P->P->A, P->P->bv64.p(), f.p(), c.p(), S());
+        downc(coarse, c.p(), c.n);
+    });
+}
+
+// ------------------------------------------------------------------ coarse operator
+int smg_coarse_create_galerkin(smg_coarse** out, smg_dirac* D, smg_prolongator* P) {
+    return guard([&] {
+        SMG_REQUIRE(out && D && P, "null argument");
+        SMG_REQUIRE(P->P->A.nx == D->D->L && P->P->A.ny == D->D->L && P->P->A.nd == 2,
+                    "prolongator does not act on this Dirac operator's lattice");
+        auto* h = new smg_coarse_s();
+        try {
+            h->C.reset(new CoarseOperator(P->P->A.nbx, P->P->A.nby, P->P->A.k));
+            h->C->build(*D->op64, *P->P, S());
+        } catch (...) { delete h; throw; }
+        *out = h;
+    });
+}
+int smg_coarse_create_from_coarse(smg_coarse** out, smg_coarse* A, smg_prolongator* P) {
+    return guard([&] {
+        SMG_REQUIRE(out && A && P, "null argument");
+        SMG_REQUIRE(P->P->A.nx == A->C->nx && P->P->A.ny == A->C->ny && P->P->A.nd == A->C->k,
+                    "prolongator does not act on this coarse operator's lattice");
+        auto* h = new smg_coarse_s();
+        try {
+            CoarseLinOp<double, c64> op(*A->C);
+            h->C.reset(new CoarseOperator(P->P->A.nbx, P->P->A.nby, P->P->A.k));
+            h->C->build(op, *P->P, S());
+        } catch (...) { delete h; throw; }
+        *out = h;
+    });
+}
+int smg_coarse_create_links(smg_coarse** out, int nx, int ny, int k, const double* links) {
+    return guard([&] {
+        SMG_REQUIRE(out && nx > 0 && ny > 0 && k > 0, "bad arguments");
+        auto* h = new smg_coarse_s();
+        try {
+            h->C.reset(new CoarseOperator(nx, ny, k));
+            Field<double> l = up(links, h->C->nlinks());
+            h->C->set_links(l.p(), S());
+            Context::get().sync();
+        } catch (...) { delete h; throw; }
+        *out = h;
+    });
+}
+int smg_coarse_destroy(smg_coarse* C) { return guard([&] { delete C; }); }
+int smg_coarse_get_links(smg_coarse* C, double* links) {
+    return guard([&] { SMG_REQUIRE(C, "null handle"); downc(links, C->C->l64.p(), C->C->nlinks()); });
+}
+int smg_coarse_apply(smg_coarse* C, int prec, const double* src, double* dst) {
+    return guard([&] {
+        SMG_REQUIRE(C, "null handle");
+        const size_t n = C->C->n();
+        Field<double> s = up(src, n), d(n);
+        if (prec == SMG_F64) {
+            CoarseLinOp<double, c64> op(*C->C);
+            op.apply(d.p(), s.p(), EPI_NONE, nullptr, nullptr, nullptr, S());
+        } else {
+            Field<float> s32(n), d32(n);
+            launch_convert<float, double>(s32.p(), s.p(), n, S());
+            if (prec == SMG_F32) {
+                CoarseLinOp<float, c32> op(*C->C);
+                op.apply(d32.p(), s32.p(), EPI_NONE, nullptr, nullptr, nullptr, S());
+            } else {
+                SMG_REQUIRE(prec == SMG_F16, "bad precision");
+                CoarseLinOp<float, ch16> op(*C->C);
+                op.apply(d32.p(), s32.p(), EPI_NONE, nullptr, nullptr, nullptr, S());
+            }
+            launch_convert<double, float>(d.p(), d32.p(), n, S());
+        }
+        downc(dst, d.p(), n);
+    });
+}
+int smg_coarse_solve(smg_coarse* C, const double* b, double* x, int maxit, double tol, int* iters) {
+    return guard([&] {
+        SMG_REQUIRE(C, "null handle");
+        const size_t n = C->C->n();
+        Field<double> bb = up(b, n), xx(n), r(n), p(n), Ap(n);
+        DevMem sc(64 * sizeof(double));
+        SMG_HIP(hipMemsetAsync(sc.p, 0, 64 * sizeof(double), S()));
+        CoarseLinOp<double, c64> op(*C->C);
+        int it = coarse_cg<double>(op, bb.p(), xx.p(), r.p(), p.p(), Ap.p(), maxit, tol, sc.as<double>(), true, S());
+        if (iters) *iters = it;
+        downc(x, xx.p(), n);
+    });
+}
+
+// ------------------------------------------------------------------ smoothers
+template <typename Op>
+static void smooth_op(Op& op, int kind, double* x, const double* b, int niter, double omega) {
+    const size_t n = op.n();
+    Field<double> xx = up(x, n), bb = up(b, n), r(n), Ar(n);
+    DevMem sc(64 * sizeof(double));
+    if (kind == SMG_SMOOTH_MR)
+        mr_smooth<double>(op, xx.p(), bb.p(), r.p(), Ar.p(), niter, false, sc.as<double>(), S());
+    else {
+        SMG_REQUIRE(kind == SMG_SMOOTH_RICHARDSON, "unknown smoother");
+        richardson_smooth<double>(op, xx.p(), bb.p(), r.p(), niter, omega, false, S());
+    }
+    downc(x, xx.p(), n);
+}
+int smg_smooth_dirac(smg_dirac* D, int kind, double* x, const double* b, int niter, double omega) {
+    return guard([&] { SMG_REQUIRE(D, "null handle"); smooth_op(*D->op64, kind, x, b, niter, omega); });
+}
+int smg_smooth_coarse(smg_coarse* C, int kind, double* x, const double* b, int niter, double omega) {
+    return guard([&] {
+        SMG_REQUIRE(C, "null handle");
+        CoarseLinOp<double, c64> op(*C->C);
+        smooth_op(op, kind, x, b, niter, omega);
+    });
+}
+int smg_lambda_max_dirac(smg_dirac* D, const double* v0, int niter, double* out) {
+    return guard([&] {
+        SMG_REQUIRE(D && out, "null argument");
+        const size_t n = D->op64->n();
+        Field<double> v = up(v0, n), Av(n);
+        *out = lambda_max<double>(*D->op64, v.p(), Av.p(), niter, S());
+    });
+}
+
+// ------------------------------------------------------------------ multigrid
+int smg_mg_setup(smg_mg** out, smg_dirac* D, const smg_mg_params* prm, const double* starts) {
+    return guard([&] {
+        SMG_REQUIRE(out && D && prm && starts, "null argument");
+        MGConfig c;
+        c.n_levels = prm->n_levels;
+        for (int i = 0; i < 4; i++) {
+            c.bx[i] = prm->bx[i];
+            c.by[i] = prm->by[i];
+            c.k[i] = prm->k[i];
+        }
+        c.null_iters = prm->null_iters;
+        c.pre = prm->pre_smooth;
+        c.post = prm->post_smooth;
+        c.w_cycle = prm->w_cycle;
+        c.coarse_maxit = prm->coarse_maxit;
+        c.coarse_tol = prm->coarse_tol;
+        c.coarse_half = prm->coarse_half;
+        SMG_REQUIRE(c.n_levels >= 2 && c.n_levels <= 5, "n_levels must be 2..5");
+        for (int l = 0; l + 1 < c.n_levels; l++)
+            SMG_REQUIRE(c.k[l] >= 1 && c.k[l] <= 64 && c.bx[l] >= 1 && c.by[l] >= 1, "bad level parameters");
+        auto* h = new smg_mg_s();
+        h->D = D;
+        try { h->mg.reset(new MG(*D->D, c, reinterpret_cast<const c64*>(starts), false)); } catch (...) { delete h; throw; }
+        *out = h;
+    });
+}
+int smg_mg_destroy(smg_mg* mg) { return guard([&] { delete mg; }); }
+int smg_mg_set_symmetric(smg_mg* mg, double damping, const double* starts) {
+    return guard([&] {
+        SMG_REQUIRE(mg && starts, "null argument");
+        mg->mg->set_symmetric(damping, reinterpret_cast<const c64*>(starts), false);
+    });
+}
+static void mg_prec_dev(smg_mg* mg, int prec, const void* b, void* x) {
+    check_prec(prec);
+    if (prec == SMG_F64) mg->mg->precondition((const c64*)b, (c64*)x, S());
+    else mg->mg->precondition((const c32*)b, (c32*)x, S());
+}
+int smg_mg_precondition(smg_mg* mg, int prec, const double* b, double* x) {
+    return guard([&] {
+        SMG_REQUIRE(mg, "null handle");
+        check_prec(prec);
+        const size_t n = mg->mg->dims[0];
+        Field<double> bb = up(b, n), xx(n);
+        if (prec == SMG_F64) mg_prec_dev(mg, prec, bb.p(), xx.p());
+        else {
+            Field<float> b32(n), x32(n);
+            launch_convert<float, double>(b32.p(), bb.p(), n, S());
+            mg_prec_dev(mg, prec, b32.p(), x32.p());
+            launch_convert<double, float>(xx.p(), x32.p(), n, S());
+        }
+        downc(x, xx.p(), n);
+    });
+}
+int smg_mg_precondition_dev(smg_mg* mg, int prec, const void* b, void* x, void* stream) {
+    return guard([&] {
+        SMG_REQUIRE(mg && b && x, "bad arguments");
+        StreamJoin j(stream);
+        mg_prec_dev(mg, prec, b, x);
+    });
+}
+int smg_mg_level_dim(smg_mg* mg, int level, size_t* n) {
+    return guard([&] {
+        SMG_REQUIRE(mg && n && level >= 0 && level < mg->mg->cfg.n_levels, "bad arguments");
+        *n = mg->mg->dims[level];
+    });
+}
+int smg_mg_null_vecs(smg_mg* mg, int level, double* out) {
+    return guard([&] {
+        SMG_REQUIRE(mg && level >= 0 && level + 1 < mg->mg->cfg.n_levels, "bad level");
+        downc(out, mg->mg->nullv[level]->p(), mg->mg->nullv[level]->n);
+    });
+}
+int smg_mg_block_vecs(smg_mg* mg, int level, double* out) {
+    return guard([&] {
+        SMG_REQUIRE(mg && level >= 0 && level + 1 < mg->mg->cfg.n_levels, "bad level");
+        downc(out, mg->mg->P[level]->bv64.p(), mg->mg->P[level]->bv64.n);
+    });
+}
+int smg_mg_coarse_links(smg_mg* mg, int level, double* out) {
+    return guard([&] {
+        SMG_REQUIRE(mg && level >= 0 && level + 1 < mg->mg->cfg.n_levels, "bad level");
+        downc(out, mg->mg->C[level]->l64.p(), mg->mg->C[level]->nlinks());
+    });
+}
+int smg_mg_omega(smg_mg* mg, int level, double* out) {
+    return guard([&] {
+        SMG_REQUIRE(mg && out && level >= 0 && level < mg->mg->cfg.n_levels, "bad level");
+        *out = mg->mg->omega[level];
+    });
+}
+
+int smg_mg_set_cycle(smg_mg* mg, int pre, int post, int coarse_maxit, double coarse_tol) {
+    return guard([&] {
+        SMG_REQUIRE(mg && pre >= 0 && post >= 0 && coarse_maxit >= 1, "bad arguments");
+        mg->mg->cfg.pre = pre;
+        mg->mg->cfg.post = post;
+        mg->mg->cfg.coarse_maxit = coarse_maxit;
+        mg->mg->cfg.coarse_tol = coarse_tol;
+    });
+}
+
+// ------------------------------------------------------------------ solvers
+static void fill_info(smg_solve_info* info, const SolveResult& r) {
+    if (!info) return;
+    info->iterations = r.iterations;
+    info->final_residual = r.final_residual;
+    info->true_residual = r.true_residual;
+    info->restarts = r.restarts;
+}
+int smg_cg_solve(smg_dirac* D, int op, smg_mg* mg, const double* b, const double* x0, double* x, int maxit,
+                 double tol, smg_solve_info* info) {
+    return guard([&] {
+        SMG_REQUIRE(D, "null handle");
+        LinOp<double>* A;
+        if (op == SMG_OP_DDAGD) A = D->op64.get();
+        else {
+            SMG_REQUIRE(op == SMG_OP_SCHUR && D->schur64, "unknown operator (SMG_OP_SCHUR needs an even L)");
+            SMG_REQUIRE(mg == nullptr, "the multigrid preconditioner acts on D^dag D only");
+            A = D->schur64.get();
+        }
+        const size_t n = A->n();
+        Field<double> bb = up(b, n), xx(n);
+        Field<double> xi;
+        if (x0) xi = up(x0, n);
+        SolveResult r = cg_solve(*A, bb.p(), xx.p(), x0 ? xi.p() : nullptr, mg ? mg->mg.get() : nullptr, maxit, tol,
+                                 S());
+        r.true_residual = true_residual(*A, bb.p(), xx.p(), S());
+        fill_info(info, r);
+        downc(x, xx.p(), n);
+    });
+}
+static SolveResult gcr_dev(smg_dirac* D, smg_mg* mg, int prec, const c64* b, c64* x, int maxit, double tol,
+                           int trunc, int refp, double inner_tol) {
+    MG* m = mg ? mg->mg.get() : nullptr;
+    SolveResult r;
+    if (prec == SMG_F64) {
+        r = gcr_solve<double>(*D->op64, b, x, m, maxit, tol, trunc, refp != 0, S());
+        r.true_residual = true_residual(*D->op64, b, x, S());
+    } else {
+        check_prec(prec);
+        r = gcr_solve_mixed(*D->op64, *D->op32, b, x, m, maxit, tol, trunc, inner_tol, S());
+    }
+    return r;
+}
+int smg_gcr_solve(smg_dirac* D, smg_mg* mg, int prec, const double* b, double* x, int maxit, double tol, int trunc,
+                  int ref_projection, double inner_tol, smg_solve_info* info) {
+    return guard([&] {
+        SMG_REQUIRE(D, "null handle");
+        const size_t n = D->op64->n();
+        Field<double> bb = up(b, n), xx(n);
+        SolveResult r = gcr_dev(D, mg, prec, bb.p(), xx.p(), maxit, tol, trunc, ref_projection, inner_tol);
+        fill_info(info, r);
+        downc(x, xx.p(), n);
+    });
+}
+int smg_gcr_solve_dev(smg_dirac* D, smg_mg* mg, int prec, const void* b, void* x, int maxit, double tol, int trunc,
+                      double inner_tol, smg_solve_info* info, void* stream) {
+    return guard([&] {
+        SMG_REQUIRE(D && b && x, "bad arguments");
+        StreamJoin j(stream);
+        SolveResult r = gcr_dev(D, mg, prec, (const c64*)b, (c64*)x, maxit, tol, trunc, 0, inner_tol);
+        fill_info(info, r);
+    });
+}
+int smg_fgmres_solve(smg_dirac* D, smg_mg* mg, const double* b, double* x, int restart, int maxit, double tol,
+                     smg_solve_info* info) {
+    return guard([&] {
+        SMG_REQUIRE(D && restart >= 1, "bad arguments");
+        const size_t n = D->op64->n();
+        Field<double> bb = up(b, n), xx(n);
+        SolveResult r = fgmres_solve(*D->op64, bb.p(), xx.p(), mg ? mg->mg.get() : nullptr, restart, maxit, tol, S());
+        fill_info(info, r);
+        downc(x, xx.p(), n);
+    });
+}
+
+// ------------------------------------------------------------------ forces
+int smg_force_forward(smg_dirac* D, const double* X, double* chi) { return smg_dirac_apply(D, SMG_D, X, chi); }
+int smg_force_backward(smg_dirac* D, const double* chi, const double* X, double* F) {
+    return guard([&] {
+        SMG_REQUIRE(D, "null handle");
+        const size_t n = 2 * (size_t)D->D->V;
+        Field<double> c = up(chi, n), xx = up(X, n);
+        DevMem f(n * sizeof(double));
+        launch_force_backward(D->D->v64(), D->D->csw, c.p(), xx.p(), f.as<double>(), S());
+        down(F, f.p, n * sizeof(double));
+    });
+}
+static void fermion_force_dev(smg_dirac* D, const c64* X, double* F, hipStream_t st) {
+    const Dirac& d = *D->D;
+    const size_t n = 2 * (size_t)d.V;
+    Field<double> chi(n);
+    launch_dslash<double>(d.v64(), chi.p(), X, d.mu_t, 0, 0, EPI_NONE, nullptr, nullptr, nullptr, 1, n, st);
+    launch_force_backward(d.v64(), d.csw, chi.p(), X, F, st);
+}
+int smg_fermion_force(smg_dirac* D, const double* X, double* F) {
+    return guard([&] {
+        SMG_REQUIRE(D, "null handle");
+        const size_t n = 2 * (size_t)D->D->V;
+        Field<double> xx = up(X, n);
+        DevMem f(n * sizeof(double));
+        fermion_force_dev(D, xx.p(), f.as<double>(), S());
+        down(F, f.p, n * sizeof(double));
+    });
+}
+int smg_fermion_force_dev(smg_dirac* D, const void* X, void* F, void* stream) {
+    return guard([&] {
+        SMG_REQUIRE(D && X && F, "bad arguments");
+        StreamJoin j(stream);
+        fermion_force_dev(D, (const c64*)X, (double*)F, S());
+    });
+}
+int smg_hopping_force(smg_dirac* D, const double* chi, const double* X, double* F) {
+    return guard([&] {
+        SMG_REQUIRE(D, "null handle");
+        const size_t n = 2 * (size_t)D->D->V;
+        Field<double> c = up(chi, n), xx = up(X, n);
+        DevMem f(n * sizeof(double));
+        launch_hopping_force(D->D->v64(), c.p(), xx.p(), 1.0, 0, f.as<double>(), S());
+        down(F, f.p, n * sizeof(double));
+    });
+}
+static void logdet_force_dev(const Dirac& d, double* F, int accumulate, hipStream_t st) {
+    const size_t V = (size_t)d.V;
+    if (!accumulate) SMG_HIP(hipMemsetAsync(F, 0, 2 * V * sizeof(double), st));
+    if (d.csw == 0.0) return;
+    DevMem w(V * sizeof(double));
+    launch_logdet_weight(d.v64(), d.csw, d.mu_t, w.as<double>(), st);
+    launch_clover_insert(d.v64(), w.as<double>(), 0.5, F, st);
+}
+int smg_logdet_force(smg_dirac* D, double* F) {
+    return guard([&] {
+        SMG_REQUIRE(D, "null handle");
+        SMG_REQUIRE(D->D->L % 2 == 0, "even-odd operators need an even L");
+        DevMem f(2 * (size_t)D->D->V * sizeof(double));
+        logdet_force_dev(*D->D, f.as<double>(), 0, S());
+        down(F, f.p, 2 * (size_t)D->D->V * sizeof(double));
+    });
+}
+// schur_deriv_plus, src/hmc_forces.cpp:763-828, accumulated into F (device)
+static void schur_deriv_plus(const Dirac& d, const c64* chi_o, const c64* psi_o, double* F, hipStream_t st) {
+    const size_t nh = (size_t)d.V, n = 2 * nh;
+    const DiracView<double> v = d.v64();
+    Field<double> tmp2(nh), tmp3(nh), chif(n), psif(n);
+    launch_eo_hop<double>(v, tmp2.p(), psi_o, 0, 1, d.mu_t, 0, 0, nullptr, EPI_NONE, nullptr, nullptr, nullptr, st);
+    // tmp3 = D_ee(-mu)^-1 g5 H (g5 chi): the diagonal commutes with g5
+    launch_eo_hop<double>(v, tmp3.p(), chi_o, 0, 1, -d.mu_t, 1, 1, nullptr, EPI_NONE, nullptr, nullptr, nullptr, st);
+    if (d.csw != 0.0) {
+        DevMem w(nh * sizeof(double));
+        launch_eo_force_weights(v, chi_o, psi_o, tmp3.p(), tmp2.p(), w.as<double>(), st);
+        launch_clover_insert(v, w.as<double>(), d.csw / 8.0, F, st);
+    }
+    launch_eo_scatter<double>(d.L, chif.p(), tmp3.p(), 0, st);
+    launch_eo_scatter<double>(d.L, chif.p(), chi_o, 1, st);
+    launch_eo_scatter<double>(d.L, psif.p(), tmp2.p(), 0, st);
+    launch_eo_scatter<double>(d.L, psif.p(), psi_o, 1, st);
+    launch_hopping_force(v, chif.p(), psif.p(), -0.5, 1, F, st);
+}
+int smg_eo_fermion_force(smg_dirac* D, const double* x_o, const double* y_o, double* F) {
+    return guard([&] {
+        SMG_REQUIRE(D, "null handle");
+        const Dirac& d = *D->D;
+        SMG_REQUIRE(d.L % 2 == 0, "even-odd operators need an even L");
+        const size_t V = (size_t)d.V;
+        Field<double> xo = up(x_o, V), yo = up(y_o, V);
+        DevMem f(2 * V * sizeof(double)), fl(2 * V * sizeof(double));
+        SMG_HIP(hipMemsetAsync(f.p, 0, 2 * V * sizeof(double), S()));
+        schur_deriv_plus(d, yo.p(), xo.p(), f.as<double>(), S());
+        // F = 2 * schur_deriv_plus + logdet force (src/hmc_forces.cpp:852-866); a real
+        // array of 2V doubles is scaled as V complex numbers
+        launch_axpby<double>(f.as<c64>(), f.as<c64>(), c64{0, 0}, c64{2.0, 0}, V, S());
+        if (d.csw != 0.0) {
+            logdet_force_dev(d, fl.as<double>(), 0, S());
+            launch_axpby<double>(f.as<c64>(), fl.as<c64>(), c64{1, 0}, c64{1, 0}, V, S());
+        }
+        down(F, f.p, 2 * V * sizeof(double));
+    });
+}
+int smg_fermion_action(smg_dirac* D, const double* phi, int maxit, double tol, double* S_out, double* X) {
+    return guard([&] {
+        SMG_REQUIRE(D && S_out, "null argument");
+        const size_t n = 2 * (size_t)D->D->V;
+        Field<double> ph = up(phi, n), xx(n);
+        cg_solve(*D->op64, ph.p(), xx.p(), nullptr, nullptr, maxit, tol, S());
+        *S_out = dev_dot<double>(ph.p(), xx.p(), n, S()).re;
+        if (X) downc(X, xx.p(), n);
+    });
+}

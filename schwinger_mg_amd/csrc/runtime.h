@@ -1,0 +1,241 @@
+// runtime.h -- host-side C++ runtime of the gfx950 Schwinger-MG library.
+//
+// Objects mirror the reference's operator API (src/dirac.h, src/prolongator.h,
+// src/coarse_op.h, src/multigrid.h, src/solvers.h) but keep every field in
+// device memory; host code only sequences kernels on one HIP stream.
+#pragma once
+#include <memory>
+#include <vector>
+#include "smg_kernels.h"
+
+namespace smg {
+
+// ------------------------------------------------------------------ memory
+struct DevMem {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevMem() = default;
+    explicit DevMem(size_t b) { alloc(b); }
+    ~DevMem() { release(); }
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
+    DevMem(DevMem&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevMem& operator=(DevMem&& o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    void alloc(size_t b);
+    void release();
+    template <typename X>
+    X* as() const { return reinterpret_cast<X*>(p); }
+};
+
+template <typename T>
+struct Field {
+    DevMem m;
+    size_t n = 0;  // complex entries
+    Field() = default;
+    explicit Field(size_t n_) : m(n_ * sizeof(cplx<T>)), n(n_) {}
+    cplx<T>* p() const { return m.as<cplx<T>>(); }
+};
+
+// Per-process execution context: one stream, reduction partials, scalars.
+struct Context {
+    hipStream_t stream = nullptr;
+    DevMem part;      // reduction partial sums
+    DevMem scal;      // device scalars (pool)
+    DevMem hostscal;  // pinned host mirror (hipHostMalloc)
+    size_t scal_used = 0;
+    static Context& get();
+    double* part_ptr() const { return part.as<double>(); }
+    double* new_scalars(size_t count);  // 64-aligned slots from the pool
+    // read `count` doubles from device into host (synchronises the stream)
+    void read(const double* dev, double* host, size_t count);
+    double finalize_read(int nparts, int nc, double* out_dev, size_t slot_count = 1);
+    void sync();
+    ~Context();
+};
+
+// ------------------------------------------------------------------ operators
+template <typename T>
+struct LinOp {
+    virtual ~LinOp() = default;
+    virtual size_t n() const = 0;
+    // dst = A src | b - A src | A src with partials {<dst,aux>, |dst|^2}; returns #partials
+    virtual int apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux,
+                      double* part, hipStream_t st) = 0;
+};
+
+// Wilson-clover(-twisted) Dirac operator on an L x L torus (src/dirac.h:8-183).
+struct Dirac {
+    int L, V;
+    double mass, r, csw, mu_t;
+    Field<double> U64;
+    Field<float> U32;
+    DevMem F01;  // clover field F01 (double [V])
+    DevMem C64;  // 0.5*csw*F01 (double [V])
+    DevMem C32;  // float [V]
+    Dirac(int L, const c64* U_host, double mass, double r, double csw, double mu_t);
+    void set_gauge(const c64* U, bool device_ptr);
+    DiracView<double> v64() const;
+    DiracView<float> v32() const;
+    template <typename T>
+    DiracView<T> view() const;
+};
+
+// D^dag D (src/dirac.h:136-140) as a fused pair of stencil launches.
+template <typename T>
+struct DdagDOp : LinOp<T> {
+    const Dirac& D;
+    Field<T> tmp;
+    explicit DdagDOp(const Dirac& d) : D(d), tmp(2 * (size_t)d.V) {}
+    size_t n() const override { return 2 * (size_t)D.V; }
+    int apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux, double* part,
+              hipStream_t st) override;
+};
+
+// M^dag M on odd sites (src/dirac.h:399-445), four half-lattice launches.
+template <typename T>
+struct SchurOp : LinOp<T> {
+    const Dirac& D;
+    Field<T> te, to;
+    explicit SchurOp(const Dirac& d) : D(d), te((size_t)d.V), to((size_t)d.V) {}
+    size_t n() const override { return (size_t)D.V; }
+    int apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux, double* part,
+              hipStream_t st) override;
+    // M (dag = 0) or M^dag (dag = 1) only
+    void apply_m(cplx<T>* dst, const cplx<T>* src, int dag, hipStream_t st);
+};
+
+// Aggregation (geometric block) prolongator, src/prolongator.h:14-142.
+struct Aggregation {
+    AggView A;
+    Field<double> bv64;
+    Field<float> bv32;
+    Aggregation(int nx, int ny, int nd, int bx, int by, int k);
+    size_t fine_n() const { return (size_t)A.nx * A.ny * A.nd; }
+    size_t coarse_n() const { return (size_t)A.nbx * A.nby * A.k; }
+    // null_vecs: k device vectors of fine_n() entries, contiguous
+    void build(const c64* null_vecs, hipStream_t st);
+    void set_block_vecs(const c64* bv_dev, hipStream_t st);
+    template <typename T>
+    const cplx<T>* bv() const;
+};
+
+// 9-point coarse operator, src/coarse_op.h:154-297, stored in FP64, FP32, FP16.
+struct CoarseOperator {
+    int nx, ny, k;
+    Field<double> l64;
+    Field<float> l32;
+    DevMem l16;
+    CoarseOperator(int nx, int ny, int k);
+    size_t n() const { return (size_t)nx * ny * k; }
+    size_t nlinks() const { return (size_t)nx * ny * 9 * k * k; }
+    // Galerkin P^dag A P by coloured probing in FP64
+    void build(LinOp<double>& A, const Aggregation& P, hipStream_t st);
+    void set_links(const c64* dev, hipStream_t st);
+    void refresh_low_precision(hipStream_t st);
+};
+
+template <typename T, typename TL>
+struct CoarseLinOp : LinOp<T> {
+    const CoarseOperator& C;
+    explicit CoarseLinOp(const CoarseOperator& c) : C(c) {}
+    size_t n() const override { return C.n(); }
+    int apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux, double* part,
+              hipStream_t st) override;
+};
+
+// ------------------------------------------------------------------ smoothers
+// MR smoother (src/smoother.cpp:26-40).  x holds the initial guess unless
+// zero_init; on return r holds the recursive residual b - A x.
+template <typename T>
+void mr_smooth(LinOp<T>& A, cplx<T>* x, const cplx<T>* b, cplx<T>* r, cplx<T>* Ar, int niter, bool zero_init,
+               double* sc, hipStream_t st);
+// Richardson (src/smoother.cpp:63-73); on return r = b - A x (recomputed)
+template <typename T>
+void richardson_smooth(LinOp<T>& A, cplx<T>* x, const cplx<T>* b, cplx<T>* r, int niter, double omega,
+                       bool zero_init, hipStream_t st);
+// power iteration (src/smoother.cpp:43-59) from the start vector v (overwritten)
+template <typename T>
+double lambda_max(LinOp<T>& A, cplx<T>* v, cplx<T>* Av, int niter, hipStream_t st);
+
+// CG on a (coarse) operator with device-side scalars (src/coarse_op.h:492-547
+// without deflation).  Returns iterations if check != 0, else -1 (not read back).
+template <typename T>
+int coarse_cg(LinOp<T>& A, const cplx<T>* b, cplx<T>* x, cplx<T>* r, cplx<T>* p, cplx<T>* Ap, int maxit, double tol,
+              double* sc, bool check, hipStream_t st);
+
+// ------------------------------------------------------------------ multigrid
+struct MGParamsC;  // C-ABI parameter struct (smg.h)
+
+struct MGConfig {
+    int n_levels = 2;
+    int bx[4] = {4, 2, 2, 2}, by[4] = {4, 2, 2, 2}, k[4] = {8, 8, 8, 8};
+    int null_iters = 20;
+    int null_inner = 5;
+    int pre = 3, post = 3;
+    int w_cycle = 1;
+    int coarse_maxit = 200;
+    double coarse_tol = 1e-12;
+    int smoother = 0;        // 0 MR, 1 Richardson (set by set_symmetric)
+    int coarse_half = 0;     // FP32 cycle: FP16 coarse links
+};
+
+template <typename T>
+struct Cycle;
+
+struct MG {
+    MGConfig cfg;
+    const Dirac* D = nullptr;
+    std::vector<std::unique_ptr<Aggregation>> P;
+    std::vector<std::unique_ptr<CoarseOperator>> C;
+    std::vector<std::unique_ptr<Field<double>>> nullv;  // per transfer, k vectors
+    std::vector<double> omega;                          // Richardson omegas (0 = MR)
+    std::unique_ptr<LinOp<double>> fine64;
+    std::unique_ptr<LinOp<float>> fine32;
+    std::unique_ptr<Cycle<double>> cyc64;
+    std::unique_ptr<Cycle<float>> cyc32;
+    std::vector<size_t> dims;   // per level
+    std::vector<int> lnx, lny, lnd;
+    // starts: every level's k start vectors (host or device FP64), level 0 first
+    MG(const Dirac& d, const MGConfig& c, const c64* starts, bool starts_on_device, bool schur = false);
+    ~MG();
+    void set_symmetric(double damping, const c64* starts, bool on_device);
+    void precondition(const c64* b, c64* x, hipStream_t st);
+    void precondition(const c32* b, c32* x, hipStream_t st);
+    template <typename T>
+    Cycle<T>& cycle();
+    LinOp<double>& level_op64(int l);
+};
+
+// ------------------------------------------------------------------ Krylov
+struct SolveResult {
+    int iterations = 0;
+    double final_residual = 0.0;   // solver's residual estimate (reference semantics)
+    double true_residual = 0.0;    // |b - A x| / |b| recomputed in FP64
+    int restarts = 0;
+};
+
+// cg_solve_core (src/solvers.cpp:16-72); x0 may be null; mg may be null.
+SolveResult cg_solve(LinOp<double>& A, const c64* b, c64* x, const c64* x0, MG* mg, int maxit, double tol,
+                     hipStream_t st);
+// fcg_solve (src/solvers.cpp:223-300) in precision T; ref_projection selects the
+// reference's conjugated projection coefficient.
+template <typename T>
+SolveResult gcr_solve(LinOp<T>& A, const cplx<T>* b, cplx<T>* x, MG* mg, int maxit, double tol, int trunc,
+                      bool ref_projection, hipStream_t st);
+// Mixed precision: FP64 iterative refinement (true residual in FP64) around FP32
+// GCR(trunc) preconditioned by the FP32 MG cycle.
+SolveResult gcr_solve_mixed(LinOp<double>& A64, LinOp<float>& A32, const c64* b, c64* x, MG* mg, int maxit,
+                            double tol, int trunc, double inner_tol, hipStream_t st);
+// fgmres_solve_generic without Ritz harvest (src/solvers.cpp:555-772)
+SolveResult fgmres_solve(LinOp<double>& A, const c64* b, c64* x, MG* mg, int restart, int maxit, double tol,
+                         hipStream_t st);
+
+double true_residual(LinOp<double>& A, const c64* b, const c64* x, hipStream_t st);
+double dev_norm(const c64* v, size_t n, hipStream_t st);
+template <typename T>
+c64 dev_dot(const cplx<T>* a, const cplx<T>* b, size_t n, hipStream_t st);
+
+}  // namespace smg

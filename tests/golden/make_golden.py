@@ -1,0 +1,102 @@
+"""Writes tests/golden/reference_vectors.npz from the reference's own code.
+
+The reference (C++17) is compiled from its sources by oracle/Makefile into
+oracle/_ref/libref.so; this script feeds it explicit seeded inputs and stores
+inputs and outputs.  Run in the container that holds the reference:
+
+    make -C oracle && python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+import oracle.ref as R  # noqa: E402
+from schwinger_mg_amd.fields import gaussian_vec, gaussian_vecs, random_gauge  # noqa: E402
+
+CONFIGS = [(0.0, 0.0), (1.0, 0.0), (1.0, 0.1), (0.0, 0.1)]
+
+
+def main():
+    assert R.available(), "oracle/_ref/libref.so is not built"
+    out = {}
+    # --- Dirac family, even-odd, clover, forces at L = 8
+    L, m = 8, 0.1
+    V = L * L
+    U = random_gauge(L, 0.5, seed=11)
+    psi = gaussian_vec(2 * V, seed=12)
+    chi = gaussian_vec(2 * V, seed=13)
+    pi = np.random.default_rng(14).normal(size=2 * V)
+    out["L8_U"], out["L8_psi"], out["L8_chi"], out["L8_pi"] = U, psi, chi, pi
+    out["L8_clover"] = R.clover(L, U)
+    out["L8_plaq"] = np.array([R.avg_plaq(L, U)])
+    out["L8_gauge_force_b2"] = R.gauge_force(L, U, 2.0)
+    out["L8_gauge_action_b2"] = np.array([R.gauge_action(L, U, 2.0)])
+    maps = R.eo_maps(L)
+    out["L8_even"], out["L8_odd"], out["L8_eoi"] = maps
+    for ci, (csw, mu) in enumerate(CONFIGS):
+        D = R.Dirac(L, U, m, 1.0, csw, mu)
+        for kind in range(4):
+            out[f"L8_c{ci}_dirac{kind}"] = D.apply(kind, psi)
+        out[f"L8_c{ci}_delta"] = D.delta(psi, pi, 0.3)
+        for kind in range(8):
+            out[f"L8_c{ci}_eo{kind}"] = D.eo(kind, psi[:V])
+        out[f"L8_c{ci}_recon"] = D.reconstruct(psi[:V], chi)
+        out[f"L8_c{ci}_recon_full"] = D.reconstruct(psi[:V])
+        out[f"L8_c{ci}_logdet"] = np.array([D.logdet()])
+        for kind in range(4):
+            out[f"L8_c{ci}_force{kind}"] = D.force(kind, psi, chi)
+        out[f"L8_c{ci}_force_eo"] = D.force(4, psi[:V], chi[:V])
+        x, it, res = D.krylov(0, psi, 500, 1e-10)
+        out[f"L8_c{ci}_cg_x"], out[f"L8_c{ci}_cg_it"] = x, np.array([it, res])
+        x, it, res = D.krylov(2, psi, 500, 1e-10, 10)
+        out[f"L8_c{ci}_fgmres_x"], out[f"L8_c{ci}_fgmres_it"] = x, np.array([it, res])
+        x, it, res = D.krylov(1, psi, 40, 1e-10, 10)
+        out[f"L8_c{ci}_gcr_x"], out[f"L8_c{ci}_gcr_it"] = x, np.array([it, res])
+        x0 = gaussian_vec(2 * V, seed=15)
+        out[f"L8_c{ci}_mr5"] = D.mr_smooth(x0, psi, 5)
+        out[f"L8_c{ci}_lambda30"] = np.array([D.lambda_max(30)])
+    out["L8_x0"] = gaussian_vec(2 * V, seed=15)
+    out["L8_pv12345"] = R.random_vec(2 * V, 12345)
+    # --- aggregation, Galerkin and multigrid at L = 16 (4x4 aggregates, k = 4)
+    L, m, b, k = 16, 0.05, 4, 4
+    V = L * L
+    U = random_gauge(L, 0.4, seed=21)
+    nulls = gaussian_vecs(k, 2 * V, seed=22)
+    rhs = gaussian_vec(2 * V, seed=23)
+    cvec = gaussian_vec((L // b) ** 2 * k, seed=24)
+    out["L16_U"], out["L16_nulls"], out["L16_rhs"], out["L16_c"] = U, nulls, rhs, cvec
+    out["L16_bv"] = R.prolong_build(L, b, k, nulls.ravel())
+    out["L16_prolong"] = R.prolong_apply(L, b, k, nulls.ravel(), 0, cvec)
+    out["L16_restrict"] = R.prolong_apply(L, b, k, nulls.ravel(), 1, rhs)
+    D = R.Dirac(L, U, m, 1.0, 1.0, 0.0)
+    links, dense = D.galerkin(b, k, nulls.ravel(), dense=True)
+    out["L16_links"], out["L16_dense"] = links, dense
+    out["L16_dense_solve"] = R.dense_solve(dense.ravel(), cvec)
+    starts = gaussian_vecs(k, 2 * V, seed=25).ravel()
+    sym = gaussian_vec(2 * V, seed=26)
+    out["L16_starts"], out["L16_sym"] = starts, sym
+    for mode in range(4):
+        x, it, res = D.mg(2, b, k, 4 * k, 20, starts, 3, 3, True, mode, rhs, 100, 1e-10, 20,
+                          sym if mode == 2 else None)
+        out[f"L16_mg2_mode{mode}_x"], out[f"L16_mg2_mode{mode}_it"] = x, np.array([it, res])
+    # 3 levels at L = 32 (level-1 aggregation: 4 consecutive coarse sites, i.e. 4x1)
+    L = 32
+    V = L * L
+    U = random_gauge(L, 0.4, seed=31)
+    n1 = (L // 4) ** 2 * k
+    starts3 = np.concatenate([gaussian_vecs(k, 2 * V, seed=32).ravel(), gaussian_vecs(k, n1, seed=33).ravel()])
+    rhs3 = gaussian_vec(2 * V, seed=34)
+    D = R.Dirac(L, U, m, 1.0, 1.0, 0.0)
+    x, _, _ = D.mg(3, 4, k, 4 * k, 10, starts3, 3, 3, True, 0, rhs3)
+    out["L32_U"], out["L32_starts"], out["L32_rhs"], out["L32_mg3_x"] = U, starts3, rhs3, x
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_vectors.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes,", len(out), "arrays")
+
+
+if __name__ == "__main__":
+    main()

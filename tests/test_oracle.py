@@ -1,0 +1,278 @@
+"""Pins the oracle (the CPU restatement used as the checker) against
+(1) the golden vectors written by the reference's own code (tests/golden/),
+(2) the live compiled reference where it is built (oracle/_ref), and
+(3) what the mathematics fixes: plane waves of the free Wilson operator, the
+    uniform-flux clover term, gamma5-hermiticity, gauge covariance, the Schur
+    decomposition, orthonormal aggregates, the dense Galerkin product and finite
+    differences of the actions.
+Each check is chosen so that a dropped term, a sign or index error or a
+transposed operand fails it.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import oracle.ref as R
+from schwinger_mg_amd.fields import (antiperiodic_y, gaussian_vec, gaussian_vecs, random_gauge,
+                                     uniform_flux_gauge)
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_vectors.npz")
+CONFIGS = [(0.0, 0.0), (1.0, 0.0), (1.0, 0.1), (0.0, 0.1)]
+
+
+@pytest.fixture(scope="module")
+def g():
+    return np.load(GOLD)
+
+
+def close(a, b, rtol=1e-13):
+    a, b = np.asarray(a), np.asarray(b)
+    scale = max(np.abs(b).max(), 1e-300)
+    return np.abs(a - b).max() <= rtol * scale
+
+
+# ------------------------------------------------------------------ golden vectors
+def test_golden_geometry_and_gauge(g):
+    L = 8
+    e, o, i = O.eo_maps(L)
+    assert np.array_equal(e, g["L8_even"]) and np.array_equal(o, g["L8_odd"]) and np.array_equal(i, g["L8_eoi"])
+    assert close(O.clover(L, g["L8_U"]), g["L8_clover"])
+    assert abs(O.avg_plaq(L, g["L8_U"]) - g["L8_plaq"][0]) < 1e-15
+    assert close(O.gauge_force(L, g["L8_U"], 2.0), g["L8_gauge_force_b2"])
+    assert abs(O.gauge_action(L, g["L8_U"], 2.0) - g["L8_gauge_action_b2"][0]) < 1e-12
+
+
+@pytest.mark.parametrize("ci", range(4))
+def test_golden_dirac_eo_forces(g, ci):
+    L, V = 8, 64
+    csw, mu = CONFIGS[ci]
+    D = O.Dirac(L, g["L8_U"], 0.1, 1.0, csw, mu)
+    psi, chi = g["L8_psi"], g["L8_chi"]
+    for kind in range(4):
+        assert close(D.apply(kind, psi), g[f"L8_c{ci}_dirac{kind}"]), kind
+    assert close(D.delta(psi, g["L8_pi"], 0.3), g[f"L8_c{ci}_delta"])
+    for kind in range(8):
+        assert close(D.eo(kind, psi[:V]), g[f"L8_c{ci}_eo{kind}"]), kind
+    assert close(D.reconstruct(psi[:V], chi), g[f"L8_c{ci}_recon"])
+    assert close(D.reconstruct(psi[:V]), g[f"L8_c{ci}_recon_full"])
+    assert abs(D.logdet() - g[f"L8_c{ci}_logdet"][0]) < 1e-12
+    assert close(D.force(psi), g[f"L8_c{ci}_force0"])
+    assert close(D.force_bilinear(chi, psi), g[f"L8_c{ci}_force1"])
+    assert close(D.hopping_force(chi, psi), g[f"L8_c{ci}_force2"])
+    if csw != 0:
+        assert close(D.logdet_force(), g[f"L8_c{ci}_force3"])
+    assert close(D.eo_force(psi[:V], chi[:V]), g[f"L8_c{ci}_force_eo"])
+
+
+@pytest.mark.parametrize("ci", range(4))
+def test_golden_solvers_smoother(g, ci):
+    L = 8
+    csw, mu = CONFIGS[ci]
+    D = O.Dirac(L, g["L8_U"], 0.1, 1.0, csw, mu)
+    A = D.op()
+    psi = g["L8_psi"]
+    x, it, res = O.cg(A, psi, 500, 1e-10)
+    assert it == g[f"L8_c{ci}_cg_it"][0] and close(x, g[f"L8_c{ci}_cg_x"], 1e-12)
+    x, it, res = O.fgmres(A, psi, 10, 500, 1e-10)
+    assert it == g[f"L8_c{ci}_fgmres_it"][0] and close(x, g[f"L8_c{ci}_fgmres_x"], 1e-12)
+    x, it, res = O.gcr(A, psi, 40, 1e-10, 10, ref_projection=True)
+    assert it == g[f"L8_c{ci}_gcr_it"][0] and close(x, g[f"L8_c{ci}_gcr_x"], 1e-12)
+    assert close(O.mr_smooth(A, g["L8_x0"], psi, 5), g[f"L8_c{ci}_mr5"])
+    assert abs(O.lambda_max(A, g["L8_pv12345"], 30) - g[f"L8_c{ci}_lambda30"][0]) < 1e-12
+
+
+def test_golden_aggregation_galerkin(g):
+    L, b, k = 16, 4, 4
+    P = O.Prolongator(L, L, 2, b, b, k).build(g["L16_nulls"].ravel())
+    assert close(P.block_vecs(), g["L16_bv"])
+    assert close(P.prolong(g["L16_c"]), g["L16_prolong"])
+    assert close(P.restrict(g["L16_rhs"]), g["L16_restrict"])
+    D = O.Dirac(L, g["L16_U"], 0.05, 1.0, 1.0, 0.0)
+    A = D.op()
+    for colored in (0, 1):
+        C = O.Coarse(A, P, colored)
+        assert close(C.links(), g["L16_links"], 1e-12)
+    assert close(O.dense_galerkin(A, P), g["L16_dense"], 1e-12)
+
+
+def test_golden_multigrid(g):
+    L, b, k = 16, 4, 4
+    D = O.Dirac(L, g["L16_U"], 0.05, 1.0, 1.0, 0.0)
+    A = D.op()
+    rhs = g["L16_rhs"]
+    mg = O.MG(A, L, L, 2, g["L16_starts"], n_levels=2, block=((b, b),), k=(k,), null_iters=20,
+              coarse_solver=1)
+    assert close(mg(rhs), g["L16_mg2_mode0_x"], 1e-12)
+    x, it, _ = O.gcr(A, rhs, 100, 1e-10, 20, mg, ref_projection=True)
+    assert it == g["L16_mg2_mode1_it"][0] and close(x, g["L16_mg2_mode1_x"], 1e-10)
+    x, it, _ = O.fgmres(A, rhs, 20, 100, 1e-10, mg)
+    assert it == g["L16_mg2_mode3_it"][0] and close(x, g["L16_mg2_mode3_x"], 1e-10)
+    mg.set_symmetric(0.8, g["L16_sym"])
+    x, it, _ = O.cg(A, rhs, 100, 1e-10, mg=mg)
+    assert it == g["L16_mg2_mode2_it"][0] and close(x, g["L16_mg2_mode2_x"], 1e-10)
+    # three levels, level-1 aggregation of 4 consecutive coarse sites = 4 x 1
+    L = 32
+    D = O.Dirac(L, g["L32_U"], 0.05, 1.0, 1.0, 0.0)
+    mg3 = O.MG(D.op(), L, L, 2, g["L32_starts"], n_levels=3, block=((4, 4), (4, 1)), k=(k, k), null_iters=10,
+               coarse_solver=1)
+    assert close(mg3(g["L32_rhs"]), g["L32_mg3_x"], 1e-12)
+    # the 9-point CG coarsest solve at a tight tolerance reproduces the dense LU
+    mg3cg = O.MG(D.op(), L, L, 2, g["L32_starts"], n_levels=3, block=((4, 4), (4, 1)), k=(k, k), null_iters=10,
+                 coarse_solver=0, coarse_tol=1e-14, coarse_maxit=5000)
+    assert close(mg3cg(g["L32_rhs"]), g["L32_mg3_x"], 1e-10)
+
+
+# ------------------------------------------------------------------ live compiled reference
+@pytest.mark.skipif(not R.available(), reason="oracle/_ref/libref.so not built (reference sources absent)")
+@pytest.mark.parametrize("L,seed", [(6, 1), (10, 2), (16, 3)])
+def test_live_reference_random(L, seed):
+    V = L * L
+    U = random_gauge(L, 0.7, seed)
+    psi = gaussian_vec(2 * V, seed + 100)
+    for csw, mu in CONFIGS:
+        Do, Dr = O.Dirac(L, U, 0.2, 1.0, csw, mu), R.Dirac(L, U, 0.2, 1.0, csw, mu)
+        for kind in range(4):
+            assert close(Do.apply(kind, psi), Dr.apply(kind, psi))
+        for kind in range(8):
+            assert close(Do.eo(kind, psi[:V]), Dr.eo(kind, psi[:V]))
+        assert close(Do.force(psi), Dr.force(0, psi))
+        assert close(Do.eo_force(psi[:V], psi[V:]), Dr.force(4, psi[:V], psi[V:]))
+
+
+# ------------------------------------------------------------------ mathematics
+def plane_wave(L, px, py, chi):
+    x = np.arange(L)
+    ph = np.exp(1j * (px * x[None, :] + py * x[:, None])).ravel()   # s = x + L*y
+    return np.stack([ph * chi[0], ph * chi[1]], axis=1).ravel(), ph
+
+
+def test_free_field_plane_waves():
+    """D(p) = 2r + m - r sum cos p_mu + i sum sigma_mu sin p_mu on U = 1."""
+    L, m, r = 8, 0.3, 1.0
+    U = np.ones(2 * L * L, np.complex128)
+    D = O.Dirac(L, U, m, r, 1.0, 0.0)          # clover vanishes at U = 1
+    s1 = np.array([[0, 1], [1, 0]]); s2 = np.array([[0, -1j], [1j, 0]])
+    chi = np.array([0.3 + 0.2j, -0.7 + 0.1j])
+    for nx, ny in [(0, 0), (1, 0), (0, 3), (2, 5), (7, 7)]:
+        px, py = 2 * np.pi * nx / L, 2 * np.pi * ny / L
+        psi, ph = plane_wave(L, px, py, chi)
+        Dp = (2 * r + m - r * (np.cos(px) + np.cos(py))) * np.eye(2) + 1j * (s1 * np.sin(px) + s2 * np.sin(py))
+        want, _ = plane_wave(L, px, py, Dp @ chi)
+        assert close(D.apply(0, psi), want, 1e-12)
+    assert np.abs(D.clover()).max() < 1e-15
+
+
+def test_uniform_flux_clover():
+    L = 8
+    U, alpha = uniform_flux_gauge(L, 3)
+    assert abs(O.avg_plaq(L, U) - np.cos(alpha)) < 1e-14
+    assert np.abs(O.clover(L, U) - np.sin(alpha)).max() < 1e-14
+
+
+@pytest.mark.parametrize("csw,mu", CONFIGS)
+def test_gamma5_hermiticity_and_normal_op(csw, mu):
+    L, V = 8, 64
+    U = random_gauge(L, 0.6, 5)
+    D = O.Dirac(L, U, 0.1, 1.0, csw, mu)
+    a, b = gaussian_vec(2 * V, 1), gaussian_vec(2 * V, 2)
+    assert abs(np.vdot(a, D.apply(0, b)) - np.vdot(D.apply(1, a), b)) < 1e-12 * np.linalg.norm(a) * np.linalg.norm(b)
+    assert close(D.apply(2, b), D.apply(1, D.apply(0, b)))
+    if mu == 0:
+        assert abs(np.vdot(a, D.apply(3, b)) - np.vdot(D.apply(3, a), b)) < 1e-11
+
+
+def test_gauge_covariance():
+    """D[U^g] = G D[U] G^dag with U_mu(s) -> g(s) U_mu(s) g(s+mu)^*."""
+    L, V = 8, 64
+    U = random_gauge(L, 0.6, 6)
+    gph = np.exp(1j * np.random.default_rng(7).uniform(-np.pi, np.pi, V))
+    s = np.arange(V); x, y = s % L, s // L
+    sxp, syp = (x + 1) % L + L * y, x + L * ((y + 1) % L)
+    Ug = U.copy()
+    Ug[:V] = gph * U[:V] * np.conj(gph[sxp])
+    Ug[V:] = gph * U[V:] * np.conj(gph[syp])
+    psi = gaussian_vec(2 * V, 8)
+    G = np.repeat(gph, 2)
+    D, Dg = O.Dirac(L, U, 0.1, 1.0, 1.0, 0.05), O.Dirac(L, Ug, 0.1, 1.0, 1.0, 0.05)
+    assert close(Dg.apply(0, G * psi), G * D.apply(0, psi), 1e-12)
+
+
+def test_schur_decomposition_solves_full_system():
+    """M x_o = b_o - D_oe D_ee^-1 b_e and reconstruct gives D x = b."""
+    L, V = 8, 64
+    U = random_gauge(L, 0.5, 9)
+    D = O.Dirac(L, U, 0.2, 1.0, 1.0, 0.0)
+    b = gaussian_vec(2 * V, 10)
+    e, o, _ = O.eo_maps(L)
+    idx = lambda sites: np.stack([2 * sites, 2 * sites + 1], 1).ravel()
+    b_e, b_o = b[idx(e)], b[idx(o)]
+    rhs_o = b_o - D.eo(0, D.eo(2, b_e))
+    M = np.stack([D.eo(5, np.eye(V)[j]) for j in range(V)], 1)
+    x_o = np.linalg.solve(M, rhs_o)
+    x = D.reconstruct(x_o, b)
+    assert close(D.apply(0, x), b, 1e-11)
+
+
+def test_aggregates_orthonormal_and_galerkin_is_dense_product():
+    L, k = 12, 3
+    V = L * L
+    U = random_gauge(L, 0.4, 11)
+    nulls = gaussian_vecs(k, 2 * V, 12)
+    P = O.Prolongator(L, L, 2, 4, 4, k).build(nulls.ravel())
+    Pm = np.stack([P.prolong(np.eye(P.coarse_dim)[j]) for j in range(P.coarse_dim)], 1)
+    assert np.abs(Pm.conj().T @ Pm - np.eye(P.coarse_dim)).max() < 1e-13      # P^dag P = I
+    assert close(P.restrict(nulls[1]), Pm.conj().T @ nulls[1])                 # R = P^dag
+    D = O.Dirac(L, U, 0.1, 1.0, 1.0, 0.0)
+    A = D.op()
+    Am = np.stack([A(np.eye(2 * V)[j]) for j in range(2 * V)], 1)
+    Ac = Pm.conj().T @ Am @ Pm
+    C = O.Coarse(A, P, 1).op()
+    Cm = np.stack([C(np.eye(P.coarse_dim)[j]) for j in range(P.coarse_dim)], 1)
+    assert close(Cm, Ac, 1e-12)
+    assert close(Cm, Cm.conj().T, 1e-12)                                        # Hermitian
+
+
+def test_antiperiodic_link_flip_leaves_gauge_quantities():
+    L = 8
+    U = random_gauge(L, 0.5, 13)
+    Ua = antiperiodic_y(U, L)
+    assert close(O.clover(L, Ua), O.clover(L, U)) and close(O.gauge_force(L, Ua, 3.0), O.gauge_force(L, U, 3.0))
+    # antiperiodic plane waves: p_y = 2 pi (n + 1/2) / L are eigenvectors of the free operator
+    U1 = antiperiodic_y(np.ones(2 * L * L, np.complex128), L)
+    D = O.Dirac(L, U1, 0.3, 1.0, 0.0, 0.0)
+    py = 2 * np.pi * 0.5 / L
+    psi, _ = plane_wave(L, 0.0, py, np.array([1.0, 0.0]))
+    # psi(x, y) = e^{i py y} is antiperiodic only if multiplied by the flip; the free
+    # operator on the flipped links maps it to D(p) psi
+    s2 = np.array([[0, -1j], [1j, 0]])
+    Dp = (2 + 0.3 - (1 + np.cos(py))) * np.eye(2) + 1j * s2 * np.sin(py)
+    want, _ = plane_wave(L, 0.0, py, Dp @ np.array([1.0, 0.0]))
+    assert close(D.apply(0, psi), want, 1e-12)
+
+
+def test_forces_central_difference():
+    """F = -dS/dtheta against central differences (the check of src/hmc_forces.cpp:869-981)."""
+    L, V, eps = 6, 36, 1e-5
+    U = random_gauge(L, 0.5, 14)
+    eta = gaussian_vec(2 * V, 15)
+    for csw in (0.0, 1.0):
+        D = O.Dirac(L, U, 0.2, 1.0, csw, 0.0)
+        phi = D.apply(1, eta)
+        X, _, _ = O.cg(D.op(), phi, 2000, 1e-14)
+        F = D.force(X)
+        Fg = O.gauge_force(L, U, 2.0)
+        for t in range(6):
+            mu, s = t % 2, (t * 7 + t // 2 * 13) % V
+            S = []
+            Sg = []
+            for sgn in (1, -1):
+                Up = U.copy()
+                Up[mu * V + s] *= np.exp(1j * sgn * eps)
+                Dp = O.Dirac(L, Up, 0.2, 1.0, csw, 0.0)
+                Xp, _, _ = O.cg(Dp.op(), phi, 2000, 1e-14)
+                S.append(np.vdot(phi, Xp).real)
+                Sg.append(O.gauge_action(L, Up, 2.0))
+            assert abs(-(S[0] - S[1]) / (2 * eps) - F[mu * V + s]) < 1e-6 * max(1.0, abs(F).max())
+            assert abs(-(Sg[0] - Sg[1]) / (2 * eps) - Fg[mu * V + s]) < 1e-6 * max(1.0, abs(Fg).max())
