@@ -38,13 +38,15 @@ def parse():
     ap.add_argument("--L", type=int, default=512)
     ap.add_argument("--mass", type=float, default=0.05)
     ap.add_argument("--sigma", type=float, default=0.4)
-    ap.add_argument("--levels", type=int, default=3)
+    ap.add_argument("--levels", type=int, default=5)
     ap.add_argument("--k", type=int, default=8)
     ap.add_argument("--null-iters", type=int, default=10)
     ap.add_argument("--pre", type=int, default=3)
     ap.add_argument("--post", type=int, default=3)
     ap.add_argument("--coarse-maxit", type=int, default=50)
     ap.add_argument("--coarse-tol", type=float, default=1e-2)
+    ap.add_argument("--coarse-solver", type=int, default=1)
+    ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--trunc", type=int, default=10)
     ap.add_argument("--inner-tol", type=float, default=1e-5)
     ap.add_argument("--fp64-steps", type=int, default=2)
@@ -91,7 +93,8 @@ def cpu_baseline(args, U, rhs, starts, threads):
     blocks = ((4, 4),) + ((2, 2),) * (args.levels - 2)
     t0 = time.time()
     mg = O.MG(A, L, L, 2, starts, n_levels=args.levels, block=blocks, k=(args.k,) * (args.levels - 1),
-              null_iters=args.null_iters, pre=args.pre, post=args.post, coarse_solver=0,
+              null_iters=args.null_iters, pre=args.pre, post=args.post,
+              coarse_solver=1 if args.coarse_solver == 1 else 0,
               coarse_maxit=args.coarse_maxit, coarse_tol=args.coarse_tol)
     setup = time.time() - t0
     t0 = time.time()
@@ -130,7 +133,8 @@ def main():
     t0 = time.time()
     mg = smg.MGHierarchy(D, starts, n_levels=args.levels, blocks=blocks, k=(args.k,) * (args.levels - 1),
                          null_iters=args.null_iters, pre_smooth=args.pre, post_smooth=args.post,
-                         coarse_maxit=args.coarse_maxit, coarse_tol=args.coarse_tol, coarse_half=True)
+                         coarse_maxit=args.coarse_maxit, coarse_tol=args.coarse_tol, coarse_half=True,
+                         coarse_solver=args.coarse_solver, use_graph=bool(args.graph))
     smg.synchronize()
     setup_s = time.time() - t0
     log(f"MG setup {setup_s:.2f} s")
@@ -200,7 +204,8 @@ def main():
                                f"{args.levels}-level MG (4x4 then 2x2 aggregates, k={args.k}), GCR({args.trunc}), "
                                "tol 1e-10 true residual",
                    "L": L, "levels": args.levels, "k": args.k, "pre": args.pre, "post": args.post,
-                   "coarse_maxit": args.coarse_maxit, "coarse_tol": args.coarse_tol},
+                   "coarse_maxit": args.coarse_maxit, "coarse_tol": args.coarse_tol,
+                   "coarse_solver": "dense inverse" if args.coarse_solver == 1 else "CG", "graph": bool(args.graph)},
         "mixed": mixed,
         "fp64_solves_per_s": round(1.0 / t_f64, 3),
         "fp64": fp64,

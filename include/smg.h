@@ -163,6 +163,9 @@ typedef struct {
     int coarse_maxit;      /* coarsest CG */
     double coarse_tol;
     int coarse_half;       /* FP32 cycle: coarse links in FP16 */
+    int coarse_solver;     /* coarsest level: 0 CG (coarse_maxit, coarse_tol); 1 dense inverse
+                              (formed once at setup; coarsest dimension <= 8192) */
+    int use_graph;         /* 1: replay the cycle from a captured hipGraph */
 } smg_mg_params;
 /* build_mg_hierarchy, src/multigrid.cpp:595-687, with every level's null-space start
  * vectors supplied (k[l] vectors of level-l dimension, level 0 first), geometric
@@ -180,8 +183,9 @@ int smg_mg_null_vecs(smg_mg* mg, int level, double* out);
 int smg_mg_block_vecs(smg_mg* mg, int level, double* out);
 int smg_mg_coarse_links(smg_mg* mg, int level, double* out);
 int smg_mg_omega(smg_mg* mg, int level, double* out);
-/* change the cycle's smoothing steps and coarsest-solve budget after setup */
-int smg_mg_set_cycle(smg_mg* mg, int pre_smooth, int post_smooth, int coarse_maxit, double coarse_tol);
+/* change the cycle's smoothing steps, coarsest-solve budget and graph replay after setup */
+int smg_mg_set_cycle(smg_mg* mg, int pre_smooth, int post_smooth, int coarse_maxit, double coarse_tol,
+                     int use_graph);
 
 /* ------------------------------------------------------------------ Krylov solvers */
 typedef struct {

@@ -36,7 +36,7 @@ class SMGError(RuntimeError):
 class MGParams(ctypes.Structure):
     _fields_ = [("n_levels", _I), ("bx", _I * 4), ("by", _I * 4), ("k", _I * 4), ("null_iters", _I),
                 ("pre_smooth", _I), ("post_smooth", _I), ("w_cycle", _I), ("coarse_maxit", _I),
-                ("coarse_tol", _D), ("coarse_half", _I)]
+                ("coarse_tol", _D), ("coarse_half", _I), ("coarse_solver", _I), ("use_graph", _I)]
 
 
 class SolveInfo(ctypes.Structure):
@@ -70,7 +70,7 @@ SIGNATURES = [
     ("smg_mg_precondition_dev", _I, [_P, _I, _P, _P, _P]), ("smg_mg_level_dim", _I, [_P, _I, _P]),
     ("smg_mg_null_vecs", _I, [_P, _I, _P]), ("smg_mg_block_vecs", _I, [_P, _I, _P]),
     ("smg_mg_coarse_links", _I, [_P, _I, _P]), ("smg_mg_omega", _I, [_P, _I, _P]),
-    ("smg_mg_set_cycle", _I, [_P, _I, _I, _I, _D]),
+    ("smg_mg_set_cycle", _I, [_P, _I, _I, _I, _D, _I]),
     ("smg_cg_solve", _I, [_P, _I, _P, _P, _P, _P, _I, _D, _P]),
     ("smg_gcr_solve", _I, [_P, _P, _I, _P, _P, _I, _D, _I, _I, _D, _P]),
     ("smg_gcr_solve_dev", _I, [_P, _P, _I, _P, _P, _I, _D, _I, _D, _P, _P]),
@@ -418,7 +418,8 @@ class MGHierarchy:
     """MGHierarchy (src/multigrid.h:41-107) built by build_mg_hierarchy."""
 
     def __init__(self, D, starts, n_levels=2, blocks=((4, 4),), k=(4,), null_iters=20, pre_smooth=3,
-                 post_smooth=3, w_cycle=True, coarse_maxit=200, coarse_tol=1e-12, coarse_half=False):
+                 post_smooth=3, w_cycle=True, coarse_maxit=200, coarse_tol=1e-12, coarse_half=False,
+                 coarse_solver=0, use_graph=False):
         p = MGParams()
         p.n_levels = n_levels
         blocks = list(blocks) + [blocks[-1]] * (4 - len(blocks))
@@ -428,6 +429,7 @@ class MGHierarchy:
             p.k[i] = k[i]
         p.null_iters, p.pre_smooth, p.post_smooth = null_iters, pre_smooth, post_smooth
         p.w_cycle, p.coarse_maxit, p.coarse_tol, p.coarse_half = int(w_cycle), coarse_maxit, coarse_tol, int(coarse_half)
+        p.coarse_solver, p.use_graph = coarse_solver, int(use_graph)
         self.params, self.D, self.n_levels = p, D, n_levels
         h = _P()
         _chk(lib().smg_mg_setup(ctypes.byref(h), D.h, ctypes.byref(p), _p(_c(starts))))
@@ -473,8 +475,8 @@ class MGHierarchy:
         _chk(lib().smg_mg_coarse_links(self.h, level, _p(out)))
         return out
 
-    def set_cycle(self, pre_smooth, post_smooth, coarse_maxit, coarse_tol):
-        _chk(lib().smg_mg_set_cycle(self.h, pre_smooth, post_smooth, coarse_maxit, coarse_tol))
+    def set_cycle(self, pre_smooth, post_smooth, coarse_maxit, coarse_tol, use_graph=False):
+        _chk(lib().smg_mg_set_cycle(self.h, pre_smooth, post_smooth, coarse_maxit, coarse_tol, int(use_graph)))
 
     def omega(self, level):
         out = _D()

@@ -2,57 +2,50 @@
 //
 // Reductions accumulate in FP64 whatever the storage precision: per-thread
 // partial, wave64 shuffle tree, LDS across the 4 waves, one partial per
-// workgroup, then a single-workgroup finalize in fixed order (deterministic).
-// Scalars that feed later kernels (MR omega, CG alpha/beta, GCR alpha) stay in
-// device memory, so a smoother or coarse solve issues no host synchronisation.
+// workgroup; the workgroup that finishes last sums the partials in index order
+// (deterministic) and runs the scalar program (reduce_finish), so no separate
+// finalize launch.  Scalars that feed later kernels (MR omega, CG alpha/beta,
+// GCR alpha) stay in device memory: a smoother or coarse solve issues no host
+// synchronisation.
 #include "smg_kernels.h"
 
 namespace smg {
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dot(const cplx<T>* __restrict__ a, const cplx<T>* __restrict__ b,
-                                                size_t n, double* part) {
+                                                size_t n, Fin fin) {
     double v[2] = {0.0, 0.0};
     for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
         const cplx<T> p = cdot(a[i], b[i]);
         v[0] += (double)p.re;
         v[1] += (double)p.im;
     }
-    block_sum<2>(v);
-    if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = v[0];
-        part[2 * blockIdx.x + 1] = v[1];
-    }
+    reduce_finish<2>(v, fin, gridDim.x, blockIdx.x);
 }
 template <typename T>
-int launch_dot_part(const cplx<T>* a, const cplx<T>* b, size_t n, double* part, hipStream_t st) {
-    int g = red_blocks(n);
-    k_dot<T><<<g, kBlock, 0, st>>>(a, b, n, part);
+void launch_dot(const cplx<T>* a, const cplx<T>* b, size_t n, const Fin& fin, hipStream_t st) {
+    k_dot<T><<<red_blocks(n), kBlock, 0, st>>>(a, b, n, fin);
     SMG_LAUNCH_CHECK();
-    return g;
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_norm2(const cplx<T>* __restrict__ a, size_t n, double* part) {
+__global__ __launch_bounds__(kBlock) void k_norm2(const cplx<T>* __restrict__ a, size_t n, Fin fin) {
     double v[1] = {0.0};
     for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
         const cplx<T> p = a[i];
         v[0] += (double)p.re * (double)p.re + (double)p.im * (double)p.im;
     }
-    block_sum<1>(v);
-    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+    reduce_finish<1>(v, fin, gridDim.x, blockIdx.x);
 }
 template <typename T>
-int launch_norm2_part(const cplx<T>* a, size_t n, double* part, hipStream_t st) {
-    int g = red_blocks(n);
-    k_norm2<T><<<g, kBlock, 0, st>>>(a, n, part);
+void launch_norm2(const cplx<T>* a, size_t n, const Fin& fin, hipStream_t st) {
+    k_norm2<T><<<red_blocks(n), kBlock, 0, st>>>(a, n, fin);
     SMG_LAUNCH_CHECK();
-    return g;
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dots(const cplx<T>* __restrict__ a, const cplx<T>* __restrict__ b,
-                                                 size_t n, double* part) {
+                                                 size_t n, Fin fin) {
     double v[3] = {0.0, 0.0, 0.0};
     for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
         const cplx<T> x = a[i];
@@ -61,125 +54,11 @@ __global__ __launch_bounds__(kBlock) void k_dots(const cplx<T>* __restrict__ a, 
         v[1] += (double)p.im;
         v[2] += (double)x.re * (double)x.re + (double)x.im * (double)x.im;
     }
-    block_sum<3>(v);
-    if (threadIdx.x == 0) {
-        part[3 * blockIdx.x] = v[0];
-        part[3 * blockIdx.x + 1] = v[1];
-        part[3 * blockIdx.x + 2] = v[2];
-    }
+    reduce_finish<3>(v, fin, gridDim.x, blockIdx.x);
 }
 template <typename T>
-int launch_dots_part(const cplx<T>* a, const cplx<T>* b, size_t n, double* part, hipStream_t st) {
-    int g = red_blocks(n);
-    k_dots<T><<<g, kBlock, 0, st>>>(a, b, n, part);
-    SMG_LAUNCH_CHECK();
-    return g;
-}
-
-// Finalize: sum partial rows, then one scalar program in thread 0 (see FinOp).
-__global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ part, int nparts, int nc,
-                                                     double* __restrict__ out, int op, double* aux) {
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int i = threadIdx.x; i < nparts; i += kBlock)
-        for (int c = 0; c < nc; c++) v[c] += part[(size_t)i * nc + c];
-    block_sum<4>(v);
-    if (threadIdx.x != 0) return;
-    int* flag = aux ? reinterpret_cast<int*>(aux + 8) : nullptr;
-    switch (op) {
-    case FIN_STORE:
-        for (int c = 0; c < nc; c++) out[c] = v[c];
-        break;
-    case FIN_MR: {
-        // src/smoother.cpp:34-37: omega = <Ar,r>/<Ar,Ar>; |<Ar,Ar>| < 1e-30 ends the sweeps,
-        // which a zero step reproduces (x and r stay unchanged from then on)
-        const double d = v[2];
-        if (fabs(d) < 1e-30) {
-            out[0] = 0.0;
-            out[1] = 0.0;
-        } else {
-            out[0] = v[0] / d;
-            out[1] = v[1] / d;
-        }
-        break;
-    }
-    case FIN_CG_INIT: {
-        const double rr = v[0];
-        aux[0] = rr;
-        aux[1] = 0.0;
-        aux[2] = aux[5] * rr;
-        aux[3] = 0.0;
-        aux[4] = rr;
-        *flag = (rr < 1e-60) ? 1 : 0;   // |b| < 1e-30: x = 0
-        break;
-    }
-    case FIN_CG_ALPHA: {
-        // src/coarse_op.h:520-523 (pAp = <p, Ap> = conj(<Ap, p>))
-        if (*flag) break;
-        const c64 pAp = {v[0], -v[1]};
-        if (sqrt(pAp.re * pAp.re + pAp.im * pAp.im) < 1e-30) {
-            *flag = 2;
-            out[0] = 0.0;
-            out[1] = 0.0;
-            break;
-        }
-        const c64 a = cdiv(c64{aux[0], aux[1]}, pAp);
-        out[0] = a.re;
-        out[1] = a.im;
-        break;
-    }
-    case FIN_CG_BETA: {
-        // src/coarse_op.h:529-540: stop on |r|/|b| < tol (or NaN/Inf), beta = rr_new/rr
-        if (*flag) break;
-        const double rr = v[0];
-        aux[3] += 1.0;
-        aux[4] = rr;
-        if (rr < aux[2] || isnan(rr) || isinf(rr)) {
-            *flag = 1;
-            break;
-        }
-        out[0] = rr / aux[0];
-        out[1] = 0.0;
-        aux[0] = rr;
-        aux[1] = 0.0;
-        break;
-    }
-    case FIN_PCG_BETA: {
-        const c64 b = cdiv(c64{v[0], v[1]}, c64{aux[0], aux[1]});
-        out[0] = b.re;
-        out[1] = b.im;
-        aux[0] = v[0];
-        aux[1] = v[1];
-        break;
-    }
-    case FIN_GCR_ALPHA: {
-        const double apap = v[2];
-        aux[0] = apap;
-        if (apap < 1e-30) {
-            out[0] = 0.0;
-            out[1] = 0.0;
-        } else {
-            out[0] = v[0] / apap;
-            out[1] = v[1] / apap;
-        }
-        break;
-    }
-    }
-}
-void launch_finalize(const double* part, int nparts, int nc, double* out, int op, double* aux, hipStream_t st) {
-    k_finalize<<<1, kBlock, 0, st>>>(part, nparts, nc, out, op, aux);
-    SMG_LAUNCH_CHECK();
-}
-
-__global__ void k_finalize_many(const double* __restrict__ part, int nparts, int nc, double* __restrict__ out) {
-    const int c = threadIdx.x;
-    if (c >= nc) return;
-    double a = 0.0;
-    for (int i = 0; i < nparts; i++) a += part[(size_t)i * nc + c];
-    out[c] = a;
-}
-void launch_finalize_many(const double* part, int nparts, int nc, double* out, hipStream_t st) {
-    SMG_REQUIRE(nc <= 64, "finalize_many: at most 64 components");
-    k_finalize_many<<<1, 64, 0, st>>>(part, nparts, nc, out);
+void launch_dots(const cplx<T>* a, const cplx<T>* b, size_t n, const Fin& fin, hipStream_t st) {
+    k_dots<T><<<red_blocks(n), kBlock, 0, st>>>(a, b, n, fin);
     SMG_LAUNCH_CHECK();
 }
 
@@ -198,7 +77,7 @@ void launch_set_scalars(double* dst, int count, double v0, double v1, double v2,
 // <V_i, w> for i < nv (swap: <w, V_i>), FP64 accumulation
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_multi_dot(VecPtrs<T> V, int nv, const cplx<T>* __restrict__ w,
-                                                      size_t n, int swap, double* part) {
+                                                      size_t n, int swap, Fin fin) {
     double acc[64];
 #pragma unroll
     for (int c = 0; c < 64; c++) acc[c] = 0.0;
@@ -213,18 +92,14 @@ __global__ __launch_bounds__(kBlock) void k_multi_dot(VecPtrs<T> V, int nv, cons
             }
         }
     }
-    block_sum<64>(acc);
-    if (threadIdx.x == 0)
-        for (int c = 0; c < 2 * nv; c++) part[(size_t)blockIdx.x * 2 * nv + c] = acc[c];
+    reduce_finish<64>(acc, fin, gridDim.x, blockIdx.x);
 }
 template <typename T>
-int launch_multi_dot(const VecPtrs<T>& V, int nv, const cplx<T>* w, size_t n, int swap, double* part,
-                     hipStream_t st) {
+void launch_multi_dot(const VecPtrs<T>& V, int nv, const cplx<T>* w, size_t n, int swap, const Fin& fin,
+                      hipStream_t st) {
     SMG_REQUIRE(nv >= 1 && nv <= 32, "multi_dot: 1..32 vectors");
-    int g = red_blocks(n);
-    k_multi_dot<T><<<g, kBlock, 0, st>>>(V, nv, w, n, swap, part);
+    k_multi_dot<T><<<red_blocks(n), kBlock, 0, st>>>(V, nv, w, n, swap, fin);
     SMG_LAUNCH_CHECK();
-    return g;
 }
 
 template <typename T>
@@ -308,7 +183,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_update(cplx<T>* __restrict__ x, cplx<T>* __restrict__ r,
                                                       const cplx<T>* __restrict__ p,
                                                       const cplx<T>* __restrict__ Ap, const double* __restrict__ s,
-                                                      const int* __restrict__ flag, size_t n, double* part) {
+                                                      const int* __restrict__ flag, size_t n, Fin fin) {
     double v[1] = {0.0};
     const bool live = (*flag == 0);
     if (live) {
@@ -320,16 +195,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(cplx<T>* __restrict__ x, c
             v[0] += (double)ri.re * (double)ri.re + (double)ri.im * (double)ri.im;
         }
     }
-    block_sum<1>(v);
-    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+    reduce_finish<1>(v, fin, gridDim.x, blockIdx.x);
 }
 template <typename T>
-int launch_cg_update(cplx<T>* x, cplx<T>* r, const cplx<T>* p, const cplx<T>* Ap, const double* s, const int* flag,
-                     size_t n, double* part, hipStream_t st) {
-    int g = red_blocks(n);
-    k_cg_update<T><<<g, kBlock, 0, st>>>(x, r, p, Ap, s, flag, n, part);
+void launch_cg_update(cplx<T>* x, cplx<T>* r, const cplx<T>* p, const cplx<T>* Ap, const double* s, const int* flag,
+                      size_t n, const Fin& fin, hipStream_t st) {
+    k_cg_update<T><<<red_blocks(n), kBlock, 0, st>>>(x, r, p, Ap, s, flag, n, fin);
     SMG_LAUNCH_CHECK();
-    return g;
 }
 
 template <typename T>
@@ -377,22 +249,15 @@ void launch_zero(cplx<T>* p, size_t n, hipStream_t st) {
     SMG_LAUNCH_CHECK();
 }
 
-__global__ __launch_bounds__(kBlock) void k_resid_norm(c64* __restrict__ r, const c64* __restrict__ b,
-                                                       const c64* __restrict__ Ax, size_t n, double* part) {
-    double v[1] = {0.0};
-    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
-        const c64 d = b[i] - Ax[i];
-        r[i] = d;
-        v[0] += d.re * d.re + d.im * d.im;
-    }
-    block_sum<1>(v);
-    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+template <typename T>
+__global__ void k_copy(cplx<T>* __restrict__ d, const cplx<T>* __restrict__ src, size_t n) {
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) d[i] = src[i];
 }
-int launch_resid_norm(c64* r, const c64* b, const c64* Ax, size_t n, double* part, hipStream_t st) {
-    int g = red_blocks(n);
-    k_resid_norm<<<g, kBlock, 0, st>>>(r, b, Ax, n, part);
+template <typename T>
+void launch_copy(cplx<T>* dst, const cplx<T>* src, size_t n, hipStream_t st) {
+    k_copy<T><<<grid_for(n), kBlock, 0, st>>>(dst, src, n);
     SMG_LAUNCH_CHECK();
-    return g;
 }
 
 __global__ void k_accum_mixed(c64* __restrict__ x, const c32* __restrict__ e, size_t n) {
@@ -416,18 +281,19 @@ void launch_g5(cplx<T>* v, size_t nsites, hipStream_t st) {
 }
 
 #define SMG_INST(T)                                                                                          \
-    template int launch_dot_part<T>(const cplx<T>*, const cplx<T>*, size_t, double*, hipStream_t);          \
-    template int launch_norm2_part<T>(const cplx<T>*, size_t, double*, hipStream_t);                        \
-    template int launch_dots_part<T>(const cplx<T>*, const cplx<T>*, size_t, double*, hipStream_t);         \
+    template void launch_dot<T>(const cplx<T>*, const cplx<T>*, size_t, const Fin&, hipStream_t);           \
+    template void launch_norm2<T>(const cplx<T>*, size_t, const Fin&, hipStream_t);                         \
+    template void launch_dots<T>(const cplx<T>*, const cplx<T>*, size_t, const Fin&, hipStream_t);          \
+    template void launch_copy<T>(cplx<T>*, const cplx<T>*, size_t, hipStream_t);                            \
     template void launch_axpby<T>(cplx<T>*, const cplx<T>*, c64, c64, size_t, hipStream_t);                 \
     template void launch_axpy_dev<T>(cplx<T>*, const cplx<T>*, const double*, int, size_t, hipStream_t);    \
     template void launch_mr_update<T>(cplx<T>*, cplx<T>*, const cplx<T>*, const double*, size_t, hipStream_t); \
-    template int launch_cg_update<T>(cplx<T>*, cplx<T>*, const cplx<T>*, const cplx<T>*, const double*,     \
-                                     const int*, size_t, double*, hipStream_t);                              \
+    template void launch_cg_update<T>(cplx<T>*, cplx<T>*, const cplx<T>*, const cplx<T>*, const double*,    \
+                                      const int*, size_t, const Fin&, hipStream_t);                          \
     template void launch_xpay_dev<T>(cplx<T>*, const cplx<T>*, const double*, const int*, size_t, hipStream_t); \
     template void launch_zero<T>(cplx<T>*, size_t, hipStream_t);                                             \
     template void launch_g5<T>(cplx<T>*, size_t, hipStream_t);                                              \
-    template int launch_multi_dot<T>(const VecPtrs<T>&, int, const cplx<T>*, size_t, int, double*, hipStream_t); \
+    template void launch_multi_dot<T>(const VecPtrs<T>&, int, const cplx<T>*, size_t, int, const Fin&, hipStream_t); \
     template void launch_gcr_orth<T>(cplx<T>*, cplx<T>*, const VecPtrs<T>&, const VecPtrs<T>&, int,         \
                                      const double*, const double* const*, size_t, hipStream_t);
 SMG_INST(double)

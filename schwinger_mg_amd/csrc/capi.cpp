@@ -114,27 +114,31 @@ int smg_eo_maps(int L, int* even, int* odd, int* eoi) {
     });
 }
 
-static double reduce_read(int g, hipStream_t st) {
-    Context& cx = Context::get();
-    DevMem out(8 * sizeof(double));
-    launch_finalize(cx.part_ptr(), g, 1, out.as<double>(), FIN_STORE, nullptr, st);
-    double h;
-    cx.read(out.as<double>(), &h, 1);
-    return h;
-}
+// a one-component FIN_STORE reduction target and its host read-back
+struct Scalar1 {
+    DevMem m{8 * sizeof(double)};
+    Fin fin() { return Context::get().fin(FIN_STORE, m.as<double>()); }
+    double read() {
+        double h;
+        Context::get().read(m.as<double>(), &h, 1);
+        return h;
+    }
+};
 
 int smg_avg_plaq(int L, const double* U, double* out) {
     return guard([&] {
         Field<double> u = up(U, 2 * (size_t)L * L);
-        int g = launch_plaq_sum(L, u.p(), Context::get().part_ptr(), S());
-        *out = reduce_read(g, S()) / ((double)L * L);
+        Scalar1 r;
+        launch_plaq_sum(L, u.p(), r.fin(), S());
+        *out = r.read() / ((double)L * L);
     });
 }
 int smg_gauge_action(int L, const double* U, double beta, double* out) {
     return guard([&] {
         Field<double> u = up(U, 2 * (size_t)L * L);
-        int g = launch_plaq_sum(L, u.p(), Context::get().part_ptr(), S());
-        *out = -beta * reduce_read(g, S());
+        Scalar1 r;
+        launch_plaq_sum(L, u.p(), r.fin(), S());
+        *out = -beta * r.read();
     });
 }
 int smg_gauge_force(int L, const double* U, double beta, double* F) {
@@ -191,16 +195,16 @@ static void dirac_kind(const Dirac& d, LinOp<T>& op, int kind, const cplx<T>* sr
     const T mu = (T)d.mu_t;
     switch (kind) {
     case SMG_D:
-        launch_dslash<T>(v, dst, src, mu, 0, 0, EPI_NONE, nullptr, nullptr, nullptr, nrhs, n, st);
+        launch_dslash<T>(v, dst, src, mu, 0, 0, EPI_NONE, nullptr, nullptr, Fin(), nrhs, n, st);
         break;
     case SMG_DDAG:
-        launch_dslash<T>(v, dst, src, -mu, 1, 1, EPI_NONE, nullptr, nullptr, nullptr, nrhs, n, st);
+        launch_dslash<T>(v, dst, src, -mu, 1, 1, EPI_NONE, nullptr, nullptr, Fin(), nrhs, n, st);
         break;
     case SMG_G5D:
-        launch_dslash<T>(v, dst, src, mu, 0, 1, EPI_NONE, nullptr, nullptr, nullptr, nrhs, n, st);
+        launch_dslash<T>(v, dst, src, mu, 0, 1, EPI_NONE, nullptr, nullptr, Fin(), nrhs, n, st);
         break;
     case SMG_DDAGD:
-        for (int i = 0; i < nrhs; i++) op.apply(dst + i * n, src + i * n, EPI_NONE, nullptr, nullptr, nullptr, st);
+        for (int i = 0; i < nrhs; i++) op.apply(dst + i * n, src + i * n, st);
         break;
     default:
         SMG_REQUIRE(false, "unknown Dirac kind");
@@ -250,17 +254,17 @@ static void eo_kind(smg_dirac* D, int kind, const cplx<T>* s, cplx<T>* d, hipStr
     else sop = D->schur32.get();
     switch (kind) {
     case SMG_EO_HOP_EO:
-        launch_eo_hop<T>(v, d, s, 1, 0, mu, 0, 0, nullptr, EPI_NONE, nullptr, nullptr, nullptr, st);
+        launch_eo_hop<T>(v, d, s, 1, 0, mu, 0, 0, nullptr, EPI_NONE, nullptr, nullptr, Fin(), st);
         break;
     case SMG_EO_HOP_OE:
-        launch_eo_hop<T>(v, d, s, 0, 0, mu, 0, 0, nullptr, EPI_NONE, nullptr, nullptr, nullptr, st);
+        launch_eo_hop<T>(v, d, s, 0, 0, mu, 0, 0, nullptr, EPI_NONE, nullptr, nullptr, Fin(), st);
         break;
     case SMG_EO_EE_INV: launch_eo_diag<T>(v, d, s, 0, 1, mu, st); break;
     case SMG_EO_OO_INV: launch_eo_diag<T>(v, d, s, 1, 1, mu, st); break;
     case SMG_EO_OO: launch_eo_diag<T>(v, d, s, 1, 0, mu, st); break;
     case SMG_EO_SCHUR: sop->apply_m(d, s, 0, st); break;
     case SMG_EO_SCHUR_DAG: sop->apply_m(d, s, 1, st); break;
-    case SMG_EO_SCHUR_DAG_SCHUR: sop->apply(d, s, EPI_NONE, nullptr, nullptr, nullptr, st); break;
+    case SMG_EO_SCHUR_DAG_SCHUR: sop->apply(d, s, st); break;
     default: SMG_REQUIRE(false, "unknown even-odd kind");
     }
 }
@@ -290,7 +294,7 @@ int smg_eo_reconstruct(smg_dirac* D, const double* x_o, const double* phi, doubl
         const size_t nh = (size_t)dd.V, n = 2 * nh;
         Field<double> xo = up(x_o, nh), h(nh), te(nh), xe(nh), full(n);
         const DiracView<double> v = dd.v64();
-        launch_eo_hop<double>(v, h.p(), xo.p(), 0, 0, dd.mu_t, 0, 0, nullptr, EPI_NONE, nullptr, nullptr, nullptr,
+        launch_eo_hop<double>(v, h.p(), xo.p(), 0, 0, dd.mu_t, 0, 0, nullptr, EPI_NONE, nullptr, nullptr, Fin(),
                               S());
         if (phi) {
             Field<double> ph = up(phi, n);
@@ -309,8 +313,9 @@ int smg_eo_logdet(smg_dirac* D, double* out) {
     return guard([&] {
         SMG_REQUIRE(D && out, "null argument");
         SMG_REQUIRE(D->D->L % 2 == 0, "even-odd operators need an even L");
-        int g = launch_logdet(D->D->v64(), D->D->mu_t, Context::get().part_ptr(), S());
-        *out = reduce_read(g, S());
+        Scalar1 r;
+        launch_logdet(D->D->v64(), D->D->mu_t, r.fin(), S());
+        *out = r.read();
     });
 }
 
@@ -433,17 +438,17 @@ int smg_coarse_apply(smg_coarse* C, int prec, const double* src, double* dst) {
         Field<double> s = up(src, n), d(n);
         if (prec == SMG_F64) {
             CoarseLinOp<double, c64> op(*C->C);
-            op.apply(d.p(), s.p(), EPI_NONE, nullptr, nullptr, nullptr, S());
+            op.apply(d.p(), s.p(), S());
         } else {
             Field<float> s32(n), d32(n);
             launch_convert<float, double>(s32.p(), s.p(), n, S());
             if (prec == SMG_F32) {
                 CoarseLinOp<float, c32> op(*C->C);
-                op.apply(d32.p(), s32.p(), EPI_NONE, nullptr, nullptr, nullptr, S());
+                op.apply(d32.p(), s32.p(), S());
             } else {
                 SMG_REQUIRE(prec == SMG_F16, "bad precision");
                 CoarseLinOp<float, ch16> op(*C->C);
-                op.apply(d32.p(), s32.p(), EPI_NONE, nullptr, nullptr, nullptr, S());
+                op.apply(d32.p(), s32.p(), S());
             }
             launch_convert<double, float>(d.p(), d32.p(), n, S());
         }
@@ -515,6 +520,9 @@ int smg_mg_setup(smg_mg** out, smg_dirac* D, const smg_mg_params* prm, const dou
         c.coarse_maxit = prm->coarse_maxit;
         c.coarse_tol = prm->coarse_tol;
         c.coarse_half = prm->coarse_half;
+        c.coarse_solver = prm->coarse_solver;
+        c.use_graph = prm->use_graph;
+        SMG_REQUIRE(c.coarse_solver == 0 || c.coarse_solver == 1, "coarse_solver must be 0 or 1");
         SMG_REQUIRE(c.n_levels >= 2 && c.n_levels <= 5, "n_levels must be 2..5");
         for (int l = 0; l + 1 < c.n_levels; l++)
             SMG_REQUIRE(c.k[l] >= 1 && c.k[l] <= 64 && c.bx[l] >= 1 && c.by[l] >= 1, "bad level parameters");
@@ -590,13 +598,15 @@ int smg_mg_omega(smg_mg* mg, int level, double* out) {
     });
 }
 
-int smg_mg_set_cycle(smg_mg* mg, int pre, int post, int coarse_maxit, double coarse_tol) {
+int smg_mg_set_cycle(smg_mg* mg, int pre, int post, int coarse_maxit, double coarse_tol, int use_graph) {
     return guard([&] {
         SMG_REQUIRE(mg && pre >= 0 && post >= 0 && coarse_maxit >= 1, "bad arguments");
         mg->mg->cfg.pre = pre;
         mg->mg->cfg.post = post;
         mg->mg->cfg.coarse_maxit = coarse_maxit;
         mg->mg->cfg.coarse_tol = coarse_tol;
+        mg->mg->cfg.use_graph = use_graph;
+        mg->mg->reset_graphs();
     });
 }
 
@@ -691,7 +701,7 @@ static void fermion_force_dev(smg_dirac* D, const c64* X, double* F, hipStream_t
     const Dirac& d = *D->D;
     const size_t n = 2 * (size_t)d.V;
     Field<double> chi(n);
-    launch_dslash<double>(d.v64(), chi.p(), X, d.mu_t, 0, 0, EPI_NONE, nullptr, nullptr, nullptr, 1, n, st);
+    launch_dslash<double>(d.v64(), chi.p(), X, d.mu_t, 0, 0, EPI_NONE, nullptr, nullptr, Fin(), 1, n, st);
     launch_force_backward(d.v64(), d.csw, chi.p(), X, F, st);
 }
 int smg_fermion_force(smg_dirac* D, const double* X, double* F) {
@@ -743,9 +753,9 @@ static void schur_deriv_plus(const Dirac& d, const c64* chi_o, const c64* psi_o,
     const size_t nh = (size_t)d.V, n = 2 * nh;
     const DiracView<double> v = d.v64();
     Field<double> tmp2(nh), tmp3(nh), chif(n), psif(n);
-    launch_eo_hop<double>(v, tmp2.p(), psi_o, 0, 1, d.mu_t, 0, 0, nullptr, EPI_NONE, nullptr, nullptr, nullptr, st);
+    launch_eo_hop<double>(v, tmp2.p(), psi_o, 0, 1, d.mu_t, 0, 0, nullptr, EPI_NONE, nullptr, nullptr, Fin(), st);
     // tmp3 = D_ee(-mu)^-1 g5 H (g5 chi): the diagonal commutes with g5
-    launch_eo_hop<double>(v, tmp3.p(), chi_o, 0, 1, -d.mu_t, 1, 1, nullptr, EPI_NONE, nullptr, nullptr, nullptr, st);
+    launch_eo_hop<double>(v, tmp3.p(), chi_o, 0, 1, -d.mu_t, 1, 1, nullptr, EPI_NONE, nullptr, nullptr, Fin(), st);
     if (d.csw != 0.0) {
         DevMem w(nh * sizeof(double));
         launch_eo_force_weights(v, chi_o, psi_o, tmp3.p(), tmp2.p(), w.as<double>(), st);

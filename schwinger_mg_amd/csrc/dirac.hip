@@ -45,8 +45,7 @@ template <typename T, int EPI>
 __global__ __launch_bounds__(kBlock) void k_dslash(DiracView<T> D, cplx<T>* __restrict__ dst,
                                                    const cplx<T>* __restrict__ src, T mu, int g5in, int g5out,
                                                    const cplx<T>* __restrict__ bvec,
-                                                   const cplx<T>* __restrict__ aux, double* __restrict__ part,
-                                                   size_t vstride) {
+                                                   const cplx<T>* __restrict__ aux, Fin fin, size_t vstride) {
     const int L = D.L, V = L * L;
     const int s = blockIdx.x * kBlock + threadIdx.x;
     const size_t off = (size_t)blockIdx.y * vstride;
@@ -89,35 +88,27 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DiracView<T> D, cplx<T>* __re
             acc[2] = (double)norm2(o0) + (double)norm2(o1);
         }
     }
-    if (EPI == EPI_DOTS) {
-        block_sum<3>(acc);
-        if (threadIdx.x == 0) {
-            double* p = part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 3;
-            p[0] = acc[0];
-            p[1] = acc[1];
-            p[2] = acc[2];
-        }
-    }
+    if (EPI == EPI_DOTS) reduce_finish<3>(acc, fin, gridDim.x * gridDim.y, blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 template <typename T>
-int launch_dslash(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T mu, int g5in, int g5out, Epi epi,
-                  const cplx<T>* b, const cplx<T>* aux, double* part, int nrhs, size_t vstride, hipStream_t st) {
+void launch_dslash(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T mu, int g5in, int g5out, Epi epi,
+                   const cplx<T>* b, const cplx<T>* aux, const Fin& fin, int nrhs, size_t vstride, hipStream_t st) {
     const int V = D.L * D.L;
     dim3 grid(grid_for(V), nrhs);
     switch (epi) {
     case EPI_NONE:
-        k_dslash<T, EPI_NONE><<<grid, kBlock, 0, st>>>(D, dst, src, mu, g5in, g5out, b, aux, part, vstride);
+        k_dslash<T, EPI_NONE><<<grid, kBlock, 0, st>>>(D, dst, src, mu, g5in, g5out, b, aux, fin, vstride);
         break;
     case EPI_RESID:
-        k_dslash<T, EPI_RESID><<<grid, kBlock, 0, st>>>(D, dst, src, mu, g5in, g5out, b, aux, part, vstride);
+        k_dslash<T, EPI_RESID><<<grid, kBlock, 0, st>>>(D, dst, src, mu, g5in, g5out, b, aux, fin, vstride);
         break;
     default:
-        k_dslash<T, EPI_DOTS><<<grid, kBlock, 0, st>>>(D, dst, src, mu, g5in, g5out, b, aux, part, vstride);
+        SMG_REQUIRE(nrhs == 1, "the reducing epilogue takes one right-hand side");
+        k_dslash<T, EPI_DOTS><<<grid, kBlock, 0, st>>>(D, dst, src, mu, g5in, g5out, b, aux, fin, vstride);
         break;
     }
     SMG_LAUNCH_CHECK();
-    return grid.x;
 }
 
 // ----------------------------------------------------------------- even-odd
@@ -128,7 +119,7 @@ __global__ __launch_bounds__(kBlock) void k_eo_hop(DiracView<T> D, cplx<T>* __re
                                                    const cplx<T>* __restrict__ src, int p, T mu, int g5src,
                                                    int g5aux, int g5out, const cplx<T>* __restrict__ aux,
                                                    const cplx<T>* __restrict__ bvec,
-                                                   const cplx<T>* __restrict__ aux2, double* __restrict__ part) {
+                                                   const cplx<T>* __restrict__ aux2, Fin fin) {
     const int L = D.L, V = L * L, Lh = L >> 1, Vh = V >> 1;
     const int i = blockIdx.x * kBlock + threadIdx.x;
     double acc[3] = {0.0, 0.0, 0.0};
@@ -180,28 +171,20 @@ __global__ __launch_bounds__(kBlock) void k_eo_hop(DiracView<T> D, cplx<T>* __re
             acc[2] = (double)norm2(o0) + (double)norm2(o1);
         }
     }
-    if (EPI == EPI_DOTS) {
-        block_sum<3>(acc);
-        if (threadIdx.x == 0) {
-            double* q = part + (size_t)blockIdx.x * 3;
-            q[0] = acc[0];
-            q[1] = acc[1];
-            q[2] = acc[2];
-        }
-    }
+    if (EPI == EPI_DOTS) reduce_finish<3>(acc, fin, gridDim.x, blockIdx.x);
 }
 
 template <typename T>
-int launch_eo_hop(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, int p, int mode, T mu, int g5in,
-                  int g5out, const cplx<T>* aux, Epi epi, const cplx<T>* b, const cplx<T>* aux2, double* part,
-                  hipStream_t st) {
+void launch_eo_hop(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, int p, int mode, T mu, int g5in,
+                   int g5out, const cplx<T>* aux, Epi epi, const cplx<T>* b, const cplx<T>* aux2, const Fin& fin,
+                   hipStream_t st) {
     SMG_REQUIRE(D.L % 2 == 0, "even-odd operators need an even L");
     const int Vh = D.L * D.L / 2;
     const int g = grid_for(Vh);
     // In mode 2 the g5 flag applies to aux (the diagonal term); src is the even tmp.
     const int g5src = (mode == 2) ? 0 : g5in, g5aux = (mode == 2) ? g5in : 0;
 #define SMG_EO(M, E)                                                                                  \
-    k_eo_hop<T, M, E><<<g, kBlock, 0, st>>>(D, dst, src, p, mu, g5src, g5aux, g5out, aux, b, aux2, part)
+    k_eo_hop<T, M, E><<<g, kBlock, 0, st>>>(D, dst, src, p, mu, g5src, g5aux, g5out, aux, b, aux2, fin)
     if (mode == 0) SMG_EO(0, EPI_NONE);
     else if (mode == 1) SMG_EO(1, EPI_NONE);
     else if (epi == EPI_NONE) SMG_EO(2, EPI_NONE);
@@ -209,7 +192,6 @@ int launch_eo_hop(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, int p
     else SMG_EO(2, EPI_DOTS);
 #undef SMG_EO
     SMG_LAUNCH_CHECK();
-    return g;
 }
 
 template <typename T>
@@ -323,7 +305,7 @@ void launch_delta_dirac(const DiracView<double>& D, c64* dst, const c64* src, co
 }
 
 // src/dirac.h:348-366: partial sums over even sites
-__global__ void k_logdet(DiracView<double> D, double mu, double* part) {
+__global__ __launch_bounds__(kBlock) void k_logdet(DiracView<double> D, double mu, Fin fin) {
     const int L = D.L, Lh = L >> 1, Vh = L * L / 2;
     const int i = blockIdx.x * kBlock + threadIdx.x;
     double v[1] = {0.0};
@@ -333,17 +315,14 @@ __global__ void k_logdet(DiracView<double> D, double mu, double* part) {
         const double d0 = D.diag + C, d1 = D.diag - C;
         v[0] = 0.5 * log(d0 * d0 + mu * mu) + 0.5 * log(d1 * d1 + mu * mu);
     }
-    block_sum<1>(v);
-    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+    reduce_finish<1>(v, fin, gridDim.x, blockIdx.x);
 }
-int launch_logdet(const DiracView<double>& D, double mu, double* part, hipStream_t st) {
-    int g = grid_for(D.L * D.L / 2);
-    k_logdet<<<g, kBlock, 0, st>>>(D, mu, part);
+void launch_logdet(const DiracView<double>& D, double mu, const Fin& fin, hipStream_t st) {
+    k_logdet<<<grid_for(D.L * D.L / 2), kBlock, 0, st>>>(D, mu, fin);
     SMG_LAUNCH_CHECK();
-    return g;
 }
 
-__global__ void k_plaq_sum(int L, const c64* __restrict__ U, double* part) {
+__global__ __launch_bounds__(kBlock) void k_plaq_sum(int L, const c64* __restrict__ U, Fin fin) {
     const int V = L * L;
     const int s = blockIdx.x * kBlock + threadIdx.x;
     double v[1] = {0.0};
@@ -352,14 +331,11 @@ __global__ void k_plaq_sum(int L, const c64* __restrict__ U, double* part) {
         const c64 P = U[s] * U[V + nb_site(L, x + 1, y)] * conj(U[nb_site(L, x, y + 1)]) * conj(U[V + s]);
         v[0] = P.re;
     }
-    block_sum<1>(v);
-    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+    reduce_finish<1>(v, fin, gridDim.x, blockIdx.x);
 }
-int launch_plaq_sum(int L, const c64* U, double* part, hipStream_t st) {
-    int g = grid_for(L * L);
-    k_plaq_sum<<<g, kBlock, 0, st>>>(L, U, part);
+void launch_plaq_sum(int L, const c64* U, const Fin& fin, hipStream_t st) {
+    k_plaq_sum<<<grid_for(L * L), kBlock, 0, st>>>(L, U, fin);
     SMG_LAUNCH_CHECK();
-    return g;
 }
 
 // ----------------------------------------------------------------- forces (FP64)
@@ -556,10 +532,10 @@ void launch_gauge_force(int L, const c64* U, double beta, double* F, hipStream_t
 
 // ----------------------------------------------------------------- instantiations
 #define SMG_INST(T)                                                                                                 \
-    template int launch_dslash<T>(const DiracView<T>&, cplx<T>*, const cplx<T>*, T, int, int, Epi, const cplx<T>*, \
-                                  const cplx<T>*, double*, int, size_t, hipStream_t);                               \
-    template int launch_eo_hop<T>(const DiracView<T>&, cplx<T>*, const cplx<T>*, int, int, T, int, int,             \
-                                  const cplx<T>*, Epi, const cplx<T>*, const cplx<T>*, double*, hipStream_t);       \
+    template void launch_dslash<T>(const DiracView<T>&, cplx<T>*, const cplx<T>*, T, int, int, Epi, const cplx<T>*, \
+                                   const cplx<T>*, const Fin&, int, size_t, hipStream_t);                            \
+    template void launch_eo_hop<T>(const DiracView<T>&, cplx<T>*, const cplx<T>*, int, int, T, int, int,             \
+                                   const cplx<T>*, Epi, const cplx<T>*, const cplx<T>*, const Fin&, hipStream_t);    \
     template void launch_eo_diag<T>(const DiracView<T>&, cplx<T>*, const cplx<T>*, int, int, T, hipStream_t);       \
     template void launch_eo_gather<T>(int, cplx<T>*, const cplx<T>*, int, hipStream_t);                             \
     template void launch_eo_scatter<T>(int, cplx<T>*, const cplx<T>*, int, hipStream_t);

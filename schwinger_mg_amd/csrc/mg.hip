@@ -134,12 +134,12 @@ __device__ __forceinline__ int cnb(int nbx, int nby, int b, int d) {
     return (bx + dx + nbx) % nbx + ((by + dy + nby) % nby) * nbx;
 }
 
-// src/coarse_op.h:282-297: one work-item per coarse dof (b, ki)
+// src/coarse_op.h:282-297.  Generic form: one work-item per coarse dof (b, ki).
 template <typename T, typename TL, int EPI>
 __global__ __launch_bounds__(kBlock) void k_coarse_apply(CoarseView<TL> C, cplx<T>* __restrict__ dst,
                                                          const cplx<T>* __restrict__ src,
                                                          const cplx<T>* __restrict__ bvec,
-                                                         const cplx<T>* __restrict__ aux, double* part) {
+                                                         const cplx<T>* __restrict__ aux, Fin fin) {
     const int k = C.k, nb = C.nx * C.ny;
     const int t = blockIdx.x * kBlock + threadIdx.x;
     double acc[3] = {0.0, 0.0, 0.0};
@@ -164,26 +164,82 @@ __global__ __launch_bounds__(kBlock) void k_coarse_apply(CoarseView<TL> C, cplx<
             acc[2] = norm2(out);
         }
     }
-    if (EPI == EPI_DOTS) {
-        block_sum<3>(acc);
-        if (threadIdx.x == 0) {
-            double* q = part + (size_t)blockIdx.x * 3;
-            q[0] = acc[0];
-            q[1] = acc[1];
-            q[2] = acc[2];
+    if (EPI == EPI_DOTS) reduce_finish<3>(acc, fin, gridDim.x, blockIdx.x);
+}
+
+// Power-of-two k: one lane per link entry (b, ki, kj), so the k x k block of each
+// direction is one coalesced read, the grid is nb*k*k lanes (the whole chip even
+// on small coarse lattices), and the row sums over kj are shuffles inside groups
+// of K lanes.
+template <typename T, typename TL, int EPI, int K>
+__global__ __launch_bounds__(kBlock) void k_coarse_apply_lanes(CoarseView<TL> C, cplx<T>* __restrict__ dst,
+                                                               const cplx<T>* __restrict__ src,
+                                                               const cplx<T>* __restrict__ bvec,
+                                                               const cplx<T>* __restrict__ aux, Fin fin) {
+    constexpr int KK = K * K;
+    const int nb = C.nx * C.ny;
+    const size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    const int b = (int)(t / KK), e = (int)(t % KK), kj = e % K;
+    cplx<T> sum = {(T)0, (T)0};
+    if (b < nb) {
+        const int bx = b % C.nx, by = b / C.nx;
+        const int xm = (bx == 0 ? C.nx - 1 : bx - 1), xp = (bx + 1 == C.nx ? 0 : bx + 1);
+        const int ym = (by == 0 ? C.ny - 1 : by - 1), yp = (by + 1 == C.ny ? 0 : by + 1);
+        const int nbr[9] = {b, xp + by * C.nx, xm + by * C.nx, bx + yp * C.nx, bx + ym * C.nx,
+                            xp + yp * C.nx, xp + ym * C.nx, xm + yp * C.nx, xm + ym * C.nx};
+        const TL* lk = C.link + (size_t)b * 9 * KK + e;
+#pragma unroll
+        for (int d = 0; d < 9; d++) sum = sum + cvt<T>(load_c(lk + d * KK)) * src[(size_t)nbr[d] * K + kj];
+    }
+#pragma unroll
+    for (int off = K / 2; off > 0; off >>= 1) {
+        sum.re += __shfl_xor(sum.re, off, 64);
+        sum.im += __shfl_xor(sum.im, off, 64);
+    }
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (b < nb && kj == 0) {
+        const size_t o = (size_t)b * K + e / K;
+        cplx<T> out = sum;
+        if (EPI == EPI_RESID) out = bvec[o] - out;
+        dst[o] = out;
+        if (EPI == EPI_DOTS) {
+            const cplx<T> p = cdot(out, aux[o]);
+            acc[0] = p.re;
+            acc[1] = p.im;
+            acc[2] = norm2(out);
         }
     }
+    if (EPI == EPI_DOTS) reduce_finish<3>(acc, fin, gridDim.x, blockIdx.x);
 }
+
+template <typename T, typename TL, int EPI>
+static void coarse_apply_dispatch(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* src, const cplx<T>* b,
+                                  const cplx<T>* aux, const Fin& fin, hipStream_t st) {
+    const size_t nb = (size_t)C.nx * C.ny;
+    const size_t lanes = nb * C.k * C.k;
+    switch (C.k) {
+#define SMG_CK(K)                                                                                         \
+    case K:                                                                                                 \
+        k_coarse_apply_lanes<T, TL, EPI, K><<<grid_for(lanes), kBlock, 0, st>>>(C, dst, src, b, aux, fin); \
+        break;
+        SMG_CK(2)
+        SMG_CK(4)
+        SMG_CK(8)
+        SMG_CK(16)
+        SMG_CK(32)
+#undef SMG_CK
+    default:
+        k_coarse_apply<T, TL, EPI><<<grid_for(nb * C.k), kBlock, 0, st>>>(C, dst, src, b, aux, fin);
+    }
+}
+
 template <typename T, typename TL>
-int launch_coarse_apply(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b,
-                        const cplx<T>* aux, double* part, hipStream_t st) {
-    const int n = C.nx * C.ny * C.k;
-    const int g = grid_for(n);
-    if (epi == EPI_NONE) k_coarse_apply<T, TL, EPI_NONE><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, part);
-    else if (epi == EPI_RESID) k_coarse_apply<T, TL, EPI_RESID><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, part);
-    else k_coarse_apply<T, TL, EPI_DOTS><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, part);
+void launch_coarse_apply(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b,
+                         const cplx<T>* aux, const Fin& fin, hipStream_t st) {
+    if (epi == EPI_NONE) coarse_apply_dispatch<T, TL, EPI_NONE>(C, dst, src, b, aux, fin, st);
+    else if (epi == EPI_RESID) coarse_apply_dispatch<T, TL, EPI_RESID>(C, dst, src, b, aux, fin, st);
+    else coarse_apply_dispatch<T, TL, EPI_DOTS>(C, dst, src, b, aux, fin, st);
     SMG_LAUNCH_CHECK();
-    return g;
 }
 
 // ---- Galerkin by coloured probing (the reference probes one column at a time,
@@ -222,15 +278,91 @@ void launch_probe_extract(int nbx, int nby, int k, int cx, int cy, int cls, int 
     SMG_LAUNCH_CHECK();
 }
 
+// ---- dense inverse of the coarsest operator (HPD: Gauss-Jordan without pivoting)
+// M (row-major n x n, zeroed by the caller) += the 9-point stencil of C
+__global__ void k_coarse_to_dense(CoarseView<c64> C, c64* __restrict__ M, size_t n) {
+    const int k = C.k;
+    const size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    const size_t nb = (size_t)C.nx * C.ny;
+    if (t >= nb * k * k) return;
+    const int b = (int)(t / (k * k)), ki = (int)((t / k) % k), kj = (int)(t % k);
+    const size_t row = (size_t)b * k + ki;
+    for (int d = 0; d < 9; d++) {
+        const int nbb = cnb(C.nx, C.ny, b, d);
+        const c64 v = C.link[(((size_t)b * 9 + d) * k + ki) * k + kj];
+        c64& m = M[row * n + (size_t)nbb * k + kj];
+        m = m + v;
+    }
+}
+void launch_coarse_to_dense(const CoarseView<c64>& C, c64* M, size_t n, hipStream_t st) {
+    k_coarse_to_dense<<<grid_for((size_t)C.nx * C.ny * C.k * C.k), kBlock, 0, st>>>(C, M, n);
+    SMG_LAUNCH_CHECK();
+}
+// step p of in-place Gauss-Jordan: the scaled pivot row and the pivot column
+__global__ void k_gj_pivot(const c64* __restrict__ M, size_t n, size_t p, c64* __restrict__ prow,
+                           c64* __restrict__ pcol) {
+    const c64 piv = M[p * n + p];
+    const c64 inv = cdiv(c64{1.0, 0.0}, piv);
+    for (size_t j = threadIdx.x; j < n; j += blockDim.x) {
+        prow[j] = (j == p) ? inv : M[p * n + j] * inv;
+        pcol[j] = (j == p) ? c64{0.0, 0.0} : M[j * n + p];
+    }
+}
+__global__ void k_gj_update(c64* __restrict__ M, size_t n, size_t p, const c64* __restrict__ prow,
+                            const c64* __restrict__ pcol) {
+    const size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n * n) return;
+    const size_t i = t / n, j = t % n;
+    if (i == p) {
+        M[t] = prow[j];
+        return;
+    }
+    const c64 a = (j == p) ? c64{0.0, 0.0} : M[t];
+    M[t] = a - pcol[i] * prow[j];
+}
+void launch_dense_invert(c64* M, size_t n, c64* prow, c64* pcol, hipStream_t st) {
+    for (size_t p = 0; p < n; p++) {
+        k_gj_pivot<<<1, kBlock, 0, st>>>(M, n, p, prow, pcol);
+        k_gj_update<<<grid_for(n * n), kBlock, 0, st>>>(M, n, p, prow, pcol);
+    }
+    SMG_LAUNCH_CHECK();
+}
+// y = M x, one wave64 per row
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dense_matvec(const cplx<T>* __restrict__ M, const cplx<T>* __restrict__ x,
+                                                         cplx<T>* __restrict__ y, size_t n) {
+    const size_t row = (size_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    double re = 0.0, im = 0.0;
+    if (row < n) {
+        const cplx<T>* mr = M + row * n;
+        for (size_t j = lane; j < n; j += 64) {
+            const cplx<T> p = mr[j] * x[j];
+            re += (double)p.re;
+            im += (double)p.im;
+        }
+    }
+    re = wave_sum(re);
+    im = wave_sum(im);
+    if (row < n && lane == 0) y[row] = cplx<T>{(T)re, (T)im};
+}
+template <typename T>
+void launch_dense_matvec(const cplx<T>* M, const cplx<T>* x, cplx<T>* y, size_t n, hipStream_t st) {
+    k_dense_matvec<T><<<(unsigned)((n + 3) / 4), kBlock, 0, st>>>(M, x, y, n);
+    SMG_LAUNCH_CHECK();
+}
+template void launch_dense_matvec<double>(const c64*, const c64*, c64*, size_t, hipStream_t);
+template void launch_dense_matvec<float>(const c32*, const c32*, c32*, size_t, hipStream_t);
+
 template void launch_prolong<double>(const AggView&, const c64*, const c64*, c64*, int, hipStream_t);
 template void launch_prolong<float>(const AggView&, const c32*, const c32*, c32*, int, hipStream_t);
 template void launch_restrict<double>(const AggView&, const c64*, const c64*, c64*, hipStream_t);
 template void launch_restrict<float>(const AggView&, const c32*, const c32*, c32*, hipStream_t);
-template int launch_coarse_apply<double, c64>(const CoarseView<c64>&, c64*, const c64*, Epi, const c64*,
-                                              const c64*, double*, hipStream_t);
-template int launch_coarse_apply<float, c32>(const CoarseView<c32>&, c32*, const c32*, Epi, const c32*,
-                                             const c32*, double*, hipStream_t);
-template int launch_coarse_apply<float, ch16>(const CoarseView<ch16>&, c32*, const c32*, Epi, const c32*,
-                                              const c32*, double*, hipStream_t);
+template void launch_coarse_apply<double, c64>(const CoarseView<c64>&, c64*, const c64*, Epi, const c64*,
+                                               const c64*, const Fin&, hipStream_t);
+template void launch_coarse_apply<float, c32>(const CoarseView<c32>&, c32*, const c32*, Epi, const c32*,
+                                              const c32*, const Fin&, hipStream_t);
+template void launch_coarse_apply<float, ch16>(const CoarseView<ch16>&, c32*, const c32*, Epi, const c32*,
+                                               const c32*, const Fin&, hipStream_t);
 
 }  // namespace smg

@@ -5,18 +5,61 @@
 #include <cstring>
 #include <algorithm>
 #include <deque>
+#include <map>
+#include <mutex>
 
 namespace smg {
 
 // ------------------------------------------------------------------ memory
+// Caching allocator: freed blocks are kept per rounded size and handed out again,
+// so solver temporaries cost no hipMalloc/hipFree (and no device synchronisation)
+// after their first use.  All library work runs on one stream, so a block
+// returned to the cache is only reused by work ordered after its last use.
+namespace {
+std::mutex g_pool_mu;
+std::multimap<size_t, void*>& pool() {
+    static auto* m = new std::multimap<size_t, void*>();
+    return *m;
+}
+size_t round_bytes(size_t b) {
+    if (b <= 256) return 256;
+    size_t r = 256;
+    while (r < b) r <<= 1;
+    if (r - b > (b >> 2) && b > (size_t(1) << 20)) r = (b + (size_t(1) << 20) - 1) & ~((size_t(1) << 20) - 1);
+    return r;
+}
+}  // namespace
 void DevMem::alloc(size_t b) {
     release();
-    if (b == 0) b = 16;
-    SMG_HIP(hipMalloc(&p, b));
-    bytes = b;
+    const size_t r = round_bytes(b);
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        auto it = pool().find(r);
+        if (it != pool().end()) {
+            p = it->second;
+            pool().erase(it);
+            bytes = r;
+            return;
+        }
+    }
+    hipError_t e = hipMalloc(&p, r);
+    if (e != hipSuccess) {
+        // give the cache back to the runtime and retry once
+        (void)hipGetLastError();
+        {
+            std::lock_guard<std::mutex> lk(g_pool_mu);
+            for (auto& kv : pool()) (void)hipFree(kv.second);
+            pool().clear();
+        }
+        SMG_HIP(hipMalloc(&p, r));
+    }
+    bytes = r;
 }
 void DevMem::release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        pool().emplace(bytes, p);
+    }
     p = nullptr;
     bytes = 0;
 }
@@ -27,6 +70,8 @@ Context& Context::get() {
         ctx = new Context();
         SMG_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         ctx->part.alloc(size_t(1) << 24);  // 2M doubles of partial sums
+        ctx->counter.alloc(256);
+        SMG_HIP(hipMemset(ctx->counter.p, 0, 256));
         void* h = nullptr;
         SMG_HIP(hipHostMalloc(&h, 4096, hipHostMallocDefault));
         ctx->hostscal.p = h;
@@ -57,8 +102,7 @@ struct Scalars {
 double dev_norm(const c64* v, size_t n, hipStream_t st) {
     Context& cx = Context::get();
     Scalars s(4);
-    int g = launch_norm2_part(v, n, cx.part_ptr(), st);
-    launch_finalize(cx.part_ptr(), g, 1, s.p(), FIN_STORE, nullptr, st);
+    launch_norm2<double>(v, n, cx.fin(FIN_STORE, s.p()), st);
     double h;
     cx.read(s.p(), &h, 1);
     return std::sqrt(h);
@@ -67,8 +111,7 @@ template <typename T>
 c64 dev_dot(const cplx<T>* a, const cplx<T>* b, size_t n, hipStream_t st) {
     Context& cx = Context::get();
     Scalars s(4);
-    int g = launch_dot_part(a, b, n, cx.part_ptr(), st);
-    launch_finalize(cx.part_ptr(), g, 2, s.p(), FIN_STORE, nullptr, st);
+    launch_dot<T>(a, b, n, cx.fin(FIN_STORE, s.p()), st);
     double h[2];
     cx.read(s.p(), h, 2);
     return c64{h[0], h[1]};
@@ -80,16 +123,16 @@ template <typename T>
 static double dev_norm_t(const cplx<T>* v, size_t n, hipStream_t st) {
     Context& cx = Context::get();
     Scalars s(4);
-    int g = launch_norm2_part(v, n, cx.part_ptr(), st);
-    launch_finalize(cx.part_ptr(), g, 1, s.p(), FIN_STORE, nullptr, st);
+    launch_norm2<T>(v, n, cx.fin(FIN_STORE, s.p()), st);
     double h;
     cx.read(s.p(), &h, 1);
     return std::sqrt(h);
 }
 
+// device copy as a kernel (graph-capturable)
 template <typename T>
 static void dcopy(cplx<T>* d, const cplx<T>* s, size_t n, hipStream_t st) {
-    SMG_HIP(hipMemcpyAsync(d, s, n * sizeof(cplx<T>), hipMemcpyDeviceToDevice, st));
+    launch_copy<T>(d, s, n, st);
 }
 
 // ------------------------------------------------------------------ Dirac
@@ -138,29 +181,29 @@ template <>
 DiracView<float> Dirac::view<float>() const { return v32(); }
 
 template <typename T>
-int DdagDOp<T>::apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux, double* part,
-                      hipStream_t st) {
+void DdagDOp<T>::apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux,
+                       const Fin& fin, hipStream_t st) {
     const DiracView<T> v = D.view<T>();
     const T mu = (T)D.mu_t;
-    launch_dslash<T>(v, tmp.p(), src, mu, 0, 0, EPI_NONE, nullptr, nullptr, nullptr, 1, 0, st);
-    return launch_dslash<T>(v, dst, tmp.p(), -mu, 1, 1, epi, b, aux, part, 1, 0, st);
+    launch_dslash<T>(v, tmp.p(), src, mu, 0, 0, EPI_NONE, nullptr, nullptr, Fin(), 1, 0, st);
+    launch_dslash<T>(v, dst, tmp.p(), -mu, 1, 1, epi, b, aux, fin, 1, 0, st);
 }
 
 template <typename T>
 void SchurOp<T>::apply_m(cplx<T>* dst, const cplx<T>* src, int dag, hipStream_t st) {
     const DiracView<T> v = D.view<T>();
     const T mu = dag ? (T)-D.mu_t : (T)D.mu_t;
-    launch_eo_hop<T>(v, te.p(), src, 0, 1, mu, dag, 0, nullptr, EPI_NONE, nullptr, nullptr, nullptr, st);
-    launch_eo_hop<T>(v, dst, te.p(), 1, 2, mu, dag, dag, src, EPI_NONE, nullptr, nullptr, nullptr, st);
+    launch_eo_hop<T>(v, te.p(), src, 0, 1, mu, dag, 0, nullptr, EPI_NONE, nullptr, nullptr, Fin(), st);
+    launch_eo_hop<T>(v, dst, te.p(), 1, 2, mu, dag, dag, src, EPI_NONE, nullptr, nullptr, Fin(), st);
 }
 template <typename T>
-int SchurOp<T>::apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux, double* part,
-                      hipStream_t st) {
+void SchurOp<T>::apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux,
+                       const Fin& fin, hipStream_t st) {
     const DiracView<T> v = D.view<T>();
     apply_m(to.p(), src, 0, st);
     const T mu = (T)-D.mu_t;
-    launch_eo_hop<T>(v, te.p(), to.p(), 0, 1, mu, 1, 0, nullptr, EPI_NONE, nullptr, nullptr, nullptr, st);
-    return launch_eo_hop<T>(v, dst, te.p(), 1, 2, mu, 1, 1, to.p(), epi, b, aux, part, st);
+    launch_eo_hop<T>(v, te.p(), to.p(), 0, 1, mu, 1, 0, nullptr, EPI_NONE, nullptr, nullptr, Fin(), st);
+    launch_eo_hop<T>(v, dst, te.p(), 1, 2, mu, 1, 1, to.p(), epi, b, aux, fin, st);
 }
 
 template struct DdagDOp<double>;
@@ -213,7 +256,7 @@ void CoarseOperator::build(LinOp<double>& A, const Aggregation& P, hipStream_t s
         for (int kj = 0; kj < k; kj++) {
             launch_probe_vector(nx, ny, k, cx, cy, cls, kj, e.p(), st);
             launch_prolong<double>(P.A, P.bv64.p(), e.p(), f.p(), 0, st);
-            A.apply(Af.p(), f.p(), EPI_NONE, nullptr, nullptr, nullptr, st);
+            A.apply(Af.p(), f.p(), st);
             launch_restrict<double>(P.A, P.bv64.p(), Af.p(), col.p(), st);
             launch_probe_extract(nx, ny, k, cx, cy, cls, kj, col.p(), l64.p(), st);
         }
@@ -230,14 +273,14 @@ void CoarseOperator::refresh_low_precision(hipStream_t st) {
 }
 
 template <typename T, typename TL>
-int CoarseLinOp<T, TL>::apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux,
-                              double* part, hipStream_t st) {
+void CoarseLinOp<T, TL>::apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux,
+                               const Fin& fin, hipStream_t st) {
     const TL* link;
     if constexpr (std::is_same<TL, c64>::value) link = C.l64.p();
     else if constexpr (std::is_same<TL, c32>::value) link = C.l32.p();
     else link = C.l16.template as<ch16>();
     CoarseView<TL> v{link, C.nx, C.ny, C.k};
-    return launch_coarse_apply<T, TL>(v, dst, src, epi, b, aux, part, st);
+    launch_coarse_apply<T, TL>(v, dst, src, epi, b, aux, fin, st);
 }
 template struct CoarseLinOp<double, c64>;
 template struct CoarseLinOp<float, c32>;
@@ -248,17 +291,16 @@ template <typename T>
 void mr_smooth(LinOp<T>& A, cplx<T>* x, const cplx<T>* b, cplx<T>* r, cplx<T>* Ar, int niter, bool zero_init,
                double* sc, hipStream_t st) {
     const size_t n = A.n();
-    double* part = Context::get().part_ptr();
+    const Fin fin = Context::get().fin(FIN_MR, sc);
     if (zero_init) {
         launch_zero<T>(x, n, st);
         dcopy(r, b, n, st);
     } else {
-        A.apply(r, x, EPI_RESID, b, nullptr, nullptr, st);
+        A.apply(r, x, EPI_RESID, b, nullptr, Fin(), st);
     }
     for (int it = 0; it < niter; it++) {
-        // Ar = A r with {<Ar,r>, |Ar|^2}; omega = <Ar,r>/|Ar|^2; x += omega r; r -= omega Ar
-        const int np = A.apply(Ar, r, EPI_DOTS, nullptr, r, part, st);
-        launch_finalize(part, np, 3, sc, FIN_MR, nullptr, st);
+        // Ar = A r with {<Ar,r>, |Ar|^2} -> omega = <Ar,r>/|Ar|^2; x += omega r; r -= omega Ar
+        A.apply(Ar, r, EPI_DOTS, nullptr, r, fin, st);
         launch_mr_update<T>(x, r, Ar, sc, n, st);
     }
 }
@@ -270,11 +312,11 @@ void richardson_smooth(LinOp<T>& A, cplx<T>* x, const cplx<T>* b, cplx<T>* r, in
     if (zero_init) launch_zero<T>(x, n, st);
     for (int it = 0; it < niter; it++) {
         if (it == 0 && zero_init) dcopy(r, b, n, st);
-        else A.apply(r, x, EPI_RESID, b, nullptr, nullptr, st);
+        else A.apply(r, x, EPI_RESID, b, nullptr, Fin(), st);
         launch_axpby<T>(x, r, c64{omega, 0.0}, c64{1.0, 0.0}, n, st);
     }
     if (niter == 0 && zero_init) dcopy(r, b, n, st);
-    else A.apply(r, x, EPI_RESID, b, nullptr, nullptr, st);
+    else A.apply(r, x, EPI_RESID, b, nullptr, Fin(), st);
 }
 
 template <typename T>
@@ -284,7 +326,7 @@ double lambda_max(LinOp<T>& A, cplx<T>* v, cplx<T>* Av, int niter, hipStream_t s
     launch_axpby<T>(v, v, c64{0.0, 0.0}, c64{1.0 / vn, 0.0}, n, st);
     double lambda = 0.0;
     for (int i = 0; i < niter; i++) {
-        A.apply(Av, v, EPI_NONE, nullptr, nullptr, nullptr, st);
+        A.apply(Av, v, st);
         lambda = dev_norm_t(Av, n, st);
         if (lambda < 1e-30) break;
         launch_axpby<T>(v, Av, c64{1.0 / lambda, 0.0}, c64{0.0, 0.0}, n, st);
@@ -297,7 +339,6 @@ int coarse_cg(LinOp<T>& A, const cplx<T>* b, cplx<T>* x, cplx<T>* r, cplx<T>* p,
               double* sc, bool check, hipStream_t st) {
     const size_t n = A.n();
     Context& cx = Context::get();
-    double* part = cx.part_ptr();
     double* alpha = sc;
     double* beta = sc + 2;
     double* aux = sc + 16;
@@ -306,14 +347,12 @@ int coarse_cg(LinOp<T>& A, const cplx<T>* b, cplx<T>* x, cplx<T>* r, cplx<T>* p,
     dcopy(r, b, n, st);
     dcopy(p, b, n, st);
     launch_set_scalars(aux + 5, 1, tol * tol, 0, 0, 0, flag, st);
-    int g = launch_norm2_part<T>(r, n, part, st);
-    launch_finalize(part, g, 1, nullptr, FIN_CG_INIT, aux, st);
+    launch_norm2<T>(r, n, cx.fin(FIN_CG_INIT, nullptr, aux), st);
+    const Fin fa = cx.fin(FIN_CG_ALPHA, alpha, aux), fb = cx.fin(FIN_CG_BETA, beta, aux);
     const int chunk = 8;
     for (int it = 0; it < maxit; it++) {
-        int np = A.apply(Ap, p, EPI_DOTS, nullptr, p, part, st);
-        launch_finalize(part, np, 3, alpha, FIN_CG_ALPHA, aux, st);
-        g = launch_cg_update<T>(x, r, p, Ap, alpha, flag, n, part, st);
-        launch_finalize(part, g, 1, beta, FIN_CG_BETA, aux, st);
+        A.apply(Ap, p, EPI_DOTS, nullptr, p, fa, st);
+        launch_cg_update<T>(x, r, p, Ap, alpha, flag, n, fb, st);
         launch_xpay_dev<T>(p, r, beta, flag, n, st);
         if (check && (it + 1) % chunk == 0 && it + 1 < maxit) {
             double h[9];
@@ -378,8 +417,15 @@ struct Cycle {
         LinOp<T>& A = *ops[lev];
         const int last = nlev - 1;
         if (lev == last) {
-            coarse_cg<T>(A, b, x, rs[lev].p(), ps[lev].p(), Ars[lev].p(), mg.cfg.coarse_maxit, mg.cfg.coarse_tol,
-                         sc[lev]->p(), false, st);
+            if (mg.cfg.coarse_solver == 1) {
+                const cplx<T>* inv;
+                if constexpr (std::is_same<T, double>::value) inv = mg.inv64.p();
+                else inv = mg.inv32.p();
+                launch_dense_matvec<T>(inv, b, x, mg.dims[lev], st);
+            } else {
+                coarse_cg<T>(A, b, x, rs[lev].p(), ps[lev].p(), Ars[lev].p(), mg.cfg.coarse_maxit,
+                             mg.cfg.coarse_tol, sc[lev]->p(), false, st);
+            }
             return;
         }
         const double om = mg.omega[lev];
@@ -390,7 +436,7 @@ struct Cycle {
         launch_restrict<T>(P.A, P.bv<T>(), rs[lev].p(), bs[lev + 1].p(), st);
         run(lev + 1, bs[lev + 1].p(), xs[lev + 1].p(), st);
         if (mg.cfg.w_cycle && lev + 1 < last) {
-            ops[lev + 1]->apply(ts[lev + 1].p(), xs[lev + 1].p(), EPI_RESID, bs[lev + 1].p(), nullptr, nullptr, st);
+            ops[lev + 1]->apply(ts[lev + 1].p(), xs[lev + 1].p(), EPI_RESID, bs[lev + 1].p(), nullptr, Fin(), st);
             run(lev + 1, ts[lev + 1].p(), e2s[lev + 1].p(), st);
             launch_axpby<T>(xs[lev + 1].p(), e2s[lev + 1].p(), c64{1.0, 0.0}, c64{1.0, 0.0}, mg.dims[lev + 1], st);
         }
@@ -479,11 +525,38 @@ MG::MG(const Dirac& d, const MGConfig& c, const c64* starts, bool on_device, boo
         lny.push_back(P.back()->A.nby);
         lnd.push_back(k);
     }
+    if (cfg.coarse_solver == 1) build_dense_inverse(st);
     cyc64.reset(new Cycle<double>(*this, fine64.get()));
     cyc32.reset(new Cycle<float>(*this, fine32.get()));
     SMG_HIP(hipStreamSynchronize(st));
 }
-MG::~MG() = default;
+MG::~MG() { reset_graphs(); }
+
+void MG::reset_graphs() {
+    for (auto& g : graph) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.g) (void)hipGraphDestroy(g.g);
+        g.exec = nullptr;
+        g.g = nullptr;
+    }
+}
+
+// The coarsest operator is Hermitian positive definite (a Galerkin projection of
+// D^dag D), so its inverse is formed once by Gauss-Jordan without pivoting, and a
+// coarsest solve becomes one dense matrix-vector product (the exact solve of
+// src/coarse_op.h:101-137 without refactorising on every call).
+void MG::build_dense_inverse(hipStream_t st) {
+    const size_t n = dims.back();
+    SMG_REQUIRE(n <= 8192, "dense coarsest inverse: the coarsest level must have <= 8192 unknowns");
+    const CoarseOperator& Cc = *C.back();
+    inv64 = Field<double>(n * n);
+    inv32 = Field<float>(n * n);
+    launch_zero<double>(inv64.p(), n * n, st);
+    launch_coarse_to_dense(CoarseView<c64>{Cc.l64.p(), Cc.nx, Cc.ny, Cc.k}, inv64.p(), n, st);
+    Field<double> prow(n), pcol(n);
+    launch_dense_invert(inv64.p(), n, prow.p(), pcol.p(), st);
+    launch_convert<float, double>(inv32.p(), inv64.p(), n * n, st);
+}
 
 // src/multigrid.cpp:146-152 with explicit power-iteration start vectors
 void MG::set_symmetric(double damping, const c64* starts, bool on_device) {
@@ -498,16 +571,51 @@ void MG::set_symmetric(double damping, const c64* starts, bool on_device) {
         const double lmax = lambda_max<double>(*cyc64->ops[l], v.p(), Av.p(), 30, st);
         omega[l] = damping / lmax;
     }
+    reset_graphs();
 }
 
-void MG::precondition(const c64* b, c64* x, hipStream_t st) { cyc64->run(0, b, x, st); }
-void MG::precondition(const c32* b, c32* x, hipStream_t st) { cyc32->run(0, b, x, st); }
+// Replays a hipGraph of the whole cycle (captured on first use with fixed
+// input/output buffers); the cycle makes no host synchronisation and no
+// allocation, so it captures as is.
+template <typename T>
+static void run_graphed(MG& mg, Cycle<T>& cyc, MG::Graph& G, Field<T>& gin, Field<T>& gout, const cplx<T>* b,
+                        cplx<T>* x, hipStream_t st) {
+    const size_t n = mg.dims[0];
+    if (!G.exec) {
+        gin = Field<T>(n);
+        gout = Field<T>(n);
+        SMG_HIP(hipStreamSynchronize(st));
+        SMG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            cyc.run(0, gin.p(), gout.p(), st);
+        } catch (...) {
+            hipGraph_t tmp;
+            (void)hipStreamEndCapture(st, &tmp);
+            if (tmp) (void)hipGraphDestroy(tmp);
+            throw;
+        }
+        SMG_HIP(hipStreamEndCapture(st, &G.g));
+        SMG_HIP(hipGraphInstantiate(&G.exec, G.g, nullptr, nullptr, 0));
+    }
+    launch_copy<T>(gin.p(), b, n, st);
+    SMG_HIP(hipGraphLaunch(G.exec, st));
+    launch_copy<T>(x, gout.p(), n, st);
+}
+
+void MG::precondition(const c64* b, c64* x, hipStream_t st) {
+    if (cfg.use_graph) run_graphed<double>(*this, *cyc64, graph[0], gin64, gout64, b, x, st);
+    else cyc64->run(0, b, x, st);
+}
+void MG::precondition(const c32* b, c32* x, hipStream_t st) {
+    if (cfg.use_graph) run_graphed<float>(*this, *cyc32, graph[1], gin32, gout32, b, x, st);
+    else cyc32->run(0, b, x, st);
+}
 
 // ------------------------------------------------------------------ Krylov
 double true_residual(LinOp<double>& A, const c64* b, const c64* x, hipStream_t st) {
     const size_t n = A.n();
     Field<double> r(n);
-    A.apply(r.p(), x, EPI_RESID, b, nullptr, nullptr, st);
+    A.apply(r.p(), x, EPI_RESID, b, nullptr, Fin(), st);
     return dev_norm(r.p(), n, st) / dev_norm(b, n, st);
 }
 
@@ -523,7 +631,6 @@ SolveResult cg_solve(LinOp<double>& A, const c64* b, c64* x, const c64* x0, MG* 
     SolveResult res;
     const size_t n = A.n();
     Context& cx = Context::get();
-    double* part = cx.part_ptr();
     const double bnorm = dev_norm(b, n, st);
     if (bnorm < 1e-30) {
         launch_zero<double>(x, n, st);
@@ -539,7 +646,7 @@ SolveResult cg_solve(LinOp<double>& A, const c64* b, c64* x, const c64* x0, MG* 
     launch_set_scalars(aux + 5, 1, 0.0, 0, 0, 0, flag, st);
     if (x0) {
         dcopy(x, x0, n, st);
-        A.apply(r.p(), x, EPI_RESID, b, nullptr, nullptr, st);
+        A.apply(r.p(), x, EPI_RESID, b, nullptr, Fin(), st);
     } else {
         launch_zero<double>(x, n, st);
         dcopy(r.p(), b, n, st);
@@ -547,23 +654,19 @@ SolveResult cg_solve(LinOp<double>& A, const c64* b, c64* x, const c64* x0, MG* 
     c64* zp = mg ? z.p() : r.p();
     if (mg) mg->precondition(r.p(), z.p(), st);
     dcopy(p.p(), zp, n, st);
-    int g = launch_dot_part<double>(r.p(), zp, n, part, st);
-    launch_finalize(part, g, 2, aux, FIN_STORE, nullptr, st);
+    launch_dot<double>(r.p(), zp, n, cx.fin(FIN_STORE, aux), st);
     int iter = 0;
     double rn = bnorm;
     while (iter < maxit) {
-        int np = A.apply(Ap.p(), p.p(), EPI_DOTS, nullptr, p.p(), part, st);
-        launch_finalize(part, np, 3, alpha, FIN_CG_ALPHA, aux, st);
-        g = launch_cg_update<double>(x, r.p(), p.p(), Ap.p(), alpha, flag, n, part, st);
-        launch_finalize(part, g, 1, rr, FIN_STORE, nullptr, st);
+        A.apply(Ap.p(), p.p(), EPI_DOTS, nullptr, p.p(), cx.fin(FIN_CG_ALPHA, alpha, aux), st);
+        launch_cg_update<double>(x, r.p(), p.p(), Ap.p(), alpha, flag, n, cx.fin(FIN_STORE, rr), st);
         iter++;
         double h;
         cx.read(rr, &h, 1);
         rn = std::sqrt(h);
         if (rn / bnorm < tol) break;
         if (mg) mg->precondition(r.p(), z.p(), st);
-        g = launch_dot_part<double>(r.p(), zp, n, part, st);
-        launch_finalize(part, g, 2, beta, FIN_PCG_BETA, aux, st);
+        launch_dot<double>(r.p(), zp, n, cx.fin(FIN_PCG_BETA, beta, aux), st);
         launch_xpay_dev<double>(p.p(), zp, beta, flag, n, st);
     }
     res.iterations = iter;
@@ -579,7 +682,6 @@ SolveResult gcr_solve(LinOp<T>& A, const cplx<T>* b, cplx<T>* x, MG* mg, int max
     const size_t n = A.n();
     SMG_REQUIRE(trunc >= 1 && trunc <= 32, "GCR truncation window must be 1..32");
     Context& cx = Context::get();
-    double* part = cx.part_ptr();
     launch_zero<T>(x, n, st);
     const double bnorm = dev_norm_t(b, n, st);
     if (bnorm < 1e-30) return res;
@@ -607,7 +709,7 @@ SolveResult gcr_solve(LinOp<T>& A, const cplx<T>* b, cplx<T>* x, MG* mg, int max
         cplx<T>* p = Pd[slot].p();
         cplx<T>* Ap = APd[slot].p();
         precond_apply<T>(mg, r.p(), p, n, st);
-        A.apply(Ap, p, EPI_NONE, nullptr, nullptr, nullptr, st);
+        A.apply(Ap, p, st);
         const int nd = (int)order.size();
         if (nd > 0) {
             VecPtrs<T> Pv, APv;
@@ -618,14 +720,12 @@ SolveResult gcr_solve(LinOp<T>& A, const cplx<T>* b, cplx<T>* x, MG* mg, int max
                 ap[i] = apap + order[i];
             }
             // beta_i from the unmodified A z (classical Gram-Schmidt, src/solvers.cpp:260-266)
-            const int g = launch_multi_dot<T>(APv, nd, Ap, n, ref_projection ? 1 : 0, part, st);
-            launch_finalize_many(part, g, 2 * nd, dots, st);
+            launch_multi_dot<T>(APv, nd, Ap, n, ref_projection ? 1 : 0, cx.fin(FIN_STORE, dots), st);
             launch_gcr_orth<T>(p, Ap, Pv, APv, nd, dots, ap, n, st);
         }
-        int g = launch_dots_part<T>(Ap, r.p(), n, part, st);
-        launch_finalize(part, g, 3, alpha, FIN_GCR_ALPHA, apap + slot, st);
-        g = launch_cg_update<T>(x, r.p(), p, Ap, alpha, reinterpret_cast<const int*>(sc.p(120)), n, part, st);
-        launch_finalize(part, g, 1, rr, FIN_STORE, nullptr, st);
+        launch_dots<T>(Ap, r.p(), n, cx.fin(FIN_GCR_ALPHA, alpha, apap + slot), st);
+        launch_cg_update<T>(x, r.p(), p, Ap, alpha, reinterpret_cast<const int*>(sc.p(120)), n,
+                            cx.fin(FIN_STORE, rr), st);
         double h[2];
         cx.read(rr, &h[0], 1);
         cx.read(apap + slot, &h[1], 1);
@@ -658,7 +758,6 @@ SolveResult gcr_solve_mixed(LinOp<double>& A64, LinOp<float>& A32, const c64* b,
     SolveResult res;
     const size_t n = A64.n();
     Context& cx = Context::get();
-    double* part = cx.part_ptr();
     launch_zero<double>(x, n, st);
     const double bnorm = dev_norm(b, n, st);
     if (bnorm < 1e-30) return res;
@@ -677,9 +776,8 @@ SolveResult gcr_solve_mixed(LinOp<double>& A64, LinOp<float>& A32, const c64* b,
         total += std::max(in.iterations, 1);
         launch_accum_mixed(x, e32.p(), n, st);
         // FP64 true residual
-        A64.apply(r.p(), x, EPI_RESID, b, nullptr, nullptr, st);
-        int g = launch_norm2_part<double>(r.p(), n, part, st);
-        launch_finalize(part, g, 1, sc.p(), FIN_STORE, nullptr, st);
+        A64.apply(r.p(), x, EPI_RESID, b, nullptr, Fin(), st);
+        launch_norm2<double>(r.p(), n, cx.fin(FIN_STORE, sc.p()), st);
         double h;
         cx.read(sc.p(), &h, 1);
         rn = std::sqrt(h);
@@ -706,7 +804,7 @@ SolveResult fgmres_solve(LinOp<double>& A, const c64* b, c64* x, MG* mg, int res
     int total = 0;
     auto cdivh = [](c64 a, c64 b) { return cdiv(a, b); };
     while (total < maxit) {
-        A.apply(Vv[0].p(), x, EPI_RESID, b, nullptr, nullptr, st);
+        A.apply(Vv[0].p(), x, EPI_RESID, b, nullptr, Fin(), st);
         const double beta = dev_norm(Vv[0].p(), n, st);
         if (beta / bnorm < tol) break;
         const int m = std::min(restart, maxit - total);
@@ -718,7 +816,7 @@ SolveResult fgmres_solve(LinOp<double>& A, const c64* b, c64* x, MG* mg, int res
         int k;
         for (k = 0; k < m; k++) {
             precond_apply<double>(mg, Vv[k].p(), Zv[k].p(), n, st);
-            A.apply(w.p(), Zv[k].p(), EPI_NONE, nullptr, nullptr, nullptr, st);
+            A.apply(w.p(), Zv[k].p(), st);
             for (int pass = 0; pass < 2; pass++)
                 for (int j = 0; j <= k; j++) {
                     c64 c = dev_dot<double>(Vv[j].p(), w.p(), n, st);

@@ -43,15 +43,22 @@ struct Field {
 struct Context {
     hipStream_t stream = nullptr;
     DevMem part;      // reduction partial sums
-    DevMem scal;      // device scalars (pool)
+    DevMem counter;   // workgroup-completion counter of the fused reductions (zero between kernels)
     DevMem hostscal;  // pinned host mirror (hipHostMalloc)
-    size_t scal_used = 0;
     static Context& get();
     double* part_ptr() const { return part.as<double>(); }
-    double* new_scalars(size_t count);  // 64-aligned slots from the pool
+    // a reduction request finishing with scalar program op into out (aux: program state)
+    Fin fin(int op, double* out, double* aux = nullptr) const {
+        Fin f;
+        f.op = op;
+        f.out = out;
+        f.aux = aux;
+        f.part = part.as<double>();
+        f.counter = counter.as<unsigned>();
+        return f;
+    }
     // read `count` doubles from device into host (synchronises the stream)
     void read(const double* dev, double* host, size_t count);
-    double finalize_read(int nparts, int nc, double* out_dev, size_t slot_count = 1);
     void sync();
     ~Context();
 };
@@ -61,9 +68,10 @@ template <typename T>
 struct LinOp {
     virtual ~LinOp() = default;
     virtual size_t n() const = 0;
-    // dst = A src | b - A src | A src with partials {<dst,aux>, |dst|^2}; returns #partials
-    virtual int apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux,
-                      double* part, hipStream_t st) = 0;
+    // dst = A src | b - A src | A src with the reduction {<dst,aux>, |dst|^2} finished by fin
+    virtual void apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux,
+                       const Fin& fin, hipStream_t st) = 0;
+    void apply(cplx<T>* dst, const cplx<T>* src, hipStream_t st) { apply(dst, src, EPI_NONE, nullptr, nullptr, Fin(), st); }
 };
 
 // Wilson-clover(-twisted) Dirac operator on an L x L torus (src/dirac.h:8-183).
@@ -90,8 +98,9 @@ struct DdagDOp : LinOp<T> {
     Field<T> tmp;
     explicit DdagDOp(const Dirac& d) : D(d), tmp(2 * (size_t)d.V) {}
     size_t n() const override { return 2 * (size_t)D.V; }
-    int apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux, double* part,
-              hipStream_t st) override;
+    void apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux, const Fin& fin,
+               hipStream_t st) override;
+    using LinOp<T>::apply;
 };
 
 // M^dag M on odd sites (src/dirac.h:399-445), four half-lattice launches.
@@ -101,8 +110,9 @@ struct SchurOp : LinOp<T> {
     Field<T> te, to;
     explicit SchurOp(const Dirac& d) : D(d), te((size_t)d.V), to((size_t)d.V) {}
     size_t n() const override { return (size_t)D.V; }
-    int apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux, double* part,
-              hipStream_t st) override;
+    void apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux, const Fin& fin,
+               hipStream_t st) override;
+    using LinOp<T>::apply;
     // M (dag = 0) or M^dag (dag = 1) only
     void apply_m(cplx<T>* dst, const cplx<T>* src, int dag, hipStream_t st);
 };
@@ -142,8 +152,9 @@ struct CoarseLinOp : LinOp<T> {
     const CoarseOperator& C;
     explicit CoarseLinOp(const CoarseOperator& c) : C(c) {}
     size_t n() const override { return C.n(); }
-    int apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux, double* part,
-              hipStream_t st) override;
+    void apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux, const Fin& fin,
+               hipStream_t st) override;
+    using LinOp<T>::apply;
 };
 
 // ------------------------------------------------------------------ smoothers
@@ -180,6 +191,8 @@ struct MGConfig {
     double coarse_tol = 1e-12;
     int smoother = 0;        // 0 MR, 1 Richardson (set by set_symmetric)
     int coarse_half = 0;     // FP32 cycle: FP16 coarse links
+    int coarse_solver = 0;   // coarsest level: 0 CG (coarse_maxit, coarse_tol), 1 dense inverse
+    int use_graph = 0;       // replay the cycle from a captured hipGraph
 };
 
 template <typename T>
@@ -198,6 +211,18 @@ struct MG {
     std::unique_ptr<Cycle<float>> cyc32;
     std::vector<size_t> dims;   // per level
     std::vector<int> lnx, lny, lnd;
+    // dense inverse of the coarsest operator (cfg.coarse_solver == 1)
+    Field<double> inv64;
+    Field<float> inv32;
+    // captured cycles (cfg.use_graph): fixed input/output buffers per precision
+    struct Graph {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t exec = nullptr;
+    } graph[2];
+    Field<double> gin64, gout64;
+    Field<float> gin32, gout32;
+    void reset_graphs();
+    void build_dense_inverse(hipStream_t st);
     // starts: every level's k start vectors (host or device FP64), level 0 first
     MG(const Dirac& d, const MGConfig& c, const c64* starts, bool starts_on_device, bool schur = false);
     ~MG();

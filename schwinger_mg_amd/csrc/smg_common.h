@@ -118,6 +118,150 @@ __device__ __forceinline__ void block_sum(double (&v)[NC]) {
     __syncthreads();
 }
 
+// ------------------------------------------------------------------ fused finalize
+// Scalar programs run on the summed reduction (see FinOp in smg_kernels.h).
+enum FinOp : int {
+    FIN_NONE = -1,      // no reduction requested
+    FIN_STORE = 0,      // out[c] = sum[c]
+    FIN_MR = 1,         // sum = {Re<Ar,r>, Im<Ar,r>, |Ar|^2}; out = <Ar,r>/|Ar|^2 (0 if |Ar|^2 < 1e-30)
+    FIN_CG_INIT = 2,    // sum = {|r|^2}: aux[0..1] = rr, aux[2] = aux[5]*rr, aux[3] = 0, flag = rr < 1e-60
+    FIN_CG_ALPHA = 3,   // sum = {Re<Ap,p>, Im<Ap,p>, .}: out = rz/pAp, pAp = conj(sum), rz = aux[0..1]
+    FIN_CG_BETA = 4,    // sum = {|r|^2}: iteration count aux[3], stop test vs aux[2], out = rr/aux[0]
+    FIN_PCG_BETA = 5,   // sum = {Re<r,z>, Im<r,z>}: out = sum/aux[0..1]; aux[0..1] = sum
+    FIN_GCR_ALPHA = 6,  // sum = {Re<Ap,r>, Im<Ap,r>, |Ap|^2}: aux[0] = |Ap|^2; out = <Ap,r>/|Ap|^2 or 0
+};
+// aux layout of the CG programs: aux[0..1] rz, aux[2] stop threshold, aux[3] iterations,
+// aux[4] last |r|^2, aux[5] tol^2, int flag at (int*)(aux + 8)
+
+// A reduction request handed to a kernel: the kernel writes one partial row per
+// workgroup to part, and the workgroup that finishes last sums the rows in index
+// order (deterministic) and runs the scalar program `op` -- no separate finalize
+// launch.  counter is a device word that is zero between kernels (the last
+// workgroup resets it).
+struct Fin {
+    int op = FIN_NONE;
+    double* out = nullptr;
+    double* aux = nullptr;
+    double* part = nullptr;
+    unsigned* counter = nullptr;
+};
+
+__device__ __forceinline__ void fin_program(int op, const double* v, int nc, double* out, double* aux) {
+    int* flag = aux ? reinterpret_cast<int*>(aux + 8) : nullptr;
+    switch (op) {
+    case FIN_STORE:
+        for (int c = 0; c < nc; c++) out[c] = v[c];
+        break;
+    case FIN_MR: {
+        // src/smoother.cpp:34-37: omega = <Ar,r>/<Ar,Ar>; |<Ar,Ar>| < 1e-30 ends the
+        // sweeps, which a zero step reproduces (x and r stay unchanged from then on)
+        const double d = v[2];
+        if (fabs(d) < 1e-30) {
+            out[0] = 0.0;
+            out[1] = 0.0;
+        } else {
+            out[0] = v[0] / d;
+            out[1] = v[1] / d;
+        }
+        break;
+    }
+    case FIN_CG_INIT: {
+        const double rr = v[0];
+        aux[0] = rr;
+        aux[1] = 0.0;
+        aux[2] = aux[5] * rr;
+        aux[3] = 0.0;
+        aux[4] = rr;
+        *flag = (rr < 1e-60) ? 1 : 0;   // |b| < 1e-30: x = 0
+        break;
+    }
+    case FIN_CG_ALPHA: {
+        // src/coarse_op.h:520-523 (pAp = <p, Ap> = conj(<Ap, p>))
+        if (*flag) break;
+        const cplx<double> pAp = {v[0], -v[1]};
+        if (sqrt(pAp.re * pAp.re + pAp.im * pAp.im) < 1e-30) {
+            *flag = 2;
+            out[0] = 0.0;
+            out[1] = 0.0;
+            break;
+        }
+        const cplx<double> a = cdiv(cplx<double>{aux[0], aux[1]}, pAp);
+        out[0] = a.re;
+        out[1] = a.im;
+        break;
+    }
+    case FIN_CG_BETA: {
+        // src/coarse_op.h:529-540: stop on |r|/|b| < tol (or NaN/Inf), beta = rr_new/rr
+        if (*flag) break;
+        const double rr = v[0];
+        aux[3] += 1.0;
+        aux[4] = rr;
+        if (rr < aux[2] || isnan(rr) || isinf(rr)) {
+            *flag = 1;
+            break;
+        }
+        out[0] = rr / aux[0];
+        out[1] = 0.0;
+        aux[0] = rr;
+        aux[1] = 0.0;
+        break;
+    }
+    case FIN_PCG_BETA: {
+        const cplx<double> b = cdiv(cplx<double>{v[0], v[1]}, cplx<double>{aux[0], aux[1]});
+        out[0] = b.re;
+        out[1] = b.im;
+        aux[0] = v[0];
+        aux[1] = v[1];
+        break;
+    }
+    case FIN_GCR_ALPHA: {
+        const double apap = v[2];
+        aux[0] = apap;
+        if (apap < 1e-30) {
+            out[0] = 0.0;
+            out[1] = 0.0;
+        } else {
+            out[0] = v[0] / apap;
+            out[1] = v[1] / apap;
+        }
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+// End of a reducing kernel: every thread holds its partial v[NC]; all threads of
+// every workgroup must call this.  nblk = total workgroups of the launch.
+template <int NC>
+__device__ __forceinline__ void reduce_finish(double (&v)[NC], const Fin& f, int nblk, int blk) {
+    block_sum<NC>(v);
+    __shared__ int last;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) f.part[(size_t)blk * NC + c] = v[c];
+        __threadfence();
+        const unsigned prev = atomicAdd(f.counter, 1u);
+        last = (prev == (unsigned)nblk - 1u);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double w[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) w[c] = 0.0;
+    // fixed-order sum: thread t adds rows t, t + kBlock, ... ; then the block tree
+    for (int i = threadIdx.x; i < nblk; i += kBlock)
+#pragma unroll
+        for (int c = 0; c < NC; c++) w[c] += __builtin_nontemporal_load(f.part + (size_t)i * NC + c);
+    block_sum<NC>(w);
+    if (threadIdx.x == 0) {
+        fin_program(f.op, w, NC, f.out, f.aux);
+        *f.counter = 0u;
+        __threadfence();
+    }
+}
+
 // Number of partial-sum blocks used for a reduction over n items: a function of
 // n only, so the summation order is deterministic.
 inline int red_blocks(size_t n) {

@@ -4,17 +4,11 @@
 
 namespace smg {
 
-// Reduction request attached to a kernel: partial sums of NC doubles per
-// workgroup are written to part[(rhs*nblk + blk)*NC + c].
-struct RedOut {
-    double* part = nullptr;
-};
-
 // Epilogue of an operator kernel.
 enum Epi : int {
     EPI_NONE = 0,   // dst = A src
     EPI_RESID = 1,  // dst = b - A src
-    EPI_DOTS = 2,   // dst = A src; partials of {Re<dst,aux>, Im<dst,aux>, |dst|^2}
+    EPI_DOTS = 2,   // dst = A src; reduction {Re<dst,aux>, Im<dst,aux>, |dst|^2} -> Fin
 };
 
 // ---------------------------------------------------------------- Dirac (dirac.hip)
@@ -31,20 +25,19 @@ struct DiracView {
 // D^dag = (g5in = g5out = 1, mu -> -mu_t); g5 D = (g5out = 1).  nrhs vectors at
 // stride vstride complex.  Returns the number of workgroups per rhs (partials).
 template <typename T>
-int launch_dslash(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T mu, int g5in, int g5out,
-                  Epi epi, const cplx<T>* b, const cplx<T>* aux, double* part, int nrhs, size_t vstride,
-                  hipStream_t st);
+void launch_dslash(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T mu, int g5in, int g5out,
+                   Epi epi, const cplx<T>* b, const cplx<T>* aux, const Fin& fin, int nrhs, size_t vstride,
+                   hipStream_t st);
 
 // Half-lattice hopping with fused diagonal.  dst parity p (0 even, 1 odd).
 //   mode 0: dst = H src
 //   mode 1: dst = Dinv_p(mu) H src
 //   mode 2: dst = g5^g5out [ D_p(mu) g5^g5in aux - H src ]
-//   mode 3: dst = H (g5 src) then g5 (the g5-sandwiched hop; used by the e/o force)
-// g5in also applies to src reads in modes 0/1.
+// g5in applies to src reads in modes 0/1 and to aux in mode 2.
 template <typename T>
-int launch_eo_hop(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, int p, int mode, T mu, int g5in,
-                  int g5out, const cplx<T>* aux, Epi epi, const cplx<T>* b, const cplx<T>* aux2, double* part,
-                  hipStream_t st);
+void launch_eo_hop(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, int p, int mode, T mu, int g5in,
+                   int g5out, const cplx<T>* aux, Epi epi, const cplx<T>* b, const cplx<T>* aux2, const Fin& fin,
+                   hipStream_t st);
 // dst = D_p(mu)^{-1} src (inv = 1) or D_p(mu) src (inv = 0) on parity p
 template <typename T>
 void launch_eo_diag(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, int p, int inv, T mu, hipStream_t st);
@@ -57,10 +50,10 @@ void launch_eo_scatter(int L, cplx<T>* full, const cplx<T>* half, int p, hipStre
 void launch_clover(int L, const c64* U, double* F01, hipStream_t st);
 void launch_delta_dirac(const DiracView<double>& D, c64* dst, const c64* src, const double* pi, double dt,
                         hipStream_t st);
-// logdet partials: sum over even sites of 0.5 log(d0^2+mu^2) + 0.5 log(d1^2+mu^2)
-int launch_logdet(const DiracView<double>& D, double mu, double* part, hipStream_t st);
-// plaquette sum partials (Re P summed over sites)
-int launch_plaq_sum(int L, const c64* U, double* part, hipStream_t st);
+// sum over even sites of 0.5 log(d0^2+mu^2) + 0.5 log(d1^2+mu^2) (FIN_STORE into fin.out)
+void launch_logdet(const DiracView<double>& D, double mu, const Fin& fin, hipStream_t st);
+// sum over sites of Re P (FIN_STORE into fin.out)
+void launch_plaq_sum(int L, const c64* U, const Fin& fin, hipStream_t st);
 
 // forces, FP64, F[mu*V + s]
 // F (=|+=) scale * hopping force Re(i c_bwd - i c_fwd)
@@ -81,41 +74,25 @@ void launch_logdet_weight(const DiracView<double>& D, double csw, double mu, dou
 void launch_gauge_force(int L, const c64* U, double beta, double* F, hipStream_t st);
 
 // ---------------------------------------------------------------- BLAS-1 (blas.hip)
-// partial reductions; return number of partial blocks
+// reductions finished in-kernel by the scalar program of fin
 template <typename T>
-int launch_dot_part(const cplx<T>* a, const cplx<T>* b, size_t n, double* part, hipStream_t st);  // NC = 2
+void launch_dot(const cplx<T>* a, const cplx<T>* b, size_t n, const Fin& fin, hipStream_t st);  // {<a,b>}
 template <typename T>
-int launch_norm2_part(const cplx<T>* a, size_t n, double* part, hipStream_t st);  // NC = 1
+void launch_norm2(const cplx<T>* a, size_t n, const Fin& fin, hipStream_t st);  // {|a|^2}
 template <typename T>
-int launch_dots_part(const cplx<T>* a, const cplx<T>* b, size_t n, double* part, hipStream_t st);  // NC = 3: <a,b>, |a|^2
+void launch_dots(const cplx<T>* a, const cplx<T>* b, size_t n, const Fin& fin, hipStream_t st);  // {<a,b>, |a|^2}
 
-// finalize: sum nparts partial rows of NC doubles, then apply a scalar program op
-enum FinOp : int {
-    FIN_STORE = 0,      // out[c] = sum[c]
-    FIN_MR = 1,         // sum = {Re<Ar,r>, Im<Ar,r>, |Ar|^2}; out = <Ar,r>/|Ar|^2 (0 if |Ar|^2 < 1e-30)
-    FIN_CG_INIT = 2,    // sum = {|r|^2}: aux[0..1] = rr, aux[2] = aux[5]*rr, aux[3] = 0, flag = rr < 1e-60
-    FIN_CG_ALPHA = 3,   // sum = {Re<Ap,p>, Im<Ap,p>}: out = rz/pAp, pAp = conj(sum), rz = aux[0..1]
-    FIN_CG_BETA = 4,    // sum = {|r|^2}: iteration count aux[3], stop test vs aux[2], out = rr/aux[0]
-    FIN_PCG_BETA = 5,   // sum = {Re<r,z>, Im<r,z>}: out = sum/aux[0..1]; aux[0..1] = sum
-    FIN_GCR_ALPHA = 6,  // sum = {Re<Ap,r>, Im<Ap,r>, |Ap|^2}: aux[0] = |Ap|^2; out = <Ap,r>/|Ap|^2 or 0
-};
-// aux layout of the CG programs: aux[0..1] rz, aux[2] stop threshold, aux[3] iterations,
-// aux[4] last |r|^2, aux[5] tol^2, int flag at (int*)(aux + 8)
-void launch_finalize(const double* part, int nparts, int nc, double* out, int op, double* aux, hipStream_t st);
-
-// sum over parts for nc <= 64 components, out[c] = sum_i part[i*nc + c]
-void launch_finalize_many(const double* part, int nparts, int nc, double* out, hipStream_t st);
 // write up to 4 constants to dst (and zero the int flag at iflag if non-null)
 void launch_set_scalars(double* dst, int count, double v0, double v1, double v2, double v3, int* iflag,
                         hipStream_t st);
-// batched dots against up to 32 vectors: part[blk*2*nv + 2i..] = <V_i, w> (swap: <w, V_i>)
+// batched dots against up to 32 vectors: fin.out[2i..2i+1] = <V_i, w> (swap: <w, V_i>)
 template <typename T>
 struct VecPtrs {
     const cplx<T>* v[32];
 };
 template <typename T>
-int launch_multi_dot(const VecPtrs<T>& V, int nv, const cplx<T>* w, size_t n, int swap, double* part,
-                     hipStream_t st);
+void launch_multi_dot(const VecPtrs<T>& V, int nv, const cplx<T>* w, size_t n, int swap, const Fin& fin,
+                      hipStream_t st);
 // GCR orthogonalisation: beta_i = dots[i]/apap[i]; p -= sum beta_i P_i; Ap -= sum beta_i AP_i
 template <typename T>
 void launch_gcr_orth(cplx<T>* p, cplx<T>* Ap, const VecPtrs<T>& P, const VecPtrs<T>& AP, int nv,
@@ -130,10 +107,10 @@ void launch_axpy_dev(cplx<T>* y, const cplx<T>* x, const double* s, int negate, 
 // MR update: x += w r; r -= w Ar  (w at device s[0..1])
 template <typename T>
 void launch_mr_update(cplx<T>* x, cplx<T>* r, const cplx<T>* Ar, const double* s, size_t n, hipStream_t st);
-// CG update: x += a p; r -= a Ap; partial |r|^2 (a at s[0..1]; skipped when flag != 0)
+// CG update: x += a p; r -= a Ap; reduction {|r|^2} (a at s[0..1]; skipped when flag != 0)
 template <typename T>
-int launch_cg_update(cplx<T>* x, cplx<T>* r, const cplx<T>* p, const cplx<T>* Ap, const double* s,
-                     const int* flag, size_t n, double* part, hipStream_t st);
+void launch_cg_update(cplx<T>* x, cplx<T>* r, const cplx<T>* p, const cplx<T>* Ap, const double* s,
+                      const int* flag, size_t n, const Fin& fin, hipStream_t st);
 // p = z + beta p (beta at s[0..1], skipped when flag != 0)
 template <typename T>
 void launch_xpay_dev(cplx<T>* p, const cplx<T>* z, const double* s, const int* flag, size_t n, hipStream_t st);
@@ -143,8 +120,9 @@ void launch_convert(cplx<To>* dst, const cplx<From>* src, size_t n, hipStream_t 
 void launch_convert_half(ch16* dst, const c64* src, size_t n, hipStream_t st);
 template <typename T>
 void launch_zero(cplx<T>* p, size_t n, hipStream_t st);
-// r = b - Ax in FP64 plus partial |r|^2 (true residual)
-int launch_resid_norm(c64* r, const c64* b, const c64* Ax, size_t n, double* part, hipStream_t st);
+// dst = src (same precision), as a kernel (graph-capturable)
+template <typename T>
+void launch_copy(cplx<T>* dst, const cplx<T>* src, size_t n, hipStream_t st);
 // x64 += x32 (iterative refinement accumulate)
 void launch_accum_mixed(c64* x, const c32* e, size_t n, hipStream_t st);
 // g5 on a spinor field (negate odd components), in place
@@ -173,8 +151,14 @@ struct CoarseView {
     int nx, ny, k;
 };
 template <typename T, typename TL>
-int launch_coarse_apply(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b,
-                        const cplx<T>* aux, double* part, hipStream_t st);
+void launch_coarse_apply(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b,
+                         const cplx<T>* aux, const Fin& fin, hipStream_t st);
+// dense coarsest solve: M += stencil (row-major n x n); in-place Gauss-Jordan
+// inverse (HPD, no pivoting); y = M x
+void launch_coarse_to_dense(const CoarseView<c64>& C, c64* M, size_t n, hipStream_t st);
+void launch_dense_invert(c64* M, size_t n, c64* prow, c64* pcol, hipStream_t st);
+template <typename T>
+void launch_dense_matvec(const cplx<T>* M, const cplx<T>* x, cplx<T>* y, size_t n, hipStream_t st);
 // Galerkin probing helpers
 void launch_probe_vector(int nbx, int nby, int k, int cx, int cy, int cls, int kj, c64* e, hipStream_t st);
 void launch_probe_extract(int nbx, int nby, int k, int cx, int cy, int cls, int kj, const c64* col, c64* link,

@@ -183,3 +183,43 @@ def test_schur_cg(smg):
     r = smg.cg_solve(D, b, 1000, 1e-10, op=smg.OP_SCHUR)
     xo, ito, _ = O.cg(Do.op(O.SCHUR_NORMAL), b, 1000, 1e-10)
     assert abs(r.iterations - ito) <= 2 and relerr(r.solution, xo) < 1e-8
+
+
+def test_dense_coarsest_matches_reference_lu(smg):
+    """The dense inverse coarsest solve reproduces the reference's dense-LU cycle."""
+    banner("mg dense coarsest vs golden L=16")
+    g = np.load(GOLD)
+    L, k = 16, 4
+    D = smg.DiracOp(L, g["L16_U"], 0.05, 1.0, 1.0, 0.0)
+    mg = smg.MGHierarchy(D, g["L16_starts"], n_levels=2, blocks=((4, 4),), k=(k,), null_iters=20, coarse_solver=1)
+    assert relerr(mg.precondition(g["L16_rhs"]), g["L16_mg2_mode0_x"]) < 1e-10
+    L = 32
+    D3 = smg.DiracOp(L, g["L32_U"], 0.05, 1.0, 1.0, 0.0)
+    mg3 = smg.MGHierarchy(D3, g["L32_starts"], n_levels=3, blocks=((4, 4), (4, 1)), k=(k, k), null_iters=10,
+                          coarse_solver=1)
+    assert relerr(mg3.precondition(g["L32_rhs"]), g["L32_mg3_x"]) < 1e-10
+
+
+@pytest.mark.parametrize("coarse_solver", [0, 1])
+def test_graph_replay_equals_eager(smg, coarse_solver):
+    """A captured cycle replays to the eager result, on repeated replays with other
+    library work (and host synchronisation) between them."""
+    banner(f"graph replay coarse_solver={coarse_solver}")
+    L, k = 32, (4, 4)
+    V = L * L
+    U = random_gauge(L, 0.4, 55)
+    dims = [2 * V, (L // 4) ** 2 * 4]
+    starts = np.concatenate([gaussian_vecs(4, n, 56 + i).ravel() for i, n in enumerate(dims)])
+    D = smg.DiracOp(L, U, 0.05, 1.0, 1.0, 0.0)
+    mg = smg.MGHierarchy(D, starts, n_levels=3, blocks=((4, 4), (2, 2)), k=k, null_iters=5, coarse_maxit=20,
+                         coarse_tol=1e-3, coarse_solver=coarse_solver)
+    bs = [gaussian_vec(2 * V, 60 + i) for i in range(3)]
+    for prec, tol in ((smg.F64, 0.0), (smg.F32, 0.0)):
+        mg.set_cycle(3, 3, 20, 1e-3, use_graph=False)
+        eager = [mg.precondition(b, prec) for b in bs]
+        mg.set_cycle(3, 3, 20, 1e-3, use_graph=True)
+        for rep in range(3):
+            for b, want in zip(bs, eager):
+                got = mg.precondition(b, prec)
+                D.apply(b)            # unrelated eager work between replays
+                assert relerr(got, want) <= tol + 1e-14, (prec, rep)
