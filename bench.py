@@ -1,13 +1,15 @@
 """Benchmark: MG-GCR solves/s on a 512 x 512 Wilson-clover Schwinger lattice.
 
-One step = one right-hand side solved to a true residual |phi - A x|/|phi| <= 1e-10
-(A = D^dag D, the reference's system, src/solvers.cpp:55-59) by mixed-precision
-GCR + multigrid: FP32 GCR and FP32 W-cycle with FP16 coarse links inside FP64
-iterative refinement on the FP64 true residual.  The same hierarchy solved in
-plain FP64 (FP64 GCR + FP64 cycle) is timed beside it, the Wilson-clover stencil
-is timed alone (HIP events on the library's stream), and the oracle's FP64 CPU
+One step = a batch of N (default 32) Gaussian right-hand sides, each solved to a
+true residual |phi - A x|/|phi| <= 1e-10 (A = D^dag D, the reference's system,
+src/solvers.cpp:55-59) by mixed-precision GCR + multigrid run on all N at once:
+FP32 GCR and FP32 W-cycle with FP16 coarse links inside FP64 iterative
+refinement on the FP64 true residual.  value = N / (seconds per step).  The same
+hierarchy solved in plain FP64 (FP64 GCR + FP64 cycle, also batched) is timed
+beside it, as is the one-right-hand-side solve; the Wilson-clover stencil is
+timed alone (HIP events on the library's stream), and the oracle's FP64 CPU
 MG-GCR solve is timed on the host cores.  Inputs are synthetic (seeded U(1)
-links theta ~ N(0, 0.4^2), Gaussian source) and resident in HBM when the timed
+links theta ~ N(0, 0.4^2), Gaussian sources) and resident in HBM when the timed
 region starts.  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1]
@@ -49,7 +51,8 @@ def parse():
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--trunc", type=int, default=10)
     ap.add_argument("--inner-tol", type=float, default=1e-5)
-    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--nrhs", type=int, default=32)
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
@@ -62,19 +65,24 @@ def stencil_bytes_per_site(prec, nrhs, csw):
 
 
 def time_stencil(smg, torch, D, prec, nrhs, V, csw, launches=50, warm=5):
-    """Average duration of one Wilson-clover launch, HIP events on the library stream."""
+    """Average duration of one Wilson-clover launch on the batched layout (RHS
+    innermost, the layout of the solve), HIP events on the library stream."""
     dt = torch.complex128 if prec == "f64" else torch.complex64
-    src = torch.randn(nrhs, 2 * V, dtype=dt, device="cuda")
+    src = torch.randn(2 * V, nrhs, dtype=dt, device="cuda")
     dst = torch.empty_like(src)
     p = smg.F64 if prec == "f64" else smg.F32
     st = torch.cuda.ExternalStream(smg.library_stream())
+
+    def launch():
+        smg._chk(smg.lib().smg_dirac_apply_batched_dev(D.h, smg.D_, p, src.data_ptr(), dst.data_ptr(), nrhs, None))
+
     for _ in range(warm):
-        D.apply_dev(smg.D_, p, src.data_ptr(), dst.data_ptr(), nrhs, None)
+        launch()
     smg.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(st)
     for _ in range(launches):
-        D.apply_dev(smg.D_, p, src.data_ptr(), dst.data_ptr(), nrhs, None)
+        launch()
     e1.record(st)
     e1.synchronize()
     ms = e0.elapsed_time(e1) / launches
@@ -119,7 +127,6 @@ def main():
     L, V = args.L, args.L * args.L
     seed = 2024
     U = random_gauge(L, args.sigma, seed)
-    rhs = gaussian_vec(2 * V, seed + 1)
     dims = [2 * V]
     nx = L
     for lev in range(args.levels - 1):
@@ -139,57 +146,72 @@ def main():
     setup_s = time.time() - t0
     log(f"MG setup {setup_s:.2f} s")
 
-    b = torch.from_numpy(rhs).cuda()
-    x = torch.zeros_like(b)
+    N = args.nrhs
+    B = gaussian_vecs(N, 2 * V, seed + 1)
+    Bd = torch.from_numpy(B).cuda()
+    Xd = torch.zeros_like(Bd)
     info = smg.SolveInfo()
+    tres = np.zeros(N)
     import ctypes
 
-    def solve(prec):
-        smg._chk(smg.lib().smg_gcr_solve_dev(D.h, mg.h, prec, b.data_ptr(), x.data_ptr(), 500, 1e-10, args.trunc,
+    def solve_batch(prec):
+        smg._chk(smg.lib().smg_gcr_solve_batched_dev(D.h, mg.h, prec, N, Bd.data_ptr(), Xd.data_ptr(), 500, 1e-10,
+                                                     min(args.trunc, 16), args.inner_tol,
+                                                     tres.ctypes.data_as(ctypes.c_void_p), ctypes.byref(info), None))
+
+    def timed(fn, steps, warm):
+        for _ in range(warm):
+            fn()
+        torch.cuda.synchronize(); smg.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize(); smg.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    t_mixed = timed(lambda: solve_batch(smg.F32), args.steps, args.warmup)
+    mixed = {"iterations": info.iterations, "restarts": info.restarts, "max_true_residual": float(tres.max())}
+    log(f"mixed x{N}: {t_mixed * 1e3:.1f} ms/batch = {N / t_mixed:.1f} solves/s, {mixed}")
+    X_mixed = Xd.cpu().numpy()
+    t_f64 = timed(lambda: solve_batch(smg.F64), args.fp64_steps, 1)
+    fp64 = {"iterations": info.iterations, "max_true_residual": float(tres.max())}
+    log(f"fp64 x{N}: {t_f64 * 1e3:.1f} ms/batch = {N / t_f64:.1f} solves/s, {fp64}")
+    X_f64 = Xd.cpu().numpy()
+    err_vs_f64 = float(np.max(np.linalg.norm(X_mixed - X_f64, axis=1) / np.linalg.norm(X_f64, axis=1)))
+
+    # one right-hand side at a time (the reference's mode of use)
+    b1 = Bd[0].contiguous()
+    x1 = torch.zeros_like(b1)
+
+    def solve_one(prec):
+        smg._chk(smg.lib().smg_gcr_solve_dev(D.h, mg.h, prec, b1.data_ptr(), x1.data_ptr(), 500, 1e-10, args.trunc,
                                              args.inner_tol, ctypes.byref(info), None))
 
-    for _ in range(args.warmup):
-        solve(smg.F32)
-    torch.cuda.synchronize(); smg.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        solve(smg.F32)
-    torch.cuda.synchronize(); smg.synchronize()
-    t_mixed = (time.perf_counter() - t0) / args.steps
-    mixed = {"iterations": info.iterations, "restarts": info.restarts, "true_residual": info.true_residual}
-    log(f"mixed: {t_mixed * 1e3:.1f} ms/solve, {mixed}")
-    x_mixed = x.cpu().numpy()
-
-    solve(smg.F64)
-    torch.cuda.synchronize(); smg.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.fp64_steps):
-        solve(smg.F64)
-    torch.cuda.synchronize(); smg.synchronize()
-    t_f64 = (time.perf_counter() - t0) / args.fp64_steps
-    fp64 = {"iterations": info.iterations, "true_residual": info.true_residual}
-    log(f"fp64: {t_f64 * 1e3:.1f} ms/solve, {fp64}")
-    x_f64 = x.cpu().numpy()
-    err_vs_f64 = float(np.linalg.norm(x_mixed - x_f64) / np.linalg.norm(x_f64))
+    t_one_mixed = timed(lambda: solve_one(smg.F32), 2, 1)
+    one = {"mixed_solves_per_s": round(1.0 / t_one_mixed, 3), "mixed_iterations": info.iterations,
+           "true_residual": info.true_residual}
+    t_one_f64 = timed(lambda: solve_one(smg.F64), 1, 1)
+    one["fp64_solves_per_s"] = round(1.0 / t_one_f64, 3)
+    log(f"single rhs: {one}")
 
     stencil = [time_stencil(smg, torch, D, p, n, V, True) for p, n in
-               [("f32", 1), ("f64", 1), ("f32", 32), ("f64", 32)]]
-    dom = stencil[3]     # FP64, 32 RHS: a 540 MB working set (beyond the 256 MB Infinity Cache)
+               [("f32", 1), ("f64", 1), ("f32", N), ("f64", N)]]
+    dom = stencil[2]     # FP32, N RHS: the fine-level stencil of the mixed solve
     log("stencil: " + ", ".join(f"{s['prec']}x{s['nrhs']} {s['GBs']:.0f} GB/s" for s in stencil))
     roofline = {"bound": "hbm", "achieved": round(dom["GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_dslash<double> (Wilson-clover D, 32 RHS, L=512)"}
+                "kernel": f"k_b_dslash<float> (Wilson-clover D, {N} RHS, L={L})"}
 
     cpu = None
     if args.cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads ...")
-        cpu = cpu_baseline(args, U, rhs, starts, threads)
+        cpu = cpu_baseline(args, U, B[0], starts, threads)
         log(f"cpu baseline: {cpu}")
 
     out = {
         "metric": "MG-GCR solves/sec at L=512, 1e-10 resid: mixed vs FP64 GPU; Dslash GB/s",
-        "value": round(1.0 / t_mixed, 3),
+        "value": round(N / t_mixed, 3),
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -200,16 +222,17 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 (FP16 coarse links) in f64 refinement",
         "data": "synthetic",
-        "config": {"workload": f"L={L} Wilson-clover m={args.mass} c_sw=1, theta~N(0,{args.sigma}^2), 1 RHS, "
+        "config": {"workload": f"L={L} Wilson-clover m={args.mass} c_sw=1, theta~N(0,{args.sigma}^2), {N} RHS, "
                                f"{args.levels}-level MG (4x4 then 2x2 aggregates, k={args.k}), GCR({args.trunc}), "
                                "tol 1e-10 true residual",
-                   "L": L, "levels": args.levels, "k": args.k, "pre": args.pre, "post": args.post,
+                   "L": L, "nrhs": N, "levels": args.levels, "k": args.k, "pre": args.pre, "post": args.post,
                    "coarse_maxit": args.coarse_maxit, "coarse_tol": args.coarse_tol,
                    "coarse_solver": "dense inverse" if args.coarse_solver == 1 else "CG", "graph": bool(args.graph)},
         "mixed": mixed,
-        "fp64_solves_per_s": round(1.0 / t_f64, 3),
+        "fp64_solves_per_s": round(N / t_f64, 3),
         "fp64": fp64,
         "speedup_mixed_vs_fp64": round(t_f64 / t_mixed, 3),
+        "single_rhs": one,
         "mixed_vs_fp64_solution_relerr": err_vs_f64,
         "setup_s": round(setup_s, 3),
         "stencil": stencil,

@@ -212,6 +212,25 @@ int smg_gcr_solve_dev(smg_dirac* D, smg_mg* mg, int prec, const void* b, void* x
 int smg_fgmres_solve(smg_dirac* D, smg_mg* mg, const double* b, double* x, int restart, int maxit, double tol,
                      smg_solve_info* info);
 
+/* ------------------------------------------------------------------ many right-hand sides
+ * N (a power of two <= 32) right-hand sides per launch.  Host and device arrays
+ * of the calls below hold the N vectors one after the other (each a reference
+ * Vec); internally they run in the batched layout v[i*N + j] (RHS innermost) so
+ * that every kernel of the cycle and solver carries all N.  No reference
+ * counterpart: the reference solves one right-hand side at a time. */
+/* the stencil on the batched layout (device pointers): src/dst[(2*s + spin)*N + j] */
+int smg_dirac_apply_batched_dev(smg_dirac* D, int kind, int prec, const void* src, void* dst, int nrhs,
+                                void* stream);
+/* one multigrid cycle applied to N right-hand sides at once (host arrays) */
+int smg_mg_precondition_batched(smg_mg* mg, int prec, int nrhs, const double* B, double* X);
+/* smg_gcr_solve for N right-hand sides in lockstep (textbook GCR projection);
+ * true_res (optional, N doubles) receives each RHS's FP64 true relative residual */
+int smg_gcr_solve_batched(smg_dirac* D, smg_mg* mg, int prec, int nrhs, const double* B, double* X, int maxit,
+                          double tol, int trunc, double inner_tol, double* true_res, smg_solve_info* info);
+int smg_gcr_solve_batched_dev(smg_dirac* D, smg_mg* mg, int prec, int nrhs, const void* B, void* X, int maxit,
+                              double tol, int trunc, double inner_tol, double* true_res, smg_solve_info* info,
+                              void* stream);
+
 /* ------------------------------------------------------------------ forces (F = -dS/dtheta) */
 /* forward half of fermion_force, src/hmc_forces.cpp:134-139: chi = D X */
 int smg_force_forward(smg_dirac* D, const double* X, double* chi);

@@ -75,6 +75,10 @@ SIGNATURES = [
     ("smg_gcr_solve", _I, [_P, _P, _I, _P, _P, _I, _D, _I, _I, _D, _P]),
     ("smg_gcr_solve_dev", _I, [_P, _P, _I, _P, _P, _I, _D, _I, _D, _P, _P]),
     ("smg_fgmres_solve", _I, [_P, _P, _P, _P, _I, _I, _D, _P]),
+    ("smg_dirac_apply_batched_dev", _I, [_P, _I, _I, _P, _P, _I, _P]),
+    ("smg_mg_precondition_batched", _I, [_P, _I, _I, _P, _P]),
+    ("smg_gcr_solve_batched", _I, [_P, _P, _I, _I, _P, _P, _I, _D, _I, _D, _P, _P]),
+    ("smg_gcr_solve_batched_dev", _I, [_P, _P, _I, _I, _P, _P, _I, _D, _I, _D, _P, _P, _P]),
     ("smg_force_forward", _I, [_P, _P, _P]), ("smg_force_backward", _I, [_P, _P, _P, _P]),
     ("smg_fermion_force", _I, [_P, _P, _P]), ("smg_fermion_force_dev", _I, [_P, _P, _P, _P]),
     ("smg_hopping_force", _I, [_P, _P, _P, _P]), ("smg_logdet_force", _I, [_P, _P]),
@@ -478,6 +482,14 @@ class MGHierarchy:
     def set_cycle(self, pre_smooth, post_smooth, coarse_maxit, coarse_tol, use_graph=False):
         _chk(lib().smg_mg_set_cycle(self.h, pre_smooth, post_smooth, coarse_maxit, coarse_tol, int(use_graph)))
 
+    def precondition_batched(self, B, prec=F64):
+        """One cycle on N right-hand sides at once; B is (N, ndof)."""
+        B = _c(B)
+        N = B.shape[0]
+        X = np.zeros_like(B)
+        _chk(lib().smg_mg_precondition_batched(self.h, prec, N, _p(B), _p(X)))
+        return X
+
     def omega(self, level):
         out = _D()
         _chk(lib().smg_mg_omega(self.h, level, ctypes.byref(out)))
@@ -526,6 +538,19 @@ def fcg_solve(D, rhs, mg, max_iter, tol, trunc=30, prec=F64, ref_projection=Fals
 
 
 gcr_solve = fcg_solve
+
+
+def gcr_solve_batched(D, B, mg, max_iter, tol, trunc=10, prec=F64, inner_tol=1e-5):
+    """GCR + MG on N right-hand sides in lockstep; B is (N, ndof).  Returns
+    (X, info, per-RHS FP64 true residuals)."""
+    B = _c(B)
+    N = B.shape[0]
+    X = np.zeros_like(B)
+    info = SolveInfo()
+    tr = np.zeros(N)
+    _chk(lib().smg_gcr_solve_batched(D.h, mg.h if mg else None, prec, N, _p(B), _p(X), max_iter, tol, trunc,
+                                     inner_tol, _p(tr), ctypes.byref(info)))
+    return X, info, tr
 
 
 def fgmres_solve(D, rhs, mg, restart, max_iter, tol):

@@ -1,10 +1,12 @@
 // capi.cpp -- the extern "C" boundary (include/smg.h) over the C++ runtime.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
 
 #include "../../include/smg.h"
 #include "runtime.h"
+#include "runtime_b.h"
 
 using namespace smg;
 
@@ -532,7 +534,12 @@ int smg_mg_setup(smg_mg** out, smg_dirac* D, const smg_mg_params* prm, const dou
         *out = h;
     });
 }
-int smg_mg_destroy(smg_mg* mg) { return guard([&] { delete mg; }); }
+int smg_mg_destroy(smg_mg* mg) {
+    return guard([&] {
+        if (mg) release_batched_cycles(mg->mg.get());
+        delete mg;
+    });
+}
 int smg_mg_set_symmetric(smg_mg* mg, double damping, const double* starts) {
     return guard([&] {
         SMG_REQUIRE(mg && starts, "null argument");
@@ -682,6 +689,96 @@ int smg_fgmres_solve(smg_dirac* D, smg_mg* mg, const double* b, double* x, int r
         SolveResult r = fgmres_solve(*D->op64, bb.p(), xx.p(), mg ? mg->mg.get() : nullptr, restart, maxit, tol, S());
         fill_info(info, r);
         downc(x, xx.p(), n);
+    });
+}
+
+// ------------------------------------------------------------------ many right-hand sides
+static void check_nrhs(int N, int maxN) {
+    SMG_REQUIRE(N >= 1 && N <= maxN && (N & (N - 1)) == 0, "nrhs must be a power of two <= " + std::to_string(maxN));
+}
+template <typename T>
+static void dirac_kind_b(const Dirac& d, int kind, const cplx<T>* src, cplx<T>* dst, int N, hipStream_t st) {
+    const DiracView<T> v = d.view<T>();
+    const T mu = (T)d.mu_t;
+    switch (kind) {
+    case SMG_D: launch_b_dslash<T>(v, dst, src, mu, 0, 0, EPI_NONE, nullptr, nullptr, FinB(), N, st); break;
+    case SMG_DDAG: launch_b_dslash<T>(v, dst, src, -mu, 1, 1, EPI_NONE, nullptr, nullptr, FinB(), N, st); break;
+    case SMG_G5D: launch_b_dslash<T>(v, dst, src, mu, 0, 1, EPI_NONE, nullptr, nullptr, FinB(), N, st); break;
+    case SMG_DDAGD: {
+        DdagDOpB<T> op(d, N);   // its workspace returns to the stream-ordered cache
+        op.apply(dst, src, st);
+        break;
+    }
+    default: SMG_REQUIRE(false, "unknown Dirac kind");
+    }
+}
+int smg_dirac_apply_batched_dev(smg_dirac* D, int kind, int prec, const void* src, void* dst, int nrhs,
+                                void* stream) {
+    return guard([&] {
+        SMG_REQUIRE(D && src && dst, "bad arguments");
+        check_nrhs(nrhs, 64);
+        check_prec(prec);
+        StreamJoin j(stream);
+        if (prec == SMG_F64) dirac_kind_b<double>(*D->D, kind, (const c64*)src, (c64*)dst, nrhs, S());
+        else dirac_kind_b<float>(*D->D, kind, (const c32*)src, (c32*)dst, nrhs, S());
+    });
+}
+int smg_mg_precondition_batched(smg_mg* mg, int prec, int nrhs, const double* B, double* X) {
+    return guard([&] {
+        SMG_REQUIRE(mg, "null handle");
+        check_nrhs(nrhs, 64);
+        check_prec(prec);
+        const size_t n = mg->mg->dims[0], nN = n * nrhs;
+        Field<double> b = up(B, nN), x(nN), bb(nN), xb(nN);
+        launch_b_pack<double, double>(bb.p(), b.p(), n, nrhs, S());
+        if (prec == SMG_F64) {
+            mg_precondition_b<double>(*mg->mg, bb.p(), xb.p(), nrhs, S());
+        } else {
+            Field<float> b32(nN), x32(nN);
+            launch_convert<float, double>(b32.p(), bb.p(), nN, S());
+            mg_precondition_b<float>(*mg->mg, b32.p(), x32.p(), nrhs, S());
+            launch_convert<double, float>(xb.p(), x32.p(), nN, S());
+        }
+        launch_b_unpack<double, double>(x.p(), xb.p(), n, nrhs, S());
+        downc(X, x.p(), nN);
+    });
+}
+static void fill_batched(const BatchedSolveResult& r, double* true_res, smg_solve_info* info) {
+    double worst = 0.0;
+    for (double v : r.true_residual) worst = std::max(worst, v);
+    if (true_res) std::copy(r.true_residual.begin(), r.true_residual.end(), true_res);
+    if (info) {
+        info->iterations = r.iterations;
+        info->final_residual = worst;
+        info->true_residual = worst;
+        info->restarts = r.restarts;
+    }
+}
+int smg_gcr_solve_batched(smg_dirac* D, smg_mg* mg, int prec, int nrhs, const double* B, double* X, int maxit,
+                          double tol, int trunc, double inner_tol, double* true_res, smg_solve_info* info) {
+    return guard([&] {
+        SMG_REQUIRE(D, "null handle");
+        check_nrhs(nrhs, 32);
+        check_prec(prec);
+        const size_t nN = 2 * (size_t)D->D->V * nrhs;
+        Field<double> b = up(B, nN), x(nN);
+        BatchedSolveResult r = gcr_solve_batched(*D->D, mg ? mg->mg.get() : nullptr, prec, b.p(), x.p(), nrhs, maxit,
+                                                 tol, trunc, inner_tol, S());
+        fill_batched(r, true_res, info);
+        downc(X, x.p(), nN);
+    });
+}
+int smg_gcr_solve_batched_dev(smg_dirac* D, smg_mg* mg, int prec, int nrhs, const void* B, void* X, int maxit,
+                              double tol, int trunc, double inner_tol, double* true_res, smg_solve_info* info,
+                              void* stream) {
+    return guard([&] {
+        SMG_REQUIRE(D && B && X, "bad arguments");
+        check_nrhs(nrhs, 32);
+        check_prec(prec);
+        StreamJoin j(stream);
+        BatchedSolveResult r = gcr_solve_batched(*D->D, mg ? mg->mg.get() : nullptr, prec, (const c64*)B, (c64*)X,
+                                                 nrhs, maxit, tol, trunc, inner_tol, S());
+        fill_batched(r, true_res, info);
     });
 }
 

@@ -1,0 +1,354 @@
+// runtime_b.cpp -- sequencing of the multi-right-hand-side solve path.
+#include "runtime_b.h"
+
+#include <algorithm>
+#include <cmath>
+#include <deque>
+
+namespace smg {
+
+namespace {
+FinB finb(int op, double* out, int ostride, double* aux = nullptr, int astride = 16) {
+    const Fin f = Context::get().fin(op, out, aux);
+    FinB b;
+    b.op = op;
+    b.out = out;
+    b.ostride = ostride;
+    b.aux = aux;
+    b.astride = astride;
+    b.part = f.part;
+    b.counter = f.counter;
+    return b;
+}
+struct ScalarsB {
+    DevMem m;
+    explicit ScalarsB(size_t n) : m(n * sizeof(double)) {
+        SMG_HIP(hipMemsetAsync(m.p, 0, n * sizeof(double), Context::get().stream));
+    }
+    double* p(size_t i = 0) const { return m.as<double>() + i; }
+};
+template <typename T>
+void bcopy(cplx<T>* d, const cplx<T>* s, size_t n, hipStream_t st) {
+    launch_copy<T>(d, s, n, st);
+}
+}  // namespace
+
+template <typename T>
+void DdagDOpB<T>::apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux,
+                        const FinB& fin, hipStream_t st) {
+    const DiracView<T> v = D.view<T>();
+    const T mu = (T)D.mu_t;
+    launch_b_dslash<T>(v, tmp.p(), src, mu, 0, 0, EPI_NONE, nullptr, nullptr, FinB(), this->N, st);
+    launch_b_dslash<T>(v, dst, tmp.p(), -mu, 1, 1, epi, b, aux, fin, this->N, st);
+}
+template struct DdagDOpB<double>;
+template struct DdagDOpB<float>;
+
+template <typename T, typename TL>
+void CoarseLinOpB<T, TL>::apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux,
+                                const FinB& fin, hipStream_t st) {
+    const TL* link;
+    if constexpr (std::is_same<TL, c64>::value) link = C.l64.p();
+    else if constexpr (std::is_same<TL, c32>::value) link = C.l32.p();
+    else link = C.l16.template as<ch16>();
+    launch_b_coarse_apply<T, TL>(CoarseView<TL>{link, C.nx, C.ny, C.k}, dst, src, epi, b, aux, fin, this->N, st);
+}
+template struct CoarseLinOpB<double, c64>;
+template struct CoarseLinOpB<float, c32>;
+template struct CoarseLinOpB<float, ch16>;
+
+// ------------------------------------------------------------------ smoothers (batched)
+// MR, src/smoother.cpp:26-40, with the residual carried recursively; per-RHS omega
+template <typename T>
+static void mr_b(LinOpB<T>& A, cplx<T>* x, const cplx<T>* b, cplx<T>* r, cplx<T>* Ar, int niter, bool zero_init,
+                 double* w, hipStream_t st) {
+    const size_t n = A.n(), nN = n * A.N;
+    if (zero_init) {
+        launch_zero<T>(x, nN, st);
+        bcopy(r, b, nN, st);
+    } else {
+        A.apply(r, x, EPI_RESID, b, nullptr, FinB(), st);
+    }
+    const FinB f = finb(FIN_MR, w, 2);
+    for (int it = 0; it < niter; it++) {
+        A.apply(Ar, r, EPI_DOTS, nullptr, r, f, st);
+        launch_b_mr_update<T>(x, r, Ar, w, n, A.N, st);
+    }
+}
+
+// Richardson, src/smoother.cpp:63-73; on return r = b - A x
+template <typename T>
+static void rich_b(LinOpB<T>& A, cplx<T>* x, const cplx<T>* b, cplx<T>* r, int niter, double omega, bool zero_init,
+                   hipStream_t st) {
+    const size_t nN = A.n() * A.N;
+    if (zero_init) launch_zero<T>(x, nN, st);
+    for (int it = 0; it < niter; it++) {
+        if (it == 0 && zero_init) bcopy(r, b, nN, st);
+        else A.apply(r, x, EPI_RESID, b, nullptr, FinB(), st);
+        launch_b_axpby_real<T>(x, r, omega, 1.0, nN, st);
+    }
+    if (niter == 0 && zero_init) bcopy(r, b, nN, st);
+    else A.apply(r, x, EPI_RESID, b, nullptr, FinB(), st);
+}
+
+// CG per RHS (src/coarse_op.h:492-547 without deflation); converged RHS take zero steps
+template <typename T>
+static void cg_b(LinOpB<T>& A, const cplx<T>* b, cplx<T>* x, cplx<T>* r, cplx<T>* p, cplx<T>* Ap, int maxit,
+                 double tol, double* sc, hipStream_t st) {
+    const size_t n = A.n(), nN = n * A.N;
+    const int N = A.N;
+    double* alpha = sc;             // [N][2]
+    double* beta = sc + 2 * 64;     // [N][2]
+    double* aux = sc + 4 * 64;      // [N][16]
+    launch_zero<T>(x, nN, st);
+    bcopy(r, b, nN, st);
+    bcopy(p, b, nN, st);
+    launch_b_cg_init(aux, 16, N, tol * tol, st);
+    launch_b_norm2<T>(r, n, finb(FIN_CG_INIT, nullptr, 0, aux), N, st);
+    const FinB fa = finb(FIN_CG_ALPHA, alpha, 2, aux), fbt = finb(FIN_CG_BETA, beta, 2, aux);
+    for (int it = 0; it < maxit; it++) {
+        A.apply(Ap, p, EPI_DOTS, nullptr, p, fa, st);
+        launch_b_update<T>(x, r, p, Ap, alpha, n, fbt, N, st);
+        launch_b_xpay<T>(p, r, beta, n, N, st);
+    }
+}
+
+// ------------------------------------------------------------------ cycle (batched)
+template <typename T>
+struct CycleB {
+    MG& mg;
+    int N;
+    std::vector<LinOpB<T>*> ops;
+    std::vector<std::unique_ptr<LinOpB<T>>> own;
+    std::vector<Field<T>> xs, bs, rs, Ars, ts, e2s, ps;
+    std::vector<std::unique_ptr<ScalarsB>> sc;
+    int nlev;
+    CycleB(MG& m, int N_) : mg(m), N(N_), nlev(m.cfg.n_levels) {
+        SMG_REQUIRE(N >= 1 && N <= 64 && (N & (N - 1)) == 0, "batched solves take a power-of-two count <= 64");
+        own.emplace_back(new DdagDOpB<T>(*m.D, N));
+        ops.push_back(own.back().get());
+        for (int l = 1; l < nlev; l++) {
+            if constexpr (std::is_same<T, double>::value)
+                own.emplace_back(new CoarseLinOpB<double, c64>(*m.C[l - 1], N));
+            else if (m.cfg.coarse_half)
+                own.emplace_back(new CoarseLinOpB<float, ch16>(*m.C[l - 1], N));
+            else
+                own.emplace_back(new CoarseLinOpB<float, c32>(*m.C[l - 1], N));
+            ops.push_back(own.back().get());
+        }
+        for (int l = 0; l < nlev; l++) {
+            const size_t nN = m.dims[l] * N;
+            const bool last = (l == nlev - 1);
+            xs.emplace_back(nN);
+            bs.emplace_back(nN);
+            rs.emplace_back(nN);
+            Ars.emplace_back(nN);
+            ts.emplace_back(l == 0 ? 1 : nN);
+            e2s.emplace_back(l == 0 ? 1 : nN);
+            ps.emplace_back(last ? nN : 1);
+            sc.emplace_back(new ScalarsB(8 * 64 + 16 * 64));
+        }
+    }
+    // src/multigrid.cpp:43-102 on N right-hand sides
+    void run(int lev, const cplx<T>* b, cplx<T>* x, hipStream_t st) {
+        LinOpB<T>& A = *ops[lev];
+        const int last = nlev - 1;
+        if (lev == last) {
+            if (mg.cfg.coarse_solver == 1) {
+                const cplx<T>* inv;
+                if constexpr (std::is_same<T, double>::value) inv = mg.inv64.p();
+                else inv = mg.inv32.p();
+                launch_b_dense_apply<T>(inv, b, x, mg.dims[lev], N, st);
+            } else {
+                cg_b<T>(A, b, x, rs[lev].p(), ps[lev].p(), Ars[lev].p(), mg.cfg.coarse_maxit, mg.cfg.coarse_tol,
+                        sc[lev]->p(), st);
+            }
+            return;
+        }
+        const double om = mg.omega[lev];
+        if (om > 0) rich_b<T>(A, x, b, rs[lev].p(), mg.cfg.pre, om, true, st);
+        else mr_b<T>(A, x, b, rs[lev].p(), Ars[lev].p(), mg.cfg.pre, true, sc[lev]->p(), st);
+        const Aggregation& P = *mg.P[lev];
+        launch_b_restrict<T>(P.A, P.bv<T>(), rs[lev].p(), bs[lev + 1].p(), N, st);
+        run(lev + 1, bs[lev + 1].p(), xs[lev + 1].p(), st);
+        if (mg.cfg.w_cycle && lev + 1 < last) {
+            ops[lev + 1]->apply(ts[lev + 1].p(), xs[lev + 1].p(), EPI_RESID, bs[lev + 1].p(), nullptr, FinB(), st);
+            run(lev + 1, ts[lev + 1].p(), e2s[lev + 1].p(), st);
+            launch_b_axpby_real<T>(xs[lev + 1].p(), e2s[lev + 1].p(), 1.0, 1.0, mg.dims[lev + 1] * N, st);
+        }
+        launch_b_prolong<T>(P.A, P.bv<T>(), xs[lev + 1].p(), x, 1, N, st);
+        if (om > 0) rich_b<T>(A, x, b, rs[lev].p(), mg.cfg.post, om, false, st);
+        else mr_b<T>(A, x, b, rs[lev].p(), Ars[lev].p(), mg.cfg.post, false, sc[lev]->p(), st);
+    }
+};
+
+namespace {
+std::map<std::pair<MG*, int>, std::unique_ptr<CycleB<double>>>& cyc64_map() {
+    static auto* m = new std::map<std::pair<MG*, int>, std::unique_ptr<CycleB<double>>>();
+    return *m;
+}
+std::map<std::pair<MG*, int>, std::unique_ptr<CycleB<float>>>& cyc32_map() {
+    static auto* m = new std::map<std::pair<MG*, int>, std::unique_ptr<CycleB<float>>>();
+    return *m;
+}
+}  // namespace
+
+template <>
+CycleB<double>& batched_cycle<double>(MG& mg, int N) {
+    auto& m = cyc64_map();
+    auto& c = m[{&mg, N}];
+    if (!c) c.reset(new CycleB<double>(mg, N));
+    return *c;
+}
+template <>
+CycleB<float>& batched_cycle<float>(MG& mg, int N) {
+    auto& m = cyc32_map();
+    auto& c = m[{&mg, N}];
+    if (!c) c.reset(new CycleB<float>(mg, N));
+    return *c;
+}
+void release_batched_cycles(MG* mg) {
+    for (auto it = cyc64_map().begin(); it != cyc64_map().end();)
+        it = (it->first.first == mg) ? cyc64_map().erase(it) : std::next(it);
+    for (auto it = cyc32_map().begin(); it != cyc32_map().end();)
+        it = (it->first.first == mg) ? cyc32_map().erase(it) : std::next(it);
+}
+
+template <typename T>
+void mg_precondition_b(MG& mg, const cplx<T>* b, cplx<T>* x, int N, hipStream_t st) {
+    batched_cycle<T>(mg, N).run(0, b, x, st);
+}
+template void mg_precondition_b<double>(MG&, const c64*, c64*, int, hipStream_t);
+template void mg_precondition_b<float>(MG&, const c32*, c32*, int, hipStream_t);
+
+// ------------------------------------------------------------------ GCR (batched)
+// fcg_solve (src/solvers.cpp:223-300) with the textbook projection, N RHS in lockstep
+template <typename T>
+int gcr_solve_b(LinOpB<T>& A, const cplx<T>* b, cplx<T>* x, MG* mg, int maxit, double tol, int trunc, double* res,
+                hipStream_t st) {
+    const size_t n = A.n(), nN = n * A.N;
+    const int N = A.N;
+    SMG_REQUIRE(trunc >= 1 && trunc <= 16, "batched GCR truncation window must be 1..16");
+    Context& cx = Context::get();
+    ScalarsB sc(64 * 2 + 64 + 64 * 32 + 17 * 64);
+    double* alpha = sc.p(0);                   // [N][2]
+    double* rr = sc.p(128);                    // [N]
+    double* dots = sc.p(192);                  // [N][32]
+    double* apap = sc.p(192 + 64 * 32);        // [slot][N]
+    launch_zero<T>(x, nN, st);
+    std::vector<double> bn(N), rn(N);
+    launch_b_norm2<T>(b, n, finb(FIN_STORE, rr, 1), N, st);
+    cx.read(rr, bn.data(), N);
+    for (int j = 0; j < N; j++) {
+        bn[j] = std::sqrt(bn[j]);
+        rn[j] = bn[j];
+    }
+    Field<T> r(nN);
+    bcopy(r.p(), b, nN, st);
+    std::vector<Field<T>> Pd, APd;
+    for (int i = 0; i <= trunc; i++) {
+        Pd.emplace_back(nN);
+        APd.emplace_back(nN);
+    }
+    std::deque<int> order;
+    std::vector<int> freeslots;
+    for (int i = trunc; i >= 0; i--) freeslots.push_back(i);
+    auto done = [&]() {
+        for (int j = 0; j < N; j++)
+            if (bn[j] > 1e-30 && rn[j] / bn[j] >= tol) return false;
+        return true;
+    };
+    int iter = 0;
+    while (iter < maxit && !done()) {
+        const int slot = freeslots.back();
+        freeslots.pop_back();
+        cplx<T>* p = Pd[slot].p();
+        cplx<T>* Ap = APd[slot].p();
+        if (mg) batched_cycle<T>(*mg, N).run(0, r.p(), p, st);
+        else bcopy(p, r.p(), nN, st);
+        A.apply(Ap, p, st);
+        const int nd = (int)order.size();
+        if (nd > 0) {
+            VecPtrs<T> Pv, APv;
+            const double* ap[16];
+            for (int i = 0; i < nd; i++) {
+                Pv.v[i] = Pd[order[i]].p();
+                APv.v[i] = APd[order[i]].p();
+                ap[i] = apap + (size_t)order[i] * N;
+            }
+            launch_b_multi_dot<T>(APv, nd, Ap, n, finb(FIN_STORE, dots, 32), N, st);
+            launch_b_gcr_orth<T>(p, Ap, Pv, APv, nd, dots, ap, n, N, st);
+        }
+        launch_b_dots<T>(Ap, r.p(), n, finb(FIN_GCR_ALPHA, alpha, 2, apap + (size_t)slot * N, 1), N, st);
+        launch_b_update<T>(x, r.p(), p, Ap, alpha, n, finb(FIN_STORE, rr, 1), N, st);
+        cx.read(rr, rn.data(), N);
+        for (int j = 0; j < N; j++) rn[j] = std::sqrt(rn[j]);
+        iter++;
+        if ((int)order.size() >= trunc) {
+            freeslots.push_back(order.front());
+            order.pop_front();
+        }
+        order.push_back(slot);
+    }
+    for (int j = 0; j < N; j++) res[j] = bn[j] > 1e-30 ? rn[j] / bn[j] : 0.0;
+    return iter;
+}
+template int gcr_solve_b<double>(LinOpB<double>&, const c64*, c64*, MG*, int, double, int, double*, hipStream_t);
+template int gcr_solve_b<float>(LinOpB<float>&, const c32*, c32*, MG*, int, double, int, double*, hipStream_t);
+
+BatchedSolveResult gcr_solve_batched(const Dirac& D, MG* mg, int prec, const c64* B, c64* X, int N, int maxit,
+                                     double tol, int trunc, double inner_tol, hipStream_t st) {
+    SMG_REQUIRE(N >= 1 && N <= 32 && (N & (N - 1)) == 0, "batched solves take a power-of-two count <= 32");
+    BatchedSolveResult out;
+    out.true_residual.assign(N, 0.0);
+    Context& cx = Context::get();
+    const size_t n = 2 * (size_t)D.V, nN = n * N;
+    DdagDOpB<double> A64(D, N);
+    Field<double> b64(nN), x64(nN), r64(nN);
+    ScalarsB sc(64);
+    launch_b_pack<double, double>(b64.p(), B, n, N, st);
+    std::vector<double> bn(N), rn(N), res(N);
+    launch_b_norm2<double>(b64.p(), n, finb(FIN_STORE, sc.p(), 1), N, st);
+    cx.read(sc.p(), bn.data(), N);
+    for (int j = 0; j < N; j++) bn[j] = std::sqrt(bn[j]);
+    if (prec == 0) {
+        out.iterations = gcr_solve_b<double>(A64, b64.p(), x64.p(), mg, maxit, tol, trunc, res.data(), st);
+    } else {
+        DdagDOpB<float> A32(D, N);
+        Field<float> r32(nN), e32(nN);
+        launch_zero<double>(x64.p(), nN, st);
+        bcopy(r64.p(), b64.p(), nN, st);
+        rn = bn;
+        int total = 0;
+        while (total < maxit) {
+            bool conv = true;
+            double worst = 0.0;
+            for (int j = 0; j < N; j++)
+                if (bn[j] > 1e-30) {
+                    worst = std::max(worst, rn[j] / bn[j]);
+                    if (rn[j] / bn[j] >= tol) conv = false;
+                }
+            if (conv) break;
+            launch_convert<float, double>(r32.p(), r64.p(), nN, st);
+            const double itol = std::max(inner_tol, 0.5 * tol / worst);
+            const int it = gcr_solve_b<float>(A32, r32.p(), e32.p(), mg, maxit - total, itol, trunc, res.data(), st);
+            total += std::max(it, 1);
+            launch_b_accum_mixed(x64.p(), e32.p(), n, N, st);
+            A64.apply(r64.p(), x64.p(), EPI_RESID, b64.p(), nullptr, FinB(), st);
+            launch_b_norm2<double>(r64.p(), n, finb(FIN_STORE, sc.p(), 1), N, st);
+            cx.read(sc.p(), rn.data(), N);
+            for (int j = 0; j < N; j++) rn[j] = std::sqrt(rn[j]);
+            out.restarts++;
+        }
+        out.iterations = total;
+    }
+    // FP64 true residuals
+    A64.apply(r64.p(), x64.p(), EPI_RESID, b64.p(), nullptr, FinB(), st);
+    launch_b_norm2<double>(r64.p(), n, finb(FIN_STORE, sc.p(), 1), N, st);
+    cx.read(sc.p(), rn.data(), N);
+    for (int j = 0; j < N; j++) out.true_residual[j] = bn[j] > 1e-30 ? std::sqrt(rn[j]) / bn[j] : 0.0;
+    launch_b_unpack<double, double>(X, x64.p(), n, N, st);
+    return out;
+}
+
+}  // namespace smg

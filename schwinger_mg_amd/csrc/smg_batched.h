@@ -1,0 +1,77 @@
+// smg_batched.h -- launchers of the multi-right-hand-side ("batched") kernels.
+//
+// A batched field holds N right-hand sides with the RHS index innermost:
+//   v[(i * N) + j], i = site*nd + dof, j = 0..N-1, N a power of two <= 64.
+// Work-items map to (i, j) with j fastest, so a wave64 reads N consecutive
+// complex numbers per dof (coalesced) and shares the site's links, clover term
+// and aggregate vectors across the N lanes of one site (broadcast loads).
+// Every scalar of the Krylov and smoother programs is per RHS: out[j*ostride..],
+// aux[j*astride..].
+#pragma once
+#include "smg_kernels.h"
+
+namespace smg {
+
+struct FinB {
+    int op = FIN_NONE;
+    double* out = nullptr;
+    int ostride = 2;
+    double* aux = nullptr;
+    int astride = 16;
+    double* part = nullptr;
+    unsigned* counter = nullptr;
+};
+
+// Per-RHS coefficient array views (device): complex a_j at s[2j], s[2j+1].
+template <typename T>
+void launch_b_dslash(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T mu, int g5in, int g5out, Epi epi,
+                     const cplx<T>* b, const cplx<T>* aux, const FinB& fin, int N, hipStream_t st);
+template <typename T, typename TL>
+void launch_b_coarse_apply(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b,
+                           const cplx<T>* aux, const FinB& fin, int N, hipStream_t st);
+template <typename T>
+void launch_b_prolong(const AggView& A, const cplx<T>* bv, const cplx<T>* coarse, cplx<T>* fine, int accumulate,
+                      int N, hipStream_t st);
+template <typename T>
+void launch_b_restrict(const AggView& A, const cplx<T>* bv, const cplx<T>* fine, cplx<T>* coarse, int N,
+                       hipStream_t st);
+template <typename T>
+void launch_b_dense_apply(const cplx<T>* M, const cplx<T>* x, cplx<T>* y, size_t n, int N, hipStream_t st);
+
+// reductions over the vector index, one result per RHS
+template <typename T>
+void launch_b_norm2(const cplx<T>* a, size_t n, const FinB& fin, int N, hipStream_t st);         // {|a_j|^2}
+template <typename T>
+void launch_b_dots(const cplx<T>* a, const cplx<T>* b, size_t n, const FinB& fin, int N, hipStream_t st);  // {<a,b>,|a|^2}
+// fin.out[j*2*nv + 2i..] = <V_i[:,j], w[:,j]> for i < nv (nv <= 16)
+template <typename T>
+void launch_b_multi_dot(const VecPtrs<T>& V, int nv, const cplx<T>* w, size_t n, const FinB& fin, int N,
+                        hipStream_t st);
+// p -= sum_i beta_ij P_i ; Ap -= sum_i beta_ij AP_i, beta_ij = dots[j*2*nv + 2i..] / apap_i[j]
+template <typename T>
+void launch_b_gcr_orth(cplx<T>* p, cplx<T>* Ap, const VecPtrs<T>& P, const VecPtrs<T>& AP, int nv,
+                       const double* dots, const double* const* apap, size_t n, int N, hipStream_t st);
+// x += w_j r ; r -= w_j Ar   (w_j at s[2j..])
+template <typename T>
+void launch_b_mr_update(cplx<T>* x, cplx<T>* r, const cplx<T>* Ar, const double* s, size_t n, int N, hipStream_t st);
+// x += a_j p ; r -= a_j Ap with {|r_j|^2} -> fin
+template <typename T>
+void launch_b_update(cplx<T>* x, cplx<T>* r, const cplx<T>* p, const cplx<T>* Ap, const double* s, size_t n,
+                     const FinB& fin, int N, hipStream_t st);
+// p = z + b_j p (b_j at s[2j..])
+template <typename T>
+void launch_b_xpay(cplx<T>* p, const cplx<T>* z, const double* s, size_t n, int N, hipStream_t st);
+// per-RHS CG program state: aux_j[5] = tol^2, flag_j = 0
+void launch_b_cg_init(double* aux, int astride, int N, double tol2, hipStream_t st);
+// y = a x + c y with one real scalar pair for all RHS
+template <typename T>
+void launch_b_axpby_real(cplx<T>* y, const cplx<T>* x, double a, double c, size_t n, hipStream_t st);
+// layout transforms: separate vectors [j][i] (n entries each) <-> batched [i][j], with precision change
+template <typename To, typename From>
+void launch_b_pack(cplx<To>* batched, const cplx<From>* sep, size_t n, int N, hipStream_t st);
+template <typename To, typename From>
+void launch_b_unpack(cplx<To>* sep, const cplx<From>* batched, size_t n, int N, hipStream_t st);
+// x64 += e32 (batched layouts); r = b - Ax with {|r_j|^2}
+void launch_b_accum_mixed(c64* x, const c32* e, size_t n, int N, hipStream_t st);
+
+}  // namespace smg

@@ -176,8 +176,12 @@ __device__ __forceinline__ void fin_program(int op, const double* v, int nc, dou
         break;
     }
     case FIN_CG_ALPHA: {
-        // src/coarse_op.h:520-523 (pAp = <p, Ap> = conj(<Ap, p>))
-        if (*flag) break;
+        // src/coarse_op.h:520-523 (pAp = <p, Ap> = conj(<Ap, p>)); a finished RHS takes a zero step
+        if (*flag) {
+            out[0] = 0.0;
+            out[1] = 0.0;
+            break;
+        }
         const cplx<double> pAp = {v[0], -v[1]};
         if (sqrt(pAp.re * pAp.re + pAp.im * pAp.im) < 1e-30) {
             *flag = 2;
