@@ -221,6 +221,15 @@ int smg_fgmres_solve(smg_dirac* D, smg_mg* mg, const double* b, double* x, int r
 /* the stencil on the batched layout (device pointers): src/dst[(2*s + spin)*N + j] */
 int smg_dirac_apply_batched_dev(smg_dirac* D, int kind, int prec, const void* src, void* dst, int nrhs,
                                 void* stream);
+/* SparseCoarseOp::apply_to (src/coarse_op.h:282-297) on N right-hand sides at
+ * once: host arrays of N coarse vectors one after the other; links read in
+ * precision prec (SMG_F64, SMG_F32, or SMG_F16 = FP16 links with FP32
+ * arithmetic).  For k in {8, 16} and N in {16, 32, 64} the product runs on the
+ * matrix cores. */
+int smg_coarse_apply_batched(smg_coarse* C, int prec, int nrhs, const double* src, double* dst);
+/* the same on device arrays in the batched layout src/dst[(b*k + kk)*N + j]
+ * (complex128 for SMG_F64, complex64 otherwise) */
+int smg_coarse_apply_batched_dev(smg_coarse* C, int prec, int nrhs, const void* src, void* dst, void* stream);
 /* one multigrid cycle applied to N right-hand sides at once (host arrays) */
 int smg_mg_precondition_batched(smg_mg* mg, int prec, int nrhs, const double* B, double* X);
 /* smg_gcr_solve for N right-hand sides in lockstep (textbook GCR projection);

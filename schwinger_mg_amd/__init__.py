@@ -76,6 +76,8 @@ SIGNATURES = [
     ("smg_gcr_solve_dev", _I, [_P, _P, _I, _P, _P, _I, _D, _I, _D, _P, _P]),
     ("smg_fgmres_solve", _I, [_P, _P, _P, _P, _I, _I, _D, _P]),
     ("smg_dirac_apply_batched_dev", _I, [_P, _I, _I, _P, _P, _I, _P]),
+    ("smg_coarse_apply_batched", _I, [_P, _I, _I, _P, _P]),
+    ("smg_coarse_apply_batched_dev", _I, [_P, _I, _I, _P, _P, _P]),
     ("smg_mg_precondition_batched", _I, [_P, _I, _I, _P, _P]),
     ("smg_gcr_solve_batched", _I, [_P, _P, _I, _I, _P, _P, _I, _D, _I, _D, _P, _P]),
     ("smg_gcr_solve_batched_dev", _I, [_P, _P, _I, _I, _P, _P, _I, _D, _I, _D, _P, _P, _P]),
@@ -370,6 +372,13 @@ class SparseCoarseOp:
     def apply_to(self, src, prec=F64):
         out = np.zeros(self.dim, np.complex128)
         _chk(lib().smg_coarse_apply(self.h, prec, _p(_c(src)), _p(out)))
+        return out
+
+    def apply_batched(self, S, prec=F64):
+        """apply_to on N right-hand sides at once; S is (N, dim)."""
+        S = _c(S)
+        out = np.zeros_like(S)
+        _chk(lib().smg_coarse_apply_batched(self.h, prec, S.shape[0], _p(S), _p(out)))
         return out
 
     def solve(self, b, max_iter=200, cg_tol=1e-12):

@@ -16,6 +16,19 @@ constexpr int kRedGrid = 1024;   // workgroups of a reducing batched kernel
 __device__ __forceinline__ int lg2(int N) { return 31 - __clz(N); }
 
 // Per-RHS reduce-and-finish.  v[NC] holds this lane's partials for RHS j = lane % N.
+//
+// Two-level, fixed-order tree: every workgroup writes its row of NC*N partials;
+// the last workgroup to finish within a group of kGroup consecutive workgroups
+// sums that group's rows in index order into a group row; the last group to
+// finish sums the group rows in index order and runs the scalar program of
+// every RHS.  Each level is at most kGroup (or nblk/kGroup) loads deep per
+// column, all issued before the adds, so the finish costs a few memory
+// latencies instead of one latency per workgroup.  The result depends only on
+// nblk, not on the order in which workgroups finish.
+// Counters: counter[0] counts finished groups, counter[1 + g] the workgroups of
+// group g; each is reset to zero by the workgroup that consumed it.
+constexpr int kGroup = 32;
+
 template <int NC, int NMAX>
 __device__ __forceinline__ void reduce_finish_b(double (&v)[NC], const FinB& f, int N, int nblk, int blk) {
     __shared__ double sm[kBlock / 64][NC * NMAX];
@@ -33,17 +46,41 @@ __device__ __forceinline__ void reduce_finish_b(double (&v)[NC], const FinB& f, 
         f.part[(size_t)blk * tot + t] = (sm[0][t] + sm[1][t]) + (sm[2][t] + sm[3][t]);
     __threadfence();
     __syncthreads();
-    if (threadIdx.x == 0) last = (atomicAdd(f.counter, 1u) == (unsigned)nblk - 1u);
+    const int ngrp = (nblk + kGroup - 1) / kGroup, g = blk / kGroup;
+    const int g0 = g * kGroup, gn = min(nblk - g0, kGroup);
+    if (threadIdx.x == 0) last = (atomicAdd(f.counter + 1 + g, 1u) == (unsigned)gn - 1u);
     __syncthreads();
     if (!last) return;
     __threadfence();
-    // component c = t % tot, rows g, g + G, ... for group g = t / tot; groups combined in order
-    const int G = tot >= kBlock ? 1 : kBlock / tot;
-    double* acc = &sm[0][0];   // G * tot <= kBlock*... fits: tot*G <= max(tot, kBlock) <= 4*NC*NMAX
-    for (int t = threadIdx.x; t < tot * G; t += kBlock) {
-        const int c = t % tot, g = t / tot;
+    double* grow = f.part + (size_t)nblk * tot;   // group rows follow the workgroup rows
+    for (int t = threadIdx.x; t < tot; t += kBlock) {
+        double r[kGroup];
+#pragma unroll
+        for (int i = 0; i < kGroup; i++) r[i] = i < gn ? f.part[(size_t)(g0 + i) * tot + t] : 0.0;
+        double a = r[0];
+#pragma unroll
+        for (int i = 1; i < kGroup; i++) a += r[i];
+        grow[(size_t)g * tot + t] = a;
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        f.counter[1 + g] = 0u;
+        last = (atomicAdd(f.counter, 1u) == (unsigned)ngrp - 1u);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double* acc = &sm[0][0];   // tot <= NC * NMAX entries
+    for (int t = threadIdx.x; t < tot; t += kBlock) {
+        double r[kGroup];
         double a = 0.0;
-        for (int i = g; i < nblk; i += G) a += f.part[(size_t)i * tot + c];
+        for (int i0 = 0; i0 < ngrp; i0 += kGroup) {
+#pragma unroll
+            for (int i = 0; i < kGroup; i++) r[i] = i0 + i < ngrp ? grow[(size_t)(i0 + i) * tot + t] : 0.0;
+#pragma unroll
+            for (int i = 0; i < kGroup; i++) a += r[i];
+        }
         acc[t] = a;
     }
     __syncthreads();
@@ -51,11 +88,7 @@ __device__ __forceinline__ void reduce_finish_b(double (&v)[NC], const FinB& f, 
         const int j = threadIdx.x;
         double w[NC];
 #pragma unroll
-        for (int c = 0; c < NC; c++) {
-            double a = 0.0;
-            for (int g = 0; g < G; g++) a += acc[g * tot + c * N + j];
-            w[c] = a;
-        }
+        for (int c = 0; c < NC; c++) w[c] = acc[c * N + j];
         fin_program(f.op, w, NC, f.out + (size_t)j * f.ostride, f.aux ? f.aux + (size_t)j * f.astride : nullptr);
     }
     __syncthreads();
@@ -192,9 +225,166 @@ __global__ __launch_bounds__(kBlock) void k_b_coarse(CoarseView<TL> C, cplx<T>* 
     }
     if (EPI == EPI_DOTS) reduce_finish_b<3, 64>(acc, fin, N, gridDim.x, blockIdx.x);
 }
+// ----------------------------------------------------------------- MFMA coarse operator
+// The complex products of the coarse stencil and of the dense coarsest inverse
+// run on the matrix cores as real GEMMs: a complex (m x k) x (k x N) product is
+//   [[Re A, -Im A], [Im A, Re A]] (2m x 2k, rows (i, c) and columns (kk, c')
+//   interleaved) x (Re X, Im X) (2k x N, rows (kk, c')),
+// whose output rows (i, c) are (Re, Im) of the complex result.  Tiles are the
+// 16x16x4 MFMA shapes in FP64 (FP64 path) or FP32 (FP32 path, coarse links
+// stored in FP16/FP32 and widened on load).  Lane layouts (checked on gfx950):
+// A[i][kk] in lane i + 16 kk and B[kk][j] in lane j + 16 kk for both; the
+// result D[i][j] sits in lane j + 16 (i / 4), register i % 4 for FP32, and in
+// lane j + 16 (i % 4), register i / 4 for FP64.  The tile row i <-> (complex row
+// ki, part c) assignment is chosen per precision so that lane (j, g) always
+// holds (Re, Im) of rows ki = 2g and 2g + 1 in registers (0, 1) and (2, 3).
+template <typename T>
+struct MfmaTile;
+template <>
+struct MfmaTile<double> {
+    typedef double acc __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc mma(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // tile row i = g + 4v  <->  ki = 2g + v/2, c = v%2
+    static __device__ __forceinline__ int row_ki(int i) { return 2 * (i & 3) + (i >> 3); }
+    static __device__ __forceinline__ int row_c(int i) { return (i >> 2) & 1; }
+};
+template <>
+struct MfmaTile<float> {
+    typedef float acc __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc mma(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // tile row i = 4g + v  <->  ki = 2g + v/2, c = v%2
+    static __device__ __forceinline__ int row_ki(int i) { return i >> 1; }
+    static __device__ __forceinline__ int row_c(int i) { return i & 1; }
+};
+
+// A-fragment entry of the real form of complex a for row part c and column part cp
+template <typename T>
+__device__ __forceinline__ T real_form(cplx<T> a, int c, int cp) {
+    return c == cp ? a.re : (c ? a.im : -a.im);
+}
+
+// One wave per coarse site b: out'[b] (2K x N) = sum_d A'_{b,d} (2K x 2K) B'_{nbr(b,d)} (2K x N);
+// RT = 2K/16 row tiles, NT = N/16 RHS tiles, K/2 MFMA steps of 4 real columns per direction.
+template <typename T, typename TL, int EPI, int K, int NT>
+__global__ __launch_bounds__(kBlock) void k_b_coarse_mfma(CoarseView<TL> C, cplx<T>* __restrict__ dst,
+                                                          const cplx<T>* __restrict__ src,
+                                                          const cplx<T>* __restrict__ bvec,
+                                                          const cplx<T>* __restrict__ aux, FinB fin) {
+    constexpr int N = 16 * NT, RT = K / 8, KS = K / 2;
+    using MT = MfmaTile<T>;
+    using acc_t = typename MT::acc;
+    const int lane = threadIdx.x & 63, lo = lane & 15, hi = lane >> 4;
+    const int nx = C.nx, nsites = C.nx * C.ny;
+    const int nw = kBlock / 64;
+    const T* srcr = reinterpret_cast<const T*>(src);
+    // A role: tile row lo -> (ki, c), column (kj = hi/2, c' = hi%2); B role: column j = lo, row (hi/2, hi%2)
+    const int a_ki = MT::row_ki(lo), a_c = MT::row_c(lo), a_kj = hi >> 1, a_cp = hi & 1;
+    double dacc[NT][3];
+#pragma unroll
+    for (int t = 0; t < NT; t++) dacc[t][0] = dacc[t][1] = dacc[t][2] = 0.0;
+    for (int b = blockIdx.x * nw + (threadIdx.x >> 6); b < nsites; b += gridDim.x * nw) {
+        const int bx = b % nx, by = b / nx;
+        const int xm = (bx == 0 ? nx - 1 : bx - 1), xp = (bx + 1 == nx ? 0 : bx + 1);
+        const int ym = (by == 0 ? C.ny - 1 : by - 1), yp = (by + 1 == C.ny ? 0 : by + 1);
+        acc_t acc[RT][NT];
+#pragma unroll
+        for (int r = 0; r < RT; r++)
+#pragma unroll
+            for (int t = 0; t < NT; t++) acc[r][t] = acc_t{0, 0, 0, 0};
+#pragma unroll
+        for (int d = 0; d < 9; d++) {
+            const int cx = (d == 1 || d == 5 || d == 6) ? xp : (d == 2 || d == 7 || d == 8) ? xm : bx;
+            const int cy = (d == 3 || d == 5 || d == 7) ? yp : (d == 4 || d == 6 || d == 8) ? ym : by;
+            const int nb = cx + cy * nx;
+            const TL* lb = C.link + ((size_t)b * 9 + d) * (K * K) + a_ki * K + a_kj;
+            const T* sb = srcr + ((size_t)nb * K * N + (size_t)(hi >> 1) * N + lo) * 2 + (hi & 1);
+#pragma unroll
+            for (int s = 0; s < KS; s++) {
+                T av[RT], bv[NT];
+#pragma unroll
+                for (int r = 0; r < RT; r++) av[r] = real_form(cvt<T>(load_c(lb + 8 * r * K + 2 * s)), a_c, a_cp);
+#pragma unroll
+                for (int t = 0; t < NT; t++) bv[t] = sb[((size_t)(2 * s) * N + 16 * t) * 2];
+#pragma unroll
+                for (int r = 0; r < RT; r++)
+#pragma unroll
+                    for (int t = 0; t < NT; t++) acc[r][t] = MT::mma(av[r], bv[t], acc[r][t]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RT; r++)
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int ki = 8 * r + 2 * hi + h;
+                    const size_t idx = ((size_t)b * K + ki) * N + 16 * t + lo;
+                    cplx<T> o = {acc[r][t][2 * h], acc[r][t][2 * h + 1]};
+                    if (EPI == EPI_RESID) o = bvec[idx] - o;
+                    dst[idx] = o;
+                    if (EPI == EPI_DOTS) {
+                        const cplx<T> p = cdot(o, aux[idx]);
+                        dacc[t][0] += p.re;
+                        dacc[t][1] += p.im;
+                        dacc[t][2] += norm2(o);
+                    }
+                }
+    }
+    if (EPI == EPI_DOTS) {
+        // lanes with equal lo hold partials of RHS 16t + lo: fold the four lane groups,
+        // then lane j < N carries RHS j into the per-RHS reduction
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                dacc[t][c] += __shfl_xor(dacc[t][c], 16, 64);
+                dacc[t][c] += __shfl_xor(dacc[t][c], 32, 64);
+            }
+        double v[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+            if (lane < N && (lane >> 4) == t)
+#pragma unroll
+                for (int c = 0; c < 3; c++) v[c] = dacc[t][c];
+        reduce_finish_b<3, 64>(v, fin, N, gridDim.x, blockIdx.x);
+    }
+}
+
+template <typename T, typename TL, int K, int NT>
+static void coarse_mfma(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b,
+                        const cplx<T>* aux, const FinB& fin, hipStream_t st) {
+    const size_t waves = (size_t)C.nx * C.ny;
+    const int nw = kBlock / 64;
+    const int g = (int)std::min<size_t>((waves + nw - 1) / nw, epi == EPI_DOTS ? kRedGrid : 1 << 16);
+    if (epi == EPI_NONE)
+        k_b_coarse_mfma<T, TL, EPI_NONE, K, NT><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
+    else if (epi == EPI_RESID)
+        k_b_coarse_mfma<T, TL, EPI_RESID, K, NT><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
+    else
+        k_b_coarse_mfma<T, TL, EPI_DOTS, K, NT><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
+}
+
 template <typename T, typename TL>
 void launch_b_coarse_apply(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b,
                            const cplx<T>* aux, const FinB& fin, int N, hipStream_t st) {
+    if ((C.k == 8 || C.k == 16) && (N == 16 || N == 32 || N == 64)) {
+        // matrix-core path
+        if (C.k == 8) {
+            if (N == 16) coarse_mfma<T, TL, 8, 1>(C, dst, src, epi, b, aux, fin, st);
+            else if (N == 32) coarse_mfma<T, TL, 8, 2>(C, dst, src, epi, b, aux, fin, st);
+            else coarse_mfma<T, TL, 8, 4>(C, dst, src, epi, b, aux, fin, st);
+        } else {
+            if (N == 16) coarse_mfma<T, TL, 16, 1>(C, dst, src, epi, b, aux, fin, st);
+            else if (N == 32) coarse_mfma<T, TL, 16, 2>(C, dst, src, epi, b, aux, fin, st);
+            else coarse_mfma<T, TL, 16, 4>(C, dst, src, epi, b, aux, fin, st);
+        }
+        SMG_LAUNCH_CHECK();
+        return;
+    }
     const size_t items = (size_t)C.nx * C.ny * C.k * N;
     if (epi == EPI_NONE)
         k_b_coarse<T, TL, EPI_NONE><<<b_grid(items, 1 << 16), kBlock, 0, st>>>(C, dst, src, b, aux, fin, N);
@@ -267,7 +457,8 @@ void launch_b_restrict(const AggView& A, const cplx<T>* bv, const cplx<T>* fine,
     SMG_LAUNCH_CHECK();
 }
 
-// Y = M X, M row-major n x n (the dense coarsest inverse); one work-item per (row, j)
+// Y = M X, M row-major n x n (the dense coarsest inverse); one work-item per (row, j).
+// Used when the matrix-core path does not apply (N < 16 or n % 8 != 0).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_dense(const cplx<T>* __restrict__ M, const cplx<T>* __restrict__ X,
                                                     cplx<T>* __restrict__ Y, size_t n, int N) {
@@ -282,8 +473,78 @@ __global__ __launch_bounds__(kBlock) void k_b_dense(const cplx<T>* __restrict__ 
         Y[t] = acc;
     }
 }
+
+// Matrix-core dense product: wave w computes one tile of 8 complex rows x N RHS
+// over column slice w / ntile (of SK equal slices); slices land in part[slice]
+// and are summed in slice order by k_b_dense_sum.
+template <typename T, int NT>
+__global__ __launch_bounds__(kBlock) void k_b_dense_mfma(const cplx<T>* __restrict__ M,
+                                                         const cplx<T>* __restrict__ X,
+                                                         cplx<T>* __restrict__ part, int n, int SK) {
+    constexpr int N = 16 * NT;
+    using MT = MfmaTile<T>;
+    using acc_t = typename MT::acc;
+    const int lane = threadIdx.x & 63, lo = lane & 15, hi = lane >> 4;
+    const int ntile = n / 8;
+    const int w = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (w >= ntile * SK) return;   // wave-uniform; no workgroup barrier follows
+    const int tile = w % ntile, slice = w / ntile;
+    const int kc = n / SK, k0 = slice * kc;
+    const int a_c = MT::row_c(lo);
+    const cplx<T>* mp = M + (size_t)(8 * tile + MT::row_ki(lo)) * n + k0 + (hi >> 1);
+    const T* xp = reinterpret_cast<const T*>(X) + ((size_t)(k0 + (hi >> 1)) * N + lo) * 2 + (hi & 1);
+    acc_t acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = acc_t{0, 0, 0, 0};
+    for (int k = 0; k < kc; k += 2) {
+        const T a = real_form(mp[k], a_c, hi & 1);
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[t] = MT::mma(a, xp[((size_t)k * N + 16 * t) * 2], acc[t]);
+    }
+    cplx<T>* out = part + (size_t)slice * n * N;
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+            out[(size_t)(8 * tile + 2 * hi + h) * N + 16 * t + lo] = cplx<T>{acc[t][2 * h], acc[t][2 * h + 1]};
+}
 template <typename T>
-void launch_b_dense_apply(const cplx<T>* M, const cplx<T>* x, cplx<T>* y, size_t n, int N, hipStream_t st) {
+__global__ __launch_bounds__(kBlock) void k_b_dense_sum(const cplx<T>* __restrict__ part, cplx<T>* __restrict__ Y,
+                                                        size_t items, int SK) {
+    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < items; t += (size_t)gridDim.x * kBlock) {
+        cplx<T> a = part[t];
+        for (int s = 1; s < SK; s++) a = a + part[(size_t)s * items + t];
+        Y[t] = a;
+    }
+}
+
+// column slices of the matrix-core dense product: about 2048+ waves, kc = n/SK even
+static int dense_slices(size_t n) {
+    int sk = 1;
+    while (sk < 64 && (n / 8) * sk < 2048 && n % (size_t)(4 * sk) == 0) sk *= 2;
+    return sk;
+}
+size_t b_dense_work(size_t n, int N) {
+    return (N >= 16 && N <= 64 && n % 8 == 0) ? (size_t)dense_slices(n) * n * N : 0;
+}
+
+template <typename T>
+void launch_b_dense_apply(const cplx<T>* M, const cplx<T>* x, cplx<T>* y, size_t n, int N, cplx<T>* work,
+                          hipStream_t st) {
+    const size_t need = b_dense_work(n, N);
+    if (need > 0 && work != nullptr) {
+        SMG_REQUIRE(n < (size_t)1 << 30, "dense coarsest: n too large");
+        const int sk = dense_slices(n);
+        const size_t waves = (n / 8) * sk;
+        const int g = (int)((waves + 3) / 4);
+        if (N == 16) k_b_dense_mfma<T, 1><<<g, kBlock, 0, st>>>(M, x, work, (int)n, sk);
+        else if (N == 32) k_b_dense_mfma<T, 2><<<g, kBlock, 0, st>>>(M, x, work, (int)n, sk);
+        else k_b_dense_mfma<T, 4><<<g, kBlock, 0, st>>>(M, x, work, (int)n, sk);
+        SMG_LAUNCH_CHECK();
+        k_b_dense_sum<T><<<b_grid(n * N, 1 << 16), kBlock, 0, st>>>(work, y, n * N, sk);
+        SMG_LAUNCH_CHECK();
+        return;
+    }
     k_b_dense<T><<<b_grid(n * N, 1 << 16), kBlock, 0, st>>>(M, x, y, n, N);
     SMG_LAUNCH_CHECK();
 }
@@ -521,7 +782,8 @@ void launch_b_accum_mixed(c64* x, const c32* e, size_t n, int N, hipStream_t st)
     template void launch_b_prolong<T>(const AggView&, const cplx<T>*, const cplx<T>*, cplx<T>*, int, int,        \
                                       hipStream_t);                                                              \
     template void launch_b_restrict<T>(const AggView&, const cplx<T>*, const cplx<T>*, cplx<T>*, int, hipStream_t); \
-    template void launch_b_dense_apply<T>(const cplx<T>*, const cplx<T>*, cplx<T>*, size_t, int, hipStream_t);   \
+    template void launch_b_dense_apply<T>(const cplx<T>*, const cplx<T>*, cplx<T>*, size_t, int, cplx<T>*, \
+                                           hipStream_t);                                                   \
     template void launch_b_norm2<T>(const cplx<T>*, size_t, const FinB&, int, hipStream_t);                     \
     template void launch_b_dots<T>(const cplx<T>*, const cplx<T>*, size_t, const FinB&, int, hipStream_t);      \
     template void launch_b_multi_dot<T>(const VecPtrs<T>&, int, const cplx<T>*, size_t, const FinB&, int,        \

@@ -723,6 +723,52 @@ int smg_dirac_apply_batched_dev(smg_dirac* D, int kind, int prec, const void* sr
         else dirac_kind_b<float>(*D->D, kind, (const c32*)src, (c32*)dst, nrhs, S());
     });
 }
+int smg_coarse_apply_batched(smg_coarse* C, int prec, int nrhs, const double* src, double* dst) {
+    return guard([&] {
+        SMG_REQUIRE(C && src && dst, "bad arguments");
+        check_nrhs(nrhs, 64);
+        SMG_REQUIRE(prec == SMG_F64 || prec == SMG_F32 || prec == SMG_F16, "prec must be SMG_F64, SMG_F32 or SMG_F16");
+        const size_t n = C->C->n(), nN = n * nrhs;
+        Field<double> s = up(src, nN), d(nN);
+        if (prec == SMG_F64) {
+            Field<double> sb(nN), db(nN);
+            launch_b_pack<double, double>(sb.p(), s.p(), n, nrhs, S());
+            CoarseLinOpB<double, c64> op(*C->C, nrhs);
+            op.apply(db.p(), sb.p(), S());
+            launch_b_unpack<double, double>(d.p(), db.p(), n, nrhs, S());
+        } else {
+            Field<float> sb(nN), db(nN);
+            launch_b_pack<float, double>(sb.p(), s.p(), n, nrhs, S());
+            if (prec == SMG_F32) {
+                CoarseLinOpB<float, c32> op(*C->C, nrhs);
+                op.apply(db.p(), sb.p(), S());
+            } else {
+                CoarseLinOpB<float, ch16> op(*C->C, nrhs);
+                op.apply(db.p(), sb.p(), S());
+            }
+            launch_b_unpack<double, float>(d.p(), db.p(), n, nrhs, S());
+        }
+        downc(dst, d.p(), nN);
+    });
+}
+int smg_coarse_apply_batched_dev(smg_coarse* C, int prec, int nrhs, const void* src, void* dst, void* stream) {
+    return guard([&] {
+        SMG_REQUIRE(C && src && dst, "bad arguments");
+        check_nrhs(nrhs, 64);
+        SMG_REQUIRE(prec == SMG_F64 || prec == SMG_F32 || prec == SMG_F16, "prec must be SMG_F64, SMG_F32 or SMG_F16");
+        StreamJoin j(stream);
+        if (prec == SMG_F64) {
+            CoarseLinOpB<double, c64> op(*C->C, nrhs);
+            op.apply((c64*)dst, (const c64*)src, S());
+        } else if (prec == SMG_F32) {
+            CoarseLinOpB<float, c32> op(*C->C, nrhs);
+            op.apply((c32*)dst, (const c32*)src, S());
+        } else {
+            CoarseLinOpB<float, ch16> op(*C->C, nrhs);
+            op.apply((c32*)dst, (const c32*)src, S());
+        }
+    });
+}
 int smg_mg_precondition_batched(smg_mg* mg, int prec, int nrhs, const double* B, double* X) {
     return guard([&] {
         SMG_REQUIRE(mg, "null handle");

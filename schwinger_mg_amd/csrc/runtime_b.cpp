@@ -123,7 +123,9 @@ struct CycleB {
     std::vector<Field<T>> xs, bs, rs, Ars, ts, e2s, ps;
     std::vector<std::unique_ptr<ScalarsB>> sc;
     int nlev;
-    CycleB(MG& m, int N_) : mg(m), N(N_), nlev(m.cfg.n_levels) {
+    Field<T> dwork;   // column-slice partials of the matrix-core dense coarsest product
+    CycleB(MG& m, int N_)
+        : mg(m), N(N_), nlev(m.cfg.n_levels), dwork(std::max<size_t>(1, b_dense_work(m.dims[m.cfg.n_levels - 1], N_))) {
         SMG_REQUIRE(N >= 1 && N <= 64 && (N & (N - 1)) == 0, "batched solves take a power-of-two count <= 64");
         own.emplace_back(new DdagDOpB<T>(*m.D, N));
         ops.push_back(own.back().get());
@@ -158,7 +160,8 @@ struct CycleB {
                 const cplx<T>* inv;
                 if constexpr (std::is_same<T, double>::value) inv = mg.inv64.p();
                 else inv = mg.inv32.p();
-                launch_b_dense_apply<T>(inv, b, x, mg.dims[lev], N, st);
+                launch_b_dense_apply<T>(inv, b, x, mg.dims[lev], N,
+                                        b_dense_work(mg.dims[lev], N) ? dwork.p() : nullptr, st);
             } else {
                 cg_b<T>(A, b, x, rs[lev].p(), ps[lev].p(), Ars[lev].p(), mg.cfg.coarse_maxit, mg.cfg.coarse_tol,
                         sc[lev]->p(), st);

@@ -35,8 +35,12 @@ void launch_b_prolong(const AggView& A, const cplx<T>* bv, const cplx<T>* coarse
 template <typename T>
 void launch_b_restrict(const AggView& A, const cplx<T>* bv, const cplx<T>* fine, cplx<T>* coarse, int N,
                        hipStream_t st);
+// y = M x with the dense n x n inverse M; work holds b_dense_work(n, N) entries
+// (the matrix-core path's column-slice partials; nullptr selects the plain kernel)
+size_t b_dense_work(size_t n, int N);
 template <typename T>
-void launch_b_dense_apply(const cplx<T>* M, const cplx<T>* x, cplx<T>* y, size_t n, int N, hipStream_t st);
+void launch_b_dense_apply(const cplx<T>* M, const cplx<T>* x, cplx<T>* y, size_t n, int N, cplx<T>* work,
+                          hipStream_t st);
 
 // reductions over the vector index, one result per RHS
 template <typename T>
