@@ -139,6 +139,15 @@ int smg_coarse_create_from_coarse(smg_coarse** out, smg_coarse* A, smg_prolongat
 int smg_coarse_create_links(smg_coarse** out, int nx, int ny, int k, const double* links);
 int smg_coarse_destroy(smg_coarse* C);
 int smg_coarse_get_links(smg_coarse* C, double* links);
+/* CoarseOp (dense A_c), src/coarse_op.h:18-59: the coarse operator as a dense
+ * n x n matrix in the reference's column layout, M[j*n + i] = A_c[i][j]
+ * (n = coarse dim <= 16384).  Equal to CoarseOp::build(D, P) for a Galerkin
+ * operator built from the same prolongator. */
+int smg_coarse_dense(smg_coarse* C, double* M);
+/* CoarseOp::solve, src/coarse_op.h:101-137: x = A_c^-1 b (FP64 Gauss-Jordan
+ * inverse; A_c must be Hermitian positive definite, as Galerkin projections of
+ * D^dag D are; n <= 8192) */
+int smg_coarse_dense_solve(smg_coarse* C, const double* b, double* x);
 /* SparseCoarseOp::apply_to, src/coarse_op.h:282-297, links read in precision prec */
 int smg_coarse_apply(smg_coarse* C, int prec, const double* src, double* dst);
 /* SparseCoarseOp::solve without deflation (CG from x0 = 0), src/coarse_op.h:492-547 */

@@ -325,6 +325,15 @@ def dense_galerkin(op, P=None):
     return cols.reshape(dim, dim)   # cols[j] = column j
 
 
+def dense_solve_galerkin(op, P, b):
+    """CoarseOp::build + CoarseOp::solve (dense LU with partial pivoting)."""
+    h = lib().orc_dense_build(op.h, P.h)
+    x = np.zeros(P.coarse_dim, np.complex128)
+    lib().orc_dense_solve(h, _ptr(_c(b)), _ptr(x))
+    lib().orc_dense_free(h)
+    return x
+
+
 def mr_smooth(op, x, b, niter):
     x = _c(x).copy(); b = _c(b)
     lib().orc_mr_smooth(op.h, _ptr(x), _ptr(b), niter)

@@ -77,6 +77,7 @@ SIGNATURES = [
     ("smg_fgmres_solve", _I, [_P, _P, _P, _P, _I, _I, _D, _P]),
     ("smg_dirac_apply_batched_dev", _I, [_P, _I, _I, _P, _P, _I, _P]),
     ("smg_coarse_apply_batched", _I, [_P, _I, _I, _P, _P]),
+    ("smg_coarse_dense", _I, [_P, _P]), ("smg_coarse_dense_solve", _I, [_P, _P, _P]),
     ("smg_coarse_apply_batched_dev", _I, [_P, _I, _I, _P, _P, _P]),
     ("smg_mg_precondition_batched", _I, [_P, _I, _I, _P, _P]),
     ("smg_gcr_solve_batched", _I, [_P, _P, _I, _I, _P, _P, _I, _D, _I, _D, _P, _P]),
@@ -380,6 +381,18 @@ class SparseCoarseOp:
         out = np.zeros_like(S)
         _chk(lib().smg_coarse_apply_batched(self.h, prec, S.shape[0], _p(S), _p(out)))
         return out
+
+    def dense(self):
+        """CoarseOp's dense matrix, returned as columns: out[j] = A_c e_j (src/coarse_op.h:18-37)."""
+        out = np.zeros(self.dim * self.dim, np.complex128)
+        _chk(lib().smg_coarse_dense(self.h, _p(out)))
+        return out.reshape(self.dim, self.dim)
+
+    def dense_solve(self, b):
+        """CoarseOp::solve (src/coarse_op.h:101-137) through the dense inverse."""
+        x = np.zeros(self.dim, np.complex128)
+        _chk(lib().smg_coarse_dense_solve(self.h, _p(_c(b)), _p(x)))
+        return x
 
     def solve(self, b, max_iter=200, cg_tol=1e-12):
         x = np.zeros(self.dim, np.complex128)

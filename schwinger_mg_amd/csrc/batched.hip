@@ -11,7 +11,15 @@
 
 namespace smg {
 
-constexpr int kRedGrid = 1024;   // workgroups of a reducing batched kernel
+// Workgroups of a reducing batched kernel: kRedGrid for up to 3 partials per RHS,
+// kRedGridMulti for the multi-dot (up to 32 per RHS).  The partial rows,
+// (nblk + nblk/32) * NC * N doubles, fit the 2M-double partial buffer, and the
+// 1 + nblk/32 completion counters fit the 1024-entry counter array.
+constexpr int kRedGrid = 8192;
+constexpr int kRedGridMulti = 1024;
+static_assert((kRedGrid + kRedGrid / 32) * 3 * 64 <= (1 << 21), "partials exceed the reduction buffer");
+static_assert((kRedGridMulti + kRedGridMulti / 32) * 32 * 32 <= (1 << 21), "partials exceed the reduction buffer");
+static_assert(1 + kRedGrid / 32 <= 1024, "completion counters exceed the counter array");
 
 __device__ __forceinline__ int lg2(int N) { return 31 - __clz(N); }
 
@@ -43,41 +51,39 @@ __device__ __forceinline__ void reduce_finish_b(double (&v)[NC], const FinB& f, 
     __syncthreads();
     const int tot = NC * N;
     for (int t = threadIdx.x; t < tot; t += kBlock)
-        f.part[(size_t)blk * tot + t] = (sm[0][t] + sm[1][t]) + (sm[2][t] + sm[3][t]);
-    __threadfence();
+        part_store(f.part + (size_t)blk * tot + t, (sm[0][t] + sm[1][t]) + (sm[2][t] + sm[3][t]));
+    stores_complete();
     __syncthreads();
     const int ngrp = (nblk + kGroup - 1) / kGroup, g = blk / kGroup;
     const int g0 = g * kGroup, gn = min(nblk - g0, kGroup);
-    if (threadIdx.x == 0) last = (atomicAdd(f.counter + 1 + g, 1u) == (unsigned)gn - 1u);
+    if (threadIdx.x == 0) last = (counter_arrive(f.counter + 1 + g) == (unsigned)gn - 1u);
     __syncthreads();
     if (!last) return;
-    __threadfence();
     double* grow = f.part + (size_t)nblk * tot;   // group rows follow the workgroup rows
     for (int t = threadIdx.x; t < tot; t += kBlock) {
         double r[kGroup];
 #pragma unroll
-        for (int i = 0; i < kGroup; i++) r[i] = i < gn ? f.part[(size_t)(g0 + i) * tot + t] : 0.0;
+        for (int i = 0; i < kGroup; i++) r[i] = i < gn ? part_load(f.part + (size_t)(g0 + i) * tot + t) : 0.0;
         double a = r[0];
 #pragma unroll
         for (int i = 1; i < kGroup; i++) a += r[i];
-        grow[(size_t)g * tot + t] = a;
+        part_store(grow + (size_t)g * tot + t, a);
     }
-    __threadfence();
+    stores_complete();
     __syncthreads();
     if (threadIdx.x == 0) {
-        f.counter[1 + g] = 0u;
-        last = (atomicAdd(f.counter, 1u) == (unsigned)ngrp - 1u);
+        counter_reset(f.counter + 1 + g);
+        last = (counter_arrive(f.counter) == (unsigned)ngrp - 1u);
     }
     __syncthreads();
     if (!last) return;
-    __threadfence();
     double* acc = &sm[0][0];   // tot <= NC * NMAX entries
     for (int t = threadIdx.x; t < tot; t += kBlock) {
         double r[kGroup];
         double a = 0.0;
         for (int i0 = 0; i0 < ngrp; i0 += kGroup) {
 #pragma unroll
-            for (int i = 0; i < kGroup; i++) r[i] = i0 + i < ngrp ? grow[(size_t)(i0 + i) * tot + t] : 0.0;
+            for (int i = 0; i < kGroup; i++) r[i] = i0 + i < ngrp ? part_load(grow + (size_t)(i0 + i) * tot + t) : 0.0;
 #pragma unroll
             for (int i = 0; i < kGroup; i++) a += r[i];
         }
@@ -91,11 +97,7 @@ __device__ __forceinline__ void reduce_finish_b(double (&v)[NC], const FinB& f, 
         for (int c = 0; c < NC; c++) w[c] = acc[c * N + j];
         fin_program(f.op, w, NC, f.out + (size_t)j * f.ostride, f.aux ? f.aux + (size_t)j * f.astride : nullptr);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        *f.counter = 0u;
-        __threadfence();
-    }
+    if (threadIdx.x == 0) counter_reset(f.counter);
 }
 
 static inline int b_grid(size_t items, int cap) {
@@ -267,22 +269,35 @@ __device__ __forceinline__ T real_form(cplx<T> a, int c, int cp) {
     return c == cp ? a.re : (c ? a.im : -a.im);
 }
 
-// One wave per coarse site b: out'[b] (2K x N) = sum_d A'_{b,d} (2K x 2K) B'_{nbr(b,d)} (2K x N);
-// RT = 2K/16 row tiles, NT = N/16 RHS tiles, K/2 MFMA steps of 4 real columns per direction.
+// Q consecutive coarse links loaded as one (or a few) wide loads
+template <typename TL, int Q>
+struct alignas(sizeof(TL) * Q > 16 ? 16 : sizeof(TL) * Q) LinkRun {
+    TL v[Q];
+};
+
+// One wave per coarse site b: out'[b] (2K x N) = sum_d A'_{b,d} (2K x 2K) B'_{nbr(b,d)} (2K x N).
+// The real-form K index of MFMA (q, c') in lane group hi is (kj = hi*Q + q, part c'),
+// Q = K/4, so a lane's Q links of one direction are contiguous (one wide load) and
+// each source entry it loads feeds both parts c'.  In FP32 with an even number of
+// 16-RHS tiles, tile t of lane column lo holds RHS 32(t/2) + 2 lo + t%2, so one
+// 16-byte load gives a lane the sources of two tiles; otherwise RHS 16t + lo.
+// Per site and direction: RT*Q*2*NT MFMAs (RT = K/8 row tiles).
 template <typename T, typename TL, int EPI, int K, int NT>
 __global__ __launch_bounds__(kBlock) void k_b_coarse_mfma(CoarseView<TL> C, cplx<T>* __restrict__ dst,
                                                           const cplx<T>* __restrict__ src,
                                                           const cplx<T>* __restrict__ bvec,
                                                           const cplx<T>* __restrict__ aux, FinB fin) {
-    constexpr int N = 16 * NT, RT = K / 8, KS = K / 2;
+    constexpr int N = 16 * NT, RT = K / 8, Q = K / 4;
+    constexpr bool PAIR = sizeof(T) == 4 && NT % 2 == 0;
+    constexpr int NL = PAIR ? NT / 2 : NT;   // source loads per (kj) per lane
+    constexpr int S = sizeof(T) == 4 ? 3 : 1;   // directions per pipeline stage
     using MT = MfmaTile<T>;
     using acc_t = typename MT::acc;
+    using Run = LinkRun<TL, Q>;
     const int lane = threadIdx.x & 63, lo = lane & 15, hi = lane >> 4;
     const int nx = C.nx, nsites = C.nx * C.ny;
     const int nw = kBlock / 64;
-    const T* srcr = reinterpret_cast<const T*>(src);
-    // A role: tile row lo -> (ki, c), column (kj = hi/2, c' = hi%2); B role: column j = lo, row (hi/2, hi%2)
-    const int a_ki = MT::row_ki(lo), a_c = MT::row_c(lo), a_kj = hi >> 1, a_cp = hi & 1;
+    const int a_ki = MT::row_ki(lo), a_c = MT::row_c(lo);
     double dacc[NT][3];
 #pragma unroll
     for (int t = 0; t < NT; t++) dacc[t][0] = dacc[t][1] = dacc[t][2] = 0.0;
@@ -295,48 +310,114 @@ __global__ __launch_bounds__(kBlock) void k_b_coarse_mfma(CoarseView<TL> C, cplx
         for (int r = 0; r < RT; r++)
 #pragma unroll
             for (int t = 0; t < NT; t++) acc[r][t] = acc_t{0, 0, 0, 0};
+        // software pipeline over the 9 directions in stages of S: the loads of stage
+        // i + 1 are issued before the MFMAs of stage i (sched_barrier keeps the order)
+        int nbr[9];
 #pragma unroll
         for (int d = 0; d < 9; d++) {
             const int cx = (d == 1 || d == 5 || d == 6) ? xp : (d == 2 || d == 7 || d == 8) ? xm : bx;
             const int cy = (d == 3 || d == 5 || d == 7) ? yp : (d == 4 || d == 6 || d == 8) ? ym : by;
-            const int nb = cx + cy * nx;
-            const TL* lb = C.link + ((size_t)b * 9 + d) * (K * K) + a_ki * K + a_kj;
-            const T* sb = srcr + ((size_t)nb * K * N + (size_t)(hi >> 1) * N + lo) * 2 + (hi & 1);
+            nbr[d] = cx + cy * nx;
+        }
+        Run lv[2][S][RT];
+        cplx<T> sv[2][S][Q][NT];   // sv[.][.][q][t] = src[nb][hi*Q + q][rhs(t, lo)]
+        auto load_stage = [&](int buf, int st0) {
 #pragma unroll
-            for (int s = 0; s < KS; s++) {
-                T av[RT], bv[NT];
+            for (int e = 0; e < S; e++) {
+                const int d = st0 + e, nb = nbr[d];
 #pragma unroll
-                for (int r = 0; r < RT; r++) av[r] = real_form(cvt<T>(load_c(lb + 8 * r * K + 2 * s)), a_c, a_cp);
+                for (int r = 0; r < RT; r++)
+                    lv[buf][e][r] = *reinterpret_cast<const Run*>(C.link + ((size_t)b * 9 + d) * (K * K) +
+                                                                  (8 * r + a_ki) * K + hi * Q);
 #pragma unroll
-                for (int t = 0; t < NT; t++) bv[t] = sb[((size_t)(2 * s) * N + 16 * t) * 2];
+                for (int q = 0; q < Q; q++) {
+                    const cplx<T>* row = src + ((size_t)nb * K + hi * Q + q) * N;
+#pragma unroll
+                    for (int u = 0; u < NL; u++) {
+                        if constexpr (PAIR) {
+                            const float4 w = *reinterpret_cast<const float4*>(row + 32 * u + 2 * lo);
+                            sv[buf][e][q][2 * u] = cplx<T>{w.x, w.y};
+                            sv[buf][e][q][2 * u + 1] = cplx<T>{w.z, w.w};
+                        } else {
+                            sv[buf][e][q][u] = row[16 * u + lo];
+                        }
+                    }
+                }
+            }
+        };
+        auto mma_stage = [&](int buf) {
+#pragma unroll
+            for (int e = 0; e < S; e++)
 #pragma unroll
                 for (int r = 0; r < RT; r++)
 #pragma unroll
-                    for (int t = 0; t < NT; t++) acc[r][t] = MT::mma(av[r], bv[t], acc[r][t]);
-            }
+                    for (int q = 0; q < Q; q++) {
+                        const cplx<T> L = cvt<T>(load_c(&lv[buf][e][r].v[q]));
+                        const T a0 = real_form(L, a_c, 0), a1 = real_form(L, a_c, 1);
+#pragma unroll
+                        for (int t = 0; t < NT; t++) {
+                            acc[r][t] = MT::mma(a0, sv[buf][e][q][t].re, acc[r][t]);
+                            acc[r][t] = MT::mma(a1, sv[buf][e][q][t].im, acc[r][t]);
+                        }
+                    }
+        };
+        load_stage(0, 0);
+#pragma unroll
+        for (int i = 0; i < 9 / S; i++) {
+            if (i + 1 < 9 / S) load_stage((i + 1) & 1, (i + 1) * S);
+            __builtin_amdgcn_sched_barrier(0);
+            mma_stage(i & 1);
         }
 #pragma unroll
         for (int r = 0; r < RT; r++)
 #pragma unroll
-            for (int t = 0; t < NT; t++)
+            for (int h = 0; h < 2; h++) {
+                const int ki = 8 * r + 2 * hi + h;
+                const size_t row = ((size_t)b * K + ki) * N;
 #pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const int ki = 8 * r + 2 * hi + h;
-                    const size_t idx = ((size_t)b * K + ki) * N + 16 * t + lo;
-                    cplx<T> o = {acc[r][t][2 * h], acc[r][t][2 * h + 1]};
-                    if (EPI == EPI_RESID) o = bvec[idx] - o;
-                    dst[idx] = o;
-                    if (EPI == EPI_DOTS) {
-                        const cplx<T> p = cdot(o, aux[idx]);
-                        dacc[t][0] += p.re;
-                        dacc[t][1] += p.im;
-                        dacc[t][2] += norm2(o);
+                for (int u = 0; u < NL; u++) {
+                    cplx<T> o[2];
+                    const int nt = PAIR ? 2 : 1;
+                    const size_t idx = row + (PAIR ? 32 * u + 2 * lo : 16 * u + lo);
+#pragma unroll
+                    for (int e = 0; e < nt; e++) {
+                        const int t = PAIR ? 2 * u + e : u;
+                        o[e] = cplx<T>{acc[r][t][2 * h], acc[r][t][2 * h + 1]};
+                    }
+                    if constexpr (PAIR) {
+                        if (EPI == EPI_RESID) {
+                            const float4 w = *reinterpret_cast<const float4*>(bvec + idx);
+                            o[0] = cplx<T>{w.x, w.y} - o[0];
+                            o[1] = cplx<T>{w.z, w.w} - o[1];
+                        }
+                        *reinterpret_cast<float4*>(dst + idx) = float4{o[0].re, o[0].im, o[1].re, o[1].im};
+                        if (EPI == EPI_DOTS) {
+                            const float4 w = *reinterpret_cast<const float4*>(aux + idx);
+                            const cplx<T> x0{w.x, w.y}, x1{w.z, w.w};
+                            const cplx<T> p0 = cdot(o[0], x0), p1 = cdot(o[1], x1);
+                            dacc[2 * u][0] += p0.re;
+                            dacc[2 * u][1] += p0.im;
+                            dacc[2 * u][2] += norm2(o[0]);
+                            dacc[2 * u + 1][0] += p1.re;
+                            dacc[2 * u + 1][1] += p1.im;
+                            dacc[2 * u + 1][2] += norm2(o[1]);
+                        }
+                    } else {
+                        if (EPI == EPI_RESID) o[0] = bvec[idx] - o[0];
+                        dst[idx] = o[0];
+                        if (EPI == EPI_DOTS) {
+                            const cplx<T> p = cdot(o[0], aux[idx]);
+                            dacc[u][0] += p.re;
+                            dacc[u][1] += p.im;
+                            dacc[u][2] += norm2(o[0]);
+                        }
                     }
                 }
+            }
     }
     if (EPI == EPI_DOTS) {
-        // lanes with equal lo hold partials of RHS 16t + lo: fold the four lane groups,
-        // then lane j < N carries RHS j into the per-RHS reduction
+        // lanes with equal lo hold partials of RHS rhs(t, lo): fold the four lane
+        // groups, then lane j < N fetches RHS j from lane lo(j), tile t(j)
 #pragma unroll
         for (int t = 0; t < NT; t++)
 #pragma unroll
@@ -344,12 +425,22 @@ __global__ __launch_bounds__(kBlock) void k_b_coarse_mfma(CoarseView<TL> C, cplx
                 dacc[t][c] += __shfl_xor(dacc[t][c], 16, 64);
                 dacc[t][c] += __shfl_xor(dacc[t][c], 32, 64);
             }
+        int tj, lj;
+        if constexpr (PAIR) {
+            tj = 2 * ((lane & 63) >> 5) + (lane & 1);
+            lj = (lane & 31) >> 1;
+        } else {
+            tj = (lane >> 4) & (NT - 1);
+            lj = lane & 15;
+        }
         double v[3] = {0.0, 0.0, 0.0};
 #pragma unroll
         for (int t = 0; t < NT; t++)
-            if (lane < N && (lane >> 4) == t)
 #pragma unroll
-                for (int c = 0; c < 3; c++) v[c] = dacc[t][c];
+            for (int c = 0; c < 3; c++) {
+                const double x = __shfl(dacc[t][c], lj, 64);
+                if (t == tj && lane < N) v[c] = x;
+            }
         reduce_finish_b<3, 64>(v, fin, N, gridDim.x, blockIdx.x);
     }
 }
@@ -604,7 +695,7 @@ template <typename T>
 void launch_b_multi_dot(const VecPtrs<T>& V, int nv, const cplx<T>* w, size_t n, const FinB& fin, int N,
                         hipStream_t st) {
     SMG_REQUIRE(nv >= 1 && nv <= 16 && N <= 32, "batched multi_dot: 1..16 vectors, N <= 32");
-    k_b_multi_dot<T><<<b_grid(n * N, 256), kBlock, 0, st>>>(V, nv, w, n * N, fin, N);
+    k_b_multi_dot<T><<<b_grid(n * N, kRedGridMulti), kBlock, 0, st>>>(V, nv, w, n * N, fin, N);
     SMG_LAUNCH_CHECK();
 }
 

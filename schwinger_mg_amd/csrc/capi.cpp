@@ -471,6 +471,43 @@ int smg_coarse_solve(smg_coarse* C, const double* b, double* x, int maxit, doubl
     });
 }
 
+// Dense A_c (CoarseOp, src/coarse_op.h:18-59): the Galerkin stencil expanded to
+// the reference's column layout mat[j][i] (M[j*n + i] = A_c[i][j])
+int smg_coarse_dense(smg_coarse* C, double* M) {
+    return guard([&] {
+        SMG_REQUIRE(C && M, "bad arguments");
+        const CoarseOperator& c = *C->C;
+        const size_t n = c.n();
+        SMG_REQUIRE(n <= 16384, "dense coarse operator: at most 16384 unknowns");
+        Field<double> d(n * n);
+        launch_zero<double>(d.p(), n * n, S());
+        launch_coarse_to_dense(CoarseView<c64>{c.l64.p(), c.nx, c.ny, c.k}, d.p(), n, S());
+        std::vector<c64> h(n * n);
+        down(reinterpret_cast<double*>(h.data()), d.p(), n * n * sizeof(c64));
+        c64* out = reinterpret_cast<c64*>(M);
+        for (size_t i = 0; i < n; i++)
+            for (size_t j = 0; j < n; j++) out[j * n + i] = h[i * n + j];
+    });
+}
+// CoarseOp::solve (src/coarse_op.h:101-137): x = A_c^-1 b through the dense inverse
+// (Gauss-Jordan without pivoting, valid for the Hermitian positive-definite Galerkin
+// projection of D^dag D)
+int smg_coarse_dense_solve(smg_coarse* C, const double* b, double* x) {
+    return guard([&] {
+        SMG_REQUIRE(C && b && x, "bad arguments");
+        const CoarseOperator& c = *C->C;
+        const size_t n = c.n();
+        SMG_REQUIRE(n <= 8192, "dense coarse solve: at most 8192 unknowns");
+        Field<double> inv(n * n), prow(n), pcol(n);
+        launch_zero<double>(inv.p(), n * n, S());
+        launch_coarse_to_dense(CoarseView<c64>{c.l64.p(), c.nx, c.ny, c.k}, inv.p(), n, S());
+        launch_dense_invert(inv.p(), n, prow.p(), pcol.p(), S());
+        Field<double> bb = up(b, n), xx(n);
+        launch_dense_matvec<double>(inv.p(), bb.p(), xx.p(), n, S());
+        downc(x, xx.p(), n);
+    });
+}
+
 // ------------------------------------------------------------------ smoothers
 template <typename Op>
 static void smooth_op(Op& op, int kind, double* x, const double* b, int niter, double omega) {

@@ -70,8 +70,8 @@ Context& Context::get() {
         ctx = new Context();
         SMG_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         ctx->part.alloc(size_t(1) << 24);  // 2M doubles of partial sums
-        ctx->counter.alloc(256);
-        SMG_HIP(hipMemset(ctx->counter.p, 0, 256));
+        ctx->counter.alloc(4096);   // 1024 completion counters
+        SMG_HIP(hipMemset(ctx->counter.p, 0, 4096));
         void* h = nullptr;
         SMG_HIP(hipHostMalloc(&h, 4096, hipHostMallocDefault));
         ctx->hostscal.p = h;

@@ -235,6 +235,29 @@ __device__ __forceinline__ void fin_program(int op, const double* v, int nc, dou
     }
 }
 
+// Cross-workgroup hand-off of reduction partials.  The partials, and the
+// completion counters, are written and read with device-scope (agent) atomic
+// accesses, which are coherent across the XCDs' L2 caches; nothing else in L2 is
+// written back or invalidated (a device-scope __threadfence() would do both, on
+// every workgroup, and evict the data the other workgroups are streaming).  A
+// wave's stores are complete once s_waitcnt returns, so every wave waits for its
+// own partial stores before the workgroup barrier that precedes the counter
+// increment; the workgroup that sees the last increment reads the partials with
+// device-scope loads.
+__device__ __forceinline__ void part_store(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double part_load(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void stores_complete() { __builtin_amdgcn_s_waitcnt(0); }
+__device__ __forceinline__ unsigned counter_arrive(unsigned* c) {
+    return __hip_atomic_fetch_add(c, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void counter_reset(unsigned* c) {
+    __hip_atomic_store(c, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // End of a reducing kernel: every thread holds its partial v[NC]; all threads of
 // every workgroup must call this.  nblk = total workgroups of the launch.
 template <int NC>
@@ -243,26 +266,24 @@ __device__ __forceinline__ void reduce_finish(double (&v)[NC], const Fin& f, int
     __shared__ int last;
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int c = 0; c < NC; c++) f.part[(size_t)blk * NC + c] = v[c];
-        __threadfence();
-        const unsigned prev = atomicAdd(f.counter, 1u);
+        for (int c = 0; c < NC; c++) part_store(f.part + (size_t)blk * NC + c, v[c]);
+        stores_complete();
+        const unsigned prev = counter_arrive(f.counter);
         last = (prev == (unsigned)nblk - 1u);
     }
     __syncthreads();
     if (!last) return;
-    __threadfence();
     double w[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) w[c] = 0.0;
     // fixed-order sum: thread t adds rows t, t + kBlock, ... ; then the block tree
     for (int i = threadIdx.x; i < nblk; i += kBlock)
 #pragma unroll
-        for (int c = 0; c < NC; c++) w[c] += __builtin_nontemporal_load(f.part + (size_t)i * NC + c);
+        for (int c = 0; c < NC; c++) w[c] += part_load(f.part + (size_t)i * NC + c);
     block_sum<NC>(w);
     if (threadIdx.x == 0) {
         fin_program(f.op, w, NC, f.out, f.aux);
-        *f.counter = 0u;
-        __threadfence();
+        counter_reset(f.counter);
     }
 }
 

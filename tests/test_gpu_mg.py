@@ -58,6 +58,23 @@ def test_aggregation_and_coarse_vs_oracle(smg, L, b, k):
         assert relerr(C2.link, C2o.links()) < 1e-11
 
 
+@pytest.mark.parametrize("L,b,k", [(16, 4, 4), (32, 4, 8)])
+def test_dense_coarse_op_vs_oracle(smg, L, b, k):
+    """CoarseOp (dense): the matrix equals the oracle's column-by-column Galerkin
+    build, and the dense solve equals its LU solve."""
+    banner(f"dense coarse op L={L} b={b} k={k}")
+    U = random_gauge(L, 0.4, 70 + k)
+    nulls = gaussian_vecs(k, 2 * L * L, 71)
+    Pg = smg.Prolongator(L, L, 2, b, b, k).build_from_vectors(nulls)
+    Po = O.Prolongator(L, L, 2, b, b, k).build(nulls.ravel())
+    D, Do = smg.DiracOp(L, U, 0.05, 1.0, 1.0, 0.0), O.Dirac(L, U, 0.05, 1.0, 1.0, 0.0)
+    C = smg.SparseCoarseOp.build(Pg, D)
+    want = O.dense_galerkin(Do.op(), Po)
+    assert relerr(C.dense(), want) < 1e-12
+    rhs = gaussian_vec(C.dim, 72)
+    assert relerr(C.dense_solve(rhs), O.dense_solve_galerkin(Do.op(), Po, rhs)) < 1e-10
+
+
 def test_coarse_solve_and_smoothers(smg):
     banner("coarse solve + smoothers L=16")
     L, b, k = 16, 4, 4
