@@ -1,0 +1,665 @@
+// schwinger_mg.hpp -- header-only C++17 re-statement of the reference's operator
+// API (cpviolator/Schwinger_MG src/types.h, lattice.h, gauge.h, dirac.h,
+// prolongator.h, coarse_op.h, smoother.h, multigrid.h, solvers.h, hmc.h) on top of
+// the C-ABI in smg.h.  Every computation runs on the GPU through libsmg.so;
+// vectors cross the boundary as the reference's host Vec (std::vector of
+// std::complex<double>).  Link with -lsmg.
+//
+// Differences from the reference, all at the boundary:
+//  * Names live in namespace schwinger_mg (`using namespace schwinger_mg;`
+//    makes a reference caller compile unchanged).
+//  * DiracOp copies the gauge links to the device when it is constructed; after
+//    changing gauge.U call DiracOp::update_gauge() (the reference reads the
+//    GaugeField through its reference on every apply).
+//  * The Krylov solvers, smoothers and coarse builds run the operator on the
+//    GPU, so they accept the OpApply / preconditioner objects this header
+//    creates (DiracOp::as_DdagD_op, SparseCoarseOp::as_op, MGHierarchy's
+//    as_precond) and throw std::invalid_argument for any other std::function.
+//  * A GPU failure throws std::runtime_error carrying smg_last_error() (the
+//    reference asserts on shape errors and has no other error path).
+#pragma once
+#include <array>
+#include <cmath>
+#include <complex>
+#include <functional>
+#include <memory>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "smg.h"
+
+namespace schwinger_mg {
+
+using cx = std::complex<double>;
+using Vec = std::vector<cx>;
+using RVec = std::vector<double>;
+using OpApply = std::function<void(const Vec& src, Vec& dst)>;
+
+namespace detail {
+inline void check(int rc) {
+    if (rc != 0) throw std::runtime_error(std::string("schwinger_mg: ") + smg_last_error());
+}
+inline const double* raw(const Vec& v) { return reinterpret_cast<const double*>(v.data()); }
+inline double* raw(Vec& v) { return reinterpret_cast<double*>(v.data()); }
+inline void fit(Vec& v, size_t n) {
+    if (v.size() != n) v.assign(n, cx(0.0, 0.0));
+}
+}  // namespace detail
+
+// random_vec, src/linalg.cpp:44-49 (host RNG stream identical to the reference's)
+inline Vec random_vec(int n, std::mt19937& rng) {
+    std::normal_distribution<double> dist(0.0, 1.0);
+    Vec v(n);
+    for (auto& x : v) {
+        const double re = dist(rng);
+        const double im = dist(rng);
+        x = cx(re, im);
+    }
+    return v;
+}
+
+// ------------------------------------------------------------------ Lattice, src/lattice.h
+struct Lattice {
+    int L, V, ndof, V_half;
+    std::vector<int> even_sites, odd_sites, eo_index;
+    explicit Lattice(int L_) : L(L_), V(L_ * L_), ndof(2 * L_ * L_), V_half(L_ * L_ / 2) { init_eo(); }
+    int idx(int x, int y) const { return ((x % L + L) % L) + L * ((y % L + L) % L); }
+    int xp(int s) const { return idx(s % L + 1, s / L); }
+    int xm(int s) const { return idx(s % L - 1, s / L); }
+    int yp(int s) const { return idx(s % L, s / L + 1); }
+    int ym(int s) const { return idx(s % L, s / L - 1); }
+    int parity(int s) const { return ((s % L) + (s / L)) % 2; }
+    void init_eo() {
+        even_sites.assign(V_half, 0);
+        odd_sites.assign(V_half, 0);
+        eo_index.assign(V, 0);
+        detail::check(smg_eo_maps(L, even_sites.data(), odd_sites.data(), eo_index.data()));
+    }
+};
+
+// ------------------------------------------------------------------ GaugeField, src/gauge.h
+struct GaugeField {
+    const Lattice& lat;
+    std::array<Vec, 2> U;
+    explicit GaugeField(const Lattice& lat_) : lat(lat_) {
+        U[0].assign(lat.V, cx(1.0, 0.0));
+        U[1].assign(lat.V, cx(1.0, 0.0));
+    }
+    void randomise(std::mt19937& rng, double width = M_PI) {
+        std::uniform_real_distribution<double> dist(-width, width);
+        for (int mu = 0; mu < 2; mu++)
+            for (int s = 0; s < lat.V; s++) U[mu][s] = std::exp(cx(0, dist(rng)));
+    }
+    cx plaq(int s) const {
+        return U[0][s] * U[1][lat.xp(s)] * std::conj(U[0][lat.yp(s)]) * std::conj(U[1][s]);
+    }
+    Vec links() const {   // U[mu*V + s], the C-ABI gauge layout
+        Vec u(2 * (size_t)lat.V);
+        std::copy(U[0].begin(), U[0].end(), u.begin());
+        std::copy(U[1].begin(), U[1].end(), u.begin() + lat.V);
+        return u;
+    }
+    double avg_plaq() const {
+        double out = 0.0;
+        const Vec u = links();
+        detail::check(smg_avg_plaq(lat.L, detail::raw(u), &out));
+        return out;
+    }
+};
+
+// ------------------------------------------------------------------ DiracOp, src/dirac.h:8-183
+struct DiracOp;
+struct DdagDApply {   // the OpApply of DiracOp::as_DdagD_op
+    const DiracOp* D;
+    void operator()(const Vec& s, Vec& d) const;
+};
+struct G5DApply {     // the OpApply of DiracOp::as_g5D_op
+    const DiracOp* D;
+    void operator()(const Vec& s, Vec& d) const;
+};
+
+struct DiracOp {
+    const Lattice& lat;
+    const GaugeField& gauge;
+    double mass, r, c_sw, mu_t;
+    RVec clover_field;   // F_01(s) (filled when c_sw != 0, as the reference)
+
+    DiracOp(const Lattice& lat_, const GaugeField& g_, double m_, double r_ = 1.0, double csw_ = 0.0,
+            double mu_t_ = 0.0)
+        : lat(lat_), gauge(g_), mass(m_), r(r_), c_sw(csw_), mu_t(mu_t_) {
+        const Vec u = gauge.links();
+        smg_dirac* h = nullptr;
+        detail::check(smg_dirac_create(&h, lat.L, detail::raw(u), mass, r, c_sw, mu_t));
+        h_.reset(h, [](smg_dirac* p) { smg_dirac_destroy(p); });
+        fetch_clover();
+    }
+    // re-read gauge.U (and recompute the clover field) after the caller changed it
+    void update_gauge() {
+        const Vec u = gauge.links();
+        detail::check(smg_dirac_set_gauge(h_.get(), detail::raw(u)));
+        fetch_clover();
+    }
+    void apply(const Vec& src, Vec& dst) const { run(SMG_D, src, dst); }
+    void apply_dag(const Vec& src, Vec& dst) const { run(SMG_DDAG, src, dst); }
+    void apply_DdagD(const Vec& src, Vec& dst) const { run(SMG_DDAGD, src, dst); }
+    void apply_g5D(const Vec& src, Vec& dst) const { run(SMG_G5D, src, dst); }
+    void apply_delta_D(const Vec& src, Vec& dst, const std::array<RVec, 2>& pi, double dt) const {
+        detail::fit(dst, lat.ndof);
+        RVec p(2 * (size_t)lat.V);
+        std::copy(pi[0].begin(), pi[0].end(), p.begin());
+        std::copy(pi[1].begin(), pi[1].end(), p.begin() + lat.V);
+        detail::check(smg_dirac_delta(h_.get(), detail::raw(src), detail::raw(dst), p.data(), dt));
+    }
+    OpApply as_DdagD_op() const { return DdagDApply{this}; }
+    OpApply as_g5D_op() const { return G5DApply{this}; }
+    smg_dirac* handle() const { return h_.get(); }
+
+   private:
+    std::shared_ptr<smg_dirac> h_;
+    void run(int kind, const Vec& src, Vec& dst) const {
+        detail::fit(dst, lat.ndof);
+        detail::check(smg_dirac_apply(h_.get(), kind, detail::raw(src), detail::raw(dst)));
+    }
+    void fetch_clover() {
+        if (c_sw == 0.0) return;
+        clover_field.assign(lat.V, 0.0);
+        detail::check(smg_dirac_clover(h_.get(), clover_field.data()));
+    }
+};
+inline void DdagDApply::operator()(const Vec& s, Vec& d) const { D->apply_DdagD(s, d); }
+inline void G5DApply::operator()(const Vec& s, Vec& d) const { D->apply_g5D(s, d); }
+
+// ------------------------------------------------------------------ EvenOddDiracOp, src/dirac.h:190-553
+struct EvenOddDiracOp {
+    const DiracOp& D;
+    const Lattice& lat;
+    explicit EvenOddDiracOp(const DiracOp& D_) : D(D_), lat(D_.lat) {}
+    void apply_eo(const Vec& src_e, Vec& dst_o) const { run(SMG_EO_HOP_EO, src_e, dst_o); }
+    void apply_oe(const Vec& src_o, Vec& dst_e) const { run(SMG_EO_HOP_OE, src_o, dst_e); }
+    void apply_ee_inv(const Vec& src, Vec& dst) const { run(SMG_EO_EE_INV, src, dst); }
+    void apply_oo_inv(const Vec& src, Vec& dst) const { run(SMG_EO_OO_INV, src, dst); }
+    void apply_oo(const Vec& src, Vec& dst) const { run(SMG_EO_OO, src, dst); }
+    void apply_schur(const Vec& src_o, Vec& dst_o) const { run(SMG_EO_SCHUR, src_o, dst_o); }
+    void apply_schur_dag(const Vec& src_o, Vec& dst_o) const { run(SMG_EO_SCHUR_DAG, src_o, dst_o); }
+    void apply_schur_dag_schur(const Vec& src_o, Vec& dst_o) const { run(SMG_EO_SCHUR_DAG_SCHUR, src_o, dst_o); }
+    Vec scatter(const Vec& half_e, const Vec& half_o) const {
+        Vec full(lat.ndof, 0.0);
+        for (int i = 0; i < lat.V_half; i++) {
+            const int se = lat.even_sites[i], so = lat.odd_sites[i];
+            full[2 * se] = half_e[2 * i];
+            full[2 * se + 1] = half_e[2 * i + 1];
+            full[2 * so] = half_o[2 * i];
+            full[2 * so + 1] = half_o[2 * i + 1];
+        }
+        return full;
+    }
+    Vec gather_even(const Vec& full) const { return gather(full, lat.even_sites); }
+    Vec gather_odd(const Vec& full) const { return gather(full, lat.odd_sites); }
+    Vec generate_pseudofermion_eo(std::mt19937& rng) const {
+        const Vec eta_o = random_vec(2 * lat.V_half, rng);
+        Vec phi_o(2 * (size_t)lat.V_half);
+        apply_schur_dag(eta_o, phi_o);
+        return phi_o;
+    }
+    Vec reconstruct_full(const Vec& x_o) const {
+        Vec out(lat.ndof);
+        detail::check(smg_eo_reconstruct(D.handle(), detail::raw(x_o), nullptr, detail::raw(out)));
+        return out;
+    }
+    Vec reconstruct(const Vec& x_o, const Vec& phi) const {
+        Vec out(lat.ndof);
+        detail::check(smg_eo_reconstruct(D.handle(), detail::raw(x_o), detail::raw(phi), detail::raw(out)));
+        return out;
+    }
+    double log_det_ee() const {
+        double out = 0.0;
+        detail::check(smg_eo_logdet(D.handle(), &out));
+        return out;
+    }
+
+   private:
+    void run(int kind, const Vec& src, Vec& dst) const {
+        detail::fit(dst, 2 * (size_t)lat.V_half);
+        detail::check(smg_eo_apply(D.handle(), kind, detail::raw(src), detail::raw(dst)));
+    }
+    Vec gather(const Vec& full, const std::vector<int>& sites) const {
+        Vec half(2 * (size_t)lat.V_half);
+        for (int i = 0; i < lat.V_half; i++) {
+            half[2 * i] = full[2 * sites[i]];
+            half[2 * i + 1] = full[2 * sites[i] + 1];
+        }
+        return half;
+    }
+};
+
+// ------------------------------------------------------------------ Prolongator, src/prolongator.h:11-141
+struct Prolongator {
+    const Lattice& lat;
+    int bx, by, k_vec, nbx, nby, nblocks, coarse_dim;
+    Prolongator(const Lattice& lat_, int bx_, int by_, int k_)
+        : lat(lat_), bx(bx_), by(by_), k_vec(k_), nbx(lat_.L / bx_), nby(lat_.L / by_) {
+        if (lat.L % bx != 0 || lat.L % by != 0) throw std::invalid_argument("Prolongator: L % block != 0");
+        nblocks = nbx * nby;
+        coarse_dim = nblocks * k_vec;
+        smg_prolongator* h = nullptr;
+        detail::check(smg_prolongator_create(&h, lat.L, lat.L, 2, bx, by, k_vec));
+        h_.reset(h, [](smg_prolongator* p) { smg_prolongator_destroy(p); });
+    }
+    void build_from_vectors(const std::vector<Vec>& null_vecs) {
+        if ((int)null_vecs.size() < k_vec) throw std::invalid_argument("build_from_vectors: too few vectors");
+        Vec all((size_t)k_vec * lat.ndof);
+        for (int i = 0; i < k_vec; i++) std::copy(null_vecs[i].begin(), null_vecs[i].end(), all.begin() + (size_t)i * lat.ndof);
+        detail::check(smg_prolongator_build(h_.get(), detail::raw(all)));
+    }
+    Vec prolong(const Vec& coarse) const {
+        Vec fine(lat.ndof);
+        detail::check(smg_prolong(h_.get(), detail::raw(coarse), detail::raw(fine)));
+        return fine;
+    }
+    Vec restrict_vec(const Vec& fine) const {
+        Vec coarse(coarse_dim);
+        detail::check(smg_restrict(h_.get(), detail::raw(fine), detail::raw(coarse)));
+        return coarse;
+    }
+    // block_vecs[b][k] (length 2*bx*by), src/prolongator.h:23
+    std::vector<std::vector<Vec>> block_vecs() const {
+        const size_t bs = 2 * (size_t)bx * by;
+        Vec all((size_t)nblocks * k_vec * bs);
+        detail::check(smg_prolongator_get(h_.get(), detail::raw(all)));
+        std::vector<std::vector<Vec>> out(nblocks, std::vector<Vec>(k_vec));
+        for (int b = 0; b < nblocks; b++)
+            for (int k = 0; k < k_vec; k++)
+                out[b][k].assign(all.begin() + ((size_t)b * k_vec + k) * bs, all.begin() + ((size_t)b * k_vec + k + 1) * bs);
+        return out;
+    }
+    smg_prolongator* handle() const { return h_.get(); }
+
+   private:
+    std::shared_ptr<smg_prolongator> h_;
+};
+
+// ------------------------------------------------------------------ coarse operators, src/coarse_op.h
+struct SparseCoarseOp;
+struct CoarseApply {   // the OpApply of SparseCoarseOp::as_op
+    const SparseCoarseOp* C;
+    void operator()(const Vec& s, Vec& d) const;
+};
+
+// SparseCoarseOp, src/coarse_op.h:154-302 (no deflation)
+struct SparseCoarseOp {
+    int nbx = 0, nby = 0, nblocks = 0, k_vec = 0, dim = 0;
+    // Galerkin A_c = P^dag A P; A must be DiracOp::as_DdagD_op()
+    void build(const Prolongator& P, const OpApply& A, int fine_dim) {
+        const DdagDApply* f = A.target<DdagDApply>();
+        if (!f) throw std::invalid_argument("SparseCoarseOp::build: A must be DiracOp::as_DdagD_op()");
+        if (fine_dim != P.lat.ndof) throw std::invalid_argument("SparseCoarseOp::build: fine_dim mismatch");
+        smg_coarse* h = nullptr;
+        detail::check(smg_coarse_create_galerkin(&h, f->D->handle(), P.handle()));
+        h_.reset(h, [](smg_coarse* p) { smg_coarse_destroy(p); });
+        nbx = P.nbx;
+        nby = P.nby;
+        nblocks = P.nblocks;
+        k_vec = P.k_vec;
+        dim = nblocks * k_vec;
+    }
+    void apply_to(const Vec& src, Vec& dst) const {
+        dst.assign(dim, cx(0.0, 0.0));
+        detail::check(smg_coarse_apply(h_.get(), SMG_F64, detail::raw(src), detail::raw(dst)));
+    }
+    OpApply as_op() const { return CoarseApply{this}; }
+    Vec solve(const Vec& b, int max_iter = 200, double cg_tol = 1e-12) const {
+        Vec x(dim);
+        int it = 0;
+        detail::check(smg_coarse_solve(h_.get(), detail::raw(b), detail::raw(x), max_iter, cg_tol, &it));
+        return x;
+    }
+    // link[b][d][ki*k + kj] flattened, src/coarse_op.h:159-166
+    Vec links() const {
+        Vec out((size_t)nblocks * 9 * k_vec * k_vec);
+        detail::check(smg_coarse_get_links(h_.get(), detail::raw(out)));
+        return out;
+    }
+    smg_coarse* handle() const { return h_.get(); }
+
+   private:
+    std::shared_ptr<smg_coarse> h_;
+};
+inline void CoarseApply::operator()(const Vec& s, Vec& d) const { C->apply_to(s, d); }
+
+// CoarseOp (dense), src/coarse_op.h:18-137: mat holds the columns of A_c
+struct CoarseOp {
+    int dim = 0;
+    std::vector<Vec> mat;
+    void build(const DiracOp& D, Prolongator& P) {
+        sparse_.build(P, D.as_DdagD_op(), D.lat.ndof);
+        dim = sparse_.dim;
+        Vec all((size_t)dim * dim);
+        detail::check(smg_coarse_dense(sparse_.handle(), detail::raw(all)));
+        mat.assign(dim, Vec());
+        for (int j = 0; j < dim; j++) mat[j].assign(all.begin() + (size_t)j * dim, all.begin() + (size_t)(j + 1) * dim);
+    }
+    Vec apply(const Vec& x) const {
+        Vec y;
+        sparse_.apply_to(x, y);
+        return y;
+    }
+    void apply_to(const Vec& src, Vec& dst) const { dst = apply(src); }
+    Vec solve(const Vec& b) const {
+        Vec x(dim);
+        detail::check(smg_coarse_dense_solve(sparse_.handle(), detail::raw(b), detail::raw(x)));
+        return x;
+    }
+
+   private:
+    SparseCoarseOp sparse_;
+};
+
+// ------------------------------------------------------------------ smoothers, src/smoother.h
+namespace detail {
+inline void smooth(const OpApply& A, int kind, Vec& x, const Vec& b, int niter, double omega, const char* who) {
+    if (const DdagDApply* f = A.target<DdagDApply>())
+        check(smg_smooth_dirac(f->D->handle(), kind, raw(x), raw(b), niter, omega));
+    else if (const CoarseApply* c = A.target<CoarseApply>())
+        check(smg_smooth_coarse(c->C->handle(), kind, raw(x), raw(b), niter, omega));
+    else
+        throw std::invalid_argument(std::string(who) + ": A must be DiracOp::as_DdagD_op() or SparseCoarseOp::as_op()");
+}
+}  // namespace detail
+inline void mr_smooth(const DiracOp& D, Vec& x, const Vec& b, int niter) {
+    detail::check(smg_smooth_dirac(D.handle(), SMG_SMOOTH_MR, detail::raw(x), detail::raw(b), niter, 0.0));
+}
+inline void mr_smooth_op(const OpApply& A, Vec& x, const Vec& b, int niter) {
+    detail::smooth(A, SMG_SMOOTH_MR, x, b, niter, 0.0, "mr_smooth_op");
+}
+inline void richardson_smooth_op(const OpApply& A, Vec& x, const Vec& b, int niter, double omega) {
+    detail::smooth(A, SMG_SMOOTH_RICHARDSON, x, b, niter, omega, "richardson_smooth_op");
+}
+// power iteration from mt19937(12345) as src/smoother.cpp:43-59 (D^dag D only)
+inline double estimate_lambda_max(const OpApply& A, int n, int niter = 20) {
+    const DdagDApply* f = A.target<DdagDApply>();
+    if (!f) throw std::invalid_argument("estimate_lambda_max: A must be DiracOp::as_DdagD_op()");
+    std::mt19937 rng_local(12345);
+    const Vec v0 = random_vec(n, rng_local);
+    double out = 0.0;
+    detail::check(smg_lambda_max_dirac(f->D->handle(), detail::raw(v0), niter, &out));
+    return out;
+}
+
+// ------------------------------------------------------------------ multigrid, src/multigrid.h
+struct MGHierarchy;
+struct MGPrecond {   // the preconditioner of MGHierarchy::as_precond
+    MGHierarchy* mg;
+    Vec operator()(const Vec& b) const;
+};
+
+struct MGHierarchy {
+    bool w_cycle = true;
+    std::vector<int> dims;   // per level
+    Vec precondition(const Vec& b) {
+        Vec x(dims.at(0));
+        detail::check(smg_mg_precondition(h_.get(), SMG_F64, detail::raw(b), detail::raw(x)));
+        return x;
+    }
+    // src/multigrid.cpp:146-152: Richardson omega = damping / lambda_max, 30 power
+    // iterations from mt19937(12345) per non-coarsest level
+    void set_symmetric(double damping = 0.8) {
+        Vec starts;
+        for (size_t l = 0; l + 1 < dims.size(); l++) {
+            std::mt19937 rng_local(12345);
+            const Vec v = random_vec(dims[l], rng_local);
+            starts.insert(starts.end(), v.begin(), v.end());
+        }
+        detail::check(smg_mg_set_symmetric(h_.get(), damping, detail::raw(starts)));
+    }
+    std::function<Vec(const Vec&)> as_precond() { return MGPrecond{this}; }
+    smg_mg* handle() const { return h_.get(); }
+    std::shared_ptr<smg_mg> h_;
+};
+inline Vec MGPrecond::operator()(const Vec& b) const { return mg->precondition(b); }
+
+// build_mg_hierarchy, src/multigrid.cpp:595-687.  The null-space start vectors are
+// drawn from rng in the reference's order (k_null vectors per level, level 0
+// first).  Levels >= 1 aggregate coarse_block_agg consecutive coarse dofs, which
+// on the coarse lattice is an (agg/k_null) x 1 block; the coarsest level is
+// solved exactly (dense inverse when its dimension is <= 8192, else CG to 1e-12).
+inline MGHierarchy build_mg_hierarchy(const DiracOp& D, int n_levels, int block_size, int k_null,
+                                      int coarse_block_agg, int null_iters, std::mt19937& rng, bool w_cycle = true,
+                                      int pre_smooth = 3, int post_smooth = 3, bool verbose = true,
+                                      const std::vector<Vec>* warm_start = nullptr) {
+    (void)verbose;
+    if (warm_start) throw std::invalid_argument("build_mg_hierarchy: warm starts are not supported");
+    if (n_levels < 2 || n_levels > 5) throw std::invalid_argument("build_mg_hierarchy: 2..5 levels");
+    if (coarse_block_agg <= 0) coarse_block_agg = 4 * k_null;
+    smg_mg_params p{};
+    p.n_levels = n_levels;
+    p.null_iters = null_iters;
+    p.pre_smooth = pre_smooth;
+    p.post_smooth = post_smooth;
+    p.w_cycle = w_cycle ? 1 : 0;
+    p.coarse_maxit = 200;
+    p.coarse_tol = 1e-12;
+    MGHierarchy mg;
+    mg.w_cycle = w_cycle;
+    mg.dims.push_back(D.lat.ndof);
+    int nx = D.lat.L;
+    for (int l = 0; l + 1 < n_levels; l++) {
+        int bx = block_size, by = block_size;
+        if (l > 0) {
+            const int prev = mg.dims.back();
+            int blk = coarse_block_agg;
+            while (prev % blk != 0 && blk > 1) blk--;
+            if (blk % k_null != 0 || nx % (blk / k_null) != 0)
+                throw std::invalid_argument("build_mg_hierarchy: coarse_block_agg must be a multiple of k_null "
+                                            "dividing the coarse lattice row");
+            bx = blk / k_null;
+            by = 1;
+        }
+        p.bx[l] = bx;
+        p.by[l] = by;
+        p.k[l] = k_null;
+        if (nx % bx != 0) throw std::invalid_argument("build_mg_hierarchy: block does not divide the lattice");
+        const int ny = (l == 0 ? D.lat.L : mg.dims.back() / k_null / nx) / by;
+        nx /= bx;
+        mg.dims.push_back(nx * ny * k_null);
+    }
+    p.coarse_solver = mg.dims.back() <= 8192 ? 1 : 0;
+    Vec starts;
+    for (int l = 0; l + 1 < n_levels; l++)
+        for (int i = 0; i < k_null; i++) {
+            const Vec v = random_vec(mg.dims[l], rng);
+            starts.insert(starts.end(), v.begin(), v.end());
+        }
+    smg_mg* h = nullptr;
+    detail::check(smg_mg_setup(&h, D.handle(), &p, detail::raw(starts)));
+    mg.h_.reset(h, [](smg_mg* q) { smg_mg_destroy(q); });
+    return mg;
+}
+
+// ------------------------------------------------------------------ Krylov solvers, src/solvers.h
+struct CGResult {
+    Vec solution;
+    int iterations;
+    double final_residual;
+};
+using FCGResult = CGResult;
+struct RitzPair {
+    double value;
+    Vec vector;
+};
+struct FGMRESResult {
+    Vec solution;
+    int iterations;
+    double final_residual;
+    std::vector<RitzPair> ritz_pairs;   // not harvested on the GPU path
+};
+
+namespace detail {
+inline const DiracOp& dirac_of(const OpApply& A, const char* who) {
+    const DdagDApply* f = A.target<DdagDApply>();
+    if (!f) throw std::invalid_argument(std::string(who) + ": A must be DiracOp::as_DdagD_op()");
+    return *f->D;
+}
+inline MGHierarchy* mg_of(const std::function<Vec(const Vec&)>& M, const char* who) {
+    const MGPrecond* p = M.target<MGPrecond>();
+    if (!p) throw std::invalid_argument(std::string(who) + ": the preconditioner must be MGHierarchy::as_precond()");
+    return p->mg;
+}
+inline CGResult cg(const OpApply& A, const Vec& rhs, const Vec* x0, MGHierarchy* mg, int max_iter, double tol,
+                   const char* who) {
+    const DiracOp& D = dirac_of(A, who);
+    CGResult r{Vec(D.lat.ndof), 0, 0.0};
+    smg_solve_info info{};
+    check(smg_cg_solve(D.handle(), SMG_OP_DDAGD, mg ? mg->handle() : nullptr, raw(rhs), x0 ? raw(*x0) : nullptr,
+                       raw(r.solution), max_iter, tol, &info));
+    r.iterations = info.iterations;
+    r.final_residual = info.final_residual;
+    return r;
+}
+}  // namespace detail
+
+// cg_solve family, src/solvers.cpp:16-96
+inline CGResult cg_solve(const OpApply& A, int n, const Vec& rhs, int max_iter, double tol) {
+    (void)n;
+    return detail::cg(A, rhs, nullptr, nullptr, max_iter, tol, "cg_solve");
+}
+inline CGResult cg_solve_x0(const OpApply& A, int n, const Vec& rhs, const Vec& x0, int max_iter, double tol) {
+    (void)n;
+    return detail::cg(A, rhs, &x0, nullptr, max_iter, tol, "cg_solve_x0");
+}
+inline CGResult cg_solve_precond(const OpApply& A, int n, const Vec& rhs,
+                                 const std::function<Vec(const Vec&)>& precond, int max_iter, double tol) {
+    (void)n;
+    return detail::cg(A, rhs, nullptr, detail::mg_of(precond, "cg_solve_precond"), max_iter, tol, "cg_solve_precond");
+}
+inline CGResult cg_solve_x0_precond(const OpApply& A, int n, const Vec& rhs, const Vec& x0,
+                                    const std::function<Vec(const Vec&)>& precond, int max_iter, double tol) {
+    (void)n;
+    return detail::cg(A, rhs, &x0, detail::mg_of(precond, "cg_solve_x0_precond"), max_iter, tol,
+                      "cg_solve_x0_precond");
+}
+
+enum class Precision { FP64 = SMG_F64, Mixed = SMG_F32 };
+
+// fcg_solve / fcg_solve_mg, src/solvers.cpp:223-300,803-811 (the reference's
+// projection coefficient, src/solvers.cpp:261, FP64)
+inline FCGResult fcg_solve(const OpApply& A, int n, const Vec& rhs, const std::function<Vec(const Vec&)>& precond,
+                           int max_iter, double tol, int trunc = 30) {
+    (void)n;
+    const DiracOp& D = detail::dirac_of(A, "fcg_solve");
+    MGHierarchy* mg = detail::mg_of(precond, "fcg_solve");
+    FCGResult r{Vec(D.lat.ndof), 0, 0.0};
+    smg_solve_info info{};
+    detail::check(smg_gcr_solve(D.handle(), mg->handle(), SMG_F64, detail::raw(rhs), detail::raw(r.solution),
+                                max_iter, tol, trunc, 1, 1e-5, &info));
+    r.iterations = info.iterations;
+    r.final_residual = info.final_residual;
+    return r;
+}
+inline FCGResult fcg_solve_mg(const OpApply& A, int n, const Vec& rhs, MGHierarchy& mg, int max_iter, double tol,
+                              int truncation = 20) {
+    return fcg_solve(A, n, rhs, mg.as_precond(), max_iter, tol, truncation);
+}
+// GCR with the textbook projection, FP64 or mixed precision (FP64 refinement of
+// the true residual around FP32 GCR and an FP32 cycle); final_residual is the
+// FP64 true residual.  No reference counterpart.
+inline FCGResult gcr_solve_mg(const OpApply& A, int n, const Vec& rhs, MGHierarchy& mg, int max_iter, double tol,
+                              int trunc = 10, Precision prec = Precision::FP64, double inner_tol = 1e-5) {
+    (void)n;
+    const DiracOp& D = detail::dirac_of(A, "gcr_solve_mg");
+    FCGResult r{Vec(D.lat.ndof), 0, 0.0};
+    smg_solve_info info{};
+    detail::check(smg_gcr_solve(D.handle(), mg.handle(), (int)prec, detail::raw(rhs), detail::raw(r.solution),
+                                max_iter, tol, trunc, 0, inner_tol, &info));
+    r.iterations = info.iterations;
+    r.final_residual = info.true_residual;
+    return r;
+}
+// fgmres_solve_generic / fgmres_solve_mg, src/solvers.cpp:555-811 (no Ritz harvest)
+inline FGMRESResult fgmres_solve_generic(const OpApply& A, int n, const Vec& rhs,
+                                         const std::function<Vec(const Vec&)>& precond, int restart, int max_iter,
+                                         double tol, int n_ritz_harvest = 4) {
+    (void)n;
+    (void)n_ritz_harvest;
+    const DiracOp& D = detail::dirac_of(A, "fgmres_solve_generic");
+    MGHierarchy* mg = detail::mg_of(precond, "fgmres_solve_generic");
+    FGMRESResult r{Vec(D.lat.ndof), 0, 0.0, {}};
+    smg_solve_info info{};
+    detail::check(smg_fgmres_solve(D.handle(), mg->handle(), detail::raw(rhs), detail::raw(r.solution), restart,
+                                   max_iter, tol, &info));
+    r.iterations = info.iterations;
+    r.final_residual = info.final_residual;
+    return r;
+}
+inline FGMRESResult fgmres_solve_mg(const OpApply& A, int n, const Vec& rhs, MGHierarchy& mg, int restart,
+                                    int max_iter, double tol, int n_ritz_harvest = 4) {
+    return fgmres_solve_generic(A, n, rhs, mg.as_precond(), restart, max_iter, tol, n_ritz_harvest);
+}
+
+// ------------------------------------------------------------------ actions and forces, src/hmc.h
+namespace detail {
+inline void to_force(const RVec& F, int V, std::array<RVec, 2>& force) {
+    force[0].assign(F.begin(), F.begin() + V);
+    force[1].assign(F.begin() + V, F.begin() + 2 * (size_t)V);
+}
+}  // namespace detail
+inline double gauge_action(const GaugeField& g, double beta) {
+    double out = 0.0;
+    const Vec u = g.links();
+    detail::check(smg_gauge_action(g.lat.L, detail::raw(u), beta, &out));
+    return out;
+}
+inline void gauge_force(const GaugeField& g, double beta, std::array<RVec, 2>& force) {
+    RVec F(2 * (size_t)g.lat.V);
+    const Vec u = g.links();
+    detail::check(smg_gauge_force(g.lat.L, detail::raw(u), beta, F.data()));
+    detail::to_force(F, g.lat.V, force);
+}
+struct FermionActionResult {
+    double action;
+    Vec X;
+    int cg_iters;
+};
+inline FermionActionResult fermion_action(const DiracOp& D, const Vec& phi, int max_iter, double tol) {
+    FermionActionResult r{0.0, Vec(D.lat.ndof), 0};
+    detail::check(smg_fermion_action(D.handle(), detail::raw(phi), max_iter, tol, &r.action, detail::raw(r.X),
+                                     &r.cg_iters));
+    return r;
+}
+// F = -dS_f/dtheta, src/hmc_forces.cpp:134-139 (chi = D X kept on the device)
+inline void fermion_force(const DiracOp& D, const Vec& X, std::array<RVec, 2>& force) {
+    RVec F(2 * (size_t)D.lat.V);
+    detail::check(smg_fermion_force(D.handle(), detail::raw(X), F.data()));
+    detail::to_force(F, D.lat.V, force);
+}
+inline void fermion_force_bilinear(const DiracOp& D, const Vec& chi, const Vec& X, std::array<RVec, 2>& force) {
+    RVec F(2 * (size_t)D.lat.V);
+    detail::check(smg_force_backward(D.handle(), detail::raw(chi), detail::raw(X), F.data()));
+    detail::to_force(F, D.lat.V, force);
+}
+inline void hopping_force_bilinear(const DiracOp& D, const Vec& chi, const Vec& X, std::array<RVec, 2>& force) {
+    RVec F(2 * (size_t)D.lat.V);
+    detail::check(smg_hopping_force(D.handle(), detail::raw(chi), detail::raw(X), F.data()));
+    detail::to_force(F, D.lat.V, force);
+}
+inline void logdet_ee_force(const DiracOp& D, std::array<RVec, 2>& force) {
+    RVec F(2 * (size_t)D.lat.V);
+    detail::check(smg_logdet_force(D.handle(), F.data()));
+    detail::to_force(F, D.lat.V, force);
+}
+inline void eo_fermion_force(const DiracOp& D, const EvenOddDiracOp& eoD, const Vec& x_o, const Vec& y_o,
+                             std::array<RVec, 2>& force) {
+    (void)eoD;
+    RVec F(2 * (size_t)D.lat.V);
+    detail::check(smg_eo_fermion_force(D.handle(), detail::raw(x_o), detail::raw(y_o), F.data()));
+    detail::to_force(F, D.lat.V, force);
+}
+// generate_pseudofermion, src/hmc_trajectory.cpp: phi = D^dag eta, eta from rng
+inline void generate_pseudofermion(const DiracOp& D, std::mt19937& rng, Vec& phi) {
+    const Vec eta = random_vec(D.lat.ndof, rng);
+    D.apply_dag(eta, phi);
+}
+
+}  // namespace schwinger_mg

@@ -263,8 +263,9 @@ int smg_hopping_force(smg_dirac* D, const double* chi, const double* X, double* 
 int smg_logdet_force(smg_dirac* D, double* F);
 /* eo_fermion_force, src/hmc_forces.cpp:830-867 (x_o, y_o half-lattice) */
 int smg_eo_fermion_force(smg_dirac* D, const double* x_o, const double* y_o, double* F);
-/* fermion_action, src/hmc_forces.cpp:113-119: S = Re phi^dag (D^dag D)^-1 phi via CG */
-int smg_fermion_action(smg_dirac* D, const double* phi, int maxit, double tol, double* S, double* X);
+/* fermion_action, src/hmc_forces.cpp:113-119: S = Re phi^dag (D^dag D)^-1 phi via CG;
+ * X (optional) receives (D^dag D)^-1 phi, iters (optional) the CG iterations */
+int smg_fermion_action(smg_dirac* D, const double* phi, int maxit, double tol, double* S, double* X, int* iters);
 
 #ifdef __cplusplus
 }

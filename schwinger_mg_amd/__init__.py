@@ -85,7 +85,7 @@ SIGNATURES = [
     ("smg_force_forward", _I, [_P, _P, _P]), ("smg_force_backward", _I, [_P, _P, _P, _P]),
     ("smg_fermion_force", _I, [_P, _P, _P]), ("smg_fermion_force_dev", _I, [_P, _P, _P, _P]),
     ("smg_hopping_force", _I, [_P, _P, _P, _P]), ("smg_logdet_force", _I, [_P, _P]),
-    ("smg_eo_fermion_force", _I, [_P, _P, _P, _P]), ("smg_fermion_action", _I, [_P, _P, _I, _D, _P, _P]),
+    ("smg_eo_fermion_force", _I, [_P, _P, _P, _P]), ("smg_fermion_action", _I, [_P, _P, _I, _D, _P, _P, _P]),
 ]
 
 _lib = None
@@ -631,5 +631,5 @@ def fermion_action(D, phi, max_iter, tol):
     """fermion_action (src/hmc_forces.cpp:113-119): returns (S, X)."""
     S = _D()
     X = np.zeros(D.ndof, np.complex128)
-    _chk(lib().smg_fermion_action(D.h, _p(_c(phi)), max_iter, tol, ctypes.byref(S), _p(X)))
+    _chk(lib().smg_fermion_action(D.h, _p(_c(phi)), max_iter, tol, ctypes.byref(S), _p(X), None))
     return S.value, X

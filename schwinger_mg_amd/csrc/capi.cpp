@@ -967,12 +967,14 @@ int smg_eo_fermion_force(smg_dirac* D, const double* x_o, const double* y_o, dou
         down(F, f.p, 2 * V * sizeof(double));
     });
 }
-int smg_fermion_action(smg_dirac* D, const double* phi, int maxit, double tol, double* S_out, double* X) {
+int smg_fermion_action(smg_dirac* D, const double* phi, int maxit, double tol, double* S_out, double* X,
+                       int* iters) {
     return guard([&] {
         SMG_REQUIRE(D && S_out, "null argument");
         const size_t n = 2 * (size_t)D->D->V;
         Field<double> ph = up(phi, n), xx(n);
-        cg_solve(*D->op64, ph.p(), xx.p(), nullptr, nullptr, maxit, tol, S());
+        const auto r = cg_solve(*D->op64, ph.p(), xx.p(), nullptr, nullptr, maxit, tol, S());
+        if (iters) *iters = r.iterations;
         *S_out = dev_dot<double>(ph.p(), xx.p(), n, S()).re;
         if (X) downc(X, xx.p(), n);
     });

@@ -37,3 +37,15 @@ def test_python_mirror_binds_every_symbol():
     m.load_library()
     bound = sorted(n for n, _, _ in m.SIGNATURES)
     assert bound == declared()
+
+
+def test_cpp_shim_compiles():
+    """include/schwinger_mg.hpp compiles as C++17 with the reference-style caller."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(root, "include"),
+                        os.path.join(root, "tests", "cpp", "dropin_test.cpp")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
