@@ -245,7 +245,7 @@ struct Prolongator {
         nblocks = nbx * nby;
         coarse_dim = nblocks * k_vec;
         smg_prolongator* h = nullptr;
-        detail::check(smg_prolongator_create(&h, lat.L, lat.L, 2, bx, by, k_vec));
+        detail::check(smg_prolongator_create(&h, lat.L, lat.L, 2, bx, by, k_vec, 0));
         h_.reset(h, [](smg_prolongator* p) { smg_prolongator_destroy(p); });
     }
     void build_from_vectors(const std::vector<Vec>& null_vecs) {

@@ -116,10 +116,16 @@ int smg_norm_dev(const void* a, size_t n, int prec, double* out, void* stream);
 /* Prolongator(lat, bx, by, k), src/prolongator.h:35-58, generalised to nd dof per
  * site so that it also restates CoarseProlongator (src/prolongator.h:149-228):
  * linear blocks of 4*k consecutive coarse dofs on an nx x ny coarse lattice are
- * the 4 x 1 aggregates with nd = k. */
-int smg_prolongator_create(smg_prolongator** out, int nx, int ny, int nd, int bx, int by, int k);
+ * the 4 x 1 aggregates with nd = k.  k is the coarse dof per aggregate.
+ * chiral = 0: the reference's aggregation (k null vectors, both chiralities in
+ * each block vector).  chiral = 1 (no reference counterpart): k/2 null vectors,
+ * each split into its gamma5 = +1 part (coarse dof j < k/2; the first nd/2 dof of
+ * a site) and its gamma5 = -1 part (coarse dof k/2 + j), so the coarse operator
+ * keeps a diagonal gamma5. */
+int smg_prolongator_create(smg_prolongator** out, int nx, int ny, int nd, int bx, int by, int k, int chiral);
 int smg_prolongator_destroy(smg_prolongator* P);
-/* Prolongator::build_from_vectors, src/prolongator.h:133-141 (k host vectors of nx*ny*nd complex) */
+/* Prolongator::build_from_vectors, src/prolongator.h:133-141 (k host vectors of nx*ny*nd
+ * complex; k/2 with chiral aggregation) */
 int smg_prolongator_build(smg_prolongator* P, const double* null_vecs);
 /* block vectors [nblocks][k][bx*by*nd] (Prolongator::block_vecs) */
 int smg_prolongator_get(smg_prolongator* P, double* bv);
@@ -175,10 +181,12 @@ typedef struct {
     int coarse_solver;     /* coarsest level: 0 CG (coarse_maxit, coarse_tol); 1 dense inverse
                               (formed once at setup; coarsest dimension <= 8192) */
     int use_graph;         /* 1: replay the cycle from a captured hipGraph */
+    int chiral;            /* 1: chiral aggregation at every level (k[l] = 2 x null vectors) */
 } smg_mg_params;
 /* build_mg_hierarchy, src/multigrid.cpp:595-687, with every level's null-space start
- * vectors supplied (k[l] vectors of level-l dimension, level 0 first), geometric
- * aggregation at every level and 9-point Galerkin operators */
+ * vectors supplied (k[l] vectors of level-l dimension, k[l]/2 with chiral
+ * aggregation; level 0 first), geometric aggregation at every level and 9-point
+ * Galerkin operators */
 int smg_mg_setup(smg_mg** out, smg_dirac* D, const smg_mg_params* prm, const double* starts);
 int smg_mg_destroy(smg_mg* mg);
 /* MGHierarchy::set_symmetric, src/multigrid.cpp:146-152 (power-iteration start

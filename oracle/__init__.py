@@ -67,7 +67,7 @@ def lib():
         L.orc_op_dim.argtypes = [_P]
         L.orc_op_free.argtypes = [_P]
         L.orc_prolong_new.restype = _P
-        L.orc_prolong_new.argtypes = [_I, _I, _I, _I, _I, _I]
+        L.orc_prolong_new.argtypes = [_I, _I, _I, _I, _I, _I, _I]
         L.orc_prolong_free.argtypes = [_P]
         L.orc_prolong_build.argtypes = [_P, _P]
         L.orc_prolong_get.argtypes = [_P, _P]
@@ -126,7 +126,7 @@ class MGParams(ctypes.Structure):
     _fields_ = [("n_levels", _I), ("bx", _I * 4), ("by", _I * 4), ("k", _I * 4),
                 ("null_iters", _I), ("pre_smooth", _I), ("post_smooth", _I), ("w_cycle", _I),
                 ("coarse_solver", _I), ("coarse_maxit", _I), ("coarse_tol", _D),
-                ("colored_build", _I)]
+                ("colored_build", _I), ("chiral", _I)]
 
 
 D_, DDAG, DDAGD, G5D = 0, 1, 2, 3
@@ -259,8 +259,9 @@ class Op:
 
 
 class Prolongator:
-    def __init__(self, nx, ny, nd, bx, by, k):
-        self.h = lib().orc_prolong_new(nx, ny, nd, bx, by, k)
+    def __init__(self, nx, ny, nd, bx, by, k, chiral=False):
+        self.h = lib().orc_prolong_new(nx, ny, nd, bx, by, k, int(chiral))
+        self.chiral = bool(chiral)
         self.nx, self.ny, self.nd, self.bx, self.by, self.k = nx, ny, nd, bx, by, k
         self.nblocks = (nx // bx) * (ny // by)
         self.coarse_dim = self.nblocks * k
@@ -360,7 +361,7 @@ def null_space(op, k, outer_iters, starts):
 class MG:
     def __init__(self, fine_op, nx, ny, nd, starts, n_levels=2, block=((4, 4),), k=(4,),
                  null_iters=20, pre=3, post=3, w_cycle=True, coarse_solver=0,
-                 coarse_maxit=200, coarse_tol=1e-12, colored=1):
+                 coarse_maxit=200, coarse_tol=1e-12, colored=1, chiral=False):
         p = MGParams()
         p.n_levels = n_levels
         for i, (bx, by) in enumerate(block):
@@ -371,6 +372,7 @@ class MG:
         p.w_cycle = int(w_cycle)
         p.coarse_solver, p.coarse_maxit, p.coarse_tol = coarse_solver, coarse_maxit, coarse_tol
         p.colored_build = colored
+        p.chiral = int(chiral)
         self.prm = p
         self.starts = _c(starts)
         self.fine = fine_op

@@ -483,14 +483,15 @@ void orc_op_free(orc_op* A) { free(A); }
  * geometric implementation with nd dof per site restates both. */
 struct orc_prolong {
     int nx, ny, nd, bx, by, k, nbx, nby, nblocks, bsites, bsize;
+    int chiral;       /* 1: k/2 null vectors, each split into its two chiralities */
     ocx* bv;          /* [nblocks][k][bsize], bsize = bx*by*nd */
     int* s2b; int* s2l;
     int* blist;       /* [nblocks][bsites] ascending site index */
 };
 
-orc_prolong* orc_prolong_new(int nx, int ny, int nd, int bx, int by, int k) {
+orc_prolong* orc_prolong_new(int nx, int ny, int nd, int bx, int by, int k, int chiral) {
     orc_prolong* P = xcalloc(1, sizeof(orc_prolong));
-    P->nx = nx; P->ny = ny; P->nd = nd; P->bx = bx; P->by = by; P->k = k;
+    P->nx = nx; P->ny = ny; P->nd = nd; P->bx = bx; P->by = by; P->k = k; P->chiral = chiral;
     P->nbx = nx / bx; P->nby = ny / by; P->nblocks = P->nbx * P->nby;
     P->bsites = bx * by; P->bsize = bx * by * nd;
     P->bv = xcalloc((size_t)P->nblocks * k * P->bsize, sizeof(ocx));
@@ -539,17 +540,25 @@ static void orthonormalise_block(orc_prolong* P, int b) {
     }
 }
 
-/* src/prolongator.h:63-74, 133-141 */
+/* src/prolongator.h:63-74, 133-141.  Chiral aggregation (no reference
+ * counterpart): coarse dof j < k/2 holds the gamma5 = +1 part of null vector j
+ * (the first nd/2 dof of every site), coarse dof k/2 + j its gamma5 = -1 part;
+ * the per-block Gram-Schmidt then runs over all k as usual (the two halves are
+ * orthogonal by support). */
 void orc_prolong_build(orc_prolong* P, const ocx* null_vecs) {
-    int V = P->nx * P->ny, nd = P->nd;
+    int V = P->nx * P->ny, nd = P->nd, nv = P->chiral ? P->k / 2 : P->k;
     size_t fd = (size_t)V * nd;
     for (int b = 0; b < P->nblocks; b++) {
         for (int k = 0; k < P->k; k++) {
             ocx* out = P->bv + ((size_t)b * P->k + k) * P->bsize;
+            int src = k % nv, chi = P->chiral ? k / nv : -1;
             for (int t = 0; t < P->bsites; t++) {
                 int s = P->blist[b * P->bsites + t];
                 int loc = P->s2l[s];
-                for (int c = 0; c < nd; c++) out[loc * nd + c] = null_vecs[k * fd + (size_t)s * nd + c];
+                for (int c = 0; c < nd; c++) {
+                    int keep = chi < 0 || (c >= nd / 2) == chi;
+                    out[loc * nd + c] = keep ? null_vecs[src * fd + (size_t)s * nd + c] : 0.0;
+                }
             }
         }
         orthonormalise_block(P, b);
@@ -866,11 +875,11 @@ orc_mg* orc_mg_new(const orc_op* fine, int nx, int ny, int nd, const orc_mg_para
     int cnx = nx, cny = ny, cnd = nd;
     size_t soff = 0;
     for (int l = 0; l + 1 < mg->nlev; l++) {
-        int k = prm->k[l], n = mg->dim[l];
-        mg->nullv[l] = xcalloc((size_t)k * n, sizeof(ocx));
-        orc_null_space(mg->op[l], k, prm->null_iters, starts + soff, mg->nullv[l]);
-        soff += (size_t)k * n;
-        mg->P[l] = orc_prolong_new(cnx, cny, cnd, prm->bx[l], prm->by[l], k);
+        int k = prm->k[l], n = mg->dim[l], nv = prm->chiral ? k / 2 : k;
+        mg->nullv[l] = xcalloc((size_t)nv * n, sizeof(ocx));
+        orc_null_space(mg->op[l], nv, prm->null_iters, starts + soff, mg->nullv[l]);
+        soff += (size_t)nv * n;
+        mg->P[l] = orc_prolong_new(cnx, cny, cnd, prm->bx[l], prm->by[l], k, prm->chiral);
         orc_prolong_build(mg->P[l], mg->nullv[l]);
         mg->C[l] = orc_coarse_build(mg->op[l], mg->P[l], prm->colored_build);
         mg->own_op[l + 1] = orc_op_coarse(mg->C[l]);
@@ -896,7 +905,8 @@ const orc_prolong* orc_mg_prolong(const orc_mg* mg, int l) { return mg->P[l]; }
 const orc_coarse* orc_mg_coarse(const orc_mg* mg, int l) { return mg->C[l]; }
 double orc_mg_omega(const orc_mg* mg, int l) { return mg->omega[l]; }
 void orc_mg_null_vecs(const orc_mg* mg, int l, ocx* out) {
-    memcpy(out, mg->nullv[l], (size_t)mg->prm.k[l] * mg->dim[l] * sizeof(ocx));
+    int nv = mg->prm.chiral ? mg->prm.k[l] / 2 : mg->prm.k[l];
+    memcpy(out, mg->nullv[l], (size_t)nv * mg->dim[l] * sizeof(ocx));
 }
 
 /* src/multigrid.cpp:146-152 with explicit power-iteration starts per level */

@@ -64,7 +64,9 @@ void orc_op_free(orc_op* A);
 
 /* ---------------- prolongator (geometric aggregation) ---------------- */
 typedef struct orc_prolong orc_prolong;
-orc_prolong* orc_prolong_new(int nx, int ny, int nd, int bx, int by, int k);
+/* k coarse dof per aggregate; chiral = 1 splits each of k/2 null vectors into its
+ * gamma5 = +1 (first nd/2 dof of a site) and gamma5 = -1 parts */
+orc_prolong* orc_prolong_new(int nx, int ny, int nd, int bx, int by, int k, int chiral);
 void orc_prolong_free(orc_prolong* P);
 void orc_prolong_build(orc_prolong* P, const ocx* null_vecs);
 void orc_prolong_get(const orc_prolong* P, ocx* bv);
@@ -108,6 +110,7 @@ typedef struct {
     int coarse_maxit;
     double coarse_tol;
     int colored_build;     /* 1 = colored probing, 0 = one probe per coarse column */
+    int chiral;            /* 1 = chiral aggregation at every level */
 } orc_mg_params;
 /* fine op acts on nx*ny sites with nd dof each.  starts: the k[l] start vectors of
  * every level, concatenated (level 0 first). */

@@ -36,7 +36,8 @@ class SMGError(RuntimeError):
 class MGParams(ctypes.Structure):
     _fields_ = [("n_levels", _I), ("bx", _I * 4), ("by", _I * 4), ("k", _I * 4), ("null_iters", _I),
                 ("pre_smooth", _I), ("post_smooth", _I), ("w_cycle", _I), ("coarse_maxit", _I),
-                ("coarse_tol", _D), ("coarse_half", _I), ("coarse_solver", _I), ("use_graph", _I)]
+                ("coarse_tol", _D), ("coarse_half", _I), ("coarse_solver", _I), ("use_graph", _I),
+                ("chiral", _I)]
 
 
 class SolveInfo(ctypes.Structure):
@@ -56,7 +57,7 @@ SIGNATURES = [
     ("smg_eo_apply", _I, [_P, _I, _P, _P]), ("smg_eo_apply_dev", _I, [_P, _I, _I, _P, _P, _P]),
     ("smg_eo_reconstruct", _I, [_P, _P, _P, _P]), ("smg_eo_logdet", _I, [_P, _P]),
     ("smg_dot_dev", _I, [_P, _P, _SZ, _I, _P, _P]), ("smg_norm_dev", _I, [_P, _SZ, _I, _P, _P]),
-    ("smg_prolongator_create", _I, [_P, _I, _I, _I, _I, _I, _I]), ("smg_prolongator_destroy", _I, [_P]),
+    ("smg_prolongator_create", _I, [_P, _I, _I, _I, _I, _I, _I, _I]), ("smg_prolongator_destroy", _I, [_P]),
     ("smg_prolongator_build", _I, [_P, _P]), ("smg_prolongator_get", _I, [_P, _P]),
     ("smg_prolongator_set", _I, [_P, _P]), ("smg_prolong", _I, [_P, _P, _P]), ("smg_restrict", _I, [_P, _P, _P]),
     ("smg_coarse_create_galerkin", _I, [_P, _P, _P]), ("smg_coarse_create_from_coarse", _I, [_P, _P, _P]),
@@ -289,16 +290,20 @@ class EvenOddDiracOp:
 class Prolongator:
     """Prolongator(lat, bx, by, k) -- src/prolongator.h:14-142, with nd dof per site
     (nd = 2 on the fine lattice; nd = k of the level above on a coarse lattice,
-    which also covers CoarseProlongator, src/prolongator.h:149-228)."""
+    which also covers CoarseProlongator, src/prolongator.h:149-228).  k is the
+    coarse dof per aggregate; chiral=True splits each of k/2 null vectors into its
+    two chiralities (no reference counterpart)."""
 
-    def __init__(self, nx, ny, nd, bx, by, k):
+    def __init__(self, nx, ny, nd, bx, by, k, chiral=False):
         self.nx, self.ny, self.nd, self.bx, self.by, self.k_vec = nx, ny, nd, bx, by, k
+        self.chiral = bool(chiral)
+        self.nvec = k // 2 if chiral else k
         self.nbx, self.nby = nx // bx, ny // by
         self.nblocks = self.nbx * self.nby
         self.coarse_dim = self.nblocks * k
         self.fine_dim = nx * ny * nd
         h = _P()
-        _chk(lib().smg_prolongator_create(ctypes.byref(h), nx, ny, nd, bx, by, k))
+        _chk(lib().smg_prolongator_create(ctypes.byref(h), nx, ny, nd, bx, by, k, int(chiral)))
         self.h = h
 
     def __del__(self):
@@ -307,7 +312,7 @@ class Prolongator:
             self.h = None
 
     def build_from_vectors(self, null_vecs):
-        nv = _c(np.asarray(null_vecs)[: self.k_vec]).ravel()
+        nv = _c(np.asarray(null_vecs)[: self.nvec]).ravel()
         _chk(lib().smg_prolongator_build(self.h, _p(nv)))
         return self
 
@@ -445,7 +450,7 @@ class MGHierarchy:
 
     def __init__(self, D, starts, n_levels=2, blocks=((4, 4),), k=(4,), null_iters=20, pre_smooth=3,
                  post_smooth=3, w_cycle=True, coarse_maxit=200, coarse_tol=1e-12, coarse_half=False,
-                 coarse_solver=0, use_graph=False):
+                 coarse_solver=0, use_graph=False, chiral=False):
         p = MGParams()
         p.n_levels = n_levels
         blocks = list(blocks) + [blocks[-1]] * (4 - len(blocks))
@@ -455,7 +460,7 @@ class MGHierarchy:
             p.k[i] = k[i]
         p.null_iters, p.pre_smooth, p.post_smooth = null_iters, pre_smooth, post_smooth
         p.w_cycle, p.coarse_maxit, p.coarse_tol, p.coarse_half = int(w_cycle), coarse_maxit, coarse_tol, int(coarse_half)
-        p.coarse_solver, p.use_graph = coarse_solver, int(use_graph)
+        p.coarse_solver, p.use_graph, p.chiral = coarse_solver, int(use_graph), int(chiral)
         self.params, self.D, self.n_levels = p, D, n_levels
         h = _P()
         _chk(lib().smg_mg_setup(ctypes.byref(h), D.h, ctypes.byref(p), _p(_c(starts))))
@@ -482,9 +487,10 @@ class MGHierarchy:
 
     def null_vecs(self, level=0):
         n = self.level_dim(level)
-        out = np.zeros(self.params.k[level] * n, np.complex128)
+        nv = self.params.k[level] // 2 if self.params.chiral else self.params.k[level]
+        out = np.zeros(nv * n, np.complex128)
         _chk(lib().smg_mg_null_vecs(self.h, level, _p(out)))
-        return out.reshape(self.params.k[level], n)
+        return out.reshape(nv, n)
 
     def block_vecs(self, level=0):
         nc = self.level_dim(level + 1)

@@ -290,7 +290,7 @@ __global__ __launch_bounds__(kBlock) void k_b_coarse_mfma(CoarseView<TL> C, cplx
     constexpr int N = 16 * NT, RT = K / 8, Q = K / 4;
     constexpr bool PAIR = sizeof(T) == 4 && NT % 2 == 0;
     constexpr int NL = PAIR ? NT / 2 : NT;   // source loads per (kj) per lane
-    constexpr int S = sizeof(T) == 4 ? 3 : 1;   // directions per pipeline stage
+    constexpr int S = (sizeof(T) == 4 && K <= 8) ? 3 : 1;   // directions per pipeline stage (K <= 8)
     using MT = MfmaTile<T>;
     using acc_t = typename MT::acc;
     using Run = LinkRun<TL, Q>;
@@ -319,8 +319,9 @@ __global__ __launch_bounds__(kBlock) void k_b_coarse_mfma(CoarseView<TL> C, cplx
             const int cy = (d == 3 || d == 5 || d == 7) ? yp : (d == 4 || d == 6 || d == 8) ? ym : by;
             nbr[d] = cx + cy * nx;
         }
-        Run lv[2][S][RT];
-        cplx<T> sv[2][S][Q][NT];   // sv[.][.][q][t] = src[nb][hi*Q + q][rhs(t, lo)]
+        constexpr int SB = K <= 8 ? S : 1;   // staged directions (K <= 8 only)
+        Run lv[2][SB][K <= 8 ? RT : 1];
+        cplx<T> sv[2][SB][K <= 8 ? Q : 1][NT];   // sv[.][.][q][t] = src[nb][hi*Q + q][rhs(t, lo)]
         auto load_stage = [&](int buf, int st0) {
 #pragma unroll
             for (int e = 0; e < S; e++) {
@@ -361,12 +362,52 @@ __global__ __launch_bounds__(kBlock) void k_b_coarse_mfma(CoarseView<TL> C, cplx
                         }
                     }
         };
-        load_stage(0, 0);
+        if constexpr (K <= 8) {
+            load_stage(0, 0);
 #pragma unroll
-        for (int i = 0; i < 9 / S; i++) {
-            if (i + 1 < 9 / S) load_stage((i + 1) & 1, (i + 1) * S);
-            __builtin_amdgcn_sched_barrier(0);
-            mma_stage(i & 1);
+            for (int i = 0; i < 9 / S; i++) {
+                if (i + 1 < 9 / S) load_stage((i + 1) & 1, (i + 1) * S);
+                __builtin_amdgcn_sched_barrier(0);
+                mma_stage(i & 1);
+            }
+        } else {
+            // large k: a direction carries RT*Q*2*NT MFMAs, enough to cover its own
+            // loads; the sources of the direction are held, the links streamed per
+            // row tile
+#pragma unroll 1
+            for (int d = 0; d < 9; d++) {
+                const int nb = nbr[d];
+                cplx<T> sw[Q][NT];
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    const cplx<T>* row = src + ((size_t)nb * K + hi * Q + q) * N;
+#pragma unroll
+                    for (int u = 0; u < NL; u++) {
+                        if constexpr (PAIR) {
+                            const float4 w = *reinterpret_cast<const float4*>(row + 32 * u + 2 * lo);
+                            sw[q][2 * u] = cplx<T>{w.x, w.y};
+                            sw[q][2 * u + 1] = cplx<T>{w.z, w.w};
+                        } else {
+                            sw[q][u] = row[16 * u + lo];
+                        }
+                    }
+                }
+                const TL* lrow = C.link + ((size_t)b * 9 + d) * (K * K) + a_ki * K + hi * Q;
+#pragma unroll
+                for (int r = 0; r < RT; r++) {
+                    const Run lr = *reinterpret_cast<const Run*>(lrow + 8 * r * K);
+#pragma unroll
+                    for (int q = 0; q < Q; q++) {
+                        const cplx<T> L = cvt<T>(load_c(&lr.v[q]));
+                        const T a0 = real_form(L, a_c, 0), a1 = real_form(L, a_c, 1);
+#pragma unroll
+                        for (int t = 0; t < NT; t++) {
+                            acc[r][t] = MT::mma(a0, sw[q][t].re, acc[r][t]);
+                            acc[r][t] = MT::mma(a1, sw[q][t].im, acc[r][t]);
+                        }
+                    }
+                }
+            }
         }
 #pragma unroll
         for (int r = 0; r < RT; r++)
@@ -462,16 +503,20 @@ static void coarse_mfma(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* sr
 template <typename T, typename TL>
 void launch_b_coarse_apply(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b,
                            const cplx<T>* aux, const FinB& fin, int N, hipStream_t st) {
-    if ((C.k == 8 || C.k == 16) && (N == 16 || N == 32 || N == 64)) {
+    if ((C.k == 8 || C.k == 16 || C.k == 32) && (N == 16 || N == 32 || N == 64)) {
         // matrix-core path
         if (C.k == 8) {
             if (N == 16) coarse_mfma<T, TL, 8, 1>(C, dst, src, epi, b, aux, fin, st);
             else if (N == 32) coarse_mfma<T, TL, 8, 2>(C, dst, src, epi, b, aux, fin, st);
             else coarse_mfma<T, TL, 8, 4>(C, dst, src, epi, b, aux, fin, st);
-        } else {
+        } else if (C.k == 16) {
             if (N == 16) coarse_mfma<T, TL, 16, 1>(C, dst, src, epi, b, aux, fin, st);
             else if (N == 32) coarse_mfma<T, TL, 16, 2>(C, dst, src, epi, b, aux, fin, st);
             else coarse_mfma<T, TL, 16, 4>(C, dst, src, epi, b, aux, fin, st);
+        } else {
+            if (N == 16) coarse_mfma<T, TL, 32, 1>(C, dst, src, epi, b, aux, fin, st);
+            else if (N == 32) coarse_mfma<T, TL, 32, 2>(C, dst, src, epi, b, aux, fin, st);
+            else coarse_mfma<T, TL, 32, 4>(C, dst, src, epi, b, aux, fin, st);
         }
         SMG_LAUNCH_CHECK();
         return;
@@ -503,8 +548,9 @@ __global__ __launch_bounds__(kBlock) void k_b_prolong(AggView A, const cplx<T>* 
         const int loc = (x % A.bx) + A.bx * (y % A.by);
         const cplx<T>* v = bv + (size_t)b * A.k * bsize + loc * A.nd + c;
         const cplx<T>* cc = coarse + ((size_t)b * A.k) * N + j;
+        const int k0 = A.chiral ? agg_chi_dof(A, c) * (A.k / 2) : 0, k1 = A.chiral ? k0 + A.k / 2 : A.k;
         cplx<T> acc = {(T)0, (T)0};
-        for (int kk = 0; kk < A.k; kk++) acc = acc + cc[(size_t)kk * N] * v[(size_t)kk * bsize];
+        for (int kk = k0; kk < k1; kk++) acc = acc + cc[(size_t)kk * N] * v[(size_t)kk * bsize];
         fine[t] = accumulate ? fine[t] + acc : acc;
     }
 }
@@ -529,12 +575,13 @@ __global__ __launch_bounds__(kBlock) void k_b_restrict(AggView A, const cplx<T>*
         const int b = (int)(row / A.k), kk = (int)(row % A.k);
         const int bxi = b % A.nbx, byi = b / A.nbx;
         const cplx<T>* v = bv + ((size_t)b * A.k + kk) * bsize;
+        const int c0 = A.chiral ? agg_chi_vec(A, kk) * (A.nd / 2) : 0, c1 = A.chiral ? c0 + A.nd / 2 : A.nd;
         cplx<T> acc = {(T)0, (T)0};
         for (int loc = 0; loc < bsites; loc++) {
             const int s = (bxi * A.bx + loc % A.bx) + A.nx * (byi * A.by + loc / A.bx);
             const cplx<T>* fp = fine + ((size_t)s * A.nd) * N + j;
-            cplx<T> a = cdot(v[loc * A.nd], fp[0]);
-            for (int c = 1; c < A.nd; c++) a = a + cdot(v[loc * A.nd + c], fp[(size_t)c * N]);
+            cplx<T> a = cdot(v[loc * A.nd + c0], fp[(size_t)c0 * N]);
+            for (int c = c0 + 1; c < c1; c++) a = a + cdot(v[loc * A.nd + c], fp[(size_t)c * N]);
             acc = acc + a;
         }
         coarse[t] = acc;

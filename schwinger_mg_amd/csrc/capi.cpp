@@ -343,11 +343,11 @@ int smg_norm_dev(const void* a, size_t n, int prec, double* out, void* stream) {
 }
 
 // ------------------------------------------------------------------ prolongator
-int smg_prolongator_create(smg_prolongator** out, int nx, int ny, int nd, int bx, int by, int k) {
+int smg_prolongator_create(smg_prolongator** out, int nx, int ny, int nd, int bx, int by, int k, int chiral) {
     return guard([&] {
         SMG_REQUIRE(out && nx > 0 && ny > 0 && nd > 0, "bad arguments");
         auto* h = new smg_prolongator_s();
-        try { h->P.reset(new Aggregation(nx, ny, nd, bx, by, k)); } catch (...) { delete h; throw; }
+        try { h->P.reset(new Aggregation(nx, ny, nd, bx, by, k, chiral)); } catch (...) { delete h; throw; }
         *out = h;
     });
 }
@@ -355,7 +355,7 @@ int smg_prolongator_destroy(smg_prolongator* P) { return guard([&] { delete P; }
 int smg_prolongator_build(smg_prolongator* P, const double* null_vecs) {
     return guard([&] {
         SMG_REQUIRE(P, "null handle");
-        Field<double> nv = up(null_vecs, P->P->fine_n() * P->P->A.k);
+        Field<double> nv = up(null_vecs, P->P->fine_n() * P->P->nvec());
         P->P->build(nv.p(), S());
         Context::get().sync();
     });
@@ -561,6 +561,7 @@ int smg_mg_setup(smg_mg** out, smg_dirac* D, const smg_mg_params* prm, const dou
         c.coarse_half = prm->coarse_half;
         c.coarse_solver = prm->coarse_solver;
         c.use_graph = prm->use_graph;
+        c.chiral = prm->chiral;
         SMG_REQUIRE(c.coarse_solver == 0 || c.coarse_solver == 1, "coarse_solver must be 0 or 1");
         SMG_REQUIRE(c.n_levels >= 2 && c.n_levels <= 5, "n_levels must be 2..5");
         for (int l = 0; l + 1 < c.n_levels; l++)

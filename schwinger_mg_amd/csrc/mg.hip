@@ -15,8 +15,9 @@ __device__ __forceinline__ void agg_site(const AggView& A, int b, int loc, int& 
     s = (bxi * A.bx + lx) + A.nx * (byi * A.by + ly);
 }
 
-// src/prolongator.h:63-74: copy vector kv into the block storage
-__global__ void k_block_gather(AggView A, c64* __restrict__ bv, const c64* __restrict__ vec, int kv) {
+// src/prolongator.h:63-74: copy one null vector into the block storage (with
+// chiral aggregation, one chirality of it)
+__global__ void k_block_gather(AggView A, c64* __restrict__ bv, const c64* __restrict__ vec, int slot, int chi) {
     const int bsites = A.bx * A.by, bsize = bsites * A.nd;
     const int nb = A.nbx * A.nby;
     const size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
@@ -25,11 +26,12 @@ __global__ void k_block_gather(AggView A, c64* __restrict__ bv, const c64* __res
     const int loc = e / A.nd, c = e % A.nd;
     int s;
     agg_site(A, b, loc, s);
-    bv[((size_t)b * A.k + kv) * bsize + e] = vec[(size_t)s * A.nd + c];
+    const bool keep = chi < 0 || agg_chi_dof(A, c) == chi;
+    bv[((size_t)b * A.k + slot) * bsize + e] = keep ? vec[(size_t)s * A.nd + c] : c64{0.0, 0.0};
 }
-void launch_block_gather(const AggView& A, c64* bv, const c64* vec, int kv, hipStream_t st) {
+void launch_block_gather(const AggView& A, c64* bv, const c64* vec, int slot, int chi, hipStream_t st) {
     const size_t n = (size_t)A.nbx * A.nby * A.bx * A.by * A.nd;
-    k_block_gather<<<grid_for(n), kBlock, 0, st>>>(A, bv, vec, kv);
+    k_block_gather<<<grid_for(n), kBlock, 0, st>>>(A, bv, vec, slot, chi);
     SMG_LAUNCH_CHECK();
 }
 
@@ -84,8 +86,10 @@ __global__ __launch_bounds__(kBlock) void k_prolong(AggView A, const cplx<T>* __
     const int bsize = A.bx * A.by * A.nd;
     const cplx<T>* v = bv + (size_t)b * A.k * bsize + loc * A.nd + c;
     const cplx<T>* cc = coarse + (size_t)b * A.k;
+    // chiral aggregation: only the coarse dof of this dof's chirality contribute
+    const int j0 = A.chiral ? agg_chi_dof(A, c) * (A.k / 2) : 0, j1 = A.chiral ? j0 + A.k / 2 : A.k;
     cplx<T> acc = {(T)0, (T)0};
-    for (int j = 0; j < A.k; j++) acc = acc + cc[j] * v[(size_t)j * bsize];
+    for (int j = j0; j < j1; j++) acc = acc + cc[j] * v[(size_t)j * bsize];
     fine[f] = accumulate ? fine[f] + acc : acc;
 }
 template <typename T>
@@ -108,13 +112,15 @@ __global__ __launch_bounds__(kBlock) void k_restrict(AggView A, const cplx<T>* _
     const int b = t / A.k, j = t % A.k;
     const int bsites = A.bx * A.by, bsize = bsites * A.nd;
     const cplx<T>* v = bv + ((size_t)b * A.k + j) * bsize;
+    // chiral aggregation: the block vector vanishes off its chirality
+    const int c0 = A.chiral ? agg_chi_vec(A, j) * (A.nd / 2) : 0, c1 = A.chiral ? c0 + A.nd / 2 : A.nd;
     cplx<T> acc = {(T)0, (T)0};
     for (int loc = 0; loc < bsites; loc++) {
         int s;
         agg_site(A, b, loc, s);
         const cplx<T>* fp = fine + (size_t)s * A.nd;
-        cplx<T> a = cdot(v[loc * A.nd], fp[0]);
-        for (int c = 1; c < A.nd; c++) a = a + cdot(v[loc * A.nd + c], fp[c]);
+        cplx<T> a = cdot(v[loc * A.nd + c0], fp[c0]);
+        for (int c = c0 + 1; c < c1; c++) a = a + cdot(v[loc * A.nd + c], fp[c]);
         acc = acc + a;
     }
     coarse[t] = acc;

@@ -212,16 +212,26 @@ template struct SchurOp<double>;
 template struct SchurOp<float>;
 
 // ------------------------------------------------------------------ aggregation
-Aggregation::Aggregation(int nx, int ny, int nd, int bx, int by, int k) {
+Aggregation::Aggregation(int nx, int ny, int nd, int bx, int by, int k, int chiral) {
     SMG_REQUIRE(bx >= 1 && by >= 1 && nx % bx == 0 && ny % by == 0, "aggregate shape must tile the lattice");
     SMG_REQUIRE(k >= 1, "k must be >= 1");
-    A = AggView{nx, ny, nd, bx, by, k, nx / bx, ny / by};
+    SMG_REQUIRE(!chiral || (k % 2 == 0 && nd % 2 == 0), "chiral aggregation needs even k and an even dof count");
+    A = AggView{nx, ny, nd, bx, by, k, nx / bx, ny / by, chiral ? 1 : 0};
     const size_t nbv = (size_t)A.nbx * A.nby * k * bx * by * nd;
     bv64 = Field<double>(nbv);
     bv32 = Field<float>(nbv);
 }
 void Aggregation::build(const c64* null_vecs, hipStream_t st) {
-    for (int kv = 0; kv < A.k; kv++) launch_block_gather(A, bv64.p(), null_vecs + (size_t)kv * fine_n(), kv, st);
+    const int nv = nvec();
+    for (int kv = 0; kv < nv; kv++) {
+        const c64* v = null_vecs + (size_t)kv * fine_n();
+        if (A.chiral) {
+            launch_block_gather(A, bv64.p(), v, kv, 0, st);        // coarse dof kv: gamma5 = +1
+            launch_block_gather(A, bv64.p(), v, nv + kv, 1, st);   // coarse dof nv + kv: gamma5 = -1
+        } else {
+            launch_block_gather(A, bv64.p(), v, kv, -1, st);
+        }
+    }
     launch_block_orthonormalize(A, bv64.p(), st);
     launch_convert<float, double>(bv32.p(), bv64.p(), bv64.n, st);
 }
@@ -508,13 +518,15 @@ MG::MG(const Dirac& d, const MGConfig& c, const c64* starts, bool on_device, boo
     LinOp<double>* cur = fine64.get();
     for (int l = 0; l + 1 < c.n_levels; l++) {
         const int k = c.k[l];
+        SMG_REQUIRE(!c.chiral || k % 2 == 0, "chiral aggregation needs an even k at every level");
+        const int nv = c.chiral ? k / 2 : k;   // null vectors of this level
         const size_t n = dims[l];
-        nullv.emplace_back(new Field<double>((size_t)k * n));
-        SMG_HIP(hipMemcpyAsync(nullv.back()->p(), starts + soff, (size_t)k * n * sizeof(c64),
+        nullv.emplace_back(new Field<double>((size_t)nv * n));
+        SMG_HIP(hipMemcpyAsync(nullv.back()->p(), starts + soff, (size_t)nv * n * sizeof(c64),
                                on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
-        soff += (size_t)k * n;
-        null_space(*cur, k, c.null_iters, c.null_inner, nullv.back()->p(), st);
-        P.emplace_back(new Aggregation(lnx[l], lny[l], lnd[l], c.bx[l], c.by[l], k));
+        soff += (size_t)nv * n;
+        null_space(*cur, nv, c.null_iters, c.null_inner, nullv.back()->p(), st);
+        P.emplace_back(new Aggregation(lnx[l], lny[l], lnd[l], c.bx[l], c.by[l], k, c.chiral));
         P.back()->build(nullv.back()->p(), st);
         C.emplace_back(new CoarseOperator(P.back()->A.nbx, P.back()->A.nby, k));
         C.back()->build(*cur, *P.back(), st);

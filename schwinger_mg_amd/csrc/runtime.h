@@ -122,10 +122,13 @@ struct Aggregation {
     AggView A;
     Field<double> bv64;
     Field<float> bv32;
-    Aggregation(int nx, int ny, int nd, int bx, int by, int k);
+    // k coarse dof per aggregate; chiral: k = 2 * (null vectors), each vector split
+    // into its two chiralities
+    Aggregation(int nx, int ny, int nd, int bx, int by, int k, int chiral = 0);
     size_t fine_n() const { return (size_t)A.nx * A.ny * A.nd; }
     size_t coarse_n() const { return (size_t)A.nbx * A.nby * A.k; }
-    // null_vecs: k device vectors of fine_n() entries, contiguous
+    int nvec() const { return A.chiral ? A.k / 2 : A.k; }
+    // null_vecs: nvec() device vectors of fine_n() entries, contiguous
     void build(const c64* null_vecs, hipStream_t st);
     void set_block_vecs(const c64* bv_dev, hipStream_t st);
     template <typename T>
@@ -193,6 +196,7 @@ struct MGConfig {
     int coarse_half = 0;     // FP32 cycle: FP16 coarse links
     int coarse_solver = 0;   // coarsest level: 0 CG (coarse_maxit, coarse_tol), 1 dense inverse
     int use_graph = 0;       // replay the cycle from a captured hipGraph
+    int chiral = 0;          // chiral aggregation: k[l] = 2 * (null vectors of level l)
 };
 
 template <typename T>

@@ -132,11 +132,19 @@ void launch_g5(cplx<T>* v, size_t nsites, hipStream_t st);
 // ---------------------------------------------------------------- MG (mg.hip)
 struct AggView {
     int nx, ny, nd;  // fine lattice and dof per site
-    int bx, by, k;   // aggregate shape and vectors
+    int bx, by, k;   // aggregate shape and coarse dof per aggregate
     int nbx, nby;    // coarse lattice
+    int chiral;      // 1: each null vector is split into its two chiralities (k = 2 * vectors)
 };
-// gather null vector kv into block storage bv[b][kv][loc*nd + c] (FP64)
-void launch_block_gather(const AggView& A, c64* bv, const c64* vec, int kv, hipStream_t st);
+// Chirality of dof c of a site: the first nd/2 dof are gamma5 = +1 (the upper spin
+// component on the fine lattice, the first half of the coarse dof on a coarse one),
+// the rest gamma5 = -1.  With chiral aggregation, coarse dof j of an aggregate has
+// chirality j / (k/2) and its block vector vanishes on the other chirality.
+__host__ __device__ inline int agg_chi_dof(const AggView& A, int c) { return c >= A.nd / 2 ? 1 : 0; }
+__host__ __device__ inline int agg_chi_vec(const AggView& A, int j) { return j >= A.k / 2 ? 1 : 0; }
+// copy null vector vec into block slot `slot` of bv[b][slot][loc*nd + c] (FP64);
+// chi = 0 or 1 keeps only the dof of that chirality (zero elsewhere), chi = -1 all
+void launch_block_gather(const AggView& A, c64* bv, const c64* vec, int slot, int chi, hipStream_t st);
 // per-aggregate modified Gram-Schmidt (FP64), src/prolongator.h:111-130
 void launch_block_orthonormalize(const AggView& A, c64* bv, hipStream_t st);
 template <typename T>

@@ -30,6 +30,7 @@ typedef struct {
     int coarse_maxit;
     double coarse_tol;
     int colored_build;
+    int chiral;
 } orc_mg_params;
 orc_dirac* orc_dirac_new(int L, const cx* U, double mass, double r, double csw, double mu_t);
 void orc_dirac_free(orc_dirac* D);

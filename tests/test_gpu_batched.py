@@ -34,15 +34,17 @@ def test_batched_stencil(smg, N):
             assert relerr(got, want) < tol, (kind, prec)
 
 
-@pytest.mark.parametrize("k", [8, 16])
+@pytest.mark.parametrize("k", [8, 16, 32])
 @pytest.mark.parametrize("N", [4, 16, 32, 64])
 def test_batched_coarse_apply(smg, k, N):
-    """Galerkin links from the oracle; N = 16, 32, 64 with k = 8, 16 run on the
-    matrix cores, N = 4 on the plain kernel."""
+    """Galerkin links from the oracle (chiral aggregation for k = 32); N = 16, 32, 64
+    with k = 8, 16, 32 run on the matrix cores, N = 4 on the plain kernel."""
     banner(f"batched coarse apply k={k} N={N}")
     L, b = 32, 4
     U = random_gauge(L, 0.4, 40 + k)
-    Po = O.Prolongator(L, L, 2, b, b, k).build(gaussian_vecs(k, 2 * L * L, 41).ravel())
+    chiral = k == 32
+    nv = k // 2 if chiral else k
+    Po = O.Prolongator(L, L, 2, b, b, k, chiral=chiral).build(gaussian_vecs(nv, 2 * L * L, 41).ravel())
     Co = O.Coarse(O.Dirac(L, U, 0.05, 1.0, 1.0, 0.0).op(), Po, 1)
     C = smg.SparseCoarseOp.from_links(L // b, L // b, k, Co.links())
     S = gaussian_vecs(N, C.dim, 42)
@@ -68,24 +70,27 @@ def test_batched_coarse_apply_rectangular(smg, N):
         assert relerr(C.apply_batched(S, smg.F64), want) < 1e-13, (nx, ny)
 
 
-def _mg(smg, L, coarse_solver, seed=10, k=4):
+def _mg(smg, L, coarse_solver, seed=10, k=4, chiral=False, w_cycle=True):
     V = L * L
     U = random_gauge(L, 0.4, seed)
     D = smg.DiracOp(L, U, 0.05, 1.0, 1.0, 0.0)
     dims = [2 * V, (L // 4) ** 2 * k]
-    starts = np.concatenate([gaussian_vecs(k, n, seed + 1 + i).ravel() for i, n in enumerate(dims)])
+    nv = k // 2 if chiral else k
+    starts = np.concatenate([gaussian_vecs(nv, n, seed + 1 + i).ravel() for i, n in enumerate(dims)])
     mg = smg.MGHierarchy(D, starts, n_levels=3, blocks=((4, 4), (2, 2)), k=(k, k), null_iters=5, coarse_maxit=30,
-                         coarse_tol=1e-4, coarse_solver=coarse_solver, coarse_half=True)
+                         coarse_tol=1e-4, coarse_solver=coarse_solver, coarse_half=True, chiral=chiral,
+                         w_cycle=w_cycle)
     return D, U, mg
 
 
-@pytest.mark.parametrize("coarse_solver,k,N", [(0, 4, 8), (1, 4, 8), (1, 8, 32), (0, 8, 16), (1, 16, 16)])
-def test_batched_cycle_equals_single(smg, coarse_solver, k, N):
-    """k = 8, 16 with N >= 16 runs the coarse levels and the dense coarsest
-    product on the matrix cores."""
-    banner(f"batched cycle coarse_solver={coarse_solver} k={k} N={N}")
+@pytest.mark.parametrize("coarse_solver,k,N,chiral", [(0, 4, 8, False), (1, 4, 8, False), (1, 8, 32, False),
+                                                      (0, 8, 16, False), (1, 16, 16, False), (0, 32, 32, True)])
+def test_batched_cycle_equals_single(smg, coarse_solver, k, N, chiral):
+    """k = 8, 16, 32 with N >= 16 runs the coarse levels and the dense coarsest
+    product on the matrix cores; k = 32 with chiral aggregation."""
+    banner(f"batched cycle coarse_solver={coarse_solver} k={k} N={N} chiral={chiral}")
     L = 32
-    D, U, mg = _mg(smg, L, coarse_solver, k=k)
+    D, U, mg = _mg(smg, L, coarse_solver, k=k, chiral=chiral, w_cycle=not chiral)
     B = gaussian_vecs(N, 2 * L * L, 20)
     want64 = np.stack([mg.precondition(B[j], smg.F64) for j in range(N)])
     want32 = np.stack([mg.precondition(B[j], smg.F32) for j in range(N)])
@@ -93,11 +98,13 @@ def test_batched_cycle_equals_single(smg, coarse_solver, k, N):
     assert relerr(mg.precondition_batched(B, smg.F32), want32) < 2e-5
 
 
-@pytest.mark.parametrize("prec,k,N", [("f64", 4, 8), ("mixed", 4, 8), ("f64", 8, 32), ("mixed", 8, 32)])
-def test_batched_gcr_solves(smg, prec, k, N):
-    banner(f"batched gcr {prec} k={k} N={N}")
+@pytest.mark.parametrize("prec,k,N,chiral", [("f64", 4, 8, False), ("mixed", 4, 8, False), ("f64", 8, 32, False),
+                                              ("mixed", 8, 32, False), ("f64", 32, 32, True),
+                                              ("mixed", 32, 32, True)])
+def test_batched_gcr_solves(smg, prec, k, N, chiral):
+    banner(f"batched gcr {prec} k={k} N={N} chiral={chiral}")
     L = 32
-    D, U, mg = _mg(smg, L, 1, k=k)
+    D, U, mg = _mg(smg, L, 0 if chiral else 1, k=k, chiral=chiral, w_cycle=not chiral)
     B = gaussian_vecs(N, 2 * L * L, 30)
     p = smg.F64 if prec == "f64" else smg.F32
     X, info, tr = smg.gcr_solve_batched(D, B, mg, 200, 1e-10, 8, prec=p, inner_tol=1e-5)

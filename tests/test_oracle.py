@@ -234,6 +234,41 @@ def test_aggregates_orthonormal_and_galerkin_is_dense_product():
     assert close(Cm, Cm.conj().T, 1e-12)                                        # Hermitian
 
 
+def test_chiral_aggregation_structure_and_galerkin():
+    """Chiral aggregation (no reference counterpart) is pinned by what it must
+    satisfy: orthonormal block vectors, P gamma5_c = gamma5 P with
+    gamma5_c = diag(+1 x k/2, -1 x k/2) per aggregate (exactly, by support), the
+    reference's non-chiral block vectors lying in the chiral span, the 9-point
+    links equal to the dense P^dag A P, and a chirality-block-diagonal coarse gamma5
+    at the next level too."""
+    L, nv = 12, 3
+    V, k = L * L, 2 * nv
+    U = random_gauge(L, 0.4, 21)
+    nulls = gaussian_vecs(nv, 2 * V, 22)
+    P = O.Prolongator(L, L, 2, 4, 4, k, chiral=True).build(nulls.ravel())
+    Pm = np.stack([P.prolong(np.eye(P.coarse_dim)[j]) for j in range(P.coarse_dim)], 1)
+    assert np.abs(Pm.conj().T @ Pm - np.eye(P.coarse_dim)).max() < 1e-13
+    g5 = np.tile([1.0, -1.0], V)
+    g5c = np.tile(np.r_[np.ones(nv), -np.ones(nv)], P.nblocks)
+    assert np.array_equal(g5[:, None] * Pm, Pm * g5c[None, :])                 # exact
+    Pn = O.Prolongator(L, L, 2, 4, 4, nv).build(nulls.ravel())                 # the reference's aggregation
+    Pnm = np.stack([Pn.prolong(np.eye(Pn.coarse_dim)[j]) for j in range(Pn.coarse_dim)], 1)
+    assert np.abs(Pm @ (Pm.conj().T @ Pnm) - Pnm).max() < 1e-12                # span(P_nonchiral) in span(P_chiral)
+    A = O.Dirac(L, U, 0.1, 1.0, 1.0, 0.0).op()
+    Am = np.stack([A(np.eye(2 * V)[j]) for j in range(2 * V)], 1)
+    C = O.Coarse(A, P, 1)
+    Cm = np.stack([C.op()(np.eye(P.coarse_dim)[j]) for j in range(P.coarse_dim)], 1)
+    assert close(Cm, Pm.conj().T @ Am @ Pm, 1e-12)
+    # next level: chirality of a coarse dof is its half of the k dof of its site
+    nb = L // 4
+    nulls2 = gaussian_vecs(nv, P.coarse_dim, 23)
+    P2 = O.Prolongator(nb, nb, k, 3, 3, k, chiral=True).build(nulls2.ravel())
+    P2m = np.stack([P2.prolong(np.eye(P2.coarse_dim)[j]) for j in range(P2.coarse_dim)], 1)
+    g5c2 = np.tile(np.r_[np.ones(nv), -np.ones(nv)], P2.nblocks)
+    assert np.array_equal(g5c[:, None] * P2m, P2m * g5c2[None, :])
+    assert np.abs(P2m.conj().T @ P2m - np.eye(P2.coarse_dim)).max() < 1e-13
+
+
 def test_antiperiodic_link_flip_leaves_gauge_quantities():
     L = 8
     U = random_gauge(L, 0.5, 13)
