@@ -40,20 +40,24 @@ def parse():
     ap.add_argument("--L", type=int, default=512)
     ap.add_argument("--mass", type=float, default=0.05)
     ap.add_argument("--sigma", type=float, default=0.4)
-    ap.add_argument("--levels", type=int, default=5)
-    ap.add_argument("--k", type=int, default=8)
+    ap.add_argument("--levels", type=int, default=3)
+    ap.add_argument("--k", type=int, default=32, help="coarse dof per aggregate (2 x null vectors when chiral)")
+    ap.add_argument("--chiral", type=int, default=1)
+    ap.add_argument("--w-cycle", type=int, default=0)
     ap.add_argument("--null-iters", type=int, default=10)
     ap.add_argument("--pre", type=int, default=3)
     ap.add_argument("--post", type=int, default=3)
-    ap.add_argument("--coarse-maxit", type=int, default=50)
-    ap.add_argument("--coarse-tol", type=float, default=1e-2)
-    ap.add_argument("--coarse-solver", type=int, default=1)
+    ap.add_argument("--coarse-maxit", type=int, default=30)
+    ap.add_argument("--coarse-tol", type=float, default=1e-1)
+    ap.add_argument("--coarse-solver", type=int, default=0)
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--trunc", type=int, default=10)
     ap.add_argument("--inner-tol", type=float, default=1e-5)
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--nrhs", type=int, default=32)
     ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--single", type=int, default=1, help="also time one right-hand side at a time")
+    ap.add_argument("--fp64", type=int, default=1, help="also time the FP64 batched solve")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
 
@@ -133,7 +137,8 @@ def main():
         b = 4 if lev == 0 else 2
         nx //= b
         dims.append(nx * nx * args.k)
-    starts = np.concatenate([gaussian_vecs(args.k, n, seed + 10 + i).ravel() for i, n in enumerate(dims[:-1])])
+    nvec = args.k // 2 if args.chiral else args.k
+    starts = np.concatenate([gaussian_vecs(nvec, n, seed + 10 + i).ravel() for i, n in enumerate(dims[:-1])])
 
     D = smg.DiracOp(L, U, args.mass, 1.0, 1.0, 0.0)
     blocks = ((4, 4),) + ((2, 2),) * (args.levels - 2)
@@ -141,7 +146,8 @@ def main():
     mg = smg.MGHierarchy(D, starts, n_levels=args.levels, blocks=blocks, k=(args.k,) * (args.levels - 1),
                          null_iters=args.null_iters, pre_smooth=args.pre, post_smooth=args.post,
                          coarse_maxit=args.coarse_maxit, coarse_tol=args.coarse_tol, coarse_half=True,
-                         coarse_solver=args.coarse_solver, use_graph=bool(args.graph))
+                         coarse_solver=args.coarse_solver, use_graph=bool(args.graph), chiral=bool(args.chiral),
+                         w_cycle=bool(args.w_cycle))
     smg.synchronize()
     setup_s = time.time() - t0
     log(f"MG setup {setup_s:.2f} s")
@@ -173,11 +179,13 @@ def main():
     mixed = {"iterations": info.iterations, "restarts": info.restarts, "max_true_residual": float(tres.max())}
     log(f"mixed x{N}: {t_mixed * 1e3:.1f} ms/batch = {N / t_mixed:.1f} solves/s, {mixed}")
     X_mixed = Xd.cpu().numpy()
-    t_f64 = timed(lambda: solve_batch(smg.F64), args.fp64_steps, 1)
-    fp64 = {"iterations": info.iterations, "max_true_residual": float(tres.max())}
-    log(f"fp64 x{N}: {t_f64 * 1e3:.1f} ms/batch = {N / t_f64:.1f} solves/s, {fp64}")
-    X_f64 = Xd.cpu().numpy()
-    err_vs_f64 = float(np.max(np.linalg.norm(X_mixed - X_f64, axis=1) / np.linalg.norm(X_f64, axis=1)))
+    t_f64, fp64, err_vs_f64 = float("nan"), None, None
+    if args.fp64:
+        t_f64 = timed(lambda: solve_batch(smg.F64), args.fp64_steps, 1)
+        fp64 = {"iterations": info.iterations, "max_true_residual": float(tres.max())}
+        log(f"fp64 x{N}: {t_f64 * 1e3:.1f} ms/batch = {N / t_f64:.1f} solves/s, {fp64}")
+        X_f64 = Xd.cpu().numpy()
+        err_vs_f64 = float(np.max(np.linalg.norm(X_mixed - X_f64, axis=1) / np.linalg.norm(X_f64, axis=1)))
 
     # one right-hand side at a time (the reference's mode of use)
     b1 = Bd[0].contiguous()
@@ -187,12 +195,14 @@ def main():
         smg._chk(smg.lib().smg_gcr_solve_dev(D.h, mg.h, prec, b1.data_ptr(), x1.data_ptr(), 500, 1e-10, args.trunc,
                                              args.inner_tol, ctypes.byref(info), None))
 
-    t_one_mixed = timed(lambda: solve_one(smg.F32), 2, 1)
-    one = {"mixed_solves_per_s": round(1.0 / t_one_mixed, 3), "mixed_iterations": info.iterations,
-           "true_residual": info.true_residual}
-    t_one_f64 = timed(lambda: solve_one(smg.F64), 1, 1)
-    one["fp64_solves_per_s"] = round(1.0 / t_one_f64, 3)
-    log(f"single rhs: {one}")
+    one = None
+    if args.single:
+        t_one_mixed = timed(lambda: solve_one(smg.F32), 2, 1)
+        one = {"mixed_solves_per_s": round(1.0 / t_one_mixed, 3), "mixed_iterations": info.iterations,
+               "true_residual": info.true_residual}
+        t_one_f64 = timed(lambda: solve_one(smg.F64), 1, 1)
+        one["fp64_solves_per_s"] = round(1.0 / t_one_f64, 3)
+        log(f"single rhs: {one}")
 
     stencil = [time_stencil(smg, torch, D, p, n, V, True) for p, n in
                [("f32", 1), ("f64", 1), ("f32", N), ("f64", N)]]
@@ -229,9 +239,9 @@ def main():
                    "coarse_maxit": args.coarse_maxit, "coarse_tol": args.coarse_tol,
                    "coarse_solver": "dense inverse" if args.coarse_solver == 1 else "CG", "graph": bool(args.graph)},
         "mixed": mixed,
-        "fp64_solves_per_s": round(N / t_f64, 3),
+        "fp64_solves_per_s": round(N / t_f64, 3) if fp64 else None,
         "fp64": fp64,
-        "speedup_mixed_vs_fp64": round(t_f64 / t_mixed, 3),
+        "speedup_mixed_vs_fp64": round(t_f64 / t_mixed, 3) if fp64 else None,
         "single_rhs": one,
         "mixed_vs_fp64_solution_relerr": err_vs_f64,
         "setup_s": round(setup_s, 3),

@@ -37,9 +37,43 @@ __device__ __forceinline__ int lg2(int N) { return 31 - __clz(N); }
 // group g; each is reset to zero by the workgroup that consumed it.
 constexpr int kGroup = 32;
 
+// Column sums of rows [r0, r1) of a row-major table of tot columns, every column
+// summed in a fixed order with all threads of the workgroup: thread t adds rows
+// r0 + q, r0 + q + Q, ... of column t % tot (q = t / tot, Q = kBlock / tot parts),
+// then the Q parts are combined in order through LDS.  out[c] (LDS) receives the
+// sums.  At most 8 loads per thread are in flight, which keeps the register
+// footprint of the calling (streaming) kernel small.
+__device__ __forceinline__ void column_sums(const double* tab, int r0, int r1, int tot, double* red, double* out) {
+    if (tot >= kBlock) {
+        for (int c = threadIdx.x; c < tot; c += kBlock) {
+            double a = 0.0;
+#pragma unroll 8
+            for (int i = r0; i < r1; i++) a += part_load(tab + (size_t)i * tot + c);
+            out[c] = a;
+        }
+        __syncthreads();
+        return;
+    }
+    const int Q = kBlock / tot, c = threadIdx.x % tot, q = threadIdx.x / tot;
+    double a = 0.0;
+    if (q < Q) {
+#pragma unroll 8
+        for (int i = r0 + q; i < r1; i += Q) a += part_load(tab + (size_t)i * tot + c);
+    }
+    red[threadIdx.x] = a;
+    __syncthreads();
+    if (threadIdx.x < tot) {
+        double sum = 0.0;
+        for (int qq = 0; qq < Q; qq++) sum += red[qq * tot + threadIdx.x];
+        out[threadIdx.x] = sum;
+    }
+    __syncthreads();
+}
+
 template <int NC, int NMAX>
 __device__ __forceinline__ void reduce_finish_b(double (&v)[NC], const FinB& f, int N, int nblk, int blk) {
     __shared__ double sm[kBlock / 64][NC * NMAX];
+    __shared__ double red[kBlock];
     __shared__ int last;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int off = N; off < 64; off <<= 1)
@@ -59,16 +93,10 @@ __device__ __forceinline__ void reduce_finish_b(double (&v)[NC], const FinB& f, 
     if (threadIdx.x == 0) last = (counter_arrive(f.counter + 1 + g) == (unsigned)gn - 1u);
     __syncthreads();
     if (!last) return;
+    double* out = &sm[0][0];   // tot <= NC * NMAX entries
     double* grow = f.part + (size_t)nblk * tot;   // group rows follow the workgroup rows
-    for (int t = threadIdx.x; t < tot; t += kBlock) {
-        double r[kGroup];
-#pragma unroll
-        for (int i = 0; i < kGroup; i++) r[i] = i < gn ? part_load(f.part + (size_t)(g0 + i) * tot + t) : 0.0;
-        double a = r[0];
-#pragma unroll
-        for (int i = 1; i < kGroup; i++) a += r[i];
-        part_store(grow + (size_t)g * tot + t, a);
-    }
+    column_sums(f.part, g0, g0 + gn, tot, red, out);
+    for (int t = threadIdx.x; t < tot; t += kBlock) part_store(grow + (size_t)g * tot + t, out[t]);
     stores_complete();
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -77,24 +105,12 @@ __device__ __forceinline__ void reduce_finish_b(double (&v)[NC], const FinB& f, 
     }
     __syncthreads();
     if (!last) return;
-    double* acc = &sm[0][0];   // tot <= NC * NMAX entries
-    for (int t = threadIdx.x; t < tot; t += kBlock) {
-        double r[kGroup];
-        double a = 0.0;
-        for (int i0 = 0; i0 < ngrp; i0 += kGroup) {
-#pragma unroll
-            for (int i = 0; i < kGroup; i++) r[i] = i0 + i < ngrp ? part_load(grow + (size_t)(i0 + i) * tot + t) : 0.0;
-#pragma unroll
-            for (int i = 0; i < kGroup; i++) a += r[i];
-        }
-        acc[t] = a;
-    }
-    __syncthreads();
+    column_sums(grow, 0, ngrp, tot, red, out);
     if (threadIdx.x < N) {
         const int j = threadIdx.x;
         double w[NC];
 #pragma unroll
-        for (int c = 0; c < NC; c++) w[c] = acc[c * N + j];
+        for (int c = 0; c < NC; c++) w[c] = out[c * N + j];
         fin_program(f.op, w, NC, f.out + (size_t)j * f.ostride, f.aux ? f.aux + (size_t)j * f.astride : nullptr);
     }
     if (threadIdx.x == 0) counter_reset(f.counter);
