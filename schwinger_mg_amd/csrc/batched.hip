@@ -7,6 +7,8 @@
 // by xor-shuffles over the lane bits above log2(N), then across the waves in LDS;
 // the last workgroup sums the partial rows in index order and runs the scalar
 // program of every RHS.
+#include <type_traits>
+
 #include "smg_batched.h"
 
 namespace smg {
@@ -113,6 +115,14 @@ __device__ __forceinline__ void reduce_finish_b(double (&v)[NC], const FinB& f, 
         for (int c = 0; c < NC; c++) w[c] = out[c * N + j];
         fin_program(f.op, w, NC, f.out + (size_t)j * f.ostride, f.aux ? f.aux + (size_t)j * f.astride : nullptr);
     }
+    if (f.done) {   // all RHS converged?  (flags written by this workgroup's threads above)
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int all = 1;
+            for (int j = 0; j < N; j++) all &= *reinterpret_cast<const int*>(f.aux + (size_t)j * f.astride + 8) != 0;
+            *f.done = all;
+        }
+    }
     if (threadIdx.x == 0) counter_reset(f.counter);
 }
 
@@ -207,11 +217,17 @@ void launch_b_dslash(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T 
 }
 
 // ----------------------------------------------------------------- coarse operator
+// link i of a coarse view in arithmetic type T
+template <typename T, typename TL>
+__device__ __forceinline__ cplx<T> link_at(const CoarseView<TL>& C, size_t i) {
+    return cvt<T>(load_c(C.link + i));
+}
 template <typename T, typename TL, int EPI>
 __global__ __launch_bounds__(kBlock) void k_b_coarse(CoarseView<TL> C, cplx<T>* __restrict__ dst,
                                                      const cplx<T>* __restrict__ src,
                                                      const cplx<T>* __restrict__ bvec,
                                                      const cplx<T>* __restrict__ aux, FinB fin, int N) {
+    if (EPI == EPI_DOTS && fin.skip && *fin.skip) return;   // CG finished for every RHS
     const int k = C.k, lN = lg2(N);
     const size_t items = ((size_t)C.nx * C.ny * k) << lN;
     double acc[3] = {0.0, 0.0, 0.0};
@@ -226,10 +242,10 @@ __global__ __launch_bounds__(kBlock) void k_b_coarse(CoarseView<TL> C, cplx<T>* 
                             xp + yp * C.nx, xp + ym * C.nx, xm + yp * C.nx, xm + ym * C.nx};
         cplx<T> out = {(T)0, (T)0};
         for (int d = 0; d < 9; d++) {
-            const TL* lrow = C.link + (((size_t)b * 9 + d) * k + ki) * k;
+            const size_t lrow = (((size_t)b * 9 + d) * k + ki) * k;
             const cplx<T>* sv = src + ((size_t)nbr[d] * k) * N + j;
             cplx<T> sum = {(T)0, (T)0};
-            for (int kj = 0; kj < k; kj++) sum = sum + cvt<T>(load_c(lrow + kj)) * sv[(size_t)kj * N];
+            for (int kj = 0; kj < k; kj++) sum = sum + link_at<T>(C, lrow + kj) * sv[(size_t)kj * N];
             out = out + sum;
         }
         if (EPI == EPI_RESID) out = bvec[t] - out;
@@ -303,6 +319,7 @@ __global__ __launch_bounds__(kBlock) void k_b_coarse_mfma(CoarseView<TL> C, cplx
                                                           const cplx<T>* __restrict__ src,
                                                           const cplx<T>* __restrict__ bvec,
                                                           const cplx<T>* __restrict__ aux, FinB fin) {
+    if (EPI == EPI_DOTS && fin.skip && *fin.skip) return;   // CG finished for every RHS
     constexpr int N = 16 * NT, RT = K / 8, Q = K / 4;
     constexpr bool PAIR = sizeof(T) == 4 && NT % 2 == 0;
     constexpr int NL = PAIR ? NT / 2 : NT;   // source loads per (kj) per lane
@@ -502,6 +519,236 @@ __global__ __launch_bounds__(kBlock) void k_b_coarse_mfma(CoarseView<TL> C, cplx
     }
 }
 
+// ----------------------------------------------------------------- bf16x3 coarse operator
+// FP32 coarse apply with FP16 links on the bf16 matrix cores (16x16x32, FP32
+// accumulation).  Every FP32 operand x is split as x = hi + lo with hi = bf16(x)
+// and lo = bf16(x - hi); the product keeps hi*hi + hi*lo + lo*hi, dropping
+// lo*lo (relative size 2^-16).  FP16 links are exact as hi + lo (11 significant
+// bits), so only the sources are truncated, to about 2^-17 relative.  bf16 has
+// FP32's exponent range, so no scaling is needed.  Three bf16 MFMAs cost about
+// 3/16 of the FP32 MFMAs they replace.
+//
+// The 9 directions x K link columns of a coarse site form one K axis of 9K complex
+// columns (d, kj), taken 16 at a time per MFMA: lane group hi holds the four
+// columns 16m + 4hi + (0..3) (one direction, four consecutive kj), each as the
+// real pair (Re, Im) -> 8 bf16 operands.  Lane layout of v_mfma_f32_16x16x32_bf16
+// (checked on gfx950): A[i][k] in lane i + 16 (k/8), element k%8; B[k][j] in lane
+// j + 16 (k/8), element k%8; D as the FP32 16x16x4 tile.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Split-precision operands by bit manipulation: hi is x with the low 16 bits of
+// its FP32 pattern cleared (a bf16, truncated), lo = x - hi (exact in FP32, its
+// upper 16 bits are the bf16 carried).  pack(a, b) places the bf16 upper halves
+// of a and b in the low / high half of a dword: v_perm_b32 with bytes (7,6,3,2)
+// of [b:a]; with the selector 0x03020706 the halves are swapped.
+__device__ __forceinline__ unsigned hi16(float x) { return __float_as_uint(x) & 0xffff0000u; }
+__device__ __forceinline__ float lo_part(float x) { return x - __uint_as_float(hi16(x)); }
+__device__ __forceinline__ unsigned pack_bf(float a, float b, unsigned sel) {
+    return __builtin_amdgcn_perm(__float_as_uint(b), __float_as_uint(a), sel);
+}
+
+// One wave per coarse site; the 9K/16 chunks are pipelined, chunk m + 1's raw
+// sources and links in flight while chunk m is split and multiplied.
+template <typename TL, int EPI, int K, int NT>
+__global__ __launch_bounds__(kBlock) void k_b_coarse_bf16(CoarseView<TL> C, c32* __restrict__ dst,
+                                                          const c32* __restrict__ src, const c32* __restrict__ bvec,
+                                                          const c32* __restrict__ aux, FinB fin) {
+    static_assert(std::is_same<TL, ch16>::value, "bf16x3 path: FP16 links");
+    if (EPI == EPI_DOTS && fin.skip && *fin.skip) return;   // CG finished for every RHS
+    constexpr int N = 16 * NT, RT = K / 8, NCH = (9 * K + 15) / 16;
+    constexpr bool PAIR = NT % 2 == 0;
+    constexpr int NL = PAIR ? NT / 2 : NT;
+    const int lane = threadIdx.x & 63, lo = lane & 15, hi = lane >> 4;
+    const int nx = C.nx, ny = C.ny, nsites = nx * ny;
+    const int nw = kBlock / 64;
+    const int a_ki = lo >> 1, a_c = lo & 1;   // FP32 tile rows: i = 4g + v <-> (ki = 2g + v/2, c = v%2)
+    // real form of row part c for the column pair (c' = 0, 1): c = 0 -> (Re L, -Im L), c = 1 -> (Im L, Re L)
+    const unsigned neg_im = a_c ? 0u : 0x80000000u;
+    const unsigned sel_a = a_c ? 0x03020706u : 0x07060302u;
+    const unsigned* links = reinterpret_cast<const unsigned*>(C.link);
+    double dacc[NT][3];
+#pragma unroll
+    for (int t = 0; t < NT; t++) dacc[t][0] = dacc[t][1] = dacc[t][2] = 0.0;
+    for (int b = blockIdx.x * nw + (threadIdx.x >> 6); b < nsites; b += gridDim.x * nw) {
+        const int bx = b % nx, by = b / nx;
+        f32x4 acc[RT][NT];
+#pragma unroll
+        for (int r = 0; r < RT; r++)
+#pragma unroll
+            for (int t = 0; t < NT; t++) acc[r][t] = f32x4{0, 0, 0, 0};
+        // raw operands of one chunk: the sources of four link columns (both RHS
+        // tiles of a pair in one 16-byte load) and the row-tile link runs
+        using SrcV = typename std::conditional<PAIR, float4, float2>::type;
+        struct Raw {
+            SrcV s[4][NL];
+            uint4 l[RT];
+        };
+        auto load_raw = [&](Raw& R, int m) {
+            const int cc0 = 16 * m + 4 * hi;
+            const bool valid = (9 * K) % 16 == 0 || cc0 < 9 * K;
+            const int cc = valid ? cc0 : 0;
+            const int d = cc / K, kj0 = cc % K;
+            const int dx = (d == 1 || d == 5 || d == 6) ? 1 : ((d == 2 || d == 7 || d == 8) ? -1 : 0);
+            const int dy = (d == 3 || d == 5 || d == 7) ? 1 : ((d == 4 || d == 6 || d == 8) ? -1 : 0);
+            const int nb = (bx + dx + nx) % nx + ((by + dy + ny) % ny) * nx;
+#pragma unroll
+            for (int e2 = 0; e2 < 4; e2++) {
+                const c32* row = src + ((size_t)nb * K + kj0 + e2) * N;
+#pragma unroll
+                for (int u = 0; u < NL; u++)
+                    R.s[e2][u] = *reinterpret_cast<const SrcV*>(row + (PAIR ? 32 * u + 2 * lo : 16 * u + lo));
+            }
+            const unsigned* lrow = links + ((size_t)b * 9 + d) * (K * K) + a_ki * K + kj0;
+#pragma unroll
+            for (int r = 0; r < RT; r++) R.l[r] = *reinterpret_cast<const uint4*>(lrow + 8 * r * K);
+            if (!valid) {   // the partial last chunk (k = 8): these columns do not exist
+#pragma unroll
+                for (int e2 = 0; e2 < 4; e2++)
+#pragma unroll
+                    for (int u = 0; u < NL; u++) R.s[e2][u] = SrcV{};
+#pragma unroll
+                for (int r = 0; r < RT; r++) R.l[r] = uint4{0u, 0u, 0u, 0u};
+            }
+        };
+        auto mma_chunk = [&](const Raw& R) {
+            u32x4 bh[NT], bl[NT];
+#pragma unroll
+            for (int e2 = 0; e2 < 4; e2++) {
+                c32 sv[NT];
+#pragma unroll
+                for (int u = 0; u < NL; u++) {
+                    if constexpr (PAIR) {
+                        const float4 w = R.s[e2][u];
+                        sv[2 * u] = c32{w.x, w.y};
+                        sv[2 * u + 1] = c32{w.z, w.w};
+                    } else {
+                        const float2 w = R.s[e2][u];
+                        sv[u] = c32{w.x, w.y};
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    bh[t][e2] = pack_bf(sv[t].re, sv[t].im, 0x07060302u);
+                    bl[t][e2] = pack_bf(lo_part(sv[t].re), lo_part(sv[t].im), 0x07060302u);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < RT; r++) {
+                const unsigned lws[4] = {R.l[r].x, R.l[r].y, R.l[r].z, R.l[r].w};
+                u32x4 ah, al;
+#pragma unroll
+                for (int e2 = 0; e2 < 4; e2++) {
+                    const float re = __half2float(__ushort_as_half((unsigned short)(lws[e2] & 0xffffu)));
+                    const float im = __uint_as_float(
+                        __float_as_uint(__half2float(__ushort_as_half((unsigned short)(lws[e2] >> 16)))) ^ neg_im);
+                    ah[e2] = pack_bf(re, im, sel_a);
+                    al[e2] = pack_bf(lo_part(re), lo_part(im), sel_a);
+                }
+                const bf16x8 ahv = __builtin_bit_cast(bf16x8, ah), alv = __builtin_bit_cast(bf16x8, al);
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    const bf16x8 bhv = __builtin_bit_cast(bf16x8, bh[t]), blv = __builtin_bit_cast(bf16x8, bl[t]);
+                    acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahv, bhv, acc[r][t], 0, 0, 0);
+                    acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahv, blv, acc[r][t], 0, 0, 0);
+                    acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alv, bhv, acc[r][t], 0, 0, 0);
+                }
+            }
+        };
+        Raw raw[2];
+        load_raw(raw[0], 0);
+#pragma unroll 2
+        for (int m = 0; m < NCH; m++) {
+            if (m + 1 < NCH) load_raw(raw[(m + 1) & 1], m + 1);
+            __builtin_amdgcn_sched_barrier(0);
+            mma_chunk(raw[m & 1]);
+        }
+#pragma unroll
+        for (int r = 0; r < RT; r++)
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int ki = 8 * r + 2 * hi + h;
+                const size_t row = ((size_t)b * K + ki) * N;
+#pragma unroll
+                for (int u = 0; u < NL; u++) {
+                    if constexpr (PAIR) {
+                        const size_t idx = row + 32 * u + 2 * lo;
+                        c32 o0{acc[r][2 * u][2 * h], acc[r][2 * u][2 * h + 1]};
+                        c32 o1{acc[r][2 * u + 1][2 * h], acc[r][2 * u + 1][2 * h + 1]};
+                        if (EPI == EPI_RESID) {
+                            const float4 w = *reinterpret_cast<const float4*>(bvec + idx);
+                            o0 = c32{w.x, w.y} - o0;
+                            o1 = c32{w.z, w.w} - o1;
+                        }
+                        *reinterpret_cast<float4*>(dst + idx) = float4{o0.re, o0.im, o1.re, o1.im};
+                        if (EPI == EPI_DOTS) {
+                            const float4 w = *reinterpret_cast<const float4*>(aux + idx);
+                            const c32 p0 = cdot(o0, c32{w.x, w.y}), p1 = cdot(o1, c32{w.z, w.w});
+                            dacc[2 * u][0] += p0.re;
+                            dacc[2 * u][1] += p0.im;
+                            dacc[2 * u][2] += norm2(o0);
+                            dacc[2 * u + 1][0] += p1.re;
+                            dacc[2 * u + 1][1] += p1.im;
+                            dacc[2 * u + 1][2] += norm2(o1);
+                        }
+                    } else {
+                        const size_t idx = row + 16 * u + lo;
+                        c32 o{acc[r][u][2 * h], acc[r][u][2 * h + 1]};
+                        if (EPI == EPI_RESID) o = bvec[idx] - o;
+                        dst[idx] = o;
+                        if (EPI == EPI_DOTS) {
+                            const c32 p = cdot(o, aux[idx]);
+                            dacc[u][0] += p.re;
+                            dacc[u][1] += p.im;
+                            dacc[u][2] += norm2(o);
+                        }
+                    }
+                }
+            }
+    }
+    if (EPI == EPI_DOTS) {
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                dacc[t][c] += __shfl_xor(dacc[t][c], 16, 64);
+                dacc[t][c] += __shfl_xor(dacc[t][c], 32, 64);
+            }
+        int tj, lj;
+        if constexpr (PAIR) {
+            tj = 2 * ((lane & 63) >> 5) + (lane & 1);
+            lj = (lane & 31) >> 1;
+        } else {
+            tj = (lane >> 4) & (NT - 1);
+            lj = lane & 15;
+        }
+        double v[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const double x = __shfl(dacc[t][c], lj, 64);
+                if (t == tj && lane < N) v[c] = x;
+            }
+        reduce_finish_b<3, 64>(v, fin, N, gridDim.x, blockIdx.x);
+    }
+}
+
+template <typename TL, int K, int NT>
+static void coarse_bf16(const CoarseView<TL>& C, c32* dst, const c32* src, Epi epi, const c32* b, const c32* aux,
+                        const FinB& fin, hipStream_t st) {
+    const size_t waves = (size_t)C.nx * C.ny;
+    const int nw = kBlock / 64;
+    const int g = (int)std::min<size_t>((waves + nw - 1) / nw, epi == EPI_DOTS ? kRedGrid : 1 << 16);
+    if (epi == EPI_NONE)
+        k_b_coarse_bf16<TL, EPI_NONE, K, NT><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
+    else if (epi == EPI_RESID)
+        k_b_coarse_bf16<TL, EPI_RESID, K, NT><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
+    else
+        k_b_coarse_bf16<TL, EPI_DOTS, K, NT><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
+}
+
 template <typename T, typename TL, int K, int NT>
 static void coarse_mfma(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b,
                         const cplx<T>* aux, const FinB& fin, hipStream_t st) {
@@ -519,6 +766,26 @@ static void coarse_mfma(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* sr
 template <typename T, typename TL>
 void launch_b_coarse_apply(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b,
                            const cplx<T>* aux, const FinB& fin, int N, hipStream_t st) {
+    if constexpr (std::is_same<T, float>::value && std::is_same<TL, ch16>::value) {
+        if ((C.k == 8 || C.k == 16 || C.k == 32) && (N == 16 || N == 32 || N == 64)) {
+            // FP16 links: bf16x3 matrix-core path
+            if (C.k == 8) {
+                if (N == 16) coarse_bf16<TL, 8, 1>(C, dst, src, epi, b, aux, fin, st);
+                else if (N == 32) coarse_bf16<TL, 8, 2>(C, dst, src, epi, b, aux, fin, st);
+                else coarse_bf16<TL, 8, 4>(C, dst, src, epi, b, aux, fin, st);
+            } else if (C.k == 16) {
+                if (N == 16) coarse_bf16<TL, 16, 1>(C, dst, src, epi, b, aux, fin, st);
+                else if (N == 32) coarse_bf16<TL, 16, 2>(C, dst, src, epi, b, aux, fin, st);
+                else coarse_bf16<TL, 16, 4>(C, dst, src, epi, b, aux, fin, st);
+            } else {
+                if (N == 16) coarse_bf16<TL, 32, 1>(C, dst, src, epi, b, aux, fin, st);
+                else if (N == 32) coarse_bf16<TL, 32, 2>(C, dst, src, epi, b, aux, fin, st);
+                else coarse_bf16<TL, 32, 4>(C, dst, src, epi, b, aux, fin, st);
+            }
+            SMG_LAUNCH_CHECK();
+            return;
+        }
+    }
     if ((C.k == 8 || C.k == 16 || C.k == 32) && (N == 16 || N == 32 || N == 64)) {
         // matrix-core path
         if (C.k == 8) {
@@ -822,6 +1089,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_update(cplx<T>* __restrict__ x, cplx<T>* __restrict__ r,
                                                      const cplx<T>* __restrict__ p, const cplx<T>* __restrict__ Ap,
                                                      const double* __restrict__ s, size_t items, FinB fin, int N) {
+    if (fin.skip && *fin.skip) return;   // CG finished for every RHS
     double v[1] = {0.0};
     for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < items; t += (size_t)gridDim.x * kBlock) {
         const int j = (int)(t & (N - 1));
@@ -842,7 +1110,8 @@ void launch_b_update(cplx<T>* x, cplx<T>* r, const cplx<T>* p, const cplx<T>* Ap
 
 template <typename T>
 __global__ void k_b_xpay(cplx<T>* __restrict__ p, const cplx<T>* __restrict__ z, const double* __restrict__ s,
-                         size_t items, int N) {
+                         size_t items, int N, const int* __restrict__ skip) {
+    if (skip && *skip) return;
     for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < items; t += (size_t)gridDim.x * kBlock) {
         const int j = (int)(t & (N - 1));
         const cplx<T> b = {(T)s[2 * j], (T)s[2 * j + 1]};
@@ -850,21 +1119,23 @@ __global__ void k_b_xpay(cplx<T>* __restrict__ p, const cplx<T>* __restrict__ z,
     }
 }
 template <typename T>
-void launch_b_xpay(cplx<T>* p, const cplx<T>* z, const double* s, size_t n, int N, hipStream_t st) {
-    k_b_xpay<T><<<b_grid(n * N, 1 << 16), kBlock, 0, st>>>(p, z, s, n * N, N);
+void launch_b_xpay(cplx<T>* p, const cplx<T>* z, const double* s, size_t n, int N, hipStream_t st,
+                   const int* skip) {
+    k_b_xpay<T><<<b_grid(n * N, 1 << 16), kBlock, 0, st>>>(p, z, s, n * N, N, skip);
     SMG_LAUNCH_CHECK();
 }
 
 // per-RHS CG program state: aux_j[5] = tol^2, flag_j = 0
-__global__ void k_b_cg_init(double* aux, int astride, int N, double tol2) {
+__global__ void k_b_cg_init(double* aux, int astride, int N, double tol2, int* done) {
     const int j = threadIdx.x;
     if (j < N) {
         aux[(size_t)j * astride + 5] = tol2;
         *reinterpret_cast<int*>(aux + (size_t)j * astride + 8) = 0;
     }
+    if (j == 0 && done) *done = 0;
 }
-void launch_b_cg_init(double* aux, int astride, int N, double tol2, hipStream_t st) {
-    k_b_cg_init<<<1, 64, 0, st>>>(aux, astride, N, tol2);
+void launch_b_cg_init(double* aux, int astride, int N, double tol2, int* done, hipStream_t st) {
+    k_b_cg_init<<<1, 64, 0, st>>>(aux, astride, N, tol2, done);
     SMG_LAUNCH_CHECK();
 }
 
@@ -949,7 +1220,7 @@ void launch_b_accum_mixed(c64* x, const c32* e, size_t n, int N, hipStream_t st)
     template void launch_b_update<T>(cplx<T>*, cplx<T>*, const cplx<T>*, const cplx<T>*, const double*, size_t,  \
                                      const FinB&, int, hipStream_t);                                             \
     template void launch_b_axpby_real<T>(cplx<T>*, const cplx<T>*, double, double, size_t, hipStream_t);    \
-    template void launch_b_xpay<T>(cplx<T>*, const cplx<T>*, const double*, size_t, int, hipStream_t);
+    template void launch_b_xpay<T>(cplx<T>*, const cplx<T>*, const double*, size_t, int, hipStream_t, const int*);
 SMG_BINST(double)
 SMG_BINST(float)
 #undef SMG_BINST

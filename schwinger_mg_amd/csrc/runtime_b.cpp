@@ -103,13 +103,19 @@ static void cg_b(LinOpB<T>& A, const cplx<T>* b, cplx<T>* x, cplx<T>* r, cplx<T>
     launch_zero<T>(x, nN, st);
     bcopy(r, b, nN, st);
     bcopy(p, b, nN, st);
-    launch_b_cg_init(aux, 16, N, tol * tol, st);
+    // once every RHS has converged the remaining iterations' kernels return at
+    // entry (device-side flag, no host synchronisation)
+    int* done = reinterpret_cast<int*>(aux + 16 * 64);
+    launch_b_cg_init(aux, 16, N, tol * tol, done, st);
     launch_b_norm2<T>(r, n, finb(FIN_CG_INIT, nullptr, 0, aux), N, st);
-    const FinB fa = finb(FIN_CG_ALPHA, alpha, 2, aux), fbt = finb(FIN_CG_BETA, beta, 2, aux);
+    FinB fa = finb(FIN_CG_ALPHA, alpha, 2, aux), fbt = finb(FIN_CG_BETA, beta, 2, aux);
+    fa.skip = done;
+    fbt.skip = done;
+    fbt.done = done;
     for (int it = 0; it < maxit; it++) {
         A.apply(Ap, p, EPI_DOTS, nullptr, p, fa, st);
         launch_b_update<T>(x, r, p, Ap, alpha, n, fbt, N, st);
-        launch_b_xpay<T>(p, r, beta, n, N, st);
+        launch_b_xpay<T>(p, r, beta, n, N, st, done);
     }
 }
 

@@ -20,6 +20,11 @@ struct FinB {
     int astride = 16;
     double* part = nullptr;
     unsigned* counter = nullptr;
+    // CG early exit: the finish of the update writes *done = 1 once every RHS's
+    // convergence flag (int at aux_j + 8) is set; kernels given skip return at
+    // entry while *skip != 0 (grid-uniform, written by an earlier kernel)
+    int* done = nullptr;
+    const int* skip = nullptr;
 };
 
 // Per-RHS coefficient array views (device): complex a_j at s[2j], s[2j+1].
@@ -62,11 +67,12 @@ void launch_b_mr_update(cplx<T>* x, cplx<T>* r, const cplx<T>* Ar, const double*
 template <typename T>
 void launch_b_update(cplx<T>* x, cplx<T>* r, const cplx<T>* p, const cplx<T>* Ap, const double* s, size_t n,
                      const FinB& fin, int N, hipStream_t st);
-// p = z + b_j p (b_j at s[2j..])
+// p = z + b_j p (b_j at s[2j..]); nothing while skip && *skip
 template <typename T>
-void launch_b_xpay(cplx<T>* p, const cplx<T>* z, const double* s, size_t n, int N, hipStream_t st);
-// per-RHS CG program state: aux_j[5] = tol^2, flag_j = 0
-void launch_b_cg_init(double* aux, int astride, int N, double tol2, hipStream_t st);
+void launch_b_xpay(cplx<T>* p, const cplx<T>* z, const double* s, size_t n, int N, hipStream_t st,
+                   const int* skip = nullptr);
+// per-RHS CG program state: aux_j[5] = tol^2, flag_j = 0; *done = 0
+void launch_b_cg_init(double* aux, int astride, int N, double tol2, int* done, hipStream_t st);
 // y = a x + c y with one real scalar pair for all RHS
 template <typename T>
 void launch_b_axpby_real(cplx<T>* y, const cplx<T>* x, double a, double c, size_t n, hipStream_t st);

@@ -118,6 +118,7 @@ void launch_xpay_dev(cplx<T>* p, const cplx<T>* z, const double* s, const int* f
 template <typename To, typename From>
 void launch_convert(cplx<To>* dst, const cplx<From>* src, size_t n, hipStream_t st);
 void launch_convert_half(ch16* dst, const c64* src, size_t n, hipStream_t st);
+
 template <typename T>
 void launch_zero(cplx<T>* p, size_t n, hipStream_t st);
 // dst = src (same precision), as a kernel (graph-capturable)

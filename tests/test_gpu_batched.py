@@ -52,8 +52,12 @@ def test_batched_coarse_apply(smg, k, N):
     want = np.stack([A(S[j]) for j in range(N)])
     assert relerr(C.apply_batched(S, smg.F64), want) < 1e-12
     assert relerr(C.apply_batched(S, smg.F32), want) < 2e-6
+    # FP16 links; with k = 8, 16, 32 and N >= 16 the batched product runs on the bf16
+    # matrix cores with the FP32 sources split into two bf16 terms (about 2^-16)
     want16 = np.stack([C.apply_to(S[j], smg.F16) for j in range(N)])
-    assert relerr(C.apply_batched(S, smg.F16), want16) < 2e-6
+    err16 = relerr(C.apply_batched(S, smg.F16), want16)
+    print(f"k={k} N={N}: FP16-link batched vs single relerr {err16:.2e}, single vs FP64 {relerr(want16, want):.2e}")
+    assert err16 < (1e-4 if N >= 16 else 2e-6)
 
 
 @pytest.mark.parametrize("N", [16, 32])
@@ -95,7 +99,11 @@ def test_batched_cycle_equals_single(smg, coarse_solver, k, N, chiral):
     want64 = np.stack([mg.precondition(B[j], smg.F64) for j in range(N)])
     want32 = np.stack([mg.precondition(B[j], smg.F32) for j in range(N)])
     assert relerr(mg.precondition_batched(B, smg.F64), want64) < 1e-10
-    assert relerr(mg.precondition_batched(B, smg.F32), want32) < 2e-5
+    # both FP32 cycles keep FP16 coarse links; for N >= 16 the batched
+    # coarse products run on the bf16 matrix cores (sources split in two bf16 terms)
+    err32 = relerr(mg.precondition_batched(B, smg.F32), want32)
+    print(f"batched vs single FP32 cycle relerr {err32:.2e}, FP32 vs FP64 cycle {relerr(want32, want64):.2e}")
+    assert err32 < (5e-4 if N >= 16 else 2e-5)
 
 
 @pytest.mark.parametrize("prec,k,N,chiral", [("f64", 4, 8, False), ("mixed", 4, 8, False), ("f64", 8, 32, False),
