@@ -3,14 +3,17 @@
 One step = a batch of N (default 32) Gaussian right-hand sides, each solved to a
 true residual |phi - A x|/|phi| <= 1e-10 (A = D^dag D, the reference's system,
 src/solvers.cpp:55-59) by mixed-precision GCR + multigrid run on all N at once:
-FP32 GCR and FP32 W-cycle with FP16 coarse links inside FP64 iterative
-refinement on the FP64 true residual.  value = N / (seconds per step).  The same
-hierarchy solved in plain FP64 (FP64 GCR + FP64 cycle, also batched) is timed
-beside it, as is the one-right-hand-side solve; the Wilson-clover stencil is
-timed alone (HIP events on the library's stream), and the oracle's FP64 CPU
-MG-GCR solve is timed on the host cores.  Inputs are synthetic (seeded U(1)
-links theta ~ N(0, 0.4^2), Gaussian sources) and resident in HBM when the timed
-region starts.  Prints one JSON line.
+FP32 GCR and an FP32 V-cycle with FP16 coarse links (bf16x3 matrix-core products)
+inside FP64 iterative refinement on the FP64 true residual.  value = N / (seconds
+per step).  The same hierarchy solved in plain FP64 (FP64 GCR + FP64 cycle, also
+batched) is timed beside it, as is the one-right-hand-side solve; the
+Wilson-clover stencil is timed alone (HIP events on the library's stream), and
+the oracle's FP64 CPU MG-GCR solve of one right-hand side on the same hierarchy
+is timed on the host cores.  Inputs are synthetic (seeded U(1) links
+theta ~ N(0, 0.4^2), Gaussian sources) and resident in HBM when the timed region
+starts.  roofline.traffic is the stencil's HBM bytes per launch from the
+FETCH_SIZE / WRITE_SIZE counter runs summarised in profiles/.  Prints one JSON
+line.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1]
 """
@@ -62,6 +65,21 @@ def parse():
     return ap.parse_args()
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_b_dslash_f32_L512_N32_hbm_traffic.md")
+
+
+def profiled_traffic(L, nrhs):
+    """HBM bytes per launch of the FP32 stencil at L=512, N=32, from the counter
+    summary (its line 'traffic_bytes_per_launch: <int>'); None if absent or if the
+    run is at another size."""
+    if L != 512 or nrhs != 32 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        if line.strip().startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1].strip().strip("`"))
+    return None
+
+
 def stencil_bytes_per_site(prec, nrhs, csw):
     sc = 16 if prec == "f64" else 8      # complex
     sr = 8 if prec == "f64" else 4       # real
@@ -70,52 +88,70 @@ def stencil_bytes_per_site(prec, nrhs, csw):
 
 def time_stencil(smg, torch, D, prec, nrhs, V, csw, launches=50, warm=5):
     """Average duration of one Wilson-clover launch on the batched layout (RHS
-    innermost, the layout of the solve), HIP events on the library stream."""
+    innermost, the layout of the solve): a HIP event pair around each launch on
+    the library's stream (the stream the kernel runs on), so host-side gaps
+    between launches are not counted."""
     dt = torch.complex128 if prec == "f64" else torch.complex64
     src = torch.randn(2 * V, nrhs, dtype=dt, device="cuda")
     dst = torch.empty_like(src)
     p = smg.F64 if prec == "f64" else smg.F32
-    st = torch.cuda.ExternalStream(smg.library_stream())
+    sp = smg.library_stream()
+    st = torch.cuda.ExternalStream(sp)
+    import ctypes
+    sarg = ctypes.c_void_p(sp)
+    torch.cuda.synchronize()
 
     def launch():
-        smg._chk(smg.lib().smg_dirac_apply_batched_dev(D.h, smg.D_, p, src.data_ptr(), dst.data_ptr(), nrhs, None))
+        smg._chk(smg.lib().smg_dirac_apply_batched_dev(D.h, smg.D_, p, src.data_ptr(), dst.data_ptr(), nrhs, sarg))
 
     for _ in range(warm):
         launch()
     smg.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(st)
-    for _ in range(launches):
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(launches)]
+    for e0, e1 in ev:
+        e0.record(st)
         launch()
-    e1.record(st)
-    e1.synchronize()
-    ms = e0.elapsed_time(e1) / launches
+        e1.record(st)
+    ev[-1][1].synchronize()
+    ms = sum(e0.elapsed_time(e1) for e0, e1 in ev) / launches
     byt = stencil_bytes_per_site(prec, nrhs, csw) * V
     return {"prec": prec, "nrhs": nrhs, "us_per_launch": ms * 1e3, "bytes_per_launch": byt,
             "GBs": byt / (ms * 1e-3) / 1e9}
 
 
-def cpu_baseline(args, U, rhs, starts, threads):
-    """The oracle's FP64 MG-GCR solve (same hierarchy shape) on the host cores."""
+def cpu_baseline(args, U, B, mg, threads):
+    """The oracle's FP64 MG-GCR solve of one right-hand side on the host cores, on
+    the same hierarchy: the block vectors and coarse links of the GPU setup are
+    copied into the oracle (its own setup, minutes at L=512, is not timed, as the
+    GPU setup is not part of value).  A bounded sample: right-hand sides are solved
+    one at a time (each capped at 30 iterations) until about 10 s have passed."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
     import oracle as O
     L = args.L
     D = O.Dirac(L, U, args.mass, 1.0, 1.0, 0.0)
     A = D.op()
-    blocks = ((4, 4),) + ((2, 2),) * (args.levels - 2)
-    t0 = time.time()
-    mg = O.MG(A, L, L, 2, starts, n_levels=args.levels, block=blocks, k=(args.k,) * (args.levels - 1),
-              null_iters=args.null_iters, pre=args.pre, post=args.post,
-              coarse_solver=1 if args.coarse_solver == 1 else 0,
-              coarse_maxit=args.coarse_maxit, coarse_tol=args.coarse_tol)
-    setup = time.time() - t0
-    t0 = time.time()
-    x, it, res = O.gcr(A, rhs, 200, 1e-10, args.trunc, mg, ref_projection=False)
-    dt = time.time() - t0
-    tres = np.linalg.norm(rhs - A(x)) / np.linalg.norm(rhs)
-    return {"value": 1.0 / dt, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"1 FP64 MG-GCR solve at L={L} after a {setup:.1f} s setup "
-                      f"({it} GCR iterations, true residual {tres:.2e})"}
+    nt = args.levels - 1
+    blocks = ((4, 4),) + ((2, 2),) * (nt - 1)
+    parts = (np.concatenate([mg.block_vecs(l).ravel() for l in range(nt)]),
+             np.concatenate([mg.coarse_links(l).ravel() for l in range(nt)]))
+    omg = O.MG(A, L, L, 2, None, n_levels=args.levels, block=blocks, k=(args.k,) * nt,
+               pre=args.pre, post=args.post, w_cycle=bool(args.w_cycle),
+               coarse_solver=1 if args.coarse_solver == 1 else 0,
+               coarse_maxit=args.coarse_maxit, coarse_tol=args.coarse_tol, chiral=bool(args.chiral), parts=parts)
+    its, tres, dt = [], [], 0.0
+    for rhs in B:   # solves one after another until about 10 s of CPU work
+        t0 = time.time()
+        x, it, res = O.gcr(A, rhs, 30, 1e-10, args.trunc, omg, ref_projection=False)
+        dt += time.time() - t0
+        its.append(it)
+        tres.append(np.linalg.norm(rhs - A(x)) / np.linalg.norm(rhs))
+        if dt >= 10.0:
+            break
+    n = len(its)
+    return {"value": round(n / dt, 5), "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"{n} right-hand side(s) solved one at a time, FP64 MG-GCR({args.trunc}) at L={L} on the same "
+                      f"{args.levels}-level hierarchy (parts copied from the GPU setup), {sum(its)} iterations in "
+                      f"{dt:.1f} s, max true residual {max(tres):.2e}"}
 
 
 def main():
@@ -209,14 +245,15 @@ def main():
     dom = stencil[2]     # FP32, N RHS: the fine-level stencil of the mixed solve
     log("stencil: " + ", ".join(f"{s['prec']}x{s['nrhs']} {s['GBs']:.0f} GB/s" for s in stencil))
     roofline = {"bound": "hbm", "achieved": round(dom["GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(L, N),
+                "algorithmic_bytes_per_launch": dom["bytes_per_launch"], "avg_launch_us": round(dom["us_per_launch"], 2),
                 "kernel": f"k_b_dslash<float> (Wilson-clover D, {N} RHS, L={L})"}
 
     cpu = None
     if args.cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", 0)) or min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads ...")
-        cpu = cpu_baseline(args, U, B[0], starts, threads)
+        cpu = cpu_baseline(args, U, B[:8], mg, threads)
         log(f"cpu baseline: {cpu}")
 
     out = {
@@ -233,8 +270,13 @@ def main():
         "dtype": "f32 (FP16 coarse links) in f64 refinement",
         "data": "synthetic",
         "config": {"workload": f"L={L} Wilson-clover m={args.mass} c_sw=1, theta~N(0,{args.sigma}^2), {N} RHS, "
-                               f"{args.levels}-level MG (4x4 then 2x2 aggregates, k={args.k}), GCR({args.trunc}), "
-                               "tol 1e-10 true residual",
+                               f"{args.levels}-level MG (4x4 then 2x2 aggregates, "
+                               + (f"{args.k // 2} null vectors split by chirality, k={args.k}" if args.chiral
+                                  else f"k={args.k}")
+                               + f", {'W' if args.w_cycle else 'V'}-cycle, MR({args.pre},{args.post}), coarsest "
+                               + ("dense inverse" if args.coarse_solver == 1
+                                  else f"CG to {args.coarse_tol:g} (<= {args.coarse_maxit} it)")
+                               + f"), GCR({args.trunc}), tol 1e-10 true residual",
                    "L": L, "nrhs": N, "levels": args.levels, "k": args.k, "pre": args.pre, "post": args.post,
                    "coarse_maxit": args.coarse_maxit, "coarse_tol": args.coarse_tol,
                    "coarse_solver": "dense inverse" if args.coarse_solver == 1 else "CG", "graph": bool(args.graph)},

@@ -94,6 +94,8 @@ def lib():
         L.orc_null_space.argtypes = [_P, _I, _I, _P, _P]
         L.orc_mg_new.restype = _P
         L.orc_mg_new.argtypes = [_P, _I, _I, _I, _P, _P]
+        L.orc_mg_assemble.restype = _P
+        L.orc_mg_assemble.argtypes = [_P, _I, _I, _I, _P, _P, _P]
         L.orc_mg_free.argtypes = [_P]
         L.orc_mg_set_symmetric.argtypes = [_P, _D, _P]
         L.orc_mg_precondition.argtypes = [_P, _P, _P]
@@ -361,7 +363,10 @@ def null_space(op, k, outer_iters, starts):
 class MG:
     def __init__(self, fine_op, nx, ny, nd, starts, n_levels=2, block=((4, 4),), k=(4,),
                  null_iters=20, pre=3, post=3, w_cycle=True, coarse_solver=0,
-                 coarse_maxit=200, coarse_tol=1e-12, colored=1, chiral=False):
+                 coarse_maxit=200, coarse_tol=1e-12, colored=1, chiral=False, parts=None):
+        """parts = (block_vecs, links): the per-level block vectors and coarse links,
+        each concatenated over levels, to assemble the hierarchy from instead of
+        running the setup (starts is then unused)."""
         p = MGParams()
         p.n_levels = n_levels
         for i, (bx, by) in enumerate(block):
@@ -374,9 +379,15 @@ class MG:
         p.colored_build = colored
         p.chiral = int(chiral)
         self.prm = p
-        self.starts = _c(starts)
         self.fine = fine_op
-        self.h = lib().orc_mg_new(fine_op.h, nx, ny, nd, ctypes.byref(p), _ptr(self.starts))
+        self.nx, self.ny = nx, ny
+        if parts is None:
+            self.starts = _c(starts)
+            self.h = lib().orc_mg_new(fine_op.h, nx, ny, nd, ctypes.byref(p), _ptr(self.starts))
+        else:
+            self.parts = (_c(parts[0]), _c(parts[1]))
+            self.h = lib().orc_mg_assemble(fine_op.h, nx, ny, nd, ctypes.byref(p), _ptr(self.parts[0]),
+                                           _ptr(self.parts[1]))
         self.n = fine_op.n
 
     def __del__(self):
@@ -395,6 +406,27 @@ class MG:
 
     def omega(self, level):
         return lib().orc_mg_omega(self.h, level)
+
+    def _shape(self, level):
+        """(nblocks, k, block size) of transfer `level`."""
+        nx, ny, nd = self.nx, self.ny, 2
+        for l in range(level + 1):
+            bx, by, k = self.prm.bx[l], self.prm.by[l], self.prm.k[l]
+            if l < level:
+                nx, ny, nd = nx // bx, ny // by, k
+        return (nx // bx) * (ny // by), k, bx * by * nd
+
+    def block_vecs(self, level):
+        nb, k, bs = self._shape(level)
+        out = np.zeros(nb * k * bs, np.complex128)
+        lib().orc_prolong_get(lib().orc_mg_prolong(self.h, level), _ptr(out))
+        return out
+
+    def coarse_links(self, level):
+        nb, k, _ = self._shape(level)
+        out = np.zeros(nb * 9 * k * k, np.complex128)
+        lib().orc_coarse_get(lib().orc_mg_coarse(self.h, level), _ptr(out))
+        return out
 
 
 def cg(op, rhs, maxit, tol, x0=None, mg=None):

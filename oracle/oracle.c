@@ -891,6 +891,41 @@ orc_mg* orc_mg_new(const orc_op* fine, int nx, int ny, int nd, const orc_mg_para
         mg->dense = orc_dense_build(mg->op[mg->nlev - 1], NULL);
     return mg;
 }
+/* The hierarchy of orc_mg_new with its parts given instead of computed: per
+ * transfer l the block vectors (orc_prolong_get layout) and the coarse links
+ * (orc_coarse_get layout), each concatenated over levels.  The cycle is
+ * src/multigrid.cpp:43-102 as in orc_mg_new; used to time the solve on a
+ * hierarchy set up elsewhere. */
+orc_mg* orc_mg_assemble(const orc_op* fine, int nx, int ny, int nd, const orc_mg_params* prm,
+                        const ocx* block_vecs, const ocx* links) {
+    orc_mg* mg = xcalloc(1, sizeof(orc_mg));
+    mg->prm = *prm;
+    mg->nlev = prm->n_levels;
+    mg->op[0] = fine;
+    mg->dim[0] = fine->n;
+    int cnx = nx, cny = ny, cnd = nd;
+    size_t boff = 0, loff = 0;
+    for (int l = 0; l + 1 < mg->nlev; l++) {
+        int k = prm->k[l];
+        mg->P[l] = orc_prolong_new(cnx, cny, cnd, prm->bx[l], prm->by[l], k, prm->chiral);
+        orc_prolong_set(mg->P[l], block_vecs + boff);
+        boff += (size_t)mg->P[l]->nblocks * k * mg->P[l]->bsize;
+        orc_coarse* C = xcalloc(1, sizeof(orc_coarse));
+        C->nx = mg->P[l]->nbx; C->ny = mg->P[l]->nby; C->k = k; C->nb = mg->P[l]->nblocks;
+        C->link = xcalloc((size_t)C->nb * 9 * k * k, sizeof(ocx));
+        orc_coarse_set(C, links + loff);
+        loff += (size_t)C->nb * 9 * k * k;
+        mg->C[l] = C;
+        mg->own_op[l + 1] = orc_op_coarse(C);
+        mg->op[l + 1] = mg->own_op[l + 1];
+        mg->dim[l + 1] = C->nb * k;
+        cnx = C->nx; cny = C->ny; cnd = k;
+    }
+    if (prm->coarse_solver == 1 && mg->nlev >= 2)
+        mg->dense = orc_dense_build(mg->op[mg->nlev - 1], NULL);
+    return mg;
+}
+
 void orc_mg_free(orc_mg* mg) {
     if (!mg) return;
     for (int l = 0; l < MAXLEV; l++) {
@@ -906,7 +941,8 @@ const orc_coarse* orc_mg_coarse(const orc_mg* mg, int l) { return mg->C[l]; }
 double orc_mg_omega(const orc_mg* mg, int l) { return mg->omega[l]; }
 void orc_mg_null_vecs(const orc_mg* mg, int l, ocx* out) {
     int nv = mg->prm.chiral ? mg->prm.k[l] / 2 : mg->prm.k[l];
-    memcpy(out, mg->nullv[l], (size_t)nv * mg->dim[l] * sizeof(ocx));
+    if (mg->nullv[l]) memcpy(out, mg->nullv[l], (size_t)nv * mg->dim[l] * sizeof(ocx));
+    else memset(out, 0, (size_t)nv * mg->dim[l] * sizeof(ocx));   /* assembled from parts */
 }
 
 /* src/multigrid.cpp:146-152 with explicit power-iteration starts per level */

@@ -116,6 +116,10 @@ typedef struct {
  * every level, concatenated (level 0 first). */
 orc_mg* orc_mg_new(const orc_op* fine, int nx, int ny, int nd, const orc_mg_params* prm,
                    const ocx* starts);
+/* the same hierarchy from given parts: block vectors and coarse links of every
+ * transfer, concatenated (level 0 first), in orc_prolong_get / orc_coarse_get layout */
+orc_mg* orc_mg_assemble(const orc_op* fine, int nx, int ny, int nd, const orc_mg_params* prm,
+                        const ocx* block_vecs, const ocx* links);
 void orc_mg_free(orc_mg* mg);
 void orc_mg_set_symmetric(orc_mg* mg, double damping, const ocx* starts);
 void orc_mg_precondition(orc_mg* mg, const ocx* b, ocx* x);

@@ -65,22 +65,28 @@ void downc(double* h, const c64* d, size_t n) { down(h, d, n * sizeof(c64)); }
 
 // Orders the library stream after the caller's stream on entry, and the caller's
 // stream after the library's work on exit.
+// Orders the library's stream after the caller's stream on entry and the
+// caller's stream after the library's on exit.  A caller that passes the
+// library's own stream needs neither.  The two events are reused (a wait takes
+// the event's state at the time of the call), one pair per host thread.
 struct StreamJoin {
     hipStream_t user;
-    explicit StreamJoin(void* s) : user(reinterpret_cast<hipStream_t>(s)) {
-        hipEvent_t ev;
-        SMG_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    bool same;
+    static hipEvent_t event(int i) {
+        thread_local hipEvent_t ev[2] = {nullptr, nullptr};
+        if (!ev[i]) SMG_HIP(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+        return ev[i];
+    }
+    explicit StreamJoin(void* s) : user(reinterpret_cast<hipStream_t>(s)), same(user == S()) {
+        if (same) return;
+        hipEvent_t ev = event(0);
         SMG_HIP(hipEventRecord(ev, user));
         SMG_HIP(hipStreamWaitEvent(S(), ev, 0));
-        SMG_HIP(hipEventDestroy(ev));
     }
     ~StreamJoin() {
-        hipEvent_t ev;
-        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess) {
-            (void)hipEventRecord(ev, S());
-            (void)hipStreamWaitEvent(user, ev, 0);
-            (void)hipEventDestroy(ev);
-        }
+        if (same) return;
+        hipEvent_t ev = event(1);
+        if (hipEventRecord(ev, S()) == hipSuccess) (void)hipStreamWaitEvent(user, ev, 0);
     }
 };
 

@@ -269,6 +269,26 @@ def test_chiral_aggregation_structure_and_galerkin():
     assert np.abs(P2m.conj().T @ P2m - np.eye(P2.coarse_dim)).max() < 1e-13
 
 
+def test_mg_assembled_from_parts_equals_built():
+    """A hierarchy assembled from another's block vectors and coarse links (the
+    CPU baseline's way of sharing the GPU setup) runs the identical cycle."""
+    L, k = 16, 4
+    U = random_gauge(L, 0.4, 31)
+    A = O.Dirac(L, U, 0.05, 1.0, 1.0, 0.0).op()
+    dims = [2 * L * L, (L // 4) ** 2 * k]
+    starts = np.concatenate([gaussian_vecs(k // 2, n, 32 + i).ravel() for i, n in enumerate(dims)])
+    kw = dict(n_levels=3, block=((4, 4), (2, 2)), k=(k, k), pre=3, post=3, w_cycle=False, coarse_solver=0,
+              coarse_maxit=30, coarse_tol=0.1, chiral=True)
+    mg = O.MG(A, L, L, 2, starts, null_iters=4, **kw)
+    parts = (np.concatenate([mg.block_vecs(l) for l in range(2)]),
+             np.concatenate([mg.coarse_links(l) for l in range(2)]))
+    mga = O.MG(A, L, L, 2, None, parts=parts, **kw)
+    assert np.array_equal(mga.block_vecs(1), mg.block_vecs(1))
+    assert np.array_equal(mga.coarse_links(1), mg.coarse_links(1))
+    b = gaussian_vec(2 * L * L, 33)
+    assert np.array_equal(mga(b), mg(b))
+
+
 def test_antiperiodic_link_flip_leaves_gauge_quantities():
     L = 8
     U = random_gauge(L, 0.5, 13)
