@@ -230,7 +230,8 @@ int smg_fgmres_solve(smg_dirac* D, smg_mg* mg, const double* b, double* x, int r
                      smg_solve_info* info);
 
 /* ------------------------------------------------------------------ many right-hand sides
- * N (a power of two <= 32) right-hand sides per launch.  Host and device arrays
+ * N right-hand sides per launch, N a power of two (<= 32 for the solver, <= 64
+ * for the cycle, the stencil and the coarse apply).  Host and device arrays
  * of the calls below hold the N vectors one after the other (each a reference
  * Vec); internally they run in the batched layout v[i*N + j] (RHS innermost) so
  * that every kernel of the cycle and solver carries all N.  No reference
@@ -241,8 +242,9 @@ int smg_dirac_apply_batched_dev(smg_dirac* D, int kind, int prec, const void* sr
 /* SparseCoarseOp::apply_to (src/coarse_op.h:282-297) on N right-hand sides at
  * once: host arrays of N coarse vectors one after the other; links read in
  * precision prec (SMG_F64, SMG_F32, or SMG_F16 = FP16 links with FP32
- * arithmetic).  For k in {8, 16} and N in {16, 32, 64} the product runs on the
- * matrix cores. */
+ * arithmetic).  For k in {8, 16, 32} and N in {16, 32, 64} the product runs on
+ * the matrix cores (FP16 links: bf16 MFMA with the FP32 sources split in two
+ * bf16 terms). */
 int smg_coarse_apply_batched(smg_coarse* C, int prec, int nrhs, const double* src, double* dst);
 /* the same on device arrays in the batched layout src/dst[(b*k + kk)*N + j]
  * (complex128 for SMG_F64, complex64 otherwise) */
