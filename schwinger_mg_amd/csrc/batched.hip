@@ -181,6 +181,48 @@ struct RhsRun {
     }
 };
 
+// Per-RHS sums of a kernel whose lanes hold runs of P right-hand sides (lane l:
+// RHS P*(l % (N/P)) + q in acc[c][q]) into the layout reduce_finish_b expects
+// (lane j < N: RHS j; lanes >= N: zero).
+template <int NC, int P>
+__device__ __forceinline__ void runs_to_lanes(double (&acc)[NC][P], double (&v)[NC], int N) {
+    if constexpr (P == 1) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) v[c] = acc[c][0];
+    } else {
+        const int lane = threadIdx.x & 63, NP = N / P;
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+#pragma unroll
+            for (int q = 0; q < P; q++)
+                for (int off = NP; off < 64; off <<= 1) acc[c][q] += __shfl_xor(acc[c][q], off, 64);
+        const int jl = lane & (N - 1);
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            double x = 0.0;
+#pragma unroll
+            for (int q = 0; q < P; q++) {
+                const double y = __shfl(acc[c][q], jl / P, 64);
+                if (jl % P == q) x = y;
+            }
+            v[c] = lane < N ? x : 0.0;
+        }
+    }
+}
+// Run length of the batched vector kernels: pairs of right-hand sides in FP32
+// (N >= 2), single ones otherwise.  run_dispatch calls f with the run length as a
+// compile-time constant.
+template <typename T, typename F>
+__host__ inline void run_dispatch(int N, F&& f) {
+    if constexpr (sizeof(T) == 4) {
+        if (N >= 2) {
+            f(std::integral_constant<int, 2>{});
+            return;
+        }
+    }
+    f(std::integral_constant<int, 1>{});
+}
+
 // One work-item per (site, run of P right-hand sides).  P = 2 in FP32 halves the
 // load instructions and the index arithmetic per byte: measured at L=512, 32 RHS,
 // 51.5 us per launch against 76.5 us with one right-hand side per work-item.
@@ -246,30 +288,7 @@ __global__ __launch_bounds__(kBlock) void k_b_dslash(DiracView<T> D, cplx<T>* __
     }
     if (EPI == EPI_DOTS) {
         double v[3];
-        if constexpr (P == 1) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) v[c] = acc[c][0];
-        } else {
-            // lane l holds RHS P*(l % (N/P)) + q: sum lanes of equal l % (N/P), then hand
-            // RHS j to lane j (lanes >= N zero) in the layout the finish expects
-            const int lane = threadIdx.x & 63, NP = N / P;
-#pragma unroll
-            for (int c = 0; c < 3; c++)
-#pragma unroll
-                for (int q = 0; q < P; q++)
-                    for (int off = NP; off < 64; off <<= 1) acc[c][q] += __shfl_xor(acc[c][q], off, 64);
-            const int jl = lane & (N - 1);
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                double x = 0.0;
-#pragma unroll
-                for (int q = 0; q < P; q++) {
-                    const double y = __shfl(acc[c][q], jl / P, 64);
-                    if (jl % P == q) x = y;
-                }
-                v[c] = lane < N ? x : 0.0;
-            }
-        }
+        runs_to_lanes<3, P>(acc, v, N);
         reduce_finish_b<3, 64>(v, fin, N, gridDim.x, blockIdx.x);
     }
 }
@@ -292,14 +311,9 @@ static void dslash_p(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T 
 template <typename T>
 void launch_b_dslash(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T mu, int g5in, int g5out, Epi epi,
                      const cplx<T>* b, const cplx<T>* aux, const FinB& fin, int N, hipStream_t st) {
-    if constexpr (sizeof(T) == 4) {
-        if (N >= 2) {
-            dslash_p<T, 2>(D, dst, src, mu, g5in, g5out, epi, b, aux, fin, N, st);
-            SMG_LAUNCH_CHECK();
-            return;
-        }
-    }
-    dslash_p<T, 1>(D, dst, src, mu, g5in, g5out, epi, b, aux, fin, N, st);
+    run_dispatch<T>(N, [&](auto pc) {
+        dslash_p<T, decltype(pc)::value>(D, dst, src, mu, g5in, g5out, epi, b, aux, fin, N, st);
+    });
     SMG_LAUNCH_CHECK();
 }
 
@@ -942,16 +956,17 @@ __global__ __launch_bounds__(kBlock) void k_b_prolong(AggView A, const cplx<T>* 
 // The same with the number of coarse dof summed per fine dof (k, or k/2 with chiral
 // aggregation) known at compile time: the KH loads of a work-item issue together
 // instead of one dependent round trip per term.
-template <typename T, int KH>
+template <typename T, int KH, int P>
 __global__ __launch_bounds__(kBlock) void k_b_prolong_k(AggView A, const cplx<T>* __restrict__ bv,
                                                         const cplx<T>* __restrict__ coarse,
                                                         cplx<T>* __restrict__ fine, int accumulate, int N) {
     const int lN = lg2(N);
-    const size_t items = ((size_t)A.nx * A.ny * A.nd) << lN;
+    const size_t runs = ((size_t)A.nx * A.ny * A.nd * N) / P;
     const int bsize = A.bx * A.by * A.nd;
-    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < items; t += (size_t)gridDim.x * kBlock) {
-        const size_t f = t >> lN;
-        const int j = (int)(t & (N - 1));
+    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < runs; t += (size_t)gridDim.x * kBlock) {
+        const size_t i = t * P;
+        const size_t f = i >> lN;
+        const int j = (int)(i & (N - 1));
         const int s = (int)(f / A.nd), c = (int)(f % A.nd);
         const int x = s % A.nx, y = s / A.nx;
         const int b = x / A.bx + A.nbx * (y / A.by);
@@ -959,16 +974,28 @@ __global__ __launch_bounds__(kBlock) void k_b_prolong_k(AggView A, const cplx<T>
         const int k0 = A.chiral ? agg_chi_dof(A, c) * KH : 0;
         const cplx<T>* v = bv + ((size_t)b * A.k + k0) * bsize + loc * A.nd + c;
         const cplx<T>* cc = coarse + ((size_t)b * A.k + k0) * N + j;
-        cplx<T> cv[KH], vv[KH];
+        RhsRun<T, P> cv[KH];
+        cplx<T> vv[KH];
 #pragma unroll
         for (int q = 0; q < KH; q++) {
-            cv[q] = cc[(size_t)q * N];
+            cv[q].load(cc + (size_t)q * N);
             vv[q] = v[(size_t)q * bsize];
         }
-        cplx<T> acc = {(T)0, (T)0};
+        RhsRun<T, P> out;
 #pragma unroll
-        for (int q = 0; q < KH; q++) acc = acc + cv[q] * vv[q];
-        fine[t] = accumulate ? fine[t] + acc : acc;
+        for (int u = 0; u < P; u++) {
+            cplx<T> acc = {(T)0, (T)0};
+#pragma unroll
+            for (int q = 0; q < KH; q++) acc = acc + cv[q].v[u] * vv[q];
+            out.v[u] = acc;
+        }
+        if (accumulate) {
+            RhsRun<T, P> old;
+            old.load(fine + i);
+#pragma unroll
+            for (int u = 0; u < P; u++) out.v[u] = old.v[u] + out.v[u];
+        }
+        out.store(fine + i);
     }
 }
 template <typename T>
@@ -977,11 +1004,18 @@ void launch_b_prolong(const AggView& A, const cplx<T>* bv, const cplx<T>* coarse
     const size_t items = (size_t)A.nx * A.ny * A.nd * N;
     const int g = b_grid(items, 1 << 16);
     const int kh = A.chiral ? A.k / 2 : A.k;
-    if (kh == 4) k_b_prolong_k<T, 4><<<g, kBlock, 0, st>>>(A, bv, coarse, fine, accumulate, N);
-    else if (kh == 8) k_b_prolong_k<T, 8><<<g, kBlock, 0, st>>>(A, bv, coarse, fine, accumulate, N);
-    else if (kh == 16) k_b_prolong_k<T, 16><<<g, kBlock, 0, st>>>(A, bv, coarse, fine, accumulate, N);
-    else if (kh == 32) k_b_prolong_k<T, 32><<<g, kBlock, 0, st>>>(A, bv, coarse, fine, accumulate, N);
-    else k_b_prolong<T><<<g, kBlock, 0, st>>>(A, bv, coarse, fine, accumulate, N);
+    if (kh != 4 && kh != 8 && kh != 16 && kh != 32) {
+        k_b_prolong<T><<<g, kBlock, 0, st>>>(A, bv, coarse, fine, accumulate, N);
+    } else {
+        run_dispatch<T>(N, [&](auto pc) {
+            constexpr int P = decltype(pc)::value;
+            const int gp = b_grid(items / P, 1 << 16);
+            if (kh == 4) k_b_prolong_k<T, 4, P><<<gp, kBlock, 0, st>>>(A, bv, coarse, fine, accumulate, N);
+            else if (kh == 8) k_b_prolong_k<T, 8, P><<<gp, kBlock, 0, st>>>(A, bv, coarse, fine, accumulate, N);
+            else if (kh == 16) k_b_prolong_k<T, 16, P><<<gp, kBlock, 0, st>>>(A, bv, coarse, fine, accumulate, N);
+            else k_b_prolong_k<T, 32, P><<<gp, kBlock, 0, st>>>(A, bv, coarse, fine, accumulate, N);
+        });
+    }
     SMG_LAUNCH_CHECK();
 }
 
@@ -1013,49 +1047,58 @@ __global__ __launch_bounds__(kBlock) void k_b_restrict(AggView A, const cplx<T>*
 // The same with the aggregate's sites BS and the dof per site summed NC (nd, or
 // nd/2 with chiral aggregation) known at compile time; the loads are issued in
 // groups of up to 16 terms.
-template <typename T, int BS, int NC>
+template <typename T, int BS, int NC, int P>
 __global__ __launch_bounds__(kBlock) void k_b_restrict_k(AggView A, const cplx<T>* __restrict__ bv,
                                                          const cplx<T>* __restrict__ fine,
                                                          cplx<T>* __restrict__ coarse, int N) {
     constexpr int NT = BS * NC, G = NT < 16 ? NT : 16;
     static_assert(NT % G == 0, "terms in whole groups");
     const int lN = lg2(N);
-    const size_t items = ((size_t)A.nbx * A.nby * A.k) << lN;
+    const size_t runs = ((size_t)A.nbx * A.nby * A.k * N) / P;
     const int bsize = BS * A.nd;
-    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < items; t += (size_t)gridDim.x * kBlock) {
-        const size_t row = t >> lN;   // b*k + kk
-        const int j = (int)(t & (N - 1));
+    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < runs; t += (size_t)gridDim.x * kBlock) {
+        const size_t i = t * P;
+        const size_t row = i >> lN;   // b*k + kk
+        const int j = (int)(i & (N - 1));
         const int b = (int)(row / A.k), kk = (int)(row % A.k);
         const int bxi = b % A.nbx, byi = b / A.nbx;
         const cplx<T>* v = bv + ((size_t)b * A.k + kk) * bsize;
         const int c0 = A.chiral ? agg_chi_vec(A, kk) * NC : 0;
-        cplx<T> acc = {(T)0, (T)0};
+        cplx<T> acc[P];
+#pragma unroll
+        for (int u = 0; u < P; u++) acc[u] = cplx<T>{(T)0, (T)0};
 #pragma unroll
         for (int g0 = 0; g0 < NT; g0 += G) {
-            cplx<T> fv[G], vv[G];
+            RhsRun<T, P> fv[G];
+            cplx<T> vv[G];
 #pragma unroll
             for (int q = 0; q < G; q++) {
                 const int loc = (g0 + q) / NC, c = c0 + (g0 + q) % NC;
                 const int sx = bxi * A.bx + loc % A.bx, sy = byi * A.by + loc / A.bx;
-                fv[q] = fine[((size_t)(sx + A.nx * sy) * A.nd + c) * N + j];
+                fv[q].load(fine + ((size_t)(sx + A.nx * sy) * A.nd + c) * N + j);
                 vv[q] = v[loc * A.nd + c];
             }
 #pragma unroll
-            for (int q = 0; q < G; q++) acc = acc + cdot(vv[q], fv[q]);
+            for (int q = 0; q < G; q++)
+#pragma unroll
+                for (int u = 0; u < P; u++) acc[u] = acc[u] + cdot(vv[q], fv[q].v[u]);
         }
-        coarse[t] = acc;
+        RhsRun<T, P> out;
+#pragma unroll
+        for (int u = 0; u < P; u++) out.v[u] = acc[u];
+        out.store(coarse + i);
     }
 }
-template <typename T, int BS>
+template <typename T, int BS, int P>
 static bool restrict_nc(const AggView& A, int nc, int g, const cplx<T>* bv, const cplx<T>* fine, cplx<T>* coarse,
                         int N, hipStream_t st) {
     switch (nc) {
-        case 1: k_b_restrict_k<T, BS, 1><<<g, kBlock, 0, st>>>(A, bv, fine, coarse, N); return true;
-        case 2: k_b_restrict_k<T, BS, 2><<<g, kBlock, 0, st>>>(A, bv, fine, coarse, N); return true;
-        case 4: k_b_restrict_k<T, BS, 4><<<g, kBlock, 0, st>>>(A, bv, fine, coarse, N); return true;
-        case 8: k_b_restrict_k<T, BS, 8><<<g, kBlock, 0, st>>>(A, bv, fine, coarse, N); return true;
-        case 16: k_b_restrict_k<T, BS, 16><<<g, kBlock, 0, st>>>(A, bv, fine, coarse, N); return true;
-        case 32: k_b_restrict_k<T, BS, 32><<<g, kBlock, 0, st>>>(A, bv, fine, coarse, N); return true;
+        case 1: k_b_restrict_k<T, BS, 1, P><<<g, kBlock, 0, st>>>(A, bv, fine, coarse, N); return true;
+        case 2: k_b_restrict_k<T, BS, 2, P><<<g, kBlock, 0, st>>>(A, bv, fine, coarse, N); return true;
+        case 4: k_b_restrict_k<T, BS, 4, P><<<g, kBlock, 0, st>>>(A, bv, fine, coarse, N); return true;
+        case 8: k_b_restrict_k<T, BS, 8, P><<<g, kBlock, 0, st>>>(A, bv, fine, coarse, N); return true;
+        case 16: k_b_restrict_k<T, BS, 16, P><<<g, kBlock, 0, st>>>(A, bv, fine, coarse, N); return true;
+        case 32: k_b_restrict_k<T, BS, 32, P><<<g, kBlock, 0, st>>>(A, bv, fine, coarse, N); return true;
         default: return false;
     }
 }
@@ -1066,8 +1109,12 @@ void launch_b_restrict(const AggView& A, const cplx<T>* bv, const cplx<T>* fine,
     const int g = b_grid(items, 1 << 16);
     const int bs = A.bx * A.by, nc = A.chiral ? A.nd / 2 : A.nd;
     bool done = false;
-    if (bs == 4) done = restrict_nc<T, 4>(A, nc, g, bv, fine, coarse, N, st);
-    else if (bs == 16) done = restrict_nc<T, 16>(A, nc, g, bv, fine, coarse, N, st);
+    run_dispatch<T>(N, [&](auto pc) {
+        constexpr int P = decltype(pc)::value;
+        const int gp = b_grid(items / P, 1 << 16);
+        if (bs == 4) done = restrict_nc<T, 4, P>(A, nc, gp, bv, fine, coarse, N, st);
+        else if (bs == 16) done = restrict_nc<T, 16, P>(A, nc, gp, bv, fine, coarse, N, st);
+    });
     if (!done) k_b_restrict<T><<<g, kBlock, 0, st>>>(A, bv, fine, coarse, N);
     SMG_LAUNCH_CHECK();
 }
@@ -1165,35 +1212,56 @@ void launch_b_dense_apply(const cplx<T>* M, const cplx<T>* x, cplx<T>* y, size_t
 }
 
 // ----------------------------------------------------------------- BLAS-1
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_b_norm2(const cplx<T>* __restrict__ a, size_t items, FinB fin, int N) {
-    double v[1] = {0.0};
-    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < items; t += (size_t)gridDim.x * kBlock)
-        v[0] += (double)norm2(a[t]);
+// The vector kernels below take runs of P right-hand sides per work-item (P = 2:
+// 16-byte accesses in FP32), chosen by run_dispatch.
+template <typename T, int P>
+__global__ __launch_bounds__(kBlock) void k_b_norm2(const cplx<T>* __restrict__ a, size_t runs, FinB fin, int N) {
+    double acc[1][P] = {};
+    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < runs; t += (size_t)gridDim.x * kBlock) {
+        RhsRun<T, P> x;
+        x.load(a + t * P);
+#pragma unroll
+        for (int q = 0; q < P; q++) acc[0][q] += (double)norm2(x.v[q]);
+    }
+    double v[1];
+    runs_to_lanes<1, P>(acc, v, N);
     reduce_finish_b<1, 64>(v, fin, N, gridDim.x, blockIdx.x);
 }
 template <typename T>
 void launch_b_norm2(const cplx<T>* a, size_t n, const FinB& fin, int N, hipStream_t st) {
-    k_b_norm2<T><<<b_grid(n * N, kRedGridVec), kBlock, 0, st>>>(a, n * N, fin, N);
+    run_dispatch<T>(N, [&](auto pc) {
+        constexpr int P = decltype(pc)::value;
+        k_b_norm2<T, P><<<b_grid(n * N / P, kRedGridVec), kBlock, 0, st>>>(a, n * N / P, fin, N);
+    });
     SMG_LAUNCH_CHECK();
 }
 
-template <typename T>
+template <typename T, int P>
 __global__ __launch_bounds__(kBlock) void k_b_dots(const cplx<T>* __restrict__ a, const cplx<T>* __restrict__ b,
-                                                   size_t items, FinB fin, int N) {
-    double v[3] = {0.0, 0.0, 0.0};
-    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < items; t += (size_t)gridDim.x * kBlock) {
-        const cplx<T> x = a[t];
-        const cplx<T> p = cdot(x, b[t]);
-        v[0] += (double)p.re;
-        v[1] += (double)p.im;
-        v[2] += (double)norm2(x);
+                                                   size_t runs, FinB fin, int N) {
+    double acc[3][P] = {};
+    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < runs; t += (size_t)gridDim.x * kBlock) {
+        RhsRun<T, P> x, y;
+        x.load(a + t * P);
+        y.load(b + t * P);
+#pragma unroll
+        for (int q = 0; q < P; q++) {
+            const cplx<T> d = cdot(x.v[q], y.v[q]);
+            acc[0][q] += (double)d.re;
+            acc[1][q] += (double)d.im;
+            acc[2][q] += (double)norm2(x.v[q]);
+        }
     }
+    double v[3];
+    runs_to_lanes<3, P>(acc, v, N);
     reduce_finish_b<3, 64>(v, fin, N, gridDim.x, blockIdx.x);
 }
 template <typename T>
 void launch_b_dots(const cplx<T>* a, const cplx<T>* b, size_t n, const FinB& fin, int N, hipStream_t st) {
-    k_b_dots<T><<<b_grid(n * N, kRedGridVec), kBlock, 0, st>>>(a, b, n * N, fin, N);
+    run_dispatch<T>(N, [&](auto pc) {
+        constexpr int P = decltype(pc)::value;
+        k_b_dots<T, P><<<b_grid(n * N / P, kRedGridVec), kBlock, 0, st>>>(a, b, n * N / P, fin, N);
+    });
     SMG_LAUNCH_CHECK();
 }
 
@@ -1261,61 +1329,104 @@ void launch_b_gcr_orth(cplx<T>* p, cplx<T>* Ap, const VecPtrs<T>& P, const VecPt
     SMG_LAUNCH_CHECK();
 }
 
-template <typename T>
+// per-RHS complex coefficients s[2j], s[2j+1] of the run starting at RHS j
+template <typename T, int P>
+__device__ __forceinline__ void run_coef(const double* __restrict__ s, int j, cplx<T> (&a)[P]) {
+#pragma unroll
+    for (int q = 0; q < P; q++) a[q] = cplx<T>{(T)s[2 * (j + q)], (T)s[2 * (j + q) + 1]};
+}
+
+template <typename T, int P>
 __global__ __launch_bounds__(kBlock) void k_b_mr_update(cplx<T>* __restrict__ x, cplx<T>* __restrict__ r,
                                                         const cplx<T>* __restrict__ Ar, const double* __restrict__ s,
-                                                        size_t items, int N) {
-    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < items; t += (size_t)gridDim.x * kBlock) {
-        const int j = (int)(t & (N - 1));
-        const cplx<T> w = {(T)s[2 * j], (T)s[2 * j + 1]};
-        const cplx<T> ri = r[t];
-        x[t] = x[t] + w * ri;
-        r[t] = ri - w * Ar[t];
+                                                        size_t runs, int N) {
+    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < runs; t += (size_t)gridDim.x * kBlock) {
+        const size_t i = t * P;
+        cplx<T> w[P];
+        run_coef<T, P>(s, (int)(i & (N - 1)), w);
+        RhsRun<T, P> xv, rv, av;
+        xv.load(x + i);
+        rv.load(r + i);
+        av.load(Ar + i);
+#pragma unroll
+        for (int q = 0; q < P; q++) {
+            xv.v[q] = xv.v[q] + w[q] * rv.v[q];
+            rv.v[q] = rv.v[q] - w[q] * av.v[q];
+        }
+        xv.store(x + i);
+        rv.store(r + i);
     }
 }
 template <typename T>
 void launch_b_mr_update(cplx<T>* x, cplx<T>* r, const cplx<T>* Ar, const double* s, size_t n, int N, hipStream_t st) {
-    k_b_mr_update<T><<<b_grid(n * N, 1 << 16), kBlock, 0, st>>>(x, r, Ar, s, n * N, N);
+    run_dispatch<T>(N, [&](auto pc) {
+        constexpr int P = decltype(pc)::value;
+        k_b_mr_update<T, P><<<b_grid(n * N / P, 1 << 16), kBlock, 0, st>>>(x, r, Ar, s, n * N / P, N);
+    });
     SMG_LAUNCH_CHECK();
 }
 
-template <typename T>
+template <typename T, int P>
 __global__ __launch_bounds__(kBlock) void k_b_update(cplx<T>* __restrict__ x, cplx<T>* __restrict__ r,
                                                      const cplx<T>* __restrict__ p, const cplx<T>* __restrict__ Ap,
-                                                     const double* __restrict__ s, size_t items, FinB fin, int N) {
+                                                     const double* __restrict__ s, size_t runs, FinB fin, int N) {
     if (fin.skip && *fin.skip) return;   // CG finished for every RHS
-    double v[1] = {0.0};
-    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < items; t += (size_t)gridDim.x * kBlock) {
-        const int j = (int)(t & (N - 1));
-        const cplx<T> a = {(T)s[2 * j], (T)s[2 * j + 1]};
-        x[t] = x[t] + a * p[t];
-        const cplx<T> ri = r[t] - a * Ap[t];
-        r[t] = ri;
-        v[0] += (double)norm2(ri);
+    double acc[1][P] = {};
+    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < runs; t += (size_t)gridDim.x * kBlock) {
+        const size_t i = t * P;
+        cplx<T> a[P];
+        run_coef<T, P>(s, (int)(i & (N - 1)), a);
+        RhsRun<T, P> xv, rv, pv, av;
+        xv.load(x + i);
+        rv.load(r + i);
+        pv.load(p + i);
+        av.load(Ap + i);
+#pragma unroll
+        for (int q = 0; q < P; q++) {
+            xv.v[q] = xv.v[q] + a[q] * pv.v[q];
+            rv.v[q] = rv.v[q] - a[q] * av.v[q];
+            acc[0][q] += (double)norm2(rv.v[q]);
+        }
+        xv.store(x + i);
+        rv.store(r + i);
     }
+    double v[1];
+    runs_to_lanes<1, P>(acc, v, N);
     reduce_finish_b<1, 64>(v, fin, N, gridDim.x, blockIdx.x);
 }
 template <typename T>
 void launch_b_update(cplx<T>* x, cplx<T>* r, const cplx<T>* p, const cplx<T>* Ap, const double* s, size_t n,
                      const FinB& fin, int N, hipStream_t st) {
-    k_b_update<T><<<b_grid(n * N, kRedGridVec), kBlock, 0, st>>>(x, r, p, Ap, s, n * N, fin, N);
+    run_dispatch<T>(N, [&](auto pc) {
+        constexpr int P = decltype(pc)::value;
+        k_b_update<T, P><<<b_grid(n * N / P, kRedGridVec), kBlock, 0, st>>>(x, r, p, Ap, s, n * N / P, fin, N);
+    });
     SMG_LAUNCH_CHECK();
 }
 
-template <typename T>
+template <typename T, int P>
 __global__ void k_b_xpay(cplx<T>* __restrict__ p, const cplx<T>* __restrict__ z, const double* __restrict__ s,
-                         size_t items, int N, const int* __restrict__ skip) {
+                         size_t runs, int N, const int* __restrict__ skip) {
     if (skip && *skip) return;
-    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < items; t += (size_t)gridDim.x * kBlock) {
-        const int j = (int)(t & (N - 1));
-        const cplx<T> b = {(T)s[2 * j], (T)s[2 * j + 1]};
-        p[t] = z[t] + b * p[t];
+    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < runs; t += (size_t)gridDim.x * kBlock) {
+        const size_t i = t * P;
+        cplx<T> b[P];
+        run_coef<T, P>(s, (int)(i & (N - 1)), b);
+        RhsRun<T, P> pv, zv;
+        pv.load(p + i);
+        zv.load(z + i);
+#pragma unroll
+        for (int q = 0; q < P; q++) pv.v[q] = zv.v[q] + b[q] * pv.v[q];
+        pv.store(p + i);
     }
 }
 template <typename T>
 void launch_b_xpay(cplx<T>* p, const cplx<T>* z, const double* s, size_t n, int N, hipStream_t st,
                    const int* skip) {
-    k_b_xpay<T><<<b_grid(n * N, 1 << 16), kBlock, 0, st>>>(p, z, s, n * N, N, skip);
+    run_dispatch<T>(N, [&](auto pc) {
+        constexpr int P = decltype(pc)::value;
+        k_b_xpay<T, P><<<b_grid(n * N / P, 1 << 16), kBlock, 0, st>>>(p, z, s, n * N / P, N, skip);
+    });
     SMG_LAUNCH_CHECK();
 }
 
