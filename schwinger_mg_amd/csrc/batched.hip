@@ -13,21 +13,25 @@
 
 namespace smg {
 
-// Workgroups of a reducing batched kernel: kRedGrid for the stencil and coarse
-// applies with fused dots, kRedGridVec for the vector reductions and updates,
-// kRedGridMulti for the multi-dot (up to 32 partials per RHS).  Every workgroup of
-// a reducing kernel waits for its partials to reach memory before it retires, so
-// fewer workgroups that loop over more items are faster: measured on gfx950 at
-// L=512, 32 RHS, the FP32 stencil with dots takes 131 us at 1024-2048 workgroups
-// and 163 us at 8192; the coarsest-level CG update 36 us at 1024 and 51 us at 8192.
-// The partial rows, (nblk + nblk/32) * NC * N doubles, fit the 2M-double partial
-// buffer, and the 1 + nblk/32 completion counters fit the 1024-entry counter array.
-constexpr int kRedGrid = 2048;
-constexpr int kRedGridVec = 1024;
+// Workgroups of a reducing batched kernel: kRedGrid for the stencil with fused
+// dots, kRedGridCoarse for the coarse apply with fused dots, kRedGridVec for the
+// vector reductions and updates, kRedGridMulti for the multi-dot (up to 32
+// partials per RHS).  Every workgroup of a reducing kernel waits for its partials
+// to reach memory before it retires, so fewer workgroups that loop over more items
+// are faster.  Measured on gfx950 at 32 RHS: the paired FP32 stencil with dots
+// (L=512) 94 us at 512 workgroups, 106 us at 2048, 139 us at 8192; the k=32 coarse
+// apply with dots 56 us at 256 workgroups and 63 us at 1024 (64^2 sites), 222 us
+// and 250 us at 256 and 2048 (128^2); the coarsest-level CG update 35 us at 512
+// and 51 us at 8192.  The partial rows, (nblk + nblk/32) * NC * N doubles, fit the
+// 2M-double partial buffer, and the 1 + nblk/32 completion counters fit the
+// 1024-entry counter array.
+constexpr int kRedGrid = 512;
+constexpr int kRedGridCoarse = 256;
+constexpr int kRedGridVec = 512;
 constexpr int kRedGridMulti = 1024;
 static_assert((kRedGrid + kRedGrid / 32) * 3 * 64 <= (1 << 21), "partials exceed the reduction buffer");
 static_assert((kRedGridMulti + kRedGridMulti / 32) * 32 * 32 <= (1 << 21), "partials exceed the reduction buffer");
-static_assert(1 + kRedGrid / 32 <= 1024, "completion counters exceed the counter array");
+static_assert(1 + kRedGridMulti / 32 <= 1024, "completion counters exceed the counter array");
 
 __device__ __forceinline__ int lg2(int N) { return 31 - __clz(N); }
 
@@ -653,7 +657,8 @@ __device__ __forceinline__ unsigned pack_bf(float a, float b, unsigned sel) {
 // One wave per coarse site; the 9K/16 chunks are pipelined, chunk m + 1's raw
 // sources and links in flight while chunk m is split and multiplied.
 template <typename TL, int EPI, int K, int NT>
-__global__ __launch_bounds__(kBlock) void k_b_coarse_bf16(CoarseView<TL> C, c32* __restrict__ dst,
+__global__ __launch_bounds__(kBlock)
+void k_b_coarse_bf16(CoarseView<TL> C, c32* __restrict__ dst,
                                                           const c32* __restrict__ src, const c32* __restrict__ bvec,
                                                           const c32* __restrict__ aux, FinB fin) {
     static_assert(std::is_same<TL, ch16>::value, "bf16x3 path: FP16 links");
@@ -757,6 +762,22 @@ __global__ __launch_bounds__(kBlock) void k_b_coarse_bf16(CoarseView<TL> C, c32*
                 }
             }
         };
+        // the epilogue's operand (b for EPI_RESID, aux for EPI_DOTS) at this lane's
+        // outputs, loaded while the last chunks are multiplied (PAIR layout)
+        float4 epo[PAIR ? RT : 1][2][PAIR ? NL : 1];
+        auto load_epo = [&]() {
+            if constexpr (PAIR && EPI != EPI_NONE) {
+                const c32* ep = EPI == EPI_RESID ? bvec : aux;
+#pragma unroll
+                for (int r = 0; r < RT; r++)
+#pragma unroll
+                    for (int h = 0; h < 2; h++)
+#pragma unroll
+                        for (int u = 0; u < NL; u++)
+                            epo[r][h][u] = *reinterpret_cast<const float4*>(
+                                ep + ((size_t)b * K + 8 * r + 2 * hi + h) * N + 32 * u + 2 * lo);
+            }
+        };
         if constexpr (NT <= 2) {
             // chunks m + 1 and m + 2 in flight while chunk m is multiplied (fully
             // unrolled, so the ring indices are compile-time): 12% faster than one
@@ -764,9 +785,11 @@ __global__ __launch_bounds__(kBlock) void k_b_coarse_bf16(CoarseView<TL> C, c32*
             Raw raw[3];
             load_raw(raw[0], 0);
             if (NCH > 1) load_raw(raw[1], 1);
+            if (NCH == 1) load_epo();
 #pragma unroll
             for (int m = 0; m < NCH; m++) {
                 if (m + 2 < NCH) load_raw(raw[(m + 2) % 3], m + 2);
+                else if (m + 2 == NCH) load_epo();   // the ring slot of chunk m + 2 is free
                 __builtin_amdgcn_sched_barrier(0);
                 mma_chunk(raw[m % 3]);
             }
@@ -779,6 +802,7 @@ __global__ __launch_bounds__(kBlock) void k_b_coarse_bf16(CoarseView<TL> C, c32*
                 __builtin_amdgcn_sched_barrier(0);
                 mma_chunk(raw[m & 1]);
             }
+            load_epo();
         }
 #pragma unroll
         for (int r = 0; r < RT; r++)
@@ -793,13 +817,13 @@ __global__ __launch_bounds__(kBlock) void k_b_coarse_bf16(CoarseView<TL> C, c32*
                         c32 o0{acc[r][2 * u][2 * h], acc[r][2 * u][2 * h + 1]};
                         c32 o1{acc[r][2 * u + 1][2 * h], acc[r][2 * u + 1][2 * h + 1]};
                         if (EPI == EPI_RESID) {
-                            const float4 w = *reinterpret_cast<const float4*>(bvec + idx);
+                            const float4 w = epo[r][h][u];
                             o0 = c32{w.x, w.y} - o0;
                             o1 = c32{w.z, w.w} - o1;
                         }
                         *reinterpret_cast<float4*>(dst + idx) = float4{o0.re, o0.im, o1.re, o1.im};
                         if (EPI == EPI_DOTS) {
-                            const float4 w = *reinterpret_cast<const float4*>(aux + idx);
+                            const float4 w = epo[r][h][u];
                             const c32 p0 = cdot(o0, c32{w.x, w.y}), p1 = cdot(o1, c32{w.z, w.w});
                             dacc[2 * u][0] += p0.re;
                             dacc[2 * u][1] += p0.im;
@@ -856,7 +880,7 @@ static void coarse_bf16(const CoarseView<TL>& C, c32* dst, const c32* src, Epi e
                         const FinB& fin, hipStream_t st) {
     const size_t waves = (size_t)C.nx * C.ny;
     const int nw = kBlock / 64;
-    const int g = (int)std::min<size_t>((waves + nw - 1) / nw, epi == EPI_DOTS ? kRedGrid : 1 << 16);
+    const int g = (int)std::min<size_t>((waves + nw - 1) / nw, epi == EPI_DOTS ? kRedGridCoarse : 1 << 16);
     if (epi == EPI_NONE)
         k_b_coarse_bf16<TL, EPI_NONE, K, NT><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
     else if (epi == EPI_RESID)
@@ -870,7 +894,7 @@ static void coarse_mfma(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* sr
                         const cplx<T>* aux, const FinB& fin, hipStream_t st) {
     const size_t waves = (size_t)C.nx * C.ny;
     const int nw = kBlock / 64;
-    const int g = (int)std::min<size_t>((waves + nw - 1) / nw, epi == EPI_DOTS ? kRedGrid : 1 << 16);
+    const int g = (int)std::min<size_t>((waves + nw - 1) / nw, epi == EPI_DOTS ? kRedGridCoarse : 1 << 16);
     if (epi == EPI_NONE)
         k_b_coarse_mfma<T, TL, EPI_NONE, K, NT><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
     else if (epi == EPI_RESID)
@@ -926,7 +950,7 @@ void launch_b_coarse_apply(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>*
     else if (epi == EPI_RESID)
         k_b_coarse<T, TL, EPI_RESID><<<b_grid(items, 1 << 16), kBlock, 0, st>>>(C, dst, src, b, aux, fin, N);
     else
-        k_b_coarse<T, TL, EPI_DOTS><<<b_grid(items, kRedGrid), kBlock, 0, st>>>(C, dst, src, b, aux, fin, N);
+        k_b_coarse<T, TL, EPI_DOTS><<<b_grid(items, kRedGridCoarse), kBlock, 0, st>>>(C, dst, src, b, aux, fin, N);
     SMG_LAUNCH_CHECK();
 }
 
