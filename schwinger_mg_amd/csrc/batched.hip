@@ -1454,17 +1454,97 @@ void launch_b_xpay(cplx<T>* p, const cplx<T>* z, const double* s, size_t n, int 
     SMG_LAUNCH_CHECK();
 }
 
-// per-RHS CG program state: aux_j[5] = tol^2, flag_j = 0
-__global__ void k_b_cg_init(double* aux, int astride, int N, double tol2, int* done) {
+// per-RHS CG program state: aux_j[5] = tol^2, flag_j = 0; the solve's done and
+// late flags cleared
+__global__ void k_b_cg_init(double* aux, int astride, int N, double tol2, int* done, int* late) {
     const int j = threadIdx.x;
     if (j < N) {
         aux[(size_t)j * astride + 5] = tol2;
         *reinterpret_cast<int*>(aux + (size_t)j * astride + 8) = 0;
     }
     if (j == 0 && done) *done = 0;
+    if (j == 0 && late) *late = 0;
 }
-void launch_b_cg_init(double* aux, int astride, int N, double tol2, int* done, hipStream_t st) {
-    k_b_cg_init<<<1, 64, 0, st>>>(aux, astride, N, tol2, done);
+void launch_b_cg_init(double* aux, int astride, int N, double tol2, int* done, int* late, hipStream_t st) {
+    k_b_cg_init<<<1, 64, 0, st>>>(aux, astride, N, tol2, done, late);
+    SMG_LAUNCH_CHECK();
+}
+
+// The two vector passes of a CG iteration after the apply (src/coarse_op.h:520-540):
+// k_b_cg_r: r -= a_j Ap with |r|^2 finished by fin (FIN_CG_BETA, which sets
+// *fin.done once every RHS has converged); k_b_cg_xp: x += a_j p, p = r + b_j p, the
+// solution update taken in the pass that reads p anyway (8 vector passes per
+// iteration instead of 9).  An RHS that has converged has a_j = 0 from then on.
+// Once the solve is done, k_b_cg_r of the next iteration finds *fin.skip set,
+// returns at entry and sets *late, which stops k_b_cg_xp: the iteration in which
+// the last RHS converged still adds its step to x.
+template <typename T, int P>
+__global__ __launch_bounds__(kBlock) void k_b_cg_r(cplx<T>* __restrict__ r, const cplx<T>* __restrict__ Ap,
+                                                   const double* __restrict__ s, size_t runs, FinB fin, int N,
+                                                   int* __restrict__ late) {
+    if (fin.skip && *fin.skip) {
+        if (late && blockIdx.x == 0 && threadIdx.x == 0) *late = 1;
+        return;
+    }
+    double acc[1][P] = {};
+    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < runs; t += (size_t)gridDim.x * kBlock) {
+        const size_t i = t * P;
+        cplx<T> a[P];
+        run_coef<T, P>(s, (int)(i & (N - 1)), a);
+        RhsRun<T, P> rv, av;
+        rv.load(r + i);
+        av.load(Ap + i);
+#pragma unroll
+        for (int q = 0; q < P; q++) {
+            rv.v[q] = rv.v[q] - a[q] * av.v[q];
+            acc[0][q] += (double)norm2(rv.v[q]);
+        }
+        rv.store(r + i);
+    }
+    double v[1];
+    runs_to_lanes<1, P>(acc, v, N);
+    reduce_finish_b<1, 64>(v, fin, N, gridDim.x, blockIdx.x);
+}
+template <typename T, int P>
+__global__ __launch_bounds__(kBlock) void k_b_cg_xp(cplx<T>* __restrict__ x, cplx<T>* __restrict__ p,
+                                                    const cplx<T>* __restrict__ r, const double* __restrict__ sa,
+                                                    const double* __restrict__ sb, size_t runs, int N,
+                                                    const int* __restrict__ late) {
+    if (late && *late) return;
+    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < runs; t += (size_t)gridDim.x * kBlock) {
+        const size_t i = t * P;
+        cplx<T> a[P], b[P];
+        run_coef<T, P>(sa, (int)(i & (N - 1)), a);
+        run_coef<T, P>(sb, (int)(i & (N - 1)), b);
+        RhsRun<T, P> xv, pv, rv;
+        xv.load(x + i);
+        pv.load(p + i);
+        rv.load(r + i);
+#pragma unroll
+        for (int q = 0; q < P; q++) {
+            xv.v[q] = xv.v[q] + a[q] * pv.v[q];
+            pv.v[q] = rv.v[q] + b[q] * pv.v[q];
+        }
+        xv.store(x + i);
+        pv.store(p + i);
+    }
+}
+template <typename T>
+void launch_b_cg_r(cplx<T>* r, const cplx<T>* Ap, const double* a, size_t n, const FinB& fin, int N, int* late,
+                   hipStream_t st) {
+    run_dispatch<T>(N, [&](auto pc) {
+        constexpr int P = decltype(pc)::value;
+        k_b_cg_r<T, P><<<b_grid(n * N / P, kRedGridVec), kBlock, 0, st>>>(r, Ap, a, n * N / P, fin, N, late);
+    });
+    SMG_LAUNCH_CHECK();
+}
+template <typename T>
+void launch_b_cg_xp(cplx<T>* x, cplx<T>* p, const cplx<T>* r, const double* a, const double* b, size_t n, int N,
+                    const int* late, hipStream_t st) {
+    run_dispatch<T>(N, [&](auto pc) {
+        constexpr int P = decltype(pc)::value;
+        k_b_cg_xp<T, P><<<b_grid(n * N / P, 1 << 16), kBlock, 0, st>>>(x, p, r, a, b, n * N / P, N, late);
+    });
     SMG_LAUNCH_CHECK();
 }
 
@@ -1549,7 +1629,11 @@ void launch_b_accum_mixed(c64* x, const c32* e, size_t n, int N, hipStream_t st)
     template void launch_b_update<T>(cplx<T>*, cplx<T>*, const cplx<T>*, const cplx<T>*, const double*, size_t,  \
                                      const FinB&, int, hipStream_t);                                             \
     template void launch_b_axpby_real<T>(cplx<T>*, const cplx<T>*, double, double, size_t, hipStream_t);    \
-    template void launch_b_xpay<T>(cplx<T>*, const cplx<T>*, const double*, size_t, int, hipStream_t, const int*);
+    template void launch_b_xpay<T>(cplx<T>*, const cplx<T>*, const double*, size_t, int, hipStream_t, const int*); \
+    template void launch_b_cg_r<T>(cplx<T>*, const cplx<T>*, const double*, size_t, const FinB&, int, int*,        \
+                                   hipStream_t);                                                                    \
+    template void launch_b_cg_xp<T>(cplx<T>*, cplx<T>*, const cplx<T>*, const double*, const double*, size_t, int, \
+                                    const int*, hipStream_t);
 SMG_BINST(double)
 SMG_BINST(float)
 #undef SMG_BINST

@@ -106,7 +106,8 @@ static void cg_b(LinOpB<T>& A, const cplx<T>* b, cplx<T>* x, cplx<T>* r, cplx<T>
     // once every RHS has converged the remaining iterations' kernels return at
     // entry (device-side flag, no host synchronisation)
     int* done = reinterpret_cast<int*>(aux + 16 * 64);
-    launch_b_cg_init(aux, 16, N, tol * tol, done, st);
+    int* late = reinterpret_cast<int*>(aux + 16 * 64 + 1);
+    launch_b_cg_init(aux, 16, N, tol * tol, done, late, st);
     launch_b_norm2<T>(r, n, finb(FIN_CG_INIT, nullptr, 0, aux), N, st);
     FinB fa = finb(FIN_CG_ALPHA, alpha, 2, aux), fbt = finb(FIN_CG_BETA, beta, 2, aux);
     fa.skip = done;
@@ -114,8 +115,8 @@ static void cg_b(LinOpB<T>& A, const cplx<T>* b, cplx<T>* x, cplx<T>* r, cplx<T>
     fbt.done = done;
     for (int it = 0; it < maxit; it++) {
         A.apply(Ap, p, EPI_DOTS, nullptr, p, fa, st);
-        launch_b_update<T>(x, r, p, Ap, alpha, n, fbt, N, st);
-        launch_b_xpay<T>(p, r, beta, n, N, st, done);
+        launch_b_cg_r<T>(r, Ap, alpha, n, fbt, N, late, st);
+        launch_b_cg_xp<T>(x, p, r, alpha, beta, n, N, late, st);
     }
 }
 

@@ -71,8 +71,17 @@ void launch_b_update(cplx<T>* x, cplx<T>* r, const cplx<T>* p, const cplx<T>* Ap
 template <typename T>
 void launch_b_xpay(cplx<T>* p, const cplx<T>* z, const double* s, size_t n, int N, hipStream_t st,
                    const int* skip = nullptr);
-// per-RHS CG program state: aux_j[5] = tol^2, flag_j = 0; *done = 0
-void launch_b_cg_init(double* aux, int astride, int N, double tol2, int* done, hipStream_t st);
+// per-RHS CG program state: aux_j[5] = tol^2, flag_j = 0; *done = *late = 0
+void launch_b_cg_init(double* aux, int astride, int N, double tol2, int* done, int* late, hipStream_t st);
+// CG vector passes after the apply: r -= a_j Ap with |r|^2 finished by fin (returns
+// at entry and sets *late when *fin.skip), then x += a_j p, p = r + b_j p (nothing
+// once *late is set)
+template <typename T>
+void launch_b_cg_r(cplx<T>* r, const cplx<T>* Ap, const double* a, size_t n, const FinB& fin, int N, int* late,
+                   hipStream_t st);
+template <typename T>
+void launch_b_cg_xp(cplx<T>* x, cplx<T>* p, const cplx<T>* r, const double* a, const double* b, size_t n, int N,
+                    const int* late, hipStream_t st);
 // y = a x + c y with one real scalar pair for all RHS
 template <typename T>
 void launch_b_axpby_real(cplx<T>* y, const cplx<T>* x, double a, double c, size_t n, hipStream_t st);
