@@ -277,6 +277,40 @@ int smg_eo_fermion_force(smg_dirac* D, const double* x_o, const double* y_o, dou
  * X (optional) receives (D^dag D)^-1 phi, iters (optional) the CG iterations */
 int smg_fermion_action(smg_dirac* D, const double* phi, int maxit, double tol, double* S, double* X, int* iters);
 
+/* ------------------------------------------------------------------ HMC molecular dynamics
+ * The gauge field evolves on the device inside the operator handle (its clover
+ * term and FP32 links follow); momenta are 2V reals pi[mu*V + s]. */
+typedef struct {
+    double beta;       /* gauge coupling of the Wilson action */
+    double tau;        /* trajectory length */
+    int n_steps;       /* MD steps */
+    int omelyan;       /* 0: leapfrog; 1: Omelyan 2MN with lambda = 0.1932 */
+    int cg_maxiter;    /* CG on D^dag D for every force and action, from x0 = 0 */
+    double cg_tol;
+    int force_accept;  /* always accept (thermalisation) */
+} smg_hmc_params;      /* HMCParams, src/hmc.h:107-119 (full lattice, no preconditioner) */
+typedef struct {
+    int accepted;
+    double dH, dKE, dSG, dSF;
+    int total_cg_iters;
+} smg_hmc_result;      /* HMCResult, src/hmc.h:120-128 (dLD = 0 on the full lattice) */
+/* the operator's current gauge field (host, 2V complex) */
+int smg_dirac_get_gauge(smg_dirac* D, double* U);
+/* plain_leapfrog_evolve, src/hmc_trajectory.cpp:243-409 (no preconditioner, no
+ * tracking): the operator's gauge field and the momenta pi (host, in/out) evolved
+ * over tau with the pseudofermion phi (host, 2V complex) fixed */
+int smg_hmc_evolve(smg_dirac* D, const smg_hmc_params* p, double* pi, const double* phi, int* cg_iters);
+/* the Hamiltonian of src/hmc_trajectory.cpp:87-103 on the current gauge field:
+ * out3 = {KE = 0.5 sum pi^2, S_g = -beta sum Re P, S_f = Re phi^dag (D^dag D)^-1 phi} */
+int smg_hmc_hamiltonian(smg_dirac* D, double beta, const double* pi, const double* phi, int cg_maxiter,
+                        double cg_tol, double* out3, int* cg_iters);
+/* hmc_trajectory, full-lattice path, src/hmc_trajectory.cpp:38-45, 198-236: momenta
+ * pi0 and the Gaussian eta of phi = D^dag eta come from the caller (drawn from its
+ * generator in that order); u is the uniform of the Metropolis test, consulted only
+ * when dH >= 0 and force_accept == 0.  A rejected trajectory restores the gauge field. */
+int smg_hmc_trajectory(smg_dirac* D, const smg_hmc_params* p, const double* pi0, const double* eta, double u,
+                       smg_hmc_result* res);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,6 +53,12 @@ def lib():
         L.orc_avg_plaq.argtypes = [_I, _P]
         L.orc_gauge_action.restype = _D
         L.orc_gauge_action.argtypes = [_I, _P, _D]
+        L.orc_kinetic_energy.restype = _D
+        L.orc_kinetic_energy.argtypes = [_I, _P]
+        L.orc_hmc_evolve.restype = _I
+        L.orc_hmc_evolve.argtypes = [_I, _P, _D, _D, _D, _D, _D, _D, _I, _I, _I, _D, _P, _P]
+        L.orc_hmc_hamiltonian.restype = _I
+        L.orc_hmc_hamiltonian.argtypes = [_I, _P, _D, _D, _D, _D, _D, _I, _D, _P, _P, _P]
         L.orc_gauge_force.argtypes = [_I, _P, _D, _P]
         L.orc_eo_maps.argtypes = [_I, _P, _P, _P]
         L.orc_dot2.argtypes = [_I, _P, _P, _P]
@@ -461,3 +467,22 @@ def clover_deriv_insert(D, w, factor, F):
     w = _c(w, np.float64); F = _c(F, np.float64).copy()
     lib().orc_clover_deriv_insert(D.h, _ptr(w), factor, _ptr(F))
     return F
+
+
+def hmc_evolve(L, U, mass, r, csw, mu_t, beta, tau, n_steps, omelyan, cg_maxit, cg_tol, pi, phi):
+    """plain_leapfrog_evolve restated: returns (U, pi, cg iterations) after the MD trajectory."""
+    U = _c(U).copy()
+    pi = np.ascontiguousarray(pi, dtype=np.float64).copy()
+    phi = _c(phi)
+    it = lib().orc_hmc_evolve(L, _ptr(U), mass, r, csw, mu_t, beta, tau, n_steps, int(omelyan), cg_maxit, cg_tol,
+                              _ptr(pi), _ptr(phi))
+    return U, pi, it
+
+
+def hmc_hamiltonian(L, U, mass, r, csw, mu_t, beta, cg_maxit, cg_tol, pi, phi):
+    """(KE, S_g, S_f) of the trajectory's Hamiltonian."""
+    U, phi = _c(U), _c(phi)
+    pi = np.ascontiguousarray(pi, dtype=np.float64)
+    out = np.zeros(3)
+    lib().orc_hmc_hamiltonian(L, _ptr(U), mass, r, csw, mu_t, beta, cg_maxit, cg_tol, _ptr(pi), _ptr(phi), _ptr(out))
+    return out

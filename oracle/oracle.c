@@ -1356,3 +1356,93 @@ void orc_eo_fermion_force(const orc_dirac* D, const ocx* x_o, const ocx* y_o, do
         free(fl);
     }
 }
+
+/* ================================================================ HMC molecular dynamics */
+/* MomentumField::kinetic_energy, src/hmc_forces.cpp:70-77: 0.5 sum pi^2 over 2V reals */
+double orc_kinetic_energy(int n, const double* pi) {
+    double ke = 0.0;
+    for (int i = 0; i < n; i++) ke += pi[i] * pi[i];
+    return 0.5 * ke;
+}
+
+/* the MD force of src/hmc_trajectory.cpp:254-343 without tracking: gauge force plus
+ * the fermion force of X = (D^dag D)^-1 phi (CG from x0 = 0), both -dS/dtheta */
+static int md_force(int L, const ocx* U, double mass, double r, double csw, double mu_t, double beta, int cg_maxit,
+                    double cg_tol, const ocx* phi, double* F) {
+    int V = L * L;
+    orc_gauge_force(L, U, beta, F);
+    orc_dirac* D = orc_dirac_new(L, U, mass, r, csw, mu_t);
+    orc_op* A = orc_op_dirac(D, ORC_DDAGD);
+    ocx* X = xcalloc(2 * (size_t)V, sizeof(ocx));
+    double res;
+    int it = orc_cg(A, phi, NULL, NULL, cg_maxit, cg_tol, X, &res);
+    double* ff = xcalloc(2 * (size_t)V, sizeof(double));
+    orc_fermion_force(D, X, ff);
+    for (int i = 0; i < 2 * V; i++) F[i] += ff[i];
+    free(ff); free(X);
+    orc_op_free(A);
+    orc_dirac_free(D);
+    return it;
+}
+/* src/hmc_trajectory.cpp:345-357 */
+static void md_update_mom(int n, double* pi, const double* F, double scale) {
+    for (int i = 0; i < n; i++) pi[i] += scale * F[i];
+}
+static void md_update_gauge(int n, ocx* U, const double* pi, double scale) {
+    for (int i = 0; i < n; i++) U[i] *= cexp(I * (scale * pi[i]));
+}
+
+/* plain_leapfrog_evolve, src/hmc_trajectory.cpp:243-409: U (2V complex) and pi (2V
+ * reals) evolved in place over tau in n_steps steps, leapfrog P(dt/2) [Q(dt) P(dt)]^(N-1)
+ * Q(dt) P(dt/2) or Omelyan (2MN, lambda = 0.1932); returns the CG iterations. */
+int orc_hmc_evolve(int L, ocx* U, double mass, double r, double csw, double mu_t, double beta, double tau,
+                   int n_steps, int omelyan, int cg_maxit, double cg_tol, double* pi, const ocx* phi) {
+    int n = 2 * L * L, cg = 0;
+    double dt = tau / n_steps;
+    double* F = xcalloc(n, sizeof(double));
+    if (omelyan) {
+        const double lambda = 0.1932;
+        cg += md_force(L, U, mass, r, csw, mu_t, beta, cg_maxit, cg_tol, phi, F);
+        md_update_mom(n, pi, F, lambda * dt);
+        for (int step = 0; step < n_steps; step++) {
+            md_update_gauge(n, U, pi, 0.5 * dt);
+            cg += md_force(L, U, mass, r, csw, mu_t, beta, cg_maxit, cg_tol, phi, F);
+            md_update_mom(n, pi, F, (1.0 - 2.0 * lambda) * dt);
+            md_update_gauge(n, U, pi, 0.5 * dt);
+            cg += md_force(L, U, mass, r, csw, mu_t, beta, cg_maxit, cg_tol, phi, F);
+            md_update_mom(n, pi, F, (step < n_steps - 1 ? 2.0 : 1.0) * lambda * dt);
+        }
+    } else {
+        cg += md_force(L, U, mass, r, csw, mu_t, beta, cg_maxit, cg_tol, phi, F);
+        md_update_mom(n, pi, F, 0.5 * dt);
+        for (int step = 0; step < n_steps - 1; step++) {
+            md_update_gauge(n, U, pi, dt);
+            cg += md_force(L, U, mass, r, csw, mu_t, beta, cg_maxit, cg_tol, phi, F);
+            md_update_mom(n, pi, F, dt);
+        }
+        md_update_gauge(n, U, pi, dt);
+        cg += md_force(L, U, mass, r, csw, mu_t, beta, cg_maxit, cg_tol, phi, F);
+        md_update_mom(n, pi, F, 0.5 * dt);
+    }
+    free(F);
+    return cg;
+}
+
+/* H = KE + S_g + S_f of src/hmc_trajectory.cpp:87-103 (S_f = Re phi^dag (D^dag D)^-1 phi,
+ * CG from x0 = 0); out = {KE, S_g, S_f}; returns the CG iterations */
+int orc_hmc_hamiltonian(int L, const ocx* U, double mass, double r, double csw, double mu_t, double beta,
+                        int cg_maxit, double cg_tol, const double* pi, const ocx* phi, double* out) {
+    int V = L * L;
+    out[0] = orc_kinetic_energy(2 * V, pi);
+    out[1] = orc_gauge_action(L, U, beta);
+    orc_dirac* D = orc_dirac_new(L, U, mass, r, csw, mu_t);
+    orc_op* A = orc_op_dirac(D, ORC_DDAGD);
+    ocx* X = xcalloc(2 * (size_t)V, sizeof(ocx));
+    double res;
+    int it = orc_cg(A, phi, NULL, NULL, cg_maxit, cg_tol, X, &res);
+    out[2] = creal(orc_dot(2 * V, phi, X));
+    free(X);
+    orc_op_free(A);
+    orc_dirac_free(D);
+    return it;
+}

@@ -145,6 +145,13 @@ void orc_clover_deriv_insert(const orc_dirac* D, const double* w, double factor,
 void orc_logdet_ee_force(const orc_dirac* D, double* F);
 void orc_eo_fermion_force(const orc_dirac* D, const ocx* x_o, const ocx* y_o, double* F);
 
+/* ---------------- HMC molecular dynamics ---------------- */
+double orc_kinetic_energy(int n, const double* pi);
+int orc_hmc_evolve(int L, ocx* U, double mass, double r, double csw, double mu_t, double beta, double tau,
+                   int n_steps, int omelyan, int cg_maxit, double cg_tol, double* pi, const ocx* phi);
+int orc_hmc_hamiltonian(int L, const ocx* U, double mass, double r, double csw, double mu_t, double beta,
+                        int cg_maxit, double cg_tol, const double* pi, const ocx* phi, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,8 @@ def lib():
         L.ref_fermion_force.argtypes = [_I, _P, _D, _D, _D, _D, _I, _P, _P, _P]
         L.ref_fermion_action.restype = _D
         L.ref_fermion_action.argtypes = [_I, _P, _D, _D, _D, _D, _P, _I, _D]
+        L.ref_leapfrog_evolve.restype = _I
+        L.ref_leapfrog_evolve.argtypes = [_I, _P, _D, _D, _D, _D, _D, _I, _I, _I, _D, _P, _P]
         _lib = L
     return _lib
 
@@ -194,3 +196,13 @@ def gauge_force(L, U, beta):
 
 def gauge_action(L, U, beta):
     return lib().ref_gauge_action(L, _ptr(_c(U)), beta)
+
+
+def leapfrog_evolve(L, U, mass, r, csw, beta, tau, n_steps, omelyan, cg_maxit, cg_tol, pi, phi):
+    """plain_leapfrog_evolve of the compiled reference: returns (U, pi, cg iterations)."""
+    U = _c(U).copy()
+    pi = np.ascontiguousarray(pi, dtype=np.float64).copy()
+    phi = _c(phi)
+    it = lib().ref_leapfrog_evolve(L, _ptr(U), mass, r, csw, beta, tau, n_steps, int(omelyan), cg_maxit, cg_tol,
+                                   _ptr(pi), _ptr(phi))
+    return U, pi, it

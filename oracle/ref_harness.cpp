@@ -347,3 +347,31 @@ API double ref_fermion_action(int L, const double* U, double m, double r, double
     Ctx c(L, U, m, r, csw, mu);
     return fermion_action(c.D, to_vec(phi, c.lat.ndof), maxit, tol).action;
 }
+
+// plain_leapfrog_evolve (src/hmc_trajectory.cpp:243-409): the gauge field U
+// (2V complex) and the momenta pi (2V reals, pi[mu*V + s]) are evolved in place
+// with the pseudofermion phi fixed; no preconditioner, no tracking.
+API int ref_leapfrog_evolve(int L, double* U, double mass, double r, double csw, double beta, double tau,
+                            int n_steps, int omelyan, int cg_maxit, double cg_tol, double* pi, const double* phi) {
+    Lattice lat(L);
+    GaugeField g(lat);
+    load_gauge_arr(g, U);
+    MomentumField mom(lat);
+    std::memcpy(mom.pi[0].data(), pi, lat.V * sizeof(double));
+    std::memcpy(mom.pi[1].data(), pi + lat.V, lat.V * sizeof(double));
+    HMCParams p{};
+    p.beta = beta;
+    p.tau = tau;
+    p.n_steps = n_steps;
+    p.cg_maxiter = cg_maxit;
+    p.cg_tol = cg_tol;
+    p.use_mg = false;
+    p.c_sw = csw;
+    p.omelyan = omelyan != 0;
+    const int iters = plain_leapfrog_evolve(g, mom, lat, mass, r, to_vec(phi, lat.ndof), p, nullptr, nullptr);
+    from_vec(g.U[0], U);
+    from_vec(g.U[1], U + 2 * lat.V);
+    std::memcpy(pi, mom.pi[0].data(), lat.V * sizeof(double));
+    std::memcpy(pi + lat.V, mom.pi[1].data(), lat.V * sizeof(double));
+    return iters;
+}

@@ -44,6 +44,17 @@ class SolveInfo(ctypes.Structure):
     _fields_ = [("iterations", _I), ("final_residual", _D), ("true_residual", _D), ("restarts", _I)]
 
 
+class HMCParams(ctypes.Structure):
+    """HMCParams (src/hmc.h:107-119), full lattice without preconditioner."""
+    _fields_ = [("beta", _D), ("tau", _D), ("n_steps", _I), ("omelyan", _I), ("cg_maxiter", _I), ("cg_tol", _D),
+                ("force_accept", _I)]
+
+
+class HMCResult(ctypes.Structure):
+    """HMCResult (src/hmc.h:120-128)."""
+    _fields_ = [("accepted", _I), ("dH", _D), ("dKE", _D), ("dSG", _D), ("dSF", _D), ("total_cg_iters", _I)]
+
+
 # (name, restype, argtypes) of every entry point declared in include/smg.h
 SIGNATURES = [
     ("smg_init", _I, [_I]), ("smg_last_error", ctypes.c_char_p, []), ("smg_version", _I, []),
@@ -87,6 +98,9 @@ SIGNATURES = [
     ("smg_fermion_force", _I, [_P, _P, _P]), ("smg_fermion_force_dev", _I, [_P, _P, _P, _P]),
     ("smg_hopping_force", _I, [_P, _P, _P, _P]), ("smg_logdet_force", _I, [_P, _P]),
     ("smg_eo_fermion_force", _I, [_P, _P, _P, _P]), ("smg_fermion_action", _I, [_P, _P, _I, _D, _P, _P, _P]),
+    ("smg_dirac_get_gauge", _I, [_P, _P]), ("smg_hmc_evolve", _I, [_P, _P, _P, _P, _P]),
+    ("smg_hmc_hamiltonian", _I, [_P, _D, _P, _P, _I, _D, _P, _P]),
+    ("smg_hmc_trajectory", _I, [_P, _P, _P, _P, _D, _P]),
 ]
 
 _lib = None
@@ -639,3 +653,40 @@ def fermion_action(D, phi, max_iter, tol):
     X = np.zeros(D.ndof, np.complex128)
     _chk(lib().smg_fermion_action(D.h, _p(_c(phi)), max_iter, tol, ctypes.byref(S), _p(X), None))
     return S.value, X
+
+
+def hmc_params(beta, tau, n_steps, omelyan=False, cg_maxiter=1000, cg_tol=1e-12, force_accept=False):
+    return HMCParams(beta, tau, n_steps, int(omelyan), cg_maxiter, cg_tol, int(force_accept))
+
+
+def gauge_of(D):
+    """The operator's current (device) gauge field, U[mu*V + s]."""
+    U = np.zeros(2 * D.V, np.complex128)
+    _chk(lib().smg_dirac_get_gauge(D.h, _p(U)))
+    return U
+
+
+def hmc_evolve(D, params, pi, phi):
+    """plain_leapfrog_evolve (src/hmc_trajectory.cpp:243-409) on the device: D's gauge
+    field evolves in place; returns (pi, cg iterations)."""
+    pi = np.ascontiguousarray(pi, dtype=np.float64).copy()
+    it = _I()
+    _chk(lib().smg_hmc_evolve(D.h, ctypes.byref(params), _p(pi), _p(_c(phi)), ctypes.byref(it)))
+    return pi, it.value
+
+
+def hmc_hamiltonian(D, beta, pi, phi, cg_maxiter=1000, cg_tol=1e-12):
+    """(KE, S_g, S_f) on D's current gauge field."""
+    out = np.zeros(3)
+    _chk(lib().smg_hmc_hamiltonian(D.h, beta, _p(np.ascontiguousarray(pi, dtype=np.float64)), _p(_c(phi)),
+                                   cg_maxiter, cg_tol, _p(out), None))
+    return out
+
+
+def hmc_trajectory(D, params, pi0, eta, u):
+    """hmc_trajectory (src/hmc_trajectory.cpp:38-236, full lattice): momenta pi0 and
+    the Gaussian eta of phi = D^dag eta given; u the Metropolis uniform."""
+    res = HMCResult()
+    _chk(lib().smg_hmc_trajectory(D.h, ctypes.byref(params), _p(np.ascontiguousarray(pi0, dtype=np.float64)),
+                                  _p(_c(eta)), u, ctypes.byref(res)))
+    return res

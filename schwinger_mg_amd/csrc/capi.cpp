@@ -986,3 +986,140 @@ int smg_fermion_action(smg_dirac* D, const double* phi, int maxit, double tol, d
         if (X) downc(X, xx.p(), n);
     });
 }
+
+// ------------------------------------------------------------------ HMC molecular dynamics
+namespace {
+// Device state of one trajectory: momenta, the two forces, the pseudofermion and
+// the CG solution, all resident for the whole evolution.
+struct MDState {
+    smg_dirac* D;
+    size_t V;
+    DevMem pi, gf, ff;
+    Field<double> phi, X;
+    MDState(smg_dirac* d, const double* pi_h, const double* phi_h)
+        : D(d), V((size_t)d->D->V), pi(2 * V * sizeof(double)), gf(2 * V * sizeof(double)),
+          ff(2 * V * sizeof(double)), phi(2 * V), X(2 * V) {
+        SMG_HIP(hipMemcpyAsync(pi.p, pi_h, 2 * V * sizeof(double), hipMemcpyHostToDevice, S()));
+        if (phi_h) SMG_HIP(hipMemcpyAsync(phi.p(), phi_h, 2 * V * sizeof(c64), hipMemcpyHostToDevice, S()));
+        SMG_HIP(hipStreamSynchronize(S()));
+    }
+    // gauge force and fermion force of X = (D^dag D)^-1 phi (CG from x0 = 0), both
+    // -dS/dtheta (src/hmc_trajectory.cpp:254-343 without tracking)
+    int force(double beta, int maxit, double tol) {
+        const Dirac& d = *D->D;
+        launch_gauge_force(d.L, d.U64.p(), beta, gf.as<double>(), S());
+        const SolveResult r = cg_solve(*D->op64, phi.p(), X.p(), nullptr, nullptr, maxit, tol, S());
+        fermion_force_dev(D, X.p(), ff.as<double>(), S());
+        return r.iterations;
+    }
+    void mom(double scale) { launch_mom_update(pi.as<double>(), gf.as<double>(), ff.as<double>(), scale, 2 * V, S()); }
+    void gauge(double scale) { D->D->update_links(pi.as<double>(), scale, S()); }
+    // {KE, S_g, S_f} on the current gauge field (src/hmc_trajectory.cpp:87-103)
+    int hamiltonian(double beta, int maxit, double tol, double* out) {
+        const Dirac& d = *D->D;
+        out[0] = 0.5 * dev_dot<double>(pi.as<c64>(), pi.as<c64>(), V, S()).re;   // 2V reals as V complex
+        Scalar1 sg;
+        launch_plaq_sum(d.L, d.U64.p(), sg.fin(), S());
+        out[1] = -beta * sg.read();
+        const SolveResult r = cg_solve(*D->op64, phi.p(), X.p(), nullptr, nullptr, maxit, tol, S());
+        out[2] = dev_dot<double>(phi.p(), X.p(), 2 * V, S()).re;
+        return r.iterations;
+    }
+    // plain_leapfrog_evolve, src/hmc_trajectory.cpp:243-409
+    int evolve(const smg_hmc_params& p) {
+        SMG_REQUIRE(p.n_steps >= 1, "n_steps must be >= 1");
+        const double dt = p.tau / p.n_steps;
+        int cg = 0;
+        if (p.omelyan) {   // P(l e) [Q(e/2) P((1-2l) e) Q(e/2) P(2l e)]^(N-1) ... P(l e), l = 0.1932
+            const double lambda = 0.1932;
+            cg += force(p.beta, p.cg_maxiter, p.cg_tol);
+            mom(lambda * dt);
+            for (int step = 0; step < p.n_steps; step++) {
+                gauge(0.5 * dt);
+                cg += force(p.beta, p.cg_maxiter, p.cg_tol);
+                mom((1.0 - 2.0 * lambda) * dt);
+                gauge(0.5 * dt);
+                cg += force(p.beta, p.cg_maxiter, p.cg_tol);
+                mom((step < p.n_steps - 1 ? 2.0 : 1.0) * lambda * dt);
+            }
+        } else {   // P(dt/2) [Q(dt) P(dt)]^(N-1) Q(dt) P(dt/2)
+            cg += force(p.beta, p.cg_maxiter, p.cg_tol);
+            mom(0.5 * dt);
+            for (int step = 0; step < p.n_steps - 1; step++) {
+                gauge(dt);
+                cg += force(p.beta, p.cg_maxiter, p.cg_tol);
+                mom(dt);
+            }
+            gauge(dt);
+            cg += force(p.beta, p.cg_maxiter, p.cg_tol);
+            mom(0.5 * dt);
+        }
+        return cg;
+    }
+    void pi_out(double* h) { down(h, pi.p, 2 * V * sizeof(double)); }
+};
+void check_hmc(const smg_hmc_params* p) {
+    SMG_REQUIRE(p, "null parameters");
+    SMG_REQUIRE(p->n_steps >= 1 && p->tau > 0.0, "tau > 0 and n_steps >= 1 required");
+    SMG_REQUIRE(p->cg_maxiter >= 1 && p->cg_tol > 0.0, "bad CG budget");
+}
+}  // namespace
+
+int smg_dirac_get_gauge(smg_dirac* D, double* U) {
+    return guard([&] {
+        SMG_REQUIRE(D && U, "null argument");
+        downc(U, D->D->U64.p(), 2 * (size_t)D->D->V);
+    });
+}
+int smg_hmc_evolve(smg_dirac* D, const smg_hmc_params* p, double* pi, const double* phi, int* cg_iters) {
+    return guard([&] {
+        SMG_REQUIRE(D && pi && phi, "null argument");
+        check_hmc(p);
+        MDState md(D, pi, phi);
+        const int it = md.evolve(*p);
+        md.pi_out(pi);
+        if (cg_iters) *cg_iters = it;
+    });
+}
+int smg_hmc_hamiltonian(smg_dirac* D, double beta, const double* pi, const double* phi, int cg_maxiter,
+                        double cg_tol, double* out3, int* cg_iters) {
+    return guard([&] {
+        SMG_REQUIRE(D && pi && phi && out3, "null argument");
+        MDState md(D, pi, phi);
+        const int it = md.hamiltonian(beta, cg_maxiter, cg_tol, out3);
+        if (cg_iters) *cg_iters = it;
+    });
+}
+int smg_hmc_trajectory(smg_dirac* D, const smg_hmc_params* p, const double* pi0, const double* eta, double u,
+                       smg_hmc_result* res) {
+    return guard([&] {
+        SMG_REQUIRE(D && pi0 && eta && res, "null argument");
+        check_hmc(p);
+        Dirac& d = *D->D;
+        const size_t V = (size_t)d.V;
+        Field<double> saved(2 * V);   // gauge_old, src/hmc_trajectory.cpp:48-50
+        launch_copy<double>(saved.p(), d.U64.p(), 2 * V, S());
+        MDState md(D, pi0, nullptr);
+        // phi = D^dag eta (generate_pseudofermion, src/hmc_trajectory.cpp:32-36)
+        Field<double> e = up(eta, 2 * V);
+        launch_dslash<double>(d.v64(), md.phi.p(), e.p(), -d.mu_t, 1, 1, EPI_NONE, nullptr, nullptr, Fin(), 1, 2 * V,
+                              S());
+        double h0[3], h1[3];
+        int cg = md.hamiltonian(p->beta, p->cg_maxiter, p->cg_tol, h0);
+        cg += md.evolve(*p);
+        cg += md.hamiltonian(p->beta, p->cg_maxiter, p->cg_tol, h1);
+        const double dH = (h1[0] + h1[1] + h1[2]) - (h0[0] + h0[1] + h0[2]);
+        const bool accept = p->force_accept || dH < 0 || u < std::exp(-dH);
+        if (!accept) {
+            launch_copy<double>(d.U64.p(), saved.p(), 2 * V, S());
+            d.refresh_derived(S());
+        }
+        SMG_HIP(hipStreamSynchronize(S()));
+        res->accepted = accept ? 1 : 0;
+        res->dH = dH;
+        res->dKE = h1[0] - h0[0];
+        res->dSG = h1[1] - h0[1];
+        res->dSF = h1[2] - h0[2];
+        res->total_cg_iters = cg;
+    });
+}

@@ -151,22 +151,19 @@ void Dirac::set_gauge(const c64* U, bool device_ptr) {
     hipStream_t st = Context::get().stream;
     SMG_HIP(hipMemcpyAsync(U64.p(), U, 2 * (size_t)V * sizeof(c64),
                            device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    refresh_derived(st);
+    if (!device_ptr) SMG_HIP(hipStreamSynchronize(st));   // the host array may go away
+}
+void Dirac::refresh_derived(hipStream_t st) {
     launch_convert<float, double>(U32.p(), U64.p(), 2 * (size_t)V, st);
     launch_clover(L, U64.p(), F01.as<double>(), st);
-    if (csw != 0.0) {
-        // C = 0.5 * csw * F01 (src/dirac.h:102), kept in FP64 and FP32
-        std::vector<double> h(V);
-        SMG_HIP(hipMemcpyAsync(h.data(), F01.p, V * sizeof(double), hipMemcpyDeviceToHost, st));
-        SMG_HIP(hipStreamSynchronize(st));
-        std::vector<float> hf(V);
-        for (int s = 0; s < V; s++) {
-            h[s] = 0.5 * csw * h[s];
-            hf[s] = (float)h[s];
-        }
-        SMG_HIP(hipMemcpyAsync(C64.p, h.data(), V * sizeof(double), hipMemcpyHostToDevice, st));
-        SMG_HIP(hipMemcpyAsync(C32.p, hf.data(), V * sizeof(float), hipMemcpyHostToDevice, st));
-        SMG_HIP(hipStreamSynchronize(st));
-    }
+    // C = 0.5 * csw * F01 (src/dirac.h:102), kept in FP64 and FP32
+    if (csw != 0.0) launch_clover_scale(F01.as<double>(), csw, C64.as<double>(), C32.as<float>(), V, st);
+}
+void Dirac::update_links(const double* pi, double scale, hipStream_t st) {
+    launch_link_update(U64.p(), U32.p(), pi, scale, 2 * (size_t)V, st);
+    launch_clover(L, U64.p(), F01.as<double>(), st);
+    if (csw != 0.0) launch_clover_scale(F01.as<double>(), csw, C64.as<double>(), C32.as<float>(), V, st);
 }
 
 DiracView<double> Dirac::v64() const {

@@ -85,6 +85,11 @@ struct Dirac {
     DevMem C32;  // float [V]
     Dirac(int L, const c64* U_host, double mass, double r, double csw, double mu_t);
     void set_gauge(const c64* U, bool device_ptr);
+    // U <- U exp(i scale pi) on the device (pi: 2V reals), then the FP32 links and
+    // the clover term are recomputed
+    void update_links(const double* pi, double scale, hipStream_t st);
+    // FP32 links, F01 and the clover term from U64
+    void refresh_derived(hipStream_t st);
     DiracView<double> v64() const;
     DiracView<float> v32() const;
     template <typename T>

@@ -72,6 +72,11 @@ void launch_eo_force_weights(const DiracView<double>& D, const c64* chi_o, const
 // logdet weight: even sites 0.5 csw (d0/(d0^2+mu^2) - d1/(d1^2+mu^2)), odd 0
 void launch_logdet_weight(const DiracView<double>& D, double csw, double mu, double* w, hipStream_t st);
 void launch_gauge_force(int L, const c64* U, double beta, double* F, hipStream_t st);
+// HMC molecular dynamics (hmc.hip): U <- U exp(i scale pi) with the FP32 copy
+// rewritten; pi <- pi + scale (gf + ff); C = 0.5 csw F01 in FP64 and FP32
+void launch_link_update(c64* U, c32* U32, const double* pi, double scale, size_t n, hipStream_t st);
+void launch_mom_update(double* pi, const double* gf, const double* ff, double scale, size_t n, hipStream_t st);
+void launch_clover_scale(const double* F01, double csw, double* C64, float* C32, int V, hipStream_t st);
 
 // ---------------------------------------------------------------- BLAS-1 (blas.hip)
 // reductions finished in-kernel by the scalar program of fin

@@ -93,6 +93,16 @@ def main():
     D = R.Dirac(L, U, m, 1.0, 1.0, 0.0)
     x, _, _ = D.mg(3, 4, k, 4 * k, 10, starts3, 3, 3, True, 0, rhs3)
     out["L32_U"], out["L32_starts"], out["L32_rhs"], out["L32_mg3_x"] = U, starts3, rhs3, x
+    # HMC molecular dynamics (plain_leapfrog_evolve): leapfrog and Omelyan at L = 6
+    L = 6
+    V = L * L
+    U = random_gauge(L, 0.4, seed=41)
+    pi = np.random.default_rng(42).normal(size=2 * V)
+    phi = gaussian_vec(2 * V, seed=43)
+    out["L6_U"], out["L6_pi"], out["L6_phi"] = U, pi, phi
+    for om in (0, 1):
+        Uf, pf, it = R.leapfrog_evolve(L, U, 0.1, 1.0, 1.0, 2.0, 0.5, 3, om, 500, 1e-12, pi, phi)
+        out[f"L6_md{om}_U"], out[f"L6_md{om}_pi"], out[f"L6_md{om}_it"] = Uf, pf, np.array([it])
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_vectors.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes,", len(out), "arrays")

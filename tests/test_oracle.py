@@ -124,6 +124,16 @@ def test_golden_multigrid(g):
     assert close(mg3cg(g["L32_rhs"]), g["L32_mg3_x"], 1e-10)
 
 
+def test_golden_hmc_md(g):
+    """plain_leapfrog_evolve (leapfrog and Omelyan) against the reference's outputs."""
+    L = 6
+    for om in (0, 1):
+        U, pi, it = O.hmc_evolve(L, g["L6_U"], 0.1, 1.0, 1.0, 0.0, 2.0, 0.5, 3, om, 500, 1e-12, g["L6_pi"],
+                                 g["L6_phi"])
+        assert close(U, g[f"L6_md{om}_U"], 1e-13) and close(pi, g[f"L6_md{om}_pi"], 1e-12)
+        assert it == g[f"L6_md{om}_it"][0]
+
+
 # ------------------------------------------------------------------ live compiled reference
 @pytest.mark.skipif(not R.available(), reason="oracle/_ref/libref.so not built (reference sources absent)")
 @pytest.mark.parametrize("L,seed", [(6, 1), (10, 2), (16, 3)])
@@ -139,6 +149,18 @@ def test_live_reference_random(L, seed):
             assert close(Do.eo(kind, psi[:V]), Dr.eo(kind, psi[:V]))
         assert close(Do.force(psi), Dr.force(0, psi))
         assert close(Do.eo_force(psi[:V], psi[V:]), Dr.force(4, psi[:V], psi[V:]))
+
+
+@pytest.mark.skipif(not R.available(), reason="oracle/_ref/libref.so not built (reference sources absent)")
+@pytest.mark.parametrize("omelyan", [0, 1])
+def test_live_reference_hmc_md(omelyan):
+    L, V = 8, 64
+    U = random_gauge(L, 0.3, 5)
+    pi = np.random.default_rng(1).standard_normal(2 * V)
+    phi = gaussian_vec(2 * V, 6)
+    Uo, po, io = O.hmc_evolve(L, U, 0.1, 1.0, 1.0, 0.0, 2.0, 0.5, 4, omelyan, 500, 1e-12, pi, phi)
+    Ur, pr, ir = R.leapfrog_evolve(L, U, 0.1, 1.0, 1.0, 2.0, 0.5, 4, omelyan, 500, 1e-12, pi, phi)
+    assert close(Uo, Ur, 1e-13) and close(po, pr, 1e-12) and io == ir
 
 
 # ------------------------------------------------------------------ mathematics
@@ -331,3 +353,23 @@ def test_forces_central_difference():
                 Sg.append(O.gauge_action(L, Up, 2.0))
             assert abs(-(S[0] - S[1]) / (2 * eps) - F[mu * V + s]) < 1e-6 * max(1.0, abs(F).max())
             assert abs(-(Sg[0] - Sg[1]) / (2 * eps) - Fg[mu * V + s]) < 1e-6 * max(1.0, abs(Fg).max())
+
+
+@pytest.mark.parametrize("omelyan", [0, 1])
+def test_hmc_md_reversible_and_second_order(omelyan):
+    """The MD integrators are time-reversible (forward, negate the momenta, forward
+    again returns to the start) and second order: dH shrinks about 4x when dt halves."""
+    L, V = 6, 36
+    U = random_gauge(L, 0.3, 7)
+    pi = np.random.default_rng(8).standard_normal(2 * V)
+    phi = gaussian_vec(2 * V, 9)
+    args = (0.1, 1.0, 1.0, 0.0, 2.0)
+    U1, p1, _ = O.hmc_evolve(L, U, *args, 0.5, 4, omelyan, 1000, 1e-14, pi, phi)
+    U2, p2, _ = O.hmc_evolve(L, U1, *args, 0.5, 4, omelyan, 1000, 1e-14, -p1, phi)
+    assert np.abs(U2 - U).max() < 1e-11 and np.abs(-p2 - pi).max() < 1e-10
+    h0 = O.hmc_hamiltonian(L, U, *args, 1000, 1e-14, pi, phi).sum()
+    dH = []
+    for n in (8, 16):
+        Uf, pf, _ = O.hmc_evolve(L, U, *args, 0.5, n, omelyan, 1000, 1e-14, pi, phi)
+        dH.append(abs(O.hmc_hamiltonian(L, Uf, *args, 1000, 1e-14, pf, phi).sum() - h0))
+    assert 3.0 < dH[0] / dH[1] < 5.0
