@@ -22,6 +22,8 @@
 #include <cmath>
 #include <complex>
 #include <functional>
+#include <iomanip>
+#include <iostream>
 #include <memory>
 #include <random>
 #include <stdexcept>
@@ -660,6 +662,261 @@ inline void eo_fermion_force(const DiracOp& D, const EvenOddDiracOp& eoD, const 
 inline void generate_pseudofermion(const DiracOp& D, std::mt19937& rng, Vec& phi) {
     const Vec eta = random_vec(D.lat.ndof, rng);
     D.apply_dag(eta, phi);
+}
+
+// ------------------------------------------------------------------ HMC, src/hmc.h, src/hmc_trajectory.cpp
+// MomentumField, src/hmc.h:31-44 (randomise draws pi[0] then pi[1] from rng)
+struct MomentumField {
+    const Lattice& lat;
+    std::array<RVec, 2> pi;
+    explicit MomentumField(const Lattice& lat_) : lat(lat_) {
+        pi[0].assign(lat.V, 0.0);
+        pi[1].assign(lat.V, 0.0);
+    }
+    void randomise(std::mt19937& rng) {
+        std::normal_distribution<double> dist(0.0, 1.0);
+        for (int mu = 0; mu < 2; mu++)
+            for (int s = 0; s < lat.V; s++) pi[mu][s] = dist(rng);
+    }
+    double kinetic_energy() const {
+        double ke = 0.0;
+        for (int mu = 0; mu < 2; mu++)
+            for (int s = 0; s < lat.V; s++) ke += pi[mu][s] * pi[mu][s];
+        return 0.5 * ke;
+    }
+};
+struct HMCParams {   // src/hmc.h:107-119
+    double beta;
+    double tau;
+    int n_steps;
+    int cg_maxiter;
+    double cg_tol;
+    bool use_mg;
+    double c_sw = 0.0;
+    double mu_t = 0.0;
+    bool use_eo = false;
+    bool omelyan = false;
+    bool force_accept = false;
+};
+struct HMCResult {   // src/hmc.h:120-128
+    bool accepted;
+    double dH, dKE, dSG, dSF, dLD;
+    int total_cg_iters;
+};
+struct ReversibilityResult {   // src/hmc.h:402-408
+    double gauge_delta, mom_delta, dH_forward, dH_backward;
+    int total_cg;
+};
+struct TrackingState;   // eigenspace tracking is not provided by this library
+
+namespace detail {
+inline smg_hmc_params hmc_c(const HMCParams& p) {
+    smg_hmc_params c{};
+    c.beta = p.beta;
+    c.tau = p.tau;
+    c.n_steps = p.n_steps;
+    c.omelyan = p.omelyan ? 1 : 0;
+    c.cg_maxiter = p.cg_maxiter;
+    c.cg_tol = p.cg_tol;
+    c.force_accept = p.force_accept ? 1 : 0;
+    return c;
+}
+inline RVec mom_flat(const MomentumField& m) {
+    RVec p(2 * (size_t)m.lat.V);
+    std::copy(m.pi[0].begin(), m.pi[0].end(), p.begin());
+    std::copy(m.pi[1].begin(), m.pi[1].end(), p.begin() + m.lat.V);
+    return p;
+}
+inline void mom_unflat(const RVec& p, MomentumField& m) {
+    m.pi[0].assign(p.begin(), p.begin() + m.lat.V);
+    m.pi[1].assign(p.begin() + m.lat.V, p.end());
+}
+// {KE, S_g, S_f} on D's gauge field; returns the CG iterations
+inline int hamiltonian(const DiracOp& D, double beta, const MomentumField& mom, const Vec& phi, int maxit,
+                       double tol, double* h) {
+    const RVec p = mom_flat(mom);
+    int it = 0;
+    check(smg_hmc_hamiltonian(D.handle(), beta, p.data(), raw(phi), maxit, tol, h, &it));
+    return it;
+}
+inline void no_extras(const std::function<Vec(const Vec&)>* precond, TrackingState* tracking, const char* who) {
+    if (precond || tracking)
+        throw std::invalid_argument(std::string(who) + ": preconditioned or tracked MD is not supported");
+}
+}  // namespace detail
+
+// plain_leapfrog_evolve, src/hmc_trajectory.cpp:243-409: gauge and mom evolve in place
+// (on the GPU, copied back); returns the CG iterations
+inline int plain_leapfrog_evolve(GaugeField& gauge, MomentumField& mom, const Lattice& lat, double mass,
+                                 double wilson_r, const Vec& phi, const HMCParams& params,
+                                 const std::function<Vec(const Vec&)>* precond = nullptr,
+                                 TrackingState* tracking = nullptr) {
+    detail::no_extras(precond, tracking, "plain_leapfrog_evolve");
+    DiracOp D(lat, gauge, mass, wilson_r, params.c_sw, params.mu_t);
+    const smg_hmc_params p = detail::hmc_c(params);
+    RVec pi = detail::mom_flat(mom);
+    int it = 0;
+    detail::check(smg_hmc_evolve(D.handle(), &p, pi.data(), detail::raw(phi), &it));
+    Vec u(2 * (size_t)lat.V);
+    detail::check(smg_dirac_get_gauge(D.handle(), detail::raw(u)));
+    gauge.U[0].assign(u.begin(), u.begin() + lat.V);
+    gauge.U[1].assign(u.begin() + lat.V, u.end());
+    detail::mom_unflat(pi, mom);
+    return it;
+}
+
+// hmc_trajectory, full-lattice path, src/hmc_trajectory.cpp:38-45, 198-236: momenta,
+// then eta (phi = D^dag eta), then (only if needed) the Metropolis uniform are drawn
+// from rng in the reference's order
+inline HMCResult hmc_trajectory(GaugeField& gauge, const Lattice& lat, double mass, double wilson_r,
+                                const HMCParams& params, std::mt19937& rng,
+                                const std::function<Vec(const Vec&)>* precond = nullptr,
+                                TrackingState* tracking = nullptr) {
+    detail::no_extras(precond, tracking, "hmc_trajectory");
+    if (params.use_eo) throw std::invalid_argument("hmc_trajectory: the even-odd trajectory is not supported");
+    const std::array<Vec, 2> old = gauge.U;
+    MomentumField mom(lat);
+    mom.randomise(rng);
+    Vec phi;
+    double h0[3], h1[3];
+    int cg = 0;
+    {
+        DiracOp D0(lat, gauge, mass, wilson_r, params.c_sw, params.mu_t);
+        generate_pseudofermion(D0, rng, phi);
+        cg += detail::hamiltonian(D0, params.beta, mom, phi, params.cg_maxiter, params.cg_tol, h0);
+    }
+    cg += plain_leapfrog_evolve(gauge, mom, lat, mass, wilson_r, phi, params);
+    {
+        DiracOp D1(lat, gauge, mass, wilson_r, params.c_sw, params.mu_t);
+        cg += detail::hamiltonian(D1, params.beta, mom, phi, params.cg_maxiter, params.cg_tol, h1);
+    }
+    const double dH = (h1[0] + h1[1] + h1[2]) - (h0[0] + h0[1] + h0[2]);
+    std::uniform_real_distribution<double> uniform(0.0, 1.0);
+    const bool accept = params.force_accept || (dH < 0) || (uniform(rng) < std::exp(-dH));
+    if (!accept) gauge.U = old;
+    return {accept, dH, h1[0] - h0[0], h1[1] - h0[1], h1[2] - h0[2], 0.0, cg};
+}
+
+// reversibility_test_plain, src/hmc_trajectory.cpp:416-509 (the gauge field is restored)
+inline ReversibilityResult reversibility_test_plain(GaugeField& gauge, const Lattice& lat, double mass,
+                                                    double wilson_r, const HMCParams& params, std::mt19937& rng,
+                                                    const std::function<Vec(const Vec&)>* precond = nullptr) {
+    detail::no_extras(precond, nullptr, "reversibility_test_plain");
+    const std::array<Vec, 2> save = gauge.U;
+    Vec phi;
+    {
+        DiracOp D0(lat, gauge, mass, wilson_r, params.c_sw, params.mu_t);
+        generate_pseudofermion(D0, rng, phi);
+    }
+    MomentumField mom(lat);
+    mom.randomise(rng);
+    const std::array<RVec, 2> pi_save = mom.pi;
+    auto H = [&](double* h) {
+        DiracOp D(lat, gauge, mass, wilson_r, params.c_sw, params.mu_t);
+        detail::hamiltonian(D, params.beta, mom, phi, params.cg_maxiter, params.cg_tol, h);
+        return h[0] + h[1] + h[2];
+    };
+    double h[3];
+    const double H0 = H(h);
+    const int cg_fwd = plain_leapfrog_evolve(gauge, mom, lat, mass, wilson_r, phi, params);
+    const double H1 = H(h);
+    for (int mu = 0; mu < 2; mu++)
+        for (auto& p : mom.pi[mu]) p = -p;
+    const int cg_bwd = plain_leapfrog_evolve(gauge, mom, lat, mass, wilson_r, phi, params);
+    const double H2 = H(h);
+    double gn = 0, gd = 0, mn = 0, md = 0;
+    for (int mu = 0; mu < 2; mu++)
+        for (int s = 0; s < lat.V; s++) {
+            gn += std::norm(save[mu][s]);
+            gd += std::norm(gauge.U[mu][s] - save[mu][s]);
+            mn += pi_save[mu][s] * pi_save[mu][s];
+            const double d = -mom.pi[mu][s] - pi_save[mu][s];
+            md += d * d;
+        }
+    gauge.U = save;
+    return {std::sqrt(gd / std::max(gn, 1e-30)), std::sqrt(md / std::max(mn, 1e-30)), H1 - H0, H2 - H1,
+            cg_fwd + cg_bwd};
+}
+
+// verify_forces, src/hmc_forces.cpp:869-981: central differences (eps = 1e-5) of the
+// gauge and fermion actions on 20 links against the analytic forces, all on the GPU;
+// true if both relative errors are below err_threshold
+inline bool verify_forces(const GaugeField& g, double beta, double mass, double wilson_r, int max_iter, double tol,
+                          double c_sw = 0.0, bool use_eo = false, double err_threshold = 1e-4, double mu_t = 0.0) {
+    const Lattice& lat = g.lat;
+    const double eps = 1e-5;
+    std::mt19937 rng(12345);
+    const int n_tests = 20;
+    std::cout << "\n=== Force Verification (" << (use_eo ? "E/O" : "Full") << ", c_sw=" << c_sw
+              << ", mu_t=" << mu_t << ") ===\n";
+    DiracOp D0(lat, g, mass, wilson_r, c_sw, mu_t);
+    std::array<RVec, 2> gf_ana, ff_ana;
+    gauge_force(g, beta, gf_ana);
+    const int nh = 2 * lat.V_half;
+    Vec phi, phi_o;
+    smg_solve_info info{};
+    if (!use_eo) {
+        const Vec eta = random_vec(lat.ndof, rng);
+        D0.apply_dag(eta, phi);
+        Vec X(lat.ndof);
+        detail::check(smg_cg_solve(D0.handle(), SMG_OP_DDAGD, nullptr, detail::raw(phi), nullptr, detail::raw(X),
+                                   max_iter, tol, &info));
+        fermion_force(D0, X, ff_ana);
+    } else {
+        EvenOddDiracOp eo(D0);
+        const Vec eta = random_vec(nh, rng);
+        eo.apply_schur_dag(eta, phi_o);
+        Vec x(nh), y(nh);
+        detail::check(smg_cg_solve(D0.handle(), SMG_OP_SCHUR, nullptr, detail::raw(phi_o), nullptr, detail::raw(x),
+                                   max_iter, tol, &info));
+        eo.apply_schur(x, y);
+        eo_fermion_force(D0, eo, x, y, ff_ana);
+    }
+    auto fermion_S = [&](const GaugeField& gf) {
+        DiracOp Dp(lat, gf, mass, wilson_r, c_sw, mu_t);
+        if (!use_eo) {
+            Vec X(lat.ndof);
+            detail::check(smg_cg_solve(Dp.handle(), SMG_OP_DDAGD, nullptr, detail::raw(phi), nullptr,
+                                       detail::raw(X), max_iter, tol, &info));
+            std::complex<double> acc(0.0, 0.0);
+            for (int i = 0; i < lat.ndof; i++) acc += std::conj(phi[i]) * X[i];
+            return acc.real();
+        }
+        EvenOddDiracOp eo(Dp);
+        Vec x(nh);
+        detail::check(smg_cg_solve(Dp.handle(), SMG_OP_SCHUR, nullptr, detail::raw(phi_o), nullptr, detail::raw(x),
+                                   max_iter, tol, &info));
+        std::complex<double> acc(0.0, 0.0);
+        for (int i = 0; i < nh; i++) acc += std::conj(phi_o[i]) * x[i];
+        double S = acc.real();
+        if (c_sw != 0.0) S -= 2.0 * eo.log_det_ee();
+        return S;
+    };
+    double max_gf_err = 0, max_ff_err = 0, max_gf_val = 0, max_ff_val = 0;
+    for (int test = 0; test < n_tests; test++) {
+        const int mu = test % 2, s = (test * 7 + test / 2 * 13) % lat.V;
+        GaugeField gp(lat), gm(lat);
+        gp.U = g.U;
+        gm.U = g.U;
+        gp.U[mu][s] *= std::exp(cx(0, eps));
+        gm.U[mu][s] *= std::exp(cx(0, -eps));
+        const double gf_num = -(gauge_action(gp, beta) - gauge_action(gm, beta)) / (2 * eps);
+        const double ff_num = -(fermion_S(gp) - fermion_S(gm)) / (2 * eps);
+        max_gf_err = std::max(max_gf_err, std::abs(gf_ana[mu][s] - gf_num));
+        max_gf_val = std::max(max_gf_val, std::abs(gf_ana[mu][s]));
+        max_ff_err = std::max(max_ff_err, std::abs(ff_ana[mu][s] - ff_num));
+        max_ff_val = std::max(max_ff_val, std::abs(ff_ana[mu][s]));
+        std::cout << "  link(" << mu << "," << std::setw(3) << s << "): gf_ana=" << std::setw(12) << gf_ana[mu][s]
+                  << " gf_num=" << std::setw(12) << gf_num << " | ff_ana=" << std::setw(12) << ff_ana[mu][s]
+                  << " ff_num=" << std::setw(12) << ff_num << "\n";
+    }
+    const double gf_rel = max_gf_err / std::max(max_gf_val, 1e-30);
+    const double ff_rel = max_ff_err / std::max(max_ff_val, 1e-30);
+    std::cout << "Gauge force:   max_err=" << std::scientific << max_gf_err << " rel=" << gf_rel
+              << (gf_rel < err_threshold ? " PASS" : " FAIL") << "\n";
+    std::cout << "Fermion force: max_err=" << std::scientific << max_ff_err << " rel=" << ff_rel
+              << (ff_rel < err_threshold ? " PASS" : " FAIL") << "\n";
+    return gf_rel < err_threshold && ff_rel < err_threshold;
 }
 
 }  // namespace schwinger_mg

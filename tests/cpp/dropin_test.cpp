@@ -44,6 +44,10 @@ void orc_mg_precondition(orc_mg* mg, const cx* b, cx* x);
 int orc_cg(const orc_op* A, const cx* rhs, const cx* x0, orc_mg* mg, int maxit, double tol, cx* x, double* res);
 void orc_fermion_force(const orc_dirac* D, const cx* X, double* F);
 void orc_gauge_force(int L, const cx* U, double beta, double* F);
+int orc_hmc_evolve(int L, cx* U, double mass, double r, double csw, double mu_t, double beta, double tau,
+                   int n_steps, int omelyan, int cg_maxit, double cg_tol, double* pi, const cx* phi);
+int orc_hmc_hamiltonian(int L, const cx* U, double mass, double r, double csw, double mu_t, double beta,
+                        int cg_maxit, double cg_tol, const double* pi, const cx* phi, double* out);
 }
 
 static int failures = 0;
@@ -158,6 +162,75 @@ int main() {
     report("gauge_force", relerr(F, Fo, lat.V), 1e-12);
     FermionActionResult fa = fermion_action(D, b, 2000, 1e-10);
     report("fermion_action X vs CG", relerr(fa.X, xcg), 1e-8);
+
+    // HMC (src/hmc_trajectory.cpp): the MD evolution and a trajectory against the oracle
+    {
+        const int Lh = 8;
+        Lattice lh(Lh);
+        GaugeField gh(lh);
+        std::mt19937 r2(7);
+        gh.randomise(r2, 0.3);
+        for (int om = 0; om < 2; om++) {
+            HMCParams hp{};
+            hp.beta = 2.0;
+            hp.tau = 0.5;
+            hp.n_steps = 4;
+            hp.cg_maxiter = 2000;
+            hp.cg_tol = 1e-12;
+            hp.c_sw = 1.0;
+            hp.omelyan = om == 1;
+            GaugeField ge(lh);
+            ge.U = gh.U;
+            MomentumField mom(lh);
+            mom.randomise(r2);
+            const Vec phi = random_vec(lh.ndof, r2);
+            RVec pio = detail::mom_flat(mom);
+            Vec Uo = gh.links();
+            plain_leapfrog_evolve(ge, mom, lh, 0.1, 1.0, phi, hp);
+            orc_hmc_evolve(Lh, Uo.data(), 0.1, 1.0, 1.0, 0.0, 2.0, 0.5, 4, om, 2000, 1e-12, pio.data(), phi.data());
+            report(om ? "plain_leapfrog_evolve (Omelyan) links" : "plain_leapfrog_evolve links",
+                   relerr(ge.links(), Uo), 1e-11);
+            report(om ? "plain_leapfrog_evolve (Omelyan) momenta" : "plain_leapfrog_evolve momenta",
+                   relerr(mom.pi, pio, lh.V), 1e-10);
+        }
+        // hmc_trajectory draws momenta, then eta, from rng: the oracle replays the draws
+        HMCParams hp{};
+        hp.beta = 2.0;
+        hp.tau = 0.5;
+        hp.n_steps = 4;
+        hp.cg_maxiter = 2000;
+        hp.cg_tol = 1e-12;
+        hp.c_sw = 1.0;
+        hp.force_accept = true;
+        GaugeField gt(lh);
+        gt.U = gh.U;
+        std::mt19937 ra(99), rb(99);
+        HMCResult hr = hmc_trajectory(gt, lh, 0.1, 1.0, hp, ra);
+        MomentumField m2(lh);
+        m2.randomise(rb);
+        const Vec eta = random_vec(lh.ndof, rb);
+        Vec phi(lh.ndof);
+        orc_dirac* Dh = orc_dirac_new(Lh, gh.links().data(), 0.1, 1.0, 1.0, 0.0);
+        orc_dirac_apply(Dh, 1, eta.data(), phi.data());
+        orc_dirac_free(Dh);
+        RVec pio = detail::mom_flat(m2);
+        Vec Uo = gh.links();
+        double h0[3], h1[3];
+        orc_hmc_hamiltonian(Lh, Uo.data(), 0.1, 1.0, 1.0, 0.0, 2.0, 2000, 1e-12, pio.data(), phi.data(), h0);
+        orc_hmc_evolve(Lh, Uo.data(), 0.1, 1.0, 1.0, 0.0, 2.0, 0.5, 4, 0, 2000, 1e-12, pio.data(), phi.data());
+        orc_hmc_hamiltonian(Lh, Uo.data(), 0.1, 1.0, 1.0, 0.0, 2.0, 2000, 1e-12, pio.data(), phi.data(), h1);
+        report("hmc_trajectory dH", std::abs(hr.dH - ((h1[0] + h1[1] + h1[2]) - (h0[0] + h0[1] + h0[2]))), 1e-8);
+        report("hmc_trajectory links", relerr(gt.links(), Uo), 1e-11);
+        GaugeField gr(lh);
+        gr.U = gh.U;
+        std::mt19937 rr(5);
+        ReversibilityResult rv = reversibility_test_plain(gr, lh, 0.1, 1.0, hp, rr);
+        report("reversibility_test_plain gauge", rv.gauge_delta, 1e-10);
+        report("reversibility_test_plain momenta", rv.mom_delta, 1e-9);
+        report("verify_forces (full, clover)", verify_forces(gh, 2.0, 0.1, 1.0, 2000, 1e-12, 1.0) ? 0.0 : 1.0, 0.0);
+        report("verify_forces (even-odd, clover)",
+               verify_forces(gh, 2.0, 0.1, 1.0, 2000, 1e-12, 1.0, true) ? 0.0 : 1.0, 0.0);
+    }
 
     orc_mg_free(mgo);
     orc_op_free(Ao);
