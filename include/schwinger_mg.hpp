@@ -110,7 +110,44 @@ struct GaugeField {
         detail::check(smg_avg_plaq(lat.L, detail::raw(u), &out));
         return out;
     }
+    // src/gauge.h:38-57: int32 V then the links; false on any failure (as the reference)
+    bool save(const std::string& path) const {
+        const Vec u = links();
+        return smg_gauge_write_raw(path.c_str(), lat.L, detail::raw(u)) == 0;
+    }
+    bool load(const std::string& path) {
+        Vec u(2 * (size_t)lat.V);
+        if (smg_gauge_read_raw(path.c_str(), lat.L, detail::raw(u)) != 0) return false;
+        U[0].assign(u.begin(), u.begin() + lat.V);
+        U[1].assign(u.begin() + lat.V, u.end());
+        return true;
+    }
 };
+
+// gauge files with the 'SWNG' header, src/hmc.h:19-29, src/hmc_forces.cpp:18-62
+using GaugeHeader = smg_gauge_header;
+inline bool save_gauge(const GaugeField& g, double beta, double mass, const std::string& filename) {
+    const Vec u = g.links();
+    const bool ok = smg_gauge_write(filename.c_str(), g.lat.L, detail::raw(u), beta, mass) == 0;
+    if (!ok) std::cerr << "Cannot open " << filename << " for writing\n";
+    return ok;
+}
+inline bool load_gauge(GaugeField& g, GaugeHeader& hdr, const std::string& filename) {
+    Vec u(2 * (size_t)g.lat.V);
+    if (smg_gauge_read(filename.c_str(), g.lat.L, detail::raw(u), &hdr) != 0) {
+        std::cerr << smg_last_error() << "\n";
+        return false;
+    }
+    g.U[0].assign(u.begin(), u.begin() + g.lat.V);
+    g.U[1].assign(u.begin() + g.lat.V, u.end());
+    return true;
+}
+// therm_cfg_path, src/hmc_utils.cpp:7-11
+inline std::string therm_cfg_path(int L, double beta, int n_therm) {
+    char buf[256];
+    detail::check(smg_therm_cfg_path(L, beta, n_therm, buf, sizeof(buf)));
+    return std::string(buf);
+}
 
 // ------------------------------------------------------------------ DiracOp, src/dirac.h:8-183
 struct DiracOp;

@@ -24,6 +24,7 @@
 #ifndef SMG_H
 #define SMG_H
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -310,6 +311,28 @@ int smg_hmc_hamiltonian(smg_dirac* D, double beta, const double* pi, const doubl
  * when dH >= 0 and force_accept == 0.  A rejected trajectory restores the gauge field. */
 int smg_hmc_trajectory(smg_dirac* D, const smg_hmc_params* p, const double* pi0, const double* eta, double u,
                        smg_hmc_result* res);
+
+/* ------------------------------------------------------------------ gauge field files (host I/O)
+ * Links are U[mu*V + s] (2V complex).  Host-only: no GPU is touched. */
+/* GaugeField::save / load, src/gauge.h:38-57: int32 V, then U[0] and U[1] as complex
+ * doubles (host byte order); reading fails if V != L*L */
+int smg_gauge_write_raw(const char* path, int L, const double* U);
+int smg_gauge_read_raw(const char* path, int L, double* U);
+/* GaugeHeader, src/hmc.h:19-25 (32 bytes, host byte order) */
+typedef struct {
+    uint32_t magic; /* 0x53574E47 = "SWNG" */
+    uint32_t L;
+    double beta;
+    double mass;
+    double avg_plaq;
+} smg_gauge_header;
+/* save_gauge / load_gauge, src/hmc_forces.cpp:18-62: the header (avg_plaq summed in
+ * site order) then the links; reading checks the magic number and L and renormalises
+ * every link to |U| = 1; hdr (optional) receives the header */
+int smg_gauge_write(const char* path, int L, const double* U, double beta, double mass);
+int smg_gauge_read(const char* path, int L, double* U, smg_gauge_header* hdr);
+/* therm_cfg_path, src/hmc_utils.cpp:7-11: "gauge_L<L>_b<beta as %f, 4 chars>_t<n>.bin" */
+int smg_therm_cfg_path(int L, double beta, int n_therm, char* out, size_t cap);
 
 #ifdef __cplusplus
 }

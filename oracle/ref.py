@@ -61,6 +61,12 @@ def lib():
         L.ref_fermion_force.argtypes = [_I, _P, _D, _D, _D, _D, _I, _P, _P, _P]
         L.ref_fermion_action.restype = _D
         L.ref_fermion_action.argtypes = [_I, _P, _D, _D, _D, _D, _P, _I, _D]
+        for name, args in (("ref_gauge_save_raw", [_I, _P, ctypes.c_char_p]),
+                           ("ref_gauge_load_raw", [_I, _P, ctypes.c_char_p]),
+                           ("ref_gauge_save", [_I, _P, _D, _D, ctypes.c_char_p]),
+                           ("ref_gauge_load", [_I, _P, _P, ctypes.c_char_p])):
+            getattr(L, name).restype = _I
+            getattr(L, name).argtypes = args
         L.ref_leapfrog_evolve.restype = _I
         L.ref_leapfrog_evolve.argtypes = [_I, _P, _D, _D, _D, _D, _D, _I, _I, _I, _D, _P, _P]
         _lib = L
@@ -206,3 +212,25 @@ def leapfrog_evolve(L, U, mass, r, csw, beta, tau, n_steps, omelyan, cg_maxit, c
     it = lib().ref_leapfrog_evolve(L, _ptr(U), mass, r, csw, beta, tau, n_steps, int(omelyan), cg_maxit, cg_tol,
                                    _ptr(pi), _ptr(phi))
     return U, pi, it
+
+
+def gauge_save_raw(L, U, path):
+    return lib().ref_gauge_save_raw(L, _ptr(_c(U)), path.encode())
+
+
+def gauge_load_raw(L, path):
+    U = np.zeros(2 * L * L, np.complex128)
+    rc = lib().ref_gauge_load_raw(L, _ptr(U), path.encode())
+    return U if rc == 0 else None
+
+
+def gauge_save(L, U, beta, mass, path):
+    return lib().ref_gauge_save(L, _ptr(_c(U)), beta, mass, path.encode())
+
+
+def gauge_load(L, path):
+    """(U, (beta, mass, avg_plaq)) or None when the reference refuses the file."""
+    U = np.zeros(2 * L * L, np.complex128)
+    h = np.zeros(3)
+    rc = lib().ref_gauge_load(L, _ptr(U), _ptr(h), path.encode())
+    return (U, tuple(h)) if rc == 0 else None

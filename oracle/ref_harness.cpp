@@ -375,3 +375,38 @@ API int ref_leapfrog_evolve(int L, double* U, double mass, double r, double csw,
     std::memcpy(pi + lat.V, mom.pi[1].data(), lat.V * sizeof(double));
     return iters;
 }
+
+// gauge files: GaugeField::save / load (src/gauge.h:38-57), save_gauge / load_gauge
+// (src/hmc_forces.cpp:18-62)
+API int ref_gauge_save_raw(int L, const double* U, const char* path) {
+    Lattice lat(L);
+    GaugeField g(lat);
+    load_gauge_arr(g, U);
+    return g.save(path) ? 0 : -1;
+}
+API int ref_gauge_load_raw(int L, double* U, const char* path) {
+    Lattice lat(L);
+    GaugeField g(lat);
+    if (!g.load(path)) return -1;
+    from_vec(g.U[0], U);
+    from_vec(g.U[1], U + 2 * lat.V);
+    return 0;
+}
+API int ref_gauge_save(int L, const double* U, double beta, double mass, const char* path) {
+    Lattice lat(L);
+    GaugeField g(lat);
+    load_gauge_arr(g, U);
+    return save_gauge(g, beta, mass, path) ? 0 : -1;
+}
+API int ref_gauge_load(int L, double* U, double* hdr3, const char* path) {
+    Lattice lat(L);
+    GaugeField g(lat);
+    GaugeHeader h{};
+    if (!load_gauge(g, h, path)) return -1;
+    from_vec(g.U[0], U);
+    from_vec(g.U[1], U + 2 * lat.V);
+    hdr3[0] = h.beta;
+    hdr3[1] = h.mass;
+    hdr3[2] = h.avg_plaq;
+    return 0;
+}

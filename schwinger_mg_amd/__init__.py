@@ -44,6 +44,11 @@ class SolveInfo(ctypes.Structure):
     _fields_ = [("iterations", _I), ("final_residual", _D), ("true_residual", _D), ("restarts", _I)]
 
 
+class GaugeHeader(ctypes.Structure):
+    """GaugeHeader (src/hmc.h:19-25)."""
+    _fields_ = [("magic", ctypes.c_uint32), ("L", ctypes.c_uint32), ("beta", _D), ("mass", _D), ("avg_plaq", _D)]
+
+
 class HMCParams(ctypes.Structure):
     """HMCParams (src/hmc.h:107-119), full lattice without preconditioner."""
     _fields_ = [("beta", _D), ("tau", _D), ("n_steps", _I), ("omelyan", _I), ("cg_maxiter", _I), ("cg_tol", _D),
@@ -101,6 +106,10 @@ SIGNATURES = [
     ("smg_dirac_get_gauge", _I, [_P, _P]), ("smg_hmc_evolve", _I, [_P, _P, _P, _P, _P]),
     ("smg_hmc_hamiltonian", _I, [_P, _D, _P, _P, _I, _D, _P, _P]),
     ("smg_hmc_trajectory", _I, [_P, _P, _P, _P, _D, _P]),
+    ("smg_gauge_write_raw", _I, [ctypes.c_char_p, _I, _P]), ("smg_gauge_read_raw", _I, [ctypes.c_char_p, _I, _P]),
+    ("smg_gauge_write", _I, [ctypes.c_char_p, _I, _P, _D, _D]),
+    ("smg_gauge_read", _I, [ctypes.c_char_p, _I, _P, _P]),
+    ("smg_therm_cfg_path", _I, [_I, _D, _I, ctypes.c_char_p, ctypes.c_size_t]),
 ]
 
 _lib = None
@@ -690,3 +699,41 @@ def hmc_trajectory(D, params, pi0, eta, u):
     _chk(lib().smg_hmc_trajectory(D.h, ctypes.byref(params), _p(np.ascontiguousarray(pi0, dtype=np.float64)),
                                   _p(_c(eta)), u, ctypes.byref(res)))
     return res
+
+
+def _hchk(rc):
+    """Status of a host-only entry point (no device initialisation)."""
+    if rc != 0:
+        raise SMGError(load_library().smg_last_error().decode())
+
+
+def gauge_save_raw(L, U, path):
+    """GaugeField::save (src/gauge.h:38-46)."""
+    _hchk(load_library().smg_gauge_write_raw(os.fsencode(path), L, _p(_c(U))))
+
+
+def gauge_load_raw(L, path):
+    """GaugeField::load (src/gauge.h:48-57)."""
+    U = np.zeros(2 * L * L, np.complex128)
+    _hchk(load_library().smg_gauge_read_raw(os.fsencode(path), L, _p(U)))
+    return U
+
+
+def save_gauge(L, U, beta, mass, path):
+    """save_gauge (src/hmc_forces.cpp:18-36): 'SWNG' header, then the links."""
+    _hchk(load_library().smg_gauge_write(os.fsencode(path), L, _p(_c(U)), beta, mass))
+
+
+def load_gauge(L, path):
+    """load_gauge (src/hmc_forces.cpp:38-62): returns (U, header)."""
+    U = np.zeros(2 * L * L, np.complex128)
+    h = GaugeHeader()
+    _hchk(load_library().smg_gauge_read(os.fsencode(path), L, _p(U), ctypes.byref(h)))
+    return U, h
+
+
+def therm_cfg_path(L, beta, n_therm):
+    """therm_cfg_path (src/hmc_utils.cpp:7-11)."""
+    buf = ctypes.create_string_buffer(256)
+    _hchk(load_library().smg_therm_cfg_path(L, beta, n_therm, buf, 256))
+    return buf.value.decode()

@@ -1,6 +1,9 @@
 // capi.cpp -- the extern "C" boundary (include/smg.h) over the C++ runtime.
 #include <algorithm>
 #include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <string>
 
@@ -1121,5 +1124,100 @@ int smg_hmc_trajectory(smg_dirac* D, const smg_hmc_params* p, const double* pi0,
         res->dSG = h1[1] - h0[1];
         res->dSF = h1[2] - h0[2];
         res->total_cg_iters = cg;
+    });
+}
+
+// ------------------------------------------------------------------ gauge field files (host I/O)
+namespace {
+// GaugeField::avg_plaq, src/gauge.h:26-36, summed in site order (the reference's
+// single-thread order)
+double host_avg_plaq(int L, const c64* U) {
+    const int V = L * L;
+    double sum = 0.0;
+    for (int s = 0; s < V; s++) {
+        const int x = s % L, y = s / L;
+        const int sxp = (x + 1) % L + L * y, syp = x + L * ((y + 1) % L);
+        const std::complex<double> u0(U[s].re, U[s].im), u1p(U[V + sxp].re, U[V + sxp].im);
+        const std::complex<double> u0p(U[syp].re, U[syp].im), u1(U[V + s].re, U[V + s].im);
+        sum += std::real(u0 * u1p * std::conj(u0p) * std::conj(u1));
+    }
+    return sum / V;
+}
+struct File {
+    FILE* f;
+    File(const char* path, const char* mode) : f(std::fopen(path, mode)) {
+        SMG_REQUIRE(f != nullptr, std::string("cannot open ") + path);
+    }
+    ~File() {
+        if (f) std::fclose(f);
+    }
+    void write(const void* p, size_t bytes) {
+        SMG_REQUIRE(std::fwrite(p, 1, bytes, f) == bytes, "short write");
+    }
+    void read(void* p, size_t bytes) {
+        SMG_REQUIRE(std::fread(p, 1, bytes, f) == bytes, "short read");
+    }
+};
+}  // namespace
+
+int smg_gauge_write_raw(const char* path, int L, const double* U) {
+    return guard([&] {
+        SMG_REQUIRE(path && U && L >= 2, "bad arguments");
+        const int V = L * L;
+        File f(path, "wb");
+        f.write(&V, sizeof(V));
+        f.write(U, 2 * (size_t)V * sizeof(c64));
+    });
+}
+int smg_gauge_read_raw(const char* path, int L, double* U) {
+    return guard([&] {
+        SMG_REQUIRE(path && U && L >= 2, "bad arguments");
+        File f(path, "rb");
+        int V = 0;
+        f.read(&V, sizeof(V));
+        SMG_REQUIRE(V == L * L, "volume in the file does not match L");
+        f.read(U, 2 * (size_t)V * sizeof(c64));
+    });
+}
+int smg_gauge_write(const char* path, int L, const double* U, double beta, double mass) {
+    return guard([&] {
+        SMG_REQUIRE(path && U && L >= 2, "bad arguments");
+        smg_gauge_header h{};
+        h.magic = 0x53574E47u;
+        h.L = (uint32_t)L;
+        h.beta = beta;
+        h.mass = mass;
+        h.avg_plaq = host_avg_plaq(L, reinterpret_cast<const c64*>(U));
+        File f(path, "wb");
+        f.write(&h, sizeof(h));
+        f.write(U, 2 * (size_t)L * L * sizeof(c64));
+    });
+}
+int smg_gauge_read(const char* path, int L, double* U, smg_gauge_header* hdr) {
+    return guard([&] {
+        SMG_REQUIRE(path && U && L >= 2, "bad arguments");
+        File f(path, "rb");
+        smg_gauge_header h{};
+        f.read(&h, sizeof(h));
+        SMG_REQUIRE(h.magic == 0x53574E47u, std::string("bad magic number in ") + path);
+        SMG_REQUIRE((int)h.L == L, "lattice size in the file does not match L");
+        const size_t n = 2 * (size_t)L * L;
+        f.read(U, n * sizeof(c64));
+        c64* u = reinterpret_cast<c64*>(U);
+        for (size_t i = 0; i < n; i++) {   // links renormalised to |U| = 1 (|U| as cabs: hypot)
+            const double a = std::hypot(u[i].re, u[i].im);
+            u[i].re /= a;
+            u[i].im /= a;
+        }
+        if (hdr) *hdr = h;
+    });
+}
+int smg_therm_cfg_path(int L, double beta, int n_therm, char* out, size_t cap) {
+    return guard([&] {
+        SMG_REQUIRE(out && cap > 0, "bad arguments");
+        const std::string p = "gauge_L" + std::to_string(L) + "_b" + std::to_string(beta).substr(0, 4) + "_t" +
+                              std::to_string(n_therm) + ".bin";
+        SMG_REQUIRE(p.size() + 1 <= cap, "output buffer too small");
+        std::memcpy(out, p.c_str(), p.size() + 1);
     });
 }
