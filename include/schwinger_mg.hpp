@@ -756,6 +756,7 @@ inline smg_hmc_params hmc_c(const HMCParams& p) {
     c.cg_maxiter = p.cg_maxiter;
     c.cg_tol = p.cg_tol;
     c.force_accept = p.force_accept ? 1 : 0;
+    c.use_eo = p.use_eo ? 1 : 0;
     return c;
 }
 inline RVec mom_flat(const MomentumField& m) {
@@ -768,17 +769,37 @@ inline void mom_unflat(const RVec& p, MomentumField& m) {
     m.pi[0].assign(p.begin(), p.begin() + m.lat.V);
     m.pi[1].assign(p.begin() + m.lat.V, p.end());
 }
-// {KE, S_g, S_f} on D's gauge field; returns the CG iterations
-inline int hamiltonian(const DiracOp& D, double beta, const MomentumField& mom, const Vec& phi, int maxit,
+// {KE, S_g, S_f, LD} on D's gauge field; returns the CG iterations
+inline int hamiltonian(const DiracOp& D, bool eo, double beta, const MomentumField& mom, const Vec& phi, int maxit,
                        double tol, double* h) {
     const RVec p = mom_flat(mom);
     int it = 0;
-    check(smg_hmc_hamiltonian(D.handle(), beta, p.data(), raw(phi), maxit, tol, h, &it));
+    check(smg_hmc_hamiltonian(D.handle(), eo ? 1 : 0, beta, p.data(), raw(phi), maxit, tol, h, &it));
     return it;
 }
 inline void no_extras(const std::function<Vec(const Vec&)>* precond, TrackingState* tracking, const char* who) {
     if (precond || tracking)
         throw std::invalid_argument(std::string(who) + ": preconditioned or tracked MD is not supported");
+}
+}  // namespace detail
+
+namespace detail {
+// the MD evolution on the GPU (eo: the even-odd trajectory's leapfrog, phi = phi_o);
+// gauge and mom are updated in place; returns the CG iterations
+inline int md_evolve(GaugeField& gauge, MomentumField& mom, const Lattice& lat, double mass, double wilson_r,
+                     const Vec& phi, const HMCParams& params, bool eo) {
+    DiracOp D(lat, gauge, mass, wilson_r, params.c_sw, params.mu_t);
+    smg_hmc_params p = hmc_c(params);
+    p.use_eo = eo ? 1 : 0;
+    RVec pi = mom_flat(mom);
+    int it = 0;
+    check(smg_hmc_evolve(D.handle(), &p, pi.data(), raw(phi), &it));
+    Vec u(2 * (size_t)lat.V);
+    check(smg_dirac_get_gauge(D.handle(), raw(u)));
+    gauge.U[0].assign(u.begin(), u.begin() + lat.V);
+    gauge.U[1].assign(u.begin() + lat.V, u.end());
+    mom_unflat(pi, mom);
+    return it;
 }
 }  // namespace detail
 
@@ -789,49 +810,39 @@ inline int plain_leapfrog_evolve(GaugeField& gauge, MomentumField& mom, const La
                                  const std::function<Vec(const Vec&)>* precond = nullptr,
                                  TrackingState* tracking = nullptr) {
     detail::no_extras(precond, tracking, "plain_leapfrog_evolve");
-    DiracOp D(lat, gauge, mass, wilson_r, params.c_sw, params.mu_t);
-    const smg_hmc_params p = detail::hmc_c(params);
-    RVec pi = detail::mom_flat(mom);
-    int it = 0;
-    detail::check(smg_hmc_evolve(D.handle(), &p, pi.data(), detail::raw(phi), &it));
-    Vec u(2 * (size_t)lat.V);
-    detail::check(smg_dirac_get_gauge(D.handle(), detail::raw(u)));
-    gauge.U[0].assign(u.begin(), u.begin() + lat.V);
-    gauge.U[1].assign(u.begin() + lat.V, u.end());
-    detail::mom_unflat(pi, mom);
-    return it;
+    return detail::md_evolve(gauge, mom, lat, mass, wilson_r, phi, params, false);
 }
 
-// hmc_trajectory, full-lattice path, src/hmc_trajectory.cpp:38-45, 198-236: momenta,
-// then eta (phi = D^dag eta), then (only if needed) the Metropolis uniform are drawn
-// from rng in the reference's order
+// hmc_trajectory, src/hmc_trajectory.cpp:38-236: momenta, then the pseudofermion's
+// Gaussians (phi = D^dag eta, or phi_o = M^dag eta_o with use_eo), then (only if
+// needed) the Metropolis uniform are drawn from rng in the reference's order
 inline HMCResult hmc_trajectory(GaugeField& gauge, const Lattice& lat, double mass, double wilson_r,
                                 const HMCParams& params, std::mt19937& rng,
                                 const std::function<Vec(const Vec&)>* precond = nullptr,
                                 TrackingState* tracking = nullptr) {
     detail::no_extras(precond, tracking, "hmc_trajectory");
-    if (params.use_eo) throw std::invalid_argument("hmc_trajectory: the even-odd trajectory is not supported");
     const std::array<Vec, 2> old = gauge.U;
     MomentumField mom(lat);
     mom.randomise(rng);
     Vec phi;
-    double h0[3], h1[3];
+    double h0[4], h1[4];
     int cg = 0;
     {
         DiracOp D0(lat, gauge, mass, wilson_r, params.c_sw, params.mu_t);
-        generate_pseudofermion(D0, rng, phi);
-        cg += detail::hamiltonian(D0, params.beta, mom, phi, params.cg_maxiter, params.cg_tol, h0);
+        if (params.use_eo) phi = EvenOddDiracOp(D0).generate_pseudofermion_eo(rng);
+        else generate_pseudofermion(D0, rng, phi);
+        cg += detail::hamiltonian(D0, params.use_eo, params.beta, mom, phi, params.cg_maxiter, params.cg_tol, h0);
     }
-    cg += plain_leapfrog_evolve(gauge, mom, lat, mass, wilson_r, phi, params);
+    cg += detail::md_evolve(gauge, mom, lat, mass, wilson_r, phi, params, params.use_eo);
     {
         DiracOp D1(lat, gauge, mass, wilson_r, params.c_sw, params.mu_t);
-        cg += detail::hamiltonian(D1, params.beta, mom, phi, params.cg_maxiter, params.cg_tol, h1);
+        cg += detail::hamiltonian(D1, params.use_eo, params.beta, mom, phi, params.cg_maxiter, params.cg_tol, h1);
     }
-    const double dH = (h1[0] + h1[1] + h1[2]) - (h0[0] + h0[1] + h0[2]);
+    const double dH = (h1[0] + h1[1] + h1[2] + h1[3]) - (h0[0] + h0[1] + h0[2] + h0[3]);
     std::uniform_real_distribution<double> uniform(0.0, 1.0);
     const bool accept = params.force_accept || (dH < 0) || (uniform(rng) < std::exp(-dH));
     if (!accept) gauge.U = old;
-    return {accept, dH, h1[0] - h0[0], h1[1] - h0[1], h1[2] - h0[2], 0.0, cg};
+    return {accept, dH, h1[0] - h0[0], h1[1] - h0[1], h1[2] - h0[2], h1[3] - h0[3], cg};
 }
 
 // reversibility_test_plain, src/hmc_trajectory.cpp:416-509 (the gauge field is restored)
@@ -850,10 +861,10 @@ inline ReversibilityResult reversibility_test_plain(GaugeField& gauge, const Lat
     const std::array<RVec, 2> pi_save = mom.pi;
     auto H = [&](double* h) {
         DiracOp D(lat, gauge, mass, wilson_r, params.c_sw, params.mu_t);
-        detail::hamiltonian(D, params.beta, mom, phi, params.cg_maxiter, params.cg_tol, h);
+        detail::hamiltonian(D, false, params.beta, mom, phi, params.cg_maxiter, params.cg_tol, h);
         return h[0] + h[1] + h[2];
     };
-    double h[3];
+    double h[4];
     const double H0 = H(h);
     const int cg_fwd = plain_leapfrog_evolve(gauge, mom, lat, mass, wilson_r, phi, params);
     const double H1 = H(h);

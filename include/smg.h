@@ -289,26 +289,32 @@ typedef struct {
     int cg_maxiter;    /* CG on D^dag D for every force and action, from x0 = 0 */
     double cg_tol;
     int force_accept;  /* always accept (thermalisation) */
-} smg_hmc_params;      /* HMCParams, src/hmc.h:107-119 (full lattice, no preconditioner) */
+    int use_eo;        /* 1: even-odd trajectory (Schur operator, log det D_ee; leapfrog), even L */
+} smg_hmc_params;      /* HMCParams, src/hmc.h:107-119 (no preconditioner) */
 typedef struct {
     int accepted;
     double dH, dKE, dSG, dSF;
     int total_cg_iters;
-} smg_hmc_result;      /* HMCResult, src/hmc.h:120-128 (dLD = 0 on the full lattice) */
+    double dLD;        /* log-det change (even-odd with clover, else 0) */
+} smg_hmc_result;      /* HMCResult, src/hmc.h:120-128 */
 /* the operator's current gauge field (host, 2V complex) */
 int smg_dirac_get_gauge(smg_dirac* D, double* U);
 /* plain_leapfrog_evolve, src/hmc_trajectory.cpp:243-409 (no preconditioner, no
  * tracking): the operator's gauge field and the momenta pi (host, in/out) evolved
- * over tau with the pseudofermion phi (host, 2V complex) fixed */
+ * over tau with the pseudofermion phi (host, 2V complex) fixed.  use_eo: the
+ * even-odd trajectory's leapfrog (src/hmc_trajectory.cpp:135-170), phi = phi_o
+ * (V complex, odd sites in eo_index order) */
 int smg_hmc_evolve(smg_dirac* D, const smg_hmc_params* p, double* pi, const double* phi, int* cg_iters);
-/* the Hamiltonian of src/hmc_trajectory.cpp:87-103 on the current gauge field:
- * out3 = {KE = 0.5 sum pi^2, S_g = -beta sum Re P, S_f = Re phi^dag (D^dag D)^-1 phi} */
-int smg_hmc_hamiltonian(smg_dirac* D, double beta, const double* pi, const double* phi, int cg_maxiter,
-                        double cg_tol, double* out3, int* cg_iters);
-/* hmc_trajectory, full-lattice path, src/hmc_trajectory.cpp:38-45, 198-236: momenta
- * pi0 and the Gaussian eta of phi = D^dag eta come from the caller (drawn from its
- * generator in that order); u is the uniform of the Metropolis test, consulted only
- * when dH >= 0 and force_accept == 0.  A rejected trajectory restores the gauge field. */
+/* the Hamiltonian of src/hmc_trajectory.cpp:87-103 (use_eo: 115-127) on the current
+ * gauge field: out4 = {KE = 0.5 sum pi^2, S_g = -beta sum Re P, S_f = Re phi^dag
+ * A^-1 phi with A = D^dag D (M^dag M), LD = -2 log det D_ee (even-odd with clover, else 0)} */
+int smg_hmc_hamiltonian(smg_dirac* D, int use_eo, double beta, const double* pi, const double* phi, int cg_maxiter,
+                        double cg_tol, double* out4, int* cg_iters);
+/* hmc_trajectory, src/hmc_trajectory.cpp:38-236: momenta pi0 and the Gaussian eta
+ * of phi = D^dag eta (use_eo: phi_o = M^dag eta_o, V complex) come from the caller
+ * (drawn from its generator in that order); u is the uniform of the Metropolis test,
+ * consulted only when dH >= 0 and force_accept == 0.  A rejected trajectory restores
+ * the gauge field. */
 int smg_hmc_trajectory(smg_dirac* D, const smg_hmc_params* p, const double* pi0, const double* eta, double u,
                        smg_hmc_result* res);
 

@@ -59,6 +59,10 @@ def lib():
         L.orc_hmc_evolve.argtypes = [_I, _P, _D, _D, _D, _D, _D, _D, _I, _I, _I, _D, _P, _P]
         L.orc_hmc_hamiltonian.restype = _I
         L.orc_hmc_hamiltonian.argtypes = [_I, _P, _D, _D, _D, _D, _D, _I, _D, _P, _P, _P]
+        L.orc_hmc_evolve_eo.restype = _I
+        L.orc_hmc_evolve_eo.argtypes = [_I, _P, _D, _D, _D, _D, _D, _D, _I, _I, _D, _P, _P]
+        L.orc_hmc_hamiltonian_eo.restype = _I
+        L.orc_hmc_hamiltonian_eo.argtypes = [_I, _P, _D, _D, _D, _D, _D, _I, _D, _P, _P, _P]
         L.orc_gauge_force.argtypes = [_I, _P, _D, _P]
         L.orc_eo_maps.argtypes = [_I, _P, _P, _P]
         L.orc_dot2.argtypes = [_I, _P, _P, _P]
@@ -485,4 +489,21 @@ def hmc_hamiltonian(L, U, mass, r, csw, mu_t, beta, cg_maxit, cg_tol, pi, phi):
     pi = np.ascontiguousarray(pi, dtype=np.float64)
     out = np.zeros(3)
     lib().orc_hmc_hamiltonian(L, _ptr(U), mass, r, csw, mu_t, beta, cg_maxit, cg_tol, _ptr(pi), _ptr(phi), _ptr(out))
+    return out
+
+
+def hmc_evolve_eo(L, U, mass, r, csw, mu_t, beta, tau, n_steps, cg_maxit, cg_tol, pi, phi_o):
+    """The even-odd trajectory's leapfrog: returns (U, pi, cg iterations)."""
+    U = _c(U).copy()
+    pi = np.ascontiguousarray(pi, dtype=np.float64).copy()
+    it = lib().orc_hmc_evolve_eo(L, _ptr(U), mass, r, csw, mu_t, beta, tau, n_steps, cg_maxit, cg_tol, _ptr(pi),
+                                 _ptr(_c(phi_o)))
+    return U, pi, it
+
+
+def hmc_hamiltonian_eo(L, U, mass, r, csw, mu_t, beta, cg_maxit, cg_tol, pi, phi_o):
+    """(KE, S_g, S_f, LD) of the even-odd trajectory."""
+    out = np.zeros(4)
+    lib().orc_hmc_hamiltonian_eo(L, _ptr(_c(U)), mass, r, csw, mu_t, beta, cg_maxit, cg_tol,
+                                 _ptr(np.ascontiguousarray(pi, dtype=np.float64)), _ptr(_c(phi_o)), _ptr(out))
     return out

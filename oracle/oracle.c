@@ -1446,3 +1446,64 @@ int orc_hmc_hamiltonian(int L, const ocx* U, double mass, double r, double csw, 
     orc_dirac_free(D);
     return it;
 }
+
+/* ---------------- even-odd HMC, src/hmc_trajectory.cpp:53-135 ----------------
+ * S_eo = phi_o^dag (M^dag M)^-1 phi_o - 2 log det D_ee on the odd-site pseudofermion
+ * phi_o (V complex = 2 V/2); CG from x0 = 0 on M^dag M; always leapfrog. */
+static int md_force_eo(int L, const ocx* U, double mass, double r, double csw, double mu_t, double beta,
+                       int cg_maxit, double cg_tol, const ocx* phi_o, double* F) {
+    int V = L * L;
+    orc_gauge_force(L, U, beta, F);
+    orc_dirac* D = orc_dirac_new(L, U, mass, r, csw, mu_t);
+    orc_op* A = orc_op_dirac(D, 100);
+    ocx* x = xcalloc((size_t)V, sizeof(ocx));
+    ocx* y = xcalloc((size_t)V, sizeof(ocx));
+    double res;
+    int it = orc_cg(A, phi_o, NULL, NULL, cg_maxit, cg_tol, x, &res);
+    orc_eo_apply(D, ORC_EO_SCHUR, x, y);
+    double* ff = xcalloc(2 * (size_t)V, sizeof(double));
+    orc_eo_fermion_force(D, x, y, ff);
+    for (int i = 0; i < 2 * V; i++) F[i] += ff[i];
+    free(ff); free(x); free(y);
+    orc_op_free(A);
+    orc_dirac_free(D);
+    return it;
+}
+/* the leapfrog of the even-odd trajectory, src/hmc_trajectory.cpp:135-170 */
+int orc_hmc_evolve_eo(int L, ocx* U, double mass, double r, double csw, double mu_t, double beta, double tau,
+                      int n_steps, int cg_maxit, double cg_tol, double* pi, const ocx* phi_o) {
+    int n = 2 * L * L, cg = 0;
+    double dt = tau / n_steps;
+    double* F = xcalloc(n, sizeof(double));
+    cg += md_force_eo(L, U, mass, r, csw, mu_t, beta, cg_maxit, cg_tol, phi_o, F);
+    md_update_mom(n, pi, F, 0.5 * dt);
+    for (int step = 0; step < n_steps - 1; step++) {
+        md_update_gauge(n, U, pi, dt);
+        cg += md_force_eo(L, U, mass, r, csw, mu_t, beta, cg_maxit, cg_tol, phi_o, F);
+        md_update_mom(n, pi, F, dt);
+    }
+    md_update_gauge(n, U, pi, dt);
+    cg += md_force_eo(L, U, mass, r, csw, mu_t, beta, cg_maxit, cg_tol, phi_o, F);
+    md_update_mom(n, pi, F, 0.5 * dt);
+    free(F);
+    return cg;
+}
+/* out = {KE, S_g, S_f = Re phi_o^dag (M^dag M)^-1 phi_o, LD = -2 log det D_ee (0 if c_sw = 0)},
+ * src/hmc_trajectory.cpp:115-127 */
+int orc_hmc_hamiltonian_eo(int L, const ocx* U, double mass, double r, double csw, double mu_t, double beta,
+                           int cg_maxit, double cg_tol, const double* pi, const ocx* phi_o, double* out) {
+    int V = L * L;
+    out[0] = orc_kinetic_energy(2 * V, pi);
+    out[1] = orc_gauge_action(L, U, beta);
+    orc_dirac* D = orc_dirac_new(L, U, mass, r, csw, mu_t);
+    orc_op* A = orc_op_dirac(D, 100);
+    ocx* x = xcalloc((size_t)V, sizeof(ocx));
+    double res;
+    int it = orc_cg(A, phi_o, NULL, NULL, cg_maxit, cg_tol, x, &res);
+    out[2] = creal(orc_dot(V, phi_o, x));
+    out[3] = csw != 0.0 ? -2.0 * orc_eo_logdet(D) : 0.0;
+    free(x);
+    orc_op_free(A);
+    orc_dirac_free(D);
+    return it;
+}

@@ -151,6 +151,10 @@ int orc_hmc_evolve(int L, ocx* U, double mass, double r, double csw, double mu_t
                    int n_steps, int omelyan, int cg_maxit, double cg_tol, double* pi, const ocx* phi);
 int orc_hmc_hamiltonian(int L, const ocx* U, double mass, double r, double csw, double mu_t, double beta,
                         int cg_maxit, double cg_tol, const double* pi, const ocx* phi, double* out);
+int orc_hmc_evolve_eo(int L, ocx* U, double mass, double r, double csw, double mu_t, double beta, double tau,
+                      int n_steps, int cg_maxit, double cg_tol, double* pi, const ocx* phi_o);
+int orc_hmc_hamiltonian_eo(int L, const ocx* U, double mass, double r, double csw, double mu_t, double beta,
+                           int cg_maxit, double cg_tol, const double* pi, const ocx* phi_o, double* out);
 
 #ifdef __cplusplus
 }

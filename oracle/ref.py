@@ -67,6 +67,8 @@ def lib():
                            ("ref_gauge_load", [_I, _P, _P, ctypes.c_char_p])):
             getattr(L, name).restype = _I
             getattr(L, name).argtypes = args
+        L.ref_hmc_draws.argtypes = [_I, ctypes.c_uint, _P, _I, _P]
+        L.ref_hmc_trajectory.argtypes = [_I, _P, _D, _D, _D, _D, _D, _I, _I, _I, _I, _D, ctypes.c_uint, _P]
         L.ref_leapfrog_evolve.restype = _I
         L.ref_leapfrog_evolve.argtypes = [_I, _P, _D, _D, _D, _D, _D, _I, _I, _I, _D, _P, _P]
         _lib = L
@@ -234,3 +236,20 @@ def gauge_load(L, path):
     h = np.zeros(3)
     rc = lib().ref_gauge_load(L, _ptr(U), _ptr(h), path.encode())
     return (U, tuple(h)) if rc == 0 else None
+
+
+def hmc_draws(L, seed, n_eta):
+    """The momenta and pseudofermion Gaussians hmc_trajectory draws from mt19937(seed)."""
+    pi = np.zeros(2 * L * L)
+    eta = np.zeros(n_eta, np.complex128)
+    lib().ref_hmc_draws(L, seed, _ptr(pi), n_eta, _ptr(eta))
+    return pi, eta
+
+
+def hmc_trajectory(L, U, mass, r, csw, beta, tau, n_steps, omelyan, use_eo, cg_maxit, cg_tol, seed):
+    """hmc_trajectory with force_accept: (final U, [dH, dKE, dSG, dSF, dLD, cg])."""
+    U = _c(U).copy()
+    out = np.zeros(6)
+    lib().ref_hmc_trajectory(L, _ptr(U), mass, r, csw, beta, tau, n_steps, int(omelyan), int(use_eo), cg_maxit,
+                             cg_tol, seed, _ptr(out))
+    return U, out

@@ -410,3 +410,45 @@ API int ref_gauge_load(int L, double* U, double* hdr3, const char* path) {
     hdr3[2] = h.avg_plaq;
     return 0;
 }
+
+// the draws of hmc_trajectory (src/hmc_trajectory.cpp:52-53, 62): momenta (pi[0] then
+// pi[1]), then the n_eta complex Gaussians of the pseudofermion, from mt19937(seed)
+API void ref_hmc_draws(int L, unsigned seed, double* pi, int n_eta, double* eta) {
+    Lattice lat(L);
+    std::mt19937 rng(seed);
+    MomentumField mom(lat);
+    mom.randomise(rng);
+    std::memcpy(pi, mom.pi[0].data(), lat.V * sizeof(double));
+    std::memcpy(pi + lat.V, mom.pi[1].data(), lat.V * sizeof(double));
+    from_vec(random_vec(n_eta, rng), eta);
+}
+// hmc_trajectory (src/hmc_trajectory.cpp:38-236) with force_accept: the final links and
+// {dH, dKE, dSG, dSF, dLD, cg iterations}
+API void ref_hmc_trajectory(int L, double* U, double mass, double r, double csw, double beta, double tau,
+                            int n_steps, int omelyan, int use_eo, int cg_maxit, double cg_tol, unsigned seed,
+                            double* out6) {
+    Lattice lat(L);
+    GaugeField g(lat);
+    load_gauge_arr(g, U);
+    HMCParams p{};
+    p.beta = beta;
+    p.tau = tau;
+    p.n_steps = n_steps;
+    p.cg_maxiter = cg_maxit;
+    p.cg_tol = cg_tol;
+    p.use_mg = false;
+    p.c_sw = csw;
+    p.use_eo = use_eo != 0;
+    p.omelyan = omelyan != 0;
+    p.force_accept = true;
+    std::mt19937 rng(seed);
+    const HMCResult res = hmc_trajectory(g, lat, mass, r, p, rng, nullptr, nullptr);
+    from_vec(g.U[0], U);
+    from_vec(g.U[1], U + 2 * lat.V);
+    out6[0] = res.dH;
+    out6[1] = res.dKE;
+    out6[2] = res.dSG;
+    out6[3] = res.dSF;
+    out6[4] = res.dLD;
+    out6[5] = res.total_cg_iters;
+}

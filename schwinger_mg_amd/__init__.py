@@ -52,12 +52,13 @@ class GaugeHeader(ctypes.Structure):
 class HMCParams(ctypes.Structure):
     """HMCParams (src/hmc.h:107-119), full lattice without preconditioner."""
     _fields_ = [("beta", _D), ("tau", _D), ("n_steps", _I), ("omelyan", _I), ("cg_maxiter", _I), ("cg_tol", _D),
-                ("force_accept", _I)]
+                ("force_accept", _I), ("use_eo", _I)]
 
 
 class HMCResult(ctypes.Structure):
     """HMCResult (src/hmc.h:120-128)."""
-    _fields_ = [("accepted", _I), ("dH", _D), ("dKE", _D), ("dSG", _D), ("dSF", _D), ("total_cg_iters", _I)]
+    _fields_ = [("accepted", _I), ("dH", _D), ("dKE", _D), ("dSG", _D), ("dSF", _D), ("total_cg_iters", _I),
+                ("dLD", _D)]
 
 
 # (name, restype, argtypes) of every entry point declared in include/smg.h
@@ -104,7 +105,7 @@ SIGNATURES = [
     ("smg_hopping_force", _I, [_P, _P, _P, _P]), ("smg_logdet_force", _I, [_P, _P]),
     ("smg_eo_fermion_force", _I, [_P, _P, _P, _P]), ("smg_fermion_action", _I, [_P, _P, _I, _D, _P, _P, _P]),
     ("smg_dirac_get_gauge", _I, [_P, _P]), ("smg_hmc_evolve", _I, [_P, _P, _P, _P, _P]),
-    ("smg_hmc_hamiltonian", _I, [_P, _D, _P, _P, _I, _D, _P, _P]),
+    ("smg_hmc_hamiltonian", _I, [_P, _I, _D, _P, _P, _I, _D, _P, _P]),
     ("smg_hmc_trajectory", _I, [_P, _P, _P, _P, _D, _P]),
     ("smg_gauge_write_raw", _I, [ctypes.c_char_p, _I, _P]), ("smg_gauge_read_raw", _I, [ctypes.c_char_p, _I, _P]),
     ("smg_gauge_write", _I, [ctypes.c_char_p, _I, _P, _D, _D]),
@@ -664,8 +665,8 @@ def fermion_action(D, phi, max_iter, tol):
     return S.value, X
 
 
-def hmc_params(beta, tau, n_steps, omelyan=False, cg_maxiter=1000, cg_tol=1e-12, force_accept=False):
-    return HMCParams(beta, tau, n_steps, int(omelyan), cg_maxiter, cg_tol, int(force_accept))
+def hmc_params(beta, tau, n_steps, omelyan=False, cg_maxiter=1000, cg_tol=1e-12, force_accept=False, use_eo=False):
+    return HMCParams(beta, tau, n_steps, int(omelyan), cg_maxiter, cg_tol, int(force_accept), int(use_eo))
 
 
 def gauge_of(D):
@@ -684,11 +685,11 @@ def hmc_evolve(D, params, pi, phi):
     return pi, it.value
 
 
-def hmc_hamiltonian(D, beta, pi, phi, cg_maxiter=1000, cg_tol=1e-12):
-    """(KE, S_g, S_f) on D's current gauge field."""
-    out = np.zeros(3)
-    _chk(lib().smg_hmc_hamiltonian(D.h, beta, _p(np.ascontiguousarray(pi, dtype=np.float64)), _p(_c(phi)),
-                                   cg_maxiter, cg_tol, _p(out), None))
+def hmc_hamiltonian(D, beta, pi, phi, cg_maxiter=1000, cg_tol=1e-12, use_eo=False):
+    """(KE, S_g, S_f, LD) on D's current gauge field (LD: even-odd with clover)."""
+    out = np.zeros(4)
+    _chk(lib().smg_hmc_hamiltonian(D.h, int(use_eo), beta, _p(np.ascontiguousarray(pi, dtype=np.float64)),
+                                   _p(_c(phi)), cg_maxiter, cg_tol, _p(out), None))
     return out
 
 

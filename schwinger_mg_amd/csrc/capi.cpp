@@ -957,6 +957,20 @@ static void schur_deriv_plus(const Dirac& d, const c64* chi_o, const c64* psi_o,
     launch_eo_scatter<double>(d.L, psif.p(), psi_o, 1, st);
     launch_hopping_force(v, chif.p(), psif.p(), -0.5, 1, F, st);
 }
+// eo_fermion_force, src/hmc_forces.cpp:830-867, on device arrays: F = 2 *
+// schur_deriv_plus(y_o, x_o) + the log-det force (a real array of 2V doubles is
+// scaled as V complex numbers)
+static void eo_force_dev(const Dirac& d, const c64* y_o, const c64* x_o, double* F, hipStream_t st) {
+    const size_t V = (size_t)d.V;
+    SMG_HIP(hipMemsetAsync(F, 0, 2 * V * sizeof(double), st));
+    schur_deriv_plus(d, y_o, x_o, F, st);
+    launch_axpby<double>(reinterpret_cast<c64*>(F), reinterpret_cast<const c64*>(F), c64{0, 0}, c64{2.0, 0}, V, st);
+    if (d.csw != 0.0) {
+        DevMem fl(2 * V * sizeof(double));
+        logdet_force_dev(d, fl.as<double>(), 0, st);
+        launch_axpby<double>(reinterpret_cast<c64*>(F), fl.as<c64>(), c64{1, 0}, c64{1, 0}, V, st);
+    }
+}
 int smg_eo_fermion_force(smg_dirac* D, const double* x_o, const double* y_o, double* F) {
     return guard([&] {
         SMG_REQUIRE(D, "null handle");
@@ -964,16 +978,8 @@ int smg_eo_fermion_force(smg_dirac* D, const double* x_o, const double* y_o, dou
         SMG_REQUIRE(d.L % 2 == 0, "even-odd operators need an even L");
         const size_t V = (size_t)d.V;
         Field<double> xo = up(x_o, V), yo = up(y_o, V);
-        DevMem f(2 * V * sizeof(double)), fl(2 * V * sizeof(double));
-        SMG_HIP(hipMemsetAsync(f.p, 0, 2 * V * sizeof(double), S()));
-        schur_deriv_plus(d, yo.p(), xo.p(), f.as<double>(), S());
-        // F = 2 * schur_deriv_plus + logdet force (src/hmc_forces.cpp:852-866); a real
-        // array of 2V doubles is scaled as V complex numbers
-        launch_axpby<double>(f.as<c64>(), f.as<c64>(), c64{0, 0}, c64{2.0, 0}, V, S());
-        if (d.csw != 0.0) {
-            logdet_force_dev(d, fl.as<double>(), 0, S());
-            launch_axpby<double>(f.as<c64>(), fl.as<c64>(), c64{1, 0}, c64{1, 0}, V, S());
-        }
+        DevMem f(2 * V * sizeof(double));
+        eo_force_dev(d, yo.p(), xo.p(), f.as<double>(), S());
         down(F, f.p, 2 * V * sizeof(double));
     });
 }
@@ -993,47 +999,67 @@ int smg_fermion_action(smg_dirac* D, const double* phi, int maxit, double tol, d
 // ------------------------------------------------------------------ HMC molecular dynamics
 namespace {
 // Device state of one trajectory: momenta, the two forces, the pseudofermion and
-// the CG solution, all resident for the whole evolution.
+// the CG solution, all resident for the whole evolution.  Full lattice: phi and X
+// have 2V complex entries, the action is Re phi^dag (D^dag D)^-1 phi.  Even-odd
+// (src/hmc_trajectory.cpp:53-135): phi_o and x_o have V complex entries (2 x V/2),
+// the action is Re phi_o^dag (M^dag M)^-1 phi_o - 2 log det D_ee.
 struct MDState {
     smg_dirac* D;
-    size_t V;
+    bool eo;
+    size_t V, nphi;
     DevMem pi, gf, ff;
-    Field<double> phi, X;
-    MDState(smg_dirac* d, const double* pi_h, const double* phi_h)
-        : D(d), V((size_t)d->D->V), pi(2 * V * sizeof(double)), gf(2 * V * sizeof(double)),
-          ff(2 * V * sizeof(double)), phi(2 * V), X(2 * V) {
+    Field<double> phi, X, Y;
+    MDState(smg_dirac* d, bool eo_, const double* pi_h, const double* phi_h)
+        : D(d), eo(eo_), V((size_t)d->D->V), nphi(eo_ ? (size_t)d->D->V : 2 * (size_t)d->D->V),
+          pi(2 * V * sizeof(double)), gf(2 * V * sizeof(double)), ff(2 * V * sizeof(double)), phi(nphi), X(nphi),
+          Y(nphi) {
+        SMG_REQUIRE(!eo || (d->D->L % 2 == 0 && d->schur64), "the even-odd trajectory needs an even L");
         SMG_HIP(hipMemcpyAsync(pi.p, pi_h, 2 * V * sizeof(double), hipMemcpyHostToDevice, S()));
-        if (phi_h) SMG_HIP(hipMemcpyAsync(phi.p(), phi_h, 2 * V * sizeof(c64), hipMemcpyHostToDevice, S()));
+        if (phi_h) SMG_HIP(hipMemcpyAsync(phi.p(), phi_h, nphi * sizeof(c64), hipMemcpyHostToDevice, S()));
         SMG_HIP(hipStreamSynchronize(S()));
     }
-    // gauge force and fermion force of X = (D^dag D)^-1 phi (CG from x0 = 0), both
-    // -dS/dtheta (src/hmc_trajectory.cpp:254-343 without tracking)
+    LinOp<double>& op() { return eo ? static_cast<LinOp<double>&>(*D->schur64) : *D->op64; }
+    // gauge force and fermion force, both -dS/dtheta: full lattice X = (D^dag D)^-1 phi
+    // and fermion_force (src/hmc_trajectory.cpp:254-343); even-odd x_o = (M^dag M)^-1
+    // phi_o, y_o = M x_o and eo_fermion_force (src/hmc_trajectory.cpp:60-110)
     int force(double beta, int maxit, double tol) {
         const Dirac& d = *D->D;
         launch_gauge_force(d.L, d.U64.p(), beta, gf.as<double>(), S());
-        const SolveResult r = cg_solve(*D->op64, phi.p(), X.p(), nullptr, nullptr, maxit, tol, S());
-        fermion_force_dev(D, X.p(), ff.as<double>(), S());
+        const SolveResult r = cg_solve(op(), phi.p(), X.p(), nullptr, nullptr, maxit, tol, S());
+        if (!eo) {
+            fermion_force_dev(D, X.p(), ff.as<double>(), S());
+        } else {
+            D->schur64->apply_m(Y.p(), X.p(), 0, S());
+            eo_force_dev(d, Y.p(), X.p(), ff.as<double>(), S());
+        }
         return r.iterations;
     }
     void mom(double scale) { launch_mom_update(pi.as<double>(), gf.as<double>(), ff.as<double>(), scale, 2 * V, S()); }
     void gauge(double scale) { D->D->update_links(pi.as<double>(), scale, S()); }
-    // {KE, S_g, S_f} on the current gauge field (src/hmc_trajectory.cpp:87-103)
+    // {KE, S_g, S_f, LD} on the current gauge field (src/hmc_trajectory.cpp:87-103, 115-127)
     int hamiltonian(double beta, int maxit, double tol, double* out) {
         const Dirac& d = *D->D;
         out[0] = 0.5 * dev_dot<double>(pi.as<c64>(), pi.as<c64>(), V, S()).re;   // 2V reals as V complex
         Scalar1 sg;
         launch_plaq_sum(d.L, d.U64.p(), sg.fin(), S());
         out[1] = -beta * sg.read();
-        const SolveResult r = cg_solve(*D->op64, phi.p(), X.p(), nullptr, nullptr, maxit, tol, S());
-        out[2] = dev_dot<double>(phi.p(), X.p(), 2 * V, S()).re;
+        const SolveResult r = cg_solve(op(), phi.p(), X.p(), nullptr, nullptr, maxit, tol, S());
+        out[2] = dev_dot<double>(phi.p(), X.p(), nphi, S()).re;
+        out[3] = 0.0;
+        if (eo && d.csw != 0.0) {
+            Scalar1 ld;
+            launch_logdet(d.v64(), d.mu_t, ld.fin(), S());
+            out[3] = -2.0 * ld.read();
+        }
         return r.iterations;
     }
-    // plain_leapfrog_evolve, src/hmc_trajectory.cpp:243-409
+    // plain_leapfrog_evolve, src/hmc_trajectory.cpp:243-409 (the even-odd trajectory
+    // always takes the leapfrog, src/hmc_trajectory.cpp:135-170)
     int evolve(const smg_hmc_params& p) {
         SMG_REQUIRE(p.n_steps >= 1, "n_steps must be >= 1");
         const double dt = p.tau / p.n_steps;
         int cg = 0;
-        if (p.omelyan) {   // P(l e) [Q(e/2) P((1-2l) e) Q(e/2) P(2l e)]^(N-1) ... P(l e), l = 0.1932
+        if (p.omelyan && !eo) {   // P(l e) [Q(e/2) P((1-2l) e) Q(e/2) P(2l e)]^(N-1) ... P(l e), l = 0.1932
             const double lambda = 0.1932;
             cg += force(p.beta, p.cg_maxiter, p.cg_tol);
             mom(lambda * dt);
@@ -1078,18 +1104,18 @@ int smg_hmc_evolve(smg_dirac* D, const smg_hmc_params* p, double* pi, const doub
     return guard([&] {
         SMG_REQUIRE(D && pi && phi, "null argument");
         check_hmc(p);
-        MDState md(D, pi, phi);
+        MDState md(D, p->use_eo != 0, pi, phi);
         const int it = md.evolve(*p);
         md.pi_out(pi);
         if (cg_iters) *cg_iters = it;
     });
 }
-int smg_hmc_hamiltonian(smg_dirac* D, double beta, const double* pi, const double* phi, int cg_maxiter,
-                        double cg_tol, double* out3, int* cg_iters) {
+int smg_hmc_hamiltonian(smg_dirac* D, int use_eo, double beta, const double* pi, const double* phi, int cg_maxiter,
+                        double cg_tol, double* out4, int* cg_iters) {
     return guard([&] {
-        SMG_REQUIRE(D && pi && phi && out3, "null argument");
-        MDState md(D, pi, phi);
-        const int it = md.hamiltonian(beta, cg_maxiter, cg_tol, out3);
+        SMG_REQUIRE(D && pi && phi && out4, "null argument");
+        MDState md(D, use_eo != 0, pi, phi);
+        const int it = md.hamiltonian(beta, cg_maxiter, cg_tol, out4);
         if (cg_iters) *cg_iters = it;
     });
 }
@@ -1102,16 +1128,18 @@ int smg_hmc_trajectory(smg_dirac* D, const smg_hmc_params* p, const double* pi0,
         const size_t V = (size_t)d.V;
         Field<double> saved(2 * V);   // gauge_old, src/hmc_trajectory.cpp:48-50
         launch_copy<double>(saved.p(), d.U64.p(), 2 * V, S());
-        MDState md(D, pi0, nullptr);
-        // phi = D^dag eta (generate_pseudofermion, src/hmc_trajectory.cpp:32-36)
-        Field<double> e = up(eta, 2 * V);
-        launch_dslash<double>(d.v64(), md.phi.p(), e.p(), -d.mu_t, 1, 1, EPI_NONE, nullptr, nullptr, Fin(), 1, 2 * V,
-                              S());
-        double h0[3], h1[3];
+        MDState md(D, p->use_eo != 0, pi0, nullptr);
+        Field<double> e = up(eta, md.nphi);
+        if (!md.eo)   // phi = D^dag eta (generate_pseudofermion, src/hmc_trajectory.cpp:32-36)
+            launch_dslash<double>(d.v64(), md.phi.p(), e.p(), -d.mu_t, 1, 1, EPI_NONE, nullptr, nullptr, Fin(), 1,
+                                  2 * V, S());
+        else          // phi_o = M^dag eta_o (generate_pseudofermion_eo, src/dirac.h:504-510)
+            D->schur64->apply_m(md.phi.p(), e.p(), 1, S());
+        double h0[4], h1[4];
         int cg = md.hamiltonian(p->beta, p->cg_maxiter, p->cg_tol, h0);
         cg += md.evolve(*p);
         cg += md.hamiltonian(p->beta, p->cg_maxiter, p->cg_tol, h1);
-        const double dH = (h1[0] + h1[1] + h1[2]) - (h0[0] + h0[1] + h0[2]);
+        const double dH = (h1[0] + h1[1] + h1[2] + h1[3]) - (h0[0] + h0[1] + h0[2] + h0[3]);
         const bool accept = p->force_accept || dH < 0 || u < std::exp(-dH);
         if (!accept) {
             launch_copy<double>(d.U64.p(), saved.p(), 2 * V, S());
@@ -1123,6 +1151,7 @@ int smg_hmc_trajectory(smg_dirac* D, const smg_hmc_params* p, const double* pi0,
         res->dKE = h1[0] - h0[0];
         res->dSG = h1[1] - h0[1];
         res->dSF = h1[2] - h0[2];
+        res->dLD = h1[3] - h0[3];
         res->total_cg_iters = cg;
     });
 }

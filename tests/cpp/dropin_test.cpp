@@ -48,6 +48,10 @@ int orc_hmc_evolve(int L, cx* U, double mass, double r, double csw, double mu_t,
                    int n_steps, int omelyan, int cg_maxit, double cg_tol, double* pi, const cx* phi);
 int orc_hmc_hamiltonian(int L, const cx* U, double mass, double r, double csw, double mu_t, double beta,
                         int cg_maxit, double cg_tol, const double* pi, const cx* phi, double* out);
+int orc_hmc_evolve_eo(int L, cx* U, double mass, double r, double csw, double mu_t, double beta, double tau,
+                      int n_steps, int cg_maxit, double cg_tol, double* pi, const cx* phi_o);
+int orc_hmc_hamiltonian_eo(int L, const cx* U, double mass, double r, double csw, double mu_t, double beta,
+                           int cg_maxit, double cg_tol, const double* pi, const cx* phi_o, double* out);
 }
 
 static int failures = 0;
@@ -221,6 +225,33 @@ int main() {
         orc_hmc_hamiltonian(Lh, Uo.data(), 0.1, 1.0, 1.0, 0.0, 2.0, 2000, 1e-12, pio.data(), phi.data(), h1);
         report("hmc_trajectory dH", std::abs(hr.dH - ((h1[0] + h1[1] + h1[2]) - (h0[0] + h0[1] + h0[2]))), 1e-8);
         report("hmc_trajectory links", relerr(gt.links(), Uo), 1e-11);
+        // the even-odd trajectory: momenta, then eta_o (V complex), phi_o = M^dag eta_o
+        {
+            HMCParams he = hp;
+            he.use_eo = true;
+            GaugeField ge(lh);
+            ge.U = gh.U;
+            std::mt19937 ea(31), eb(31);
+            HMCResult er = hmc_trajectory(ge, lh, 0.1, 1.0, he, ea);
+            MomentumField m3(lh);
+            m3.randomise(eb);
+            const Vec eta_o = random_vec(lh.V, eb);
+            Vec phi_o(lh.V);
+            orc_dirac* De = orc_dirac_new(Lh, gh.links().data(), 0.1, 1.0, 1.0, 0.0);
+            orc_eo_apply(De, 6, eta_o.data(), phi_o.data());
+            orc_dirac_free(De);
+            RVec pe = detail::mom_flat(m3);
+            Vec Ue = gh.links();
+            double e0[4], e1[4];
+            orc_hmc_hamiltonian_eo(Lh, Ue.data(), 0.1, 1.0, 1.0, 0.0, 2.0, 2000, 1e-12, pe.data(), phi_o.data(), e0);
+            orc_hmc_evolve_eo(Lh, Ue.data(), 0.1, 1.0, 1.0, 0.0, 2.0, 0.5, 4, 2000, 1e-12, pe.data(), phi_o.data());
+            orc_hmc_hamiltonian_eo(Lh, Ue.data(), 0.1, 1.0, 1.0, 0.0, 2.0, 2000, 1e-12, pe.data(), phi_o.data(), e1);
+            double dHe = 0;
+            for (int i = 0; i < 4; i++) dHe += e1[i] - e0[i];
+            report("hmc_trajectory (even-odd) dH", std::abs(er.dH - dHe), 1e-8);
+            report("hmc_trajectory (even-odd) dLD", std::abs(er.dLD - (e1[3] - e0[3])), 1e-9);
+            report("hmc_trajectory (even-odd) links", relerr(ge.links(), Ue), 1e-11);
+        }
         GaugeField gr(lh);
         gr.U = gh.U;
         std::mt19937 rr(5);

@@ -103,6 +103,12 @@ def main():
     for om in (0, 1):
         Uf, pf, it = R.leapfrog_evolve(L, U, 0.1, 1.0, 1.0, 2.0, 0.5, 3, om, 500, 1e-12, pi, phi)
         out[f"L6_md{om}_U"], out[f"L6_md{om}_pi"], out[f"L6_md{om}_it"] = Uf, pf, np.array([it])
+    # hmc_trajectory (force_accept) on its own draws from mt19937(55): full lattice, even-odd
+    for eo in (0, 1):
+        pi_d, eta_d = R.hmc_draws(L, 55, V if eo else 2 * V)
+        Uf, res = R.hmc_trajectory(L, U, 0.1, 1.0, 1.0, 2.0, 0.5, 3, 0, eo, 500, 1e-12, 55)
+        out[f"L6_traj{eo}_pi"], out[f"L6_traj{eo}_eta"] = pi_d, eta_d
+        out[f"L6_traj{eo}_U"], out[f"L6_traj{eo}_out"] = Uf, res
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_vectors.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes,", len(out), "arrays")

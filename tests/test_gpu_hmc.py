@@ -56,6 +56,11 @@ def test_hamiltonian_vs_oracle(smg):
     D = smg.DiracOp(L, U, MASS, R_W, 1.0, 0.0)
     hg = smg.hmc_hamiltonian(D, BETA, pi, phi, 2000, 1e-12)
     ho = O.hmc_hamiltonian(L, U, MASS, R_W, 1.0, 0.0, BETA, 2000, 1e-12, pi, phi)
+    assert np.all(np.abs(hg[:3] - ho) <= 1e-10 * np.abs(ho)) and hg[3] == 0.0
+    # even-odd: S_f on the Schur operator and LD = -2 log det D_ee
+    phi_o = gaussian_vec(V, 83)
+    hg = smg.hmc_hamiltonian(D, BETA, pi, phi_o, 2000, 1e-12, use_eo=True)
+    ho = O.hmc_hamiltonian_eo(L, U, MASS, R_W, 1.0, 0.0, BETA, 2000, 1e-12, pi, phi_o)
     assert np.all(np.abs(hg - ho) <= 1e-10 * np.abs(ho))
 
 
@@ -110,3 +115,45 @@ def test_trajectory_metropolis(smg):
     res2 = smg.hmc_trajectory(D2, p, pi, eta, 2.0)
     assert res2.accepted == 1 and relerr(smg.gauge_of(D2), Uo) < 1e-11
     assert abs(res2.dH - dHo) < 1e-8
+
+
+@pytest.mark.parametrize("csw", [0.0, 1.0])
+def test_eo_evolve_and_trajectory_vs_oracle(smg, csw):
+    """The even-odd trajectory (src/hmc_trajectory.cpp:53-135): leapfrog on the Schur
+    force, phi_o = M^dag eta_o, dH with the log-det term."""
+    banner(f"hmc even-odd csw={csw}")
+    L, V = 16, 256
+    U = random_gauge(L, 0.3, 110)
+    pi = np.random.default_rng(111).standard_normal(2 * V)
+    eta = gaussian_vec(V, 112)
+    Do = O.Dirac(L, U, MASS, R_W, csw, 0.0)
+    phi_o = Do.eo(6, eta)   # M^dag eta_o
+    D = smg.DiracOp(L, U, MASS, R_W, csw, 0.0)
+    p = smg.hmc_params(BETA, 0.5, 4, cg_maxiter=2000, cg_tol=1e-12, use_eo=True)
+    pg, _ = smg.hmc_evolve(D, p, pi, phi_o)
+    Uo, po, _ = O.hmc_evolve_eo(L, U, MASS, R_W, csw, 0.0, BETA, 0.5, 4, 2000, 1e-12, pi, phi_o)
+    print(f"eo evolve: U {relerr(smg.gauge_of(D), Uo):.2e}, pi {relerr(pg, po):.2e}")
+    assert relerr(smg.gauge_of(D), Uo) < 1e-11 and relerr(pg, po) < 1e-10
+    h0 = O.hmc_hamiltonian_eo(L, U, MASS, R_W, csw, 0.0, BETA, 2000, 1e-12, pi, phi_o)
+    h1 = O.hmc_hamiltonian_eo(L, Uo, MASS, R_W, csw, 0.0, BETA, 2000, 1e-12, po, phi_o)
+    D2 = smg.DiracOp(L, U, MASS, R_W, csw, 0.0)
+    p.force_accept = 1
+    res = smg.hmc_trajectory(D2, p, pi, eta, 0.5)
+    assert res.accepted == 1 and relerr(smg.gauge_of(D2), Uo) < 1e-11
+    assert abs(res.dH - (h1 - h0).sum()) < 1e-8 and abs(res.dLD - (h1[3] - h0[3])) < 1e-9
+    if csw == 0.0:
+        assert res.dLD == 0.0
+
+
+def test_trajectory_golden(smg):
+    """hmc_trajectory of the compiled reference (full lattice and even-odd) on its own
+    draws, reproduced from the same momenta and Gaussians."""
+    banner("hmc trajectory golden L=6")
+    g = np.load(GOLD)
+    for eo in (0, 1):
+        D = smg.DiracOp(6, g["L6_U"], MASS, R_W, 1.0, 0.0)
+        p = smg.hmc_params(BETA, 0.5, 3, cg_maxiter=500, cg_tol=1e-12, force_accept=True, use_eo=bool(eo))
+        res = smg.hmc_trajectory(D, p, g[f"L6_traj{eo}_pi"], g[f"L6_traj{eo}_eta"], 0.5)
+        want = g[f"L6_traj{eo}_out"]
+        assert relerr(smg.gauge_of(D), g[f"L6_traj{eo}_U"]) < 1e-11
+        assert abs(res.dH - want[0]) < 1e-9 and abs(res.dLD - want[4]) < 1e-9

@@ -134,6 +134,31 @@ def test_golden_hmc_md(g):
         assert it == g[f"L6_md{om}_it"][0]
 
 
+def test_golden_hmc_trajectory(g):
+    """hmc_trajectory (full lattice and even-odd) reproduced by the oracle from the
+    reference's draws: final links and dH components."""
+    L = 6
+    U = g["L6_U"]
+    for eo in (0, 1):
+        pi, eta = g[f"L6_traj{eo}_pi"], g[f"L6_traj{eo}_eta"]
+        D = O.Dirac(L, U, 0.1, 1.0, 1.0, 0.0)
+        args = (0.1, 1.0, 1.0, 0.0, 2.0)
+        if eo:
+            phi = D.eo(6, eta)
+            h0 = O.hmc_hamiltonian_eo(L, U, *args, 500, 1e-12, pi, phi)
+            Uf, pf, _ = O.hmc_evolve_eo(L, U, *args, 0.5, 3, 500, 1e-12, pi, phi)
+            h1 = O.hmc_hamiltonian_eo(L, Uf, *args, 500, 1e-12, pf, phi)
+        else:
+            phi = D.apply(1, eta)
+            h0 = np.r_[O.hmc_hamiltonian(L, U, *args, 500, 1e-12, pi, phi), 0.0]
+            Uf, pf, _ = O.hmc_evolve(L, U, *args, 0.5, 3, 0, 500, 1e-12, pi, phi)
+            h1 = np.r_[O.hmc_hamiltonian(L, Uf, *args, 500, 1e-12, pf, phi), 0.0]
+        want = g[f"L6_traj{eo}_out"]
+        assert close(Uf, g[f"L6_traj{eo}_U"], 1e-13)
+        d = h1 - h0
+        assert abs(d.sum() - want[0]) < 1e-10 and np.all(np.abs(d - want[1:5]) < 1e-10)
+
+
 # ------------------------------------------------------------------ live compiled reference
 @pytest.mark.skipif(not R.available(), reason="oracle/_ref/libref.so not built (reference sources absent)")
 @pytest.mark.parametrize("L,seed", [(6, 1), (10, 2), (16, 3)])
