@@ -51,13 +51,15 @@ inline void fit(Vec& v, size_t n) {
 }
 }  // namespace detail
 
-// random_vec, src/linalg.cpp:44-49 (host RNG stream identical to the reference's)
+// random_vec, src/linalg.cpp:44-49: the reference writes cx(dist(rng), dist(rng)),
+// whose two draws g++ evaluates right to left, so the imaginary part takes the
+// first draw; written out here so that every compiler gives the g++ stream
 inline Vec random_vec(int n, std::mt19937& rng) {
     std::normal_distribution<double> dist(0.0, 1.0);
     Vec v(n);
     for (auto& x : v) {
-        const double re = dist(rng);
         const double im = dist(rng);
+        const double re = dist(rng);
         x = cx(re, im);
     }
     return v;
@@ -327,7 +329,7 @@ struct CoarseApply {   // the OpApply of SparseCoarseOp::as_op
     void operator()(const Vec& s, Vec& d) const;
 };
 
-// SparseCoarseOp, src/coarse_op.h:154-302 (no deflation)
+// SparseCoarseOp, src/coarse_op.h:154-548 (setup_deflation: the TRLM branch)
 struct SparseCoarseOp {
     int nbx = 0, nby = 0, nblocks = 0, k_vec = 0, dim = 0;
     // Galerkin A_c = P^dag A P; A must be DiracOp::as_DdagD_op()
@@ -349,12 +351,66 @@ struct SparseCoarseOp {
         detail::check(smg_coarse_apply(h_.get(), SMG_F64, detail::raw(src), detail::raw(dst)));
     }
     OpApply as_op() const { return CoarseApply{this}; }
+    // CG from x0 = 0, or from the deflated guess once setup_deflation has run
     Vec solve(const Vec& b, int max_iter = 200, double cg_tol = 1e-12) const {
         Vec x(dim);
         int it = 0;
         detail::check(smg_coarse_solve(h_.get(), detail::raw(b), detail::raw(x), max_iter, cg_tol, &it));
         return x;
     }
+    // src/coarse_op.cpp:10-54; the deflation space lives on the device, and
+    // defl_vecs / defl_vals below are host copies taken when it is set up
+    void setup_deflation(int n_ev, int n_kr = 0, int max_restarts = 100, double tol = 1e-10,
+                         const std::string& solver = "trlm", double feast_emax = 0.0) {
+        (void)feast_emax;
+        if (solver != "trlm") throw std::invalid_argument("setup_deflation: only the 'trlm' solver is provided");
+        if (dim <= 0 || n_ev <= 0) return;
+        std::mt19937 rng(42);
+        const Vec start = random_vec(dim, rng);
+        smg_trlm_info info{};
+        detail::check(smg_coarse_setup_deflation(h_.get(), n_ev, n_kr, max_restarts, tol, detail::raw(start), &info));
+        if (!info.converged)
+            std::cerr << "SparseCoarseOp::setup_deflation: TRLM did not converge (restarts=" << info.restarts
+                      << ")\n";
+        pull_deflation();
+    }
+    // src/coarse_op.h:176-200: keeps the vectors whose fresh Rayleigh quotient is
+    // positive and whose relative residual is below res_threshold
+    int validate_deflation(double res_threshold = 0.1) {
+        int kept = 0;
+        for (size_t i = 0; i < defl_vecs.size(); i++) {
+            Vec Av;
+            apply_to(defl_vecs[i], Av);
+            cx vav = 0.0;
+            for (int j = 0; j < dim; j++) vav += std::conj(defl_vecs[i][j]) * Av[j];
+            const double rq = vav.real();
+            double rn = 0.0, an = 0.0;
+            for (int j = 0; j < dim; j++) {
+                rn += std::norm(Av[j] - rq * defl_vecs[i][j]);
+                an += std::norm(Av[j]);
+            }
+            const double rel = an > 1e-60 ? std::sqrt(rn / an) : 1.0;
+            if (rq > 1e-14 && rel < res_threshold) {
+                if ((size_t)kept < i) defl_vecs[kept] = std::move(defl_vecs[i]);
+                defl_vals[kept] = rq;
+                kept++;
+            }
+        }
+        defl_vecs.resize(kept);
+        defl_vals.resize(kept);
+        push_deflation();
+        return kept;
+    }
+    // after editing defl_vecs / defl_vals on the host, hand them to the device solve
+    void push_deflation() {
+        const int nd = (int)defl_vals.size();
+        Vec all((size_t)nd * dim);
+        for (int i = 0; i < nd; i++) std::copy(defl_vecs[i].begin(), defl_vecs[i].end(), all.begin() + (size_t)i * dim);
+        detail::check(smg_coarse_set_deflation(h_.get(), nd, nd ? defl_vals.data() : nullptr,
+                                               nd ? detail::raw(all) : nullptr));
+    }
+    std::vector<Vec> defl_vecs;
+    std::vector<double> defl_vals;
     // link[b][d][ki*k + kj] flattened, src/coarse_op.h:159-166
     Vec links() const {
         Vec out((size_t)nblocks * 9 * k_vec * k_vec);
@@ -364,6 +420,15 @@ struct SparseCoarseOp {
     smg_coarse* handle() const { return h_.get(); }
 
    private:
+    void pull_deflation() {
+        int nd = 0;
+        detail::check(smg_coarse_deflation(h_.get(), &nd, nullptr, nullptr));
+        defl_vals.assign(nd, 0.0);
+        Vec all((size_t)nd * dim);
+        if (nd) detail::check(smg_coarse_deflation(h_.get(), &nd, defl_vals.data(), detail::raw(all)));
+        defl_vecs.assign(nd, Vec());
+        for (int i = 0; i < nd; i++) defl_vecs[i].assign(all.begin() + (size_t)i * dim, all.begin() + (size_t)(i + 1) * dim);
+    }
     std::shared_ptr<smg_coarse> h_;
 };
 inline void CoarseApply::operator()(const Vec& s, Vec& d) const { C->apply_to(s, d); }
@@ -635,6 +700,86 @@ inline FGMRESResult fgmres_solve_generic(const OpApply& A, int n, const Vec& rhs
 inline FGMRESResult fgmres_solve_mg(const OpApply& A, int n, const Vec& rhs, MGHierarchy& mg, int restart,
                                     int max_iter, double tol, int n_ritz_harvest = 4) {
     return fgmres_solve_generic(A, n, rhs, mg.as_precond(), restart, max_iter, tol, n_ritz_harvest);
+}
+
+// ------------------------------------------------------------------ eigensolvers, src/eigensolver.h
+struct TRLMResult {   // src/eigensolver.h:19-25
+    std::vector<Vec> eigvecs;
+    std::vector<double> eigvals;
+    int iterations;
+    int num_restarts;
+    bool converged;
+};
+struct TRLMState;   // saving the Krylov space (state_out) is not provided by this library
+
+// lanczos_eigen, src/eigensolver_core.cpp:103-265: A_cols[j][i] = A_ij; evecs[c] the
+// eigenvector of the c-th (ascending) eigenvalue
+inline void lanczos_eigen(std::vector<Vec>& A_cols, int n, RVec& evals, std::vector<Vec>& evecs) {
+    Vec a((size_t)n * n), v((size_t)n * n);
+    for (int j = 0; j < n; j++) std::copy(A_cols[j].begin(), A_cols[j].begin() + n, a.begin() + (size_t)j * n);
+    evals.assign(n, 0.0);
+    detail::check(smg_dense_eigh(n, detail::raw(a), evals.data(), detail::raw(v)));
+    evecs.assign(n, Vec(n));
+    for (int c = 0; c < n; c++)
+        for (int r = 0; r < n; r++) evecs[c][r] = v[(size_t)r * n + c];
+}
+
+// trlm_eigensolver, src/eigensolver_core.cpp:691-1057, on DiracOp::as_DdagD_op() or
+// SparseCoarseOp::as_op(); the Chebyshev filter (poly_deg > 0) and state_out are not
+// provided
+inline TRLMResult trlm_eigensolver(const OpApply& A, int n, int n_ev, int n_kr = 0, int max_restarts = 100,
+                                   double tol = 1e-10, int poly_deg = 0, double a_min = 0.0, double a_max = 0.0,
+                                   TRLMState* state_out = nullptr, const Vec* start_vec = nullptr) {
+    (void)a_min;
+    (void)a_max;
+    if (poly_deg > 0) throw std::invalid_argument("trlm_eigensolver: the Chebyshev filter is not provided");
+    if (state_out) throw std::invalid_argument("trlm_eigensolver: state_out is not provided");
+    Vec start;
+    if (start_vec && (int)start_vec->size() == n) {
+        start = *start_vec;
+    } else {
+        std::mt19937 rng(42);
+        start = random_vec(n, rng);
+    }
+    TRLMResult r{std::vector<Vec>(n_ev, Vec(n)), std::vector<double>(n_ev, 0.0), 0, 0, false};
+    Vec all((size_t)n_ev * n);
+    smg_trlm_info info{};
+    if (const DdagDApply* f = A.target<DdagDApply>()) {
+        if (n != f->D->lat.ndof) throw std::invalid_argument("trlm_eigensolver: n mismatch");
+        detail::check(smg_dirac_trlm(f->D->handle(), SMG_OP_DDAGD, n_ev, n_kr, max_restarts, tol, detail::raw(start),
+                                     r.eigvals.data(), detail::raw(all), &info));
+    } else if (const CoarseApply* c = A.target<CoarseApply>()) {
+        if (n != c->C->dim) throw std::invalid_argument("trlm_eigensolver: n mismatch");
+        detail::check(smg_coarse_trlm(c->C->handle(), n_ev, n_kr, max_restarts, tol, detail::raw(start),
+                                      r.eigvals.data(), detail::raw(all), &info));
+    } else {
+        throw std::invalid_argument("trlm_eigensolver: A must be DiracOp::as_DdagD_op() or SparseCoarseOp::as_op()");
+    }
+    for (int i = 0; i < n_ev; i++) r.eigvecs[i].assign(all.begin() + (size_t)i * n, all.begin() + (size_t)(i + 1) * n);
+    r.iterations = info.iterations;
+    r.num_restarts = info.restarts;
+    r.converged = info.converged != 0;
+    return r;
+}
+
+// cg_solve_deflated, src/solvers.cpp:111-213 (precond: MGHierarchy::as_precond() or
+// the identity, an empty std::function); defl_vals is not read, as in the reference
+inline CGResult cg_solve_deflated(const OpApply& A, int n, const Vec& rhs,
+                                  const std::function<Vec(const Vec&)>& precond, const std::vector<Vec>& defl_vecs,
+                                  const std::vector<double>& defl_vals, int max_iter, double tol) {
+    (void)defl_vals;
+    const DiracOp& D = detail::dirac_of(A, "cg_solve_deflated");
+    MGHierarchy* mg = precond ? detail::mg_of(precond, "cg_solve_deflated") : nullptr;
+    const int nd = (int)defl_vecs.size();
+    Vec U((size_t)nd * n);
+    for (int i = 0; i < nd; i++) std::copy(defl_vecs[i].begin(), defl_vecs[i].begin() + n, U.begin() + (size_t)i * n);
+    CGResult r{Vec(n), 0, 0.0};
+    smg_solve_info info{};
+    detail::check(smg_cg_solve_deflated(D.handle(), SMG_OP_DDAGD, mg ? mg->handle() : nullptr, detail::raw(rhs), nd,
+                                        nd ? detail::raw(U) : nullptr, detail::raw(r.solution), max_iter, tol, &info));
+    r.iterations = info.iterations;
+    r.final_residual = info.final_residual;
+    return r;
 }
 
 // ------------------------------------------------------------------ actions and forces, src/hmc.h

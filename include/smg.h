@@ -157,7 +157,9 @@ int smg_coarse_dense(smg_coarse* C, double* M);
 int smg_coarse_dense_solve(smg_coarse* C, const double* b, double* x);
 /* SparseCoarseOp::apply_to, src/coarse_op.h:282-297, links read in precision prec */
 int smg_coarse_apply(smg_coarse* C, int prec, const double* src, double* dst);
-/* SparseCoarseOp::solve without deflation (CG from x0 = 0), src/coarse_op.h:492-547 */
+/* SparseCoarseOp::solve, src/coarse_op.h:492-547: CG from x0 = 0, or from the
+ * deflated guess once smg_coarse_setup_deflation / smg_coarse_set_deflation has
+ * given C a deflation space */
 int smg_coarse_solve(smg_coarse* C, const double* b, double* x, int maxit, double tol, int* iters);
 
 /* ------------------------------------------------------------------ smoothers */
@@ -229,6 +231,53 @@ int smg_gcr_solve_dev(smg_dirac* D, smg_mg* mg, int prec, const void* b, void* x
 /* fgmres_solve_generic / fgmres_solve_mg with n_ritz_harvest = 0, src/solvers.cpp:555-800 */
 int smg_fgmres_solve(smg_dirac* D, smg_mg* mg, const double* b, double* x, int restart, int maxit, double tol,
                      smg_solve_info* info);
+
+/* ------------------------------------------------------------------ eigensolvers and deflation */
+/* random_vec (src/linalg.cpp:44-49) from std::mt19937(seed): n complex numbers of
+ * unit-variance normal parts, in the order the reference's g++ build draws them.
+ * With seed 42 this is the default start vector of trlm_eigensolver
+ * (src/eigensolver_core.cpp:720-727).  Host only. */
+int smg_random_vec(int n, unsigned seed, double* out);
+/* lanczos_eigen (src/eigensolver_core.cpp:103-265; declared src/eigensolver.h:30):
+ * every eigenpair of the n x n Hermitian matrix A, complex, in the reference's
+ * column layout A[j*n + i] = A_ij (A_cols[j][i]); evals ascending; evecs[row*n + col]
+ * holds component row of eigenvector col (the reference's evecs[col][row]).  Host only. */
+int smg_dense_eigh(int n, const double* A, double* evals, double* evecs);
+/* trlm_eigensolver with poly_deg = 0 (src/eigensolver_core.cpp:691-1057; declared
+ * src/eigensolver.h:68-73) on D^dag D (op SMG_OP_DDAGD) or M^dag M (SMG_OP_SCHUR):
+ * the n_ev smallest eigenpairs, evecs[n_ev][n].  n_kr <= 0 takes the reference's
+ * default Krylov size.  start: the start vector (the reference's default is
+ * smg_random_vec(n, 42)).  info (may be NULL) reports a run that stopped at
+ * max_restarts unconverged, as the reference's TRLMResult does. */
+typedef struct {
+    int iterations; /* operator applications (TRLMResult::iterations) */
+    int restarts;   /* TRLMResult::num_restarts */
+    int converged;  /* TRLMResult::converged */
+} smg_trlm_info;
+int smg_dirac_trlm(smg_dirac* D, int op, int n_ev, int n_kr, int max_restarts, double tol, const double* start,
+                   double* evals, double* evecs, smg_trlm_info* info);
+/* trlm_eigensolver as smg_dirac_trlm, on the coarse operator (SparseCoarseOp::as_op,
+ * src/coarse_op.h:299-301) over its dimension; nothing is stored in C */
+int smg_coarse_trlm(smg_coarse* C, int n_ev, int n_kr, int max_restarts, double tol, const double* start,
+                    double* evals, double* evecs, smg_trlm_info* info);
+/* SparseCoarseOp::setup_deflation with solver "trlm" (src/coarse_op.cpp:10-54;
+ * declared src/coarse_op.h:480-483): TRLM on A_c, the pairs kept in C; every later
+ * smg_coarse_solve then starts from the deflated guess x0 = sum_i (v_i^dag b / l_i) v_i
+ * (SparseCoarseOp::solve, src/coarse_op.h:492-547).  start: as for smg_dirac_trlm
+ * over the coarse dimension. */
+int smg_coarse_setup_deflation(smg_coarse* C, int n_ev, int n_kr, int max_restarts, double tol, const double* start,
+                               smg_trlm_info* info);
+/* SparseCoarseOp::defl_vals / defl_vecs (src/coarse_op.h:169-170): read (evals,
+ * evecs[ndefl][n] may be NULL) or replace (ndefl = 0 clears) the deflation space */
+int smg_coarse_deflation(smg_coarse* C, int* ndefl, double* evals, double* evecs);
+int smg_coarse_set_deflation(smg_coarse* C, int ndefl, const double* evals, const double* evecs);
+/* cg_solve_deflated (src/solvers.cpp:111-213; declared src/solvers.h:50-56):
+ * CG from x0 = U E^-1 U^dag b, E = U^dag A U, with the preconditioned residual
+ * projected, z = M^-1 r - U E^-1 (AU)^dag M^-1 r; M = the FP64 MG cycle or none
+ * (mg = NULL); op as smg_cg_solve.  defl_vecs[ndefl][n]; the reference's defl_vals
+ * are not read by the solver and are not taken. */
+int smg_cg_solve_deflated(smg_dirac* D, int op, smg_mg* mg, const double* b, int ndefl, const double* defl_vecs,
+                          double* x, int maxit, double tol, smg_solve_info* info);
 
 /* ------------------------------------------------------------------ many right-hand sides
  * N right-hand sides per launch, N a power of two (<= 32 for the solver, <= 64

@@ -59,6 +59,13 @@ def lib():
         L.orc_hmc_evolve.argtypes = [_I, _P, _D, _D, _D, _D, _D, _D, _I, _I, _I, _D, _P, _P]
         L.orc_hmc_hamiltonian.restype = _I
         L.orc_hmc_hamiltonian.argtypes = [_I, _P, _D, _D, _D, _D, _D, _I, _D, _P, _P, _P]
+        L.orc_lanczos_eigen.argtypes = [_I, _P, _P, _P]
+        L.orc_cg_deflated.restype = _I
+        L.orc_cg_deflated.argtypes = [_P, _P, _P, _I, _P, _I, _D, _P, _P]
+        L.orc_trlm.restype = _I
+        L.orc_trlm.argtypes = [_P, _I, _I, _I, _D, _P, _P, _P, _P, _P]
+        L.orc_deflated_coarse_cg.restype = _I
+        L.orc_deflated_coarse_cg.argtypes = [_P, _I, _P, _P, _P, _I, _D, _P]
         L.orc_hmc_evolve_eo.restype = _I
         L.orc_hmc_evolve_eo.argtypes = [_I, _P, _D, _D, _D, _D, _D, _D, _I, _I, _D, _P, _P]
         L.orc_hmc_hamiltonian_eo.restype = _I
@@ -507,3 +514,44 @@ def hmc_hamiltonian_eo(L, U, mass, r, csw, mu_t, beta, cg_maxit, cg_tol, pi, phi
     lib().orc_hmc_hamiltonian_eo(L, _ptr(_c(U)), mass, r, csw, mu_t, beta, cg_maxit, cg_tol,
                                  _ptr(np.ascontiguousarray(pi, dtype=np.float64)), _ptr(_c(phi_o)), _ptr(out))
     return out
+
+
+def lanczos_eigen(A):
+    """Dense Hermitian eigensolver of lanczos_eigen: (ascending eigenvalues, eigenvector columns)."""
+    n = A.shape[0]
+    cols = np.ascontiguousarray(np.asarray(A, np.complex128).T)
+    ev = np.zeros(n)
+    vecs = np.zeros((n, n), np.complex128)
+    lib().orc_lanczos_eigen(n, _ptr(cols), _ptr(ev), _ptr(vecs))
+    return ev, vecs
+
+
+def trlm(op, n_ev, n_kr, max_restarts, tol, start):
+    """Thick-restart Lanczos: (converged, restarts, eigenvalues, eigenvectors [n_ev][n],
+    operator applications)."""
+    ev = np.zeros(n_ev)
+    vecs = np.zeros((n_ev, op.n), np.complex128)
+    rs, its = ctypes.c_int(), ctypes.c_int()
+    conv = lib().orc_trlm(op.h, n_ev, n_kr, max_restarts, tol, _ptr(_c(start)), _ptr(ev), _ptr(vecs),
+                          ctypes.byref(rs), ctypes.byref(its))
+    return bool(conv), rs.value, ev, vecs, its.value
+
+
+def deflated_coarse_cg(op, vecs, vals, b, maxit, tol):
+    """SparseCoarseOp::solve with deflation vectors: (x, iterations)."""
+    x = np.zeros(op.n, np.complex128)
+    vecs = np.ascontiguousarray(vecs, np.complex128)
+    vals = np.ascontiguousarray(vals, np.float64)
+    it = lib().orc_deflated_coarse_cg(op.h, len(vals), _ptr(vecs), _ptr(vals), _ptr(_c(b)), maxit, tol, _ptr(x))
+    return x, it
+
+
+def cg_deflated(op, b, U, maxit, tol, mg=None):
+    """cg_solve_deflated with the deflation vectors U [ndefl][n] and the MG cycle (or
+    none) as preconditioner: (x, iterations, final residual)."""
+    x = np.zeros(op.n, np.complex128)
+    U = np.ascontiguousarray(U, np.complex128).reshape(-1, op.n)
+    res = np.zeros(1)
+    it = lib().orc_cg_deflated(op.h, _ptr(_c(b)), mg.h if mg is not None else None, U.shape[0], _ptr(U), maxit,
+                               tol, _ptr(x), _ptr(res))
+    return x, it, res[0]

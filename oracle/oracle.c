@@ -1507,3 +1507,324 @@ int orc_hmc_hamiltonian_eo(int L, const ocx* U, double mass, double r, double cs
     orc_dirac_free(D);
     return it;
 }
+
+/* ================================================================ eigensolvers */
+/* lanczos_eigen, src/eigensolver_core.cpp:103-265: A (n x n Hermitian, column-major
+ * A[j*n + i]) -> ascending eigenvalues and evecs[row*n + col] (column col = eigenvector
+ * col).  Lanczos tridiagonalisation from e_0 with two MGS passes, then implicit QL
+ * with the Wilkinson shift on the real tridiagonal. */
+void orc_lanczos_eigen(int n, const ocx* A, double* evals, ocx* evecs) {
+    if (n <= 0) return;
+    if (n == 1) { evals[0] = creal(A[0]); evecs[0] = 1.0; return; }
+    ocx* Q = xcalloc((size_t)n * n, sizeof(ocx));      /* Q[j*n + i]: vector j */
+    double* al = xcalloc(n, sizeof(double));
+    double* be = xcalloc(n + 1, sizeof(double));
+    ocx* w = xcalloc(n, sizeof(ocx));
+#define MATVEC(x, y) do { for (int i_ = 0; i_ < n; i_++) (y)[i_] = 0; \
+        for (int j_ = 0; j_ < n; j_++) { ocx xj_ = (x)[j_]; for (int i_ = 0; i_ < n; i_++) (y)[i_] += A[(size_t)j_ * n + i_] * xj_; } } while (0)
+    Q[0] = 1.0;
+    MATVEC(Q, w);
+    al[0] = creal(orc_dot(n, Q, w));
+    for (int i = 0; i < n; i++) w[i] -= al[0] * Q[i];
+    for (int j = 1; j < n; j++) {
+        ocx* qj = Q + (size_t)j * n;
+        be[j] = orc_norm(n, w);
+        if (be[j] < 1e-14) {
+            int found = 0;
+            for (int t = 0; t < n && !found; t++) {
+                for (int i = 0; i < n; i++) qj[i] = 0;
+                qj[t] = 1.0;
+                for (int pass = 0; pass < 2; pass++)
+                    for (int i = 0; i < j; i++) {
+                        ocx pr = orc_dot(n, Q + (size_t)i * n, qj);
+                        for (int k = 0; k < n; k++) qj[k] -= pr * Q[(size_t)i * n + k];
+                    }
+                double nq = orc_norm(n, qj);
+                if (nq > 1e-10) { for (int k = 0; k < n; k++) qj[k] /= nq; found = 1; }
+            }
+            be[j] = 0.0;
+        } else {
+            for (int i = 0; i < n; i++) qj[i] = w[i] / be[j];
+        }
+        for (int pass = 0; pass < 2; pass++)
+            for (int i = 0; i < j; i++) {
+                ocx pr = orc_dot(n, Q + (size_t)i * n, qj);
+                for (int k = 0; k < n; k++) qj[k] -= pr * Q[(size_t)i * n + k];
+            }
+        double nq = orc_norm(n, qj);
+        if (nq > 1e-14) for (int k = 0; k < n; k++) qj[k] /= nq;
+        MATVEC(qj, w);
+        al[j] = creal(orc_dot(n, qj, w));
+        for (int i = 0; i < n; i++) w[i] -= al[j] * qj[i] + be[j] * Q[(size_t)(j - 1) * n + i];
+        for (int pass = 0; pass < 2; pass++)
+            for (int i = 0; i <= j; i++) {
+                ocx pr = orc_dot(n, Q + (size_t)i * n, w);
+                for (int k = 0; k < n; k++) w[k] -= pr * Q[(size_t)i * n + k];
+            }
+    }
+#undef MATVEC
+    double* d = xcalloc(n, sizeof(double));
+    double* e = xcalloc(n, sizeof(double));
+    double* Z = xcalloc((size_t)n * n, sizeof(double));   /* Z[k*n + i] */
+    for (int i = 0; i < n; i++) { d[i] = al[i]; Z[(size_t)i * n + i] = 1.0; }
+    for (int i = 0; i < n - 1; i++) e[i] = be[i + 1];
+    for (int l = 0; l < n; l++) {
+        int iter = 0;
+        for (;;) {
+            int m;
+            for (m = l; m < n - 1; m++) {
+                double dd = fabs(d[m]) + fabs(d[m + 1]);
+                if (fabs(e[m]) <= 1e-15 * dd) break;
+            }
+            if (m == l) break;
+            if (++iter > 200) break;
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = hypot(g, 1.0);
+            g = d[m] - d[l] + e[l] / (g + copysign(r, g));
+            double s = 1.0, c = 1.0, p = 0.0;
+            int early = 0;
+            for (int i = m - 1; i >= l; i--) {
+                double f = s * e[i], b = c * e[i];
+                r = hypot(f, g);
+                e[i + 1] = r;
+                if (r < 1e-30) { d[i + 1] -= p; e[m] = 0.0; early = 1; break; }
+                s = f / r;
+                c = g / r;
+                g = d[i + 1] - p;
+                r = (d[i] - g) * s + 2.0 * c * b;
+                p = s * r;
+                d[i + 1] = g + p;
+                g = c * r - b;
+                for (int k = 0; k < n; k++) {
+                    double t = Z[(size_t)k * n + i + 1];
+                    Z[(size_t)k * n + i + 1] = s * Z[(size_t)k * n + i] + c * t;
+                    Z[(size_t)k * n + i] = c * Z[(size_t)k * n + i] - s * t;
+                }
+            }
+            if (!early) { d[l] -= p; e[l] = g; e[m] = 0.0; }
+        }
+    }
+    /* ascending order (stable, as std::sort on distinct values) then back-transform */
+    int* idx = xcalloc(n, sizeof(int));
+    for (int i = 0; i < n; i++) idx[i] = i;
+    for (int i = 1; i < n; i++) {
+        int v = idx[i], j = i;
+        while (j > 0 && d[idx[j - 1]] > d[v]) { idx[j] = idx[j - 1]; j--; }
+        idx[j] = v;
+    }
+    for (int col = 0; col < n; col++) {
+        evals[col] = d[idx[col]];
+        for (int row = 0; row < n; row++) {
+            ocx v = 0;
+            for (int j = 0; j < n; j++) v += Q[(size_t)j * n + row] * Z[(size_t)j * n + idx[col]];
+            evecs[(size_t)row * n + col] = v;
+        }
+    }
+    free(idx); free(d); free(e); free(Z); free(Q); free(al); free(be); free(w);
+}
+
+/* trlm_eigensolver without the Chebyshev filter (poly_deg = 0), src/eigensolver_core.cpp:
+ * 691-1057: the n_ev smallest eigenpairs of the Hermitian operator A (dimension n), Krylov
+ * space n_kr, start vector start (n complex; the reference draws random_vec(n,
+ * mt19937(42))).  evecs: n_ev vectors of n; returns 1 if converged. */
+int orc_trlm(const orc_op* A, int n_ev, int n_kr, int max_restarts, double tol, const ocx* start,
+             double* evals, ocx* evecs, int* restarts_out, int* iters_out) {
+    int n = A->n;
+    if (n_kr <= 0) {   /* the default Krylov space without the Chebyshev filter */
+        n_kr = 2 * n_ev + 6 > n_ev + 32 ? 2 * n_ev + 6 : n_ev + 32;
+        if (n_kr > n) n_kr = n;
+    }
+    if (n_kr > n) n_kr = n;
+    if (n_kr < n_ev + 6) n_kr = n_ev + 6 < n ? n_ev + 6 : n;
+    ocx* K = xcalloc((size_t)(n_kr + 1) * n, sizeof(ocx));
+    double* alpha = xcalloc(n_kr, sizeof(double));
+    double* beta = xcalloc(n_kr, sizeof(double));
+    double* resid = xcalloc(n_kr, sizeof(double));
+    ocx* r = xcalloc(n, sizeof(ocx));
+    memcpy(K, start, n * sizeof(ocx));
+    double nv = orc_norm(n, K);
+    for (int i = 0; i < n; i++) K[i] /= nv;
+    int num_locked = 0, num_converged = 0, num_keep = 0, converged = 0, restart, iters = 0;
+#define KV(j) (K + (size_t)(j) * n)
+    for (restart = 0; restart < max_restarts && !converged; restart++) {
+        iters += n_kr - num_keep;   /* operator applications */
+        for (int j = num_keep; j < n_kr; j++) {   /* lanczosStep */
+            A->fn(A, KV(j), r);
+            alpha[j] = creal(orc_dot(n, KV(j), r));
+            for (int i = 0; i < n; i++) r[i] -= alpha[j] * KV(j)[i];
+            int start_i = j > num_keep ? j - 1 : 0;
+            for (int i = start_i; i < j; i++)
+                for (int k = 0; k < n; k++) r[k] -= beta[i] * KV(i)[k];
+            for (int pass = 0; pass < 2; pass++)
+                for (int k = 0; k <= j; k++) {
+                    ocx pr = orc_dot(n, KV(k), r);
+                    for (int i = 0; i < n; i++) r[i] -= pr * KV(k)[i];
+                }
+            beta[j] = orc_norm(n, r);
+            if (beta[j] > 1e-14)
+                for (int i = 0; i < n; i++) KV(j + 1)[i] = r[i] / beta[j];
+        }
+        int dim = n_kr - num_locked, arrow = num_keep - num_locked;
+        ocx* M = xcalloc((size_t)dim * dim, sizeof(ocx));
+        for (int i = 0; i < dim; i++) M[(size_t)i * dim + i] = alpha[i + num_locked];
+        for (int i = 0; i < arrow; i++) {
+            M[(size_t)arrow * dim + i] = beta[i + num_locked];
+            M[(size_t)i * dim + arrow] = beta[i + num_locked];
+        }
+        for (int i = arrow; i < dim - 1; i++) {
+            M[(size_t)(i + 1) * dim + i] = beta[i + num_locked];
+            M[(size_t)i * dim + i + 1] = beta[i + num_locked];
+        }
+        double* ev = xcalloc(dim, sizeof(double));
+        ocx* Z = xcalloc((size_t)dim * dim, sizeof(ocx));   /* Z[row*dim + col] */
+        orc_lanczos_eigen(dim, M, ev, Z);
+        for (int i = 0; i < dim; i++) {
+            resid[i + num_locked] = fabs(beta[n_kr - 1] * creal(Z[(size_t)(dim - 1) * dim + i]));
+            alpha[i + num_locked] = ev[i];
+        }
+        int it_locked = 0, it_conv;
+        for (int i = 1; i < dim; i++) {
+            if (resid[i + num_locked] < 1e-15 * fabs(alpha[i + num_locked])) it_locked = i;
+            else break;
+        }
+        it_conv = it_locked;
+        for (int i = it_locked + 1; i < dim; i++) {
+            if (resid[i + num_locked] < tol * fabs(alpha[i + num_locked])) it_conv = i;
+            else break;
+        }
+        int keep = it_conv + (n_kr - num_converged) / 2;
+        if (keep > n_kr - num_locked - 12) keep = n_kr - num_locked - 12;
+        if (keep < 1) keep = 1;
+        if (keep > dim - 1) keep = dim - 1;
+        ocx* rot = xcalloc((size_t)keep * n, sizeof(ocx));
+        for (int i = 0; i < keep; i++)
+            for (int j = 0; j < dim; j++) {
+                ocx c = Z[(size_t)j * dim + i];
+                for (int k = 0; k < n; k++) rot[(size_t)i * n + k] += c * KV(num_locked + j)[k];
+            }
+        memcpy(KV(num_locked), rot, (size_t)keep * n * sizeof(ocx));
+        memcpy(KV(num_locked + keep), KV(n_kr), n * sizeof(ocx));
+        for (int i = 0; i < keep; i++) beta[i + num_locked] = beta[n_kr - 1] * creal(Z[(size_t)(dim - 1) * dim + i]);
+        free(rot); free(M); free(ev); free(Z);
+        num_converged = num_locked + it_conv;
+        num_keep = num_locked + keep;
+        num_locked += it_locked;
+        if (num_converged >= n_ev) {
+            converged = 1;
+            for (int i = 1; i < n_kr; i++) {   /* ascending, insertion sort */
+                int j = i;
+                while (j > 0 && alpha[j - 1] > alpha[j]) {
+                    double t = alpha[j]; alpha[j] = alpha[j - 1]; alpha[j - 1] = t;
+                    for (int k = 0; k < n; k++) { ocx s = KV(j)[k]; KV(j)[k] = KV(j - 1)[k]; KV(j - 1)[k] = s; }
+                    j--;
+                }
+            }
+        }
+    }
+    for (int i = 0; i < n_ev; i++) {
+        evals[i] = alpha[i];
+        memcpy(evecs + (size_t)i * n, KV(i), n * sizeof(ocx));
+    }
+#undef KV
+    if (restarts_out) *restarts_out = restart;
+    if (iters_out) *iters_out = iters;
+    free(K); free(alpha); free(beta); free(resid); free(r);
+    return converged;
+}
+
+/* cg_solve_deflated, src/solvers.cpp:111-213: E = U^dag A U (E_ij = <u_i, A u_j>),
+ * x0 = U E^-1 U^dag b, then preconditioned CG whose preconditioned residual is
+ * projected, z = M^-1 r - U E^-1 (AU)^dag M^-1 r; E solves by CoarseOp::solve
+ * (partial-pivot elimination, src/coarse_op.h:101-137, as orc_dense_solve) */
+int orc_cg_deflated(const orc_op* A, const ocx* rhs, orc_mg* mg, int ndefl, const ocx* U, int maxit, double tol,
+                    ocx* x, double* final_res) {
+    int n = A->n;
+    memset(x, 0, n * sizeof(ocx));
+    double rhs_norm = orc_norm(n, rhs);
+    if (rhs_norm < 1e-30) { *final_res = 0; return 0; }
+    ocx* AU = xcalloc((size_t)(ndefl > 0 ? ndefl : 1) * n, sizeof(ocx));
+    for (int i = 0; i < ndefl; i++) A->fn(A, U + (size_t)i * n, AU + (size_t)i * n);
+    orc_dense E = {ndefl, xcalloc((size_t)(ndefl > 0 ? ndefl * ndefl : 1), sizeof(ocx))};
+    for (int i = 0; i < ndefl; i++)
+        for (int j = 0; j < ndefl; j++) E.mat[(size_t)j * ndefl + i] = orc_dot(n, U + (size_t)i * n, AU + (size_t)j * n);
+    ocx* t = xcalloc(ndefl + 1, sizeof(ocx));
+    ocx* c = xcalloc(ndefl + 1, sizeof(ocx));
+    for (int i = 0; i < ndefl; i++) t[i] = orc_dot(n, U + (size_t)i * n, rhs);
+    if (ndefl) orc_dense_solve(&E, t, c);
+    for (int i = 0; i < ndefl; i++)
+        for (int k = 0; k < n; k++) x[k] += c[i] * U[(size_t)i * n + k];
+    ocx* r = xcalloc(n, sizeof(ocx));
+    ocx* z = xcalloc(n, sizeof(ocx));
+    ocx* p = xcalloc(n, sizeof(ocx));
+    ocx* Ap = xcalloc(n, sizeof(ocx));
+    A->fn(A, x, Ap);
+    for (int i = 0; i < n; i++) r[i] = rhs[i] - Ap[i];
+#define PROJECT()                                                                          \
+    do {                                                                                   \
+        precond(mg, n, r, z);                                                              \
+        for (int i = 0; i < ndefl; i++) t[i] = orc_dot(n, AU + (size_t)i * n, z);          \
+        if (ndefl) orc_dense_solve(&E, t, c);                                              \
+        for (int i = 0; i < ndefl; i++)                                                    \
+            for (int k = 0; k < n; k++) z[k] -= c[i] * U[(size_t)i * n + k];               \
+    } while (0)
+    PROJECT();
+    memcpy(p, z, n * sizeof(ocx));
+    ocx rz = orc_dot(n, r, z);
+    int iter = 0;
+    while (iter < maxit) {
+        A->fn(A, p, Ap);
+        ocx alpha = rz / orc_dot(n, p, Ap);
+        for (int i = 0; i < n; i++) { x[i] += alpha * p[i]; r[i] -= alpha * Ap[i]; }
+        iter++;
+        if (orc_norm(n, r) / rhs_norm < tol) break;
+        PROJECT();
+        ocx rz_new = orc_dot(n, r, z);
+        ocx beta = rz_new / rz;
+        rz = rz_new;
+        for (int i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+    }
+#undef PROJECT
+    *final_res = orc_norm(n, r) / rhs_norm;
+    free(AU); free(E.mat); free(t); free(c); free(r); free(z); free(p); free(Ap);
+    return iter;
+}
+
+/* SparseCoarseOp::solve with deflation vectors, src/coarse_op.h:492-547: x0 = sum_i
+ * (v_i^dag b / lambda_i) v_i (lambda_i > 1e-14), then CG from x0 with the stopping rule
+ * |r|/|b| < tol checked after each update; returns the iterations */
+int orc_deflated_coarse_cg(const orc_op* A, int ndefl, const ocx* vecs, const double* vals, const ocx* b,
+                           int maxit, double tol, ocx* x) {
+    int n = A->n;
+    memset(x, 0, n * sizeof(ocx));
+    double bnorm = orc_norm(n, b);
+    if (bnorm < 1e-30) return 0;
+    for (int i = 0; i < ndefl; i++)
+        if (vals[i] > 1e-14) {
+            ocx c = orc_dot(n, vecs + (size_t)i * n, b) / vals[i];
+            for (int k = 0; k < n; k++) x[k] += c * vecs[(size_t)i * n + k];
+        }
+    ocx* r = xcalloc(n, sizeof(ocx));
+    ocx* p = xcalloc(n, sizeof(ocx));
+    ocx* Ap = xcalloc(n, sizeof(ocx));
+    A->fn(A, x, Ap);
+    for (int i = 0; i < n; i++) r[i] = b[i] - Ap[i];
+    memcpy(p, r, n * sizeof(ocx));
+    ocx rr = orc_dot(n, r, r);
+    int it = 0;
+    while (it < maxit) {
+        A->fn(A, p, Ap);
+        ocx pAp = orc_dot(n, p, Ap);
+        if (cabs(pAp) < 1e-30) break;
+        ocx a = rr / pAp;
+        for (int i = 0; i < n; i++) { x[i] += a * p[i]; r[i] -= a * Ap[i]; }
+        it++;
+        double rn = orc_norm(n, r);
+        if (rn / bnorm < tol || isnan(rn) || isinf(rn)) break;
+        ocx rn2 = orc_dot(n, r, r);
+        ocx be = rn2 / rr;
+        rr = rn2;
+        for (int i = 0; i < n; i++) p[i] = r[i] + be * p[i];
+    }
+    free(r); free(p); free(Ap);
+    return it;
+}

@@ -145,6 +145,15 @@ void orc_clover_deriv_insert(const orc_dirac* D, const double* w, double factor,
 void orc_logdet_ee_force(const orc_dirac* D, double* F);
 void orc_eo_fermion_force(const orc_dirac* D, const ocx* x_o, const ocx* y_o, double* F);
 
+/* ---------------- eigensolvers and deflation ---------------- */
+void orc_lanczos_eigen(int n, const ocx* A, double* evals, ocx* evecs);
+int orc_trlm(const orc_op* A, int n_ev, int n_kr, int max_restarts, double tol, const ocx* start,
+             double* evals, ocx* evecs, int* restarts_out, int* iters_out);
+int orc_cg_deflated(const orc_op* A, const ocx* rhs, orc_mg* mg, int ndefl, const ocx* U, int maxit, double tol,
+                    ocx* x, double* final_res);
+int orc_deflated_coarse_cg(const orc_op* A, int ndefl, const ocx* vecs, const double* vals, const ocx* b,
+                           int maxit, double tol, ocx* x);
+
 /* ---------------- HMC molecular dynamics ---------------- */
 double orc_kinetic_energy(int n, const double* pi);
 int orc_hmc_evolve(int L, ocx* U, double mass, double r, double csw, double mu_t, double beta, double tau,

@@ -69,6 +69,14 @@ def lib():
             getattr(L, name).argtypes = args
         L.ref_hmc_draws.argtypes = [_I, ctypes.c_uint, _P, _I, _P]
         L.ref_hmc_trajectory.argtypes = [_I, _P, _D, _D, _D, _D, _D, _I, _I, _I, _I, _D, ctypes.c_uint, _P]
+        L.ref_lanczos_eigen.argtypes = [_I, _P, _P, _P]
+        L.ref_trlm_dirac.restype = _I
+        L.ref_trlm_dirac.argtypes = [_I, _P, _D, _D, _D, _D, _I, _I, _I, _D, _P, _P]
+        L.ref_cg_deflated.restype = _I
+        L.ref_cg_deflated.argtypes = [_I, _P, _D, _D, _D, _D, _I, _P, _P, _I, _D, _P, _P]
+        L.ref_coarse_deflated_solve.restype = _I
+        L.ref_coarse_deflated_solve.argtypes = [_I, _P, _D, _D, _D, _D, _I, _I, _P, _I, _I, _I, _D, _P, _I, _D,
+                                                _P, _P]
         L.ref_leapfrog_evolve.restype = _I
         L.ref_leapfrog_evolve.argtypes = [_I, _P, _D, _D, _D, _D, _D, _I, _I, _I, _D, _P, _P]
         _lib = L
@@ -253,3 +261,41 @@ def hmc_trajectory(L, U, mass, r, csw, beta, tau, n_steps, omelyan, use_eo, cg_m
     lib().ref_hmc_trajectory(L, _ptr(U), mass, r, csw, beta, tau, n_steps, int(omelyan), int(use_eo), cg_maxit,
                              cg_tol, seed, _ptr(out))
     return U, out
+
+
+def lanczos_eigen(A):
+    """lanczos_eigen of a Hermitian matrix: (ascending eigenvalues, eigenvector columns)."""
+    n = A.shape[0]
+    cols = np.ascontiguousarray(np.asarray(A, np.complex128).T)
+    ev = np.zeros(n)
+    vecs = np.zeros((n, n), np.complex128)
+    lib().ref_lanczos_eigen(n, _ptr(cols), _ptr(ev), _ptr(vecs))
+    return ev, vecs
+
+
+def trlm_dirac(L, U, m, r, csw, mu, n_ev, n_kr, max_restarts, tol):
+    """trlm_eigensolver on D^dag D: (restarts or -1, eigenvalues, eigenvectors [n_ev][2V])."""
+    ev = np.zeros(n_ev)
+    vecs = np.zeros((n_ev, 2 * L * L), np.complex128)
+    rs = lib().ref_trlm_dirac(L, _ptr(_c(U)), m, r, csw, mu, n_ev, n_kr, max_restarts, tol, _ptr(ev), _ptr(vecs))
+    return rs, ev, vecs
+
+
+def coarse_deflated_solve(L, U, m, r, csw, mu, b, k, nulls, n_ev, n_kr, max_restarts, tol, rhs, maxit, cg_tol):
+    """TRLM deflation of the Galerkin coarse operator, then its deflated CG solve:
+    (eigenvalues, x)."""
+    ev = np.zeros(n_ev)
+    x = np.zeros(len(rhs), np.complex128)
+    nd = lib().ref_coarse_deflated_solve(L, _ptr(_c(U)), m, r, csw, mu, b, k, _ptr(_c(nulls)), n_ev, n_kr,
+                                         max_restarts, tol, _ptr(_c(rhs)), maxit, cg_tol, _ptr(ev), _ptr(x))
+    return ev[:nd], x
+
+
+def cg_deflated(L, U, m, r, csw, mu, vecs, rhs, maxit, tol):
+    """cg_solve_deflated on D^dag D, identity preconditioner: (x, iterations, final residual)."""
+    vecs = np.ascontiguousarray(vecs, np.complex128)
+    x = np.zeros(L * L * 2, np.complex128)
+    res = np.zeros(1)
+    it = lib().ref_cg_deflated(L, _ptr(_c(U)), m, r, csw, mu, vecs.shape[0], _ptr(vecs), _ptr(_c(rhs)), maxit, tol,
+                               _ptr(x), _ptr(res))
+    return x, it, res[0]

@@ -452,3 +452,64 @@ API void ref_hmc_trajectory(int L, double* U, double mass, double r, double csw,
     out6[4] = res.dLD;
     out6[5] = res.total_cg_iters;
 }
+
+// lanczos_eigen (src/eigensolver_core.cpp:103-265): A column-major (A[j*n + i]);
+// evecs[row*n + col]
+API void ref_lanczos_eigen(int n, const double* A, double* evals, double* evecs) {
+    std::vector<Vec> cols(n);
+    for (int j = 0; j < n; j++) cols[j] = to_vec(A + 2 * (size_t)j * n, n);
+    RVec ev;
+    std::vector<Vec> vecs;
+    lanczos_eigen(cols, n, ev, vecs);
+    std::memcpy(evals, ev.data(), n * sizeof(double));
+    for (int row = 0; row < n; row++) from_vec(vecs[row], evecs + 2 * (size_t)row * n);
+}
+// trlm_eigensolver on D^dag D (default start vector: random_vec(n, mt19937(42)))
+API int ref_trlm_dirac(int L, const double* U, double m, double r, double csw, double mu, int n_ev, int n_kr,
+                       int max_restarts, double tol, double* evals, double* evecs) {
+    Ctx c(L, U, m, r, csw, mu);
+    TRLMResult res = trlm_eigensolver(c.D.as_DdagD_op(), c.lat.ndof, n_ev, n_kr, max_restarts, tol);
+    for (int i = 0; i < n_ev; i++) {
+        evals[i] = res.eigvals[i];
+        from_vec(res.eigvecs[i], evecs + 2 * (size_t)i * c.lat.ndof);
+    }
+    return res.converged ? res.num_restarts : -1;
+}
+// SparseCoarseOp::setup_deflation (TRLM) then solve (src/coarse_op.cpp:10-54,
+// src/coarse_op.h:492-547) on the Galerkin operator of b x b aggregates with k vectors
+API int ref_coarse_deflated_solve(int L, const double* U, double m, double r, double csw, double mu, int b, int k,
+                                  const double* nulls, int n_ev, int n_kr, int max_restarts, double tol,
+                                  const double* rhs, int maxit, double cg_tol, double* evals, double* x) {
+    Ctx c(L, U, m, r, csw, mu);
+    Prolongator P(c.lat, b, b, k);
+    std::vector<Vec> nv(k);
+    for (int i = 0; i < k; i++) nv[i] = to_vec(nulls + 2 * (size_t)i * c.lat.ndof, c.lat.ndof);
+    P.build_from_vectors(nv);
+    SparseCoarseOp S;
+    S.build(P, c.D.as_DdagD_op(), c.lat.ndof);
+    // the TRLM branch of setup_deflation (src/coarse_op.cpp:39-52); the function itself
+    // also holds the FEAST branch, which needs the Fortran library this image lacks
+    if (n_kr <= 0) n_kr = std::min(2 * n_ev + 10, S.dim);
+    if (n_kr > S.dim) n_kr = S.dim;
+    if (n_kr < n_ev + 2) n_kr = std::min(n_ev + 6, S.dim);
+    TRLMResult tr = trlm_eigensolver(S.as_op(), S.dim, n_ev, n_kr, max_restarts, tol);
+    S.defl_vecs = std::move(tr.eigvecs);
+    S.defl_vals = std::move(tr.eigvals);
+    for (int i = 0; i < (int)S.defl_vals.size(); i++) evals[i] = S.defl_vals[i];
+    from_vec(S.solve(to_vec(rhs, S.dim), maxit, cg_tol), x);
+    return (int)S.defl_vals.size();
+}
+// cg_solve_deflated (src/solvers.cpp:111-213) on D^dag D with the identity
+// preconditioner and the ndefl deflation vectors vecs[ndefl][ndof]
+API int ref_cg_deflated(int L, const double* U, double m, double r, double csw, double mu, int ndefl,
+                        const double* vecs, const double* rhs, int maxit, double tol, double* x, double* res) {
+    Ctx c(L, U, m, r, csw, mu);
+    std::vector<Vec> dv(ndefl);
+    for (int i = 0; i < ndefl; i++) dv[i] = to_vec(vecs + 2 * (size_t)i * c.lat.ndof, c.lat.ndof);
+    std::vector<double> dl(ndefl, 1.0);   // the eigenvalues are not read by the solver
+    auto ident = [](const Vec& v) { return v; };
+    CGResult o = cg_solve_deflated(c.D.as_DdagD_op(), c.lat.ndof, to_vec(rhs, c.lat.ndof), ident, dv, dl, maxit, tol);
+    from_vec(o.solution, x);
+    *res = o.final_residual;
+    return o.iterations;
+}

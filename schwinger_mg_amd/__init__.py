@@ -111,6 +111,12 @@ SIGNATURES = [
     ("smg_gauge_write", _I, [ctypes.c_char_p, _I, _P, _D, _D]),
     ("smg_gauge_read", _I, [ctypes.c_char_p, _I, _P, _P]),
     ("smg_therm_cfg_path", _I, [_I, _D, _I, ctypes.c_char_p, ctypes.c_size_t]),
+    ("smg_random_vec", _I, [_I, ctypes.c_uint, _P]), ("smg_dense_eigh", _I, [_I, _P, _P, _P]),
+    ("smg_dirac_trlm", _I, [_P, _I, _I, _I, _I, _D, _P, _P, _P, _P]),
+    ("smg_coarse_trlm", _I, [_P, _I, _I, _I, _D, _P, _P, _P, _P]),
+    ("smg_coarse_setup_deflation", _I, [_P, _I, _I, _I, _D, _P, _P]),
+    ("smg_coarse_deflation", _I, [_P, _P, _P, _P]), ("smg_coarse_set_deflation", _I, [_P, _I, _P, _P]),
+    ("smg_cg_solve_deflated", _I, [_P, _I, _P, _P, _I, _P, _P, _I, _D, _P]),
 ]
 
 _lib = None
@@ -433,6 +439,45 @@ class SparseCoarseOp:
     def as_op(self):
         return self.apply_to
 
+    def setup_deflation(self, n_ev, n_kr=0, max_restarts=100, tol=1e-10, solver="trlm", start_vec=None):
+        """SparseCoarseOp::setup_deflation (src/coarse_op.cpp:10-54), TRLM branch: the
+        n_ev smallest eigenpairs of A_c become the deflation space of solve().
+        start_vec defaults to the reference's random_vec(dim, mt19937(42))."""
+        if solver != "trlm":
+            raise SMGError(f"setup_deflation: solver {solver!r} is not provided (only 'trlm')")
+        start = random_vec(self.dim, 42) if start_vec is None else _c(start_vec)
+        info = TRLMInfo()
+        _chk(lib().smg_coarse_setup_deflation(self.h, n_ev, n_kr, max_restarts, tol, _p(start),
+                                              ctypes.byref(info)))
+        self.last_trlm = info
+        return info
+
+    def _deflation(self):
+        nd = _I()
+        _chk(lib().smg_coarse_deflation(self.h, ctypes.byref(nd), None, None))
+        vals = np.zeros(nd.value)
+        vecs = np.zeros((nd.value, self.dim), np.complex128)
+        if nd.value:
+            _chk(lib().smg_coarse_deflation(self.h, ctypes.byref(nd), _p(vals), _p(vecs)))
+        return vals, vecs
+
+    @property
+    def defl_vals(self):
+        """SparseCoarseOp::defl_vals (src/coarse_op.h:170)."""
+        return self._deflation()[0]
+
+    @property
+    def defl_vecs(self):
+        """SparseCoarseOp::defl_vecs (src/coarse_op.h:169), one row per vector."""
+        return self._deflation()[1]
+
+    def set_deflation(self, vals, vecs):
+        """Replace the deflation space (an empty one switches deflation off)."""
+        vals = np.ascontiguousarray(vals, np.float64)
+        vecs = _c(vecs).reshape(len(vals), -1) if len(vals) else np.zeros((0, self.dim), np.complex128)
+        _chk(lib().smg_coarse_set_deflation(self.h, len(vals), _p(vals) if len(vals) else None,
+                                            _p(vecs) if len(vals) else None))
+
 
 # ------------------------------------------------------------------ smoothers
 def mr_smooth(D, x, b, niter):
@@ -612,6 +657,76 @@ def fgmres_solve(D, rhs, mg, restart, max_iter, tol):
     _chk(lib().smg_fgmres_solve(D.h, mg.h if mg else None, _p(_c(rhs)), _p(x), restart, max_iter, tol,
                                 ctypes.byref(info)))
     return _res(x, info)
+
+
+def cg_solve_deflated(D, rhs, mg, defl_vecs, defl_vals, max_iter, tol, op=OP_DDAGD):
+    """cg_solve_deflated (src/solvers.cpp:111-213): CG started in the deflation space
+    U (rows of defl_vecs) with the projected preconditioned residual; mg = the FP64
+    cycle as preconditioner or None.  defl_vals is accepted as in the reference and,
+    as there, not read."""
+    n = D.ndof if op == OP_DDAGD else D.V
+    U = _c(defl_vecs).reshape(-1, n) if len(defl_vecs) else np.zeros((0, n), np.complex128)
+    x = np.zeros(n, np.complex128)
+    info = SolveInfo()
+    _chk(lib().smg_cg_solve_deflated(D.h, op, mg.h if mg else None, _p(_c(rhs)), U.shape[0],
+                                     _p(U) if U.shape[0] else None, _p(x), max_iter, tol, ctypes.byref(info)))
+    return _res(x, info)
+
+
+# ------------------------------------------------------------------ eigensolvers
+class TRLMInfo(ctypes.Structure):
+    _fields_ = [("iterations", _I), ("restarts", _I), ("converged", _I)]
+
+
+@dataclass
+class TRLMResult:
+    """TRLMResult (src/eigensolver.h:19-25)."""
+    eigvecs: np.ndarray   # (n_ev, n): eigvecs[i] is eigenvector i
+    eigvals: np.ndarray
+    iterations: int
+    num_restarts: int
+    converged: bool
+
+
+def random_vec(n, seed):
+    """random_vec(n, std::mt19937(seed)) (src/linalg.cpp:44-49), drawn as the
+    reference's g++ build draws it."""
+    out = np.zeros(n, np.complex128)
+    _chk(load_library().smg_random_vec(n, seed, _p(out)))
+    return out
+
+
+def lanczos_eigen(A_cols):
+    """lanczos_eigen (src/eigensolver_core.cpp:103-265) on the Hermitian matrix given
+    as its columns, A_cols[j][i] = A_ij: (ascending eigenvalues, evecs) with evecs[c]
+    the eigenvector of eigenvalue c, as the reference's evecs[c]."""
+    cols = _c(A_cols)
+    n = cols.shape[0]
+    ev = np.zeros(n)
+    out = np.zeros((n, n), np.complex128)   # out[row, col]
+    if n:
+        _chk(load_library().smg_dense_eigh(n, _p(cols), _p(ev), _p(out)))
+    return ev, np.ascontiguousarray(out.T)
+
+
+def trlm_eigensolver(A, n_ev, n_kr=0, max_restarts=100, tol=1e-10, start_vec=None, op=OP_DDAGD):
+    """trlm_eigensolver (src/eigensolver_core.cpp:691-1057) without the Chebyshev
+    filter.  A = a DiracOp (the operator D^dag D for op OP_DDAGD, the reference's
+    D.as_DdagD_op(), or M^dag M for OP_SCHUR) or a SparseCoarseOp (its as_op()); start_vec
+    defaults to random_vec(n, 42) as in the reference."""
+    coarse = isinstance(A, SparseCoarseOp)
+    n = A.dim if coarse else (A.ndof if op == OP_DDAGD else A.V)
+    start = random_vec(n, 42) if start_vec is None else _c(start_vec)
+    ev = np.zeros(n_ev)
+    vecs = np.zeros((n_ev, n), np.complex128)
+    info = TRLMInfo()
+    if coarse:
+        _chk(lib().smg_coarse_trlm(A.h, n_ev, n_kr, max_restarts, tol, _p(start), _p(ev), _p(vecs),
+                                   ctypes.byref(info)))
+    else:
+        _chk(lib().smg_dirac_trlm(A.h, op, n_ev, n_kr, max_restarts, tol, _p(start), _p(ev), _p(vecs),
+                                  ctypes.byref(info)))
+    return TRLMResult(vecs, ev, info.iterations, info.restarts, bool(info.converged))
 
 
 # ------------------------------------------------------------------ forces

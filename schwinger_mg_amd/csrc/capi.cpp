@@ -5,9 +5,11 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <random>
 #include <string>
 
 #include "../../include/smg.h"
+#include "eigen.h"
 #include "runtime.h"
 #include "runtime_b.h"
 
@@ -25,6 +27,7 @@ struct smg_prolongator_s {
 };
 struct smg_coarse_s {
     std::unique_ptr<CoarseOperator> C;
+    Eigenpairs defl;   // SparseCoarseOp::defl_vecs / defl_vals (empty: no deflation)
 };
 struct smg_mg_s {
     std::unique_ptr<MG> mg;
@@ -470,6 +473,14 @@ int smg_coarse_solve(smg_coarse* C, const double* b, double* x, int maxit, doubl
     return guard([&] {
         SMG_REQUIRE(C, "null handle");
         const size_t n = C->C->n();
+        if (!C->defl.vals.empty()) {   // deflated initial guess, then the same CG
+            Field<double> bb = up(b, n), xx(n);
+            CoarseLinOp<double, c64> op(*C->C);
+            const int it = deflated_coarse_cg(op, C->defl, bb.p(), xx.p(), maxit, tol, S());
+            if (iters) *iters = it;
+            downc(x, xx.p(), n);
+            return;
+        }
         Field<double> bb = up(b, n), xx(n), r(n), p(n), Ap(n);
         DevMem sc(64 * sizeof(double));
         SMG_HIP(hipMemsetAsync(sc.p, 0, 64 * sizeof(double), S()));
@@ -1248,5 +1259,125 @@ int smg_therm_cfg_path(int L, double beta, int n_therm, char* out, size_t cap) {
                               std::to_string(n_therm) + ".bin";
         SMG_REQUIRE(p.size() + 1 <= cap, "output buffer too small");
         std::memcpy(out, p.c_str(), p.size() + 1);
+    });
+}
+
+// ---------------------------------------------------------------- eigensolvers and deflation
+int smg_random_vec(int n, unsigned seed, double* out) {
+    return guard([&] {
+        SMG_REQUIRE(n >= 0 && (n == 0 || out), "bad arguments");
+        std::mt19937 rng(seed);
+        std::normal_distribution<double> dist(0.0, 1.0);
+        for (int i = 0; i < n; i++) {
+            // the reference's cx(dist(rng), dist(rng)) as g++ evaluates it: the
+            // imaginary part's draw first
+            const double im = dist(rng);
+            const double re = dist(rng);
+            out[2 * i] = re;
+            out[2 * i + 1] = im;
+        }
+    });
+}
+int smg_dense_eigh(int n, const double* A, double* evals, double* evecs) {
+    return guard([&] {
+        SMG_REQUIRE(n >= 0 && (n == 0 || (A && evals && evecs)), "bad arguments");
+        dense_eigh(n, reinterpret_cast<const c64*>(A), evals, reinterpret_cast<c64*>(evecs));
+    });
+}
+static void trlm_out(const Eigenpairs& E, size_t n, double* evals, double* evecs, smg_trlm_info* info) {
+    const int ne = (int)E.vals.size();
+    if (evals) std::copy(E.vals.begin(), E.vals.end(), evals);
+    if (evecs && ne) downc(evecs, E.vecs.p(), (size_t)ne * n);
+    if (info) {
+        info->iterations = E.iterations;
+        info->restarts = E.restarts;
+        info->converged = E.converged ? 1 : 0;
+    }
+}
+int smg_dirac_trlm(smg_dirac* D, int op, int n_ev, int n_kr, int max_restarts, double tol, const double* start,
+                   double* evals, double* evecs, smg_trlm_info* info) {
+    return guard([&] {
+        SMG_REQUIRE(D && start, "bad arguments");
+        LinOp<double>* A;
+        if (op == SMG_OP_DDAGD) A = D->op64.get();
+        else {
+            SMG_REQUIRE(op == SMG_OP_SCHUR && D->schur64, "unknown operator (SMG_OP_SCHUR needs an even L)");
+            A = D->schur64.get();
+        }
+        const size_t n = A->n();
+        SMG_REQUIRE(n_ev >= 1 && (size_t)n_ev < n, "need 1 <= n_ev < dimension");
+        Field<double> s0 = up(start, n);
+        Eigenpairs E = trlm(*A, n_ev, n_kr, max_restarts, tol, s0.p(), S());
+        trlm_out(E, n, evals, evecs, info);
+    });
+}
+int smg_coarse_trlm(smg_coarse* C, int n_ev, int n_kr, int max_restarts, double tol, const double* start,
+                    double* evals, double* evecs, smg_trlm_info* info) {
+    return guard([&] {
+        SMG_REQUIRE(C && start, "bad arguments");
+        const size_t n = C->C->n();
+        SMG_REQUIRE(n_ev >= 1 && (size_t)n_ev < n, "need 1 <= n_ev < dimension");
+        Field<double> s0 = up(start, n);
+        CoarseLinOp<double, c64> op(*C->C);
+        Eigenpairs E = trlm(op, n_ev, n_kr, max_restarts, tol, s0.p(), S());
+        trlm_out(E, n, evals, evecs, info);
+    });
+}
+int smg_coarse_setup_deflation(smg_coarse* C, int n_ev, int n_kr, int max_restarts, double tol, const double* start,
+                               smg_trlm_info* info) {
+    return guard([&] {
+        SMG_REQUIRE(C && start, "bad arguments");
+        const int dim = (int)C->C->n();
+        if (dim <= 0 || n_ev <= 0) return;
+        if (n_kr <= 0) n_kr = std::min(2 * n_ev + 10, dim);
+        if (n_kr > dim) n_kr = dim;
+        if (n_kr < n_ev + 2) n_kr = std::min(n_ev + 6, dim);
+        SMG_REQUIRE(n_ev < dim, "need n_ev < coarse dimension");
+        Field<double> s0 = up(start, (size_t)dim);
+        CoarseLinOp<double, c64> op(*C->C);
+        C->defl = trlm(op, n_ev, n_kr, max_restarts, tol, s0.p(), S());
+        trlm_out(C->defl, (size_t)dim, nullptr, nullptr, info);
+    });
+}
+int smg_coarse_deflation(smg_coarse* C, int* ndefl, double* evals, double* evecs) {
+    return guard([&] {
+        SMG_REQUIRE(C, "null handle");
+        if (ndefl) *ndefl = (int)C->defl.vals.size();
+        trlm_out(C->defl, C->C->n(), evals, evecs, nullptr);
+    });
+}
+int smg_coarse_set_deflation(smg_coarse* C, int ndefl, const double* evals, const double* evecs) {
+    return guard([&] {
+        SMG_REQUIRE(C && ndefl >= 0 && (ndefl == 0 || (evals && evecs)), "bad arguments");
+        const size_t n = C->C->n();
+        Eigenpairs E;
+        if (ndefl > 0) {
+            E.vals.assign(evals, evals + ndefl);
+            E.vecs = up(evecs, (size_t)ndefl * n);
+            Context::get().sync();
+        }
+        E.converged = true;
+        C->defl = std::move(E);
+    });
+}
+int smg_cg_solve_deflated(smg_dirac* D, int op, smg_mg* mg, const double* b, int ndefl, const double* defl_vecs,
+                          double* x, int maxit, double tol, smg_solve_info* info) {
+    return guard([&] {
+        SMG_REQUIRE(D && ndefl >= 0 && (ndefl == 0 || defl_vecs), "bad arguments");
+        LinOp<double>* A;
+        if (op == SMG_OP_DDAGD) A = D->op64.get();
+        else {
+            SMG_REQUIRE(op == SMG_OP_SCHUR && D->schur64, "unknown operator (SMG_OP_SCHUR needs an even L)");
+            SMG_REQUIRE(mg == nullptr, "the multigrid preconditioner acts on D^dag D only");
+            A = D->schur64.get();
+        }
+        const size_t n = A->n();
+        Field<double> bb = up(b, n), xx(n), uu;
+        if (ndefl) uu = up(defl_vecs, (size_t)ndefl * n);
+        SolveResult r = cg_deflated(*A, mg ? mg->mg.get() : nullptr, ndefl ? uu.p() : nullptr, ndefl, bb.p(), xx.p(),
+                                    maxit, tol, S());
+        r.true_residual = true_residual(*A, bb.p(), xx.p(), S());
+        fill_info(info, r);
+        downc(x, xx.p(), n);
     });
 }

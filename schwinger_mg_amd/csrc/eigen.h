@@ -1,0 +1,41 @@
+// eigen.h -- eigensolvers and deflation: the dense Hermitian eigensolver of the
+// small Lanczos problems (host), thick-restart Lanczos on a device operator, and
+// the deflated solves built on its eigenpairs.
+#pragma once
+#include <vector>
+
+#include "runtime.h"
+
+namespace smg {
+
+// lanczos_eigen (src/eigensolver_core.cpp:103-265) for a small dense Hermitian
+// matrix on the host: A column-major (A[j*n + i]); ascending evals; evecs[row*n + col]
+void dense_eigh(int n, const c64* A, double* evals, c64* evecs);
+
+// trlm_eigensolver without the Chebyshev filter (src/eigensolver_core.cpp:691-1057):
+// the n_ev smallest eigenpairs of the Hermitian operator A.  start: the device start
+// vector (normalised inside).  The Krylov vectors stay on the device; the full
+// reorthogonalisation is two passes of classical Gram-Schmidt (batched dots and one
+// linear combination per pass) in place of the reference's sequential modified
+// Gram-Schmidt, equal in exact arithmetic.
+struct Eigenpairs {
+    std::vector<double> vals;
+    Field<double> vecs;   // n_ev vectors of A.n(), one after the other
+    int restarts = 0;
+    int iterations = 0;   // operator applications
+    bool converged = false;
+};
+Eigenpairs trlm(LinOp<double>& A, int n_ev, int n_kr, int max_restarts, double tol, const c64* start,
+                hipStream_t st);
+
+// SparseCoarseOp::solve with deflation vectors (src/coarse_op.h:492-547): x0 =
+// sum_i (v_i^dag b / lambda_i) v_i, then CG; returns the iterations
+int deflated_coarse_cg(LinOp<double>& A, const Eigenpairs& E, const c64* b, c64* x, int maxit, double tol,
+                       hipStream_t st);
+
+// cg_solve_deflated (src/solvers.cpp:111-213): projected (preconditioned) CG with the
+// deflation space U (ndefl device vectors), E = U^dag A U inverted on the host
+SolveResult cg_deflated(LinOp<double>& A, MG* mg, const c64* U, int ndefl, const c64* b, c64* x, int maxit,
+                        double tol, hipStream_t st);
+
+}  // namespace smg

@@ -37,6 +37,7 @@ struct Field {
     Field() = default;
     explicit Field(size_t n_) : m(n_ * sizeof(cplx<T>)), n(n_) {}
     cplx<T>* p() const { return m.as<cplx<T>>(); }
+    cplx<T>* p(size_t i) const { return m.as<cplx<T>>() + i; }
 };
 
 // Per-process execution context: one stream, reduction partials, scalars.

@@ -98,6 +98,10 @@ struct VecPtrs {
 template <typename T>
 void launch_multi_dot(const VecPtrs<T>& V, int nv, const cplx<T>* w, size_t n, int swap, const Fin& fin,
                       hipStream_t st);
+// y = sum c_i V_i (mode 0), y += sum c_i V_i (1), y -= sum c_i V_i (2); nv <= 32 vectors,
+// complex coefficients c_i at coef[2i], coef[2i+1] in device memory (eigen.hip)
+template <typename T>
+void launch_lincomb(cplx<T>* y, const VecPtrs<T>& V, int nv, const double* coef, int mode, size_t n, hipStream_t st);
 // GCR orthogonalisation: beta_i = dots[i]/apap[i]; p -= sum beta_i P_i; Ap -= sum beta_i AP_i
 template <typename T>
 void launch_gcr_orth(cplx<T>* p, cplx<T>* Ap, const VecPtrs<T>& P, const VecPtrs<T>& AP, int nv,

@@ -4,6 +4,7 @@
 // through libsmg.so.  Every result is checked against the oracle (liboracle.so,
 // the CPU restatement) on identical inputs; the program prints one line per check
 // and exits non-zero on the first failure.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -52,6 +53,11 @@ int orc_hmc_evolve_eo(int L, cx* U, double mass, double r, double csw, double mu
                       int n_steps, int cg_maxit, double cg_tol, double* pi, const cx* phi_o);
 int orc_hmc_hamiltonian_eo(int L, const cx* U, double mass, double r, double csw, double mu_t, double beta,
                            int cg_maxit, double cg_tol, const double* pi, const cx* phi_o, double* out);
+void orc_lanczos_eigen(int n, const cx* A, double* evals, cx* evecs);
+int orc_trlm(const orc_op* A, int n_ev, int n_kr, int max_restarts, double tol, const cx* start, double* evals,
+             cx* evecs, int* restarts_out, int* iters_out);
+int orc_cg_deflated(const orc_op* A, const cx* rhs, orc_mg* mg, int ndefl, const cx* U, int maxit, double tol, cx* x,
+                    double* final_res);
 }
 
 static int failures = 0;
@@ -261,6 +267,85 @@ int main() {
         report("verify_forces (full, clover)", verify_forces(gh, 2.0, 0.1, 1.0, 2000, 1e-12, 1.0) ? 0.0 : 1.0, 0.0);
         report("verify_forces (even-odd, clover)",
                verify_forces(gh, 2.0, 0.1, 1.0, 2000, 1e-12, 1.0, true) ? 0.0 : 1.0, 0.0);
+    }
+
+    // eigensolvers and deflation (src/eigensolver.h, src/coarse_op.h, src/solvers.h)
+    {
+        std::mt19937 r1(12345);
+        const Vec rv = random_vec(64, r1);
+        Vec rl(64);
+        smg_random_vec(64, 12345, reinterpret_cast<double*>(rl.data()));
+        report("random_vec (the g++ draw order)", relerr(rv, rl), 0.0);
+        const int nh = 10;
+        std::vector<Vec> Hc(nh, Vec(nh));
+        std::mt19937 rh(7);
+        std::normal_distribution<double> nd(0.0, 1.0);
+        for (int j = 0; j < nh; j++)
+            for (int i = 0; i <= j; i++) {
+                const cx z = i == j ? cx(nd(rh), 0.0) : cx(nd(rh), nd(rh));
+                Hc[j][i] = z;
+                Hc[i][j] = std::conj(z);
+            }
+        RVec ev;
+        std::vector<Vec> evec;
+        lanczos_eigen(Hc, nh, ev, evec);
+        Vec flat((size_t)nh * nh), ovec((size_t)nh * nh);
+        for (int j = 0; j < nh; j++)
+            for (int i = 0; i < nh; i++) flat[(size_t)j * nh + i] = Hc[j][i];
+        RVec eo_(nh);
+        orc_lanczos_eigen(nh, flat.data(), eo_.data(), ovec.data());
+        double de = 0;
+        for (int i = 0; i < nh; i++) de = std::max(de, std::abs(ev[i] - eo_[i]));
+        report("lanczos_eigen eigenvalues", de, 1e-13);
+
+        const int n_ev = 6, n = lat.ndof;
+        TRLMResult tr = trlm_eigensolver(A, n, n_ev, 24, 100, 1e-10);
+        std::mt19937 r42(42);
+        const Vec st = random_vec(n, r42);
+        RVec evo(n_ev);
+        Vec vo((size_t)n_ev * n);
+        int rs = 0, its = 0;
+        orc_trlm(Ao, n_ev, 24, 100, 1e-10, st.data(), evo.data(), vo.data(), &rs, &its);
+        de = 0;
+        for (int i = 0; i < n_ev; i++) de = std::max(de, std::abs(tr.eigvals[i] - evo[i]) / evo[n_ev - 1]);
+        report("trlm_eigensolver eigenvalues", de, 1e-10);
+        report("trlm_eigensolver restarts / iterations",
+               (tr.converged && tr.num_restarts == rs && tr.iterations == its) ? 0.0 : 1.0, 0.0);
+
+        Vec U((size_t)n_ev * n);
+        for (int i = 0; i < n_ev; i++) std::copy(tr.eigvecs[i].begin(), tr.eigvecs[i].end(), U.begin() + (size_t)i * n);
+        CGResult cd = cg_solve_deflated(A, n, b, {}, tr.eigvecs, tr.eigvals, 2000, 1e-10);
+        Vec xd(n);
+        double rd = 0;
+        const int itd = orc_cg_deflated(Ao, b.data(), nullptr, n_ev, U.data(), 2000, 1e-10, xd.data(), &rd);
+        report("cg_solve_deflated", relerr(cd.solution, xd), 1e-8);
+        report("cg_solve_deflated iterations", std::abs(cd.iterations - itd), 1.0);
+        // mg holds the symmetric cycle here (set_symmetric above), which the oracle's
+        // hierarchy does not: the same solution, in fewer iterations
+        CGResult cm = cg_solve_deflated(A, n, b, mg.as_precond(), tr.eigvecs, tr.eigvals, 200, 1e-10);
+        report("cg_solve_deflated (symmetric MG) solution", relerr(cm.solution, xd), 1e-8);
+        report("cg_solve_deflated (symmetric MG) fewer iterations", cm.iterations < cd.iterations ? 0.0 : 1.0, 0.0);
+
+        // SparseCoarseOp::setup_deflation + solve: x solves A_c x = b_c
+        Prolongator P(lat, 4, 4, 4);
+        std::vector<Vec> nv;
+        for (int i = 0; i < 4; i++) nv.push_back(random_vec(lat.ndof, rng));
+        P.build_from_vectors(nv);
+        SparseCoarseOp S;
+        S.build(P, A, lat.ndof);
+        S.setup_deflation(8);
+        const Vec bc = random_vec(S.dim, rng);
+        const Vec xc = S.solve(bc, 500, 1e-12);
+        Vec Axc;
+        S.apply_to(xc, Axc);
+        report("SparseCoarseOp deflation eigenvalues ascend",
+               (S.defl_vals.size() == 8 && S.defl_vals[0] > 0 && S.defl_vals[0] <= S.defl_vals[7]) ? 0.0 : 1.0, 0.0);
+        report("SparseCoarseOp::solve (deflated) residual", relerr(Axc, bc), 1e-11);
+        TRLMResult tc = trlm_eigensolver(S.as_op(), S.dim, 8, 0, 100, 1e-10);
+        de = 0;
+        for (int i = 0; i < 8; i++) de = std::max(de, std::abs(tc.eigvals[i] - S.defl_vals[i]));
+        report("trlm_eigensolver on SparseCoarseOp::as_op", de, 1e-9 * S.defl_vals[7]);
+        report("SparseCoarseOp::validate_deflation", S.validate_deflation() == 8 ? 0.0 : 1.0, 0.0);
     }
 
     orc_mg_free(mgo);

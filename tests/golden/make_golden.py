@@ -109,6 +109,30 @@ def main():
         Uf, res = R.hmc_trajectory(L, U, 0.1, 1.0, 1.0, 2.0, 0.5, 3, 0, eo, 500, 1e-12, 55)
         out[f"L6_traj{eo}_pi"], out[f"L6_traj{eo}_eta"] = pi_d, eta_d
         out[f"L6_traj{eo}_U"], out[f"L6_traj{eo}_out"] = Uf, res
+    # eigensolvers: lanczos_eigen on a random and on a degenerate Hermitian matrix,
+    # trlm_eigensolver on D^dag D (its default start vector), cg_solve_deflated with
+    # those vectors, SparseCoarseOp::setup_deflation (TRLM) + solve at L = 16
+    rng = np.random.default_rng(61)
+    H = rng.standard_normal((12, 12)) + 1j * rng.standard_normal((12, 12))
+    H = H + H.conj().T
+    Q, _ = np.linalg.qr(rng.standard_normal((8, 8)) + 1j * rng.standard_normal((8, 8)))
+    Hd = Q @ np.diag([1.0, 1.0, 2.0, 2.0, 3.0, 3.0, 4.0, 4.0]) @ Q.conj().T
+    for name, M in (("H12", H), ("H8deg", Hd)):
+        ev, vecs = R.lanczos_eigen(M)
+        out[f"{name}"], out[f"{name}_evals"], out[f"{name}_evecs"] = M, ev, vecs
+    out["rv42_128"], out["rv42_64"] = R.random_vec(128, 42), R.random_vec(64, 42)
+    L = 8
+    U = random_gauge(L, 0.3, seed=62)
+    out["E8_U"] = U
+    for tag, (ne, nk) in (("a", (6, 24)), ("b", (4, 0))):
+        rs, ev, vecs = R.trlm_dirac(L, U, 0.05, 1.0, 1.0, 0.0, ne, nk, 100, 1e-10)
+        out[f"E8_trlm{tag}_rs"], out[f"E8_trlm{tag}_evals"], out[f"E8_trlm{tag}_evecs"] = np.array([rs]), ev, vecs
+    rhs = gaussian_vec(2 * L * L, seed=63)
+    x, it, res = R.cg_deflated(L, U, 0.05, 1.0, 1.0, 0.0, out["E8_trlma_evecs"], rhs, 500, 1e-10)
+    out["E8_rhs"], out["E8_cgdefl_x"], out["E8_cgdefl_it"] = rhs, x, np.array([it, res])
+    ev, x = R.coarse_deflated_solve(16, out["L16_U"], 0.05, 1.0, 1.0, 0.0, 4, 4, out["L16_nulls"].ravel(), 8, 0,
+                                    100, 1e-10, out["L16_c"], 200, 1e-12)
+    out["L16_defl_evals"], out["L16_defl_x"] = ev, x
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_vectors.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes,", len(out), "arrays")

@@ -398,3 +398,84 @@ def test_hmc_md_reversible_and_second_order(omelyan):
         Uf, pf, _ = O.hmc_evolve(L, U, *args, 0.5, n, omelyan, 1000, 1e-14, pi, phi)
         dH.append(abs(O.hmc_hamiltonian(L, Uf, *args, 1000, 1e-14, pf, phi).sum() - h0))
     assert 3.0 < dH[0] / dH[1] < 5.0
+
+
+# ------------------------------------------------------------------ eigensolvers and deflation
+def _phase_aligned(a, b):
+    """b scaled by the unit phase that best aligns it with a (eigenvectors are fixed
+    up to a phase only)."""
+    ov = np.vdot(b, a)
+    return b * (ov / abs(ov))
+
+
+def test_golden_lanczos_eigen(g):
+    """lanczos_eigen (src/eigensolver_core.cpp:103-265) on a random Hermitian matrix
+    and on a degenerate one whose Krylov space from e_0 closes early (the invariant-
+    subspace branch): eigenvalues against the reference and against numpy's eigh,
+    eigenvectors against the reference's to the last bits."""
+    for name in ("H12", "H8deg"):
+        A = g[name]
+        ev, vecs = O.lanczos_eigen(A)
+        assert np.abs(ev - g[f"{name}_evals"]).max() < 1e-13
+        assert np.abs(ev - np.linalg.eigvalsh(A)).max() < 1e-12
+        ref = g[f"{name}_evecs"]
+        if name == "H12":
+            assert np.abs(vecs - ref).max() < 1e-12
+        else:   # a degenerate eigenspace fixes only its projector
+            for c in range(0, 8, 2):
+                Pv, Pr = vecs[:, c:c + 2], ref[:, c:c + 2]
+                assert np.abs(Pv @ Pv.conj().T - Pr @ Pr.conj().T).max() < 1e-12
+        assert np.abs(A @ vecs - vecs * ev).max() < 1e-12
+        assert np.abs(vecs.conj().T @ vecs - np.eye(len(ev))).max() < 1e-12
+
+
+@pytest.mark.parametrize("tag,n_ev,n_kr", [("a", 6, 24), ("b", 4, 0)])
+def test_golden_trlm(g, tag, n_ev, n_kr):
+    """trlm_eigensolver on D^dag D from the reference's default start vector
+    (random_vec(n, mt19937(42)), stored with the fixtures): the same
+    restart count, eigenvalues and (phase-aligned) eigenvectors; the eigenvalues are the
+    n_ev smallest of the dense operator; n_kr = 0 takes the default Krylov size."""
+    L = 8
+    D = O.Dirac(L, g["E8_U"], 0.05, 1.0, 1.0, 0.0)
+    conv, rs, ev, vecs, _ = O.trlm(D.op(), n_ev, n_kr, 100, 1e-10, g["rv42_128"])
+    assert conv and rs == int(g[f"E8_trlm{tag}_rs"][0])
+    assert np.abs(ev - g[f"E8_trlm{tag}_evals"]).max() < 1e-12 * ev.max()
+    for i in range(n_ev):
+        ref = g[f"E8_trlm{tag}_evecs"][i]
+        assert np.linalg.norm(_phase_aligned(ref, vecs[i]) - ref) < 1e-8
+    M = O.dense_galerkin(D.op())
+    assert np.allclose(ev, np.linalg.eigvalsh(M)[:n_ev], rtol=1e-9, atol=0)
+
+
+def test_golden_cg_deflated(g):
+    """cg_solve_deflated with the TRLM vectors, identity preconditioner: the same
+    iterations and solution; the solution solves D^dag D x = b."""
+    L = 8
+    D = O.Dirac(L, g["E8_U"], 0.05, 1.0, 1.0, 0.0)
+    x, it, res = O.cg_deflated(D.op(), g["E8_rhs"], g["E8_trlma_evecs"], 500, 1e-10)
+    assert it == int(g["E8_cgdefl_it"][0]) and abs(res - g["E8_cgdefl_it"][1]) < 1e-14
+    assert np.abs(x - g["E8_cgdefl_x"]).max() < 1e-12 * np.abs(x).max()
+    _, it0, _ = O.cg_deflated(D.op(), g["E8_rhs"], g["E8_trlma_evecs"][:0], 500, 1e-10)
+    xcg, itcg, _ = O.cg(D.op(), g["E8_rhs"], 500, 1e-10)
+    assert it0 == itcg and it < it0   # no deflation space: plain CG; deflation saves iterations
+    assert np.linalg.norm(D.op()(x) - g["E8_rhs"]) < 1e-9 * np.linalg.norm(g["E8_rhs"])
+
+
+def test_golden_coarse_deflation(g):
+    """SparseCoarseOp::setup_deflation (TRLM) + solve on the L = 16 Galerkin operator:
+    the eigenvalues are the smallest of the dense A_c and the reference's; the solve
+    is the dense solution."""
+    L, b, k = 16, 4, 4
+    P = O.Prolongator(L, L, 2, b, b, k).build(g["L16_nulls"].ravel())
+    D = O.Dirac(L, g["L16_U"], 0.05, 1.0, 1.0, 0.0)
+    C = O.Coarse(D.op(), P, 1)
+    A = C.op()
+    n = A.n
+    n_ev = 8
+    n_kr = min(2 * n_ev + 10, n)
+    conv, rs, ev, vecs, _ = O.trlm(A, n_ev, n_kr, 100, 1e-10, g["rv42_64"])
+    assert conv and np.abs(ev - g["L16_defl_evals"]).max() < 1e-12 * ev.max()
+    assert np.allclose(ev, np.linalg.eigvalsh(O.dense_galerkin(A))[:n_ev], rtol=1e-9, atol=0)
+    x, it = O.deflated_coarse_cg(A, vecs, ev, g["L16_c"], 200, 1e-12)
+    assert np.abs(x - g["L16_defl_x"]).max() < 1e-11 * np.abs(x).max()
+    assert np.abs(x - g["L16_dense_solve"]).max() < 1e-9 * np.abs(x).max()
