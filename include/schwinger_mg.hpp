@@ -519,6 +519,22 @@ struct MGHierarchy {
         detail::check(smg_mg_set_symmetric(h_.get(), damping, detail::raw(starts)));
     }
     std::function<Vec(const Vec&)> as_precond() { return MGPrecond{this}; }
+    // src/multigrid.cpp:111-141; the coarsest level's space (the reference's
+    // levels[last].defl_vecs / defl_vals) is copied back into the two members below
+    void update_coarsest_deflation(int n_defl, int lobpcg_iters = 3) {
+        detail::check(smg_mg_update_coarsest_deflation(h_.get(), n_defl, lobpcg_iters));
+        int nd = 0;
+        detail::check(smg_mg_coarsest_deflation(h_.get(), &nd, nullptr, nullptr));
+        const int dim = dims.back();
+        coarsest_defl_vals.assign(nd, 0.0);
+        Vec all((size_t)nd * dim);
+        if (nd) detail::check(smg_mg_coarsest_deflation(h_.get(), &nd, coarsest_defl_vals.data(), detail::raw(all)));
+        coarsest_defl_vecs.assign(nd, Vec());
+        for (int i = 0; i < nd; i++)
+            coarsest_defl_vecs[i].assign(all.begin() + (size_t)i * dim, all.begin() + (size_t)(i + 1) * dim);
+    }
+    std::vector<Vec> coarsest_defl_vecs;
+    std::vector<double> coarsest_defl_vals;
     smg_mg* handle() const { return h_.get(); }
     std::shared_ptr<smg_mg> h_;
 };
@@ -712,16 +728,15 @@ struct TRLMResult {   // src/eigensolver.h:19-25
 };
 struct TRLMState;   // saving the Krylov space (state_out) is not provided by this library
 
-// lanczos_eigen, src/eigensolver_core.cpp:103-265: A_cols[j][i] = A_ij; evecs[c] the
-// eigenvector of the c-th (ascending) eigenvalue
+// lanczos_eigen, src/eigensolver_core.cpp:103-265: A_cols[j][i] = A_ij; ascending
+// evals; evecs[row][col] = component row of eigenvector col
 inline void lanczos_eigen(std::vector<Vec>& A_cols, int n, RVec& evals, std::vector<Vec>& evecs) {
     Vec a((size_t)n * n), v((size_t)n * n);
     for (int j = 0; j < n; j++) std::copy(A_cols[j].begin(), A_cols[j].begin() + n, a.begin() + (size_t)j * n);
     evals.assign(n, 0.0);
     detail::check(smg_dense_eigh(n, detail::raw(a), evals.data(), detail::raw(v)));
     evecs.assign(n, Vec(n));
-    for (int c = 0; c < n; c++)
-        for (int r = 0; r < n; r++) evecs[c][r] = v[(size_t)r * n + c];
+    for (int r = 0; r < n; r++) std::copy(v.begin() + (size_t)r * n, v.begin() + (size_t)(r + 1) * n, evecs[r].begin());
 }
 
 // trlm_eigensolver, src/eigensolver_core.cpp:691-1057, on DiracOp::as_DdagD_op() or
@@ -759,6 +774,45 @@ inline TRLMResult trlm_eigensolver(const OpApply& A, int n, int n_ev, int n_kr =
     r.iterations = info.iterations;
     r.num_restarts = info.restarts;
     r.converged = info.converged != 0;
+    return r;
+}
+
+struct LOBPCGResult {   // src/eigensolver.h:6-10
+    std::vector<Vec> eigvecs;
+    std::vector<double> eigvals;
+    int iterations;
+};
+// the identity preconditioner, recognised by lobpcg_update (the reference's callers
+// pass a lambda returning its argument)
+struct IdentityPrecond {
+    Vec operator()(const Vec& v) const { return v; }
+};
+inline std::function<Vec(const Vec&)> identity_precond() { return IdentityPrecond{}; }
+
+// lobpcg_update, src/eigensolver_core.cpp:279-533, on DiracOp::as_DdagD_op() (precond:
+// MGHierarchy::as_precond(), identity_precond() or an empty std::function) or on
+// SparseCoarseOp::as_op() (identity only); k <= 32, run as one block on the device
+inline LOBPCGResult lobpcg_update(const OpApply& A, int n, int k, const std::vector<Vec>& X0,
+                                  const std::function<Vec(const Vec&)>& precond, int max_iter = 5,
+                                  double tol = 1e-8) {
+    const bool ident = !precond || precond.target<IdentityPrecond>() != nullptr;
+    Vec x0((size_t)k * n), x((size_t)k * n);
+    for (int i = 0; i < k; i++) std::copy(X0[i].begin(), X0[i].begin() + n, x0.begin() + (size_t)i * n);
+    LOBPCGResult r{std::vector<Vec>(k), std::vector<double>(k, 0.0), 0};
+    if (const DdagDApply* f = A.target<DdagDApply>()) {
+        if (n != f->D->lat.ndof) throw std::invalid_argument("lobpcg_update: n mismatch");
+        MGHierarchy* mg = ident ? nullptr : detail::mg_of(precond, "lobpcg_update");
+        detail::check(smg_dirac_lobpcg(f->D->handle(), mg ? mg->handle() : nullptr, k, detail::raw(x0), max_iter,
+                                       tol, r.eigvals.data(), detail::raw(x), &r.iterations));
+    } else if (const CoarseApply* c = A.target<CoarseApply>()) {
+        if (n != c->C->dim) throw std::invalid_argument("lobpcg_update: n mismatch");
+        if (!ident) throw std::invalid_argument("lobpcg_update: the coarse operator takes the identity preconditioner");
+        detail::check(smg_coarse_lobpcg(c->C->handle(), k, detail::raw(x0), max_iter, tol, r.eigvals.data(),
+                                        detail::raw(x), &r.iterations));
+    } else {
+        throw std::invalid_argument("lobpcg_update: A must be DiracOp::as_DdagD_op() or SparseCoarseOp::as_op()");
+    }
+    for (int i = 0; i < k; i++) r.eigvecs[i].assign(x.begin() + (size_t)i * n, x.begin() + (size_t)(i + 1) * n);
     return r;
 }
 

@@ -271,6 +271,23 @@ int smg_coarse_setup_deflation(smg_coarse* C, int n_ev, int n_kr, int max_restar
  * evecs[ndefl][n] may be NULL) or replace (ndefl = 0 clears) the deflation space */
 int smg_coarse_deflation(smg_coarse* C, int* ndefl, double* evals, double* evecs);
 int smg_coarse_set_deflation(smg_coarse* C, int ndefl, const double* evals, const double* evecs);
+/* lobpcg_update (src/eigensolver_core.cpp:279-533; declared src/eigensolver.h:39-44)
+ * on D^dag D from the k start vectors X0[k][n] (1 <= k <= 32), preconditioned by the
+ * FP64 MG cycle (mg) or not (mg = NULL, the reference's identity): evals[k] (the final
+ * Rayleigh quotients, as the reference returns them) and X[k][n].  The k vectors run
+ * as one block of multi-right-hand-side launches. */
+int smg_dirac_lobpcg(smg_dirac* D, smg_mg* mg, int k, const double* X0, int max_iter, double tol, double* evals,
+                     double* X, int* iterations);
+/* lobpcg_update on the coarse operator with the identity preconditioner */
+int smg_coarse_lobpcg(smg_coarse* C, int k, const double* X0, int max_iter, double tol, double* evals, double* X,
+                      int* iterations);
+/* MGHierarchy::update_coarsest_deflation (src/multigrid.cpp:111-141): lobpcg_iters
+ * LOBPCG iterations (tol 1e-10) on the coarsest operator, warm-started from the
+ * stored space when it holds n_defl vectors, else from random_vec(dim) drawn n_defl
+ * times from mt19937(42); n_defl <= 32.  smg_mg_coarsest_deflation reads the space
+ * (evals[ndefl], evecs[ndefl][dim]; either may be NULL). */
+int smg_mg_update_coarsest_deflation(smg_mg* mg, int n_defl, int lobpcg_iters);
+int smg_mg_coarsest_deflation(smg_mg* mg, int* ndefl, double* evals, double* evecs);
 /* cg_solve_deflated (src/solvers.cpp:111-213; declared src/solvers.h:50-56):
  * CG from x0 = U E^-1 U^dag b, E = U^dag A U, with the preconditioned residual
  * projected, z = M^-1 r - U E^-1 (AU)^dag M^-1 r; M = the FP64 MG cycle or none

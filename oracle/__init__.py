@@ -60,6 +60,8 @@ def lib():
         L.orc_hmc_hamiltonian.restype = _I
         L.orc_hmc_hamiltonian.argtypes = [_I, _P, _D, _D, _D, _D, _D, _I, _D, _P, _P, _P]
         L.orc_lanczos_eigen.argtypes = [_I, _P, _P, _P]
+        L.orc_lobpcg.restype = _I
+        L.orc_lobpcg.argtypes = [_P, _P, _I, _P, _I, _D, _P, _P]
         L.orc_cg_deflated.restype = _I
         L.orc_cg_deflated.argtypes = [_P, _P, _P, _I, _P, _I, _D, _P, _P]
         L.orc_trlm.restype = _I
@@ -555,3 +557,13 @@ def cg_deflated(op, b, U, maxit, tol, mg=None):
     it = lib().orc_cg_deflated(op.h, _ptr(_c(b)), mg.h if mg is not None else None, U.shape[0], _ptr(U), maxit,
                                tol, _ptr(x), _ptr(res))
     return x, it, res[0]
+
+
+def lobpcg(op, X0, max_iter, tol, mg=None):
+    """lobpcg_update from the k start vectors X0 [k][n]: (eigenvalues, X [k][n], iterations)."""
+    X0 = np.ascontiguousarray(X0, np.complex128).reshape(-1, op.n)
+    k = X0.shape[0]
+    X = np.zeros_like(X0)
+    ev = np.zeros(k)
+    it = lib().orc_lobpcg(op.h, mg.h if mg is not None else None, k, _ptr(X0), max_iter, tol, _ptr(X), _ptr(ev))
+    return ev, X, it

@@ -1732,6 +1732,182 @@ int orc_trlm(const orc_op* A, int n_ev, int n_kr, int max_restarts, double tol, 
     return converged;
 }
 
+/* lobpcg_update, src/eigensolver_core.cpp:279-533: block LOBPCG on [X, W, P] with
+ * W = M^-1 R (M = the MG cycle or the identity), the Rayleigh-Ritz problem reduced by
+ * the Cholesky factor of the Gram matrix and solved by orc_lanczos_eigen; a failed
+ * Cholesky falls back to one preconditioned steepest-descent step per vector.
+ * The closing re-orthonormalisation updates AX_i by -<X_j,X_i> X_j, as the reference
+ * does, so AX is A X only up to that term.  X0, X: [k][n]; returns the iterations run. */
+static void lob_mgs(int n, int k, ocx* X, ocx* AX) {
+    for (int i = 0; i < k; i++) {
+        for (int j = 0; j < i; j++) {
+            ocx pr = orc_dot(n, X + (size_t)j * n, X + (size_t)i * n);
+            for (int t = 0; t < n; t++) X[(size_t)i * n + t] -= pr * X[(size_t)j * n + t];
+            if (AX)   /* the reference subtracts pr X_j (not pr AX_j) from AX_i (:514-517) */
+                for (int t = 0; t < n; t++) AX[(size_t)i * n + t] -= pr * X[(size_t)j * n + t];
+        }
+        double nv = orc_norm(n, X + (size_t)i * n);
+        if (nv > 1e-14) {
+            for (int t = 0; t < n; t++) X[(size_t)i * n + t] /= nv;
+            if (AX)
+                for (int t = 0; t < n; t++) AX[(size_t)i * n + t] /= nv;
+        }
+    }
+}
+int orc_lobpcg(const orc_op* A, orc_mg* mg, int k, const ocx* X0, int max_iter, double tol, ocx* Xout,
+               double* evals) {
+    int n = A->n;
+    size_t kn = (size_t)k * n;
+    ocx* X = xcalloc(kn, sizeof(ocx));
+    ocx* AX = xcalloc(kn, sizeof(ocx));
+    ocx* W = xcalloc(kn, sizeof(ocx));
+    ocx* AW = xcalloc(kn, sizeof(ocx));
+    ocx* P = xcalloc(kn, sizeof(ocx));
+    ocx* AP = xcalloc(kn, sizeof(ocx));
+    ocx* R = xcalloc(kn, sizeof(ocx));
+    ocx* Xn = xcalloc(kn, sizeof(ocx));
+    ocx* AXn = xcalloc(kn, sizeof(ocx));
+    double* theta = xcalloc(k, sizeof(double));
+    int sd = 3 * k;
+    ocx* H = xcalloc((size_t)sd * sd, sizeof(ocx));   /* H[j*sd + i] = <S_i, A S_j> (columns) */
+    ocx* M = xcalloc((size_t)sd * sd, sizeof(ocx));
+    ocx* Lc = xcalloc((size_t)sd * sd, sizeof(ocx));  /* Lc[j*sd + i] = L[j][i], the row-j factor */
+    ocx* Y = xcalloc((size_t)sd * sd, sizeof(ocx));
+    ocx* Ht = xcalloc((size_t)sd * sd, sizeof(ocx));
+    ocx* z = xcalloc(sd, sizeof(ocx));
+    double* ev = xcalloc(sd, sizeof(double));
+    ocx* evec = xcalloc((size_t)sd * sd, sizeof(ocx));
+    ocx* Yb = xcalloc((size_t)sd * sd, sizeof(ocx));   /* Yb[col*sd + j] */
+    memcpy(X, X0, kn * sizeof(ocx));
+    lob_mgs(n, k, X, NULL);
+    for (int i = 0; i < k; i++) A->fn(A, X + (size_t)i * n, AX + (size_t)i * n);
+    int have_P = 0, iter;
+    for (iter = 0; iter < max_iter; iter++) {
+        double max_r = 0;
+        for (int i = 0; i < k; i++) {
+            theta[i] = creal(orc_dot(n, X + (size_t)i * n, AX + (size_t)i * n));
+            for (int t = 0; t < n; t++) R[(size_t)i * n + t] = AX[(size_t)i * n + t] - theta[i] * X[(size_t)i * n + t];
+            double ta = fabs(theta[i]) > 1e-30 ? fabs(theta[i]) : 1e-30;
+            double rn = orc_norm(n, R + (size_t)i * n) / ta;
+            if (rn > max_r) max_r = rn;
+        }
+        if (max_r < tol) break;
+        for (int i = 0; i < k; i++) {
+            precond(mg, n, R + (size_t)i * n, W + (size_t)i * n);
+            A->fn(A, W + (size_t)i * n, AW + (size_t)i * n);
+        }
+        int sdim = (have_P ? 3 : 2) * k;
+        const ocx* S[3] = {X, W, P};
+        const ocx* AS[3] = {AX, AW, AP};
+#define SV(j) (S[(j) / k] + (size_t)((j) % k) * n)
+#define ASV(j) (AS[(j) / k] + (size_t)((j) % k) * n)
+        for (int j = 0; j < sdim; j++)
+            for (int i = 0; i < sdim; i++) {
+                H[(size_t)j * sd + i] = orc_dot(n, SV(i), ASV(j));
+                M[(size_t)j * sd + i] = orc_dot(n, SV(i), SV(j));
+            }
+        for (int i = 0; i < sdim; i++)
+            for (int j = i + 1; j < sdim; j++) {
+                ocx ah = 0.5 * (H[(size_t)j * sd + i] + conj(H[(size_t)i * sd + j]));
+                H[(size_t)j * sd + i] = ah; H[(size_t)i * sd + j] = conj(ah);
+                ocx am = 0.5 * (M[(size_t)j * sd + i] + conj(M[(size_t)i * sd + j]));
+                M[(size_t)j * sd + i] = am; M[(size_t)i * sd + j] = conj(am);
+            }
+        /* Cholesky M = L L^dag, row by row: L[j][i] for i < j, then L[j][j] */
+        memset(Lc, 0, (size_t)sd * sd * sizeof(ocx));
+        int chol_ok = 1;
+        for (int j = 0; j < sdim && chol_ok; j++) {
+            for (int i = 0; i < j; i++) {
+                ocx sm = M[(size_t)j * sd + i];
+                for (int p = 0; p < i; p++) sm -= Lc[(size_t)i * sd + p] * conj(Lc[(size_t)j * sd + p]);
+                Lc[(size_t)j * sd + i] = sm / Lc[(size_t)i * sd + i];
+            }
+            ocx sm = M[(size_t)j * sd + j];
+            for (int p = 0; p < j; p++) sm -= Lc[(size_t)j * sd + p] * conj(Lc[(size_t)j * sd + p]);
+            double d = creal(sm);
+            if (d < 1e-14) chol_ok = 0;
+            else Lc[(size_t)j * sd + j] = sqrt(d);
+        }
+        if (!chol_ok) {
+            for (int i = 0; i < k; i++) {
+                ocx* xi = X + (size_t)i * n;
+                ocx xAw = orc_dot(n, xi, AW + (size_t)i * n);
+                ocx wAw = orc_dot(n, W + (size_t)i * n, AW + (size_t)i * n);
+                if (cabs(wAw) > 1e-30) {
+                    ocx al = -xAw / wAw;
+                    for (int t = 0; t < n; t++) xi[t] += al * W[(size_t)i * n + t];
+                }
+                double nv = orc_norm(n, xi);
+                if (nv > 1e-14) for (int t = 0; t < n; t++) xi[t] /= nv;
+            }
+            lob_mgs(n, k, X, NULL);
+            for (int i = 0; i < k; i++) A->fn(A, X + (size_t)i * n, AX + (size_t)i * n);
+            have_P = 0;
+            continue;
+        }
+        /* Ht = L^-1 H L^-dag: Y_col = L^-1 H_col, then L^dag z = Y_col */
+        for (int col = 0; col < sdim; col++) {
+            for (int i = 0; i < sdim; i++) {
+                ocx sm = H[(size_t)col * sd + i];
+                for (int p = 0; p < i; p++) sm -= Lc[(size_t)i * sd + p] * Y[(size_t)col * sd + p];
+                Y[(size_t)col * sd + i] = sm / Lc[(size_t)i * sd + i];
+            }
+            for (int i = sdim - 1; i >= 0; i--) {
+                ocx sm = Y[(size_t)col * sd + i];
+                for (int p = i + 1; p < sdim; p++) sm -= conj(Lc[(size_t)p * sd + i]) * z[p];
+                z[i] = sm / Lc[(size_t)i * sd + i];
+            }
+            for (int i = 0; i < sdim; i++) Ht[(size_t)col * sdim + i] = z[i];   /* packed sdim x sdim */
+        }
+        for (int i = 0; i < sdim; i++)
+            for (int j = i + 1; j < sdim; j++) {
+                ocx a = 0.5 * (Ht[(size_t)j * sdim + i] + conj(Ht[(size_t)i * sdim + j]));
+                Ht[(size_t)j * sdim + i] = a; Ht[(size_t)i * sdim + j] = conj(a);
+            }
+        orc_lanczos_eigen(sdim, Ht, ev, evec);   /* evec[row*sdim + col] */
+        for (int col = 0; col < k; col++)
+            for (int i = sdim - 1; i >= 0; i--) {
+                ocx sm = evec[(size_t)i * sdim + col];
+                for (int p = i + 1; p < sdim; p++) sm -= conj(Lc[(size_t)p * sd + i]) * Yb[(size_t)col * sd + p];
+                Yb[(size_t)col * sd + i] = sm / Lc[(size_t)i * sd + i];
+            }
+        for (int i = 0; i < k; i++) {
+            ocx* xn = Xn + (size_t)i * n;
+            ocx* axn = AXn + (size_t)i * n;
+            memset(xn, 0, n * sizeof(ocx));
+            memset(axn, 0, n * sizeof(ocx));
+            for (int j = 0; j < sdim; j++) {
+                ocx c = Yb[(size_t)i * sd + j];
+                const ocx* sv = SV(j);
+                const ocx* asv = ASV(j);
+                for (int t = 0; t < n; t++) { xn[t] += c * sv[t]; axn[t] += c * asv[t]; }
+            }
+        }
+#undef SV
+#undef ASV
+        for (int i = 0; i < k; i++) {
+            ocx* p = P + (size_t)i * n;
+            ocx* ap = AP + (size_t)i * n;
+            for (int t = 0; t < n; t++) {
+                p[t] = Xn[(size_t)i * n + t] - X[(size_t)i * n + t];
+                ap[t] = AXn[(size_t)i * n + t] - AX[(size_t)i * n + t];
+            }
+            double np = orc_norm(n, p);
+            if (np > 1e-14)
+                for (int t = 0; t < n; t++) { p[t] /= np; ap[t] /= np; }
+        }
+        memcpy(X, Xn, kn * sizeof(ocx));
+        memcpy(AX, AXn, kn * sizeof(ocx));
+        have_P = 1;
+        lob_mgs(n, k, X, AX);
+    }
+    for (int i = 0; i < k; i++) evals[i] = creal(orc_dot(n, X + (size_t)i * n, AX + (size_t)i * n));
+    memcpy(Xout, X, kn * sizeof(ocx));
+    free(X); free(AX); free(W); free(AW); free(P); free(AP); free(R); free(Xn); free(AXn); free(theta);
+    free(H); free(M); free(Lc); free(Y); free(Ht); free(z); free(ev); free(evec); free(Yb);
+    return iter;
+}
+
 /* cg_solve_deflated, src/solvers.cpp:111-213: E = U^dag A U (E_ij = <u_i, A u_j>),
  * x0 = U E^-1 U^dag b, then preconditioned CG whose preconditioned residual is
  * projected, z = M^-1 r - U E^-1 (AU)^dag M^-1 r; E solves by CoarseOp::solve

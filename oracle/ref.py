@@ -72,6 +72,8 @@ def lib():
         L.ref_lanczos_eigen.argtypes = [_I, _P, _P, _P]
         L.ref_trlm_dirac.restype = _I
         L.ref_trlm_dirac.argtypes = [_I, _P, _D, _D, _D, _D, _I, _I, _I, _D, _P, _P]
+        L.ref_lobpcg_dirac.restype = _I
+        L.ref_lobpcg_dirac.argtypes = [_I, _P, _D, _D, _D, _D, _I, _P, _I, _D, _P, _P]
         L.ref_cg_deflated.restype = _I
         L.ref_cg_deflated.argtypes = [_I, _P, _D, _D, _D, _D, _I, _P, _P, _I, _D, _P, _P]
         L.ref_coarse_deflated_solve.restype = _I
@@ -299,3 +301,13 @@ def cg_deflated(L, U, m, r, csw, mu, vecs, rhs, maxit, tol):
     it = lib().ref_cg_deflated(L, _ptr(_c(U)), m, r, csw, mu, vecs.shape[0], _ptr(vecs), _ptr(_c(rhs)), maxit, tol,
                                _ptr(x), _ptr(res))
     return x, it, res[0]
+
+
+def lobpcg_dirac(L, U, m, r, csw, mu, X0, max_iter, tol):
+    """lobpcg_update on D^dag D, identity preconditioner: (eigenvalues, X [k][2V], iterations)."""
+    X0 = np.ascontiguousarray(X0, np.complex128)
+    k = X0.shape[0]
+    ev = np.zeros(k)
+    X = np.zeros_like(X0)
+    it = lib().ref_lobpcg_dirac(L, _ptr(_c(U)), m, r, csw, mu, k, _ptr(X0), max_iter, tol, _ptr(ev), _ptr(X))
+    return ev, X, it

@@ -513,3 +513,18 @@ API int ref_cg_deflated(int L, const double* U, double m, double r, double csw, 
     *res = o.final_residual;
     return o.iterations;
 }
+// lobpcg_update (src/eigensolver_core.cpp:279-533) on D^dag D, identity
+// preconditioner, from k start vectors X0[k][ndof]; returns the iterations
+API int ref_lobpcg_dirac(int L, const double* U, double m, double r, double csw, double mu, int k, const double* X0,
+                         int max_iter, double tol, double* evals, double* X) {
+    Ctx c(L, U, m, r, csw, mu);
+    std::vector<Vec> x0(k);
+    for (int i = 0; i < k; i++) x0[i] = to_vec(X0 + 2 * (size_t)i * c.lat.ndof, c.lat.ndof);
+    auto ident = [](const Vec& v) { return v; };
+    LOBPCGResult o = lobpcg_update(c.D.as_DdagD_op(), c.lat.ndof, k, x0, ident, max_iter, tol);
+    for (int i = 0; i < k; i++) {
+        evals[i] = o.eigvals[i];
+        from_vec(o.eigvecs[i], X + 2 * (size_t)i * c.lat.ndof);
+    }
+    return o.iterations;
+}

@@ -117,6 +117,10 @@ SIGNATURES = [
     ("smg_coarse_setup_deflation", _I, [_P, _I, _I, _I, _D, _P, _P]),
     ("smg_coarse_deflation", _I, [_P, _P, _P, _P]), ("smg_coarse_set_deflation", _I, [_P, _I, _P, _P]),
     ("smg_cg_solve_deflated", _I, [_P, _I, _P, _P, _I, _P, _P, _I, _D, _P]),
+    ("smg_dirac_lobpcg", _I, [_P, _P, _I, _P, _I, _D, _P, _P, _P]),
+    ("smg_coarse_lobpcg", _I, [_P, _I, _P, _I, _D, _P, _P, _P]),
+    ("smg_mg_update_coarsest_deflation", _I, [_P, _I, _I]),
+    ("smg_mg_coarsest_deflation", _I, [_P, _P, _P, _P]),
 ]
 
 _lib = None
@@ -545,6 +549,20 @@ class MGHierarchy:
         _chk(lib().smg_mg_level_dim(self.h, level, ctypes.byref(n)))
         return n.value
 
+    def update_coarsest_deflation(self, n_defl, lobpcg_iters=3):
+        """MGHierarchy::update_coarsest_deflation (src/multigrid.cpp:111-141)."""
+        _chk(lib().smg_mg_update_coarsest_deflation(self.h, n_defl, lobpcg_iters))
+
+    def coarsest_deflation(self):
+        """(defl_vals, defl_vecs) of the coarsest level (src/multigrid.h:28-29)."""
+        nd = _I()
+        _chk(lib().smg_mg_coarsest_deflation(self.h, ctypes.byref(nd), None, None))
+        vals = np.zeros(nd.value)
+        vecs = np.zeros((nd.value, self.level_dim(self.n_levels - 1)), np.complex128)
+        if nd.value:
+            _chk(lib().smg_mg_coarsest_deflation(self.h, ctypes.byref(nd), _p(vals), _p(vecs)))
+        return vals, vecs
+
     def set_symmetric(self, damping, starts):
         """set_symmetric (src/multigrid.cpp:146-152) with explicit power-iteration starts."""
         _chk(lib().smg_mg_set_symmetric(self.h, damping, _p(_c(starts))))
@@ -698,15 +716,15 @@ def random_vec(n, seed):
 
 def lanczos_eigen(A_cols):
     """lanczos_eigen (src/eigensolver_core.cpp:103-265) on the Hermitian matrix given
-    as its columns, A_cols[j][i] = A_ij: (ascending eigenvalues, evecs) with evecs[c]
-    the eigenvector of eigenvalue c, as the reference's evecs[c]."""
+    as its columns, A_cols[j][i] = A_ij: (ascending eigenvalues, evecs) with
+    evecs[row][col] component row of eigenvector col, the reference's layout."""
     cols = _c(A_cols)
     n = cols.shape[0]
     ev = np.zeros(n)
-    out = np.zeros((n, n), np.complex128)   # out[row, col]
+    out = np.zeros((n, n), np.complex128)
     if n:
         _chk(load_library().smg_dense_eigh(n, _p(cols), _p(ev), _p(out)))
-    return ev, np.ascontiguousarray(out.T)
+    return ev, out
 
 
 def trlm_eigensolver(A, n_ev, n_kr=0, max_restarts=100, tol=1e-10, start_vec=None, op=OP_DDAGD):
@@ -727,6 +745,33 @@ def trlm_eigensolver(A, n_ev, n_kr=0, max_restarts=100, tol=1e-10, start_vec=Non
         _chk(lib().smg_dirac_trlm(A.h, op, n_ev, n_kr, max_restarts, tol, _p(start), _p(ev), _p(vecs),
                                   ctypes.byref(info)))
     return TRLMResult(vecs, ev, info.iterations, info.restarts, bool(info.converged))
+
+
+@dataclass
+class LOBPCGResult:
+    """LOBPCGResult (src/eigensolver.h:6-10)."""
+    eigvecs: np.ndarray   # (k, n)
+    eigvals: np.ndarray
+    iterations: int
+
+
+def lobpcg_update(A, X0, precond=None, max_iter=5, tol=1e-8):
+    """lobpcg_update (src/eigensolver_core.cpp:279-533) from the k start vectors X0
+    (rows).  A = a DiracOp (the operator D^dag D) or a SparseCoarseOp; precond = an
+    MGHierarchy (its FP64 cycle; fine operator only) or None for the identity."""
+    X0 = _c(X0)
+    k, n = X0.shape
+    ev = np.zeros(k)
+    X = np.zeros_like(X0)
+    it = _I()
+    if isinstance(A, SparseCoarseOp):
+        if precond is not None:
+            raise SMGError("lobpcg_update: the coarse operator takes the identity preconditioner")
+        _chk(lib().smg_coarse_lobpcg(A.h, k, _p(X0), max_iter, tol, _p(ev), _p(X), ctypes.byref(it)))
+    else:
+        _chk(lib().smg_dirac_lobpcg(A.h, precond.h if precond else None, k, _p(X0), max_iter, tol, _p(ev), _p(X),
+                                    ctypes.byref(it)))
+    return LOBPCGResult(X, ev, it.value)
 
 
 # ------------------------------------------------------------------ forces

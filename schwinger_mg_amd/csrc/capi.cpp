@@ -1381,3 +1381,69 @@ int smg_cg_solve_deflated(smg_dirac* D, int op, smg_mg* mg, const double* b, int
         downc(x, xx.p(), n);
     });
 }
+
+static int next_pow2(int k) {
+    int N = 1;
+    while (N < k) N <<= 1;
+    return N;
+}
+static void lobpcg_out(const LobpcgResult& r, size_t n, double* evals, double* X, int* iterations) {
+    if (evals) std::copy(r.vals.begin(), r.vals.end(), evals);
+    if (X) downc(X, r.vecs.p(), r.vals.size() * n);
+    if (iterations) *iterations = r.iterations;
+}
+int smg_dirac_lobpcg(smg_dirac* D, smg_mg* mg, int k, const double* X0, int max_iter, double tol, double* evals,
+                     double* X, int* iterations) {
+    return guard([&] {
+        SMG_REQUIRE(D && X0 && k >= 1 && k <= 32, "bad arguments (1 <= k <= 32)");
+        const size_t n = 2 * (size_t)D->D->V;
+        DdagDOpB<double> A(*D->D, next_pow2(k));
+        Field<double> x0 = up(X0, (size_t)k * n);
+        LobpcgResult r = lobpcg(A, mg ? mg->mg.get() : nullptr, k, x0.p(), max_iter, tol, S());
+        lobpcg_out(r, n, evals, X, iterations);
+    });
+}
+int smg_coarse_lobpcg(smg_coarse* C, int k, const double* X0, int max_iter, double tol, double* evals, double* X,
+                      int* iterations) {
+    return guard([&] {
+        SMG_REQUIRE(C && X0 && k >= 1 && k <= 32, "bad arguments (1 <= k <= 32)");
+        const size_t n = C->C->n();
+        CoarseLinOpB<double, c64> A(*C->C, next_pow2(k));
+        Field<double> x0 = up(X0, (size_t)k * n);
+        LobpcgResult r = lobpcg(A, nullptr, k, x0.p(), max_iter, tol, S());
+        lobpcg_out(r, n, evals, X, iterations);
+    });
+}
+int smg_mg_update_coarsest_deflation(smg_mg* mg, int n_defl, int lobpcg_iters) {
+    return guard([&] {
+        SMG_REQUIRE(mg, "null handle");
+        MG& m = *mg->mg;
+        SMG_REQUIRE(!m.C.empty(), "the hierarchy has no coarse level");
+        const size_t dim = m.dims.back();
+        if (dim == 0 || n_defl <= 0) return;
+        SMG_REQUIRE(n_defl <= 32 && (size_t)n_defl <= dim, "n_defl: 1..32 and at most the coarsest dimension");
+        Field<double> x0((size_t)n_defl * dim);
+        if ((int)m.defl_vals.size() >= n_defl) {   // warm start
+            launch_copy<double>(x0.p(), m.defl_vecs.p(), (size_t)n_defl * dim, S());
+        } else {   // cold start: random_vec(dim, rng) n_defl times from one mt19937(42)
+            std::vector<double> h(2 * (size_t)n_defl * dim);
+            SMG_REQUIRE(smg_random_vec((int)(n_defl * dim), 42, h.data()) == 0, "random_vec failed");
+            SMG_HIP(hipMemcpyAsync(x0.p(), h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, S()));
+            SMG_HIP(hipStreamSynchronize(S()));
+        }
+        CoarseLinOpB<double, c64> A(*m.C.back(), next_pow2(n_defl));
+        LobpcgResult r = lobpcg(A, nullptr, n_defl, x0.p(), lobpcg_iters, 1e-10, S());
+        m.defl_vals = r.vals;
+        m.defl_vecs = std::move(r.vecs);
+    });
+}
+int smg_mg_coarsest_deflation(smg_mg* mg, int* ndefl, double* evals, double* evecs) {
+    return guard([&] {
+        SMG_REQUIRE(mg, "null handle");
+        const MG& m = *mg->mg;
+        const int nd = (int)m.defl_vals.size();
+        if (ndefl) *ndefl = nd;
+        if (evals) std::copy(m.defl_vals.begin(), m.defl_vals.end(), evals);
+        if (evecs && nd) downc(evecs, m.defl_vecs.p(), (size_t)nd * m.dims.back());
+    });
+}

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "runtime.h"
+#include "runtime_b.h"
 
 namespace smg {
 
@@ -37,5 +38,19 @@ int deflated_coarse_cg(LinOp<double>& A, const Eigenpairs& E, const c64* b, c64*
 // deflation space U (ndefl device vectors), E = U^dag A U inverted on the host
 SolveResult cg_deflated(LinOp<double>& A, MG* mg, const c64* U, int ndefl, const c64* b, c64* x, int maxit,
                         double tol, hipStream_t st);
+
+// lobpcg_update (src/eigensolver_core.cpp:279-533) on the k vectors X0 (device,
+// [k][n]) as one block of N = next power of two >= k columns: every operator and
+// preconditioner application is one multi-right-hand-side launch, the Gram matrices
+// of [X, W, P] are block contractions, and the orthonormalisations are Cholesky QR
+// (equal in exact arithmetic to the reference's modified Gram-Schmidt, with AX
+// updated as the reference updates it).  mg: the FP64 cycle as preconditioner, or
+// null for the identity.  k <= 32.
+struct LobpcgResult {
+    std::vector<double> vals;
+    Field<double> vecs;   // k vectors of A.n(), one after the other
+    int iterations = 0;
+};
+LobpcgResult lobpcg(LinOpB<double>& A, MG* mg, int k, const c64* X0, int max_iter, double tol, hipStream_t st);
 
 }  // namespace smg

@@ -405,4 +405,285 @@ SolveResult cg_deflated(LinOp<double>& A, MG* mg, const c64* U, int nd, const c6
     return res;
 }
 
+
+// ----------------------------------------------------------------- LOBPCG
+namespace {
+// Block-vector workspace of the LOBPCG iteration and its host-side Gram helper.
+struct BlockWork {
+    size_t n;
+    int N, k;   // block width and the number of live columns
+    DevMem part, gout, coef;
+    BlockWork(size_t n_, int N_, int k_)
+        : n(n_), N(N_), k(k_),
+          part((size_t)blk_gram_grid(n_) * 2 * kGramPairsMax * sizeof(double)),
+          gout(2 * (size_t)kGramPairsMax * sizeof(double)),
+          coef(2 * 3 * (size_t)N_ * N_ * sizeof(c64)) {}
+    static constexpr int kGramPairsMax = 4096;
+    // G[a][b] = <L_a, R_b>, a over the nl*N left columns, b over the nr*N right ones
+    std::vector<cxd> gram(const std::vector<const c64*>& L, const std::vector<const c64*>& R, hipStream_t st) {
+        const int nl = (int)L.size(), nr = (int)R.size(), CL = nl * N, CR = nr * N;
+        std::vector<cxd> G((size_t)CL * CR);
+        BlockPtrs lp{};
+        for (int i = 0; i < nl; i++) lp.p[i] = L[i];
+        int chunk = std::max(1, std::min(nr, kGramPairsMax / (CL * N)));
+        while (chunk > 1 && (nl + chunk) * N > 3072) chunk--;
+        std::vector<double> h;
+        for (int r0 = 0; r0 < nr; r0 += chunk) {
+            const int nc = std::min(chunk, nr - r0);
+            BlockPtrs rp{};
+            for (int i = 0; i < nc; i++) rp.p[i] = R[r0 + i];
+            launch_blk_gram(lp, nl, rp, nc, N, n, part.as<double>(), gout.as<double>(), st);
+            h.resize(2 * (size_t)CL * nc * N);
+            SMG_HIP(hipMemcpyAsync(h.data(), gout.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+            SMG_HIP(hipStreamSynchronize(st));
+            for (int a = 0; a < CL; a++)
+                for (int b = 0; b < nc * N; b++)
+                    G[(size_t)a * CR + r0 * N + b] = cxd(h[2 * ((size_t)a * nc * N + b)], h[2 * ((size_t)a * nc * N + b) + 1]);
+        }
+        return G;
+    }
+    // out0 = sum_s S_s C0[s], out1 = sum_s S_s C1[s] (host coefficient matrices
+    // (ns*N) x N, row-major; C1 empty for one output)
+    void combine(c64* out0, c64* out1, const std::vector<const c64*>& S, const std::vector<cxd>& C0,
+                 const std::vector<cxd>& C1, hipStream_t st) {
+        const int ns = (int)S.size();
+        const size_t m = (size_t)ns * N * N;
+        std::vector<c64> h(2 * m);
+        for (size_t i = 0; i < m; i++) {
+            h[i] = c64{C0[i].real(), C0[i].imag()};
+            if (!C1.empty()) h[m + i] = c64{C1[i].real(), C1[i].imag()};
+        }
+        SMG_HIP(hipMemcpyAsync(coef.p, h.data(), (C1.empty() ? m : 2 * m) * sizeof(c64), hipMemcpyHostToDevice, st));
+        BlockPtrs sp{};
+        for (int i = 0; i < ns; i++) sp.p[i] = S[i];
+        launch_blk_combine(out0, out1, sp, ns, N, k, coef.as<c64>(), C1.empty() ? nullptr : coef.as<c64>() + m, n,
+                           st);
+        SMG_HIP(hipStreamSynchronize(st));   // the host coefficients go out of scope
+    }
+    // per-column y_j = a_j x_j + b_j y_j
+    void cols(c64* y, const c64* x, const std::vector<cxd>& a, const std::vector<cxd>& b, hipStream_t st) {
+        std::vector<c64> h(2 * (size_t)N);
+        for (int j = 0; j < N; j++) {
+            h[j] = c64{a[j].real(), a[j].imag()};
+            h[N + j] = c64{b[j].real(), b[j].imag()};
+        }
+        SMG_HIP(hipMemcpyAsync(coef.p, h.data(), h.size() * sizeof(c64), hipMemcpyHostToDevice, st));
+        launch_blk_cols(y, x, coef.as<c64>(), coef.as<c64>() + N, N, n, st);
+        SMG_HIP(hipStreamSynchronize(st));
+    }
+};
+}  // namespace
+
+LobpcgResult lobpcg(LinOpB<double>& A, MG* mg, int k, const c64* X0, int max_iter, double tol, hipStream_t st) {
+    const size_t n = A.n();
+    const int N = A.N;
+    SMG_REQUIRE(k >= 1 && k <= N && N <= 32 && (N & (N - 1)) == 0, "lobpcg: 1 <= k <= N <= 32, N a power of two");
+    SMG_REQUIRE((size_t)k <= n, "lobpcg: k exceeds the dimension");
+    BlockWork bw(n, N, k);
+    const size_t bn = n * N;
+    Field<double> X(bn), AX(bn), W(bn), AW(bn), P(bn), AP(bn), Xn(bn), AXn(bn), Pn(bn), APn(bn), R(bn), sep(bn);
+    launch_zero<double>(sep.p(), bn, st);
+    launch_copy<double>(sep.p(), X0, (size_t)k * n, st);
+    launch_b_pack<double, double>(X.p(), sep.p(), n, N, st);
+    std::vector<cxd> one(N, 1.0), zero(N, 0.0);
+    auto ident = [&](int ns) {   // (ns*N) x N zero matrix
+        return std::vector<cxd>((size_t)ns * N * N, 0.0);
+    };
+    // X <- X R^-1 with R from the Cholesky factor of X^dag X (the reference's modified
+    // Gram-Schmidt, src/eigensolver_core.cpp:300-307 and :510-523); with AX, also
+    // AX <- X R^-1 + (AX - X) diag(1/R_ii), the reference's update of AX
+    auto chol_qr = [&](bool with_ax) {
+        std::vector<cxd> G = bw.gram({X.p()}, {X.p()}, st);
+        std::vector<cxd> Rm((size_t)k * k, 0.0);
+        for (int i = 0; i < k; i++) {
+            cxd d = G[(size_t)i * N + i];
+            for (int p = 0; p < i; p++) d -= std::norm(Rm[(size_t)p * k + i]);
+            const double rii = std::sqrt(std::max(d.real(), 0.0));
+            Rm[(size_t)i * k + i] = rii > 1e-14 ? rii : 1.0;   // the reference leaves such a vector unscaled
+            for (int j = i + 1; j < k; j++) {
+                cxd s = G[(size_t)i * N + j];
+                for (int p = 0; p < i; p++) s -= std::conj(Rm[(size_t)p * k + i]) * Rm[(size_t)p * k + j];
+                Rm[(size_t)i * k + j] = s / Rm[(size_t)i * k + i];
+            }
+        }
+        std::vector<cxd> Ri((size_t)k * k, 0.0);   // R^-1, upper triangular
+        for (int c = 0; c < k; c++) {
+            Ri[(size_t)c * k + c] = 1.0 / Rm[(size_t)c * k + c];
+            for (int i = c - 1; i >= 0; i--) {
+                cxd s = 0.0;
+                for (int p = i + 1; p <= c; p++) s += Rm[(size_t)i * k + p] * Ri[(size_t)p * k + c];
+                Ri[(size_t)i * k + c] = -s / Rm[(size_t)i * k + i];
+            }
+        }
+        std::vector<cxd> C0 = ident(with_ax ? 2 : 1), C1;
+        for (int j = 0; j < k; j++)
+            for (int c = 0; c < k; c++) C0[(size_t)j * N + c] = Ri[(size_t)j * k + c];
+        if (with_ax) {
+            C1 = ident(2);
+            for (int j = 0; j < k; j++)
+                for (int c = 0; c < k; c++) C1[(size_t)j * N + c] = Ri[(size_t)j * k + c];
+            for (int c = 0; c < k; c++) {
+                const double inv = 1.0 / Rm[(size_t)c * k + c].real();
+                C1[(size_t)c * N + c] -= inv;
+                C1[((size_t)N + c) * N + c] = inv;
+            }
+            bw.combine(Xn.p(), AXn.p(), {X.p(), AX.p()}, C0, C1, st);
+            std::swap(X, Xn);
+            std::swap(AX, AXn);
+        } else {
+            bw.combine(Xn.p(), nullptr, {X.p()}, C0, {}, st);
+            std::swap(X, Xn);
+        }
+    };
+    chol_qr(false);
+    A.apply(AX.p(), X.p(), st);
+    bool have_P = false;
+    int iter;
+    for (iter = 0; iter < max_iter; iter++) {
+        std::vector<cxd> G = bw.gram({X.p()}, {AX.p()}, st);
+        std::vector<cxd> mth(N, 0.0);
+        std::vector<double> theta(k);
+        for (int i = 0; i < k; i++) {
+            theta[i] = G[(size_t)i * N + i].real();
+            mth[i] = -theta[i];
+        }
+        launch_copy<double>(R.p(), AX.p(), bn, st);
+        bw.cols(R.p(), X.p(), mth, one, st);
+        std::vector<cxd> GR = bw.gram({R.p()}, {R.p()}, st);
+        double max_r = 0.0;
+        for (int i = 0; i < k; i++)
+            max_r = std::max(max_r, std::sqrt(GR[(size_t)i * N + i].real()) / std::max(std::abs(theta[i]), 1e-30));
+        if (max_r < tol) break;
+        if (mg) mg_precondition_b<double>(*mg, R.p(), W.p(), N, st);
+        else launch_copy<double>(W.p(), R.p(), bn, st);
+        A.apply(AW.p(), W.p(), st);
+        const int nb = have_P ? 3 : 2, sdim = nb * k;
+        std::vector<const c64*> S = {X.p(), W.p(), P.p()}, AS = {AX.p(), AW.p(), AP.p()};
+        S.resize(nb);
+        AS.resize(nb);
+        std::vector<cxd> GM = bw.gram(S, S, st), GH = bw.gram(S, AS, st);
+        const int CL = nb * N;
+        auto at = [&](const std::vector<cxd>& Gm, int i, int j) {   // <S_i, T_j> in the reference's indexing
+            return Gm[(size_t)((i / k) * N + i % k) * CL + (j / k) * N + j % k];
+        };
+        // H_cols[j][i] = <S_i, A S_j>, M_cols[j][i] = <S_i, S_j>, then Hermitised
+        std::vector<cxd> H((size_t)sdim * sdim), M((size_t)sdim * sdim);
+        for (int j = 0; j < sdim; j++)
+            for (int i = 0; i < sdim; i++) {
+                H[(size_t)j * sdim + i] = at(GH, i, j);
+                M[(size_t)j * sdim + i] = at(GM, i, j);
+            }
+        std::vector<cxd> xAw(k), wAw(k);
+        for (int i = 0; i < k; i++) {
+            xAw[i] = H[(size_t)(k + i) * sdim + i];
+            wAw[i] = H[(size_t)(k + i) * sdim + k + i];
+        }
+        for (int i = 0; i < sdim; i++)
+            for (int j = i + 1; j < sdim; j++) {
+                const cxd ah = 0.5 * (H[(size_t)j * sdim + i] + std::conj(H[(size_t)i * sdim + j]));
+                H[(size_t)j * sdim + i] = ah;
+                H[(size_t)i * sdim + j] = std::conj(ah);
+                const cxd am = 0.5 * (M[(size_t)j * sdim + i] + std::conj(M[(size_t)i * sdim + j]));
+                M[(size_t)j * sdim + i] = am;
+                M[(size_t)i * sdim + j] = std::conj(am);
+            }
+        // M = L L^dag, L[j][i] (i <= j) stored Lc[j*sdim + i]
+        std::vector<cxd> Lc((size_t)sdim * sdim, 0.0);
+        bool chol_ok = true;
+        for (int j = 0; j < sdim && chol_ok; j++) {
+            for (int i = 0; i < j; i++) {
+                cxd s = M[(size_t)j * sdim + i];
+                for (int p = 0; p < i; p++) s -= Lc[(size_t)i * sdim + p] * std::conj(Lc[(size_t)j * sdim + p]);
+                Lc[(size_t)j * sdim + i] = s / Lc[(size_t)i * sdim + i];
+            }
+            cxd s = M[(size_t)j * sdim + j];
+            for (int p = 0; p < j; p++) s -= Lc[(size_t)j * sdim + p] * std::conj(Lc[(size_t)j * sdim + p]);
+            if (s.real() < 1e-14) chol_ok = false;
+            else Lc[(size_t)j * sdim + j] = std::sqrt(s.real());
+        }
+        if (!chol_ok) {   // one preconditioned steepest-descent step per vector (:396-423)
+            std::vector<cxd> al(N, 0.0);
+            for (int i = 0; i < k; i++)
+                if (std::abs(wAw[i]) > 1e-30) al[i] = -xAw[i] / wAw[i];
+            bw.cols(X.p(), W.p(), al, one, st);
+            std::vector<cxd> GX = bw.gram({X.p()}, {X.p()}, st);
+            std::vector<cxd> sc(N, 1.0);
+            for (int i = 0; i < k; i++) {
+                const double nv = std::sqrt(GX[(size_t)i * N + i].real());
+                if (nv > 1e-14) sc[i] = 1.0 / nv;
+            }
+            bw.cols(X.p(), nullptr, zero, sc, st);
+            chol_qr(false);
+            A.apply(AX.p(), X.p(), st);
+            have_P = false;
+            continue;
+        }
+        // Ht = L^-1 H L^-dag, Hermitised, diagonalised
+        std::vector<cxd> Y((size_t)sdim * sdim), Ht((size_t)sdim * sdim), z(sdim);
+        for (int col = 0; col < sdim; col++) {
+            for (int i = 0; i < sdim; i++) {
+                cxd s = H[(size_t)col * sdim + i];
+                for (int p = 0; p < i; p++) s -= Lc[(size_t)i * sdim + p] * Y[(size_t)col * sdim + p];
+                Y[(size_t)col * sdim + i] = s / Lc[(size_t)i * sdim + i];
+            }
+            for (int i = sdim - 1; i >= 0; i--) {
+                cxd s = Y[(size_t)col * sdim + i];
+                for (int p = i + 1; p < sdim; p++) s -= std::conj(Lc[(size_t)p * sdim + i]) * z[p];
+                z[i] = s / Lc[(size_t)i * sdim + i];
+            }
+            for (int i = 0; i < sdim; i++) Ht[(size_t)col * sdim + i] = z[i];
+        }
+        for (int i = 0; i < sdim; i++)
+            for (int j = i + 1; j < sdim; j++) {
+                const cxd a = 0.5 * (Ht[(size_t)j * sdim + i] + std::conj(Ht[(size_t)i * sdim + j]));
+                Ht[(size_t)j * sdim + i] = a;
+                Ht[(size_t)i * sdim + j] = std::conj(a);
+            }
+        std::vector<c64> Hc((size_t)sdim * sdim), Ev((size_t)sdim * sdim);
+        for (size_t i = 0; i < Hc.size(); i++) Hc[i] = c64{Ht[i].real(), Ht[i].imag()};
+        std::vector<double> ev(sdim);
+        dense_eigh(sdim, Hc.data(), ev.data(), Ev.data());   // Ev[row*sdim + col]
+        // y = L^-dag z for the k lowest
+        std::vector<cxd> Yb((size_t)k * sdim, 0.0);
+        for (int col = 0; col < k; col++)
+            for (int i = sdim - 1; i >= 0; i--) {
+                cxd s = to_std(Ev[(size_t)i * sdim + col]);
+                for (int p = i + 1; p < sdim; p++) s -= std::conj(Lc[(size_t)p * sdim + i]) * Yb[(size_t)col * sdim + p];
+                Yb[(size_t)col * sdim + i] = s / Lc[(size_t)i * sdim + i];
+            }
+        // X_new = S y, P = X_new - X (and their A images), one launch each for S and AS
+        std::vector<cxd> CX = ident(nb), CP;
+        for (int c = 0; c < k; c++)
+            for (int j = 0; j < sdim; j++) CX[((size_t)(j / k) * N + j % k) * N + c] = Yb[(size_t)c * sdim + j];
+        CP = CX;
+        for (int c = 0; c < k; c++) CP[(size_t)c * N + c] -= 1.0;
+        bw.combine(Xn.p(), Pn.p(), S, CX, CP, st);
+        bw.combine(AXn.p(), APn.p(), AS, CX, CP, st);
+        std::vector<cxd> GP = bw.gram({Pn.p()}, {Pn.p()}, st);
+        std::vector<cxd> sc(N, 1.0);
+        for (int i = 0; i < k; i++) {
+            const double np = std::sqrt(GP[(size_t)i * N + i].real());
+            if (np > 1e-14) sc[i] = 1.0 / np;
+        }
+        bw.cols(Pn.p(), nullptr, zero, sc, st);
+        bw.cols(APn.p(), nullptr, zero, sc, st);
+        std::swap(X, Xn);
+        std::swap(AX, AXn);
+        std::swap(P, Pn);
+        std::swap(AP, APn);
+        have_P = true;
+        chol_qr(true);
+    }
+    LobpcgResult out;
+    out.iterations = iter;
+    std::vector<cxd> G = bw.gram({X.p()}, {AX.p()}, st);
+    out.vals.resize(k);
+    for (int i = 0; i < k; i++) out.vals[i] = G[(size_t)i * N + i].real();
+    launch_b_unpack<double, double>(sep.p(), X.p(), n, N, st);
+    out.vecs = Field<double>((size_t)k * n);
+    launch_copy<double>(out.vecs.p(), sep.p(), (size_t)k * n, st);
+    SMG_HIP(hipStreamSynchronize(st));
+    return out;
+}
+
 }  // namespace smg

@@ -221,6 +221,9 @@ struct MG {
     std::unique_ptr<Cycle<float>> cyc32;
     std::vector<size_t> dims;   // per level
     std::vector<int> lnx, lny, lnd;
+    // coarsest-level deflation space (MGHierarchy::levels[last].defl_vecs / defl_vals)
+    std::vector<double> defl_vals;
+    Field<double> defl_vecs;
     // dense inverse of the coarsest operator (cfg.coarse_solver == 1)
     Field<double> inv64;
     Field<float> inv32;

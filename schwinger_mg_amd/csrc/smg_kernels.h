@@ -102,6 +102,22 @@ void launch_multi_dot(const VecPtrs<T>& V, int nv, const cplx<T>* w, size_t n, i
 // complex coefficients c_i at coef[2i], coef[2i+1] in device memory (eigen.hip)
 template <typename T>
 void launch_lincomb(cplx<T>* y, const VecPtrs<T>& V, int nv, const double* coef, int mode, size_t n, hipStream_t st);
+// block vectors (N vectors of n entries, v[i*N + j], N a power of two <= 32)
+constexpr int kMaxBlocks = 6;
+struct BlockPtrs {
+    const c64* p[kMaxBlocks];
+};
+// G[a][b] = <L_a, R_b> over the nl*N left and nr*N right vectors, complex, into out
+// (device, 2*nl*N*nr*N doubles); part holds blk_gram_grid(n) * that many doubles
+int blk_gram_grid(size_t n);
+void launch_blk_gram(const BlockPtrs& L, int nl, const BlockPtrs& R, int nr, int N, size_t n, double* part,
+                     double* out, hipStream_t st);
+// out_m = sum_s S_s C_m[s] (C_m: (ns*N) x N complex, row-major; out1/C1 may be null);
+// only the first kact columns of each source are read
+void launch_blk_combine(c64* out0, c64* out1, const BlockPtrs& S, int ns, int N, int kact, const c64* C0,
+                        const c64* C1, size_t n, hipStream_t st);
+// y_j = a_j x_j + b_j y_j per column j (x may be null)
+void launch_blk_cols(c64* y, const c64* x, const c64* a, const c64* b, int N, size_t n, hipStream_t st);
 // GCR orthogonalisation: beta_i = dots[i]/apap[i]; p -= sum beta_i P_i; Ap -= sum beta_i AP_i
 template <typename T>
 void launch_gcr_orth(cplx<T>* p, cplx<T>* Ap, const VecPtrs<T>& P, const VecPtrs<T>& AP, int nv,

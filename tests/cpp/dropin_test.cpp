@@ -58,6 +58,7 @@ int orc_trlm(const orc_op* A, int n_ev, int n_kr, int max_restarts, double tol, 
              cx* evecs, int* restarts_out, int* iters_out);
 int orc_cg_deflated(const orc_op* A, const cx* rhs, orc_mg* mg, int ndefl, const cx* U, int maxit, double tol, cx* x,
                     double* final_res);
+int orc_lobpcg(const orc_op* A, orc_mg* mg, int k, const cx* X0, int max_iter, double tol, cx* Xout, double* evals);
 }
 
 static int failures = 0;
@@ -297,6 +298,10 @@ int main() {
         double de = 0;
         for (int i = 0; i < nh; i++) de = std::max(de, std::abs(ev[i] - eo_[i]));
         report("lanczos_eigen eigenvalues", de, 1e-13);
+        double dv = 0;
+        for (int r = 0; r < nh; r++)
+            for (int c = 0; c < nh; c++) dv = std::max(dv, std::abs(evec[r][c] - ovec[(size_t)r * nh + c]));
+        report("lanczos_eigen evecs[row][col]", dv, 1e-12);
 
         const int n_ev = 6, n = lat.ndof;
         TRLMResult tr = trlm_eigensolver(A, n, n_ev, 24, 100, 1e-10);
@@ -346,6 +351,32 @@ int main() {
         for (int i = 0; i < 8; i++) de = std::max(de, std::abs(tc.eigvals[i] - S.defl_vals[i]));
         report("trlm_eigensolver on SparseCoarseOp::as_op", de, 1e-9 * S.defl_vals[7]);
         report("SparseCoarseOp::validate_deflation", S.validate_deflation() == 8 ? 0.0 : 1.0, 0.0);
+
+        // lobpcg_update (identity preconditioner) against the oracle from the same block
+        const int kl = 8;
+        std::vector<Vec> X0;
+        Vec x0f;
+        for (int i = 0; i < kl; i++) {
+            X0.push_back(random_vec(n, rng));
+            x0f.insert(x0f.end(), X0.back().begin(), X0.back().end());
+        }
+        LOBPCGResult lr = lobpcg_update(A, n, kl, X0, identity_precond(), 8, 1e-10);
+        Vec xl((size_t)kl * n), xg;
+        RVec el(kl);
+        const int itl = orc_lobpcg(Ao, nullptr, kl, x0f.data(), 8, 1e-10, xl.data(), el.data());
+        de = 0;
+        for (int i = 0; i < kl; i++) {
+            de = std::max(de, std::abs(lr.eigvals[i] - el[i]) / std::abs(el[0]));
+            xg.insert(xg.end(), lr.eigvecs[i].begin(), lr.eigvecs[i].end());
+        }
+        report("lobpcg_update Rayleigh quotients", de, 1e-9);
+        report("lobpcg_update vectors", relerr(xg, xl), 1e-7);
+        report("lobpcg_update iterations", std::abs(lr.iterations - itl), 0.0);
+        mg.update_coarsest_deflation(8, 3);
+        report("MGHierarchy::update_coarsest_deflation",
+               mg.coarsest_defl_vals.size() == 8 && mg.coarsest_defl_vecs[0].size() == (size_t)mg.dims.back() ? 0.0
+                                                                                                       : 1.0,
+               0.0);
     }
 
     orc_mg_free(mgo);

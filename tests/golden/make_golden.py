@@ -133,6 +133,12 @@ def main():
     ev, x = R.coarse_deflated_solve(16, out["L16_U"], 0.05, 1.0, 1.0, 0.0, 4, 4, out["L16_nulls"].ravel(), 8, 0,
                                     100, 1e-10, out["L16_c"], 200, 1e-12)
     out["L16_defl_evals"], out["L16_defl_x"] = ev, x
+    # lobpcg_update on D^dag D (identity preconditioner) from random_vec draws
+    X0 = R.random_vec(6 * 128, 77).reshape(6, 128)
+    out["E8_lob_X0"] = X0
+    for its in (10, 40):
+        ev, X, it = R.lobpcg_dirac(8, out["E8_U"], 0.05, 1.0, 1.0, 0.0, X0, its, 1e-10)
+        out[f"E8_lob{its}_evals"], out[f"E8_lob{its}_X"], out[f"E8_lob{its}_it"] = ev, X, np.array([it])
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_vectors.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes,", len(out), "arrays")

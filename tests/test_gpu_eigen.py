@@ -38,11 +38,11 @@ def test_lanczos_eigen_vs_oracle_and_golden(smg):
     for name in ("H12", "H8deg"):
         A = g[name]
         ev, vecs = smg.lanczos_eigen(np.ascontiguousarray(A.T))   # columns A_cols[j][i] = A_ij
-        evo, vo = O.lanczos_eigen(A)
+        evo, vo = O.lanczos_eigen(A)   # both [row][col]
         assert np.abs(ev - evo).max() < 1e-13 and np.abs(ev - g[f"{name}_evals"]).max() < 1e-13
-        assert np.abs(A @ vecs.T - vecs.T * ev).max() < 1e-12
+        assert np.abs(A @ vecs - vecs * ev).max() < 1e-12
         if name == "H12":
-            assert np.abs(vecs.T - vo).max() < 1e-12
+            assert np.abs(vecs - vo).max() < 1e-12
     ev, vecs = smg.lanczos_eigen(np.array([[2.5]]))
     assert ev[0] == 2.5 and vecs[0, 0] == 1.0
 
@@ -192,3 +192,93 @@ def test_trlm_large_properties(smg):
     conv, rs, evo, _, _ = O.trlm(A, n_ev, 0, 200, 1e-10, smg.random_vec(2 * L * L, 42))
     print(f"restarts {res.num_restarts} vs {rs}; lowest {res.eigvals[0]:.12e} vs {evo[0]:.12e}")
     assert abs(res.eigvals[0] - evo[0]) < 1e-10 * evo[-1]
+
+
+# ------------------------------------------------------------------ LOBPCG
+@pytest.mark.parametrize("its", [10, 40])
+def test_lobpcg_golden_and_oracle(smg, its):
+    """lobpcg_update on D^dag D as one 8-column block (k = 6 padded): Cholesky QR
+    and block Gram matrices in place of the reference's sequential Gram-Schmidt and
+    dots, equal in exact arithmetic."""
+    banner(f"lobpcg golden L=8 iterations={its}")
+    g = np.load(GOLD)
+    D = smg.DiracOp(8, g["E8_U"], 0.05, 1.0, 1.0, 0.0)
+    res = smg.lobpcg_update(D, g["E8_lob_X0"], None, its, 1e-10)
+    ref = g[f"E8_lob{its}_evals"]
+    print(f"evals relerr {relerr(res.eigvals, ref):.2e}, X relerr {relerr(res.eigvecs, g[f'E8_lob{its}_X']):.2e}")
+    assert res.iterations == int(g[f"E8_lob{its}_it"][0])
+    assert np.abs(res.eigvals - ref).max() < 1e-9 * np.abs(ref).max()
+    assert relerr(res.eigvecs, g[f"E8_lob{its}_X"]) < 1e-7
+
+
+@pytest.mark.parametrize("k", [4, 16])
+def test_lobpcg_mg_preconditioned_vs_oracle(smg, k):
+    banner(f"lobpcg with the MG preconditioner L=16 k={k}")
+    L, kn = 16, 4
+    U = random_gauge(L, 0.3, 270)
+    starts = gaussian_vecs(kn, 2 * L * L, 271).ravel()
+    D = smg.DiracOp(L, U, 0.05, 1.0, 1.0, 0.0)
+    mg = smg.MGHierarchy(D, starts, n_levels=2, blocks=((4, 4),), k=(kn,), null_iters=10, coarse_solver=1)
+    Do = O.Dirac(L, U, 0.05, 1.0, 1.0, 0.0)
+    mgo = O.MG(Do.op(), L, L, 2, starts, n_levels=2, block=((4, 4),), k=(kn,), null_iters=10, coarse_solver=1)
+    X0 = gaussian_vecs(k, 2 * L * L, 272)
+    res = smg.lobpcg_update(D, X0, mg, 5, 1e-10)
+    evo, Xo, ito = O.lobpcg(Do.op(), X0, 5, 1e-10, mg=mgo)
+    print(f"evals {res.eigvals[:4]} vs {evo[:4]}")
+    assert res.iterations == ito
+    assert np.abs(res.eigvals - evo).max() < 1e-8 * np.abs(evo).max()
+    assert relerr(res.eigvecs, Xo) < 1e-6
+
+
+def test_lobpcg_coarse_and_coarsest_deflation(smg):
+    """lobpcg_update on a coarse operator, and MGHierarchy::update_coarsest_deflation:
+    cold start from random_vec(dim) drawn n_defl times from mt19937(42), then a warm
+    start from the stored space."""
+    banner("lobpcg coarse / update_coarsest_deflation")
+    L, kn = 16, 4
+    U = random_gauge(L, 0.3, 280)
+    starts = gaussian_vecs(kn, 2 * L * L, 281).ravel()
+    D = smg.DiracOp(L, U, 0.05, 1.0, 1.0, 0.0)
+    mg = smg.MGHierarchy(D, starts, n_levels=2, blocks=((4, 4),), k=(kn,), null_iters=10)
+    dim = mg.level_dim(1)
+    Po = O.Prolongator(L, L, 2, 4, 4, kn)
+    Po.set_block_vecs(mg.block_vecs(0))
+    Co = O.Coarse(O.Dirac(L, U, 0.05, 1.0, 1.0, 0.0).op(), Po, 1)
+    n_defl = 8
+    mg.update_coarsest_deflation(n_defl, 3)
+    vals, vecs = mg.coarsest_deflation()
+    X0 = smg.random_vec(n_defl * dim, 42).reshape(n_defl, dim)
+    evo, Xo, _ = O.lobpcg(Co.op(), X0, 3, 1e-10)
+    assert np.abs(vals - evo).max() < 1e-9 * np.abs(evo).max() and relerr(vecs, Xo) < 1e-7
+    mg.update_coarsest_deflation(n_defl, 3)   # warm start
+    vals2, vecs2 = mg.coarsest_deflation()
+    evo2, Xo2, _ = O.lobpcg(Co.op(), Xo, 3, 1e-10)
+    assert np.abs(vals2 - evo2).max() < 1e-9 * np.abs(evo2).max() and relerr(vecs2, Xo2) < 1e-6
+    # the coarse operator directly
+    P = smg.Prolongator(L, L, 2, 4, 4, kn)
+    P.set_block_vecs(mg.block_vecs(0))
+    C = smg.SparseCoarseOp.build(P, D)
+    res = smg.lobpcg_update(C, X0, None, 3, 1e-10)
+    assert np.abs(res.eigvals - evo).max() < 1e-9 * np.abs(evo).max()
+
+
+def test_lobpcg_large_block_of_16(smg):
+    """L = 64 (n = 8192), a 16-vector block: X stays orthonormal, its Ritz values lie
+    above the 16 lowest eigenvalues (Cauchy interlacing), and the Rayleigh quotients
+    follow the oracle's from the same start over 30 iterations."""
+    banner("lobpcg L=64 k=16")
+    L, k = 64, 16
+    U = random_gauge(L, 0.3, 290)
+    D = smg.DiracOp(L, U, 0.05, 1.0, 1.0, 0.0)
+    tr = smg.trlm_eigensolver(D, k, 0, 200, 1e-10)
+    X0 = gaussian_vecs(k, 2 * L * L, 291)
+    res = smg.lobpcg_update(D, X0, None, 30, 1e-10)
+    X = res.eigvecs
+    assert np.abs(X.conj() @ X.T - np.eye(k)).max() < 1e-10
+    A = O.Dirac(L, U, 0.05, 1.0, 1.0, 0.0).op()
+    AX = np.array([A(x) for x in X])
+    ritz = np.linalg.eigvalsh(X.conj() @ AX.T)
+    print(f"ritz[:4] {ritz[:4]}, trlm[:4] {tr.eigvals[:4]}")
+    assert np.all(ritz >= tr.eigvals - 1e-10 * tr.eigvals[-1])
+    evo, _, ito = O.lobpcg(A, X0, 30, 1e-10)
+    assert res.iterations == ito and np.abs(res.eigvals - evo).max() < 1e-7 * np.abs(evo).max()

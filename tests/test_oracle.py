@@ -479,3 +479,22 @@ def test_golden_coarse_deflation(g):
     x, it = O.deflated_coarse_cg(A, vecs, ev, g["L16_c"], 200, 1e-12)
     assert np.abs(x - g["L16_defl_x"]).max() < 1e-11 * np.abs(x).max()
     assert np.abs(x - g["L16_dense_solve"]).max() < 1e-9 * np.abs(x).max()
+
+
+@pytest.mark.parametrize("its", [10, 40])
+def test_golden_lobpcg(g, its):
+    """lobpcg_update (src/eigensolver_core.cpp:279-533) from the stored start block:
+    the same iterations, Rayleigh quotients and vectors as the reference.  The
+    reference updates AX_i by -<X_j,X_i> X_j in its closing re-orthonormalisation,
+    so its Rayleigh quotients are not those of A; the restatement keeps that."""
+    D = O.Dirac(8, g["E8_U"], 0.05, 1.0, 1.0, 0.0)
+    ev, X, it = O.lobpcg(D.op(), g["E8_lob_X0"], its, 1e-10)
+    assert it == int(g[f"E8_lob{its}_it"][0])
+    assert np.abs(ev - g[f"E8_lob{its}_evals"]).max() < 1e-12
+    assert np.abs(X - g[f"E8_lob{its}_X"]).max() < 1e-11
+    # X is orthonormal and spans a space whose Ritz values approach the lowest modes
+    assert np.abs(X.conj() @ X.T - np.eye(6)).max() < 1e-12
+    low = np.linalg.eigvalsh(O.dense_galerkin(D.op()))[:6]
+    A = np.array([D.op()(x) for x in X])
+    ritz = np.linalg.eigvalsh(X.conj() @ A.T)
+    assert np.all(ritz >= low - 1e-12)
