@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--coarse-maxit", type=int, default=30)
     ap.add_argument("--coarse-tol", type=float, default=1e-1)
     ap.add_argument("--coarse-solver", type=int, default=0)
+    ap.add_argument("--coarse-bf16", type=int, default=1,
+                    help="BF16 vectors in the coarsest CG of the FP32 cycle (FP16 links either way)")
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--trunc", type=int, default=10)
     ap.add_argument("--inner-tol", type=float, default=1e-5)
@@ -181,7 +183,8 @@ def main():
     t0 = time.time()
     mg = smg.MGHierarchy(D, starts, n_levels=args.levels, blocks=blocks, k=(args.k,) * (args.levels - 1),
                          null_iters=args.null_iters, pre_smooth=args.pre, post_smooth=args.post,
-                         coarse_maxit=args.coarse_maxit, coarse_tol=args.coarse_tol, coarse_half=True,
+                         coarse_maxit=args.coarse_maxit, coarse_tol=args.coarse_tol,
+                         coarse_half=2 if args.coarse_bf16 else 1,
                          coarse_solver=args.coarse_solver, use_graph=bool(args.graph), chiral=bool(args.chiral),
                          w_cycle=bool(args.w_cycle))
     smg.synchronize()
@@ -267,7 +270,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (FP16 coarse links) in f64 refinement",
+        "dtype": "f32 (FP16 coarse links" + (", BF16 coarsest-CG vectors" if args.coarse_bf16 else "")
+                 + ") in f64 refinement",
         "data": "synthetic",
         "config": {"workload": f"L={L} Wilson-clover m={args.mass} c_sw=1, theta~N(0,{args.sigma}^2), {N} RHS, "
                                f"{args.levels}-level MG (4x4 then 2x2 aggregates, "
@@ -279,7 +283,8 @@ def main():
                                + f"), GCR({args.trunc}), tol 1e-10 true residual",
                    "L": L, "nrhs": N, "levels": args.levels, "k": args.k, "pre": args.pre, "post": args.post,
                    "coarse_maxit": args.coarse_maxit, "coarse_tol": args.coarse_tol,
-                   "coarse_solver": "dense inverse" if args.coarse_solver == 1 else "CG", "graph": bool(args.graph)},
+                   "coarse_solver": "dense inverse" if args.coarse_solver == 1 else "CG", "graph": bool(args.graph),
+                   "coarsest_cg_vectors": "bf16" if args.coarse_bf16 else "fp32"},
         "mixed": mixed,
         "fp64_solves_per_s": round(N / t_f64, 3) if fp64 else None,
         "fp64": fp64,

@@ -33,6 +33,7 @@ extern "C" {
 #define SMG_F64 0
 #define SMG_F32 1
 #define SMG_F16 2 /* coarse-link storage only (FP32 arithmetic) */
+#define SMG_BF16V 3 /* FP16 coarse links with BF16 vector storage (FP32 arithmetic) */
 
 /* Dirac operator kinds: D, D^dag, D^dag D, gamma5 D (src/dirac.h:115-152) */
 #define SMG_D 0
@@ -180,7 +181,8 @@ typedef struct {
     int w_cycle;
     int coarse_maxit;      /* coarsest CG */
     double coarse_tol;
-    int coarse_half;       /* FP32 cycle: coarse links in FP16 */
+    int coarse_half;       /* FP32 cycle: 1 = coarse links in FP16; 2 = also BF16 vectors in the
+                              batched coarsest CG (N >= 16), half its vector traffic */
     int coarse_solver;     /* coarsest level: 0 CG (coarse_maxit, coarse_tol); 1 dense inverse
                               (formed once at setup; coarsest dimension <= 8192) */
     int use_graph;         /* 1: replay the cycle from a captured hipGraph */
@@ -311,7 +313,9 @@ int smg_dirac_apply_batched_dev(smg_dirac* D, int kind, int prec, const void* sr
  * precision prec (SMG_F64, SMG_F32, or SMG_F16 = FP16 links with FP32
  * arithmetic).  For k in {8, 16, 32} and N in {16, 32, 64} the product runs on
  * the matrix cores (FP16 links: bf16 MFMA with the FP32 sources split in two
- * bf16 terms). */
+ * bf16 terms).  SMG_BF16V (host entry only; k in {8, 16, 32}, N in {16, 32, 64}):
+ * FP16 links with the sources and results rounded to BF16, the storage of the
+ * batched coarsest CG of a hierarchy with coarse_half = 2. */
 int smg_coarse_apply_batched(smg_coarse* C, int prec, int nrhs, const double* src, double* dst);
 /* the same on device arrays in the batched layout src/dst[(b*k + kk)*N + j]
  * (complex128 for SMG_F64, complex64 otherwise) */

@@ -20,7 +20,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsmg.so")
 
-F64, F32, F16 = 0, 1, 2
+F64, F32, F16, BF16V = 0, 1, 2, 3
 D_, DDAG, DDAGD, G5D = 0, 1, 2, 3
 OP_DDAGD, OP_SCHUR = 0, 1
 SMOOTH_MR, SMOOTH_RICHARDSON = 0, 1

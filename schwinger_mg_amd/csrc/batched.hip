@@ -654,14 +654,34 @@ __device__ __forceinline__ unsigned pack_bf(float a, float b, unsigned sel) {
     return __builtin_amdgcn_perm(__float_as_uint(b), __float_as_uint(a), sel);
 }
 
+// BF16 storage of a complex vector entry: one dword, the bf16 of Re in the low
+// half and of Im in the high half (the layout of a B-operand pair of the MFMA),
+// rounded to nearest even.
+__device__ __forceinline__ unsigned bf16_rne(float x) {
+    const unsigned u = __float_as_uint(x);
+    return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ unsigned pack_h(c32 v) { return bf16_rne(v.re) | (bf16_rne(v.im) << 16); }
+__device__ __forceinline__ c32 unpack_h(unsigned w) {
+    return c32{__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u)};
+}
+
 // One wave per coarse site; the 9K/16 chunks are pipelined, chunk m + 1's raw
-// sources and links in flight while chunk m is split and multiplied.
-template <typename TL, int EPI, int K, int NT>
+// sources and links in flight while chunk m is split and multiplied.  HV: source,
+// destination and the dot partner are BF16 vectors (cb16 entries), so the source
+// needs no split (two MFMAs per product instead of three); EPI_RESID is FP32 only.
+template <typename TL, int EPI, int K, int NT, bool HV = false>
 __global__ __launch_bounds__(kBlock)
-void k_b_coarse_bf16(CoarseView<TL> C, c32* __restrict__ dst,
-                                                          const c32* __restrict__ src, const c32* __restrict__ bvec,
-                                                          const c32* __restrict__ aux, FinB fin) {
+void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __restrict__ srcv,
+                     const c32* __restrict__ bvec, const void* __restrict__ auxv, FinB fin) {
     static_assert(std::is_same<TL, ch16>::value, "bf16x3 path: FP16 links");
+    static_assert(!HV || EPI != EPI_RESID, "BF16 vectors: no residual epilogue");
+    c32* __restrict__ dst = static_cast<c32*>(dstv);
+    const c32* __restrict__ src = static_cast<const c32*>(srcv);
+    const c32* __restrict__ aux = static_cast<const c32*>(auxv);
+    unsigned* __restrict__ dsth = static_cast<unsigned*>(dstv);
+    const unsigned* __restrict__ srch = static_cast<const unsigned*>(srcv);
+    const unsigned* __restrict__ auxh = static_cast<const unsigned*>(auxv);
     if (EPI == EPI_DOTS && fin.skip && *fin.skip) return;   // CG finished for every RHS
     constexpr int N = 16 * NT, RT = K / 8, NCH = (9 * K + 15) / 16;
     constexpr bool PAIR = NT % 2 == 0;
@@ -686,7 +706,9 @@ void k_b_coarse_bf16(CoarseView<TL> C, c32* __restrict__ dst,
             for (int t = 0; t < NT; t++) acc[r][t] = f32x4{0, 0, 0, 0};
         // raw operands of one chunk: the sources of four link columns (both RHS
         // tiles of a pair in one 16-byte load) and the row-tile link runs
-        using SrcV = typename std::conditional<PAIR, float4, float2>::type;
+        using SrcF = typename std::conditional<PAIR, float4, float2>::type;
+        using SrcH = typename std::conditional<PAIR, uint2, unsigned>::type;
+        using SrcV = typename std::conditional<HV, SrcH, SrcF>::type;
         struct Raw {
             SrcV s[4][NL];
             uint4 l[RT];
@@ -701,10 +723,13 @@ void k_b_coarse_bf16(CoarseView<TL> C, c32* __restrict__ dst,
             const int nb = (bx + dx + nx) % nx + ((by + dy + ny) % ny) * nx;
 #pragma unroll
             for (int e2 = 0; e2 < 4; e2++) {
-                const c32* row = src + ((size_t)nb * K + kj0 + e2) * N;
+                const size_t row = ((size_t)nb * K + kj0 + e2) * N;
 #pragma unroll
-                for (int u = 0; u < NL; u++)
-                    R.s[e2][u] = *reinterpret_cast<const SrcV*>(row + (PAIR ? 32 * u + 2 * lo : 16 * u + lo));
+                for (int u = 0; u < NL; u++) {
+                    const size_t o = row + (PAIR ? 32 * u + 2 * lo : 16 * u + lo);
+                    if constexpr (HV) R.s[e2][u] = *reinterpret_cast<const SrcV*>(srch + o);
+                    else R.s[e2][u] = *reinterpret_cast<const SrcV*>(src + o);
+                }
             }
             const unsigned* lrow = links + ((size_t)b * 9 + d) * (K * K) + a_ki * K + kj0;
 #pragma unroll
@@ -722,22 +747,34 @@ void k_b_coarse_bf16(CoarseView<TL> C, c32* __restrict__ dst,
             u32x4 bh[NT], bl[NT];
 #pragma unroll
             for (int e2 = 0; e2 < 4; e2++) {
-                c32 sv[NT];
+                if constexpr (HV) {   // the stored dwords are the operand pairs
 #pragma unroll
-                for (int u = 0; u < NL; u++) {
-                    if constexpr (PAIR) {
-                        const float4 w = R.s[e2][u];
-                        sv[2 * u] = c32{w.x, w.y};
-                        sv[2 * u + 1] = c32{w.z, w.w};
-                    } else {
-                        const float2 w = R.s[e2][u];
-                        sv[u] = c32{w.x, w.y};
+                    for (int u = 0; u < NL; u++) {
+                        if constexpr (PAIR) {
+                            bh[2 * u][e2] = R.s[e2][u].x;
+                            bh[2 * u + 1][e2] = R.s[e2][u].y;
+                        } else {
+                            bh[u][e2] = R.s[e2][u];
+                        }
                     }
-                }
+                } else {
+                    c32 sv[NT];
 #pragma unroll
-                for (int t = 0; t < NT; t++) {
-                    bh[t][e2] = pack_bf(sv[t].re, sv[t].im, 0x07060302u);
-                    bl[t][e2] = pack_bf(lo_part(sv[t].re), lo_part(sv[t].im), 0x07060302u);
+                    for (int u = 0; u < NL; u++) {
+                        if constexpr (PAIR) {
+                            const float4 w = R.s[e2][u];
+                            sv[2 * u] = c32{w.x, w.y};
+                            sv[2 * u + 1] = c32{w.z, w.w};
+                        } else {
+                            const float2 w = R.s[e2][u];
+                            sv[u] = c32{w.x, w.y};
+                        }
+                    }
+#pragma unroll
+                    for (int t = 0; t < NT; t++) {
+                        bh[t][e2] = pack_bf(sv[t].re, sv[t].im, 0x07060302u);
+                        bl[t][e2] = pack_bf(lo_part(sv[t].re), lo_part(sv[t].im), 0x07060302u);
+                    }
                 }
             }
 #pragma unroll
@@ -755,27 +792,32 @@ void k_b_coarse_bf16(CoarseView<TL> C, c32* __restrict__ dst,
                 const bf16x8 ahv = __builtin_bit_cast(bf16x8, ah), alv = __builtin_bit_cast(bf16x8, al);
 #pragma unroll
                 for (int t = 0; t < NT; t++) {
-                    const bf16x8 bhv = __builtin_bit_cast(bf16x8, bh[t]), blv = __builtin_bit_cast(bf16x8, bl[t]);
+                    const bf16x8 bhv = __builtin_bit_cast(bf16x8, bh[t]);
                     acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahv, bhv, acc[r][t], 0, 0, 0);
-                    acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahv, blv, acc[r][t], 0, 0, 0);
+                    if constexpr (!HV) {
+                        const bf16x8 blv = __builtin_bit_cast(bf16x8, bl[t]);
+                        acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahv, blv, acc[r][t], 0, 0, 0);
+                    }
                     acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alv, bhv, acc[r][t], 0, 0, 0);
                 }
             }
         };
         // the epilogue's operand (b for EPI_RESID, aux for EPI_DOTS) at this lane's
         // outputs, loaded while the last chunks are multiplied (PAIR layout)
-        float4 epo[PAIR ? RT : 1][2][PAIR ? NL : 1];
+        using EpoV = typename std::conditional<HV, uint2, float4>::type;
+        EpoV epo[PAIR ? RT : 1][2][PAIR ? NL : 1];
         auto load_epo = [&]() {
             if constexpr (PAIR && EPI != EPI_NONE) {
-                const c32* ep = EPI == EPI_RESID ? bvec : aux;
 #pragma unroll
                 for (int r = 0; r < RT; r++)
 #pragma unroll
                     for (int h = 0; h < 2; h++)
 #pragma unroll
-                        for (int u = 0; u < NL; u++)
-                            epo[r][h][u] = *reinterpret_cast<const float4*>(
-                                ep + ((size_t)b * K + 8 * r + 2 * hi + h) * N + 32 * u + 2 * lo);
+                        for (int u = 0; u < NL; u++) {
+                            const size_t o = ((size_t)b * K + 8 * r + 2 * hi + h) * N + 32 * u + 2 * lo;
+                            if constexpr (HV) epo[r][h][u] = *reinterpret_cast<const uint2*>(auxh + o);
+                            else epo[r][h][u] = *reinterpret_cast<const float4*>((EPI == EPI_RESID ? bvec : aux) + o);
+                        }
             }
         };
         if constexpr (NT <= 2) {
@@ -812,7 +854,34 @@ void k_b_coarse_bf16(CoarseView<TL> C, c32* __restrict__ dst,
                 const size_t row = ((size_t)b * K + ki) * N;
 #pragma unroll
                 for (int u = 0; u < NL; u++) {
-                    if constexpr (PAIR) {
+                    if constexpr (PAIR && HV) {
+                        const size_t idx = row + 32 * u + 2 * lo;
+                        const unsigned w0 = pack_h(c32{acc[r][2 * u][2 * h], acc[r][2 * u][2 * h + 1]});
+                        const unsigned w1 = pack_h(c32{acc[r][2 * u + 1][2 * h], acc[r][2 * u + 1][2 * h + 1]});
+                        *reinterpret_cast<uint2*>(dsth + idx) = uint2{w0, w1};
+                        if (EPI == EPI_DOTS) {   // dots of the stored (rounded) values
+                            const c32 o0 = unpack_h(w0), o1 = unpack_h(w1);
+                            const uint2 w = epo[r][h][u];
+                            const c32 p0 = cdot(o0, unpack_h(w.x)), p1 = cdot(o1, unpack_h(w.y));
+                            dacc[2 * u][0] += p0.re;
+                            dacc[2 * u][1] += p0.im;
+                            dacc[2 * u][2] += norm2(o0);
+                            dacc[2 * u + 1][0] += p1.re;
+                            dacc[2 * u + 1][1] += p1.im;
+                            dacc[2 * u + 1][2] += norm2(o1);
+                        }
+                    } else if constexpr (HV) {
+                        const size_t idx = row + 16 * u + lo;
+                        const unsigned w = pack_h(c32{acc[r][u][2 * h], acc[r][u][2 * h + 1]});
+                        dsth[idx] = w;
+                        if (EPI == EPI_DOTS) {
+                            const c32 o = unpack_h(w);
+                            const c32 p = cdot(o, unpack_h(auxh[idx]));
+                            dacc[u][0] += p.re;
+                            dacc[u][1] += p.im;
+                            dacc[u][2] += norm2(o);
+                        }
+                    } else if constexpr (PAIR) {
                         const size_t idx = row + 32 * u + 2 * lo;
                         c32 o0{acc[r][2 * u][2 * h], acc[r][2 * u][2 * h + 1]};
                         c32 o1{acc[r][2 * u + 1][2 * h], acc[r][2 * u + 1][2 * h + 1]};
@@ -875,18 +944,18 @@ void k_b_coarse_bf16(CoarseView<TL> C, c32* __restrict__ dst,
     }
 }
 
-template <typename TL, int K, int NT>
-static void coarse_bf16(const CoarseView<TL>& C, c32* dst, const c32* src, Epi epi, const c32* b, const c32* aux,
+template <typename TL, int K, int NT, bool HV = false>
+static void coarse_bf16(const CoarseView<TL>& C, void* dst, const void* src, Epi epi, const c32* b, const void* aux,
                         const FinB& fin, hipStream_t st) {
     const size_t waves = (size_t)C.nx * C.ny;
     const int nw = kBlock / 64;
     const int g = (int)std::min<size_t>((waves + nw - 1) / nw, epi == EPI_DOTS ? kRedGridCoarse : 1 << 16);
     if (epi == EPI_NONE)
-        k_b_coarse_bf16<TL, EPI_NONE, K, NT><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
-    else if (epi == EPI_RESID)
+        k_b_coarse_bf16<TL, EPI_NONE, K, NT, HV><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
+    else if (epi == EPI_DOTS)
+        k_b_coarse_bf16<TL, EPI_DOTS, K, NT, HV><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
+    else if constexpr (!HV)
         k_b_coarse_bf16<TL, EPI_RESID, K, NT><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
-    else
-        k_b_coarse_bf16<TL, EPI_DOTS, K, NT><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
 }
 
 template <typename T, typename TL, int K, int NT>
@@ -951,6 +1020,26 @@ void launch_b_coarse_apply(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>*
         k_b_coarse<T, TL, EPI_RESID><<<b_grid(items, 1 << 16), kBlock, 0, st>>>(C, dst, src, b, aux, fin, N);
     else
         k_b_coarse<T, TL, EPI_DOTS><<<b_grid(items, kRedGridCoarse), kBlock, 0, st>>>(C, dst, src, b, aux, fin, N);
+    SMG_LAUNCH_CHECK();
+}
+
+// The coarse apply on BF16 vectors (FP16 links): epi EPI_NONE or EPI_DOTS
+void launch_b_coarse_apply_h(const CoarseView<ch16>& C, cb16* dst, const cb16* src, Epi epi, const cb16* aux,
+                             const FinB& fin, int N, hipStream_t st) {
+    SMG_REQUIRE((C.k == 8 || C.k == 16 || C.k == 32) && (N == 16 || N == 32 || N == 64) && epi != EPI_RESID,
+                "BF16-vector coarse apply: k in {8,16,32}, N in {16,32,64}, no residual epilogue");
+    auto go = [&](auto kc, auto ntc) {
+        constexpr int K = decltype(kc)::value, NT = decltype(ntc)::value;
+        coarse_bf16<ch16, K, NT, true>(C, dst, src, epi, nullptr, aux, fin, st);
+    };
+    auto by_n = [&](auto kc) {
+        if (N == 16) go(kc, std::integral_constant<int, 1>{});
+        else if (N == 32) go(kc, std::integral_constant<int, 2>{});
+        else go(kc, std::integral_constant<int, 4>{});
+    };
+    if (C.k == 8) by_n(std::integral_constant<int, 8>{});
+    else if (C.k == 16) by_n(std::integral_constant<int, 16>{});
+    else by_n(std::integral_constant<int, 32>{});
     SMG_LAUNCH_CHECK();
 }
 
@@ -1545,6 +1634,144 @@ void launch_b_cg_xp(cplx<T>* x, cplx<T>* p, const cplx<T>* r, const double* a, c
         constexpr int P = decltype(pc)::value;
         k_b_cg_xp<T, P><<<b_grid(n * N / P, 1 << 16), kBlock, 0, st>>>(x, p, r, a, b, n * N / P, N, late);
     });
+    SMG_LAUNCH_CHECK();
+}
+
+// ----------------------------------------------------------------- BF16-vector CG
+// The coarsest-level CG of the FP32 cycle with r, p, Ap and x stored as BF16 (the
+// solve runs to a loose tolerance, 0.1 by default, which BF16's 8-bit mantissa
+// resolves; FP32 arithmetic throughout): each work-item takes a run of 4 right-hand
+// sides (one 16-byte access), N >= 16.  Semantics as k_b_cg_r / k_b_cg_xp.
+struct HRun {
+    c32 v[4];
+    __device__ __forceinline__ void load(const cb16* p) {
+        const uint4 w = *reinterpret_cast<const uint4*>(p);
+        v[0] = unpack_h(w.x);
+        v[1] = unpack_h(w.y);
+        v[2] = unpack_h(w.z);
+        v[3] = unpack_h(w.w);
+    }
+    __device__ __forceinline__ uint4 store(cb16* p) {   // returns the stored dwords
+        const uint4 w{pack_h(v[0]), pack_h(v[1]), pack_h(v[2]), pack_h(v[3])};
+        *reinterpret_cast<uint4*>(p) = w;
+        return w;
+    }
+};
+// r = p = bf16(b), x = 0, |r|^2 (of the stored r) finished by fin (FIN_CG_INIT)
+__global__ __launch_bounds__(kBlock) void k_h_cg_init(const c32* __restrict__ b, cb16* __restrict__ r,
+                                                      cb16* __restrict__ p, cb16* __restrict__ x, size_t runs,
+                                                      FinB fin, int N) {
+    double acc[1][4] = {};
+    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < runs; t += (size_t)gridDim.x * kBlock) {
+        const size_t i = t * 4;
+        const float4 b0 = *reinterpret_cast<const float4*>(b + i), b1 = *reinterpret_cast<const float4*>(b + i + 2);
+        HRun rv;
+        rv.v[0] = c32{b0.x, b0.y};
+        rv.v[1] = c32{b0.z, b0.w};
+        rv.v[2] = c32{b1.x, b1.y};
+        rv.v[3] = c32{b1.z, b1.w};
+        const uint4 w = rv.store(r + i);
+        *reinterpret_cast<uint4*>(p + i) = w;
+        *reinterpret_cast<uint4*>(x + i) = uint4{0u, 0u, 0u, 0u};
+        const unsigned ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[0][q] += (double)norm2(unpack_h(ws[q]));
+    }
+    double v[1];
+    runs_to_lanes<1, 4>(acc, v, N);
+    reduce_finish_b<1, 64>(v, fin, N, gridDim.x, blockIdx.x);
+}
+__global__ __launch_bounds__(kBlock) void k_h_cg_r(cb16* __restrict__ r, const cb16* __restrict__ Ap,
+                                                   const double* __restrict__ s, size_t runs, FinB fin, int N,
+                                                   int* __restrict__ late) {
+    if (fin.skip && *fin.skip) {
+        if (late && blockIdx.x == 0 && threadIdx.x == 0) *late = 1;
+        return;
+    }
+    double acc[1][4] = {};
+    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < runs; t += (size_t)gridDim.x * kBlock) {
+        const size_t i = t * 4;
+        c32 a[4];
+        run_coef<float, 4>(s, (int)(i & (N - 1)), a);
+        HRun rv, av;
+        rv.load(r + i);
+        av.load(Ap + i);
+#pragma unroll
+        for (int q = 0; q < 4; q++) rv.v[q] = rv.v[q] - a[q] * av.v[q];
+        const uint4 w = rv.store(r + i);
+        const unsigned ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[0][q] += (double)norm2(unpack_h(ws[q]));
+    }
+    double v[1];
+    runs_to_lanes<1, 4>(acc, v, N);
+    reduce_finish_b<1, 64>(v, fin, N, gridDim.x, blockIdx.x);
+}
+__global__ __launch_bounds__(kBlock) void k_h_cg_xp(cb16* __restrict__ x, cb16* __restrict__ p,
+                                                    const cb16* __restrict__ r, const double* __restrict__ sa,
+                                                    const double* __restrict__ sb, size_t runs, int N,
+                                                    const int* __restrict__ late) {
+    if (late && *late) return;
+    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < runs; t += (size_t)gridDim.x * kBlock) {
+        const size_t i = t * 4;
+        c32 a[4], b[4];
+        run_coef<float, 4>(sa, (int)(i & (N - 1)), a);
+        run_coef<float, 4>(sb, (int)(i & (N - 1)), b);
+        HRun xv, pv, rv;
+        xv.load(x + i);
+        pv.load(p + i);
+        rv.load(r + i);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            xv.v[q] = xv.v[q] + a[q] * pv.v[q];
+            pv.v[q] = rv.v[q] + b[q] * pv.v[q];
+        }
+        xv.store(x + i);
+        pv.store(p + i);
+    }
+}
+__global__ __launch_bounds__(kBlock) void k_h_to_f32(c32* __restrict__ out, const cb16* __restrict__ in,
+                                                     size_t runs) {
+    const size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= runs) return;
+    HRun v;
+    v.load(in + 4 * t);
+    *reinterpret_cast<float4*>(out + 4 * t) = float4{v.v[0].re, v.v[0].im, v.v[1].re, v.v[1].im};
+    *reinterpret_cast<float4*>(out + 4 * t + 2) = float4{v.v[2].re, v.v[2].im, v.v[3].re, v.v[3].im};
+}
+__global__ __launch_bounds__(kBlock) void k_h_from_f32(cb16* __restrict__ out, const c32* __restrict__ in,
+                                                       size_t runs) {
+    const size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= runs) return;
+    const float4 a = *reinterpret_cast<const float4*>(in + 4 * t), b = *reinterpret_cast<const float4*>(in + 4 * t + 2);
+    *reinterpret_cast<uint4*>(out + 4 * t) =
+        uint4{pack_h(c32{a.x, a.y}), pack_h(c32{a.z, a.w}), pack_h(c32{b.x, b.y}), pack_h(c32{b.z, b.w})};
+}
+void launch_h_from_f32(cb16* out, const c32* in, size_t n, int N, hipStream_t st) {
+    SMG_REQUIRE(N % 4 == 0, "BF16 conversion: N a multiple of 4");
+    const size_t runs = n * N / 4;
+    k_h_from_f32<<<(int)((runs + kBlock - 1) / kBlock), kBlock, 0, st>>>(out, in, runs);
+    SMG_LAUNCH_CHECK();
+}
+void launch_h_cg_init(const c32* b, cb16* r, cb16* p, cb16* x, size_t n, const FinB& fin, int N, hipStream_t st) {
+    SMG_REQUIRE(N >= 16 && N <= 64 && (N & (N - 1)) == 0, "BF16 CG: N in {16, 32, 64}");
+    k_h_cg_init<<<b_grid(n * N / 4, kRedGridVec), kBlock, 0, st>>>(b, r, p, x, n * N / 4, fin, N);
+    SMG_LAUNCH_CHECK();
+}
+void launch_h_cg_r(cb16* r, const cb16* Ap, const double* a, size_t n, const FinB& fin, int N, int* late,
+                   hipStream_t st) {
+    k_h_cg_r<<<b_grid(n * N / 4, kRedGridVec), kBlock, 0, st>>>(r, Ap, a, n * N / 4, fin, N, late);
+    SMG_LAUNCH_CHECK();
+}
+void launch_h_cg_xp(cb16* x, cb16* p, const cb16* r, const double* a, const double* b, size_t n, int N,
+                    const int* late, hipStream_t st) {
+    k_h_cg_xp<<<b_grid(n * N / 4, 1 << 16), kBlock, 0, st>>>(x, p, r, a, b, n * N / 4, N, late);
+    SMG_LAUNCH_CHECK();
+}
+void launch_h_to_f32(c32* out, const cb16* in, size_t n, int N, hipStream_t st) {
+    SMG_REQUIRE(N % 4 == 0, "BF16 conversion: N a multiple of 4");
+    const size_t runs = n * N / 4;
+    k_h_to_f32<<<(int)((runs + kBlock - 1) / kBlock), kBlock, 0, st>>>(out, in, runs);
     SMG_LAUNCH_CHECK();
 }
 

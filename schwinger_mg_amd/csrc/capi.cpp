@@ -785,10 +785,20 @@ int smg_coarse_apply_batched(smg_coarse* C, int prec, int nrhs, const double* sr
     return guard([&] {
         SMG_REQUIRE(C && src && dst, "bad arguments");
         check_nrhs(nrhs, 64);
-        SMG_REQUIRE(prec == SMG_F64 || prec == SMG_F32 || prec == SMG_F16, "prec must be SMG_F64, SMG_F32 or SMG_F16");
+        SMG_REQUIRE(prec == SMG_F64 || prec == SMG_F32 || prec == SMG_F16 || prec == SMG_BF16V,
+                    "prec must be SMG_F64, SMG_F32, SMG_F16 or SMG_BF16V");
         const size_t n = C->C->n(), nN = n * nrhs;
         Field<double> s = up(src, nN), d(nN);
-        if (prec == SMG_F64) {
+        if (prec == SMG_BF16V) {
+            Field<float> sb(nN), db(nN);
+            DevMem sh(nN * sizeof(cb16)), dh(nN * sizeof(cb16));
+            launch_b_pack<float, double>(sb.p(), s.p(), n, nrhs, S());
+            launch_h_from_f32(sh.as<cb16>(), sb.p(), n, nrhs, S());
+            launch_b_coarse_apply_h(CoarseView<ch16>{C->C->l16.as<ch16>(), C->C->nx, C->C->ny, C->C->k},
+                                    dh.as<cb16>(), sh.as<cb16>(), EPI_NONE, nullptr, FinB(), nrhs, S());
+            launch_h_to_f32(db.p(), dh.as<cb16>(), n, nrhs, S());
+            launch_b_unpack<double, float>(d.p(), db.p(), n, nrhs, S());
+        } else if (prec == SMG_F64) {
             Field<double> sb(nN), db(nN);
             launch_b_pack<double, double>(sb.p(), s.p(), n, nrhs, S());
             CoarseLinOpB<double, c64> op(*C->C, nrhs);

@@ -120,6 +120,31 @@ static void cg_b(LinOpB<T>& A, const cplx<T>* b, cplx<T>* x, cplx<T>* r, cplx<T>
     }
 }
 
+// The same CG on the FP16-link coarse operator with BF16 vectors (k_h_* kernels):
+// half the vector traffic of cg_b<float>, for the loose coarsest-level tolerance
+static void cg_h(const CoarseOperator& C, const c32* b, c32* x, cb16* r, cb16* p, cb16* Ap, cb16* xh, int N,
+                 int maxit, double tol, double* sc, hipStream_t st) {
+    const size_t n = C.n();
+    double* alpha = sc;
+    double* beta = sc + 2 * 64;
+    double* aux = sc + 4 * 64;
+    int* done = reinterpret_cast<int*>(aux + 16 * 64);
+    int* late = reinterpret_cast<int*>(aux + 16 * 64 + 1);
+    launch_b_cg_init(aux, 16, N, tol * tol, done, late, st);
+    launch_h_cg_init(b, r, p, xh, n, finb(FIN_CG_INIT, nullptr, 0, aux), N, st);
+    FinB fa = finb(FIN_CG_ALPHA, alpha, 2, aux), fbt = finb(FIN_CG_BETA, beta, 2, aux);
+    fa.skip = done;
+    fbt.skip = done;
+    fbt.done = done;
+    const CoarseView<ch16> v{C.l16.as<ch16>(), C.nx, C.ny, C.k};
+    for (int it = 0; it < maxit; it++) {
+        launch_b_coarse_apply_h(v, Ap, p, EPI_DOTS, p, fa, N, st);
+        launch_h_cg_r(r, Ap, alpha, n, fbt, N, late, st);
+        launch_h_cg_xp(xh, p, r, alpha, beta, n, N, late, st);
+    }
+    launch_h_to_f32(x, xh, n, N, st);
+}
+
 // ------------------------------------------------------------------ cycle (batched)
 template <typename T>
 struct CycleB {
@@ -131,9 +156,22 @@ struct CycleB {
     std::vector<std::unique_ptr<ScalarsB>> sc;
     int nlev;
     Field<T> dwork;   // column-slice partials of the matrix-core dense coarsest product
+    bool hcg = false;  // coarsest CG on BF16 vectors (FP32 cycle, cfg.coarse_half == 2)
+    DevMem hr, hp, hAp, hx;
     CycleB(MG& m, int N_)
         : mg(m), N(N_), nlev(m.cfg.n_levels), dwork(std::max<size_t>(1, b_dense_work(m.dims[m.cfg.n_levels - 1], N_))) {
         SMG_REQUIRE(N >= 1 && N <= 64 && (N & (N - 1)) == 0, "batched solves take a power-of-two count <= 64");
+        if constexpr (std::is_same<T, float>::value) {
+            const int kc = m.C.empty() ? 0 : m.C.back()->k;
+            if (m.cfg.coarse_half == 2 && m.cfg.coarse_solver == 0 && N >= 16 && (kc == 8 || kc == 16 || kc == 32)) {
+                hcg = true;
+                const size_t bytes = m.dims[nlev - 1] * N * sizeof(cb16);
+                hr.alloc(bytes);
+                hp.alloc(bytes);
+                hAp.alloc(bytes);
+                hx.alloc(bytes);
+            }
+        }
         own.emplace_back(new DdagDOpB<T>(*m.D, N));
         ops.push_back(own.back().get());
         for (int l = 1; l < nlev; l++) {
@@ -169,6 +207,13 @@ struct CycleB {
                 else inv = mg.inv32.p();
                 launch_b_dense_apply<T>(inv, b, x, mg.dims[lev], N,
                                         b_dense_work(mg.dims[lev], N) ? dwork.p() : nullptr, st);
+            } else if constexpr (std::is_same<T, float>::value) {
+                if (hcg)
+                    cg_h(*mg.C[lev - 1], b, x, hr.as<cb16>(), hp.as<cb16>(), hAp.as<cb16>(), hx.as<cb16>(), N,
+                         mg.cfg.coarse_maxit, mg.cfg.coarse_tol, sc[lev]->p(), st);
+                else
+                    cg_b<T>(A, b, x, rs[lev].p(), ps[lev].p(), Ars[lev].p(), mg.cfg.coarse_maxit, mg.cfg.coarse_tol,
+                            sc[lev]->p(), st);
             } else {
                 cg_b<T>(A, b, x, rs[lev].p(), ps[lev].p(), Ars[lev].p(), mg.cfg.coarse_maxit, mg.cfg.coarse_tol,
                         sc[lev]->p(), st);

@@ -122,3 +122,63 @@ def test_batched_gcr_solves(smg, prec, k, N, chiral):
         assert np.linalg.norm(B[j] - A(X[j])) / np.linalg.norm(B[j]) < 1e-10
         r = smg.fcg_solve(D, B[j], mg, 200, 1e-12, 8)
         assert relerr(X[j], r.solution) < 1e-8
+
+
+def _bf16(x):
+    """Round each real component to BF16 (nearest even), as the device stores it."""
+    f = np.ascontiguousarray(x.view(np.float64).astype(np.float32))
+    u = f.view(np.uint32).astype(np.uint64)
+    r = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16).astype(np.uint32)
+    return r.view(np.float32).astype(np.float64).view(np.complex128).reshape(x.shape)
+
+
+@pytest.mark.parametrize("k,N", [(8, 16), (16, 32), (32, 32), (32, 64)])
+def test_coarse_apply_bf16_vectors(smg, k, N):
+    """FP16 links with BF16 sources and results (the storage of the coarsest CG with
+    coarse_half = 2): each result is the BF16 rounding of the FP16-link product of
+    the BF16-rounded sources, to the nearest BF16 or its neighbour."""
+    banner(f"coarse apply BF16 vectors k={k} N={N}")
+    L, b = 32, 4
+    U = random_gauge(L, 0.4, 60 + k)
+    chiral = k == 32
+    nv = k // 2 if chiral else k
+    Po = O.Prolongator(L, L, 2, b, b, k, chiral=chiral).build(gaussian_vecs(nv, 2 * L * L, 61).ravel())
+    Co = O.Coarse(O.Dirac(L, U, 0.05, 1.0, 1.0, 0.0).op(), Po, 1)
+    C = smg.SparseCoarseOp.from_links(L // b, L // b, k, Co.links())
+    S = gaussian_vecs(N, C.dim, 62)
+    Sr = _bf16(S)
+    exact = np.stack([C.apply_to(Sr[j], smg.F16) for j in range(N)])   # FP16 links, FP32 arithmetic
+    got = C.apply_batched(S, smg.BF16V)
+    assert np.array_equal(got, _bf16(got))   # every result is a BF16 value
+    want = _bf16(exact)
+    g, w, e = got.view(np.float64), want.view(np.float64), exact.view(np.float64)
+    same = np.mean(g == w)
+    ulp = np.maximum(np.abs(e), np.abs(g)) * 2.0 ** -7   # one BF16 step at the value (upper bound)
+    print(f"k={k} N={N}: {100 * same:.2f}% bit-equal to the rounded product")
+    assert same > 0.99 and np.all(np.abs(g - w) <= ulp + 1e-30)
+
+
+@pytest.mark.parametrize("w_cycle", [False, True])
+def test_mixed_solve_with_bf16_coarsest_cg(smg, w_cycle):
+    """coarse_half = 2 (BF16 vectors in the coarsest CG of the batched FP32 cycle):
+    the mixed batched solve reaches the same 1e-10 true residual as with FP32
+    vectors, in about as many iterations, and the same solution."""
+    banner(f"mixed solve, BF16 coarsest CG, w_cycle={w_cycle}")
+    L, k, N = 32, 32, 32
+    V = L * L
+    U = random_gauge(L, 0.4, 70)
+    D = smg.DiracOp(L, U, 0.05, 1.0, 1.0, 0.0)
+    dims = [2 * V, (L // 4) ** 2 * k]
+    starts = np.concatenate([gaussian_vecs(k // 2, n, 71 + i).ravel() for i, n in enumerate(dims)])
+    B = gaussian_vecs(N, 2 * V, 72)
+    out = {}
+    for half in (1, 2):
+        mg = smg.MGHierarchy(D, starts, n_levels=3, blocks=((4, 4), (2, 2)), k=(k, k), null_iters=5,
+                             coarse_maxit=30, coarse_tol=0.1, coarse_solver=0, coarse_half=half, chiral=True,
+                             w_cycle=w_cycle)
+        X, info, tr = smg.gcr_solve_batched(D, B, mg, 200, 1e-10, 10, prec=smg.F32, inner_tol=1e-5)
+        assert (tr < 1e-10).all(), tr
+        out[half] = (X, info.iterations)
+    print(f"iterations: FP32 coarsest CG {out[1][1]}, BF16 coarsest CG {out[2][1]}")
+    assert out[2][1] <= out[1][1] + 3
+    assert relerr(out[2][0], out[1][0]) < 1e-8
