@@ -1044,6 +1044,153 @@ inline HMCResult hmc_trajectory(GaugeField& gauge, const Lattice& lat, double ma
     return {accept, dH, h1[0] - h0[0], h1[1] - h0[1], h1[2] - h0[2], h1[3] - h0[3], cg};
 }
 
+// ------------------------------------------------------------------ multi-timescale MG HMC, src/hmc_multiscale.cpp
+// CoarseDeflState, src/hmc.h:308-324
+struct CoarseDeflState {
+    std::vector<Vec> eigvecs;
+    std::vector<double> eigvals;
+    Vec lowmode_solution(const Vec& phi, const Prolongator& P) const {
+        Vec pc = P.restrict_vec(phi);
+        const int nc = (int)pc.size();
+        Vec xc(nc, cx(0.0, 0.0));
+        for (size_t i = 0; i < eigvecs.size(); i++)
+            if (eigvals[i] > 1e-14) {
+                cx c = 0.0;
+                for (int t = 0; t < nc; t++) c += std::conj(eigvecs[i][t]) * pc[t];
+                c /= eigvals[i];
+                for (int t = 0; t < nc; t++) xc[t] += c * eigvecs[i][t];
+            }
+        return P.prolong(xc);
+    }
+};
+enum class OuterIntegrator { Leapfrog, Omelyan, FGI, FGI_QPQPQ };   // src/hmc.h:326-335
+struct MGMultiScaleParams {   // src/hmc.h:346-365
+    double beta;
+    double tau = 1.0;
+    int n_outer = 4;
+    int n_inner = 5;
+    int cg_maxiter = 500;
+    double cg_tol = 1e-10;
+    int inner_smooth = 3;
+    OuterIntegrator outer_type = OuterIntegrator::Leapfrog;
+    int defl_refresh = 0;
+    int mg_perturb_freq = 0;
+    double c_sw = 0.0;
+    double mu_t = 0.0;
+    int inner_tracking = 0;
+    bool use_eo = false;
+};
+struct MGMultiScaleResult {   // src/hmc.h:367-376
+    bool accepted;
+    double dH;
+    double dKE, dSG, dSF;
+    int highmode_cg_iters;
+    int lowmode_force_evals;
+    double highmode_time;
+    double lowmode_time;
+};
+struct EigenForecastState;   // forecasting is not provided by this library
+
+namespace detail {
+// an smg_coarse handle on P's coarse space holding the deflation state (its operator
+// is not used)
+inline std::shared_ptr<smg_coarse> defl_handle(const Prolongator& P, const CoarseDeflState& cd) {
+    Vec zero((size_t)P.nblocks * 9 * P.k_vec * P.k_vec, cx(0.0, 0.0));
+    smg_coarse* h = nullptr;
+    check(smg_coarse_create_links(&h, P.nbx, P.nby, P.k_vec, raw(zero)));
+    std::shared_ptr<smg_coarse> C(h, [](smg_coarse* p) { smg_coarse_destroy(p); });
+    const int nd = (int)cd.eigvals.size();
+    Vec all((size_t)nd * P.coarse_dim);
+    for (int i = 0; i < nd; i++) std::copy(cd.eigvecs[i].begin(), cd.eigvecs[i].end(), all.begin() + (size_t)i * P.coarse_dim);
+    check(smg_coarse_set_deflation(h, nd, nd ? cd.eigvals.data() : nullptr, nd ? raw(all) : nullptr));
+    return C;
+}
+inline void defl_read(smg_coarse* C, int dim, CoarseDeflState& cd) {
+    int nd = 0;
+    check(smg_coarse_deflation(C, &nd, nullptr, nullptr));
+    cd.eigvals.assign(nd, 0.0);
+    Vec all((size_t)nd * dim);
+    if (nd) check(smg_coarse_deflation(C, &nd, cd.eigvals.data(), raw(all)));
+    cd.eigvecs.assign(nd, Vec());
+    for (int i = 0; i < nd; i++) cd.eigvecs[i].assign(all.begin() + (size_t)i * dim, all.begin() + (size_t)(i + 1) * dim);
+}
+}  // namespace detail
+
+// coarse_lowmode_force, src/hmc_multiscale.cpp:237-276
+inline void coarse_lowmode_force(const DiracOp& D, const CoarseDeflState& cdefl, const Prolongator& P, const Vec& phi,
+                                 std::array<RVec, 2>& force, int smooth_iters = 0) {
+    auto C = detail::defl_handle(P, cdefl);
+    RVec F(2 * (size_t)D.lat.V);
+    detail::check(smg_coarse_lowmode_force(D.handle(), P.handle(), C.get(), detail::raw(phi), smooth_iters, F.data()));
+    detail::to_force(F, D.lat.V, force);
+}
+// evolve_coarse_deflation without forecast, src/hmc_multiscale.cpp:278-338
+inline void evolve_coarse_deflation(CoarseDeflState& cdefl, const SparseCoarseOp& Ac_new,
+                                    EigenForecastState* forecast = nullptr) {
+    if (forecast) throw std::invalid_argument("evolve_coarse_deflation: forecasting is not provided");
+    if (cdefl.eigvecs.empty()) return;
+    const int nd = (int)cdefl.eigvals.size();
+    Vec zero((size_t)Ac_new.nblocks * 9 * Ac_new.k_vec * Ac_new.k_vec, cx(0.0, 0.0));
+    smg_coarse* h = nullptr;
+    detail::check(smg_coarse_create_links(&h, Ac_new.nbx, Ac_new.nby, Ac_new.k_vec, detail::raw(zero)));
+    std::shared_ptr<smg_coarse> C(h, [](smg_coarse* p) { smg_coarse_destroy(p); });
+    Vec all((size_t)nd * Ac_new.dim);
+    for (int i = 0; i < nd; i++) std::copy(cdefl.eigvecs[i].begin(), cdefl.eigvecs[i].end(), all.begin() + (size_t)i * Ac_new.dim);
+    detail::check(smg_coarse_set_deflation(h, nd, cdefl.eigvals.data(), detail::raw(all)));
+    detail::check(smg_coarse_evolve_deflation(h, Ac_new.handle()));
+    detail::defl_read(h, Ac_new.dim, cdefl);
+}
+
+// hmc_trajectory_mg_multiscale, src/hmc_multiscale.cpp:340-888: leapfrog or Omelyan outer
+// integrator; mg_precond is MGHierarchy::as_precond() (its FP64 cycle preconditions the
+// outer solves), identity_precond() or empty.  The reference's rng order is kept: the
+// momenta, two pseudofermions (the second is used), then the Metropolis uniform when
+// dH >= 0.  Forecasting, pre_solve, prolongator perturbation, tracking and the FGI
+// integrators are not provided.
+inline MGMultiScaleResult hmc_trajectory_mg_multiscale(GaugeField& gauge, const Lattice& lat, double mass,
+                                                       double wilson_r, const MGMultiScaleParams& params,
+                                                       CoarseDeflState& cdefl, Prolongator& P,
+                                                       std::function<Vec(const Vec&)>& mg_precond, std::mt19937& rng,
+                                                       EigenForecastState* forecast = nullptr,
+                                                       const std::function<void()>* pre_solve = nullptr,
+                                                       MGHierarchy* mg_hierarchy = nullptr,
+                                                       TrackingState* tracking = nullptr) {
+    if (forecast || pre_solve || mg_hierarchy || tracking || params.mg_perturb_freq || params.inner_tracking ||
+        params.use_eo)
+        throw std::invalid_argument("hmc_trajectory_mg_multiscale: forecasting, pre_solve, perturbation, tracking "
+                                    "and even-odd are not provided");
+    if (params.outer_type != OuterIntegrator::Leapfrog && params.outer_type != OuterIntegrator::Omelyan)
+        throw std::invalid_argument("hmc_trajectory_mg_multiscale: the FGI integrators are not provided");
+    const bool ident = !mg_precond || mg_precond.target<IdentityPrecond>() != nullptr;
+    MGHierarchy* mg = ident ? nullptr : detail::mg_of(mg_precond, "hmc_trajectory_mg_multiscale");
+    const std::array<Vec, 2> old = gauge.U;
+    MomentumField mom(lat);
+    mom.randomise(rng);
+    (void)random_vec(lat.ndof, rng);   // the first generate_pseudofermion, overwritten (:380-383)
+    const Vec eta = random_vec(lat.ndof, rng);
+    DiracOp D(lat, gauge, mass, wilson_r, params.c_sw, params.mu_t);
+    auto C = detail::defl_handle(P, cdefl);
+    smg_ms_params cp{params.beta, params.tau, params.n_outer, params.n_inner,
+                     params.outer_type == OuterIntegrator::Omelyan ? 1 : 0, params.inner_smooth, params.defl_refresh,
+                     params.cg_maxiter, params.cg_tol, 1};
+    const RVec pi0 = detail::mom_flat(mom);
+    smg_ms_result r{};
+    detail::check(smg_hmc_ms_trajectory(D.handle(), mg ? mg->handle() : nullptr, P.handle(), C.get(), &cp, pi0.data(),
+                                        detail::raw(eta), 0.0, &r));
+    detail::defl_read(C.get(), P.coarse_dim, cdefl);
+    std::uniform_real_distribution<double> uniform(0.0, 1.0);
+    const bool accept = (r.dH < 0) || (uniform(rng) < std::exp(-r.dH));
+    if (accept) {
+        Vec u(2 * (size_t)lat.V);
+        detail::check(smg_dirac_get_gauge(D.handle(), detail::raw(u)));
+        gauge.U[0].assign(u.begin(), u.begin() + lat.V);
+        gauge.U[1].assign(u.begin() + lat.V, u.end());
+    } else {
+        gauge.U = old;
+    }
+    return {accept, r.dH, r.dKE, r.dSG, r.dSF, r.highmode_cg_iters, r.lowmode_force_evals, 0.0, 0.0};
+}
+
 // reversibility_test_plain, src/hmc_trajectory.cpp:416-509 (the gauge field is restored)
 inline ReversibilityResult reversibility_test_plain(GaugeField& gauge, const Lattice& lat, double mass,
                                                     double wilson_r, const HMCParams& params, std::mt19937& rng,

@@ -388,6 +388,51 @@ int smg_hmc_hamiltonian(smg_dirac* D, int use_eo, double beta, const double* pi,
 int smg_hmc_trajectory(smg_dirac* D, const smg_hmc_params* p, const double* pi0, const double* eta, double u,
                        smg_hmc_result* res);
 
+/* ------------------------------------------------------------------ multi-timescale MG HMC
+ * hmc_trajectory_mg_multiscale (src/hmc_multiscale.cpp:340-888) without eigenspace
+ * tracking, forecasting or prolongator perturbation: the outer integrator (leapfrog,
+ * or Omelyan with lambda = 0.1932) kicks with F_g + F_f - F_low, F_f from the CG solve
+ * of D^dag D X = phi (preconditioned by mg's FP64 cycle when mg is given); the inner
+ * leapfrog, n_inner steps per outer half-step, kicks with F_low =
+ * coarse_lowmode_force.  The deflation state (the reference's CoarseDeflState) is the
+ * deflation space of an smg_coarse handle on P's coarse space (smg_coarse_setup_deflation
+ * or smg_coarse_set_deflation); every defl_refresh-th inner step it is Rayleigh-Ritz
+ * evolved on the Galerkin operator of P on the current gauge field and stays evolved,
+ * as in the reference, also when the trajectory is rejected. */
+typedef struct {
+    double beta, tau;
+    int n_outer, n_inner;
+    int omelyan;       /* outer integrator: 0 leapfrog, 1 Omelyan */
+    int inner_smooth;  /* MR sweeps around the low-mode solution (0: the raw projection) */
+    int defl_refresh;  /* Rayleigh-Ritz refresh every N inner steps (0: never) */
+    int cg_maxiter;
+    double cg_tol;
+    int force_accept;
+} smg_ms_params;       /* MGMultiScaleParams, src/hmc.h:346-365 (the used fields) */
+typedef struct {
+    int accepted;
+    double dH, dKE, dSG, dSF;
+    int highmode_cg_iters;   /* all outer and Hamiltonian solves */
+    int lowmode_force_evals; /* inner-force evaluations */
+} smg_ms_result;             /* MGMultiScaleResult, src/hmc.h:367-376 (without the timings) */
+/* coarse_lowmode_force (src/hmc_multiscale.cpp:237-276): F = -dS/dtheta of the low-mode
+ * solution of D^dag D x = phi through P and defl's deflation space (smooth_iters MR
+ * sweeps before and after, 0 for the raw projection); F[mu*V + s] */
+int smg_coarse_lowmode_force(smg_dirac* D, smg_prolongator* P, smg_coarse* defl, const double* phi, int smooth_iters,
+                             double* F);
+/* evolve_coarse_deflation without forecast (src/hmc_multiscale.cpp:278-338): defl's
+ * space rotated by rr_evolve (src/eigen_evolve.cpp:48-108) on A_new */
+int smg_coarse_evolve_deflation(smg_coarse* defl, smg_coarse* A_new);
+/* the molecular dynamics from momenta pi (2V reals, in/out) and phi (2V complex); the
+ * gauge field of D evolves in place */
+int smg_hmc_ms_evolve(smg_dirac* D, smg_mg* mg, smg_prolongator* P, smg_coarse* defl, const smg_ms_params* p,
+                      double* pi, const double* phi, int* cg_iters, int* lowmode_evals);
+/* the trajectory: pi0 and eta are the momenta and the Gaussian of the pseudofermion
+ * phi = D^dag eta (the reference draws two pseudofermions and keeps the second; eta is
+ * that second draw), u the Metropolis uniform */
+int smg_hmc_ms_trajectory(smg_dirac* D, smg_mg* mg, smg_prolongator* P, smg_coarse* defl, const smg_ms_params* p,
+                          const double* pi0, const double* eta, double u, smg_ms_result* res);
+
 /* ------------------------------------------------------------------ gauge field files (host I/O)
  * Links are U[mu*V + s] (2V complex).  Host-only: no GPU is touched. */
 /* GaugeField::save / load, src/gauge.h:38-57: int32 V, then U[0] and U[1] as complex

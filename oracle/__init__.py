@@ -60,6 +60,11 @@ def lib():
         L.orc_hmc_hamiltonian.restype = _I
         L.orc_hmc_hamiltonian.argtypes = [_I, _P, _D, _D, _D, _D, _D, _I, _D, _P, _P, _P]
         L.orc_lanczos_eigen.argtypes = [_I, _P, _P, _P]
+        L.orc_rr_evolve.argtypes = [_P, _I, _P, _P]
+        L.orc_coarse_lowmode_force.argtypes = [_P, _P, _I, _P, _P, _P, _I, _P]
+        L.orc_hmc_ms_evolve.restype = _I
+        L.orc_hmc_ms_evolve.argtypes = [_I, _P, _D, _D, _D, _D, _D, _D, _I, _I, _I, _I, _I, _I, _D, _P, _I, _P, _P,
+                                        _P, _P, _P]
         L.orc_lobpcg.restype = _I
         L.orc_lobpcg.argtypes = [_P, _P, _I, _P, _I, _D, _P, _P]
         L.orc_cg_deflated.restype = _I
@@ -567,3 +572,34 @@ def lobpcg(op, X0, max_iter, tol, mg=None):
     ev = np.zeros(k)
     it = lib().orc_lobpcg(op.h, mg.h if mg is not None else None, k, _ptr(X0), max_iter, tol, _ptr(X), _ptr(ev))
     return ev, X, it
+
+
+def rr_evolve(op, V):
+    """rr_evolve: (rotated vectors [k][n], ascending Ritz values)."""
+    V = np.ascontiguousarray(V, np.complex128).copy()
+    vals = np.zeros(V.shape[0])
+    lib().orc_rr_evolve(op.h, V.shape[0], _ptr(V), _ptr(vals))
+    return V, vals
+
+
+def coarse_lowmode_force(D, P, V, vals, phi, smooth):
+    """coarse_lowmode_force: F[2V] (F[mu*V + s])."""
+    V = np.ascontiguousarray(V, np.complex128)
+    F = np.zeros(2 * D.L * D.L)
+    lib().orc_coarse_lowmode_force(D.h, P.h, V.shape[0], _ptr(V), _ptr(np.ascontiguousarray(vals, np.float64)),
+                                   _ptr(_c(phi)), smooth, _ptr(F))
+    return F
+
+
+def hmc_ms_evolve(L, U, mass, r, csw, mu_t, beta, tau, n_outer, n_inner, omelyan, inner_smooth, defl_refresh,
+                  cg_maxit, cg_tol, P, V, vals, pi, phi):
+    """The multi-timescale MD: (U, pi, V, vals, outer CG iterations, low-mode evaluations)."""
+    U = _c(U).copy()
+    pi = np.ascontiguousarray(pi, np.float64).copy()
+    V = np.ascontiguousarray(V, np.complex128).copy()
+    vals = np.ascontiguousarray(vals, np.float64).copy()
+    nl = ctypes.c_int()
+    it = lib().orc_hmc_ms_evolve(L, _ptr(U), mass, r, csw, mu_t, beta, tau, n_outer, n_inner, omelyan, inner_smooth,
+                                 defl_refresh, cg_maxit, cg_tol, P.h, V.shape[0], _ptr(V), _ptr(vals), _ptr(pi),
+                                 _ptr(_c(phi)), ctypes.byref(nl))
+    return U, pi, V, vals, it, nl.value

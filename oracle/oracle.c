@@ -1428,6 +1428,162 @@ int orc_hmc_evolve(int L, ocx* U, double mass, double r, double csw, double mu_t
     return cg;
 }
 
+/* rr_evolve, src/eigen_evolve.cpp:48-108: Rayleigh-Ritz of the k vectors V [k][n]
+ * (in/out) on A: M_ij = <v_i, A v_j> (i <= j, M_ji = conj), diagonalised by
+ * orc_lanczos_eigen; v_i <- sum_j evecs[j][i] v_j, vals <- the ascending eigenvalues */
+void orc_rr_evolve(const orc_op* A, int k, ocx* V, double* vals) {
+    int n = A->n;
+    ocx* AV = xcalloc((size_t)k * n, sizeof(ocx));
+    for (int i = 0; i < k; i++) A->fn(A, V + (size_t)i * n, AV + (size_t)i * n);
+    ocx* M = xcalloc((size_t)k * k, sizeof(ocx));   /* M[j*k + i] = M_ij (columns) */
+    for (int j = 0; j < k; j++)
+        for (int i = 0; i <= j; i++) {
+            ocx val = orc_dot(n, V + (size_t)i * n, AV + (size_t)j * n);
+            M[(size_t)j * k + i] = val;
+            if (i != j) M[(size_t)i * k + j] = conj(val);
+        }
+    double* ev = xcalloc(k, sizeof(double));
+    ocx* Z = xcalloc((size_t)k * k, sizeof(ocx));   /* Z[row*k + col] */
+    orc_lanczos_eigen(k, M, ev, Z);
+    ocx* W = xcalloc((size_t)k * n, sizeof(ocx));
+    for (int i = 0; i < k; i++) {
+        for (int j = 0; j < k; j++) {
+            ocx c = Z[(size_t)j * k + i];
+            for (int t = 0; t < n; t++) W[(size_t)i * n + t] += c * V[(size_t)j * n + t];
+        }
+        vals[i] = ev[i];
+    }
+    memcpy(V, W, (size_t)k * n * sizeof(ocx));
+    free(AV); free(M); free(ev); free(Z); free(W);
+}
+
+/* CoarseDeflState::lowmode_solution, src/hmc.h:311-323: P (sum_i v_i v_i^dag / l_i) P^dag phi
+ * over the l_i > 1e-14 */
+static void lowmode_solution(const orc_prolong* P, int k, const ocx* V, const double* vals, const ocx* phi,
+                             ocx* out) {
+    int nc = orc_prolong_coarse_dim(P);
+    ocx* pc = xcalloc(nc, sizeof(ocx));
+    ocx* xc = xcalloc(nc, sizeof(ocx));
+    orc_restrict_apply(P, phi, pc);
+    for (int i = 0; i < k; i++)
+        if (vals[i] > 1e-14) {
+            ocx c = orc_dot(nc, V + (size_t)i * nc, pc) / vals[i];
+            for (int t = 0; t < nc; t++) xc[t] += c * V[(size_t)i * nc + t];
+        }
+    orc_prolong_apply(P, xc, out);
+    free(pc); free(xc);
+}
+
+/* coarse_lowmode_force, src/hmc_multiscale.cpp:237-276: smooth = 0 takes the force of
+ * the low-mode solution; otherwise x = MR(smooth) on D^dag D x = phi, plus the low-mode
+ * solution of its residual, then MR(smooth) again from x; F = fermion_force(D, x) */
+void orc_coarse_lowmode_force(const orc_dirac* D, const orc_prolong* P, int k, const ocx* V, const double* vals,
+                              const ocx* phi, int smooth, double* F) {
+    orc_op* A = orc_op_dirac(D, ORC_DDAGD);
+    int n = A->n;
+    ocx* x = xcalloc(n, sizeof(ocx));
+    if (smooth <= 0) {
+        lowmode_solution(P, k, V, vals, phi, x);
+    } else {
+        ocx* Ax = xcalloc(n, sizeof(ocx));
+        ocx* e = xcalloc(n, sizeof(ocx));
+        orc_mr_smooth(A, x, phi, smooth);
+        A->fn(A, x, Ax);
+        for (int i = 0; i < n; i++) Ax[i] = phi[i] - Ax[i];
+        lowmode_solution(P, k, V, vals, Ax, e);
+        for (int i = 0; i < n; i++) x[i] += e[i];
+        orc_mr_smooth(A, x, phi, smooth);
+        free(Ax); free(e);
+    }
+    orc_fermion_force(D, x, F);
+    free(x);
+    orc_op_free(A);
+}
+
+/* The molecular dynamics of hmc_trajectory_mg_multiscale, src/hmc_multiscale.cpp:340-888,
+ * without eigenspace tracking, forecasting or prolongator perturbation: the outer
+ * (leapfrog or Omelyan, lambda = 0.1932) integrator kicks with F_g + F_f - F_low (F_f from
+ * the CG solve of D^dag D X = phi; the oracle's CG is unpreconditioned), the inner leapfrog
+ * (n_inner steps per call) with F_low = coarse_lowmode_force; every defl_refresh-th inner
+ * step the deflation space V [k][nc] (in/out) is Rayleigh-Ritz evolved on the Galerkin
+ * operator of P on the current gauge field (evolve_coarse_deflation, :278-338).  U and pi
+ * evolve in place; returns the outer CG iterations, *lowmode_evals the F_low count. */
+int orc_hmc_ms_evolve(int L, ocx* U, double mass, double r, double csw, double mu_t, double beta, double tau,
+                      int n_outer, int n_inner, int omelyan, int inner_smooth, int defl_refresh, int cg_maxit,
+                      double cg_tol, const orc_prolong* P, int k, ocx* V, double* vals, double* pi, const ocx* phi,
+                      int* lowmode_evals) {
+    int n = 2 * L * L, cg = 0, nlow = 0, counter = 0;
+    double h = tau / n_outer;
+    double* F = xcalloc(n, sizeof(double));
+    double* fl = xcalloc(n, sizeof(double));
+#define LOW_FORCE()                                                                        \
+    do {                                                                                   \
+        orc_dirac* Dc = orc_dirac_new(L, U, mass, r, csw, mu_t);                           \
+        orc_coarse_lowmode_force(Dc, P, k, V, vals, phi, inner_smooth, fl);                \
+        orc_dirac_free(Dc);                                                                \
+    } while (0)
+#define INNER_FORCE()  /* compute_inner_force, :593-599 (the one that counts) */            \
+    do {                                                                                   \
+        LOW_FORCE();                                                                       \
+        nlow++;                                                                            \
+    } while (0)
+#define OUTER_FORCE()  /* compute_outer_force, :446-512 */                                  \
+    do {                                                                                   \
+        cg += md_force(L, U, mass, r, csw, mu_t, beta, cg_maxit, cg_tol, phi, F);          \
+        LOW_FORCE();                                                                       \
+        for (int i = 0; i < n; i++) F[i] -= fl[i];                                         \
+    } while (0)
+    /* inner_integrator(dt_total), :667-685 */
+#define INNER(DT)                                                                          \
+    do {                                                                                   \
+        double dti = (DT) / n_inner;                                                       \
+        INNER_FORCE();                                                                     \
+        md_update_mom(n, pi, fl, 0.5 * dti);                                               \
+        for (int i = 0; i < n_inner; i++) {                                                \
+            md_update_gauge(n, U, pi, dti);                                                \
+            counter++;                                                                     \
+            if (defl_refresh > 0 && counter % defl_refresh == 0) {                         \
+                orc_dirac* Dr = orc_dirac_new(L, U, mass, r, csw, mu_t);                   \
+                orc_op* Ar = orc_op_dirac(Dr, ORC_DDAGD);                                  \
+                orc_coarse* Cr = orc_coarse_build(Ar, P, 1);                               \
+                orc_op* Ac = orc_op_coarse(Cr);                                            \
+                orc_rr_evolve(Ac, k, V, vals);                                             \
+                orc_op_free(Ac); orc_coarse_free(Cr); orc_op_free(Ar); orc_dirac_free(Dr); \
+            }                                                                              \
+            INNER_FORCE();                                                                 \
+            md_update_mom(n, pi, fl, (i < n_inner - 1) ? dti : 0.5 * dti);                 \
+        }                                                                                  \
+    } while (0)
+    if (omelyan) {
+        const double lam = 0.1932;
+        OUTER_FORCE();
+        md_update_mom(n, pi, F, lam * h);
+        for (int o = 0; o < n_outer; o++) {
+            INNER(0.5 * h);
+            OUTER_FORCE();
+            md_update_mom(n, pi, F, (1.0 - 2.0 * lam) * h);
+            INNER(0.5 * h);
+            OUTER_FORCE();
+            md_update_mom(n, pi, F, (o < n_outer - 1) ? 2.0 * lam * h : lam * h);
+        }
+    } else {
+        OUTER_FORCE();
+        md_update_mom(n, pi, F, 0.5 * h);
+        for (int o = 0; o < n_outer; o++) {
+            INNER(h);
+            OUTER_FORCE();
+            md_update_mom(n, pi, F, (o < n_outer - 1) ? h : 0.5 * h);
+        }
+    }
+#undef INNER
+#undef OUTER_FORCE
+#undef INNER_FORCE
+#undef LOW_FORCE
+    free(F); free(fl);
+    if (lowmode_evals) *lowmode_evals = nlow;
+    return cg;
+}
+
 /* H = KE + S_g + S_f of src/hmc_trajectory.cpp:87-103 (S_f = Re phi^dag (D^dag D)^-1 phi,
  * CG from x0 = 0); out = {KE, S_g, S_f}; returns the CG iterations */
 int orc_hmc_hamiltonian(int L, const ocx* U, double mass, double r, double csw, double mu_t, double beta,

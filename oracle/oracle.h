@@ -160,6 +160,13 @@ int orc_deflated_coarse_cg(const orc_op* A, int ndefl, const ocx* vecs, const do
 double orc_kinetic_energy(int n, const double* pi);
 int orc_hmc_evolve(int L, ocx* U, double mass, double r, double csw, double mu_t, double beta, double tau,
                    int n_steps, int omelyan, int cg_maxit, double cg_tol, double* pi, const ocx* phi);
+void orc_rr_evolve(const orc_op* A, int k, ocx* V, double* vals);
+void orc_coarse_lowmode_force(const orc_dirac* D, const orc_prolong* P, int k, const ocx* V, const double* vals,
+                              const ocx* phi, int smooth, double* F);
+int orc_hmc_ms_evolve(int L, ocx* U, double mass, double r, double csw, double mu_t, double beta, double tau,
+                      int n_outer, int n_inner, int omelyan, int inner_smooth, int defl_refresh, int cg_maxit,
+                      double cg_tol, const orc_prolong* P, int k, ocx* V, double* vals, double* pi, const ocx* phi,
+                      int* lowmode_evals);
 int orc_hmc_hamiltonian(int L, const ocx* U, double mass, double r, double csw, double mu_t, double beta,
                         int cg_maxit, double cg_tol, const double* pi, const ocx* phi, double* out);
 int orc_hmc_evolve_eo(int L, ocx* U, double mass, double r, double csw, double mu_t, double beta, double tau,

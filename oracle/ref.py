@@ -74,6 +74,8 @@ def lib():
         L.ref_trlm_dirac.argtypes = [_I, _P, _D, _D, _D, _D, _I, _I, _I, _D, _P, _P]
         L.ref_lobpcg_dirac.restype = _I
         L.ref_lobpcg_dirac.argtypes = [_I, _P, _D, _D, _D, _D, _I, _P, _I, _D, _P, _P]
+        L.ref_hmc_ms_trajectory.argtypes = [_I, _P, _D, _D, _D, _D, _D, _I, _I, _I, _I, _I, _I, _D, _I, _I, _P, _I,
+                                            ctypes.c_uint, _P, _P, _P, _P, _P]
         L.ref_cg_deflated.restype = _I
         L.ref_cg_deflated.argtypes = [_I, _P, _D, _D, _D, _D, _I, _P, _P, _I, _D, _P, _P]
         L.ref_coarse_deflated_solve.restype = _I
@@ -311,3 +313,18 @@ def lobpcg_dirac(L, U, m, r, csw, mu, X0, max_iter, tol):
     X = np.zeros_like(X0)
     it = lib().ref_lobpcg_dirac(L, _ptr(_c(U)), m, r, csw, mu, k, _ptr(X0), max_iter, tol, _ptr(ev), _ptr(X))
     return ev, X, it
+
+
+def hmc_ms_trajectory(L, U, mass, r, csw, beta, tau, n_outer, n_inner, omelyan, inner_smooth, defl_refresh,
+                      cg_maxit, cg_tol, b, k, nulls, n_defl, seed):
+    """hmc_trajectory_mg_multiscale (identity mg_precond) from mt19937(seed): (U after,
+    out = [accepted, dH, dKE, dSG, dSF, cg, lowmode evals], (vals0, vecs0), (vals1, vecs1))."""
+    U = _c(U).copy()
+    nc = (L // b) ** 2 * k
+    out = np.zeros(7)
+    v0 = np.zeros(n_defl); V0 = np.zeros((n_defl, nc), np.complex128)
+    v1 = np.zeros(n_defl); V1 = np.zeros((n_defl, nc), np.complex128)
+    lib().ref_hmc_ms_trajectory(L, _ptr(U), mass, r, csw, beta, tau, n_outer, n_inner, omelyan, inner_smooth,
+                                defl_refresh, cg_maxit, cg_tol, b, k, _ptr(_c(nulls)), n_defl, seed, _ptr(out),
+                                _ptr(v0), _ptr(V0), _ptr(v1), _ptr(V1))
+    return U, out, (v0, V0), (v1, V1)

@@ -528,3 +528,61 @@ API int ref_lobpcg_dirac(int L, const double* U, double m, double r, double csw,
     }
     return o.iterations;
 }
+// hmc_trajectory_mg_multiscale (src/hmc_multiscale.cpp:340-888) with the identity as
+// mg_precond, no tracking or forecast; the deflation state is the TRLM space of the
+// b x b Galerkin operator of the given null vectors (SparseCoarseOp::setup_deflation,
+// n_ev = n_defl).  out = {accepted, dH, dKE, dSG, dSF, highmode CG iterations, low-mode
+// force evaluations}; U in/out; the deflation space before (vals0, vecs0) and after
+// (vals1, vecs1) the trajectory.
+API void ref_hmc_ms_trajectory(int L, double* U, double mass, double r, double csw, double beta, double tau,
+                               int n_outer, int n_inner, int omelyan, int inner_smooth, int defl_refresh,
+                               int cg_maxit, double cg_tol, int b, int k, const double* nulls, int n_defl,
+                               unsigned seed, double* out, double* vals0, double* vecs0, double* vals1,
+                               double* vecs1) {
+    Lattice lat(L);
+    GaugeField g(lat);
+    load_gauge_arr(g, U);
+    DiracOp D(lat, g, mass, r, csw, 0.0);
+    Prolongator P(lat, b, b, k);
+    std::vector<Vec> nv(k);
+    for (int i = 0; i < k; i++) nv[i] = to_vec(nulls + 2 * (size_t)i * lat.ndof, lat.ndof);
+    P.build_from_vectors(nv);
+    SparseCoarseOp S;
+    S.build(P, D.as_DdagD_op(), lat.ndof);
+    int n_kr = std::min(2 * n_defl + 10, S.dim);
+    TRLMResult tr = trlm_eigensolver(S.as_op(), S.dim, n_defl, n_kr, 100, 1e-10);
+    CoarseDeflState cd;
+    cd.eigvecs = tr.eigvecs;
+    cd.eigvals = tr.eigvals;
+    for (int i = 0; i < n_defl; i++) {
+        vals0[i] = cd.eigvals[i];
+        from_vec(cd.eigvecs[i], vecs0 + 2 * (size_t)i * S.dim);
+    }
+    MGMultiScaleParams p;
+    p.beta = beta;
+    p.tau = tau;
+    p.n_outer = n_outer;
+    p.n_inner = n_inner;
+    p.cg_maxiter = cg_maxit;
+    p.cg_tol = cg_tol;
+    p.inner_smooth = inner_smooth;
+    p.outer_type = omelyan ? OuterIntegrator::Omelyan : OuterIntegrator::Leapfrog;
+    p.defl_refresh = defl_refresh;
+    p.c_sw = csw;
+    std::function<Vec(const Vec&)> ident = [](const Vec& v) { return v; };
+    std::mt19937 rng(seed);
+    MGMultiScaleResult res = hmc_trajectory_mg_multiscale(g, lat, mass, r, p, cd, P, ident, rng);
+    out[0] = res.accepted ? 1.0 : 0.0;
+    out[1] = res.dH;
+    out[2] = res.dKE;
+    out[3] = res.dSG;
+    out[4] = res.dSF;
+    out[5] = res.highmode_cg_iters;
+    out[6] = res.lowmode_force_evals;
+    for (int i = 0; i < n_defl; i++) {
+        vals1[i] = cd.eigvals[i];
+        from_vec(cd.eigvecs[i], vecs1 + 2 * (size_t)i * S.dim);
+    }
+    from_vec(g.U[0], U);
+    from_vec(g.U[1], U + 2 * (size_t)lat.V);
+}

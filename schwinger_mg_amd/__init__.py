@@ -118,6 +118,10 @@ SIGNATURES = [
     ("smg_coarse_deflation", _I, [_P, _P, _P, _P]), ("smg_coarse_set_deflation", _I, [_P, _I, _P, _P]),
     ("smg_cg_solve_deflated", _I, [_P, _I, _P, _P, _I, _P, _P, _I, _D, _P]),
     ("smg_dirac_lobpcg", _I, [_P, _P, _I, _P, _I, _D, _P, _P, _P]),
+    ("smg_coarse_lowmode_force", _I, [_P, _P, _P, _P, _I, _P]),
+    ("smg_coarse_evolve_deflation", _I, [_P, _P]),
+    ("smg_hmc_ms_evolve", _I, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    ("smg_hmc_ms_trajectory", _I, [_P, _P, _P, _P, _P, _P, _P, _D, _P]),
     ("smg_coarse_lobpcg", _I, [_P, _I, _P, _I, _D, _P, _P, _P]),
     ("smg_mg_update_coarsest_deflation", _I, [_P, _I, _I]),
     ("smg_mg_coarsest_deflation", _I, [_P, _P, _P, _P]),
@@ -772,6 +776,57 @@ def lobpcg_update(A, X0, precond=None, max_iter=5, tol=1e-8):
         _chk(lib().smg_dirac_lobpcg(A.h, precond.h if precond else None, k, _p(X0), max_iter, tol, _p(ev), _p(X),
                                     ctypes.byref(it)))
     return LOBPCGResult(X, ev, it.value)
+
+
+# ------------------------------------------------------------------ multi-timescale MG HMC
+class MSParams(ctypes.Structure):
+    """MGMultiScaleParams (src/hmc.h:346-365), the fields the device trajectory uses."""
+    _fields_ = [("beta", _D), ("tau", _D), ("n_outer", _I), ("n_inner", _I), ("omelyan", _I),
+                ("inner_smooth", _I), ("defl_refresh", _I), ("cg_maxiter", _I), ("cg_tol", _D),
+                ("force_accept", _I)]
+
+
+class MSResult(ctypes.Structure):
+    """MGMultiScaleResult (src/hmc.h:367-376) without the timings."""
+    _fields_ = [("accepted", _I), ("dH", _D), ("dKE", _D), ("dSG", _D), ("dSF", _D),
+                ("highmode_cg_iters", _I), ("lowmode_force_evals", _I)]
+
+
+def ms_params(beta, tau=1.0, n_outer=4, n_inner=5, omelyan=False, inner_smooth=3, defl_refresh=0, cg_maxiter=500,
+              cg_tol=1e-10, force_accept=False):
+    return MSParams(beta, tau, n_outer, n_inner, int(omelyan), inner_smooth, defl_refresh, cg_maxiter, cg_tol,
+                    int(force_accept))
+
+
+def coarse_lowmode_force(D, P, defl, phi, smooth_iters=0):
+    """coarse_lowmode_force (src/hmc_multiscale.cpp:237-276) with the deflation space of
+    the SparseCoarseOp defl on P's coarse space: F[mu*V + s]."""
+    F = np.zeros(2 * D.V)
+    _chk(lib().smg_coarse_lowmode_force(D.h, P.h, defl.h, _p(_c(phi)), smooth_iters, _p(F)))
+    return F
+
+
+def evolve_coarse_deflation(defl, A_new):
+    """evolve_coarse_deflation without forecast (src/hmc_multiscale.cpp:278-338)."""
+    _chk(lib().smg_coarse_evolve_deflation(defl.h, A_new.h))
+
+
+def hmc_ms_evolve(D, P, defl, params, pi, phi, mg=None):
+    """The multi-timescale molecular dynamics: (pi, outer CG iterations, low-mode force evaluations)."""
+    pi = np.ascontiguousarray(pi, np.float64).copy()
+    it, nl = _I(), _I()
+    _chk(lib().smg_hmc_ms_evolve(D.h, mg.h if mg else None, P.h, defl.h, ctypes.byref(params), _p(pi),
+                                 _p(_c(phi)), ctypes.byref(it), ctypes.byref(nl)))
+    return pi, it.value, nl.value
+
+
+def hmc_trajectory_mg_multiscale(D, P, defl, params, pi0, eta, u, mg=None):
+    """hmc_trajectory_mg_multiscale (src/hmc_multiscale.cpp:340-888) from the momenta pi0
+    and the pseudofermion Gaussian eta (phi = D^dag eta), u the Metropolis uniform."""
+    res = MSResult()
+    _chk(lib().smg_hmc_ms_trajectory(D.h, mg.h if mg else None, P.h, defl.h, ctypes.byref(params),
+                                     _p(np.ascontiguousarray(pi0, np.float64)), _p(_c(eta)), u, ctypes.byref(res)))
+    return res
 
 
 # ------------------------------------------------------------------ forces

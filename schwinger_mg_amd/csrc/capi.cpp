@@ -1177,6 +1177,197 @@ int smg_hmc_trajectory(smg_dirac* D, const smg_hmc_params* p, const double* pi0,
     });
 }
 
+// ------------------------------------------------------------------ multi-timescale MG HMC
+namespace {
+// coarse_lowmode_force, src/hmc_multiscale.cpp:237-276, on the device; the MR
+// smoother carries its residual recursively, the residual handed to the low-mode
+// solution is recomputed
+void lowmode_force_dev(smg_dirac* D, const Aggregation& P, const Eigenpairs& E, const c64* phi, int smooth, double* F,
+                       hipStream_t st) {
+    LinOp<double>& A = *D->op64;
+    const size_t n = A.n();
+    Field<double> x(n), r(n), Ar(n), e(n);
+    if (smooth <= 0) {
+        lowmode_solution(P, E, phi, x.p(), st);
+    } else {
+        DevMem sc(64 * sizeof(double));
+        mr_smooth<double>(A, x.p(), phi, r.p(), Ar.p(), smooth, true, sc.as<double>(), st);
+        A.apply(Ar.p(), x.p(), st);
+        launch_copy<double>(r.p(), phi, n, st);
+        launch_axpby<double>(r.p(), Ar.p(), c64{-1.0, 0.0}, c64{1.0, 0.0}, n, st);
+        lowmode_solution(P, E, r.p(), e.p(), st);
+        launch_axpby<double>(x.p(), e.p(), c64{1.0, 0.0}, c64{1.0, 0.0}, n, st);
+        mr_smooth<double>(A, x.p(), phi, r.p(), Ar.p(), smooth, false, sc.as<double>(), st);
+    }
+    fermion_force_dev(D, x.p(), F, st);
+}
+
+// The molecular dynamics of hmc_trajectory_mg_multiscale (src/hmc_multiscale.cpp:340-888)
+// without tracking, forecasting or prolongator perturbation.  The full-lattice MD
+// state (momenta, forces, phi) of the plain trajectory, plus the low-mode force and
+// the deflation space (held by the smg_coarse handle, updated in place).
+struct MSState : MDState {
+    MG* mg;
+    const Aggregation& P;
+    Eigenpairs& E;
+    DevMem fl, zero;
+    int lowmode_evals = 0, counter = 0;
+    MSState(smg_dirac* d, MG* m, const Aggregation& P_, Eigenpairs& E_, const double* pi_h, const double* phi_h)
+        : MDState(d, false, pi_h, phi_h), mg(m), P(P_), E(E_), fl(2 * V * sizeof(double)),
+          zero(2 * V * sizeof(double)) {
+        SMG_HIP(hipMemsetAsync(zero.p, 0, 2 * V * sizeof(double), S()));
+    }
+    void low_force(int smooth) { lowmode_force_dev(D, P, E, phi.p(), smooth, fl.as<double>(), S()); }
+    // compute_outer_force (:446-512): F_g + F_f - F_low, the solve MG-preconditioned when
+    // a hierarchy is given
+    int outer_force(const smg_ms_params& p) {
+        const Dirac& d = *D->D;
+        launch_gauge_force(d.L, d.U64.p(), p.beta, gf.as<double>(), S());
+        const SolveResult r = cg_solve(*D->op64, phi.p(), X.p(), nullptr, mg, p.cg_maxiter, p.cg_tol, S());
+        fermion_force_dev(D, X.p(), ff.as<double>(), S());
+        low_force(p.inner_smooth);
+        launch_mom_update(ff.as<double>(), fl.as<double>(), zero.as<double>(), -1.0, 2 * V, S());   // ff -= fl
+        return r.iterations;
+    }
+    void kick_low(double scale) {
+        launch_mom_update(pi.as<double>(), fl.as<double>(), zero.as<double>(), scale, 2 * V, S());
+    }
+    // inner_integrator (:667-685) with maybe_refresh_deflation (:601-614)
+    void inner(const smg_ms_params& p, double dt_total) {
+        const int ni = p.n_inner;
+        const double dti = dt_total / ni;
+        low_force(p.inner_smooth);
+        lowmode_evals++;
+        kick_low(0.5 * dti);
+        for (int i = 0; i < ni; i++) {
+            gauge(dti);
+            counter++;
+            if (p.defl_refresh > 0 && counter % p.defl_refresh == 0) {   // evolve_coarse_deflation
+                CoarseOperator Cn(P.A.nbx, P.A.nby, P.A.k);
+                Cn.build(*D->op64, P, S());
+                CoarseLinOp<double, c64> op(Cn);
+                rr_evolve(op, E, S());
+            }
+            low_force(p.inner_smooth);
+            lowmode_evals++;
+            kick_low(i < ni - 1 ? dti : 0.5 * dti);
+        }
+    }
+    int evolve(const smg_ms_params& p) {
+        const double h = p.tau / p.n_outer;
+        int cg = 0;
+        if (p.omelyan) {
+            const double lam = 0.1932;
+            cg += outer_force(p);
+            mom(lam * h);
+            for (int o = 0; o < p.n_outer; o++) {
+                inner(p, 0.5 * h);
+                cg += outer_force(p);
+                mom((1.0 - 2.0 * lam) * h);
+                inner(p, 0.5 * h);
+                cg += outer_force(p);
+                mom((o < p.n_outer - 1 ? 2.0 : 1.0) * lam * h);
+            }
+        } else {
+            cg += outer_force(p);
+            mom(0.5 * h);
+            for (int o = 0; o < p.n_outer; o++) {
+                inner(p, h);
+                cg += outer_force(p);
+                mom(o < p.n_outer - 1 ? h : 0.5 * h);
+            }
+        }
+        return cg;
+    }
+    // KE + S_g + S_f with the (preconditioned) CG of the trajectory
+    int ham(const smg_ms_params& p, double* h3) {
+        double h4[4];
+        const Dirac& d = *D->D;
+        h4[0] = 0.5 * dev_dot<double>(pi.as<c64>(), pi.as<c64>(), V, S()).re;
+        Scalar1 sg;
+        launch_plaq_sum(d.L, d.U64.p(), sg.fin(), S());
+        h4[1] = -p.beta * sg.read();
+        const SolveResult r = cg_solve(*D->op64, phi.p(), X.p(), nullptr, mg, p.cg_maxiter, p.cg_tol, S());
+        h4[2] = dev_dot<double>(phi.p(), X.p(), nphi, S()).re;
+        for (int i = 0; i < 3; i++) h3[i] = h4[i];
+        return r.iterations;
+    }
+};
+void check_ms(smg_dirac* D, smg_prolongator* P, smg_coarse* defl, const smg_ms_params* p) {
+    SMG_REQUIRE(D && P && defl && p, "null argument");
+    SMG_REQUIRE(p->n_outer >= 1 && p->n_inner >= 1 && p->tau > 0.0, "tau > 0, n_outer >= 1 and n_inner >= 1");
+    SMG_REQUIRE(p->cg_maxiter >= 1 && p->cg_tol > 0.0, "bad CG budget");
+    SMG_REQUIRE(P->P->fine_n() == 2 * (size_t)D->D->V && P->P->A.nd == 2, "the prolongator is not of this lattice");
+    SMG_REQUIRE(defl->defl.vals.empty() || defl->C->n() == P->P->coarse_n(),
+                "the deflation space is not of the prolongator's coarse space");
+}
+}  // namespace
+
+int smg_coarse_lowmode_force(smg_dirac* D, smg_prolongator* P, smg_coarse* defl, const double* phi, int smooth_iters,
+                             double* F) {
+    return guard([&] {
+        SMG_REQUIRE(D && P && defl && phi && F, "null argument");
+        SMG_REQUIRE(P->P->fine_n() == 2 * (size_t)D->D->V && defl->C->n() == P->P->coarse_n(), "shape mismatch");
+        const size_t V = (size_t)D->D->V;
+        Field<double> ph = up(phi, 2 * V);
+        DevMem f(2 * V * sizeof(double));
+        lowmode_force_dev(D, *P->P, defl->defl, ph.p(), smooth_iters, f.as<double>(), S());
+        down(F, f.p, 2 * V * sizeof(double));
+    });
+}
+int smg_coarse_evolve_deflation(smg_coarse* defl, smg_coarse* A_new) {
+    return guard([&] {
+        SMG_REQUIRE(defl && A_new && defl->C->n() == A_new->C->n(), "bad arguments");
+        CoarseLinOp<double, c64> op(*A_new->C);
+        rr_evolve(op, defl->defl, S());
+    });
+}
+int smg_hmc_ms_evolve(smg_dirac* D, smg_mg* mg, smg_prolongator* P, smg_coarse* defl, const smg_ms_params* p,
+                      double* pi, const double* phi, int* cg_iters, int* lowmode_evals) {
+    return guard([&] {
+        check_ms(D, P, defl, p);
+        SMG_REQUIRE(pi && phi, "null argument");
+        MSState ms(D, mg ? mg->mg.get() : nullptr, *P->P, defl->defl, pi, phi);
+        const int it = ms.evolve(*p);
+        ms.pi_out(pi);
+        if (cg_iters) *cg_iters = it;
+        if (lowmode_evals) *lowmode_evals = ms.lowmode_evals;
+    });
+}
+int smg_hmc_ms_trajectory(smg_dirac* D, smg_mg* mg, smg_prolongator* P, smg_coarse* defl, const smg_ms_params* p,
+                          const double* pi0, const double* eta, double u, smg_ms_result* res) {
+    return guard([&] {
+        check_ms(D, P, defl, p);
+        SMG_REQUIRE(pi0 && eta && res, "null argument");
+        Dirac& d = *D->D;
+        const size_t V = (size_t)d.V;
+        Field<double> saved(2 * V);
+        launch_copy<double>(saved.p(), d.U64.p(), 2 * V, S());
+        MSState ms(D, mg ? mg->mg.get() : nullptr, *P->P, defl->defl, pi0, nullptr);
+        Field<double> e = up(eta, 2 * V);   // phi = D^dag eta (generate_pseudofermion)
+        launch_dslash<double>(d.v64(), ms.phi.p(), e.p(), -d.mu_t, 1, 1, EPI_NONE, nullptr, nullptr, Fin(), 1, 2 * V,
+                              S());
+        double h0[3], h1[3];
+        int cg = ms.ham(*p, h0);
+        cg += ms.evolve(*p);
+        cg += ms.ham(*p, h1);
+        const double dH = (h1[0] + h1[1] + h1[2]) - (h0[0] + h0[1] + h0[2]);
+        const bool accept = p->force_accept || dH < 0 || u < std::exp(-dH);
+        if (!accept) {   // the gauge field is restored; the deflation space keeps its evolution
+            launch_copy<double>(d.U64.p(), saved.p(), 2 * V, S());
+            d.refresh_derived(S());
+        }
+        SMG_HIP(hipStreamSynchronize(S()));
+        res->accepted = accept ? 1 : 0;
+        res->dH = dH;
+        res->dKE = h1[0] - h0[0];
+        res->dSG = h1[1] - h0[1];
+        res->dSF = h1[2] - h0[2];
+        res->highmode_cg_iters = cg;
+        res->lowmode_force_evals = ms.lowmode_evals;
+    });
+}
+
 // ------------------------------------------------------------------ gauge field files (host I/O)
 namespace {
 // GaugeField::avg_plaq, src/gauge.h:26-36, summed in site order (the reference's

@@ -39,6 +39,13 @@ int deflated_coarse_cg(LinOp<double>& A, const Eigenpairs& E, const c64* b, c64*
 SolveResult cg_deflated(LinOp<double>& A, MG* mg, const c64* U, int ndefl, const c64* b, c64* x, int maxit,
                         double tol, hipStream_t st);
 
+// rr_evolve (src/eigen_evolve.cpp:48-108): the space E.vecs Rayleigh-Ritz rotated on A
+// (M_ij = <v_i, A v_j>, i <= j, diagonalised by dense_eigh), E.vals the Ritz values
+void rr_evolve(LinOp<double>& A, Eigenpairs& E, hipStream_t st);
+// CoarseDeflState::lowmode_solution (src/hmc.h:311-323): out = P (sum_i v_i v_i^dag / l_i)
+// P^dag v over the l_i > 1e-14, P the aggregation whose coarse space holds E
+void lowmode_solution(const Aggregation& P, const Eigenpairs& E, const c64* v, c64* out, hipStream_t st);
+
 // lobpcg_update (src/eigensolver_core.cpp:279-533) on the k vectors X0 (device,
 // [k][n]) as one block of N = next power of two >= k columns: every operator and
 // preconditioner application is one multi-right-hand-side launch, the Gram matrices

@@ -406,6 +406,69 @@ SolveResult cg_deflated(LinOp<double>& A, MG* mg, const c64* U, int nd, const c6
 }
 
 
+// ----------------------------------------------------------------- Rayleigh-Ritz, low modes
+namespace {
+// host copies of <V_i, w> for the m device vectors V (stride n)
+std::vector<cxd> dots_h(const c64* V, int m, const c64* w, size_t n, hipStream_t st) {
+    DevMem out(64 * (size_t)((m + 31) / 32) * sizeof(double));
+    Context& cx = Context::get();
+    for (int c0 = 0; c0 < m; c0 += 32) {
+        const int nv = std::min(32, m - c0);
+        VecPtrs<double> P{};
+        for (int i = 0; i < nv; i++) P.v[i] = V + (size_t)(c0 + i) * n;
+        launch_multi_dot<double>(P, nv, w, n, 0, cx.fin(FIN_STORE, out.as<double>() + 2 * c0), st);
+    }
+    std::vector<double> h(64 * (size_t)((m + 31) / 32));
+    SMG_HIP(hipMemcpyAsync(h.data(), out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    SMG_HIP(hipStreamSynchronize(st));
+    std::vector<cxd> d(m);
+    for (int i = 0; i < m; i++) d[i] = cxd(h[2 * i], h[2 * i + 1]);
+    return d;
+}
+}  // namespace
+
+void rr_evolve(LinOp<double>& A, Eigenpairs& E, hipStream_t st) {
+    const int k = (int)E.vals.size();
+    if (k == 0) return;
+    const size_t n = A.n();
+    Field<double> AV((size_t)k * n), W((size_t)k * n);
+    for (int j = 0; j < k; j++) A.apply(AV.p((size_t)j * n), E.vecs.p((size_t)j * n), st);
+    std::vector<c64> M((size_t)k * k);   // columns: M[j*k + i] = M_ij
+    for (int j = 0; j < k; j++) {
+        const std::vector<cxd> d = dots_h(E.vecs.p(), j + 1, AV.p((size_t)j * n), n, st);
+        for (int i = 0; i <= j; i++) {
+            M[(size_t)j * k + i] = c64{d[i].real(), d[i].imag()};
+            if (i != j) M[(size_t)i * k + j] = c64{d[i].real(), -d[i].imag()};
+        }
+    }
+    std::vector<double> ev(k);
+    std::vector<c64> Z((size_t)k * k);   // Z[row*k + col]
+    dense_eigh(k, M.data(), ev.data(), Z.data());
+    DevMem coef(64 * (size_t)((k + 31) / 32) * sizeof(double));
+    std::vector<cxd> c(k);
+    for (int i = 0; i < k; i++) {   // v_i <- sum_j Z[j][i] v_j
+        for (int j = 0; j < k; j++) c[j] = to_std(Z[(size_t)j * k + i]);
+        combine(W.p((size_t)i * n), E.vecs.p(), k, c, n, coef, st);
+    }
+    E.vecs = std::move(W);
+    E.vals = ev;
+}
+
+void lowmode_solution(const Aggregation& P, const Eigenpairs& E, const c64* v, c64* out, hipStream_t st) {
+    const size_t nc = P.coarse_n();
+    const int k = (int)E.vals.size();
+    Field<double> pc(nc), xc(nc);
+    launch_restrict<double>(P.A, P.bv<double>(), v, pc.p(), st);
+    launch_zero<double>(xc.p(), nc, st);
+    if (k > 0) {
+        std::vector<cxd> c = dots_h(E.vecs.p(), k, pc.p(), nc, st);
+        for (int i = 0; i < k; i++) c[i] = E.vals[i] > 1e-14 ? c[i] / E.vals[i] : cxd(0.0);
+        DevMem coef(64 * (size_t)((k + 31) / 32) * sizeof(double));
+        combine(xc.p(), E.vecs.p(), k, c, nc, coef, st);
+    }
+    launch_prolong<double>(P.A, P.bv<double>(), xc.p(), out, 0, st);
+}
+
 // ----------------------------------------------------------------- LOBPCG
 namespace {
 // Block-vector workspace of the LOBPCG iteration and its host-side Gram helper.

@@ -59,6 +59,14 @@ int orc_trlm(const orc_op* A, int n_ev, int n_kr, int max_restarts, double tol, 
 int orc_cg_deflated(const orc_op* A, const cx* rhs, orc_mg* mg, int ndefl, const cx* U, int maxit, double tol, cx* x,
                     double* final_res);
 int orc_lobpcg(const orc_op* A, orc_mg* mg, int k, const cx* X0, int max_iter, double tol, cx* Xout, double* evals);
+typedef struct orc_prolong orc_prolong;
+orc_prolong* orc_prolong_new(int nx, int ny, int nd, int bx, int by, int k, int chiral);
+void orc_prolong_build(orc_prolong* P, const cx* null_vecs);
+void orc_prolong_free(orc_prolong* P);
+int orc_hmc_ms_evolve(int L, cx* U, double mass, double r, double csw, double mu_t, double beta, double tau,
+                      int n_outer, int n_inner, int omelyan, int inner_smooth, int defl_refresh, int cg_maxit,
+                      double cg_tol, const orc_prolong* P, int k, cx* V, double* vals, double* pi, const cx* phi,
+                      int* lowmode_evals);
 }
 
 static int failures = 0;
@@ -377,6 +385,73 @@ int main() {
                mg.coarsest_defl_vals.size() == 8 && mg.coarsest_defl_vecs[0].size() == (size_t)mg.dims.back() ? 0.0
                                                                                                        : 1.0,
                0.0);
+    }
+
+    // hmc_trajectory_mg_multiscale (Omelyan outer, smoothed low-mode force, refresh every
+    // 2 inner steps) against the oracle's MD + Hamiltonian on the same draws
+    {
+        const int Lm = 16;
+        Lattice lm(Lm);
+        GaugeField gm(lm);
+        std::mt19937 rg(77);
+        gm.randomise(rg, 0.3);
+        const GaugeField g0 = gm;
+        DiracOp Dm(lm, gm, 0.1, 1.0, 1.0, 0.0);
+        Prolongator Pm(lm, 4, 4, 4);
+        std::vector<Vec> nv;
+        Vec nvf;
+        for (int i = 0; i < 4; i++) {
+            nv.push_back(random_vec(lm.ndof, rg));
+            nvf.insert(nvf.end(), nv.back().begin(), nv.back().end());
+        }
+        Pm.build_from_vectors(nv);
+        SparseCoarseOp Sm;
+        Sm.build(Pm, Dm.as_DdagD_op(), lm.ndof);
+        Sm.setup_deflation(6);
+        CoarseDeflState cd{Sm.defl_vecs, Sm.defl_vals};
+        const CoarseDeflState cd0 = cd;
+        MGMultiScaleParams mp;
+        mp.beta = 2.0;
+        mp.tau = 0.3;
+        mp.n_outer = 3;
+        mp.n_inner = 3;
+        mp.cg_tol = 1e-12;
+        mp.inner_smooth = 3;
+        mp.outer_type = OuterIntegrator::Omelyan;
+        mp.defl_refresh = 2;
+        mp.c_sw = 1.0;
+        std::function<Vec(const Vec&)> idp = identity_precond();
+        std::mt19937 rt(5), rc(5);
+        MGMultiScaleResult mr = hmc_trajectory_mg_multiscale(gm, lm, 0.1, 1.0, mp, cd, Pm, idp, rt);
+        // the oracle from the same draws
+        MomentumField mo(lm);
+        mo.randomise(rc);
+        (void)random_vec(lm.ndof, rc);
+        const Vec eta = random_vec(lm.ndof, rc);
+        Vec Uo = g0.links(), phi(lm.ndof);
+        orc_dirac* Dq = orc_dirac_new(Lm, Uo.data(), 0.1, 1.0, 1.0, 0.0);
+        orc_dirac_apply(Dq, 1, eta.data(), phi.data());
+        orc_dirac_free(Dq);
+        orc_prolong* Po = orc_prolong_new(Lm, Lm, 2, 4, 4, 4, 0);
+        orc_prolong_build(Po, nvf.data());
+        Vec Vd;
+        for (auto& v : cd0.eigvecs) Vd.insert(Vd.end(), v.begin(), v.end());
+        RVec vals = cd0.eigvals, pio = detail::mom_flat(mo);
+        double e0[3], e1[3];
+        orc_hmc_hamiltonian(Lm, Uo.data(), 0.1, 1.0, 1.0, 0.0, 2.0, 500, 1e-12, pio.data(), phi.data(), e0);
+        int nl = 0;
+        orc_hmc_ms_evolve(Lm, Uo.data(), 0.1, 1.0, 1.0, 0.0, 2.0, 0.3, 3, 3, 1, 3, 2, 500, 1e-12, Po, 6, Vd.data(),
+                          vals.data(), pio.data(), phi.data(), &nl);
+        orc_hmc_hamiltonian(Lm, Uo.data(), 0.1, 1.0, 1.0, 0.0, 2.0, 500, 1e-12, pio.data(), phi.data(), e1);
+        orc_prolong_free(Po);
+        const double dHo = (e1[0] + e1[1] + e1[2]) - (e0[0] + e0[1] + e0[2]);
+        report("hmc_trajectory_mg_multiscale dH", std::abs(mr.dH - dHo), 1e-8);
+        report("hmc_trajectory_mg_multiscale low-mode evaluations", std::abs(mr.lowmode_force_evals - nl), 0.0);
+        double dv = 0;
+        for (int i = 0; i < 6; i++) dv = std::max(dv, std::abs(cd.eigvals[i] - vals[i]));
+        report("hmc_trajectory_mg_multiscale evolved deflation", dv, 1e-10);
+        if (mr.accepted) report("hmc_trajectory_mg_multiscale links", relerr(gm.links(), Uo), 1e-10);
+        else report("hmc_trajectory_mg_multiscale rejected: links restored", relerr(gm.links(), g0.links()), 0.0);
     }
 
     orc_mg_free(mgo);

@@ -139,6 +139,18 @@ def main():
     for its in (10, 40):
         ev, X, it = R.lobpcg_dirac(8, out["E8_U"], 0.05, 1.0, 1.0, 0.0, X0, its, 1e-10)
         out[f"E8_lob{its}_evals"], out[f"E8_lob{its}_X"], out[f"E8_lob{its}_it"] = ev, X, np.array([it])
+    # hmc_trajectory_mg_multiscale at L = 16 (4x4 aggregates of 4 null vectors, 6 deflation
+    # vectors), Omelyan outer with the smoothed low-mode force and a Rayleigh-Ritz refresh
+    # every 2 inner steps, from the draws of mt19937(31)
+    L = 16
+    U = random_gauge(L, 0.3, seed=91)
+    nulls = gaussian_vecs(4, 2 * L * L, seed=92)
+    Uf, res, (v0, V0), (v1, V1) = R.hmc_ms_trajectory(L, U, 0.1, 1.0, 1.0, 2.0, 0.3, 3, 3, 1, 3, 2, 500, 1e-12, 4, 4,
+                                                    nulls, 6, 31)
+    pi_d, eta_d = R.hmc_draws(L, 31, 4 * L * L)
+    out["MS_U"], out["MS_nulls"], out["MS_pi"], out["MS_eta2"] = U, nulls, pi_d, eta_d[2 * L * L:]
+    out["MS_out"], out["MS_Uf"] = res, Uf
+    out["MS_vals0"], out["MS_vecs0"], out["MS_vals1"], out["MS_vecs1"] = v0, V0, v1, V1
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_vectors.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes,", len(out), "arrays")

@@ -498,3 +498,51 @@ def test_golden_lobpcg(g, its):
     A = np.array([D.op()(x) for x in X])
     ritz = np.linalg.eigvalsh(X.conj() @ A.T)
     assert np.all(ritz >= low - 1e-12)
+
+
+# ------------------------------------------------------------------ multi-timescale MG HMC
+def test_golden_hmc_ms_trajectory(g):
+    """hmc_trajectory_mg_multiscale (src/hmc_multiscale.cpp:340-888; Omelyan outer, the
+    MR-smoothed coarse_lowmode_force inner, Rayleigh-Ritz refresh every 2 inner steps)
+    restated: from the reference's draws and initial deflation space, the same dH,
+    final gauge field and evolved space."""
+    L = 16
+    U = g["MS_U"]
+    Do = O.Dirac(L, U, 0.1, 1.0, 1.0, 0.0)
+    phi = Do.apply(1, g["MS_eta2"])
+    P = O.Prolongator(L, L, 2, 4, 4, 4).build(g["MS_nulls"].ravel())
+    Uo, po, Vo, vo, it, nl = O.hmc_ms_evolve(L, U, 0.1, 1.0, 1.0, 0.0, 2.0, 0.3, 3, 3, 1, 3, 2, 500, 1e-12, P,
+                                             g["MS_vecs0"], g["MS_vals0"], g["MS_pi"], phi)
+    h0 = O.hmc_hamiltonian(L, U, 0.1, 1.0, 1.0, 0.0, 2.0, 500, 1e-12, g["MS_pi"], phi)
+    h1 = O.hmc_hamiltonian(L, Uo, 0.1, 1.0, 1.0, 0.0, 2.0, 500, 1e-12, po, phi)
+    out = g["MS_out"]
+    assert out[0] == 1.0 and nl == int(out[6])
+    assert abs((h1 - h0).sum() - out[1]) < 1e-9
+    assert np.abs(h1 - h0 - out[2:5]).max() < 1e-9
+    assert np.abs(Uo - g["MS_Uf"]).max() < 1e-11
+    assert np.abs(vo - g["MS_vals1"]).max() < 1e-12
+
+
+def test_rr_evolve_and_lowmode_projection():
+    """rr_evolve returns the Ritz pairs of the subspace (a subspace holding exact
+    eigenvectors returns those); the low-mode solution restricted to the deflation
+    space inverts A_c there: P^dag D^dag D x_low = P^dag phi projected on the space
+    equals that projection (the low-mode force then is the force of x_low)."""
+    L, b, k = 16, 4, 4
+    U = random_gauge(L, 0.3, 93)
+    Do = O.Dirac(L, U, 0.1, 1.0, 1.0, 0.0)
+    P = O.Prolongator(L, L, 2, b, b, k).build(gaussian_vecs(k, 2 * L * L, 94).ravel())
+    C = O.Coarse(Do.op(), P, 1)
+    A = O.dense_galerkin(C.op()).T   # dense_galerkin returns the columns
+    w, Z = np.linalg.eigh(A)
+    Q, _ = np.linalg.qr(Z[:, :5] @ (np.random.default_rng(95).standard_normal((5, 5))
+                                    + 1j * np.random.default_rng(96).standard_normal((5, 5))))
+    V, vals = O.rr_evolve(C.op(), Q.T.copy())
+    assert np.allclose(vals, w[:5], rtol=1e-12, atol=0)
+    for i in range(5):
+        assert np.linalg.norm(A @ V[i] - vals[i] * V[i]) < 1e-10 * vals[-1]
+    # the smooth = 0 low-mode force is the fermion force of P (sum v v^dag / l) P^dag phi
+    phi = gaussian_vec(2 * L * L, 97)
+    xc = sum((np.vdot(V[i], P.restrict(phi)) / vals[i]) * V[i] for i in range(5))
+    F = O.coarse_lowmode_force(Do, P, V, vals, phi, 0)
+    assert np.abs(F - Do.force(P.prolong(xc))).max() < 1e-12 * np.abs(F).max()
