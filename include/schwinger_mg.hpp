@@ -1044,6 +1044,16 @@ inline HMCResult hmc_trajectory(GaugeField& gauge, const Lattice& lat, double ma
     return {accept, dH, h1[0] - h0[0], h1[1] - h0[1], h1[2] - h0[2], h1[3] - h0[3], cg};
 }
 
+// ------------------------------------------------------------------ U(1) heatbath (an addition: the reference has none)
+// n_sweeps checkerboard heatbath sweeps of the Wilson action on gauge.U (L even), Philox
+// keyed by seed, sweeps numbered from first_sweep (see smg_heatbath)
+inline void heatbath(GaugeField& gauge, double beta, int n_sweeps, unsigned long long seed, unsigned first_sweep = 0) {
+    Vec u = gauge.links();
+    detail::check(smg_heatbath(gauge.lat.L, detail::raw(u), beta, n_sweeps, seed, first_sweep));
+    gauge.U[0].assign(u.begin(), u.begin() + gauge.lat.V);
+    gauge.U[1].assign(u.begin() + gauge.lat.V, u.end());
+}
+
 // ------------------------------------------------------------------ multi-timescale MG HMC, src/hmc_multiscale.cpp
 // CoarseDeflState, src/hmc.h:308-324
 struct CoarseDeflState {

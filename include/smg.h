@@ -433,6 +433,18 @@ int smg_hmc_ms_evolve(smg_dirac* D, smg_mg* mg, smg_prolongator* P, smg_coarse* 
 int smg_hmc_ms_trajectory(smg_dirac* D, smg_mg* mg, smg_prolongator* P, smg_coarse* defl, const smg_ms_params* p,
                           const double* pi0, const double* eta, double u, smg_ms_result* res);
 
+/* ------------------------------------------------------------------ U(1) heatbath
+ * No counterpart in the reference, which thermalises by HMC: the thermalisation of the
+ * quenched configurations (Wilson action S = -beta sum_x Re P(x)).  n_sweeps sweeps of
+ * the checkerboard heatbath (per sweep: mu = 0, 1, each parity 0, 1), every link drawn
+ * from exp(beta Re(U A)), A its staple sum, by the Best-Fisher von Mises sampler on
+ * Philox4x32-10 numbers keyed by seed and counted by (link, sweep): the field depends
+ * only on (U, beta, seed, sweep numbers).  Sweeps are numbered from first_sweep, so
+ * n sweeps then m more equal n + m at once.  L even. */
+int smg_dirac_heatbath(smg_dirac* D, double beta, int n_sweeps, unsigned long long seed, unsigned first_sweep);
+/* the same on host links U[mu*V + s] (in/out) */
+int smg_heatbath(int L, double* U, double beta, int n_sweeps, unsigned long long seed, unsigned first_sweep);
+
 /* ------------------------------------------------------------------ gauge field files (host I/O)
  * Links are U[mu*V + s] (2V complex).  Host-only: no GPU is touched. */
 /* GaugeField::save / load, src/gauge.h:38-57: int32 V, then U[0] and U[1] as complex

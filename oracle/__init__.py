@@ -65,6 +65,10 @@ def lib():
         L.orc_hmc_ms_evolve.restype = _I
         L.orc_hmc_ms_evolve.argtypes = [_I, _P, _D, _D, _D, _D, _D, _D, _I, _I, _I, _I, _I, _I, _D, _P, _I, _P, _P,
                                         _P, _P, _P]
+        L.orc_philox4x32.argtypes = [_P, _P, _P]
+        L.orc_von_mises.restype = _D
+        L.orc_von_mises.argtypes = [_D, ctypes.c_uint32, ctypes.c_uint32, _P]
+        L.orc_heatbath.argtypes = [_I, _P, _D, _I, ctypes.c_uint64, ctypes.c_uint32]
         L.orc_lobpcg.restype = _I
         L.orc_lobpcg.argtypes = [_P, _P, _I, _P, _I, _D, _P, _P]
         L.orc_cg_deflated.restype = _I
@@ -603,3 +607,23 @@ def hmc_ms_evolve(L, U, mass, r, csw, mu_t, beta, tau, n_outer, n_inner, omelyan
                                  defl_refresh, cg_maxit, cg_tol, P.h, V.shape[0], _ptr(V), _ptr(vals), _ptr(pi),
                                  _ptr(_c(phi)), ctypes.byref(nl))
     return U, pi, V, vals, it, nl.value
+
+
+def philox4x32(ctr, key):
+    """Philox4x32-10 of the 4-word counter and 2-word key."""
+    c = np.ascontiguousarray(ctr, np.uint32); k = np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32(_ptr(c), _ptr(k), _ptr(out))
+    return out
+
+
+def von_mises(kappa, c0, c1, seed):
+    k = np.array([seed & 0xFFFFFFFF, seed >> 32], np.uint32)
+    return lib().orc_von_mises(kappa, c0, c1, _ptr(k))
+
+
+def heatbath(L, U, beta, n_sweeps, seed, first_sweep=0):
+    """The checkerboard U(1) heatbath: the new links."""
+    U = _c(U).copy()
+    lib().orc_heatbath(L, _ptr(U), beta, n_sweeps, seed, first_sweep)
+    return U

@@ -15,6 +15,7 @@
  */
 #include "oracle.h"
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -1355,6 +1356,94 @@ void orc_eo_fermion_force(const orc_dirac* D, const ocx* x_o, const ocx* y_o, do
         for (int i = 0; i < 2 * V; i++) F[i] += fl[i];
         free(fl);
     }
+}
+
+/* ================================================================ U(1) heatbath
+ * No reference counterpart (the reference thermalises by HMC).  Checkerboard heatbath of
+ * the Wilson action S = -beta sum_x Re P(x): links U_mu(x) of one direction and one
+ * parity are independent given the rest; each is drawn from exp(beta Re(U A)), A the sum
+ * of its two staples, as U = e^{i phi} conj(A)/|A| with phi from the von Mises density
+ * exp(kappa cos phi), kappa = beta |A| (Best & Fisher 1979, Appl. Statist. 28:152-157).
+ * Random numbers: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC11), key = seed, counter
+ * = (link index mu*V + x, 4*sweep + 2*mu + parity, attempt, 0). */
+static inline uint32_t mulhilo32(uint32_t a, uint32_t b, uint32_t* hi) {
+    uint64_t p = (uint64_t)a * b;
+    *hi = (uint32_t)(p >> 32);
+    return (uint32_t)p;
+}
+void orc_philox4x32(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; r++) {
+        uint32_t hi0, hi1;
+        uint32_t lo0 = mulhilo32(0xD2511F53u, c0, &hi0);
+        uint32_t lo1 = mulhilo32(0xCD9E8D57u, c2, &hi1);
+        uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+#define ORC_PI 3.14159265358979323846
+static inline double u32_to_unit(uint32_t x) { return ((double)x + 0.5) * (1.0 / 4294967296.0); }
+/* one von Mises(kappa) draw from the Philox stream (ctr[2] counts attempts); a stream
+ * that rejects 1000 times returns nan (the caller keeps the old link) */
+double orc_von_mises(double kappa, uint32_t ctr0, uint32_t ctr1, const uint32_t key[2]) {
+    uint32_t ctr[4] = {ctr0, ctr1, 0u, 0u}, o[4];
+    if (kappa < 1e-8) {
+        orc_philox4x32(ctr, key, o);
+        return ORC_PI * (2.0 * u32_to_unit(o[0]) - 1.0);
+    }
+    double tau = 1.0 + sqrt(1.0 + 4.0 * kappa * kappa);
+    double rho = (tau - sqrt(2.0 * tau)) / (2.0 * kappa);
+    double rr = (1.0 + rho * rho) / (2.0 * rho);
+    for (uint32_t a = 0; a < 1000; a++) {
+        ctr[2] = a;
+        orc_philox4x32(ctr, key, o);
+        double u1 = u32_to_unit(o[0]), u2 = u32_to_unit(o[1]), u3 = u32_to_unit(o[2]);
+        double z = cos(ORC_PI * u1);
+        double f = (1.0 + rr * z) / (rr + z);
+        double c = kappa * (rr - f);
+        if (c * (2.0 - c) - u2 > 0.0 || log(c / u2) + 1.0 - c >= 0.0) {
+            double ph = acos(fmax(-1.0, fmin(1.0, f)));
+            return u3 > 0.5 ? ph : -ph;
+        }
+    }
+    return NAN;
+}
+/* the staple sum A of U_mu(x): Re(U_mu(x) A) is the x-dependent part of sum Re P */
+static ocx staple(int L, const ocx* U, int mu, int s) {
+    int V = L * L, x = s % L, y = s / L;
+    int xp = (x + 1) % L + L * y, xm = (x - 1 + L) % L + L * y;
+    int yp = x + L * ((y + 1) % L), ym = x + L * ((y - 1 + L) % L);
+    const ocx* U0 = U;
+    const ocx* U1 = U + V;
+    if (mu == 0) {
+        int xp_ym = (x + 1) % L + L * ((y - 1 + L) % L);
+        return U1[xp] * conj(U0[yp]) * conj(U1[s]) + conj(U0[ym]) * conj(U1[xp_ym]) * U1[ym];
+    }
+    int xm_yp = (x - 1 + L) % L + L * ((y + 1) % L);
+    return conj(U0[s]) * conj(U1[xp]) * U0[yp] + U0[xm] * conj(U0[xm_yp]) * conj(U1[xm]);
+}
+/* n_sweeps sweeps (each: mu = 0, 1; parity 0, 1), the first numbered first_sweep */
+void orc_heatbath(int L, ocx* U, double beta, int n_sweeps, uint64_t seed, uint32_t first_sweep) {
+    int V = L * L;
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (int sw = 0; sw < n_sweeps; sw++)
+        for (int mu = 0; mu < 2; mu++)
+            for (int par = 0; par < 2; par++)
+                for (int s = 0; s < V; s++) {
+                    int x = s % L, y = s / L;
+                    if (((x + y) & 1) != par) continue;
+                    ocx A = staple(L, U, mu, s);
+                    double a = cabs(A);
+                    uint32_t sub = 4u * (first_sweep + (uint32_t)sw) + 2u * (uint32_t)mu + (uint32_t)par;
+                    double ph = orc_von_mises(beta * a, (uint32_t)(mu * V + s), sub, key);
+                    if (isnan(ph)) continue;
+                    ocx dir = a > 0.0 ? conj(A) / a : 1.0;
+                    U[(size_t)mu * V + s] = cexp(I * ph) * dir;
+                }
 }
 
 /* ================================================================ HMC molecular dynamics */

@@ -19,6 +19,7 @@
 #ifndef SMG_ORACLE_H
 #define SMG_ORACLE_H
 #include <complex.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -155,6 +156,11 @@ int orc_cg_deflated(const orc_op* A, const ocx* rhs, orc_mg* mg, int ndefl, cons
                     ocx* x, double* final_res);
 int orc_deflated_coarse_cg(const orc_op* A, int ndefl, const ocx* vecs, const double* vals, const ocx* b,
                            int maxit, double tol, ocx* x);
+
+/* ---------------- U(1) heatbath (no reference counterpart) ---------------- */
+void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double orc_von_mises(double kappa, uint32_t ctr0, uint32_t ctr1, const uint32_t key[2]);
+void orc_heatbath(int L, ocx* U, double beta, int n_sweeps, uint64_t seed, uint32_t first_sweep);
 
 /* ---------------- HMC molecular dynamics ---------------- */
 double orc_kinetic_energy(int n, const double* pi);

@@ -122,6 +122,8 @@ SIGNATURES = [
     ("smg_coarse_evolve_deflation", _I, [_P, _P]),
     ("smg_hmc_ms_evolve", _I, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     ("smg_hmc_ms_trajectory", _I, [_P, _P, _P, _P, _P, _P, _P, _D, _P]),
+    ("smg_dirac_heatbath", _I, [_P, _D, _I, ctypes.c_ulonglong, ctypes.c_uint]),
+    ("smg_heatbath", _I, [_I, _P, _D, _I, ctypes.c_ulonglong, ctypes.c_uint]),
     ("smg_coarse_lobpcg", _I, [_P, _I, _P, _I, _D, _P, _P, _P]),
     ("smg_mg_update_coarsest_deflation", _I, [_P, _I, _I]),
     ("smg_mg_coarsest_deflation", _I, [_P, _P, _P, _P]),
@@ -827,6 +829,20 @@ def hmc_trajectory_mg_multiscale(D, P, defl, params, pi0, eta, u, mg=None):
     _chk(lib().smg_hmc_ms_trajectory(D.h, mg.h if mg else None, P.h, defl.h, ctypes.byref(params),
                                      _p(np.ascontiguousarray(pi0, np.float64)), _p(_c(eta)), u, ctypes.byref(res)))
     return res
+
+
+# ------------------------------------------------------------------ U(1) heatbath
+def heatbath(L, U, beta, n_sweeps, seed, first_sweep=0):
+    """n_sweeps checkerboard U(1) heatbath sweeps of the Wilson action on host links
+    U[mu*V + s] (no reference counterpart): returns the new links."""
+    U = _c(U).copy()
+    _chk(lib().smg_heatbath(L, _p(U), beta, n_sweeps, seed, first_sweep))
+    return U
+
+
+def dirac_heatbath(D, beta, n_sweeps, seed, first_sweep=0):
+    """The heatbath on the links of a DiracOp, in place on the device."""
+    _chk(lib().smg_dirac_heatbath(D.h, beta, n_sweeps, seed, first_sweep))
 
 
 # ------------------------------------------------------------------ forces

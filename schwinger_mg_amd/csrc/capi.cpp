@@ -1368,6 +1368,26 @@ int smg_hmc_ms_trajectory(smg_dirac* D, smg_mg* mg, smg_prolongator* P, smg_coar
     });
 }
 
+// ------------------------------------------------------------------ U(1) heatbath (no reference counterpart)
+int smg_dirac_heatbath(smg_dirac* D, double beta, int n_sweeps, unsigned long long seed, unsigned first_sweep) {
+    return guard([&] {
+        SMG_REQUIRE(D && n_sweeps >= 0, "bad arguments");
+        Dirac& d = *D->D;
+        launch_heatbath(d.U64.p(), d.L, beta, n_sweeps, seed, first_sweep, S());
+        d.refresh_derived(S());
+        SMG_HIP(hipStreamSynchronize(S()));
+    });
+}
+int smg_heatbath(int L, double* U, double beta, int n_sweeps, unsigned long long seed, unsigned first_sweep) {
+    return guard([&] {
+        SMG_REQUIRE(U && L >= 2 && L % 2 == 0 && n_sweeps >= 0, "bad arguments (L even)");
+        const size_t n = 2 * (size_t)L * L;
+        Field<double> u = up(U, n);
+        launch_heatbath(u.p(), L, beta, n_sweeps, seed, first_sweep, S());
+        downc(U, u.p(), n);
+    });
+}
+
 // ------------------------------------------------------------------ gauge field files (host I/O)
 namespace {
 // GaugeField::avg_plaq, src/gauge.h:26-36, summed in site order (the reference's

@@ -77,6 +77,10 @@ void launch_gauge_force(int L, const c64* U, double beta, double* F, hipStream_t
 void launch_link_update(c64* U, c32* U32, const double* pi, double scale, size_t n, hipStream_t st);
 void launch_mom_update(double* pi, const double* gf, const double* ff, double scale, size_t n, hipStream_t st);
 void launch_clover_scale(const double* F01, double csw, double* C64, float* C32, int V, hipStream_t st);
+// n_sweeps checkerboard U(1) heatbath sweeps of the Wilson action on the FP64 links
+// U[mu*V + x] (L even), the first numbered first_sweep
+void launch_heatbath(c64* U, int L, double beta, int n_sweeps, unsigned long long seed, unsigned first_sweep,
+                     hipStream_t st);
 
 // ---------------------------------------------------------------- BLAS-1 (blas.hip)
 // reductions finished in-kernel by the scalar program of fin

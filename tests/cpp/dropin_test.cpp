@@ -6,6 +6,7 @@
 // and exits non-zero on the first failure.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -59,6 +60,7 @@ int orc_trlm(const orc_op* A, int n_ev, int n_kr, int max_restarts, double tol, 
 int orc_cg_deflated(const orc_op* A, const cx* rhs, orc_mg* mg, int ndefl, const cx* U, int maxit, double tol, cx* x,
                     double* final_res);
 int orc_lobpcg(const orc_op* A, orc_mg* mg, int k, const cx* X0, int max_iter, double tol, cx* Xout, double* evals);
+void orc_heatbath(int L, cx* U, double beta, int n_sweeps, uint64_t seed, uint32_t first_sweep);
 typedef struct orc_prolong orc_prolong;
 orc_prolong* orc_prolong_new(int nx, int ny, int nd, int bx, int by, int k, int chiral);
 void orc_prolong_build(orc_prolong* P, const cx* null_vecs);
@@ -452,6 +454,17 @@ int main() {
         report("hmc_trajectory_mg_multiscale evolved deflation", dv, 1e-10);
         if (mr.accepted) report("hmc_trajectory_mg_multiscale links", relerr(gm.links(), Uo), 1e-10);
         else report("hmc_trajectory_mg_multiscale rejected: links restored", relerr(gm.links(), g0.links()), 0.0);
+    }
+
+    {   // the heatbath addition: 10 sweeps at beta = 4 against the oracle
+        Lattice lh(16);
+        GaugeField gh(lh);
+        std::mt19937 rh(3);
+        gh.randomise(rh, 0.35);
+        Vec uo = gh.links();
+        heatbath(gh, 4.0, 10, 99);
+        orc_heatbath(16, uo.data(), 4.0, 10, 99, 0);
+        report("heatbath (addition) links", relerr(gh.links(), uo), 1e-10);
     }
 
     orc_mg_free(mgo);
