@@ -64,6 +64,8 @@ def parse():
     ap.add_argument("--single", type=int, default=1, help="also time one right-hand side at a time")
     ap.add_argument("--fp64", type=int, default=1, help="also time the FP64 batched solve")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--lattice-batch", type=int, default=1,
+                    help="also time the 40-lattice batch of the workload (one launch over all lattices)")
     return ap.parse_args()
 
 
@@ -119,6 +121,51 @@ def time_stencil(smg, torch, D, prec, nrhs, V, csw, launches=50, warm=5):
     byt = stencil_bytes_per_site(prec, nrhs, csw) * V
     return {"prec": prec, "nrhs": nrhs, "us_per_launch": ms * 1e3, "bytes_per_launch": byt,
             "GBs": byt / (ms * 1e-3) / 1e9}
+
+
+def lattice_batch(smg, torch, gaussian_vec, random_gauge):
+    """The configuration-3 batch of 40 lattices (L in {64, 96, 128, 192, 256}, 8 each,
+    theta ~ N(0, 0.4^2), independent seeds, m = 0.1, c_sw = 1) as one Lattices handle:
+    the FP64 Wilson-clover D over all 40 in one launch (event-timed on the library
+    stream, 50 launches) and the lockstep CG on D^dag D of all 40 to 1e-10 (host-array
+    entry, wall time including its PCIe copies of b and x)."""
+    import ctypes
+    Ls = [L for L in (64, 96, 128, 192, 256) for _ in range(8)]
+    Us = [random_gauge(L, 0.4, 800 + i) for i, L in enumerate(Ls)]
+    B = smg.Lattices(Ls, Us, 0.1, 1.0, 1.0, 0.0)
+    n = int(B.offsets[-1])
+    sites = n // 2
+    src = torch.randn(n, dtype=torch.complex128, device="cuda")
+    dst = torch.empty_like(src)
+    sp = smg.library_stream()
+    st = torch.cuda.ExternalStream(sp)
+    sarg = ctypes.c_void_p(sp)
+    torch.cuda.synchronize()
+
+    def launch():
+        smg._chk(smg.lib().smg_lattices_apply_dev(B.h, smg.D_, src.data_ptr(), dst.data_ptr(), sarg))
+
+    for _ in range(5):
+        launch()
+    smg.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(50)]
+    for e0, e1 in ev:
+        e0.record(st)
+        launch()
+        e1.record(st)
+    ev[-1][1].synchronize()
+    us = sum(e0.elapsed_time(e1) for e0, e1 in ev) / 50 * 1e3
+    byt = stencil_bytes_per_site("f64", 1, True) * sites
+    bs = [gaussian_vec(2 * L * L, 900 + i) for i, L in enumerate(Ls)]
+    B.cg(bs, 5000, 1e-10)   # warm-up
+    smg.synchronize()
+    t0 = time.perf_counter()
+    _, its, tr = B.cg(bs, 5000, 1e-10)
+    dt = time.perf_counter() - t0
+    return {"lattices": len(Ls), "L": [64, 96, 128, 192, 256], "sites": sites,
+            "dslash_f64_us": round(us, 2), "dslash_f64_GBs": round(byt / (us * 1e-6) / 1e9, 1),
+            "cg_batch_s": round(dt, 4), "cg_lattice_solves_per_s": round(len(Ls) / dt, 2),
+            "cg_iterations": [int(its.min()), int(its.max())], "max_true_residual": float(tr.max())}
 
 
 def cpu_baseline(args, U, B, mg, threads):
@@ -245,6 +292,10 @@ def main():
 
     stencil = [time_stencil(smg, torch, D, p, n, V, True) for p, n in
                [("f32", 1), ("f64", 1), ("f32", N), ("f64", N)]]
+    lbatch = None
+    if args.lattice_batch:
+        lbatch = lattice_batch(smg, torch, gaussian_vec, random_gauge)
+        log(f"lattice batch: {lbatch}")
     dom = stencil[2]     # FP32, N RHS: the fine-level stencil of the mixed solve
     log("stencil: " + ", ".join(f"{s['prec']}x{s['nrhs']} {s['GBs']:.0f} GB/s" for s in stencil))
     roofline = {"bound": "hbm", "achieved": round(dom["GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -293,6 +344,7 @@ def main():
         "mixed_vs_fp64_solution_relerr": err_vs_f64,
         "setup_s": round(setup_s, 3),
         "stencil": stencil,
+        "lattice_batch": lbatch,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

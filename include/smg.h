@@ -63,6 +63,7 @@ typedef struct smg_dirac_s smg_dirac;
 typedef struct smg_prolongator_s smg_prolongator;
 typedef struct smg_coarse_s smg_coarse;
 typedef struct smg_mg_s smg_mg;
+typedef struct smg_lattices_s smg_lattices;
 
 /* ------------------------------------------------------------------ runtime */
 /* Select the HIP device and create the library's stream.  Optional: the first
@@ -432,6 +433,26 @@ int smg_hmc_ms_evolve(smg_dirac* D, smg_mg* mg, smg_prolongator* P, smg_coarse* 
  * that second draw), u the Metropolis uniform */
 int smg_hmc_ms_trajectory(smg_dirac* D, smg_mg* mg, smg_prolongator* P, smg_coarse* defl, const smg_ms_params* p,
                           const double* pi0, const double* eta, double u, smg_ms_result* res);
+
+/* ------------------------------------------------------------------ many lattices in one launch
+ * No reference counterpart (the reference runs one lattice at a time): G Wilson-clover
+ * lattices of different sizes (L[g] >= 2; links U of lattice g follow those of g-1, each
+ * U[mu*V + s]; masses mass[g]; r, csw, mu_t shared) whose operators and CG solves run
+ * as one launch per kernel for all of them.  Vectors are the lattices' vectors one after
+ * the other (2 V_g complex each, psi[2s + spin]); smg_lattices_size gives the total. */
+int smg_lattices_create(smg_lattices** out, int G, const int* L, const double* U, const double* mass, double r,
+                        double csw, double mu_t);
+int smg_lattices_destroy(smg_lattices* H);
+int smg_lattices_size(smg_lattices* H, size_t* n);
+/* kind SMG_D, SMG_DDAG, SMG_DDAGD or SMG_G5D on every lattice (DiracOp::apply and
+ * friends, src/dirac.h:93-152, per lattice); _dev: FP64 device arrays */
+int smg_lattices_apply(smg_lattices* H, int kind, const double* src, double* dst);
+int smg_lattices_apply_dev(smg_lattices* H, int kind, const void* src, void* dst, void* stream);
+/* cg_solve on D^dag D (src/solvers.cpp:16-72) for every lattice at once from x0 = 0:
+ * per-lattice scalars and stopping rule |r_g|/|b_g| < tol (a converged lattice takes no
+ * further steps); iters[g] and the FP64 true residuals true_res[g] (either may be NULL) */
+int smg_lattices_cg(smg_lattices* H, const double* b, double* x, int maxit, double tol, int* iters,
+                    double* true_res);
 
 /* ------------------------------------------------------------------ U(1) heatbath
  * No counterpart in the reference, which thermalises by HMC: the thermalisation of the

@@ -123,6 +123,10 @@ SIGNATURES = [
     ("smg_hmc_ms_evolve", _I, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     ("smg_hmc_ms_trajectory", _I, [_P, _P, _P, _P, _P, _P, _P, _D, _P]),
     ("smg_dirac_heatbath", _I, [_P, _D, _I, ctypes.c_ulonglong, ctypes.c_uint]),
+    ("smg_lattices_create", _I, [_P, _I, _P, _P, _P, _D, _D, _D]), ("smg_lattices_destroy", _I, [_P]),
+    ("smg_lattices_size", _I, [_P, _P]), ("smg_lattices_apply", _I, [_P, _I, _P, _P]),
+    ("smg_lattices_apply_dev", _I, [_P, _I, _P, _P, _P]),
+    ("smg_lattices_cg", _I, [_P, _P, _P, _I, _D, _P, _P]),
     ("smg_heatbath", _I, [_I, _P, _D, _I, ctypes.c_ulonglong, ctypes.c_uint]),
     ("smg_coarse_lobpcg", _I, [_P, _I, _P, _I, _D, _P, _P, _P]),
     ("smg_mg_update_coarsest_deflation", _I, [_P, _I, _I]),
@@ -829,6 +833,51 @@ def hmc_trajectory_mg_multiscale(D, P, defl, params, pi0, eta, u, mg=None):
     _chk(lib().smg_hmc_ms_trajectory(D.h, mg.h if mg else None, P.h, defl.h, ctypes.byref(params),
                                      _p(np.ascontiguousarray(pi0, np.float64)), _p(_c(eta)), u, ctypes.byref(res)))
     return res
+
+
+# ------------------------------------------------------------------ many lattices in one launch
+class Lattices:
+    """G Wilson-clover lattices of different sizes whose operators and CG solves run as
+    one launch per kernel (no reference counterpart).  Us: the lattices' link arrays
+    (each U[mu*V + s]); vectors are lists of per-lattice vectors."""
+
+    def __init__(self, Ls, Us, masses, r=1.0, csw=0.0, mu_t=0.0):
+        self.Ls = [int(L) for L in Ls]
+        self.G = len(self.Ls)
+        Ucat = np.concatenate([_c(U) for U in Us])
+        Larr = np.ascontiguousarray(self.Ls, np.int32)
+        m = np.ascontiguousarray(np.broadcast_to(masses, (self.G,)), np.float64)
+        h = _P()
+        _chk(lib().smg_lattices_create(ctypes.byref(h), self.G, _p(Larr), _p(Ucat), _p(m), r, csw, mu_t))
+        self.h = h
+        self.sizes = [2 * L * L for L in self.Ls]
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)])
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.smg_lattices_destroy(self.h)
+            self.h = None
+
+    def _cat(self, vs):
+        return np.concatenate([_c(v) for v in vs])
+
+    def _split(self, v):
+        return [v[self.offsets[g]:self.offsets[g + 1]] for g in range(self.G)]
+
+    def apply(self, kind, vs):
+        src = self._cat(vs)
+        dst = np.zeros_like(src)
+        _chk(lib().smg_lattices_apply(self.h, kind, _p(src), _p(dst)))
+        return self._split(dst)
+
+    def cg(self, bs, max_iter, tol):
+        """CG on D^dag D for every lattice at once: (solutions, iterations, true residuals)."""
+        b = self._cat(bs)
+        x = np.zeros_like(b)
+        it = np.zeros(self.G, np.int32)
+        tr = np.zeros(self.G)
+        _chk(lib().smg_lattices_cg(self.h, _p(b), _p(x), max_iter, tol, _p(it), _p(tr)))
+        return self._split(x), it, tr
 
 
 # ------------------------------------------------------------------ U(1) heatbath

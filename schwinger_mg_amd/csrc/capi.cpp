@@ -11,6 +11,7 @@
 #include "../../include/smg.h"
 #include "eigen.h"
 #include "runtime.h"
+#include "smg_lattices.h"
 #include "runtime_b.h"
 
 using namespace smg;
@@ -28,6 +29,13 @@ struct smg_prolongator_s {
 struct smg_coarse_s {
     std::unique_ptr<CoarseOperator> C;
     Eigenpairs defl;   // SparseCoarseOp::defl_vecs / defl_vals (empty: no deflation)
+};
+struct smg_lattices_s {
+    std::vector<std::unique_ptr<Dirac>> D;   // per lattice: links, clover, parameters
+    std::vector<LatDesc> h;                  // host copy of the table
+    DevMem d;                                // the table on the device
+    size_t sites = 0;
+    int nwg = 0;
 };
 struct smg_mg_s {
     std::unique_ptr<MG> mg;
@@ -1385,6 +1393,154 @@ int smg_heatbath(int L, double* U, double beta, int n_sweeps, unsigned long long
         Field<double> u = up(U, n);
         launch_heatbath(u.p(), L, beta, n_sweeps, seed, first_sweep, S());
         downc(U, u.p(), n);
+    });
+}
+
+// ------------------------------------------------------------------ many lattices in one launch
+int smg_lattices_create(smg_lattices** out, int G, const int* L, const double* U, const double* mass, double r,
+                        double csw, double mu_t) {
+    return guard([&] {
+        SMG_REQUIRE(out && L && U && mass && G >= 1 && G <= 4096, "bad arguments (1 <= G <= 4096)");
+        auto* h = new smg_lattices_s();
+        try {
+            size_t off = 0;
+            for (int g = 0; g < G; g++) {
+                SMG_REQUIRE(L[g] >= 2, "every L must be >= 2");
+                h->D.emplace_back(new Dirac(L[g], reinterpret_cast<const c64*>(U) + off, mass[g], r, csw, mu_t));
+                off += 2 * (size_t)L[g] * L[g];
+                const DiracView<double> v = h->D.back()->v64();
+                LatDesc e;
+                e.U = v.U;
+                e.C = v.C;
+                e.diag = v.diag;
+                e.r = v.r;
+                e.mu = mu_t;
+                e.L = L[g];
+                e.wg0 = h->nwg;
+                e.site0 = (long long)h->sites;
+                h->h.push_back(e);
+                h->sites += (size_t)L[g] * L[g];
+                h->nwg += (L[g] * L[g] + kBlock - 1) / kBlock;
+            }
+            SMG_REQUIRE((size_t)h->nwg * 3 <= (size_t(1) << 21), "too many sites for one launch's partial sums");
+            h->d.alloc(h->h.size() * sizeof(LatDesc));
+            SMG_HIP(hipMemcpyAsync(h->d.p, h->h.data(), h->h.size() * sizeof(LatDesc), hipMemcpyHostToDevice, S()));
+            Context::get().sync();
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+int smg_lattices_destroy(smg_lattices* H) { return guard([&] { delete H; }); }
+int smg_lattices_size(smg_lattices* H, size_t* n) {
+    return guard([&] {
+        SMG_REQUIRE(H && n, "null argument");
+        *n = 2 * H->sites;
+    });
+}
+namespace {
+LatFin latfin(int op, double* out, double* aux) {
+    const Fin f = Context::get().fin(op, out, aux);
+    LatFin l;
+    l.op = op;
+    l.out = out;
+    l.aux = aux;
+    l.part = f.part;
+    l.counter = f.counter;
+    return l;
+}
+void ml_apply(smg_lattices* H, int kind, c64* dst, const c64* src, c64* tmp, const LatFin& skip_fin,
+              const LatFin* dots, const c64* aux, hipStream_t st) {
+    const LatDesc* d = H->d.as<LatDesc>();
+    const int G = (int)H->h.size();
+    if (kind == SMG_DDAGD) {
+        launch_ml_dslash(d, G, H->nwg, tmp, src, 1.0, 0, 0, EPI_NONE, nullptr, skip_fin, st);
+        launch_ml_dslash(d, G, H->nwg, dst, tmp, -1.0, 1, 1, dots ? EPI_DOTS : EPI_NONE, aux, dots ? *dots : skip_fin,
+                         st);
+        return;
+    }
+    SMG_REQUIRE(kind == SMG_D || kind == SMG_DDAG || kind == SMG_G5D, "unknown operator kind");
+    const double ms = kind == SMG_DDAG ? -1.0 : 1.0;
+    const int g5in = kind == SMG_DDAG ? 1 : 0, g5out = kind == SMG_D ? 0 : 1;
+    launch_ml_dslash(d, G, H->nwg, dst, src, ms, g5in, g5out, EPI_NONE, nullptr, skip_fin, st);
+}
+}  // namespace
+int smg_lattices_apply(smg_lattices* H, int kind, const double* src, double* dst) {
+    return guard([&] {
+        SMG_REQUIRE(H && src && dst, "null argument");
+        const size_t n = 2 * H->sites;
+        Field<double> s_ = up(src, n), d_(n), t(n);
+        ml_apply(H, kind, d_.p(), s_.p(), t.p(), LatFin(), nullptr, nullptr, S());
+        downc(dst, d_.p(), n);
+    });
+}
+int smg_lattices_apply_dev(smg_lattices* H, int kind, const void* src, void* dst, void* stream) {
+    return guard([&] {
+        SMG_REQUIRE(H && src && dst, "null argument");
+        StreamJoin j(stream);
+        Field<double> t(kind == SMG_DDAGD ? 2 * H->sites : 1);
+        ml_apply(H, kind, static_cast<c64*>(dst), static_cast<const c64*>(src), t.p(), LatFin(), nullptr, nullptr, S());
+    });
+}
+int smg_lattices_cg(smg_lattices* H, const double* b, double* x, int maxit, double tol, int* iters,
+                    double* true_res) {
+    return guard([&] {
+        SMG_REQUIRE(H && b && x && maxit >= 0 && tol > 0.0, "bad arguments");
+        const int G = (int)H->h.size();
+        const size_t n = 2 * H->sites;
+        const LatDesc* d = H->d.as<LatDesc>();
+        Field<double> bb = up(b, n), xx(n), r(n), p(n), Ap(n), t(n);
+        DevMem sc((4 * (size_t)G + 16 * (size_t)G + 4 * (size_t)G) * sizeof(double) + 64);
+        double* alpha = sc.as<double>();
+        double* beta = alpha + 2 * G;
+        double* aux = beta + 2 * G;
+        double* nrm = aux + 16 * G;   // [G] |b - A x|^2, then [G] |b|^2
+        int* done = reinterpret_cast<int*>(nrm + 4 * G);
+        int* late = done + 1;
+        launch_ml_cg_state(aux, G, tol * tol, done, late, S());
+        launch_ml_cg_init(d, G, H->nwg, bb.p(), xx.p(), r.p(), p.p(), latfin(FIN_CG_INIT, nullptr, aux), S());
+        LatFin skip;
+        skip.skip = done;
+        LatFin fa = latfin(FIN_CG_ALPHA, alpha, aux), fbt = latfin(FIN_CG_BETA, beta, aux);
+        fa.skip = done;
+        fbt.skip = done;
+        fbt.done = done;
+        for (int it = 0; it < maxit; it++) {
+            ml_apply(H, SMG_DDAGD, Ap.p(), p.p(), t.p(), skip, &fa, p.p(), S());
+            launch_ml_cg_r(d, G, H->nwg, r.p(), Ap.p(), alpha, fbt, late, S());
+            launch_ml_cg_xp(d, G, H->nwg, xx.p(), p.p(), r.p(), alpha, beta, late, S());
+            if (it % 32 == 31) {   // stop enqueueing once every lattice has converged
+                int h = 0;
+                SMG_HIP(hipMemcpyAsync(&h, done, sizeof(int), hipMemcpyDeviceToHost, S()));
+                SMG_HIP(hipStreamSynchronize(S()));
+                if (h) break;
+            }
+        }
+        // iterations and FP64 true residuals per lattice
+        std::vector<double> ah(16 * (size_t)G);
+        SMG_HIP(hipMemcpyAsync(ah.data(), aux, ah.size() * sizeof(double), hipMemcpyDeviceToHost, S()));
+        ml_apply(H, SMG_DDAGD, Ap.p(), xx.p(), t.p(), LatFin(), nullptr, nullptr, S());
+        launch_axpby<double>(Ap.p(), bb.p(), c64{1.0, 0.0}, c64{-1.0, 0.0}, n, S());   // b - A x
+        launch_ml_norm2(d, G, H->nwg, Ap.p(), latfin(FIN_STORE, nrm, nullptr), S());
+        // FIN_STORE writes one value per lattice at out + t*ostride: use stride 1
+        std::vector<double> hn(2 * (size_t)G);
+        SMG_HIP(hipStreamSynchronize(S()));
+        {
+            LatFin fs = latfin(FIN_STORE, nrm + 2 * G, nullptr);
+            fs.ostride = 1;
+            launch_ml_norm2(d, G, H->nwg, bb.p(), fs, S());
+        }
+        SMG_HIP(hipMemcpyAsync(hn.data(), nrm, 2 * G * sizeof(double), hipMemcpyDeviceToHost, S()));
+        std::vector<double> hb(G);
+        SMG_HIP(hipMemcpyAsync(hb.data(), nrm + 2 * G, G * sizeof(double), hipMemcpyDeviceToHost, S()));
+        SMG_HIP(hipStreamSynchronize(S()));
+        for (int g = 0; g < G; g++) {
+            if (iters) iters[g] = (int)ah[(size_t)g * 16 + 3];
+            if (true_res) true_res[g] = hb[g] > 0.0 ? std::sqrt(hn[2 * g] / hb[g]) : 0.0;
+        }
+        downc(x, xx.p(), n);
     });
 }
 
