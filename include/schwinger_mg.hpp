@@ -535,6 +535,21 @@ struct MGHierarchy {
     }
     std::vector<Vec> coarsest_defl_vecs;
     std::vector<double> coarsest_defl_vals;
+    // levels[0].Ac.build(D, P0) followed by rebuild_deeper_levels (src/multigrid.cpp:
+    // 163-194): call after the hierarchy's DiracOp has taken new links (update_gauge)
+    void rebuild_deeper_levels() { detail::check(smg_mg_rebuild(h_.get())); }
+    // src/multigrid.cpp:754-794 with solver "trlm"; fine_op is the hierarchy's own
+    // operator, whose level-1 Galerkin operator the library already holds
+    void setup_sparse_coarse(const OpApply& fine_op, int fine_dim, int n_defl, double cg_tol = 1e-12,
+                             int max_cg_iter = 200, const std::string& solver = "trlm", double feast_emax = 0.0) {
+        (void)fine_op;
+        (void)fine_dim;
+        (void)feast_emax;
+        if (solver != "trlm") throw std::invalid_argument("setup_sparse_coarse: only the 'trlm' solver is provided");
+        std::mt19937 rng(42);
+        const Vec start = random_vec(dims.at(1), rng);
+        detail::check(smg_mg_setup_sparse_coarse(h_.get(), n_defl, cg_tol, max_cg_iter, detail::raw(start)));
+    }
     smg_mg* handle() const { return h_.get(); }
     std::shared_ptr<smg_mg> h_;
 };

@@ -290,6 +290,20 @@ int smg_coarse_lobpcg(smg_coarse* C, int k, const double* X0, int max_iter, doub
  * times from mt19937(42); n_defl <= 32.  smg_mg_coarsest_deflation reads the space
  * (evals[ndefl], evecs[ndefl][dim]; either may be NULL). */
 int smg_mg_update_coarsest_deflation(smg_mg* mg, int n_defl, int lobpcg_iters);
+/* MGHierarchy::setup_sparse_coarse (src/multigrid.cpp:754-794; declared
+ * src/multigrid.h:73-80) with solver "trlm": TRLM deflation (n_defl vectors, n_kr =
+ * 2 n_defl + 10, tol 1e-10, from start) of the level-1 Galerkin operator, then the FP64
+ * cycle's coarsest solve is its deflated CG (SparseCoarseOp::solve, to cg_tol in at most
+ * max_cg_iter steps).  The reference wires the level-1 operator as the coarsest solve of
+ * any hierarchy, which only fits two levels: other depths are refused.  The FP32 cycle
+ * keeps its own coarsest solver. */
+int smg_mg_setup_sparse_coarse(smg_mg* mg, int n_defl, double cg_tol, int max_cg_iter, const double* start);
+/* levels[0].Ac.build(D, P0) then MGHierarchy::rebuild_deeper_levels (src/multigrid.cpp:
+ * 163-194): every level's Galerkin operator re-projected, with the existing
+ * prolongators, for the current gauge field of the hierarchy's operator (after
+ * smg_dirac_set_gauge or an HMC update); the dense coarsest inverse follows, the
+ * deflation spaces are kept as they are (as in the reference) */
+int smg_mg_rebuild(smg_mg* mg);
 int smg_mg_coarsest_deflation(smg_mg* mg, int* ndefl, double* evals, double* evecs);
 /* cg_solve_deflated (src/solvers.cpp:111-213; declared src/solvers.h:50-56):
  * CG from x0 = U E^-1 U^dag b, E = U^dag A U, with the preconditioned residual

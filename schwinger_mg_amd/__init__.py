@@ -131,6 +131,7 @@ SIGNATURES = [
     ("smg_coarse_lobpcg", _I, [_P, _I, _P, _I, _D, _P, _P, _P]),
     ("smg_mg_update_coarsest_deflation", _I, [_P, _I, _I]),
     ("smg_mg_coarsest_deflation", _I, [_P, _P, _P, _P]),
+    ("smg_mg_setup_sparse_coarse", _I, [_P, _I, _D, _I, _P]), ("smg_mg_rebuild", _I, [_P]),
 ]
 
 _lib = None
@@ -562,6 +563,18 @@ class MGHierarchy:
     def update_coarsest_deflation(self, n_defl, lobpcg_iters=3):
         """MGHierarchy::update_coarsest_deflation (src/multigrid.cpp:111-141)."""
         _chk(lib().smg_mg_update_coarsest_deflation(self.h, n_defl, lobpcg_iters))
+
+    def setup_sparse_coarse(self, n_defl, cg_tol=1e-12, max_cg_iter=200, start_vec=None):
+        """MGHierarchy::setup_sparse_coarse (src/multigrid.cpp:754-794), TRLM: the FP64
+        cycle's coarsest solve becomes deflated CG (two-level hierarchies)."""
+        dim = self.level_dim(1)
+        start = random_vec(dim, 42) if start_vec is None else _c(start_vec)
+        _chk(lib().smg_mg_setup_sparse_coarse(self.h, n_defl, cg_tol, max_cg_iter, _p(start)))
+
+    def rebuild(self):
+        """Re-project every level's Galerkin operator for the operator's current gauge
+        field (Ac.build + rebuild_deeper_levels, src/multigrid.cpp:163-194)."""
+        _chk(lib().smg_mg_rebuild(self.h))
 
     def coarsest_deflation(self):
         """(defl_vals, defl_vecs) of the coarsest level (src/multigrid.h:28-29)."""

@@ -1759,6 +1759,34 @@ int smg_cg_solve_deflated(smg_dirac* D, int op, smg_mg* mg, const double* b, int
     });
 }
 
+int smg_mg_rebuild(smg_mg* mg) {
+    return guard([&] {
+        SMG_REQUIRE(mg, "null handle");
+        mg->mg->rebuild(S());
+    });
+}
+int smg_mg_setup_sparse_coarse(smg_mg* mg, int n_defl, double cg_tol, int max_cg_iter, const double* start) {
+    return guard([&] {
+        SMG_REQUIRE(mg && start && n_defl >= 0 && max_cg_iter >= 0 && cg_tol > 0.0, "bad arguments");
+        MG& m = *mg->mg;
+        SMG_REQUIRE(m.cfg.n_levels == 2,
+                    "setup_sparse_coarse wires the level-1 operator as the coarsest solve: two-level hierarchies");
+        const int dim = (int)m.C[0]->n();
+        // SparseCoarseOp::setup_deflation(n_defl, 0, 100, 1e-10) on the level-1 operator
+        int n_kr = std::min(2 * n_defl + 10, dim);
+        if (n_kr < n_defl + 2) n_kr = std::min(n_defl + 6, dim);
+        std::unique_ptr<Eigenpairs> E(new Eigenpairs());
+        if (n_defl > 0) {
+            SMG_REQUIRE(n_defl < dim, "n_defl must be below the coarse dimension");
+            Field<double> s0 = up(start, (size_t)dim);
+            CoarseLinOp<double, c64> op(*m.C[0]);
+            *E = trlm(op, n_defl, n_kr, 100, 1e-10, s0.p(), S());
+        }
+        m.sparse_defl = std::move(E);
+        m.sparse_maxit = max_cg_iter;
+        m.sparse_tol = cg_tol;
+    });
+}
 static int next_pow2(int k) {
     int N = 1;
     while (N < k) N <<= 1;

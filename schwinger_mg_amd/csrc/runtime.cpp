@@ -1,6 +1,8 @@
 // runtime.cpp -- host-side sequencing of the gfx950 kernels.
 #include "runtime.h"
 
+#include "eigen.h"
+
 #include <cmath>
 #include <cstring>
 #include <algorithm>
@@ -424,6 +426,12 @@ struct Cycle {
         LinOp<T>& A = *ops[lev];
         const int last = nlev - 1;
         if (lev == last) {
+            if constexpr (std::is_same<T, double>::value) {
+                if (mg.sparse_defl) {   // SparseCoarseOp::solve with its deflation space
+                    deflated_coarse_cg(A, *mg.sparse_defl, b, x, mg.sparse_maxit, mg.sparse_tol, st);
+                    return;
+                }
+            }
             if (mg.cfg.coarse_solver == 1) {
                 const cplx<T>* inv;
                 if constexpr (std::is_same<T, double>::value) inv = mg.inv64.p();
@@ -541,6 +549,19 @@ MG::MG(const Dirac& d, const MGConfig& c, const c64* starts, bool on_device, boo
 }
 MG::~MG() { reset_graphs(); }
 
+void MG::rebuild(hipStream_t st) {
+    LinOp<double>* cur = fine64.get();
+    std::vector<std::unique_ptr<LinOp<double>>> ops64;
+    for (size_t l = 0; l < C.size(); l++) {
+        C[l]->build(*cur, *P[l], st);
+        ops64.emplace_back(new CoarseLinOp<double, c64>(*C[l]));
+        cur = ops64.back().get();
+    }
+    if (cfg.coarse_solver == 1) build_dense_inverse(st);
+    reset_graphs();   // captured cycles are recaptured on their next use
+    SMG_HIP(hipStreamSynchronize(st));
+}
+
 void MG::reset_graphs() {
     for (auto& g : graph) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -612,7 +633,8 @@ static void run_graphed(MG& mg, Cycle<T>& cyc, MG::Graph& G, Field<T>& gin, Fiel
 }
 
 void MG::precondition(const c64* b, c64* x, hipStream_t st) {
-    if (cfg.use_graph) run_graphed<double>(*this, *cyc64, graph[0], gin64, gout64, b, x, st);
+    // the deflated coarsest CG reads its scalars back on the host: not capturable
+    if (cfg.use_graph && !sparse_defl) run_graphed<double>(*this, *cyc64, graph[0], gin64, gout64, b, x, st);
     else cyc64->run(0, b, x, st);
 }
 void MG::precondition(const c32* b, c32* x, hipStream_t st) {

@@ -207,6 +207,7 @@ struct MGConfig {
 
 template <typename T>
 struct Cycle;
+struct Eigenpairs;
 
 struct MG {
     MGConfig cfg;
@@ -224,6 +225,11 @@ struct MG {
     // coarsest-level deflation space (MGHierarchy::levels[last].defl_vecs / defl_vals)
     std::vector<double> defl_vals;
     Field<double> defl_vecs;
+    // setup_sparse_coarse: the coarsest solve of the FP64 cycle becomes the deflated CG
+    // of SparseCoarseOp::solve with this space (two-level hierarchies)
+    std::unique_ptr<Eigenpairs> sparse_defl;
+    int sparse_maxit = 200;
+    double sparse_tol = 1e-12;
     // dense inverse of the coarsest operator (cfg.coarse_solver == 1)
     Field<double> inv64;
     Field<float> inv32;
@@ -240,6 +246,10 @@ struct MG {
     MG(const Dirac& d, const MGConfig& c, const c64* starts, bool starts_on_device, bool schur = false);
     ~MG();
     void set_symmetric(double damping, const c64* starts, bool on_device);
+    // the Galerkin operators of every level re-projected for the current gauge field of
+    // the fine operator, with the existing prolongators (Ac.build at level 0, then
+    // MGHierarchy::rebuild_deeper_levels, src/multigrid.cpp:163-194)
+    void rebuild(hipStream_t st);
     void precondition(const c64* b, c64* x, hipStream_t st);
     void precondition(const c32* b, c32* x, hipStream_t st);
     template <typename T>
