@@ -628,8 +628,15 @@ struct FGMRESResult {
     Vec solution;
     int iterations;
     double final_residual;
-    std::vector<RitzPair> ritz_pairs;   // not harvested on the GPU path
+    std::vector<RitzPair> ritz_pairs;
 };
+
+// the identity preconditioner, recognised by lobpcg_update and fgmres_solve_generic
+// (the reference's callers pass a lambda returning its argument)
+struct IdentityPrecond {
+    Vec operator()(const Vec& v) const { return v; }
+};
+inline std::function<Vec(const Vec&)> identity_precond() { return IdentityPrecond{}; }
 
 namespace detail {
 inline const DiracOp& dirac_of(const OpApply& A, const char* who) {
@@ -712,20 +719,30 @@ inline FCGResult gcr_solve_mg(const OpApply& A, int n, const Vec& rhs, MGHierarc
     r.final_residual = info.true_residual;
     return r;
 }
-// fgmres_solve_generic / fgmres_solve_mg, src/solvers.cpp:555-811 (no Ritz harvest)
+// fgmres_solve_generic / fgmres_solve_mg, src/solvers.cpp:555-811, with the Ritz
+// harvest and intra-solve deflation of n_ritz_harvest pairs; precond:
+// MGHierarchy::as_precond(), identity_precond() or an empty std::function
 inline FGMRESResult fgmres_solve_generic(const OpApply& A, int n, const Vec& rhs,
                                          const std::function<Vec(const Vec&)>& precond, int restart, int max_iter,
                                          double tol, int n_ritz_harvest = 4) {
     (void)n;
-    (void)n_ritz_harvest;
     const DiracOp& D = detail::dirac_of(A, "fgmres_solve_generic");
-    MGHierarchy* mg = detail::mg_of(precond, "fgmres_solve_generic");
-    FGMRESResult r{Vec(D.lat.ndof), 0, 0.0, {}};
+    const bool ident = !precond || precond.target<IdentityPrecond>() != nullptr;
+    MGHierarchy* mg = ident ? nullptr : detail::mg_of(precond, "fgmres_solve_generic");
+    const int k = n_ritz_harvest > 0 ? n_ritz_harvest : 0;
+    const size_t nd = (size_t)D.lat.ndof;
+    FGMRESResult r{Vec(nd), 0, 0.0, {}};
     smg_solve_info info{};
-    detail::check(smg_fgmres_solve(D.handle(), mg->handle(), detail::raw(rhs), detail::raw(r.solution), restart,
-                                   max_iter, tol, &info));
+    std::vector<double> vals(2 * (size_t)std::max(k, 1));
+    Vec vecs(2 * (size_t)std::max(k, 1) * nd);
+    int npairs = 0;
+    detail::check(smg_fgmres_solve_ritz(D.handle(), mg ? mg->handle() : nullptr, detail::raw(rhs),
+                                        detail::raw(r.solution), restart, max_iter, tol, k, &info, &npairs,
+                                        vals.data(), detail::raw(vecs)));
     r.iterations = info.iterations;
     r.final_residual = info.final_residual;
+    for (int q = 0; q < npairs; q++)
+        r.ritz_pairs.push_back({vals[q], Vec(vecs.begin() + (ptrdiff_t)(q * nd), vecs.begin() + (ptrdiff_t)((q + 1) * nd))});
     return r;
 }
 inline FGMRESResult fgmres_solve_mg(const OpApply& A, int n, const Vec& rhs, MGHierarchy& mg, int restart,
@@ -797,13 +814,6 @@ struct LOBPCGResult {   // src/eigensolver.h:6-10
     std::vector<double> eigvals;
     int iterations;
 };
-// the identity preconditioner, recognised by lobpcg_update (the reference's callers
-// pass a lambda returning its argument)
-struct IdentityPrecond {
-    Vec operator()(const Vec& v) const { return v; }
-};
-inline std::function<Vec(const Vec&)> identity_precond() { return IdentityPrecond{}; }
-
 // lobpcg_update, src/eigensolver_core.cpp:279-533, on DiracOp::as_DdagD_op() (precond:
 // MGHierarchy::as_precond(), identity_precond() or an empty std::function) or on
 // SparseCoarseOp::as_op() (identity only); k <= 32, run as one block on the device

@@ -234,6 +234,13 @@ int smg_gcr_solve_dev(smg_dirac* D, smg_mg* mg, int prec, const void* b, void* x
 /* fgmres_solve_generic / fgmres_solve_mg with n_ritz_harvest = 0, src/solvers.cpp:555-800 */
 int smg_fgmres_solve(smg_dirac* D, smg_mg* mg, const double* b, double* x, int restart, int maxit, double tol,
                      smg_solve_info* info);
+/* fgmres_solve_generic / fgmres_solve_mg with n_ritz_harvest = n_ritz >= 0,
+ * src/solvers.cpp:555-800 (harvest and intra-solve deflation :601-611, :693-751; the
+ * FGMRESResult::ritz_pairs of src/solvers.h:18-23): *n_pairs (<= 2 n_ritz) Ritz pairs,
+ * values in ritz_vals[2 n_ritz], vectors one after the other in ritz_vecs (2 n_ritz
+ * vectors of ndof complex).  mg may be null (identity preconditioner). */
+int smg_fgmres_solve_ritz(smg_dirac* D, smg_mg* mg, const double* b, double* x, int restart, int maxit, double tol,
+                          int n_ritz, smg_solve_info* info, int* n_pairs, double* ritz_vals, double* ritz_vecs);
 
 /* ------------------------------------------------------------------ eigensolvers and deflation */
 /* random_vec (src/linalg.cpp:44-49) from std::mt19937(seed): n complex numbers of

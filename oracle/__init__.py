@@ -140,6 +140,8 @@ def lib():
         L.orc_fcg.argtypes = [_P, _P, _P, _I, _D, _I, _I, _P, _P]
         L.orc_fgmres.restype = _I
         L.orc_fgmres.argtypes = [_P, _P, _P, _I, _I, _D, _P, _P]
+        L.orc_fgmres_ritz.restype = _I
+        L.orc_fgmres_ritz.argtypes = [_P, _P, _P, _I, _I, _D, _I, _P, _P, _P, _P, _P]
         L.orc_coarse_cg.restype = _I
         L.orc_coarse_cg.argtypes = [_P, _P, _I, _D, _P]
         L.orc_fermion_force.argtypes = [_P, _P, _P]
@@ -477,6 +479,18 @@ def fgmres(op, rhs, restart, maxit, tol, mg=None):
     rhs = _c(rhs); x = np.zeros(op.n, np.complex128); res = np.zeros(1)
     it = lib().orc_fgmres(op.h, _ptr(rhs), mg.h if mg else None, restart, maxit, tol, _ptr(x), _ptr(res))
     return x, it, res[0]
+
+
+def fgmres_ritz(op, rhs, restart, maxit, tol, n_ritz, mg=None):
+    """fgmres_solve_generic (src/solvers.cpp:555-772) with n_ritz_harvest = n_ritz:
+    (x, iterations, residual, ritz values, ritz vectors [k][n])."""
+    rhs = _c(rhs); x = np.zeros(op.n, np.complex128); res = np.zeros(1); nr = np.zeros(1, np.int32)
+    cap = 2 * max(n_ritz, 1) + restart
+    vals = np.zeros(cap); vecs = np.zeros((cap, op.n), np.complex128)
+    it = lib().orc_fgmres_ritz(op.h, _ptr(rhs), mg.h if mg else None, restart, maxit, tol, n_ritz,
+                               _ptr(x), _ptr(res), _ptr(nr), _ptr(vals), _ptr(vecs))
+    k = int(nr[0])
+    return x, it, res[0], vals[:k].copy(), vecs[:k].copy()
 
 
 def coarse_cg(op, b, maxit, tol):

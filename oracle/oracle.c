@@ -1135,9 +1135,16 @@ int orc_fcg(const orc_op* A, const ocx* rhs, orc_mg* mg, int maxit, double tol, 
     return iter;
 }
 
-/* src/solvers.cpp:555-772 with n_ritz_harvest = 0 (no Ritz harvest). */
-int orc_fgmres(const orc_op* A, const ocx* rhs, orc_mg* mg, int restart, int maxit, double tol,
-               ocx* x, double* final_res) {
+/* fgmres_solve_generic, src/solvers.cpp:555-772: right-preconditioned FGMRES(restart)
+ * with two Gram-Schmidt passes and complex Givens rotations; with n_ritz > 0, after each
+ * restart cycle of at least n_ritz steps, the n_ritz lowest eigenpairs of G = H~^dag H~
+ * (the unreduced Hessenberg) give Ritz pairs sum_j evecs[j][p] Z_j (normalised), which
+ * deflate the later preconditioned vectors (Z_k += sum (v^dag V_k / l) v); beyond
+ * 2 n_ritz pairs the n_ritz smallest are kept.  The reference sorts with std::sort; an
+ * insertion sort gives the same order for distinct values.  Returns the iterations;
+ * *n_ritz_out pairs in ritz_vals / ritz_vecs [.][n] (capacity 2 n_ritz + restart). */
+int orc_fgmres_ritz(const orc_op* A, const ocx* rhs, orc_mg* mg, int restart, int maxit, double tol,
+                    int n_ritz, ocx* x, double* final_res, int* n_ritz_out, double* ritz_vals, ocx* ritz_vecs) {
     int n = A->n;
     double rhs_norm = orc_norm(n, rhs);
     memset(x, 0, n * sizeof(ocx));
@@ -1155,6 +1162,11 @@ int orc_fgmres(const orc_op* A, const ocx* rhs, orc_mg* mg, int restart, int max
     ocx* cs = xcalloc(m0, sizeof(ocx));
     ocx* sn = xcalloc(m0, sizeof(ocx));
     ocx* y = xcalloc(m0, sizeof(ocx));
+    ocx* Hr = xcalloc((size_t)(m0 + 1) * m0, sizeof(ocx));   /* unreduced Hessenberg */
+    int cap = 2 * (n_ritz > 0 ? n_ritz : 1) + m0, nr = 0;
+    double* rv = xcalloc(cap, sizeof(double));               /* harvested Ritz pairs */
+    ocx** rvec = xcalloc(cap, sizeof(ocx*));
+    if (n_ritz_out) *n_ritz_out = 0;
     while (total < maxit) {
         A->fn(A, x, Ax);
         for (int i = 0; i < n; i++) V[0][i] = rhs[i] - Ax[i];
@@ -1163,11 +1175,14 @@ int orc_fgmres(const orc_op* A, const ocx* rhs, orc_mg* mg, int restart, int max
         int m = restart < maxit - total ? restart : maxit - total;
         vscale(n, V[0], 1.0 / beta);
         memset(H, 0, (size_t)(m0 + 1) * m0 * sizeof(ocx));
+        memset(Hr, 0, (size_t)(m0 + 1) * m0 * sizeof(ocx));
         memset(g, 0, (m0 + 1) * sizeof(ocx));
         g[0] = beta;
         int k;
         for (k = 0; k < m; k++) {
             precond(mg, n, V[k], Z[k]);
+            for (int q = 0; q < nr; q++)   /* intra-solve deflation by earlier Ritz pairs */
+                if (rv[q] > 1e-14) axpy(n, orc_dot(n, rvec[q], V[k]) / rv[q], rvec[q], Z[k]);
             A->fn(A, Z[k], w);
             for (int j = 0; j <= k; j++) {
                 H[j * m0 + k] = orc_dot(n, V[j], w);
@@ -1179,6 +1194,7 @@ int orc_fgmres(const orc_op* A, const ocx* rhs, orc_mg* mg, int restart, int max
                 axpy(n, -c, V[j], w);
             }
             H[(k + 1) * m0 + k] = orc_norm(n, w);
+            for (int j = 0; j <= k + 1; j++) Hr[j * m0 + k] = H[j * m0 + k];
             if (cabs(H[(k + 1) * m0 + k]) < 1e-14) { k++; break; }
             memcpy(V[k + 1], w, n * sizeof(ocx));
             vscale(n, V[k + 1], 1.0 / H[(k + 1) * m0 + k]);
@@ -1205,16 +1221,64 @@ int orc_fgmres(const orc_op* A, const ocx* rhs, orc_mg* mg, int restart, int max
             y[i] = s / H[i * m0 + i];
         }
         for (int j = 0; j < kk; j++) axpy(n, y[j], Z[j], x);
+        if (kk >= n_ritz && n_ritz > 0) {   /* Ritz pairs of G = H~^dag H~ (:707-751) */
+            ocx* G = xcalloc((size_t)kk * kk, sizeof(ocx));   /* G[j*kk + i] */
+            for (int j = 0; j < kk; j++)
+                for (int i = 0; i < kk; i++)
+                    for (int p = 0; p <= kk; p++) G[(size_t)j * kk + i] += conj(Hr[p * m0 + i]) * Hr[p * m0 + j];
+            for (int i = 0; i < kk; i++)
+                for (int j = i + 1; j < kk; j++) {
+                    ocx av = 0.5 * (G[(size_t)j * kk + i] + conj(G[(size_t)i * kk + j]));
+                    G[(size_t)j * kk + i] = av;
+                    G[(size_t)i * kk + j] = conj(av);
+                }
+            double* ev = xcalloc(kk, sizeof(double));
+            ocx* Ev = xcalloc((size_t)kk * kk, sizeof(ocx));   /* Ev[row*kk + col] */
+            orc_lanczos_eigen(kk, G, ev, Ev);
+            int nh = n_ritz < kk ? n_ritz : kk;
+            for (int p = 0; p < nh; p++) {
+                ocx* v = xcalloc(n, sizeof(ocx));
+                for (int j = 0; j < kk; j++) axpy(n, Ev[(size_t)j * kk + p], Z[j], v);
+                double nv = orc_norm(n, v);
+                if (nv > 1e-14) vscale(n, v, 1.0 / nv);
+                rv[nr] = ev[p];
+                rvec[nr] = v;
+                nr++;
+            }
+            if (nr > 2 * n_ritz) {   /* keep the n_ritz smallest (insertion sort by value) */
+                for (int a = 1; a < nr; a++)
+                    for (int b2 = a; b2 > 0 && rv[b2 - 1] > rv[b2]; b2--) {
+                        double tv = rv[b2]; rv[b2] = rv[b2 - 1]; rv[b2 - 1] = tv;
+                        ocx* tp = rvec[b2]; rvec[b2] = rvec[b2 - 1]; rvec[b2 - 1] = tp;
+                    }
+                for (int q = n_ritz; q < nr; q++) free(rvec[q]);
+                nr = n_ritz;
+            }
+            free(G); free(ev); free(Ev);
+        }
         if (cabs(g[kk]) / rhs_norm < tol) break;
     }
     A->fn(A, x, Ax);
     double res = 0;
     for (int i = 0; i < n; i++) { ocx d = rhs[i] - Ax[i]; res += creal(d) * creal(d) + cimag(d) * cimag(d); }
     *final_res = sqrt(res) / rhs_norm;
+    if (n_ritz_out) *n_ritz_out = nr;
+    for (int q = 0; q < nr; q++) {
+        if (ritz_vals) ritz_vals[q] = rv[q];
+        if (ritz_vecs) memcpy(ritz_vecs + (size_t)q * n, rvec[q], n * sizeof(ocx));
+        free(rvec[q]);
+    }
+    free(rv); free(rvec); free(Hr);
     for (int i = 0; i <= m0; i++) free(V[i]);
     for (int i = 0; i < m0; i++) free(Z[i]);
     free(V); free(Z); free(H); free(g); free(cs); free(sn); free(y); free(Ax); free(w);
     return total;
+}
+
+/* src/solvers.cpp:555-772 with n_ritz_harvest = 0 (no Ritz harvest). */
+int orc_fgmres(const orc_op* A, const ocx* rhs, orc_mg* mg, int restart, int maxit, double tol,
+               ocx* x, double* final_res) {
+    return orc_fgmres_ritz(A, rhs, mg, restart, maxit, tol, 0, x, final_res, NULL, NULL, NULL);
 }
 
 /* ================================================================ forces */

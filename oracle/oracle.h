@@ -137,6 +137,8 @@ int orc_fcg(const orc_op* A, const ocx* rhs, orc_mg* mg, int maxit, double tol, 
 int orc_fgmres(const orc_op* A, const ocx* rhs, orc_mg* mg, int restart, int maxit, double tol,
                ocx* x, double* final_res);
 int orc_coarse_cg(const orc_op* A, const ocx* b, int maxit, double tol, ocx* x);
+int orc_fgmres_ritz(const orc_op* A, const ocx* rhs, orc_mg* mg, int restart, int maxit, double tol,
+                    int n_ritz, ocx* x, double* final_res, int* n_ritz_out, double* ritz_vals, ocx* ritz_vecs);
 
 /* ---------------- forces ---------------- */
 void orc_fermion_force(const orc_dirac* D, const ocx* X, double* F);

@@ -52,6 +52,8 @@ def lib():
         L.ref_lambda_max.argtypes = [_I, _P, _D, _D, _D, _D, _I]
         L.ref_krylov.restype = _I
         L.ref_krylov.argtypes = [_I, _P, _D, _D, _D, _D, _I, _P, _I, _D, _I, _P, _P]
+        L.ref_fgmres_ritz.restype = _I
+        L.ref_fgmres_ritz.argtypes = [_I, _P, _D, _D, _D, _D, _P, _I, _D, _I, _I, _P, _P, _P, _P, _P]
         L.ref_mg.restype = _I
         L.ref_mg.argtypes = [_I, _P, _D, _D, _D, _D, _I, _I, _I, _I, _I, _P, _I, _I, _I, _I,
                              _P, _P, _I, _D, _I, _P, _P]
@@ -162,6 +164,16 @@ class Dirac:
         it = lib().ref_krylov(self.L, _ptr(self.U), *self.args, solver, _ptr(rhs), maxit, tol, trunc,
                               _ptr(x), _ptr(res))
         return x, it, res[0]
+
+    def fgmres_ritz(self, rhs, maxit, tol, restart, n_ritz):
+        rhs = _c(rhs); n = 2 * self.V
+        x = np.zeros(n, np.complex128); res = np.zeros(1); nr = np.zeros(1, np.int32)
+        cap = 2 * n_ritz + restart
+        vals = np.zeros(cap); vecs = np.zeros((cap, n), np.complex128)
+        it = lib().ref_fgmres_ritz(self.L, _ptr(self.U), *self.args, _ptr(rhs), maxit, tol, restart, n_ritz,
+                                   _ptr(x), _ptr(res), _ptr(nr), _ptr(vals), _ptr(vecs))
+        k = int(nr[0])
+        return x, it, res[0], vals[:k].copy(), vecs[:k].copy()
 
     def mg(self, n_levels, block, k, coarse_block, null_iters, starts, pre, post, w_cycle, mode, b,
            maxit=100, tol=1e-10, trunc=20, sym_starts=None):

@@ -221,6 +221,26 @@ API int ref_krylov(int L, const double* U, double m, double r, double csw, doubl
     from_vec(o.solution, x); *res = o.final_residual; return o.iterations;
 }
 
+// fgmres_solve_generic on D^dag D, identity preconditioner, with n_ritz_harvest =
+// n_ritz: x, the residual, and the harvested pairs (*nr_out of them; vals, vecs
+// [.][2V] with room for 2 n_ritz + restart)
+API int ref_fgmres_ritz(int L, const double* U, double m, double r, double csw, double mu,
+                        const double* rhs, int maxit, double tol, int restart, int n_ritz,
+                        double* x, double* res, int* nr_out, double* vals, double* vecs) {
+    Ctx c(L, U, m, r, csw, mu);
+    OpApply A = c.D.as_DdagD_op();
+    auto ident = [](const Vec& v) { return v; };
+    auto o = fgmres_solve_generic(A, c.lat.ndof, to_vec(rhs, c.lat.ndof), ident, restart, maxit, tol, n_ritz);
+    from_vec(o.solution, x);
+    *res = o.final_residual;
+    *nr_out = (int)o.ritz_pairs.size();
+    for (size_t q = 0; q < o.ritz_pairs.size(); q++) {
+        vals[q] = o.ritz_pairs[q].value;
+        from_vec(o.ritz_pairs[q].vector, vecs + 2 * q * c.lat.ndof);
+    }
+    return o.iterations;
+}
+
 // Multigrid built from the reference's building blocks exactly as
 // build_mg_hierarchy (src/multigrid.cpp:595-687) assembles them, but with every
 // level's null-space start vectors supplied (the reference's warm_start).

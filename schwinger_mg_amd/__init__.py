@@ -13,7 +13,7 @@ if libsmg.so is missing or no GPU is present the first call raises.
 """
 import ctypes
 import os
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import numpy as np
 
@@ -93,6 +93,7 @@ SIGNATURES = [
     ("smg_gcr_solve", _I, [_P, _P, _I, _P, _P, _I, _D, _I, _I, _D, _P]),
     ("smg_gcr_solve_dev", _I, [_P, _P, _I, _P, _P, _I, _D, _I, _D, _P, _P]),
     ("smg_fgmres_solve", _I, [_P, _P, _P, _P, _I, _I, _D, _P]),
+    ("smg_fgmres_solve_ritz", _I, [_P, _P, _P, _P, _I, _I, _D, _I, _P, _P, _P, _P]),
     ("smg_dirac_apply_batched_dev", _I, [_P, _I, _I, _P, _P, _I, _P]),
     ("smg_coarse_apply_batched", _I, [_P, _I, _I, _P, _P]),
     ("smg_coarse_dense", _I, [_P, _P]), ("smg_coarse_dense_solve", _I, [_P, _P, _P]),
@@ -691,13 +692,35 @@ def gcr_solve_batched(D, B, mg, max_iter, tol, trunc=10, prec=F64, inner_tol=1e-
     return X, info, tr
 
 
-def fgmres_solve(D, rhs, mg, restart, max_iter, tol):
-    """fgmres_solve_generic / fgmres_solve_mg with no Ritz harvest (src/solvers.cpp:555-800)."""
+@dataclass
+class RitzPair:
+    """RitzPair (src/solvers.h:13-16)."""
+    value: float
+    vector: np.ndarray
+
+
+@dataclass
+class FGMRESResult(SolveResult):
+    """FGMRESResult (src/solvers.h:18-23): the solve and the harvested Ritz pairs."""
+    ritz_pairs: list = field(default_factory=list)
+
+
+def fgmres_solve(D, rhs, mg, restart, max_iter, tol, n_ritz_harvest=4):
+    """fgmres_solve_generic / fgmres_solve_mg (src/solvers.cpp:555-800), with the
+    reference's default n_ritz_harvest = 4: after every restart cycle of at least
+    n_ritz_harvest steps the lowest Ritz pairs of the cycle's Hessenberg are kept
+    (at most 2 n_ritz_harvest) and deflate the later preconditioned vectors.
+    mg = None: the identity preconditioner."""
     x = np.zeros(D.ndof, np.complex128)
     info = SolveInfo()
-    _chk(lib().smg_fgmres_solve(D.h, mg.h if mg else None, _p(_c(rhs)), _p(x), restart, max_iter, tol,
-                                ctypes.byref(info)))
-    return _res(x, info)
+    k = max(int(n_ritz_harvest), 0)
+    vals = np.zeros(max(2 * k, 1))
+    vecs = np.zeros((max(2 * k, 1), D.ndof), np.complex128)
+    npairs = ctypes.c_int(0)
+    _chk(lib().smg_fgmres_solve_ritz(D.h, mg.h if mg else None, _p(_c(rhs)), _p(x), restart, max_iter, tol, k,
+                                     ctypes.byref(info), ctypes.byref(npairs), _p(vals), _p(vecs)))
+    pairs = [RitzPair(float(vals[q]), vecs[q].copy()) for q in range(npairs.value)]
+    return FGMRESResult(x, info.iterations, info.final_residual, info.true_residual, info.restarts, pairs)
 
 
 def cg_solve_deflated(D, rhs, mg, defl_vecs, defl_vals, max_iter, tol, op=OP_DDAGD):

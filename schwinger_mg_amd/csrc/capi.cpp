@@ -758,6 +758,26 @@ int smg_fgmres_solve(smg_dirac* D, smg_mg* mg, const double* b, double* x, int r
     });
 }
 
+int smg_fgmres_solve_ritz(smg_dirac* D, smg_mg* mg, const double* b, double* x, int restart, int maxit, double tol,
+                          int n_ritz, smg_solve_info* info, int* n_pairs, double* ritz_vals, double* ritz_vecs) {
+    return guard([&] {
+        SMG_REQUIRE(D && b && x && restart >= 1 && n_ritz >= 0, "bad arguments");
+        SMG_REQUIRE(n_ritz == 0 || (n_pairs && ritz_vals && ritz_vecs), "ritz output buffers required");
+        const size_t n = D->op64->n();
+        Field<double> bb = up(b, n), xx(n);
+        RitzHarvest rh(n_ritz, restart, n);
+        SolveResult r = fgmres_solve(*D->op64, bb.p(), xx.p(), mg ? mg->mg.get() : nullptr, restart, maxit, tol, S(),
+                                     n_ritz > 0 ? &rh : nullptr);
+        fill_info(info, r);
+        downc(x, xx.p(), n);
+        if (n_pairs) *n_pairs = n_ritz > 0 ? rh.count() : 0;
+        if (n_ritz > 0 && rh.count() > 0) {
+            std::copy(rh.vals.begin(), rh.vals.end(), ritz_vals);
+            downc(ritz_vecs, rh.vecs.p(), (size_t)rh.count() * n);
+        }
+    });
+}
+
 // ------------------------------------------------------------------ many right-hand sides
 static void check_nrhs(int N, int maxN) {
     SMG_REQUIRE(N >= 1 && N <= maxN && (N & (N - 1)) == 0, "nrhs must be a power of two <= " + std::to_string(maxN));

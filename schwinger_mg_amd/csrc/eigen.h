@@ -53,6 +53,25 @@ void lowmode_solution(const Aggregation& P, const Eigenpairs& E, const c64* v, c
 // (equal in exact arithmetic to the reference's modified Gram-Schmidt, with AX
 // updated as the reference updates it).  mg: the FP64 cycle as preconditioner, or
 // null for the identity.  k <= 32.
+// The Ritz pairs harvested inside fgmres_solve_generic (src/solvers.cpp:601-611,
+// 693-751): after a restart cycle of kk >= n_ritz steps, the n_ritz lowest
+// eigenpairs (dense_eigh) of G = H~^dag H~, H~ the unreduced (kk+1) x kk Hessenberg,
+// give the normalised vectors sum_j evecs[j][p] Z_j; beyond 2 n_ritz stored pairs the
+// n_ritz of smallest value are kept.  Later preconditioned vectors are deflated by
+// every stored pair of value > 1e-14: z += (<v_q, v> / l_q) v_q.
+struct RitzHarvest {
+    int n_ritz;
+    size_t n;
+    std::vector<double> vals;
+    Field<double> vecs;   // room for 3 n_ritz vectors of n
+    DevMem coef;
+    RitzHarvest(int n_ritz, int restart, size_t n);
+    int count() const { return (int)vals.size(); }
+    void deflate(const c64* v, c64* z, hipStream_t st);
+    // Hraw row-major with ld columns (rows 0..kk); Z: kk vectors, stride n
+    void harvest(const std::vector<c64>& Hraw, int ld, int kk, const c64* Z, hipStream_t st);
+};
+
 struct LobpcgResult {
     std::vector<double> vals;
     Field<double> vecs;   // k vectors of A.n(), one after the other

@@ -749,4 +749,76 @@ LobpcgResult lobpcg(LinOpB<double>& A, MG* mg, int k, const c64* X0, int max_ite
     return out;
 }
 
+// ----------------------------------------------------------------- FGMRES Ritz harvest
+RitzHarvest::RitzHarvest(int n_ritz_, int restart, size_t n_)
+    : n_ritz(n_ritz_), n(n_), vecs((size_t)3 * std::max(n_ritz_, 1) * n_),
+      coef(64 * (size_t)((std::max(3 * n_ritz_, restart) + 31) / 32) * sizeof(double)) {}
+
+void RitzHarvest::deflate(const c64* v, c64* z, hipStream_t st) {
+    const int m = count();
+    if (m == 0) return;
+    std::vector<cxd> c = dots_h(vecs.p(), m, v, n, st);
+    std::vector<double> h(2 * (size_t)m);
+    for (int q = 0; q < m; q++) {
+        const cxd a = vals[q] > 1e-14 ? c[q] / vals[q] : cxd(0.0);
+        h[2 * q] = a.real();
+        h[2 * q + 1] = a.imag();
+    }
+    SMG_HIP(hipMemcpyAsync(coef.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    for (int c0 = 0; c0 < m; c0 += 32) {
+        const int nv = std::min(32, m - c0);
+        VecPtrs<double> P{};
+        for (int i = 0; i < nv; i++) P.v[i] = vecs.p((size_t)(c0 + i) * n);
+        launch_lincomb<double>(z, P, nv, coef.as<double>() + 2 * c0, 1, n, st);
+    }
+    SMG_HIP(hipStreamSynchronize(st));
+}
+
+void RitzHarvest::harvest(const std::vector<c64>& Hraw, int ld, int kk, const c64* Z, hipStream_t st) {
+    if (n_ritz <= 0 || kk < n_ritz) return;
+    auto hr = [&](int p, int i) { return to_std(Hraw[(size_t)p * ld + i]); };
+    std::vector<cxd> G((size_t)kk * kk, 0.0);   // G[j*kk + i] = sum_p conj(H~_pi) H~_pj
+    for (int j = 0; j < kk; j++)
+        for (int i = 0; i < kk; i++) {
+            cxd s = 0.0;
+            for (int p = 0; p <= kk; p++) s += std::conj(hr(p, i)) * hr(p, j);
+            G[(size_t)j * kk + i] = s;
+        }
+    for (int i = 0; i < kk; i++)
+        for (int j = i + 1; j < kk; j++) {
+            const cxd av = 0.5 * (G[(size_t)j * kk + i] + std::conj(G[(size_t)i * kk + j]));
+            G[(size_t)j * kk + i] = av;
+            G[(size_t)i * kk + j] = std::conj(av);
+        }
+    std::vector<c64> Gc((size_t)kk * kk), E((size_t)kk * kk);
+    for (size_t i = 0; i < Gc.size(); i++) Gc[i] = c64{G[i].real(), G[i].imag()};
+    std::vector<double> ev(kk);
+    dense_eigh(kk, Gc.data(), ev.data(), E.data());
+    const int nh = std::min(n_ritz, kk);
+    std::vector<cxd> c(kk);
+    for (int p = 0; p < nh; p++) {
+        const int q = count();
+        c64* out = vecs.p((size_t)q * n);
+        for (int j = 0; j < kk; j++) c[j] = to_std(E[(size_t)j * kk + p]);
+        combine(out, Z, kk, c, n, coef, st);
+        const double nv = dev_norm(out, n, st);
+        if (nv > 1e-14) launch_axpby<double>(out, out, c64{0, 0}, c64{1.0 / nv, 0}, n, st);
+        vals.push_back(ev[p]);
+    }
+    if (count() > 2 * n_ritz) {   // keep the n_ritz smallest, in ascending order
+        std::vector<int> idx(count());
+        std::iota(idx.begin(), idx.end(), 0);
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return vals[a] < vals[b]; });
+        Field<double> kept((size_t)3 * n_ritz * n);
+        std::vector<double> kv(n_ritz);
+        for (int q = 0; q < n_ritz; q++) {
+            launch_copy<double>(kept.p((size_t)q * n), vecs.p((size_t)idx[q] * n), n, st);
+            kv[q] = vals[idx[q]];
+        }
+        vecs = std::move(kept);
+        vals = kv;
+    }
+    SMG_HIP(hipStreamSynchronize(st));
+}
+
 }  // namespace smg

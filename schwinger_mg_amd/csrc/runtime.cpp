@@ -820,17 +820,18 @@ SolveResult gcr_solve_mixed(LinOp<double>& A64, LinOp<float>& A32, const c64* b,
     return res;
 }
 
-// src/solvers.cpp:555-772 with n_ritz_harvest = 0
+// src/solvers.cpp:555-772; the Ritz harvest and intra-solve deflation when ritz
 SolveResult fgmres_solve(LinOp<double>& A, const c64* b, c64* x, MG* mg, int restart, int maxit, double tol,
-                         hipStream_t st) {
+                         hipStream_t st, RitzHarvest* ritz) {
     SolveResult res;
     const size_t n = A.n();
     launch_zero<double>(x, n, st);
     const double bnorm = dev_norm(b, n, st);
     if (bnorm < 1e-30) return res;
-    std::vector<Field<double>> Vv, Zv;
+    std::vector<Field<double>> Vv;
     for (int i = 0; i <= restart; i++) Vv.emplace_back(n);
-    for (int i = 0; i < restart; i++) Zv.emplace_back(n);
+    Field<double> Z((size_t)restart * n);   // preconditioned vectors, contiguous for the harvest
+    auto zk = [&](int k) { return Z.p((size_t)k * n); };
     Field<double> w(n);
     int total = 0;
     auto cdivh = [](c64 a, c64 b) { return cdiv(a, b); };
@@ -842,12 +843,14 @@ SolveResult fgmres_solve(LinOp<double>& A, const c64* b, c64* x, MG* mg, int res
         launch_axpby<double>(Vv[0].p(), Vv[0].p(), c64{0, 0}, c64{1.0 / beta, 0}, n, st);
         std::vector<c64> H((size_t)(restart + 1) * restart, c64{0, 0}), g(restart + 1, c64{0, 0}),
             cs(restart, c64{0, 0}), sn(restart, c64{0, 0}), y(restart, c64{0, 0});
+        std::vector<c64> Hraw(ritz ? H.size() : 0, c64{0, 0});   // the Hessenberg before the rotations
         auto h = [&](int i, int j) -> c64& { return H[(size_t)i * restart + j]; };
         g[0] = c64{beta, 0};
         int k;
         for (k = 0; k < m; k++) {
-            precond_apply<double>(mg, Vv[k].p(), Zv[k].p(), n, st);
-            A.apply(w.p(), Zv[k].p(), st);
+            precond_apply<double>(mg, Vv[k].p(), zk(k), n, st);
+            if (ritz) ritz->deflate(Vv[k].p(), zk(k), st);
+            A.apply(w.p(), zk(k), st);
             for (int pass = 0; pass < 2; pass++)
                 for (int j = 0; j <= k; j++) {
                     c64 c = dev_dot<double>(Vv[j].p(), w.p(), n, st);
@@ -856,6 +859,8 @@ SolveResult fgmres_solve(LinOp<double>& A, const c64* b, c64* x, MG* mg, int res
                 }
             const double hn = dev_norm(w.p(), n, st);
             h(k + 1, k) = c64{hn, 0};
+            if (ritz)
+                for (int j = 0; j <= k + 1; j++) Hraw[(size_t)j * restart + k] = h(j, k);
             if (hn < 1e-14) {
                 k++;
                 break;
@@ -886,7 +891,8 @@ SolveResult fgmres_solve(LinOp<double>& A, const c64* b, c64* x, MG* mg, int res
             for (int j = i + 1; j < kk; j++) s = s - h(i, j) * y[j];
             y[i] = cdivh(s, h(i, i));
         }
-        for (int j = 0; j < kk; j++) launch_axpby<double>(x, Zv[j].p(), y[j], c64{1, 0}, n, st);
+        for (int j = 0; j < kk; j++) launch_axpby<double>(x, zk(j), y[j], c64{1, 0}, n, st);
+        if (ritz) ritz->harvest(Hraw, restart, kk, Z.p(), st);
         if (std::sqrt(norm2(g[kk])) / bnorm < tol) break;
     }
     res.iterations = total;

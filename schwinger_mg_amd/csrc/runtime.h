@@ -208,6 +208,7 @@ struct MGConfig {
 template <typename T>
 struct Cycle;
 struct Eigenpairs;
+struct RitzHarvest;
 
 struct MG {
     MGConfig cfg;
@@ -277,9 +278,10 @@ SolveResult gcr_solve(LinOp<T>& A, const cplx<T>* b, cplx<T>* x, MG* mg, int max
 // GCR(trunc) preconditioned by the FP32 MG cycle.
 SolveResult gcr_solve_mixed(LinOp<double>& A64, LinOp<float>& A32, const c64* b, c64* x, MG* mg, int maxit,
                             double tol, int trunc, double inner_tol, hipStream_t st);
-// fgmres_solve_generic without Ritz harvest (src/solvers.cpp:555-772)
+// fgmres_solve_generic (src/solvers.cpp:555-772): no Ritz harvest when ritz is null,
+// else the harvest and intra-solve deflation that ritz carries
 SolveResult fgmres_solve(LinOp<double>& A, const c64* b, c64* x, MG* mg, int restart, int maxit, double tol,
-                         hipStream_t st);
+                         hipStream_t st, RitzHarvest* ritz = nullptr);
 
 double true_residual(LinOp<double>& A, const c64* b, const c64* x, hipStream_t st);
 double dev_norm(const c64* v, size_t n, hipStream_t st);

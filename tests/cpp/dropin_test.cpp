@@ -60,6 +60,8 @@ int orc_trlm(const orc_op* A, int n_ev, int n_kr, int max_restarts, double tol, 
 int orc_cg_deflated(const orc_op* A, const cx* rhs, orc_mg* mg, int ndefl, const cx* U, int maxit, double tol, cx* x,
                     double* final_res);
 int orc_lobpcg(const orc_op* A, orc_mg* mg, int k, const cx* X0, int max_iter, double tol, cx* Xout, double* evals);
+int orc_fgmres_ritz(const orc_op* A, const cx* rhs, orc_mg* mg, int restart, int maxit, double tol, int n_ritz,
+                    cx* x, double* final_res, int* n_ritz_out, double* ritz_vals, cx* ritz_vecs);
 void orc_heatbath(int L, cx* U, double beta, int n_sweeps, uint64_t seed, uint32_t first_sweep);
 typedef struct orc_prolong orc_prolong;
 orc_prolong* orc_prolong_new(int nx, int ny, int nd, int bx, int by, int k, int chiral);
@@ -382,6 +384,29 @@ int main() {
         report("lobpcg_update Rayleigh quotients", de, 1e-9);
         report("lobpcg_update vectors", relerr(xg, xl), 1e-7);
         report("lobpcg_update iterations", std::abs(lr.iterations - itl), 0.0);
+        // fgmres_solve_generic with the Ritz harvest (identity preconditioner, two cycles)
+        {
+            const Vec bf = random_vec(n, rng);
+            FGMRESResult fr = fgmres_solve_generic(A, n, bf, identity_precond(), 10, 20, 1e-10, 3);
+            Vec xo(n), vo((size_t)(2 * 3 + 10) * n);
+            RVec lo(2 * 3 + 10);
+            double reso = 0;
+            int nro = 0;
+            const int ito = orc_fgmres_ritz(Ao, bf.data(), nullptr, 10, 20, 1e-10, 3, xo.data(), &reso, &nro,
+                                            lo.data(), vo.data());
+            report("fgmres_solve_generic (Ritz harvest) solution", relerr(fr.solution, xo), 1e-11);
+            report("fgmres_solve_generic (Ritz harvest) iterations", std::abs(fr.iterations - ito), 0.0);
+            report("fgmres_solve_generic ritz_pairs count", std::abs((int)fr.ritz_pairs.size() - nro), 0.0);
+            double dv = 0;
+            Vec vg;
+            for (size_t q = 0; q < fr.ritz_pairs.size() && (int)q < nro; q++) {
+                dv = std::max(dv, std::abs(fr.ritz_pairs[q].value - lo[q]) / std::abs(lo[nro - 1]));
+                vg.insert(vg.end(), fr.ritz_pairs[q].vector.begin(), fr.ritz_pairs[q].vector.end());
+            }
+            vo.resize((size_t)nro * n);
+            report("fgmres_solve_generic Ritz values", dv, 1e-10);
+            report("fgmres_solve_generic Ritz vectors", relerr(vg, vo), 1e-8);
+        }
         mg.update_coarsest_deflation(8, 3);
         report("MGHierarchy::update_coarsest_deflation",
                mg.coarsest_defl_vals.size() == 8 && mg.coarsest_defl_vecs[0].size() == (size_t)mg.dims.back() ? 0.0

@@ -151,6 +151,18 @@ def main():
     out["MS_U"], out["MS_nulls"], out["MS_pi"], out["MS_eta2"] = U, nulls, pi_d, eta_d[2 * L * L:]
     out["MS_out"], out["MS_Uf"] = res, Uf
     out["MS_vals0"], out["MS_vecs0"], out["MS_vals1"], out["MS_vecs1"] = v0, V0, v1, V1
+    # fgmres_solve_generic on D^dag D (identity preconditioner, restart 10) with the Ritz
+    # harvest of 2 and 4 pairs: after two restart cycles and at convergence
+    L = 8
+    for ci in (0, 2):
+        csw, mu = CONFIGS[ci]
+        D = R.Dirac(L, out["L8_U"], 0.1, 1.0, csw, mu)
+        for nr in (2, 4):
+            for maxit in (20, 500):
+                x, it, res, vals, vecs = D.fgmres_ritz(out["L8_psi"], maxit, 1e-10, 10, nr)
+                key = f"L8_c{ci}_fgr{nr}_{maxit}"
+                out[key + "_x"], out[key + "_it"], out[key + "_vals"], out[key + "_vecs"] = \
+                    x, np.array([it, res]), vals, vecs
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_vectors.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes,", len(out), "arrays")

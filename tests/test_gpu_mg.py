@@ -171,10 +171,14 @@ def test_mg_cycle_and_solvers(smg, levels, L, blocks):
     r = smg.fcg_solve(D, b, mg, 12, 1e-10, 20, ref_projection=True)
     xo, ito, _ = O.gcr(A, b, 12, 1e-10, 20, mgo, ref_projection=True)
     assert r.iterations == ito and relerr(r.solution, xo) < 1e-7
-    # FGMRES
+    # FGMRES, with the reference's default harvest of 4 Ritz pairs and without
     r = smg.fgmres_solve(D, b, mg, 20, 100, 1e-10)
-    xo, ito, _ = O.fgmres(A, b, 20, 100, 1e-10, mgo)
+    xo, ito, _, vals, _ = O.fgmres_ritz(A, b, 20, 100, 1e-10, 4, mgo)
     assert abs(r.iterations - ito) <= 1 and relerr(r.solution, xo) < 1e-8
+    assert len(r.ritz_pairs) == len(vals)
+    r = smg.fgmres_solve(D, b, mg, 20, 100, 1e-10, n_ritz_harvest=0)
+    xo, ito, _ = O.fgmres(A, b, 20, 100, 1e-10, mgo)
+    assert abs(r.iterations - ito) <= 1 and relerr(r.solution, xo) < 1e-8 and r.ritz_pairs == []
     # mixed precision: FP32 GCR + FP32 cycle inside FP64 refinement reaches 1e-10
     r = smg.fcg_solve(D, b, mg, 200, 1e-10, 10, prec=smg.F32, inner_tol=1e-5)
     assert r.true_residual < 1e-10
@@ -213,7 +217,7 @@ def test_golden_multigrid(smg):
                          coarse_maxit=5000, coarse_tol=1e-15)
     # the reference solves the coarsest level by dense LU; a CG to 1e-15 reproduces it
     assert relerr(mg.precondition(g["L16_rhs"]), g["L16_mg2_mode0_x"]) < 1e-8
-    r = smg.fgmres_solve(D, g["L16_rhs"], mg, 20, 100, 1e-10)
+    r = smg.fgmres_solve(D, g["L16_rhs"], mg, 20, 100, 1e-10, n_ritz_harvest=0)
     assert abs(r.iterations - g["L16_mg2_mode3_it"][0]) <= 1
     assert relerr(r.solution, g["L16_mg2_mode3_x"]) < 1e-8
 

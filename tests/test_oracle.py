@@ -546,3 +546,48 @@ def test_rr_evolve_and_lowmode_projection():
     xc = sum((np.vdot(V[i], P.restrict(phi)) / vals[i]) * V[i] for i in range(5))
     F = O.coarse_lowmode_force(Do, P, V, vals, phi, 0)
     assert np.abs(F - Do.force(P.prolong(xc))).max() < 1e-12 * np.abs(F).max()
+
+
+# ------------------------------------------------------------------ FGMRES Ritz harvest
+@pytest.mark.parametrize("ci", (0, 2))
+@pytest.mark.parametrize("nr", (2, 4))
+def test_golden_fgmres_ritz(g, ci, nr):
+    """fgmres_solve_generic with n_ritz_harvest (src/solvers.cpp:555-772) against the
+    reference's own output: after two restart cycles every pair agrees to rounding;
+    at convergence the solution and iteration count agree exactly, while the pairs of
+    the last cycles, whose Krylov space starts from a residual 1e-9 of |b| (formed
+    with that much cancellation), agree to 1e-4 relative."""
+    csw, mu = CONFIGS[ci]
+    A = O.Dirac(8, g["L8_U"], 0.1, 1.0, csw, mu).op()
+    psi = g["L8_psi"]
+    key = f"L8_c{ci}_fgr{nr}_20"
+    x, it, res, vals, vecs = O.fgmres_ritz(A, psi, 10, 20, 1e-10, nr)
+    assert it == g[key + "_it"][0] == 20 and close(x, g[key + "_x"], 1e-12)
+    assert len(vals) == len(g[key + "_vals"]) == 2 * nr
+    assert close(vals, g[key + "_vals"], 1e-11) and close(vecs, g[key + "_vecs"], 1e-9)
+    key = f"L8_c{ci}_fgr{nr}_500"
+    x, it, res, vals, vecs = O.fgmres_ritz(A, psi, 10, 500, 1e-10, nr)
+    assert it == g[key + "_it"][0] and close(x, g[key + "_x"], 1e-12)
+    assert abs(res - g[key + "_it"][1]) < 1e-12
+    assert len(vals) == len(g[key + "_vals"])
+    assert np.all(np.abs(vals - g[key + "_vals"]) <= 1e-4 * np.abs(g[key + "_vals"]))
+
+
+def test_fgmres_ritz_first_cycle_identity():
+    """After one cycle with the identity preconditioner Z = V is orthonormal and
+    A Z = V_{k+1} H~, so each harvested pair (u, l) satisfies |A u|^2 = l, |u| = 1, and
+    the l are the smallest squared singular values of A restricted to the Krylov
+    space -- ascending and at least sigma_min(A)^2."""
+    L = 8
+    U = random_gauge(L, 0.5, seed=3)
+    A = O.Dirac(L, U, 0.1, 1.0, 1.0, 0.05).op()
+    b = gaussian_vec(2 * L * L, seed=4)
+    _, it, _, vals, vecs = O.fgmres_ritz(A, b, 12, 12, 1e-14, 5)
+    assert it == 12 and len(vals) == 5 and np.all(np.diff(vals) >= 0)
+    Adense = np.stack([A(e) for e in np.eye(2 * L * L, dtype=np.complex128)], axis=1)
+    smin2 = np.linalg.svd(Adense, compute_uv=False)[-1] ** 2
+    for lv, u in zip(vals, vecs):
+        Au = Adense @ u
+        assert abs(np.linalg.norm(u) - 1) < 1e-12
+        assert abs(np.vdot(Au, Au).real - lv) < 1e-10 * max(lv, 1.0)
+        assert lv >= smin2 * (1 - 1e-10)
