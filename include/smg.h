@@ -455,6 +455,40 @@ int smg_hmc_ms_evolve(smg_dirac* D, smg_mg* mg, smg_prolongator* P, smg_coarse* 
 int smg_hmc_ms_trajectory(smg_dirac* D, smg_mg* mg, smg_prolongator* P, smg_coarse* defl, const smg_ms_params* p,
                           const double* pi0, const double* eta, double u, smg_ms_result* res);
 
+/* ------------------------------------------------------------------ nested integrator, gauge force on the fine scale
+ * No reference counterpart: the reference's multi-timescale split
+ * (src/hmc_multiscale.cpp:424-506) puts the gauge force on the outer scale; this one
+ * puts the fermion force alone on the outer scale (n_outer steps over tau, the leapfrog
+ * or Omelyan sequences of plain_leapfrog_evolve, src/hmc_trajectory.cpp:243-409) and
+ * makes each outer position update Q(h) n_gauge steps of h / n_gauge of the inner
+ * integrator, which kicks with the gauge force alone.  The solves of D^dag D X = phi
+ * (every outer kick and both Hamiltonians) are FP64 CG, preconditioned by mg's FP64
+ * cycle when mg is given (solver = SMG_F64), or mixed-precision GCR(10) with mg's FP32
+ * cycle inside FP64 refinement on the FP64 true residual (solver = SMG_F32).  The
+ * hierarchy is not rebuilt during the trajectory (smg_mg_rebuild between trajectories). */
+typedef struct {
+    double beta, tau;
+    int n_outer;        /* outer steps (fermion force) */
+    int n_gauge;        /* inner steps per outer position update (gauge force) */
+    int omelyan_outer;  /* 0 leapfrog, 1 Omelyan 2MN (lambda = 0.1932) */
+    int omelyan_inner;
+    int solver;         /* SMG_F64 or SMG_F32 (needs mg) */
+    int cg_maxiter;
+    double cg_tol;
+    double inner_tol;   /* inner tolerance of the mixed-precision solver */
+    int force_accept;
+} smg_nested_params;
+/* the molecular dynamics from momenta pi (2V reals, in/out) and phi (2V complex); the
+ * gauge field of D evolves in place; *solver_iters = the iterations of all solves */
+int smg_hmc_nested_evolve(smg_dirac* D, smg_mg* mg, const smg_nested_params* p, double* pi, const double* phi,
+                          int* solver_iters);
+/* the trajectory: momenta pi0, phi = D^dag eta, H = KE + S_g + S_f before and after,
+ * Metropolis with the caller's uniform u (accept if force_accept, dH < 0 or
+ * u < exp(-dH)); a rejected trajectory restores the gauge field.  total_cg_iters = the
+ * iterations of every solve, dLD = 0 */
+int smg_hmc_nested_trajectory(smg_dirac* D, smg_mg* mg, const smg_nested_params* p, const double* pi0,
+                              const double* eta, double u, smg_hmc_result* res);
+
 /* ------------------------------------------------------------------ many lattices in one launch
  * No reference counterpart (the reference runs one lattice at a time): G Wilson-clover
  * lattices of different sizes (L[g] >= 2; links U of lattice g follow those of g-1, each

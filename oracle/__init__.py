@@ -57,6 +57,8 @@ def lib():
         L.orc_kinetic_energy.argtypes = [_I, _P]
         L.orc_hmc_evolve.restype = _I
         L.orc_hmc_evolve.argtypes = [_I, _P, _D, _D, _D, _D, _D, _D, _I, _I, _I, _D, _P, _P]
+        L.orc_hmc_nested_evolve.restype = _I
+        L.orc_hmc_nested_evolve.argtypes = [_I, _P, _D, _D, _D, _D, _D, _D, _I, _I, _I, _I, _I, _D, _P, _P]
         L.orc_hmc_hamiltonian.restype = _I
         L.orc_hmc_hamiltonian.argtypes = [_I, _P, _D, _D, _D, _D, _D, _I, _D, _P, _P, _P]
         L.orc_lanczos_eigen.argtypes = [_I, _P, _P, _P]
@@ -503,6 +505,17 @@ def clover_deriv_insert(D, w, factor, F):
     w = _c(w, np.float64); F = _c(F, np.float64).copy()
     lib().orc_clover_deriv_insert(D.h, _ptr(w), factor, _ptr(F))
     return F
+
+
+def hmc_nested_evolve(L, U, mass, r, csw, mu_t, beta, tau, n_outer, n_gauge, omelyan_outer, omelyan_inner,
+                      cg_maxit, cg_tol, pi, phi):
+    """The nested integrator with the gauge force on the fine scale: (U, pi, CG iterations)."""
+    U = _c(U).copy()
+    pi = np.ascontiguousarray(pi, dtype=np.float64).copy()
+    phi = _c(phi)
+    it = lib().orc_hmc_nested_evolve(L, _ptr(U), mass, r, csw, mu_t, beta, tau, n_outer, n_gauge, int(omelyan_outer),
+                                     int(omelyan_inner), cg_maxit, cg_tol, _ptr(pi), _ptr(phi))
+    return U, pi, it
 
 
 def hmc_evolve(L, U, mass, r, csw, mu_t, beta, tau, n_steps, omelyan, cg_maxit, cg_tol, pi, phi):

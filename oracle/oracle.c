@@ -1581,6 +1581,78 @@ int orc_hmc_evolve(int L, ocx* U, double mass, double r, double csw, double mu_t
     return cg;
 }
 
+/* The nested (Sexton-Weingarten) integrator with the gauge force on the fine scale; no
+ * reference counterpart (the reference's multi-timescale split, src/hmc_multiscale.cpp:
+ * 424-506, puts the gauge force on the outer scale).  The outer integrator -- leapfrog
+ * P(h/2) [Q(h) P(h)]^(N-1) Q(h) P(h/2) or Omelyan 2MN, lambda = 0.1932, the step
+ * sequences of src/hmc_trajectory.cpp:243-409 -- kicks with the fermion force alone;
+ * each of its position updates Q(h) is n_gauge steps of the inner integrator (leapfrog
+ * or Omelyan) of step h / n_gauge kicking with the gauge force alone.  Parity is pinned
+ * by reductions: beta = 0 makes every inner integrator one drift, which is the plain
+ * trajectory of the fermion force; phi = 0 makes the fermion force vanish, which is
+ * the plain trajectory of the gauge force with the inner steps; and by reversibility. */
+static void nested_inner(int L, ocx* U, double beta, double h, int n_gauge, int omelyan, double* pi, double* F) {
+    int n = 2 * L * L;
+    double dt = h / n_gauge;
+    if (omelyan) {
+        const double lambda = 0.1932;
+        orc_gauge_force(L, U, beta, F);
+        md_update_mom(n, pi, F, lambda * dt);
+        for (int s = 0; s < n_gauge; s++) {
+            md_update_gauge(n, U, pi, 0.5 * dt);
+            orc_gauge_force(L, U, beta, F);
+            md_update_mom(n, pi, F, (1.0 - 2.0 * lambda) * dt);
+            md_update_gauge(n, U, pi, 0.5 * dt);
+            orc_gauge_force(L, U, beta, F);
+            md_update_mom(n, pi, F, (s < n_gauge - 1 ? 2.0 : 1.0) * lambda * dt);
+        }
+    } else {
+        orc_gauge_force(L, U, beta, F);
+        md_update_mom(n, pi, F, 0.5 * dt);
+        for (int s = 0; s < n_gauge - 1; s++) {
+            md_update_gauge(n, U, pi, dt);
+            orc_gauge_force(L, U, beta, F);
+            md_update_mom(n, pi, F, dt);
+        }
+        md_update_gauge(n, U, pi, dt);
+        orc_gauge_force(L, U, beta, F);
+        md_update_mom(n, pi, F, 0.5 * dt);
+    }
+}
+int orc_hmc_nested_evolve(int L, ocx* U, double mass, double r, double csw, double mu_t, double beta, double tau,
+                          int n_outer, int n_gauge, int omelyan_outer, int omelyan_inner, int cg_maxit,
+                          double cg_tol, double* pi, const ocx* phi) {
+    int n = 2 * L * L, cg = 0;
+    double h = tau / n_outer;
+    double* F = xcalloc(n, sizeof(double));
+    /* md_force at beta = 0 is the fermion force alone */
+#define FFORCE() (cg += md_force(L, U, mass, r, csw, mu_t, 0.0, cg_maxit, cg_tol, phi, F))
+    if (omelyan_outer) {
+        const double lambda = 0.1932;
+        FFORCE();
+        md_update_mom(n, pi, F, lambda * h);
+        for (int o = 0; o < n_outer; o++) {
+            nested_inner(L, U, beta, 0.5 * h, n_gauge, omelyan_inner, pi, F);
+            FFORCE();
+            md_update_mom(n, pi, F, (1.0 - 2.0 * lambda) * h);
+            nested_inner(L, U, beta, 0.5 * h, n_gauge, omelyan_inner, pi, F);
+            FFORCE();
+            md_update_mom(n, pi, F, (o < n_outer - 1 ? 2.0 : 1.0) * lambda * h);
+        }
+    } else {
+        FFORCE();
+        md_update_mom(n, pi, F, 0.5 * h);
+        for (int o = 0; o < n_outer; o++) {
+            nested_inner(L, U, beta, h, n_gauge, omelyan_inner, pi, F);
+            FFORCE();
+            md_update_mom(n, pi, F, o < n_outer - 1 ? h : 0.5 * h);
+        }
+    }
+#undef FFORCE
+    free(F);
+    return cg;
+}
+
 /* rr_evolve, src/eigen_evolve.cpp:48-108: Rayleigh-Ritz of the k vectors V [k][n]
  * (in/out) on A: M_ij = <v_i, A v_j> (i <= j, M_ji = conj), diagonalised by
  * orc_lanczos_eigen; v_i <- sum_j evecs[j][i] v_j, vals <- the ascending eigenvalues */

@@ -175,6 +175,9 @@ int orc_hmc_ms_evolve(int L, ocx* U, double mass, double r, double csw, double m
                       int n_outer, int n_inner, int omelyan, int inner_smooth, int defl_refresh, int cg_maxit,
                       double cg_tol, const orc_prolong* P, int k, ocx* V, double* vals, double* pi, const ocx* phi,
                       int* lowmode_evals);
+int orc_hmc_nested_evolve(int L, ocx* U, double mass, double r, double csw, double mu_t, double beta, double tau,
+                          int n_outer, int n_gauge, int omelyan_outer, int omelyan_inner, int cg_maxit,
+                          double cg_tol, double* pi, const ocx* phi);
 int orc_hmc_hamiltonian(int L, const ocx* U, double mass, double r, double csw, double mu_t, double beta,
                         int cg_maxit, double cg_tol, const double* pi, const ocx* phi, double* out);
 int orc_hmc_evolve_eo(int L, ocx* U, double mass, double r, double csw, double mu_t, double beta, double tau,

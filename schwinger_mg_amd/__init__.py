@@ -123,6 +123,8 @@ SIGNATURES = [
     ("smg_coarse_evolve_deflation", _I, [_P, _P]),
     ("smg_hmc_ms_evolve", _I, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     ("smg_hmc_ms_trajectory", _I, [_P, _P, _P, _P, _P, _P, _P, _D, _P]),
+    ("smg_hmc_nested_evolve", _I, [_P, _P, _P, _P, _P, _P]),
+    ("smg_hmc_nested_trajectory", _I, [_P, _P, _P, _P, _P, _D, _P]),
     ("smg_dirac_heatbath", _I, [_P, _D, _I, ctypes.c_ulonglong, ctypes.c_uint]),
     ("smg_lattices_create", _I, [_P, _I, _P, _P, _P, _D, _D, _D]), ("smg_lattices_destroy", _I, [_P]),
     ("smg_lattices_size", _I, [_P, _P]), ("smg_lattices_apply", _I, [_P, _I, _P, _P]),
@@ -868,6 +870,40 @@ def hmc_trajectory_mg_multiscale(D, P, defl, params, pi0, eta, u, mg=None):
     res = MSResult()
     _chk(lib().smg_hmc_ms_trajectory(D.h, mg.h if mg else None, P.h, defl.h, ctypes.byref(params),
                                      _p(np.ascontiguousarray(pi0, np.float64)), _p(_c(eta)), u, ctypes.byref(res)))
+    return res
+
+
+# ------------------------------------------------------------------ nested integrator, gauge force on the fine scale
+class NestedParams(ctypes.Structure):
+    """The nested integrator (no reference counterpart): the fermion force on the
+    outer scale, n_gauge inner gauge-force steps per outer position update."""
+    _fields_ = [("beta", _D), ("tau", _D), ("n_outer", _I), ("n_gauge", _I), ("omelyan_outer", _I),
+                ("omelyan_inner", _I), ("solver", _I), ("cg_maxiter", _I), ("cg_tol", _D), ("inner_tol", _D),
+                ("force_accept", _I)]
+
+
+def nested_params(beta, tau=1.0, n_outer=1, n_gauge=8, omelyan_outer=True, omelyan_inner=True, solver=F64,
+                  cg_maxiter=1000, cg_tol=1e-10, inner_tol=1e-5, force_accept=False):
+    return NestedParams(beta, tau, n_outer, n_gauge, int(omelyan_outer), int(omelyan_inner), solver, cg_maxiter,
+                        cg_tol, inner_tol, int(force_accept))
+
+
+def hmc_nested_evolve(D, params, pi, phi, mg=None):
+    """The nested molecular dynamics: D's gauge field evolves in place; (pi, solver iterations)."""
+    pi = np.ascontiguousarray(pi, np.float64).copy()
+    it = _I()
+    _chk(lib().smg_hmc_nested_evolve(D.h, mg.h if mg else None, ctypes.byref(params), _p(pi), _p(_c(phi)),
+                                     ctypes.byref(it)))
+    return pi, it.value
+
+
+def hmc_trajectory_nested(D, params, pi0, eta, u, mg=None):
+    """One trajectory of the nested integrator from the momenta pi0 and the Gaussian eta
+    of phi = D^dag eta, u the Metropolis uniform: an HMCResult (dLD = 0)."""
+    res = HMCResult()
+    _chk(lib().smg_hmc_nested_trajectory(D.h, mg.h if mg else None, ctypes.byref(params),
+                                         _p(np.ascontiguousarray(pi0, np.float64)), _p(_c(eta)), u,
+                                         ctypes.byref(res)))
     return res
 
 

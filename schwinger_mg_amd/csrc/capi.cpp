@@ -1396,6 +1396,148 @@ int smg_hmc_ms_trajectory(smg_dirac* D, smg_mg* mg, smg_prolongator* P, smg_coar
     });
 }
 
+// ------------------------------------------------------------------ nested integrator, gauge force on the fine scale
+namespace {
+// The fermion force on the outer scale, the gauge force on the fine scale: each outer
+// position update Q(h) is n_gauge inner steps of h / n_gauge kicking with the gauge
+// force alone; the outer kicks use the fermion force alone.  The solves of D^dag D X =
+// phi are FP64 CG (preconditioned by the FP64 cycle when a hierarchy is given) or the
+// mixed-precision GCR(10) inside FP64 refinement.
+struct NestedState : MDState {
+    smg_mg* mgh;
+    const smg_nested_params& p;
+    DevMem zero;
+    int iters = 0, solves = 0;
+    NestedState(smg_dirac* d, smg_mg* m, const smg_nested_params& p_, const double* pi_h, const double* phi_h)
+        : MDState(d, false, pi_h, phi_h), mgh(m), p(p_), zero(2 * V * sizeof(double)) {
+        SMG_HIP(hipMemsetAsync(zero.p, 0, 2 * V * sizeof(double), S()));
+    }
+    void solve() {
+        SolveResult r;
+        if (p.solver == SMG_F32)
+            r = gcr_dev(D, mgh, SMG_F32, phi.p(), X.p(), p.cg_maxiter, p.cg_tol, 10, 0, p.inner_tol);
+        else
+            r = cg_solve(*D->op64, phi.p(), X.p(), nullptr, mgh ? mgh->mg.get() : nullptr, p.cg_maxiter, p.cg_tol,
+                         S());
+        iters += r.iterations;
+        solves++;
+    }
+    void fermion_kick(double scale) {
+        solve();
+        fermion_force_dev(D, X.p(), ff.as<double>(), S());
+        launch_mom_update(pi.as<double>(), ff.as<double>(), zero.as<double>(), scale, 2 * V, S());
+    }
+    void gauge_kick(double scale) {
+        const Dirac& d = *D->D;
+        launch_gauge_force(d.L, d.U64.p(), p.beta, gf.as<double>(), S());
+        launch_mom_update(pi.as<double>(), gf.as<double>(), zero.as<double>(), scale, 2 * V, S());
+    }
+    void inner(double h) {
+        const int ng = p.n_gauge;
+        const double dt = h / ng;
+        if (p.omelyan_inner) {
+            const double lam = 0.1932;
+            gauge_kick(lam * dt);
+            for (int s = 0; s < ng; s++) {
+                gauge(0.5 * dt);
+                gauge_kick((1.0 - 2.0 * lam) * dt);
+                gauge(0.5 * dt);
+                gauge_kick((s < ng - 1 ? 2.0 : 1.0) * lam * dt);
+            }
+        } else {
+            gauge_kick(0.5 * dt);
+            for (int s = 0; s < ng - 1; s++) {
+                gauge(dt);
+                gauge_kick(dt);
+            }
+            gauge(dt);
+            gauge_kick(0.5 * dt);
+        }
+    }
+    void evolve() {
+        const double h = p.tau / p.n_outer;
+        if (p.omelyan_outer) {
+            const double lam = 0.1932;
+            fermion_kick(lam * h);
+            for (int o = 0; o < p.n_outer; o++) {
+                inner(0.5 * h);
+                fermion_kick((1.0 - 2.0 * lam) * h);
+                inner(0.5 * h);
+                fermion_kick((o < p.n_outer - 1 ? 2.0 : 1.0) * lam * h);
+            }
+        } else {
+            fermion_kick(0.5 * h);
+            for (int o = 0; o < p.n_outer; o++) {
+                inner(h);
+                fermion_kick(o < p.n_outer - 1 ? h : 0.5 * h);
+            }
+        }
+    }
+    // KE, S_g, S_f with the trajectory's solver
+    void ham(double* h3) {
+        const Dirac& d = *D->D;
+        h3[0] = 0.5 * dev_dot<double>(pi.as<c64>(), pi.as<c64>(), V, S()).re;
+        Scalar1 sg;
+        launch_plaq_sum(d.L, d.U64.p(), sg.fin(), S());
+        h3[1] = -p.beta * sg.read();
+        solve();
+        h3[2] = dev_dot<double>(phi.p(), X.p(), nphi, S()).re;
+    }
+};
+void check_nested(smg_dirac* D, smg_mg* mg, const smg_nested_params* p) {
+    SMG_REQUIRE(D && p, "null argument");
+    SMG_REQUIRE(p->n_outer >= 1 && p->n_gauge >= 1 && p->tau > 0.0, "tau > 0, n_outer >= 1 and n_gauge >= 1");
+    SMG_REQUIRE(p->cg_maxiter >= 1 && p->cg_tol > 0.0, "bad solver budget");
+    check_prec(p->solver);
+    SMG_REQUIRE(p->solver == SMG_F64 || mg, "the mixed-precision solver needs a hierarchy");
+}
+}  // namespace
+
+int smg_hmc_nested_evolve(smg_dirac* D, smg_mg* mg, const smg_nested_params* p, double* pi, const double* phi,
+                          int* solver_iters) {
+    return guard([&] {
+        check_nested(D, mg, p);
+        SMG_REQUIRE(pi && phi, "null argument");
+        NestedState ns(D, mg, *p, pi, phi);
+        ns.evolve();
+        ns.pi_out(pi);
+        if (solver_iters) *solver_iters = ns.iters;
+    });
+}
+int smg_hmc_nested_trajectory(smg_dirac* D, smg_mg* mg, const smg_nested_params* p, const double* pi0,
+                              const double* eta, double u, smg_hmc_result* res) {
+    return guard([&] {
+        check_nested(D, mg, p);
+        SMG_REQUIRE(pi0 && eta && res, "null argument");
+        Dirac& d = *D->D;
+        const size_t V = (size_t)d.V;
+        Field<double> saved(2 * V);
+        launch_copy<double>(saved.p(), d.U64.p(), 2 * V, S());
+        NestedState ns(D, mg, *p, pi0, nullptr);
+        Field<double> e = up(eta, 2 * V);   // phi = D^dag eta
+        launch_dslash<double>(d.v64(), ns.phi.p(), e.p(), -d.mu_t, 1, 1, EPI_NONE, nullptr, nullptr, Fin(), 1, 2 * V,
+                              S());
+        double h0[3], h1[3];
+        ns.ham(h0);
+        ns.evolve();
+        ns.ham(h1);
+        const double dH = (h1[0] + h1[1] + h1[2]) - (h0[0] + h0[1] + h0[2]);
+        const bool accept = p->force_accept || dH < 0 || u < std::exp(-dH);
+        if (!accept) {
+            launch_copy<double>(d.U64.p(), saved.p(), 2 * V, S());
+            d.refresh_derived(S());
+        }
+        SMG_HIP(hipStreamSynchronize(S()));
+        res->accepted = accept ? 1 : 0;
+        res->dH = dH;
+        res->dKE = h1[0] - h0[0];
+        res->dSG = h1[1] - h0[1];
+        res->dSF = h1[2] - h0[2];
+        res->dLD = 0.0;
+        res->total_cg_iters = ns.iters;
+    });
+}
+
 // ------------------------------------------------------------------ U(1) heatbath (no reference counterpart)
 int smg_dirac_heatbath(smg_dirac* D, double beta, int n_sweeps, unsigned long long seed, unsigned first_sweep) {
     return guard([&] {
