@@ -16,6 +16,10 @@ FETCH_SIZE / WRITE_SIZE counter runs summarised in profiles/.  Prints one JSON
 line.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1]
+
+With --workload hmc it times trajectories of the nested Omelyan HMC at the same L
+(beta 6, m 0.02, the fermion force on one outer step, the gauge force on 8 inner
+steps, every solve the mixed-precision MG) and prints that line instead.
 """
 import argparse
 import json
@@ -27,6 +31,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+
+from schwinger_mg_amd.fields import gaussian_vec, gaussian_vecs, random_gauge  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
@@ -66,6 +72,20 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--lattice-batch", type=int, default=1,
                     help="also time the 40-lattice batch of the workload (one launch over all lattices)")
+    ap.add_argument("--workload", choices=("solve", "hmc"), default="solve",
+                    help="solve: the batched MG solve (default); hmc: trajectories of the nested Omelyan HMC")
+    ap.add_argument("--beta", type=float, default=6.0, help="hmc: gauge coupling")
+    ap.add_argument("--hmc-mass", type=float, default=0.02, help="hmc: fermion mass")
+    ap.add_argument("--n-outer", type=int, default=1, help="hmc: outer (fermion-force) steps")
+    ap.add_argument("--n-gauge", type=int, default=8, help="hmc: gauge-force steps per outer position update")
+    ap.add_argument("--therm", type=int, default=200, help="hmc: heatbath sweeps before the trajectories")
+    ap.add_argument("--hmc-null-iters", type=int, default=40, help="hmc: null-space iterations of the hierarchy")
+    ap.add_argument("--hmc-coarse-maxit", type=int, default=100, help="hmc: coarsest CG iterations")
+    ap.add_argument("--hmc-coarse-tol", type=float, default=1e-2, help="hmc: coarsest CG tolerance")
+    ap.add_argument("--mg-refresh", type=int, default=1,
+                    help="hmc: refresh the hierarchy every N outer position updates (0: Galerkin rebuild between "
+                         "trajectories only)")
+    ap.add_argument("--mg-refresh-iters", type=int, default=2, help="hmc: near-null iterations per refresh")
     return ap.parse_args()
 
 
@@ -203,19 +223,8 @@ def cpu_baseline(args, U, B, mg, threads):
                       f"{dt:.1f} s, max true residual {max(tres):.2e}"}
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
-              file=sys.stderr)
-        sys.exit(2)
-    import torch
-    import schwinger_mg_amd as smg
-    from schwinger_mg_amd.fields import gaussian_vec, gaussian_vecs, random_gauge
-
-    L, V = args.L, args.L * args.L
-    seed = 2024
-    U = random_gauge(L, args.sigma, seed)
+def build_mg(smg, args, D, L, seed):
+    V = L * L
     dims = [2 * V]
     nx = L
     for lev in range(args.levels - 1):
@@ -224,16 +233,120 @@ def main():
         dims.append(nx * nx * args.k)
     nvec = args.k // 2 if args.chiral else args.k
     starts = np.concatenate([gaussian_vecs(nvec, n, seed + 10 + i).ravel() for i, n in enumerate(dims[:-1])])
-
-    D = smg.DiracOp(L, U, args.mass, 1.0, 1.0, 0.0)
     blocks = ((4, 4),) + ((2, 2),) * (args.levels - 2)
+    return smg.MGHierarchy(D, starts, n_levels=args.levels, blocks=blocks, k=(args.k,) * (args.levels - 1),
+                           null_iters=args.null_iters, pre_smooth=args.pre, post_smooth=args.post,
+                           coarse_maxit=args.coarse_maxit, coarse_tol=args.coarse_tol,
+                           coarse_half=2 if args.coarse_bf16 else 1,
+                           coarse_solver=args.coarse_solver, use_graph=bool(args.graph), chiral=bool(args.chiral),
+                           w_cycle=bool(args.w_cycle))
+
+
+def hmc_workload(args):
+    """Trajectories of the nested HMC: Omelyan with the fermion force on the outer scale
+    (n_outer steps) and the gauge force on the fine scale (n_gauge Omelyan steps per
+    outer position update), tau = 1, every solve the mixed-precision MG GCR to 1e-10
+    inside FP64 refinement, FP64 dH and Metropolis; the hierarchy's Galerkin operators
+    are rebuilt between trajectories.  The links start from heatbath-thermalised
+    quenched configurations."""
+    import torch
+    import schwinger_mg_amd as smg
+    L, V, seed = args.L, args.L * args.L, 2024
+    D = smg.DiracOp(L, random_gauge(L, args.sigma, seed), args.hmc_mass, 1.0, 1.0, 0.0)
     t0 = time.time()
-    mg = smg.MGHierarchy(D, starts, n_levels=args.levels, blocks=blocks, k=(args.k,) * (args.levels - 1),
-                         null_iters=args.null_iters, pre_smooth=args.pre, post_smooth=args.post,
-                         coarse_maxit=args.coarse_maxit, coarse_tol=args.coarse_tol,
-                         coarse_half=2 if args.coarse_bf16 else 1,
-                         coarse_solver=args.coarse_solver, use_graph=bool(args.graph), chiral=bool(args.chiral),
-                         w_cycle=bool(args.w_cycle))
+    smg.dirac_heatbath(D, args.beta, args.therm, seed)
+    smg.synchronize()
+    therm_s = time.time() - t0
+    plaq0 = smg.avg_plaq(L, smg.gauge_of(D))
+    log(f"heatbath {args.therm} sweeps {therm_s:.2f} s, plaquette {plaq0:.6f}")
+    # the thermalised field at small mass wants a better near-null space and coarsest
+    # solve than the random-field solve workload
+    args.null_iters, args.coarse_maxit, args.coarse_tol = args.hmc_null_iters, args.hmc_coarse_maxit, \
+        args.hmc_coarse_tol
+    t0 = time.time()
+    mg = build_mg(smg, args, D, L, seed)
+    smg.synchronize()
+    setup_s = time.time() - t0
+    log(f"MG setup {setup_s:.2f} s")
+    p = smg.nested_params(args.beta, 1.0, args.n_outer, args.n_gauge, True, True, smg.F32, 500, 1e-10,
+                          args.inner_tol, mg_refresh=args.mg_refresh, mg_refresh_iters=args.mg_refresh_iters)
+    rng = np.random.default_rng(seed + 5)
+    recs, times = [], []
+    for t in range(args.warmup + args.steps):
+        pi0 = rng.normal(size=2 * V)
+        eta = rng.normal(size=2 * V) + 1j * rng.normal(size=2 * V)
+        u = rng.uniform()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        res = smg.hmc_trajectory_nested(D, p, pi0, eta, u, mg=mg)
+        if args.mg_refresh:   # the hierarchy followed the proposal; bring it to the kept field
+            mg.refresh(args.mg_refresh_iters)
+        else:
+            mg.rebuild()
+        smg.synchronize()
+        dt = time.perf_counter() - t1
+        plaq = smg.avg_plaq(L, smg.gauge_of(D))
+        rec = {"dH": res.dH, "accepted": res.accepted, "plaquette": plaq, "solver_iters": res.total_cg_iters,
+               "s": round(dt, 4)}
+        log(f"trajectory {t}: {rec}")
+        if t >= args.warmup:
+            recs.append(rec)
+            times.append(dt)
+    total = sum(times)
+    dHs = np.array([r["dH"] for r in recs])
+    out = {
+        "metric": f"HMC trajectories/sec at L={L} (nested Omelyan, mixed-precision {args.levels}-level MG)",
+        "value": round(len(recs) / total, 4),
+        "unit": "trajectories/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total / len(recs) * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 MD and dH; solves f32 (FP16 coarse links) in f64 refinement",
+        "data": "synthetic",
+        "config": {"workload": f"L={L} beta={args.beta} m={args.hmc_mass} c_sw=1, tau=1, Omelyan outer "
+                               f"n={args.n_outer} (fermion force), Omelyan inner n={args.n_gauge} (gauge force), "
+                               f"{args.therm} heatbath sweeps, {args.levels}-level MG ({args.null_iters} null "
+                               f"iterations, coarsest CG to {args.coarse_tol:g}) GCR({args.trunc}) to 1e-10, "
+                               + (f"hierarchy refreshed ({args.mg_refresh_iters} near-null iterations) every "
+                                  f"{args.mg_refresh} outer position updates" if args.mg_refresh
+                                  else "Galerkin rebuild between trajectories"),
+                   "L": L, "beta": args.beta, "mass": args.hmc_mass, "n_outer": args.n_outer,
+                   "n_gauge": args.n_gauge, "mg_refresh": args.mg_refresh,
+                   "mg_refresh_iters": args.mg_refresh_iters},
+        "acceptance": float(np.mean([r["accepted"] for r in recs])),
+        "mean_dH": float(dHs.mean()),
+        "mean_exp_minus_dH": float(np.exp(-dHs).mean()),
+        "plaquette_start": plaq0,
+        "plaquette_end": recs[-1]["plaquette"],
+        "setup_s": round(setup_s, 3),
+        "therm_s": round(therm_s, 3),
+        "trajectories": recs,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "hmc":
+        hmc_workload(args)
+        return
+    import torch
+    import schwinger_mg_amd as smg
+
+    L, V = args.L, args.L * args.L
+    seed = 2024
+    U = random_gauge(L, args.sigma, seed)
+    D = smg.DiracOp(L, U, args.mass, 1.0, 1.0, 0.0)
+    t0 = time.time()
+    mg = build_mg(smg, args, D, L, seed)
     smg.synchronize()
     setup_s = time.time() - t0
     log(f"MG setup {setup_s:.2f} s")

@@ -311,6 +311,12 @@ int smg_mg_setup_sparse_coarse(smg_mg* mg, int n_defl, double cg_tol, int max_cg
  * smg_dirac_set_gauge or an HMC update); the dense coarsest inverse follows, the
  * deflation spaces are kept as they are (as in the reference) */
 int smg_mg_rebuild(smg_mg* mg);
+/* build_full_mg with warm_start (src/mg_builder.cpp:3-31; the reference warm-starts the
+ * fine level with 5 null iterations): every level's near-null vectors iterated
+ * null_iters more times from their current values on the current gauge field, then
+ * the aggregations, the Galerkin operators and the dense coarsest inverse rebuilt;
+ * the symmetric-cycle weights and deflation spaces are kept */
+int smg_mg_refresh(smg_mg* mg, int null_iters);
 int smg_mg_coarsest_deflation(smg_mg* mg, int* ndefl, double* evals, double* evecs);
 /* cg_solve_deflated (src/solvers.cpp:111-213; declared src/solvers.h:50-56):
  * CG from x0 = U E^-1 U^dag b, E = U^dag A U, with the preconditioned residual
@@ -464,8 +470,9 @@ int smg_hmc_ms_trajectory(smg_dirac* D, smg_mg* mg, smg_prolongator* P, smg_coar
  * integrator, which kicks with the gauge force alone.  The solves of D^dag D X = phi
  * (every outer kick and both Hamiltonians) are FP64 CG, preconditioned by mg's FP64
  * cycle when mg is given (solver = SMG_F64), or mixed-precision GCR(10) with mg's FP32
- * cycle inside FP64 refinement on the FP64 true residual (solver = SMG_F32).  The
- * hierarchy is not rebuilt during the trajectory (smg_mg_rebuild between trajectories). */
+ * cycle inside FP64 refinement on the FP64 true residual (solver = SMG_F32).  With
+ * mg_refresh > 0 the hierarchy follows the evolving field: after every mg_refresh-th
+ * outer position update it is refreshed as by smg_mg_refresh(mg, mg_refresh_iters). */
 typedef struct {
     double beta, tau;
     int n_outer;        /* outer steps (fermion force) */
@@ -477,6 +484,8 @@ typedef struct {
     double cg_tol;
     double inner_tol;   /* inner tolerance of the mixed-precision solver */
     int force_accept;
+    int mg_refresh;     /* refresh the hierarchy every N outer position updates (0: never) */
+    int mg_refresh_iters;
 } smg_nested_params;
 /* the molecular dynamics from momenta pi (2V reals, in/out) and phi (2V complex); the
  * gauge field of D evolves in place; *solver_iters = the iterations of all solves */

@@ -123,7 +123,7 @@ SIGNATURES = [
     ("smg_coarse_evolve_deflation", _I, [_P, _P]),
     ("smg_hmc_ms_evolve", _I, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     ("smg_hmc_ms_trajectory", _I, [_P, _P, _P, _P, _P, _P, _P, _D, _P]),
-    ("smg_hmc_nested_evolve", _I, [_P, _P, _P, _P, _P, _P]),
+    ("smg_hmc_nested_evolve", _I, [_P, _P, _P, _P, _P, _P]), ("smg_mg_refresh", _I, [_P, _I]),
     ("smg_hmc_nested_trajectory", _I, [_P, _P, _P, _P, _P, _D, _P]),
     ("smg_dirac_heatbath", _I, [_P, _D, _I, ctypes.c_ulonglong, ctypes.c_uint]),
     ("smg_lattices_create", _I, [_P, _I, _P, _P, _P, _D, _D, _D]), ("smg_lattices_destroy", _I, [_P]),
@@ -574,6 +574,12 @@ class MGHierarchy:
         start = random_vec(dim, 42) if start_vec is None else _c(start_vec)
         _chk(lib().smg_mg_setup_sparse_coarse(self.h, n_defl, cg_tol, max_cg_iter, _p(start)))
 
+    def refresh(self, null_iters=5):
+        """build_full_mg with warm_start (src/mg_builder.cpp:3-31): null_iters more
+        near-null iterations from the current vectors on the current gauge field at
+        every level, then the aggregations and Galerkin operators rebuilt."""
+        _chk(lib().smg_mg_refresh(self.h, null_iters))
+
     def rebuild(self):
         """Re-project every level's Galerkin operator for the operator's current gauge
         field (Ac.build + rebuild_deeper_levels, src/multigrid.cpp:163-194)."""
@@ -879,13 +885,14 @@ class NestedParams(ctypes.Structure):
     outer scale, n_gauge inner gauge-force steps per outer position update."""
     _fields_ = [("beta", _D), ("tau", _D), ("n_outer", _I), ("n_gauge", _I), ("omelyan_outer", _I),
                 ("omelyan_inner", _I), ("solver", _I), ("cg_maxiter", _I), ("cg_tol", _D), ("inner_tol", _D),
-                ("force_accept", _I)]
+                ("force_accept", _I), ("mg_refresh", _I), ("mg_refresh_iters", _I)]
 
 
 def nested_params(beta, tau=1.0, n_outer=1, n_gauge=8, omelyan_outer=True, omelyan_inner=True, solver=F64,
-                  cg_maxiter=1000, cg_tol=1e-10, inner_tol=1e-5, force_accept=False):
+                  cg_maxiter=1000, cg_tol=1e-10, inner_tol=1e-5, force_accept=False, mg_refresh=0,
+                  mg_refresh_iters=2):
     return NestedParams(beta, tau, n_outer, n_gauge, int(omelyan_outer), int(omelyan_inner), solver, cg_maxiter,
-                        cg_tol, inner_tol, int(force_accept))
+                        cg_tol, inner_tol, int(force_accept), mg_refresh, mg_refresh_iters)
 
 
 def hmc_nested_evolve(D, params, pi, phi, mg=None):

@@ -1454,6 +1454,11 @@ struct NestedState : MDState {
             gauge_kick(0.5 * dt);
         }
     }
+    int moved = 0;
+    // after every mg_refresh-th outer position update the hierarchy follows the field
+    void follow() {
+        if (p.mg_refresh > 0 && mgh && ++moved % p.mg_refresh == 0) mgh->mg->refresh(p.mg_refresh_iters, S());
+    }
     void evolve() {
         const double h = p.tau / p.n_outer;
         if (p.omelyan_outer) {
@@ -1461,14 +1466,17 @@ struct NestedState : MDState {
             fermion_kick(lam * h);
             for (int o = 0; o < p.n_outer; o++) {
                 inner(0.5 * h);
+                follow();
                 fermion_kick((1.0 - 2.0 * lam) * h);
                 inner(0.5 * h);
+                follow();
                 fermion_kick((o < p.n_outer - 1 ? 2.0 : 1.0) * lam * h);
             }
         } else {
             fermion_kick(0.5 * h);
             for (int o = 0; o < p.n_outer; o++) {
                 inner(h);
+                follow();
                 fermion_kick(o < p.n_outer - 1 ? h : 0.5 * h);
             }
         }
@@ -1490,6 +1498,7 @@ void check_nested(smg_dirac* D, smg_mg* mg, const smg_nested_params* p) {
     SMG_REQUIRE(p->cg_maxiter >= 1 && p->cg_tol > 0.0, "bad solver budget");
     check_prec(p->solver);
     SMG_REQUIRE(p->solver == SMG_F64 || mg, "the mixed-precision solver needs a hierarchy");
+    SMG_REQUIRE(p->mg_refresh >= 0 && p->mg_refresh_iters >= 0, "bad refresh parameters");
 }
 }  // namespace
 
@@ -1925,6 +1934,12 @@ int smg_mg_rebuild(smg_mg* mg) {
     return guard([&] {
         SMG_REQUIRE(mg, "null handle");
         mg->mg->rebuild(S());
+    });
+}
+int smg_mg_refresh(smg_mg* mg, int null_iters) {
+    return guard([&] {
+        SMG_REQUIRE(mg && null_iters >= 0, "bad arguments");
+        mg->mg->refresh(null_iters, S());
     });
 }
 int smg_mg_setup_sparse_coarse(smg_mg* mg, int n_defl, double cg_tol, int max_cg_iter, const double* start) {

@@ -562,6 +562,22 @@ void MG::rebuild(hipStream_t st) {
     SMG_HIP(hipStreamSynchronize(st));
 }
 
+void MG::refresh(int iters, hipStream_t st) {
+    LinOp<double>* cur = fine64.get();
+    std::vector<std::unique_ptr<LinOp<double>>> ops64;
+    for (size_t l = 0; l < C.size(); l++) {
+        const int nv = cfg.chiral ? cfg.k[l] / 2 : cfg.k[l];
+        if (iters > 0) null_space(*cur, nv, iters, cfg.null_inner, nullv[l]->p(), st);
+        P[l]->build(nullv[l]->p(), st);
+        C[l]->build(*cur, *P[l], st);
+        ops64.emplace_back(new CoarseLinOp<double, c64>(*C[l]));
+        cur = ops64.back().get();
+    }
+    if (cfg.coarse_solver == 1) build_dense_inverse(st);
+    reset_graphs();
+    SMG_HIP(hipStreamSynchronize(st));
+}
+
 void MG::reset_graphs() {
     for (auto& g : graph) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);

@@ -251,6 +251,11 @@ struct MG {
     // the fine operator, with the existing prolongators (Ac.build at level 0, then
     // MGHierarchy::rebuild_deeper_levels, src/multigrid.cpp:163-194)
     void rebuild(hipStream_t st);
+    // build_full_mg with warm_start (src/mg_builder.cpp:3-31, null iterations 5 when
+    // warm): every level's near-null vectors iterated iters more times from their
+    // current values on the current operator, then the aggregation and the Galerkin
+    // operators rebuilt (the reference warm-starts the fine level only)
+    void refresh(int iters, hipStream_t st);
     void precondition(const c64* b, c64* x, hipStream_t st);
     void precondition(const c32* b, c32* x, hipStream_t st);
     template <typename T>

@@ -66,3 +66,36 @@ def test_setup_sparse_coarse(smg):
                           n_levels=3, blocks=((4, 4), (2, 2)), k=(k, k), null_iters=5)
     with pytest.raises(smg.SMGError):
         mg3.setup_sparse_coarse(4)
+
+
+def test_refresh_is_warm_started_setup(smg):
+    """MGHierarchy refresh (build_full_mg with warm_start, src/mg_builder.cpp:3-31) after
+    a gauge change equals a new hierarchy set up on the new field from the current
+    near-null vectors of every level with the same iteration count; refresh(0) equals
+    the Galerkin rebuild."""
+    banner("MG refresh = warm-started setup")
+    L, k = 16, 4
+    U1 = random_gauge(L, 0.3, 410)
+    U2 = U1 * np.exp(0.3j * np.random.default_rng(411).standard_normal(U1.shape))
+    D = smg.DiracOp(L, U1, 0.05, 1.0, 1.0, 0.0)
+    dims = [2 * L * L, (L // 4) ** 2 * k]
+    starts = np.concatenate([gaussian_vecs(k, n, 412 + i).ravel() for i, n in enumerate(dims)])
+    kw = dict(n_levels=3, blocks=((4, 4), (2, 2)), k=(k, k), coarse_solver=1)
+    mg = smg.MGHierarchy(D, starts, null_iters=5, **kw)
+    D.set_gauge(U2)
+    warm = np.concatenate([mg.null_vecs(0).ravel(), mg.null_vecs(1).ravel()])
+    # two more hierarchies set up on the old field: one rebuilt, one refreshed with 0 iterations
+    D.set_gauge(U1)
+    mg_rb = smg.MGHierarchy(D, starts, null_iters=5, **kw)
+    mg0 = smg.MGHierarchy(D, starts, null_iters=5, **kw)
+    D.set_gauge(U2)
+    mg_rb.rebuild()
+    mg0.refresh(0)
+    b = gaussian_vec(2 * L * L, 415)
+    assert relerr(mg0.precondition(b), mg_rb.precondition(b)) < 1e-13
+    mg.refresh(3)
+    fresh = smg.MGHierarchy(D, warm, null_iters=3, **kw)
+    assert relerr(mg.null_vecs(0), fresh.null_vecs(0)) < 1e-12
+    assert relerr(mg.precondition(b), fresh.precondition(b)) < 1e-11
+    r = smg.fcg_solve(D, b, mg, 100, 1e-10, 10)
+    assert r.true_residual < 1.1e-10
