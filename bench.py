@@ -86,6 +86,8 @@ def parse():
                     help="hmc: refresh the hierarchy every N outer position updates (0: Galerkin rebuild between "
                          "trajectories only)")
     ap.add_argument("--mg-refresh-iters", type=int, default=2, help="hmc: near-null iterations per refresh")
+    ap.add_argument("--lobpcg", type=int, default=16,
+                    help="hmc: low modes of D^dag D tracked by LOBPCG (one block) after every trajectory (0: none)")
     return ap.parse_args()
 
 
@@ -270,6 +272,14 @@ def hmc_workload(args):
     log(f"MG setup {setup_s:.2f} s")
     p = smg.nested_params(args.beta, 1.0, args.n_outer, args.n_gauge, True, True, smg.F32, 500, 1e-10,
                           args.inner_tol, mg_refresh=args.mg_refresh, mg_refresh_iters=args.mg_refresh_iters)
+    X = None
+    if args.lobpcg:   # the low modes, tracked as one block of right-hand sides
+        t0 = time.time()
+        lr = smg.lobpcg_update(D, gaussian_vecs(args.lobpcg, 2 * V, seed + 7), precond=mg, max_iter=10, tol=1e-8,
+                               exact_ax=True)
+        X = lr.eigvecs
+        log(f"LOBPCG {args.lobpcg} modes from random starts: {time.time() - t0:.2f} s, {lr.iterations} iterations, "
+            f"lowest {lr.eigvals[0]:.4e}")
     rng = np.random.default_rng(seed + 5)
     recs, times = [], []
     for t in range(args.warmup + args.steps):
@@ -288,6 +298,14 @@ def hmc_workload(args):
         plaq = smg.avg_plaq(L, smg.gauge_of(D))
         rec = {"dH": res.dH, "accepted": res.accepted, "plaquette": plaq, "solver_iters": res.total_cg_iters,
                "s": round(dt, 4)}
+        if X is not None:   # warm-started from the previous trajectory's modes
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            lr = smg.lobpcg_update(D, X, precond=mg, max_iter=3, tol=1e-8, exact_ax=True)
+            rec["lobpcg_s"] = round(time.perf_counter() - t1, 4)
+            rec["lobpcg_iters"] = lr.iterations
+            rec["lowest_modes"] = [float(v) for v in lr.eigvals[:4]]
+            X = lr.eigvecs
         log(f"trajectory {t}: {rec}")
         if t >= args.warmup:
             recs.append(rec)
@@ -324,6 +342,7 @@ def hmc_workload(args):
         "plaquette_end": recs[-1]["plaquette"],
         "setup_s": round(setup_s, 3),
         "therm_s": round(therm_s, 3),
+        "lobpcg_modes": args.lobpcg,
         "trajectories": recs,
     }
     print(json.dumps(out), flush=True)

@@ -288,6 +288,13 @@ int smg_coarse_set_deflation(smg_coarse* C, int ndefl, const double* evals, cons
  * as one block of multi-right-hand-side launches. */
 int smg_dirac_lobpcg(smg_dirac* D, smg_mg* mg, int k, const double* X0, int max_iter, double tol, double* evals,
                      double* X, int* iterations);
+/* smg_dirac_lobpcg with exact_ax = 1: the re-orthonormalisation of X keeps AX = A X
+ * (AX_i -= <X_j, X_i> AX_j) where the reference subtracts <X_j, X_i> X_j
+ * (src/eigensolver_core.cpp:514-517), which leaves AX inconsistent and the warm-started
+ * Rayleigh quotients non-monotone; exact_ax = 0 is smg_dirac_lobpcg.  No reference
+ * counterpart. */
+int smg_dirac_lobpcg_ex(smg_dirac* D, smg_mg* mg, int k, const double* X0, int max_iter, double tol, int exact_ax,
+                        double* evals, double* X, int* iterations);
 /* lobpcg_update on the coarse operator with the identity preconditioner */
 int smg_coarse_lobpcg(smg_coarse* C, int k, const double* X0, int max_iter, double tol, double* evals, double* X,
                       int* iterations);

@@ -71,6 +71,8 @@ def lib():
         L.orc_von_mises.restype = _D
         L.orc_von_mises.argtypes = [_D, ctypes.c_uint32, ctypes.c_uint32, _P]
         L.orc_heatbath.argtypes = [_I, _P, _D, _I, ctypes.c_uint64, ctypes.c_uint32]
+        L.orc_lobpcg_ex.restype = _I
+        L.orc_lobpcg_ex.argtypes = [_P, _P, _I, _P, _I, _D, _I, _P, _P]
         L.orc_lobpcg.restype = _I
         L.orc_lobpcg.argtypes = [_P, _P, _I, _P, _I, _D, _P, _P]
         L.orc_cg_deflated.restype = _I
@@ -595,13 +597,15 @@ def cg_deflated(op, b, U, maxit, tol, mg=None):
     return x, it, res[0]
 
 
-def lobpcg(op, X0, max_iter, tol, mg=None):
-    """lobpcg_update from the k start vectors X0 [k][n]: (eigenvalues, X [k][n], iterations)."""
+def lobpcg(op, X0, max_iter, tol, mg=None, exact_ax=False):
+    """lobpcg_update from the k start vectors X0 [k][n]: (eigenvalues, X [k][n], iterations);
+    exact_ax keeps AX = A X through the re-orthonormalisation (the reference does not)."""
     X0 = np.ascontiguousarray(X0, np.complex128).reshape(-1, op.n)
     k = X0.shape[0]
     X = np.zeros_like(X0)
     ev = np.zeros(k)
-    it = lib().orc_lobpcg(op.h, mg.h if mg is not None else None, k, _ptr(X0), max_iter, tol, _ptr(X), _ptr(ev))
+    it = lib().orc_lobpcg_ex(op.h, mg.h if mg is not None else None, k, _ptr(X0), max_iter, tol, int(exact_ax),
+                             _ptr(X), _ptr(ev))
     return ev, X, it
 
 

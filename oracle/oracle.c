@@ -2119,13 +2119,16 @@ int orc_trlm(const orc_op* A, int n_ev, int n_kr, int max_restarts, double tol, 
  * Cholesky falls back to one preconditioned steepest-descent step per vector.
  * The closing re-orthonormalisation updates AX_i by -<X_j,X_i> X_j, as the reference
  * does, so AX is A X only up to that term.  X0, X: [k][n]; returns the iterations run. */
+static int lob_exact_ax = 0;   /* 1: AX_i -= pr AX_j, which keeps AX = A X */
 static void lob_mgs(int n, int k, ocx* X, ocx* AX) {
     for (int i = 0; i < k; i++) {
         for (int j = 0; j < i; j++) {
             ocx pr = orc_dot(n, X + (size_t)j * n, X + (size_t)i * n);
             for (int t = 0; t < n; t++) X[(size_t)i * n + t] -= pr * X[(size_t)j * n + t];
-            if (AX)   /* the reference subtracts pr X_j (not pr AX_j) from AX_i (:514-517) */
-                for (int t = 0; t < n; t++) AX[(size_t)i * n + t] -= pr * X[(size_t)j * n + t];
+            if (AX) {   /* the reference subtracts pr X_j (not pr AX_j) from AX_i (:514-517) */
+                const ocx* src = lob_exact_ax ? AX + (size_t)j * n : X + (size_t)j * n;
+                for (int t = 0; t < n; t++) AX[(size_t)i * n + t] -= pr * src[t];
+            }
         }
         double nv = orc_norm(n, X + (size_t)i * n);
         if (nv > 1e-14) {
@@ -2134,6 +2137,15 @@ static void lob_mgs(int n, int k, ocx* X, ocx* AX) {
                 for (int t = 0; t < n; t++) AX[(size_t)i * n + t] /= nv;
         }
     }
+}
+/* orc_lobpcg with exact_ax = 1: the re-orthonormalisation keeps AX = A X (AX_i -= pr AX_j)
+ * in place of the reference's AX_i -= pr X_j; no reference counterpart */
+int orc_lobpcg_ex(const orc_op* A, orc_mg* mg, int k, const ocx* X0, int max_iter, double tol, int exact_ax,
+                  ocx* Xout, double* evals) {
+    lob_exact_ax = exact_ax != 0;
+    int it = orc_lobpcg(A, mg, k, X0, max_iter, tol, Xout, evals);
+    lob_exact_ax = 0;
+    return it;
 }
 int orc_lobpcg(const orc_op* A, orc_mg* mg, int k, const ocx* X0, int max_iter, double tol, ocx* Xout,
                double* evals) {

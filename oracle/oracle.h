@@ -154,6 +154,8 @@ int orc_trlm(const orc_op* A, int n_ev, int n_kr, int max_restarts, double tol, 
              double* evals, ocx* evecs, int* restarts_out, int* iters_out);
 int orc_lobpcg(const orc_op* A, orc_mg* mg, int k, const ocx* X0, int max_iter, double tol, ocx* Xout,
                double* evals);
+int orc_lobpcg_ex(const orc_op* A, orc_mg* mg, int k, const ocx* X0, int max_iter, double tol, int exact_ax,
+                  ocx* Xout, double* evals);
 int orc_cg_deflated(const orc_op* A, const ocx* rhs, orc_mg* mg, int ndefl, const ocx* U, int maxit, double tol,
                     ocx* x, double* final_res);
 int orc_deflated_coarse_cg(const orc_op* A, int ndefl, const ocx* vecs, const double* vals, const ocx* b,

@@ -118,6 +118,7 @@ SIGNATURES = [
     ("smg_coarse_setup_deflation", _I, [_P, _I, _I, _I, _D, _P, _P]),
     ("smg_coarse_deflation", _I, [_P, _P, _P, _P]), ("smg_coarse_set_deflation", _I, [_P, _I, _P, _P]),
     ("smg_cg_solve_deflated", _I, [_P, _I, _P, _P, _I, _P, _P, _I, _D, _P]),
+    ("smg_dirac_lobpcg_ex", _I, [_P, _P, _I, _P, _I, _D, _I, _P, _P, _P]),
     ("smg_dirac_lobpcg", _I, [_P, _P, _I, _P, _I, _D, _P, _P, _P]),
     ("smg_coarse_lowmode_force", _I, [_P, _P, _P, _P, _I, _P]),
     ("smg_coarse_evolve_deflation", _I, [_P, _P]),
@@ -809,19 +810,26 @@ class LOBPCGResult:
     iterations: int
 
 
-def lobpcg_update(A, X0, precond=None, max_iter=5, tol=1e-8):
+def lobpcg_update(A, X0, precond=None, max_iter=5, tol=1e-8, exact_ax=False):
     """lobpcg_update (src/eigensolver_core.cpp:279-533) from the k start vectors X0
     (rows).  A = a DiracOp (the operator D^dag D) or a SparseCoarseOp; precond = an
-    MGHierarchy (its FP64 cycle; fine operator only) or None for the identity."""
+    MGHierarchy (its FP64 cycle; fine operator only) or None for the identity.
+    exact_ax (DiracOp only) keeps AX = A X through the re-orthonormalisation, where
+    the reference's update does not."""
     X0 = _c(X0)
     k, n = X0.shape
     ev = np.zeros(k)
     X = np.zeros_like(X0)
     it = _I()
     if isinstance(A, SparseCoarseOp):
+        if exact_ax:
+            raise SMGError("lobpcg_update: exact_ax is offered on the Dirac operator")
         if precond is not None:
             raise SMGError("lobpcg_update: the coarse operator takes the identity preconditioner")
         _chk(lib().smg_coarse_lobpcg(A.h, k, _p(X0), max_iter, tol, _p(ev), _p(X), ctypes.byref(it)))
+    elif exact_ax:
+        _chk(lib().smg_dirac_lobpcg_ex(A.h, precond.h if precond else None, k, _p(X0), max_iter, tol, 1, _p(ev),
+                                       _p(X), ctypes.byref(it)))
     else:
         _chk(lib().smg_dirac_lobpcg(A.h, precond.h if precond else None, k, _p(X0), max_iter, tol, _p(ev), _p(X),
                                     ctypes.byref(it)))

@@ -1985,6 +1985,17 @@ int smg_dirac_lobpcg(smg_dirac* D, smg_mg* mg, int k, const double* X0, int max_
         lobpcg_out(r, n, evals, X, iterations);
     });
 }
+int smg_dirac_lobpcg_ex(smg_dirac* D, smg_mg* mg, int k, const double* X0, int max_iter, double tol, int exact_ax,
+                        double* evals, double* X, int* iterations) {
+    return guard([&] {
+        SMG_REQUIRE(D && X0 && k >= 1 && k <= 32, "bad arguments (1 <= k <= 32)");
+        const size_t n = 2 * (size_t)D->D->V;
+        DdagDOpB<double> A(*D->D, next_pow2(k));
+        Field<double> x0 = up(X0, (size_t)k * n);
+        LobpcgResult r = lobpcg(A, mg ? mg->mg.get() : nullptr, k, x0.p(), max_iter, tol, S(), exact_ax != 0);
+        lobpcg_out(r, n, evals, X, iterations);
+    });
+}
 int smg_coarse_lobpcg(smg_coarse* C, int k, const double* X0, int max_iter, double tol, double* evals, double* X,
                       int* iterations) {
     return guard([&] {

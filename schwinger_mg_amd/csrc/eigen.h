@@ -77,6 +77,9 @@ struct LobpcgResult {
     Field<double> vecs;   // k vectors of A.n(), one after the other
     int iterations = 0;
 };
-LobpcgResult lobpcg(LinOpB<double>& A, MG* mg, int k, const c64* X0, int max_iter, double tol, hipStream_t st);
+// exact_ax: the re-orthonormalisation keeps AX = A X (AX <- AX R^-1) in place of the
+// reference's update
+LobpcgResult lobpcg(LinOpB<double>& A, MG* mg, int k, const c64* X0, int max_iter, double tol, hipStream_t st,
+                    bool exact_ax = false);
 
 }  // namespace smg

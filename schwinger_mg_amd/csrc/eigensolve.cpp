@@ -537,7 +537,8 @@ struct BlockWork {
 };
 }  // namespace
 
-LobpcgResult lobpcg(LinOpB<double>& A, MG* mg, int k, const c64* X0, int max_iter, double tol, hipStream_t st) {
+LobpcgResult lobpcg(LinOpB<double>& A, MG* mg, int k, const c64* X0, int max_iter, double tol, hipStream_t st,
+                    bool exact_ax) {
     const size_t n = A.n();
     const int N = A.N;
     SMG_REQUIRE(k >= 1 && k <= N && N <= 32 && (N & (N - 1)) == 0, "lobpcg: 1 <= k <= N <= 32, N a power of two");
@@ -583,12 +584,17 @@ LobpcgResult lobpcg(LinOpB<double>& A, MG* mg, int k, const c64* X0, int max_ite
             for (int c = 0; c < k; c++) C0[(size_t)j * N + c] = Ri[(size_t)j * k + c];
         if (with_ax) {
             C1 = ident(2);
-            for (int j = 0; j < k; j++)
-                for (int c = 0; c < k; c++) C1[(size_t)j * N + c] = Ri[(size_t)j * k + c];
-            for (int c = 0; c < k; c++) {
-                const double inv = 1.0 / Rm[(size_t)c * k + c].real();
-                C1[(size_t)c * N + c] -= inv;
-                C1[((size_t)N + c) * N + c] = inv;
+            if (exact_ax) {   // AX <- AX R^-1
+                for (int j = 0; j < k; j++)
+                    for (int c = 0; c < k; c++) C1[((size_t)N + j) * N + c] = Ri[(size_t)j * k + c];
+            } else {
+                for (int j = 0; j < k; j++)
+                    for (int c = 0; c < k; c++) C1[(size_t)j * N + c] = Ri[(size_t)j * k + c];
+                for (int c = 0; c < k; c++) {
+                    const double inv = 1.0 / Rm[(size_t)c * k + c].real();
+                    C1[(size_t)c * N + c] -= inv;
+                    C1[((size_t)N + c) * N + c] = inv;
+                }
             }
             bw.combine(Xn.p(), AXn.p(), {X.p(), AX.p()}, C0, C1, st);
             std::swap(X, Xn);

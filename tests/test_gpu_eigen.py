@@ -230,6 +230,33 @@ def test_lobpcg_mg_preconditioned_vs_oracle(smg, k):
     assert relerr(res.eigvecs, Xo) < 1e-6
 
 
+@pytest.mark.parametrize("precond", [False, True])
+def test_lobpcg_exact_ax_vs_oracle(smg, precond):
+    """The LOBPCG variant that keeps AX = A X, on the device against the oracle's, from
+    the same block; its returned quotients are those of the returned vectors."""
+    banner(f"lobpcg exact AX L=16 precond={precond}")
+    L, kn, k = 16, 4, 8
+    U = random_gauge(L, 0.3, 280)
+    starts = gaussian_vecs(kn, 2 * L * L, 281).ravel()
+    D = smg.DiracOp(L, U, 0.05, 1.0, 1.0, 0.0)
+    Do = O.Dirac(L, U, 0.05, 1.0, 1.0, 0.0)
+    mg = mgo = None
+    if precond:
+        mg = smg.MGHierarchy(D, starts, n_levels=2, blocks=((4, 4),), k=(kn,), null_iters=10, coarse_solver=1)
+        mgo = O.MG(Do.op(), L, L, 2, starts, n_levels=2, block=((4, 4),), k=(kn,), null_iters=10, coarse_solver=1)
+    X0 = gaussian_vecs(k, 2 * L * L, 282)
+    res = smg.lobpcg_update(D, X0, mg, 6, 1e-10, exact_ax=True)
+    evo, Xo, ito = O.lobpcg(Do.op(), X0, 6, 1e-10, mg=mgo, exact_ax=True)
+    assert res.iterations == ito
+    assert np.abs(res.eigvals - evo).max() < 1e-8 * np.abs(evo).max()
+    # each vector agrees up to its phase (the Ritz vectors' phases are arbitrary)
+    ph = np.array([np.vdot(g, o) for g, o in zip(res.eigvecs, Xo)])
+    assert relerr(res.eigvecs * (ph / np.abs(ph))[:, None], Xo) < 1e-6
+    A = Do.op()
+    rq = np.array([np.vdot(x, A(x)).real for x in res.eigvecs])
+    assert np.abs(res.eigvals - rq).max() < 1e-10 * rq.max()
+
+
 def test_lobpcg_coarse_and_coarsest_deflation(smg):
     """lobpcg_update on a coarse operator, and MGHierarchy::update_coarsest_deflation:
     cold start from random_vec(dim) drawn n_defl times from mt19937(42), then a warm

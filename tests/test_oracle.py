@@ -591,3 +591,32 @@ def test_fgmres_ritz_first_cycle_identity():
         assert abs(np.linalg.norm(u) - 1) < 1e-12
         assert abs(np.vdot(Au, Au).real - lv) < 1e-10 * max(lv, 1.0)
         assert lv >= smin2 * (1 - 1e-10)
+
+
+def test_lobpcg_exact_ax_consistent_and_converging():
+    """The LOBPCG variant that keeps AX = A X: the returned AX-based quotients equal
+    those recomputed from X on the dense operator after every warm restart, and a long
+    run reaches the k lowest eigenvalues of numpy's eigh.  The reference's update
+    (exact_ax = 0) fails the first property on the same start."""
+    L, k = 8, 4
+    U = random_gauge(L, 0.4, 31)
+    A = O.Dirac(L, U, 0.1, 1.0, 1.0, 0.0).op()
+    Ad = np.stack([A(e) for e in np.eye(2 * L * L, dtype=np.complex128)], axis=1)
+    exact = np.linalg.eigvalsh(Ad)[:k]
+    X = gaussian_vecs(k, 2 * L * L, 32)
+    X0 = X.copy()
+    for rep in range(6):
+        ev, X, _ = O.lobpcg(A, X, 3, 1e-14, exact_ax=True)
+        rq = np.array([np.vdot(x, Ad @ x).real for x in X])
+        assert np.abs(ev - rq).max() < 1e-10 * rq.max()
+    ev, X, _ = O.lobpcg(A, X, 400, 1e-12, exact_ax=True)
+    # the Rayleigh quotients approach the k lowest eigenvalues (the reference's restart
+    # rules leave the last digits to slow convergence), and their sum bounds theirs
+    assert np.abs(np.sort(ev) - exact).max() < 2e-3 * exact.max()
+    assert ev.sum() >= exact.sum() * (1 - 1e-12)
+    worst = 0.0
+    for rep in range(6):
+        ev, X0, _ = O.lobpcg(A, X0, 3, 1e-14, exact_ax=False)
+        rq = np.array([np.vdot(x, Ad @ x).real for x in X0])
+        worst = max(worst, np.abs(ev - rq).max() / rq.max())
+    assert worst > 1e-3
