@@ -174,45 +174,53 @@ __global__ __launch_bounds__(kBlock) void k_coarse_apply(CoarseView<TL> C, cplx<
 }
 
 // Power-of-two k: one lane per link entry (b, ki, kj), so the k x k block of each
-// direction is one coalesced read, the grid is nb*k*k lanes (the whole chip even
-// on small coarse lattices), and the row sums over kj are shuffles inside groups
-// of K lanes.
+// direction is one coalesced read, and the row sums over kj are shuffles inside
+// groups of K lanes.  The workgroups (at most kLanesGrid) stride over the nb*k*k
+// entries: a reduction then finishes over few partial rows (each workgroup waits
+// for its partials to reach memory before it retires), and the stride, a multiple
+// of kBlock and so of K, keeps every row inside one wave and one iteration; the
+// loop bound is uniform per workgroup, so every lane reaches every shuffle.
+constexpr int kLanesGrid = 2048;
 template <typename T, typename TL, int EPI, int K>
 __global__ __launch_bounds__(kBlock) void k_coarse_apply_lanes(CoarseView<TL> C, cplx<T>* __restrict__ dst,
                                                                const cplx<T>* __restrict__ src,
                                                                const cplx<T>* __restrict__ bvec,
                                                                const cplx<T>* __restrict__ aux, Fin fin) {
+    static_assert(kBlock % K == 0 && K <= 64, "a row of K lanes must sit inside one wave");
     constexpr int KK = K * K;
     const int nb = C.nx * C.ny;
-    const size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    const int b = (int)(t / KK), e = (int)(t % KK), kj = e % K;
-    cplx<T> sum = {(T)0, (T)0};
-    if (b < nb) {
-        const int bx = b % C.nx, by = b / C.nx;
-        const int xm = (bx == 0 ? C.nx - 1 : bx - 1), xp = (bx + 1 == C.nx ? 0 : bx + 1);
-        const int ym = (by == 0 ? C.ny - 1 : by - 1), yp = (by + 1 == C.ny ? 0 : by + 1);
-        const int nbr[9] = {b, xp + by * C.nx, xm + by * C.nx, bx + yp * C.nx, bx + ym * C.nx,
-                            xp + yp * C.nx, xp + ym * C.nx, xm + yp * C.nx, xm + ym * C.nx};
-        const TL* lk = C.link + (size_t)b * 9 * KK + e;
-#pragma unroll
-        for (int d = 0; d < 9; d++) sum = sum + cvt<T>(load_c(lk + d * KK)) * src[(size_t)nbr[d] * K + kj];
-    }
-#pragma unroll
-    for (int off = K / 2; off > 0; off >>= 1) {
-        sum.re += __shfl_xor(sum.re, off, 64);
-        sum.im += __shfl_xor(sum.im, off, 64);
-    }
+    const size_t total = (size_t)nb * KK;
     double acc[3] = {0.0, 0.0, 0.0};
-    if (b < nb && kj == 0) {
-        const size_t o = (size_t)b * K + e / K;
-        cplx<T> out = sum;
-        if (EPI == EPI_RESID) out = bvec[o] - out;
-        dst[o] = out;
-        if (EPI == EPI_DOTS) {
-            const cplx<T> p = cdot(out, aux[o]);
-            acc[0] = p.re;
-            acc[1] = p.im;
-            acc[2] = norm2(out);
+    for (size_t t0 = (size_t)blockIdx.x * kBlock; t0 < total; t0 += (size_t)gridDim.x * kBlock) {
+        const size_t t = t0 + threadIdx.x;
+        const int b = (int)(t / KK), e = (int)(t % KK), kj = e % K;
+        cplx<T> sum = {(T)0, (T)0};
+        if (b < nb) {
+            const int bx = b % C.nx, by = b / C.nx;
+            const int xm = (bx == 0 ? C.nx - 1 : bx - 1), xp = (bx + 1 == C.nx ? 0 : bx + 1);
+            const int ym = (by == 0 ? C.ny - 1 : by - 1), yp = (by + 1 == C.ny ? 0 : by + 1);
+            const int nbr[9] = {b, xp + by * C.nx, xm + by * C.nx, bx + yp * C.nx, bx + ym * C.nx,
+                                xp + yp * C.nx, xp + ym * C.nx, xm + yp * C.nx, xm + ym * C.nx};
+            const TL* lk = C.link + (size_t)b * 9 * KK + e;
+#pragma unroll
+            for (int d = 0; d < 9; d++) sum = sum + cvt<T>(load_c(lk + d * KK)) * src[(size_t)nbr[d] * K + kj];
+        }
+#pragma unroll
+        for (int off = K / 2; off > 0; off >>= 1) {
+            sum.re += __shfl_xor(sum.re, off, 64);
+            sum.im += __shfl_xor(sum.im, off, 64);
+        }
+        if (b < nb && kj == 0) {
+            const size_t o = (size_t)b * K + e / K;
+            cplx<T> out = sum;
+            if (EPI == EPI_RESID) out = bvec[o] - out;
+            dst[o] = out;
+            if (EPI == EPI_DOTS) {
+                const cplx<T> p = cdot(out, aux[o]);
+                acc[0] += p.re;
+                acc[1] += p.im;
+                acc[2] += norm2(out);
+            }
         }
     }
     if (EPI == EPI_DOTS) reduce_finish<3>(acc, fin, gridDim.x, blockIdx.x);
@@ -226,7 +234,8 @@ static void coarse_apply_dispatch(const CoarseView<TL>& C, cplx<T>* dst, const c
     switch (C.k) {
 #define SMG_CK(K)                                                                                         \
     case K:                                                                                                 \
-        k_coarse_apply_lanes<T, TL, EPI, K><<<grid_for(lanes), kBlock, 0, st>>>(C, dst, src, b, aux, fin); \
+        k_coarse_apply_lanes<T, TL, EPI, K><<<std::min(grid_for(lanes), kLanesGrid), kBlock, 0, st>>>(    \
+            C, dst, src, b, aux, fin);                                                                      \
         break;
         SMG_CK(2)
         SMG_CK(4)
