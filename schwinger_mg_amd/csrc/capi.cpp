@@ -765,15 +765,16 @@ int smg_fgmres_solve_ritz(smg_dirac* D, smg_mg* mg, const double* b, double* x, 
         SMG_REQUIRE(n_ritz == 0 || (n_pairs && ritz_vals && ritz_vecs), "ritz output buffers required");
         const size_t n = D->op64->n();
         Field<double> bb = up(b, n), xx(n);
-        RitzHarvest rh(n_ritz, restart, n);
+        std::unique_ptr<RitzHarvest> rh;
+        if (n_ritz > 0) rh.reset(new RitzHarvest(n_ritz, restart, n));
         SolveResult r = fgmres_solve(*D->op64, bb.p(), xx.p(), mg ? mg->mg.get() : nullptr, restart, maxit, tol, S(),
-                                     n_ritz > 0 ? &rh : nullptr);
+                                     rh.get());
         fill_info(info, r);
         downc(x, xx.p(), n);
-        if (n_pairs) *n_pairs = n_ritz > 0 ? rh.count() : 0;
-        if (n_ritz > 0 && rh.count() > 0) {
-            std::copy(rh.vals.begin(), rh.vals.end(), ritz_vals);
-            downc(ritz_vecs, rh.vecs.p(), (size_t)rh.count() * n);
+        if (n_pairs) *n_pairs = rh ? rh->count() : 0;
+        if (rh && rh->count() > 0) {
+            std::copy(rh->vals.begin(), rh->vals.end(), ritz_vals);
+            downc(ritz_vecs, rh->vecs.p(), (size_t)rh->count() * n);
         }
     });
 }
