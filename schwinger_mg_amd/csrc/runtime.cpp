@@ -244,7 +244,7 @@ template <>
 const c32* Aggregation::bv<float>() const { return bv32.p(); }
 
 // ------------------------------------------------------------------ coarse operator
-static int color_period(int nb) {
+int color_period(int nb) {
     if (nb < 3) return nb;
     for (int c = 3; c <= nb; c++)
         if (nb % c == 0) return c;
@@ -534,7 +534,7 @@ MG::MG(const Dirac& d, const MGConfig& c, const c64* starts, bool on_device, boo
         P.emplace_back(new Aggregation(lnx[l], lny[l], lnd[l], c.bx[l], c.by[l], k, c.chiral));
         P.back()->build(nullv.back()->p(), st);
         C.emplace_back(new CoarseOperator(P.back()->A.nbx, P.back()->A.nby, k));
-        C.back()->build(*cur, *P.back(), st);
+        galerkin(l, st);
         ops64.emplace_back(new CoarseLinOp<double, c64>(*C.back()));
         cur = ops64.back().get();
         dims.push_back(C.back()->n());
@@ -549,14 +549,21 @@ MG::MG(const Dirac& d, const MGConfig& c, const c64* starts, bool on_device, boo
 }
 MG::~MG() { reset_graphs(); }
 
-void MG::rebuild(hipStream_t st) {
-    LinOp<double>* cur = fine64.get();
-    std::vector<std::unique_ptr<LinOp<double>>> ops64;
-    for (size_t l = 0; l < C.size(); l++) {
-        C[l]->build(*cur, *P[l], st);
-        ops64.emplace_back(new CoarseLinOp<double, c64>(*C[l]));
-        cur = ops64.back().get();
+// level l's Galerkin operator from level l's operator (the fine D^dag D or the
+// coarse operator of level l - 1), probed 32 columns at a time
+void MG::galerkin(size_t l, hipStream_t st) {
+    constexpr int kProbeBatch = 32;
+    if (l == 0) {
+        DdagDOpB<double> A(*D, kProbeBatch);
+        C[0]->build_b(A, *P[0], st);
+    } else {
+        CoarseLinOpB<double, c64> A(*C[l - 1], kProbeBatch);
+        C[l]->build_b(A, *P[l], st);
     }
+}
+
+void MG::rebuild(hipStream_t st) {
+    for (size_t l = 0; l < C.size(); l++) galerkin(l, st);
     if (cfg.coarse_solver == 1) build_dense_inverse(st);
     reset_graphs();   // captured cycles are recaptured on their next use
     SMG_HIP(hipStreamSynchronize(st));
@@ -569,7 +576,7 @@ void MG::refresh(int iters, hipStream_t st) {
         const int nv = cfg.chiral ? cfg.k[l] / 2 : cfg.k[l];
         if (iters > 0) null_space(*cur, nv, iters, cfg.null_inner, nullv[l]->p(), st);
         P[l]->build(nullv[l]->p(), st);
-        C[l]->build(*cur, *P[l], st);
+        galerkin(l, st);
         ops64.emplace_back(new CoarseLinOp<double, c64>(*C[l]));
         cur = ops64.back().get();
     }

@@ -142,6 +142,12 @@ struct Aggregation {
 };
 
 // 9-point coarse operator, src/coarse_op.h:154-297, stored in FP64, FP32, FP16.
+template <typename T>
+struct LinOpB;
+
+// the colour period of the probing along a coarse direction of nb sites
+int color_period(int nb);
+
 struct CoarseOperator {
     int nx, ny, k;
     Field<double> l64;
@@ -152,6 +158,9 @@ struct CoarseOperator {
     size_t nlinks() const { return (size_t)nx * ny * 9 * k * k; }
     // Galerkin P^dag A P by coloured probing in FP64
     void build(LinOp<double>& A, const Aggregation& P, hipStream_t st);
+    // the same probing with A.N probes at a time, each batch prolonged, applied and
+    // restricted as one block of right-hand sides
+    void build_b(LinOpB<double>& A, const Aggregation& P, hipStream_t st);
     void set_links(const c64* dev, hipStream_t st);
     void refresh_low_precision(hipStream_t st);
 };
@@ -251,6 +260,7 @@ struct MG {
     // the fine operator, with the existing prolongators (Ac.build at level 0, then
     // MGHierarchy::rebuild_deeper_levels, src/multigrid.cpp:163-194)
     void rebuild(hipStream_t st);
+    void galerkin(size_t level, hipStream_t st);
     // build_full_mg with warm_start (src/mg_builder.cpp:3-31, null iterations 5 when
     // warm): every level's near-null vectors iterated iters more times from their
     // current values on the current operator, then the aggregation and the Galerkin

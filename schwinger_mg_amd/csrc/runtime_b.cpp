@@ -57,6 +57,33 @@ template struct CoarseLinOpB<double, c64>;
 template struct CoarseLinOpB<float, c32>;
 template struct CoarseLinOpB<float, ch16>;
 
+// ------------------------------------------------------------------ Galerkin build (batched probing)
+void CoarseOperator::build_b(LinOpB<double>& A, const Aggregation& P, hipStream_t st) {
+    SMG_REQUIRE(P.A.nbx == nx && P.A.nby == ny && P.A.k == k, "coarse operator / aggregation mismatch");
+    SMG_REQUIRE(A.n() == P.fine_n(), "operator / aggregation mismatch");
+    const int N = A.N;
+    const size_t nc = n(), nf = P.fine_n();
+    launch_zero<double>(l64.p(), l64.n, st);
+    Field<double> sep(nc * N), eb(nc * N), fb(nf * N), afb(nf * N), cb(nc * N);
+    launch_zero<double>(sep.p(), nc * N, st);   // columns past the last probe stay zero
+    const int cx = color_period(nx), cy = color_period(ny);
+    const int total = cx * cy * k;   // probes (class, kj), class-major as in build
+    for (int p0 = 0; p0 < total; p0 += N) {
+        const int m = std::min(N, total - p0);
+        for (int q = 0; q < m; q++)
+            launch_probe_vector(nx, ny, k, cx, cy, (p0 + q) / k, (p0 + q) % k, sep.p((size_t)q * nc), st);
+        launch_b_pack<double, double>(eb.p(), sep.p(), nc, N, st);
+        launch_b_prolong<double>(P.A, P.bv64.p(), eb.p(), fb.p(), 0, N, st);
+        A.apply(afb.p(), fb.p(), st);
+        launch_b_restrict<double>(P.A, P.bv64.p(), afb.p(), cb.p(), N, st);
+        launch_b_unpack<double, double>(sep.p(), cb.p(), nc, N, st);
+        for (int q = 0; q < m; q++)
+            launch_probe_extract(nx, ny, k, cx, cy, (p0 + q) / k, (p0 + q) % k, sep.p((size_t)q * nc), l64.p(), st);
+    }
+    refresh_low_precision(st);
+    SMG_HIP(hipStreamSynchronize(st));
+}
+
 // ------------------------------------------------------------------ smoothers (batched)
 // MR, src/smoother.cpp:26-40, with the residual carried recursively; per-RHS omega
 template <typename T>
