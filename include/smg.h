@@ -188,6 +188,8 @@ typedef struct {
                               (formed once at setup; coarsest dimension <= 8192) */
     int use_graph;         /* 1: replay the cycle from a captured hipGraph */
     int chiral;            /* 1: chiral aggregation at every level (k[l] = 2 x null vectors) */
+    int null_prec;         /* near-null MR sweeps: 0 FP64 (the reference's); 1 FP32 on the FP32
+                              operator, vectors and orthogonalisation kept in FP64 */
 } smg_mg_params;
 /* build_mg_hierarchy, src/multigrid.cpp:595-687, with every level's null-space start
  * vectors supplied (k[l] vectors of level-l dimension, k[l]/2 with chiral

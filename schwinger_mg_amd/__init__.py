@@ -37,7 +37,7 @@ class MGParams(ctypes.Structure):
     _fields_ = [("n_levels", _I), ("bx", _I * 4), ("by", _I * 4), ("k", _I * 4), ("null_iters", _I),
                 ("pre_smooth", _I), ("post_smooth", _I), ("w_cycle", _I), ("coarse_maxit", _I),
                 ("coarse_tol", _D), ("coarse_half", _I), ("coarse_solver", _I), ("use_graph", _I),
-                ("chiral", _I)]
+                ("chiral", _I), ("null_prec", _I)]
 
 
 class SolveInfo(ctypes.Structure):
@@ -538,7 +538,7 @@ class MGHierarchy:
 
     def __init__(self, D, starts, n_levels=2, blocks=((4, 4),), k=(4,), null_iters=20, pre_smooth=3,
                  post_smooth=3, w_cycle=True, coarse_maxit=200, coarse_tol=1e-12, coarse_half=False,
-                 coarse_solver=0, use_graph=False, chiral=False):
+                 coarse_solver=0, use_graph=False, chiral=False, null_prec=F64):
         p = MGParams()
         p.n_levels = n_levels
         blocks = list(blocks) + [blocks[-1]] * (4 - len(blocks))
@@ -549,6 +549,7 @@ class MGHierarchy:
         p.null_iters, p.pre_smooth, p.post_smooth = null_iters, pre_smooth, post_smooth
         p.w_cycle, p.coarse_maxit, p.coarse_tol, p.coarse_half = int(w_cycle), coarse_maxit, coarse_tol, int(coarse_half)
         p.coarse_solver, p.use_graph, p.chiral = coarse_solver, int(use_graph), int(chiral)
+        p.null_prec = 0 if null_prec == F64 else 1
         self.params, self.D, self.n_levels = p, D, n_levels
         h = _P()
         _chk(lib().smg_mg_setup(ctypes.byref(h), D.h, ctypes.byref(p), _p(_c(starts))))

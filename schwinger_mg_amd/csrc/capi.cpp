@@ -590,6 +590,8 @@ int smg_mg_setup(smg_mg** out, smg_dirac* D, const smg_mg_params* prm, const dou
         c.coarse_solver = prm->coarse_solver;
         c.use_graph = prm->use_graph;
         c.chiral = prm->chiral;
+        c.null_prec = prm->null_prec;
+        SMG_REQUIRE(c.null_prec == 0 || c.null_prec == 1, "null_prec must be 0 or 1");
         SMG_REQUIRE(c.coarse_solver == 0 || c.coarse_solver == 1, "coarse_solver must be 0 or 1");
         SMG_REQUIRE(c.n_levels >= 2 && c.n_levels <= 5, "n_levels must be 2..5");
         for (int l = 0; l + 1 < c.n_levels; l++)

@@ -468,18 +468,30 @@ Cycle<float>& MG::cycle<float>() { return *cyc32; }
 
 LinOp<double>& MG::level_op64(int l) { return *cyc64->ops[l]; }
 
-// src/multigrid.cpp:545-587: inverse iteration with 5 MR sweeps per outer step
-static void null_space(LinOp<double>& A, int k, int outer, int inner, c64* vecs, hipStream_t st) {
+// src/multigrid.cpp:545-587: inverse iteration with 5 MR sweeps per outer step.
+// With T = float the MR sweeps run in FP32 on the FP32 operator (the vectors, their
+// orthogonalisation and normalisation stay FP64): the right-hand side is rounded to
+// FP32 before the sweeps and the iterate widened after them.
+template <typename T>
+static void null_space(LinOp<T>& A, int k, int outer, int inner, c64* vecs, hipStream_t st) {
     const size_t n = A.n();
-    Field<double> x(n), r(n), Ar(n), rhs(n);
+    Field<double> x(n), rhs(n);
+    const size_t nlow = std::is_same<T, double>::value ? 0 : n;   // FP32 copies of x and the rhs
+    Field<T> xs(nlow), r(n), Ar(n), rhss(nlow);
     Scalars sc(64);
     for (int i = 0; i < k; i++) {
         c64* v = vecs + (size_t)i * n;
         double nv = dev_norm(v, n, st);
         launch_axpby<double>(v, v, c64{0, 0}, c64{1.0 / nv, 0}, n, st);
         for (int it = 0; it < outer; it++) {
-            dcopy(rhs.p(), v, n, st);
-            mr_smooth<double>(A, x.p(), rhs.p(), r.p(), Ar.p(), inner, true, sc.p(), st);
+            if constexpr (std::is_same<T, double>::value) {
+                dcopy(rhs.p(), v, n, st);
+                mr_smooth<double>(A, x.p(), rhs.p(), r.p(), Ar.p(), inner, true, sc.p(), st);
+            } else {
+                launch_convert<T, double>(rhss.p(), v, n, st);
+                mr_smooth<T>(A, xs.p(), rhss.p(), r.p(), Ar.p(), inner, true, sc.p(), st);
+                launch_convert<double, T>(x.p(), xs.p(), n, st);
+            }
             for (int j = 0; j < i; j++) {
                 c64 pr = dev_dot<double>(vecs + (size_t)j * n, x.p(), n, st);
                 launch_axpby<double>(x.p(), vecs + (size_t)j * n, c64{-pr.re, -pr.im}, c64{1, 0}, n, st);
@@ -497,6 +509,20 @@ static void null_space(LinOp<double>& A, int k, int outer, int inner, c64* vecs,
         }
         double nv = dev_norm(v, n, st);
         if (nv > 1e-14) launch_axpby<double>(v, v, c64{0, 0}, c64{1.0 / nv, 0}, n, st);
+    }
+}
+
+// level l's near-null vectors by inverse iteration on A (level l's FP64 operator),
+// the MR sweeps in FP64 or, with null_prec = 1, in FP32 on the same operator's FP32
+// form (the fine FP32 D^dag D, or the FP32 links of the coarse operator of level l - 1)
+void MG::near_null(size_t l, LinOp<double>& A, int nv, int iters, c64* vecs, hipStream_t st) {
+    if (cfg.null_prec == 0) {
+        null_space<double>(A, nv, iters, cfg.null_inner, vecs, st);
+    } else if (l == 0) {
+        null_space<float>(*fine32, nv, iters, cfg.null_inner, vecs, st);
+    } else {
+        CoarseLinOp<float, c32> A32(*C[l - 1]);
+        null_space<float>(A32, nv, iters, cfg.null_inner, vecs, st);
     }
 }
 
@@ -530,7 +556,7 @@ MG::MG(const Dirac& d, const MGConfig& c, const c64* starts, bool on_device, boo
         SMG_HIP(hipMemcpyAsync(nullv.back()->p(), starts + soff, (size_t)nv * n * sizeof(c64),
                                on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
         soff += (size_t)nv * n;
-        null_space(*cur, nv, c.null_iters, c.null_inner, nullv.back()->p(), st);
+        near_null(l, *cur, nv, c.null_iters, nullv.back()->p(), st);
         P.emplace_back(new Aggregation(lnx[l], lny[l], lnd[l], c.bx[l], c.by[l], k, c.chiral));
         P.back()->build(nullv.back()->p(), st);
         C.emplace_back(new CoarseOperator(P.back()->A.nbx, P.back()->A.nby, k));
@@ -574,7 +600,7 @@ void MG::refresh(int iters, hipStream_t st) {
     std::vector<std::unique_ptr<LinOp<double>>> ops64;
     for (size_t l = 0; l < C.size(); l++) {
         const int nv = cfg.chiral ? cfg.k[l] / 2 : cfg.k[l];
-        if (iters > 0) null_space(*cur, nv, iters, cfg.null_inner, nullv[l]->p(), st);
+        if (iters > 0) near_null(l, *cur, nv, iters, nullv[l]->p(), st);
         P[l]->build(nullv[l]->p(), st);
         galerkin(l, st);
         ops64.emplace_back(new CoarseLinOp<double, c64>(*C[l]));

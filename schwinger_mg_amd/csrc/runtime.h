@@ -212,6 +212,7 @@ struct MGConfig {
     int coarse_solver = 0;   // coarsest level: 0 CG (coarse_maxit, coarse_tol), 1 dense inverse
     int use_graph = 0;       // replay the cycle from a captured hipGraph
     int chiral = 0;          // chiral aggregation: k[l] = 2 * (null vectors of level l)
+    int null_prec = 0;       // near-null MR sweeps: 0 FP64, 1 FP32 (vectors and orthogonalisation FP64)
 };
 
 template <typename T>
@@ -261,6 +262,7 @@ struct MG {
     // MGHierarchy::rebuild_deeper_levels, src/multigrid.cpp:163-194)
     void rebuild(hipStream_t st);
     void galerkin(size_t level, hipStream_t st);
+    void near_null(size_t level, LinOp<double>& A, int nv, int iters, c64* vecs, hipStream_t st);
     // build_full_mg with warm_start (src/mg_builder.cpp:3-31, null iterations 5 when
     // warm): every level's near-null vectors iterated iters more times from their
     // current values on the current operator, then the aggregation and the Galerkin

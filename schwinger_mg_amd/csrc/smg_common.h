@@ -250,7 +250,14 @@ __device__ __forceinline__ void part_store(double* p, double v) {
 __device__ __forceinline__ double part_load(const double* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void stores_complete() { __builtin_amdgcn_s_waitcnt(0); }
+// s_waitcnt orders the hardware, but the compiler models it as touching no memory, so
+// compiler-only fences on both sides keep the partial stores before it and the
+// counter increment after it (no cache writeback is emitted for a signal fence).
+__device__ __forceinline__ void stores_complete() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 __device__ __forceinline__ unsigned counter_arrive(unsigned* c) {
     return __hip_atomic_fetch_add(c, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

@@ -79,3 +79,16 @@ def test_files_byte_identical_to_the_reference(smg, tmp_path):
     Ug, hg = smg.load_gauge(L, b)
     assert np.array_equal(Ur, Ug) and hr == (hg.beta, hg.mass, hg.avg_plaq)
     assert R.gauge_load(L + 2, a) is None
+
+
+def test_reference_written_fixtures(smg, tmp_path):
+    """The committed fixtures written by the reference (tests/golden/make_gauge_files.py)
+    equal this library's files for the same field."""
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    L = 8
+    U = random_gauge(L, 0.6, 5)
+    a, b = str(tmp_path / "a.bin"), str(tmp_path / "b.bin")
+    smg.gauge_save_raw(L, U, a)
+    smg.save_gauge(L, U, 4.0, 0.1, b)
+    assert open(a, "rb").read() == open(os.path.join(gold, "gauge_L8_raw.bin"), "rb").read()
+    assert open(b, "rb").read() == open(os.path.join(gold, "gauge_L8_swng.bin"), "rb").read()

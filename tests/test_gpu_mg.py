@@ -272,3 +272,31 @@ def test_graph_replay_equals_eager(smg, coarse_solver):
                 got = mg.precondition(b, prec)
                 D.apply(b)            # unrelated eager work between replays
                 assert relerr(got, want) <= tol + 1e-14, (prec, rep)
+
+
+def test_coarse_apply_strided_large_lattice(smg):
+    """L = 128, 4 x 4 aggregates, k = 32 (16 null vectors, chiral): 32 x 32 coarse sites
+    of 32 x 32 link entries per direction, 1,048,576 lanes against the 2048 x 256 of one
+    pass, so the single-RHS apply's workgroups take two strides and its fused dot
+    products (the coarse CG's) finish over the rows each workgroup accumulated."""
+    banner("strided single-RHS coarse apply L=128 b=4 k=32")
+    L, b, k = 128, 4, 32
+    U = random_gauge(L, 0.4, 90)
+    Po = O.Prolongator(L, L, 2, b, b, k, chiral=True).build(gaussian_vecs(k // 2, 2 * L * L, 91).ravel())
+    Co = O.Coarse(O.Dirac(L, U, 0.05, 1.0, 1.0, 0.0).op(), Po, 1)
+    C = smg.SparseCoarseOp.from_links(L // b, L // b, k, Co.links())
+    assert C.dim * k > 2048 * 256
+    A = Co.op()
+    c = gaussian_vec(C.dim, 92)
+    want = A(c)
+    assert relerr(C.apply_to(c, smg.F64), want) < 1e-12
+    assert relerr(C.apply_to(c, smg.F32), want) < 1e-5
+    assert relerr(C.apply_to(c, smg.F16), want) < 5e-3
+    x = C.solve(c, 300, 1e-10)
+    xo, ito = O.coarse_cg(A, c, 300, 1e-10)
+    print(f"coarse CG: {C.last_iterations} its (oracle {ito}), relerr {relerr(x, xo):.2e}")
+    assert abs(C.last_iterations - ito) <= 1
+    assert relerr(x, xo) < 1e-8
+    # both stop at the iteration cap on this ill-conditioned operator: equal residuals
+    rg, ro = (np.linalg.norm(c - A(v)) / np.linalg.norm(c) for v in (x, xo))
+    assert abs(rg - ro) < 1e-3 * ro
