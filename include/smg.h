@@ -114,6 +114,16 @@ int smg_eo_logdet(smg_dirac* D, double* out);
 int smg_dot_dev(const void* a, const void* b, size_t n, int prec, double* out2, void* stream);
 /* norm (src/linalg.cpp:20) */
 int smg_norm_dev(const void* a, size_t n, int prec, double* out, void* stream);
+/* No counterpart in the reference: smg_dot_dev for callers that cannot allocate
+ * (code under stream capture up to the final read-back, tight serving loops).  It
+ * calls no hipMalloc/hipFree: the reduction's partial rows (at most 1024
+ * workgroups), its completion counter and its result live in a 16.5 KiB file-scope
+ * device array that each GPU holds until the process ends.  Kernels and the
+ * read-back run on `stream` itself (NULL: the null stream), which is synchronised
+ * before the call returns; the result is bit-identical to smg_dot_dev's.  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (other host
+ * threads, other streams) race and must not be issued. */
+int smg_dot_dev_noalloc(const void* a, const void* b, size_t n, int prec, double* out2, void* stream);
 
 /* ------------------------------------------------------------------ prolongator */
 /* Prolongator(lat, bx, by, k), src/prolongator.h:35-58, generalised to nd dof per

@@ -74,6 +74,7 @@ SIGNATURES = [
     ("smg_eo_apply", _I, [_P, _I, _P, _P]), ("smg_eo_apply_dev", _I, [_P, _I, _I, _P, _P, _P]),
     ("smg_eo_reconstruct", _I, [_P, _P, _P, _P]), ("smg_eo_logdet", _I, [_P, _P]),
     ("smg_dot_dev", _I, [_P, _P, _SZ, _I, _P, _P]), ("smg_norm_dev", _I, [_P, _SZ, _I, _P, _P]),
+    ("smg_dot_dev_noalloc", _I, [_P, _P, _SZ, _I, _P, _P]),
     ("smg_prolongator_create", _I, [_P, _I, _I, _I, _I, _I, _I, _I]), ("smg_prolongator_destroy", _I, [_P]),
     ("smg_prolongator_build", _I, [_P, _P]), ("smg_prolongator_get", _I, [_P, _P]),
     ("smg_prolongator_set", _I, [_P, _P]), ("smg_prolong", _I, [_P, _P, _P]), ("smg_restrict", _I, [_P, _P, _P]),

@@ -113,6 +113,7 @@ int smg_init(int device) {
     return guard([&] {
         SMG_HIP(hipSetDevice(device));
         Context::get();
+        noalloc_scratch();   // this device's address of the allocation-free entry's scratch
     });
 }
 const char* smg_last_error(void) { return g_err.c_str(); }
@@ -350,6 +351,13 @@ int smg_dot_dev(const void* a, const void* b, size_t n, int prec, double* out2, 
                                 : dev_dot<float>((const c32*)a, (const c32*)b, n, S());
         out2[0] = d.re;
         out2[1] = d.im;
+    });
+}
+int smg_dot_dev_noalloc(const void* a, const void* b, size_t n, int prec, double* out2, void* stream) {
+    return guard([&] {
+        SMG_REQUIRE(a && b && out2, "null argument");
+        check_prec(prec);
+        noalloc_dot(a, b, n, prec == SMG_F64 ? 1 : 0, out2, reinterpret_cast<hipStream_t>(stream));
     });
 }
 int smg_norm_dev(const void* a, size_t n, int prec, double* out, void* stream) {

@@ -202,4 +202,10 @@ void launch_probe_vector(int nbx, int nby, int k, int cx, int cy, int cls, int k
 void launch_probe_extract(int nbx, int nby, int k, int cx, int cy, int cls, int kj, const c64* col, c64* link,
                           hipStream_t st);
 
+// allocation-free dot product (noalloc.hip): partials, counter and result in a
+// file-scope device array; noalloc_scratch() is the current device's address of it
+double* noalloc_scratch();
+size_t noalloc_dot_rows(size_t n);
+void noalloc_dot(const void* a, const void* b, size_t n, int f64, double* out2, hipStream_t st);
+
 }  // namespace smg
