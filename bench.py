@@ -86,6 +86,8 @@ def parse():
                     help="hmc: refresh the hierarchy every N outer position updates (0: Galerkin rebuild between "
                          "trajectories only)")
     ap.add_argument("--mg-refresh-iters", type=int, default=2, help="hmc: near-null iterations per refresh")
+    ap.add_argument("--chrono", type=int, default=1,
+                    help="hmc: every solve after the first starts from the previous solution")
     ap.add_argument("--lobpcg", type=int, default=16,
                     help="hmc: low modes of D^dag D tracked by LOBPCG (one block) after every trajectory (0: none)")
     return ap.parse_args()
@@ -271,7 +273,8 @@ def hmc_workload(args):
     setup_s = time.time() - t0
     log(f"MG setup {setup_s:.2f} s")
     p = smg.nested_params(args.beta, 1.0, args.n_outer, args.n_gauge, True, True, smg.F32, 500, 1e-10,
-                          args.inner_tol, mg_refresh=args.mg_refresh, mg_refresh_iters=args.mg_refresh_iters)
+                          args.inner_tol, mg_refresh=args.mg_refresh, mg_refresh_iters=args.mg_refresh_iters,
+                          chrono_guess=bool(args.chrono))
     X = None
     if args.lobpcg:   # the low modes, tracked as one block of right-hand sides
         t0 = time.time()
@@ -289,15 +292,15 @@ def hmc_workload(args):
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         res = smg.hmc_trajectory_nested(D, p, pi0, eta, u, mg=mg)
-        if args.mg_refresh:   # the hierarchy followed the proposal; bring it to the kept field
-            mg.refresh(args.mg_refresh_iters)
-        else:
+        if not args.mg_refresh:
             mg.rebuild()
+        elif res.accepted:   # a rejected trajectory has already refreshed on the kept field
+            mg.refresh(args.mg_refresh_iters)
         smg.synchronize()
         dt = time.perf_counter() - t1
         plaq = smg.avg_plaq(L, smg.gauge_of(D))
         rec = {"dH": res.dH, "accepted": res.accepted, "plaquette": plaq, "solver_iters": res.total_cg_iters,
-               "s": round(dt, 4)}
+               "max_true_residual": res.max_true_residual, "s": round(dt, 4)}
         if X is not None:   # warm-started from the previous trajectory's modes
             torch.cuda.synchronize()
             t1 = time.perf_counter()

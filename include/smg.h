@@ -413,6 +413,9 @@ typedef struct {
     double dH, dKE, dSG, dSF;
     int total_cg_iters;
     double dLD;        /* log-det change (even-odd with clover, else 0) */
+    double max_true_residual;   /* no reference counterpart: the largest FP64 true residual
+                                   |phi - A X|/|phi| of the trajectory's solves (a solve that stopped
+                                   at its iteration budget shows here) */
 } smg_hmc_result;      /* HMCResult, src/hmc.h:120-128 */
 /* the operator's current gauge field (host, 2V complex) */
 int smg_dirac_get_gauge(smg_dirac* D, double* U);
@@ -491,7 +494,9 @@ int smg_hmc_ms_trajectory(smg_dirac* D, smg_mg* mg, smg_prolongator* P, smg_coar
  * cycle when mg is given (solver = SMG_F64), or mixed-precision GCR(10) with mg's FP32
  * cycle inside FP64 refinement on the FP64 true residual (solver = SMG_F32).  With
  * mg_refresh > 0 the hierarchy follows the evolving field: after every mg_refresh-th
- * outer position update it is refreshed as by smg_mg_refresh(mg, mg_refresh_iters). */
+ * outer position update it is refreshed as by smg_mg_refresh(mg, mg_refresh_iters), and
+ * a rejected trajectory restores the near-null vectors it started from and rebuilds the
+ * aggregations and Galerkin operators on the restored field. */
 typedef struct {
     double beta, tau;
     int n_outer;        /* outer steps (fermion force) */
@@ -505,6 +510,7 @@ typedef struct {
     int force_accept;
     int mg_refresh;     /* refresh the hierarchy every N outer position updates (0: never) */
     int mg_refresh_iters;
+    int chrono_guess;   /* 1: every solve after the first starts from the previous solution */
 } smg_nested_params;
 /* the molecular dynamics from momenta pi (2V reals, in/out) and phi (2V complex); the
  * gauge field of D evolves in place; *solver_iters = the iterations of all solves */
@@ -570,6 +576,32 @@ int smg_gauge_write(const char* path, int L, const double* U, double beta, doubl
 int smg_gauge_read(const char* path, int L, double* U, smg_gauge_header* hdr);
 /* therm_cfg_path, src/hmc_utils.cpp:7-11: "gauge_L<L>_b<beta as %f, 4 chars>_t<n>.bin" */
 int smg_therm_cfg_path(int L, double beta, int n_therm, char* out, size_t cap);
+
+/* ------------------------------------------------------------------ launch-shape autotuner
+ * No reference counterpart.  A per-process table keeps, per kernel family and shape,
+ * the variant and the reducing-launch workgroup count a timing search picked.  The
+ * workgroup count fixes a reduction's summation order, so runs with the same table
+ * give bit-identical results; shapes without an entry use the built-in defaults.
+ * Families (shape words a..e, choice):
+ *   SMG_TUNE_COARSE_BF16: the FP16-link matrix-core coarse apply; (nx, ny, k, nrhs,
+ *     4*bf16_vectors + epilogue[0 none, 1 residual, 2 dots]); choice = herm + 2*g, herm = 1
+ *     reads 5 of the 9 link blocks of a site (Hermitian Galerkin operators), g indexes the
+ *     workgroup counts {128, 256, 512, 1024, 2048} of the launch with dot products;
+ *   SMG_TUNE_DSLASH_DOTS: the batched stencil with dot products; (L, nrhs, bytes per
+ *     real, 0, 0); choice = g. */
+#define SMG_TUNE_COARSE_BF16 0
+#define SMG_TUNE_DSLASH_DOTS 1
+/* times the candidates of every kernel of mg's batched cycle for nrhs (16, 32 or 64)
+ * right-hand sides and keeps the fastest; *entries = table entries added (may be NULL) */
+int smg_tune_mg(smg_mg* mg, int nrhs, int* entries);
+/* write / read the table (text, one "family a b c d e choice" line per entry; reading
+ * adds to the table, a malformed line is an error and adds nothing) */
+int smg_tune_save(const char* path);
+int smg_tune_load(const char* path);
+int smg_tune_clear(void);
+/* the choice stored for (family, shape[5]) in *choice, -1 if none; set one by hand */
+int smg_tune_get(int family, const int* shape, int* choice);
+int smg_tune_set(int family, const int* shape, int choice);
 
 #ifdef __cplusplus
 }

@@ -58,7 +58,7 @@ class HMCParams(ctypes.Structure):
 class HMCResult(ctypes.Structure):
     """HMCResult (src/hmc.h:120-128)."""
     _fields_ = [("accepted", _I), ("dH", _D), ("dKE", _D), ("dSG", _D), ("dSF", _D), ("total_cg_iters", _I),
-                ("dLD", _D)]
+                ("dLD", _D), ("max_true_residual", _D)]
 
 
 # (name, restype, argtypes) of every entry point declared in include/smg.h
@@ -74,6 +74,9 @@ SIGNATURES = [
     ("smg_eo_apply", _I, [_P, _I, _P, _P]), ("smg_eo_apply_dev", _I, [_P, _I, _I, _P, _P, _P]),
     ("smg_eo_reconstruct", _I, [_P, _P, _P, _P]), ("smg_eo_logdet", _I, [_P, _P]),
     ("smg_dot_dev", _I, [_P, _P, _SZ, _I, _P, _P]), ("smg_norm_dev", _I, [_P, _SZ, _I, _P, _P]),
+    ("smg_tune_mg", _I, [_P, _I, _P]), ("smg_tune_save", _I, [ctypes.c_char_p]),
+    ("smg_tune_load", _I, [ctypes.c_char_p]), ("smg_tune_clear", _I, []), ("smg_tune_get", _I, [_I, _P, _P]),
+    ("smg_tune_set", _I, [_I, _P, _I]),
     ("smg_dot_dev_noalloc", _I, [_P, _P, _SZ, _I, _P, _P]),
     ("smg_prolongator_create", _I, [_P, _I, _I, _I, _I, _I, _I, _I]), ("smg_prolongator_destroy", _I, [_P]),
     ("smg_prolongator_build", _I, [_P, _P]), ("smg_prolongator_get", _I, [_P, _P]),
@@ -895,14 +898,14 @@ class NestedParams(ctypes.Structure):
     outer scale, n_gauge inner gauge-force steps per outer position update."""
     _fields_ = [("beta", _D), ("tau", _D), ("n_outer", _I), ("n_gauge", _I), ("omelyan_outer", _I),
                 ("omelyan_inner", _I), ("solver", _I), ("cg_maxiter", _I), ("cg_tol", _D), ("inner_tol", _D),
-                ("force_accept", _I), ("mg_refresh", _I), ("mg_refresh_iters", _I)]
+                ("force_accept", _I), ("mg_refresh", _I), ("mg_refresh_iters", _I), ("chrono_guess", _I)]
 
 
 def nested_params(beta, tau=1.0, n_outer=1, n_gauge=8, omelyan_outer=True, omelyan_inner=True, solver=F64,
                   cg_maxiter=1000, cg_tol=1e-10, inner_tol=1e-5, force_accept=False, mg_refresh=0,
-                  mg_refresh_iters=2):
+                  mg_refresh_iters=2, chrono_guess=False):
     return NestedParams(beta, tau, n_outer, n_gauge, int(omelyan_outer), int(omelyan_inner), solver, cg_maxiter,
-                        cg_tol, inner_tol, int(force_accept), mg_refresh, mg_refresh_iters)
+                        cg_tol, inner_tol, int(force_accept), mg_refresh, mg_refresh_iters, int(chrono_guess))
 
 
 def hmc_nested_evolve(D, params, pi, phi, mg=None):
@@ -1107,3 +1110,41 @@ def therm_cfg_path(L, beta, n_therm):
     buf = ctypes.create_string_buffer(256)
     _hchk(load_library().smg_therm_cfg_path(L, beta, n_therm, buf, 256))
     return buf.value.decode()
+
+
+# ------------------------------------------------------------------ launch-shape autotuner
+TUNE_COARSE_BF16, TUNE_DSLASH_DOTS = 0, 1
+
+
+def tune_mg(mg, nrhs):
+    """Time the launch shapes of mg's batched cycle for nrhs right-hand sides and keep
+    the fastest per shape; returns the number of table entries added."""
+    n = _I()
+    _chk(lib().smg_tune_mg(mg.h, nrhs, ctypes.byref(n)))
+    return n.value
+
+
+def tune_save(path):
+    """The table is host memory: these calls need no GPU."""
+    _hchk(load_library().smg_tune_save(str(path).encode()))
+
+
+def tune_load(path):
+    _hchk(load_library().smg_tune_load(str(path).encode()))
+
+
+def tune_clear():
+    _hchk(load_library().smg_tune_clear())
+
+
+def tune_get(family, shape):
+    sh = np.ascontiguousarray(shape, np.int32)
+    c = _I()
+    _hchk(load_library().smg_tune_get(family, _p(sh), ctypes.byref(c)))
+    return c.value
+
+
+def tune_set(family, shape, choice):
+    sh = np.ascontiguousarray(shape, np.int32)
+    _hchk(load_library().smg_tune_set(family, _p(sh), choice))
+

@@ -10,6 +10,7 @@
 #include <type_traits>
 
 #include "smg_batched.h"
+#include "tune.h"
 
 namespace smg {
 
@@ -307,9 +308,12 @@ static void dslash_p(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T 
     else if (epi == EPI_RESID)
         k_b_dslash<T, EPI_RESID, P><<<b_grid(items, 1 << 16), kBlock, 0, st>>>(D, dst, src, mu, g5in, g5out, b, aux,
                                                                              fin, N);
-    else
-        k_b_dslash<T, EPI_DOTS, P><<<b_grid(items, kRedGrid), kBlock, 0, st>>>(D, dst, src, mu, g5in, g5out, b, aux,
-                                                                             fin, N);
+    else {
+        const int choice = tune_lookup(TuneKey{TUNE_DSLASH_DOTS, D.L, N, (int)sizeof(T), 0, 0});
+        const int g = choice < 0 ? kRedGrid : kTuneGrids[choice];
+        k_b_dslash<T, EPI_DOTS, P><<<b_grid(items, g), kBlock, 0, st>>>(D, dst, src, mu, g5in, g5out, b, aux, fin,
+                                                                      N);
+    }
 }
 
 template <typename T>
@@ -670,12 +674,18 @@ __device__ __forceinline__ c32 unpack_h(unsigned w) {
 // sources and links in flight while chunk m is split and multiplied.  HV: source,
 // destination and the dot partner are BF16 vectors (cb16 entries), so the source
 // needs no split (two MFMAs per product instead of three); EPI_RESID is FP32 only.
-template <typename TL, int EPI, int K, int NT, bool HV = false>
+// HERM (C.herm set, K >= 16 so that a chunk lies in one direction): the blocks of the
+// backward directions are read as the conjugate transposes of the neighbours' forward
+// blocks, L[b][d] = L[nbr(b,d)][rev d]^dag, so a site reads 5 of its 9 link blocks and
+// the links' HBM traffic falls to 5/9; a lane then gathers its four link columns of a
+// row as four dwords of one column of the stored block instead of one 16-byte run.
+template <typename TL, int EPI, int K, int NT, bool HV = false, bool HERM = false>
 __global__ __launch_bounds__(kBlock)
 void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __restrict__ srcv,
                      const c32* __restrict__ bvec, const void* __restrict__ auxv, FinB fin) {
     static_assert(std::is_same<TL, ch16>::value, "bf16x3 path: FP16 links");
     static_assert(!HV || EPI != EPI_RESID, "BF16 vectors: no residual epilogue");
+    static_assert(!HERM || K % 16 == 0, "Hermitian reads: a chunk of 16 columns inside one direction");
     c32* __restrict__ dst = static_cast<c32*>(dstv);
     const c32* __restrict__ src = static_cast<const c32*>(srcv);
     const c32* __restrict__ aux = static_cast<const c32*>(auxv);
@@ -712,6 +722,7 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
         struct Raw {
             SrcV s[4][NL];
             uint4 l[RT];
+            unsigned cj;   // sign bit flipped into Im L (a conjugated, transposed block)
         };
         auto load_raw = [&](Raw& R, int m) {
             const int cc0 = 16 * m + 4 * hi;
@@ -731,9 +742,19 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                     else R.s[e2][u] = *reinterpret_cast<const SrcV*>(src + o);
                 }
             }
-            const unsigned* lrow = links + ((size_t)b * 9 + d) * (K * K) + a_ki * K + kj0;
+            const bool bwd = HERM && (d == 2 || d == 4 || d == 7 || d == 8);
+            R.cj = bwd ? 0x80000000u : 0u;
+            if (bwd) {   // L[b][d][ki][kj] = conj(L[nb][rev d][kj][ki])
+                const int rd = d == 2 ? 1 : d == 4 ? 3 : d == 7 ? 6 : 5;
+                const unsigned* lcol = links + ((size_t)nb * 9 + rd) * (K * K) + (size_t)kj0 * K + a_ki;
 #pragma unroll
-            for (int r = 0; r < RT; r++) R.l[r] = *reinterpret_cast<const uint4*>(lrow + 8 * r * K);
+                for (int r = 0; r < RT; r++)
+                    R.l[r] = uint4{lcol[8 * r], lcol[K + 8 * r], lcol[2 * K + 8 * r], lcol[3 * K + 8 * r]};
+            } else {
+                const unsigned* lrow = links + ((size_t)b * 9 + d) * (K * K) + a_ki * K + kj0;
+#pragma unroll
+                for (int r = 0; r < RT; r++) R.l[r] = *reinterpret_cast<const uint4*>(lrow + 8 * r * K);
+            }
             if (!valid) {   // the partial last chunk (k = 8): these columns do not exist
 #pragma unroll
                 for (int e2 = 0; e2 < 4; e2++)
@@ -785,7 +806,8 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                 for (int e2 = 0; e2 < 4; e2++) {
                     const float re = __half2float(__ushort_as_half((unsigned short)(lws[e2] & 0xffffu)));
                     const float im = __uint_as_float(
-                        __float_as_uint(__half2float(__ushort_as_half((unsigned short)(lws[e2] >> 16)))) ^ neg_im);
+                        __float_as_uint(__half2float(__ushort_as_half((unsigned short)(lws[e2] >> 16)))) ^ neg_im ^
+                        R.cj);
                     ah[e2] = pack_bf(re, im, sel_a);
                     al[e2] = pack_bf(lo_part(re), lo_part(im), sel_a);
                 }
@@ -944,18 +966,33 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
     }
 }
 
+template <typename TL, int K, int NT, bool HV, bool HERM>
+static void coarse_bf16_v(const CoarseView<TL>& C, void* dst, const void* src, Epi epi, const c32* b,
+                          const void* aux, const FinB& fin, int gdots, hipStream_t st) {
+    const size_t waves = (size_t)C.nx * C.ny;
+    const int nw = kBlock / 64;
+    const int g = (int)std::min<size_t>((waves + nw - 1) / nw, epi == EPI_DOTS ? gdots : 1 << 16);
+    if (epi == EPI_NONE)
+        k_b_coarse_bf16<TL, EPI_NONE, K, NT, HV, HERM><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
+    else if (epi == EPI_DOTS)
+        k_b_coarse_bf16<TL, EPI_DOTS, K, NT, HV, HERM><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
+    else if constexpr (!HV)
+        k_b_coarse_bf16<TL, EPI_RESID, K, NT, false, HERM><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
+}
+// The tuner's choice for this shape, else the defaults: the Hermitian reads whenever
+// the links allow them, kRedGridCoarse workgroups with fused dot products.
 template <typename TL, int K, int NT, bool HV = false>
 static void coarse_bf16(const CoarseView<TL>& C, void* dst, const void* src, Epi epi, const c32* b, const void* aux,
                         const FinB& fin, hipStream_t st) {
-    const size_t waves = (size_t)C.nx * C.ny;
-    const int nw = kBlock / 64;
-    const int g = (int)std::min<size_t>((waves + nw - 1) / nw, epi == EPI_DOTS ? kRedGridCoarse : 1 << 16);
-    if (epi == EPI_NONE)
-        k_b_coarse_bf16<TL, EPI_NONE, K, NT, HV><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
-    else if (epi == EPI_DOTS)
-        k_b_coarse_bf16<TL, EPI_DOTS, K, NT, HV><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
-    else if constexpr (!HV)
-        k_b_coarse_bf16<TL, EPI_RESID, K, NT><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
+    const int choice = tune_lookup(TuneKey{TUNE_COARSE_BF16, C.nx, C.ny, K, 16 * NT, (HV ? 4 : 0) + (int)epi});
+    const int gdots = choice < 0 ? kRedGridCoarse : kTuneGrids[choice >> 1];
+    if constexpr (K % 16 == 0) {
+        if (C.herm && (choice < 0 || (choice & 1))) {
+            coarse_bf16_v<TL, K, NT, HV, true>(C, dst, src, epi, b, aux, fin, gdots, st);
+            return;
+        }
+    }
+    coarse_bf16_v<TL, K, NT, HV, false>(C, dst, src, epi, b, aux, fin, gdots, st);
 }
 
 template <typename T, typename TL, int K, int NT>

@@ -13,6 +13,7 @@
 #include "runtime.h"
 #include "smg_lattices.h"
 #include "runtime_b.h"
+#include "tune.h"
 
 using namespace smg;
 
@@ -833,7 +834,7 @@ int smg_coarse_apply_batched(smg_coarse* C, int prec, int nrhs, const double* sr
             DevMem sh(nN * sizeof(cb16)), dh(nN * sizeof(cb16));
             launch_b_pack<float, double>(sb.p(), s.p(), n, nrhs, S());
             launch_h_from_f32(sh.as<cb16>(), sb.p(), n, nrhs, S());
-            launch_b_coarse_apply_h(CoarseView<ch16>{C->C->l16.as<ch16>(), C->C->nx, C->C->ny, C->C->k},
+            launch_b_coarse_apply_h(C->C->view16(),
                                     dh.as<cb16>(), sh.as<cb16>(), EPI_NONE, nullptr, FinB(), nrhs, S());
             launch_h_to_f32(db.p(), dh.as<cb16>(), n, nrhs, S());
             launch_b_unpack<double, float>(d.p(), db.p(), n, nrhs, S());
@@ -1069,6 +1070,7 @@ struct MDState {
     size_t V, nphi;
     DevMem pi, gf, ff;
     Field<double> phi, X, Y;
+    double max_tres = 0.0;   // the largest FP64 true residual |phi - A X|/|phi| of the solves
     MDState(smg_dirac* d, bool eo_, const double* pi_h, const double* phi_h)
         : D(d), eo(eo_), V((size_t)d->D->V), nphi(eo_ ? (size_t)d->D->V : 2 * (size_t)d->D->V),
           pi(2 * V * sizeof(double)), gf(2 * V * sizeof(double)), ff(2 * V * sizeof(double)), phi(nphi), X(nphi),
@@ -1086,6 +1088,7 @@ struct MDState {
         const Dirac& d = *D->D;
         launch_gauge_force(d.L, d.U64.p(), beta, gf.as<double>(), S());
         const SolveResult r = cg_solve(op(), phi.p(), X.p(), nullptr, nullptr, maxit, tol, S());
+        max_tres = std::max(max_tres, true_residual(op(), phi.p(), X.p(), S()));
         if (!eo) {
             fermion_force_dev(D, X.p(), ff.as<double>(), S());
         } else {
@@ -1104,6 +1107,7 @@ struct MDState {
         launch_plaq_sum(d.L, d.U64.p(), sg.fin(), S());
         out[1] = -beta * sg.read();
         const SolveResult r = cg_solve(op(), phi.p(), X.p(), nullptr, nullptr, maxit, tol, S());
+        max_tres = std::max(max_tres, true_residual(op(), phi.p(), X.p(), S()));
         out[2] = dev_dot<double>(phi.p(), X.p(), nphi, S()).re;
         out[3] = 0.0;
         if (eo && d.csw != 0.0) {
@@ -1213,6 +1217,7 @@ int smg_hmc_trajectory(smg_dirac* D, const smg_hmc_params* p, const double* pi0,
         res->dSF = h1[2] - h0[2];
         res->dLD = h1[3] - h0[3];
         res->total_cg_iters = cg;
+        res->max_true_residual = md.max_tres;
     });
 }
 
@@ -1419,17 +1424,25 @@ struct NestedState : MDState {
     const smg_nested_params& p;
     DevMem zero;
     int iters = 0, solves = 0;
+    bool have_x = false;      // X holds a solution on an earlier field (the chronological guess)
+    double max_tres = 0.0;    // the largest FP64 true residual of the trajectory's solves
     NestedState(smg_dirac* d, smg_mg* m, const smg_nested_params& p_, const double* pi_h, const double* phi_h)
         : MDState(d, false, pi_h, phi_h), mgh(m), p(p_), zero(2 * V * sizeof(double)) {
         SMG_HIP(hipMemsetAsync(zero.p, 0, 2 * V * sizeof(double), S()));
     }
     void solve() {
         SolveResult r;
-        if (p.solver == SMG_F32)
-            r = gcr_dev(D, mgh, SMG_F32, phi.p(), X.p(), p.cg_maxiter, p.cg_tol, 10, 0, p.inner_tol);
-        else
-            r = cg_solve(*D->op64, phi.p(), X.p(), nullptr, mgh ? mgh->mg.get() : nullptr, p.cg_maxiter, p.cg_tol,
-                         S());
+        const bool guess = p.chrono_guess && have_x;
+        if (p.solver == SMG_F32) {
+            r = gcr_solve_mixed(*D->op64, *D->op32, phi.p(), X.p(), mgh->mg.get(), p.cg_maxiter, p.cg_tol, 10,
+                                p.inner_tol, S(), guess);
+        } else {
+            r = cg_solve(*D->op64, phi.p(), X.p(), guess ? X.p() : nullptr, mgh ? mgh->mg.get() : nullptr,
+                         p.cg_maxiter, p.cg_tol, S());
+            r.true_residual = true_residual(*D->op64, phi.p(), X.p(), S());
+        }
+        have_x = true;
+        max_tres = std::max(max_tres, r.true_residual);
         iters += r.iterations;
         solves++;
     }
@@ -1533,6 +1546,14 @@ int smg_hmc_nested_trajectory(smg_dirac* D, smg_mg* mg, const smg_nested_params*
         const size_t V = (size_t)d.V;
         Field<double> saved(2 * V);
         launch_copy<double>(saved.p(), d.U64.p(), 2 * V, S());
+        // a hierarchy that follows the proposal keeps its near-null vectors for a rejection
+        std::vector<Field<double>> saved_null;
+        const bool follows = p->mg_refresh > 0 && mg;
+        if (follows)
+            for (auto& v : mg->mg->nullv) {
+                saved_null.emplace_back(v->n);
+                launch_copy<double>(saved_null.back().p(), v->p(), v->n, S());
+            }
         NestedState ns(D, mg, *p, pi0, nullptr);
         Field<double> e = up(eta, 2 * V);   // phi = D^dag eta
         launch_dslash<double>(d.v64(), ns.phi.p(), e.p(), -d.mu_t, 1, 1, EPI_NONE, nullptr, nullptr, Fin(), 1, 2 * V,
@@ -1546,6 +1567,14 @@ int smg_hmc_nested_trajectory(smg_dirac* D, smg_mg* mg, const smg_nested_params*
         if (!accept) {
             launch_copy<double>(d.U64.p(), saved.p(), 2 * V, S());
             d.refresh_derived(S());
+            // the hierarchy followed the rejected proposal: its near-null vectors come back and
+            // the aggregations and Galerkin operators are rebuilt on the kept field, which
+            // restores the hierarchy the trajectory started from
+            if (follows) {
+                for (size_t l = 0; l < saved_null.size(); l++)
+                    launch_copy<double>(mg->mg->nullv[l]->p(), saved_null[l].p(), saved_null[l].n, S());
+                mg->mg->refresh(0, S());
+            }
         }
         SMG_HIP(hipStreamSynchronize(S()));
         res->accepted = accept ? 1 : 0;
@@ -1555,6 +1584,7 @@ int smg_hmc_nested_trajectory(smg_dirac* D, smg_mg* mg, const smg_nested_params*
         res->dSF = h1[2] - h0[2];
         res->dLD = 0.0;
         res->total_cg_iters = ns.iters;
+        res->max_true_residual = ns.max_tres;
     });
 }
 
@@ -2049,5 +2079,41 @@ int smg_mg_coarsest_deflation(smg_mg* mg, int* ndefl, double* evals, double* eve
         if (ndefl) *ndefl = nd;
         if (evals) std::copy(m.defl_vals.begin(), m.defl_vals.end(), evals);
         if (evecs && nd) downc(evecs, m.defl_vecs.p(), (size_t)nd * m.dims.back());
+    });
+}
+
+// ------------------------------------------------------------------ launch-shape autotuner
+int smg_tune_mg(smg_mg* mg, int nrhs, int* entries) {
+    return guard([&] {
+        SMG_REQUIRE(mg, "null handle");
+        const int n = tune_mg(*mg->mg, nrhs, S());
+        if (entries) *entries = n;
+    });
+}
+int smg_tune_save(const char* path) {
+    return guard([&] {
+        SMG_REQUIRE(path, "null path");
+        tune_save(path);
+    });
+}
+int smg_tune_load(const char* path) {
+    return guard([&] {
+        SMG_REQUIRE(path, "null path");
+        tune_load(path);
+    });
+}
+int smg_tune_clear(void) {
+    return guard([&] { tune_clear(); });
+}
+int smg_tune_get(int family, const int* shape, int* choice) {
+    return guard([&] {
+        SMG_REQUIRE(shape && choice, "null argument");
+        *choice = tune_lookup(TuneKey{family, shape[0], shape[1], shape[2], shape[3], shape[4]});
+    });
+}
+int smg_tune_set(int family, const int* shape, int choice) {
+    return guard([&] {
+        SMG_REQUIRE(shape, "null argument");
+        tune_store(TuneKey{family, shape[0], shape[1], shape[2], shape[3], shape[4]}, choice);
     });
 }

@@ -309,6 +309,29 @@ __global__ void k_coarse_to_dense(CoarseView<c64> C, c64* __restrict__ M, size_t
         m = m + v;
     }
 }
+// backward direction j (0..3) of a site: -x, -y, -x+y, -x-y (indices 2, 4, 7, 8 of the
+// 9-direction order {0, +x, -x, +y, -y, +x+y, +x-y, -x+y, -x-y}); from the neighbour it
+// reaches, the way back is +x, +y, +x-y, +x+y (indices 1, 3, 6, 5)
+__global__ void k_half_hermitian(ch16* __restrict__ link, int nx, int ny, int k) {
+    const size_t kk = (size_t)k * k;
+    const size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= (size_t)nx * ny * 4 * kk) return;
+    const int b = (int)(t / (4 * kk)), j = (int)(t / kk % 4);
+    const int e = (int)(t % kk), ki = e / k, kj = e % k;
+    const int d = (j == 0) ? 2 : (j == 1) ? 4 : (j == 2) ? 7 : 8;
+    const int rd = (j == 0) ? 1 : (j == 1) ? 3 : (j == 2) ? 6 : 5;
+    const int dx = (j == 1) ? 0 : -1, dy = (j == 0) ? 0 : (j == 1 || j == 3) ? -1 : 1;
+    const int bx = b % nx, by = b / nx;
+    const int n = (bx + dx + nx) % nx + ((by + dy + ny) % ny) * nx;
+    const ch16 v = link[((size_t)n * 9 + rd) * kk + (size_t)kj * k + ki];
+    link[((size_t)b * 9 + d) * kk + e] = ch16{v.re, __hneg(v.im)};
+}
+void launch_half_hermitian(ch16* link, int nx, int ny, int k, hipStream_t st) {
+    SMG_REQUIRE(nx >= 3 && ny >= 3, "Hermitian FP16 links need coarse extents >= 3");
+    k_half_hermitian<<<grid_for((size_t)nx * ny * 4 * k * k), kBlock, 0, st>>>(link, nx, ny, k);
+    SMG_LAUNCH_CHECK();
+}
+
 void launch_coarse_to_dense(const CoarseView<c64>& C, c64* M, size_t n, hipStream_t st) {
     k_coarse_to_dense<<<grid_for((size_t)C.nx * C.ny * C.k * C.k), kBlock, 0, st>>>(C, M, n);
     SMG_LAUNCH_CHECK();

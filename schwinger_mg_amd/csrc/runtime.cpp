@@ -258,6 +258,7 @@ CoarseOperator::CoarseOperator(int nx_, int ny_, int k_)
 
 void CoarseOperator::build(LinOp<double>& A, const Aggregation& P, hipStream_t st) {
     SMG_REQUIRE(P.A.nbx == nx && P.A.nby == ny && P.A.k == k, "coarse operator / aggregation mismatch");
+    hermitian = true;
     launch_zero<double>(l64.p(), l64.n, st);
     Field<double> e(n()), f(P.fine_n()), Af(P.fine_n()), col(n());
     const int cx = color_period(nx), cy = color_period(ny);
@@ -273,12 +274,17 @@ void CoarseOperator::build(LinOp<double>& A, const Aggregation& P, hipStream_t s
     SMG_HIP(hipStreamSynchronize(st));
 }
 void CoarseOperator::set_links(const c64* dev, hipStream_t st) {
+    hermitian = false;   // links from the caller: no symmetry is assumed
     dcopy(l64.p(), dev, nlinks(), st);
     refresh_low_precision(st);
 }
 void CoarseOperator::refresh_low_precision(hipStream_t st) {
     launch_convert<float, double>(l32.p(), l64.p(), nlinks(), st);
     launch_convert_half(l16.as<ch16>(), l64.p(), nlinks(), st);
+    // P^dag A P of a Hermitian A is Hermitian up to roundoff; the FP16 copy is made so
+    // exactly, and its matrix-core applies then read 5 of the 9 blocks of a site
+    herm16 = (hermitian && nx >= 3 && ny >= 3) ? 1 : 0;
+    if (herm16) launch_half_hermitian(l16.as<ch16>(), nx, ny, k, st);
 }
 
 template <typename T, typename TL>
@@ -834,18 +840,26 @@ template SolveResult gcr_solve<double>(LinOp<double>&, const c64*, c64*, MG*, in
 template SolveResult gcr_solve<float>(LinOp<float>&, const c32*, c32*, MG*, int, double, int, bool, hipStream_t);
 
 SolveResult gcr_solve_mixed(LinOp<double>& A64, LinOp<float>& A32, const c64* b, c64* x, MG* mg, int maxit,
-                            double tol, int trunc, double inner_tol, hipStream_t st) {
+                            double tol, int trunc, double inner_tol, hipStream_t st, bool use_x0) {
     SolveResult res;
     const size_t n = A64.n();
     Context& cx = Context::get();
-    launch_zero<double>(x, n, st);
+    if (!use_x0) launch_zero<double>(x, n, st);
     const double bnorm = dev_norm(b, n, st);
-    if (bnorm < 1e-30) return res;
+    if (bnorm < 1e-30) {
+        launch_zero<double>(x, n, st);
+        return res;
+    }
     Field<double> r(n);
     Field<float> r32(n), e32(n);
     Scalars sc(8);
-    dcopy(r.p(), b, n, st);
     double rn = bnorm;
+    if (use_x0) {
+        A64.apply(r.p(), x, EPI_RESID, b, nullptr, Fin(), st);
+        rn = dev_norm(r.p(), n, st);
+    } else {
+        dcopy(r.p(), b, n, st);
+    }
     int total = 0;
     while (total < maxit) {
         if (rn / bnorm < tol) break;

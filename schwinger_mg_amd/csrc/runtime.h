@@ -153,6 +153,9 @@ struct CoarseOperator {
     Field<double> l64;
     Field<float> l32;
     DevMem l16;
+    bool hermitian = false;   // a Galerkin projection of a Hermitian operator (build, build_b)
+    int herm16 = 0;           // l16 made exactly Hermitian (hermitian and both extents >= 3)
+    CoarseView<ch16> view16() const { return CoarseView<ch16>{l16.as<ch16>(), nx, ny, k, herm16}; }
     CoarseOperator(int nx, int ny, int k);
     size_t n() const { return (size_t)nx * ny * k; }
     size_t nlinks() const { return (size_t)nx * ny * 9 * k * k; }
@@ -293,8 +296,9 @@ SolveResult gcr_solve(LinOp<T>& A, const cplx<T>* b, cplx<T>* x, MG* mg, int max
                       bool ref_projection, hipStream_t st);
 // Mixed precision: FP64 iterative refinement (true residual in FP64) around FP32
 // GCR(trunc) preconditioned by the FP32 MG cycle.
+// With use_x0 the solve starts from the x it is given (its FP64 true residual first).
 SolveResult gcr_solve_mixed(LinOp<double>& A64, LinOp<float>& A32, const c64* b, c64* x, MG* mg, int maxit,
-                            double tol, int trunc, double inner_tol, hipStream_t st);
+                            double tol, int trunc, double inner_tol, hipStream_t st, bool use_x0 = false);
 // fgmres_solve_generic (src/solvers.cpp:555-772): no Ritz harvest when ritz is null,
 // else the harvest and intra-solve deflation that ritz carries
 SolveResult fgmres_solve(LinOp<double>& A, const c64* b, c64* x, MG* mg, int restart, int maxit, double tol,

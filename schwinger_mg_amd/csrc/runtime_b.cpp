@@ -1,9 +1,11 @@
 // runtime_b.cpp -- sequencing of the multi-right-hand-side solve path.
 #include "runtime_b.h"
+#include "tune.h"
 
 #include <algorithm>
 #include <cmath>
 #include <deque>
+#include <functional>
 
 namespace smg {
 
@@ -51,7 +53,9 @@ void CoarseLinOpB<T, TL>::apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const
     if constexpr (std::is_same<TL, c64>::value) link = C.l64.p();
     else if constexpr (std::is_same<TL, c32>::value) link = C.l32.p();
     else link = C.l16.template as<ch16>();
-    launch_b_coarse_apply<T, TL>(CoarseView<TL>{link, C.nx, C.ny, C.k}, dst, src, epi, b, aux, fin, this->N, st);
+    const int herm = std::is_same<TL, ch16>::value ? C.herm16 : 0;
+    launch_b_coarse_apply<T, TL>(CoarseView<TL>{link, C.nx, C.ny, C.k, herm}, dst, src, epi, b, aux, fin, this->N,
+                                 st);
 }
 template struct CoarseLinOpB<double, c64>;
 template struct CoarseLinOpB<float, c32>;
@@ -60,6 +64,7 @@ template struct CoarseLinOpB<float, ch16>;
 // ------------------------------------------------------------------ Galerkin build (batched probing)
 void CoarseOperator::build_b(LinOpB<double>& A, const Aggregation& P, hipStream_t st) {
     SMG_REQUIRE(P.A.nbx == nx && P.A.nby == ny && P.A.k == k, "coarse operator / aggregation mismatch");
+    hermitian = true;
     SMG_REQUIRE(A.n() == P.fine_n(), "operator / aggregation mismatch");
     const int N = A.N;
     const size_t nc = n(), nf = P.fine_n();
@@ -163,7 +168,7 @@ static void cg_h(const CoarseOperator& C, const c32* b, c32* x, cb16* r, cb16* p
     fa.skip = done;
     fbt.skip = done;
     fbt.done = done;
-    const CoarseView<ch16> v{C.l16.as<ch16>(), C.nx, C.ny, C.k};
+    const CoarseView<ch16> v = C.view16();
     for (int it = 0; it < maxit; it++) {
         launch_b_coarse_apply_h(v, Ap, p, EPI_DOTS, p, fa, N, st);
         launch_h_cg_r(r, Ap, alpha, n, fbt, N, late, st);
@@ -431,6 +436,110 @@ BatchedSolveResult gcr_solve_batched(const Dirac& D, MG* mg, int prec, const c64
     for (int j = 0; j < N; j++) out.true_residual[j] = bn[j] > 1e-30 ? std::sqrt(rn[j]) / bn[j] : 0.0;
     launch_b_unpack<double, double>(X, x64.p(), n, N, st);
     return out;
+}
+
+
+// ------------------------------------------------------------------ autotuner searches
+namespace {
+// average duration of one call of f, from HIP events around `reps` calls after two
+// untimed calls (every call on the library stream st)
+double time_us(const std::function<void()>& f, hipStream_t st, int reps = 6) {
+    hipEvent_t e0, e1;
+    SMG_HIP(hipEventCreate(&e0));
+    SMG_HIP(hipEventCreate(&e1));
+    f();
+    f();
+    SMG_HIP(hipEventRecord(e0, st));
+    for (int i = 0; i < reps; i++) f();
+    SMG_HIP(hipEventRecord(e1, st));
+    SMG_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    SMG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return 1e3 * ms / reps;
+}
+// times every candidate choice of key with launch(), keeps the fastest in the table
+int pick(const TuneKey& key, const std::vector<int>& cands, const std::function<void()>& launch, hipStream_t st) {
+    double best = 1e300;
+    int arg = cands.front();
+    for (int c : cands) {
+        tune_store(key, c);
+        const double t = time_us(launch, st);
+        if (t < best) {
+            best = t;
+            arg = c;
+        }
+    }
+    tune_store(key, arg);
+    return arg;
+}
+}  // namespace
+
+// The launch shapes of the batched cycle for N right-hand sides: for every coarse
+// level's FP16-link matrix-core apply (each epilogue, and the BF16-vector form of the
+// coarsest CG) the link-read variant and the workgroup count of its reducing launch,
+// and the workgroup count of the fine stencil's reducing launch (FP32 and FP64).
+int tune_mg(MG& mg, int N, hipStream_t st) {
+    SMG_REQUIRE(N == 16 || N == 32 || N == 64, "tuning: N must be 16, 32 or 64");
+    const int before = (int)tune_size();
+    Field<double> out((size_t)3 * 64);
+    const FinB fin = finb(FIN_STORE, reinterpret_cast<double*>(out.p()), 3);
+    for (size_t l = 0; l < mg.C.size(); l++) {
+        const CoarseOperator& C = *mg.C[l];
+        if (C.k != 8 && C.k != 16 && C.k != 32) continue;
+        const size_t nN = C.n() * N;
+        Field<float> src(nN), dst(nN), aux(nN);
+        launch_zero<float>(src.p(), nN, st);
+        launch_zero<float>(aux.p(), nN, st);
+        CoarseLinOpB<float, ch16> op(C, N);
+        const int sites = C.nx * C.ny, waves_per_wg = kBlock / 64;
+        std::vector<int> herm = {0};
+        if (C.herm16 && C.k % 16 == 0) herm.push_back(1);
+        std::vector<int> dots;
+        for (int h : herm)
+            for (int g = 0; g < kTuneNGrids; g++)
+                if (g == 0 || kTuneGrids[g] <= (sites + waves_per_wg - 1) / waves_per_wg) dots.push_back(h + 2 * g);
+        for (int epi : {(int)EPI_NONE, (int)EPI_RESID, (int)EPI_DOTS}) {
+            const TuneKey key{TUNE_COARSE_BF16, C.nx, C.ny, C.k, N, epi};
+            pick(key, epi == EPI_DOTS ? dots : herm, [&] {
+                op.apply(dst.p(), src.p(), (Epi)epi, aux.p(), aux.p(), fin, st);
+            }, st);
+        }
+        const bool last = l + 1 == mg.C.size();
+        if (last && mg.cfg.coarse_half == 2 && mg.cfg.coarse_solver == 0) {   // BF16-vector coarsest CG
+            DevMem hs(nN * sizeof(cb16)), hd(nN * sizeof(cb16));
+            SMG_HIP(hipMemsetAsync(hs.p, 0, nN * sizeof(cb16), st));
+            const TuneKey key{TUNE_COARSE_BF16, C.nx, C.ny, C.k, N, 4 + (int)EPI_DOTS};
+            pick(key, dots, [&] {
+                launch_b_coarse_apply_h(C.view16(), hd.as<cb16>(), hs.as<cb16>(), EPI_DOTS, hs.as<cb16>(), fin, N,
+                                        st);
+            }, st);
+        }
+    }
+    // the fine stencil with fused dots (the second pass of D^dag D in the smoother and CG)
+    const Dirac& D = *mg.D;
+    const size_t nf = 2 * (size_t)D.V * N;
+    std::vector<int> grids;
+    for (int g = 0; g < kTuneNGrids; g++) grids.push_back(g);
+    {
+        Field<float> src(nf), dst(nf);
+        launch_zero<float>(src.p(), nf, st);
+        pick(TuneKey{TUNE_DSLASH_DOTS, D.L, N, (int)sizeof(float), 0, 0}, grids, [&] {
+            launch_b_dslash<float>(D.v32(), dst.p(), src.p(), 0.0f, 1, 1, EPI_DOTS, nullptr, src.p(), fin, N, st);
+        }, st);
+    }
+    {
+        Field<double> src(nf), dst(nf);
+        launch_zero<double>(src.p(), nf, st);
+        pick(TuneKey{TUNE_DSLASH_DOTS, D.L, N, (int)sizeof(double), 0, 0}, grids, [&] {
+            launch_b_dslash<double>(D.v64(), dst.p(), src.p(), 0.0, 1, 1, EPI_DOTS, nullptr, src.p(), fin, N, st);
+        }, st);
+    }
+    mg.reset_graphs();   // captured cycles hold the old launch shapes
+    release_batched_cycles(&mg);
+    SMG_HIP(hipStreamSynchronize(st));
+    return (int)tune_size() - before;
 }
 
 }  // namespace smg

@@ -51,6 +51,9 @@ template <typename T>
 void mg_precondition_b(MG& mg, const cplx<T>* b, cplx<T>* x, int N, hipStream_t st);
 // drop the batched cycles built for mg (before mg is destroyed)
 void release_batched_cycles(MG* mg);
+// the autotuner's timing search over the launch shapes of mg's batched cycle for N
+// right-hand sides (tune.h); returns the number of new table entries
+int tune_mg(MG& mg, int N, hipStream_t st);
 
 // GCR(trunc) on N right-hand sides in lockstep (independent Krylov spaces, per-RHS
 // scalars), MG preconditioned; batched layouts; returns per-RHS relative residuals

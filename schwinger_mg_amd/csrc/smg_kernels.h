@@ -187,7 +187,14 @@ template <typename TL>
 struct CoarseView {
     const TL* link;  // [nb][9][k*k]
     int nx, ny, k;
+    // 1: the backward blocks (-x, -y, -x+y, -x-y) are, bit for bit, the conjugate
+    // transposes of the neighbours' forward blocks (+x, +y, +x-y, +x+y), so a kernel may
+    // read 5 of the 9 blocks (a Hermitian operator on a lattice of extent >= 3)
+    int herm = 0;
 };
+// the FP16 links of a Hermitian operator made exactly Hermitian: every backward block
+// overwritten with the conjugate transpose of the neighbour's forward block (extents >= 3)
+void launch_half_hermitian(ch16* link, int nx, int ny, int k, hipStream_t st);
 template <typename T, typename TL>
 void launch_coarse_apply(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b,
                          const cplx<T>* aux, const Fin& fin, hipStream_t st);

@@ -151,3 +151,98 @@ def test_config4_L128_force_vs_finite_differences(smg, config4, form):
     worst = np.abs(c["fd"] - F[c["links"]]).max() / np.abs(F).max()
     print(f"C4 {form}: max |FD - F| / max |F| = {worst:.2e}")
     assert worst < 1e-7
+
+
+@pytest.fixture(scope="module")
+def config5_small(smg):
+    """Configuration 5's integrator at L = 32: beta = 6, m = 0.02, c_sw = 1, tau = 1,
+    Omelyan outer steps (fermion force) n_outer = 32, Omelyan inner steps (gauge force)
+    n_gauge = 8 per outer position update, links after 200 heatbath sweeps; the oracle's
+    trajectory (FP64 CG to 1e-13) from the same momenta and pseudofermion."""
+    L, V, m, beta, n_outer = 32, 32 * 32, 0.02, 6.0, 32
+    D = smg.DiracOp(L, random_gauge(L, 0.3, 5001), m, 1.0, 1.0, 0.0)
+    smg.dirac_heatbath(D, beta, 200, 5002)
+    U = smg.gauge_of(D)
+    rng = np.random.default_rng(5003)
+    pi0 = rng.normal(size=2 * V)
+    eta = rng.normal(size=2 * V) + 1j * rng.normal(size=2 * V)
+    Do = O.Dirac(L, U, m, 1.0, 1.0, 0.0)
+    phi = Do.apply(1, eta)
+    h0, h1 = np.zeros(4), np.zeros(4)
+    O.lib().orc_hmc_hamiltonian(L, O._ptr(U), m, 1.0, 1.0, 0.0, beta, 20000, 1e-13, O._ptr(pi0), O._ptr(phi),
+                                O._ptr(h0))
+    Uo, po, _ = O.hmc_nested_evolve(L, U, m, 1.0, 1.0, 0.0, beta, 1.0, n_outer, 8, 1, 1, 20000, 1e-13, pi0, phi)
+    O.lib().orc_hmc_hamiltonian(L, O._ptr(Uo), m, 1.0, 1.0, 0.0, beta, 20000, 1e-13, O._ptr(po), O._ptr(phi),
+                                O._ptr(h1))
+    dHo = h1[:3].sum() - h0[:3].sum()
+    return dict(L=L, V=V, m=m, beta=beta, n_outer=n_outer, U=U, pi0=pi0, eta=eta, Uo=Uo, dHo=dHo,
+                hscale=max(1.0, np.abs(h0[:3]).max()))
+
+
+@pytest.mark.parametrize("solver", ["fp64", "mixed"])
+def test_config5_nested_omelyan_reduced_vs_oracle(smg, config5_small, solver):
+    """The device trajectory equals the oracle's: with FP64 CG to dH 1e-8 (relative to
+    the Hamiltonian's terms) and the links to 1e-10; with the mixed-precision MG solver,
+    the chronological initial guess and the hierarchy refreshed every second outer
+    position update, to the solver's 1e-12 tolerance.  At this step count |dH| < 1."""
+    c = config5_small
+    L, V = c["L"], c["V"]
+    banner(f"config 5 reduced: L={L} beta=6 m=0.02 n_outer={c['n_outer']} n_gauge=8, {solver}")
+    D = smg.DiracOp(L, c["U"], c["m"], 1.0, 1.0, 0.0)
+    if solver == "fp64":
+        p = smg.nested_params(c["beta"], 1.0, c["n_outer"], 8, True, True, smg.F64, 20000, 1e-13)
+        mg = None
+    else:
+        starts = gaussian_vecs(4, 2 * V, 5004)
+        mg = smg.MGHierarchy(D, starts, n_levels=2, blocks=((4, 4),), k=(8,), null_iters=20, chiral=True,
+                             coarse_half=True, coarse_maxit=200, coarse_tol=1e-2)
+        p = smg.nested_params(c["beta"], 1.0, c["n_outer"], 8, True, True, smg.F32, 500, 1e-12, mg_refresh=2,
+                              mg_refresh_iters=2, chrono_guess=True)
+    res = smg.hmc_trajectory_nested(D, p, c["pi0"], c["eta"], 0.05, mg=mg)
+    dU = np.abs(smg.gauge_of(D) - c["Uo"]).max()
+    print(f"C5 reduced {solver}: dH {res.dH:.6e} (oracle {c['dHo']:.6e}), accepted {res.accepted}, "
+          f"solver iterations {res.total_cg_iters}, max true residual {res.max_true_residual:.2e}, |dU| {dU:.2e}")
+    assert abs(c["dHo"]) < 1.0 and res.accepted == 1
+    if solver == "fp64":
+        assert abs(res.dH - c["dHo"]) < 1e-8 * c["hscale"]
+        assert dU < 1e-10
+        assert res.max_true_residual < 1e-12
+    else:
+        assert abs(res.dH - c["dHo"]) < 1e-6 * c["hscale"]
+        assert dU < 1e-7
+        assert res.max_true_residual < 1e-12
+
+
+def test_nested_reject_with_refresh_restores_field_and_hierarchy(smg):
+    """A rejected trajectory whose hierarchy followed the proposal: the links and the
+    near-null vectors come back and the hierarchy is rebuilt on them, so the next solve on
+    the kept field is the solve of the hierarchy the trajectory started from."""
+    banner("nested reject with MG refresh L=32")
+    L, V, m = 32, 32 * 32, 0.05
+    U = random_gauge(L, 0.3, 5101)
+    starts = gaussian_vecs(4, 2 * V, 5102)
+
+    def hierarchy(D):
+        return smg.MGHierarchy(D, starts, n_levels=2, blocks=((4, 4),), k=(8,), null_iters=20, chiral=True,
+                               coarse_half=True, coarse_maxit=200, coarse_tol=1e-2)
+    D = smg.DiracOp(L, U, m, 1.0, 1.0, 0.0)
+    mg = hierarchy(D)
+    rng = np.random.default_rng(5103)
+    pi0 = rng.normal(size=2 * V)
+    eta = rng.normal(size=2 * V) + 1j * rng.normal(size=2 * V)
+    # two outer steps over tau = 1 at beta = 6: a violent proposal, rejected with u ~ 1
+    p = smg.nested_params(6.0, 1.0, 2, 2, True, True, smg.F32, 500, 1e-10, mg_refresh=1, mg_refresh_iters=2)
+    res = smg.hmc_trajectory_nested(D, p, pi0, eta, 1.0 - 1e-16, mg=mg)
+    assert res.dH > 0 and res.accepted == 0
+    assert res.max_true_residual < 1e-10
+    assert np.array_equal(smg.gauge_of(D), U)
+    b = gaussian_vec(2 * V, 5104)
+    got = smg.fcg_solve(D, b, mg, 500, 1e-10, trunc=10, prec=smg.F32)
+    D2 = smg.DiracOp(L, U, m, 1.0, 1.0, 0.0)
+    mg2 = hierarchy(D2)
+    ref = smg.fcg_solve(D2, b, mg2, 500, 1e-10, trunc=10, prec=smg.F32)
+    print(f"after reject: {got.iterations} iterations (fresh hierarchy {ref.iterations}), "
+          f"solution difference {relerr(got.solution, ref.solution):.2e}")
+    assert got.true_residual < 1e-10
+    assert got.iterations == ref.iterations
+    assert relerr(got.solution, ref.solution) < 1e-13

@@ -1,0 +1,73 @@
+"""The Hermitian link reads of the FP16-link matrix-core coarse apply and the launch-shape
+autotuner.  A Galerkin operator's FP16 links are made exactly Hermitian, so the apply that
+reads 5 of a site's 9 link blocks returns, bit for bit, what the 9-block apply returns, and
+both agree with the oracle's FP64 operator to the FP16 rounding.  A tuned table, saved and
+loaded again, reproduces a batched mixed-precision solve bit for bit."""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import banner, relerr
+from schwinger_mg_amd.fields import gaussian_vecs, random_gauge
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def clean_table(smg):
+    smg.tune_clear()
+    yield
+    smg.tune_clear()
+
+
+@pytest.mark.parametrize("k,N", [(16, 16), (32, 32), (32, 64)])
+def test_hermitian_reads_bit_identical(smg, clean_table, k, N):
+    banner(f"hermitian coarse reads k={k} N={N}")
+    L, b = 32, 4
+    V = L * L
+    U = random_gauge(L, 0.4, 300 + k)
+    D = smg.DiracOp(L, U, 0.05, 1.0, 1.0, 0.0)
+    nulls = gaussian_vecs(k // 2, 2 * V, 301)
+    P = smg.Prolongator(L, L, 2, b, b, k, chiral=True).build_from_vectors(nulls)
+    C = smg.SparseCoarseOp.build(P, D)            # Galerkin: Hermitian, 8 x 8 coarse sites
+    S = gaussian_vecs(N, C.dim, 302)
+    nb = L // b
+    out = {}
+    for prec in (smg.F16, smg.BF16V):
+        hv = 4 if prec == smg.BF16V else 0
+        for herm in (0, 1):
+            smg.tune_set(smg.TUNE_COARSE_BF16, [nb, nb, k, N, hv + 0], herm)
+            out[prec, herm] = C.apply_batched(S, prec)
+        assert np.array_equal(out[prec, 0], out[prec, 1]), prec
+    Po = O.Prolongator(L, L, 2, b, b, k, chiral=True)
+    Po.set_block_vecs(P.block_vecs)
+    A = O.Coarse(O.Dirac(L, U, 0.05, 1.0, 1.0, 0.0).op(), Po, 1).op()
+    want = np.stack([A(S[j]) for j in range(N)])
+    e16, ebf = relerr(out[smg.F16, 1], want), relerr(out[smg.BF16V, 1], want)
+    print(f"FP16 links vs FP64 oracle: {e16:.2e}; with BF16 vectors {ebf:.2e}")
+    assert e16 < 2e-3 and ebf < 2e-2
+
+
+def test_tuned_table_reproduces_solve(smg, clean_table, tmp_path):
+    banner("autotuner: tune, save, load, bit-identical batched solve L=64")
+    L, k, N = 64, 32, 32
+    V = L * L
+    U = random_gauge(L, 0.4, 310)
+    D = smg.DiracOp(L, U, 0.05, 1.0, 1.0, 0.0)
+    dims = [2 * V, (L // 4) ** 2 * k]
+    starts = np.concatenate([gaussian_vecs(k // 2, n, 311 + i).ravel() for i, n in enumerate(dims)])
+    mg = smg.MGHierarchy(D, starts, n_levels=3, blocks=((4, 4), (2, 2)), k=(k, k), null_iters=10, w_cycle=False,
+                         coarse_maxit=30, coarse_tol=0.1, coarse_half=2, chiral=True)
+    added = smg.tune_mg(mg, N)
+    assert added >= 6
+    path = str(tmp_path / "tune.txt")
+    smg.tune_save(path)
+    B = gaussian_vecs(N, 2 * V, 312)
+    X1, i1, t1 = smg.gcr_solve_batched(D, B, mg, 500, 1e-10, trunc=10, prec=smg.F32)
+    smg.tune_clear()
+    smg.tune_load(path)
+    X2, i2, t2 = smg.gcr_solve_batched(D, B, mg, 500, 1e-10, trunc=10, prec=smg.F32)
+    print(f"tuned: {added} shapes, {i1.iterations} iterations, max true residual {t1.max():.2e}")
+    print(open(path).read())
+    assert np.array_equal(X1, X2) and i1.iterations == i2.iterations
+    assert t1.max() < 1e-10
