@@ -37,6 +37,17 @@ from schwinger_mg_amd.fields import gaussian_vec, gaussian_vecs, random_gauge  #
 HBM_PEAK_GBS = 8000.0
 
 
+def marker(torch, name, start):
+    """roctx range around a timed step (a no-op where torch has no marker support)."""
+    try:
+        if start:
+            torch.cuda.nvtx.range_push(name)
+        else:
+            torch.cuda.nvtx.range_pop()
+    except Exception:
+        pass
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -72,8 +83,12 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--lattice-batch", type=int, default=1,
                     help="also time the 40-lattice batch of the workload (one launch over all lattices)")
-    ap.add_argument("--workload", choices=("solve", "hmc"), default="solve",
-                    help="solve: the batched MG solve (default); hmc: trajectories of the nested Omelyan HMC")
+    ap.add_argument("--workload", choices=("solve", "hmc", "kernels"), default="solve",
+                    help="solve: the batched MG solve (default); hmc: trajectories of the nested Omelyan HMC; "
+                         "kernels: the hierarchy's setup, then only the stencil and coarse-operator launches (and a "
+                         "plain copy of 128 MiB) that the counter runs summarised in profiles/ measure")
+    ap.add_argument("--tune", type=int, default=1,
+                    help="run the launch-shape autotuner on the hierarchy before timing (its time is reported)")
     ap.add_argument("--beta", type=float, default=6.0, help="hmc: gauge coupling")
     ap.add_argument("--hmc-mass", type=float, default=0.02, help="hmc: fermion mass")
     ap.add_argument("--n-outer", type=int, default=1, help="hmc: outer (fermion-force) steps")
@@ -93,17 +108,20 @@ def parse():
     return ap.parse_args()
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_b_dslash_f32_L512_N32_hbm_traffic.md")
+TRAFFIC_PROFILES = {
+    "dslash_f32": os.path.join(ROOT, "profiles", "k_b_dslash_f32_L512_N32_hbm_traffic.md"),
+    "coarse": os.path.join(ROOT, "profiles", "k_b_coarse_L512_N32_hbm_traffic.md"),
+}
 
 
-def profiled_traffic(L, nrhs):
-    """HBM bytes per launch of the FP32 stencil at L=512, N=32, from the counter
-    summary (its line 'traffic_bytes_per_launch: <int>'); None if absent or if the
-    run is at another size."""
-    if L != 512 or nrhs != 32 or not os.path.exists(TRAFFIC_PROFILE):
+def profiled_traffic(L, nrhs, kernel="dslash_f32", key="traffic_bytes_per_launch"):
+    """HBM bytes per launch of a kernel at L=512, N=32 from its counter summary (a line
+    '<key>: <int>'); None if absent or if the run is at another size."""
+    path = TRAFFIC_PROFILES[kernel]
+    if L != 512 or nrhs != 32 or not os.path.exists(path):
         return None
-    for line in open(TRAFFIC_PROFILE):
-        if line.strip().startswith("traffic_bytes_per_launch:"):
+    for line in open(path):
+        if line.strip().startswith(key + ":"):
             return int(line.split(":")[1].strip().strip("`"))
     return None
 
@@ -145,6 +163,51 @@ def time_stencil(smg, torch, D, prec, nrhs, V, csw, launches=50, warm=5):
     byt = stencil_bytes_per_site(prec, nrhs, csw) * V
     return {"prec": prec, "nrhs": nrhs, "us_per_launch": ms * 1e3, "bytes_per_launch": byt,
             "GBs": byt / (ms * 1e-3) / 1e9}
+
+
+PEAKS = {"hbm": 8000.0, "bf16": 2500.0, "f64": 78.6, "f32": 157.3}   # GB/s; TFLOP/s (dense MFMA)
+
+
+def time_coarse(smg, torch, mg, level, prec, nrhs, k, launches=50, warm=5):
+    """Average duration of one launch of the hierarchy's level-`level` coarse operator on
+    the batched layout with the fused dot products (the launch shape of the smoother and
+    the coarse CG): a HIP event pair around each launch on the library's stream.
+    Algorithmic bytes per launch Vc (9 k^2 s_L + 2 k s_psi N) (links of s_L bytes per
+    complex entry, sources read and results written once); flops 72 k^2 Vc N."""
+    import ctypes
+    n = mg.level_dim(level)
+    vc = n // k
+    sl, sp, dt = {smg.F64: (16, 16, torch.complex128), smg.F16: (4, 8, torch.complex64),
+                  smg.BF16V: (4, 4, torch.int32)}[prec]
+    if prec == smg.BF16V:     # bf16 pairs: random bits of moderate exponent
+        src = (torch.randint(0, 1 << 14, (n * nrhs,), dtype=torch.int32, device="cuda") | 0x3c003c00)
+    else:
+        src = torch.randn(n * nrhs, dtype=dt, device="cuda")
+    dst = torch.empty_like(src)
+    sp_ = smg.library_stream()
+    st = torch.cuda.ExternalStream(sp_)
+    sarg = ctypes.c_void_p(sp_)
+    torch.cuda.synchronize()
+
+    def launch():
+        smg._chk(smg.lib().smg_mg_coarse_apply_dev(mg.h, level, prec, nrhs, src.data_ptr(), dst.data_ptr(), 1, sarg))
+
+    for _ in range(warm):
+        launch()
+    smg.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(launches)]
+    for e0, e1 in ev:
+        e0.record(st)
+        launch()
+        e1.record(st)
+    ev[-1][1].synchronize()
+    us = sum(e0.elapsed_time(e1) for e0, e1 in ev) / launches * 1e3
+    byt = vc * (9 * k * k * sl + 2 * k * sp * nrhs)
+    flop = 72 * k * k * vc * nrhs
+    return {"level": level, "prec": {smg.F64: "f64", smg.F16: "f16 links, f32 vectors",
+                                     smg.BF16V: "f16 links, bf16 vectors"}[prec],
+            "sites": vc, "k": k, "nrhs": nrhs, "us_per_launch": us, "bytes_per_launch": byt,
+            "GBs": byt / (us * 1e-6) / 1e9, "TFLOPs": flop / (us * 1e-6) / 1e12}
 
 
 def lattice_batch(smg, torch, gaussian_vec, random_gauge):
@@ -351,6 +414,34 @@ def hmc_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def kernels_workload(args):
+    """The hierarchy of the solve workload, tuned, then 20 launches of each timed kernel
+    (the FP32 and FP64 stencils at N RHS, every coarse level's apply with dots in FP16
+    links / FP32 vectors, BF16 vectors at the coarsest, FP64) and a plain 128 MiB copy
+    whose counters calibrate the HBM byte counters.  Prints one line per kernel."""
+    import torch
+    import schwinger_mg_amd as smg
+    L, V, N = args.L, args.L * args.L, args.nrhs
+    D = smg.DiracOp(L, random_gauge(L, args.sigma, 2024), args.mass, 1.0, 1.0, 0.0)
+    mg = build_mg(smg, args, D, L, 2024)
+    if args.tune:
+        smg.tune_mg(mg, N)
+    for p in ("f32", "f64"):
+        s = time_stencil(smg, torch, D, p, N, V, True, launches=20, warm=2)
+        print(f"stencil {p} x{N}: {s['us_per_launch']:.1f} us, {s['GBs']:.0f} GB/s", flush=True)
+    for lev in range(1, args.levels):
+        for prec in (smg.F16, smg.F64) + ((smg.BF16V,) if lev == args.levels - 1 else ()):
+            c = time_coarse(smg, torch, mg, lev, prec, N, args.k, launches=20, warm=2)
+            print(f"coarse level {lev} {c['prec']}: {c['us_per_launch']:.1f} us, {c['GBs']:.0f} GB/s, "
+                  f"{c['TFLOPs']:.1f} TFLOP/s", flush=True)
+    a = torch.empty(1 << 25, dtype=torch.float32, device="cuda").uniform_()
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print("copy 128 MiB x3 done", flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -359,6 +450,9 @@ def main():
         sys.exit(2)
     if args.workload == "hmc":
         hmc_workload(args)
+        return
+    if args.workload == "kernels":
+        kernels_workload(args)
         return
     import torch
     import schwinger_mg_amd as smg
@@ -372,8 +466,18 @@ def main():
     smg.synchronize()
     setup_s = time.time() - t0
     log(f"MG setup {setup_s:.2f} s")
-
     N = args.nrhs
+    tune_s, tuned, table = None, None, None
+    if args.tune and N in (16, 32, 64):
+        t0 = time.time()
+        tuned = smg.tune_mg(mg, N)
+        tune_s = time.time() - t0
+        import tempfile
+        with tempfile.TemporaryDirectory() as td:   # the table as the library writes it
+            p = os.path.join(td, "tune.txt")
+            smg.tune_save(p)
+            table = [l.strip() for l in open(p) if not l.startswith("#")]
+        log(f"autotuner: {tuned} launch shapes in {tune_s:.2f} s: {table}")
     B = gaussian_vecs(N, 2 * V, seed + 1)
     Bd = torch.from_numpy(B).cuda()
     Xd = torch.zeros_like(Bd)
@@ -386,23 +490,25 @@ def main():
                                                      min(args.trunc, 16), args.inner_tol,
                                                      tres.ctypes.data_as(ctypes.c_void_p), ctypes.byref(info), None))
 
-    def timed(fn, steps, warm):
+    def timed(fn, steps, warm, name):
         for _ in range(warm):
             fn()
         torch.cuda.synchronize(); smg.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
+            marker(torch, name, True)    # a marker range per timed step (kernel traces select by it)
             fn()
+            marker(torch, name, False)
         torch.cuda.synchronize(); smg.synchronize()
         return (time.perf_counter() - t0) / steps
 
-    t_mixed = timed(lambda: solve_batch(smg.F32), args.steps, args.warmup)
+    t_mixed = timed(lambda: solve_batch(smg.F32), args.steps, args.warmup, "mixed_batch")
     mixed = {"iterations": info.iterations, "restarts": info.restarts, "max_true_residual": float(tres.max())}
     log(f"mixed x{N}: {t_mixed * 1e3:.1f} ms/batch = {N / t_mixed:.1f} solves/s, {mixed}")
     X_mixed = Xd.cpu().numpy()
     t_f64, fp64, err_vs_f64 = float("nan"), None, None
     if args.fp64:
-        t_f64 = timed(lambda: solve_batch(smg.F64), args.fp64_steps, 1)
+        t_f64 = timed(lambda: solve_batch(smg.F64), args.fp64_steps, 1, "fp64_batch")
         fp64 = {"iterations": info.iterations, "max_true_residual": float(tres.max())}
         log(f"fp64 x{N}: {t_f64 * 1e3:.1f} ms/batch = {N / t_f64:.1f} solves/s, {fp64}")
         X_f64 = Xd.cpu().numpy()
@@ -418,25 +524,64 @@ def main():
 
     one = None
     if args.single:
-        t_one_mixed = timed(lambda: solve_one(smg.F32), 2, 1)
+        t_one_mixed = timed(lambda: solve_one(smg.F32), 2, 1, "mixed_single")
         one = {"mixed_solves_per_s": round(1.0 / t_one_mixed, 3), "mixed_iterations": info.iterations,
                "true_residual": info.true_residual}
-        t_one_f64 = timed(lambda: solve_one(smg.F64), 1, 1)
+        t_one_f64 = timed(lambda: solve_one(smg.F64), 1, 1, "fp64_single")
         one["fp64_solves_per_s"] = round(1.0 / t_one_f64, 3)
         log(f"single rhs: {one}")
 
     stencil = [time_stencil(smg, torch, D, p, n, V, True) for p, n in
                [("f32", 1), ("f64", 1), ("f32", N), ("f64", N)]]
+    coarse = []
+    if N in (16, 32, 64) and args.k in (8, 16, 32):
+        for lev in range(1, args.levels):
+            coarse.append(time_coarse(smg, torch, mg, lev, smg.F16, N, args.k))
+        if args.coarse_bf16:
+            coarse.append(time_coarse(smg, torch, mg, args.levels - 1, smg.BF16V, N, args.k))
+        if args.fp64:
+            for lev in range(1, args.levels):
+                coarse.append(time_coarse(smg, torch, mg, lev, smg.F64, N, args.k))
+        for c in coarse:
+            log(f"coarse level {c['level']} ({c['prec']}): {c['us_per_launch']:.1f} us, {c['GBs']:.0f} GB/s, "
+                f"{c['TFLOPs']:.1f} TFLOP/s")
     lbatch = None
     if args.lattice_batch:
         lbatch = lattice_batch(smg, torch, gaussian_vec, random_gauge)
         log(f"lattice batch: {lbatch}")
-    dom = stencil[2]     # FP32, N RHS: the fine-level stencil of the mixed solve
     log("stencil: " + ", ".join(f"{s['prec']}x{s['nrhs']} {s['GBs']:.0f} GB/s" for s in stencil))
-    roofline = {"bound": "hbm", "achieved": round(dom["GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(L, N),
-                "algorithmic_bytes_per_launch": dom["bytes_per_launch"], "avg_launch_us": round(dom["us_per_launch"], 2),
-                "kernel": f"k_b_dslash<float> (Wilson-clover D, {N} RHS, L={L})"}
+
+    def line(name, t, traffic, per_cycle, mfma_peak=None):
+        r = {"kernel": name, "bound": "hbm", "achieved": round(t["GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": round(t["GBs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+             "algorithmic_bytes_per_launch": t["bytes_per_launch"], "avg_launch_us": round(t["us_per_launch"], 2),
+             "launches_per_cycle": per_cycle, "ms_per_cycle": round(per_cycle * t["us_per_launch"] / 1e3, 3)}
+        if mfma_peak:
+            r["mfma_TFLOPs"] = round(t["TFLOPs"], 2)
+            r["mfma_frac"] = round(t["TFLOPs"] / mfma_peak, 4)
+        return r
+
+    # the mixed solve's kernels; launches per V-cycle: the stencil twice per D^dag D
+    # (3 + 3 MR sweeps, the residual and the GCR apply), each coarse level's apply in
+    # its 6 MR sweeps and residual, the coarsest in every CG iteration
+    lines = [line(f"k_b_dslash<float> (Wilson-clover D, {N} RHS, L={L})", stencil[2],
+                  profiled_traffic(L, N, "dslash_f32"), 2 * (args.pre + args.post + 2))]
+    last = args.levels - 1
+    for c in coarse:
+        if c["prec"].endswith("f32 vectors") and not (c["level"] == last and args.coarse_bf16):
+            lines.append(line(f"k_b_coarse_bf16 (FP16 links, FP32 vectors, level {c['level']}, "
+                              f"{c['sites']} sites, k={c['k']}, {N} RHS)", c,
+                              profiled_traffic(L, N, "coarse", f"level{c['level']}_f16"),
+                              args.pre + args.post + 1, PEAKS["bf16"]))
+        elif c["prec"].endswith("bf16 vectors"):
+            lines.append(line(f"k_b_coarse_bf16 (FP16 links, BF16 vectors, level {c['level']} coarsest CG, "
+                              f"{c['sites']} sites, k={c['k']}, {N} RHS)", c,
+                              profiled_traffic(L, N, "coarse", f"level{c['level']}_bf16v"),
+                              args.coarse_maxit, PEAKS["bf16"]))
+    fp64_lines = [line(f"k_b_coarse_mfma<double> (FP64, level {c['level']}, {c['sites']} sites, k={c['k']}, "
+                       f"{N} RHS)", c, None, args.coarse_maxit if c["level"] == last else args.pre + args.post + 1,
+                       PEAKS["f64"]) for c in coarse if c["prec"] == "f64"]
+    roofline = max(lines, key=lambda r: r["ms_per_cycle"])   # the mixed solve's dominant kernel
 
     cpu = None
     if args.cpu_baseline:
@@ -478,9 +623,14 @@ def main():
         "single_rhs": one,
         "mixed_vs_fp64_solution_relerr": err_vs_f64,
         "setup_s": round(setup_s, 3),
+        "tune_s": round(tune_s, 3) if tune_s is not None else None,
+        "tuned_shapes": tuned,
+        "tuned_table": table,
         "stencil": stencil,
         "lattice_batch": lbatch,
         "roofline": roofline,
+        "roofline_kernels": lines,
+        "roofline_fp64_kernels": fp64_lines,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -11,16 +11,20 @@
 //  * DiracOp copies the gauge links to the device when it is constructed; after
 //    changing gauge.U call DiracOp::update_gauge() (the reference reads the
 //    GaugeField through its reference on every apply).
-//  * The Krylov solvers, smoothers and coarse builds run the operator on the
-//    GPU, so they accept the OpApply / preconditioner objects this header
-//    creates (DiracOp::as_DdagD_op, SparseCoarseOp::as_op, MGHierarchy's
-//    as_precond) and throw std::invalid_argument for any other std::function.
+//  * The Krylov solvers, smoothers and the coarse build take any OpApply and any
+//    preconditioner std::function.  Those this header creates
+//    (DiracOp::as_DdagD_op, SparseCoarseOp::as_op, MGHierarchy::as_precond)
+//    run on the GPU as they are; any other callable (a lambda, as the
+//    reference's callers write) is called back on the host for each application
+//    while the Krylov vectors and reductions stay on the GPU.  The mixed-precision
+//    solve, the eigensolvers and the HMC drivers take this header's objects only.
 //  * A GPU failure throws std::runtime_error carrying smg_last_error() (the
 //    reference asserts on shape errors and has no other error path).
 #pragma once
 #include <array>
 #include <cmath>
 #include <complex>
+#include <exception>
 #include <functional>
 #include <iomanip>
 #include <iostream>
@@ -64,6 +68,28 @@ inline Vec random_vec(int n, std::mt19937& rng) {
     }
     return v;
 }
+
+// the host-vector vocabulary of src/linalg.h:5-11 (dot, norm, axpy, add, scale, zeros)
+// for callers that combine solver results on the host, as the reference's do; sums run
+// in index order
+inline cx dot(const Vec& a, const Vec& b) {   // sum conj(a_i) b_i
+    cx s(0.0, 0.0);
+    for (size_t i = 0; i < a.size(); i++) s += std::conj(a[i]) * b[i];
+    return s;
+}
+inline double norm(const Vec& v) { return std::sqrt(std::real(dot(v, v))); }
+inline void axpy(cx a, const Vec& x, Vec& y) {
+    for (size_t i = 0; i < y.size(); i++) y[i] += a * x[i];
+}
+inline Vec add(const Vec& a, const Vec& b, cx alpha = 1.0, cx beta = 1.0) {
+    Vec r(a.size());
+    for (size_t i = 0; i < a.size(); i++) r[i] = alpha * a[i] + beta * b[i];
+    return r;
+}
+inline void scale(Vec& v, cx a) {
+    for (auto& x : v) x *= a;
+}
+inline Vec zeros(int n) { return Vec(n, cx(0.0, 0.0)); }
 
 // ------------------------------------------------------------------ Lattice, src/lattice.h
 struct Lattice {
@@ -332,20 +358,8 @@ struct CoarseApply {   // the OpApply of SparseCoarseOp::as_op
 // SparseCoarseOp, src/coarse_op.h:154-548 (setup_deflation: the TRLM branch)
 struct SparseCoarseOp {
     int nbx = 0, nby = 0, nblocks = 0, k_vec = 0, dim = 0;
-    // Galerkin A_c = P^dag A P; A must be DiracOp::as_DdagD_op()
-    void build(const Prolongator& P, const OpApply& A, int fine_dim) {
-        const DdagDApply* f = A.target<DdagDApply>();
-        if (!f) throw std::invalid_argument("SparseCoarseOp::build: A must be DiracOp::as_DdagD_op()");
-        if (fine_dim != P.lat.ndof) throw std::invalid_argument("SparseCoarseOp::build: fine_dim mismatch");
-        smg_coarse* h = nullptr;
-        detail::check(smg_coarse_create_galerkin(&h, f->D->handle(), P.handle()));
-        h_.reset(h, [](smg_coarse* p) { smg_coarse_destroy(p); });
-        nbx = P.nbx;
-        nby = P.nby;
-        nblocks = P.nblocks;
-        k_vec = P.k_vec;
-        dim = nblocks * k_vec;
-    }
+    // Galerkin A_c = P^dag A P (any operator A on the prolongator's fine lattice)
+    void build(const Prolongator& P, const OpApply& A, int fine_dim);
     void apply_to(const Vec& src, Vec& dst) const {
         dst.assign(dim, cx(0.0, 0.0));
         detail::check(smg_coarse_apply(h_.get(), SMG_F64, detail::raw(src), detail::raw(dst)));
@@ -463,14 +477,7 @@ struct CoarseOp {
 
 // ------------------------------------------------------------------ smoothers, src/smoother.h
 namespace detail {
-inline void smooth(const OpApply& A, int kind, Vec& x, const Vec& b, int niter, double omega, const char* who) {
-    if (const DdagDApply* f = A.target<DdagDApply>())
-        check(smg_smooth_dirac(f->D->handle(), kind, raw(x), raw(b), niter, omega));
-    else if (const CoarseApply* c = A.target<CoarseApply>())
-        check(smg_smooth_coarse(c->C->handle(), kind, raw(x), raw(b), niter, omega));
-    else
-        throw std::invalid_argument(std::string(who) + ": A must be DiracOp::as_DdagD_op() or SparseCoarseOp::as_op()");
-}
+inline void smooth(const OpApply& A, int kind, Vec& x, const Vec& b, int niter, double omega, const char* who);
 }  // namespace detail
 inline void mr_smooth(const DiracOp& D, Vec& x, const Vec& b, int niter) {
     detail::check(smg_smooth_dirac(D.handle(), SMG_SMOOTH_MR, detail::raw(x), detail::raw(b), niter, 0.0));
@@ -481,16 +488,8 @@ inline void mr_smooth_op(const OpApply& A, Vec& x, const Vec& b, int niter) {
 inline void richardson_smooth_op(const OpApply& A, Vec& x, const Vec& b, int niter, double omega) {
     detail::smooth(A, SMG_SMOOTH_RICHARDSON, x, b, niter, omega, "richardson_smooth_op");
 }
-// power iteration from mt19937(12345) as src/smoother.cpp:43-59 (D^dag D only)
-inline double estimate_lambda_max(const OpApply& A, int n, int niter = 20) {
-    const DdagDApply* f = A.target<DdagDApply>();
-    if (!f) throw std::invalid_argument("estimate_lambda_max: A must be DiracOp::as_DdagD_op()");
-    std::mt19937 rng_local(12345);
-    const Vec v0 = random_vec(n, rng_local);
-    double out = 0.0;
-    detail::check(smg_lambda_max_dirac(f->D->handle(), detail::raw(v0), niter, &out));
-    return out;
-}
+// power iteration from mt19937(12345) as src/smoother.cpp:43-59, any operator
+inline double estimate_lambda_max(const OpApply& A, int n, int niter = 20);
 
 // ------------------------------------------------------------------ multigrid, src/multigrid.h
 struct MGHierarchy;
@@ -554,6 +553,129 @@ struct MGHierarchy {
     std::shared_ptr<smg_mg> h_;
 };
 inline Vec MGPrecond::operator()(const Vec& b) const { return mg->precondition(b); }
+
+// the identity preconditioner, recognised by lobpcg_update and fgmres_solve_generic
+// (the reference's callers pass a lambda returning its argument)
+struct IdentityPrecond {
+    Vec operator()(const Vec& v) const { return v; }
+};
+inline std::function<Vec(const Vec&)> identity_precond() { return IdentityPrecond{}; }
+
+// ------------------------------------------------------------------ any OpApply / preconditioner
+namespace detail {
+// A std::function handed to the library as a host callback (smg_op_fn).  An exception
+// it throws is kept and rethrown once the library call has returned.
+struct CallbackBox {
+    const OpApply* op = nullptr;
+    const std::function<Vec(const Vec&)>* prec = nullptr;
+    size_t n = 0;
+    std::exception_ptr err;
+};
+inline int op_trampoline(void* ctx, const double* src, double* dst) {
+    auto* b = static_cast<CallbackBox*>(ctx);
+    try {
+        const cx* s = reinterpret_cast<const cx*>(src);
+        Vec in(s, s + b->n), out(b->n);   // pre-sized: DiracOp's operators write without resizing
+        (*b->op)(in, out);
+        if (out.size() != b->n) throw std::length_error("OpApply returned a vector of another size");
+        std::copy(out.begin(), out.end(), reinterpret_cast<cx*>(dst));
+        return 0;
+    } catch (...) {
+        b->err = std::current_exception();
+        return 1;
+    }
+}
+inline int prec_trampoline(void* ctx, const double* src, double* dst) {
+    auto* b = static_cast<CallbackBox*>(ctx);
+    try {
+        const cx* s = reinterpret_cast<const cx*>(src);
+        const Vec out = (*b->prec)(Vec(s, s + b->n));
+        if (out.size() != b->n) throw std::length_error("the preconditioner returned a vector of another size");
+        std::copy(out.begin(), out.end(), reinterpret_cast<cx*>(dst));
+        return 0;
+    } catch (...) {
+        b->err = std::current_exception();
+        return 1;
+    }
+}
+// the library's description of A: this header's operators by handle, anything else
+// as a callback through box (which must outlive the call)
+inline smg_op make_op(const OpApply& A, size_t n, CallbackBox& box) {
+    if (!A) throw std::invalid_argument("schwinger_mg: empty OpApply");
+    smg_op o{};
+    o.n = n;
+    if (const DdagDApply* f = A.target<DdagDApply>()) {
+        o.dirac = f->D->handle();
+    } else if (const CoarseApply* c = A.target<CoarseApply>()) {
+        o.coarse = c->C->handle();
+    } else {
+        box.op = &A;
+        box.n = n;
+        o.fn = op_trampoline;
+        o.ctx = &box;
+    }
+    return o;
+}
+inline smg_precond make_prec(const std::function<Vec(const Vec&)>* M, size_t n, CallbackBox& box) {
+    smg_precond p{};
+    if (!M || !*M || M->target<IdentityPrecond>()) return p;   // none
+    if (const MGPrecond* g = M->target<MGPrecond>()) {
+        p.mg = g->mg->handle();
+        return p;
+    }
+    box.prec = M;
+    box.n = n;
+    p.fn = prec_trampoline;
+    p.ctx = &box;
+    return p;
+}
+// the status of a library call that may have called back: a callback's own exception
+// first, then the library's error
+inline void check_cb(int rc, const CallbackBox& a, const CallbackBox& m) {
+    if (a.err) std::rethrow_exception(a.err);
+    if (m.err) std::rethrow_exception(m.err);
+    check(rc);
+}
+inline void check_size(const Vec& v, int n, const char* who) {
+    if (n <= 0 || v.size() != (size_t)n)
+        throw std::invalid_argument(std::string(who) + ": the vector's size is not n");
+}
+}  // namespace detail
+
+inline void SparseCoarseOp::build(const Prolongator& P, const OpApply& A, int fine_dim) {
+    if (fine_dim != P.lat.ndof) throw std::invalid_argument("SparseCoarseOp::build: fine_dim mismatch");
+    smg_coarse* h = nullptr;
+    if (const DdagDApply* f = A.target<DdagDApply>()) {
+        detail::check(smg_coarse_create_galerkin(&h, f->D->handle(), P.handle()));
+    } else {
+        detail::CallbackBox box, none;
+        const smg_op op = detail::make_op(A, (size_t)fine_dim, box);
+        detail::check_cb(smg_coarse_create_galerkin_op(&h, &op, P.handle()), box, none);
+    }
+    h_.reset(h, [](smg_coarse* p) { smg_coarse_destroy(p); });
+    nbx = P.nbx;
+    nby = P.nby;
+    nblocks = P.nblocks;
+    k_vec = P.k_vec;
+    dim = nblocks * k_vec;
+}
+namespace detail {
+inline void smooth(const OpApply& A, int kind, Vec& x, const Vec& b, int niter, double omega, const char* who) {
+    check_size(b, (int)x.size(), who);
+    CallbackBox box, none;
+    const smg_op op = make_op(A, x.size(), box);
+    check_cb(smg_smooth_op(&op, kind, raw(x), raw(b), niter, omega), box, none);
+}
+}  // namespace detail
+inline double estimate_lambda_max(const OpApply& A, int n, int niter) {
+    std::mt19937 rng_local(12345);
+    const Vec v0 = random_vec(n, rng_local);
+    double out = 0.0;
+    detail::CallbackBox box, none;
+    const smg_op op = detail::make_op(A, (size_t)n, box);
+    detail::check_cb(smg_lambda_max_op(&op, detail::raw(v0), niter, &out), box, none);
+    return out;
+}
 
 // build_mg_hierarchy, src/multigrid.cpp:595-687.  The null-space start vectors are
 // drawn from rng in the reference's order (k_null vectors per level, level 0
@@ -631,12 +753,6 @@ struct FGMRESResult {
     std::vector<RitzPair> ritz_pairs;
 };
 
-// the identity preconditioner, recognised by lobpcg_update and fgmres_solve_generic
-// (the reference's callers pass a lambda returning its argument)
-struct IdentityPrecond {
-    Vec operator()(const Vec& v) const { return v; }
-};
-inline std::function<Vec(const Vec&)> identity_precond() { return IdentityPrecond{}; }
 
 namespace detail {
 inline const DiracOp& dirac_of(const OpApply& A, const char* who) {
@@ -649,38 +765,37 @@ inline MGHierarchy* mg_of(const std::function<Vec(const Vec&)>& M, const char* w
     if (!p) throw std::invalid_argument(std::string(who) + ": the preconditioner must be MGHierarchy::as_precond()");
     return p->mg;
 }
-inline CGResult cg(const OpApply& A, const Vec& rhs, const Vec* x0, MGHierarchy* mg, int max_iter, double tol,
-                   const char* who) {
-    const DiracOp& D = dirac_of(A, who);
-    CGResult r{Vec(D.lat.ndof), 0, 0.0};
+inline CGResult cg(const OpApply& A, int n, const Vec& rhs, const Vec* x0,
+                   const std::function<Vec(const Vec&)>* precond, int max_iter, double tol, const char* who) {
+    check_size(rhs, n, who);
+    if (x0) check_size(*x0, n, who);
+    CallbackBox ba, bm;
+    const smg_op op = make_op(A, (size_t)n, ba);
+    const smg_precond pc = make_prec(precond, (size_t)n, bm);
+    CGResult r{Vec(n), 0, 0.0};
     smg_solve_info info{};
-    check(smg_cg_solve(D.handle(), SMG_OP_DDAGD, mg ? mg->handle() : nullptr, raw(rhs), x0 ? raw(*x0) : nullptr,
-                       raw(r.solution), max_iter, tol, &info));
+    check_cb(smg_cg_solve_op(&op, &pc, raw(rhs), x0 ? raw(*x0) : nullptr, raw(r.solution), max_iter, tol, &info),
+             ba, bm);
     r.iterations = info.iterations;
     r.final_residual = info.final_residual;
     return r;
 }
 }  // namespace detail
 
-// cg_solve family, src/solvers.cpp:16-96
+// cg_solve family, src/solvers.cpp:16-96 (any operator and preconditioner)
 inline CGResult cg_solve(const OpApply& A, int n, const Vec& rhs, int max_iter, double tol) {
-    (void)n;
-    return detail::cg(A, rhs, nullptr, nullptr, max_iter, tol, "cg_solve");
+    return detail::cg(A, n, rhs, nullptr, nullptr, max_iter, tol, "cg_solve");
 }
 inline CGResult cg_solve_x0(const OpApply& A, int n, const Vec& rhs, const Vec& x0, int max_iter, double tol) {
-    (void)n;
-    return detail::cg(A, rhs, &x0, nullptr, max_iter, tol, "cg_solve_x0");
+    return detail::cg(A, n, rhs, &x0, nullptr, max_iter, tol, "cg_solve_x0");
 }
 inline CGResult cg_solve_precond(const OpApply& A, int n, const Vec& rhs,
                                  const std::function<Vec(const Vec&)>& precond, int max_iter, double tol) {
-    (void)n;
-    return detail::cg(A, rhs, nullptr, detail::mg_of(precond, "cg_solve_precond"), max_iter, tol, "cg_solve_precond");
+    return detail::cg(A, n, rhs, nullptr, &precond, max_iter, tol, "cg_solve_precond");
 }
 inline CGResult cg_solve_x0_precond(const OpApply& A, int n, const Vec& rhs, const Vec& x0,
                                     const std::function<Vec(const Vec&)>& precond, int max_iter, double tol) {
-    (void)n;
-    return detail::cg(A, rhs, &x0, detail::mg_of(precond, "cg_solve_x0_precond"), max_iter, tol,
-                      "cg_solve_x0_precond");
+    return detail::cg(A, n, rhs, &x0, &precond, max_iter, tol, "cg_solve_x0_precond");
 }
 
 enum class Precision { FP64 = SMG_F64, Mixed = SMG_F32 };
@@ -689,13 +804,15 @@ enum class Precision { FP64 = SMG_F64, Mixed = SMG_F32 };
 // projection coefficient, src/solvers.cpp:261, FP64)
 inline FCGResult fcg_solve(const OpApply& A, int n, const Vec& rhs, const std::function<Vec(const Vec&)>& precond,
                            int max_iter, double tol, int trunc = 30) {
-    (void)n;
-    const DiracOp& D = detail::dirac_of(A, "fcg_solve");
-    MGHierarchy* mg = detail::mg_of(precond, "fcg_solve");
-    FCGResult r{Vec(D.lat.ndof), 0, 0.0};
+    detail::check_size(rhs, n, "fcg_solve");
+    detail::CallbackBox ba, bm;
+    const smg_op op = detail::make_op(A, (size_t)n, ba);
+    const smg_precond pc = detail::make_prec(&precond, (size_t)n, bm);
+    FCGResult r{Vec(n), 0, 0.0};
     smg_solve_info info{};
-    detail::check(smg_gcr_solve(D.handle(), mg->handle(), SMG_F64, detail::raw(rhs), detail::raw(r.solution),
-                                max_iter, tol, trunc, 1, 1e-5, &info));
+    detail::check_cb(smg_gcr_solve_op(&op, &pc, detail::raw(rhs), detail::raw(r.solution), max_iter, tol, trunc, 1,
+                                      &info),
+                     ba, bm);
     r.iterations = info.iterations;
     r.final_residual = info.final_residual;
     return r;
@@ -725,20 +842,20 @@ inline FCGResult gcr_solve_mg(const OpApply& A, int n, const Vec& rhs, MGHierarc
 inline FGMRESResult fgmres_solve_generic(const OpApply& A, int n, const Vec& rhs,
                                          const std::function<Vec(const Vec&)>& precond, int restart, int max_iter,
                                          double tol, int n_ritz_harvest = 4) {
-    (void)n;
-    const DiracOp& D = detail::dirac_of(A, "fgmres_solve_generic");
-    const bool ident = !precond || precond.target<IdentityPrecond>() != nullptr;
-    MGHierarchy* mg = ident ? nullptr : detail::mg_of(precond, "fgmres_solve_generic");
+    detail::check_size(rhs, n, "fgmres_solve_generic");
+    detail::CallbackBox ba, bm;
+    const smg_op op = detail::make_op(A, (size_t)n, ba);
+    const smg_precond pc = detail::make_prec(&precond, (size_t)n, bm);
     const int k = n_ritz_harvest > 0 ? n_ritz_harvest : 0;
-    const size_t nd = (size_t)D.lat.ndof;
+    const size_t nd = (size_t)n;
     FGMRESResult r{Vec(nd), 0, 0.0, {}};
     smg_solve_info info{};
     std::vector<double> vals(2 * (size_t)std::max(k, 1));
     Vec vecs(2 * (size_t)std::max(k, 1) * nd);
     int npairs = 0;
-    detail::check(smg_fgmres_solve_ritz(D.handle(), mg ? mg->handle() : nullptr, detail::raw(rhs),
-                                        detail::raw(r.solution), restart, max_iter, tol, k, &info, &npairs,
-                                        vals.data(), detail::raw(vecs)));
+    detail::check_cb(smg_fgmres_solve_op(&op, &pc, detail::raw(rhs), detail::raw(r.solution), restart, max_iter, tol,
+                                         k, &info, &npairs, vals.data(), detail::raw(vecs)),
+                     ba, bm);
     r.iterations = info.iterations;
     r.final_residual = info.final_residual;
     for (int q = 0; q < npairs; q++)

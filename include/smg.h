@@ -254,6 +254,49 @@ int smg_fgmres_solve(smg_dirac* D, smg_mg* mg, const double* b, double* x, int r
 int smg_fgmres_solve_ritz(smg_dirac* D, smg_mg* mg, const double* b, double* x, int restart, int maxit, double tol,
                           int n_ritz, smg_solve_info* info, int* n_pairs, double* ritz_vals, double* ritz_vecs);
 
+/* ------------------------------------------------------------------ caller-supplied operators
+ * The reference's plug-in types: OpApply = std::function<void(const Vec& src, Vec& dst)>
+ * (src/types.h:10) and the preconditioner std::function<Vec(const Vec&)>
+ * (src/solvers.h:45-84).  An smg_op names exactly one of: a Dirac handle (its D^dag D,
+ * on the device), a coarse handle (its 9-point operator, on the device) or a host
+ * callback fn(ctx, src, dst) on n complex numbers (interleaved doubles; return 0, or
+ * nonzero to abort the call with an error).  An smg_precond names a hierarchy (its
+ * FP64 cycle), a host callback, or neither (the identity).  The Krylov vectors,
+ * reductions and updates stay on the device; a callback's input is copied to the host
+ * before each call and its output back after it.  Callbacks run on the calling thread,
+ * one at a time. */
+typedef int (*smg_op_fn)(void* ctx, const double* src, double* dst);
+typedef struct {
+    size_t n;           /* dimension (complex entries): required for fn, checked otherwise (0: any) */
+    smg_op_fn fn;
+    void* ctx;
+    smg_dirac* dirac;
+    smg_coarse* coarse;
+} smg_op;
+typedef struct {
+    smg_op_fn fn;
+    void* ctx;
+    smg_mg* mg;
+} smg_precond;
+/* cg_solve / cg_solve_x0 / cg_solve_precond / cg_solve_x0_precond, src/solvers.cpp:16-96,
+ * on any operator and preconditioner (x0 may be NULL; M may be NULL: none) */
+int smg_cg_solve_op(const smg_op* A, const smg_precond* M, const double* b, const double* x0, double* x, int maxit,
+                    double tol, smg_solve_info* info);
+/* fcg_solve, src/solvers.cpp:223-300 (ref_projection = 1: the reference's coefficient) */
+int smg_gcr_solve_op(const smg_op* A, const smg_precond* M, const double* b, double* x, int maxit, double tol,
+                     int trunc, int ref_projection, smg_solve_info* info);
+/* fgmres_solve_generic, src/solvers.cpp:555-800, with n_ritz >= 0 Ritz pairs harvested
+ * (outputs as smg_fgmres_solve_ritz) */
+int smg_fgmres_solve_op(const smg_op* A, const smg_precond* M, const double* b, double* x, int restart, int maxit,
+                        double tol, int n_ritz, smg_solve_info* info, int* n_pairs, double* ritz_vals,
+                        double* ritz_vecs);
+/* mr_smooth_op / richardson_smooth_op, src/smoother.cpp:26-73 */
+int smg_smooth_op(const smg_op* A, int kind, double* x, const double* b, int niter, double omega);
+/* estimate_lambda_max, src/smoother.cpp:43-59, from the start vector v0 */
+int smg_lambda_max_op(const smg_op* A, const double* v0, int niter, double* out);
+/* SparseCoarseOp::build(P, A, fine_dim), src/coarse_op.h:241-279, for any operator A */
+int smg_coarse_create_galerkin_op(smg_coarse** out, const smg_op* A, smg_prolongator* P);
+
 /* ------------------------------------------------------------------ eigensolvers and deflation */
 /* random_vec (src/linalg.cpp:44-49) from std::mt19937(seed): n complex numbers of
  * unit-variance normal parts, in the order the reference's g++ build draws them.
@@ -369,6 +412,14 @@ int smg_coarse_apply_batched(smg_coarse* C, int prec, int nrhs, const double* sr
 int smg_coarse_apply_batched_dev(smg_coarse* C, int prec, int nrhs, const void* src, void* dst, void* stream);
 /* one multigrid cycle applied to N right-hand sides at once (host arrays) */
 int smg_mg_precondition_batched(smg_mg* mg, int prec, int nrhs, const double* B, double* X);
+/* the hierarchy's level-`level` (>= 1) coarse operator on nrhs device vectors in the
+ * batched layout (SparseCoarseOp::apply_to, src/coarse_op.h:282-297, as the cycle runs
+ * it): prec SMG_F64 / SMG_F32 (links of that precision), SMG_F16 (FP16 links, FP32
+ * vectors) or SMG_BF16V (FP16 links, BF16 vectors: one dword per complex entry); with
+ * dots = 1 the launch also reduces {<dst, src>, |dst|^2} per right-hand side, as the
+ * smoother and the coarse CG do (results discarded) */
+int smg_mg_coarse_apply_dev(smg_mg* mg, int level, int prec, int nrhs, const void* src, void* dst, int dots,
+                            void* stream);
 /* smg_gcr_solve for N right-hand sides in lockstep (textbook GCR projection);
  * true_res (optional, N doubles) receives each RHS's FP64 true relative residual */
 int smg_gcr_solve_batched(smg_dirac* D, smg_mg* mg, int prec, int nrhs, const double* B, double* X, int maxit,

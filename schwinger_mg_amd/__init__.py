@@ -74,6 +74,12 @@ SIGNATURES = [
     ("smg_eo_apply", _I, [_P, _I, _P, _P]), ("smg_eo_apply_dev", _I, [_P, _I, _I, _P, _P, _P]),
     ("smg_eo_reconstruct", _I, [_P, _P, _P, _P]), ("smg_eo_logdet", _I, [_P, _P]),
     ("smg_dot_dev", _I, [_P, _P, _SZ, _I, _P, _P]), ("smg_norm_dev", _I, [_P, _SZ, _I, _P, _P]),
+    ("smg_cg_solve_op", _I, [_P, _P, _P, _P, _P, _I, _D, _P]),
+    ("smg_gcr_solve_op", _I, [_P, _P, _P, _P, _I, _D, _I, _I, _P]),
+    ("smg_fgmres_solve_op", _I, [_P, _P, _P, _P, _I, _I, _D, _I, _P, _P, _P, _P]),
+    ("smg_smooth_op", _I, [_P, _I, _P, _P, _I, _D]), ("smg_lambda_max_op", _I, [_P, _P, _I, _P]),
+    ("smg_coarse_create_galerkin_op", _I, [_P, _P, _P]),
+    ("smg_mg_coarse_apply_dev", _I, [_P, _I, _I, _I, _P, _P, _I, _P]),
     ("smg_tune_mg", _I, [_P, _I, _P]), ("smg_tune_save", _I, [ctypes.c_char_p]),
     ("smg_tune_load", _I, [ctypes.c_char_p]), ("smg_tune_clear", _I, []), ("smg_tune_get", _I, [_I, _P, _P]),
     ("smg_tune_set", _I, [_I, _P, _I]),

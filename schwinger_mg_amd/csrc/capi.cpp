@@ -877,6 +877,44 @@ int smg_coarse_apply_batched_dev(smg_coarse* C, int prec, int nrhs, const void* 
         }
     });
 }
+int smg_mg_coarse_apply_dev(smg_mg* mg, int level, int prec, int nrhs, const void* src, void* dst, int dots,
+                            void* stream) {
+    return guard([&] {
+        SMG_REQUIRE(mg && src && dst, "bad arguments");
+        MG& m = *mg->mg;
+        SMG_REQUIRE(level >= 1 && level < (int)m.dims.size(), "level must name a coarse level of the hierarchy");
+        check_nrhs(nrhs, 64);
+        SMG_REQUIRE(prec == SMG_F64 || prec == SMG_F32 || prec == SMG_F16 || prec == SMG_BF16V,
+                    "prec must be SMG_F64, SMG_F32, SMG_F16 or SMG_BF16V");
+        const CoarseOperator& C = *m.C[level - 1];
+        StreamJoin j(stream);
+        // the dot products {<dst, src>, |dst|^2} per right-hand side go to a scratch block
+        static thread_local std::unique_ptr<Field<double>> dots_out;
+        if (!dots_out) dots_out.reset(new Field<double>(3 * 64));
+        FinB fin;
+        if (dots) {
+            const Fin f = Context::get().fin(FIN_STORE, reinterpret_cast<double*>(dots_out->p()));
+            fin.op = FIN_STORE;
+            fin.out = f.out;
+            fin.ostride = 3;
+            fin.part = f.part;
+            fin.counter = f.counter;
+        }
+        const Epi epi = dots ? EPI_DOTS : EPI_NONE;
+        if (prec == SMG_F64) {
+            CoarseLinOpB<double, c64> op(C, nrhs);
+            op.apply((c64*)dst, (const c64*)src, epi, nullptr, (const c64*)src, fin, S());
+        } else if (prec == SMG_F32) {
+            CoarseLinOpB<float, c32> op(C, nrhs);
+            op.apply((c32*)dst, (const c32*)src, epi, nullptr, (const c32*)src, fin, S());
+        } else if (prec == SMG_F16) {
+            CoarseLinOpB<float, ch16> op(C, nrhs);
+            op.apply((c32*)dst, (const c32*)src, epi, nullptr, (const c32*)src, fin, S());
+        } else {
+            launch_b_coarse_apply_h(C.view16(), (cb16*)dst, (const cb16*)src, epi, (const cb16*)src, fin, nrhs, S());
+        }
+    });
+}
 int smg_mg_precondition_batched(smg_mg* mg, int prec, int nrhs, const double* B, double* X) {
     return guard([&] {
         SMG_REQUIRE(mg, "null handle");
@@ -2079,6 +2117,166 @@ int smg_mg_coarsest_deflation(smg_mg* mg, int* ndefl, double* evals, double* eve
         if (ndefl) *ndefl = nd;
         if (evals) std::copy(m.defl_vals.begin(), m.defl_vals.end(), evals);
         if (evecs && nd) downc(evecs, m.defl_vecs.p(), (size_t)nd * m.dims.back());
+    });
+}
+
+// ------------------------------------------------------------------ caller-supplied operators
+namespace {
+// The operator of an smg_op: D^dag D of a Dirac handle, a coarse handle's operator, or
+// the caller's host callback.  A callback is called with host arrays of n complex
+// numbers; its input is copied from the device before the call and its output back
+// after it, and the Krylov vectors, reductions and updates stay on the device.
+struct CallbackOp : LinOp<double> {
+    size_t nn;
+    smg_op_fn fn;
+    void* ctx;
+    std::vector<c64> hin, hout;
+    CallbackOp(size_t n_, smg_op_fn f, void* c) : nn(n_), fn(f), ctx(c), hin(n_), hout(n_) {}
+    size_t n() const override { return nn; }
+    void apply(c64* dst, const c64* src, Epi epi, const c64* b, const c64* aux, const Fin& fin,
+               hipStream_t st) override {
+        SMG_HIP(hipMemcpyAsync(hin.data(), src, nn * sizeof(c64), hipMemcpyDeviceToHost, st));
+        SMG_HIP(hipStreamSynchronize(st));
+        SMG_REQUIRE(fn(ctx, reinterpret_cast<const double*>(hin.data()), reinterpret_cast<double*>(hout.data())) == 0,
+                    "the operator callback reported a failure");
+        SMG_HIP(hipMemcpyAsync(dst, hout.data(), nn * sizeof(c64), hipMemcpyHostToDevice, st));
+        SMG_HIP(hipStreamSynchronize(st));   // hout may be overwritten by the next call
+        if (epi == EPI_RESID) launch_axpby<double>(dst, b, c64{1.0, 0.0}, c64{-1.0, 0.0}, nn, st);
+        else if (epi == EPI_DOTS) launch_dots<double>(dst, aux, nn, fin, st);
+    }
+    using LinOp<double>::apply;
+};
+struct CallbackPrec : Preconditioner {
+    size_t nn;
+    smg_op_fn fn;
+    void* ctx;
+    std::vector<c64> hin, hout;
+    CallbackPrec(size_t n_, smg_op_fn f, void* c) : nn(n_), fn(f), ctx(c), hin(n_), hout(n_) {}
+    void precondition(const c64* b, c64* x, hipStream_t st) override {
+        SMG_HIP(hipMemcpyAsync(hin.data(), b, nn * sizeof(c64), hipMemcpyDeviceToHost, st));
+        SMG_HIP(hipStreamSynchronize(st));
+        SMG_REQUIRE(fn(ctx, reinterpret_cast<const double*>(hin.data()), reinterpret_cast<double*>(hout.data())) == 0,
+                    "the preconditioner callback reported a failure");
+        SMG_HIP(hipMemcpyAsync(x, hout.data(), nn * sizeof(c64), hipMemcpyHostToDevice, st));
+        SMG_HIP(hipStreamSynchronize(st));
+    }
+    using Preconditioner::precondition;
+};
+// the operator an smg_op describes (own holds what had to be built)
+LinOp<double>& op_of(const smg_op* A, std::unique_ptr<LinOp<double>>& own) {
+    SMG_REQUIRE(A, "null operator");
+    const int set = (A->dirac != nullptr) + (A->coarse != nullptr) + (A->fn != nullptr);
+    SMG_REQUIRE(set == 1, "an smg_op names exactly one of dirac, coarse, fn");
+    if (A->dirac) {
+        SMG_REQUIRE(A->n == 0 || A->n == A->dirac->op64->n(), "smg_op: n does not match the Dirac operator");
+        return *A->dirac->op64;
+    }
+    if (A->coarse) {
+        SMG_REQUIRE(A->n == 0 || A->n == A->coarse->C->n(), "smg_op: n does not match the coarse operator");
+        own.reset(new CoarseLinOp<double, c64>(*A->coarse->C));
+        return *own;
+    }
+    SMG_REQUIRE(A->n > 0, "smg_op: a callback operator needs its dimension n");
+    own.reset(new CallbackOp(A->n, A->fn, A->ctx));
+    return *own;
+}
+Preconditioner* prec_of(const smg_precond* M, size_t n, std::unique_ptr<Preconditioner>& own) {
+    if (!M || (!M->mg && !M->fn)) return nullptr;   // the identity
+    SMG_REQUIRE(!(M->mg && M->fn), "an smg_precond names at most one of mg, fn");
+    if (M->mg) {
+        SMG_REQUIRE(M->mg->mg->dims[0] == n, "smg_precond: the hierarchy's fine dimension does not match");
+        return M->mg->mg.get();
+    }
+    own.reset(new CallbackPrec(n, M->fn, M->ctx));
+    return own.get();
+}
+}  // namespace
+
+int smg_cg_solve_op(const smg_op* A, const smg_precond* M, const double* b, const double* x0, double* x, int maxit,
+                    double tol, smg_solve_info* info) {
+    return guard([&] {
+        SMG_REQUIRE(b && x, "null argument");
+        std::unique_ptr<LinOp<double>> oa;
+        std::unique_ptr<Preconditioner> om;
+        LinOp<double>& op = op_of(A, oa);
+        const size_t n = op.n();
+        Preconditioner* pm = prec_of(M, n, om);
+        Field<double> bb = up(b, n), xx(n), xi;
+        if (x0) xi = up(x0, n);
+        SolveResult r = cg_solve(op, bb.p(), xx.p(), x0 ? xi.p() : nullptr, pm, maxit, tol, S());
+        r.true_residual = true_residual(op, bb.p(), xx.p(), S());
+        fill_info(info, r);
+        downc(x, xx.p(), n);
+    });
+}
+int smg_gcr_solve_op(const smg_op* A, const smg_precond* M, const double* b, double* x, int maxit, double tol,
+                     int trunc, int ref_projection, smg_solve_info* info) {
+    return guard([&] {
+        SMG_REQUIRE(b && x, "null argument");
+        std::unique_ptr<LinOp<double>> oa;
+        std::unique_ptr<Preconditioner> om;
+        LinOp<double>& op = op_of(A, oa);
+        const size_t n = op.n();
+        Preconditioner* pm = prec_of(M, n, om);
+        Field<double> bb = up(b, n), xx(n);
+        SolveResult r = gcr_solve<double>(op, bb.p(), xx.p(), pm, maxit, tol, trunc, ref_projection != 0, S());
+        r.true_residual = true_residual(op, bb.p(), xx.p(), S());
+        fill_info(info, r);
+        downc(x, xx.p(), n);
+    });
+}
+int smg_fgmres_solve_op(const smg_op* A, const smg_precond* M, const double* b, double* x, int restart, int maxit,
+                        double tol, int n_ritz, smg_solve_info* info, int* n_pairs, double* ritz_vals,
+                        double* ritz_vecs) {
+    return guard([&] {
+        SMG_REQUIRE(b && x && restart >= 1 && n_ritz >= 0, "bad arguments");
+        SMG_REQUIRE(n_ritz == 0 || (n_pairs && ritz_vals && ritz_vecs), "ritz output buffers required");
+        std::unique_ptr<LinOp<double>> oa;
+        std::unique_ptr<Preconditioner> om;
+        LinOp<double>& op = op_of(A, oa);
+        const size_t n = op.n();
+        Preconditioner* pm = prec_of(M, n, om);
+        Field<double> bb = up(b, n), xx(n);
+        std::unique_ptr<RitzHarvest> rh;
+        if (n_ritz > 0) rh.reset(new RitzHarvest(n_ritz, restart, n));
+        SolveResult r = fgmres_solve(op, bb.p(), xx.p(), pm, restart, maxit, tol, S(), rh.get());
+        fill_info(info, r);
+        downc(x, xx.p(), n);
+        if (n_pairs) *n_pairs = rh ? rh->count() : 0;
+        if (rh && rh->count() > 0) {
+            std::copy(rh->vals.begin(), rh->vals.end(), ritz_vals);
+            downc(ritz_vecs, rh->vecs.p(), (size_t)rh->count() * n);
+        }
+    });
+}
+int smg_smooth_op(const smg_op* A, int kind, double* x, const double* b, int niter, double omega) {
+    return guard([&] {
+        SMG_REQUIRE(x && b, "null argument");
+        std::unique_ptr<LinOp<double>> oa;
+        smooth_op(op_of(A, oa), kind, x, b, niter, omega);
+    });
+}
+int smg_lambda_max_op(const smg_op* A, const double* v0, int niter, double* out) {
+    return guard([&] {
+        SMG_REQUIRE(v0 && out, "null argument");
+        std::unique_ptr<LinOp<double>> oa;
+        LinOp<double>& op = op_of(A, oa);
+        Field<double> v = up(v0, op.n()), Av(op.n());
+        *out = lambda_max<double>(op, v.p(), Av.p(), niter, S());
+    });
+}
+int smg_coarse_create_galerkin_op(smg_coarse** out, const smg_op* A, smg_prolongator* P) {
+    return guard([&] {
+        SMG_REQUIRE(out && P, "null argument");
+        std::unique_ptr<LinOp<double>> oa;
+        LinOp<double>& op = op_of(A, oa);
+        SMG_REQUIRE(op.n() == P->P->fine_n(), "the operator's dimension does not match the prolongator's fine side");
+        auto* h = new smg_coarse_s();
+        try {
+            h->C.reset(new CoarseOperator(P->P->A.nbx, P->P->A.nby, P->P->A.k));
+            h->C->build(op, *P->P, S());
+        } catch (...) { delete h; throw; }
+        *out = h;
     });
 }
 

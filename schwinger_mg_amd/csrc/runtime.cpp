@@ -2,6 +2,7 @@
 #include "runtime.h"
 
 #include "eigen.h"
+#include "runtime_b.h"
 
 #include <cmath>
 #include <cstring>
@@ -618,6 +619,7 @@ void MG::refresh(int iters, hipStream_t st) {
 }
 
 void MG::reset_graphs() {
+    release_batched_cycles(this);   // the batched cycles' graphs too
     for (auto& g : graph) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
         if (g.g) (void)hipGraphDestroy(g.g);
@@ -706,13 +708,13 @@ double true_residual(LinOp<double>& A, const c64* b, const c64* x, hipStream_t s
 }
 
 template <typename T>
-static void precond_apply(MG* mg, const cplx<T>* r, cplx<T>* z, size_t n, hipStream_t st) {
+static void precond_apply(Preconditioner* mg, const cplx<T>* r, cplx<T>* z, size_t n, hipStream_t st) {
     if (mg) mg->precondition(r, z, st);
     else dcopy(z, r, n, st);
 }
 
 // src/solvers.cpp:16-72
-SolveResult cg_solve(LinOp<double>& A, const c64* b, c64* x, const c64* x0, MG* mg, int maxit, double tol,
+SolveResult cg_solve(LinOp<double>& A, const c64* b, c64* x, const c64* x0, Preconditioner* mg, int maxit, double tol,
                      hipStream_t st) {
     SolveResult res;
     const size_t n = A.n();
@@ -762,7 +764,7 @@ SolveResult cg_solve(LinOp<double>& A, const c64* b, c64* x, const c64* x0, MG* 
 
 // src/solvers.cpp:223-300
 template <typename T>
-SolveResult gcr_solve(LinOp<T>& A, const cplx<T>* b, cplx<T>* x, MG* mg, int maxit, double tol, int trunc,
+SolveResult gcr_solve(LinOp<T>& A, const cplx<T>* b, cplx<T>* x, Preconditioner* mg, int maxit, double tol, int trunc,
                       bool ref_projection, hipStream_t st) {
     SolveResult res;
     const size_t n = A.n();
@@ -836,10 +838,12 @@ SolveResult gcr_solve(LinOp<T>& A, const cplx<T>* b, cplx<T>* x, MG* mg, int max
     res.final_residual = rn / bnorm;
     return res;
 }
-template SolveResult gcr_solve<double>(LinOp<double>&, const c64*, c64*, MG*, int, double, int, bool, hipStream_t);
-template SolveResult gcr_solve<float>(LinOp<float>&, const c32*, c32*, MG*, int, double, int, bool, hipStream_t);
+template SolveResult gcr_solve<double>(LinOp<double>&, const c64*, c64*, Preconditioner*, int, double, int, bool,
+                                        hipStream_t);
+template SolveResult gcr_solve<float>(LinOp<float>&, const c32*, c32*, Preconditioner*, int, double, int, bool,
+                                       hipStream_t);
 
-SolveResult gcr_solve_mixed(LinOp<double>& A64, LinOp<float>& A32, const c64* b, c64* x, MG* mg, int maxit,
+SolveResult gcr_solve_mixed(LinOp<double>& A64, LinOp<float>& A32, const c64* b, c64* x, Preconditioner* mg, int maxit,
                             double tol, int trunc, double inner_tol, hipStream_t st, bool use_x0) {
     SolveResult res;
     const size_t n = A64.n();
@@ -884,7 +888,7 @@ SolveResult gcr_solve_mixed(LinOp<double>& A64, LinOp<float>& A32, const c64* b,
 }
 
 // src/solvers.cpp:555-772; the Ritz harvest and intra-solve deflation when ritz
-SolveResult fgmres_solve(LinOp<double>& A, const c64* b, c64* x, MG* mg, int restart, int maxit, double tol,
+SolveResult fgmres_solve(LinOp<double>& A, const c64* b, c64* x, Preconditioner* mg, int restart, int maxit, double tol,
                          hipStream_t st, RitzHarvest* ritz) {
     SolveResult res;
     const size_t n = A.n();

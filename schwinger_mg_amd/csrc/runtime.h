@@ -223,7 +223,18 @@ struct Cycle;
 struct Eigenpairs;
 struct RitzHarvest;
 
-struct MG {
+// A preconditioner z = M r on device vectors: the MG cycle, or a caller's callable
+// (the reference's std::function<Vec(const Vec&)>, src/solvers.h:45-84).
+struct Preconditioner {
+    virtual ~Preconditioner() = default;
+    virtual void precondition(const c64* b, c64* x, hipStream_t st) = 0;
+    virtual void precondition(const c32* b, c32* x, hipStream_t st) {
+        (void)b, (void)x, (void)st;
+        throw Error(-1, "this preconditioner has no FP32 form");
+    }
+};
+
+struct MG : Preconditioner {
     MGConfig cfg;
     const Dirac* D = nullptr;
     std::vector<std::unique_ptr<Aggregation>> P;
@@ -271,8 +282,8 @@ struct MG {
     // current values on the current operator, then the aggregation and the Galerkin
     // operators rebuilt (the reference warm-starts the fine level only)
     void refresh(int iters, hipStream_t st);
-    void precondition(const c64* b, c64* x, hipStream_t st);
-    void precondition(const c32* b, c32* x, hipStream_t st);
+    void precondition(const c64* b, c64* x, hipStream_t st) override;
+    void precondition(const c32* b, c32* x, hipStream_t st) override;
     template <typename T>
     Cycle<T>& cycle();
     LinOp<double>& level_op64(int l);
@@ -287,21 +298,21 @@ struct SolveResult {
 };
 
 // cg_solve_core (src/solvers.cpp:16-72); x0 may be null; mg may be null.
-SolveResult cg_solve(LinOp<double>& A, const c64* b, c64* x, const c64* x0, MG* mg, int maxit, double tol,
+SolveResult cg_solve(LinOp<double>& A, const c64* b, c64* x, const c64* x0, Preconditioner* mg, int maxit, double tol,
                      hipStream_t st);
 // fcg_solve (src/solvers.cpp:223-300) in precision T; ref_projection selects the
 // reference's conjugated projection coefficient.
 template <typename T>
-SolveResult gcr_solve(LinOp<T>& A, const cplx<T>* b, cplx<T>* x, MG* mg, int maxit, double tol, int trunc,
+SolveResult gcr_solve(LinOp<T>& A, const cplx<T>* b, cplx<T>* x, Preconditioner* mg, int maxit, double tol, int trunc,
                       bool ref_projection, hipStream_t st);
 // Mixed precision: FP64 iterative refinement (true residual in FP64) around FP32
 // GCR(trunc) preconditioned by the FP32 MG cycle.
 // With use_x0 the solve starts from the x it is given (its FP64 true residual first).
-SolveResult gcr_solve_mixed(LinOp<double>& A64, LinOp<float>& A32, const c64* b, c64* x, MG* mg, int maxit,
+SolveResult gcr_solve_mixed(LinOp<double>& A64, LinOp<float>& A32, const c64* b, c64* x, Preconditioner* mg, int maxit,
                             double tol, int trunc, double inner_tol, hipStream_t st, bool use_x0 = false);
 // fgmres_solve_generic (src/solvers.cpp:555-772): no Ritz harvest when ritz is null,
 // else the harvest and intra-solve deflation that ritz carries
-SolveResult fgmres_solve(LinOp<double>& A, const c64* b, c64* x, MG* mg, int restart, int maxit, double tol,
+SolveResult fgmres_solve(LinOp<double>& A, const c64* b, c64* x, Preconditioner* mg, int restart, int maxit, double tol,
                          hipStream_t st, RitzHarvest* ritz = nullptr);
 
 double true_residual(LinOp<double>& A, const c64* b, const c64* x, hipStream_t st);

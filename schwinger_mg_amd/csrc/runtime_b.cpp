@@ -190,6 +190,14 @@ struct CycleB {
     Field<T> dwork;   // column-slice partials of the matrix-core dense coarsest product
     bool hcg = false;  // coarsest CG on BF16 vectors (FP32 cycle, cfg.coarse_half == 2)
     DevMem hr, hp, hAp, hx;
+    // the cycle captured as a hipGraph (cfg.use_graph) on fixed input/output buffers
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    Field<T> gin, gout;
+    ~CycleB() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
     CycleB(MG& m, int N_)
         : mg(m), N(N_), nlev(m.cfg.n_levels), dwork(std::max<size_t>(1, b_dense_work(m.dims[m.cfg.n_levels - 1], N_))) {
         SMG_REQUIRE(N >= 1 && N <= 64 && (N & (N - 1)) == 0, "batched solves take a power-of-two count <= 64");
@@ -301,9 +309,38 @@ void release_batched_cycles(MG* mg) {
         it = (it->first.first == mg) ? cyc32_map().erase(it) : std::next(it);
 }
 
+// With cfg.use_graph the cycle is captured into a hipGraph on its first use, on fixed
+// input/output buffers, and replayed: every kernel of the cycle reads its scalars on
+// the device, none synchronises, allocates or clears memory with a memset, so it
+// captures as is.  A hierarchy rebuilt, refreshed or retuned drops its batched cycles,
+// and so their graphs.
 template <typename T>
 void mg_precondition_b(MG& mg, const cplx<T>* b, cplx<T>* x, int N, hipStream_t st) {
-    batched_cycle<T>(mg, N).run(0, b, x, st);
+    CycleB<T>& c = batched_cycle<T>(mg, N);
+    if (!mg.cfg.use_graph) {
+        c.run(0, b, x, st);
+        return;
+    }
+    const size_t nN = mg.dims[0] * N;
+    if (!c.gexec) {
+        c.gin = Field<T>(nN);
+        c.gout = Field<T>(nN);
+        SMG_HIP(hipStreamSynchronize(st));
+        SMG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            c.run(0, c.gin.p(), c.gout.p(), st);
+        } catch (...) {
+            hipGraph_t tmp = nullptr;
+            (void)hipStreamEndCapture(st, &tmp);
+            if (tmp) (void)hipGraphDestroy(tmp);
+            throw;
+        }
+        SMG_HIP(hipStreamEndCapture(st, &c.graph));
+        SMG_HIP(hipGraphInstantiate(&c.gexec, c.graph, nullptr, nullptr, 0));
+    }
+    launch_copy<T>(c.gin.p(), b, nN, st);
+    SMG_HIP(hipGraphLaunch(c.gexec, st));
+    launch_copy<T>(x, c.gout.p(), nN, st);
 }
 template void mg_precondition_b<double>(MG&, const c64*, c64*, int, hipStream_t);
 template void mg_precondition_b<float>(MG&, const c32*, c32*, int, hipStream_t);

@@ -182,3 +182,31 @@ def test_mixed_solve_with_bf16_coarsest_cg(smg, w_cycle):
     print(f"iterations: FP32 coarsest CG {out[1][1]}, BF16 coarsest CG {out[2][1]}")
     assert out[2][1] <= out[1][1] + 3
     assert relerr(out[2][0], out[1][0]) < 1e-8
+
+
+@pytest.mark.parametrize("half", [1, 2])
+def test_batched_cycle_graph_replays_equal_eager(smg, half):
+    """The batched cycle captured as a hipGraph (use_graph): three replays per precision,
+    with eager work between them, equal the eager cycle bit for bit (FP32 with FP16 links
+    and, for half = 2, BF16 coarsest-CG vectors; FP64), and a batched mixed solve through
+    the graph equals the eager solve."""
+    banner(f"batched cycle hipGraph replays, coarse_half={half}")
+    L, k, N = 32, 32, 32
+    V = L * L
+    U = random_gauge(L, 0.4, 90)
+    D = smg.DiracOp(L, U, 0.05, 1.0, 1.0, 0.0)
+    dims = [2 * V, (L // 4) ** 2 * k]
+    starts = np.concatenate([gaussian_vecs(k // 2, n, 91 + i).ravel() for i, n in enumerate(dims)])
+    mg = smg.MGHierarchy(D, starts, n_levels=3, blocks=((4, 4), (2, 2)), k=(k, k), null_iters=5, coarse_maxit=30,
+                         coarse_tol=0.1, coarse_half=half, chiral=True, w_cycle=False)
+    B = gaussian_vecs(N, 2 * V, 92)
+    eager = {p: mg.precondition_batched(B, p) for p in (smg.F32, smg.F64)}
+    Xe, ie, te = smg.gcr_solve_batched(D, B, mg, 200, 1e-10, 10, prec=smg.F32)
+    mg.set_cycle(3, 3, 30, 0.1, use_graph=True)
+    for rep in range(3):
+        for p in (smg.F32, smg.F64):
+            assert np.array_equal(mg.precondition_batched(B, p), eager[p]), (rep, p)
+        D.apply(B[rep])     # eager work between the replays
+    Xg, ig, tg = smg.gcr_solve_batched(D, B, mg, 200, 1e-10, 10, prec=smg.F32)
+    assert np.array_equal(Xg, Xe) and ig.iterations == ie.iterations
+    assert (tg < 1e-10).all()
