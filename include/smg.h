@@ -639,9 +639,12 @@ int smg_therm_cfg_path(int L, double beta, int n_therm, char* out, size_t cap);
  *     reads 5 of the 9 link blocks of a site (Hermitian Galerkin operators), g indexes the
  *     workgroup counts {128, 256, 512, 1024, 2048} of the launch with dot products;
  *   SMG_TUNE_DSLASH_DOTS: the batched stencil with dot products; (L, nrhs, bytes per
- *     real, 0, 0); choice = g. */
+ *     real, 0, 0); choice = g;
+ *   SMG_TUNE_DSLASH_STORE: the batched stencil's stores; (L, nrhs, bytes per real,
+ *     epilogue, 0); choice 1 = non-temporal stores, 0 = ordinary (bit-identical results). */
 #define SMG_TUNE_COARSE_BF16 0
 #define SMG_TUNE_DSLASH_DOTS 1
+#define SMG_TUNE_DSLASH_STORE 2
 /* times the candidates of every kernel of mg's batched cycle for nrhs (16, 32 or 64)
  * right-hand sides and keeps the fastest; *entries = table entries added (may be NULL) */
 int smg_tune_mg(smg_mg* mg, int nrhs, int* entries);

@@ -1119,7 +1119,7 @@ def therm_cfg_path(L, beta, n_therm):
 
 
 # ------------------------------------------------------------------ launch-shape autotuner
-TUNE_COARSE_BF16, TUNE_DSLASH_DOTS = 0, 1
+TUNE_COARSE_BF16, TUNE_DSLASH_DOTS, TUNE_DSLASH_STORE = 0, 1, 2
 
 
 def tune_mg(mg, nrhs):

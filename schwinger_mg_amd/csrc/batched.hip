@@ -184,6 +184,19 @@ struct RhsRun {
             for (int q = 0; q < P; q++) p[q] = v[q];
         }
     }
+    // a streaming store: the result is not read again by this kernel, so it is written
+    // with the non-temporal hint and does not displace the sources held in the caches
+    __device__ __forceinline__ void store_nt(cplx<T>* __restrict__ p) const {
+        using V4 = typename std::conditional<sizeof(T) == 4, float, double>::type;
+        typedef V4 v2 __attribute__((ext_vector_type(2)));
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        if constexpr (P == 2 && sizeof(T) == 4) {
+            __builtin_nontemporal_store(f4{v[0].re, v[0].im, v[1].re, v[1].im}, reinterpret_cast<f4*>(p));
+        } else {
+#pragma unroll
+            for (int q = 0; q < P; q++) __builtin_nontemporal_store(v2{v[q].re, v[q].im}, reinterpret_cast<v2*>(p + q));
+        }
+    }
 };
 
 // Per-RHS sums of a kernel whose lanes hold runs of P right-hand sides (lane l:
@@ -231,7 +244,7 @@ __host__ inline void run_dispatch(int N, F&& f) {
 // One work-item per (site, run of P right-hand sides).  P = 2 in FP32 halves the
 // load instructions and the index arithmetic per byte: measured at L=512, 32 RHS,
 // 51.5 us per launch against 76.5 us with one right-hand side per work-item.
-template <typename T, int EPI, int P>
+template <typename T, int EPI, int P, bool NT = false>
 __global__ __launch_bounds__(kBlock) void k_b_dslash(DiracView<T> D, cplx<T>* __restrict__ dst,
                                                      const cplx<T>* __restrict__ src, T mu, int g5in, int g5out,
                                                      const cplx<T>* __restrict__ bvec,
@@ -288,8 +301,13 @@ __global__ __launch_bounds__(kBlock) void k_b_dslash(DiracView<T> D, cplx<T>* __
                 acc[2][q] += (double)norm2(o0) + (double)norm2(o1);
             }
         }
-        out[0].store(dst + i0);
-        out[1].store(dst + i1);
+        if constexpr (NT) {
+            out[0].store_nt(dst + i0);
+            out[1].store_nt(dst + i1);
+        } else {
+            out[0].store(dst + i0);
+            out[1].store(dst + i1);
+        }
     }
     if (EPI == EPI_DOTS) {
         double v[3];
@@ -298,22 +316,30 @@ __global__ __launch_bounds__(kBlock) void k_b_dslash(DiracView<T> D, cplx<T>* __
     }
 }
 
-template <typename T, int P>
-static void dslash_p(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T mu, int g5in, int g5out, Epi epi,
-                     const cplx<T>* b, const cplx<T>* aux, const FinB& fin, int N, hipStream_t st) {
+template <typename T, int P, bool NT>
+static void dslash_pn(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T mu, int g5in, int g5out, Epi epi,
+                      const cplx<T>* b, const cplx<T>* aux, const FinB& fin, int N, hipStream_t st) {
     const size_t items = (size_t)D.L * D.L * N / P;
     if (epi == EPI_NONE)
-        k_b_dslash<T, EPI_NONE, P><<<b_grid(items, 1 << 16), kBlock, 0, st>>>(D, dst, src, mu, g5in, g5out, b, aux,
-                                                                            fin, N);
+        k_b_dslash<T, EPI_NONE, P, NT><<<b_grid(items, 1 << 16), kBlock, 0, st>>>(D, dst, src, mu, g5in, g5out, b,
+                                                                                aux, fin, N);
     else if (epi == EPI_RESID)
-        k_b_dslash<T, EPI_RESID, P><<<b_grid(items, 1 << 16), kBlock, 0, st>>>(D, dst, src, mu, g5in, g5out, b, aux,
-                                                                             fin, N);
+        k_b_dslash<T, EPI_RESID, P, NT><<<b_grid(items, 1 << 16), kBlock, 0, st>>>(D, dst, src, mu, g5in, g5out, b,
+                                                                                 aux, fin, N);
     else {
         const int choice = tune_lookup(TuneKey{TUNE_DSLASH_DOTS, D.L, N, (int)sizeof(T), 0, 0});
         const int g = choice < 0 ? kRedGrid : kTuneGrids[choice];
-        k_b_dslash<T, EPI_DOTS, P><<<b_grid(items, g), kBlock, 0, st>>>(D, dst, src, mu, g5in, g5out, b, aux, fin,
-                                                                      N);
+        k_b_dslash<T, EPI_DOTS, P, NT><<<b_grid(items, g), kBlock, 0, st>>>(D, dst, src, mu, g5in, g5out, b, aux,
+                                                                          fin, N);
     }
+}
+// the store hint the tuner picked for this shape and epilogue (default: ordinary stores)
+template <typename T, int P>
+static void dslash_p(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T mu, int g5in, int g5out, Epi epi,
+                     const cplx<T>* b, const cplx<T>* aux, const FinB& fin, int N, hipStream_t st) {
+    const int nt = tune_lookup(TuneKey{TUNE_DSLASH_STORE, D.L, N, (int)sizeof(T), (int)epi, 0});
+    if (nt == 1) dslash_pn<T, P, true>(D, dst, src, mu, g5in, g5out, epi, b, aux, fin, N, st);
+    else dslash_pn<T, P, false>(D, dst, src, mu, g5in, g5out, epi, b, aux, fin, N, st);
 }
 
 template <typename T>

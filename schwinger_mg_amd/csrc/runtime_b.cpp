@@ -559,20 +559,23 @@ int tune_mg(MG& mg, int N, hipStream_t st) {
     const size_t nf = 2 * (size_t)D.V * N;
     std::vector<int> grids;
     for (int g = 0; g < kTuneNGrids; g++) grids.push_back(g);
-    {
-        Field<float> src(nf), dst(nf);
-        launch_zero<float>(src.p(), nf, st);
-        pick(TuneKey{TUNE_DSLASH_DOTS, D.L, N, (int)sizeof(float), 0, 0}, grids, [&] {
-            launch_b_dslash<float>(D.v32(), dst.p(), src.p(), 0.0f, 1, 1, EPI_DOTS, nullptr, src.p(), fin, N, st);
+    // the stencil's store hint per epilogue (the first pass of D^dag D, the residual and
+    // the pass with dots), then the grid of the pass with dots
+    auto stencil = [&](auto zero) {
+        using T = decltype(zero);
+        Field<T> src(nf), dst(nf);
+        launch_zero<T>(src.p(), nf, st);
+        const DiracView<T> v = D.view<T>();
+        for (int epi : {(int)EPI_NONE, (int)EPI_RESID, (int)EPI_DOTS})
+            pick(TuneKey{TUNE_DSLASH_STORE, D.L, N, (int)sizeof(T), epi, 0}, {0, 1}, [&] {
+                launch_b_dslash<T>(v, dst.p(), src.p(), zero, 1, 1, (Epi)epi, src.p(), src.p(), fin, N, st);
+            }, st);
+        pick(TuneKey{TUNE_DSLASH_DOTS, D.L, N, (int)sizeof(T), 0, 0}, grids, [&] {
+            launch_b_dslash<T>(v, dst.p(), src.p(), zero, 1, 1, EPI_DOTS, nullptr, src.p(), fin, N, st);
         }, st);
-    }
-    {
-        Field<double> src(nf), dst(nf);
-        launch_zero<double>(src.p(), nf, st);
-        pick(TuneKey{TUNE_DSLASH_DOTS, D.L, N, (int)sizeof(double), 0, 0}, grids, [&] {
-            launch_b_dslash<double>(D.v64(), dst.p(), src.p(), 0.0, 1, 1, EPI_DOTS, nullptr, src.p(), fin, N, st);
-        }, st);
-    }
+    };
+    stencil(0.0f);
+    stencil(0.0);
     mg.reset_graphs();   // captured cycles hold the old launch shapes
     release_batched_cycles(&mg);
     SMG_HIP(hipStreamSynchronize(st));

@@ -19,7 +19,9 @@ enum TuneFamily : int {
     TUNE_COARSE_BF16 = 0,
     // k_b_dslash with fused dot products; shape (L, N, prec, 0, 0); choice = grid index
     TUNE_DSLASH_DOTS = 1,
-    TUNE_FAMILIES = 2,
+    // k_b_dslash stores; shape (L, N, prec, epi, 0); choice 1: non-temporal stores, 0: ordinary
+    TUNE_DSLASH_STORE = 2,
+    TUNE_FAMILIES = 3,
 };
 using TuneKey = std::array<int, 6>;   // family, then five shape words
 

@@ -71,3 +71,54 @@ def test_tuned_table_reproduces_solve(smg, clean_table, tmp_path):
     print(open(path).read())
     assert np.array_equal(X1, X2) and i1.iterations == i2.iterations
     assert t1.max() < 1e-10
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_stencil_store_hints_bit_identical(smg, clean_table, prec):
+    """The batched stencil with non-temporal stores (a tuner choice) writes exactly what
+    the one with ordinary stores writes, for every epilogue the solvers use."""
+    import torch
+    banner(f"stencil store hints {prec}")
+    L, N = 64, 32
+    V = L * L
+    U = random_gauge(L, 0.4, 320)
+    D = smg.DiracOp(L, U, 0.05, 1.0, 1.0, 0.0)
+    dt = torch.complex64 if prec == "f32" else torch.complex128
+    p = smg.F32 if prec == "f32" else smg.F64
+    g = torch.Generator().manual_seed(321)
+    src = torch.randn(2 * V, N, dtype=dt, generator=g).cuda()
+    stream = torch.cuda.current_stream().cuda_stream
+    outs = []
+    for nt in (0, 1):
+        for epi in (0, 1, 2):
+            smg.tune_set(smg.TUNE_DSLASH_STORE, [L, N, 4 if prec == "f32" else 8, epi, 0], nt)
+        dst = torch.empty_like(src)
+        for kind in range(4):
+            smg._chk(smg.lib().smg_dirac_apply_batched_dev(D.h, kind, p, src.data_ptr(), dst.data_ptr(), N, stream))
+            torch.cuda.synchronize()
+            outs.append(dst.cpu().clone())
+    for kind in range(4):
+        assert torch.equal(outs[kind], outs[4 + kind]), kind
+
+
+def test_stencil_store_hints_same_solve(smg, clean_table):
+    """A batched mixed solve with non-temporal stencil stores in every epilogue equals,
+    bit for bit, the solve with ordinary stores."""
+    banner("stencil store hints: batched solve")
+    L, k, N = 32, 16, 16
+    V = L * L
+    U = random_gauge(L, 0.4, 330)
+    D = smg.DiracOp(L, U, 0.05, 1.0, 1.0, 0.0)
+    starts = gaussian_vecs(k // 2, 2 * V, 331)
+    mg = smg.MGHierarchy(D, starts, n_levels=2, blocks=((4, 4),), k=(k,), null_iters=10, chiral=True,
+                         coarse_half=True, coarse_maxit=50, coarse_tol=1e-2)
+    B = gaussian_vecs(N, 2 * V, 332)
+    X = []
+    for nt in (0, 1):
+        for bytes_ in (4, 8):
+            for epi in (0, 1, 2):
+                smg.tune_set(smg.TUNE_DSLASH_STORE, [L, N, bytes_, epi, 0], nt)
+        Xn, info, tr = smg.gcr_solve_batched(D, B, mg, 200, 1e-10, trunc=10, prec=smg.F32)
+        assert tr.max() < 1e-10
+        X.append(Xn)
+    assert np.array_equal(X[0], X[1])
