@@ -83,6 +83,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--lattice-batch", type=int, default=1,
                     help="also time the 40-lattice batch of the workload (one launch over all lattices)")
+    ap.add_argument("--lattice-mg", type=int, default=1,
+                    help="lattice batch: also the three-level MG over all 40 lattices with GCR")
     ap.add_argument("--workload", choices=("solve", "hmc", "kernels"), default="solve",
                     help="solve: the batched MG solve (default); hmc: trajectories of the nested Omelyan HMC; "
                          "kernels: the hierarchy's setup, then only the stencil and coarse-operator launches (and a "
@@ -210,12 +212,15 @@ def time_coarse(smg, torch, mg, level, prec, nrhs, k, launches=50, warm=5):
             "GBs": byt / (us * 1e-6) / 1e9, "TFLOPs": flop / (us * 1e-6) / 1e12}
 
 
-def lattice_batch(smg, torch, gaussian_vec, random_gauge):
+def lattice_batch(smg, torch, gaussian_vec, random_gauge, args):
     """The configuration-3 batch of 40 lattices (L in {64, 96, 128, 192, 256}, 8 each,
     theta ~ N(0, 0.4^2), independent seeds, m = 0.1, c_sw = 1) as one Lattices handle:
     the FP64 Wilson-clover D over all 40 in one launch (event-timed on the library
-    stream, 50 launches) and the lockstep CG on D^dag D of all 40 to 1e-10 (host-array
-    entry, wall time including its PCIe copies of b and x)."""
+    stream, 50 launches), the lockstep CG on D^dag D of all 40 to 1e-10, and the
+    three-level MG (4x4 then 2x2 aggregates, 16 near-null vectors per level, chiral:
+    k = 32) over all 40 with GCR to 1e-10, every kernel one launch for all lattices
+    and the V-cycle replayed from a hipGraph (solves: host-array entries, wall time
+    including their PCIe copies of b and x; the cycle event-timed on device arrays)."""
     import ctypes
     Ls = [L for L in (64, 96, 128, 192, 256) for _ in range(8)]
     Us = [random_gauge(L, 0.4, 800 + i) for i, L in enumerate(Ls)]
@@ -249,10 +254,54 @@ def lattice_batch(smg, torch, gaussian_vec, random_gauge):
     t0 = time.perf_counter()
     _, its, tr = B.cg(bs, 5000, 1e-10)
     dt = time.perf_counter() - t0
-    return {"lattices": len(Ls), "L": [64, 96, 128, 192, 256], "sites": sites,
-            "dslash_f64_us": round(us, 2), "dslash_f64_GBs": round(byt / (us * 1e-6) / 1e9, 1),
-            "cg_batch_s": round(dt, 4), "cg_lattice_solves_per_s": round(len(Ls) / dt, 2),
-            "cg_iterations": [int(its.min()), int(its.max())], "max_true_residual": float(tr.max())}
+    out = {"lattices": len(Ls), "L": [64, 96, 128, 192, 256], "sites": sites,
+           "dslash_f64_us": round(us, 2), "dslash_f64_GBs": round(byt / (us * 1e-6) / 1e9, 1),
+           "cg_batch_s": round(dt, 4), "cg_lattice_solves_per_s": round(len(Ls) / dt, 2),
+           "cg_iterations": [int(its.min()), int(its.max())], "max_true_residual": float(tr.max())}
+    if not args.lattice_mg:
+        return out
+    kw = dict(n_levels=3, blocks=((4, 4), (2, 2)), k=(32, 32), null_iters=args.null_iters, pre_smooth=args.pre,
+              post_smooth=args.post, coarse_maxit=args.coarse_maxit, coarse_tol=args.coarse_tol, chiral=True)
+    starts = []
+    for g, L in enumerate(Ls):
+        sz = smg.mg_start_sizes(L, kw["n_levels"], kw["blocks"], kw["k"], kw["chiral"])
+        starts.append(np.concatenate([gaussian_vecs(nv, m, 1000 + 10 * g + l).ravel()
+                                      for l, (nv, m) in enumerate(sz)]))
+    smg.synchronize()
+    t0 = time.perf_counter()
+    M = smg.LatticesMG(B, starts, use_graph=True, **kw)
+    smg.synchronize()
+    setup_s = time.perf_counter() - t0
+    del starts
+    trunc = 7
+    M.gcr(bs, 500, 1e-10, trunc)   # warm-up (captures the cycle)
+    smg.synchronize()
+    t0 = time.perf_counter()
+    _, mits, mtr, _ = M.gcr(bs, 500, 1e-10, trunc)
+    mdt = time.perf_counter() - t0
+    x = torch.empty_like(src)
+
+    def cycle():
+        smg._chk(smg.lib().smg_lattices_mg_precondition_dev(M.h, src.data_ptr(), x.data_ptr(), sarg))
+
+    for _ in range(3):
+        cycle()
+    smg.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(10)]
+    for e0, e1 in ev:
+        e0.record(st)
+        cycle()
+        e1.record(st)
+    ev[-1][1].synchronize()
+    cyc_ms = sum(e0.elapsed_time(e1) for e0, e1 in ev) / 10
+    out.update({"mg": "3 levels, 4x4 then 2x2 aggregates, 16 near-null vectors (chiral, k=32), FP64, "
+                      f"MR({args.pre},{args.post}), coarsest CG {args.coarse_maxit} x tol {args.coarse_tol}, "
+                      f"GCR({trunc}), graph-replayed cycle",
+                "mg_setup_s": round(setup_s, 2), "mg_cycle_ms": round(cyc_ms, 3),
+                "mg_gcr_batch_s": round(mdt, 4), "mg_lattice_solves_per_s": round(len(Ls) / mdt, 2),
+                "mg_iterations": [int(mits.min()), int(mits.max())], "mg_max_true_residual": float(mtr.max())})
+    del M
+    return out
 
 
 def cpu_baseline(args, U, B, mg, threads):
@@ -547,7 +596,7 @@ def main():
                 f"{c['TFLOPs']:.1f} TFLOP/s")
     lbatch = None
     if args.lattice_batch:
-        lbatch = lattice_batch(smg, torch, gaussian_vec, random_gauge)
+        lbatch = lattice_batch(smg, torch, gaussian_vec, random_gauge, args)
         log(f"lattice batch: {lbatch}")
     log("stencil: " + ", ".join(f"{s['prec']}x{s['nrhs']} {s['GBs']:.0f} GB/s" for s in stencil))
 

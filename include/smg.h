@@ -64,6 +64,7 @@ typedef struct smg_prolongator_s smg_prolongator;
 typedef struct smg_coarse_s smg_coarse;
 typedef struct smg_mg_s smg_mg;
 typedef struct smg_lattices_s smg_lattices;
+typedef struct smg_lattices_mg_s smg_lattices_mg;
 
 /* ------------------------------------------------------------------ runtime */
 /* Select the HIP device and create the library's stream.  Optional: the first
@@ -593,6 +594,27 @@ int smg_lattices_apply_dev(smg_lattices* H, int kind, const void* src, void* dst
  * further steps); iters[g] and the FP64 true residuals true_res[g] (either may be NULL) */
 int smg_lattices_cg(smg_lattices* H, const double* b, double* x, int maxit, double tol, int* iters,
                     double* true_res);
+/* Multigrid over the lattices of H (H must outlive it): every lattice's hierarchy is
+ * the one smg_mg_setup builds for that lattice alone (build_mg_hierarchy,
+ * src/multigrid.cpp:595-687; starts: every lattice's start vectors in smg_mg_setup's
+ * layout, lattice after lattice), then held in one concatenated storage per level.
+ * Every kernel of the cycle runs once for all lattices, in FP64; k[l] must be a power of
+ * two in 2..64, aggregates square, the coarsest solve CG (coarse_solver 0); use_graph
+ * replays the cycle from a captured hipGraph.  */
+int smg_lattices_mg_setup(smg_lattices_mg** out, smg_lattices* H, const smg_mg_params* prm, const double* starts);
+int smg_lattices_mg_destroy(smg_lattices_mg* M);
+/* MGHierarchy::precondition (src/multigrid.cpp:155-157) on every lattice: one V-cycle
+ * x = M b on the concatenated vectors; _dev: FP64 device arrays */
+int smg_lattices_mg_precondition(smg_lattices_mg* M, const double* b, double* x);
+int smg_lattices_mg_precondition_dev(smg_lattices_mg* M, const void* b, void* x, void* stream);
+/* fcg_solve_mg (truncated GCR, src/solvers.cpp:223-300, 803-811) with the cycle on
+ * every lattice at once from x0 = 0: one window of trunc <= 7 directions, per-lattice
+ * projections and steps, a lattice stopping on |r_g|/|b_g| < tol; iters[g] and the
+ * FP64 true residuals true_res[g] (either may be NULL); total = iterations run */
+int smg_lattices_gcr(smg_lattices_mg* M, const double* b, double* x, int maxit, double tol, int trunc, int* iters,
+                     double* true_res, int* total);
+int smg_lattices_gcr_dev(smg_lattices_mg* M, const void* b, void* x, int maxit, double tol, int trunc, int* iters,
+                         double* true_res, int* total, void* stream);
 
 /* ------------------------------------------------------------------ U(1) heatbath
  * No counterpart in the reference, which thermalises by HMC: the thermalisation of the

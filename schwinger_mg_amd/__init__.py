@@ -141,6 +141,11 @@ SIGNATURES = [
     ("smg_lattices_size", _I, [_P, _P]), ("smg_lattices_apply", _I, [_P, _I, _P, _P]),
     ("smg_lattices_apply_dev", _I, [_P, _I, _P, _P, _P]),
     ("smg_lattices_cg", _I, [_P, _P, _P, _I, _D, _P, _P]),
+    ("smg_lattices_mg_setup", _I, [_P, _P, _P, _P]), ("smg_lattices_mg_destroy", _I, [_P]),
+    ("smg_lattices_mg_precondition", _I, [_P, _P, _P]),
+    ("smg_lattices_mg_precondition_dev", _I, [_P, _P, _P, _P]),
+    ("smg_lattices_gcr", _I, [_P, _P, _P, _I, _D, _I, _P, _P, _P]),
+    ("smg_lattices_gcr_dev", _I, [_P, _P, _P, _I, _D, _I, _P, _P, _P, _P]),
     ("smg_heatbath", _I, [_I, _P, _D, _I, ctypes.c_ulonglong, ctypes.c_uint]),
     ("smg_coarse_lobpcg", _I, [_P, _I, _P, _I, _D, _P, _P, _P]),
     ("smg_mg_update_coarsest_deflation", _I, [_P, _I, _I]),
@@ -543,23 +548,43 @@ def estimate_lambda_max(D, v0, niter):
 
 
 # ------------------------------------------------------------------ multigrid
+def _mg_params(n_levels, blocks, k, null_iters, pre_smooth, post_smooth, w_cycle, coarse_maxit, coarse_tol,
+               coarse_half, coarse_solver, use_graph, chiral, null_prec):
+    p = MGParams()
+    p.n_levels = n_levels
+    blocks = list(blocks) + [blocks[-1]] * (4 - len(blocks))
+    k = list(k) + [k[-1]] * (4 - len(k))
+    for i in range(4):
+        p.bx[i], p.by[i] = blocks[i]
+        p.k[i] = k[i]
+    p.null_iters, p.pre_smooth, p.post_smooth = null_iters, pre_smooth, post_smooth
+    p.w_cycle, p.coarse_maxit, p.coarse_tol, p.coarse_half = int(w_cycle), coarse_maxit, coarse_tol, int(coarse_half)
+    p.coarse_solver, p.use_graph, p.chiral = coarse_solver, int(use_graph), int(chiral)
+    p.null_prec = 0 if null_prec == F64 else 1
+    return p
+
+
+def mg_start_sizes(L, n_levels, blocks, k, chiral):
+    """The near-null start vectors an L x L lattice's hierarchy takes: one
+    (count, length) pair per non-coarsest level (k[l] vectors, k[l] / 2 with chiral
+    aggregation, of that level's dimension)."""
+    blocks = list(blocks) + [blocks[-1]] * (4 - len(blocks))
+    k = list(k) + [k[-1]] * (4 - len(k))
+    out, nx, ny, nd = [], L, L, 2
+    for l in range(n_levels - 1):
+        out.append((k[l] // 2 if chiral else k[l], nx * ny * nd))
+        nx, ny, nd = nx // blocks[l][0], ny // blocks[l][1], k[l]
+    return out
+
+
 class MGHierarchy:
     """MGHierarchy (src/multigrid.h:41-107) built by build_mg_hierarchy."""
 
     def __init__(self, D, starts, n_levels=2, blocks=((4, 4),), k=(4,), null_iters=20, pre_smooth=3,
                  post_smooth=3, w_cycle=True, coarse_maxit=200, coarse_tol=1e-12, coarse_half=False,
                  coarse_solver=0, use_graph=False, chiral=False, null_prec=F64):
-        p = MGParams()
-        p.n_levels = n_levels
-        blocks = list(blocks) + [blocks[-1]] * (4 - len(blocks))
-        k = list(k) + [k[-1]] * (4 - len(k))
-        for i in range(4):
-            p.bx[i], p.by[i] = blocks[i]
-            p.k[i] = k[i]
-        p.null_iters, p.pre_smooth, p.post_smooth = null_iters, pre_smooth, post_smooth
-        p.w_cycle, p.coarse_maxit, p.coarse_tol, p.coarse_half = int(w_cycle), coarse_maxit, coarse_tol, int(coarse_half)
-        p.coarse_solver, p.use_graph, p.chiral = coarse_solver, int(use_graph), int(chiral)
-        p.null_prec = 0 if null_prec == F64 else 1
+        p = _mg_params(n_levels, blocks, k, null_iters, pre_smooth, post_smooth, w_cycle, coarse_maxit, coarse_tol,
+                       coarse_half, coarse_solver, use_graph, chiral, null_prec)
         self.params, self.D, self.n_levels = p, D, n_levels
         h = _P()
         _chk(lib().smg_mg_setup(ctypes.byref(h), D.h, ctypes.byref(p), _p(_c(starts))))
@@ -976,6 +1001,47 @@ class Lattices:
         tr = np.zeros(self.G)
         _chk(lib().smg_lattices_cg(self.h, _p(b), _p(x), max_iter, tol, _p(it), _p(tr)))
         return self._split(x), it, tr
+
+
+class LatticesMG:
+    """Multigrid over the lattices of a Lattices batch (no reference counterpart): every
+    lattice's hierarchy is the one MGHierarchy builds for it alone (starts: one start
+    array per lattice, as MGHierarchy takes it), held in one storage per level, and
+    every kernel of the cycle and of GCR runs once for all lattices (FP64)."""
+
+    def __init__(self, B, starts, n_levels=3, blocks=((4, 4), (2, 2)), k=(8, 8), null_iters=20, pre_smooth=3,
+                 post_smooth=3, w_cycle=False, coarse_maxit=50, coarse_tol=1e-2, use_graph=False, chiral=True,
+                 null_prec=F64):
+        p = _mg_params(n_levels, blocks, k, null_iters, pre_smooth, post_smooth, w_cycle, coarse_maxit, coarse_tol,
+                       False, 0, use_graph, chiral, null_prec)
+        self.B, self.params = B, p
+        cat = np.concatenate([_c(s) for s in starts])
+        h = _P()
+        _chk(lib().smg_lattices_mg_setup(ctypes.byref(h), B.h, ctypes.byref(p), _p(cat)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.smg_lattices_mg_destroy(self.h)
+            self.h = None
+
+    def precondition(self, bs):
+        """One V-cycle on every lattice: the list of x = M b."""
+        b = self.B._cat(bs)
+        x = np.zeros_like(b)
+        _chk(lib().smg_lattices_mg_precondition(self.h, _p(b), _p(x)))
+        return self.B._split(x)
+
+    def gcr(self, bs, max_iter, tol, trunc=7):
+        """GCR(trunc) with the cycle on every lattice at once from x = 0:
+        (solutions, iterations per lattice, FP64 true residuals, iterations run)."""
+        b = self.B._cat(bs)
+        x = np.zeros_like(b)
+        it = np.zeros(self.B.G, np.int32)
+        tr = np.zeros(self.B.G)
+        tot = _I()
+        _chk(lib().smg_lattices_gcr(self.h, _p(b), _p(x), max_iter, tol, trunc, _p(it), _p(tr), ctypes.byref(tot)))
+        return self.B._split(x), it, tr, tot.value
 
 
 # ------------------------------------------------------------------ U(1) heatbath

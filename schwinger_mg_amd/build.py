@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsmg.so")
 BUILD = os.path.join(HERE, "_build")
-SOURCES = ["dirac.hip", "blas.hip", "mg.hip", "batched.hip", "hmc.hip", "eigen.hip", "lattices.hip", "noalloc.hip", "tune.cpp", "eigensolve.cpp", "runtime.cpp", "runtime_b.cpp", "capi.cpp"]
+SOURCES = ["dirac.hip", "blas.hip", "mg.hip", "batched.hip", "hmc.hip", "eigen.hip", "lattices.hip", "lattices_mg.hip", "noalloc.hip", "tune.cpp", "eigensolve.cpp", "runtime.cpp", "runtime_b.cpp", "runtime_lat.cpp", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: no v_pk_*_f32 generation (these are avoided on this platform)
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",

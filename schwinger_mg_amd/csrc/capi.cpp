@@ -13,6 +13,7 @@
 #include "runtime.h"
 #include "smg_lattices.h"
 #include "runtime_b.h"
+#include "runtime_lat.h"
 #include "tune.h"
 
 using namespace smg;
@@ -41,6 +42,10 @@ struct smg_lattices_s {
 struct smg_mg_s {
     std::unique_ptr<MG> mg;
     smg_dirac* D;
+};
+struct smg_lattices_mg_s {
+    std::unique_ptr<LatticeMG> mg;
+    smg_lattices* H;
 };
 
 namespace {
@@ -579,32 +584,38 @@ int smg_lambda_max_dirac(smg_dirac* D, const double* v0, int niter, double* out)
 }
 
 // ------------------------------------------------------------------ multigrid
+namespace {
+MGConfig cfg_of(const smg_mg_params* prm) {
+    MGConfig c;
+    c.n_levels = prm->n_levels;
+    for (int i = 0; i < 4; i++) {
+        c.bx[i] = prm->bx[i];
+        c.by[i] = prm->by[i];
+        c.k[i] = prm->k[i];
+    }
+    c.null_iters = prm->null_iters;
+    c.pre = prm->pre_smooth;
+    c.post = prm->post_smooth;
+    c.w_cycle = prm->w_cycle;
+    c.coarse_maxit = prm->coarse_maxit;
+    c.coarse_tol = prm->coarse_tol;
+    c.coarse_half = prm->coarse_half;
+    c.coarse_solver = prm->coarse_solver;
+    c.use_graph = prm->use_graph;
+    c.chiral = prm->chiral;
+    c.null_prec = prm->null_prec;
+    SMG_REQUIRE(c.null_prec == 0 || c.null_prec == 1, "null_prec must be 0 or 1");
+    SMG_REQUIRE(c.coarse_solver == 0 || c.coarse_solver == 1, "coarse_solver must be 0 or 1");
+    SMG_REQUIRE(c.n_levels >= 2 && c.n_levels <= 5, "n_levels must be 2..5");
+    for (int l = 0; l + 1 < c.n_levels; l++)
+        SMG_REQUIRE(c.k[l] >= 1 && c.k[l] <= 64 && c.bx[l] >= 1 && c.by[l] >= 1, "bad level parameters");
+    return c;
+}
+}  // namespace
 int smg_mg_setup(smg_mg** out, smg_dirac* D, const smg_mg_params* prm, const double* starts) {
     return guard([&] {
         SMG_REQUIRE(out && D && prm && starts, "null argument");
-        MGConfig c;
-        c.n_levels = prm->n_levels;
-        for (int i = 0; i < 4; i++) {
-            c.bx[i] = prm->bx[i];
-            c.by[i] = prm->by[i];
-            c.k[i] = prm->k[i];
-        }
-        c.null_iters = prm->null_iters;
-        c.pre = prm->pre_smooth;
-        c.post = prm->post_smooth;
-        c.w_cycle = prm->w_cycle;
-        c.coarse_maxit = prm->coarse_maxit;
-        c.coarse_tol = prm->coarse_tol;
-        c.coarse_half = prm->coarse_half;
-        c.coarse_solver = prm->coarse_solver;
-        c.use_graph = prm->use_graph;
-        c.chiral = prm->chiral;
-        c.null_prec = prm->null_prec;
-        SMG_REQUIRE(c.null_prec == 0 || c.null_prec == 1, "null_prec must be 0 or 1");
-        SMG_REQUIRE(c.coarse_solver == 0 || c.coarse_solver == 1, "coarse_solver must be 0 or 1");
-        SMG_REQUIRE(c.n_levels >= 2 && c.n_levels <= 5, "n_levels must be 2..5");
-        for (int l = 0; l + 1 < c.n_levels; l++)
-            SMG_REQUIRE(c.k[l] >= 1 && c.k[l] <= 64 && c.bx[l] >= 1 && c.by[l] >= 1, "bad level parameters");
+        const MGConfig c = cfg_of(prm);
         auto* h = new smg_mg_s();
         h->D = D;
         try { h->mg.reset(new MG(*D->D, c, reinterpret_cast<const c64*>(starts), false)); } catch (...) { delete h; throw; }
@@ -1791,6 +1802,63 @@ int smg_lattices_cg(smg_lattices* H, const double* b, double* x, int maxit, doub
             if (true_res) true_res[g] = hb[g] > 0.0 ? std::sqrt(hn[2 * g] / hb[g]) : 0.0;
         }
         downc(x, xx.p(), n);
+    });
+}
+
+// ------------------------------------------------------------------ multigrid over many lattices
+int smg_lattices_mg_setup(smg_lattices_mg** out, smg_lattices* H, const smg_mg_params* prm, const double* starts) {
+    return guard([&] {
+        SMG_REQUIRE(out && H && prm && starts, "null argument");
+        const MGConfig c = cfg_of(prm);
+        std::vector<const Dirac*> dirac;
+        for (const auto& d : H->D) dirac.push_back(d.get());
+        auto* h = new smg_lattices_mg_s();
+        h->H = H;
+        try {
+            h->mg.reset(new LatticeMG(dirac, H->d.as<LatDesc>(), H->nwg, c, reinterpret_cast<const c64*>(starts)));
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+int smg_lattices_mg_destroy(smg_lattices_mg* M) { return guard([&] { delete M; }); }
+int smg_lattices_mg_precondition(smg_lattices_mg* M, const double* b, double* x) {
+    return guard([&] {
+        SMG_REQUIRE(M && b && x, "null argument");
+        const size_t n = 2 * M->H->sites;
+        Field<double> bb = up(b, n), xx(n);
+        M->mg->precondition(bb.p(), xx.p(), S());
+        downc(x, xx.p(), n);
+    });
+}
+int smg_lattices_mg_precondition_dev(smg_lattices_mg* M, const void* b, void* x, void* stream) {
+    return guard([&] {
+        SMG_REQUIRE(M && b && x, "null argument");
+        StreamJoin j(stream);
+        M->mg->precondition(static_cast<const c64*>(b), static_cast<c64*>(x), S());
+    });
+}
+int smg_lattices_gcr(smg_lattices_mg* M, const double* b, double* x, int maxit, double tol, int trunc, int* iters,
+                     double* true_res, int* total) {
+    return guard([&] {
+        SMG_REQUIRE(M && b && x && maxit >= 0 && tol > 0.0, "bad arguments");
+        const size_t n = 2 * M->H->sites;
+        Field<double> bb = up(b, n), xx(n);
+        const int t = M->mg->gcr(bb.p(), xx.p(), maxit, tol, trunc, iters, true_res, S());
+        if (total) *total = t;
+        downc(x, xx.p(), n);
+    });
+}
+int smg_lattices_gcr_dev(smg_lattices_mg* M, const void* b, void* x, int maxit, double tol, int trunc, int* iters,
+                         double* true_res, int* total, void* stream) {
+    return guard([&] {
+        SMG_REQUIRE(M && b && x && maxit >= 0 && tol > 0.0, "bad arguments");
+        StreamJoin j(stream);
+        const int t = M->mg->gcr(static_cast<const c64*>(b), static_cast<c64*>(x), maxit, tol, trunc, iters,
+                                 true_res, S());
+        if (total) *total = t;
     });
 }
 

@@ -14,61 +14,6 @@
 
 namespace smg {
 
-namespace {
-// the lattice of this workgroup (binary search of the wg0 table), broadcast via LDS
-__device__ __forceinline__ int my_lattice(const LatDesc* __restrict__ d, int G) {
-    __shared__ int g_sh;
-    if (threadIdx.x == 0) {
-        int lo = 0, hi = G - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (d[mid].wg0 <= (int)blockIdx.x) lo = mid;
-            else hi = mid - 1;
-        }
-        g_sh = lo;
-    }
-    __syncthreads();
-    return g_sh;
-}
-
-// End of a reducing kernel: every thread holds its partial v[NC] for this workgroup's
-// lattice g; all threads of every workgroup call this.  Lattice t's sum is the sum of
-// its workgroups' rows in index order; fin_program then runs with out + t*ostride and
-// aux + t*astride.  With done set, *done = 1 once every lattice's flag is set.
-template <int NC>
-__device__ __forceinline__ void finish_seg(double (&v)[NC], const LatFin& f, const LatDesc* __restrict__ d, int G) {
-    block_sum<NC>(v);
-    __shared__ int last;
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int c = 0; c < NC; c++) part_store(f.part + (size_t)blockIdx.x * NC + c, v[c]);
-        stores_complete();
-        last = (counter_arrive(f.counter) == gridDim.x - 1u);
-    }
-    __syncthreads();
-    if (!last) return;
-    for (int t = threadIdx.x; t < G; t += kBlock) {
-        const int b0 = d[t].wg0, b1 = (t + 1 < G) ? d[t + 1].wg0 : (int)gridDim.x;
-        double w[NC];
-#pragma unroll
-        for (int c = 0; c < NC; c++) w[c] = 0.0;
-        for (int b = b0; b < b1; b++)
-#pragma unroll
-            for (int c = 0; c < NC; c++) w[c] += part_load(f.part + (size_t)b * NC + c);
-        fin_program(f.op, w, NC, f.out ? f.out + (size_t)t * f.ostride : nullptr,
-                    f.aux ? f.aux + (size_t)t * f.astride : nullptr);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (f.done) {
-            int all = 1;
-            for (int t = 0; t < G; t++) all &= *reinterpret_cast<const int*>(f.aux + (size_t)t * f.astride + 8) != 0;
-            *f.done = all;
-        }
-        counter_reset(f.counter);
-    }
-}
-}  // namespace
 
 // dst = g5^g5out D(mu) g5^g5in src on every lattice; EPI_DOTS adds {<dst, aux>, |dst|^2}
 // per lattice into fin
