@@ -534,6 +534,9 @@ struct MGHierarchy {
     }
     std::vector<Vec> coarsest_defl_vecs;
     std::vector<double> coarsest_defl_vals;
+    // the level-0 near-null vectors of the setup (src/multigrid.h, null_vecs_l0), kept
+    // for warm-started rebuilds
+    std::vector<Vec> null_vecs_l0;
     // levels[0].Ac.build(D, P0) followed by rebuild_deeper_levels (src/multigrid.cpp:
     // 163-194): call after the hierarchy's DiracOp has taken new links (update_gauge)
     void rebuild_deeper_levels() { detail::check(smg_mg_rebuild(h_.get())); }
@@ -679,15 +682,16 @@ inline double estimate_lambda_max(const OpApply& A, int n, int niter) {
 
 // build_mg_hierarchy, src/multigrid.cpp:595-687.  The null-space start vectors are
 // drawn from rng in the reference's order (k_null vectors per level, level 0
-// first).  Levels >= 1 aggregate coarse_block_agg consecutive coarse dofs, which
-// on the coarse lattice is an (agg/k_null) x 1 block; the coarsest level is
-// solved exactly (dense inverse when its dimension is <= 8192, else CG to 1e-12).
+// first); with warm_start, its vectors replace the first level-0 draws, which are
+// then not taken from rng (src/multigrid.cpp:443-447).  Levels >= 1 aggregate
+// coarse_block_agg consecutive coarse dofs, which on the coarse lattice is an
+// (agg/k_null) x 1 block; the coarsest level is solved exactly (dense inverse when
+// its dimension is <= 8192, else CG to 1e-12).
 inline MGHierarchy build_mg_hierarchy(const DiracOp& D, int n_levels, int block_size, int k_null,
                                       int coarse_block_agg, int null_iters, std::mt19937& rng, bool w_cycle = true,
                                       int pre_smooth = 3, int post_smooth = 3, bool verbose = true,
                                       const std::vector<Vec>* warm_start = nullptr) {
     (void)verbose;
-    if (warm_start) throw std::invalid_argument("build_mg_hierarchy: warm starts are not supported");
     if (n_levels < 2 || n_levels > 5) throw std::invalid_argument("build_mg_hierarchy: 2..5 levels");
     if (coarse_block_agg <= 0) coarse_block_agg = 4 * k_null;
     smg_mg_params p{};
@@ -726,12 +730,67 @@ inline MGHierarchy build_mg_hierarchy(const DiracOp& D, int n_levels, int block_
     Vec starts;
     for (int l = 0; l + 1 < n_levels; l++)
         for (int i = 0; i < k_null; i++) {
+            if (l == 0 && warm_start && i < (int)warm_start->size()) {
+                const Vec& w = (*warm_start)[i];
+                detail::check_size(w, mg.dims[0], "build_mg_hierarchy (warm_start)");
+                starts.insert(starts.end(), w.begin(), w.end());
+                continue;
+            }
             const Vec v = random_vec(mg.dims[l], rng);
             starts.insert(starts.end(), v.begin(), v.end());
         }
     smg_mg* h = nullptr;
     detail::check(smg_mg_setup(&h, D.handle(), &p, detail::raw(starts)));
     mg.h_.reset(h, [](smg_mg* q) { smg_mg_destroy(q); });
+    Vec nv((size_t)k_null * mg.dims[0]);
+    detail::check(smg_mg_null_vecs(h, 0, detail::raw(nv)));
+    for (int i = 0; i < k_null; i++)
+        mg.null_vecs_l0.emplace_back(nv.begin() + (size_t)i * mg.dims[0], nv.begin() + (size_t)(i + 1) * mg.dims[0]);
+    return mg;
+}
+
+// build_mg_hierarchy_warm, src/multigrid.cpp:688-701
+inline MGHierarchy build_mg_hierarchy_warm(const DiracOp& D, int n_levels, int block_size, int k_null,
+                                           int coarse_block_agg, int null_iters, std::mt19937& rng,
+                                           const std::vector<Vec>& warm_null_vecs, bool w_cycle = true,
+                                           int pre_smooth = 3, int post_smooth = 3, bool verbose = true) {
+    return build_mg_hierarchy(D, n_levels, block_size, k_null, coarse_block_agg, null_iters, rng, w_cycle, pre_smooth,
+                              post_smooth, verbose, &warm_null_vecs);
+}
+
+// MGConfig / SolverConfig (src/config.h:16-39) and build_full_mg (src/mg_builder.cpp:
+// 3-31): 20 null-space iterations, 5 when warm-started, then the sparse coarse
+// operator with TRLM deflation when n_defl > 0.  The FEAST null space is not provided.
+struct MGConfig {
+    int mg_levels = 1;
+    int block_size = 4;
+    int k_null = 4;
+    int coarse_block = 0;   // 0 = k_null * 4
+    bool w_cycle = true;
+    int pre_smooth = 3;
+    int post_smooth = 3;
+    bool no_mg = false;
+    bool symmetric_mg = false;
+    int resolved_coarse_block() const { return coarse_block > 0 ? coarse_block : k_null * 4; }
+};
+struct SolverConfig {
+    double tol = 1e-10;
+    int krylov = 30;
+    int max_iter = 300;
+    std::string eigensolver = "trlm";
+    double feast_emax = 0.0;
+    int n_defl = 0;
+};
+inline MGHierarchy build_full_mg(const DiracOp& D, const MGConfig& mcfg, const SolverConfig& scfg,
+                                 std::mt19937& rng, int n_defl = 0, bool verbose = true,
+                                 const std::vector<Vec>* warm_start = nullptr) {
+    if (scfg.eigensolver == "feast") throw std::invalid_argument("build_full_mg: the FEAST null space is not provided");
+    const int null_iters = warm_start ? 5 : 20;
+    MGHierarchy mg = build_mg_hierarchy(D, mcfg.mg_levels, mcfg.block_size, mcfg.k_null, mcfg.resolved_coarse_block(),
+                                        null_iters, rng, mcfg.w_cycle, mcfg.pre_smooth, mcfg.post_smooth, verbose,
+                                        warm_start);
+    if (n_defl > 0) mg.setup_sparse_coarse(D.as_DdagD_op(), D.lat.ndof, n_defl, 1e-12, 200, scfg.eigensolver,
+                                           scfg.feast_emax);
     return mg;
 }
 

@@ -374,8 +374,9 @@ int smg_mg_setup_sparse_coarse(smg_mg* mg, int n_defl, double cg_tol, int max_cg
  * smg_dirac_set_gauge or an HMC update); the dense coarsest inverse follows, the
  * deflation spaces are kept as they are (as in the reference) */
 int smg_mg_rebuild(smg_mg* mg);
-/* build_full_mg with warm_start (src/mg_builder.cpp:3-31; the reference warm-starts the
- * fine level with 5 null iterations): every level's near-null vectors iterated
+/* No reference counterpart (an extension; the reference's warm start, build_full_mg
+ * with warm_start in src/mg_builder.cpp:3-31, restarts the fine level only with 5 null
+ * iterations and redraws the deeper levels): every level's near-null vectors iterated
  * null_iters more times from their current values on the current gauge field, then
  * the aggregations, the Galerkin operators and the dense coarsest inverse rebuilt;
  * the symmetric-cycle weights and deflation spaces are kept */

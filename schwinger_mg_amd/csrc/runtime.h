@@ -277,10 +277,11 @@ struct MG : Preconditioner {
     void rebuild(hipStream_t st);
     void galerkin(size_t level, hipStream_t st);
     void near_null(size_t level, LinOp<double>& A, int nv, int iters, c64* vecs, hipStream_t st);
-    // build_full_mg with warm_start (src/mg_builder.cpp:3-31, null iterations 5 when
-    // warm): every level's near-null vectors iterated iters more times from their
-    // current values on the current operator, then the aggregation and the Galerkin
-    // operators rebuilt (the reference warm-starts the fine level only)
+    // An extension with no reference counterpart (the reference's warm start,
+    // build_full_mg in src/mg_builder.cpp:3-31, restarts the fine level only and
+    // redraws the deeper levels): every level's near-null vectors iterated iters more
+    // times from their current values on the current operator, then the aggregation
+    // and the Galerkin operators rebuilt
     void refresh(int iters, hipStream_t st);
     void precondition(const c64* b, c64* x, hipStream_t st) override;
     void precondition(const c32* b, c32* x, hipStream_t st) override;

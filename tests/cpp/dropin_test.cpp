@@ -153,6 +153,27 @@ int main() {
     orc_mg_precondition(mgo, b.data(), xo.data());
     report("MGHierarchy::precondition (3 levels, W)", relerr(xg, xo), 1e-10);
 
+    // build_mg_hierarchy_warm (src/multigrid.cpp:688-701): the level-0 starts are the
+    // previous hierarchy's null vectors and take no draws from rng; level 1 draws
+    std::mt19937 rng_w(77);
+    std::mt19937 rng_w_copy = rng_w;
+    MGHierarchy mgw = build_mg_hierarchy_warm(D, 3, 4, 4, 16, 5, rng_w, mg.null_vecs_l0, true, 3, 3, false);
+    Vec wstarts;
+    for (const Vec& v : mg.null_vecs_l0) wstarts.insert(wstarts.end(), v.begin(), v.end());
+    for (int i = 0; i < 4; i++) {
+        const Vec v = random_vec(mg.dims[1], rng_w_copy);
+        wstarts.insert(wstarts.end(), v.begin(), v.end());
+    }
+    report("warm start: level 0 takes no rng draws", rng_w() == rng_w_copy() ? 0.0 : 1.0, 0.0);
+    p.null_iters = 5;
+    orc_mg* mgwo = orc_mg_new(Ao, L, L, 2, &p, wstarts.data());
+    p.null_iters = 10;
+    const Vec xw = mgw.precondition(b);
+    Vec xwo(lat.ndof);
+    orc_mg_precondition(mgwo, b.data(), xwo.data());
+    report("build_mg_hierarchy_warm precondition", relerr(xw, xwo), 1e-10);
+    orc_mg_free(mgwo);
+
     // solvers (src/solvers.cpp)
     CGResult cg = cg_solve(A, lat.ndof, b, 2000, 1e-10);
     Vec xcg(lat.ndof);
