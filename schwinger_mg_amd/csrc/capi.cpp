@@ -997,12 +997,11 @@ int smg_force_backward(smg_dirac* D, const double* chi, const double* X, double*
         down(F, f.p, n * sizeof(double));
     });
 }
+// fermion_force (src/hmc_forces.cpp:134-139) in one kernel: chi = D X is recomputed
+// per tile inside the force kernel, never written to memory
 static void fermion_force_dev(smg_dirac* D, const c64* X, double* F, hipStream_t st) {
     const Dirac& d = *D->D;
-    const size_t n = 2 * (size_t)d.V;
-    Field<double> chi(n);
-    launch_dslash<double>(d.v64(), chi.p(), X, d.mu_t, 0, 0, EPI_NONE, nullptr, nullptr, Fin(), 1, n, st);
-    launch_force_backward(d.v64(), d.csw, chi.p(), X, F, st);
+    launch_force_fused(d.v64(), d.csw, d.mu_t, X, F, st);
 }
 int smg_fermion_force(smg_dirac* D, const double* X, double* F) {
     return guard([&] {

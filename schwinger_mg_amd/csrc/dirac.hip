@@ -41,6 +41,32 @@ __device__ __forceinline__ void wilson_hop(T r, cplx<T> u0, cplx<T> u0b, cplx<T>
 }
 
 // ----------------------------------------------------------------- full lattice
+// (D psi)(s) at site s = x + L*y (src/dirac.h:93-117): the clover(-twisted) diagonal and
+// the hopping sum; g5in: psi's lower spin negated on load
+template <typename T>
+__device__ __forceinline__ void dslash_site(const DiracView<T>& D, const cplx<T>* __restrict__ src, T mu, int g5in,
+                                            int s, int x, int y, cplx<T>& o0, cplx<T>& o1) {
+    const int L = D.L, V = L * L;
+    const int sxp = (x + 1 == L) ? s - x : s + 1;
+    const int sxm = (x == 0) ? s + L - 1 : s - 1;
+    const int syp = (y + 1 == L) ? x : s + L;
+    const int sym = (y == 0) ? s + V - L : s - L;
+    cplx<T> s0, s1, xf0, xf1, xb0, xb1, yf0, yf1, yb0, yb1;
+    ld_sp(src, s, g5in, s0, s1);
+    ld_sp(src, sxp, g5in, xf0, xf1);
+    ld_sp(src, sxm, g5in, xb0, xb1);
+    ld_sp(src, syp, g5in, yf0, yf1);
+    ld_sp(src, sym, g5in, yb0, yb1);
+    const cplx<T>* U0 = D.U;
+    const cplx<T>* U1 = D.U + V;
+    cplx<T> h0, h1;
+    wilson_hop(D.r, U0[s], U0[sxm], U1[s], U1[sym], xf0, xf1, xb0, xb1, yf0, yf1, yb0, yb1, h0, h1);
+    const T C = D.C ? D.C[s] : (T)0;
+    const cplx<T> e0 = mk<T>(D.diag + C, mu), e1 = mk<T>(D.diag - C, -mu);
+    o0 = e0 * s0 + h0;
+    o1 = e1 * s1 + h1;
+}
+
 template <typename T, int EPI>
 __global__ __launch_bounds__(kBlock) void k_dslash(DiracView<T> D, cplx<T>* __restrict__ dst,
                                                    const cplx<T>* __restrict__ src, T mu, int g5in, int g5out,
@@ -53,24 +79,8 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DiracView<T> D, cplx<T>* __re
     dst += off;
     double acc[3] = {0.0, 0.0, 0.0};
     if (s < V) {
-        const int x = s % L, y = s / L;
-        const int sxp = (x + 1 == L) ? s - x : s + 1;
-        const int sxm = (x == 0) ? s + L - 1 : s - 1;
-        const int syp = (y + 1 == L) ? x : s + L;
-        const int sym = (y == 0) ? s + V - L : s - L;
-        cplx<T> s0, s1, xf0, xf1, xb0, xb1, yf0, yf1, yb0, yb1;
-        ld_sp(src, s, g5in, s0, s1);
-        ld_sp(src, sxp, g5in, xf0, xf1);
-        ld_sp(src, sxm, g5in, xb0, xb1);
-        ld_sp(src, syp, g5in, yf0, yf1);
-        ld_sp(src, sym, g5in, yb0, yb1);
-        const cplx<T>* U0 = D.U;
-        const cplx<T>* U1 = D.U + V;
-        cplx<T> h0, h1;
-        wilson_hop(D.r, U0[s], U0[sxm], U1[s], U1[sym], xf0, xf1, xb0, xb1, yf0, yf1, yb0, yb1, h0, h1);
-        const T C = D.C ? D.C[s] : (T)0;
-        const cplx<T> e0 = mk<T>(D.diag + C, mu), e1 = mk<T>(D.diag - C, -mu);
-        cplx<T> o0 = e0 * s0 + h0, o1 = e1 * s1 + h1;
+        cplx<T> o0, o1;
+        dslash_site(D, src, mu, g5in, s, s % L, s / L, o0, o1);
         if (g5out) o1 = -o1;
         if (EPI == EPI_RESID) {
             const cplx<T>* bb = bvec + off;
@@ -343,12 +353,13 @@ void launch_plaq_sum(int L, const c64* U, const Fin& fin, hipStream_t st) {
 // F_mu(s) = Re(i c_bwd - i c_fwd),
 //   c_fwd = chi(s)^dag (r - sigma_mu) U_mu(s) X(s+mu),
 //   c_bwd = chi(s+mu)^dag (r + sigma_mu) U_mu(s)^* X(s).
-__device__ __forceinline__ void hop_force_site(const DiracView<double>& D, const c64* __restrict__ chi,
-                                               const c64* __restrict__ X, int s, int sfx, int sfy, double& f0,
-                                               double& f1) {
+// chi at s (c0, c1), at s + x (cx0, cx1) and at s + y (cy0, cy1)
+__device__ __forceinline__ void hop_force_vals(const DiracView<double>& D, const c64* __restrict__ X, int s, int sfx,
+                                               int sfy, c64 c0, c64 c1, c64 cx0, c64 cx1, c64 cy0, c64 cy1,
+                                               double& f0, double& f1) {
     const int V = D.L * D.L;
     const double r = D.r;
-    const c64 c0 = chi[2 * s], c1 = chi[2 * s + 1], x0 = X[2 * s], x1 = X[2 * s + 1];
+    const c64 x0 = X[2 * s], x1 = X[2 * s + 1];
     {
         const c64 u = D.U[s], ud = conj(u);
         const c64 ux0 = u * X[2 * sfx], ux1 = u * X[2 * sfx + 1];
@@ -356,7 +367,7 @@ __device__ __forceinline__ void hop_force_site(const DiracView<double>& D, const
         const c64 cf = cdot(c0, fw0) + cdot(c1, fw1);
         const c64 udx0 = ud * x0, udx1 = ud * x1;
         const c64 bw0 = r * udx0 + udx1, bw1 = udx0 + r * udx1;
-        const c64 cb = cdot(chi[2 * sfx], bw0) + cdot(chi[2 * sfx + 1], bw1);
+        const c64 cb = cdot(cx0, bw0) + cdot(cx1, bw1);
         f0 = -(cb.im - cf.im);   // Re(i z) = -Im z
     }
     {
@@ -366,9 +377,15 @@ __device__ __forceinline__ void hop_force_site(const DiracView<double>& D, const
         const c64 cf = cdot(c0, fw0) + cdot(c1, fw1);
         const c64 udx0 = ud * x0, udx1 = ud * x1;
         const c64 bw0 = r * udx0 - mul_i(udx1), bw1 = mul_i(udx0) + r * udx1;
-        const c64 cb = cdot(chi[2 * sfy], bw0) + cdot(chi[2 * sfy + 1], bw1);
+        const c64 cb = cdot(cy0, bw0) + cdot(cy1, bw1);
         f1 = -(cb.im - cf.im);
     }
+}
+__device__ __forceinline__ void hop_force_site(const DiracView<double>& D, const c64* __restrict__ chi,
+                                               const c64* __restrict__ X, int s, int sfx, int sfy, double& f0,
+                                               double& f1) {
+    hop_force_vals(D, X, s, sfx, sfy, chi[2 * s], chi[2 * s + 1], chi[2 * sfx], chi[2 * sfx + 1], chi[2 * sfy],
+                   chi[2 * sfy + 1], f0, f1);
 }
 
 __global__ void k_hopping_force(DiracView<double> D, const c64* __restrict__ chi, const c64* __restrict__ X,
@@ -465,6 +482,66 @@ __global__ __launch_bounds__(kBlock) void k_force_backward(DiracView<double> D, 
 void launch_force_backward(const DiracView<double>& D, double csw, const c64* chi, const c64* X, double* F,
                            hipStream_t st) {
     k_force_backward<<<grid_for(D.L * D.L), kBlock, 0, st>>>(D, csw, chi, X, F);
+    SMG_LAUNCH_CHECK();
+}
+
+// The whole fermion force from X alone: chi = D X is recomputed, never stored.  A
+// workgroup owns a kForceTile x kForceTile tile of sites; it first evaluates chi on
+// the tile and a one-site halo into LDS (the stencil of dslash_site, periodic wrap;
+// lattices narrower than the tile wrap onto themselves, which gives the same values),
+// then each work-item forms its site's hopping force and clover insertion from the
+// LDS chi, whose eight clover sites and two hopping neighbours all lie in the halo.
+constexpr int kForceTile = 16;
+static_assert(kForceTile * kForceTile == kBlock, "one work-item per tile site");
+__global__ __launch_bounds__(kBlock) void k_force_fused(DiracView<double> D, double csw, double mu,
+                                                        const c64* __restrict__ X, double* __restrict__ F) {
+    constexpr int TH = kForceTile + 2;
+    __shared__ c64 chi_s[TH * TH][2];
+    const int L = D.L, V = L * L;
+    const int ntx = (L + kForceTile - 1) / kForceTile;
+    const int tx0 = ((int)blockIdx.x % ntx) * kForceTile, ty0 = ((int)blockIdx.x / ntx) * kForceTile;
+    for (int i = threadIdx.x; i < TH * TH; i += kBlock) {
+        const int gx = (tx0 + i % TH - 1 + L) % L, gy = (ty0 + i / TH - 1 + L) % L;
+        c64 o0, o1;
+        dslash_site(D, X, mu, 0, gx + L * gy, gx, gy, o0, o1);
+        chi_s[i][0] = o0;
+        chi_s[i][1] = o1;
+    }
+    __syncthreads();
+    const int lx = threadIdx.x % kForceTile, ly = threadIdx.x / kForceTile;
+    const int x = tx0 + lx, y = ty0 + ly;
+    if (x >= L || y >= L) return;
+    const int s = x + L * y;
+    auto li = [&](int dx, int dy) { return (ly + 1 + dy) * TH + (lx + 1 + dx); };
+    const int c = li(0, 0), cx = li(1, 0), cy = li(0, 1);
+    double f0, f1;
+    hop_force_vals(D, X, s, nb_site(L, x + 1, y), nb_site(L, x, y + 1), chi_s[c][0], chi_s[c][1], chi_s[cx][0],
+                   chi_s[cx][1], chi_s[cy][0], chi_s[cy][1], f0, f1);
+    if (csw != 0.0) {
+        // sigma_3 weight at lattice site t, chi from the LDS entry of the same site
+        auto W = [&](int t) {
+            const int tx = t % L, ty = t / L;
+            int dx = tx - x, dy = ty - y;   // the offset in -1..1, undoing the wrap
+            if (dx > 1) dx -= L;
+            if (dx < -1) dx += L;
+            if (dy > 1) dy -= L;
+            if (dy < -1) dy += L;
+            const int k = li(dx, dy);
+            const c64 a = cdot(chi_s[k][0], X[2 * t]), b = cdot(chi_s[k][1], X[2 * t + 1]);
+            return a.re - b.re;
+        };
+        double g0, g1;
+        clover_insert_site(D, s, x, y, W, g0, g1);
+        f0 += (csw / 4.0) * g0;
+        f1 += (csw / 4.0) * g1;
+    }
+    F[s] = f0;
+    F[V + s] = f1;
+}
+void launch_force_fused(const DiracView<double>& D, double csw, double mu, const c64* X, double* F,
+                        hipStream_t st) {
+    const int nt = (D.L + kForceTile - 1) / kForceTile;
+    k_force_fused<<<nt * nt, kBlock, 0, st>>>(D, csw, mu, X, F);
     SMG_LAUNCH_CHECK();
 }
 

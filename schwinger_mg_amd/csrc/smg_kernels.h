@@ -66,6 +66,10 @@ void launch_sigma3_weight(int V, const c64* chi, const c64* X, double* w, hipStr
 // fused backward: F = hopping(chi, X) + (csw/4) clover insertion of w(chi, X)
 void launch_force_backward(const DiracView<double>& D, double csw, const c64* chi, const c64* X, double* F,
                            hipStream_t st);
+// the whole force from X: chi = D X recomputed per tile in LDS (never stored), then the
+// hopping force and clover insertion as launch_force_backward
+void launch_force_fused(const DiracView<double>& D, double csw, double mu, const c64* X, double* F,
+                        hipStream_t st);
 // e/o weights for the Schur force: w[s] on both parities
 void launch_eo_force_weights(const DiracView<double>& D, const c64* chi_o, const c64* psi_o, const c64* tmp3,
                              const c64* tmp2, double* w, hipStream_t st);

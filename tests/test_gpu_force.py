@@ -35,6 +35,24 @@ def test_forces_vs_oracle(smg, ci):
     assert abs(smg.gauge_action(L, U, 4.0) - O.gauge_action(L, U, 4.0)) < 1e-11 * abs(O.gauge_action(L, U, 4.0))
 
 
+@pytest.mark.parametrize("L", [2, 6, 8, 12, 16, 30, 48])
+@pytest.mark.parametrize("ci", range(4))
+def test_fused_force_recomputes_chi(smg, L, ci):
+    """fermion_force runs one kernel that recomputes chi = D X per tile in LDS: it equals
+    the two-pass forward (chi stored) + backward pair and the oracle, on lattices smaller
+    than, equal to and not a multiple of the 16 x 16 tile."""
+    csw, mu = CONFIGS[ci]
+    banner(f"fused force L={L} csw={csw} mu={mu}")
+    V = L * L
+    U = random_gauge(L, 0.5, 70 + L)
+    X = gaussian_vec(2 * V, 71 + L)
+    Dg = smg.DiracOp(L, U, 0.1, 1.0, csw, mu)
+    fused = smg.fermion_force(Dg, X)
+    two_pass = smg.fermion_force_bilinear(Dg, smg.force_forward(Dg, X), X)
+    assert relerr(fused, two_pass) < 1e-14
+    assert relerr(fused, O.Dirac(L, U, 0.1, 1.0, csw, mu).force(X)) < 1e-12
+
+
 def test_forces_golden(smg):
     banner("forces golden L=8")
     g = np.load(GOLD)
