@@ -85,6 +85,12 @@ def parse():
                     help="also time the 40-lattice batch of the workload (one launch over all lattices)")
     ap.add_argument("--lattice-mg", type=int, default=1,
                     help="lattice batch: also the three-level MG over all 40 lattices with GCR")
+    ap.add_argument("--summarize-trace", default=None, metavar="DIR",
+                    help="print the per-range kernel table of a rocprofv3 kernel+marker trace and exit")
+    ap.add_argument("--range", default="mixed_batch", help="--summarize-trace: the marker range name")
+    ap.add_argument("--summarize-pmc", nargs="+", default=None, metavar="DIR",
+                    help="print per-kernel counter averages of rocprofv3 --pmc runs and exit")
+    ap.add_argument("--kernel-filter", default=None, help="--summarize-pmc: kernels whose name contains this")
     ap.add_argument("--workload", choices=("solve", "hmc", "kernels"), default="solve",
                     help="solve: the batched MG solve (default); hmc: trajectories of the nested Omelyan HMC; "
                          "kernels: the hierarchy's setup, then only the stencil and coarse-operator launches (and a "
@@ -252,7 +258,9 @@ def lattice_batch(smg, torch, gaussian_vec, random_gauge, args):
     B.cg(bs, 5000, 1e-10)   # warm-up
     smg.synchronize()
     t0 = time.perf_counter()
+    marker(torch, "lattice_cg", True)
     _, its, tr = B.cg(bs, 5000, 1e-10)
+    marker(torch, "lattice_cg", False)
     dt = time.perf_counter() - t0
     out = {"lattices": len(Ls), "L": [64, 96, 128, 192, 256], "sites": sites,
            "dslash_f64_us": round(us, 2), "dslash_f64_GBs": round(byt / (us * 1e-6) / 1e9, 1),
@@ -261,7 +269,8 @@ def lattice_batch(smg, torch, gaussian_vec, random_gauge, args):
     if not args.lattice_mg:
         return out
     kw = dict(n_levels=3, blocks=((4, 4), (2, 2)), k=(32, 32), null_iters=args.null_iters, pre_smooth=args.pre,
-              post_smooth=args.post, coarse_maxit=args.coarse_maxit, coarse_tol=args.coarse_tol, chiral=True)
+              post_smooth=args.post, coarse_maxit=args.coarse_maxit, coarse_tol=args.coarse_tol, chiral=True,
+              coarse_half=bool(args.coarse_bf16))
     starts = []
     for g, L in enumerate(Ls):
         sz = smg.mg_start_sizes(L, kw["n_levels"], kw["blocks"], kw["k"], kw["chiral"])
@@ -277,7 +286,9 @@ def lattice_batch(smg, torch, gaussian_vec, random_gauge, args):
     M.gcr(bs, 500, 1e-10, trunc)   # warm-up (captures the cycle)
     smg.synchronize()
     t0 = time.perf_counter()
+    marker(torch, "lattice_mg_gcr", True)
     _, mits, mtr, _ = M.gcr(bs, 500, 1e-10, trunc)
+    marker(torch, "lattice_mg_gcr", False)
     mdt = time.perf_counter() - t0
     x = torch.empty_like(src)
 
@@ -294,7 +305,8 @@ def lattice_batch(smg, torch, gaussian_vec, random_gauge, args):
         e1.record(st)
     ev[-1][1].synchronize()
     cyc_ms = sum(e0.elapsed_time(e1) for e0, e1 in ev) / 10
-    out.update({"mg": "3 levels, 4x4 then 2x2 aggregates, 16 near-null vectors (chiral, k=32), FP64, "
+    out.update({"mg": "3 levels, 4x4 then 2x2 aggregates, 16 near-null vectors (chiral, k=32), FP64 vectors, "
+                      f"{'FP16' if args.coarse_bf16 else 'FP64'} coarse links, "
                       f"MR({args.pre},{args.post}), coarsest CG {args.coarse_maxit} x tol {args.coarse_tol}, "
                       f"GCR({trunc}), graph-replayed cycle",
                 "mg_setup_s": round(setup_s, 2), "mg_cycle_ms": round(cyc_ms, 3),
@@ -463,6 +475,64 @@ def hmc_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def summarize_trace(d, range_name):
+    """Markdown table of the kernels whose start lies inside each marker range named
+    range_name, from a rocprofv3 --kernel-trace --marker-trace csv directory (per range:
+    launches, ms and average us of every kernel)."""
+    import collections
+    import csv
+    import glob
+    kt, mk = [], []
+    for p in glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True):
+        kt += list(csv.DictReader(open(p)))
+    for p in glob.glob(os.path.join(d, "**", "*marker_api_trace.csv"), recursive=True):
+        mk += list(csv.DictReader(open(p)))
+    rng = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in mk
+           if range_name in r.get("Function", "") + r.get("Name", "") + r.get("Message", "")]
+    agg = collections.defaultdict(lambda: [0, 0.0])
+    busy = 0.0
+    for r in kt:
+        s0, e0 = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+        if any(a <= s0 <= b for a, b in rng):
+            g = agg[r["Kernel_Name"].split("(")[0].replace("void ", "")]
+            g[0] += 1
+            g[1] += (e0 - s0) / 1e3
+            busy += (e0 - s0) / 1e6
+    nb = max(1, len(rng))
+    span = sum(b - a for a, b in rng) / 1e6 / nb
+    out = [f"{len(rng)} ranges named {range_name}; per range: span {span:.2f} ms, kernel busy {busy / nb:.2f} ms, "
+           f"{sum(v[0] for v in agg.values()) / nb:.0f} launches", "",
+           "| kernel | launches / range | ms / range | average us |", "|---|---|---|---|"]
+    for k, (n, us) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        out.append(f"| `{k}` | {n / nb:.0f} | {us / nb / 1e3:.2f} | {us / n:.1f} |")
+    return "\n".join(out)
+
+
+def summarize_pmc(dirs, pattern=None):
+    """Markdown table of per-dispatch counter averages per kernel (and grid) from
+    rocprofv3 --pmc csv directories; counters named *_SIZE are in KiB."""
+    import collections
+    import csv
+    import glob
+    agg = collections.defaultdict(lambda: collections.defaultdict(list))
+    names = set()
+    for d in dirs:
+        for p in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            for r in csv.DictReader(open(p)):
+                k = (r["Kernel_Name"].split("(")[0].replace("void ", ""), r.get("Grid_Size", ""))
+                if pattern and pattern not in k[0]:
+                    continue
+                agg[k][r["Counter_Name"]].append(float(r["Counter_Value"]))
+                names.add(r["Counter_Name"])
+    cols = sorted(names)
+    out = ["| kernel | grid | dispatches | " + " | ".join(cols) + " |", "|---|---|---|" + "---|" * len(cols)]
+    for (k, g), cs in sorted(agg.items()):
+        n = max(len(v) for v in cs.values())
+        vals = [f"{sum(cs[c]) / len(cs[c]):.6g}" if cs.get(c) else "" for c in cols]
+        out.append(f"| `{k}` | {g} | {n} | " + " | ".join(vals) + " |")
+    return "\n".join(out)
+
+
 def kernels_workload(args):
     """The hierarchy of the solve workload, tuned, then 20 launches of each timed kernel
     (the FP32 and FP64 stencils at N RHS, every coarse level's apply with dots in FP16
@@ -493,6 +563,12 @@ def kernels_workload(args):
 
 def main():
     args = parse()
+    if args.summarize_trace:
+        print(summarize_trace(args.summarize_trace, args.range))
+        return
+    if args.summarize_pmc:
+        print(summarize_pmc(args.summarize_pmc, args.kernel_filter))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
               file=sys.stderr)

@@ -599,9 +599,10 @@ int smg_lattices_cg(smg_lattices* H, const double* b, double* x, int maxit, doub
  * the one smg_mg_setup builds for that lattice alone (build_mg_hierarchy,
  * src/multigrid.cpp:595-687; starts: every lattice's start vectors in smg_mg_setup's
  * layout, lattice after lattice), then held in one concatenated storage per level.
- * Every kernel of the cycle runs once for all lattices, in FP64; k[l] must be a power of
- * two in 2..64, aggregates square, the coarsest solve CG (coarse_solver 0); use_graph
- * replays the cycle from a captured hipGraph.  */
+ * Every kernel of the cycle runs once for all lattices, with FP64 vectors and arithmetic;
+ * coarse_half != 0 reads the coarse links in FP16 (the single-lattice FP16 copies),
+ * else FP64.  k[l] must be a power of two in 2..64, aggregates square, the coarsest
+ * solve CG (coarse_solver 0); use_graph replays the cycle from a captured hipGraph.  */
 int smg_lattices_mg_setup(smg_lattices_mg** out, smg_lattices* H, const smg_mg_params* prm, const double* starts);
 int smg_lattices_mg_destroy(smg_lattices_mg* M);
 /* MGHierarchy::precondition (src/multigrid.cpp:155-157) on every lattice: one V-cycle

@@ -1007,13 +1007,14 @@ class LatticesMG:
     """Multigrid over the lattices of a Lattices batch (no reference counterpart): every
     lattice's hierarchy is the one MGHierarchy builds for it alone (starts: one start
     array per lattice, as MGHierarchy takes it), held in one storage per level, and
-    every kernel of the cycle and of GCR runs once for all lattices (FP64)."""
+    every kernel of the cycle and of GCR runs once for all lattices (FP64 vectors and
+    arithmetic; coarse_half: the coarse links read in FP16)."""
 
     def __init__(self, B, starts, n_levels=3, blocks=((4, 4), (2, 2)), k=(8, 8), null_iters=20, pre_smooth=3,
                  post_smooth=3, w_cycle=False, coarse_maxit=50, coarse_tol=1e-2, use_graph=False, chiral=True,
-                 null_prec=F64):
+                 null_prec=F64, coarse_half=False):
         p = _mg_params(n_levels, blocks, k, null_iters, pre_smooth, post_smooth, w_cycle, coarse_maxit, coarse_tol,
-                       False, 0, use_graph, chiral, null_prec)
+                       coarse_half, 0, use_graph, chiral, null_prec)
         self.B, self.params = B, p
         cat = np.concatenate([_c(s) for s in starts])
         h = _P()

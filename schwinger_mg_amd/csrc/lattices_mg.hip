@@ -85,9 +85,10 @@ __global__ __launch_bounds__(kBlock) void k_seg_prolong(const SegDesc* __restric
 // stride over its nx*nx*K*K entries in steps of a multiple of kBlock (so of K): every
 // row stays inside one wave and one step, and the loop bound is uniform per workgroup.
 // EPI_RESID: dst = bvec - A src; EPI_DOTS: {<dst, aux>, |dst|^2} per lattice into fin.
-template <int EPI, int K>
+// TL: the links' storage (c64, or ch16: FP16 links widened on load, FP64 arithmetic).
+template <int EPI, int K, typename TL>
 __global__ __launch_bounds__(kBlock) void k_seg_coarse(const SegDesc* __restrict__ ld, int G,
-                                                       const c64* __restrict__ links, const c64* __restrict__ src,
+                                                       const TL* __restrict__ links, const c64* __restrict__ src,
                                                        c64* __restrict__ dst, const c64* __restrict__ bvec,
                                                        const c64* __restrict__ aux, LatFin fin) {
     static_assert(kBlock % K == 0 && K <= 64, "a row of K lanes must sit inside one wave");
@@ -107,10 +108,10 @@ __global__ __launch_bounds__(kBlock) void k_seg_coarse(const SegDesc* __restrict
             const int ym = by == 0 ? n - 1 : by - 1, yp = by + 1 == n ? 0 : by + 1;
             const int nbr[9] = {b, xp + by * n, xm + by * n, bx + yp * n, bx + ym * n,
                                 xp + yp * n, xp + ym * n, xm + yp * n, xm + ym * n};
-            const c64* lk = links + C.link_off + (size_t)b * 9 * KK + e;
+            const TL* lk = links + C.link_off + (size_t)b * 9 * KK + e;
             const c64* s = src + C.off + kj;
 #pragma unroll
-            for (int d = 0; d < 9; d++) sum = sum + lk[d * KK] * s[(size_t)nbr[d] * K];
+            for (int d = 0; d < 9; d++) sum = sum + cvt<double>(load_c(lk + d * KK)) * s[(size_t)nbr[d] * K];
         }
 #pragma unroll
         for (int off = K / 2; off > 0; off >>= 1) {
@@ -300,13 +301,13 @@ void launch_seg_prolong(const SegDesc* fd, const SegDesc* cd, int G, int nwg_f, 
     k_seg_prolong<<<nwg_f, kBlock, 0, st>>>(fd, cd, G, bv, coarse, fine, accumulate, bx, by, chiral);
     SMG_LAUNCH_CHECK();
 }
-template <int EPI>
-static void seg_coarse_k(const SegDesc* ld, int G, int nwg, int K, const c64* links, const c64* src, c64* dst,
+template <int EPI, typename TL>
+static void seg_coarse_k(const SegDesc* ld, int G, int nwg, int K, const TL* links, const c64* src, c64* dst,
                          const c64* bvec, const c64* aux, const LatFin& fin, hipStream_t st) {
     switch (K) {
-#define SMG_SK(k)                                                                                  \
-    case k:                                                                                        \
-        k_seg_coarse<EPI, k><<<nwg, kBlock, 0, st>>>(ld, G, links, src, dst, bvec, aux, fin);      \
+#define SMG_SK(k)                                                                                      \
+    case k:                                                                                            \
+        k_seg_coarse<EPI, k, TL><<<nwg, kBlock, 0, st>>>(ld, G, links, src, dst, bvec, aux, fin);      \
         break;
         SMG_SK(2) SMG_SK(4) SMG_SK(8) SMG_SK(16) SMG_SK(32) SMG_SK(64)
 #undef SMG_SK
@@ -314,13 +315,18 @@ static void seg_coarse_k(const SegDesc* ld, int G, int nwg, int K, const c64* li
         throw Error(-1, "many-lattice coarse operator: k must be a power of two in 2..64");
     }
 }
-void launch_seg_coarse(const SegDesc* lanes, int G, int nwg_lanes, int K, const c64* links, const c64* src,
-                       c64* dst, Epi epi, const c64* bvec, const c64* aux, const LatFin& fin, hipStream_t st) {
+template <typename TL>
+void launch_seg_coarse(const SegDesc* lanes, int G, int nwg_lanes, int K, const TL* links, const c64* src, c64* dst,
+                       Epi epi, const c64* bvec, const c64* aux, const LatFin& fin, hipStream_t st) {
     if (epi == EPI_NONE) seg_coarse_k<EPI_NONE>(lanes, G, nwg_lanes, K, links, src, dst, bvec, aux, fin, st);
     else if (epi == EPI_RESID) seg_coarse_k<EPI_RESID>(lanes, G, nwg_lanes, K, links, src, dst, bvec, aux, fin, st);
     else seg_coarse_k<EPI_DOTS>(lanes, G, nwg_lanes, K, links, src, dst, bvec, aux, fin, st);
     SMG_LAUNCH_CHECK();
 }
+template void launch_seg_coarse<c64>(const SegDesc*, int, int, int, const c64*, const c64*, c64*, Epi, const c64*,
+                                     const c64*, const LatFin&, hipStream_t);
+template void launch_seg_coarse<ch16>(const SegDesc*, int, int, int, const ch16*, const c64*, c64*, Epi, const c64*,
+                                      const c64*, const LatFin&, hipStream_t);
 void launch_seg_mr_update(const SegDesc* d, int G, int nwg, c64* x, c64* r, const c64* Ar, const double* w,
                           hipStream_t st) {
     k_seg_mr_update<<<nwg, kBlock, 0, st>>>(d, G, x, r, Ar, w);

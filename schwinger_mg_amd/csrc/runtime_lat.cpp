@@ -114,7 +114,10 @@ LatticeMG::LatticeMG(const std::vector<const Dirac*>& dirac, const LatDesc* fine
             hent[l].push_back(s);
         }
     for (int l = 0; l + 1 < nlev; l++) bv.emplace_back(bvlen[l]);
-    for (int l = 1; l < nlev; l++) links.emplace_back(lklen[l - 1]);
+    for (int l = 1; l < nlev; l++) {
+        links.emplace_back(lklen[l - 1]);
+        links16.emplace_back(cfg.coarse_half ? lklen[l - 1] * sizeof(ch16) : 0);
+    }
     // build each lattice's hierarchy and copy it in
     size_t soff = 0;
     for (int g = 0; g < G; g++) {
@@ -131,6 +134,9 @@ LatticeMG::LatticeMG(const std::vector<const Dirac*>& dirac, const LatDesc* fine
                         "many-lattice multigrid: coarse link count differs");
             SMG_HIP(hipMemcpyAsync(links[l].p(hent[l + 1][g].link_off), C.l64.p(), C.nlinks() * sizeof(c64),
                                    hipMemcpyDeviceToDevice, st));
+            if (cfg.coarse_half)   // the single-lattice FP16 copy (made exactly Hermitian where it can be)
+                SMG_HIP(hipMemcpyAsync(links16[l].as<ch16>() + hent[l + 1][g].link_off, C.l16.as<ch16>(),
+                                       C.nlinks() * sizeof(ch16), hipMemcpyDeviceToDevice, st));
         }
         SMG_HIP(hipStreamSynchronize(st));   // before the single-lattice hierarchy is freed
     }
@@ -187,8 +193,12 @@ void LatticeMG::apply(int l, c64* dst, const c64* src, Epi epi, const c64* bvec,
         if (epi == EPI_RESID) launch_axpby<double>(dst, bvec, c64{1.0, 0.0}, c64{-1.0, 0.0}, len[0], st);
         return;
     }
-    launch_seg_coarse(lanes[l].as<SegDesc>(), G, nwg_lanes[l], kdim[l], links[l - 1].p(), src, dst, epi, bvec, aux, f,
-                      st);
+    if (cfg.coarse_half)
+        launch_seg_coarse(lanes[l].as<SegDesc>(), G, nwg_lanes[l], kdim[l], links16[l - 1].as<ch16>(), src, dst, epi,
+                          bvec, aux, f, st);
+    else
+        launch_seg_coarse(lanes[l].as<SegDesc>(), G, nwg_lanes[l], kdim[l], links[l - 1].p(), src, dst, epi, bvec,
+                          aux, f, st);
 }
 
 // MR sweeps (src/smoother.cpp:20-45) with each lattice's own step

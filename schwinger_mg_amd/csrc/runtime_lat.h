@@ -22,6 +22,7 @@ struct LatticeMG {
     std::vector<int> kdim;                     // dof per site of every level
     std::vector<Field<double>> bv;             // per transfer: the block vectors, lattice after lattice
     std::vector<Field<double>> links;          // per coarse level l >= 1 (index l - 1), lattice after lattice
+    std::vector<DevMem> links16;               // the same in FP16 (cfg.coarse_half: the cycle reads these)
     // cycle vectors per level (ts, e2s: the second coarse solve of a W-cycle), the
     // coarsest CG's, the fine operator's intermediate
     std::vector<Field<double>> xs, bs, rs, Ars, ts, e2s;

@@ -136,14 +136,15 @@ void launch_ml_norm2(const LatDesc* d, int G, int nwg, const c64* v, const LatFi
 // multigrid over many lattices (lattices_mg.hip).  fd / cd: a finer level's and the
 // next coarser level's tables (one work-item per vector entry); transfers over bx x by
 // aggregates; the coarse operator runs on a table of one work-item per link entry
-// (K = nd of that level, a power of two <= 64).  nwg: the workgroups of the table the
-// launch runs on.
+// (K = nd of that level, a power of two <= 64; links FP64 or FP16, vectors and
+// arithmetic FP64).  nwg: the workgroups of the table the launch runs on.
 void launch_seg_restrict(const SegDesc* fd, const SegDesc* cd, int G, int nwg_c, const c64* bv, const c64* fine,
                          c64* coarse, int bx, int by, int chiral, hipStream_t st);
 void launch_seg_prolong(const SegDesc* fd, const SegDesc* cd, int G, int nwg_f, const c64* bv, const c64* coarse,
                         c64* fine, int accumulate, int bx, int by, int chiral, hipStream_t st);
-void launch_seg_coarse(const SegDesc* lanes, int G, int nwg_lanes, int K, const c64* links, const c64* src,
-                       c64* dst, Epi epi, const c64* bvec, const c64* aux, const LatFin& fin, hipStream_t st);
+template <typename TL>
+void launch_seg_coarse(const SegDesc* lanes, int G, int nwg_lanes, int K, const TL* links, const c64* src, c64* dst,
+                       Epi epi, const c64* bvec, const c64* aux, const LatFin& fin, hipStream_t st);
 void launch_seg_mr_update(const SegDesc* d, int G, int nwg, c64* x, c64* r, const c64* Ar, const double* w,
                           hipStream_t st);
 void launch_seg_cg_init(const SegDesc* d, int G, int nwg, const c64* b, c64* x, c64* r, c64* p, const LatFin& fin,

@@ -82,6 +82,20 @@ def test_lattices_gcr_zero_rhs_and_staggered_convergence(smg):
         assert abs(int(its[g]) - r.iterations) <= 1
 
 
+def test_lattices_gcr_fp16_links(smg):
+    """The cycle on FP16 coarse links (FP64 vectors) inside the FP64 GCR: every lattice
+    still reaches a 1e-10 FP64 true residual, within two iterations of the FP64-link solve."""
+    banner("many-lattice GCR with FP16 coarse links")
+    Us, B, M, starts = _setup(smg)
+    M16 = smg.LatticesMG(B, starts, coarse_half=True, **KW)
+    bs = [gaussian_vec(2 * L * L, 650 + g) for g, L in enumerate(LS)]
+    _, its, tr, _ = M.gcr(bs, 200, 1e-10, trunc=7)
+    x16, its16, tr16, _ = M16.gcr(bs, 200, 1e-10, trunc=7)
+    print(f"FP64 links {list(its)}, FP16 links {list(its16)}; true residuals {list(tr16)}")
+    assert (tr16 < 1.5e-10).all()
+    assert (its16 <= its + 2).all()
+
+
 def test_lattices_cycle_graph_replay(smg):
     banner("many-lattice V-cycle: hipGraph replays vs the eager cycle")
     Us, B, M, starts = _setup(smg, use_graph=False)
