@@ -794,6 +794,66 @@ inline MGHierarchy build_full_mg(const DiracOp& D, const MGConfig& mcfg, const S
     return mg;
 }
 
+// ------------------------------------------------------------------ even-odd multigrid, src/eo_multigrid.h
+// EvenOddMG (src/eo_multigrid.h:229-287) on the Schur operator M^dag M of the odd
+// half-lattice: build draws k_vec start vectors from rng (random_vec, as
+// compute_near_null_space_generic does) and estimate_lambda_max's mt19937(12345)
+// start, then the library builds the near-null vectors, the aggregation of the odd
+// sites of bx x by blocks (bx even), the dense Galerkin operator (copied back into
+// Ac.mat, column j = P^dag M^dag M P e_j) and omega = 0.8 / lambda_max on the device.
+// precondition runs the V-cycle with the caller's A_MdagM as the smoothing operator
+// (called on the host, like any OpApply the library does not recognise).
+struct EvenOddMG {
+    struct Prolongator {   // EvenOddProlongator's shape (src/eo_multigrid.h:26-58)
+        int bx, by, nbx, nby, nblocks, k_vec, coarse_dim;
+    } P_eo;
+    struct CoarseOp {      // EvenOddCoarseOp (src/eo_multigrid.h:158-223): dim and the columns
+        int dim = 0;
+        std::vector<Vec> mat;
+    } Ac;
+    int fine_half_dim;
+    int pre_smooth, post_smooth;
+    double richardson_omega = 0.0;
+
+    EvenOddMG(const Lattice& lat, int bx, int by, int k_vec, int pre = 3, int post = 3)
+        : P_eo{bx, by, lat.L / bx, lat.L / by, (lat.L / bx) * (lat.L / by), k_vec, (lat.L / bx) * (lat.L / by) * k_vec},
+          fine_half_dim(2 * lat.V_half), pre_smooth(pre), post_smooth(post) {}
+
+    void build(const DiracOp& D, int null_iters, std::mt19937& rng) {
+        Vec starts;
+        for (int i = 0; i < P_eo.k_vec; i++) {
+            const Vec v = random_vec(fine_half_dim, rng);
+            starts.insert(starts.end(), v.begin(), v.end());
+        }
+        std::mt19937 rng_local(12345);
+        const Vec lam = random_vec(fine_half_dim, rng_local);
+        smg_eo_mg* h = nullptr;
+        detail::check(smg_eo_mg_setup(&h, D.handle(), P_eo.bx, P_eo.by, P_eo.k_vec, pre_smooth, post_smooth,
+                                      null_iters, detail::raw(starts), detail::raw(lam)));
+        h_.reset(h, [](smg_eo_mg* q) { smg_eo_mg_destroy(q); });
+        double lmax = 0.0;
+        detail::check(smg_eo_mg_info(h, &richardson_omega, &lmax, &Ac.dim));
+        Vec m((size_t)Ac.dim * Ac.dim);
+        detail::check(smg_eo_mg_coarse_matrix(h, detail::raw(m)));
+        Ac.mat.assign(Ac.dim, Vec(Ac.dim));
+        for (int i = 0; i < Ac.dim; i++)
+            for (int j = 0; j < Ac.dim; j++) Ac.mat[j][i] = m[(size_t)i * Ac.dim + j];
+        std::cout << "  E/O MG: fine_half_dim=" << fine_half_dim << " coarse_dim=" << Ac.dim
+                  << " lambda_max=" << lmax << "\n";
+    }
+
+    Vec precondition(const Vec& b, const OpApply& A_MdagM) const {
+        if (!h_) throw std::logic_error("EvenOddMG::precondition before build");
+        detail::check_size(b, fine_half_dim, "EvenOddMG::precondition");
+        Vec x(fine_half_dim);
+        detail::CallbackBox box, none;
+        const smg_op op = detail::make_op(A_MdagM, (size_t)fine_half_dim, box);
+        detail::check_cb(smg_eo_mg_precondition(h_.get(), &op, detail::raw(b), detail::raw(x)), box, none);
+        return x;
+    }
+    std::shared_ptr<smg_eo_mg> h_;
+};
+
 // ------------------------------------------------------------------ Krylov solvers, src/solvers.h
 struct CGResult {
     Vec solution;

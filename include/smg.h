@@ -65,6 +65,7 @@ typedef struct smg_coarse_s smg_coarse;
 typedef struct smg_mg_s smg_mg;
 typedef struct smg_lattices_s smg_lattices;
 typedef struct smg_lattices_mg_s smg_lattices_mg;
+typedef struct smg_eo_mg_s smg_eo_mg;
 
 /* ------------------------------------------------------------------ runtime */
 /* Select the HIP device and create the library's stream.  Optional: the first
@@ -291,6 +292,30 @@ int smg_gcr_solve_op(const smg_op* A, const smg_precond* M, const double* b, dou
 int smg_fgmres_solve_op(const smg_op* A, const smg_precond* M, const double* b, double* x, int restart, int maxit,
                         double tol, int n_ritz, smg_solve_info* info, int* n_pairs, double* ritz_vals,
                         double* ritz_vecs);
+/* ------------------------------------------------------------------ even-odd multigrid
+ * EvenOddMG (src/eo_multigrid.h:26-287; the reference defines it and never calls it):
+ * two levels on the Schur operator M^dag M of the odd half-lattice (vectors of V = L*L
+ * complex numbers, odd half-index order, two spins).  Setup (EvenOddMG::build,
+ * :229-254): k near-null vectors of M^dag M by the generic inverse iteration
+ * (compute_near_null_space_generic, null_iters outer steps of 5 MR sweeps) from the
+ * caller's k start vectors (starts: k x V, the reference's rng draws), the
+ * aggregation of the odd sites of bx x by blocks (EvenOddProlongator; bx even,
+ * bx and by dividing L), the dense coarse operator P^dag M^dag M P (EvenOddCoarseOp,
+ * column by column) and Richardson's omega = 0.8 / lambda_max (estimate_lambda_max,
+ * 30 power iterations from lambda_start).  precondition (:257-286): Richardson
+ * pre-smoothing from zero, the residual restricted, the dense coarse solve, prolonged
+ * and added, Richardson post-smoothing; op is the smoothing operator (NULL: the
+ * setup's M^dag M; or a callback on V complex numbers, the reference's A_MdagM). */
+int smg_eo_mg_setup(smg_eo_mg** out, smg_dirac* D, int bx, int by, int k, int pre_smooth, int post_smooth,
+                    int null_iters, const double* starts, const double* lambda_start);
+int smg_eo_mg_destroy(smg_eo_mg* M);
+/* richardson_omega, the lambda_max estimate and the coarse dimension (any may be NULL) */
+int smg_eo_mg_info(smg_eo_mg* M, double* omega, double* lambda_max, int* coarse_dim);
+/* the dense coarse operator, row-major dim x dim complex (EvenOddCoarseOp::mat, transposed) */
+int smg_eo_mg_coarse_matrix(smg_eo_mg* M, double* out);
+int smg_eo_mg_precondition(smg_eo_mg* M, const smg_op* op, const double* b, double* x);
+int smg_eo_mg_precondition_dev(smg_eo_mg* M, const void* b, void* x, void* stream);
+
 /* mr_smooth_op / richardson_smooth_op, src/smoother.cpp:26-73 */
 int smg_smooth_op(const smg_op* A, int kind, double* x, const double* b, int niter, double omega);
 /* estimate_lambda_max, src/smoother.cpp:43-59, from the start vector v0 */

@@ -58,6 +58,7 @@ def lib():
         L.ref_mg.argtypes = [_I, _P, _D, _D, _D, _D, _I, _I, _I, _I, _I, _P, _I, _I, _I, _I,
                              _P, _P, _I, _D, _I, _P, _P]
         L.ref_gauge_force.argtypes = [_I, _P, _D, _P]
+        L.ref_eo_mg.argtypes = [_I, _P, _D, _D, _D, _D, _I, _I, _I, _I, _I, _I, ctypes.c_uint, _P, _P, _P, _P, _P]
         L.ref_gauge_action.restype = _D
         L.ref_gauge_action.argtypes = [_I, _P, _D]
         L.ref_fermion_force.argtypes = [_I, _P, _D, _D, _D, _D, _I, _P, _P, _P]
@@ -340,3 +341,19 @@ def hmc_ms_trajectory(L, U, mass, r, csw, beta, tau, n_outer, n_inner, omelyan, 
                                 defl_refresh, cg_maxit, cg_tol, b, k, _ptr(_c(nulls)), n_defl, seed, _ptr(out),
                                 _ptr(v0), _ptr(V0), _ptr(v1), _ptr(V1))
     return U, out, (v0, V0), (v1, V1)
+
+
+def eo_mg(L, U, mass, r, csw, mu, bx, by, k, null_iters, pre, post, seed, b):
+    """EvenOddMG built with mt19937(seed), then x = precondition(b): (x, omega,
+    the dense coarse matrix M[i][j])."""
+    V = L * L
+    dim = (L // bx) * (L // by) * k
+    x = np.zeros(V, np.complex128)
+    om = np.zeros(1)
+    cd = np.zeros(1, np.int32)
+    cols = np.zeros((dim, dim), np.complex128)
+    lib().ref_eo_mg(L, _ptr(_c(U)), mass, r, csw, mu, bx, by, k, null_iters, pre, post, seed, _ptr(_c(b)), _ptr(x),
+                    _ptr(om), _ptr(cd), _ptr(cols))
+    assert int(cd[0]) == dim
+    return x, float(om[0]), cols.T.copy()
+

@@ -12,7 +12,9 @@
 #include "solvers.h"
 #include "multigrid.h"
 #include "hmc.h"
+#include "eo_multigrid.h"
 #include <cstring>
+#include <iostream>
 #include <deque>
 #include <random>
 
@@ -605,4 +607,26 @@ API void ref_hmc_ms_trajectory(int L, double* U, double mass, double r, double c
     }
     from_vec(g.U[0], U);
     from_vec(g.U[1], U + 2 * (size_t)lat.V);
+}
+
+// EvenOddMG (src/eo_multigrid.h:229-287): build on the Schur M^dag M with mt19937(seed),
+// then one precondition of b; the Richardson weight, the coarse dimension and the
+// dense coarse matrix (column j at mat + 2*j*dim) are returned.  The build's report
+// line on std::cout is discarded.
+API void ref_eo_mg(int L, const double* U, double m, double r, double csw, double mu, int bx, int by, int k,
+                   int null_iters, int pre, int post, unsigned seed, const double* b, double* x, double* omega,
+                   int* coarse_dim, double* mat) {
+    Ctx c(L, U, m, r, csw, mu);
+    EvenOddMG mg(c.lat, bx, by, k, pre, post);
+    std::mt19937 rng(seed);
+    std::streambuf* keep = std::cout.rdbuf(nullptr);
+    mg.build(c.D, null_iters, rng);
+    std::cout.rdbuf(keep);
+    EvenOddDiracOp eoD(c.D);
+    OpApply A = [&eoD](const Vec& s, Vec& d) { eoD.apply_schur_dag_schur(s, d); };
+    from_vec(mg.precondition(to_vec(b, 2 * c.lat.V_half), A), x);
+    *omega = mg.richardson_omega;
+    *coarse_dim = mg.Ac.dim;
+    if (mat)
+        for (int j = 0; j < mg.Ac.dim; j++) from_vec(mg.Ac.mat[j], mat + 2 * (size_t)j * mg.Ac.dim);
 }

@@ -146,6 +146,9 @@ SIGNATURES = [
     ("smg_lattices_mg_precondition_dev", _I, [_P, _P, _P, _P]),
     ("smg_lattices_gcr", _I, [_P, _P, _P, _I, _D, _I, _P, _P, _P]),
     ("smg_lattices_gcr_dev", _I, [_P, _P, _P, _I, _D, _I, _P, _P, _P, _P]),
+    ("smg_eo_mg_setup", _I, [_P, _P, _I, _I, _I, _I, _I, _I, _P, _P]), ("smg_eo_mg_destroy", _I, [_P]),
+    ("smg_eo_mg_info", _I, [_P, _P, _P, _P]), ("smg_eo_mg_coarse_matrix", _I, [_P, _P]),
+    ("smg_eo_mg_precondition", _I, [_P, _P, _P, _P]), ("smg_eo_mg_precondition_dev", _I, [_P, _P, _P, _P]),
     ("smg_heatbath", _I, [_I, _P, _D, _I, ctypes.c_ulonglong, ctypes.c_uint]),
     ("smg_coarse_lobpcg", _I, [_P, _I, _P, _I, _D, _P, _P, _P]),
     ("smg_mg_update_coarsest_deflation", _I, [_P, _I, _I]),
@@ -1001,6 +1004,74 @@ class Lattices:
         tr = np.zeros(self.G)
         _chk(lib().smg_lattices_cg(self.h, _p(b), _p(x), max_iter, tol, _p(it), _p(tr)))
         return self._split(x), it, tr
+
+
+# a host callable as an smg_op (the boundary's std::function operator)
+_OPFN = ctypes.CFUNCTYPE(_I, _P, _P, _P)
+
+
+class _SmgOp(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_size_t), ("fn", _OPFN), ("ctx", _P), ("dirac", _P), ("coarse", _P)]
+
+
+def _callback_op(f, n):
+    """An smg_op calling f(vector) -> vector on n complex numbers; keep the returned
+    pair alive for the duration of the call."""
+    def cb(ctx, src, dst):
+        try:
+            v = np.ctypeslib.as_array(ctypes.cast(src, ctypes.POINTER(ctypes.c_double)), (2 * n,)).view(np.complex128)
+            out = np.ascontiguousarray(f(v.copy()), np.complex128)
+            if out.shape != (n,):
+                return 1
+            ctypes.memmove(dst, out.ctypes.data, 16 * n)
+            return 0
+        except Exception:   # reported by the library as a failed call
+            return 1
+    fn = _OPFN(cb)
+    return _SmgOp(n, fn, None, None, None), fn
+
+
+class EvenOddMG:
+    """EvenOddMG (src/eo_multigrid.h:229-287): two-level multigrid on the Schur operator
+    M^dag M of D's odd half-lattice.  starts: the k near-null start vectors (k x V), as
+    the reference draws them from its rng (random_vec(V, rng) k times); lambda_start:
+    the power iteration's start (the reference's random_vec(V, mt19937(12345)))."""
+
+    def __init__(self, D, bx, by, k, pre_smooth=3, post_smooth=3, null_iters=20, starts=None, lambda_start=None):
+        V = D.V
+        starts = np.ascontiguousarray(starts, np.complex128).reshape(k, V)
+        lam = random_vec(V, 12345) if lambda_start is None else _c(lambda_start)
+        h = _P()
+        _chk(lib().smg_eo_mg_setup(ctypes.byref(h), D.h, bx, by, k, pre_smooth, post_smooth, null_iters, _p(starts),
+                                   _p(lam)))
+        self.h, self.D, self.n = h, D, V
+        om, lm, cd = _D(), _D(), _I()
+        _chk(lib().smg_eo_mg_info(h, ctypes.byref(om), ctypes.byref(lm), ctypes.byref(cd)))
+        self.richardson_omega, self.lambda_max, self.coarse_dim = om.value, lm.value, cd.value
+        self.fine_half_dim = V
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.smg_eo_mg_destroy(self.h)
+            self.h = None
+
+    def coarse_matrix(self):
+        """The dense P^dag M^dag M P, M[i][j]."""
+        out = np.zeros((self.coarse_dim, self.coarse_dim), np.complex128)
+        _chk(lib().smg_eo_mg_coarse_matrix(self.h, _p(out)))
+        return out
+
+    def precondition(self, b, A_MdagM=None):
+        """One cycle x = M b; A_MdagM: the smoothing operator (None: the setup's M^dag M on
+        the device; else a callable on half-lattice vectors, called on the host)."""
+        x = np.zeros(self.n, np.complex128)
+        if A_MdagM is None:
+            _chk(lib().smg_eo_mg_precondition(self.h, None, _p(_c(b)), _p(x)))
+        else:
+            op, keep = _callback_op(A_MdagM, self.n)
+            _chk(lib().smg_eo_mg_precondition(self.h, ctypes.byref(op), _p(_c(b)), _p(x)))
+            del keep
+        return x
 
 
 class LatticesMG:

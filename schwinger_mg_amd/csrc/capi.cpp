@@ -43,6 +43,9 @@ struct smg_mg_s {
     std::unique_ptr<MG> mg;
     smg_dirac* D;
 };
+struct smg_eo_mg_s {
+    std::unique_ptr<EOMG> mg;
+};
 struct smg_lattices_mg_s {
     std::unique_ptr<LatticeMG> mg;
     smg_lattices* H;
@@ -2380,5 +2383,55 @@ int smg_tune_set(int family, const int* shape, int choice) {
     return guard([&] {
         SMG_REQUIRE(shape, "null argument");
         tune_store(TuneKey{family, shape[0], shape[1], shape[2], shape[3], shape[4]}, choice);
+    });
+}
+
+// ------------------------------------------------------------------ even-odd multigrid
+int smg_eo_mg_setup(smg_eo_mg** out, smg_dirac* D, int bx, int by, int k, int pre_smooth, int post_smooth,
+                    int null_iters, const double* starts, const double* lambda_start) {
+    return guard([&] {
+        SMG_REQUIRE(out && D && starts && lambda_start, "null argument");
+        auto* h = new smg_eo_mg_s();
+        try {
+            h->mg.reset(new EOMG(*D->D, bx, by, k, pre_smooth, post_smooth, null_iters,
+                                 reinterpret_cast<const c64*>(starts), reinterpret_cast<const c64*>(lambda_start)));
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+int smg_eo_mg_destroy(smg_eo_mg* M) { return guard([&] { delete M; }); }
+int smg_eo_mg_info(smg_eo_mg* M, double* omega, double* lambda_max, int* coarse_dim) {
+    return guard([&] {
+        SMG_REQUIRE(M, "null handle");
+        if (omega) *omega = M->mg->omega;
+        if (lambda_max) *lambda_max = M->mg->lmax;
+        if (coarse_dim) *coarse_dim = (int)M->mg->dim;
+    });
+}
+int smg_eo_mg_coarse_matrix(smg_eo_mg* M, double* out) {
+    return guard([&] {
+        SMG_REQUIRE(M && out, "null argument");
+        downc(out, M->mg->mat.p(), M->mg->dim * M->mg->dim);
+    });
+}
+int smg_eo_mg_precondition(smg_eo_mg* M, const smg_op* op, const double* b, double* x) {
+    return guard([&] {
+        SMG_REQUIRE(M && b && x, "null argument");
+        std::unique_ptr<LinOp<double>> own;
+        LinOp<double>& A = op ? op_of(op, own) : M->mg->A;
+        const size_t n = M->mg->A.n();
+        Field<double> bb = up(b, n), xx(n);
+        M->mg->precondition(A, bb.p(), xx.p(), S());
+        downc(x, xx.p(), n);
+    });
+}
+int smg_eo_mg_precondition_dev(smg_eo_mg* M, const void* b, void* x, void* stream) {
+    return guard([&] {
+        SMG_REQUIRE(M && b && x, "null argument");
+        StreamJoin j(stream);
+        M->mg->precondition(M->mg->A, static_cast<const c64*>(b), static_cast<c64*>(x), S());
     });
 }

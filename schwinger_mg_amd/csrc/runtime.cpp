@@ -533,6 +533,68 @@ void MG::near_null(size_t l, LinOp<double>& A, int nv, int iters, c64* vecs, hip
     }
 }
 
+// the half-lattice's row length, after checking the block shape
+static int eo_half_extent(const Dirac& d, int bx, int by) {
+    SMG_REQUIRE(d.L % 2 == 0 && bx >= 2 && bx % 2 == 0 && by >= 1 && d.L % bx == 0 && d.L % by == 0,
+                "even-odd multigrid: even bx, and bx, by dividing an even L");
+    return d.L / 2;
+}
+
+// src/eo_multigrid.h:229-254 (EvenOddMG::build)
+EOMG::EOMG(const Dirac& d, int bx_, int by_, int k_, int pre_, int post_, int null_iters, const c64* starts,
+           const c64* lam_start)
+    : D(d), bx(bx_), by(by_), k(k_), pre(pre_), post(post_), A(d),
+      P(eo_half_extent(d, bx_, by_), d.L, 2, bx_ / 2, by_, k_) {
+    SMG_REQUIRE(k >= 1 && pre >= 0 && post >= 0 && null_iters >= 0, "even-odd multigrid: bad parameters");
+    hipStream_t st = Context::get().stream;
+    const size_t n = A.n();
+    dim = P.coarse_n();
+    SMG_REQUIRE(dim <= 8192, "even-odd multigrid: the dense coarse operator needs <= 8192 unknowns");
+    Field<double> vecs((size_t)k * n);
+    SMG_HIP(hipMemcpyAsync(vecs.p(), starts, (size_t)k * n * sizeof(c64), hipMemcpyHostToDevice, st));
+    null_space<double>(A, k, null_iters, 5, vecs.p(), st);   // compute_near_null_space_generic, 5 MR steps
+    P.build(vecs.p(), st);
+    // EvenOddCoarseOp::build (src/eo_multigrid.h:164-176): column j = P^dag A P e_j
+    mat = Field<double>(dim * dim);
+    Field<double> e(dim), f(n), Af(n), col(dim);
+    static const c64 one{1.0, 0.0};
+    for (size_t j = 0; j < dim; j++) {
+        launch_zero<double>(e.p(), dim, st);
+        SMG_HIP(hipMemcpyAsync(e.p(j), &one, sizeof(c64), hipMemcpyHostToDevice, st));
+        launch_prolong<double>(P.A, P.bv64.p(), e.p(), f.p(), 0, st);
+        A.apply(Af.p(), f.p(), st);
+        launch_restrict<double>(P.A, P.bv64.p(), Af.p(), col.p(), st);
+        SMG_HIP(hipMemcpy2DAsync(mat.p(j), dim * sizeof(c64), col.p(), sizeof(c64), sizeof(c64), dim,
+                                 hipMemcpyDeviceToDevice, st));
+    }
+    // the dense solve of src/eo_multigrid.h:188-220 (LU without pivoting) as the
+    // inverse, formed once by Gauss-Jordan without pivoting
+    inv = Field<double>(dim * dim);
+    launch_copy<double>(inv.p(), mat.p(), dim * dim, st);
+    Field<double> prow(dim), pcol(dim);
+    launch_dense_invert(inv.p(), dim, prow.p(), pcol.p(), st);
+    // estimate_lambda_max(A, n, 30) from the caller's start vector; omega = 0.8 / lambda_max
+    Field<double> v(n), Av(n);
+    SMG_HIP(hipMemcpyAsync(v.p(), lam_start, n * sizeof(c64), hipMemcpyHostToDevice, st));
+    lmax = lambda_max<double>(A, v.p(), Av.p(), 30, st);
+    omega = 0.8 / lmax;
+    r = Field<double>(n);
+    rc = Field<double>(dim);
+    ec = Field<double>(dim);
+    SMG_HIP(hipStreamSynchronize(st));
+}
+
+// src/eo_multigrid.h:257-286: Richardson pre-smoothing from x = 0, the residual
+// restricted, the dense coarse solve prolonged and added, Richardson post-smoothing
+void EOMG::precondition(LinOp<double>& Aop, const c64* b, c64* x, hipStream_t st) {
+    SMG_REQUIRE(Aop.n() == A.n(), "even-odd multigrid: the operator's size differs from the half-lattice's");
+    richardson_smooth<double>(Aop, x, b, r.p(), pre, omega, true, st);   // leaves r = b - A x
+    launch_restrict<double>(P.A, P.bv64.p(), r.p(), rc.p(), st);
+    launch_dense_matvec<double>(inv.p(), rc.p(), ec.p(), dim, st);
+    launch_prolong<double>(P.A, P.bv64.p(), ec.p(), x, 1, st);
+    richardson_smooth<double>(Aop, x, b, r.p(), post, omega, false, st);
+}
+
 // src/multigrid.cpp:595-687 with geometric aggregation at every level and the
 // 9-point Galerkin operator at every coarse level.
 MG::MG(const Dirac& d, const MGConfig& c, const c64* starts, bool on_device, bool schur) : cfg(c), D(&d) {

@@ -290,6 +290,31 @@ struct MG : Preconditioner {
     LinOp<double>& level_op64(int l);
 };
 
+// Even-odd multigrid on the Schur operator M^dag M of the odd half-lattice
+// (EvenOddMG, src/eo_multigrid.h:26-287): near-null vectors of M^dag M by the generic
+// inverse iteration, aggregation of the odd sites of bx x by blocks, a dense Galerkin
+// operator P^dag M^dag M P probed column by column and inverted once, Richardson
+// smoothing with omega = 0.8 / lambda_max.  The odd sites of a bx x by block (bx even)
+// are, in the half-lattice's (L/2) x L grid of odd sites (half-index y*L/2 + x/2), the
+// (bx/2) x by block in the same row-major order, so the prolongator is the geometric
+// Aggregation on that grid.
+struct EOMG {
+    const Dirac& D;
+    int bx, by, k, pre, post;
+    SchurOp<double> A;
+    Aggregation P;
+    size_t dim = 0;
+    Field<double> mat, inv;   // P^dag A P and its inverse, row-major dim x dim
+    double lmax = 0.0, omega = 0.0;
+    Field<double> r, rc, ec;
+    // starts: k host vectors of the half-lattice (V entries), the null-space starts;
+    // lam_start: the power iteration's start vector (host)
+    EOMG(const Dirac& d, int bx, int by, int k, int pre, int post, int null_iters, const c64* starts,
+         const c64* lam_start);
+    // x = M b with the smoothing operator Aop (the setup's A, or a caller's)
+    void precondition(LinOp<double>& Aop, const c64* b, c64* x, hipStream_t st);
+};
+
 // ------------------------------------------------------------------ Krylov
 struct SolveResult {
     int iterations = 0;

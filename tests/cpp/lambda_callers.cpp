@@ -9,7 +9,7 @@
 // trip per application, except that a callback operator's dot products are a pass
 // of their own instead of being fused into the stencil (another summation order).
 // An operator that is not a Dirac operator at all and a callback that throws are
-// checked too.  Prints one line per check; exits non-zero on a failure.
+// checked too, and so is EvenOddMG preconditioning with a lambda smoothing operator.  Prints one line per check; exits non-zero on a failure.
 #include <cmath>
 #include <cstdio>
 #include <random>
@@ -150,6 +150,35 @@ int main() {
         caught = std::string(e.what()) == "operator refused";
     }
     report("an OpApply's exception reaches the caller", caught ? 0.0 : 1.0, 0.0);
+
+    // EvenOddMG (src/eo_multigrid.h:229-287) as its reference would be used: built from
+    // an rng, preconditioning with a lambda around apply_schur_dag_schur; against the
+    // C-ABI cycle on the same starts (a copy of the rng) and the setup's own operator
+    EvenOddDiracOp eoD(D);
+    OpApply AMM = [&eoD](const Vec& s, Vec& d) { eoD.apply_schur_dag_schur(s, d); };
+    EvenOddMG emg(lat, 4, 4, 4, 3, 3);
+    std::mt19937 rng_e(5), rng_e2 = rng_e;
+    emg.build(D, 8, rng_e);
+    const Vec bo = random_vec(emg.fine_half_dim, rng);
+    const Vec xe = emg.precondition(bo, AMM);
+    Vec st;
+    for (int i = 0; i < 4; i++) {
+        const Vec v = random_vec(emg.fine_half_dim, rng_e2);
+        st.insert(st.end(), v.begin(), v.end());
+    }
+    std::mt19937 rng_l(12345);
+    const Vec lam = random_vec(emg.fine_half_dim, rng_l);
+    smg_eo_mg* eh = nullptr;
+    int erc = smg_eo_mg_setup(&eh, D.handle(), 4, 4, 4, 3, 3, 8, reinterpret_cast<const double*>(st.data()),
+                              reinterpret_cast<const double*>(lam.data()));
+    Vec xn(emg.fine_half_dim);
+    if (erc == 0)
+        erc = smg_eo_mg_precondition(eh, nullptr, reinterpret_cast<const double*>(bo.data()),
+                                     reinterpret_cast<double*>(xn.data()));
+    report("EvenOddMG setup and precondition calls", erc, 0);
+    report("EvenOddMG::precondition(lambda) vs the device operator", relerr(xe, xn), 1e-13);
+    report("EvenOddMG coarse dimension", std::abs(emg.Ac.dim - 64), 0);
+    smg_eo_mg_destroy(eh);
 
     orc_op_free(Ao);
     orc_dirac_free(Do);

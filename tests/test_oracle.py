@@ -620,3 +620,20 @@ def test_lobpcg_exact_ax_consistent_and_converging():
         rq = np.array([np.vdot(x, Ad @ x).real for x in X0])
         worst = max(worst, np.abs(ev - rq).max() / rq.max())
     assert worst > 1e-3
+
+
+# ------------------------------------------------------------------ even-odd multigrid
+@pytest.mark.parametrize("ci", range(3))
+def test_eo_mg_oracle_vs_reference_fixture(ci):
+    """The numpy EvenOddMG (oracle/eo_mg.py) against the reference's own EvenOddMG
+    (tests/golden/eo_mg_vectors.npz): Richardson weight, dense Galerkin matrix and one
+    preconditioner application."""
+    from oracle.eo_mg import EvenOddMG
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "eo_mg_vectors.npz"))
+    L, csw, mu, bx, by, k, ni, pre, post, _ = g[f"c{ci}_params"]
+    L, bx, by, k, ni, pre, post = (int(v) for v in (L, bx, by, k, ni, pre, post))
+    M = EvenOddMG(O.Dirac(L, g[f"c{ci}_U"], 0.1, 1.0, csw, mu), L, bx, by, k, ni, pre, post, g[f"c{ci}_starts"],
+                  g[f"c{ci}_lam"])
+    assert abs(M.omega / g[f"c{ci}_omega"][0] - 1) < 1e-13
+    assert close(M.mat, g[f"c{ci}_mat"], 1e-13)
+    assert close(M.precondition(g[f"c{ci}_b"]), g[f"c{ci}_x"], 1e-12)
