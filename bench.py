@@ -322,8 +322,8 @@ def cpu_baseline(args, U, B, mg, threads):
     copied into the oracle (its own setup, minutes at L=512, is not timed, as the
     GPU setup is not part of value).  A bounded sample: right-hand sides are solved
     one at a time (each capped at 30 iterations) until about 10 s have passed."""
-    os.environ["OMP_NUM_THREADS"] = str(threads)
     import oracle as O
+    threads = O.set_threads(threads)   # the OpenMP runtime may predate any environment change
     L = args.L
     D = O.Dirac(L, U, args.mass, 1.0, 1.0, 0.0)
     A = D.op()

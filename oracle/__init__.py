@@ -36,6 +36,8 @@ def lib():
         if not os.path.exists(path):
             raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
         L = ctypes.CDLL(path)
+        L.orc_set_threads.argtypes = [_I]
+        L.orc_get_threads.restype = _I
         L.orc_dirac_new.restype = _P
         L.orc_dirac_new.argtypes = [_I, _P, _D, _D, _D, _D]
         L.orc_dirac_free.argtypes = [_P]
@@ -167,6 +169,13 @@ class MGParams(ctypes.Structure):
 
 D_, DDAG, DDAGD, G5D = 0, 1, 2, 3
 SCHUR_NORMAL = 100
+
+
+def set_threads(n):
+    """The OpenMP thread count of the oracle (and of anything sharing its runtime);
+    returns the count now in effect."""
+    lib().orc_set_threads(n)
+    return lib().orc_get_threads()
 
 
 def eo_maps(L):

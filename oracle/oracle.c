@@ -18,6 +18,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <omp.h>
 
 #define OMP_MIN 4096
 #define OMP_FOR _Pragma("omp parallel for schedule(static) if(n > OMP_MIN)")
@@ -2397,3 +2398,8 @@ int orc_deflated_coarse_cg(const orc_op* A, int ndefl, const ocx* vecs, const do
     free(r); free(p); free(Ap);
     return it;
 }
+
+/* the OpenMP thread count of the oracle's parallel loops (the process's OpenMP runtime
+ * may have started before OMP_NUM_THREADS was set, so it is set here explicitly) */
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads(void) { return omp_get_max_threads(); }

@@ -38,6 +38,8 @@ void orc_gauge_force(int L, const ocx* U, double beta, double* F);
 typedef struct orc_dirac orc_dirac;
 orc_dirac* orc_dirac_new(int L, const ocx* U, double mass, double r, double csw, double mu_t);
 void orc_dirac_free(orc_dirac* D);
+void orc_set_threads(int n);
+int orc_get_threads(void);
 const double* orc_dirac_clover(const orc_dirac* D);
 enum { ORC_D = 0, ORC_DDAG = 1, ORC_DDAGD = 2, ORC_G5D = 3 };
 void orc_dirac_apply(const orc_dirac* D, int kind, const ocx* src, ocx* dst);
