@@ -690,10 +690,13 @@ int smg_therm_cfg_path(int L, double beta, int n_therm, char* out, size_t cap);
  *   SMG_TUNE_DSLASH_DOTS: the batched stencil with dot products; (L, nrhs, bytes per
  *     real, 0, 0); choice = g;
  *   SMG_TUNE_DSLASH_STORE: the batched stencil's stores; (L, nrhs, bytes per real,
- *     epilogue, 0); choice 1 = non-temporal stores, 0 = ordinary (bit-identical results). */
+ *     epilogue, 0); choice 1 = non-temporal stores, 0 = ordinary (bit-identical results);
+ *   SMG_TUNE_VEC_RED: the coarsest BF16-vector CG's residual update with |r|^2;
+ *     (coarse unknowns per RHS, nrhs, 0, 0, 0); choice = g. */
 #define SMG_TUNE_COARSE_BF16 0
 #define SMG_TUNE_DSLASH_DOTS 1
 #define SMG_TUNE_DSLASH_STORE 2
+#define SMG_TUNE_VEC_RED 3
 /* times the candidates of every kernel of mg's batched cycle for nrhs (16, 32 or 64)
  * right-hand sides and keeps the fastest; *entries = table entries added (may be NULL) */
 int smg_tune_mg(smg_mg* mg, int nrhs, int* entries);

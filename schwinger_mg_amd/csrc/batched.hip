@@ -1823,7 +1823,10 @@ void launch_h_cg_init(const c32* b, cb16* r, cb16* p, cb16* x, size_t n, const F
 }
 void launch_h_cg_r(cb16* r, const cb16* Ap, const double* a, size_t n, const FinB& fin, int N, int* late,
                    hipStream_t st) {
-    k_h_cg_r<<<b_grid(n * N / 4, kRedGridVec), kBlock, 0, st>>>(r, Ap, a, n * N / 4, fin, N, late);
+    // the tuner's workgroup count for this shape (it fixes the summation order), else the default
+    const int choice = tune_lookup(TuneKey{TUNE_VEC_RED, (int)n, N, 0, 0, 0});
+    const int cap = choice < 0 ? kRedGridVec : kTuneGrids[choice];
+    k_h_cg_r<<<b_grid(n * N / 4, cap), kBlock, 0, st>>>(r, Ap, a, n * N / 4, fin, N, late);
     SMG_LAUNCH_CHECK();
 }
 void launch_h_cg_xp(cb16* x, cb16* p, const cb16* r, const double* a, const double* b, size_t n, int N,

@@ -552,6 +552,15 @@ int tune_mg(MG& mg, int N, hipStream_t st) {
                 launch_b_coarse_apply_h(C.view16(), hd.as<cb16>(), hs.as<cb16>(), EPI_DOTS, hs.as<cb16>(), fin, N,
                                         st);
             }, st);
+            // the CG's residual update with |r|^2 (alpha = 0 leaves r as it is)
+            Field<double> a0((size_t)N);
+            launch_zero<double>(a0.p(), N, st);
+            std::vector<int> vgrids;
+            for (int g = 0; g < kTuneNGrids; g++) vgrids.push_back(g);
+            pick(TuneKey{TUNE_VEC_RED, (int)C.n(), N, 0, 0, 0}, vgrids, [&] {
+                launch_h_cg_r(hd.as<cb16>(), hs.as<cb16>(), reinterpret_cast<const double*>(a0.p()), C.n(), fin, N,
+                              nullptr, st);
+            }, st);
         }
     }
     // the fine stencil with fused dots (the second pass of D^dag D in the smoother and CG)

@@ -21,7 +21,10 @@ enum TuneFamily : int {
     TUNE_DSLASH_DOTS = 1,
     // k_b_dslash stores; shape (L, N, prec, epi, 0); choice 1: non-temporal stores, 0: ordinary
     TUNE_DSLASH_STORE = 2,
-    TUNE_FAMILIES = 3,
+    // k_h_cg_r (the BF16-vector CG's residual update with |r|^2); shape (n, N, 0, 0, 0);
+    // choice = grid index
+    TUNE_VEC_RED = 3,
+    TUNE_FAMILIES = 4,
 };
 using TuneKey = std::array<int, 6>;   // family, then five shape words
 

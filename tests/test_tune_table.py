@@ -39,6 +39,8 @@ def test_refusals(smg, tmp_path):
         smg.tune_set(smg.TUNE_DSLASH_DOTS, [512, 32, 4, 0, 0], 5)      # no sixth grid
     with pytest.raises(smg.SMGError):
         smg.tune_set(7, [1, 2, 3, 4, 5], 0)                              # no such family
+    with pytest.raises(smg.SMGError):
+        smg.tune_set(smg.TUNE_VEC_RED, [131072, 32, 0, 0, 0], 5)         # no sixth grid
     bad = str(tmp_path / "bad.txt")
     open(bad, "w").write("0 64 64 32 32 2 1\n0 64 64 32 32 x 1\n")
     with pytest.raises(smg.SMGError):
