@@ -90,7 +90,14 @@ def parse():
     ap.add_argument("--range", default="mixed_batch", help="--summarize-trace: the marker range name")
     ap.add_argument("--summarize-pmc", nargs="+", default=None, metavar="DIR",
                     help="print per-kernel counter averages of rocprofv3 --pmc runs and exit")
-    ap.add_argument("--kernel-filter", default=None, help="--summarize-pmc: kernels whose name contains this")
+    ap.add_argument("--summarize-kernels", default=None, metavar="DIR",
+                    help="print per-kernel, per-grid durations of a rocprofv3 kernel trace and exit")
+    ap.add_argument("--kernel-filter", default=None,
+                    help="--summarize-pmc / --summarize-kernels: kernels whose name contains this")
+    ap.add_argument("--kernels-select", default=None,
+                    help="kernels workload: time only the kernels whose name contains this")
+    ap.add_argument("--tune-load", default=None, metavar="PATH", help="kernels workload: read this tuner table")
+    ap.add_argument("--tune-save", default=None, metavar="PATH", help="kernels workload: write the tuner table")
     ap.add_argument("--workload", choices=("solve", "hmc", "kernels"), default="solve",
                     help="solve: the batched MG solve (default); hmc: trajectories of the nested Omelyan HMC; "
                          "kernels: the hierarchy's setup, then only the stencil and coarse-operator launches (and a "
@@ -142,9 +149,9 @@ def stencil_bytes_per_site(prec, nrhs, csw):
 
 def time_stencil(smg, torch, D, prec, nrhs, V, csw, launches=50, warm=5):
     """Average duration of one Wilson-clover launch on the batched layout (RHS
-    innermost, the layout of the solve): a HIP event pair around each launch on
-    the library's stream (the stream the kernel runs on), so host-side gaps
-    between launches are not counted."""
+    innermost, the layout of the solve): a HIP event pair on the library's stream (the
+    stream the kernel runs on) around `launches` back-to-back launches, as the solve
+    issues them, divided by their number."""
     dt = torch.complex128 if prec == "f64" else torch.complex64
     src = torch.randn(2 * V, nrhs, dtype=dt, device="cuda")
     dst = torch.empty_like(src)
@@ -161,13 +168,13 @@ def time_stencil(smg, torch, D, prec, nrhs, V, csw, launches=50, warm=5):
     for _ in range(warm):
         launch()
     smg.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(launches)]
-    for e0, e1 in ev:
-        e0.record(st)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(launches):
         launch()
-        e1.record(st)
-    ev[-1][1].synchronize()
-    ms = sum(e0.elapsed_time(e1) for e0, e1 in ev) / launches
+    e1.record(st)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / launches
     byt = stencil_bytes_per_site(prec, nrhs, csw) * V
     return {"prec": prec, "nrhs": nrhs, "us_per_launch": ms * 1e3, "bytes_per_launch": byt,
             "GBs": byt / (ms * 1e-3) / 1e9}
@@ -179,7 +186,8 @@ PEAKS = {"hbm": 8000.0, "bf16": 2500.0, "f64": 78.6, "f32": 157.3}   # GB/s; TFL
 def time_coarse(smg, torch, mg, level, prec, nrhs, k, launches=50, warm=5):
     """Average duration of one launch of the hierarchy's level-`level` coarse operator on
     the batched layout with the fused dot products (the launch shape of the smoother and
-    the coarse CG): a HIP event pair around each launch on the library's stream.
+    the coarse CG): a HIP event pair around `launches` back-to-back launches on the
+    library's stream, divided by their number.
     Algorithmic bytes per launch Vc (9 k^2 s_L + 2 k s_psi N) (links of s_L bytes per
     complex entry, sources read and results written once); flops 72 k^2 Vc N."""
     import ctypes
@@ -203,13 +211,13 @@ def time_coarse(smg, torch, mg, level, prec, nrhs, k, launches=50, warm=5):
     for _ in range(warm):
         launch()
     smg.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(launches)]
-    for e0, e1 in ev:
-        e0.record(st)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(launches):
         launch()
-        e1.record(st)
-    ev[-1][1].synchronize()
-    us = sum(e0.elapsed_time(e1) for e0, e1 in ev) / launches * 1e3
+    e1.record(st)
+    e1.synchronize()
+    us = e0.elapsed_time(e1) / launches * 1e3
     byt = vc * (9 * k * k * sl + 2 * k * sp * nrhs)
     flop = 72 * k * k * vc * nrhs
     return {"level": level, "prec": {smg.F64: "f64", smg.F16: "f16 links, f32 vectors",
@@ -222,7 +230,7 @@ def lattice_batch(smg, torch, gaussian_vec, random_gauge, args):
     """The configuration-3 batch of 40 lattices (L in {64, 96, 128, 192, 256}, 8 each,
     theta ~ N(0, 0.4^2), independent seeds, m = 0.1, c_sw = 1) as one Lattices handle:
     the FP64 Wilson-clover D over all 40 in one launch (event-timed on the library
-    stream, 50 launches), the lockstep CG on D^dag D of all 40 to 1e-10, and the
+    stream over 50 back-to-back launches), the lockstep CG on D^dag D of all 40 to 1e-10, and the
     three-level MG (4x4 then 2x2 aggregates, 16 near-null vectors per level, chiral:
     k = 32) over all 40 with GCR to 1e-10, every kernel one launch for all lattices
     and the V-cycle replayed from a hipGraph (solves: host-array entries, wall time
@@ -246,13 +254,13 @@ def lattice_batch(smg, torch, gaussian_vec, random_gauge, args):
     for _ in range(5):
         launch()
     smg.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(50)]
-    for e0, e1 in ev:
-        e0.record(st)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(50):
         launch()
-        e1.record(st)
-    ev[-1][1].synchronize()
-    us = sum(e0.elapsed_time(e1) for e0, e1 in ev) / 50 * 1e3
+    e1.record(st)
+    e1.synchronize()
+    us = e0.elapsed_time(e1) / 50 * 1e3
     byt = stencil_bytes_per_site("f64", 1, True) * sites
     bs = [gaussian_vec(2 * L * L, 900 + i) for i, L in enumerate(Ls)]
     B.cg(bs, 5000, 1e-10)   # warm-up
@@ -508,6 +516,25 @@ def summarize_trace(d, range_name):
     return "\n".join(out)
 
 
+def summarize_kernels(d, pattern=None):
+    """Markdown table of launches, average and minimum duration per kernel and grid from
+    a rocprofv3 --kernel-trace csv directory."""
+    import collections
+    import csv
+    import glob
+    agg = collections.defaultdict(list)
+    for p in glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True):
+        for r in csv.DictReader(open(p)):
+            k = (r["Kernel_Name"].split("(")[0].replace("void ", ""), r.get("Grid_Size", ""))
+            if pattern and pattern not in k[0]:
+                continue
+            agg[k].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    out = ["| kernel | grid | launches | average us | min us |", "|---|---|---|---|---|"]
+    for (k, g), ts in sorted(agg.items()):
+        out.append(f"| `{k}` | {g} | {len(ts)} | {sum(ts) / len(ts):.2f} | {min(ts):.2f} |")
+    return "\n".join(out)
+
+
 def summarize_pmc(dirs, pattern=None):
     """Markdown table of per-dispatch counter averages per kernel (and grid) from
     rocprofv3 --pmc csv directories; counters named *_SIZE are in KiB."""
@@ -534,24 +561,41 @@ def summarize_pmc(dirs, pattern=None):
 
 
 def kernels_workload(args):
-    """The hierarchy of the solve workload, tuned, then 20 launches of each timed kernel
-    (the FP32 and FP64 stencils at N RHS, every coarse level's apply with dots in FP16
-    links / FP32 vectors, BF16 vectors at the coarsest, FP64) and a plain 128 MiB copy
-    whose counters calibrate the HBM byte counters.  Prints one line per kernel."""
+    """The hierarchy of the solve workload, tuned (or with the table of --tune-load), then
+    20 launches of each timed kernel (the FP32 and FP64 stencils at N RHS, every coarse
+    level's apply with dots in FP16 links / FP32 vectors, BF16 vectors at the coarsest,
+    FP64; --kernels-select keeps those whose name contains the text), each inside a
+    marker range of its name, and a plain 128 MiB copy whose counters calibrate the HBM
+    byte counters.  Prints one line per kernel."""
     import torch
     import schwinger_mg_amd as smg
     L, V, N = args.L, args.L * args.L, args.nrhs
     D = smg.DiracOp(L, random_gauge(L, args.sigma, 2024), args.mass, 1.0, 1.0, 0.0)
     mg = build_mg(smg, args, D, L, 2024)
-    if args.tune:
+    if args.tune_load:
+        smg.tune_load(args.tune_load)
+    elif args.tune:
         smg.tune_mg(mg, N)
+    if args.tune_save:
+        smg.tune_save(args.tune_save)
+    sel = args.kernels_select or ""
     for p in ("f32", "f64"):
+        name = f"stencil_{p}"
+        if sel not in name:
+            continue
+        marker(torch, name, True)
         s = time_stencil(smg, torch, D, p, N, V, True, launches=20, warm=2)
-        print(f"stencil {p} x{N}: {s['us_per_launch']:.1f} us, {s['GBs']:.0f} GB/s", flush=True)
+        marker(torch, name, False)
+        print(f"{name} x{N}: {s['us_per_launch']:.1f} us, {s['GBs']:.0f} GB/s", flush=True)
     for lev in range(1, args.levels):
         for prec in (smg.F16, smg.F64) + ((smg.BF16V,) if lev == args.levels - 1 else ()):
+            name = f"coarse_l{lev}_" + {smg.F16: "f16", smg.F64: "f64", smg.BF16V: "bf16v"}[prec]
+            if sel not in name:
+                continue
+            marker(torch, name, True)
             c = time_coarse(smg, torch, mg, lev, prec, N, args.k, launches=20, warm=2)
-            print(f"coarse level {lev} {c['prec']}: {c['us_per_launch']:.1f} us, {c['GBs']:.0f} GB/s, "
+            marker(torch, name, False)
+            print(f"{name} ({c['prec']}): {c['us_per_launch']:.1f} us, {c['GBs']:.0f} GB/s, "
                   f"{c['TFLOPs']:.1f} TFLOP/s", flush=True)
     a = torch.empty(1 << 25, dtype=torch.float32, device="cuda").uniform_()
     b = torch.empty_like(a)
@@ -568,6 +612,9 @@ def main():
         return
     if args.summarize_pmc:
         print(summarize_pmc(args.summarize_pmc, args.kernel_filter))
+        return
+    if args.summarize_kernels:
+        print(summarize_kernels(args.summarize_kernels, args.kernel_filter))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
@@ -704,7 +751,8 @@ def main():
                               profiled_traffic(L, N, "coarse", f"level{c['level']}_bf16v"),
                               args.coarse_maxit, PEAKS["bf16"]))
     fp64_lines = [line(f"k_b_coarse_mfma<double> (FP64, level {c['level']}, {c['sites']} sites, k={c['k']}, "
-                       f"{N} RHS)", c, None, args.coarse_maxit if c["level"] == last else args.pre + args.post + 1,
+                       f"{N} RHS)", c, profiled_traffic(L, N, "coarse", f"level{c['level']}_f64"),
+                       args.coarse_maxit if c["level"] == last else args.pre + args.post + 1,
                        PEAKS["f64"]) for c in coarse if c["prec"] == "f64"]
     roofline = max(lines, key=lambda r: r["ms_per_cycle"])   # the mixed solve's dominant kernel
 
