@@ -692,11 +692,15 @@ int smg_therm_cfg_path(int L, double beta, int n_therm, char* out, size_t cap);
  *   SMG_TUNE_DSLASH_STORE: the batched stencil's stores; (L, nrhs, bytes per real,
  *     epilogue, 0); choice 1 = non-temporal stores, 0 = ordinary (bit-identical results);
  *   SMG_TUNE_VEC_RED: the coarsest BF16-vector CG's residual update with |r|^2;
- *     (coarse unknowns per RHS, nrhs, 0, 0, 0); choice = g. */
+ *     (coarse unknowns per RHS, nrhs, 0, 0, 0); choice = g;
+ *   SMG_TUNE_TRANSFER: the batched restriction / prolongation; (nx, ny, 128*nd + k of the
+ *     finer level, nrhs, 2*FP64 + prolong); choice 1 = a workgroup per aggregate with its
+ *     blocks staged in LDS, 0 = one work-item per output (the same sums). */
 #define SMG_TUNE_COARSE_BF16 0
 #define SMG_TUNE_DSLASH_DOTS 1
 #define SMG_TUNE_DSLASH_STORE 2
 #define SMG_TUNE_VEC_RED 3
+#define SMG_TUNE_TRANSFER 4
 /* times the candidates of every kernel of mg's batched cycle for nrhs (16, 32 or 64)
  * right-hand sides and keeps the fastest; *entries = table entries added (may be NULL) */
 int smg_tune_mg(smg_mg* mg, int nrhs, int* entries);

@@ -1174,9 +1174,103 @@ __global__ __launch_bounds__(kBlock) void k_b_prolong_k(AggView A, const cplx<T>
         out.store(fine + i);
     }
 }
+// ---- transfers with a workgroup per aggregate: the aggregate's coarse block (k x N),
+// block vectors (k x bsize) and, for the restriction, fine block (bsize x N) are staged
+// in LDS once with 16-byte copies, and every output is formed from LDS in the same
+// order of terms as the kernels above (the same sums).  Each coarse entry and block
+// vector is then read from HBM once per launch instead of once per output that uses it.
+template <typename T>
+__device__ __forceinline__ void stage16(cplx<T>* __restrict__ dst, const cplx<T>* __restrict__ src, int n) {
+    const int n16 = n * (int)sizeof(cplx<T>) / 16;
+    const uint4* s = reinterpret_cast<const uint4*>(src);
+    uint4* d = reinterpret_cast<uint4*>(dst);
+    for (int i = threadIdx.x; i < n16; i += kBlock) d[i] = s[i];
+}
+// fine(s, c)[j] (+)= sum_kk v_{b,kk}(loc, c) coarse[b*k + kk][j], kk over c's chirality
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_prolong_lds(AggView A, const cplx<T>* __restrict__ bv,
+                                                          const cplx<T>* __restrict__ coarse,
+                                                          cplx<T>* __restrict__ fine, int accumulate, int N) {
+    extern __shared__ uint4 smem_pl[];
+    const int k = A.k, bsize = A.bx * A.by * A.nd, lN = lg2(N);
+    cplx<T>* cs = reinterpret_cast<cplx<T>*>(smem_pl);   // [k][N]
+    cplx<T>* vs = cs + k * N;                             // [k][bsize]
+    const int b = blockIdx.x;
+    stage16(cs, coarse + (size_t)b * k * N, k * N);
+    stage16(vs, bv + (size_t)b * k * bsize, k * bsize);
+    __syncthreads();
+    const int bxi = b % A.nbx, byi = b / A.nbx;
+    const int kh = A.chiral ? k / 2 : k;
+    for (int o = threadIdx.x; o < (bsize << lN); o += kBlock) {
+        const int e = o >> lN, j = o & (N - 1);
+        const int loc = e / A.nd, c = e % A.nd;
+        const int k0 = A.chiral ? agg_chi_dof(A, c) * kh : 0;
+        cplx<T> acc = {(T)0, (T)0};
+        for (int q = 0; q < kh; q++) acc = acc + cs[(k0 + q) * N + j] * vs[(k0 + q) * bsize + e];
+        const int sx = bxi * A.bx + loc % A.bx, sy = byi * A.by + loc / A.bx;
+        cplx<T>* out = fine + (((size_t)(sx + A.nx * sy) * A.nd + c) << lN) + j;
+        *out = accumulate ? *out + acc : acc;
+    }
+}
+// coarse[b*k + kk][j] = sum over the aggregate's (loc, c) of conj(v_{b,kk}(loc, c)) fine(s, c)[j]
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_restrict_lds(AggView A, const cplx<T>* __restrict__ bv,
+                                                           const cplx<T>* __restrict__ fine,
+                                                           cplx<T>* __restrict__ coarse, int N) {
+    extern __shared__ uint4 smem_rs[];
+    const int k = A.k, bs = A.bx * A.by, bsize = bs * A.nd, lN = lg2(N);
+    cplx<T>* fs = reinterpret_cast<cplx<T>*>(smem_rs);   // [bsize][N]
+    cplx<T>* vs = fs + (bsize << lN);                      // [k][bsize]
+    const int b = blockIdx.x;
+    const int bxi = b % A.nbx, byi = b / A.nbx;
+    // a site's nd x N entries are contiguous: stage site by site
+    const int per16 = (A.nd << lN) * (int)sizeof(cplx<T>) / 16;
+    for (int i = threadIdx.x; i < bs * per16; i += kBlock) {
+        const int loc = i / per16, w = i % per16;
+        const int sx = bxi * A.bx + loc % A.bx, sy = byi * A.by + loc / A.bx;
+        const uint4* s = reinterpret_cast<const uint4*>(fine + ((size_t)(sx + A.nx * sy) * A.nd << lN));
+        reinterpret_cast<uint4*>(fs)[loc * per16 + w] = s[w];
+    }
+    stage16(vs, bv + (size_t)b * k * bsize, k * bsize);
+    __syncthreads();
+    const int nc = A.chiral ? A.nd / 2 : A.nd;
+    for (int o = threadIdx.x; o < (k << lN); o += kBlock) {
+        const int kk = o >> lN, j = o & (N - 1);
+        const int c0 = A.chiral ? agg_chi_vec(A, kk) * nc : 0;
+        const cplx<T>* v = vs + kk * bsize;
+        cplx<T> acc = {(T)0, (T)0};
+        for (int loc = 0; loc < bs; loc++)
+            for (int c = c0; c < c0 + nc; c++) acc = acc + cdot(v[loc * A.nd + c], fs[((loc * A.nd + c) << lN) + j]);
+        coarse[(((size_t)b * k + kk) << lN) + j] = acc;
+    }
+}
+constexpr size_t kTransferLdsMax = 64 * 1024;
+// LDS bytes of the staged transfer, or 0 where it does not apply (sizes that are not
+// whole 16-byte runs, or more than kTransferLdsMax)
+size_t b_transfer_lds_bytes(const AggView& A, int N, size_t cbytes, bool prolong) {
+    const size_t bsize = (size_t)A.bx * A.by * A.nd, k = A.k;
+    const size_t n = prolong ? k * N + k * bsize : bsize * N + k * bsize;
+    const bool whole = (k * N * cbytes) % 16 == 0 && (k * bsize * cbytes) % 16 == 0 &&
+                       ((size_t)A.nd * N * cbytes) % 16 == 0;
+    if ((N & (N - 1)) != 0 || !whole || n * cbytes > kTransferLdsMax) return 0;
+    return n * cbytes;
+}
+static int transfer_choice(const AggView& A, int N, size_t cbytes, bool prolong) {
+    return tune_lookup(TuneKey{TUNE_TRANSFER, A.nx, A.ny, 128 * A.nd + A.k, N,
+                               2 * (cbytes == sizeof(c64) ? 1 : 0) + (prolong ? 1 : 0)});
+}
+
 template <typename T>
 void launch_b_prolong(const AggView& A, const cplx<T>* bv, const cplx<T>* coarse, cplx<T>* fine, int accumulate,
                       int N, hipStream_t st) {
+    if (transfer_choice(A, N, sizeof(cplx<T>), true) == 1) {
+        const size_t lds = b_transfer_lds_bytes(A, N, sizeof(cplx<T>), true);
+        if (lds) {
+            k_b_prolong_lds<T><<<A.nbx * A.nby, kBlock, lds, st>>>(A, bv, coarse, fine, accumulate, N);
+            SMG_LAUNCH_CHECK();
+            return;
+        }
+    }
     const size_t items = (size_t)A.nx * A.ny * A.nd * N;
     const int g = b_grid(items, 1 << 16);
     const int kh = A.chiral ? A.k / 2 : A.k;
@@ -1281,6 +1375,14 @@ static bool restrict_nc(const AggView& A, int nc, int g, const cplx<T>* bv, cons
 template <typename T>
 void launch_b_restrict(const AggView& A, const cplx<T>* bv, const cplx<T>* fine, cplx<T>* coarse, int N,
                        hipStream_t st) {
+    if (transfer_choice(A, N, sizeof(cplx<T>), false) == 1) {
+        const size_t lds = b_transfer_lds_bytes(A, N, sizeof(cplx<T>), false);
+        if (lds) {
+            k_b_restrict_lds<T><<<A.nbx * A.nby, kBlock, lds, st>>>(A, bv, fine, coarse, N);
+            SMG_LAUNCH_CHECK();
+            return;
+        }
+    }
     const size_t items = (size_t)A.nbx * A.nby * A.k * N;
     const int g = b_grid(items, 1 << 16);
     const int bs = A.bx * A.by, nc = A.chiral ? A.nd / 2 : A.nd;

@@ -563,6 +563,29 @@ int tune_mg(MG& mg, int N, hipStream_t st) {
             }, st);
         }
     }
+    // the transfers of every level, FP32 (the mixed cycle) and FP64: the per-output kernel
+    // or the workgroup per aggregate with LDS-staged blocks
+    for (size_t l = 0; l < mg.P.size(); l++) {
+        const AggView& A = mg.P[l]->A;
+        auto transfers = [&](auto zero) {
+            using T = decltype(zero);
+            const size_t nf = mg.P[l]->fine_n() * N, nc = mg.P[l]->coarse_n() * N;
+            Field<T> f(nf), c(nc);
+            launch_zero<T>(f.p(), nf, st);
+            launch_zero<T>(c.p(), nc, st);
+            const int prec = sizeof(T) == 8 ? 1 : 0;
+            if (b_transfer_lds_bytes(A, N, sizeof(cplx<T>), true))
+                pick(TuneKey{TUNE_TRANSFER, A.nx, A.ny, 128 * A.nd + A.k, N, 2 * prec + 1}, {0, 1}, [&] {
+                    launch_b_prolong<T>(A, mg.P[l]->bv<T>(), c.p(), f.p(), 1, N, st);
+                }, st);
+            if (b_transfer_lds_bytes(A, N, sizeof(cplx<T>), false))
+                pick(TuneKey{TUNE_TRANSFER, A.nx, A.ny, 128 * A.nd + A.k, N, 2 * prec}, {0, 1}, [&] {
+                    launch_b_restrict<T>(A, mg.P[l]->bv<T>(), f.p(), c.p(), N, st);
+                }, st);
+        };
+        transfers(0.0f);
+        transfers(0.0);
+    }
     // the fine stencil with fused dots (the second pass of D^dag D in the smoother and CG)
     const Dirac& D = *mg.D;
     const size_t nf = 2 * (size_t)D.V * N;

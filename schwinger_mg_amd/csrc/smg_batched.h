@@ -34,6 +34,10 @@ void launch_b_dslash(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T 
 template <typename T, typename TL>
 void launch_b_coarse_apply(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b,
                            const cplx<T>* aux, const FinB& fin, int N, hipStream_t st);
+// LDS bytes of the staged (workgroup-per-aggregate) transfer for this aggregation, N and
+// bytes per complex, or 0 where it does not apply; the tuner's TUNE_TRANSFER choice 1
+// selects it
+size_t b_transfer_lds_bytes(const AggView& A, int N, size_t cbytes, bool prolong);
 template <typename T>
 void launch_b_prolong(const AggView& A, const cplx<T>* bv, const cplx<T>* coarse, cplx<T>* fine, int accumulate,
                       int N, hipStream_t st);

@@ -24,7 +24,11 @@ enum TuneFamily : int {
     // k_h_cg_r (the BF16-vector CG's residual update with |r|^2); shape (n, N, 0, 0, 0);
     // choice = grid index
     TUNE_VEC_RED = 3,
-    TUNE_FAMILIES = 4,
+    // the batched transfers; shape (nx, ny, 128 nd + k, N, 2 (FP64) + (1 prolong, 0 restrict))
+    // of the finer lattice; choice 1: a workgroup per aggregate with its blocks staged in
+    // LDS, 0: one work-item per output reading through the caches (the same sums)
+    TUNE_TRANSFER = 4,
+    TUNE_FAMILIES = 5,
 };
 using TuneKey = std::array<int, 6>;   // family, then five shape words
 
