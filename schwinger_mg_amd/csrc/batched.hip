@@ -1854,19 +1854,34 @@ __global__ __launch_bounds__(kBlock) void k_h_cg_r(cb16* __restrict__ r, const c
         return;
     }
     double acc[1][4] = {};
-    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < runs; t += (size_t)gridDim.x * kBlock) {
-        const size_t i = t * 4;
-        c32 a[4];
-        run_coef<float, 4>(s, (int)(i & (N - 1)), a);
-        HRun rv, av;
-        rv.load(r + i);
-        av.load(Ap + i);
+    // kU runs per trip, their loads issued together (a work-item takes its runs in the
+    // same order as a one-run loop, so the sums are the same)
+    constexpr int kU = 8;
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    for (size_t t0 = (size_t)blockIdx.x * kBlock + threadIdx.x; t0 < runs; t0 += kU * stride) {
+        HRun rv[kU], av[kU];
 #pragma unroll
-        for (int q = 0; q < 4; q++) rv.v[q] = rv.v[q] - a[q] * av.v[q];
-        const uint4 w = rv.store(r + i);
-        const unsigned ws[4] = {w.x, w.y, w.z, w.w};
+        for (int u = 0; u < kU; u++) {
+            const size_t t = t0 + u * stride;
+            if (t < runs) {
+                rv[u].load(r + t * 4);
+                av[u].load(Ap + t * 4);
+            }
+        }
 #pragma unroll
-        for (int q = 0; q < 4; q++) acc[0][q] += (double)norm2(unpack_h(ws[q]));
+        for (int u = 0; u < kU; u++) {
+            const size_t t = t0 + u * stride;
+            if (t >= runs) break;
+            const size_t i = t * 4;
+            c32 a[4];
+            run_coef<float, 4>(s, (int)(i & (N - 1)), a);
+#pragma unroll
+            for (int q = 0; q < 4; q++) rv[u].v[q] = rv[u].v[q] - a[q] * av[u].v[q];
+            const uint4 w = rv[u].store(r + i);
+            const unsigned ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[0][q] += (double)norm2(unpack_h(ws[q]));
+        }
     }
     double v[1];
     runs_to_lanes<1, 4>(acc, v, N);

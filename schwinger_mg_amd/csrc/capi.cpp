@@ -1663,10 +1663,14 @@ int smg_heatbath(int L, double* U, double beta, int n_sweeps, unsigned long long
 int smg_lattices_create(smg_lattices** out, int G, const int* L, const double* U, const double* mass, double r,
                         double csw, double mu_t) {
     return guard([&] {
-        SMG_REQUIRE(out && L && U && mass && G >= 1 && G <= 4096, "bad arguments (1 <= G <= 4096)");
+        // one completion counter per lattice (of the context's 1024) in every reducing launch
+        SMG_REQUIRE(out && L && U && mass && G >= 1 && G <= 1023, "bad arguments (1 <= G <= 1023)");
         auto* h = new smg_lattices_s();
         try {
             size_t off = 0;
+            std::vector<long long> sites(G);
+            for (int g = 0; g < G; g++) sites[g] = (long long)L[g] * L[g];
+            const std::vector<int> nwg = lattice_workgroups(sites);
             for (int g = 0; g < G; g++) {
                 SMG_REQUIRE(L[g] >= 2, "every L must be >= 2");
                 h->D.emplace_back(new Dirac(L[g], reinterpret_cast<const c64*>(U) + off, mass[g], r, csw, mu_t));
@@ -1680,10 +1684,11 @@ int smg_lattices_create(smg_lattices** out, int G, const int* L, const double* U
                 e.mu = mu_t;
                 e.L = L[g];
                 e.wg0 = h->nwg;
+                e.nwg = nwg[g];
                 e.site0 = (long long)h->sites;
                 h->h.push_back(e);
                 h->sites += (size_t)L[g] * L[g];
-                h->nwg += (L[g] * L[g] + kBlock - 1) / kBlock;
+                h->nwg += nwg[g];
             }
             SMG_REQUIRE((size_t)h->nwg * 3 <= (size_t(1) << 21), "too many sites for one launch's partial sums");
             h->d.alloc(h->h.size() * sizeof(LatDesc));

@@ -2,12 +2,12 @@
 //
 // G lattices (sizes L_g, V_g = L_g^2 sites) are concatenated: their links, clover terms
 // and vectors lie one after the other (vector of lattice g at complex offset
-// 2 * site0_g, in the reference's psi[2s + spin] layout).  A launch runs one work-item
-// per site of every lattice; workgroups are assigned to lattices by a table (lattice g
-// owns workgroups wg0_g .. wg0_{g+1} - 1), so no workgroup straddles two lattices and a
+// 2 * site0_g, in the reference's psi[2s + spin] layout).  Workgroups are assigned to
+// lattices by a table (lattice g owns workgroups wg0_g .. wg0_g + nwg_g - 1, whose
+// work-items stride over its sites), so no workgroup straddles two lattices and a
 // reduction finishes per lattice: each workgroup stores its partial row, and the last
-// workgroup to arrive sums every lattice's rows in index order and runs that lattice's
-// scalar program (deterministic, as the single-lattice reductions).  The CG on D^dag D
+// workgroup of each lattice sums that lattice's rows in index order and runs its scalar
+// program (deterministic, as the single-lattice reductions).  The CG on D^dag D
 // then advances every lattice in lockstep with its own scalars and convergence flag,
 // one launch per kernel for all of them.  FP64; every kernel is HBM-bound.
 #include "smg_lattices.h"
@@ -24,11 +24,10 @@ __global__ __launch_bounds__(kBlock) void k_ml_dslash(const LatDesc* __restrict_
     const int g = my_lattice(d, G);
     const LatDesc D = d[g];
     const int L = D.L, V = L * L;
-    const int s = ((int)blockIdx.x - D.wg0) * kBlock + threadIdx.x;
     const size_t o = 2 * (size_t)D.site0;
     double acc[3] = {0.0, 0.0, 0.0};
     if (fin.skip && *fin.skip) return;   // grid-uniform: every lattice has converged
-    if (s < V) {
+    for (int s = ((int)blockIdx.x - D.wg0) * kBlock + threadIdx.x; s < V; s += D.nwg * kBlock) {
         const c64* ps = src + o;
         const int x = s % L, y = s / L;
         const int sxp = (x + 1 == L) ? s - x : s + 1;
@@ -63,25 +62,26 @@ __global__ __launch_bounds__(kBlock) void k_ml_dslash(const LatDesc* __restrict_
         if (EPI == EPI_DOTS) {
             const c64 a0 = aux[o + 2 * s], a1 = aux[o + 2 * s + 1];
             const c64 d0 = cdot(o0, a0), d1 = cdot(o1, a1);
-            acc[0] = d0.re + d1.re;
-            acc[1] = d0.im + d1.im;
-            acc[2] = norm2(o0) + norm2(o1);
+            acc[0] += d0.re + d1.re;
+            acc[1] += d0.im + d1.im;
+            acc[2] += norm2(o0) + norm2(o1);
         }
     }
-    if (EPI == EPI_DOTS) finish_seg<3>(acc, fin, d, G);
+    if (EPI == EPI_DOTS) finish_seg<3>(acc, fin, d, G, g);
 }
 
 // CG passes (src/solvers.cpp:16-72 per lattice): init r = p = b, x = 0 with |r|^2;
-// r -= a_g Ap with |r|^2; x += a_g p, p = r + b_g p; |v|^2 per lattice
+// r -= a_g Ap with |r|^2; x += a_g p, p = r + b_g p; |v|^2 per lattice.  The work-items
+// of lattice g stride over its sites.
 __global__ __launch_bounds__(kBlock) void k_ml_cg_init(const LatDesc* __restrict__ d, int G,
                                                        const c64* __restrict__ b, c64* __restrict__ x,
                                                        c64* __restrict__ r, c64* __restrict__ p, LatFin fin) {
     const int g = my_lattice(d, G);
-    const int V = d[g].L * d[g].L;
-    const int s = ((int)blockIdx.x - d[g].wg0) * kBlock + threadIdx.x;
+    const LatDesc D = d[g];
+    const int V = D.L * D.L;
     double acc[1] = {0.0};
-    if (s < V) {
-        const size_t i = 2 * ((size_t)d[g].site0 + s);
+    for (int s = ((int)blockIdx.x - D.wg0) * kBlock + threadIdx.x; s < V; s += D.nwg * kBlock) {
+        const size_t i = 2 * ((size_t)D.site0 + s);
         const c64 b0 = b[i], b1 = b[i + 1];
         r[i] = b0;
         r[i + 1] = b1;
@@ -89,9 +89,9 @@ __global__ __launch_bounds__(kBlock) void k_ml_cg_init(const LatDesc* __restrict
         p[i + 1] = b1;
         x[i] = c64{0.0, 0.0};
         x[i + 1] = c64{0.0, 0.0};
-        acc[0] = norm2(b0) + norm2(b1);
+        acc[0] += norm2(b0) + norm2(b1);
     }
-    finish_seg<1>(acc, fin, d, G);
+    finish_seg<1>(acc, fin, d, G, g);
 }
 __global__ __launch_bounds__(kBlock) void k_ml_cg_r(const LatDesc* __restrict__ d, int G, c64* __restrict__ r,
                                                     const c64* __restrict__ Ap, const double* __restrict__ alpha,
@@ -101,18 +101,18 @@ __global__ __launch_bounds__(kBlock) void k_ml_cg_r(const LatDesc* __restrict__ 
         return;
     }
     const int g = my_lattice(d, G);
-    const int V = d[g].L * d[g].L;
-    const int s = ((int)blockIdx.x - d[g].wg0) * kBlock + threadIdx.x;
+    const LatDesc D = d[g];
+    const int V = D.L * D.L;
+    const c64 a{alpha[2 * g], alpha[2 * g + 1]};
     double acc[1] = {0.0};
-    if (s < V) {
-        const size_t i = 2 * ((size_t)d[g].site0 + s);
-        const c64 a{alpha[2 * g], alpha[2 * g + 1]};
+    for (int s = ((int)blockIdx.x - D.wg0) * kBlock + threadIdx.x; s < V; s += D.nwg * kBlock) {
+        const size_t i = 2 * ((size_t)D.site0 + s);
         const c64 r0 = r[i] - a * Ap[i], r1 = r[i + 1] - a * Ap[i + 1];
         r[i] = r0;
         r[i + 1] = r1;
-        acc[0] = norm2(r0) + norm2(r1);
+        acc[0] += norm2(r0) + norm2(r1);
     }
-    finish_seg<1>(acc, fin, d, G);
+    finish_seg<1>(acc, fin, d, G, g);
 }
 __global__ __launch_bounds__(kBlock) void k_ml_cg_xp(const LatDesc* __restrict__ d, int G, c64* __restrict__ x,
                                                      c64* __restrict__ p, const c64* __restrict__ r,
@@ -120,29 +120,30 @@ __global__ __launch_bounds__(kBlock) void k_ml_cg_xp(const LatDesc* __restrict__
                                                      const double* __restrict__ beta, const int* __restrict__ late) {
     if (late && *late) return;
     const int g = my_lattice(d, G);
-    const int V = d[g].L * d[g].L;
-    const int s = ((int)blockIdx.x - d[g].wg0) * kBlock + threadIdx.x;
-    if (s >= V) return;
-    const size_t i = 2 * ((size_t)d[g].site0 + s);
+    const LatDesc D = d[g];
+    const int V = D.L * D.L;
     const c64 a{alpha[2 * g], alpha[2 * g + 1]}, bt{beta[2 * g], beta[2 * g + 1]};
+    for (int s = ((int)blockIdx.x - D.wg0) * kBlock + threadIdx.x; s < V; s += D.nwg * kBlock) {
+        const size_t i = 2 * ((size_t)D.site0 + s);
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
-        const c64 pv = p[i + q];
-        x[i + q] = x[i + q] + a * pv;
-        p[i + q] = r[i + q] + bt * pv;
+        for (int q = 0; q < 2; q++) {
+            const c64 pv = p[i + q];
+            x[i + q] = x[i + q] + a * pv;
+            p[i + q] = r[i + q] + bt * pv;
+        }
     }
 }
 __global__ __launch_bounds__(kBlock) void k_ml_norm2(const LatDesc* __restrict__ d, int G, const c64* __restrict__ v,
                                                      LatFin fin) {
     const int g = my_lattice(d, G);
-    const int V = d[g].L * d[g].L;
-    const int s = ((int)blockIdx.x - d[g].wg0) * kBlock + threadIdx.x;
+    const LatDesc D = d[g];
+    const int V = D.L * D.L;
     double acc[1] = {0.0};
-    if (s < V) {
-        const size_t i = 2 * ((size_t)d[g].site0 + s);
-        acc[0] = norm2(v[i]) + norm2(v[i + 1]);
+    for (int s = ((int)blockIdx.x - D.wg0) * kBlock + threadIdx.x; s < V; s += D.nwg * kBlock) {
+        const size_t i = 2 * ((size_t)D.site0 + s);
+        acc[0] += norm2(v[i]) + norm2(v[i + 1]);
     }
-    finish_seg<1>(acc, fin, d, G);
+    finish_seg<1>(acc, fin, d, G, g);
 }
 // per-lattice CG state: aux_g[5] = tol^2, flag_g = 0; *done = *late = 0
 __global__ void k_ml_cg_state(double* aux, int G, double tol2, int* done, int* late) {

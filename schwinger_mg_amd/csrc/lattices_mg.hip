@@ -80,27 +80,29 @@ __global__ __launch_bounds__(kBlock) void k_seg_prolong(const SegDesc* __restric
 }
 
 // dst[b, ki] = sum_d sum_kj L[b][d][ki][kj] src[nbr(b, d), kj] (src/coarse_op.h:282-297)
-// with one lane per link entry (b, ki, kj): each k x k block is one coalesced read and
-// the row sums over kj are shuffles in groups of K lanes.  The workgroups of a lattice
-// stride over its nx*nx*K*K entries in steps of a multiple of kBlock (so of K): every
-// row stays inside one wave and one step, and the loop bound is uniform per workgroup.
-// EPI_RESID: dst = bvec - A src; EPI_DOTS: {<dst, aux>, |dst|^2} per lattice into fin.
-// TL: the links' storage (c64, or ch16: FP16 links widened on load, FP64 arithmetic).
+// with a run of kSegRun (= 4, or K if smaller) consecutive link entries (b, ki, kj..) per
+// lane: one 16-byte load per direction for FP16 links, K / kSegRun lanes per row, whose
+// sums over kj end in a shuffle tree inside the row's lanes.  The workgroups of a lattice
+// stride over its nx*nx*K*K/kSegRun lanes in steps of a multiple of kBlock (so of the
+// row's lanes): every row stays inside one wave and one step, and the loop bound is
+// uniform per workgroup.  EPI_RESID: dst = bvec - A src; EPI_DOTS: {<dst, aux>, |dst|^2}
+// per lattice into fin.  TL: the links' storage (c64, or ch16: FP16 links widened on
+// load, FP64 arithmetic).
 template <int EPI, int K, typename TL>
 __global__ __launch_bounds__(kBlock) void k_seg_coarse(const SegDesc* __restrict__ ld, int G,
                                                        const TL* __restrict__ links, const c64* __restrict__ src,
                                                        c64* __restrict__ dst, const c64* __restrict__ bvec,
                                                        const c64* __restrict__ aux, LatFin fin) {
-    static_assert(kBlock % K == 0 && K <= 64, "a row of K lanes must sit inside one wave");
+    constexpr int V = seg_coarse_run(K), RL = K / V, KK = K * K, RK = K * RL;
+    static_assert(kBlock % RL == 0 && RL <= 64, "a row's lanes must sit inside one wave");
     if (EPI == EPI_DOTS && fin.skip && *fin.skip) return;   // grid-uniform
-    constexpr int KK = K * K;
     const int g = my_lattice(ld, G);
     const SegDesc C = ld[g];
     const int n = C.nx, nb = n * n;
     double acc[3] = {0.0, 0.0, 0.0};
     for (long long t0 = (long long)((int)blockIdx.x - C.wg0) * kBlock; t0 < C.n; t0 += (long long)C.nwg * kBlock) {
         const long long t = t0 + threadIdx.x;
-        const int b = (int)(t / KK), e = (int)(t % KK), kj = e % K;
+        const int b = (int)(t / RK), e = (int)(t % RK), ki = e / RL, kj0 = (e % RL) * V;
         c64 sum{0.0, 0.0};
         if (b < nb) {
             const int bx = b % n, by = b / n;
@@ -108,18 +110,32 @@ __global__ __launch_bounds__(kBlock) void k_seg_coarse(const SegDesc* __restrict
             const int ym = by == 0 ? n - 1 : by - 1, yp = by + 1 == n ? 0 : by + 1;
             const int nbr[9] = {b, xp + by * n, xm + by * n, bx + yp * n, bx + ym * n,
                                 xp + yp * n, xp + ym * n, xm + yp * n, xm + ym * n};
-            const TL* lk = links + C.link_off + (size_t)b * 9 * KK + e;
-            const c64* s = src + C.off + kj;
+            const TL* lk = links + C.link_off + (size_t)b * 9 * KK + ki * K + kj0;
+            const c64* s = src + C.off + kj0;
 #pragma unroll
-            for (int d = 0; d < 9; d++) sum = sum + cvt<double>(load_c(lk + d * KK)) * s[(size_t)nbr[d] * K];
+            for (int d = 0; d < 9; d++) {
+                TL lv[V];
+                if constexpr (std::is_same<TL, ch16>::value && V == 4) {
+                    const uint4 w = *reinterpret_cast<const uint4*>(lk + d * KK);
+                    const unsigned ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                    for (int v = 0; v < 4; v++) lv[v] = __builtin_bit_cast(ch16, ws[v]);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < V; v++) lv[v] = lk[d * KK + v];
+                }
+                const c64* sv = s + (size_t)nbr[d] * K;
+#pragma unroll
+                for (int v = 0; v < V; v++) sum = sum + cvt<double>(load_c(&lv[v])) * sv[v];
+            }
         }
 #pragma unroll
-        for (int off = K / 2; off > 0; off >>= 1) {
+        for (int off = RL / 2; off > 0; off >>= 1) {
             sum.re += __shfl_xor(sum.re, off, 64);
             sum.im += __shfl_xor(sum.im, off, 64);
         }
-        if (b < nb && kj == 0) {
-            const size_t o = C.off + (size_t)b * K + e / K;
+        if (b < nb && kj0 == 0) {
+            const size_t o = C.off + (size_t)b * K + ki;
             c64 out = sum;
             if (EPI == EPI_RESID) out = bvec[o] - out;
             dst[o] = out;
@@ -131,7 +147,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_coarse(const SegDesc* __restrict
             }
         }
     }
-    if (EPI == EPI_DOTS) finish_seg<3>(acc, fin, ld, G);
+    if (EPI == EPI_DOTS) finish_seg<3>(acc, fin, ld, G, g);
 }
 
 // vector passes with per-lattice scalars (complex a_g at s[2g], s[2g + 1])
@@ -164,7 +180,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_cg_init(const SegDesc* __restric
         x[i] = c64{0.0, 0.0};
         acc[0] += norm2(bv);
     }
-    finish_seg<1>(acc, fin, d, G);
+    finish_seg<1>(acc, fin, d, G, R.g);
 }
 // r -= alpha Ap, {|r|^2}; once every lattice has stopped, marks *late and does nothing
 __global__ __launch_bounds__(kBlock) void k_seg_cg_r(const SegDesc* __restrict__ d, int G, c64* __restrict__ r,
@@ -184,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_cg_r(const SegDesc* __restrict__
         r[i] = rv;
         acc[0] += norm2(rv);
     }
-    finish_seg<1>(acc, fin, d, G);
+    finish_seg<1>(acc, fin, d, G, R.g);
 }
 // x += alpha p; p = r + beta p on lattices whose CG goes on (stop flag of cg_aux clear;
 // a stopped lattice's alpha is zero, so its x stays as it is)
@@ -224,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_multi_dot(const SegDesc* __restr
             acc[2 * v + 1] += p.im;
         }
     }
-    finish_seg<2 * NV>(acc, fin, d, G);
+    finish_seg<2 * NV>(acc, fin, d, G, R.g);
 }
 // z -= sum_i beta_i P_i, Az -= sum_i beta_i AP_i with beta_i = <AP_i, Az> / |AP_i|^2 on
 // lattice g (classical Gram-Schmidt against the kept directions, src/solvers.cpp:260-266;
@@ -265,7 +281,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_dots(const SegDesc* __restrict__
         acc[1] += p.im;
         acc[2] += norm2(av);
     }
-    finish_seg<3>(acc, fin, d, G);
+    finish_seg<3>(acc, fin, d, G, R.g);
 }
 // x += a_g z, r -= a_g Az on lattices with active[g] != 0; {|r|^2} per lattice
 __global__ __launch_bounds__(kBlock) void k_seg_update(const SegDesc* __restrict__ d, int G, c64* __restrict__ x,
@@ -287,7 +303,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_update(const SegDesc* __restrict
         }
         acc[0] += norm2(rv);
     }
-    finish_seg<1>(acc, fin, d, G);
+    finish_seg<1>(acc, fin, d, G, R.g);
 }
 
 // ------------------------------------------------------------------ launchers

@@ -34,18 +34,17 @@ struct ScLayout {
     }
 };
 
-int nwg_for(long long items) {
-    const long long w = (items + kBlock - 1) / kBlock;
-    return (int)std::max(1LL, std::min<long long>(w, kSegCap));
-}
-
-// a device table and its workgroup count; fills wg0 and nwg of every entry
+// a device table and its workgroup count; fills wg0 and nwg of every entry (the
+// lattices share a budget of workgroups, each at most kSegCap)
 int upload(std::vector<SegDesc>& h, DevMem& d, hipStream_t st) {
+    std::vector<long long> n;
+    for (const SegDesc& s : h) n.push_back(s.n);
+    const std::vector<int> w = lattice_workgroups(n);
     int wg = 0;
-    for (SegDesc& s : h) {
-        s.wg0 = wg;
-        s.nwg = nwg_for(s.n);
-        wg += s.nwg;
+    for (size_t g = 0; g < h.size(); g++) {
+        h[g].wg0 = wg;
+        h[g].nwg = std::min(w[g], kSegCap);
+        wg += h[g].nwg;
     }
     d.alloc(h.size() * sizeof(SegDesc));
     SMG_HIP(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(SegDesc), hipMemcpyHostToDevice, st));
@@ -149,7 +148,7 @@ LatticeMG::LatticeMG(const std::vector<const Dirac*>& dirac, const LatDesc* fine
         nwg_ent[l] = upload(hent[l], ent[l], st);
         if (l >= 1) {
             std::vector<SegDesc> hl = hent[l];
-            for (SegDesc& s : hl) s.n = (long long)s.nx * s.nx * s.nd * s.nd;
+            for (SegDesc& s : hl) s.n = (long long)s.nx * s.nx * s.nd * s.nd / seg_coarse_run(s.nd);
             nwg_lanes[l] = upload(hl, lanes[l], st);
         }
     }

@@ -14,7 +14,8 @@ struct LatticeMG {
     const LatDesc* fine = nullptr;   // the lattices' table (device), one work-item per site
     int nwg_sites = 0;
     // per level: the table of one work-item per vector entry, and (levels >= 1) of one
-    // per link entry, with their launch sizes; the level's concatenated length
+    // per run of a row's link entries (the coarse operator), with their launch sizes; the
+    // level's concatenated length
     std::vector<std::vector<SegDesc>> hent;
     std::vector<DevMem> ent, lanes;
     std::vector<int> nwg_ent, nwg_lanes;
