@@ -603,10 +603,11 @@ int smg_hmc_nested_trajectory(smg_dirac* D, smg_mg* mg, const smg_nested_params*
 
 /* ------------------------------------------------------------------ many lattices in one launch
  * No reference counterpart (the reference runs one lattice at a time): G Wilson-clover
- * lattices of different sizes (L[g] >= 2; links U of lattice g follow those of g-1, each
- * U[mu*V + s]; masses mass[g]; r, csw, mu_t shared) whose operators and CG solves run
- * as one launch per kernel for all of them.  Vectors are the lattices' vectors one after
- * the other (2 V_g complex each, psi[2s + spin]); smg_lattices_size gives the total. */
+ * lattices (1 <= G <= 1023) of different sizes (L[g] >= 2; links U of lattice g follow
+ * those of g-1, each U[mu*V + s]; masses mass[g]; r, csw, mu_t shared) whose operators
+ * and CG solves run as one launch per kernel for all of them.  Vectors are the lattices'
+ * vectors one after the other (2 V_g complex each, psi[2s + spin]); smg_lattices_size
+ * gives the total. */
 int smg_lattices_create(smg_lattices** out, int G, const int* L, const double* U, const double* mass, double r,
                         double csw, double mu_t);
 int smg_lattices_destroy(smg_lattices* H);
