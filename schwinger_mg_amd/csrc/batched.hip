@@ -696,15 +696,23 @@ __device__ __forceinline__ c32 unpack_h(unsigned w) {
     return c32{__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u)};
 }
 
-// One wave per coarse site; the 9K/16 chunks are pipelined, chunk m + 1's raw
-// sources and links in flight while chunk m is split and multiplied.  HV: source,
-// destination and the dot partner are BF16 vectors (cb16 entries), so the source
-// needs no split (two MFMAs per product instead of three); EPI_RESID is FP32 only.
-// HERM (C.herm set, K >= 16 so that a chunk lies in one direction): the blocks of the
-// backward directions are read as the conjugate transposes of the neighbours' forward
-// blocks, L[b][d] = L[nbr(b,d)][rev d]^dag, so a site reads 5 of its 9 link blocks and
-// the links' HBM traffic falls to 5/9; a lane then gathers its four link columns of a
-// row as four dwords of one column of the stored block instead of one 16-byte run.
+// One wave per coarse site; the 9K/16 chunks are pipelined, chunks m + 1 and m + 2
+// in flight while chunk m is split and multiplied.  The complex products use the
+// links in their stored order: A[i][k] of row tile rt holds link row ki = 16 rt + i
+// and columns k = (kj, c') as (Re L, Im L), so every link entry is split into bf16
+// hi + lo once, by one lane, and a lane's four columns of a row are one 16-byte load.
+// The complex sign goes into the source operand, in two forms per right-hand side:
+// (Re x, -Im x) gives Re(L x) and (Im x, Re x) gives Im(L x), so every tile of 16
+// right-hand sides takes two MFMA column tiles (part 0: real, part 1: imaginary).
+// A lane of the result holds rows 16 rt + 4 hi + v (v = 0..3) of one right-hand side,
+// real part in part 0, imaginary part in part 1.  K = 8 fills rows 0..7 of the tile.
+// HV: source, destination and the dot partner are BF16 vectors (cb16 entries), so the
+// source needs no split (two MFMAs per product instead of three); EPI_RESID is FP32
+// only.  HERM (C.herm set, K >= 16 so that a chunk lies in one direction): the blocks
+// of the backward directions are read as the conjugate transposes of the neighbours'
+// forward blocks, L[b][d] = L[nbr(b,d)][rev d]^dag, so a site reads 5 of its 9 link
+// blocks and the links' HBM traffic falls to 5/9; a lane then gathers its four link
+// columns of a row as four dwords of one column of the stored block.
 template <typename TL, int EPI, int K, int NT, bool HV = false, bool HERM = false>
 __global__ __launch_bounds__(kBlock)
 void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __restrict__ srcv,
@@ -712,6 +720,7 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
     static_assert(std::is_same<TL, ch16>::value, "bf16x3 path: FP16 links");
     static_assert(!HV || EPI != EPI_RESID, "BF16 vectors: no residual epilogue");
     static_assert(!HERM || K % 16 == 0, "Hermitian reads: a chunk of 16 columns inside one direction");
+    static_assert(K == 8 || K == 16 || K == 32, "k in {8, 16, 32}");
     c32* __restrict__ dst = static_cast<c32*>(dstv);
     const c32* __restrict__ src = static_cast<const c32*>(srcv);
     const c32* __restrict__ aux = static_cast<const c32*>(auxv);
@@ -719,36 +728,35 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
     const unsigned* __restrict__ srch = static_cast<const unsigned*>(srcv);
     const unsigned* __restrict__ auxh = static_cast<const unsigned*>(auxv);
     if (EPI == EPI_DOTS && fin.skip && *fin.skip) return;   // CG finished for every RHS
-    constexpr int N = 16 * NT, RT = K / 8, NCH = (9 * K + 15) / 16;
+    constexpr int N = 16 * NT, RT = (K + 15) / 16, NCH = (9 * K + 15) / 16;
     constexpr bool PAIR = NT % 2 == 0;
     constexpr int NL = PAIR ? NT / 2 : NT;
+    constexpr unsigned kSel = 0x07060302u;   // pack_bf: bf16(a) low half, bf16(b) high half
     const int lane = threadIdx.x & 63, lo = lane & 15, hi = lane >> 4;
     const int nx = C.nx, ny = C.ny, nsites = nx * ny;
     const int nw = kBlock / 64;
-    const int a_ki = lo >> 1, a_c = lo & 1;   // FP32 tile rows: i = 4g + v <-> (ki = 2g + v/2, c = v%2)
-    // real form of row part c for the column pair (c' = 0, 1): c = 0 -> (Re L, -Im L), c = 1 -> (Im L, Re L)
-    const unsigned neg_im = a_c ? 0u : 0x80000000u;
-    const unsigned sel_a = a_c ? 0x03020706u : 0x07060302u;
+    // K = 8: rows 8..15 of the tile do not exist (zero links, no result)
+    const bool row_ok = K >= 16 || lo < K;        // this lane's link row
+    const bool out_ok = K >= 16 || 4 * hi < K;    // this lane's result rows
     const unsigned* links = reinterpret_cast<const unsigned*>(C.link);
     double dacc[NT][3];
 #pragma unroll
     for (int t = 0; t < NT; t++) dacc[t][0] = dacc[t][1] = dacc[t][2] = 0.0;
     for (int b = blockIdx.x * nw + (threadIdx.x >> 6); b < nsites; b += gridDim.x * nw) {
         const int bx = b % nx, by = b / nx;
-        f32x4 acc[RT][NT];
+        f32x4 acc[RT][NT][2];
 #pragma unroll
         for (int r = 0; r < RT; r++)
 #pragma unroll
-            for (int t = 0; t < NT; t++) acc[r][t] = f32x4{0, 0, 0, 0};
+            for (int t = 0; t < NT; t++) acc[r][t][0] = acc[r][t][1] = f32x4{0, 0, 0, 0};
         // raw operands of one chunk: the sources of four link columns (both RHS
-        // tiles of a pair in one 16-byte load) and the row-tile link runs
+        // tiles of a pair in one 16-byte load) and the link rows of the row tiles
         using SrcF = typename std::conditional<PAIR, float4, float2>::type;
         using SrcH = typename std::conditional<PAIR, uint2, unsigned>::type;
         using SrcV = typename std::conditional<HV, SrcH, SrcF>::type;
         struct Raw {
             SrcV s[4][NL];
             uint4 l[RT];
-            unsigned cj;   // sign bit flipped into Im L (a conjugated, transposed block)
         };
         auto load_raw = [&](Raw& R, int m) {
             const int cc0 = 16 * m + 4 * hi;
@@ -769,17 +777,19 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                 }
             }
             const bool bwd = HERM && (d == 2 || d == 4 || d == 7 || d == 8);
-            R.cj = bwd ? 0x80000000u : 0u;
-            if (bwd) {   // L[b][d][ki][kj] = conj(L[nb][rev d][kj][ki])
-                const int rd = d == 2 ? 1 : d == 4 ? 3 : d == 7 ? 6 : 5;
-                const unsigned* lcol = links + ((size_t)nb * 9 + rd) * (K * K) + (size_t)kj0 * K + a_ki;
 #pragma unroll
-                for (int r = 0; r < RT; r++)
-                    R.l[r] = uint4{lcol[8 * r], lcol[K + 8 * r], lcol[2 * K + 8 * r], lcol[3 * K + 8 * r]};
-            } else {
-                const unsigned* lrow = links + ((size_t)b * 9 + d) * (K * K) + a_ki * K + kj0;
-#pragma unroll
-                for (int r = 0; r < RT; r++) R.l[r] = *reinterpret_cast<const uint4*>(lrow + 8 * r * K);
+            for (int r = 0; r < RT; r++) {
+                const int ki = 16 * r + lo;
+                if (!row_ok) {
+                    R.l[r] = uint4{0u, 0u, 0u, 0u};
+                } else if (bwd) {   // L[b][d][ki][kj] = conj(L[nb][rev d][kj][ki]): Im's sign bit flipped
+                    const int rd = d == 2 ? 1 : d == 4 ? 3 : d == 7 ? 6 : 5;
+                    const unsigned* lcol = links + ((size_t)nb * 9 + rd) * (K * K) + (size_t)kj0 * K + ki;
+                    R.l[r] = uint4{lcol[0] ^ 0x80000000u, lcol[K] ^ 0x80000000u, lcol[2 * K] ^ 0x80000000u,
+                                   lcol[3 * K] ^ 0x80000000u};
+                } else {
+                    R.l[r] = *reinterpret_cast<const uint4*>(links + ((size_t)b * 9 + d) * (K * K) + ki * K + kj0);
+                }
             }
             if (!valid) {   // the partial last chunk (k = 8): these columns do not exist
 #pragma unroll
@@ -791,87 +801,87 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
             }
         };
         auto mma_chunk = [&](const Raw& R) {
-            u32x4 bh[NT], bl[NT];
-#pragma unroll
-            for (int e2 = 0; e2 < 4; e2++) {
-                if constexpr (HV) {   // the stored dwords are the operand pairs
-#pragma unroll
-                    for (int u = 0; u < NL; u++) {
-                        if constexpr (PAIR) {
-                            bh[2 * u][e2] = R.s[e2][u].x;
-                            bh[2 * u + 1][e2] = R.s[e2][u].y;
-                        } else {
-                            bh[u][e2] = R.s[e2][u];
-                        }
-                    }
-                } else {
-                    c32 sv[NT];
-#pragma unroll
-                    for (int u = 0; u < NL; u++) {
-                        if constexpr (PAIR) {
-                            const float4 w = R.s[e2][u];
-                            sv[2 * u] = c32{w.x, w.y};
-                            sv[2 * u + 1] = c32{w.z, w.w};
-                        } else {
-                            const float2 w = R.s[e2][u];
-                            sv[u] = c32{w.x, w.y};
-                        }
-                    }
-#pragma unroll
-                    for (int t = 0; t < NT; t++) {
-                        bh[t][e2] = pack_bf(sv[t].re, sv[t].im, 0x07060302u);
-                        bl[t][e2] = pack_bf(lo_part(sv[t].re), lo_part(sv[t].im), 0x07060302u);
-                    }
-                }
-            }
+            // link operands of every row tile: (Re L, Im L) as bf16 hi and lo
+            u32x4 ah[RT], al[RT];
 #pragma unroll
             for (int r = 0; r < RT; r++) {
                 const unsigned lws[4] = {R.l[r].x, R.l[r].y, R.l[r].z, R.l[r].w};
-                u32x4 ah, al;
 #pragma unroll
                 for (int e2 = 0; e2 < 4; e2++) {
                     const float re = __half2float(__ushort_as_half((unsigned short)(lws[e2] & 0xffffu)));
-                    const float im = __uint_as_float(
-                        __float_as_uint(__half2float(__ushort_as_half((unsigned short)(lws[e2] >> 16)))) ^ neg_im ^
-                        R.cj);
-                    ah[e2] = pack_bf(re, im, sel_a);
-                    al[e2] = pack_bf(lo_part(re), lo_part(im), sel_a);
+                    const float im = __half2float(__ushort_as_half((unsigned short)(lws[e2] >> 16)));
+                    ah[r][e2] = pack_bf(re, im, kSel);
+                    al[r][e2] = pack_bf(lo_part(re), lo_part(im), kSel);
                 }
-                const bf16x8 ahv = __builtin_bit_cast(bf16x8, ah), alv = __builtin_bit_cast(bf16x8, al);
+            }
+            // one RHS tile at a time: its two source forms, then its products
 #pragma unroll
-                for (int t = 0; t < NT; t++) {
-                    const bf16x8 bhv = __builtin_bit_cast(bf16x8, bh[t]);
-                    acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahv, bhv, acc[r][t], 0, 0, 0);
-                    if constexpr (!HV) {
-                        const bf16x8 blv = __builtin_bit_cast(bf16x8, bl[t]);
-                        acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahv, blv, acc[r][t], 0, 0, 0);
+            for (int t = 0; t < NT; t++) {
+                const int u = PAIR ? t / 2 : t;
+                u32x4 bh[2], bl[2];
+#pragma unroll
+                for (int e2 = 0; e2 < 4; e2++) {
+                    if constexpr (HV) {   // the stored dword is (Re, Im) in bf16
+                        unsigned w;
+                        if constexpr (PAIR) w = (t & 1) ? R.s[e2][u].y : R.s[e2][u].x;
+                        else w = R.s[e2][u];
+                        bh[0][e2] = w ^ 0x80000000u;                      // (Re x, -Im x)
+                        bh[1][e2] = __builtin_amdgcn_alignbit(w, w, 16);  // (Im x, Re x)
+                    } else {
+                        c32 sv;
+                        if constexpr (PAIR) {
+                            const float4 w = R.s[e2][u];
+                            sv = (t & 1) ? c32{w.z, w.w} : c32{w.x, w.y};
+                        } else {
+                            const float2 w = R.s[e2][u];
+                            sv = c32{w.x, w.y};
+                        }
+                        const float lr = lo_part(sv.re), li = lo_part(sv.im);
+                        bh[0][e2] = pack_bf(sv.re, sv.im, kSel) ^ 0x80000000u;
+                        bh[1][e2] = pack_bf(sv.im, sv.re, kSel);
+                        bl[0][e2] = pack_bf(lr, li, kSel) ^ 0x80000000u;
+                        bl[1][e2] = pack_bf(li, lr, kSel);
                     }
-                    acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alv, bhv, acc[r][t], 0, 0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < RT; r++) {
+                    const bf16x8 ahv = __builtin_bit_cast(bf16x8, ah[r]), alv = __builtin_bit_cast(bf16x8, al[r]);
+#pragma unroll
+                    for (int p = 0; p < 2; p++) {
+                        const bf16x8 bhv = __builtin_bit_cast(bf16x8, bh[p]);
+                        acc[r][t][p] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahv, bhv, acc[r][t][p], 0, 0, 0);
+                        if constexpr (!HV) {
+                            const bf16x8 blv = __builtin_bit_cast(bf16x8, bl[p]);
+                            acc[r][t][p] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahv, blv, acc[r][t][p], 0, 0, 0);
+                        }
+                        acc[r][t][p] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alv, bhv, acc[r][t][p], 0, 0, 0);
+                    }
                 }
             }
         };
         // the epilogue's operand (b for EPI_RESID, aux for EPI_DOTS) at this lane's
-        // outputs, loaded while the last chunks are multiplied (PAIR layout)
+        // results, loaded while the last chunks are multiplied (PAIR layout)
         using EpoV = typename std::conditional<HV, uint2, float4>::type;
-        EpoV epo[PAIR ? RT : 1][2][PAIR ? NL : 1];
+        EpoV epo[PAIR ? RT : 1][PAIR ? 4 : 1][PAIR ? NL : 1];
         auto load_epo = [&]() {
             if constexpr (PAIR && EPI != EPI_NONE) {
+                if (out_ok) {
 #pragma unroll
-                for (int r = 0; r < RT; r++)
+                    for (int r = 0; r < RT; r++)
 #pragma unroll
-                    for (int h = 0; h < 2; h++)
+                        for (int v = 0; v < 4; v++)
 #pragma unroll
-                        for (int u = 0; u < NL; u++) {
-                            const size_t o = ((size_t)b * K + 8 * r + 2 * hi + h) * N + 32 * u + 2 * lo;
-                            if constexpr (HV) epo[r][h][u] = *reinterpret_cast<const uint2*>(auxh + o);
-                            else epo[r][h][u] = *reinterpret_cast<const float4*>((EPI == EPI_RESID ? bvec : aux) + o);
-                        }
+                            for (int u = 0; u < NL; u++) {
+                                const size_t o = ((size_t)b * K + 16 * r + 4 * hi + v) * N + 32 * u + 2 * lo;
+                                if constexpr (HV) epo[r][v][u] = *reinterpret_cast<const uint2*>(auxh + o);
+                                else epo[r][v][u] = *reinterpret_cast<const float4*>((EPI == EPI_RESID ? bvec : aux) + o);
+                            }
+                }
             }
         };
         if constexpr (NT <= 2) {
             // chunks m + 1 and m + 2 in flight while chunk m is multiplied (fully
-            // unrolled, so the ring indices are compile-time): 12% faster than one
-            // chunk ahead at k = 32, N = 32
+            // unrolled, so the ring indices are compile-time)
             Raw raw[3];
             load_raw(raw[0], 0);
             if (NCH > 1) load_raw(raw[1], 1);
@@ -894,75 +904,77 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
             }
             load_epo();
         }
+        if (out_ok) {
 #pragma unroll
-        for (int r = 0; r < RT; r++)
+            for (int r = 0; r < RT; r++)
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int ki = 8 * r + 2 * hi + h;
-                const size_t row = ((size_t)b * K + ki) * N;
+                for (int v = 0; v < 4; v++) {
+                    const int ki = 16 * r + 4 * hi + v;
+                    const size_t row = ((size_t)b * K + ki) * N;
 #pragma unroll
-                for (int u = 0; u < NL; u++) {
-                    if constexpr (PAIR && HV) {
-                        const size_t idx = row + 32 * u + 2 * lo;
-                        const unsigned w0 = pack_h(c32{acc[r][2 * u][2 * h], acc[r][2 * u][2 * h + 1]});
-                        const unsigned w1 = pack_h(c32{acc[r][2 * u + 1][2 * h], acc[r][2 * u + 1][2 * h + 1]});
-                        *reinterpret_cast<uint2*>(dsth + idx) = uint2{w0, w1};
-                        if (EPI == EPI_DOTS) {   // dots of the stored (rounded) values
-                            const c32 o0 = unpack_h(w0), o1 = unpack_h(w1);
-                            const uint2 w = epo[r][h][u];
-                            const c32 p0 = cdot(o0, unpack_h(w.x)), p1 = cdot(o1, unpack_h(w.y));
-                            dacc[2 * u][0] += p0.re;
-                            dacc[2 * u][1] += p0.im;
-                            dacc[2 * u][2] += norm2(o0);
-                            dacc[2 * u + 1][0] += p1.re;
-                            dacc[2 * u + 1][1] += p1.im;
-                            dacc[2 * u + 1][2] += norm2(o1);
-                        }
-                    } else if constexpr (HV) {
-                        const size_t idx = row + 16 * u + lo;
-                        const unsigned w = pack_h(c32{acc[r][u][2 * h], acc[r][u][2 * h + 1]});
-                        dsth[idx] = w;
-                        if (EPI == EPI_DOTS) {
-                            const c32 o = unpack_h(w);
-                            const c32 p = cdot(o, unpack_h(auxh[idx]));
-                            dacc[u][0] += p.re;
-                            dacc[u][1] += p.im;
-                            dacc[u][2] += norm2(o);
-                        }
-                    } else if constexpr (PAIR) {
-                        const size_t idx = row + 32 * u + 2 * lo;
-                        c32 o0{acc[r][2 * u][2 * h], acc[r][2 * u][2 * h + 1]};
-                        c32 o1{acc[r][2 * u + 1][2 * h], acc[r][2 * u + 1][2 * h + 1]};
-                        if (EPI == EPI_RESID) {
-                            const float4 w = epo[r][h][u];
-                            o0 = c32{w.x, w.y} - o0;
-                            o1 = c32{w.z, w.w} - o1;
-                        }
-                        *reinterpret_cast<float4*>(dst + idx) = float4{o0.re, o0.im, o1.re, o1.im};
-                        if (EPI == EPI_DOTS) {
-                            const float4 w = epo[r][h][u];
-                            const c32 p0 = cdot(o0, c32{w.x, w.y}), p1 = cdot(o1, c32{w.z, w.w});
-                            dacc[2 * u][0] += p0.re;
-                            dacc[2 * u][1] += p0.im;
-                            dacc[2 * u][2] += norm2(o0);
-                            dacc[2 * u + 1][0] += p1.re;
-                            dacc[2 * u + 1][1] += p1.im;
-                            dacc[2 * u + 1][2] += norm2(o1);
-                        }
-                    } else {
-                        const size_t idx = row + 16 * u + lo;
-                        c32 o{acc[r][u][2 * h], acc[r][u][2 * h + 1]};
-                        if (EPI == EPI_RESID) o = bvec[idx] - o;
-                        dst[idx] = o;
-                        if (EPI == EPI_DOTS) {
-                            const c32 p = cdot(o, aux[idx]);
-                            dacc[u][0] += p.re;
-                            dacc[u][1] += p.im;
-                            dacc[u][2] += norm2(o);
+                    for (int u = 0; u < NL; u++) {
+                        if constexpr (PAIR && HV) {
+                            const size_t idx = row + 32 * u + 2 * lo;
+                            const unsigned w0 = pack_h(c32{acc[r][2 * u][0][v], acc[r][2 * u][1][v]});
+                            const unsigned w1 = pack_h(c32{acc[r][2 * u + 1][0][v], acc[r][2 * u + 1][1][v]});
+                            *reinterpret_cast<uint2*>(dsth + idx) = uint2{w0, w1};
+                            if (EPI == EPI_DOTS) {   // dots of the stored (rounded) values
+                                const c32 o0 = unpack_h(w0), o1 = unpack_h(w1);
+                                const uint2 w = epo[r][v][u];
+                                const c32 p0 = cdot(o0, unpack_h(w.x)), p1 = cdot(o1, unpack_h(w.y));
+                                dacc[2 * u][0] += p0.re;
+                                dacc[2 * u][1] += p0.im;
+                                dacc[2 * u][2] += norm2(o0);
+                                dacc[2 * u + 1][0] += p1.re;
+                                dacc[2 * u + 1][1] += p1.im;
+                                dacc[2 * u + 1][2] += norm2(o1);
+                            }
+                        } else if constexpr (HV) {
+                            const size_t idx = row + 16 * u + lo;
+                            const unsigned w = pack_h(c32{acc[r][u][0][v], acc[r][u][1][v]});
+                            dsth[idx] = w;
+                            if (EPI == EPI_DOTS) {
+                                const c32 o = unpack_h(w);
+                                const c32 p = cdot(o, unpack_h(auxh[idx]));
+                                dacc[u][0] += p.re;
+                                dacc[u][1] += p.im;
+                                dacc[u][2] += norm2(o);
+                            }
+                        } else if constexpr (PAIR) {
+                            const size_t idx = row + 32 * u + 2 * lo;
+                            c32 o0{acc[r][2 * u][0][v], acc[r][2 * u][1][v]};
+                            c32 o1{acc[r][2 * u + 1][0][v], acc[r][2 * u + 1][1][v]};
+                            if (EPI == EPI_RESID) {
+                                const float4 w = epo[r][v][u];
+                                o0 = c32{w.x, w.y} - o0;
+                                o1 = c32{w.z, w.w} - o1;
+                            }
+                            *reinterpret_cast<float4*>(dst + idx) = float4{o0.re, o0.im, o1.re, o1.im};
+                            if (EPI == EPI_DOTS) {
+                                const float4 w = epo[r][v][u];
+                                const c32 p0 = cdot(o0, c32{w.x, w.y}), p1 = cdot(o1, c32{w.z, w.w});
+                                dacc[2 * u][0] += p0.re;
+                                dacc[2 * u][1] += p0.im;
+                                dacc[2 * u][2] += norm2(o0);
+                                dacc[2 * u + 1][0] += p1.re;
+                                dacc[2 * u + 1][1] += p1.im;
+                                dacc[2 * u + 1][2] += norm2(o1);
+                            }
+                        } else {
+                            const size_t idx = row + 16 * u + lo;
+                            c32 o{acc[r][u][0][v], acc[r][u][1][v]};
+                            if (EPI == EPI_RESID) o = bvec[idx] - o;
+                            dst[idx] = o;
+                            if (EPI == EPI_DOTS) {
+                                const c32 p = cdot(o, aux[idx]);
+                                dacc[u][0] += p.re;
+                                dacc[u][1] += p.im;
+                                dacc[u][2] += norm2(o);
+                            }
                         }
                     }
                 }
-            }
+        }
     }
     if (EPI == EPI_DOTS) {
 #pragma unroll
