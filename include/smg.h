@@ -685,22 +685,23 @@ int smg_therm_cfg_path(int L, double beta, int n_therm, char* out, size_t cap);
  * give bit-identical results; shapes without an entry use the built-in defaults.
  * Families (shape words a..e, choice):
  *   SMG_TUNE_COARSE_BF16: the FP16-link matrix-core coarse apply; (nx, ny, k, nrhs,
- *     4*bf16_vectors + epilogue[0 none, 1 residual, 2 dots]); choice = herm + 2*g, herm = 1
+ *     4*bf16_vectors + epilogue[0 none, 1 residual, 2 dots, 3 the CG's two-partner dots]);
+ *     choice = herm + 2*g, herm = 1
  *     reads 5 of the 9 link blocks of a site (Hermitian Galerkin operators), g indexes the
  *     workgroup counts {128, 256, 512, 1024, 2048} of the launch with dot products;
  *   SMG_TUNE_DSLASH_DOTS: the batched stencil with dot products; (L, nrhs, bytes per
  *     real, 0, 0); choice = g;
  *   SMG_TUNE_DSLASH_STORE: the batched stencil's stores; (L, nrhs, bytes per real,
  *     epilogue, 0); choice 1 = non-temporal stores, 0 = ordinary (bit-identical results);
- *   SMG_TUNE_VEC_RED: the coarsest BF16-vector CG's residual update with |r|^2;
- *     (coarse unknowns per RHS, nrhs, 0, 0, 0); choice = g;
+ *   SMG_TUNE_CG_UPDATE: the coarsest BF16-vector CG's vector update (r, x and p in one
+ *     pass, no reduction); (coarse unknowns per RHS, nrhs, 0, 0, 0); choice = g;
  *   SMG_TUNE_TRANSFER: the batched restriction / prolongation; (nx, ny, 128*nd + k of the
  *     finer level, nrhs, 2*FP64 + prolong); choice 1 = a workgroup per aggregate with its
  *     blocks staged in LDS, 0 = one work-item per output (the same sums). */
 #define SMG_TUNE_COARSE_BF16 0
 #define SMG_TUNE_DSLASH_DOTS 1
 #define SMG_TUNE_DSLASH_STORE 2
-#define SMG_TUNE_VEC_RED 3
+#define SMG_TUNE_CG_UPDATE 3
 #define SMG_TUNE_TRANSFER 4
 /* times the candidates of every kernel of mg's batched cycle for nrhs (16, 32 or 64)
  * right-hand sides and keeps the fastest; *entries = table entries added (may be NULL) */

@@ -1257,7 +1257,7 @@ def therm_cfg_path(L, beta, n_therm):
 
 
 # ------------------------------------------------------------------ launch-shape autotuner
-TUNE_COARSE_BF16, TUNE_DSLASH_DOTS, TUNE_DSLASH_STORE, TUNE_VEC_RED, TUNE_TRANSFER = 0, 1, 2, 3, 4
+TUNE_COARSE_BF16, TUNE_DSLASH_DOTS, TUNE_DSLASH_STORE, TUNE_CG_UPDATE, TUNE_TRANSFER = 0, 1, 2, 3, 4
 
 
 def tune_mg(mg, nrhs):

@@ -716,9 +716,10 @@ __device__ __forceinline__ c32 unpack_h(unsigned w) {
 template <typename TL, int EPI, int K, int NT, bool HV = false, bool HERM = false>
 __global__ __launch_bounds__(kBlock)
 void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __restrict__ srcv,
-                     const c32* __restrict__ bvec, const void* __restrict__ auxv, FinB fin) {
+                     const void* __restrict__ bv, const void* __restrict__ auxv, FinB fin) {
     static_assert(std::is_same<TL, ch16>::value, "bf16x3 path: FP16 links");
     static_assert(!HV || EPI != EPI_RESID, "BF16 vectors: no residual epilogue");
+    static_assert(HV || EPI != EPI_CG, "the CG epilogue: BF16 vectors");
     static_assert(!HERM || K % 16 == 0, "Hermitian reads: a chunk of 16 columns inside one direction");
     static_assert(K == 8 || K == 16 || K == 32, "k in {8, 16, 32}");
     c32* __restrict__ dst = static_cast<c32*>(dstv);
@@ -727,7 +728,14 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
     unsigned* __restrict__ dsth = static_cast<unsigned*>(dstv);
     const unsigned* __restrict__ srch = static_cast<const unsigned*>(srcv);
     const unsigned* __restrict__ auxh = static_cast<const unsigned*>(auxv);
-    if (EPI == EPI_DOTS && fin.skip && *fin.skip) return;   // CG finished for every RHS
+    const c32* __restrict__ bvec = static_cast<const c32*>(bv);
+    const unsigned* __restrict__ bh2 = static_cast<const unsigned*>(bv);   // EPI_CG: r
+    constexpr bool RED = EPI == EPI_DOTS || EPI == EPI_CG;
+    constexpr int NC = EPI == EPI_CG ? 5 : 3;   // reduced values per RHS
+    if (RED && fin.skip && *fin.skip) {   // CG finished for every RHS
+        if (fin.late && blockIdx.x == 0 && threadIdx.x == 0) *fin.late = 1;
+        return;
+    }
     constexpr int N = 16 * NT, RT = (K + 15) / 16, NCH = (9 * K + 15) / 16;
     constexpr bool PAIR = NT % 2 == 0;
     constexpr int NL = PAIR ? NT / 2 : NT;
@@ -739,9 +747,11 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
     const bool row_ok = K >= 16 || lo < K;        // this lane's link row
     const bool out_ok = K >= 16 || 4 * hi < K;    // this lane's result rows
     const unsigned* links = reinterpret_cast<const unsigned*>(C.link);
-    double dacc[NT][3];
+    double dacc[NT][NC];
 #pragma unroll
-    for (int t = 0; t < NT; t++) dacc[t][0] = dacc[t][1] = dacc[t][2] = 0.0;
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) dacc[t][c] = 0.0;
     for (int b = blockIdx.x * nw + (threadIdx.x >> 6); b < nsites; b += gridDim.x * nw) {
         const int bx = b % nx, by = b / nx;
         f32x4 acc[RT][NT][2];
@@ -859,10 +869,12 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                 }
             }
         };
-        // the epilogue's operand (b for EPI_RESID, aux for EPI_DOTS) at this lane's
-        // results, loaded while the last chunks are multiplied (PAIR layout)
+        // the epilogue's operands (b for EPI_RESID, aux for EPI_DOTS, aux and b for
+        // EPI_CG) at this lane's results, loaded while the last chunks are multiplied
+        // (PAIR layout)
         using EpoV = typename std::conditional<HV, uint2, float4>::type;
-        EpoV epo[PAIR ? RT : 1][PAIR ? 4 : 1][PAIR ? NL : 1];
+        constexpr int NE = EPI == EPI_CG ? 2 : 1;
+        EpoV epo[PAIR ? RT : 1][PAIR ? 4 : 1][PAIR ? NL : 1][NE];
         auto load_epo = [&]() {
             if constexpr (PAIR && EPI != EPI_NONE) {
                 if (out_ok) {
@@ -873,8 +885,12 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
 #pragma unroll
                             for (int u = 0; u < NL; u++) {
                                 const size_t o = ((size_t)b * K + 16 * r + 4 * hi + v) * N + 32 * u + 2 * lo;
-                                if constexpr (HV) epo[r][v][u] = *reinterpret_cast<const uint2*>(auxh + o);
-                                else epo[r][v][u] = *reinterpret_cast<const float4*>((EPI == EPI_RESID ? bvec : aux) + o);
+                                if constexpr (HV) {
+                                    epo[r][v][u][0] = *reinterpret_cast<const uint2*>(auxh + o);
+                                    if constexpr (EPI == EPI_CG) epo[r][v][u][1] = *reinterpret_cast<const uint2*>(bh2 + o);
+                                } else {
+                                    epo[r][v][u][0] = *reinterpret_cast<const float4*>((EPI == EPI_RESID ? bvec : aux) + o);
+                                }
                             }
                 }
             }
@@ -918,40 +934,53 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                             const unsigned w0 = pack_h(c32{acc[r][2 * u][0][v], acc[r][2 * u][1][v]});
                             const unsigned w1 = pack_h(c32{acc[r][2 * u + 1][0][v], acc[r][2 * u + 1][1][v]});
                             *reinterpret_cast<uint2*>(dsth + idx) = uint2{w0, w1};
-                            if (EPI == EPI_DOTS) {   // dots of the stored (rounded) values
+                            if (RED) {   // dots of the stored (rounded) values
                                 const c32 o0 = unpack_h(w0), o1 = unpack_h(w1);
-                                const uint2 w = epo[r][v][u];
+                                const uint2 w = epo[r][v][u][0];
                                 const c32 p0 = cdot(o0, unpack_h(w.x)), p1 = cdot(o1, unpack_h(w.y));
                                 dacc[2 * u][0] += p0.re;
                                 dacc[2 * u][1] += p0.im;
-                                dacc[2 * u][2] += norm2(o0);
+                                dacc[2 * u][NC - 1] += norm2(o0);
                                 dacc[2 * u + 1][0] += p1.re;
                                 dacc[2 * u + 1][1] += p1.im;
-                                dacc[2 * u + 1][2] += norm2(o1);
+                                dacc[2 * u + 1][NC - 1] += norm2(o1);
+                                if constexpr (EPI == EPI_CG) {
+                                    const uint2 z = epo[r][v][u][NE - 1];
+                                    const c32 q0 = cdot(o0, unpack_h(z.x)), q1 = cdot(o1, unpack_h(z.y));
+                                    dacc[2 * u][2] += q0.re;
+                                    dacc[2 * u][3] += q0.im;
+                                    dacc[2 * u + 1][2] += q1.re;
+                                    dacc[2 * u + 1][3] += q1.im;
+                                }
                             }
                         } else if constexpr (HV) {
                             const size_t idx = row + 16 * u + lo;
                             const unsigned w = pack_h(c32{acc[r][u][0][v], acc[r][u][1][v]});
                             dsth[idx] = w;
-                            if (EPI == EPI_DOTS) {
+                            if (RED) {
                                 const c32 o = unpack_h(w);
                                 const c32 p = cdot(o, unpack_h(auxh[idx]));
                                 dacc[u][0] += p.re;
                                 dacc[u][1] += p.im;
-                                dacc[u][2] += norm2(o);
+                                dacc[u][NC - 1] += norm2(o);
+                                if constexpr (EPI == EPI_CG) {
+                                    const c32 q = cdot(o, unpack_h(bh2[idx]));
+                                    dacc[u][2] += q.re;
+                                    dacc[u][3] += q.im;
+                                }
                             }
                         } else if constexpr (PAIR) {
                             const size_t idx = row + 32 * u + 2 * lo;
                             c32 o0{acc[r][2 * u][0][v], acc[r][2 * u][1][v]};
                             c32 o1{acc[r][2 * u + 1][0][v], acc[r][2 * u + 1][1][v]};
                             if (EPI == EPI_RESID) {
-                                const float4 w = epo[r][v][u];
+                                const float4 w = epo[r][v][u][0];
                                 o0 = c32{w.x, w.y} - o0;
                                 o1 = c32{w.z, w.w} - o1;
                             }
                             *reinterpret_cast<float4*>(dst + idx) = float4{o0.re, o0.im, o1.re, o1.im};
                             if (EPI == EPI_DOTS) {
-                                const float4 w = epo[r][v][u];
+                                const float4 w = epo[r][v][u][0];
                                 const c32 p0 = cdot(o0, c32{w.x, w.y}), p1 = cdot(o1, c32{w.z, w.w});
                                 dacc[2 * u][0] += p0.re;
                                 dacc[2 * u][1] += p0.im;
@@ -976,11 +1005,11 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                 }
         }
     }
-    if (EPI == EPI_DOTS) {
+    if (RED) {
 #pragma unroll
         for (int t = 0; t < NT; t++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
+            for (int c = 0; c < NC; c++) {
                 dacc[t][c] += __shfl_xor(dacc[t][c], 16, 64);
                 dacc[t][c] += __shfl_xor(dacc[t][c], 32, 64);
             }
@@ -992,35 +1021,40 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
             tj = (lane >> 4) & (NT - 1);
             lj = lane & 15;
         }
-        double v[3] = {0.0, 0.0, 0.0};
+        double v[NC];
+#pragma unroll
+        for (int c = 0; c < NC; c++) v[c] = 0.0;
 #pragma unroll
         for (int t = 0; t < NT; t++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
+            for (int c = 0; c < NC; c++) {
                 const double x = __shfl(dacc[t][c], lj, 64);
                 if (t == tj && lane < N) v[c] = x;
             }
-        reduce_finish_b<3, 64>(v, fin, N, gridDim.x, blockIdx.x);
+        reduce_finish_b<NC, 64>(v, fin, N, gridDim.x, blockIdx.x);
     }
 }
 
 template <typename TL, int K, int NT, bool HV, bool HERM>
-static void coarse_bf16_v(const CoarseView<TL>& C, void* dst, const void* src, Epi epi, const c32* b,
+static void coarse_bf16_v(const CoarseView<TL>& C, void* dst, const void* src, Epi epi, const void* b,
                           const void* aux, const FinB& fin, int gdots, hipStream_t st) {
     const size_t waves = (size_t)C.nx * C.ny;
     const int nw = kBlock / 64;
-    const int g = (int)std::min<size_t>((waves + nw - 1) / nw, epi == EPI_DOTS ? gdots : 1 << 16);
+    const bool red = epi == EPI_DOTS || epi == EPI_CG;
+    const int g = (int)std::min<size_t>((waves + nw - 1) / nw, red ? gdots : 1 << 16);
     if (epi == EPI_NONE)
         k_b_coarse_bf16<TL, EPI_NONE, K, NT, HV, HERM><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
     else if (epi == EPI_DOTS)
         k_b_coarse_bf16<TL, EPI_DOTS, K, NT, HV, HERM><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
     else if constexpr (!HV)
         k_b_coarse_bf16<TL, EPI_RESID, K, NT, false, HERM><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
+    else if (epi == EPI_CG)
+        k_b_coarse_bf16<TL, EPI_CG, K, NT, true, HERM><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
 }
 // The tuner's choice for this shape, else the defaults: the Hermitian reads whenever
 // the links allow them, kRedGridCoarse workgroups with fused dot products.
 template <typename TL, int K, int NT, bool HV = false>
-static void coarse_bf16(const CoarseView<TL>& C, void* dst, const void* src, Epi epi, const c32* b, const void* aux,
+static void coarse_bf16(const CoarseView<TL>& C, void* dst, const void* src, Epi epi, const void* b, const void* aux,
                         const FinB& fin, hipStream_t st) {
     const int choice = tune_lookup(TuneKey{TUNE_COARSE_BF16, C.nx, C.ny, K, 16 * NT, (HV ? 4 : 0) + (int)epi});
     const int gdots = choice < 0 ? kRedGridCoarse : kTuneGrids[choice >> 1];
@@ -1050,6 +1084,7 @@ static void coarse_mfma(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* sr
 template <typename T, typename TL>
 void launch_b_coarse_apply(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b,
                            const cplx<T>* aux, const FinB& fin, int N, hipStream_t st) {
+    SMG_REQUIRE(epi == EPI_NONE || epi == EPI_RESID || epi == EPI_DOTS, "coarse apply: epilogue");
     if constexpr (std::is_same<T, float>::value && std::is_same<TL, ch16>::value) {
         if ((C.k == 8 || C.k == 16 || C.k == 32) && (N == 16 || N == 32 || N == 64)) {
             // FP16 links: bf16x3 matrix-core path
@@ -1100,12 +1135,13 @@ void launch_b_coarse_apply(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>*
 
 // The coarse apply on BF16 vectors (FP16 links): epi EPI_NONE or EPI_DOTS
 void launch_b_coarse_apply_h(const CoarseView<ch16>& C, cb16* dst, const cb16* src, Epi epi, const cb16* aux,
-                             const FinB& fin, int N, hipStream_t st) {
+                             const FinB& fin, int N, hipStream_t st, const cb16* b) {
     SMG_REQUIRE((C.k == 8 || C.k == 16 || C.k == 32) && (N == 16 || N == 32 || N == 64) && epi != EPI_RESID,
                 "BF16-vector coarse apply: k in {8,16,32}, N in {16,32,64}, no residual epilogue");
+    SMG_REQUIRE(epi != EPI_CG || (aux && b), "BF16-vector coarse apply: the CG epilogue takes aux and b");
     auto go = [&](auto kc, auto ntc) {
         constexpr int K = decltype(kc)::value, NT = decltype(ntc)::value;
-        coarse_bf16<ch16, K, NT, true>(C, dst, src, epi, nullptr, aux, fin, st);
+        coarse_bf16<ch16, K, NT, true>(C, dst, src, epi, b, aux, fin, st);
     };
     auto by_n = [&](auto kc) {
         if (N == 16) go(kc, std::integral_constant<int, 1>{});
@@ -1858,66 +1894,34 @@ __global__ __launch_bounds__(kBlock) void k_h_cg_init(const c32* __restrict__ b,
     runs_to_lanes<1, 4>(acc, v, N);
     reduce_finish_b<1, 64>(v, fin, N, gridDim.x, blockIdx.x);
 }
-__global__ __launch_bounds__(kBlock) void k_h_cg_r(cb16* __restrict__ r, const cb16* __restrict__ Ap,
-                                                   const double* __restrict__ s, size_t runs, FinB fin, int N,
-                                                   int* __restrict__ late) {
-    if (fin.skip && *fin.skip) {
-        if (late && blockIdx.x == 0 && threadIdx.x == 0) *late = 1;
-        return;
-    }
-    double acc[1][4] = {};
-    // kU runs per trip, their loads issued together (a work-item takes its runs in the
-    // same order as a one-run loop, so the sums are the same)
-    constexpr int kU = 8;
-    const size_t stride = (size_t)gridDim.x * kBlock;
-    for (size_t t0 = (size_t)blockIdx.x * kBlock + threadIdx.x; t0 < runs; t0 += kU * stride) {
-        HRun rv[kU], av[kU];
-#pragma unroll
-        for (int u = 0; u < kU; u++) {
-            const size_t t = t0 + u * stride;
-            if (t < runs) {
-                rv[u].load(r + t * 4);
-                av[u].load(Ap + t * 4);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kU; u++) {
-            const size_t t = t0 + u * stride;
-            if (t >= runs) break;
-            const size_t i = t * 4;
-            c32 a[4];
-            run_coef<float, 4>(s, (int)(i & (N - 1)), a);
-#pragma unroll
-            for (int q = 0; q < 4; q++) rv[u].v[q] = rv[u].v[q] - a[q] * av[u].v[q];
-            const uint4 w = rv[u].store(r + i);
-            const unsigned ws[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-            for (int q = 0; q < 4; q++) acc[0][q] += (double)norm2(unpack_h(ws[q]));
-        }
-    }
-    double v[1];
-    runs_to_lanes<1, 4>(acc, v, N);
-    reduce_finish_b<1, 64>(v, fin, N, gridDim.x, blockIdx.x);
-}
-__global__ __launch_bounds__(kBlock) void k_h_cg_xp(cb16* __restrict__ x, cb16* __restrict__ p,
-                                                    const cb16* __restrict__ r, const double* __restrict__ sa,
-                                                    const double* __restrict__ sb, size_t runs, int N,
-                                                    const int* __restrict__ late) {
+// one CG step after the apply q = A p (EPI_CG, FIN_CG_STEP): r -= a q, x += a p,
+// p = r + b p with r as stored (rounded to BF16), a_j at a[2j] and b_j at aux[16j + 6]
+__global__ __launch_bounds__(kBlock) void k_h_cg_step(cb16* __restrict__ x, cb16* __restrict__ p,
+                                                      cb16* __restrict__ r, const cb16* __restrict__ q,
+                                                      const double* __restrict__ sa, const double* __restrict__ aux,
+                                                      size_t runs, int N, const int* __restrict__ late) {
     if (late && *late) return;
     for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < runs; t += (size_t)gridDim.x * kBlock) {
         const size_t i = t * 4;
+        const int j0 = (int)(i & (N - 1));
         c32 a[4], b[4];
-        run_coef<float, 4>(sa, (int)(i & (N - 1)), a);
-        run_coef<float, 4>(sb, (int)(i & (N - 1)), b);
-        HRun xv, pv, rv;
+        run_coef<float, 4>(sa, j0, a);
+#pragma unroll
+        for (int k = 0; k < 4; k++) b[k] = c32{(float)aux[16 * (j0 + k) + 6], (float)aux[16 * (j0 + k) + 7]};
+        HRun xv, pv, rv, qv;
+        rv.load(r + i);
+        qv.load(q + i);
         xv.load(x + i);
         pv.load(p + i);
-        rv.load(r + i);
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            xv.v[q] = xv.v[q] + a[q] * pv.v[q];
-            pv.v[q] = rv.v[q] + b[q] * pv.v[q];
+        for (int k = 0; k < 4; k++) {
+            rv.v[k] = rv.v[k] - a[k] * qv.v[k];
+            xv.v[k] = xv.v[k] + a[k] * pv.v[k];
         }
+        const uint4 w = rv.store(r + i);
+        const unsigned ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) pv.v[k] = unpack_h(ws[k]) + b[k] * pv.v[k];
         xv.store(x + i);
         pv.store(p + i);
     }
@@ -1950,17 +1954,13 @@ void launch_h_cg_init(const c32* b, cb16* r, cb16* p, cb16* x, size_t n, const F
     k_h_cg_init<<<b_grid(n * N / 4, kRedGridVec), kBlock, 0, st>>>(b, r, p, x, n * N / 4, fin, N);
     SMG_LAUNCH_CHECK();
 }
-void launch_h_cg_r(cb16* r, const cb16* Ap, const double* a, size_t n, const FinB& fin, int N, int* late,
-                   hipStream_t st) {
-    // the tuner's workgroup count for this shape (it fixes the summation order), else the default
-    const int choice = tune_lookup(TuneKey{TUNE_VEC_RED, (int)n, N, 0, 0, 0});
-    const int cap = choice < 0 ? kRedGridVec : kTuneGrids[choice];
-    k_h_cg_r<<<b_grid(n * N / 4, cap), kBlock, 0, st>>>(r, Ap, a, n * N / 4, fin, N, late);
-    SMG_LAUNCH_CHECK();
-}
-void launch_h_cg_xp(cb16* x, cb16* p, const cb16* r, const double* a, const double* b, size_t n, int N,
-                    const int* late, hipStream_t st) {
-    k_h_cg_xp<<<b_grid(n * N / 4, 1 << 16), kBlock, 0, st>>>(x, p, r, a, b, n * N / 4, N, late);
+void launch_h_cg_step(cb16* x, cb16* p, cb16* r, const cb16* q, const double* a, const double* aux, size_t n, int N,
+                      const int* late, hipStream_t st) {
+    SMG_REQUIRE(N >= 16 && N <= 64 && (N & (N - 1)) == 0, "BF16 CG: N in {16, 32, 64}");
+    // the tuner's workgroup count for this shape, else the whole vector in one sweep
+    const int choice = tune_lookup(TuneKey{TUNE_CG_UPDATE, (int)n, N, 0, 0, 0});
+    const int cap = choice < 0 ? 1 << 16 : kTuneGrids[choice];
+    k_h_cg_step<<<b_grid(n * N / 4, cap), kBlock, 0, st>>>(x, p, r, q, a, aux, n * N / 4, N, late);
     SMG_LAUNCH_CHECK();
 }
 void launch_h_to_f32(c32* out, const cb16* in, size_t n, int N, hipStream_t st) {

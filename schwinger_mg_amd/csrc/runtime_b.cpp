@@ -153,26 +153,28 @@ static void cg_b(LinOpB<T>& A, const cplx<T>* b, cplx<T>* x, cplx<T>* r, cplx<T>
 }
 
 // The same CG on the FP16-link coarse operator with BF16 vectors (k_h_* kernels):
-// half the vector traffic of cg_b<float>, for the loose coarsest-level tolerance
+// half the vector traffic of cg_b<float>, for the loose coarsest-level tolerance, and
+// two launches per iteration: the apply q = A p with the sums <q,p>, <q,r>, |q|^2,
+// whose finish computes alpha, the new |r|^2 (from those sums), the stop test and
+// beta (FIN_CG_STEP), then one pass r -= alpha q, x += alpha p, p = r + beta p.  The
+// iterates are those of cg_b in exact arithmetic.
 static void cg_h(const CoarseOperator& C, const c32* b, c32* x, cb16* r, cb16* p, cb16* Ap, cb16* xh, int N,
                  int maxit, double tol, double* sc, hipStream_t st) {
     const size_t n = C.n();
     double* alpha = sc;
-    double* beta = sc + 2 * 64;
     double* aux = sc + 4 * 64;
     int* done = reinterpret_cast<int*>(aux + 16 * 64);
     int* late = reinterpret_cast<int*>(aux + 16 * 64 + 1);
     launch_b_cg_init(aux, 16, N, tol * tol, done, late, st);
     launch_h_cg_init(b, r, p, xh, n, finb(FIN_CG_INIT, nullptr, 0, aux), N, st);
-    FinB fa = finb(FIN_CG_ALPHA, alpha, 2, aux), fbt = finb(FIN_CG_BETA, beta, 2, aux);
-    fa.skip = done;
-    fbt.skip = done;
-    fbt.done = done;
+    FinB fs = finb(FIN_CG_STEP, alpha, 2, aux);
+    fs.skip = done;
+    fs.done = done;
+    fs.late = late;
     const CoarseView<ch16> v = C.view16();
     for (int it = 0; it < maxit; it++) {
-        launch_b_coarse_apply_h(v, Ap, p, EPI_DOTS, p, fa, N, st);
-        launch_h_cg_r(r, Ap, alpha, n, fbt, N, late, st);
-        launch_h_cg_xp(xh, p, r, alpha, beta, n, N, late, st);
+        launch_b_coarse_apply_h(v, Ap, p, EPI_CG, p, fs, N, st, r);
+        launch_h_cg_step(xh, p, r, Ap, alpha, aux, n, N, late, st);
     }
     launch_h_to_f32(x, xh, n, N, st);
 }
@@ -520,7 +522,7 @@ int pick(const TuneKey& key, const std::vector<int>& cands, const std::function<
 int tune_mg(MG& mg, int N, hipStream_t st) {
     SMG_REQUIRE(N == 16 || N == 32 || N == 64, "tuning: N must be 16, 32 or 64");
     const int before = (int)tune_size();
-    Field<double> out((size_t)3 * 64);
+    Field<double> out((size_t)5 * 64);
     const FinB fin = finb(FIN_STORE, reinterpret_cast<double*>(out.p()), 3);
     for (size_t l = 0; l < mg.C.size(); l++) {
         const CoarseOperator& C = *mg.C[l];
@@ -547,19 +549,25 @@ int tune_mg(MG& mg, int N, hipStream_t st) {
         if (last && mg.cfg.coarse_half == 2 && mg.cfg.coarse_solver == 0) {   // BF16-vector coarsest CG
             DevMem hs(nN * sizeof(cb16)), hd(nN * sizeof(cb16));
             SMG_HIP(hipMemsetAsync(hs.p, 0, nN * sizeof(cb16), st));
-            const TuneKey key{TUNE_COARSE_BF16, C.nx, C.ny, C.k, N, 4 + (int)EPI_DOTS};
-            pick(key, dots, [&] {
-                launch_b_coarse_apply_h(C.view16(), hd.as<cb16>(), hs.as<cb16>(), EPI_DOTS, hs.as<cb16>(), fin, N,
-                                        st);
-            }, st);
-            // the CG's residual update with |r|^2 (alpha = 0 leaves r as it is)
-            Field<double> a0((size_t)N);
-            launch_zero<double>(a0.p(), N, st);
+            const FinB fin5 = finb(FIN_STORE, reinterpret_cast<double*>(out.p()), 5);
+            for (int epi : {(int)EPI_DOTS, (int)EPI_CG})
+                pick(TuneKey{TUNE_COARSE_BF16, C.nx, C.ny, C.k, N, 4 + epi}, dots, [&] {
+                    launch_b_coarse_apply_h(C.view16(), hd.as<cb16>(), hs.as<cb16>(), (Epi)epi, hs.as<cb16>(),
+                                            epi == EPI_CG ? fin5 : fin, N, st, hs.as<cb16>());
+                }, st);
+            // the CG's vector update (zero coefficients leave the vectors as they are)
+            Field<double> a0((size_t)16 * 64);
+            launch_zero<double>(a0.p(), (size_t)16 * 64, st);
             std::vector<int> vgrids;
             for (int g = 0; g < kTuneNGrids; g++) vgrids.push_back(g);
-            pick(TuneKey{TUNE_VEC_RED, (int)C.n(), N, 0, 0, 0}, vgrids, [&] {
-                launch_h_cg_r(hd.as<cb16>(), hs.as<cb16>(), reinterpret_cast<const double*>(a0.p()), C.n(), fin, N,
-                              nullptr, st);
+            DevMem hx(nN * sizeof(cb16)), hq(nN * sizeof(cb16));
+            SMG_HIP(hipMemsetAsync(hx.p, 0, nN * sizeof(cb16), st));
+            SMG_HIP(hipMemsetAsync(hq.p, 0, nN * sizeof(cb16), st));
+            SMG_HIP(hipMemsetAsync(hd.p, 0, nN * sizeof(cb16), st));
+            pick(TuneKey{TUNE_CG_UPDATE, (int)C.n(), N, 0, 0, 0}, vgrids, [&] {
+                launch_h_cg_step(hx.as<cb16>(), hs.as<cb16>(), hd.as<cb16>(), hq.as<cb16>(),
+                                 reinterpret_cast<const double*>(a0.p()), reinterpret_cast<const double*>(a0.p()),
+                                 C.n(), N, nullptr, st);
             }, st);
         }
     }

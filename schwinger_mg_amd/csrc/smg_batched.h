@@ -25,6 +25,9 @@ struct FinB {
     // entry while *skip != 0 (grid-uniform, written by an earlier kernel)
     int* done = nullptr;
     const int* skip = nullptr;
+    // set to 1 by a kernel that returned at entry on skip (the kernel after it in the
+    // same iteration then returns too: its coefficients were not refreshed)
+    int* late = nullptr;
 };
 
 // Per-RHS coefficient array views (device): complex a_j at s[2j], s[2j+1].
@@ -98,17 +101,18 @@ void launch_b_unpack(cplx<To>* sep, const cplx<From>* batched, size_t n, int N, 
 void launch_b_accum_mixed(c64* x, const c32* e, size_t n, int N, hipStream_t st);
 
 // BF16 vectors (one dword per complex entry: bf16 Re low, bf16 Im high), N in
-// {16, 32, 64}: the coarse apply with FP16 links (EPI_NONE or EPI_DOTS) and the CG
-// passes of launch_b_cg_r / launch_b_cg_xp; init: r = p = bf16(b), x = 0 with
+// {16, 32, 64}: the coarse apply with FP16 links (EPI_NONE, EPI_DOTS, or EPI_CG with
+// the second partner b) and the CG's vector update; init: r = p = bf16(b), x = 0 with
 // |r|^2 -> fin; to_f32 converts a batched BF16 vector to FP32
 using cb16 = unsigned;
 void launch_b_coarse_apply_h(const CoarseView<ch16>& C, cb16* dst, const cb16* src, Epi epi, const cb16* aux,
-                             const FinB& fin, int N, hipStream_t st);
+                             const FinB& fin, int N, hipStream_t st, const cb16* b = nullptr);
 void launch_h_cg_init(const c32* b, cb16* r, cb16* p, cb16* x, size_t n, const FinB& fin, int N, hipStream_t st);
-void launch_h_cg_r(cb16* r, const cb16* Ap, const double* a, size_t n, const FinB& fin, int N, int* late,
-                   hipStream_t st);
-void launch_h_cg_xp(cb16* x, cb16* p, const cb16* r, const double* a, const double* b, size_t n, int N,
-                    const int* late, hipStream_t st);
+// one CG step after the apply q = A p with EPI_CG and FIN_CG_STEP: r -= a_j q,
+// x += a_j p, p = r + b_j p (a_j at a[2j], b_j at aux[16j + 6]); returns at entry
+// when *late is set
+void launch_h_cg_step(cb16* x, cb16* p, cb16* r, const cb16* q, const double* a, const double* aux, size_t n, int N,
+                      const int* late, hipStream_t st);
 void launch_h_to_f32(c32* out, const cb16* in, size_t n, int N, hipStream_t st);
 void launch_h_from_f32(cb16* out, const c32* in, size_t n, int N, hipStream_t st);
 

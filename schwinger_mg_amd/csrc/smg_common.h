@@ -129,9 +129,12 @@ enum FinOp : int {
     FIN_CG_BETA = 4,    // sum = {|r|^2}: iteration count aux[3], stop test vs aux[2], out = rr/aux[0]
     FIN_PCG_BETA = 5,   // sum = {Re<r,z>, Im<r,z>}: out = sum/aux[0..1]; aux[0..1] = sum
     FIN_GCR_ALPHA = 6,  // sum = {Re<Ap,r>, Im<Ap,r>, |Ap|^2}: aux[0] = |Ap|^2; out = <Ap,r>/|Ap|^2 or 0
+    // sum = {Re<q,p>, Im<q,p>, Re<q,r>, Im<q,r>, |q|^2} with q = A p: one whole CG step,
+    // out = alpha, aux[6..7] = beta, |r - alpha q|^2 from the sums (see fin_program)
+    FIN_CG_STEP = 7,
 };
 // aux layout of the CG programs: aux[0..1] rz, aux[2] stop threshold, aux[3] iterations,
-// aux[4] last |r|^2, aux[5] tol^2, int flag at (int*)(aux + 8)
+// aux[4] last |r|^2, aux[5] tol^2, aux[6..7] beta (FIN_CG_STEP), int flag at (int*)(aux + 8)
 
 // A reduction request handed to a kernel: the kernel writes one partial row per
 // workgroup to part, and the workgroup that finishes last sums the rows in index
@@ -206,6 +209,37 @@ __device__ __forceinline__ void fin_program(int op, const double* v, int nc, dou
         }
         out[0] = rr / aux[0];
         out[1] = 0.0;
+        aux[0] = rr;
+        aux[1] = 0.0;
+        break;
+    }
+    case FIN_CG_STEP: {
+        // FIN_CG_ALPHA and FIN_CG_BETA in one program, for an update kernel that applies
+        // r -= alpha q, x += alpha p, p = r + beta p in one pass.  The new |r|^2 comes from
+        // the apply's sums, |r - alpha q|^2 = |r|^2 - 2 Re(alpha conj(<q,r>)) + |alpha|^2 |q|^2
+        // (<q,r> = sum conj(q) r).  A finished RHS takes zero steps from then on (alpha =
+        // beta = 0: x and r stay, p = r).
+        aux[6] = 0.0;
+        aux[7] = 0.0;
+        out[0] = 0.0;
+        out[1] = 0.0;
+        if (*flag) break;
+        const cplx<double> pAp = {v[0], -v[1]};
+        if (sqrt(pAp.re * pAp.re + pAp.im * pAp.im) < 1e-30) {
+            *flag = 2;
+            break;
+        }
+        const cplx<double> a = cdiv(cplx<double>{aux[0], aux[1]}, pAp);
+        out[0] = a.re;
+        out[1] = a.im;
+        const double rr = aux[0] - 2.0 * (a.re * v[2] + a.im * v[3]) + (a.re * a.re + a.im * a.im) * v[4];
+        aux[3] += 1.0;
+        aux[4] = rr;
+        if (rr < aux[2] || isnan(rr) || isinf(rr)) {
+            *flag = 1;
+            break;
+        }
+        aux[6] = rr / aux[0];
         aux[0] = rr;
         aux[1] = 0.0;
         break;

@@ -9,6 +9,9 @@ enum Epi : int {
     EPI_NONE = 0,   // dst = A src
     EPI_RESID = 1,  // dst = b - A src
     EPI_DOTS = 2,   // dst = A src; reduction {Re<dst,aux>, Im<dst,aux>, |dst|^2} -> Fin
+    // dst = A src; reduction {Re<dst,aux>, Im<dst,aux>, Re<dst,b>, Im<dst,b>, |dst|^2} -> Fin
+    // (the BF16-vector coarse apply of the coarsest CG: aux = p, b = r)
+    EPI_CG = 3,
 };
 
 // ---------------------------------------------------------------- Dirac (dirac.hip)

@@ -21,9 +21,9 @@ enum TuneFamily : int {
     TUNE_DSLASH_DOTS = 1,
     // k_b_dslash stores; shape (L, N, prec, epi, 0); choice 1: non-temporal stores, 0: ordinary
     TUNE_DSLASH_STORE = 2,
-    // k_h_cg_r (the BF16-vector CG's residual update with |r|^2); shape (n, N, 0, 0, 0);
-    // choice = grid index
-    TUNE_VEC_RED = 3,
+    // k_h_cg_step (the BF16-vector CG's vector update, no reduction: the grid does not
+    // change the results); shape (n, N, 0, 0, 0); choice = grid index
+    TUNE_CG_UPDATE = 3,
     // the batched transfers; shape (nx, ny, 128 nd + k, N, 2 (FP64) + (1 prolong, 0 restrict))
     // of the finer lattice; choice 1: a workgroup per aggregate with its blocks staged in
     // LDS, 0: one work-item per output reading through the caches (the same sums)

@@ -42,7 +42,7 @@ def test_refusals(smg, tmp_path):
     with pytest.raises(smg.SMGError):
         smg.tune_set(smg.TUNE_TRANSFER, [512, 512, 128 * 2 + 32, 32, 1], 2)   # choices 0 and 1
     with pytest.raises(smg.SMGError):
-        smg.tune_set(smg.TUNE_VEC_RED, [131072, 32, 0, 0, 0], 5)         # no sixth grid
+        smg.tune_set(smg.TUNE_CG_UPDATE, [131072, 32, 0, 0, 0], 5)         # no sixth grid
     bad = str(tmp_path / "bad.txt")
     open(bad, "w").write("0 64 64 32 32 2 1\n0 64 64 32 32 x 1\n")
     with pytest.raises(smg.SMGError):
