@@ -1662,32 +1662,39 @@ __device__ __forceinline__ void run_coef(const double* __restrict__ s, int j, cp
     for (int q = 0; q < P; q++) a[q] = cplx<T>{(T)s[2 * (j + q)], (T)s[2 * (j + q) + 1]};
 }
 
-template <typename T, int P>
+// MODE 0: x += w r, r -= w Ar; 1 (the first sweep from x = 0, r = b): x = w b,
+// r = b - w Ar; 2 (the last sweep when the residual is not used again): x += w r
+template <typename T, int P, int MODE>
 __global__ __launch_bounds__(kBlock) void k_b_mr_update(cplx<T>* __restrict__ x, cplx<T>* __restrict__ r,
-                                                        const cplx<T>* __restrict__ Ar, const double* __restrict__ s,
-                                                        size_t runs, int N) {
+                                                        const cplx<T>* __restrict__ Ar, const cplx<T>* __restrict__ b,
+                                                        const double* __restrict__ s, size_t runs, int N) {
     for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < runs; t += (size_t)gridDim.x * kBlock) {
         const size_t i = t * P;
         cplx<T> w[P];
         run_coef<T, P>(s, (int)(i & (N - 1)), w);
         RhsRun<T, P> xv, rv, av;
-        xv.load(x + i);
-        rv.load(r + i);
-        av.load(Ar + i);
+        if (MODE != 1) xv.load(x + i);
+        rv.load((MODE == 1 ? b : r) + i);
+        if (MODE != 2) av.load(Ar + i);
 #pragma unroll
         for (int q = 0; q < P; q++) {
-            xv.v[q] = xv.v[q] + w[q] * rv.v[q];
-            rv.v[q] = rv.v[q] - w[q] * av.v[q];
+            xv.v[q] = MODE == 1 ? w[q] * rv.v[q] : xv.v[q] + w[q] * rv.v[q];
+            if (MODE != 2) rv.v[q] = rv.v[q] - w[q] * av.v[q];
         }
         xv.store(x + i);
-        rv.store(r + i);
+        if (MODE != 2) rv.store(r + i);
     }
 }
 template <typename T>
-void launch_b_mr_update(cplx<T>* x, cplx<T>* r, const cplx<T>* Ar, const double* s, size_t n, int N, hipStream_t st) {
+void launch_b_mr_update(cplx<T>* x, cplx<T>* r, const cplx<T>* Ar, const double* s, size_t n, int N, hipStream_t st,
+                        int mode, const cplx<T>* b) {
+    SMG_REQUIRE(mode >= 0 && mode <= 2 && (mode != 1 || b), "MR update: mode 0, 1 (with b) or 2");
     run_dispatch<T>(N, [&](auto pc) {
         constexpr int P = decltype(pc)::value;
-        k_b_mr_update<T, P><<<b_grid(n * N / P, 1 << 16), kBlock, 0, st>>>(x, r, Ar, s, n * N / P, N);
+        const int g = b_grid(n * N / P, 1 << 16);
+        if (mode == 0) k_b_mr_update<T, P, 0><<<g, kBlock, 0, st>>>(x, r, Ar, b, s, n * N / P, N);
+        else if (mode == 1) k_b_mr_update<T, P, 1><<<g, kBlock, 0, st>>>(x, r, Ar, b, s, n * N / P, N);
+        else k_b_mr_update<T, P, 2><<<g, kBlock, 0, st>>>(x, r, Ar, b, s, n * N / P, N);
     });
     SMG_LAUNCH_CHECK();
 }
@@ -2047,7 +2054,7 @@ void launch_b_accum_mixed(c64* x, const c32* e, size_t n, int N, hipStream_t st)
     template void launch_b_gcr_orth<T>(cplx<T>*, cplx<T>*, const VecPtrs<T>&, const VecPtrs<T>&, int,            \
                                        const double*, const double* const*, size_t, int, hipStream_t);           \
     template void launch_b_mr_update<T>(cplx<T>*, cplx<T>*, const cplx<T>*, const double*, size_t, int,          \
-                                        hipStream_t);                                                            \
+                                        hipStream_t, int, const cplx<T>*);                                       \
     template void launch_b_update<T>(cplx<T>*, cplx<T>*, const cplx<T>*, const cplx<T>*, const double*, size_t,  \
                                      const FinB&, int, hipStream_t);                                             \
     template void launch_b_axpby_real<T>(cplx<T>*, const cplx<T>*, double, double, size_t, hipStream_t);    \

@@ -90,28 +90,38 @@ void CoarseOperator::build_b(LinOpB<double>& A, const Aggregation& P, hipStream_
 }
 
 // ------------------------------------------------------------------ smoothers (batched)
-// MR, src/smoother.cpp:26-40, with the residual carried recursively; per-RHS omega
+// MR, src/smoother.cpp:26-40, with the residual carried recursively; per-RHS omega.
+// From x = 0 the first sweep reads b in place of r = b (no zero fill, no copy); with
+// need_r false (the post-smoothing, whose residual is not read again) the last sweep
+// updates x only.  On return r = b - A x when need_r.
 template <typename T>
 static void mr_b(LinOpB<T>& A, cplx<T>* x, const cplx<T>* b, cplx<T>* r, cplx<T>* Ar, int niter, bool zero_init,
-                 double* w, hipStream_t st) {
+                 bool need_r, double* w, hipStream_t st) {
     const size_t n = A.n(), nN = n * A.N;
+    const FinB f = finb(FIN_MR, w, 2);
+    int it = 0;
     if (zero_init) {
-        launch_zero<T>(x, nN, st);
-        bcopy(r, b, nN, st);
+        if (niter == 0) {
+            launch_zero<T>(x, nN, st);
+            bcopy(r, b, nN, st);
+            return;
+        }
+        A.apply(Ar, b, EPI_DOTS, nullptr, b, f, st);
+        launch_b_mr_update<T>(x, r, Ar, w, n, A.N, st, 1, b);   // x = w b, r = b - w Ar
+        it = 1;
     } else {
         A.apply(r, x, EPI_RESID, b, nullptr, FinB(), st);
     }
-    const FinB f = finb(FIN_MR, w, 2);
-    for (int it = 0; it < niter; it++) {
+    for (; it < niter; it++) {
         A.apply(Ar, r, EPI_DOTS, nullptr, r, f, st);
-        launch_b_mr_update<T>(x, r, Ar, w, n, A.N, st);
+        launch_b_mr_update<T>(x, r, Ar, w, n, A.N, st, it + 1 == niter && !need_r ? 2 : 0);
     }
 }
 
-// Richardson, src/smoother.cpp:63-73; on return r = b - A x
+// Richardson, src/smoother.cpp:63-73; on return r = b - A x when need_r
 template <typename T>
 static void rich_b(LinOpB<T>& A, cplx<T>* x, const cplx<T>* b, cplx<T>* r, int niter, double omega, bool zero_init,
-                   hipStream_t st) {
+                   bool need_r, hipStream_t st) {
     const size_t nN = A.n() * A.N;
     if (zero_init) launch_zero<T>(x, nN, st);
     for (int it = 0; it < niter; it++) {
@@ -119,6 +129,7 @@ static void rich_b(LinOpB<T>& A, cplx<T>* x, const cplx<T>* b, cplx<T>* r, int n
         else A.apply(r, x, EPI_RESID, b, nullptr, FinB(), st);
         launch_b_axpby_real<T>(x, r, omega, 1.0, nN, st);
     }
+    if (!need_r) return;
     if (niter == 0 && zero_init) bcopy(r, b, nN, st);
     else A.apply(r, x, EPI_RESID, b, nullptr, FinB(), st);
 }
@@ -263,8 +274,8 @@ struct CycleB {
             return;
         }
         const double om = mg.omega[lev];
-        if (om > 0) rich_b<T>(A, x, b, rs[lev].p(), mg.cfg.pre, om, true, st);
-        else mr_b<T>(A, x, b, rs[lev].p(), Ars[lev].p(), mg.cfg.pre, true, sc[lev]->p(), st);
+        if (om > 0) rich_b<T>(A, x, b, rs[lev].p(), mg.cfg.pre, om, true, true, st);
+        else mr_b<T>(A, x, b, rs[lev].p(), Ars[lev].p(), mg.cfg.pre, true, true, sc[lev]->p(), st);
         const Aggregation& P = *mg.P[lev];
         launch_b_restrict<T>(P.A, P.bv<T>(), rs[lev].p(), bs[lev + 1].p(), N, st);
         run(lev + 1, bs[lev + 1].p(), xs[lev + 1].p(), st);
@@ -274,8 +285,8 @@ struct CycleB {
             launch_b_axpby_real<T>(xs[lev + 1].p(), e2s[lev + 1].p(), 1.0, 1.0, mg.dims[lev + 1] * N, st);
         }
         launch_b_prolong<T>(P.A, P.bv<T>(), xs[lev + 1].p(), x, 1, N, st);
-        if (om > 0) rich_b<T>(A, x, b, rs[lev].p(), mg.cfg.post, om, false, st);
-        else mr_b<T>(A, x, b, rs[lev].p(), Ars[lev].p(), mg.cfg.post, false, sc[lev]->p(), st);
+        if (om > 0) rich_b<T>(A, x, b, rs[lev].p(), mg.cfg.post, om, false, false, st);
+        else mr_b<T>(A, x, b, rs[lev].p(), Ars[lev].p(), mg.cfg.post, false, false, sc[lev]->p(), st);
     }
 };
 

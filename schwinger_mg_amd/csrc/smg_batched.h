@@ -67,9 +67,11 @@ void launch_b_multi_dot(const VecPtrs<T>& V, int nv, const cplx<T>* w, size_t n,
 template <typename T>
 void launch_b_gcr_orth(cplx<T>* p, cplx<T>* Ap, const VecPtrs<T>& P, const VecPtrs<T>& AP, int nv,
                        const double* dots, const double* const* apap, size_t n, int N, hipStream_t st);
-// x += w_j r ; r -= w_j Ar   (w_j at s[2j..])
+// mode 0: x += w_j r ; r -= w_j Ar   (w_j at s[2j..]); 1: x = w_j b ; r = b - w_j Ar (the
+// first sweep from x = 0); 2: x += w_j r only (r is not read again)
 template <typename T>
-void launch_b_mr_update(cplx<T>* x, cplx<T>* r, const cplx<T>* Ar, const double* s, size_t n, int N, hipStream_t st);
+void launch_b_mr_update(cplx<T>* x, cplx<T>* r, const cplx<T>* Ar, const double* s, size_t n, int N, hipStream_t st,
+                        int mode = 0, const cplx<T>* b = nullptr);
 // x += a_j p ; r -= a_j Ap with {|r_j|^2} -> fin
 template <typename T>
 void launch_b_update(cplx<T>* x, cplx<T>* r, const cplx<T>* p, const cplx<T>* Ap, const double* s, size_t n,
