@@ -69,7 +69,10 @@ def parse():
     ap.add_argument("--post", type=int, default=3)
     ap.add_argument("--coarse-maxit", type=int, default=30)
     ap.add_argument("--coarse-tol", type=float, default=1e-1)
-    ap.add_argument("--coarse-solver", type=int, default=0)
+    ap.add_argument("--coarse-solver", type=int, default=2,
+                    help="0 CG, 1 dense inverse, 2 CG in FP64 and a Chebyshev iteration in the mixed cycle")
+    ap.add_argument("--cheb-steps", type=int, default=40,
+                    help="coarse-solver 2: Chebyshev steps of the mixed cycle's coarsest solve")
     ap.add_argument("--coarse-bf16", type=int, default=1,
                     help="BF16 vectors in the coarsest CG of the FP32 cycle (FP16 links either way)")
     ap.add_argument("--graph", type=int, default=1)
@@ -183,13 +186,15 @@ def time_stencil(smg, torch, D, prec, nrhs, V, csw, launches=50, warm=5):
 PEAKS = {"hbm": 8000.0, "bf16": 2500.0, "f64": 78.6, "f32": 157.3}   # GB/s; TFLOP/s (dense MFMA)
 
 
-def time_coarse(smg, torch, mg, level, prec, nrhs, k, launches=50, warm=5):
+def time_coarse(smg, torch, mg, level, prec, nrhs, k, launches=50, warm=5, cheb=False):
     """Average duration of one launch of the hierarchy's level-`level` coarse operator on
     the batched layout with the fused dot products (the launch shape of the smoother and
-    the coarse CG): a HIP event pair around `launches` back-to-back launches on the
-    library's stream, divided by their number.
+    the coarse CG), or with cheb (BF16 vectors) one step of the Chebyshev coarsest solve:
+    a HIP event pair around `launches` back-to-back launches on the library's stream,
+    divided by their number.
     Algorithmic bytes per launch Vc (9 k^2 s_L + 2 k s_psi N) (links of s_L bytes per
-    complex entry, sources read and results written once); flops 72 k^2 Vc N."""
+    complex entry, sources read and results written once), plus 4 k s_psi N for the
+    Chebyshev step's residual and iterate (each read and written once); flops 72 k^2 Vc N."""
     import ctypes
     n = mg.level_dim(level)
     vc = n // k
@@ -206,7 +211,8 @@ def time_coarse(smg, torch, mg, level, prec, nrhs, k, launches=50, warm=5):
     torch.cuda.synchronize()
 
     def launch():
-        smg._chk(smg.lib().smg_mg_coarse_apply_dev(mg.h, level, prec, nrhs, src.data_ptr(), dst.data_ptr(), 1, sarg))
+        smg._chk(smg.lib().smg_mg_coarse_apply_dev(mg.h, level, prec, nrhs, src.data_ptr(), dst.data_ptr(),
+                                                   2 if cheb else 1, sarg))
 
     for _ in range(warm):
         launch()
@@ -218,10 +224,10 @@ def time_coarse(smg, torch, mg, level, prec, nrhs, k, launches=50, warm=5):
     e1.record(st)
     e1.synchronize()
     us = e0.elapsed_time(e1) / launches * 1e3
-    byt = vc * (9 * k * k * sl + 2 * k * sp * nrhs)
+    byt = vc * (9 * k * k * sl + (6 if cheb else 2) * k * sp * nrhs)
     flop = 72 * k * k * vc * nrhs
-    return {"level": level, "prec": {smg.F64: "f64", smg.F16: "f16 links, f32 vectors",
-                                     smg.BF16V: "f16 links, bf16 vectors"}[prec],
+    return {"level": level, "cheb": cheb, "prec": {smg.F64: "f64", smg.F16: "f16 links, f32 vectors",
+                                                   smg.BF16V: "f16 links, bf16 vectors"}[prec],
             "sites": vc, "k": k, "nrhs": nrhs, "us_per_launch": us, "bytes_per_launch": byt,
             "GBs": byt / (us * 1e-6) / 1e9, "TFLOPs": flop / (us * 1e-6) / 1e12}
 
@@ -399,6 +405,7 @@ def hmc_workload(args):
     # solve than the random-field solve workload
     args.null_iters, args.coarse_maxit, args.coarse_tol = args.hmc_null_iters, args.hmc_coarse_maxit, \
         args.hmc_coarse_tol
+    args.coarse_solver = 0   # the trajectories' solves are single-right-hand-side: coarsest CG
     t0 = time.time()
     mg = build_mg(smg, args, D, L, seed)
     smg.synchronize()
@@ -597,6 +604,13 @@ def kernels_workload(args):
             marker(torch, name, False)
             print(f"{name} ({c['prec']}): {c['us_per_launch']:.1f} us, {c['GBs']:.0f} GB/s, "
                   f"{c['TFLOPs']:.1f} TFLOP/s", flush=True)
+        name = f"coarse_l{lev}_cheb"
+        if lev == args.levels - 1 and args.coarse_solver == 2 and sel in name:
+            marker(torch, name, True)
+            c = time_coarse(smg, torch, mg, lev, smg.BF16V, N, args.k, launches=20, warm=2, cheb=True)
+            marker(torch, name, False)
+            print(f"{name} (Chebyshev step, {c['prec']}): {c['us_per_launch']:.1f} us, {c['GBs']:.0f} GB/s, "
+                  f"{c['TFLOPs']:.1f} TFLOP/s", flush=True)
     a = torch.empty(1 << 25, dtype=torch.float32, device="cuda").uniform_()
     b = torch.empty_like(a)
     for _ in range(3):
@@ -674,11 +688,22 @@ def main():
         torch.cuda.synchronize(); smg.synchronize()
         return (time.perf_counter() - t0) / steps
 
+    cheb = args.coarse_solver == 2
+
+    def cycle_for(mixed_batched):
+        """coarse_solver 2: the batched FP32 cycle's coarsest Chebyshev iteration takes
+        cheb_steps; the FP64 and single-right-hand-side cycles keep CG (coarse_maxit)."""
+        if cheb:
+            mg.set_cycle(args.pre, args.post, args.cheb_steps if mixed_batched else args.coarse_maxit,
+                         args.coarse_tol, use_graph=bool(args.graph))
+
+    cycle_for(True)
     t_mixed = timed(lambda: solve_batch(smg.F32), args.steps, args.warmup, "mixed_batch")
     mixed = {"iterations": info.iterations, "restarts": info.restarts, "max_true_residual": float(tres.max())}
     log(f"mixed x{N}: {t_mixed * 1e3:.1f} ms/batch = {N / t_mixed:.1f} solves/s, {mixed}")
     X_mixed = Xd.cpu().numpy()
     t_f64, fp64, err_vs_f64 = float("nan"), None, None
+    cycle_for(False)
     if args.fp64:
         t_f64 = timed(lambda: solve_batch(smg.F64), args.fp64_steps, 1, "fp64_batch")
         fp64 = {"iterations": info.iterations, "max_true_residual": float(tres.max())}
@@ -710,7 +735,7 @@ def main():
         for lev in range(1, args.levels):
             coarse.append(time_coarse(smg, torch, mg, lev, smg.F16, N, args.k))
         if args.coarse_bf16:
-            coarse.append(time_coarse(smg, torch, mg, args.levels - 1, smg.BF16V, N, args.k))
+            coarse.append(time_coarse(smg, torch, mg, args.levels - 1, smg.BF16V, N, args.k, cheb=cheb))
         if args.fp64:
             for lev in range(1, args.levels):
                 coarse.append(time_coarse(smg, torch, mg, lev, smg.F64, N, args.k))
@@ -746,10 +771,11 @@ def main():
                               profiled_traffic(L, N, "coarse", f"level{c['level']}_f16"),
                               args.pre + args.post + 1, PEAKS["bf16"]))
         elif c["prec"].endswith("bf16 vectors"):
-            lines.append(line(f"k_b_coarse_bf16 (FP16 links, BF16 vectors, level {c['level']} coarsest CG, "
+            what = "Chebyshev step" if c["cheb"] else "CG"
+            lines.append(line(f"k_b_coarse_bf16 (FP16 links, BF16 vectors, level {c['level']} coarsest {what}, "
                               f"{c['sites']} sites, k={c['k']}, {N} RHS)", c,
-                              profiled_traffic(L, N, "coarse", f"level{c['level']}_bf16v"),
-                              args.coarse_maxit, PEAKS["bf16"]))
+                              profiled_traffic(L, N, "coarse", f"level{c['level']}_" + ("cheb" if c["cheb"] else "bf16v")),
+                              args.cheb_steps if c["cheb"] else args.coarse_maxit, PEAKS["bf16"]))
     fp64_lines = [line(f"k_b_coarse_mfma<double> (FP64, level {c['level']}, {c['sites']} sites, k={c['k']}, "
                        f"{N} RHS)", c, profiled_traffic(L, N, "coarse", f"level{c['level']}_f64"),
                        args.coarse_maxit if c["level"] == last else args.pre + args.post + 1,
@@ -774,7 +800,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (FP16 coarse links" + (", BF16 coarsest-CG vectors" if args.coarse_bf16 else "")
+        "dtype": "f32 (FP16 coarse links" + (", BF16 coarsest vectors" if args.coarse_bf16 else "")
                  + ") in f64 refinement",
         "data": "synthetic",
         "config": {"workload": f"L={L} Wilson-clover m={args.mass} c_sw=1, theta~N(0,{args.sigma}^2), {N} RHS, "
@@ -783,11 +809,14 @@ def main():
                                   else f"k={args.k}")
                                + f", {'W' if args.w_cycle else 'V'}-cycle, MR({args.pre},{args.post}), coarsest "
                                + ("dense inverse" if args.coarse_solver == 1
-                                  else f"CG to {args.coarse_tol:g} (<= {args.coarse_maxit} it)")
+                                  else (f"{args.cheb_steps}-step Chebyshev in the mixed cycle, CG to "
+                                        f"{args.coarse_tol:g} (<= {args.coarse_maxit} it) in the FP64 cycle" if cheb
+                                        else f"CG to {args.coarse_tol:g} (<= {args.coarse_maxit} it)"))
                                + f"), GCR({args.trunc}), tol 1e-10 true residual",
                    "L": L, "nrhs": N, "levels": args.levels, "k": args.k, "pre": args.pre, "post": args.post,
                    "coarse_maxit": args.coarse_maxit, "coarse_tol": args.coarse_tol,
-                   "coarse_solver": "dense inverse" if args.coarse_solver == 1 else "CG", "graph": bool(args.graph),
+                   "coarse_solver": {0: "CG", 1: "dense inverse", 2: "Chebyshev (mixed), CG (FP64)"}[args.coarse_solver],
+                   "cheb_steps": args.cheb_steps if cheb else None, "graph": bool(args.graph),
                    "coarsest_cg_vectors": "bf16" if args.coarse_bf16 else "fp32"},
         "mixed": mixed,
         "fp64_solves_per_s": round(N / t_f64, 3) if fp64 else None,

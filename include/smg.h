@@ -197,7 +197,10 @@ typedef struct {
     int coarse_half;       /* FP32 cycle: 1 = coarse links in FP16; 2 = also BF16 vectors in the
                               batched coarsest CG (N >= 16), half its vector traffic */
     int coarse_solver;     /* coarsest level: 0 CG (coarse_maxit, coarse_tol); 1 dense inverse
-                              (formed once at setup; coarsest dimension <= 8192) */
+                              (formed once at setup; coarsest dimension <= 8192); 2 CG in the
+                              FP64 cycles and, in the batched FP32 cycle with coarse_half 2, a
+                              Chebyshev iteration of coarse_maxit steps on the coarsest
+                              operator's spectral interval (Lanczos at setup): no reductions */
     int use_graph;         /* 1: replay the cycle from a captured hipGraph */
     int chiral;            /* 1: chiral aggregation at every level (k[l] = 2 x null vectors) */
     int null_prec;         /* near-null MR sweeps: 0 FP64 (the reference's); 1 FP32 on the FP32
@@ -439,12 +442,24 @@ int smg_coarse_apply_batched(smg_coarse* C, int prec, int nrhs, const double* sr
 int smg_coarse_apply_batched_dev(smg_coarse* C, int prec, int nrhs, const void* src, void* dst, void* stream);
 /* one multigrid cycle applied to N right-hand sides at once (host arrays) */
 int smg_mg_precondition_batched(smg_mg* mg, int prec, int nrhs, const double* B, double* X);
+/* The coarsest-level solve of the batched FP32 cycle on nrhs right-hand sides (host
+ * arrays [nrhs][coarsest dimension], complex128; computed in the cycle's precisions):
+ * SparseCoarseOp::solve, src/coarse_op.h:492-547, as CG (coarse_solver 0; BF16 vectors
+ * with coarse_half 2), or the dense inverse (1), or (2, with coarse_half 2 and nrhs >= 16)
+ * the Chebyshev iteration, which has no reference counterpart */
+int smg_mg_coarsest_batched(smg_mg* mg, int nrhs, const double* B, double* X);
+/* the spectral interval [lo, hi] of a coarse_solver 2 hierarchy's Chebyshev iteration:
+ * the extreme Ritz values of 60 Lanczos steps on the FP64 coarsest operator, hi
+ * widened by 10% (no reference counterpart) */
+int smg_mg_cheb_interval(smg_mg* mg, double* lo, double* hi);
 /* the hierarchy's level-`level` (>= 1) coarse operator on nrhs device vectors in the
  * batched layout (SparseCoarseOp::apply_to, src/coarse_op.h:282-297, as the cycle runs
  * it): prec SMG_F64 / SMG_F32 (links of that precision), SMG_F16 (FP16 links, FP32
  * vectors) or SMG_BF16V (FP16 links, BF16 vectors: one dword per complex entry); with
  * dots = 1 the launch also reduces {<dst, src>, |dst|^2} per right-hand side, as the
- * smoother and the coarse CG do (results discarded) */
+ * smoother and the coarse CG do (results discarded); with dots = 2 (SMG_BF16V) it is one
+ * step of the Chebyshev coarsest solve (src = the direction, dst = the next direction,
+ * the residual and iterate in library scratch; no reference counterpart) */
 int smg_mg_coarse_apply_dev(smg_mg* mg, int level, int prec, int nrhs, const void* src, void* dst, int dots,
                             void* stream);
 /* smg_gcr_solve for N right-hand sides in lockstep (textbook GCR projection);

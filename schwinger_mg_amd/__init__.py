@@ -109,6 +109,7 @@ SIGNATURES = [
     ("smg_coarse_dense", _I, [_P, _P]), ("smg_coarse_dense_solve", _I, [_P, _P, _P]),
     ("smg_coarse_apply_batched_dev", _I, [_P, _I, _I, _P, _P, _P]),
     ("smg_mg_precondition_batched", _I, [_P, _I, _I, _P, _P]),
+    ("smg_mg_coarsest_batched", _I, [_P, _I, _P, _P]), ("smg_mg_cheb_interval", _I, [_P, _P, _P]),
     ("smg_gcr_solve_batched", _I, [_P, _P, _I, _I, _P, _P, _I, _D, _I, _D, _P, _P]),
     ("smg_gcr_solve_batched_dev", _I, [_P, _P, _I, _I, _P, _P, _I, _D, _I, _D, _P, _P, _P]),
     ("smg_force_forward", _I, [_P, _P, _P]), ("smg_force_backward", _I, [_P, _P, _P, _P]),
@@ -676,6 +677,20 @@ class MGHierarchy:
         X = np.zeros_like(B)
         _chk(lib().smg_mg_precondition_batched(self.h, prec, N, _p(B), _p(X)))
         return X
+
+    def coarsest_batched(self, B):
+        """The batched FP32 cycle's coarsest-level solve on N right-hand sides at once;
+        B is (N, coarsest dimension)."""
+        B = _c(B)
+        X = np.zeros_like(B)
+        _chk(lib().smg_mg_coarsest_batched(self.h, B.shape[0], _p(B), _p(X)))
+        return X
+
+    def cheb_interval(self):
+        """(lo, hi): the spectral interval of the Chebyshev coarsest solve (coarse_solver=2)."""
+        lo, hi = _D(), _D()
+        _chk(lib().smg_mg_cheb_interval(self.h, ctypes.byref(lo), ctypes.byref(hi)))
+        return lo.value, hi.value
 
     def omega(self, level):
         out = _D()

@@ -720,6 +720,7 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
     static_assert(std::is_same<TL, ch16>::value, "bf16x3 path: FP16 links");
     static_assert(!HV || EPI != EPI_RESID, "BF16 vectors: no residual epilogue");
     static_assert(HV || EPI != EPI_CG, "the CG epilogue: BF16 vectors");
+    static_assert(HV || EPI != EPI_CHEB, "the Chebyshev epilogue: BF16 vectors");
     static_assert(!HERM || K % 16 == 0, "Hermitian reads: a chunk of 16 columns inside one direction");
     static_assert(K == 8 || K == 16 || K == 32, "k in {8, 16, 32}");
     c32* __restrict__ dst = static_cast<c32*>(dstv);
@@ -729,7 +730,10 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
     const unsigned* __restrict__ srch = static_cast<const unsigned*>(srcv);
     const unsigned* __restrict__ auxh = static_cast<const unsigned*>(auxv);
     const c32* __restrict__ bvec = static_cast<const c32*>(bv);
-    const unsigned* __restrict__ bh2 = static_cast<const unsigned*>(bv);   // EPI_CG: r
+    const unsigned* __restrict__ bh2 = static_cast<const unsigned*>(bv);   // EPI_CG, EPI_CHEB: r
+    // EPI_CHEB updates r (bv) and x (auxv) in place at this wave's own site only
+    unsigned* const rwh = const_cast<unsigned*>(bh2);
+    unsigned* const xwh = const_cast<unsigned*>(auxh);
     constexpr bool RED = EPI == EPI_DOTS || EPI == EPI_CG;
     constexpr int NC = EPI == EPI_CG ? 5 : 3;   // reduced values per RHS
     if (RED && fin.skip && *fin.skip) {   // CG finished for every RHS
@@ -873,7 +877,7 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
         // EPI_CG) at this lane's results, loaded while the last chunks are multiplied
         // (PAIR layout)
         using EpoV = typename std::conditional<HV, uint2, float4>::type;
-        constexpr int NE = EPI == EPI_CG ? 2 : 1;
+        constexpr int NE = EPI == EPI_CG ? 2 : (EPI == EPI_CHEB ? 3 : 1);
         EpoV epo[PAIR ? RT : 1][PAIR ? 4 : 1][PAIR ? NL : 1][NE];
         auto load_epo = [&]() {
             if constexpr (PAIR && EPI != EPI_NONE) {
@@ -887,7 +891,10 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                                 const size_t o = ((size_t)b * K + 16 * r + 4 * hi + v) * N + 32 * u + 2 * lo;
                                 if constexpr (HV) {
                                     epo[r][v][u][0] = *reinterpret_cast<const uint2*>(auxh + o);
-                                    if constexpr (EPI == EPI_CG) epo[r][v][u][1] = *reinterpret_cast<const uint2*>(bh2 + o);
+                                    if constexpr (EPI == EPI_CG || EPI == EPI_CHEB)
+                                        epo[r][v][u][1] = *reinterpret_cast<const uint2*>(bh2 + o);
+                                    if constexpr (EPI == EPI_CHEB)   // the previous direction at this site
+                                        epo[r][v][u][2] = *reinterpret_cast<const uint2*>(srch + o);
                                 } else {
                                     epo[r][v][u][0] = *reinterpret_cast<const float4*>((EPI == EPI_RESID ? bvec : aux) + o);
                                 }
@@ -929,7 +936,29 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                     const size_t row = ((size_t)b * K + ki) * N;
 #pragma unroll
                     for (int u = 0; u < NL; u++) {
-                        if constexpr (PAIR && HV) {
+                        if constexpr (PAIR && HV && EPI == EPI_CHEB) {
+                            // r -= q, d_new = c1 d + c2 r, x += d_new, from the unrounded q
+                            const size_t idx = row + 32 * u + 2 * lo;
+                            const uint2 xw = epo[r][v][u][0], rw = epo[r][v][u][1], dw = epo[r][v][u][2];
+                            const c32 q0{acc[r][2 * u][0][v], acc[r][2 * u][1][v]};
+                            const c32 q1{acc[r][2 * u + 1][0][v], acc[r][2 * u + 1][1][v]};
+                            const c32 r0 = unpack_h(rw.x) - q0, r1 = unpack_h(rw.y) - q1;
+                            const c32 d0 = fin.cheb_c1 * unpack_h(dw.x) + fin.cheb_c2 * r0;
+                            const c32 d1 = fin.cheb_c1 * unpack_h(dw.y) + fin.cheb_c2 * r1;
+                            const c32 x0 = unpack_h(xw.x) + d0, x1 = unpack_h(xw.y) + d1;
+                            *reinterpret_cast<uint2*>(rwh + idx) = uint2{pack_h(r0), pack_h(r1)};
+                            *reinterpret_cast<uint2*>(dsth + idx) = uint2{pack_h(d0), pack_h(d1)};
+                            *reinterpret_cast<uint2*>(xwh + idx) = uint2{pack_h(x0), pack_h(x1)};
+                        } else if constexpr (HV && EPI == EPI_CHEB) {
+                            const size_t idx = row + 16 * u + lo;
+                            const c32 q{acc[r][u][0][v], acc[r][u][1][v]};
+                            const c32 rn = unpack_h(bh2[idx]) - q;
+                            const c32 dn = fin.cheb_c1 * unpack_h(srch[idx]) + fin.cheb_c2 * rn;
+                            const c32 xn = unpack_h(auxh[idx]) + dn;
+                            rwh[idx] = pack_h(rn);
+                            dsth[idx] = pack_h(dn);
+                            xwh[idx] = pack_h(xn);
+                        } else if constexpr (PAIR && HV) {
                             const size_t idx = row + 32 * u + 2 * lo;
                             const unsigned w0 = pack_h(c32{acc[r][2 * u][0][v], acc[r][2 * u][1][v]});
                             const unsigned w1 = pack_h(c32{acc[r][2 * u + 1][0][v], acc[r][2 * u + 1][1][v]});
@@ -1050,6 +1079,8 @@ static void coarse_bf16_v(const CoarseView<TL>& C, void* dst, const void* src, E
         k_b_coarse_bf16<TL, EPI_RESID, K, NT, false, HERM><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
     else if (epi == EPI_CG)
         k_b_coarse_bf16<TL, EPI_CG, K, NT, true, HERM><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
+    else if (epi == EPI_CHEB)
+        k_b_coarse_bf16<TL, EPI_CHEB, K, NT, true, HERM><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
 }
 // The tuner's choice for this shape, else the defaults: the Hermitian reads whenever
 // the links allow them, kRedGridCoarse workgroups with fused dot products.
@@ -1133,12 +1164,18 @@ void launch_b_coarse_apply(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>*
     SMG_LAUNCH_CHECK();
 }
 
-// The coarse apply on BF16 vectors (FP16 links): epi EPI_NONE or EPI_DOTS
+// The coarse apply on BF16 vectors (FP16 links): epi EPI_NONE, EPI_DOTS, EPI_CG or EPI_CHEB
 void launch_b_coarse_apply_h(const CoarseView<ch16>& C, cb16* dst, const cb16* src, Epi epi, const cb16* aux,
                              const FinB& fin, int N, hipStream_t st, const cb16* b) {
     SMG_REQUIRE((C.k == 8 || C.k == 16 || C.k == 32) && (N == 16 || N == 32 || N == 64) && epi != EPI_RESID,
                 "BF16-vector coarse apply: k in {8,16,32}, N in {16,32,64}, no residual epilogue");
     SMG_REQUIRE(epi != EPI_CG || (aux && b), "BF16-vector coarse apply: the CG epilogue takes aux and b");
+    // the Chebyshev step writes r (b) and x (aux) in place and the new direction to dst,
+    // while other workgroups read the old direction (src) at their neighbours
+    SMG_REQUIRE(epi != EPI_CHEB || (aux && b && dst != src && aux != src && b != src && dst != aux && dst != b &&
+                                    aux != b),
+                "BF16-vector coarse apply: the Chebyshev epilogue takes distinct r (b), x (aux), d (src) and "
+                "d_new (dst)");
     auto go = [&](auto kc, auto ntc) {
         constexpr int K = decltype(kc)::value, NT = decltype(ntc)::value;
         coarse_bf16<ch16, K, NT, true>(C, dst, src, epi, b, aux, fin, st);
@@ -1901,6 +1938,25 @@ __global__ __launch_bounds__(kBlock) void k_h_cg_init(const c32* __restrict__ b,
     runs_to_lanes<1, 4>(acc, v, N);
     reduce_finish_b<1, 64>(v, fin, N, gridDim.x, blockIdx.x);
 }
+// The first Chebyshev step from x = 0: r = bf16(b), d = bf16(b / theta), x = d
+__global__ __launch_bounds__(kBlock) void k_h_cheb_init(const c32* __restrict__ b, cb16* __restrict__ r,
+                                                        cb16* __restrict__ d, cb16* __restrict__ x, size_t runs,
+                                                        float inv_theta) {
+    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < runs; t += (size_t)gridDim.x * kBlock) {
+        const size_t i = t * 4;
+        const float4 b0 = *reinterpret_cast<const float4*>(b + i), b1 = *reinterpret_cast<const float4*>(b + i + 2);
+        HRun rv, dv;
+        rv.v[0] = c32{b0.x, b0.y};
+        rv.v[1] = c32{b0.z, b0.w};
+        rv.v[2] = c32{b1.x, b1.y};
+        rv.v[3] = c32{b1.z, b1.w};
+        rv.store(r + i);
+#pragma unroll
+        for (int k = 0; k < 4; k++) dv.v[k] = inv_theta * rv.v[k];
+        const uint4 w = dv.store(d + i);
+        *reinterpret_cast<uint4*>(x + i) = w;
+    }
+}
 // one CG step after the apply q = A p (EPI_CG, FIN_CG_STEP): r -= a q, x += a p,
 // p = r + b p with r as stored (rounded to BF16), a_j at a[2j] and b_j at aux[16j + 6]
 __global__ __launch_bounds__(kBlock) void k_h_cg_step(cb16* __restrict__ x, cb16* __restrict__ p,
@@ -1959,6 +2015,12 @@ void launch_h_from_f32(cb16* out, const c32* in, size_t n, int N, hipStream_t st
 void launch_h_cg_init(const c32* b, cb16* r, cb16* p, cb16* x, size_t n, const FinB& fin, int N, hipStream_t st) {
     SMG_REQUIRE(N >= 16 && N <= 64 && (N & (N - 1)) == 0, "BF16 CG: N in {16, 32, 64}");
     k_h_cg_init<<<b_grid(n * N / 4, kRedGridVec), kBlock, 0, st>>>(b, r, p, x, n * N / 4, fin, N);
+    SMG_LAUNCH_CHECK();
+}
+void launch_h_cheb_init(const c32* b, cb16* r, cb16* d, cb16* x, size_t n, int N, double theta, hipStream_t st) {
+    SMG_REQUIRE(N >= 16 && N <= 64 && (N & (N - 1)) == 0, "BF16 Chebyshev: N in {16, 32, 64}");
+    SMG_REQUIRE(theta > 0.0, "BF16 Chebyshev: the interval's centre must be positive");
+    k_h_cheb_init<<<b_grid(n * N / 4, 1 << 16), kBlock, 0, st>>>(b, r, d, x, n * N / 4, (float)(1.0 / theta));
     SMG_LAUNCH_CHECK();
 }
 void launch_h_cg_step(cb16* x, cb16* p, cb16* r, const cb16* q, const double* a, const double* aux, size_t n, int N,

@@ -608,7 +608,7 @@ MGConfig cfg_of(const smg_mg_params* prm) {
     c.chiral = prm->chiral;
     c.null_prec = prm->null_prec;
     SMG_REQUIRE(c.null_prec == 0 || c.null_prec == 1, "null_prec must be 0 or 1");
-    SMG_REQUIRE(c.coarse_solver == 0 || c.coarse_solver == 1, "coarse_solver must be 0 or 1");
+    SMG_REQUIRE(c.coarse_solver >= 0 && c.coarse_solver <= 2, "coarse_solver must be 0, 1 or 2");
     SMG_REQUIRE(c.n_levels >= 2 && c.n_levels <= 5, "n_levels must be 2..5");
     for (int l = 0; l + 1 < c.n_levels; l++)
         SMG_REQUIRE(c.k[l] >= 1 && c.k[l] <= 64 && c.bx[l] >= 1 && c.by[l] >= 1, "bad level parameters");
@@ -914,6 +914,8 @@ int smg_mg_coarse_apply_dev(smg_mg* mg, int level, int prec, int nrhs, const voi
             fin.part = f.part;
             fin.counter = f.counter;
         }
+        SMG_REQUIRE(dots >= 0 && dots <= 2 && (dots < 2 || prec == SMG_BF16V),
+                    "dots must be 0, 1 or (SMG_BF16V only) 2");
         const Epi epi = dots ? EPI_DOTS : EPI_NONE;
         if (prec == SMG_F64) {
             CoarseLinOpB<double, c64> op(C, nrhs);
@@ -924,6 +926,17 @@ int smg_mg_coarse_apply_dev(smg_mg* mg, int level, int prec, int nrhs, const voi
         } else if (prec == SMG_F16) {
             CoarseLinOpB<float, ch16> op(C, nrhs);
             op.apply((c32*)dst, (const c32*)src, epi, nullptr, (const c32*)src, fin, S());
+        } else if (dots == 2) {
+            // one Chebyshev step: r and x are scratch here (updated in place, discarded)
+            static thread_local std::unique_ptr<DevMem> rx;
+            const size_t bytes = C.n() * nrhs * sizeof(cb16);
+            if (!rx || rx->bytes < 2 * bytes) rx.reset(new DevMem(2 * bytes));
+            FinB fc;
+            fc.cheb_c1 = 0.5f;
+            fc.cheb_c2 = 0.25f;
+            cb16* r = rx->as<cb16>();
+            launch_b_coarse_apply_h(C.view16(), (cb16*)dst, (const cb16*)src, EPI_CHEB, r + C.n() * nrhs, fc, nrhs,
+                                    S(), r);
         } else {
             launch_b_coarse_apply_h(C.view16(), (cb16*)dst, (const cb16*)src, epi, (const cb16*)src, fin, nrhs, S());
         }
@@ -945,6 +958,30 @@ int smg_mg_precondition_batched(smg_mg* mg, int prec, int nrhs, const double* B,
             mg_precondition_b<float>(*mg->mg, b32.p(), x32.p(), nrhs, S());
             launch_convert<double, float>(xb.p(), x32.p(), nN, S());
         }
+        launch_b_unpack<double, double>(x.p(), xb.p(), n, nrhs, S());
+        downc(X, x.p(), nN);
+    });
+}
+int smg_mg_cheb_interval(smg_mg* mg, double* lo, double* hi) {
+    return guard([&] {
+        SMG_REQUIRE(mg && lo && hi, "bad arguments");
+        SMG_REQUIRE(mg->mg->cfg.coarse_solver == 2, "the hierarchy has no Chebyshev coarsest solve (coarse_solver 2)");
+        *lo = mg->mg->cheb_lo;
+        *hi = mg->mg->cheb_hi;
+    });
+}
+int smg_mg_coarsest_batched(smg_mg* mg, int nrhs, const double* B, double* X) {
+    return guard([&] {
+        SMG_REQUIRE(mg && B && X, "bad arguments");
+        check_nrhs(nrhs, 64);
+        MG& m = *mg->mg;
+        const size_t n = m.dims.back(), nN = n * nrhs;
+        Field<double> b = up(B, nN), x(nN), bb(nN), xb(nN);
+        Field<float> b32(nN), x32(nN);
+        launch_b_pack<double, double>(bb.p(), b.p(), n, nrhs, S());
+        launch_convert<float, double>(b32.p(), bb.p(), nN, S());
+        coarsest_solve_b32(m, b32.p(), x32.p(), nrhs, S());
+        launch_convert<double, float>(xb.p(), x32.p(), nN, S());
         launch_b_unpack<double, double>(x.p(), xb.p(), n, nrhs, S());
         downc(X, x.p(), nN);
     });

@@ -29,6 +29,13 @@ struct Eigenpairs {
 Eigenpairs trlm(LinOp<double>& A, int n_ev, int n_kr, int max_restarts, double tol, const c64* start,
                 hipStream_t st);
 
+// Extreme eigenvalue estimates of the Hermitian operator A: the smallest and largest
+// eigenvalues of the tridiagonal of m Lanczos steps (full reorthogonalisation) from
+// the device start vector.  lo bounds the smallest eigenvalue from above, hi the
+// largest from below.  No reference counterpart: they give the spectral interval of
+// the coarsest level's Chebyshev iteration.
+void lanczos_extremes(LinOp<double>& A, int m, const c64* start, double& lo, double& hi, hipStream_t st);
+
 // SparseCoarseOp::solve with deflation vectors (src/coarse_op.h:492-547): x0 =
 // sum_i (v_i^dag b / lambda_i) v_i, then CG; returns the iterations
 int deflated_coarse_cg(LinOp<double>& A, const Eigenpairs& E, const c64* b, c64* x, int maxit, double tol,

@@ -281,6 +281,42 @@ Eigenpairs trlm(LinOp<double>& A, int n_ev, int n_kr, int max_restarts, double t
     return out;
 }
 
+void lanczos_extremes(LinOp<double>& A, int m, const c64* start, double& lo, double& hi, hipStream_t st) {
+    const size_t n = A.n();
+    SMG_REQUIRE(m >= 2, "Lanczos bounds: at least two steps");
+    if ((size_t)m > n) m = (int)n;
+    Field<double> K((size_t)(m + 1) * n), r(n);
+    DevMem coef(64 * (size_t)((m + 1 + 31) / 32) * sizeof(double));
+    auto kv = [&](int j) { return K.p((size_t)j * n); };
+    launch_copy<double>(kv(0), start, n, st);
+    {
+        const double nv = dev_norm(kv(0), n, st);
+        SMG_REQUIRE(nv > 0.0, "Lanczos bounds: zero start vector");
+        launch_axpby<double>(kv(0), kv(0), c64{0.0, 0.0}, c64{1.0 / nv, 0.0}, n, st);
+    }
+    std::vector<double> alpha, beta;
+    for (int j = 0; j < m; j++) {
+        A.apply(r.p(), kv(j), st);
+        alpha.push_back(dev_dot<double>(kv(j), r.p(), n, st).re);
+        axpy_h(r.p(), kv(j), -alpha[j], n, st);
+        if (j > 0) axpy_h(r.p(), kv(j - 1), -beta[j - 1], n, st);
+        cgs2(K.p(), j + 1, r.p(), n, coef, st);
+        beta.push_back(dev_norm(r.p(), n, st));
+        if (j + 1 == m || beta[j] <= 1e-14 * std::abs(alpha[j])) break;   // invariant subspace
+        launch_axpby<double>(kv(j + 1), r.p(), c64{1.0 / beta[j], 0.0}, c64{0.0, 0.0}, n, st);
+    }
+    const int k = (int)alpha.size();
+    std::vector<c64> T((size_t)k * k, c64{0.0, 0.0}), Z((size_t)k * k);
+    std::vector<double> ev(k);
+    for (int i = 0; i < k; i++) {
+        T[(size_t)i * k + i].re = alpha[i];
+        if (i + 1 < k) T[(size_t)i * k + i + 1].re = T[(size_t)(i + 1) * k + i].re = beta[i];
+    }
+    dense_eigh(k, T.data(), ev.data(), Z.data());
+    lo = ev.front();
+    hi = ev.back();
+}
+
 int deflated_coarse_cg(LinOp<double>& A, const Eigenpairs& E, const c64* b, c64* x, int maxit, double tol,
                        hipStream_t st) {
     const size_t n = A.n();

@@ -10,6 +10,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <random>
 
 namespace smg {
 
@@ -638,6 +639,7 @@ MG::MG(const Dirac& d, const MGConfig& c, const c64* starts, bool on_device, boo
         lnd.push_back(k);
     }
     if (cfg.coarse_solver == 1) build_dense_inverse(st);
+    if (cfg.coarse_solver == 2) cheb_bounds(st);
     cyc64.reset(new Cycle<double>(*this, fine64.get()));
     cyc32.reset(new Cycle<float>(*this, fine32.get()));
     SMG_HIP(hipStreamSynchronize(st));
@@ -660,6 +662,7 @@ void MG::galerkin(size_t l, hipStream_t st) {
 void MG::rebuild(hipStream_t st) {
     for (size_t l = 0; l < C.size(); l++) galerkin(l, st);
     if (cfg.coarse_solver == 1) build_dense_inverse(st);
+    if (cfg.coarse_solver == 2) cheb_bounds(st);
     reset_graphs();   // captured cycles are recaptured on their next use
     SMG_HIP(hipStreamSynchronize(st));
 }
@@ -676,6 +679,7 @@ void MG::refresh(int iters, hipStream_t st) {
         cur = ops64.back().get();
     }
     if (cfg.coarse_solver == 1) build_dense_inverse(st);
+    if (cfg.coarse_solver == 2) cheb_bounds(st);
     reset_graphs();
     SMG_HIP(hipStreamSynchronize(st));
 }
@@ -705,6 +709,26 @@ void MG::build_dense_inverse(hipStream_t st) {
     Field<double> prow(n), pcol(n);
     launch_dense_invert(inv64.p(), n, prow.p(), pcol.p(), st);
     launch_convert<float, double>(inv32.p(), inv64.p(), n * n, st);
+}
+
+// The coarsest operator's spectral interval for the Chebyshev coarsest solve: Lanczos on
+// the FP64 Galerkin operator from a fixed pseudo-random start (so the interval, and the
+// Chebyshev coefficients, depend only on the operator)
+void MG::cheb_bounds(hipStream_t st) {
+    const CoarseOperator& Cc = *C.back();
+    CoarseLinOp<double, c64> A(Cc);
+    const size_t n = A.n();
+    std::vector<c64> h(n);
+    std::mt19937_64 rng(0x5eed);
+    std::normal_distribution<double> g;
+    for (auto& v : h) v = c64{g(rng), g(rng)};
+    Field<double> start(n);
+    SMG_HIP(hipMemcpyAsync(start.p(), h.data(), n * sizeof(c64), hipMemcpyHostToDevice, st));
+    double lo = 0.0, hi = 0.0;
+    lanczos_extremes(A, kChebLanczos, start.p(), lo, hi, st);
+    SMG_REQUIRE(lo > 0.0 && hi > lo, "Chebyshev coarsest solve: the coarsest operator is not positive definite");
+    cheb_lo = lo;
+    cheb_hi = kChebMargin * hi;
 }
 
 // src/multigrid.cpp:146-152 with explicit power-iteration start vectors

@@ -212,7 +212,10 @@ struct MGConfig {
     double coarse_tol = 1e-12;
     int smoother = 0;        // 0 MR, 1 Richardson (set by set_symmetric)
     int coarse_half = 0;     // FP32 cycle: FP16 coarse links
-    int coarse_solver = 0;   // coarsest level: 0 CG (coarse_maxit, coarse_tol), 1 dense inverse
+    // coarsest level: 0 CG (coarse_maxit, coarse_tol), 1 dense inverse, 2 CG in the FP64
+    // cycles and, in the batched FP32 cycle with BF16 coarsest vectors (coarse_half 2), a
+    // Chebyshev iteration of coarse_maxit steps on the Lanczos-estimated spectrum
+    int coarse_solver = 0;
     int use_graph = 0;       // replay the cycle from a captured hipGraph
     int chiral = 0;          // chiral aggregation: k[l] = 2 * (null vectors of level l)
     int null_prec = 0;       // near-null MR sweeps: 0 FP64, 1 FP32 (vectors and orthogonalisation FP64)
@@ -258,6 +261,13 @@ struct MG : Preconditioner {
     // dense inverse of the coarsest operator (cfg.coarse_solver == 1)
     Field<double> inv64;
     Field<float> inv32;
+    // spectral interval [cheb_lo, cheb_hi] of the coarsest operator's Chebyshev iteration
+    // (cfg.coarse_solver == 2): the extreme Ritz values of kChebLanczos Lanczos steps,
+    // the upper one widened by kChebMargin so that it bounds the largest eigenvalue
+    double cheb_lo = 0.0, cheb_hi = 0.0;
+    static constexpr int kChebLanczos = 60;
+    static constexpr double kChebMargin = 1.1;
+    void cheb_bounds(hipStream_t st);
     // captured cycles (cfg.use_graph): fixed input/output buffers per precision
     struct Graph {
         hipGraph_t g = nullptr;

@@ -190,6 +190,39 @@ static void cg_h(const CoarseOperator& C, const c32* b, c32* x, cb16* r, cb16* p
     launch_h_to_f32(x, xh, n, N, st);
 }
 
+// The coarsest solve as a Chebyshev iteration on the BF16 vectors (cfg.coarse_solver 2):
+// on the spectral interval [lo, hi] of the coarsest operator, found at setup, the
+// polynomial needs no inner products, so a step has no reduction and no host read, and
+// every step after the first is one coarse apply with its vector updates fused into the
+// epilogue (EPI_CHEB).  The three-term recurrence of Saad, Iterative Methods for Sparse
+// Linear Systems (2nd ed.), Algorithm 12.1, from x = 0:
+//   theta = (hi + lo)/2, delta = (hi - lo)/2, sigma = theta/delta, rho_0 = 1/sigma,
+//   r_0 = b, d_0 = b/theta, x_1 = d_0;
+//   step k >= 1: r_k = r_{k-1} - A d_{k-1}, rho_k = 1/(2 sigma - rho_{k-1}),
+//                d_k = rho_k rho_{k-1} d_{k-1} + (2 rho_k/delta) r_k, x_{k+1} = x_k + d_k.
+// steps coarse applies; the directions alternate between d0 and d1.
+static void cheb_h(const CoarseOperator& C, const c32* b, c32* x, cb16* r, cb16* d0, cb16* d1, cb16* xh, int N,
+                   int steps, double lo, double hi, hipStream_t st) {
+    SMG_REQUIRE(hi > lo && lo > 0.0, "Chebyshev coarsest solve: no spectral interval");
+    const size_t n = C.n();
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    launch_h_cheb_init(b, r, d0, xh, n, N, theta, st);
+    const CoarseView<ch16> v = C.view16();
+    double rho = 1.0 / sigma;
+    cb16* dold = d0;
+    cb16* dnew = d1;
+    for (int k = 1; k <= steps; k++) {
+        const double rho_k = 1.0 / (2.0 * sigma - rho);
+        FinB f;
+        f.cheb_c1 = (float)(rho_k * rho);
+        f.cheb_c2 = (float)(2.0 * rho_k / delta);
+        launch_b_coarse_apply_h(v, dnew, dold, EPI_CHEB, xh, f, N, st, r);
+        std::swap(dold, dnew);
+        rho = rho_k;
+    }
+    launch_h_to_f32(x, xh, n, N, st);
+}
+
 // ------------------------------------------------------------------ cycle (batched)
 template <typename T>
 struct CycleB {
@@ -202,6 +235,7 @@ struct CycleB {
     int nlev;
     Field<T> dwork;   // column-slice partials of the matrix-core dense coarsest product
     bool hcg = false;  // coarsest CG on BF16 vectors (FP32 cycle, cfg.coarse_half == 2)
+    bool hcheb = false;   // coarsest Chebyshev on BF16 vectors (the same, cfg.coarse_solver == 2)
     DevMem hr, hp, hAp, hx;
     // the cycle captured as a hipGraph (cfg.use_graph) on fixed input/output buffers
     hipGraph_t graph = nullptr;
@@ -216,8 +250,10 @@ struct CycleB {
         SMG_REQUIRE(N >= 1 && N <= 64 && (N & (N - 1)) == 0, "batched solves take a power-of-two count <= 64");
         if constexpr (std::is_same<T, float>::value) {
             const int kc = m.C.empty() ? 0 : m.C.back()->k;
-            if (m.cfg.coarse_half == 2 && m.cfg.coarse_solver == 0 && N >= 16 && (kc == 8 || kc == 16 || kc == 32)) {
-                hcg = true;
+            if (m.cfg.coarse_half == 2 && (m.cfg.coarse_solver == 0 || m.cfg.coarse_solver == 2) && N >= 16 &&
+                (kc == 8 || kc == 16 || kc == 32)) {
+                hcg = m.cfg.coarse_solver == 0;
+                hcheb = m.cfg.coarse_solver == 2;
                 const size_t bytes = m.dims[nlev - 1] * N * sizeof(cb16);
                 hr.alloc(bytes);
                 hp.alloc(bytes);
@@ -264,6 +300,9 @@ struct CycleB {
                 if (hcg)
                     cg_h(*mg.C[lev - 1], b, x, hr.as<cb16>(), hp.as<cb16>(), hAp.as<cb16>(), hx.as<cb16>(), N,
                          mg.cfg.coarse_maxit, mg.cfg.coarse_tol, sc[lev]->p(), st);
+                else if (hcheb)
+                    cheb_h(*mg.C[lev - 1], b, x, hr.as<cb16>(), hp.as<cb16>(), hAp.as<cb16>(), hx.as<cb16>(), N,
+                           mg.cfg.coarse_maxit, mg.cheb_lo, mg.cheb_hi, st);
                 else
                     cg_b<T>(A, b, x, rs[lev].p(), ps[lev].p(), Ars[lev].p(), mg.cfg.coarse_maxit, mg.cfg.coarse_tol,
                             sc[lev]->p(), st);
@@ -314,6 +353,10 @@ CycleB<float>& batched_cycle<float>(MG& mg, int N) {
     auto& c = m[{&mg, N}];
     if (!c) c.reset(new CycleB<float>(mg, N));
     return *c;
+}
+void coarsest_solve_b32(MG& mg, const c32* b, c32* x, int N, hipStream_t st) {
+    CycleB<float>& c = batched_cycle<float>(mg, N);
+    c.run(c.nlev - 1, b, x, st);
 }
 void release_batched_cycles(MG* mg) {
     for (auto it = cyc64_map().begin(); it != cyc64_map().end();)
@@ -557,6 +600,15 @@ int tune_mg(MG& mg, int N, hipStream_t st) {
             }, st);
         }
         const bool last = l + 1 == mg.C.size();
+        if (last && mg.cfg.coarse_half == 2 && mg.cfg.coarse_solver == 2) {   // BF16-vector coarsest Chebyshev
+            DevMem hs(nN * sizeof(cb16)), hd(nN * sizeof(cb16)), hr(nN * sizeof(cb16)), hx(nN * sizeof(cb16));
+            for (DevMem* m : {&hs, &hd, &hr, &hx}) launch_zero<float>(m->as<c32>(), nN / 2, st);   // BF16 pairs
+            FinB f;
+            pick(TuneKey{TUNE_COARSE_BF16, C.nx, C.ny, C.k, N, 4 + (int)EPI_CHEB}, herm, [&] {
+                launch_b_coarse_apply_h(C.view16(), hd.as<cb16>(), hs.as<cb16>(), EPI_CHEB, hx.as<cb16>(), f, N, st,
+                                        hr.as<cb16>());
+            }, st);
+        }
         if (last && mg.cfg.coarse_half == 2 && mg.cfg.coarse_solver == 0) {   // BF16-vector coarsest CG
             DevMem hs(nN * sizeof(cb16)), hd(nN * sizeof(cb16));
             SMG_HIP(hipMemsetAsync(hs.p, 0, nN * sizeof(cb16), st));

@@ -49,6 +49,10 @@ template <typename T>
 CycleB<T>& batched_cycle(MG& mg, int N);
 template <typename T>
 void mg_precondition_b(MG& mg, const cplx<T>* b, cplx<T>* x, int N, hipStream_t st);
+// the coarsest-level solve of the batched FP32 cycle on N right-hand sides (FP32
+// batched layout, coarsest dimension): the BF16 CG, the BF16 Chebyshev iteration, FP32
+// CG or the dense inverse, as cfg selects
+void coarsest_solve_b32(MG& mg, const c32* b, c32* x, int N, hipStream_t st);
 // drop the batched cycles built for mg (before mg is destroyed)
 void release_batched_cycles(MG* mg);
 // the autotuner's timing search over the launch shapes of mg's batched cycle for N

@@ -28,6 +28,9 @@ struct FinB {
     // set to 1 by a kernel that returned at entry on skip (the kernel after it in the
     // same iteration then returns too: its coefficients were not refreshed)
     int* late = nullptr;
+    // EPI_CHEB: the step's coefficients d_new = c1 d + c2 r (fixed by the iteration
+    // number and the spectral interval, the same for every right-hand side)
+    float cheb_c1 = 0.0f, cheb_c2 = 0.0f;
 };
 
 // Per-RHS coefficient array views (device): complex a_j at s[2j], s[2j+1].
@@ -116,6 +119,9 @@ void launch_h_cg_init(const c32* b, cb16* r, cb16* p, cb16* x, size_t n, const F
 void launch_h_cg_step(cb16* x, cb16* p, cb16* r, const cb16* q, const double* a, const double* aux, size_t n, int N,
                       const int* late, hipStream_t st);
 void launch_h_to_f32(c32* out, const cb16* in, size_t n, int N, hipStream_t st);
+// the first step of the Chebyshev iteration from x = 0: r = bf16(b), d = bf16(b / theta),
+// x = d; each later step is the coarse apply with EPI_CHEB
+void launch_h_cheb_init(const c32* b, cb16* r, cb16* d, cb16* x, size_t n, int N, double theta, hipStream_t st);
 void launch_h_from_f32(cb16* out, const c32* in, size_t n, int N, hipStream_t st);
 
 }  // namespace smg

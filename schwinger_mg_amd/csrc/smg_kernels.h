@@ -12,6 +12,10 @@ enum Epi : int {
     // dst = A src; reduction {Re<dst,aux>, Im<dst,aux>, Re<dst,b>, Im<dst,b>, |dst|^2} -> Fin
     // (the BF16-vector coarse apply of the coarsest CG: aux = p, b = r)
     EPI_CG = 3,
+    // one fused Chebyshev step of the BF16-vector coarse apply (src = d, the previous
+    // direction, read at the neighbours): q = A d, then at each output r -= q,
+    // dst = c1 d + c2 r (the new direction), aux += dst (x); no reduction
+    EPI_CHEB = 4,
 };
 
 // ---------------------------------------------------------------- Dirac (dirac.hip)
