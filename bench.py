@@ -71,7 +71,7 @@ def parse():
     ap.add_argument("--coarse-tol", type=float, default=1e-1)
     ap.add_argument("--coarse-solver", type=int, default=2,
                     help="0 CG, 1 dense inverse, 2 CG in FP64 and a Chebyshev iteration in the mixed cycle")
-    ap.add_argument("--cheb-steps", type=int, default=40,
+    ap.add_argument("--cheb-steps", type=int, default=32,
                     help="coarse-solver 2: Chebyshev steps of the mixed cycle's coarsest solve")
     ap.add_argument("--coarse-bf16", type=int, default=1,
                     help="BF16 vectors in the coarsest CG of the FP32 cycle (FP16 links either way)")
@@ -128,6 +128,7 @@ def parse():
 
 TRAFFIC_PROFILES = {
     "dslash_f32": os.path.join(ROOT, "profiles", "k_b_dslash_f32_L512_N32_hbm_traffic.md"),
+    "ddagd_f32": os.path.join(ROOT, "profiles", "k_b_ddagd_f32_L512_N32_hbm_traffic.md"),
     "coarse": os.path.join(ROOT, "profiles", "k_b_coarse_L512_N32_hbm_traffic.md"),
 }
 
@@ -150,14 +151,18 @@ def stencil_bytes_per_site(prec, nrhs, csw):
     return nrhs * 2 * 2 * sc + 2 * sc + (sr if csw else 0)
 
 
-def time_stencil(smg, torch, D, prec, nrhs, V, csw, launches=50, warm=5):
+def time_stencil(smg, torch, D, prec, nrhs, V, csw, launches=48, warm=6, same=True, kind=None):
     """Average duration of one Wilson-clover launch on the batched layout (RHS
     innermost, the layout of the solve): a HIP event pair on the library's stream (the
-    stream the kernel runs on) around `launches` back-to-back launches, as the solve
-    issues them, divided by their number."""
+    stream the kernel runs on) around `launches` back-to-back launches, divided by their
+    number.  The launches form a chain over three vectors, each reading the vector the
+    launch before it wrote, as the solve's launches do (its input then comes from HBM, not
+    from the last-level cache that a re-read of one unchanged input would hit); the
+    duration of launches that all read one unchanged vector is reported beside it
+    (same_input_us).  kind: the Dirac kind (default D; D^dag D runs as one fused
+    launch where the tuner chose it)."""
     dt = torch.complex128 if prec == "f64" else torch.complex64
-    src = torch.randn(2 * V, nrhs, dtype=dt, device="cuda")
-    dst = torch.empty_like(src)
+    bufs = [torch.randn(2 * V, nrhs, dtype=dt, device="cuda") for _ in range(3)]
     p = smg.F64 if prec == "f64" else smg.F32
     sp = smg.library_stream()
     st = torch.cuda.ExternalStream(sp)
@@ -165,22 +170,34 @@ def time_stencil(smg, torch, D, prec, nrhs, V, csw, launches=50, warm=5):
     sarg = ctypes.c_void_p(sp)
     torch.cuda.synchronize()
 
-    def launch():
-        smg._chk(smg.lib().smg_dirac_apply_batched_dev(D.h, smg.D_, p, src.data_ptr(), dst.data_ptr(), nrhs, sarg))
+    def launch(i, chain):
+        src, dst = (bufs[i % 3], bufs[(i + 1) % 3]) if chain else (bufs[0], bufs[1])
+        smg._chk(smg.lib().smg_dirac_apply_batched_dev(D.h, smg.D_ if kind is None else kind, p, src.data_ptr(),
+                                                       dst.data_ptr(), nrhs, sarg))
 
-    for _ in range(warm):
-        launch()
-    smg.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(st)
-    for _ in range(launches):
-        launch()
-    e1.record(st)
-    e1.synchronize()
-    ms = e0.elapsed_time(e1) / launches
+    def run(chain):
+        for i in range(warm):
+            launch(i, chain)
+        smg.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for i in range(launches):
+            launch(i, chain)
+        e1.record(st)
+        e1.synchronize()
+        for b in bufs:   # the chain grows its vectors by the operator's norm: rescale
+            b.mul_(1.0 / max(float(b.abs().max()), 1e-30))
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / launches
+
+    ms_same = run(False) if same else None
+    ms = run(True)
     byt = stencil_bytes_per_site(prec, nrhs, csw) * V
-    return {"prec": prec, "nrhs": nrhs, "us_per_launch": ms * 1e3, "bytes_per_launch": byt,
-            "GBs": byt / (ms * 1e-3) / 1e9}
+    out = {"prec": prec, "nrhs": nrhs, "us_per_launch": ms * 1e3, "bytes_per_launch": byt,
+           "GBs": byt / (ms * 1e-3) / 1e9}
+    if same:
+        out.update(same_input_us=ms_same * 1e3, same_input_GBs=byt / (ms_same * 1e-3) / 1e9)
+    return out
 
 
 PEAKS = {"hbm": 8000.0, "bf16": 2500.0, "f64": 78.6, "f32": 157.3}   # GB/s; TFLOP/s (dense MFMA)
@@ -587,13 +604,13 @@ def kernels_workload(args):
         smg.tune_save(args.tune_save)
     sel = args.kernels_select or ""
     for p in ("f32", "f64"):
-        name = f"stencil_{p}"
-        if sel not in name:
-            continue
-        marker(torch, name, True)
-        s = time_stencil(smg, torch, D, p, N, V, True, launches=20, warm=2)
-        marker(torch, name, False)
-        print(f"{name} x{N}: {s['us_per_launch']:.1f} us, {s['GBs']:.0f} GB/s", flush=True)
+        for name, kind in ((f"stencil_{p}", None), (f"ddagd_{p}", smg.DDAGD)):
+            if sel not in name:
+                continue
+            marker(torch, name, True)
+            s = time_stencil(smg, torch, D, p, N, V, True, launches=20, warm=2, same=False, kind=kind)
+            marker(torch, name, False)
+            print(f"{name} x{N}: {s['us_per_launch']:.1f} us, {s['GBs']:.0f} GB/s", flush=True)
     for lev in range(1, args.levels):
         for prec in (smg.F16, smg.F64) + ((smg.BF16V,) if lev == args.levels - 1 else ()):
             name = f"coarse_l{lev}_" + {smg.F16: "f16", smg.F64: "f64", smg.BF16V: "bf16v"}[prec]
@@ -730,6 +747,8 @@ def main():
 
     stencil = [time_stencil(smg, torch, D, p, n, V, True) for p, n in
                [("f32", 1), ("f64", 1), ("f32", N), ("f64", N)]]
+    ddagd = [dict(time_stencil(smg, torch, D, p, N, V, True, same=False, kind=smg.DDAGD), kernel="DdagD")
+             for p in ("f32", "f64")]
     coarse = []
     if N in (16, 32, 64) and args.k in (8, 16, 32):
         for lev in range(1, args.levels):
@@ -758,11 +777,16 @@ def main():
             r["mfma_frac"] = round(t["TFLOPs"] / mfma_peak, 4)
         return r
 
-    # the mixed solve's kernels; launches per V-cycle: the stencil twice per D^dag D
-    # (3 + 3 MR sweeps, the residual and the GCR apply), each coarse level's apply in
-    # its 6 MR sweeps and residual, the coarsest in every CG iteration
-    lines = [line(f"k_b_dslash<float> (Wilson-clover D, {N} RHS, L={L})", stencil[2],
-                  profiled_traffic(L, N, "dslash_f32"), 2 * (args.pre + args.post + 2))]
+    # the mixed solve's kernels; launches per V-cycle: D^dag D (one fused launch, or
+    # two stencil launches) in the 3 + 3 MR sweeps, the residual and the GCR apply, each
+    # coarse level's apply in its 6 MR sweeps and residual, the coarsest in every step
+    fused = N in (16, 32, 64) and L % 16 == 0
+    if fused:
+        lines = [line(f"k_b_ddagd<float> (D^dag D in one launch, {N} RHS, L={L})", ddagd[0],
+                      profiled_traffic(L, N, "ddagd_f32"), args.pre + args.post + 2)]
+    else:
+        lines = [line(f"k_b_dslash<float> (Wilson-clover D, {N} RHS, L={L})", stencil[2],
+                      profiled_traffic(L, N, "dslash_f32"), 2 * (args.pre + args.post + 2))]
     last = args.levels - 1
     for c in coarse:
         if c["prec"].endswith("f32 vectors") and not (c["level"] == last and args.coarse_bf16):
@@ -829,6 +853,7 @@ def main():
         "tuned_shapes": tuned,
         "tuned_table": table,
         "stencil": stencil,
+        "ddagd": ddagd,
         "lattice_batch": lbatch,
         "roofline": roofline,
         "roofline_kernels": lines,

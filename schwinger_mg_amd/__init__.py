@@ -1272,7 +1272,7 @@ def therm_cfg_path(L, beta, n_therm):
 
 
 # ------------------------------------------------------------------ launch-shape autotuner
-TUNE_COARSE_BF16, TUNE_DSLASH_DOTS, TUNE_DSLASH_STORE, TUNE_CG_UPDATE, TUNE_TRANSFER = 0, 1, 2, 3, 4
+TUNE_COARSE_BF16, TUNE_DSLASH_DOTS, TUNE_DSLASH_STORE, TUNE_CG_UPDATE, TUNE_TRANSFER, TUNE_DDAGD = 0, 1, 2, 3, 4, 5
 
 
 def tune_mg(mg, nrhs):

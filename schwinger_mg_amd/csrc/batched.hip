@@ -351,6 +351,193 @@ void launch_b_dslash(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T 
     SMG_LAUNCH_CHECK();
 }
 
+// ----------------------------------------------------------------- fused D^dag D
+// D^dag D x in one launch.  A workgroup takes a kDdTX x kDdTY tile of sites and R
+// right-hand sides.  It forms the intermediate t = D x on the tile and its one-site ring
+// in LDS, with the arithmetic of the two-launch form's first pass (k_b_dslash, g5in =
+// g5out = 0, mu).  Then it forms g5 D(-mu) g5 t on the tile from LDS (the second pass).
+// The intermediate never goes to memory: the launch reads x once (the ring's halo again
+// from L2) and writes the result once, where the two-launch form writes and re-reads t.
+// The ring is recomputed by the neighbouring tiles: 180 of 128 sites for 16 x 8 tiles.
+// A workgroup keeps one group of R right-hand sides for the whole launch, so that its
+// threads' dot-product partials belong to fixed right-hand sides.
+constexpr int kDdTX = 16, kDdTY = 8, kDdRX = kDdTX + 2, kDdRY = kDdTY + 2, kDdRing = kDdRX * kDdRY;
+template <typename T>
+constexpr int ddagd_rhs() { return sizeof(T) == 4 ? 16 : 8; }
+
+template <typename T, int EPI, int P, bool NT>
+__global__ __launch_bounds__(kBlock) void k_b_ddagd(DiracView<T> D, cplx<T>* __restrict__ dst,
+                                                    const cplx<T>* __restrict__ src, T mu,
+                                                    const cplx<T>* __restrict__ bvec,
+                                                    const cplx<T>* __restrict__ aux, FinB fin, int N) {
+    constexpr int R = ddagd_rhs<T>(), RP = R / P;
+    static_assert(kBlock % RP == 0, "a thread's runs must keep their right-hand sides");
+    __shared__ cplx<T> tmp[kDdRing][2][R];
+    const int L = D.L, V = L * L;
+    const int G = N / R, ntx = L / kDdTX, ntile = ntx * (L / kDdTY);
+    const int j0 = (int)(blockIdx.x % G) * R;
+    double acc[3][P];
+#pragma unroll
+    for (int q = 0; q < P; q++) acc[0][q] = acc[1][q] = acc[2][q] = 0.0;
+    for (int tile = blockIdx.x / G; tile < ntile; tile += gridDim.x / G) {
+        const int x0 = (tile % ntx) * kDdTX, y0 = (tile / ntx) * kDdTY;
+        // t = D(mu) x on the tile and its ring
+        for (int i = threadIdx.x; i < kDdRing * RP; i += kBlock) {
+            const int rs = i / RP, run = i % RP;
+            const int x = (x0 - 1 + rs % kDdRX + L) % L, y = (y0 - 1 + rs / kDdRX + L) % L;
+            const int s = x + L * y, j = j0 + run * P;
+            const int sxp = (x + 1 == L) ? s - x : s + 1;
+            const int sxm = (x == 0) ? s + L - 1 : s - 1;
+            const int syp = (y + 1 == L) ? x : s + L;
+            const int sym = (y == 0) ? s + V - L : s - L;
+            const int ss[5] = {s, sxp, sxm, syp, sym};
+            RhsRun<T, P> in[5][2];
+#pragma unroll
+            for (int n = 0; n < 5; n++)
+#pragma unroll
+                for (int sp = 0; sp < 2; sp++) in[n][sp].load(src + (2 * (size_t)ss[n] + sp) * N + j);
+            const cplx<T> u0 = D.U[s], u0b = D.U[sxm], u1 = D.U[V + s], u1b = D.U[V + sym];
+            const T C = D.C ? D.C[s] : (T)0;
+#pragma unroll
+            for (int q = 0; q < P; q++) {
+                cplx<T> h0, h1;
+                hop_b(D.r, u0, u0b, u1, u1b, in[1][0].v[q], in[1][1].v[q], in[2][0].v[q], in[2][1].v[q],
+                      in[3][0].v[q], in[3][1].v[q], in[4][0].v[q], in[4][1].v[q], h0, h1);
+                tmp[rs][0][run * P + q] = mk<T>(D.diag + C, mu) * in[0][0].v[q] + h0;
+                tmp[rs][1][run * P + q] = mk<T>(D.diag - C, -mu) * in[0][1].v[q] + h1;
+            }
+        }
+        __syncthreads();
+        // out = g5 D(-mu) g5 t on the tile
+        for (int i = threadIdx.x; i < kDdTX * kDdTY * RP; i += kBlock) {
+            const int ts = i / RP, run = i % RP;
+            const int lx = ts % kDdTX, ly = ts / kDdTX;
+            const int x = x0 + lx, y = y0 + ly, s = x + L * y, j = j0 + run * P;
+            const int sxm = (x == 0) ? s + L - 1 : s - 1;
+            const int sym = (y == 0) ? s + V - L : s - L;
+            const int c = (lx + 1) + (ly + 1) * kDdRX;
+            const int cs[5] = {c, c + 1, c - 1, c + kDdRX, c - kDdRX};
+            cplx<T> in[5][2][P];
+#pragma unroll
+            for (int n = 0; n < 5; n++)
+#pragma unroll
+                for (int sp = 0; sp < 2; sp++)
+#pragma unroll
+                    for (int q = 0; q < P; q++) in[n][sp][q] = tmp[cs[n]][sp][run * P + q];
+            const cplx<T> u0 = D.U[s], u0b = D.U[sxm], u1 = D.U[V + s], u1b = D.U[V + sym];
+            const T C = D.C ? D.C[s] : (T)0;
+            const size_t i0 = (2 * (size_t)s) * N + j, i1 = i0 + N;
+            RhsRun<T, P> out[2], bb[2], ax[2];
+            if (EPI == EPI_RESID) {
+                bb[0].load(bvec + i0);
+                bb[1].load(bvec + i1);
+            }
+            if (EPI == EPI_DOTS) {
+                ax[0].load(aux + i0);
+                ax[1].load(aux + i1);
+            }
+#pragma unroll
+            for (int q = 0; q < P; q++) {
+                const T g1 = (T)-1;   // gamma5 on the input: lower spin negated
+                cplx<T> h0, h1;
+                hop_b(D.r, u0, u0b, u1, u1b, in[1][0][q], g1 * in[1][1][q], in[2][0][q], g1 * in[2][1][q],
+                      in[3][0][q], g1 * in[3][1][q], in[4][0][q], g1 * in[4][1][q], h0, h1);
+                cplx<T> o0 = mk<T>(D.diag + C, -mu) * in[0][0][q] + h0;
+                cplx<T> o1 = mk<T>(D.diag - C, mu) * (g1 * in[0][1][q]) + h1;
+                o1 = -o1;   // gamma5 on the output
+                if (EPI == EPI_RESID) {
+                    o0 = bb[0].v[q] - o0;
+                    o1 = bb[1].v[q] - o1;
+                }
+                out[0].v[q] = o0;
+                out[1].v[q] = o1;
+                if (EPI == EPI_DOTS) {
+                    const cplx<T> d0 = cdot(o0, ax[0].v[q]), d1 = cdot(o1, ax[1].v[q]);
+                    acc[0][q] += (double)d0.re + (double)d1.re;
+                    acc[1][q] += (double)d0.im + (double)d1.im;
+                    acc[2][q] += (double)norm2(o0) + (double)norm2(o1);
+                }
+            }
+            if constexpr (NT) {
+                out[0].store_nt(dst + i0);
+                out[1].store_nt(dst + i1);
+            } else {
+                out[0].store(dst + i0);
+                out[1].store(dst + i1);
+            }
+        }
+        __syncthreads();   // the next tile overwrites t
+    }
+    if (EPI == EPI_DOTS) {
+        // lanes with equal lane % RP hold the same right-hand sides: fold them, then lane
+        // j < N takes right-hand side j from lane (j - j0) / P of its group (zero outside)
+        const int lane = threadIdx.x & 63;
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int q = 0; q < P; q++)
+                for (int off = RP; off < 64; off <<= 1) acc[c][q] += __shfl_xor(acc[c][q], off, 64);
+        const int jr = lane - j0;
+        const bool mine = lane < N && jr >= 0 && jr < R;
+        double v[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double x = 0.0;
+#pragma unroll
+            for (int q = 0; q < P; q++) {
+                const double y = __shfl(acc[c][q], mine ? jr / P : 0, 64);
+                if (mine && jr % P == q) x = y;
+            }
+            v[c] = x;
+        }
+        reduce_finish_b<3, 64>(v, fin, N, gridDim.x, blockIdx.x);
+    }
+}
+
+template <typename T, int P, bool NT>
+static void ddagd_pn(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T mu, Epi epi, const cplx<T>* b,
+                     const cplx<T>* aux, const FinB& fin, int N, int cap, hipStream_t st) {
+    const int G = N / ddagd_rhs<T>();
+    const int units = (D.L / kDdTX) * (D.L / kDdTY) * G;
+    const int g = std::max(G, std::min(units, cap) / G * G);   // a multiple of G
+    if (epi == EPI_NONE)
+        k_b_ddagd<T, EPI_NONE, P, NT><<<g, kBlock, 0, st>>>(D, dst, src, mu, b, aux, fin, N);
+    else if (epi == EPI_RESID)
+        k_b_ddagd<T, EPI_RESID, P, NT><<<g, kBlock, 0, st>>>(D, dst, src, mu, b, aux, fin, N);
+    else
+        k_b_ddagd<T, EPI_DOTS, P, NT><<<g, kBlock, 0, st>>>(D, dst, src, mu, b, aux, fin, N);
+}
+
+bool b_ddagd_fits(int L, int N, size_t real_bytes) {
+    const int R = real_bytes == 4 ? ddagd_rhs<float>() : ddagd_rhs<double>();
+    return L % kDdTX == 0 && L % kDdTY == 0 && N >= R && N % R == 0;
+}
+
+// The tuner's TUNE_DDAGD choice for this shape: 0 the two launches, 1 + 2 g + nt the
+// fused launch with kTuneGrids[g] workgroups (reducing launches; the others take one
+// workgroup per tile and group) and non-temporal stores when nt.  Without an entry: the
+// fused launch, ordinary stores, kRedGrid workgroups with dot products.  Returns false
+// when the caller runs the two launches.
+template <typename T>
+bool launch_b_ddagd(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T mu, Epi epi, const cplx<T>* b,
+                    const cplx<T>* aux, const FinB& fin, int N, hipStream_t st) {
+    SMG_REQUIRE(epi == EPI_NONE || epi == EPI_RESID || epi == EPI_DOTS, "fused D^dag D: epilogue");
+    if (!b_ddagd_fits(D.L, N, sizeof(T))) return false;
+    const int choice = tune_lookup(TuneKey{TUNE_DDAGD, D.L, N, (int)sizeof(T), (int)epi, 0});
+    if (choice == 0) return false;
+    const bool nt = choice > 0 && ((choice - 1) & 1);
+    const int cap = epi == EPI_DOTS ? (choice > 0 ? kTuneGrids[(choice - 1) >> 1] : kRedGrid) : 1 << 20;
+    constexpr int P = sizeof(T) == 4 ? 2 : 1;
+    if (nt) ddagd_pn<T, P, true>(D, dst, src, mu, epi, b, aux, fin, N, cap, st);
+    else ddagd_pn<T, P, false>(D, dst, src, mu, epi, b, aux, fin, N, cap, st);
+    SMG_LAUNCH_CHECK();
+    return true;
+}
+template bool launch_b_ddagd<double>(const DiracView<double>&, c64*, const c64*, double, Epi, const c64*, const c64*,
+                                     const FinB&, int, hipStream_t);
+template bool launch_b_ddagd<float>(const DiracView<float>&, c32*, const c32*, float, Epi, const c32*, const c32*,
+                                    const FinB&, int, hipStream_t);
+
 // ----------------------------------------------------------------- coarse operator
 // link i of a coarse view in arithmetic type T
 template <typename T, typename TL>

@@ -40,6 +40,7 @@ void DdagDOpB<T>::apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>
                         const FinB& fin, hipStream_t st) {
     const DiracView<T> v = D.view<T>();
     const T mu = (T)D.mu_t;
+    if (launch_b_ddagd<T>(v, dst, src, mu, epi, b, aux, fin, this->N, st)) return;
     launch_b_dslash<T>(v, tmp.p(), src, mu, 0, 0, EPI_NONE, nullptr, nullptr, FinB(), this->N, st);
     launch_b_dslash<T>(v, dst, tmp.p(), -mu, 1, 1, epi, b, aux, fin, this->N, st);
 }
@@ -679,6 +680,28 @@ int tune_mg(MG& mg, int N, hipStream_t st) {
     };
     stencil(0.0f);
     stencil(0.0);
+    // D^dag D per epilogue: the two stencil launches (their choices above) or the fused
+    // launch with ordinary or non-temporal stores (and, with dots, its workgroup count)
+    auto ddagd = [&](auto zero) {
+        using T = decltype(zero);
+        if (!b_ddagd_fits(D.L, N, sizeof(T))) return;
+        DdagDOpB<T> A(D, N);
+        Field<T> src(nf), dst(nf);
+        launch_zero<T>(src.p(), nf, st);
+        for (int epi : {(int)EPI_NONE, (int)EPI_RESID, (int)EPI_DOTS}) {
+            std::vector<int> cands = {0, 1, 2};
+            if (epi == EPI_DOTS)
+                for (int g = 1; g < kTuneNGrids; g++) {
+                    cands.push_back(1 + 2 * g);
+                    cands.push_back(2 + 2 * g);
+                }
+            pick(TuneKey{TUNE_DDAGD, D.L, N, (int)sizeof(T), epi, 0}, cands, [&] {
+                A.apply(dst.p(), src.p(), (Epi)epi, src.p(), src.p(), fin, st);
+            }, st);
+        }
+    };
+    ddagd(0.0f);
+    ddagd(0.0);
     mg.reset_graphs();   // captured cycles hold the old launch shapes
     release_batched_cycles(&mg);
     SMG_HIP(hipStreamSynchronize(st));

@@ -37,6 +37,13 @@ struct FinB {
 template <typename T>
 void launch_b_dslash(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T mu, int g5in, int g5out, Epi epi,
                      const cplx<T>* b, const cplx<T>* aux, const FinB& fin, int N, hipStream_t st);
+// D^dag D in one launch (the tile's intermediate D x kept in LDS); false when the shape
+// does not fit it (L a multiple of 16, N a multiple of 16 in FP32 or 8 in FP64) or the
+// tuner chose the two launches, and the caller runs launch_b_dslash twice
+template <typename T>
+bool launch_b_ddagd(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T mu, Epi epi, const cplx<T>* b,
+                    const cplx<T>* aux, const FinB& fin, int N, hipStream_t st);
+bool b_ddagd_fits(int L, int N, size_t real_bytes);   // the fused launch's shape condition
 template <typename T, typename TL>
 void launch_b_coarse_apply(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b,
                            const cplx<T>* aux, const FinB& fin, int N, hipStream_t st);

@@ -28,7 +28,11 @@ enum TuneFamily : int {
     // of the finer lattice; choice 1: a workgroup per aggregate with its blocks staged in
     // LDS, 0: one work-item per output reading through the caches (the same sums)
     TUNE_TRANSFER = 4,
-    TUNE_FAMILIES = 5,
+    // D^dag D of the batched path; shape (L, N, prec bytes, epi, 0); choice 0: two stencil
+    // launches, 1 + 2 g + nt: one fused launch (kTuneGrids[g] workgroups with dot
+    // products; non-temporal stores when nt)
+    TUNE_DDAGD = 5,
+    TUNE_FAMILIES = 6,
 };
 using TuneKey = std::array<int, 6>;   // family, then five shape words
 

@@ -210,3 +210,81 @@ def test_batched_cycle_graph_replays_equal_eager(smg, half):
     Xg, ig, tg = smg.gcr_solve_batched(D, B, mg, 200, 1e-10, 10, prec=smg.F32)
     assert np.array_equal(Xg, Xe) and ig.iterations == ie.iterations
     assert (tg < 1e-10).all()
+
+
+@pytest.mark.parametrize("L", [32, 48])
+def test_fused_ddagd_equals_two_launches(smg, L):
+    """D^dag D of the batched path in one launch (the tile's D x kept in LDS) equals the
+    two stencil launches entry for entry (the same arithmetic per entry), with ordinary
+    and non-temporal stores, in FP32 and FP64, and the oracle's D^dag D."""
+    import torch
+    banner(f"fused D^dag D L={L}")
+    V = L * L
+    U = random_gauge(L, 0.5, 500 + L)
+    D = smg.DiracOp(L, U, 0.1, 1.0, 1.0, 0.07)
+    Do = O.Dirac(L, U, 0.1, 1.0, 1.0, 0.07)
+    try:
+        for N, prec, dt, nbytes, tol in ((16, smg.F32, torch.complex64, 4, 2e-6), (32, smg.F32, torch.complex64, 4, 2e-6),
+                                         (8, smg.F64, torch.complex128, 8, 1e-12), (32, smg.F64, torch.complex128, 8, 1e-12)):
+            B = gaussian_vecs(N, 2 * V, 501)
+            src = torch.from_numpy(np.ascontiguousarray(B.T)).cuda().to(dt).contiguous()
+            want = np.stack([Do.apply(2, B[j]) for j in range(N)])
+            got = {}
+            for choice in (0, 1, 2):   # two launches; fused; fused with non-temporal stores
+                smg.tune_set(5, (L, N, nbytes, 0, 0), choice)
+                dst = torch.empty_like(src)
+                smg._chk(smg.lib().smg_dirac_apply_batched_dev(D.h, 2, prec, src.data_ptr(), dst.data_ptr(), N,
+                                                               torch.cuda.current_stream().cuda_stream))
+                torch.cuda.synchronize()
+                got[choice] = dst.cpu().numpy().astype(np.complex128).T
+                assert relerr(got[choice], want) < tol, (N, prec, choice)
+            same = np.mean(got[1] == got[0])
+            print(f"N={N} prec={prec}: fused equals the two launches on {100 * same:.3f}% of entries, "
+                  f"max relative difference {relerr(got[1], got[0]):.1e}")
+            assert np.array_equal(got[1], got[2])
+            assert relerr(got[1], got[0]) < (1e-6 if prec == smg.F32 else 1e-14)
+    finally:
+        smg.tune_clear()
+
+
+def test_fused_ddagd_cycle_and_solve(smg):
+    """The batched cycle and the mixed and FP64 batched solves with the fused D^dag D
+    (every epilogue: plain, residual, dot products) against the two-launch form: the
+    same iterations and the 1e-10 true residual, the cycle to roundoff.  The coarsest
+    solve is the dense inverse, so the cycle is a smooth function of the rounding (a
+    coarsest CG's stopping test, in BF16 above all, turns last-bit differences into
+    1e-3 ones)."""
+    banner("fused D^dag D in the batched cycle and solves")
+    L, k, N = 32, 32, 32
+    V = L * L
+    U = random_gauge(L, 0.4, 510)
+    D = smg.DiracOp(L, U, 0.05, 1.0, 1.0, 0.0)
+    dims = [2 * V, (L // 4) ** 2 * k]
+    starts = np.concatenate([gaussian_vecs(k // 2, n, 511 + i).ravel() for i, n in enumerate(dims)])
+    mg = smg.MGHierarchy(D, starts, n_levels=3, blocks=((4, 4), (2, 2)), k=(k, k), null_iters=5, coarse_maxit=30,
+                         coarse_tol=0.1, coarse_half=1, coarse_solver=1, chiral=True, w_cycle=False)
+    B = gaussian_vecs(N, 2 * V, 512)
+    A = O.Dirac(L, U, 0.05, 1.0, 1.0, 0.0).op()
+    out = {}
+    try:
+        for choice in (0, 1):
+            for nbytes in (4, 8):
+                for epi in (0, 1, 2):
+                    smg.tune_set(5, (L, N, nbytes, epi, 0), choice)
+            mg.set_cycle(3, 3, 30, 0.1)   # drop the captured cycles
+            cyc = {p: mg.precondition_batched(B, p) for p in (smg.F32, smg.F64)}
+            sol = {}
+            for p in (smg.F32, smg.F64):
+                X, info, tr = smg.gcr_solve_batched(D, B, mg, 200, 1e-10, 10, prec=p)
+                o = max(np.linalg.norm(B[j] - A(X[j])) / np.linalg.norm(B[j]) for j in range(N))
+                assert o < 1e-10, (choice, p, o)
+                sol[p] = (X, info.iterations)
+            out[choice] = (cyc, sol)
+    finally:
+        smg.tune_clear()
+    (c0, s0), (c1, s1) = out[0], out[1]
+    print(f"cycle FP32 {relerr(c1[smg.F32], c0[smg.F32]):.1e}, FP64 {relerr(c1[smg.F64], c0[smg.F64]):.1e}; "
+          f"iterations {s0[smg.F32][1]}/{s1[smg.F32][1]} (mixed), {s0[smg.F64][1]}/{s1[smg.F64][1]} (FP64)")
+    assert relerr(c1[smg.F32], c0[smg.F32]) < 1e-5 and relerr(c1[smg.F64], c0[smg.F64]) < 1e-12
+    assert s1[smg.F32][1] == s0[smg.F32][1] and s1[smg.F64][1] == s0[smg.F64][1]
+    assert relerr(s1[smg.F64][0], s0[smg.F64][0]) < 1e-9

@@ -43,6 +43,10 @@ def test_refusals(smg, tmp_path):
         smg.tune_set(smg.TUNE_TRANSFER, [512, 512, 128 * 2 + 32, 32, 1], 2)   # choices 0 and 1
     with pytest.raises(smg.SMGError):
         smg.tune_set(smg.TUNE_CG_UPDATE, [131072, 32, 0, 0, 0], 5)         # no sixth grid
+    smg.tune_set(smg.TUNE_DDAGD, [512, 32, 4, 2, 0], 10)                   # fused, fifth grid, NT
+    with pytest.raises(smg.SMGError):
+        smg.tune_set(smg.TUNE_DDAGD, [512, 32, 4, 2, 0], 11)               # no sixth grid
+    smg.tune_clear()
     bad = str(tmp_path / "bad.txt")
     open(bad, "w").write("0 64 64 32 32 2 1\n0 64 64 32 32 x 1\n")
     with pytest.raises(smg.SMGError):
