@@ -65,12 +65,17 @@ def parse():
     ap.add_argument("--chiral", type=int, default=1)
     ap.add_argument("--w-cycle", type=int, default=0)
     ap.add_argument("--null-iters", type=int, default=10)
-    ap.add_argument("--pre", type=int, default=3)
-    ap.add_argument("--post", type=int, default=3)
+    ap.add_argument("--pre", type=int, default=1, help="mixed batched cycle: pre-smoothing steps")
+    ap.add_argument("--post", type=int, default=2, help="mixed batched cycle: post-smoothing steps")
+    ap.add_argument("--mr-pre", type=int, default=3,
+                    help="MR pre-smoothing sweeps of the FP64, single-right-hand-side, lattice-batch and CPU cycles")
+    ap.add_argument("--mr-post", type=int, default=3, help="their MR post-smoothing sweeps")
     ap.add_argument("--coarse-maxit", type=int, default=30)
     ap.add_argument("--coarse-tol", type=float, default=1e-1)
     ap.add_argument("--coarse-solver", type=int, default=2,
                     help="0 CG, 1 dense inverse, 2 CG in FP64 and a Chebyshev iteration in the mixed cycle")
+    ap.add_argument("--cheb-smooth", type=float, default=12.0,
+                    help="Chebyshev smoothing in the mixed batched cycle on [hi / R, hi] (0: MR)")
     ap.add_argument("--cheb-steps", type=int, default=32,
                     help="coarse-solver 2: Chebyshev steps of the mixed cycle's coarsest solve")
     ap.add_argument("--coarse-bf16", type=int, default=1,
@@ -299,8 +304,8 @@ def lattice_batch(smg, torch, gaussian_vec, random_gauge, args):
            "cg_iterations": [int(its.min()), int(its.max())], "max_true_residual": float(tr.max())}
     if not args.lattice_mg:
         return out
-    kw = dict(n_levels=3, blocks=((4, 4), (2, 2)), k=(32, 32), null_iters=args.null_iters, pre_smooth=args.pre,
-              post_smooth=args.post, coarse_maxit=args.coarse_maxit, coarse_tol=args.coarse_tol, chiral=True,
+    kw = dict(n_levels=3, blocks=((4, 4), (2, 2)), k=(32, 32), null_iters=args.null_iters, pre_smooth=args.mr_pre,
+              post_smooth=args.mr_post, coarse_maxit=args.coarse_maxit, coarse_tol=args.coarse_tol, chiral=True,
               coarse_half=bool(args.coarse_bf16))
     starts = []
     for g, L in enumerate(Ls):
@@ -338,7 +343,7 @@ def lattice_batch(smg, torch, gaussian_vec, random_gauge, args):
     cyc_ms = sum(e0.elapsed_time(e1) for e0, e1 in ev) / 10
     out.update({"mg": "3 levels, 4x4 then 2x2 aggregates, 16 near-null vectors (chiral, k=32), FP64 vectors, "
                       f"{'FP16' if args.coarse_bf16 else 'FP64'} coarse links, "
-                      f"MR({args.pre},{args.post}), coarsest CG {args.coarse_maxit} x tol {args.coarse_tol}, "
+                      f"MR({args.mr_pre},{args.mr_post}), coarsest CG {args.coarse_maxit} x tol {args.coarse_tol}, "
                       f"GCR({trunc}), graph-replayed cycle",
                 "mg_setup_s": round(setup_s, 2), "mg_cycle_ms": round(cyc_ms, 3),
                 "mg_gcr_batch_s": round(mdt, 4), "mg_lattice_solves_per_s": round(len(Ls) / mdt, 2),
@@ -363,7 +368,7 @@ def cpu_baseline(args, U, B, mg, threads):
     parts = (np.concatenate([mg.block_vecs(l).ravel() for l in range(nt)]),
              np.concatenate([mg.coarse_links(l).ravel() for l in range(nt)]))
     omg = O.MG(A, L, L, 2, None, n_levels=args.levels, block=blocks, k=(args.k,) * nt,
-               pre=args.pre, post=args.post, w_cycle=bool(args.w_cycle),
+               pre=args.mr_pre, post=args.mr_post, w_cycle=bool(args.w_cycle),
                coarse_solver=1 if args.coarse_solver == 1 else 0,
                coarse_maxit=args.coarse_maxit, coarse_tol=args.coarse_tol, chiral=bool(args.chiral), parts=parts)
     its, tres, dt = [], [], 0.0
@@ -394,7 +399,7 @@ def build_mg(smg, args, D, L, seed):
     starts = np.concatenate([gaussian_vecs(nvec, n, seed + 10 + i).ravel() for i, n in enumerate(dims[:-1])])
     blocks = ((4, 4),) + ((2, 2),) * (args.levels - 2)
     return smg.MGHierarchy(D, starts, n_levels=args.levels, blocks=blocks, k=(args.k,) * (args.levels - 1),
-                           null_iters=args.null_iters, pre_smooth=args.pre, post_smooth=args.post,
+                           null_iters=args.null_iters, pre_smooth=args.mr_pre, post_smooth=args.mr_post,
                            coarse_maxit=args.coarse_maxit, coarse_tol=args.coarse_tol,
                            coarse_half=2 if args.coarse_bf16 else 1,
                            coarse_solver=args.coarse_solver, use_graph=bool(args.graph), chiral=bool(args.chiral),
@@ -708,12 +713,19 @@ def main():
     cheb = args.coarse_solver == 2
 
     def cycle_for(mixed_batched):
-        """coarse_solver 2: the batched FP32 cycle's coarsest Chebyshev iteration takes
-        cheb_steps; the FP64 and single-right-hand-side cycles keep CG (coarse_maxit)."""
-        if cheb:
-            mg.set_cycle(args.pre, args.post, args.cheb_steps if mixed_batched else args.coarse_maxit,
-                         args.coarse_tol, use_graph=bool(args.graph))
+        """The mixed batched cycle: pre / post smoothing steps (Chebyshev with cheb_smooth)
+        and, with coarse_solver 2, the coarsest Chebyshev iteration's cheb_steps; the FP64
+        and single-right-hand-side cycles: MR(mr_pre, mr_post) and CG (coarse_maxit)."""
+        if mixed_batched:
+            mg.set_cycle(args.pre, args.post, args.cheb_steps if cheb else args.coarse_maxit, args.coarse_tol,
+                         use_graph=bool(args.graph))
+        else:
+            mg.set_cycle(args.mr_pre, args.mr_post, args.coarse_maxit, args.coarse_tol, use_graph=bool(args.graph))
 
+    if args.cheb_smooth > 0:
+        smooth_hi = mg.set_cheb_smoother(args.cheb_smooth)
+        log(f"Chebyshev smoothing intervals: " + ", ".join(f"level {l} [{h / args.cheb_smooth:.4g}, {h:.4g}]"
+                                                          for l, h in enumerate(smooth_hi)))
     cycle_for(True)
     t_mixed = timed(lambda: solve_batch(smg.F32), args.steps, args.warmup, "mixed_batch")
     mixed = {"iterations": info.iterations, "restarts": info.restarts, "max_true_residual": float(tres.max())}
@@ -778,8 +790,10 @@ def main():
         return r
 
     # the mixed solve's kernels; launches per V-cycle: D^dag D (one fused launch, or
-    # two stencil launches) in the 3 + 3 MR sweeps, the residual and the GCR apply, each
-    # coarse level's apply in its 6 MR sweeps and residual, the coarsest in every step
+    # two stencil launches) in the pre + post smoothing steps (Chebyshev or MR), the
+    # post-smoothing's residual and the GCR apply, each coarse level's apply in its
+    # smoothing steps and residual, the coarsest in every step.  The Chebyshev steps'
+    # launches carry the vector updates in their epilogues; the lines time the plain apply
     fused = N in (16, 32, 64) and L % 16 == 0
     if fused:
         lines = [line(f"k_b_ddagd<float> (D^dag D in one launch, {N} RHS, L={L})", ddagd[0],
@@ -802,7 +816,7 @@ def main():
                               args.cheb_steps if c["cheb"] else args.coarse_maxit, PEAKS["bf16"]))
     fp64_lines = [line(f"k_b_coarse_mfma<double> (FP64, level {c['level']}, {c['sites']} sites, k={c['k']}, "
                        f"{N} RHS)", c, profiled_traffic(L, N, "coarse", f"level{c['level']}_f64"),
-                       args.coarse_maxit if c["level"] == last else args.pre + args.post + 1,
+                       args.coarse_maxit if c["level"] == last else args.mr_pre + args.mr_post + 1,
                        PEAKS["f64"]) for c in coarse if c["prec"] == "f64"]
     roofline = max(lines, key=lambda r: r["ms_per_cycle"])   # the mixed solve's dominant kernel
 
@@ -831,13 +845,20 @@ def main():
                                f"{args.levels}-level MG (4x4 then 2x2 aggregates, "
                                + (f"{args.k // 2} null vectors split by chirality, k={args.k}" if args.chiral
                                   else f"k={args.k}")
-                               + f", {'W' if args.w_cycle else 'V'}-cycle, MR({args.pre},{args.post}), coarsest "
+                               + f", {'W' if args.w_cycle else 'V'}-cycle, "
+                               + (f"Chebyshev({args.pre},{args.post}) smoothing on [hi/{args.cheb_smooth:g}, hi] in the "
+                                  f"mixed cycle, MR({args.mr_pre},{args.mr_post}) in the FP64 cycle"
+                                  if args.cheb_smooth > 0 else f"MR({args.pre},{args.post}) in the mixed cycle, "
+                                  f"MR({args.mr_pre},{args.mr_post}) in the FP64 cycle")
+                               + ", coarsest "
                                + ("dense inverse" if args.coarse_solver == 1
                                   else (f"{args.cheb_steps}-step Chebyshev in the mixed cycle, CG to "
                                         f"{args.coarse_tol:g} (<= {args.coarse_maxit} it) in the FP64 cycle" if cheb
                                         else f"CG to {args.coarse_tol:g} (<= {args.coarse_maxit} it)"))
                                + f"), GCR({args.trunc}), tol 1e-10 true residual",
                    "L": L, "nrhs": N, "levels": args.levels, "k": args.k, "pre": args.pre, "post": args.post,
+                   "smoother": (f"Chebyshev on [hi/{args.cheb_smooth:g}, hi]" if args.cheb_smooth > 0 else "MR"),
+                   "fp64_smoother": f"MR({args.mr_pre},{args.mr_post})",
                    "coarse_maxit": args.coarse_maxit, "coarse_tol": args.coarse_tol,
                    "coarse_solver": {0: "CG", 1: "dense inverse", 2: "Chebyshev (mixed), CG (FP64)"}[args.coarse_solver],
                    "cheb_steps": args.cheb_steps if cheb else None, "graph": bool(args.graph),

@@ -452,6 +452,17 @@ int smg_mg_coarsest_batched(smg_mg* mg, int nrhs, const double* B, double* X);
  * the extreme Ritz values of 60 Lanczos steps on the FP64 coarsest operator, hi
  * widened by 10% (no reference counterpart) */
 int smg_mg_cheb_interval(smg_mg* mg, double* lo, double* hi);
+/* Chebyshev smoothing in the batched FP32 cycle (ratio > 0; 0 restores MR): every
+ * non-coarsest level smooths with pre_smooth / post_smooth Chebyshev steps on
+ * [hi / ratio, hi], hi = 1.1 x a 40-step power-iteration estimate of the level's largest
+ * eigenvalue (re-estimated by smg_mg_rebuild and smg_mg_refresh); the FP64 and
+ * single-right-hand-side cycles keep MR (src/smoother.cpp:9-40).  fused = 1: each step is
+ * one launch of the level's operator with the vector updates in its epilogue where the
+ * operator has that form (the fine D^dag D with L a multiple of 16 and nrhs of 16; the
+ * FP16-link coarse operators with k in {8, 16, 32}), else the apply and separate
+ * passes; fused = 0: always the latter.  hi (optional, n_levels - 1 entries) receives
+ * each level's upper end.  No reference counterpart. */
+int smg_mg_set_cheb_smoother(smg_mg* mg, double ratio, int fused, double* hi);
 /* the hierarchy's level-`level` (>= 1) coarse operator on nrhs device vectors in the
  * batched layout (SparseCoarseOp::apply_to, src/coarse_op.h:282-297, as the cycle runs
  * it): prec SMG_F64 / SMG_F32 (links of that precision), SMG_F16 (FP16 links, FP32

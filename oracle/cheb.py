@@ -3,8 +3,9 @@
 TEST INFRASTRUCTURE ONLY: imported by tests/ and nowhere else.
 
 No reference counterpart: the reference's coarsest solve is CG
-(SparseCoarseOp::solve, src/coarse_op.h:492-547).  This is the algorithm of the
-library's coarse_solver 2 option, Chebyshev acceleration for a Hermitian
+(SparseCoarseOp::solve, src/coarse_op.h:492-547) and its smoother MR
+(src/smoother.cpp:9-40).  This is the algorithm of the library's coarse_solver 2
+option and of its optional Chebyshev smoother, Chebyshev acceleration for a Hermitian
 positive-definite operator whose spectrum lies in [lo, hi] (Saad, Iterative Methods
 for Sparse Linear Systems, 2nd ed., Algorithm 12.1), from x = 0.
 """
@@ -28,6 +29,24 @@ def cheb_solve(A, b, lo, hi, steps):
         x = x + d
         rho = rho_k
     return x
+
+
+def cheb_smooth(A, b, lo, hi, niter, x0=None):
+    """The batched FP32 cycle's Chebyshev smoother (niter applications of A).  From
+    x = 0: x = cheb_solve(A, b, lo, hi, niter - 1), returned with its residual
+    r = b - A x (pre-smoothing).  From x0: x0 + cheb_solve(A, b - A x0, lo, hi, niter),
+    the post-smoothing's niter steps after its residual; r is then not formed (None).
+    Residual polynomials: T_n((theta - A) / delta) / T_n(sigma) with n = niter from
+    x = 0 and n = niter + 1 (times the residual of x0) from x0."""
+    if x0 is None:
+        if niter == 0:
+            return np.zeros_like(np.asarray(b, dtype=np.complex128)), np.array(b, dtype=np.complex128)
+        x = cheb_solve(A, b, lo, hi, niter - 1)
+        return x, np.asarray(b) - A(x)
+    x0 = np.asarray(x0, dtype=np.complex128)
+    if niter == 0:
+        return x0.copy(), None
+    return x0 + cheb_solve(A, np.asarray(b) - A(x0), lo, hi, niter), None
 
 
 def residual_factor(lam, lo, hi, steps):

@@ -110,6 +110,7 @@ SIGNATURES = [
     ("smg_coarse_apply_batched_dev", _I, [_P, _I, _I, _P, _P, _P]),
     ("smg_mg_precondition_batched", _I, [_P, _I, _I, _P, _P]),
     ("smg_mg_coarsest_batched", _I, [_P, _I, _P, _P]), ("smg_mg_cheb_interval", _I, [_P, _P, _P]),
+    ("smg_mg_set_cheb_smoother", _I, [_P, _D, _I, _P]),
     ("smg_gcr_solve_batched", _I, [_P, _P, _I, _I, _P, _P, _I, _D, _I, _D, _P, _P]),
     ("smg_gcr_solve_batched_dev", _I, [_P, _P, _I, _I, _P, _P, _I, _D, _I, _D, _P, _P, _P]),
     ("smg_force_forward", _I, [_P, _P, _P]), ("smg_force_backward", _I, [_P, _P, _P, _P]),
@@ -685,6 +686,15 @@ class MGHierarchy:
         X = np.zeros_like(B)
         _chk(lib().smg_mg_coarsest_batched(self.h, B.shape[0], _p(B), _p(X)))
         return X
+
+    def set_cheb_smoother(self, ratio, fused=True):
+        """Chebyshev smoothing on [hi / ratio, hi] at every non-coarsest level of the batched
+        FP32 cycle (ratio > 0); 0 restores MR.  fused: each step one launch of the level's
+        operator where it has that form, else the apply and separate vector passes.
+        Returns each non-coarsest level's hi."""
+        hi = np.zeros(self.n_levels - 1)
+        _chk(lib().smg_mg_set_cheb_smoother(self.h, float(ratio), int(bool(fused)), _p(hi)))
+        return hi
 
     def cheb_interval(self):
         """(lo, hi): the spectral interval of the Chebyshev coarsest solve (coarse_solver=2)."""

@@ -427,14 +427,18 @@ __global__ __launch_bounds__(kBlock) void k_b_ddagd(DiracView<T> D, cplx<T>* __r
             const cplx<T> u0 = D.U[s], u0b = D.U[sxm], u1 = D.U[V + s], u1b = D.U[V + sym];
             const T C = D.C ? D.C[s] : (T)0;
             const size_t i0 = (2 * (size_t)s) * N + j, i1 = i0 + N;
-            RhsRun<T, P> out[2], bb[2], ax[2];
-            if (EPI == EPI_RESID) {
+            RhsRun<T, P> out[2], bb[2], ax[2], dd[2];
+            if (EPI == EPI_RESID || EPI == EPI_CHEB) {   // b, or the Chebyshev residual r
                 bb[0].load(bvec + i0);
                 bb[1].load(bvec + i1);
             }
-            if (EPI == EPI_DOTS) {
+            if (EPI == EPI_DOTS || (EPI == EPI_CHEB && fin.cheb_mode != 1)) {   // aux, or the iterate x
                 ax[0].load(aux + i0);
                 ax[1].load(aux + i1);
+            }
+            if (EPI == EPI_CHEB && fin.cheb_mode != 1) {   // the direction at this site
+                dd[0].load(src + i0);
+                dd[1].load(src + i1);
             }
 #pragma unroll
             for (int q = 0; q < P; q++) {
@@ -445,7 +449,7 @@ __global__ __launch_bounds__(kBlock) void k_b_ddagd(DiracView<T> D, cplx<T>* __r
                 cplx<T> o0 = mk<T>(D.diag + C, -mu) * in[0][0][q] + h0;
                 cplx<T> o1 = mk<T>(D.diag - C, mu) * (g1 * in[0][1][q]) + h1;
                 o1 = -o1;   // gamma5 on the output
-                if (EPI == EPI_RESID) {
+                if (EPI == EPI_RESID || EPI == EPI_CHEB) {   // Chebyshev: the new residual r - A d
                     o0 = bb[0].v[q] - o0;
                     o1 = bb[1].v[q] - o1;
                 }
@@ -457,8 +461,30 @@ __global__ __launch_bounds__(kBlock) void k_b_ddagd(DiracView<T> D, cplx<T>* __r
                     acc[1][q] += (double)d0.im + (double)d1.im;
                     acc[2][q] += (double)norm2(o0) + (double)norm2(o1);
                 }
+                if (EPI == EPI_CHEB && fin.cheb_mode != 1) {   // d_new = c1 d + c2 r, x += d_new
+#pragma unroll
+                    for (int sp = 0; sp < 2; sp++) {
+                        dd[sp].v[q] = (T)fin.cheb_c1 * dd[sp].v[q] + (T)fin.cheb_c2 * out[sp].v[q];
+                        ax[sp].v[q] = ax[sp].v[q] + dd[sp].v[q];
+                    }
+                }
             }
-            if constexpr (NT) {
+            if constexpr (EPI == EPI_CHEB) {   // r, d and x; r only; x only (fin.cheb_mode)
+                cplx<T>* const rw = const_cast<cplx<T>*>(bvec);
+                cplx<T>* const xw = const_cast<cplx<T>*>(aux);
+                if (fin.cheb_mode != 2) {
+                    out[0].store(rw + i0);
+                    out[1].store(rw + i1);
+                }
+                if (fin.cheb_mode == 0) {
+                    dd[0].store(dst + i0);
+                    dd[1].store(dst + i1);
+                }
+                if (fin.cheb_mode != 1) {
+                    ax[0].store(xw + i0);
+                    ax[1].store(xw + i1);
+                }
+            } else if constexpr (NT) {
                 out[0].store_nt(dst + i0);
                 out[1].store_nt(dst + i1);
             } else {
@@ -504,6 +530,8 @@ static void ddagd_pn(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T 
         k_b_ddagd<T, EPI_NONE, P, NT><<<g, kBlock, 0, st>>>(D, dst, src, mu, b, aux, fin, N);
     else if (epi == EPI_RESID)
         k_b_ddagd<T, EPI_RESID, P, NT><<<g, kBlock, 0, st>>>(D, dst, src, mu, b, aux, fin, N);
+    else if (epi == EPI_CHEB)
+        k_b_ddagd<T, EPI_CHEB, P, NT><<<g, kBlock, 0, st>>>(D, dst, src, mu, b, aux, fin, N);
     else
         k_b_ddagd<T, EPI_DOTS, P, NT><<<g, kBlock, 0, st>>>(D, dst, src, mu, b, aux, fin, N);
 }
@@ -521,10 +549,14 @@ bool b_ddagd_fits(int L, int N, size_t real_bytes) {
 template <typename T>
 bool launch_b_ddagd(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T mu, Epi epi, const cplx<T>* b,
                     const cplx<T>* aux, const FinB& fin, int N, hipStream_t st) {
-    SMG_REQUIRE(epi == EPI_NONE || epi == EPI_RESID || epi == EPI_DOTS, "fused D^dag D: epilogue");
+    SMG_REQUIRE(epi == EPI_NONE || epi == EPI_RESID || epi == EPI_DOTS || epi == EPI_CHEB, "fused D^dag D: epilogue");
+    // EPI_CHEB: r (b) and x (aux) updated in place, the new direction to dst
+    SMG_REQUIRE(epi != EPI_CHEB || (b && aux && dst != src && dst != b && dst != aux && src != b && src != aux &&
+                                    b != aux),
+                "fused D^dag D: the Chebyshev epilogue takes distinct r, x, d and d_new");
     if (!b_ddagd_fits(D.L, N, sizeof(T))) return false;
     const int choice = tune_lookup(TuneKey{TUNE_DDAGD, D.L, N, (int)sizeof(T), (int)epi, 0});
-    if (choice == 0) return false;
+    if (choice == 0 && epi != EPI_CHEB) return false;   // the Chebyshev step has no two-launch form
     const bool nt = choice > 0 && ((choice - 1) & 1);
     const int cap = epi == EPI_DOTS ? (choice > 0 ? kTuneGrids[(choice - 1) >> 1] : kRedGrid) : 1 << 20;
     constexpr int P = sizeof(T) == 4 ? 2 : 1;
@@ -907,7 +939,6 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
     static_assert(std::is_same<TL, ch16>::value, "bf16x3 path: FP16 links");
     static_assert(!HV || EPI != EPI_RESID, "BF16 vectors: no residual epilogue");
     static_assert(HV || EPI != EPI_CG, "the CG epilogue: BF16 vectors");
-    static_assert(HV || EPI != EPI_CHEB, "the Chebyshev epilogue: BF16 vectors");
     static_assert(!HERM || K % 16 == 0, "Hermitian reads: a chunk of 16 columns inside one direction");
     static_assert(K == 8 || K == 16 || K == 32, "k in {8, 16, 32}");
     c32* __restrict__ dst = static_cast<c32*>(dstv);
@@ -921,6 +952,9 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
     // EPI_CHEB updates r (bv) and x (auxv) in place at this wave's own site only
     unsigned* const rwh = const_cast<unsigned*>(bh2);
     unsigned* const xwh = const_cast<unsigned*>(auxh);
+    c32* const rwf = const_cast<c32*>(bvec);
+    c32* const xwf = const_cast<c32*>(aux);
+    const int cmode = fin.cheb_mode;   // EPI_CHEB: 0 r, d and x; 1 r only; 2 x only
     constexpr bool RED = EPI == EPI_DOTS || EPI == EPI_CG;
     constexpr int NC = EPI == EPI_CG ? 5 : 3;   // reduced values per RHS
     if (RED && fin.skip && *fin.skip) {   // CG finished for every RHS
@@ -1083,7 +1117,12 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                                     if constexpr (EPI == EPI_CHEB)   // the previous direction at this site
                                         epo[r][v][u][2] = *reinterpret_cast<const uint2*>(srch + o);
                                 } else {
-                                    epo[r][v][u][0] = *reinterpret_cast<const float4*>((EPI == EPI_RESID ? bvec : aux) + o);
+                                    if (EPI != EPI_CHEB || cmode != 1)
+                                        epo[r][v][u][0] = *reinterpret_cast<const float4*>((EPI == EPI_RESID ? bvec : aux) + o);
+                                    if constexpr (EPI == EPI_CHEB) {
+                                        epo[r][v][u][1] = *reinterpret_cast<const float4*>(bvec + o);
+                                        if (cmode != 1) epo[r][v][u][2] = *reinterpret_cast<const float4*>(src + o);
+                                    }
                                 }
                             }
                 }
@@ -1136,6 +1175,30 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                             *reinterpret_cast<uint2*>(rwh + idx) = uint2{pack_h(r0), pack_h(r1)};
                             *reinterpret_cast<uint2*>(dsth + idx) = uint2{pack_h(d0), pack_h(d1)};
                             *reinterpret_cast<uint2*>(xwh + idx) = uint2{pack_h(x0), pack_h(x1)};
+                        } else if constexpr (PAIR && !HV && EPI == EPI_CHEB) {
+                            // FP32 vectors: r -= q, then (mode) d_new = c1 d + c2 r, x += d_new
+                            const size_t idx = row + 32 * u + 2 * lo;
+                            const float4 rw = epo[r][v][u][1];
+                            const c32 r0 = c32{rw.x, rw.y} - c32{acc[r][2 * u][0][v], acc[r][2 * u][1][v]};
+                            const c32 r1 = c32{rw.z, rw.w} - c32{acc[r][2 * u + 1][0][v], acc[r][2 * u + 1][1][v]};
+                            if (cmode != 2) *reinterpret_cast<float4*>(rwf + idx) = float4{r0.re, r0.im, r1.re, r1.im};
+                            if (cmode != 1) {
+                                const float4 xw = epo[r][v][u][0], dw = epo[r][v][u][2];
+                                const c32 d0 = fin.cheb_c1 * c32{dw.x, dw.y} + fin.cheb_c2 * r0;
+                                const c32 d1 = fin.cheb_c1 * c32{dw.z, dw.w} + fin.cheb_c2 * r1;
+                                if (cmode == 0) *reinterpret_cast<float4*>(dst + idx) = float4{d0.re, d0.im, d1.re, d1.im};
+                                *reinterpret_cast<float4*>(xwf + idx) =
+                                    float4{xw.x + d0.re, xw.y + d0.im, xw.z + d1.re, xw.w + d1.im};
+                            }
+                        } else if constexpr (!HV && EPI == EPI_CHEB) {
+                            const size_t idx = row + 16 * u + lo;
+                            const c32 rn = bvec[idx] - c32{acc[r][u][0][v], acc[r][u][1][v]};
+                            if (cmode != 2) rwf[idx] = rn;
+                            if (cmode != 1) {
+                                const c32 dn = fin.cheb_c1 * src[idx] + fin.cheb_c2 * rn;
+                                if (cmode == 0) dst[idx] = dn;
+                                xwf[idx] = aux[idx] + dn;
+                            }
                         } else if constexpr (HV && EPI == EPI_CHEB) {
                             const size_t idx = row + 16 * u + lo;
                             const c32 q{acc[r][u][0][v], acc[r][u][1][v]};
@@ -1262,12 +1325,12 @@ static void coarse_bf16_v(const CoarseView<TL>& C, void* dst, const void* src, E
         k_b_coarse_bf16<TL, EPI_NONE, K, NT, HV, HERM><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
     else if (epi == EPI_DOTS)
         k_b_coarse_bf16<TL, EPI_DOTS, K, NT, HV, HERM><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
+    else if (epi == EPI_CHEB)
+        k_b_coarse_bf16<TL, EPI_CHEB, K, NT, HV, HERM><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
     else if constexpr (!HV)
         k_b_coarse_bf16<TL, EPI_RESID, K, NT, false, HERM><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
     else if (epi == EPI_CG)
         k_b_coarse_bf16<TL, EPI_CG, K, NT, true, HERM><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
-    else if (epi == EPI_CHEB)
-        k_b_coarse_bf16<TL, EPI_CHEB, K, NT, true, HERM><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
 }
 // The tuner's choice for this shape, else the defaults: the Hermitian reads whenever
 // the links allow them, kRedGridCoarse workgroups with fused dot products.
@@ -1299,10 +1362,21 @@ static void coarse_mfma(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* sr
         k_b_coarse_mfma<T, TL, EPI_DOTS, K, NT><<<g, kBlock, 0, st>>>(C, dst, src, b, aux, fin);
 }
 
+bool b_coarse_cheb_fits(int k, int N, bool f16_links_f32) {
+    return f16_links_f32 && (k == 8 || k == 16 || k == 32) && (N == 16 || N == 32 || N == 64);
+}
+
 template <typename T, typename TL>
 void launch_b_coarse_apply(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b,
                            const cplx<T>* aux, const FinB& fin, int N, hipStream_t st) {
-    SMG_REQUIRE(epi == EPI_NONE || epi == EPI_RESID || epi == EPI_DOTS, "coarse apply: epilogue");
+    SMG_REQUIRE(epi == EPI_NONE || epi == EPI_RESID || epi == EPI_DOTS || epi == EPI_CHEB, "coarse apply: epilogue");
+    // the Chebyshev step (r, x in b, aux; the direction in src): FP16 links on the matrix cores
+    SMG_REQUIRE(epi != EPI_CHEB || (b_coarse_cheb_fits(C.k, N, std::is_same<T, float>::value &&
+                                                                   std::is_same<TL, ch16>::value) &&
+                                    b && aux && dst != src && dst != b && dst != aux && src != b && src != aux &&
+                                    b != aux),
+                "coarse apply: the Chebyshev epilogue needs FP16 links, FP32 vectors, k in {8,16,32}, "
+                "N in {16,32,64} and distinct r, x, d, d_new");
     if constexpr (std::is_same<T, float>::value && std::is_same<TL, ch16>::value) {
         if ((C.k == 8 || C.k == 16 || C.k == 32) && (N == 16 || N == 32 || N == 64)) {
             // FP16 links: bf16x3 matrix-core path
@@ -2225,6 +2299,34 @@ void launch_h_to_f32(c32* out, const cb16* in, size_t n, int N, hipStream_t st) 
     k_h_to_f32<<<(int)((runs + kBlock - 1) / kBlock), kBlock, 0, st>>>(out, in, runs);
     SMG_LAUNCH_CHECK();
 }
+
+// The first step of Chebyshev smoothing: from x = 0 (zero_init) r = b, d = b / theta,
+// x = d; else (r = b - A x already formed) d = r / theta, x += d
+template <typename T>
+__global__ void k_b_cheb_init(cplx<T>* __restrict__ d, cplx<T>* __restrict__ x, cplx<T>* __restrict__ r,
+                              const cplx<T>* __restrict__ b, size_t items, T inv_theta, int zero_init) {
+    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < items; t += (size_t)gridDim.x * kBlock) {
+        if (zero_init) {
+            const cplx<T> v = b[t];
+            r[t] = v;
+            d[t] = inv_theta * v;
+            x[t] = inv_theta * v;
+        } else {
+            const cplx<T> v = inv_theta * r[t];
+            d[t] = v;
+            x[t] = x[t] + v;
+        }
+    }
+}
+template <typename T>
+void launch_b_cheb_init(cplx<T>* d, cplx<T>* x, cplx<T>* r, const cplx<T>* b, size_t n, double theta, bool zero_init,
+                        hipStream_t st) {
+    SMG_REQUIRE(theta > 0.0, "Chebyshev smoothing: the interval's centre must be positive");
+    k_b_cheb_init<T><<<b_grid(n, 1 << 16), kBlock, 0, st>>>(d, x, r, b, n, (T)(1.0 / theta), zero_init ? 1 : 0);
+    SMG_LAUNCH_CHECK();
+}
+template void launch_b_cheb_init<float>(c32*, c32*, c32*, const c32*, size_t, double, bool, hipStream_t);
+template void launch_b_cheb_init<double>(c64*, c64*, c64*, const c64*, size_t, double, bool, hipStream_t);
 
 template <typename T>
 __global__ void k_b_axpby_real(cplx<T>* __restrict__ y, const cplx<T>* __restrict__ x, T a, T c, size_t items) {

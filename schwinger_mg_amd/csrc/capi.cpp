@@ -962,6 +962,16 @@ int smg_mg_precondition_batched(smg_mg* mg, int prec, int nrhs, const double* B,
         downc(X, x.p(), nN);
     });
 }
+int smg_mg_set_cheb_smoother(smg_mg* mg, double ratio, int fused, double* hi) {
+    return guard([&] {
+        SMG_REQUIRE(mg, "null handle");
+        SMG_REQUIRE(fused == 0 || fused == 1, "fused must be 0 or 1");
+        MG& m = *mg->mg;
+        m.set_cheb_smoother(ratio, fused == 1, S());
+        if (hi)
+            for (int l = 0; l + 1 < m.cfg.n_levels; l++) hi[l] = m.cheb_smooth_hi[l];
+    });
+}
 int smg_mg_cheb_interval(smg_mg* mg, double* lo, double* hi) {
     return guard([&] {
         SMG_REQUIRE(mg && lo && hi, "bad arguments");

@@ -640,6 +640,7 @@ MG::MG(const Dirac& d, const MGConfig& c, const c64* starts, bool on_device, boo
     }
     if (cfg.coarse_solver == 1) build_dense_inverse(st);
     if (cfg.coarse_solver == 2) cheb_bounds(st);
+    if (cheb_smooth_ratio > 0.0) set_cheb_smoother(cheb_smooth_ratio, cheb_smooth_fused, st);
     cyc64.reset(new Cycle<double>(*this, fine64.get()));
     cyc32.reset(new Cycle<float>(*this, fine32.get()));
     SMG_HIP(hipStreamSynchronize(st));
@@ -663,6 +664,7 @@ void MG::rebuild(hipStream_t st) {
     for (size_t l = 0; l < C.size(); l++) galerkin(l, st);
     if (cfg.coarse_solver == 1) build_dense_inverse(st);
     if (cfg.coarse_solver == 2) cheb_bounds(st);
+    if (cheb_smooth_ratio > 0.0) set_cheb_smoother(cheb_smooth_ratio, cheb_smooth_fused, st);
     reset_graphs();   // captured cycles are recaptured on their next use
     SMG_HIP(hipStreamSynchronize(st));
 }
@@ -680,6 +682,7 @@ void MG::refresh(int iters, hipStream_t st) {
     }
     if (cfg.coarse_solver == 1) build_dense_inverse(st);
     if (cfg.coarse_solver == 2) cheb_bounds(st);
+    if (cheb_smooth_ratio > 0.0) set_cheb_smoother(cheb_smooth_ratio, cheb_smooth_fused, st);
     reset_graphs();
     SMG_HIP(hipStreamSynchronize(st));
 }
@@ -729,6 +732,26 @@ void MG::cheb_bounds(hipStream_t st) {
     SMG_REQUIRE(lo > 0.0 && hi > lo, "Chebyshev coarsest solve: the coarsest operator is not positive definite");
     cheb_lo = lo;
     cheb_hi = kChebMargin * hi;
+}
+
+void MG::set_cheb_smoother(double ratio, bool fused, hipStream_t st) {
+    cheb_smooth_ratio = ratio > 0.0 ? ratio : 0.0;
+    cheb_smooth_fused = fused;
+    cheb_smooth_hi.assign(cfg.n_levels, 0.0);
+    if (cheb_smooth_ratio > 0.0) {
+        for (int l = 0; l + 1 < cfg.n_levels; l++) {
+            const size_t n = dims[l];
+            std::vector<c64> h(n);
+            std::mt19937_64 rng(0x5eed + l);
+            std::normal_distribution<double> g;
+            for (auto& v : h) v = c64{g(rng), g(rng)};
+            Field<double> v(n), Av(n);
+            SMG_HIP(hipMemcpyAsync(v.p(), h.data(), n * sizeof(c64), hipMemcpyHostToDevice, st));
+            cheb_smooth_hi[l] = kChebMargin * lambda_max<double>(*cyc64->ops[l], v.p(), Av.p(), 40, st);
+        }
+    }
+    reset_graphs();
+    SMG_HIP(hipStreamSynchronize(st));
 }
 
 // src/multigrid.cpp:146-152 with explicit power-iteration start vectors

@@ -268,6 +268,15 @@ struct MG : Preconditioner {
     static constexpr int kChebLanczos = 60;
     static constexpr double kChebMargin = 1.1;
     void cheb_bounds(hipStream_t st);
+    // Chebyshev smoothing in the batched FP32 cycle (ratio > 0): level l's smoother is the
+    // Chebyshev iteration on [hi_l / ratio, hi_l], hi_l = kChebMargin x the power-iteration
+    // estimate of level l's largest eigenvalue; MR otherwise.  Fused: each step one launch
+    // of the level's operator with the updates in its epilogue, where the operator has
+    // that form; else the apply and separate vector passes
+    double cheb_smooth_ratio = 0.0;
+    bool cheb_smooth_fused = true;
+    std::vector<double> cheb_smooth_hi;
+    void set_cheb_smoother(double ratio, bool fused, hipStream_t st);
     // captured cycles (cfg.use_graph): fixed input/output buffers per precision
     struct Graph {
         hipGraph_t g = nullptr;

@@ -44,6 +44,22 @@ void DdagDOpB<T>::apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>
     launch_b_dslash<T>(v, tmp.p(), src, mu, 0, 0, EPI_NONE, nullptr, nullptr, FinB(), this->N, st);
     launch_b_dslash<T>(v, dst, tmp.p(), -mu, 1, 1, epi, b, aux, fin, this->N, st);
 }
+static FinB cheb_fin(double c1, double c2, int mode) {
+    SMG_REQUIRE(mode >= 0 && mode <= 2, "Chebyshev step: mode");
+    FinB f;
+    f.cheb_c1 = (float)c1;
+    f.cheb_c2 = (float)c2;
+    f.cheb_mode = mode;
+    return f;
+}
+template <typename T>
+void DdagDOpB<T>::cheb_step(cplx<T>* d_new, const cplx<T>* d, cplx<T>* r, cplx<T>* x, double c1, double c2, int mode,
+                            hipStream_t st) {
+    SMG_REQUIRE(fused_cheb(), "Chebyshev step: the lattice or N does not fit the fused D^dag D");
+    const DiracView<T> v = D.view<T>();
+    SMG_REQUIRE(launch_b_ddagd<T>(v, d_new, d, (T)D.mu_t, EPI_CHEB, r, x, cheb_fin(c1, c2, mode), this->N, st),
+                "Chebyshev step: fused D^dag D refused");
+}
 template struct DdagDOpB<double>;
 template struct DdagDOpB<float>;
 
@@ -57,6 +73,12 @@ void CoarseLinOpB<T, TL>::apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const
     const int herm = std::is_same<TL, ch16>::value ? C.herm16 : 0;
     launch_b_coarse_apply<T, TL>(CoarseView<TL>{link, C.nx, C.ny, C.k, herm}, dst, src, epi, b, aux, fin, this->N,
                                  st);
+}
+template <typename T, typename TL>
+void CoarseLinOpB<T, TL>::cheb_step(cplx<T>* d_new, const cplx<T>* d, cplx<T>* r, cplx<T>* x, double c1, double c2,
+                                    int mode, hipStream_t st) {
+    SMG_REQUIRE(fused_cheb(), "Chebyshev step: the coarse operator's fused step needs FP16 links, FP32 vectors");
+    apply(d_new, d, EPI_CHEB, r, x, cheb_fin(c1, c2, mode), st);
 }
 template struct CoarseLinOpB<double, c64>;
 template struct CoarseLinOpB<float, c32>;
@@ -133,6 +155,52 @@ static void rich_b(LinOpB<T>& A, cplx<T>* x, const cplx<T>* b, cplx<T>* r, int n
     if (!need_r) return;
     if (niter == 0 && zero_init) bcopy(r, b, nN, st);
     else A.apply(r, x, EPI_RESID, b, nullptr, FinB(), st);
+}
+
+// Chebyshev smoothing on [lo, hi] (the three-term recurrence of the coarsest Chebyshev
+// solve, cheb_h): niter applications of A.  From x = 0 (zero_init) r starts as b, else
+// r = b - A x; with need_r, r = b - A x on return.
+template <typename T>
+static void cheb_smooth_b(LinOpB<T>& A, cplx<T>* x, const cplx<T>* b, cplx<T>* r, cplx<T>* d, cplx<T>* q, int niter,
+                          bool zero_init, bool need_r, double lo, double hi, bool fused, hipStream_t st) {
+    const size_t nN = A.n() * A.N;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    if (niter > 0 && fused && A.fused_cheb()) {
+        // (r = b - A x,) d_0 = r / theta, x_1 = x_0 + d_0 in one pass; each step one launch
+        // of A with the updates in its epilogue, the directions alternating between d and q
+        if (!zero_init) A.apply(r, x, EPI_RESID, b, nullptr, FinB(), st);
+        launch_b_cheb_init<T>(d, x, r, b, nN, theta, zero_init, st);
+        cplx<T>* dc = d;
+        cplx<T>* dn = q;
+        for (int k = 1; k <= niter; k++) {
+            const double rho_k = 1.0 / (2.0 * sigma - rho);
+            const int mode = k < niter ? 0 : (need_r ? 1 : 2);
+            A.cheb_step(dn, dc, r, x, rho_k * rho, 2.0 * rho_k / delta, mode, st);
+            std::swap(dc, dn);
+            rho = rho_k;
+        }
+        return;
+    }
+    if (zero_init) {
+        launch_zero<T>(x, nN, st);
+        bcopy(r, b, nN, st);
+    } else {
+        A.apply(r, x, EPI_RESID, b, nullptr, FinB(), st);
+    }
+    if (niter == 0) return;
+    launch_zero<T>(d, nN, st);
+    launch_b_axpby_real<T>(d, r, 1.0 / theta, 1.0, nN, st);   // d_0 = r / theta
+    launch_b_axpby_real<T>(x, d, 1.0, 1.0, nN, st);           // x_1 = x_0 + d_0
+    for (int k = 1; k <= niter; k++) {
+        A.apply(q, d, st);
+        launch_b_axpby_real<T>(r, q, -1.0, 1.0, nN, st);      // r_k = r_{k-1} - A d_{k-1}
+        if (k == niter && need_r) break;
+        const double rho_k = 1.0 / (2.0 * sigma - rho);
+        launch_b_axpby_real<T>(d, r, 2.0 * rho_k / delta, rho_k * rho, nN, st);
+        launch_b_axpby_real<T>(x, d, 1.0, 1.0, nN, st);
+        rho = rho_k;
+    }
 }
 
 // CG per RHS (src/coarse_op.h:492-547 without deflation); converged RHS take zero steps
@@ -231,7 +299,7 @@ struct CycleB {
     int N;
     std::vector<LinOpB<T>*> ops;
     std::vector<std::unique_ptr<LinOpB<T>>> own;
-    std::vector<Field<T>> xs, bs, rs, Ars, ts, e2s, ps;
+    std::vector<Field<T>> xs, bs, rs, Ars, ts, e2s, ps, cds;   // cds: Chebyshev smoother directions
     std::vector<std::unique_ptr<ScalarsB>> sc;
     int nlev;
     Field<T> dwork;   // column-slice partials of the matrix-core dense coarsest product
@@ -283,6 +351,7 @@ struct CycleB {
             ts.emplace_back(l == 0 ? 1 : nN);
             e2s.emplace_back(l == 0 ? 1 : nN);
             ps.emplace_back(last ? nN : 1);
+            cds.emplace_back(!last && std::is_same<T, float>::value && m.cheb_smooth_ratio > 0.0 ? nN : 1);
             sc.emplace_back(new ScalarsB(8 * 64 + 16 * 64));
         }
     }
@@ -314,7 +383,11 @@ struct CycleB {
             return;
         }
         const double om = mg.omega[lev];
-        if (om > 0) rich_b<T>(A, x, b, rs[lev].p(), mg.cfg.pre, om, true, true, st);
+        const bool cheb = std::is_same<T, float>::value && mg.cheb_smooth_ratio > 0.0;
+        const double chi = cheb ? mg.cheb_smooth_hi[lev] : 0.0, clo = cheb ? chi / mg.cheb_smooth_ratio : 0.0;
+        if (cheb) cheb_smooth_b<T>(A, x, b, rs[lev].p(), cds[lev].p(), Ars[lev].p(), mg.cfg.pre, true, true, clo, chi,
+                                    mg.cheb_smooth_fused, st);
+        else if (om > 0) rich_b<T>(A, x, b, rs[lev].p(), mg.cfg.pre, om, true, true, st);
         else mr_b<T>(A, x, b, rs[lev].p(), Ars[lev].p(), mg.cfg.pre, true, true, sc[lev]->p(), st);
         const Aggregation& P = *mg.P[lev];
         launch_b_restrict<T>(P.A, P.bv<T>(), rs[lev].p(), bs[lev + 1].p(), N, st);
@@ -325,7 +398,9 @@ struct CycleB {
             launch_b_axpby_real<T>(xs[lev + 1].p(), e2s[lev + 1].p(), 1.0, 1.0, mg.dims[lev + 1] * N, st);
         }
         launch_b_prolong<T>(P.A, P.bv<T>(), xs[lev + 1].p(), x, 1, N, st);
-        if (om > 0) rich_b<T>(A, x, b, rs[lev].p(), mg.cfg.post, om, false, false, st);
+        if (cheb) cheb_smooth_b<T>(A, x, b, rs[lev].p(), cds[lev].p(), Ars[lev].p(), mg.cfg.post, false, false, clo,
+                                    chi, mg.cheb_smooth_fused, st);
+        else if (om > 0) rich_b<T>(A, x, b, rs[lev].p(), mg.cfg.post, om, false, false, st);
         else mr_b<T>(A, x, b, rs[lev].p(), Ars[lev].p(), mg.cfg.post, false, false, sc[lev]->p(), st);
     }
 };

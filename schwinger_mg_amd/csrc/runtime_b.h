@@ -17,6 +17,14 @@ struct LinOpB {
     void apply(cplx<T>* dst, const cplx<T>* src, hipStream_t st) {
         apply(dst, src, EPI_NONE, nullptr, nullptr, FinB(), st);
     }
+    // whether cheb_step runs as one launch for this operator and N
+    virtual bool fused_cheb() const { return false; }
+    // one Chebyshev step r -= A d, d_new = c1 d + c2 r, x += d_new in the apply's
+    // epilogue; mode 0 writes r, d_new and x, 1 r only, 2 x only
+    virtual void cheb_step(cplx<T>* d_new, const cplx<T>* d, cplx<T>* r, cplx<T>* x, double c1, double c2, int mode,
+                           hipStream_t st) {
+        SMG_REQUIRE(false, "Chebyshev step: no fused form for this operator");
+    }
 };
 
 template <typename T>
@@ -27,6 +35,9 @@ struct DdagDOpB : LinOpB<T> {
     size_t n() const override { return 2 * (size_t)D.V; }
     void apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux, const FinB& fin,
                hipStream_t st) override;
+    bool fused_cheb() const override { return b_ddagd_fits(D.L, this->N, sizeof(T)); }
+    void cheb_step(cplx<T>* d_new, const cplx<T>* d, cplx<T>* r, cplx<T>* x, double c1, double c2, int mode,
+                   hipStream_t st) override;
     using LinOpB<T>::apply;
 };
 
@@ -37,6 +48,11 @@ struct CoarseLinOpB : LinOpB<T> {
     size_t n() const override { return C.n(); }
     void apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux, const FinB& fin,
                hipStream_t st) override;
+    bool fused_cheb() const override {
+        return b_coarse_cheb_fits(C.k, this->N, std::is_same<T, float>::value && std::is_same<TL, ch16>::value);
+    }
+    void cheb_step(cplx<T>* d_new, const cplx<T>* d, cplx<T>* r, cplx<T>* x, double c1, double c2, int mode,
+                   hipStream_t st) override;
     using LinOpB<T>::apply;
 };
 

@@ -29,8 +29,11 @@ struct FinB {
     // same iteration then returns too: its coefficients were not refreshed)
     int* late = nullptr;
     // EPI_CHEB: the step's coefficients d_new = c1 d + c2 r (fixed by the iteration
-    // number and the spectral interval, the same for every right-hand side)
+    // number and the spectral interval, the same for every right-hand side); what the
+    // step writes (FP32 vectors): 0 r, d_new and x; 1 r only (the last step of a smoothing
+    // that returns its residual); 2 x only (the last step of one that does not)
     float cheb_c1 = 0.0f, cheb_c2 = 0.0f;
+    int cheb_mode = 0;
 };
 
 // Per-RHS coefficient array views (device): complex a_j at s[2j], s[2j+1].
@@ -44,6 +47,9 @@ template <typename T>
 bool launch_b_ddagd(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T mu, Epi epi, const cplx<T>* b,
                     const cplx<T>* aux, const FinB& fin, int N, hipStream_t st);
 bool b_ddagd_fits(int L, int N, size_t real_bytes);   // the fused launch's shape condition
+// whether launch_b_coarse_apply takes EPI_CHEB (FP16 links with FP32 vectors on the
+// matrix cores: k in {8, 16, 32}, N in {16, 32, 64})
+bool b_coarse_cheb_fits(int k, int N, bool f16_links_f32);
 template <typename T, typename TL>
 void launch_b_coarse_apply(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b,
                            const cplx<T>* aux, const FinB& fin, int N, hipStream_t st);
@@ -101,6 +107,11 @@ void launch_b_cg_r(cplx<T>* r, const cplx<T>* Ap, const double* a, size_t n, con
 template <typename T>
 void launch_b_cg_xp(cplx<T>* x, cplx<T>* p, const cplx<T>* r, const double* a, const double* b, size_t n, int N,
                     const int* late, hipStream_t st);
+// the first step of Chebyshev smoothing on n entries (all RHS): from x = 0 (zero_init)
+// r = b, d = b / theta, x = d; else d = r / theta, x += d
+template <typename T>
+void launch_b_cheb_init(cplx<T>* d, cplx<T>* x, cplx<T>* r, const cplx<T>* b, size_t n, double theta, bool zero_init,
+                        hipStream_t st);
 // y = a x + c y with one real scalar pair for all RHS
 template <typename T>
 void launch_b_axpby_real(cplx<T>* y, const cplx<T>* x, double a, double c, size_t n, hipStream_t st);

@@ -5,7 +5,7 @@ operator it is checked on equals the oracle's dense Galerkin product."""
 import numpy as np
 
 import oracle as O
-from oracle.cheb import cheb_solve, links_to_dense, residual_factor
+from oracle.cheb import cheb_smooth, cheb_solve, links_to_dense, residual_factor
 from schwinger_mg_amd.fields import gaussian_vec, gaussian_vecs, random_gauge
 
 
@@ -33,6 +33,29 @@ def test_cheb_residual_is_the_chebyshev_polynomial():
     # steps = 0 is x = b / theta
     x0 = cheb_solve(lambda v: lam * v, b, lo, hi, 0)
     assert np.allclose(x0, b / (0.5 * (lo + hi)), rtol=1e-15, atol=0)
+
+
+def test_cheb_smoother_residual_polynomials():
+    """The smoother on a diagonal operator: from x = 0 its residual (returned, and
+    b - A x) is T_n((theta - A)/delta)/T_n(sigma) b with n = niter; from x0 the new
+    residual is the same with n = niter + 1 times b - A x0.  T_n from numpy's basis."""
+    rng = np.random.default_rng(7)
+    lo, hi = 0.3, 6.0
+    lam = np.concatenate([rng.uniform(0.01, hi, 100), [lo, hi]])
+    b = rng.normal(size=lam.size) + 1j * rng.normal(size=lam.size)
+    x0 = rng.normal(size=lam.size) + 1j * rng.normal(size=lam.size)
+    A = lambda v: lam * v
+    for niter in (1, 2, 3, 8):
+        x, r = cheb_smooth(A, b, lo, hi, niter)
+        want = residual_factor(lam, lo, hi, niter - 1) * b
+        assert np.abs(r - want).max() < 1e-10 * np.abs(b).max(), niter
+        assert np.abs((b - lam * x) - want).max() < 1e-10 * np.abs(b).max(), niter
+        x1, r1 = cheb_smooth(A, b, lo, hi, niter, x0)
+        assert r1 is None
+        want1 = residual_factor(lam, lo, hi, niter) * (b - lam * x0)
+        assert np.abs((b - lam * x1) - want1).max() < 1e-10 * np.abs(b - lam * x0).max(), niter
+    x, r = cheb_smooth(A, b, lo, hi, 0)
+    assert not np.any(x) and np.array_equal(r, b)
 
 
 def test_cheb_on_a_dense_hermitian_operator():

@@ -1,0 +1,144 @@
+"""The Chebyshev smoother of the batched FP32 cycle (set_cheb_smoother): the cycle
+against a numpy cycle built from the hierarchy's own block vectors and Galerkin links
+with the oracle's fine operator and the oracle's Chebyshev restatement; the one-launch
+steps (fine D^dag D and FP16-link coarse apply with the updates in their epilogues)
+against the apply plus separate vector passes; graph replays; mixed solves that reach
+the 1e-10 true residual with it."""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import banner, relerr
+from oracle.cheb import cheb_smooth, links_to_dense
+from schwinger_mg_amd.fields import gaussian_vecs, random_gauge
+
+pytestmark = pytest.mark.gpu
+
+MASS, R, CSW = 0.05, 1.0, 1.0
+RATIO = 8.0
+
+
+def _hierarchy(smg, L, levels, k, seed, coarse_half, coarse_solver=1, coarse_maxit=30):
+    V = L * L
+    U = random_gauge(L, 0.4, seed)
+    D = smg.DiracOp(L, U, MASS, R, CSW, 0.0)
+    dims = [2 * V, (L // 4) ** 2 * k]
+    starts = np.concatenate([gaussian_vecs(k // 2, n, seed + 1 + i).ravel() for i, n in enumerate(dims[:levels - 1])])
+    blocks = ((4, 4), (2, 2))[:levels - 1]
+    mg = smg.MGHierarchy(D, starts, n_levels=levels, blocks=blocks, k=(k,) * (levels - 1), null_iters=5,
+                         coarse_maxit=coarse_maxit, coarse_tol=0.1, coarse_solver=coarse_solver,
+                         coarse_half=coarse_half, chiral=True, w_cycle=False)
+    return U, D, mg
+
+
+def _numpy_cycle(mg, U, L, levels, k, hi, pre=3, post=3):
+    """The V-cycle with Chebyshev smoothing on [hi_l / RATIO, hi_l] in FP64: the oracle's
+    fine operator, the hierarchy's block vectors in the oracle's transfers, dense
+    operators from its Galerkin links, the coarsest level solved exactly."""
+    A0 = O.Dirac(L, U, MASS, R, CSW, 0.0).op()
+    ops, Ps = [A0], []
+    nx, nd = L, 2
+    bl = ((4, 4), (2, 2))
+    for l in range(levels - 1):
+        bx, by = bl[l]
+        P = O.Prolongator(nx, nx, nd, bx, by, k, chiral=True)
+        P.set_block_vecs(mg.block_vecs(l))
+        Ps.append(P)
+        nx, nd = nx // bx, k
+        M = links_to_dense(mg.coarse_links(l), nx, nx, k)
+        ops.append(M)
+    Minv = np.linalg.inv(ops[-1])
+    apply = [ops[0]] + [(lambda M: (lambda v: M @ v))(M) for M in ops[1:-1]]
+
+    def cycle(l, b):
+        if l == levels - 1:
+            return Minv @ b
+        x, r = cheb_smooth(apply[l], b, hi[l] / RATIO, hi[l], pre)
+        x = x + Ps[l].prolong(cycle(l + 1, Ps[l].restrict(r)))
+        x, _ = cheb_smooth(apply[l], b, hi[l] / RATIO, hi[l], post, x0=x)
+        return x
+
+    return lambda b: cycle(0, b)
+
+
+@pytest.mark.parametrize("levels,L,N", [(2, 16, 16), (3, 32, 32)])
+def test_cheb_smoothed_cycle_vs_numpy(smg, levels, L, N):
+    """FP32 links at level 1 (its smoothing through the apply and separate passes), the
+    fine level's steps one fused D^dag D launch each (L a multiple of 16, N of 16), the
+    dense coarsest inverse: one cycle on N right-hand sides equals the FP64 numpy cycle
+    to FP32 accuracy, and differs clearly from the MR-smoothed cycle."""
+    banner(f"Chebyshev-smoothed cycle vs numpy levels={levels} L={L} N={N}")
+    k = 16
+    U, D, mg = _hierarchy(smg, L, levels, k, 400 + L, coarse_half=0)
+    B = gaussian_vecs(N, 2 * L * L, 410 + L)
+    x_mr = mg.precondition_batched(B, smg.F32)
+    hi = mg.set_cheb_smoother(RATIO)
+    assert len(hi) == levels - 1 and np.all(hi > 0)
+    X = mg.precondition_batched(B, smg.F32)
+    cycle = _numpy_cycle(mg, U, L, levels, k, hi)
+    errs = [relerr(X[j], cycle(B[j])) for j in range(N)]
+    print(f"max relative deviation from the FP64 numpy cycle {max(errs):.2e}; from MR {relerr(X, x_mr):.2e}")
+    assert max(errs) < 2e-5
+    assert relerr(X, x_mr) > 1e-2
+    mg.set_cheb_smoother(0.0)
+    assert relerr(mg.precondition_batched(B, smg.F32), x_mr) < 1e-12   # MR restored
+
+
+@pytest.mark.parametrize("N", [16, 32])
+def test_fused_steps_equal_separate_passes(smg, N):
+    """FP16 links at level 1: both levels' smoothing steps are single launches (fine
+    fused D^dag D, coarse matrix-core apply with the Chebyshev epilogue); the cycle
+    equals the one whose steps are the apply plus separate vector passes, to FP32
+    rounding."""
+    banner(f"fused vs separate Chebyshev smoothing N={N}")
+    L, k = 32, 16
+    U, D, mg = _hierarchy(smg, L, 3, k, 420, coarse_half=1)
+    B = gaussian_vecs(N, 2 * L * L, 421)
+    h1 = mg.set_cheb_smoother(RATIO, fused=True)
+    Xf = mg.precondition_batched(B, smg.F32)
+    h0 = mg.set_cheb_smoother(RATIO, fused=False)
+    Xs = mg.precondition_batched(B, smg.F32)
+    assert np.array_equal(h1, h0)
+    d = relerr(Xf, Xs)
+    print(f"fused vs separate passes {d:.2e}")
+    assert d < 2e-6
+
+
+def test_cheb_smoothed_cycle_graph_replays_equal_eager(smg):
+    """The Chebyshev-smoothed cycle (no reductions in its smoothers) captured as a
+    hipGraph: three replays with eager work between them equal the eager cycle bit for
+    bit."""
+    L, k, N = 32, 16, 32
+    U, D, mg = _hierarchy(smg, L, 3, k, 430, coarse_half=2, coarse_solver=2, coarse_maxit=32)
+    mg.set_cheb_smoother(RATIO)
+    B = gaussian_vecs(N, 2 * L * L, 431)
+    mg.set_cycle(3, 3, 32, 0.1, use_graph=False)
+    eager = mg.precondition_batched(B, smg.F32)
+    mg.set_cycle(3, 3, 32, 0.1, use_graph=True)
+    for rep in range(3):
+        assert np.array_equal(mg.precondition_batched(B, smg.F32), eager), rep
+        D.apply(B[rep])
+    mg.set_cycle(3, 3, 32, 0.1, use_graph=False)
+
+
+@pytest.mark.parametrize("levels", [2, 3])
+def test_mixed_solve_with_cheb_smoother(smg, levels):
+    """The mixed batched solve (FP32 GCR and cycle inside FP64 refinement) with the
+    Chebyshev smoother and the Chebyshev coarsest solve reaches the 1e-10 FP64 true
+    residual (the oracle's operator), in no more iterations than with MR + 3, and the
+    same solution."""
+    banner(f"mixed solve, Chebyshev smoother, {levels} levels")
+    L, k, N = 32, 32, 32
+    U, D, mg = _hierarchy(smg, L, levels, k, 440, coarse_half=2, coarse_solver=2, coarse_maxit=40)
+    A = O.Dirac(L, U, MASS, R, CSW, 0.0).op()
+    B = gaussian_vecs(N, 2 * L * L, 441)
+    out = {}
+    for ratio in (0.0, RATIO):
+        mg.set_cheb_smoother(ratio)
+        X, info, tr = smg.gcr_solve_batched(D, B, mg, 200, 1e-10, 10, prec=smg.F32, inner_tol=1e-5)
+        o = np.array([np.linalg.norm(B[j] - A(X[j])) / np.linalg.norm(B[j]) for j in range(N)])
+        assert o.max() < 1e-10, (ratio, o.max())
+        out[ratio] = (X, info.iterations)
+    print(f"iterations: MR {out[0.0][1]}, Chebyshev {out[RATIO][1]}")
+    assert out[RATIO][1] <= out[0.0][1] + 3
+    assert relerr(out[RATIO][0], out[0.0][0]) < 1e-8
