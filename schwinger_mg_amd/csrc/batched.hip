@@ -365,6 +365,20 @@ constexpr int kDdTX = 16, kDdTY = 8, kDdRX = kDdTX + 2, kDdRY = kDdTY + 2, kDdRi
 template <typename T>
 constexpr int ddagd_rhs() { return sizeof(T) == 4 ? 16 : 8; }
 
+// The operands of a Chebyshev step (EPI_CHEB): d_new (dst) apart from everything it
+// reads; x (aux) apart from d, r and the new r; the new r (fin.cheb_rout, required from a
+// start) apart from d, r, x and d_new; r in place (b) only apart from d.  From a start d
+// and r may be the same vector (b, or the residual).
+static bool cheb_ptrs_ok(const void* dst, const void* src, const void* b, const void* aux, const FinB& f) {
+    const void* ro = f.cheb_rout;
+    if (!b || !aux || f.cheb_mode < 0 || f.cheb_mode > 2 || f.cheb_start < 0 || f.cheb_start > 2) return false;
+    if (dst == src || dst == b || dst == aux || (ro && dst == ro)) return false;
+    if (aux == src || aux == b || (ro && aux == ro)) return false;
+    if (ro && (ro == src || ro == b)) return false;
+    if (f.cheb_start == 0) return src != b;
+    return ro != nullptr;
+}
+
 template <typename T, int EPI, int P, bool NT>
 __global__ __launch_bounds__(kBlock) void k_b_ddagd(DiracView<T> D, cplx<T>* __restrict__ dst,
                                                     const cplx<T>* __restrict__ src, T mu,
@@ -428,15 +442,18 @@ __global__ __launch_bounds__(kBlock) void k_b_ddagd(DiracView<T> D, cplx<T>* __r
             const T C = D.C ? D.C[s] : (T)0;
             const size_t i0 = (2 * (size_t)s) * N + j, i1 = i0 + N;
             RhsRun<T, P> out[2], bb[2], ax[2], dd[2];
+            // EPI_CHEB (FinB): what the step reads, by its start and mode
+            const int cst = EPI == EPI_CHEB ? fin.cheb_start : 0, cmo = EPI == EPI_CHEB ? fin.cheb_mode : 0;
+            const bool need_d = cmo != 1 || cst != 0, need_x = cst != 1 && (cmo != 1 || cst == 2);
             if (EPI == EPI_RESID || EPI == EPI_CHEB) {   // b, or the Chebyshev residual r
                 bb[0].load(bvec + i0);
                 bb[1].load(bvec + i1);
             }
-            if (EPI == EPI_DOTS || (EPI == EPI_CHEB && fin.cheb_mode != 1)) {   // aux, or the iterate x
+            if (EPI == EPI_DOTS || (EPI == EPI_CHEB && need_x)) {   // aux, or the iterate x
                 ax[0].load(aux + i0);
                 ax[1].load(aux + i1);
             }
-            if (EPI == EPI_CHEB && fin.cheb_mode != 1) {   // the direction at this site
+            if (EPI == EPI_CHEB && need_d) {   // the direction at this site
                 dd[0].load(src + i0);
                 dd[1].load(src + i1);
             }
@@ -449,9 +466,13 @@ __global__ __launch_bounds__(kBlock) void k_b_ddagd(DiracView<T> D, cplx<T>* __r
                 cplx<T> o0 = mk<T>(D.diag + C, -mu) * in[0][0][q] + h0;
                 cplx<T> o1 = mk<T>(D.diag - C, mu) * (g1 * in[0][1][q]) + h1;
                 o1 = -o1;   // gamma5 on the output
-                if (EPI == EPI_RESID || EPI == EPI_CHEB) {   // Chebyshev: the new residual r - A d
+                if (EPI == EPI_RESID) {
                     o0 = bb[0].v[q] - o0;
                     o1 = bb[1].v[q] - o1;
+                }
+                if (EPI == EPI_CHEB) {   // the new residual r - s A d
+                    o0 = bb[0].v[q] - (T)fin.cheb_s * o0;
+                    o1 = bb[1].v[q] - (T)fin.cheb_s * o1;
                 }
                 out[0].v[q] = o0;
                 out[1].v[q] = o1;
@@ -461,26 +482,28 @@ __global__ __launch_bounds__(kBlock) void k_b_ddagd(DiracView<T> D, cplx<T>* __r
                     acc[1][q] += (double)d0.im + (double)d1.im;
                     acc[2][q] += (double)norm2(o0) + (double)norm2(o1);
                 }
-                if (EPI == EPI_CHEB && fin.cheb_mode != 1) {   // d_new = c1 d + c2 r, x += d_new
+                if (EPI == EPI_CHEB && need_d) {   // d_new = c1 d + c2 r, x += d_new
 #pragma unroll
                     for (int sp = 0; sp < 2; sp++) {
-                        dd[sp].v[q] = (T)fin.cheb_c1 * dd[sp].v[q] + (T)fin.cheb_c2 * out[sp].v[q];
-                        ax[sp].v[q] = ax[sp].v[q] + dd[sp].v[q];
+                        const cplx<T> dp = (T)fin.cheb_s * dd[sp].v[q];   // the previous direction
+                        const cplx<T> xp = cst == 1 ? dp : (cst == 2 ? ax[sp].v[q] + dp : ax[sp].v[q]);
+                        dd[sp].v[q] = (T)fin.cheb_c1 * dp + (T)fin.cheb_c2 * out[sp].v[q];
+                        ax[sp].v[q] = cmo == 1 ? xp : xp + dd[sp].v[q];
                     }
                 }
             }
             if constexpr (EPI == EPI_CHEB) {   // r, d and x; r only; x only (fin.cheb_mode)
-                cplx<T>* const rw = const_cast<cplx<T>*>(bvec);
+                cplx<T>* const rw = fin.cheb_rout ? static_cast<cplx<T>*>(fin.cheb_rout) : const_cast<cplx<T>*>(bvec);
                 cplx<T>* const xw = const_cast<cplx<T>*>(aux);
-                if (fin.cheb_mode != 2) {
+                if (cmo != 2) {
                     out[0].store(rw + i0);
                     out[1].store(rw + i1);
                 }
-                if (fin.cheb_mode == 0) {
+                if (cmo == 0) {
                     dd[0].store(dst + i0);
                     dd[1].store(dst + i1);
                 }
-                if (fin.cheb_mode != 1) {
+                if (cmo != 1 || cst != 0) {
                     ax[0].store(xw + i0);
                     ax[1].store(xw + i1);
                 }
@@ -550,10 +573,8 @@ template <typename T>
 bool launch_b_ddagd(const DiracView<T>& D, cplx<T>* dst, const cplx<T>* src, T mu, Epi epi, const cplx<T>* b,
                     const cplx<T>* aux, const FinB& fin, int N, hipStream_t st) {
     SMG_REQUIRE(epi == EPI_NONE || epi == EPI_RESID || epi == EPI_DOTS || epi == EPI_CHEB, "fused D^dag D: epilogue");
-    // EPI_CHEB: r (b) and x (aux) updated in place, the new direction to dst
-    SMG_REQUIRE(epi != EPI_CHEB || (b && aux && dst != src && dst != b && dst != aux && src != b && src != aux &&
-                                    b != aux),
-                "fused D^dag D: the Chebyshev epilogue takes distinct r, x, d and d_new");
+    SMG_REQUIRE(epi != EPI_CHEB || cheb_ptrs_ok(dst, src, b, aux, fin),
+                "fused D^dag D: the Chebyshev epilogue's r, x, d, d_new (and new r) must not overlap as given");
     if (!b_ddagd_fits(D.L, N, sizeof(T))) return false;
     const int choice = tune_lookup(TuneKey{TUNE_DDAGD, D.L, N, (int)sizeof(T), (int)epi, 0});
     if (choice == 0 && epi != EPI_CHEB) return false;   // the Chebyshev step has no two-launch form
@@ -952,9 +973,12 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
     // EPI_CHEB updates r (bv) and x (auxv) in place at this wave's own site only
     unsigned* const rwh = const_cast<unsigned*>(bh2);
     unsigned* const xwh = const_cast<unsigned*>(auxh);
-    c32* const rwf = const_cast<c32*>(bvec);
+    // FP32 vectors: the new residual to fin.cheb_rout where given (FinB's cheb_start)
+    c32* const rwf = fin.cheb_rout ? static_cast<c32*>(fin.cheb_rout) : const_cast<c32*>(bvec);
     c32* const xwf = const_cast<c32*>(aux);
     const int cmode = fin.cheb_mode;   // EPI_CHEB: 0 r, d and x; 1 r only; 2 x only
+    const int cst = HV ? 0 : fin.cheb_start;
+    const bool need_d = cmode != 1 || cst != 0, need_x = cst != 1 && (cmode != 1 || cst == 2);
     constexpr bool RED = EPI == EPI_DOTS || EPI == EPI_CG;
     constexpr int NC = EPI == EPI_CG ? 5 : 3;   // reduced values per RHS
     if (RED && fin.skip && *fin.skip) {   // CG finished for every RHS
@@ -1117,11 +1141,11 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                                     if constexpr (EPI == EPI_CHEB)   // the previous direction at this site
                                         epo[r][v][u][2] = *reinterpret_cast<const uint2*>(srch + o);
                                 } else {
-                                    if (EPI != EPI_CHEB || cmode != 1)
+                                    if (EPI != EPI_CHEB || need_x)
                                         epo[r][v][u][0] = *reinterpret_cast<const float4*>((EPI == EPI_RESID ? bvec : aux) + o);
                                     if constexpr (EPI == EPI_CHEB) {
                                         epo[r][v][u][1] = *reinterpret_cast<const float4*>(bvec + o);
-                                        if (cmode != 1) epo[r][v][u][2] = *reinterpret_cast<const float4*>(src + o);
+                                        if (need_d) epo[r][v][u][2] = *reinterpret_cast<const float4*>(src + o);
                                     }
                                 }
                             }
@@ -1176,28 +1200,35 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                             *reinterpret_cast<uint2*>(dsth + idx) = uint2{pack_h(d0), pack_h(d1)};
                             *reinterpret_cast<uint2*>(xwh + idx) = uint2{pack_h(x0), pack_h(x1)};
                         } else if constexpr (PAIR && !HV && EPI == EPI_CHEB) {
-                            // FP32 vectors: r -= q, then (mode) d_new = c1 d + c2 r, x += d_new
+                            // FP32 vectors: r -= s q, then (mode) d_new = c1 d + c2 r, x += d_new
                             const size_t idx = row + 32 * u + 2 * lo;
                             const float4 rw = epo[r][v][u][1];
-                            const c32 r0 = c32{rw.x, rw.y} - c32{acc[r][2 * u][0][v], acc[r][2 * u][1][v]};
-                            const c32 r1 = c32{rw.z, rw.w} - c32{acc[r][2 * u + 1][0][v], acc[r][2 * u + 1][1][v]};
+                            const float sc = fin.cheb_s;
+                            const c32 r0 = c32{rw.x, rw.y} - sc * c32{acc[r][2 * u][0][v], acc[r][2 * u][1][v]};
+                            const c32 r1 =
+                                c32{rw.z, rw.w} - sc * c32{acc[r][2 * u + 1][0][v], acc[r][2 * u + 1][1][v]};
                             if (cmode != 2) *reinterpret_cast<float4*>(rwf + idx) = float4{r0.re, r0.im, r1.re, r1.im};
-                            if (cmode != 1) {
+                            if (need_d) {
                                 const float4 xw = epo[r][v][u][0], dw = epo[r][v][u][2];
-                                const c32 d0 = fin.cheb_c1 * c32{dw.x, dw.y} + fin.cheb_c2 * r0;
-                                const c32 d1 = fin.cheb_c1 * c32{dw.z, dw.w} + fin.cheb_c2 * r1;
+                                const c32 p0 = sc * c32{dw.x, dw.y}, p1 = sc * c32{dw.z, dw.w};
+                                const c32 x0 = cst == 1 ? p0 : (cst == 2 ? c32{xw.x, xw.y} + p0 : c32{xw.x, xw.y});
+                                const c32 x1 = cst == 1 ? p1 : (cst == 2 ? c32{xw.z, xw.w} + p1 : c32{xw.z, xw.w});
+                                const c32 d0 = fin.cheb_c1 * p0 + fin.cheb_c2 * r0;
+                                const c32 d1 = fin.cheb_c1 * p1 + fin.cheb_c2 * r1;
                                 if (cmode == 0) *reinterpret_cast<float4*>(dst + idx) = float4{d0.re, d0.im, d1.re, d1.im};
-                                *reinterpret_cast<float4*>(xwf + idx) =
-                                    float4{xw.x + d0.re, xw.y + d0.im, xw.z + d1.re, xw.w + d1.im};
+                                const c32 y0 = cmode == 1 ? x0 : x0 + d0, y1 = cmode == 1 ? x1 : x1 + d1;
+                                *reinterpret_cast<float4*>(xwf + idx) = float4{y0.re, y0.im, y1.re, y1.im};
                             }
                         } else if constexpr (!HV && EPI == EPI_CHEB) {
                             const size_t idx = row + 16 * u + lo;
-                            const c32 rn = bvec[idx] - c32{acc[r][u][0][v], acc[r][u][1][v]};
+                            const c32 rn = bvec[idx] - fin.cheb_s * c32{acc[r][u][0][v], acc[r][u][1][v]};
                             if (cmode != 2) rwf[idx] = rn;
-                            if (cmode != 1) {
-                                const c32 dn = fin.cheb_c1 * src[idx] + fin.cheb_c2 * rn;
+                            if (need_d) {
+                                const c32 dp = fin.cheb_s * src[idx];
+                                const c32 xp = cst == 1 ? dp : (cst == 2 ? aux[idx] + dp : aux[idx]);
+                                const c32 dn = fin.cheb_c1 * dp + fin.cheb_c2 * rn;
                                 if (cmode == 0) dst[idx] = dn;
-                                xwf[idx] = aux[idx] + dn;
+                                xwf[idx] = cmode == 1 ? xp : xp + dn;
                             }
                         } else if constexpr (HV && EPI == EPI_CHEB) {
                             const size_t idx = row + 16 * u + lo;
@@ -1373,10 +1404,9 @@ void launch_b_coarse_apply(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>*
     // the Chebyshev step (r, x in b, aux; the direction in src): FP16 links on the matrix cores
     SMG_REQUIRE(epi != EPI_CHEB || (b_coarse_cheb_fits(C.k, N, std::is_same<T, float>::value &&
                                                                    std::is_same<TL, ch16>::value) &&
-                                    b && aux && dst != src && dst != b && dst != aux && src != b && src != aux &&
-                                    b != aux),
+                                    cheb_ptrs_ok(dst, src, b, aux, fin)),
                 "coarse apply: the Chebyshev epilogue needs FP16 links, FP32 vectors, k in {8,16,32}, "
-                "N in {16,32,64} and distinct r, x, d, d_new");
+                "N in {16,32,64} and r, x, d, d_new (and new r) that do not overlap as given");
     if constexpr (std::is_same<T, float>::value && std::is_same<TL, ch16>::value) {
         if ((C.k == 8 || C.k == 16 || C.k == 32) && (N == 16 || N == 32 || N == 64)) {
             // FP16 links: bf16x3 matrix-core path
@@ -1434,7 +1464,7 @@ void launch_b_coarse_apply_h(const CoarseView<ch16>& C, cb16* dst, const cb16* s
     // the Chebyshev step writes r (b) and x (aux) in place and the new direction to dst,
     // while other workgroups read the old direction (src) at their neighbours
     SMG_REQUIRE(epi != EPI_CHEB || (aux && b && dst != src && aux != src && b != src && dst != aux && dst != b &&
-                                    aux != b),
+                                    aux != b && fin.cheb_start == 0 && !fin.cheb_rout && fin.cheb_mode == 0),
                 "BF16-vector coarse apply: the Chebyshev epilogue takes distinct r (b), x (aux), d (src) and "
                 "d_new (dst)");
     auto go = [&](auto kc, auto ntc) {
@@ -2299,34 +2329,6 @@ void launch_h_to_f32(c32* out, const cb16* in, size_t n, int N, hipStream_t st) 
     k_h_to_f32<<<(int)((runs + kBlock - 1) / kBlock), kBlock, 0, st>>>(out, in, runs);
     SMG_LAUNCH_CHECK();
 }
-
-// The first step of Chebyshev smoothing: from x = 0 (zero_init) r = b, d = b / theta,
-// x = d; else (r = b - A x already formed) d = r / theta, x += d
-template <typename T>
-__global__ void k_b_cheb_init(cplx<T>* __restrict__ d, cplx<T>* __restrict__ x, cplx<T>* __restrict__ r,
-                              const cplx<T>* __restrict__ b, size_t items, T inv_theta, int zero_init) {
-    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < items; t += (size_t)gridDim.x * kBlock) {
-        if (zero_init) {
-            const cplx<T> v = b[t];
-            r[t] = v;
-            d[t] = inv_theta * v;
-            x[t] = inv_theta * v;
-        } else {
-            const cplx<T> v = inv_theta * r[t];
-            d[t] = v;
-            x[t] = x[t] + v;
-        }
-    }
-}
-template <typename T>
-void launch_b_cheb_init(cplx<T>* d, cplx<T>* x, cplx<T>* r, const cplx<T>* b, size_t n, double theta, bool zero_init,
-                        hipStream_t st) {
-    SMG_REQUIRE(theta > 0.0, "Chebyshev smoothing: the interval's centre must be positive");
-    k_b_cheb_init<T><<<b_grid(n, 1 << 16), kBlock, 0, st>>>(d, x, r, b, n, (T)(1.0 / theta), zero_init ? 1 : 0);
-    SMG_LAUNCH_CHECK();
-}
-template void launch_b_cheb_init<float>(c32*, c32*, c32*, const c32*, size_t, double, bool, hipStream_t);
-template void launch_b_cheb_init<double>(c64*, c64*, c64*, const c64*, size_t, double, bool, hipStream_t);
 
 template <typename T>
 __global__ void k_b_axpby_real(cplx<T>* __restrict__ y, const cplx<T>* __restrict__ x, T a, T c, size_t items) {

@@ -44,20 +44,12 @@ void DdagDOpB<T>::apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>
     launch_b_dslash<T>(v, tmp.p(), src, mu, 0, 0, EPI_NONE, nullptr, nullptr, FinB(), this->N, st);
     launch_b_dslash<T>(v, dst, tmp.p(), -mu, 1, 1, epi, b, aux, fin, this->N, st);
 }
-static FinB cheb_fin(double c1, double c2, int mode) {
-    SMG_REQUIRE(mode >= 0 && mode <= 2, "Chebyshev step: mode");
-    FinB f;
-    f.cheb_c1 = (float)c1;
-    f.cheb_c2 = (float)c2;
-    f.cheb_mode = mode;
-    return f;
-}
 template <typename T>
-void DdagDOpB<T>::cheb_step(cplx<T>* d_new, const cplx<T>* d, cplx<T>* r, cplx<T>* x, double c1, double c2, int mode,
+void DdagDOpB<T>::cheb_step(cplx<T>* d_new, const cplx<T>* d, const cplx<T>* r, cplx<T>* x, const FinB& f,
                             hipStream_t st) {
     SMG_REQUIRE(fused_cheb(), "Chebyshev step: the lattice or N does not fit the fused D^dag D");
     const DiracView<T> v = D.view<T>();
-    SMG_REQUIRE(launch_b_ddagd<T>(v, d_new, d, (T)D.mu_t, EPI_CHEB, r, x, cheb_fin(c1, c2, mode), this->N, st),
+    SMG_REQUIRE(launch_b_ddagd<T>(v, d_new, d, (T)D.mu_t, EPI_CHEB, r, x, f, this->N, st),
                 "Chebyshev step: fused D^dag D refused");
 }
 template struct DdagDOpB<double>;
@@ -75,10 +67,10 @@ void CoarseLinOpB<T, TL>::apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const
                                  st);
 }
 template <typename T, typename TL>
-void CoarseLinOpB<T, TL>::cheb_step(cplx<T>* d_new, const cplx<T>* d, cplx<T>* r, cplx<T>* x, double c1, double c2,
-                                    int mode, hipStream_t st) {
+void CoarseLinOpB<T, TL>::cheb_step(cplx<T>* d_new, const cplx<T>* d, const cplx<T>* r, cplx<T>* x, const FinB& f,
+                                    hipStream_t st) {
     SMG_REQUIRE(fused_cheb(), "Chebyshev step: the coarse operator's fused step needs FP16 links, FP32 vectors");
-    apply(d_new, d, EPI_CHEB, r, x, cheb_fin(c1, c2, mode), st);
+    apply(d_new, d, EPI_CHEB, r, x, f, st);
 }
 template struct CoarseLinOpB<double, c64>;
 template struct CoarseLinOpB<float, c32>;
@@ -166,18 +158,41 @@ static void cheb_smooth_b(LinOpB<T>& A, cplx<T>* x, const cplx<T>* b, cplx<T>* r
     const size_t nN = A.n() * A.N;
     const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
     double rho = 1.0 / sigma;
-    if (niter > 0 && fused && A.fused_cheb()) {
-        // (r = b - A x,) d_0 = r / theta, x_1 = x_0 + d_0 in one pass; each step one launch
-        // of A with the updates in its epilogue, the directions alternating between d and q
-        if (!zero_init) A.apply(r, x, EPI_RESID, b, nullptr, FinB(), st);
-        launch_b_cheb_init<T>(d, x, r, b, nN, theta, zero_init, st);
-        cplx<T>* dc = d;
-        cplx<T>* dn = q;
+    if (niter > 0 && fused && A.fused_cheb() && zero_init == need_r) {
+        // Each step one launch of A with the updates in its epilogue.  The first one also
+        // does the initial pass: its direction and residual are b (from x = 0) or the
+        // residual r0 = b - A x0 (in r), scaled by 1 / theta, and its new residual goes to
+        // r (from x = 0, where r stays) or q (r0 is still read there).  The directions
+        // take whichever of r, d, q holds neither the residual nor the current direction.
+        const cplx<T>* dc = b;
+        const cplx<T>* rc = b;
+        cplx<T>* rn = r;
+        if (!zero_init) {
+            A.apply(r, x, EPI_RESID, b, nullptr, FinB(), st);
+            dc = rc = r;
+            rn = q;
+        }
+        cplx<T>* const buf[3] = {r, d, q};
         for (int k = 1; k <= niter; k++) {
             const double rho_k = 1.0 / (2.0 * sigma - rho);
-            const int mode = k < niter ? 0 : (need_r ? 1 : 2);
-            A.cheb_step(dn, dc, r, x, rho_k * rho, 2.0 * rho_k / delta, mode, st);
-            std::swap(dc, dn);
+            FinB f;
+            f.cheb_c1 = (float)(rho_k * rho);
+            f.cheb_c2 = (float)(2.0 * rho_k / delta);
+            f.cheb_mode = k < niter ? 0 : (need_r ? 1 : 2);
+            if (k == 1) {
+                f.cheb_start = zero_init ? 1 : 2;
+                f.cheb_s = (float)(1.0 / theta);
+                f.cheb_rout = rn;
+            }
+            cplx<T>* dn = nullptr;
+            for (cplx<T>* c : buf)
+                if (c != rn && c != dc && (k == 1 || c != rc)) {
+                    dn = c;
+                    break;
+                }
+            A.cheb_step(dn, dc, rc, x, f, st);
+            if (k == 1) rc = rn;
+            dc = dn;
             rho = rho_k;
         }
         return;

@@ -19,9 +19,9 @@ struct LinOpB {
     }
     // whether cheb_step runs as one launch for this operator and N
     virtual bool fused_cheb() const { return false; }
-    // one Chebyshev step r -= A d, d_new = c1 d + c2 r, x += d_new in the apply's
-    // epilogue; mode 0 writes r, d_new and x, 1 r only, 2 x only
-    virtual void cheb_step(cplx<T>* d_new, const cplx<T>* d, cplx<T>* r, cplx<T>* x, double c1, double c2, int mode,
+    // one Chebyshev step in the apply's epilogue, as f's EPI_CHEB fields give it:
+    // r_new = r - s A d, d_new = c1 s d + c2 r_new, x updated (FinB)
+    virtual void cheb_step(cplx<T>* d_new, const cplx<T>* d, const cplx<T>* r, cplx<T>* x, const FinB& f,
                            hipStream_t st) {
         SMG_REQUIRE(false, "Chebyshev step: no fused form for this operator");
     }
@@ -36,7 +36,7 @@ struct DdagDOpB : LinOpB<T> {
     void apply(cplx<T>* dst, const cplx<T>* src, Epi epi, const cplx<T>* b, const cplx<T>* aux, const FinB& fin,
                hipStream_t st) override;
     bool fused_cheb() const override { return b_ddagd_fits(D.L, this->N, sizeof(T)); }
-    void cheb_step(cplx<T>* d_new, const cplx<T>* d, cplx<T>* r, cplx<T>* x, double c1, double c2, int mode,
+    void cheb_step(cplx<T>* d_new, const cplx<T>* d, const cplx<T>* r, cplx<T>* x, const FinB& f,
                    hipStream_t st) override;
     using LinOpB<T>::apply;
 };
@@ -51,7 +51,7 @@ struct CoarseLinOpB : LinOpB<T> {
     bool fused_cheb() const override {
         return b_coarse_cheb_fits(C.k, this->N, std::is_same<T, float>::value && std::is_same<TL, ch16>::value);
     }
-    void cheb_step(cplx<T>* d_new, const cplx<T>* d, cplx<T>* r, cplx<T>* x, double c1, double c2, int mode,
+    void cheb_step(cplx<T>* d_new, const cplx<T>* d, const cplx<T>* r, cplx<T>* x, const FinB& f,
                    hipStream_t st) override;
     using LinOpB<T>::apply;
 };

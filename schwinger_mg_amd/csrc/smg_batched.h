@@ -31,9 +31,19 @@ struct FinB {
     // EPI_CHEB: the step's coefficients d_new = c1 d + c2 r (fixed by the iteration
     // number and the spectral interval, the same for every right-hand side); what the
     // step writes (FP32 vectors): 0 r, d_new and x; 1 r only (the last step of a smoothing
-    // that returns its residual); 2 x only (the last step of one that does not)
+    // that returns its residual; x too from a start); 2 x only (the last step of one
+    // that does not)
     float cheb_c1 = 0.0f, cheb_c2 = 0.0f;
     int cheb_mode = 0;
+    // FP32 vectors: the first step of a smoothing takes the place of its initial pass.
+    // cheb_start 1 (from x = 0): the input d and r are both b, the direction b / theta
+    // (cheb_s = 1 / theta scales A d and d), x_1 = b / theta is not read; 2 (from the
+    // residual r0 = b - A x0): d and r are both r0, scaled the same way, and
+    // x_1 = x0 + r0 / theta.  The new r then goes to cheb_rout (r is an input the step
+    // must not overwrite); 0 continues (cheb_s = 1, r in place unless cheb_rout is given)
+    int cheb_start = 0;
+    float cheb_s = 1.0f;
+    void* cheb_rout = nullptr;
 };
 
 // Per-RHS coefficient array views (device): complex a_j at s[2j], s[2j+1].
@@ -107,11 +117,6 @@ void launch_b_cg_r(cplx<T>* r, const cplx<T>* Ap, const double* a, size_t n, con
 template <typename T>
 void launch_b_cg_xp(cplx<T>* x, cplx<T>* p, const cplx<T>* r, const double* a, const double* b, size_t n, int N,
                     const int* late, hipStream_t st);
-// the first step of Chebyshev smoothing on n entries (all RHS): from x = 0 (zero_init)
-// r = b, d = b / theta, x = d; else d = r / theta, x += d
-template <typename T>
-void launch_b_cheb_init(cplx<T>* d, cplx<T>* x, cplx<T>* r, const cplx<T>* b, size_t n, double theta, bool zero_init,
-                        hipStream_t st);
 // y = a x + c y with one real scalar pair for all RHS
 template <typename T>
 void launch_b_axpby_real(cplx<T>* y, const cplx<T>* x, double a, double c, size_t n, hipStream_t st);
