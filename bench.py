@@ -76,6 +76,8 @@ def parse():
                     help="0 CG, 1 dense inverse, 2 CG in FP64 and a Chebyshev iteration in the mixed cycle")
     ap.add_argument("--cheb-smooth", type=float, default=12.0,
                     help="Chebyshev smoothing in the mixed batched cycle on [hi / R, hi] (0: MR)")
+    ap.add_argument("--cheb-lo-scale", type=float, default=1.0,
+                    help="coarse-solver 2: the Chebyshev interval's lower end times this")
     ap.add_argument("--cheb-steps", type=int, default=32,
                     help="coarse-solver 2: Chebyshev steps of the mixed cycle's coarsest solve")
     ap.add_argument("--coarse-bf16", type=int, default=1,
@@ -722,6 +724,9 @@ def main():
         else:
             mg.set_cycle(args.mr_pre, args.mr_post, args.coarse_maxit, args.coarse_tol, use_graph=bool(args.graph))
 
+    if cheb and args.cheb_lo_scale != 1.0:
+        lo, hi = mg.cheb_interval()
+        mg.set_cheb_interval(lo * args.cheb_lo_scale, hi)
     if args.cheb_smooth > 0:
         smooth_hi = mg.set_cheb_smoother(args.cheb_smooth)
         log(f"Chebyshev smoothing intervals: " + ", ".join(f"level {l} [{h / args.cheb_smooth:.4g}, {h:.4g}]"

@@ -452,6 +452,10 @@ int smg_mg_coarsest_batched(smg_mg* mg, int nrhs, const double* B, double* X);
  * the extreme Ritz values of 60 Lanczos steps on the FP64 coarsest operator, hi
  * widened by 10% (no reference counterpart) */
 int smg_mg_cheb_interval(smg_mg* mg, double* lo, double* hi);
+/* replace that interval (0 < lo < hi) until the next smg_mg_rebuild / smg_mg_refresh
+ * estimates it again: a lo above the smallest eigenvalue trades accuracy on the lowest
+ * modes for a faster fall of the rest (no reference counterpart) */
+int smg_mg_set_cheb_interval(smg_mg* mg, double lo, double hi);
 /* Chebyshev smoothing in the batched FP32 cycle (ratio > 0; 0 restores MR): every
  * non-coarsest level smooths with pre_smooth / post_smooth Chebyshev steps on
  * [hi / ratio, hi], hi = 1.1 x a 40-step power-iteration estimate of the level's largest

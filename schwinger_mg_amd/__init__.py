@@ -110,6 +110,7 @@ SIGNATURES = [
     ("smg_coarse_apply_batched_dev", _I, [_P, _I, _I, _P, _P, _P]),
     ("smg_mg_precondition_batched", _I, [_P, _I, _I, _P, _P]),
     ("smg_mg_coarsest_batched", _I, [_P, _I, _P, _P]), ("smg_mg_cheb_interval", _I, [_P, _P, _P]),
+    ("smg_mg_set_cheb_interval", _I, [_P, _D, _D]),
     ("smg_mg_set_cheb_smoother", _I, [_P, _D, _I, _P]),
     ("smg_gcr_solve_batched", _I, [_P, _P, _I, _I, _P, _P, _I, _D, _I, _D, _P, _P]),
     ("smg_gcr_solve_batched_dev", _I, [_P, _P, _I, _I, _P, _P, _I, _D, _I, _D, _P, _P, _P]),
@@ -701,6 +702,10 @@ class MGHierarchy:
         lo, hi = _D(), _D()
         _chk(lib().smg_mg_cheb_interval(self.h, ctypes.byref(lo), ctypes.byref(hi)))
         return lo.value, hi.value
+
+    def set_cheb_interval(self, lo, hi):
+        """Replace the coarsest Chebyshev interval until the next rebuild or refresh."""
+        _chk(lib().smg_mg_set_cheb_interval(self.h, float(lo), float(hi)))
 
     def omega(self, level):
         out = _D()

@@ -972,6 +972,16 @@ int smg_mg_set_cheb_smoother(smg_mg* mg, double ratio, int fused, double* hi) {
             for (int l = 0; l + 1 < m.cfg.n_levels; l++) hi[l] = m.cheb_smooth_hi[l];
     });
 }
+int smg_mg_set_cheb_interval(smg_mg* mg, double lo, double hi) {
+    return guard([&] {
+        SMG_REQUIRE(mg, "null handle");
+        SMG_REQUIRE(mg->mg->cfg.coarse_solver == 2, "the hierarchy has no Chebyshev coarsest solve (coarse_solver 2)");
+        SMG_REQUIRE(lo > 0.0 && hi > lo, "the interval needs 0 < lo < hi");
+        mg->mg->cheb_lo = lo;
+        mg->mg->cheb_hi = hi;
+        mg->mg->reset_graphs();   // captured cycles hold the old coefficients
+    });
+}
 int smg_mg_cheb_interval(smg_mg* mg, double* lo, double* hi) {
     return guard([&] {
         SMG_REQUIRE(mg && lo && hi, "bad arguments");

@@ -103,3 +103,23 @@ def test_mixed_solve_with_cheb_coarsest(smg, levels):
     print(f"iterations: BF16 CG {out[0][1]}, Chebyshev {out[2][1]}")
     assert out[2][1] <= out[0][1] + 3
     assert relerr(out[2][0], out[0][0]) < 1e-8
+
+
+def test_set_cheb_interval(smg, cheb_mg):
+    """A caller's interval replaces the Lanczos one (the solve then follows the oracle's
+    iteration on it), invalid ones are refused, and a rebuild estimates it again."""
+    mg, M = cheb_mg["mg"], cheb_mg["M"]
+    lo, hi = mg.cheb_interval()
+    mg.set_cheb_interval(2.0 * lo, hi)
+    assert mg.cheb_interval() == (2.0 * lo, hi)
+    B = gaussian_vecs(16, M.shape[0], 304)
+    X = mg.coarsest_batched(B)
+    errs = [np.linalg.norm(X[j] - cheb_solve(lambda v: M @ v, B[j], 2.0 * lo, hi, STEPS))
+            / np.linalg.norm(cheb_solve(lambda v: M @ v, B[j], 2.0 * lo, hi, STEPS)) for j in range(16)]
+    assert max(errs) < 3e-2
+    for bad in ((0.0, hi), (hi, lo), (-1.0, 1.0)):
+        with pytest.raises(smg.SMGError):
+            mg.set_cheb_interval(*bad)
+    mg.rebuild()
+    lo2, hi2 = mg.cheb_interval()
+    assert abs(lo2 - lo) <= 1e-9 * lo and abs(hi2 - hi) <= 1e-9 * hi
