@@ -1620,6 +1620,120 @@ __global__ __launch_bounds__(kBlock) void k_b_restrict_lds(AggView A, const cplx
         coarse[(((size_t)b * k + kk) << lN) + j] = acc;
     }
 }
+// ---- transfers on the matrix cores (FP32 vectors, N a multiple of 16): one wave per
+// 16 x 16 tile of one chirality half of one aggregate, v_mfma_f32_16x16x4_f32 with each
+// complex product as four real ones (FP32 products and FP32 sums, as the kernels above
+// form them, in another order).  Entry e (0 .. kc-1) of half h is dof h*nc + e % nc of
+// the aggregate's site e / nc (nh = 2 halves with chiral aggregation, else 1).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ size_t half_entry_site(const AggView& A, int b, int loc) {
+    const int sx = (b % A.nbx) * A.bx + loc % A.bx, sy = (b / A.nbx) * A.by + loc / A.bx;
+    return (size_t)sx + (size_t)A.nx * sy;
+}
+// coarse[b*k + kk][j] = sum over half h's KC entries of conj(V[kk][e]) fine[e][j], kk in
+// half h: A = the 16 block vectors of the tile (rows) by 4 entries, B = 4 entries by 16 RHS
+template <int KC>
+__global__ __launch_bounds__(kBlock) void k_b_restrict_mfma(AggView A, const c32* __restrict__ bv,
+                                                             const c32* __restrict__ fine, c32* __restrict__ coarse,
+                                                             int N) {
+    const int nh = A.chiral ? 2 : 1, kh = A.k / nh, nc = A.nd / nh, nmt = kh >> 4, njc = N >> 4;
+    const int bsize = A.bx * A.by * A.nd;
+    size_t w = (size_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (w >= (size_t)A.nbx * A.nby * nh * nmt * njc) return;
+    const int jc = (int)(w % njc);
+    w /= njc;
+    const int mt = (int)(w % nmt);
+    w /= nmt;
+    const int h = (int)(w % nh), b = (int)(w / nh);
+    const int lane = threadIdx.x & 63, m = lane & 15, q = lane >> 4;
+    const int j = jc * 16 + m, kk0 = h * kh + mt * 16;
+    const c32* vrow = bv + ((size_t)b * A.k + kk0 + m) * bsize;
+    c32 va[KC / 4], fa[KC / 4];
+#pragma unroll
+    for (int s = 0; s < KC / 4; s++) {
+        const int e = 4 * s + q, loc = e / nc, c = h * nc + e % nc;
+        va[s] = vrow[loc * A.nd + c];
+        fa[s] = fine[(half_entry_site(A, b, loc) * A.nd + c) * N + j];
+    }
+    f32x4 cr = {0.0f, 0.0f, 0.0f, 0.0f}, ci = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < KC / 4; s++) {   // conj(v) f = (vr fr + vi fi) + i (vr fi - vi fr)
+        cr = __builtin_amdgcn_mfma_f32_16x16x4f32(va[s].re, fa[s].re, cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x4f32(va[s].im, fa[s].im, cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x4f32(va[s].re, fa[s].im, ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x4f32(-va[s].im, fa[s].re, ci, 0, 0, 0);
+    }
+    // lane (m, q) holds rows 4q .. 4q+3 of column m
+#pragma unroll
+    for (int i = 0; i < 4; i++) coarse[((size_t)b * A.k + kk0 + 4 * q + i) * N + j] = c32{cr[i], ci[i]};
+}
+// fine[e][j] (+)= sum over half h's KH block vectors of V[kk][e] coarse[kk][j]: A = 16
+// entries (rows) by 4 block vectors, B = 4 block vectors by 16 RHS
+template <int KH>
+__global__ __launch_bounds__(kBlock) void k_b_prolong_mfma(AggView A, const c32* __restrict__ bv,
+                                                            const c32* __restrict__ coarse, c32* __restrict__ fine,
+                                                            int accumulate, int N) {
+    const int nh = A.chiral ? 2 : 1, nc = A.nd / nh, kc = A.bx * A.by * nc, nmt = kc >> 4, njc = N >> 4;
+    const int bsize = A.bx * A.by * A.nd;
+    size_t w = (size_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (w >= (size_t)A.nbx * A.nby * nh * nmt * njc) return;
+    const int jc = (int)(w % njc);
+    w /= njc;
+    const int mt = (int)(w % nmt);
+    w /= nmt;
+    const int h = (int)(w % nh), b = (int)(w / nh);
+    const int lane = threadIdx.x & 63, m = lane & 15, q = lane >> 4;
+    const int j = jc * 16 + m;
+    const int ea = mt * 16 + m;   // this lane's A row: entry ea of the half
+    const c32* vcol = bv + ((size_t)b * A.k + h * KH) * bsize + (ea / nc) * A.nd + h * nc + ea % nc;
+    const c32* crow = coarse + ((size_t)b * A.k + h * KH) * N + j;
+    c32 va[KH / 4], ca[KH / 4];
+#pragma unroll
+    for (int s = 0; s < KH / 4; s++) {
+        va[s] = vcol[(size_t)(4 * s + q) * bsize];
+        ca[s] = crow[(size_t)(4 * s + q) * N];
+    }
+    f32x4 fr = {0.0f, 0.0f, 0.0f, 0.0f}, fi = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < KH / 4; s++) {   // v c = (vr cr - vi ci) + i (vr ci + vi cr)
+        fr = __builtin_amdgcn_mfma_f32_16x16x4f32(va[s].re, ca[s].re, fr, 0, 0, 0);
+        fr = __builtin_amdgcn_mfma_f32_16x16x4f32(-va[s].im, ca[s].im, fr, 0, 0, 0);
+        fi = __builtin_amdgcn_mfma_f32_16x16x4f32(va[s].re, ca[s].im, fi, 0, 0, 0);
+        fi = __builtin_amdgcn_mfma_f32_16x16x4f32(va[s].im, ca[s].re, fi, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int e = mt * 16 + 4 * q + i, loc = e / nc, c = h * nc + e % nc;
+        c32* out = fine + (half_entry_site(A, b, loc) * A.nd + c) * N + j;
+        const c32 v{fr[i], fi[i]};
+        *out = accumulate ? *out + v : v;
+    }
+}
+// whether the matrix-core transfers take this aggregation and N (the tuner's
+// TUNE_TRANSFER choice 2): N a multiple of 16, 16 or 32 block vectors per half, and
+// 16, 32 or 64 entries per half
+bool b_transfer_mfma_fits(const AggView& A, int N) {
+    const int nh = A.chiral ? 2 : 1;
+    if (A.k % nh || A.nd % nh || N % 16) return false;
+    const int kh = A.k / nh, kc = A.bx * A.by * (A.nd / nh);
+    return (kh == 16 || kh == 32) && (kc == 16 || kc == 32 || kc == 64);
+}
+static void restrict_mfma(const AggView& A, const c32* bv, const c32* fine, c32* coarse, int N, hipStream_t st) {
+    const int nh = A.chiral ? 2 : 1, kc = A.bx * A.by * (A.nd / nh);
+    const size_t tiles = (size_t)A.nbx * A.nby * nh * ((A.k / nh) >> 4) * (N >> 4);
+    const unsigned g = (unsigned)((tiles + kBlock / 64 - 1) / (kBlock / 64));
+    if (kc == 16) k_b_restrict_mfma<16><<<g, kBlock, 0, st>>>(A, bv, fine, coarse, N);
+    else if (kc == 32) k_b_restrict_mfma<32><<<g, kBlock, 0, st>>>(A, bv, fine, coarse, N);
+    else k_b_restrict_mfma<64><<<g, kBlock, 0, st>>>(A, bv, fine, coarse, N);
+}
+static void prolong_mfma(const AggView& A, const c32* bv, const c32* coarse, c32* fine, int accumulate, int N,
+                         hipStream_t st) {
+    const int nh = A.chiral ? 2 : 1, kh = A.k / nh, kc = A.bx * A.by * (A.nd / nh);
+    const size_t tiles = (size_t)A.nbx * A.nby * nh * (kc >> 4) * (N >> 4);
+    const unsigned g = (unsigned)((tiles + kBlock / 64 - 1) / (kBlock / 64));
+    if (kh == 16) k_b_prolong_mfma<16><<<g, kBlock, 0, st>>>(A, bv, coarse, fine, accumulate, N);
+    else k_b_prolong_mfma<32><<<g, kBlock, 0, st>>>(A, bv, coarse, fine, accumulate, N);
+}
 constexpr size_t kTransferLdsMax = 64 * 1024;
 // LDS bytes of the staged transfer, or 0 where it does not apply (sizes that are not
 // whole 16-byte runs, or more than kTransferLdsMax)
@@ -1639,7 +1753,15 @@ static int transfer_choice(const AggView& A, int N, size_t cbytes, bool prolong)
 template <typename T>
 void launch_b_prolong(const AggView& A, const cplx<T>* bv, const cplx<T>* coarse, cplx<T>* fine, int accumulate,
                       int N, hipStream_t st) {
-    if (transfer_choice(A, N, sizeof(cplx<T>), true) == 1) {
+    const int choice = transfer_choice(A, N, sizeof(cplx<T>), true);
+    if constexpr (std::is_same<T, float>::value) {
+        if (choice == 2 && b_transfer_mfma_fits(A, N)) {
+            prolong_mfma(A, bv, coarse, fine, accumulate, N, st);
+            SMG_LAUNCH_CHECK();
+            return;
+        }
+    }
+    if (choice == 1) {
         const size_t lds = b_transfer_lds_bytes(A, N, sizeof(cplx<T>), true);
         if (lds) {
             k_b_prolong_lds<T><<<A.nbx * A.nby, kBlock, lds, st>>>(A, bv, coarse, fine, accumulate, N);
@@ -1751,7 +1873,15 @@ static bool restrict_nc(const AggView& A, int nc, int g, const cplx<T>* bv, cons
 template <typename T>
 void launch_b_restrict(const AggView& A, const cplx<T>* bv, const cplx<T>* fine, cplx<T>* coarse, int N,
                        hipStream_t st) {
-    if (transfer_choice(A, N, sizeof(cplx<T>), false) == 1) {
+    const int choice = transfer_choice(A, N, sizeof(cplx<T>), false);
+    if constexpr (std::is_same<T, float>::value) {
+        if (choice == 2 && b_transfer_mfma_fits(A, N)) {
+            restrict_mfma(A, bv, fine, coarse, N, st);
+            SMG_LAUNCH_CHECK();
+            return;
+        }
+    }
+    if (choice == 1) {
         const size_t lds = b_transfer_lds_bytes(A, N, sizeof(cplx<T>), false);
         if (lds) {
             k_b_restrict_lds<T><<<A.nbx * A.nby, kBlock, lds, st>>>(A, bv, fine, coarse, N);

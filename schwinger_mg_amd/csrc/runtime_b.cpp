@@ -736,14 +736,17 @@ int tune_mg(MG& mg, int N, hipStream_t st) {
             launch_zero<T>(f.p(), nf, st);
             launch_zero<T>(c.p(), nc, st);
             const int prec = sizeof(T) == 8 ? 1 : 0;
-            if (b_transfer_lds_bytes(A, N, sizeof(cplx<T>), true))
-                pick(TuneKey{TUNE_TRANSFER, A.nx, A.ny, 128 * A.nd + A.k, N, 2 * prec + 1}, {0, 1}, [&] {
-                    launch_b_prolong<T>(A, mg.P[l]->bv<T>(), c.p(), f.p(), 1, N, st);
+            const bool mfma = sizeof(T) == 4 && b_transfer_mfma_fits(A, N);
+            for (int pro = 0; pro < 2; pro++) {
+                std::vector<int> cands{0};
+                if (b_transfer_lds_bytes(A, N, sizeof(cplx<T>), pro == 1)) cands.push_back(1);
+                if (mfma) cands.push_back(2);
+                if (cands.size() < 2) continue;
+                pick(TuneKey{TUNE_TRANSFER, A.nx, A.ny, 128 * A.nd + A.k, N, 2 * prec + pro}, cands, [&] {
+                    if (pro) launch_b_prolong<T>(A, mg.P[l]->bv<T>(), c.p(), f.p(), 1, N, st);
+                    else launch_b_restrict<T>(A, mg.P[l]->bv<T>(), f.p(), c.p(), N, st);
                 }, st);
-            if (b_transfer_lds_bytes(A, N, sizeof(cplx<T>), false))
-                pick(TuneKey{TUNE_TRANSFER, A.nx, A.ny, 128 * A.nd + A.k, N, 2 * prec}, {0, 1}, [&] {
-                    launch_b_restrict<T>(A, mg.P[l]->bv<T>(), f.p(), c.p(), N, st);
-                }, st);
+            }
         };
         transfers(0.0f);
         transfers(0.0);

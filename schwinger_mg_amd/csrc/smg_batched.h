@@ -67,6 +67,10 @@ void launch_b_coarse_apply(const CoarseView<TL>& C, cplx<T>* dst, const cplx<T>*
 // bytes per complex, or 0 where it does not apply; the tuner's TUNE_TRANSFER choice 1
 // selects it
 size_t b_transfer_lds_bytes(const AggView& A, int N, size_t cbytes, bool prolong);
+// whether the matrix-core transfers (FP32; the tuner's TUNE_TRANSFER choice 2) apply:
+// N a multiple of 16, 16 or 32 block vectors and 16, 32 or 64 fine entries per
+// chirality half of an aggregate
+bool b_transfer_mfma_fits(const AggView& A, int N);
 template <typename T>
 void launch_b_prolong(const AggView& A, const cplx<T>* bv, const cplx<T>* coarse, cplx<T>* fine, int accumulate,
                       int N, hipStream_t st);

@@ -21,7 +21,8 @@ std::map<TuneKey, int>& table() {
 bool valid(const TuneKey& k, int choice) {
     if (k[0] < 0 || k[0] >= TUNE_FAMILIES || choice < 0) return false;
     if (k[0] == TUNE_COARSE_BF16) return (choice >> 1) < kTuneNGrids;
-    if (k[0] == TUNE_DSLASH_STORE || k[0] == TUNE_TRANSFER) return choice <= 1;
+    if (k[0] == TUNE_DSLASH_STORE) return choice <= 1;
+    if (k[0] == TUNE_TRANSFER) return choice <= 2;
     if (k[0] == TUNE_DDAGD) return choice <= 2 * kTuneNGrids;   // 0, or 1 + 2 g + nt
     return choice < kTuneNGrids;
 }

@@ -26,7 +26,8 @@ enum TuneFamily : int {
     TUNE_CG_UPDATE = 3,
     // the batched transfers; shape (nx, ny, 128 nd + k, N, 2 (FP64) + (1 prolong, 0 restrict))
     // of the finer lattice; choice 1: a workgroup per aggregate with its blocks staged in
-    // LDS, 0: one work-item per output reading through the caches (the same sums)
+    // LDS, 0: one work-item per output reading through the caches (the same sums), 2
+    // (FP32): a wave per 16 x 16 tile on the matrix cores
     TUNE_TRANSFER = 4,
     // D^dag D of the batched path; shape (L, N, prec bytes, epi, 0); choice 0: two stencil
     // launches, 1 + 2 g + nt: one fused launch (kTuneGrids[g] workgroups with dot

@@ -82,17 +82,26 @@ def _numpy_cycle(mg, U, L, levels, k, hi, pre=3, post=3, half=False):
     return lambda b: cycle(0, b)
 
 
-@pytest.mark.parametrize("levels,L,N,pre,post", [(2, 16, 16, 3, 3), (3, 32, 32, 3, 3), (3, 32, 32, 1, 2),
-                                                 (2, 16, 16, 1, 1), (2, 16, 16, 2, 1)])
-def test_cheb_smoothed_cycle_vs_numpy(smg, levels, L, N, pre, post):
+@pytest.mark.parametrize("levels,L,N,pre,post,k,mfma", [(2, 16, 16, 3, 3, 16, 0), (3, 32, 32, 3, 3, 16, 0),
+                                                        (3, 32, 32, 1, 2, 16, 0), (2, 16, 16, 1, 1, 16, 0),
+                                                        (2, 16, 16, 2, 1, 16, 0), (3, 32, 32, 1, 2, 32, 1),
+                                                        (3, 32, 16, 3, 3, 32, 1)])
+def test_cheb_smoothed_cycle_vs_numpy(smg, levels, L, N, pre, post, k, mfma):
     """FP32 links at level 1 (its smoothing through the apply and separate passes), the
     fine level's steps one fused D^dag D launch each (L a multiple of 16, N of 16; the
     first step of each smoothing also doing its initial pass, from x = 0 or from the
     residual), the dense coarsest inverse: one cycle on N right-hand sides equals the
-    FP64 numpy cycle to FP32 accuracy, and differs clearly from the MR-smoothed cycle."""
-    banner(f"Chebyshev-smoothed cycle vs numpy levels={levels} L={L} N={N} pre={pre} post={post}")
-    k = 16
+    FP64 numpy cycle to FP32 accuracy, and differs clearly from the MR-smoothed cycle.
+    With mfma, every restriction and prolongation runs on the matrix cores (k = 32:
+    16 block vectors per chirality)."""
+    banner(f"Chebyshev-smoothed cycle vs numpy levels={levels} L={L} N={N} pre={pre} post={post} k={k} "
+           f"mfma={mfma}")
     U, D, mg = _hierarchy(smg, L, levels, k, 400 + L, coarse_half=0)
+    smg.tune_clear()
+    if mfma:   # the transfers' fine lattices: (L, L, 2 dof) and (L/4, L/4, k dof)
+        for nx, w in ((L, 128 * 2 + k), (L // 4, 128 * k + k))[:levels - 1]:
+            for direction in (0, 1):
+                smg.tune_set(smg.TUNE_TRANSFER, [nx, nx, w, N, direction], 2)
     mg.set_cycle(pre, post, 30, 0.1)
     B = gaussian_vecs(N, 2 * L * L, 410 + L)
     x_mr = mg.precondition_batched(B, smg.F32)
@@ -118,6 +127,7 @@ def test_cheb_smoothed_cycle_vs_numpy(smg, levels, L, N, pre, post):
     assert relerr(X, x_mr) > 1e-2
     mg.set_cheb_smoother(0.0)
     assert relerr(mg.precondition_batched(B, smg.F32), x_mr) < 1e-12   # MR restored
+    smg.tune_clear()
 
 
 @pytest.mark.parametrize("N,pre,post", [(16, 3, 3), (32, 3, 3), (32, 1, 2), (32, 1, 1)])

@@ -155,3 +155,35 @@ def test_staged_transfers_same_cycle(smg, clean_table, prec):
     assert e < (1e-13 if prec == "f64" else 1e-5)
     assert out[0][1] == out[1][1] and out[1][2].max() < 1e-10
 
+
+
+@pytest.mark.parametrize("N", [16, 32])
+def test_mfma_transfers_same_cycle(smg, clean_table, N):
+    """The FP32 restriction and prolongation on the matrix cores (a wave per 16 x 16 tile
+    of an aggregate's chirality half; k = 32, so 16 block vectors per half) against the
+    per-output kernels: the batched three-level cycle agrees to FP32 roundoff (the same
+    products, summed in another order), and the mixed batched solve takes the same
+    iterations to a 1e-10 FP64 true residual."""
+    banner(f"matrix-core transfers: batched cycle N={N}")
+    L, k = 32, 32
+    V = L * L
+    U = random_gauge(L, 0.4, 350)
+    D = smg.DiracOp(L, U, 0.05, 1.0, 1.0, 0.0)
+    dims = [2 * V, (L // 4) ** 2 * k]
+    starts = np.concatenate([gaussian_vecs(k // 2, n, 351 + i).ravel() for i, n in enumerate(dims)])
+    mg = smg.MGHierarchy(D, starts, n_levels=3, blocks=((4, 4), (2, 2)), k=(k, k), null_iters=6, w_cycle=False,
+                         coarse_maxit=30, coarse_tol=0.1, coarse_half=True, chiral=True)
+    B = gaussian_vecs(N, 2 * V, 352)
+    shapes = [(L, L, 128 * 2 + k), (L // 4, L // 4, 128 * k + k)]
+    out = []
+    for choice in (0, 2):
+        for nx, ny, w in shapes:
+            for direction in (0, 1):
+                smg.tune_set(smg.TUNE_TRANSFER, [nx, ny, w, N, direction], choice)
+        X = mg.precondition_batched(B, smg.F32)
+        Xs, info, tr = smg.gcr_solve_batched(D, B, mg, 200, 1e-10, trunc=10, prec=smg.F32)
+        out.append((X, info.iterations, tr))
+    e = relerr(out[1][0], out[0][0])
+    print(f"cycle matrix-core vs per-output transfers: relerr {e:.2e}; iterations {out[0][1]} / {out[1][1]}")
+    assert e < 1e-5
+    assert abs(out[0][1] - out[1][1]) <= 1 and out[1][2].max() < 1e-10
