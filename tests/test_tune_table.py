@@ -39,8 +39,9 @@ def test_refusals(smg, tmp_path):
         smg.tune_set(smg.TUNE_DSLASH_DOTS, [512, 32, 4, 0, 0], 5)      # no sixth grid
     with pytest.raises(smg.SMGError):
         smg.tune_set(7, [1, 2, 3, 4, 5], 0)                              # no such family
+    smg.tune_set(smg.TUNE_TRANSFER, [512, 512, 128 * 2 + 32, 32, 1], 2)       # matrix cores
     with pytest.raises(smg.SMGError):
-        smg.tune_set(smg.TUNE_TRANSFER, [512, 512, 128 * 2 + 32, 32, 1], 2)   # choices 0 and 1
+        smg.tune_set(smg.TUNE_TRANSFER, [512, 512, 128 * 2 + 32, 32, 1], 3)   # choices 0, 1, 2
     with pytest.raises(smg.SMGError):
         smg.tune_set(smg.TUNE_CG_UPDATE, [131072, 32, 0, 0, 0], 5)         # no sixth grid
     smg.tune_set(smg.TUNE_DDAGD, [512, 32, 4, 2, 0], 10)                   # fused, fifth grid, NT
