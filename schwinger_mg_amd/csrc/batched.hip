@@ -978,6 +978,7 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
     c32* const xwf = const_cast<c32*>(aux);
     const int cmode = fin.cheb_mode;   // EPI_CHEB: 0 r, d and x; 1 r only; 2 x only
     const int cst = HV ? 0 : fin.cheb_start;
+    const int hxacc = HV ? fin.cheb_xacc : 0;   // BF16 Chebyshev: FinB's cheb_xacc
     const bool need_d = cmode != 1 || cst != 0, need_x = cst != 1 && (cmode != 1 || cst == 2);
     constexpr bool RED = EPI == EPI_DOTS || EPI == EPI_CG;
     constexpr int NC = EPI == EPI_CG ? 5 : 3;   // reduced values per RHS
@@ -1135,7 +1136,8 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                             for (int u = 0; u < NL; u++) {
                                 const size_t o = ((size_t)b * K + 16 * r + 4 * hi + v) * N + 32 * u + 2 * lo;
                                 if constexpr (HV) {
-                                    epo[r][v][u][0] = *reinterpret_cast<const uint2*>(auxh + o);
+                                    if (EPI != EPI_CHEB || hxacc != 1)
+                                        epo[r][v][u][0] = *reinterpret_cast<const uint2*>(auxh + o);
                                     if constexpr (EPI == EPI_CG || EPI == EPI_CHEB)
                                         epo[r][v][u][1] = *reinterpret_cast<const uint2*>(bh2 + o);
                                     if constexpr (EPI == EPI_CHEB)   // the previous direction at this site
@@ -1195,10 +1197,16 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                             const c32 r0 = unpack_h(rw.x) - q0, r1 = unpack_h(rw.y) - q1;
                             const c32 d0 = fin.cheb_c1 * unpack_h(dw.x) + fin.cheb_c2 * r0;
                             const c32 d1 = fin.cheb_c1 * unpack_h(dw.y) + fin.cheb_c2 * r1;
-                            const c32 x0 = unpack_h(xw.x) + d0, x1 = unpack_h(xw.y) + d1;
                             *reinterpret_cast<uint2*>(rwh + idx) = uint2{pack_h(r0), pack_h(r1)};
                             *reinterpret_cast<uint2*>(dsth + idx) = uint2{pack_h(d0), pack_h(d1)};
-                            *reinterpret_cast<uint2*>(xwh + idx) = uint2{pack_h(x0), pack_h(x1)};
+                            if (hxacc != 1) {   // x += d_new (and d when the last step left x alone)
+                                c32 x0 = unpack_h(xw.x) + d0, x1 = unpack_h(xw.y) + d1;
+                                if (hxacc == 2) {
+                                    x0 = x0 + unpack_h(dw.x);
+                                    x1 = x1 + unpack_h(dw.y);
+                                }
+                                *reinterpret_cast<uint2*>(xwh + idx) = uint2{pack_h(x0), pack_h(x1)};
+                            }
                         } else if constexpr (PAIR && !HV && EPI == EPI_CHEB) {
                             // FP32 vectors: r -= s q, then (mode) d_new = c1 d + c2 r, x += d_new
                             const size_t idx = row + 32 * u + 2 * lo;
@@ -1234,11 +1242,11 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                             const size_t idx = row + 16 * u + lo;
                             const c32 q{acc[r][u][0][v], acc[r][u][1][v]};
                             const c32 rn = unpack_h(bh2[idx]) - q;
-                            const c32 dn = fin.cheb_c1 * unpack_h(srch[idx]) + fin.cheb_c2 * rn;
-                            const c32 xn = unpack_h(auxh[idx]) + dn;
+                            const c32 dp = unpack_h(srch[idx]);
+                            const c32 dn = fin.cheb_c1 * dp + fin.cheb_c2 * rn;
                             rwh[idx] = pack_h(rn);
                             dsth[idx] = pack_h(dn);
-                            xwh[idx] = pack_h(xn);
+                            if (hxacc != 1) xwh[idx] = pack_h(unpack_h(auxh[idx]) + dn + (hxacc == 2 ? dp : c32{0.0f, 0.0f}));
                         } else if constexpr (PAIR && HV) {
                             const size_t idx = row + 32 * u + 2 * lo;
                             const unsigned w0 = pack_h(c32{acc[r][2 * u][0][v], acc[r][2 * u][1][v]});
@@ -1464,7 +1472,8 @@ void launch_b_coarse_apply_h(const CoarseView<ch16>& C, cb16* dst, const cb16* s
     // the Chebyshev step writes r (b) and x (aux) in place and the new direction to dst,
     // while other workgroups read the old direction (src) at their neighbours
     SMG_REQUIRE(epi != EPI_CHEB || (aux && b && dst != src && aux != src && b != src && dst != aux && dst != b &&
-                                    aux != b && fin.cheb_start == 0 && !fin.cheb_rout && fin.cheb_mode == 0),
+                                    aux != b && fin.cheb_start == 0 && !fin.cheb_rout && fin.cheb_mode == 0 &&
+                                    fin.cheb_xacc >= 0 && fin.cheb_xacc <= 2),
                 "BF16-vector coarse apply: the Chebyshev epilogue takes distinct r (b), x (aux), d (src) and "
                 "d_new (dst)");
     auto go = [&](auto kc, auto ntc) {

@@ -300,6 +300,8 @@ static void cheb_h(const CoarseOperator& C, const c32* b, c32* x, cb16* r, cb16*
         FinB f;
         f.cheb_c1 = (float)(rho_k * rho);
         f.cheb_c2 = (float)(2.0 * rho_k / delta);
+        // x is read and written every other step (x += d_{k-1} + d_k), and at the last
+        f.cheb_xacc = (k % 2 == 1 && k < steps) ? 1 : (k % 2 == 0 ? 2 : 0);
         launch_b_coarse_apply_h(v, dnew, dold, EPI_CHEB, xh, f, N, st, r);
         std::swap(dold, dnew);
         rho = rho_k;

@@ -44,6 +44,9 @@ struct FinB {
     int cheb_start = 0;
     float cheb_s = 1.0f;
     void* cheb_rout = nullptr;
+    // BF16 vectors: 0 x += d_new; 1 x left alone (read and written every other step);
+    // 2 x += d + d_new, d the direction the step reads (the previous step's d_new)
+    int cheb_xacc = 0;
 };
 
 // Per-RHS coefficient array views (device): complex a_j at s[2j], s[2j+1].
