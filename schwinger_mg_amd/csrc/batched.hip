@@ -2058,29 +2058,50 @@ void launch_b_dots(const cplx<T>* a, const cplx<T>* b, size_t n, const FinB& fin
     SMG_LAUNCH_CHECK();
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_b_multi_dot(VecPtrs<T> V, int nv, const cplx<T>* __restrict__ w,
-                                                        size_t items, FinB fin, int N) {
-    double v[32];
+// <V_i, w> for i < NV per right-hand side, runs of P right-hand sides per work-item
+// (16-byte accesses in FP32); NV is a compile-time count so that only its 2 NV sums
+// are held
+template <typename T, int NV, int P>
+__global__ __launch_bounds__(kBlock) void k_b_multi_dot(VecPtrs<T> V, const cplx<T>* __restrict__ w, size_t runs,
+                                                        FinB fin, int N) {
+    double acc[2 * NV][P] = {};
+    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < runs; t += (size_t)gridDim.x * kBlock) {
+        RhsRun<T, P> we;
+        we.load(w + t * P);
 #pragma unroll
-    for (int c = 0; c < 32; c++) v[c] = 0.0;
-    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < items; t += (size_t)gridDim.x * kBlock) {
-        const cplx<T> we = w[t];
+        for (int i = 0; i < NV; i++) {
+            RhsRun<T, P> ve;
+            ve.load(V.v[i] + t * P);
 #pragma unroll
-        for (int i = 0; i < 16; i++)
-            if (i < nv) {
-                const cplx<T> p = cdot(V.v[i][t], we);
-                v[2 * i] += (double)p.re;
-                v[2 * i + 1] += (double)p.im;
+            for (int q = 0; q < P; q++) {
+                const cplx<T> d = cdot(ve.v[q], we.v[q]);
+                acc[2 * i][q] += (double)d.re;
+                acc[2 * i + 1][q] += (double)d.im;
             }
+        }
     }
-    reduce_finish_b<32, 32>(v, fin, N, gridDim.x, blockIdx.x);
+    double v[2 * NV];
+    runs_to_lanes<2 * NV, P>(acc, v, N);
+    reduce_finish_b<2 * NV, 32>(v, fin, N, gridDim.x, blockIdx.x);
+}
+template <typename T, int NV>
+static void multi_dot_nv(const VecPtrs<T>& V, int nv, const cplx<T>* w, size_t n, const FinB& fin, int N,
+                         hipStream_t st) {
+    if (nv == NV) {
+        run_dispatch<T>(N, [&](auto pc) {
+            constexpr int P = decltype(pc)::value;
+            k_b_multi_dot<T, NV, P><<<b_grid(n * N / P, kRedGridMulti), kBlock, 0, st>>>(V, w, n * N / P, fin, N);
+        });
+        return;
+    }
+    if constexpr (NV > 1) multi_dot_nv<T, NV - 1>(V, nv, w, n, fin, N, st);
 }
 template <typename T>
 void launch_b_multi_dot(const VecPtrs<T>& V, int nv, const cplx<T>* w, size_t n, const FinB& fin, int N,
                         hipStream_t st) {
     SMG_REQUIRE(nv >= 1 && nv <= 16 && N <= 32, "batched multi_dot: 1..16 vectors, N <= 32");
-    k_b_multi_dot<T><<<b_grid(n * N, kRedGridMulti), kBlock, 0, st>>>(V, nv, w, n * N, fin, N);
+    SMG_REQUIRE(fin.ostride >= 2 * nv, "batched multi_dot: the output stride holds 2 nv sums per RHS");
+    multi_dot_nv<T, 16>(V, nv, w, n, fin, N, st);
     SMG_LAUNCH_CHECK();
 }
 
@@ -2088,10 +2109,10 @@ template <typename T>
 struct ApapB {
     const double* a[16];
 };
-template <typename T>
+template <typename T, int P>
 __global__ __launch_bounds__(kBlock) void k_b_gcr_orth(cplx<T>* __restrict__ p, cplx<T>* __restrict__ Ap,
-                                                       VecPtrs<T> P, VecPtrs<T> AP, int nv,
-                                                       const double* __restrict__ dots, ApapB<T> apap, size_t items,
+                                                       VecPtrs<T> Pv, VecPtrs<T> APv, int nv,
+                                                       const double* __restrict__ dots, ApapB<T> apap, size_t runs,
                                                        int N) {
     __shared__ cplx<T> beta[16 * 64];
     for (int q = threadIdx.x; q < nv * N; q += kBlock) {
@@ -2101,16 +2122,25 @@ __global__ __launch_bounds__(kBlock) void k_b_gcr_orth(cplx<T>* __restrict__ p, 
         beta[i * N + j] = d > 0.0 ? cplx<T>{(T)(dd[0] / d), (T)(dd[1] / d)} : cplx<T>{(T)0, (T)0};
     }
     __syncthreads();
-    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < items; t += (size_t)gridDim.x * kBlock) {
-        const int j = (int)(t & (N - 1));
-        cplx<T> pe = p[t], ae = Ap[t];
+    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < runs; t += (size_t)gridDim.x * kBlock) {
+        const size_t e = t * P;
+        const int j = (int)(e & (N - 1));
+        RhsRun<T, P> pe, ae;
+        pe.load(p + e);
+        ae.load(Ap + e);
         for (int i = 0; i < nv; i++) {
-            const cplx<T> bb = beta[i * N + j];
-            pe = pe - bb * P.v[i][t];
-            ae = ae - bb * AP.v[i][t];
+            RhsRun<T, P> pi, ai;
+            pi.load(Pv.v[i] + e);
+            ai.load(APv.v[i] + e);
+#pragma unroll
+            for (int q = 0; q < P; q++) {
+                const cplx<T> bb = beta[i * N + j + q];
+                pe.v[q] = pe.v[q] - bb * pi.v[q];
+                ae.v[q] = ae.v[q] - bb * ai.v[q];
+            }
         }
-        p[t] = pe;
-        Ap[t] = ae;
+        pe.store(p + e);
+        ae.store(Ap + e);
     }
 }
 template <typename T>
@@ -2118,7 +2148,10 @@ void launch_b_gcr_orth(cplx<T>* p, cplx<T>* Ap, const VecPtrs<T>& P, const VecPt
                        const double* dots, const double* const* apap, size_t n, int N, hipStream_t st) {
     ApapB<T> a;
     for (int i = 0; i < nv; i++) a.a[i] = apap[i];
-    k_b_gcr_orth<T><<<b_grid(n * N, 1 << 14), kBlock, 0, st>>>(p, Ap, P, AP, nv, dots, a, n * N, N);
+    run_dispatch<T>(N, [&](auto pc) {
+        constexpr int RP = decltype(pc)::value;
+        k_b_gcr_orth<T, RP><<<b_grid(n * N / RP, 1 << 14), kBlock, 0, st>>>(p, Ap, P, AP, nv, dots, a, n * N / RP, N);
+    });
     SMG_LAUNCH_CHECK();
 }
 
