@@ -2109,11 +2109,13 @@ template <typename T>
 struct ApapB {
     const double* a[16];
 };
-template <typename T, int P>
+// With DOTS the new Ap's sums {<Ap, r>, |Ap|^2} per RHS are reduced into fin as well
+template <typename T, int P, bool DOTS>
 __global__ __launch_bounds__(kBlock) void k_b_gcr_orth(cplx<T>* __restrict__ p, cplx<T>* __restrict__ Ap,
                                                        VecPtrs<T> Pv, VecPtrs<T> APv, int nv,
                                                        const double* __restrict__ dots, ApapB<T> apap, size_t runs,
-                                                       int N) {
+                                                       int N, const cplx<T>* __restrict__ r, FinB fin) {
+    double acc[3][P] = {};
     __shared__ cplx<T> beta[16 * 64];
     for (int q = threadIdx.x; q < nv * N; q += kBlock) {
         const int i = q / N, j = q % N;
@@ -2141,16 +2143,40 @@ __global__ __launch_bounds__(kBlock) void k_b_gcr_orth(cplx<T>* __restrict__ p, 
         }
         pe.store(p + e);
         ae.store(Ap + e);
+        if constexpr (DOTS) {
+            RhsRun<T, P> re;
+            re.load(r + e);
+#pragma unroll
+            for (int q = 0; q < P; q++) {
+                const cplx<T> d = cdot(ae.v[q], re.v[q]);
+                acc[0][q] += (double)d.re;
+                acc[1][q] += (double)d.im;
+                acc[2][q] += (double)norm2(ae.v[q]);
+            }
+        }
+    }
+    if constexpr (DOTS) {
+        double v[3];
+        runs_to_lanes<3, P>(acc, v, N);
+        reduce_finish_b<3, 64>(v, fin, N, gridDim.x, blockIdx.x);
     }
 }
 template <typename T>
 void launch_b_gcr_orth(cplx<T>* p, cplx<T>* Ap, const VecPtrs<T>& P, const VecPtrs<T>& AP, int nv,
-                       const double* dots, const double* const* apap, size_t n, int N, hipStream_t st) {
+                       const double* dots, const double* const* apap, size_t n, int N, hipStream_t st,
+                       const cplx<T>* r, const FinB* fin) {
+    SMG_REQUIRE(nv >= 1 && nv <= 16 && N <= 64, "batched GCR orthogonalisation: 1..16 directions, N <= 64");
     ApapB<T> a;
     for (int i = 0; i < nv; i++) a.a[i] = apap[i];
     run_dispatch<T>(N, [&](auto pc) {
         constexpr int RP = decltype(pc)::value;
-        k_b_gcr_orth<T, RP><<<b_grid(n * N / RP, 1 << 14), kBlock, 0, st>>>(p, Ap, P, AP, nv, dots, a, n * N / RP, N);
+        const size_t runs = n * N / RP;
+        if (fin)
+            k_b_gcr_orth<T, RP, true><<<b_grid(runs, kRedGridVec), kBlock, 0, st>>>(p, Ap, P, AP, nv, dots, a, runs,
+                                                                                    N, r, *fin);
+        else
+            k_b_gcr_orth<T, RP, false><<<b_grid(runs, 1 << 14), kBlock, 0, st>>>(p, Ap, P, AP, nv, dots, a, runs, N,
+                                                                                 r, FinB());
     });
     SMG_LAUNCH_CHECK();
 }
@@ -2577,7 +2603,8 @@ void launch_b_accum_mixed(c64* x, const c32* e, size_t n, int N, hipStream_t st)
     template void launch_b_multi_dot<T>(const VecPtrs<T>&, int, const cplx<T>*, size_t, const FinB&, int,        \
                                         hipStream_t);                                                            \
     template void launch_b_gcr_orth<T>(cplx<T>*, cplx<T>*, const VecPtrs<T>&, const VecPtrs<T>&, int,            \
-                                       const double*, const double* const*, size_t, int, hipStream_t);           \
+                                       const double*, const double* const*, size_t, int, hipStream_t,            \
+                                       const cplx<T>*, const FinB*);                                             \
     template void launch_b_mr_update<T>(cplx<T>*, cplx<T>*, const cplx<T>*, const double*, size_t, int,          \
                                         hipStream_t, int, const cplx<T>*);                                       \
     template void launch_b_update<T>(cplx<T>*, cplx<T>*, const cplx<T>*, const cplx<T>*, const double*, size_t,  \

@@ -539,8 +539,12 @@ int gcr_solve_b(LinOpB<T>& A, const cplx<T>* b, cplx<T>* x, MG* mg, int maxit, d
         cplx<T>* Ap = APd[slot].p();
         if (mg) batched_cycle<T>(*mg, N).run(0, r.p(), p, st);
         else bcopy(p, r.p(), nN, st);
-        A.apply(Ap, p, st);
         const int nd = (int)order.size();
+        // alpha = <Ap, r> / |Ap|^2 from the sums of the pass that forms Ap last: the apply
+        // (no earlier directions) or the orthogonalisation
+        const FinB fa = finb(FIN_GCR_ALPHA, alpha, 2, apap + (size_t)slot * N, 1);
+        if (nd == 0) A.apply(Ap, p, EPI_DOTS, nullptr, r.p(), fa, st);
+        else A.apply(Ap, p, st);
         if (nd > 0) {
             VecPtrs<T> Pv, APv;
             const double* ap[16];
@@ -550,9 +554,8 @@ int gcr_solve_b(LinOpB<T>& A, const cplx<T>* b, cplx<T>* x, MG* mg, int maxit, d
                 ap[i] = apap + (size_t)order[i] * N;
             }
             launch_b_multi_dot<T>(APv, nd, Ap, n, finb(FIN_STORE, dots, 32), N, st);
-            launch_b_gcr_orth<T>(p, Ap, Pv, APv, nd, dots, ap, n, N, st);
+            launch_b_gcr_orth<T>(p, Ap, Pv, APv, nd, dots, ap, n, N, st, r.p(), &fa);
         }
-        launch_b_dots<T>(Ap, r.p(), n, finb(FIN_GCR_ALPHA, alpha, 2, apap + (size_t)slot * N, 1), N, st);
         launch_b_update<T>(x, r.p(), p, Ap, alpha, n, finb(FIN_STORE, rr, 1), N, st);
         cx.read(rr, rn.data(), N);
         for (int j = 0; j < N; j++) rn[j] = std::sqrt(rn[j]);

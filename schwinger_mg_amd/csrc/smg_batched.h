@@ -96,10 +96,12 @@ void launch_b_dots(const cplx<T>* a, const cplx<T>* b, size_t n, const FinB& fin
 template <typename T>
 void launch_b_multi_dot(const VecPtrs<T>& V, int nv, const cplx<T>* w, size_t n, const FinB& fin, int N,
                         hipStream_t st);
-// p -= sum_i beta_ij P_i ; Ap -= sum_i beta_ij AP_i, beta_ij = dots[j*2*nv + 2i..] / apap_i[j]
+// p -= sum_i beta_ij P_i ; Ap -= sum_i beta_ij AP_i, beta_ij = dots[j*32 + 2i..] / apap_i[j];
+// with fin, also {<Ap, r>, |Ap|^2} of the new Ap per RHS reduced into fin
 template <typename T>
 void launch_b_gcr_orth(cplx<T>* p, cplx<T>* Ap, const VecPtrs<T>& P, const VecPtrs<T>& AP, int nv,
-                       const double* dots, const double* const* apap, size_t n, int N, hipStream_t st);
+                       const double* dots, const double* const* apap, size_t n, int N, hipStream_t st,
+                       const cplx<T>* r = nullptr, const FinB* fin = nullptr);
 // mode 0: x += w_j r ; r -= w_j Ar   (w_j at s[2j..]); 1: x = w_j b ; r = b - w_j Ar (the
 // first sweep from x = 0); 2: x += w_j r only (r is not read again)
 template <typename T>
