@@ -456,15 +456,15 @@ int smg_mg_cheb_interval(smg_mg* mg, double* lo, double* hi);
  * estimates it again: a lo above the smallest eigenvalue trades accuracy on the lowest
  * modes for a faster fall of the rest (no reference counterpart) */
 int smg_mg_set_cheb_interval(smg_mg* mg, double lo, double hi);
-/* Chebyshev smoothing in the batched FP32 cycle (ratio > 0; 0 restores MR): every
- * non-coarsest level smooths with pre_smooth / post_smooth Chebyshev steps on
+/* Chebyshev smoothing in the batched cycles, FP32 and FP64 (ratio > 0; 0 restores MR):
+ * every non-coarsest level smooths with pre_smooth / post_smooth Chebyshev steps on
  * [hi / ratio, hi], hi = 1.1 x a 40-step power-iteration estimate of the level's largest
- * eigenvalue (re-estimated by smg_mg_rebuild and smg_mg_refresh); the FP64 and
+ * eigenvalue (re-estimated by smg_mg_rebuild and smg_mg_refresh); the
  * single-right-hand-side cycles keep MR (src/smoother.cpp:9-40).  fused = 1: each step is
  * one launch of the level's operator with the vector updates in its epilogue where the
- * operator has that form (the fine D^dag D with L a multiple of 16 and nrhs of 16; the
- * FP16-link coarse operators with k in {8, 16, 32}), else the apply and separate
- * passes; fused = 0: always the latter.  hi (optional, n_levels - 1 entries) receives
+ * operator has that form (the fine D^dag D with L a multiple of 16 and nrhs a multiple
+ * of 16 in FP32, 8 in FP64; the FP16-link coarse operators with k in {8, 16, 32} in the
+ * FP32 cycle), else the apply and separate passes; fused = 0: always the latter.  hi (optional, n_levels - 1 entries) receives
  * each level's upper end.  No reference counterpart. */
 int smg_mg_set_cheb_smoother(smg_mg* mg, double ratio, int fused, double* hi);
 /* the hierarchy's level-`level` (>= 1) coarse operator on nrhs device vectors in the

@@ -690,7 +690,7 @@ class MGHierarchy:
 
     def set_cheb_smoother(self, ratio, fused=True):
         """Chebyshev smoothing on [hi / ratio, hi] at every non-coarsest level of the batched
-        FP32 cycle (ratio > 0); 0 restores MR.  fused: each step one launch of the level's
+        cycles, FP32 and FP64 (ratio > 0); 0 restores MR.  fused: each step one launch of the level's
         operator where it has that form, else the apply and separate vector passes.
         Returns each non-coarsest level's hi."""
         hi = np.zeros(self.n_levels - 1)

@@ -976,6 +976,7 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
     // FP32 vectors: the new residual to fin.cheb_rout where given (FinB's cheb_start)
     c32* const rwf = fin.cheb_rout ? static_cast<c32*>(fin.cheb_rout) : const_cast<c32*>(bvec);
     c32* const xwf = const_cast<c32*>(aux);
+    const float cc1 = (float)fin.cheb_c1, cc2 = (float)fin.cheb_c2, ccs = (float)fin.cheb_s;
     const int cmode = fin.cheb_mode;   // EPI_CHEB: 0 r, d and x; 1 r only; 2 x only
     const int cst = HV ? 0 : fin.cheb_start;
     const int hxacc = HV ? fin.cheb_xacc : 0;   // BF16 Chebyshev: FinB's cheb_xacc
@@ -1195,8 +1196,8 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                             const c32 q0{acc[r][2 * u][0][v], acc[r][2 * u][1][v]};
                             const c32 q1{acc[r][2 * u + 1][0][v], acc[r][2 * u + 1][1][v]};
                             const c32 r0 = unpack_h(rw.x) - q0, r1 = unpack_h(rw.y) - q1;
-                            const c32 d0 = fin.cheb_c1 * unpack_h(dw.x) + fin.cheb_c2 * r0;
-                            const c32 d1 = fin.cheb_c1 * unpack_h(dw.y) + fin.cheb_c2 * r1;
+                            const c32 d0 = cc1 * unpack_h(dw.x) + cc2 * r0;
+                            const c32 d1 = cc1 * unpack_h(dw.y) + cc2 * r1;
                             *reinterpret_cast<uint2*>(rwh + idx) = uint2{pack_h(r0), pack_h(r1)};
                             *reinterpret_cast<uint2*>(dsth + idx) = uint2{pack_h(d0), pack_h(d1)};
                             if (hxacc != 1) {   // x += d_new (and d when the last step left x alone)
@@ -1211,7 +1212,7 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                             // FP32 vectors: r -= s q, then (mode) d_new = c1 d + c2 r, x += d_new
                             const size_t idx = row + 32 * u + 2 * lo;
                             const float4 rw = epo[r][v][u][1];
-                            const float sc = fin.cheb_s;
+                            const float sc = ccs;
                             const c32 r0 = c32{rw.x, rw.y} - sc * c32{acc[r][2 * u][0][v], acc[r][2 * u][1][v]};
                             const c32 r1 =
                                 c32{rw.z, rw.w} - sc * c32{acc[r][2 * u + 1][0][v], acc[r][2 * u + 1][1][v]};
@@ -1221,20 +1222,20 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                                 const c32 p0 = sc * c32{dw.x, dw.y}, p1 = sc * c32{dw.z, dw.w};
                                 const c32 x0 = cst == 1 ? p0 : (cst == 2 ? c32{xw.x, xw.y} + p0 : c32{xw.x, xw.y});
                                 const c32 x1 = cst == 1 ? p1 : (cst == 2 ? c32{xw.z, xw.w} + p1 : c32{xw.z, xw.w});
-                                const c32 d0 = fin.cheb_c1 * p0 + fin.cheb_c2 * r0;
-                                const c32 d1 = fin.cheb_c1 * p1 + fin.cheb_c2 * r1;
+                                const c32 d0 = cc1 * p0 + cc2 * r0;
+                                const c32 d1 = cc1 * p1 + cc2 * r1;
                                 if (cmode == 0) *reinterpret_cast<float4*>(dst + idx) = float4{d0.re, d0.im, d1.re, d1.im};
                                 const c32 y0 = cmode == 1 ? x0 : x0 + d0, y1 = cmode == 1 ? x1 : x1 + d1;
                                 *reinterpret_cast<float4*>(xwf + idx) = float4{y0.re, y0.im, y1.re, y1.im};
                             }
                         } else if constexpr (!HV && EPI == EPI_CHEB) {
                             const size_t idx = row + 16 * u + lo;
-                            const c32 rn = bvec[idx] - fin.cheb_s * c32{acc[r][u][0][v], acc[r][u][1][v]};
+                            const c32 rn = bvec[idx] - ccs * c32{acc[r][u][0][v], acc[r][u][1][v]};
                             if (cmode != 2) rwf[idx] = rn;
                             if (need_d) {
-                                const c32 dp = fin.cheb_s * src[idx];
+                                const c32 dp = ccs * src[idx];
                                 const c32 xp = cst == 1 ? dp : (cst == 2 ? aux[idx] + dp : aux[idx]);
-                                const c32 dn = fin.cheb_c1 * dp + fin.cheb_c2 * rn;
+                                const c32 dn = cc1 * dp + cc2 * rn;
                                 if (cmode == 0) dst[idx] = dn;
                                 xwf[idx] = cmode == 1 ? xp : xp + dn;
                             }
@@ -1243,7 +1244,7 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                             const c32 q{acc[r][u][0][v], acc[r][u][1][v]};
                             const c32 rn = unpack_h(bh2[idx]) - q;
                             const c32 dp = unpack_h(srch[idx]);
-                            const c32 dn = fin.cheb_c1 * dp + fin.cheb_c2 * rn;
+                            const c32 dn = cc1 * dp + cc2 * rn;
                             rwh[idx] = pack_h(rn);
                             dsth[idx] = pack_h(dn);
                             if (hxacc != 1) xwh[idx] = pack_h(unpack_h(auxh[idx]) + dn + (hxacc == 2 ? dp : c32{0.0f, 0.0f}));

@@ -932,8 +932,8 @@ int smg_mg_coarse_apply_dev(smg_mg* mg, int level, int prec, int nrhs, const voi
             const size_t bytes = C.n() * nrhs * sizeof(cb16);
             if (!rx || rx->bytes < 2 * bytes) rx.reset(new DevMem(2 * bytes));
             FinB fc;
-            fc.cheb_c1 = 0.5f;
-            fc.cheb_c2 = 0.25f;
+            fc.cheb_c1 = 0.5;
+            fc.cheb_c2 = 0.25;
             cb16* r = rx->as<cb16>();
             launch_b_coarse_apply_h(C.view16(), (cb16*)dst, (const cb16*)src, EPI_CHEB, r + C.n() * nrhs, fc, nrhs,
                                     S(), r);

@@ -268,7 +268,7 @@ struct MG : Preconditioner {
     static constexpr int kChebLanczos = 60;
     static constexpr double kChebMargin = 1.1;
     void cheb_bounds(hipStream_t st);
-    // Chebyshev smoothing in the batched FP32 cycle (ratio > 0): level l's smoother is the
+    // Chebyshev smoothing in the batched cycles (ratio > 0): level l's smoother is the
     // Chebyshev iteration on [hi_l / ratio, hi_l], hi_l = kChebMargin x the power-iteration
     // estimate of level l's largest eigenvalue; MR otherwise.  Fused: each step one launch
     // of the level's operator with the updates in its epilogue, where the operator has

@@ -176,12 +176,12 @@ static void cheb_smooth_b(LinOpB<T>& A, cplx<T>* x, const cplx<T>* b, cplx<T>* r
         for (int k = 1; k <= niter; k++) {
             const double rho_k = 1.0 / (2.0 * sigma - rho);
             FinB f;
-            f.cheb_c1 = (float)(rho_k * rho);
-            f.cheb_c2 = (float)(2.0 * rho_k / delta);
+            f.cheb_c1 = rho_k * rho;
+            f.cheb_c2 = 2.0 * rho_k / delta;
             f.cheb_mode = k < niter ? 0 : (need_r ? 1 : 2);
             if (k == 1) {
                 f.cheb_start = zero_init ? 1 : 2;
-                f.cheb_s = (float)(1.0 / theta);
+                f.cheb_s = 1.0 / theta;
                 f.cheb_rout = rn;
             }
             cplx<T>* dn = nullptr;
@@ -298,8 +298,8 @@ static void cheb_h(const CoarseOperator& C, const c32* b, c32* x, cb16* r, cb16*
     for (int k = 1; k <= steps; k++) {
         const double rho_k = 1.0 / (2.0 * sigma - rho);
         FinB f;
-        f.cheb_c1 = (float)(rho_k * rho);
-        f.cheb_c2 = (float)(2.0 * rho_k / delta);
+        f.cheb_c1 = rho_k * rho;
+        f.cheb_c2 = 2.0 * rho_k / delta;
         // x is read and written every other step (x += d_{k-1} + d_k), and at the last
         f.cheb_xacc = (k % 2 == 1 && k < steps) ? 1 : (k % 2 == 0 ? 2 : 0);
         launch_b_coarse_apply_h(v, dnew, dold, EPI_CHEB, xh, f, N, st, r);
@@ -368,7 +368,7 @@ struct CycleB {
             ts.emplace_back(l == 0 ? 1 : nN);
             e2s.emplace_back(l == 0 ? 1 : nN);
             ps.emplace_back(last ? nN : 1);
-            cds.emplace_back(!last && std::is_same<T, float>::value && m.cheb_smooth_ratio > 0.0 ? nN : 1);
+            cds.emplace_back(!last && m.cheb_smooth_ratio > 0.0 ? nN : 1);
             sc.emplace_back(new ScalarsB(8 * 64 + 16 * 64));
         }
     }
@@ -400,7 +400,7 @@ struct CycleB {
             return;
         }
         const double om = mg.omega[lev];
-        const bool cheb = std::is_same<T, float>::value && mg.cheb_smooth_ratio > 0.0;
+        const bool cheb = mg.cheb_smooth_ratio > 0.0;
         const double chi = cheb ? mg.cheb_smooth_hi[lev] : 0.0, clo = cheb ? chi / mg.cheb_smooth_ratio : 0.0;
         if (cheb) cheb_smooth_b<T>(A, x, b, rs[lev].p(), cds[lev].p(), Ars[lev].p(), mg.cfg.pre, true, true, clo, chi,
                                     mg.cheb_smooth_fused, st);

@@ -33,7 +33,7 @@ struct FinB {
     // step writes (FP32 vectors): 0 r, d_new and x; 1 r only (the last step of a smoothing
     // that returns its residual; x too from a start); 2 x only (the last step of one
     // that does not)
-    float cheb_c1 = 0.0f, cheb_c2 = 0.0f;
+    double cheb_c1 = 0.0, cheb_c2 = 0.0;
     int cheb_mode = 0;
     // FP32 vectors: the first step of a smoothing takes the place of its initial pass.
     // cheb_start 1 (from x = 0): the input d and r are both b, the direction b / theta
@@ -42,7 +42,7 @@ struct FinB {
     // x_1 = x0 + r0 / theta.  The new r then goes to cheb_rout (r is an input the step
     // must not overwrite); 0 continues (cheb_s = 1, r in place unless cheb_rout is given)
     int cheb_start = 0;
-    float cheb_s = 1.0f;
+    double cheb_s = 1.0;
     void* cheb_rout = nullptr;
     // BF16 vectors: 0 x += d_new; 1 x left alone (read and written every other step);
     // 2 x += d + d_new, d the direction the step reads (the previous step's d_new)

@@ -1,5 +1,5 @@
-"""The Chebyshev smoother of the batched FP32 cycle (set_cheb_smoother): the cycle
-against a numpy cycle built from the hierarchy's own block vectors and Galerkin links
+"""The Chebyshev smoother of the batched cycles (set_cheb_smoother): the FP32 and FP64
+cycles against a numpy cycle built from the hierarchy's own block vectors and Galerkin links
 with the oracle's fine operator and the oracle's Chebyshev restatement; the one-launch
 steps (fine D^dag D and FP16-link coarse apply with the updates in their epilogues)
 against the apply plus separate vector passes; graph replays; mixed solves that reach
@@ -121,10 +121,16 @@ def test_cheb_smoothed_cycle_vs_numpy(smg, levels, L, N, pre, post, k, mfma):
         print(f"level {l}: hi / 1.1 = {hi[l] / 1.1:.6g}, largest eigenvalue {lmax:.6g}")
         assert 0.9 * lmax <= hi[l] / 1.1 <= lmax * (1 + 1e-9)
     cycle = _numpy_cycle(mg, U, L, levels, k, hi, pre, post)
-    errs = [relerr(X[j], cycle(B[j])) for j in range(N)]
+    ref = [cycle(B[j]) for j in range(N)]
+    errs = [relerr(X[j], ref[j]) for j in range(N)]
     print(f"max relative deviation from the FP64 numpy cycle {max(errs):.2e}; from MR {relerr(X, x_mr):.2e}")
     assert max(errs) < 2e-5
     assert relerr(X, x_mr) > 1e-2
+    # the FP64 batched cycle with the same smoother (FP64 links, dense coarsest inverse)
+    X64 = mg.precondition_batched(B, smg.F64)
+    e64 = max(relerr(X64[j], ref[j]) for j in range(N))
+    print(f"FP64 cycle: max relative deviation from the numpy cycle {e64:.2e}")
+    assert e64 < 1e-10
     mg.set_cheb_smoother(0.0)
     assert relerr(mg.precondition_batched(B, smg.F32), x_mr) < 1e-12   # MR restored
     smg.tune_clear()
