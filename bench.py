@@ -78,6 +78,8 @@ def parse():
                     help="Chebyshev smoothing in the mixed batched cycle on [hi / R, hi] (0: MR)")
     ap.add_argument("--cheb-lo-scale", type=float, default=1.0,
                     help="coarse-solver 2: the Chebyshev interval's lower end times this")
+    ap.add_argument("--cheb-hi-scale", type=float, default=1.0,
+                    help="coarse-solver 2: the Chebyshev interval's upper end times this")
     ap.add_argument("--cheb-steps", type=int, default=32,
                     help="coarse-solver 2: Chebyshev steps of the mixed cycle's coarsest solve")
     ap.add_argument("--coarse-bf16", type=int, default=1,
@@ -724,9 +726,9 @@ def main():
         else:
             mg.set_cycle(args.mr_pre, args.mr_post, args.coarse_maxit, args.coarse_tol, use_graph=bool(args.graph))
 
-    if cheb and args.cheb_lo_scale != 1.0:
+    if cheb and (args.cheb_lo_scale != 1.0 or args.cheb_hi_scale != 1.0):
         lo, hi = mg.cheb_interval()
-        mg.set_cheb_interval(lo * args.cheb_lo_scale, hi)
+        mg.set_cheb_interval(lo * args.cheb_lo_scale, hi * args.cheb_hi_scale)
     if args.cheb_smooth > 0:
         smooth_hi = mg.set_cheb_smoother(args.cheb_smooth)
         log(f"Chebyshev smoothing intervals: " + ", ".join(f"level {l} [{h / args.cheb_smooth:.4g}, {h:.4g}]"
