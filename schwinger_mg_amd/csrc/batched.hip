@@ -375,6 +375,13 @@ static bool cheb_ptrs_ok(const void* dst, const void* src, const void* b, const 
     if (dst == src || dst == b || dst == aux || (ro && dst == ro)) return false;
     if (aux == src || aux == b || (ro && aux == ro)) return false;
     if (ro && (ro == src || ro == b)) return false;
+    // x left alone (1) or caught up (2), the latter possibly with a term read from
+    // cheb_xadd, which the step must not write
+    if (f.cheb_xacc < 0 || f.cheb_xacc > 2) return false;
+    const void* xa = f.cheb_xadd;
+    if (xa && (f.cheb_xacc != 2 || xa == aux || xa == ro || (xa == dst && f.cheb_mode == 0) ||
+               (xa == b && f.cheb_mode != 2)))
+        return false;
     if (f.cheb_start == 0) return src != b;
     return ro != nullptr;
 }
@@ -444,7 +451,14 @@ __global__ __launch_bounds__(kBlock) void k_b_ddagd(DiracView<T> D, cplx<T>* __r
             RhsRun<T, P> out[2], bb[2], ax[2], dd[2];
             // EPI_CHEB (FinB): what the step reads, by its start and mode
             const int cst = EPI == EPI_CHEB ? fin.cheb_start : 0, cmo = EPI == EPI_CHEB ? fin.cheb_mode : 0;
-            const bool need_d = cmo != 1 || cst != 0, need_x = cst != 1 && (cmo != 1 || cst == 2);
+            const int xac = EPI == EPI_CHEB ? fin.cheb_xacc : 0;
+            const bool need_d = cmo != 1 || cst != 0, need_x = cst != 1 && xac != 1 && (cmo != 1 || cst == 2);
+            const cplx<T>* const xad = static_cast<const cplx<T>*>(fin.cheb_xadd);
+            RhsRun<T, P> xa[2];
+            if (EPI == EPI_CHEB && xac == 2 && xad) {   // the catch-up term added to x
+                xa[0].load(xad + i0);
+                xa[1].load(xad + i1);
+            }
             if (EPI == EPI_RESID || EPI == EPI_CHEB) {   // b, or the Chebyshev residual r
                 bb[0].load(bvec + i0);
                 bb[1].load(bvec + i1);
@@ -486,7 +500,11 @@ __global__ __launch_bounds__(kBlock) void k_b_ddagd(DiracView<T> D, cplx<T>* __r
 #pragma unroll
                     for (int sp = 0; sp < 2; sp++) {
                         const cplx<T> dp = (T)fin.cheb_s * dd[sp].v[q];   // the previous direction
-                        const cplx<T> xp = cst == 1 ? dp : (cst == 2 ? ax[sp].v[q] + dp : ax[sp].v[q]);
+                        cplx<T> xp = cst == 1 ? dp : (cst == 2 ? ax[sp].v[q] + dp : ax[sp].v[q]);
+                        if (xac == 2) {   // x was left alone by the step before: add its direction too
+                            xp = xp + dp;
+                            if (xad) xp = xp + (T)fin.cheb_xadd_s * xa[sp].v[q];
+                        }
                         dd[sp].v[q] = (T)fin.cheb_c1 * dp + (T)fin.cheb_c2 * out[sp].v[q];
                         ax[sp].v[q] = cmo == 1 ? xp : xp + dd[sp].v[q];
                     }
@@ -503,7 +521,7 @@ __global__ __launch_bounds__(kBlock) void k_b_ddagd(DiracView<T> D, cplx<T>* __r
                     dd[0].store(dst + i0);
                     dd[1].store(dst + i1);
                 }
-                if (cmo != 1 || cst != 0) {
+                if ((cmo != 1 || cst != 0) && xac != 1) {
                     ax[0].store(xw + i0);
                     ax[1].store(xw + i1);
                 }
@@ -980,7 +998,10 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
     const int cmode = fin.cheb_mode;   // EPI_CHEB: 0 r, d and x; 1 r only; 2 x only
     const int cst = HV ? 0 : fin.cheb_start;
     const int hxacc = HV ? fin.cheb_xacc : 0;   // BF16 Chebyshev: FinB's cheb_xacc
-    const bool need_d = cmode != 1 || cst != 0, need_x = cst != 1 && (cmode != 1 || cst == 2);
+    const int fxac = HV ? 0 : fin.cheb_xacc;   // FP32 vectors: FinB's cheb_xacc and cheb_xadd
+    const bool need_d = cmode != 1 || cst != 0, need_x = cst != 1 && fxac != 1 && (cmode != 1 || cst == 2);
+    const c32* const fxad = static_cast<const c32*>(fin.cheb_xadd);
+    const float fxas = (float)fin.cheb_xadd_s;
     constexpr bool RED = EPI == EPI_DOTS || EPI == EPI_CG;
     constexpr int NC = EPI == EPI_CG ? 5 : 3;   // reduced values per RHS
     if (RED && fin.skip && *fin.skip) {   // CG finished for every RHS
@@ -1220,13 +1241,22 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                             if (need_d) {
                                 const float4 xw = epo[r][v][u][0], dw = epo[r][v][u][2];
                                 const c32 p0 = sc * c32{dw.x, dw.y}, p1 = sc * c32{dw.z, dw.w};
-                                const c32 x0 = cst == 1 ? p0 : (cst == 2 ? c32{xw.x, xw.y} + p0 : c32{xw.x, xw.y});
-                                const c32 x1 = cst == 1 ? p1 : (cst == 2 ? c32{xw.z, xw.w} + p1 : c32{xw.z, xw.w});
+                                c32 x0 = cst == 1 ? p0 : (cst == 2 ? c32{xw.x, xw.y} + p0 : c32{xw.x, xw.y});
+                                c32 x1 = cst == 1 ? p1 : (cst == 2 ? c32{xw.z, xw.w} + p1 : c32{xw.z, xw.w});
+                                if (fxac == 2) {   // x was left alone by the step before
+                                    x0 = x0 + p0;
+                                    x1 = x1 + p1;
+                                    if (fxad) {
+                                        const float4 a = *reinterpret_cast<const float4*>(fxad + idx);
+                                        x0 = x0 + fxas * c32{a.x, a.y};
+                                        x1 = x1 + fxas * c32{a.z, a.w};
+                                    }
+                                }
                                 const c32 d0 = cc1 * p0 + cc2 * r0;
                                 const c32 d1 = cc1 * p1 + cc2 * r1;
                                 if (cmode == 0) *reinterpret_cast<float4*>(dst + idx) = float4{d0.re, d0.im, d1.re, d1.im};
                                 const c32 y0 = cmode == 1 ? x0 : x0 + d0, y1 = cmode == 1 ? x1 : x1 + d1;
-                                *reinterpret_cast<float4*>(xwf + idx) = float4{y0.re, y0.im, y1.re, y1.im};
+                                if (fxac != 1) *reinterpret_cast<float4*>(xwf + idx) = float4{y0.re, y0.im, y1.re, y1.im};
                             }
                         } else if constexpr (!HV && EPI == EPI_CHEB) {
                             const size_t idx = row + 16 * u + lo;
@@ -1234,10 +1264,11 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                             if (cmode != 2) rwf[idx] = rn;
                             if (need_d) {
                                 const c32 dp = ccs * src[idx];
-                                const c32 xp = cst == 1 ? dp : (cst == 2 ? aux[idx] + dp : aux[idx]);
+                                c32 xp = cst == 1 ? dp : (cst == 2 ? aux[idx] + dp : aux[idx]);
+                                if (fxac == 2) xp = xp + dp + (fxad ? fxas * fxad[idx] : c32{0.0f, 0.0f});
                                 const c32 dn = cc1 * dp + cc2 * rn;
                                 if (cmode == 0) dst[idx] = dn;
-                                xwf[idx] = cmode == 1 ? xp : xp + dn;
+                                if (fxac != 1) xwf[idx] = cmode == 1 ? xp : xp + dn;
                             }
                         } else if constexpr (HV && EPI == EPI_CHEB) {
                             const size_t idx = row + 16 * u + lo;
@@ -1474,7 +1505,7 @@ void launch_b_coarse_apply_h(const CoarseView<ch16>& C, cb16* dst, const cb16* s
     // while other workgroups read the old direction (src) at their neighbours
     SMG_REQUIRE(epi != EPI_CHEB || (aux && b && dst != src && aux != src && b != src && dst != aux && dst != b &&
                                     aux != b && fin.cheb_start == 0 && !fin.cheb_rout && fin.cheb_mode == 0 &&
-                                    fin.cheb_xacc >= 0 && fin.cheb_xacc <= 2),
+                                    fin.cheb_xacc >= 0 && fin.cheb_xacc <= 2 && !fin.cheb_xadd),
                 "BF16-vector coarse apply: the Chebyshev epilogue takes distinct r (b), x (aux), d (src) and "
                 "d_new (dst)");
     auto go = [&](auto kc, auto ntc) {
@@ -1679,10 +1710,12 @@ __global__ __launch_bounds__(kBlock) void k_b_restrict_mfma(AggView A, const c32
 }
 // fine[e][j] (+)= sum over half h's KH block vectors of V[kk][e] coarse[kk][j]: A = 16
 // entries (rows) by 4 block vectors, B = 4 block vectors by 16 RHS
+// accumulate 0: fine = P c; 1: fine += P c; 2: fine = s base + P c
 template <int KH>
 __global__ __launch_bounds__(kBlock) void k_b_prolong_mfma(AggView A, const c32* __restrict__ bv,
                                                             const c32* __restrict__ coarse, c32* __restrict__ fine,
-                                                            int accumulate, int N) {
+                                                            int accumulate, int N, const c32* __restrict__ base,
+                                                            float bs) {
     const int nh = A.chiral ? 2 : 1, nc = A.nd / nh, kc = A.bx * A.by * nc, nmt = kc >> 4, njc = N >> 4;
     const int bsize = A.bx * A.by * A.nd;
     size_t w = (size_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
@@ -1716,7 +1749,8 @@ __global__ __launch_bounds__(kBlock) void k_b_prolong_mfma(AggView A, const c32*
         const int e = mt * 16 + 4 * q + i, loc = e / nc, c = h * nc + e % nc;
         c32* out = fine + (half_entry_site(A, b, loc) * A.nd + c) * N + j;
         const c32 v{fr[i], fi[i]};
-        *out = accumulate ? *out + v : v;
+        if (accumulate == 2) *out = bs * base[out - fine] + v;
+        else *out = accumulate ? *out + v : v;
     }
 }
 // whether the matrix-core transfers take this aggregation and N (the tuner's
@@ -1737,12 +1771,12 @@ static void restrict_mfma(const AggView& A, const c32* bv, const c32* fine, c32*
     else k_b_restrict_mfma<64><<<g, kBlock, 0, st>>>(A, bv, fine, coarse, N);
 }
 static void prolong_mfma(const AggView& A, const c32* bv, const c32* coarse, c32* fine, int accumulate, int N,
-                         hipStream_t st) {
+                         hipStream_t st, const c32* base, float bs) {
     const int nh = A.chiral ? 2 : 1, kh = A.k / nh, kc = A.bx * A.by * (A.nd / nh);
     const size_t tiles = (size_t)A.nbx * A.nby * nh * (kc >> 4) * (N >> 4);
     const unsigned g = (unsigned)((tiles + kBlock / 64 - 1) / (kBlock / 64));
-    if (kh == 16) k_b_prolong_mfma<16><<<g, kBlock, 0, st>>>(A, bv, coarse, fine, accumulate, N);
-    else k_b_prolong_mfma<32><<<g, kBlock, 0, st>>>(A, bv, coarse, fine, accumulate, N);
+    if (kh == 16) k_b_prolong_mfma<16><<<g, kBlock, 0, st>>>(A, bv, coarse, fine, accumulate, N, base, bs);
+    else k_b_prolong_mfma<32><<<g, kBlock, 0, st>>>(A, bv, coarse, fine, accumulate, N, base, bs);
 }
 constexpr size_t kTransferLdsMax = 64 * 1024;
 // LDS bytes of the staged transfer, or 0 where it does not apply (sizes that are not
@@ -1762,14 +1796,20 @@ static int transfer_choice(const AggView& A, int N, size_t cbytes, bool prolong)
 
 template <typename T>
 void launch_b_prolong(const AggView& A, const cplx<T>* bv, const cplx<T>* coarse, cplx<T>* fine, int accumulate,
-                      int N, hipStream_t st) {
+                      int N, hipStream_t st, const cplx<T>* base, double bs) {
+    SMG_REQUIRE(accumulate >= 0 && accumulate <= 2 && (accumulate != 2 || (base && base != fine)),
+                "batched prolongation: accumulate 0, 1 or 2 (with a base vector apart from the output)");
     const int choice = transfer_choice(A, N, sizeof(cplx<T>), true);
     if constexpr (std::is_same<T, float>::value) {
         if (choice == 2 && b_transfer_mfma_fits(A, N)) {
-            prolong_mfma(A, bv, coarse, fine, accumulate, N, st);
+            prolong_mfma(A, bv, coarse, fine, accumulate, N, st, base, (float)bs);
             SMG_LAUNCH_CHECK();
             return;
         }
+    }
+    if (accumulate == 2) {   // the other kernels: fine = s base first, then fine += P c
+        launch_b_axpby_real<T>(fine, base, bs, 0.0, (size_t)A.nx * A.ny * A.nd * N, st);
+        accumulate = 1;
     }
     if (choice == 1) {
         const size_t lds = b_transfer_lds_bytes(A, N, sizeof(cplx<T>), true);
@@ -2532,7 +2572,7 @@ void launch_h_to_f32(c32* out, const cb16* in, size_t n, int N, hipStream_t st) 
 template <typename T>
 __global__ void k_b_axpby_real(cplx<T>* __restrict__ y, const cplx<T>* __restrict__ x, T a, T c, size_t items) {
     for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < items; t += (size_t)gridDim.x * kBlock)
-        y[t] = a * x[t] + c * y[t];
+        y[t] = c == (T)0 ? a * x[t] : a * x[t] + c * y[t];   // c = 0 does not read y
 }
 template <typename T>
 void launch_b_axpby_real(cplx<T>* y, const cplx<T>* x, double a, double c, size_t n, hipStream_t st) {
@@ -2595,7 +2635,7 @@ void launch_b_accum_mixed(c64* x, const c32* e, size_t n, int N, hipStream_t st)
     template void launch_b_dslash<T>(const DiracView<T>&, cplx<T>*, const cplx<T>*, T, int, int, Epi,           \
                                      const cplx<T>*, const cplx<T>*, const FinB&, int, hipStream_t);            \
     template void launch_b_prolong<T>(const AggView&, const cplx<T>*, const cplx<T>*, cplx<T>*, int, int,        \
-                                      hipStream_t);                                                              \
+                                      hipStream_t, const cplx<T>*, double);                                      \
     template void launch_b_restrict<T>(const AggView&, const cplx<T>*, const cplx<T>*, cplx<T>*, int, hipStream_t); \
     template void launch_b_dense_apply<T>(const cplx<T>*, const cplx<T>*, cplx<T>*, size_t, int, cplx<T>*, \
                                            hipStream_t);                                                   \

@@ -151,10 +151,13 @@ static void rich_b(LinOpB<T>& A, cplx<T>* x, const cplx<T>* b, cplx<T>* r, int n
 
 // Chebyshev smoothing on [lo, hi] (the three-term recurrence of the coarsest Chebyshev
 // solve, cheb_h): niter applications of A.  From x = 0 (zero_init) r starts as b, else
-// r = b - A x; with need_r, r = b - A x on return.
+// r = b - A x; with need_r, r = b - A x on return.  Returns 0, or s != 0 when x was left
+// unwritten and is s b (a single fused step from x = 0 when the caller allows it with
+// implicit_x; the caller then forms x itself).
 template <typename T>
-static void cheb_smooth_b(LinOpB<T>& A, cplx<T>* x, const cplx<T>* b, cplx<T>* r, cplx<T>* d, cplx<T>* q, int niter,
-                          bool zero_init, bool need_r, double lo, double hi, bool fused, hipStream_t st) {
+static double cheb_smooth_b(LinOpB<T>& A, cplx<T>* x, const cplx<T>* b, cplx<T>* r, cplx<T>* d, cplx<T>* q,
+                            int niter, bool zero_init, bool need_r, double lo, double hi, bool fused, hipStream_t st,
+                            bool implicit_x = false) {
     const size_t nN = A.n() * A.N;
     const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
     double rho = 1.0 / sigma;
@@ -173,6 +176,10 @@ static void cheb_smooth_b(LinOpB<T>& A, cplx<T>* x, const cplx<T>* b, cplx<T>* r
             rn = q;
         }
         cplx<T>* const buf[3] = {r, d, q};
+        // from x0 with two steps, the first leaves x alone and the second adds r0 / theta
+        // and both directions (r0 stays in r: the last step writes no direction)
+        const bool defer_x = !zero_init && niter == 2;
+        const bool skip_x = zero_init && niter == 1 && implicit_x;   // x = b / theta, unwritten
         for (int k = 1; k <= niter; k++) {
             const double rho_k = 1.0 / (2.0 * sigma - rho);
             FinB f;
@@ -183,6 +190,11 @@ static void cheb_smooth_b(LinOpB<T>& A, cplx<T>* x, const cplx<T>* b, cplx<T>* r
                 f.cheb_start = zero_init ? 1 : 2;
                 f.cheb_s = 1.0 / theta;
                 f.cheb_rout = rn;
+                if (defer_x || skip_x) f.cheb_xacc = 1;
+            } else if (k == 2 && defer_x) {
+                f.cheb_xacc = 2;
+                f.cheb_xadd = r;
+                f.cheb_xadd_s = 1.0 / theta;
             }
             cplx<T>* dn = nullptr;
             for (cplx<T>* c : buf)
@@ -195,7 +207,7 @@ static void cheb_smooth_b(LinOpB<T>& A, cplx<T>* x, const cplx<T>* b, cplx<T>* r
             dc = dn;
             rho = rho_k;
         }
-        return;
+        return skip_x ? 1.0 / theta : 0.0;
     }
     if (zero_init) {
         launch_zero<T>(x, nN, st);
@@ -203,7 +215,7 @@ static void cheb_smooth_b(LinOpB<T>& A, cplx<T>* x, const cplx<T>* b, cplx<T>* r
     } else {
         A.apply(r, x, EPI_RESID, b, nullptr, FinB(), st);
     }
-    if (niter == 0) return;
+    if (niter == 0) return 0.0;
     launch_zero<T>(d, nN, st);
     launch_b_axpby_real<T>(d, r, 1.0 / theta, 1.0, nN, st);   // d_0 = r / theta
     launch_b_axpby_real<T>(x, d, 1.0, 1.0, nN, st);           // x_1 = x_0 + d_0
@@ -216,6 +228,7 @@ static void cheb_smooth_b(LinOpB<T>& A, cplx<T>* x, const cplx<T>* b, cplx<T>* r
         launch_b_axpby_real<T>(x, d, 1.0, 1.0, nN, st);
         rho = rho_k;
     }
+    return 0.0;
 }
 
 // CG per RHS (src/coarse_op.h:492-547 without deflation); converged RHS take zero steps
@@ -402,8 +415,11 @@ struct CycleB {
         const double om = mg.omega[lev];
         const bool cheb = mg.cheb_smooth_ratio > 0.0;
         const double chi = cheb ? mg.cheb_smooth_hi[lev] : 0.0, clo = cheb ? chi / mg.cheb_smooth_ratio : 0.0;
-        if (cheb) cheb_smooth_b<T>(A, x, b, rs[lev].p(), cds[lev].p(), Ars[lev].p(), mg.cfg.pre, true, true, clo, chi,
-                                    mg.cheb_smooth_fused, st);
+        // a single fused pre-smoothing step leaves x = s b unwritten: the prolongation forms it
+        double ximp = 0.0;
+        if (cheb)
+            ximp = cheb_smooth_b<T>(A, x, b, rs[lev].p(), cds[lev].p(), Ars[lev].p(), mg.cfg.pre, true, true, clo, chi,
+                                  mg.cheb_smooth_fused, st, true);
         else if (om > 0) rich_b<T>(A, x, b, rs[lev].p(), mg.cfg.pre, om, true, true, st);
         else mr_b<T>(A, x, b, rs[lev].p(), Ars[lev].p(), mg.cfg.pre, true, true, sc[lev]->p(), st);
         const Aggregation& P = *mg.P[lev];
@@ -414,7 +430,8 @@ struct CycleB {
             run(lev + 1, ts[lev + 1].p(), e2s[lev + 1].p(), st);
             launch_b_axpby_real<T>(xs[lev + 1].p(), e2s[lev + 1].p(), 1.0, 1.0, mg.dims[lev + 1] * N, st);
         }
-        launch_b_prolong<T>(P.A, P.bv<T>(), xs[lev + 1].p(), x, 1, N, st);
+        if (ximp != 0.0) launch_b_prolong<T>(P.A, P.bv<T>(), xs[lev + 1].p(), x, 2, N, st, b, ximp);
+        else launch_b_prolong<T>(P.A, P.bv<T>(), xs[lev + 1].p(), x, 1, N, st);
         if (cheb) cheb_smooth_b<T>(A, x, b, rs[lev].p(), cds[lev].p(), Ars[lev].p(), mg.cfg.post, false, false, clo,
                                     chi, mg.cheb_smooth_fused, st);
         else if (om > 0) rich_b<T>(A, x, b, rs[lev].p(), mg.cfg.post, om, false, false, st);

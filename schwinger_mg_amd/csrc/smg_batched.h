@@ -44,9 +44,13 @@ struct FinB {
     int cheb_start = 0;
     double cheb_s = 1.0;
     void* cheb_rout = nullptr;
-    // BF16 vectors: 0 x += d_new; 1 x left alone (read and written every other step);
-    // 2 x += d + d_new, d the direction the step reads (the previous step's d_new)
+    // 0 x += d_new; 1 x left alone (neither read nor written); 2 x += d + d_new, d the
+    // direction the step reads (the previous step's d_new, which the step before did not
+    // add), plus cheb_xadd_s times the vector cheb_xadd where given (FP32 and FP64
+    // vectors: the scaled residual a first step from x0 did not add)
     int cheb_xacc = 0;
+    const void* cheb_xadd = nullptr;
+    double cheb_xadd_s = 0.0;
 };
 
 // Per-RHS coefficient array views (device): complex a_j at s[2j], s[2j+1].
@@ -76,7 +80,7 @@ size_t b_transfer_lds_bytes(const AggView& A, int N, size_t cbytes, bool prolong
 bool b_transfer_mfma_fits(const AggView& A, int N);
 template <typename T>
 void launch_b_prolong(const AggView& A, const cplx<T>* bv, const cplx<T>* coarse, cplx<T>* fine, int accumulate,
-                      int N, hipStream_t st);
+                      int N, hipStream_t st, const cplx<T>* base = nullptr, double bs = 0.0);
 template <typename T>
 void launch_b_restrict(const AggView& A, const cplx<T>* bv, const cplx<T>* fine, cplx<T>* coarse, int N,
                        hipStream_t st);
