@@ -2277,17 +2277,19 @@ __global__ __launch_bounds__(kBlock) void k_b_update(cplx<T>* __restrict__ x, cp
         cplx<T> a[P];
         run_coef<T, P>(s, (int)(i & (N - 1)), a);
         RhsRun<T, P> xv, rv, pv, av;
-        xv.load(x + i);
         rv.load(r + i);
-        pv.load(p + i);
         av.load(Ap + i);
+        if (x) {   // without x, the caller forms it later from the directions
+            xv.load(x + i);
+            pv.load(p + i);
+        }
 #pragma unroll
         for (int q = 0; q < P; q++) {
-            xv.v[q] = xv.v[q] + a[q] * pv.v[q];
+            if (x) xv.v[q] = xv.v[q] + a[q] * pv.v[q];
             rv.v[q] = rv.v[q] - a[q] * av.v[q];
             acc[0][q] += (double)norm2(rv.v[q]);
         }
-        xv.store(x + i);
+        if (x) xv.store(x + i);
         rv.store(r + i);
     }
     double v[1];
@@ -2300,6 +2302,47 @@ void launch_b_update(cplx<T>* x, cplx<T>* r, const cplx<T>* p, const cplx<T>* Ap
     run_dispatch<T>(N, [&](auto pc) {
         constexpr int P = decltype(pc)::value;
         k_b_update<T, P><<<b_grid(n * N / P, kRedGridVec), kBlock, 0, st>>>(x, r, p, Ap, s, n * N / P, fin, N);
+    });
+    SMG_LAUNCH_CHECK();
+}
+
+// x (+)= sum_{i < m} a_i,j P_i (a_i,j complex at coef.a[i][2j..]): the batched GCR's
+// deferred iterate update
+template <typename T>
+struct CoefB {
+    const double* a[16];
+};
+template <typename T, int P>
+__global__ __launch_bounds__(kBlock) void k_b_combine(cplx<T>* __restrict__ x, VecPtrs<T> Pv, CoefB<T> coef, int m,
+                                                      int accumulate, size_t runs, int N) {
+    for (size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x; t < runs; t += (size_t)gridDim.x * kBlock) {
+        const size_t e = t * P;
+        const int j = (int)(e & (N - 1));
+        RhsRun<T, P> xv;
+        if (accumulate) xv.load(x + e);
+        else
+#pragma unroll
+            for (int q = 0; q < P; q++) xv.v[q] = cplx<T>{(T)0, (T)0};
+        for (int i = 0; i < m; i++) {
+            cplx<T> a[P];
+            run_coef<T, P>(coef.a[i], j, a);
+            RhsRun<T, P> pv;
+            pv.load(Pv.v[i] + e);
+#pragma unroll
+            for (int q = 0; q < P; q++) xv.v[q] = xv.v[q] + a[q] * pv.v[q];
+        }
+        xv.store(x + e);
+    }
+}
+template <typename T>
+void launch_b_combine(cplx<T>* x, const VecPtrs<T>& P, const double* const* coef, int m, int accumulate, size_t n,
+                      int N, hipStream_t st) {
+    SMG_REQUIRE(m >= 1 && m <= 16, "batched combination: 1..16 vectors");
+    CoefB<T> c;
+    for (int i = 0; i < m; i++) c.a[i] = coef[i];
+    run_dispatch<T>(N, [&](auto pc) {
+        constexpr int RP = decltype(pc)::value;
+        k_b_combine<T, RP><<<b_grid(n * N / RP, 1 << 14), kBlock, 0, st>>>(x, P, c, m, accumulate, n * N / RP, N);
     });
     SMG_LAUNCH_CHECK();
 }
@@ -2648,6 +2691,8 @@ void launch_b_accum_mixed(c64* x, const c32* e, size_t n, int N, hipStream_t st)
                                        const cplx<T>*, const FinB*);                                             \
     template void launch_b_mr_update<T>(cplx<T>*, cplx<T>*, const cplx<T>*, const double*, size_t, int,          \
                                         hipStream_t, int, const cplx<T>*);                                       \
+    template void launch_b_combine<T>(cplx<T>*, const VecPtrs<T>&, const double* const*, int, int, size_t, int,   \
+                                      hipStream_t);                                                              \
     template void launch_b_update<T>(cplx<T>*, cplx<T>*, const cplx<T>*, const cplx<T>*, const double*, size_t,  \
                                      const FinB&, int, hipStream_t);                                             \
     template void launch_b_axpby_real<T>(cplx<T>*, const cplx<T>*, double, double, size_t, hipStream_t);    \

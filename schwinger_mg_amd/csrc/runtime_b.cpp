@@ -520,12 +520,11 @@ int gcr_solve_b(LinOpB<T>& A, const cplx<T>* b, cplx<T>* x, MG* mg, int maxit, d
     const int N = A.N;
     SMG_REQUIRE(trunc >= 1 && trunc <= 16, "batched GCR truncation window must be 1..16");
     Context& cx = Context::get();
-    ScalarsB sc(64 * 2 + 64 + 64 * 32 + 17 * 64);
-    double* alpha = sc.p(0);                   // [N][2]
+    ScalarsB sc(64 * 2 + 64 + 64 * 32 + 17 * 64 + 17 * 128);
     double* rr = sc.p(128);                    // [N]
     double* dots = sc.p(192);                  // [N][32]
     double* apap = sc.p(192 + 64 * 32);        // [slot][N]
-    launch_zero<T>(x, nN, st);
+    double* alphas = sc.p(192 + 64 * 32 + 17 * 64);   // [slot][N][2]
     std::vector<double> bn(N), rn(N);
     launch_b_norm2<T>(b, n, finb(FIN_STORE, rr, 1), N, st);
     cx.read(rr, bn.data(), N);
@@ -543,6 +542,24 @@ int gcr_solve_b(LinOpB<T>& A, const cplx<T>* b, cplx<T>* x, MG* mg, int maxit, d
     std::deque<int> order;
     std::vector<int> freeslots;
     for (int i = trunc; i >= 0; i--) freeslots.push_back(i);
+    // x is formed from the directions and their alphas, x = sum alpha_i p_i, at the end and
+    // before a slot whose direction is not yet in x is reused, instead of at every step
+    std::vector<int> pend;
+    bool xset = false;
+    auto flush = [&]() {
+        for (size_t i0 = 0; i0 < pend.size(); i0 += 16) {   // at most 16 directions per pass
+            const int m = (int)std::min<size_t>(16, pend.size() - i0);
+            VecPtrs<T> Pv;
+            const double* co[16];
+            for (int i = 0; i < m; i++) {
+                Pv.v[i] = Pd[pend[i0 + i]].p();
+                co[i] = alphas + (size_t)pend[i0 + i] * 128;
+            }
+            launch_b_combine<T>(x, Pv, co, m, xset ? 1 : 0, n, N, st);
+            xset = true;
+        }
+        pend.clear();
+    };
     auto done = [&]() {
         for (int j = 0; j < N; j++)
             if (bn[j] > 1e-30 && rn[j] / bn[j] >= tol) return false;
@@ -559,6 +576,7 @@ int gcr_solve_b(LinOpB<T>& A, const cplx<T>* b, cplx<T>* x, MG* mg, int maxit, d
         const int nd = (int)order.size();
         // alpha = <Ap, r> / |Ap|^2 from the sums of the pass that forms Ap last: the apply
         // (no earlier directions) or the orthogonalisation
+        double* const alpha = alphas + (size_t)slot * 128;
         const FinB fa = finb(FIN_GCR_ALPHA, alpha, 2, apap + (size_t)slot * N, 1);
         if (nd == 0) A.apply(Ap, p, EPI_DOTS, nullptr, r.p(), fa, st);
         else A.apply(Ap, p, st);
@@ -573,16 +591,20 @@ int gcr_solve_b(LinOpB<T>& A, const cplx<T>* b, cplx<T>* x, MG* mg, int maxit, d
             launch_b_multi_dot<T>(APv, nd, Ap, n, finb(FIN_STORE, dots, 32), N, st);
             launch_b_gcr_orth<T>(p, Ap, Pv, APv, nd, dots, ap, n, N, st, r.p(), &fa);
         }
-        launch_b_update<T>(x, r.p(), p, Ap, alpha, n, finb(FIN_STORE, rr, 1), N, st);
+        launch_b_update<T>(nullptr, r.p(), p, Ap, alpha, n, finb(FIN_STORE, rr, 1), N, st);
+        pend.push_back(slot);
         cx.read(rr, rn.data(), N);
         for (int j = 0; j < N; j++) rn[j] = std::sqrt(rn[j]);
         iter++;
         if ((int)order.size() >= trunc) {
+            if (std::find(pend.begin(), pend.end(), order.front()) != pend.end()) flush();
             freeslots.push_back(order.front());
             order.pop_front();
         }
         order.push_back(slot);
     }
+    flush();
+    if (!xset) launch_zero<T>(x, nN, st);   // no step taken
     for (int j = 0; j < N; j++) res[j] = bn[j] > 1e-30 ? rn[j] / bn[j] : 0.0;
     return iter;
 }

@@ -111,7 +111,11 @@ void launch_b_gcr_orth(cplx<T>* p, cplx<T>* Ap, const VecPtrs<T>& P, const VecPt
 template <typename T>
 void launch_b_mr_update(cplx<T>* x, cplx<T>* r, const cplx<T>* Ar, const double* s, size_t n, int N, hipStream_t st,
                         int mode = 0, const cplx<T>* b = nullptr);
-// x += a_j p ; r -= a_j Ap with {|r_j|^2} -> fin
+// x (+, with accumulate)= sum_{i < m} a_ij P_i, a_ij complex at coef[i][2j..] (m <= 16)
+template <typename T>
+void launch_b_combine(cplx<T>* x, const VecPtrs<T>& P, const double* const* coef, int m, int accumulate, size_t n,
+                      int N, hipStream_t st);
+// x += a_j p ; r -= a_j Ap with {|r_j|^2} -> fin (x = nullptr: r only)
 template <typename T>
 void launch_b_update(cplx<T>* x, cplx<T>* r, const cplx<T>* p, const cplx<T>* Ap, const double* s, size_t n,
                      const FinB& fin, int N, hipStream_t st);
