@@ -631,8 +631,7 @@ BatchedSolveResult gcr_solve_batched(const Dirac& D, MG* mg, int prec, const c64
     } else {
         DdagDOpB<float> A32(D, N);
         Field<float> r32(nN), e32(nN);
-        launch_zero<double>(x64.p(), nN, st);
-        bcopy(r64.p(), b64.p(), nN, st);
+        const c64* rsrc = b64.p();   // the residual of x = 0 is b; then r64
         rn = bn;
         int total = 0;
         while (total < maxit) {
@@ -644,24 +643,29 @@ BatchedSolveResult gcr_solve_batched(const Dirac& D, MG* mg, int prec, const c64
                     if (rn[j] / bn[j] >= tol) conv = false;
                 }
             if (conv) break;
-            launch_convert<float, double>(r32.p(), r64.p(), nN, st);
+            launch_convert<float, double>(r32.p(), rsrc, nN, st);
             const double itol = std::max(inner_tol, 0.5 * tol / worst);
             const int it = gcr_solve_b<float>(A32, r32.p(), e32.p(), mg, maxit - total, itol, trunc, res.data(), st);
+            if (rsrc == b64.p()) launch_convert<double, float>(x64.p(), e32.p(), nN, st);   // x = e
+            else launch_b_accum_mixed(x64.p(), e32.p(), n, N, st);
             total += std::max(it, 1);
-            launch_b_accum_mixed(x64.p(), e32.p(), n, N, st);
+            rsrc = r64.p();
             A64.apply(r64.p(), x64.p(), EPI_RESID, b64.p(), nullptr, FinB(), st);
             launch_b_norm2<double>(r64.p(), n, finb(FIN_STORE, sc.p(), 1), N, st);
             cx.read(sc.p(), rn.data(), N);
             for (int j = 0; j < N; j++) rn[j] = std::sqrt(rn[j]);
             out.restarts++;
         }
+        if (rsrc == b64.p()) launch_zero<double>(x64.p(), nN, st);   // no refinement step: x = 0
         out.iterations = total;
     }
-    // FP64 true residuals
-    A64.apply(r64.p(), x64.p(), EPI_RESID, b64.p(), nullptr, FinB(), st);
-    launch_b_norm2<double>(r64.p(), n, finb(FIN_STORE, sc.p(), 1), N, st);
-    cx.read(sc.p(), rn.data(), N);
-    for (int j = 0; j < N; j++) out.true_residual[j] = bn[j] > 1e-30 ? std::sqrt(rn[j]) / bn[j] : 0.0;
+    if (prec == 0) {   // FP64 true residuals (the refinement loop above ends with them in rn)
+        A64.apply(r64.p(), x64.p(), EPI_RESID, b64.p(), nullptr, FinB(), st);
+        launch_b_norm2<double>(r64.p(), n, finb(FIN_STORE, sc.p(), 1), N, st);
+        cx.read(sc.p(), rn.data(), N);
+        for (int j = 0; j < N; j++) rn[j] = std::sqrt(rn[j]);
+    }
+    for (int j = 0; j < N; j++) out.true_residual[j] = bn[j] > 1e-30 ? rn[j] / bn[j] : 0.0;
     launch_b_unpack<double, double>(X, x64.p(), n, N, st);
     return out;
 }
