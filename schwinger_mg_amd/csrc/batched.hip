@@ -1146,9 +1146,12 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
         // (PAIR layout)
         using EpoV = typename std::conditional<HV, uint2, float4>::type;
         constexpr int NE = EPI == EPI_CG ? 2 : (EPI == EPI_CHEB ? 3 : 1);
-        EpoV epo[PAIR ? RT : 1][PAIR ? 4 : 1][PAIR ? NL : 1][NE];
+        // the FP32-vector Chebyshev step loads its three operands in the epilogue itself:
+        // held from the last chunks on, they would cost the kernel its second wave per SIMD
+        constexpr bool PRE = !(EPI == EPI_CHEB && !HV);
+        EpoV epo[PAIR && PRE ? RT : 1][PAIR && PRE ? 4 : 1][PAIR && PRE ? NL : 1][NE];
         auto load_epo = [&]() {
-            if constexpr (PAIR && EPI != EPI_NONE) {
+            if constexpr (PAIR && EPI != EPI_NONE && PRE) {
                 if (out_ok) {
 #pragma unroll
                     for (int r = 0; r < RT; r++)
@@ -1232,14 +1235,16 @@ void k_b_coarse_bf16(CoarseView<TL> C, void* __restrict__ dstv, const void* __re
                         } else if constexpr (PAIR && !HV && EPI == EPI_CHEB) {
                             // FP32 vectors: r -= s q, then (mode) d_new = c1 d + c2 r, x += d_new
                             const size_t idx = row + 32 * u + 2 * lo;
-                            const float4 rw = epo[r][v][u][1];
+                            const float4 rw = *reinterpret_cast<const float4*>(bvec + idx);
                             const float sc = ccs;
                             const c32 r0 = c32{rw.x, rw.y} - sc * c32{acc[r][2 * u][0][v], acc[r][2 * u][1][v]};
                             const c32 r1 =
                                 c32{rw.z, rw.w} - sc * c32{acc[r][2 * u + 1][0][v], acc[r][2 * u + 1][1][v]};
                             if (cmode != 2) *reinterpret_cast<float4*>(rwf + idx) = float4{r0.re, r0.im, r1.re, r1.im};
                             if (need_d) {
-                                const float4 xw = epo[r][v][u][0], dw = epo[r][v][u][2];
+                                const float4 xw = need_x ? *reinterpret_cast<const float4*>(aux + idx)
+                                                         : float4{0.0f, 0.0f, 0.0f, 0.0f};
+                                const float4 dw = *reinterpret_cast<const float4*>(src + idx);
                                 const c32 p0 = sc * c32{dw.x, dw.y}, p1 = sc * c32{dw.z, dw.w};
                                 c32 x0 = cst == 1 ? p0 : (cst == 2 ? c32{xw.x, xw.y} + p0 : c32{xw.x, xw.y});
                                 c32 x1 = cst == 1 ? p1 : (cst == 2 ? c32{xw.z, xw.w} + p1 : c32{xw.z, xw.w});
